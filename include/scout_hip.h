@@ -1,0 +1,168 @@
+/* scout_hip.h -- C ABI of libscout_hip.so, the MI355X repair engine behind the scout_*
+ * binaries.
+ *
+ * Plain pointers and sizes only. Every function returns SCOUT_OK (0) or an error code;
+ * scout_last_error() describes the last failure of the calling thread. The library never
+ * exits the process. Host pointers are caller-owned; device state and pinned staging are
+ * owned by the context until scout_ctx_destroy.
+ *
+ * Each entry point names the reference interface it replaces (paths relative to the
+ * reference repository). One host thread drives a context; calls on one context must not
+ * overlap.
+ */
+#ifndef SCOUT_HIP_H
+#define SCOUT_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  SCOUT_OK = 0,
+  SCOUT_ERR_ARG = 1,         /* bad argument or size */
+  SCOUT_ERR_HIP = 2,         /* a HIP runtime call failed */
+  SCOUT_ERR_STATE = 3,       /* call out of order (e.g. repair before load) */
+  SCOUT_ERR_BUDGET = 4,      /* the request exceeds the context's HBM or pinned budget */
+  SCOUT_ERR_UNSUPPORTED = 5  /* input outside what the engine handles exactly */
+};
+
+/* Maintained solutions (scout_state_upload / scout_state_download). */
+enum {
+  SCOUT_STATE_MIS = 0,    /* uint8 per vertex, 1 = member */
+  SCOUT_STATE_COLOR = 1,  /* int32 per vertex */
+  SCOUT_STATE_LEVEL = 2,  /* int32 per vertex, -1 = unreached */
+  SCOUT_STATE_LABEL = 3   /* int32 per vertex: smallest vertex id of its component */
+};
+
+/* Classification rules for scout_classify. */
+enum {
+  SCOUT_CLASSIFY_CURRENT = 0, /* insert iff absent from the current graph (coloring/bfs/cc) */
+  SCOUT_CLASSIFY_BASE = 1     /* insert iff absent from the loaded base graph (mis: every
+                                 batch is parsed against the initial graph) */
+};
+
+/* Repair strategies for scout_repair_bfs / scout_repair_cc. Every strategy returns the
+ * same exact answer; they differ only in work. */
+enum {
+  SCOUT_STRATEGY_LOCAL = 0,     /* local repair of the affected region only */
+  SCOUT_STRATEGY_RECOMPUTE = 1, /* recompute the whole solution on the device */
+  SCOUT_STRATEGY_LEARNED = 2    /* let the trained scheduler choose per batch */
+};
+
+typedef struct scout_ctx scout_ctx;
+
+typedef struct {
+  int64_t hbm_bytes;    /* 0 = no limit beyond the device */
+  int64_t pinned_bytes; /* 0 = no limit */
+} scout_budget;
+
+/* Per-batch statistics (fields a repair does not use stay 0). */
+typedef struct {
+  int64_t lines;        /* batch lines handed to the repair */
+  int64_t insertions;   /* lines labelled insertion */
+  int64_t deletions;    /* lines labelled deletion */
+  int64_t noop;         /* lines that needed no repair work */
+  int64_t seeds;        /* vertices the repair started from */
+  int64_t touched;      /* vertices whose rows were scanned */
+  int64_t rounds;       /* device rounds */
+  int64_t fallback;     /* 1 if the batch was settled by a full recompute */
+  int64_t splits;       /* CC: new components created by deletions */
+  int64_t conflicts;    /* coloring: insertions with equal colours; mis: edits that wrote */
+  int64_t sub_batches;  /* mis: sub-batches */
+  int64_t edges_scanned;/* adjacency entries read by the repair kernels */
+  double repair_ms;     /* device time of the repair, from HIP events on the compute stream */
+  double apply_ms;      /* device time of classification + graph update */
+} scout_stats;
+
+const char* scout_last_error(void);
+
+/* Device set-up, pinned and device pools. Replaces gpu_warmup_and_device_setup
+ * (core/cli_setup.h:54-58) and the alloc_pair / alloc_and_upload pools
+ * (core/gpu_common.h:43-72). budget may be NULL. */
+int scout_ctx_create(int gpu_id, const scout_budget* budget, scout_ctx** out);
+int scout_ctx_destroy(scout_ctx* ctx);
+int scout_device_name(scout_ctx* ctx, char* buf, int len);
+/* Bytes of device memory free / total (the reference's cudaMemGetInfo uses,
+ * src/mis/scout_mis.cu:51-72,808-814). */
+int scout_mem_info(scout_ctx* ctx, int64_t* free_bytes, int64_t* total_bytes);
+
+/* Loads the base graph into HBM, streaming the (mmap'd) .egr arrays through pinned staging
+ * on the copy stream. nindex is int32 (wide = 0) or uint64 (wide = 1), n+1 entries.
+ * Replaces the per-round row fetch HybridGraph::neighbors_into (core/FlashGraph.h:376-385)
+ * and the chunked nlist upload of full_recompute_cc_gpu (src/cc/scout_cc.cu:157-215). */
+int scout_graph_load(scout_ctx* ctx, int32_t n, int64_t entries, const void* nindex, int wide,
+                     const int32_t* nlist);
+/* out[0..5] = n, base entries, inserted-row entries, logical entries, rows sorted (0/1). */
+int scout_graph_info(scout_ctx* ctx, int64_t* out);
+/* The logical graph as a CSR in scratchpad-compaction row order (base rows minus deleted
+ * entries, then inserted entries in insertion order): core/flash_compactor.h:23-95.
+ * off has n+1 entries; col may be NULL to query off only. */
+int scout_graph_download(scout_ctx* ctx, int64_t* off, int32_t* col);
+/* Merges the inserted rows and deletions into a fresh base CSR in HBM (the scratchpad
+ * compaction of core/flash_compactor.h:23-119, done on the device). */
+int scout_graph_compact(scout_ctx* ctx);
+/* Degree of every vertex of the logical graph into a host array of n int64. */
+int scout_graph_degrees(scout_ctx* ctx, int64_t* deg);
+
+/* Solution state. Replace setMISFromHost / getMISToHost (src/mis/scout_mis.cu:480-494) and
+ * the colour / level uploads (src/coloring/scout_coloring.cu:313, src/bfs/scout_bfs.cu:562). */
+int scout_state_upload(scout_ctx* ctx, int kind, const void* host, int64_t n);
+int scout_state_download(scout_ctx* ctx, int kind, void* host, int64_t n);
+int scout_set_source(scout_ctx* ctx, int32_t source);
+
+/* Stages one batch of 0-indexed (u, v) pairs (2*m int32) into pinned memory and sends it to
+ * the device on the copy stream; the compute stream waits for it before the next call that
+ * uses the channel (channel 0 or 1). Replaces the pinned-pool H2D of launchBatch
+ * (src/mis/scout_mis.cu:639-647) and the per-round H2D of gpu_relax_round
+ * (src/bfs/scout_bfs.cu:223-229). */
+int scout_stage_batch(scout_ctx* ctx, int channel, const int32_t* uv, int64_t m);
+/* Same, from a device pointer already in HBM (no PCIe transfer). */
+int scout_stage_batch_device(scout_ctx* ctx, int channel, const int32_t* d_uv, int64_t m);
+
+/* Labels every staged line insertion (1) or deletion (0) against the rule's graph
+ * (parse_batch_file, core/cli_setup.h:87-103). ins may be NULL. */
+int scout_classify(scout_ctx* ctx, int channel, int rule, uint8_t* ins);
+/* Applies the staged lines to the graph with the scratchpad rules of
+ * DeltaOverlay::insertEdge/removeEdge (core/FlashGraph.h:227-310). conditional = 1 applies
+ * an insertion only if absent and a deletion only if present at apply time (MIS,
+ * src/mis/scout_mis.cu:743-749). Pairs repeated inside the batch are applied in line order. */
+int scout_apply(scout_ctx* ctx, int channel, int conditional);
+
+/* Batch repairs. Each runs after scout_apply on the same channel and leaves the solution
+ * exact for the updated graph. */
+/* BFS levels: replaces processBatch + gpu_relax_round + gpu_deletion_round
+ * (src/bfs/scout_bfs.cu:177-272,325-515). */
+int scout_repair_bfs(scout_ctx* ctx, int channel, int strategy, scout_stats* st);
+/* CC min-id labels: replaces processBatch + bidirectional_reachability_check
+ * (src/cc/scout_cc.cu:300-441). */
+int scout_repair_cc(scout_ctx* ctx, int channel, int strategy, scout_stats* st);
+/* Colouring: replaces processBatch + gpu_color_round (src/coloring/scout_coloring.cu:172-277). */
+int scout_repair_color(scout_ctx* ctx, int channel, scout_stats* st);
+/* MIS: replaces launchBatch + syncAndCollect (src/mis/scout_mis.cu:500-720); max_csr is the
+ * reference's MAX_CSR_ELEMENTS sub-batch budget (src/mis/scout_mis.cu:800-815). */
+int scout_repair_mis(scout_ctx* ctx, int channel, int64_t max_csr, scout_stats* st);
+
+/* Full recomputes. BFS from `source` replaces gpu_full_bfs (src/bfs/scout_bfs.cu:274-323);
+ * CC replaces full_recompute_cc_gpu (src/cc/scout_cc.cu:118-278). */
+int scout_bfs_full(scout_ctx* ctx, int32_t source);
+int scout_cc_full(scout_ctx* ctx);
+/* Greedy-ascending initial MIS and colouring (tools/compute_initial_mis.py:32-43,
+ * tools/greedy_color.cpp:37-52), computed on the device. */
+int scout_mis_greedy(scout_ctx* ctx);
+int scout_color_greedy(scout_ctx* ctx);
+
+/* Waits for all work of the context. */
+int scout_sync(scout_ctx* ctx);
+
+/* Synthetic inputs generated in HBM (no reference counterpart; the reference's generators
+ * are Python tools). R-MAT with quadrant probabilities (a, b, c, 1-a-b-c), 2^scale
+ * vertices, edge_factor * 2^scale generated edges, symmetrised, self-loops and duplicates
+ * removed, rows sorted. Bit-identical to the host generator for the same seed. */
+int scout_gen_rmat(scout_ctx* ctx, int scale, int edge_factor, double a, double b, double c, uint64_t seed);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

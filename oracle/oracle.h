@@ -1,0 +1,85 @@
+/* oracle.h -- CPU restatement of the SCOUT reference's hot path, used ONLY as the checker.
+ *
+ * This is test infrastructure. Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg load it. The product path (libscout_hip.so and the scout_* binaries)
+ * never links, loads or calls it.
+ *
+ * What each function restates (reference file:line) and what pins it is written next to
+ * the function in oracle.cpp. Everything is integer arithmetic.
+ */
+#ifndef SCOUT_ORACLE_H
+#define SCOUT_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_graph orc_graph;
+
+/* Base graph + scratchpad overlay (core/FlashGraph.h DeltaOverlay/HybridGraph semantics). */
+orc_graph* orc_graph_open(const char* egr_path);
+orc_graph* orc_graph_from_csr(int32_t n, const int64_t* off, const int32_t* col);
+void orc_graph_free(orc_graph* g);
+int32_t orc_n(const orc_graph* g);
+int64_t orc_base_edges(const orc_graph* g);
+
+/* Classification "insert iff !isAdjacent(u,v)" of every pair (core/cli_setup.h:87-103). */
+void orc_classify(orc_graph* g, const int32_t* uv, int64_t m, uint8_t* ins);
+/* insertEdge/removeEdge per line, unconditionally (coloring/bfs/cc processBatch). */
+void orc_apply(orc_graph* g, const int32_t* uv, const uint8_t* ins, int64_t m);
+/* MIS's conditional apply (src/mis/scout_mis.cu:743-749). */
+void orc_apply_conditional(orc_graph* g, const int32_t* uv, const uint8_t* ins, int64_t m);
+int orc_is_adjacent(orc_graph* g, int32_t u, int32_t v);
+int64_t orc_degree(orc_graph* g, int32_t v);
+int64_t orc_row(orc_graph* g, int32_t v, int32_t* out, int64_t cap);
+/* Logical CSR in compactor row order (core/flash_compactor.h:30-95). off has n+1 entries;
+ * col may be NULL to query the size. Returns the entry count, or -1 on an inconsistent
+ * delta (negative degree / degree mismatch, the compactor's error exits). */
+int64_t orc_logical_csr(orc_graph* g, int64_t* off, int32_t* col);
+/* Durable wide .egr of the logical graph (compact_flash_graph). 0 on success. */
+int orc_compact_to(orc_graph* g, const char* path);
+
+/* Closed forms. */
+void orc_bfs(orc_graph* g, int32_t src, int32_t* level);
+int32_t orc_cc(orc_graph* g, int32_t* label);
+
+/* Per-batch repairs (graph already updated with the batch). */
+int64_t orc_color_batch(orc_graph* g, const int32_t* uv, const uint8_t* ins, int64_t m, int32_t* color,
+                        int64_t* stats /* [conflicts, rounds] */);
+int64_t orc_mis_max_csr(double avg_degree, double free_bytes);
+int orc_mis_batch(orc_graph* g, const int32_t* uv, const uint8_t* ins, int64_t m, int64_t max_csr, uint8_t* mis,
+                  int64_t* stats /* [sub_batches, rows, writes] */);
+
+/* Validity checks; each returns the number of violations. */
+int64_t orc_check_bfs(orc_graph* g, int32_t src, const int32_t* level);
+int64_t orc_check_cc(orc_graph* g, const int32_t* label);
+int64_t orc_check_coloring(orc_graph* g, const int32_t* color);
+int64_t orc_check_mis_independent(orc_graph* g, const uint8_t* mis);
+int64_t orc_mis_maximality_violations(orc_graph* g, const uint8_t* mis);
+
+/* Initial states by greedy ascending id (tools/compute_initial_mis.py:32-43,
+ * tools/greedy_color.cpp:37-52). */
+void orc_greedy_mis(orc_graph* g, uint8_t* mis);
+void orc_greedy_color(orc_graph* g, int32_t* color);
+
+/* The reference's own per-edit algorithms restated on the CPU (CPU baseline):
+ * BFS processBatch (src/bfs/scout_bfs.cu:436-515) and CC processBatch
+ * (src/cc/scout_cc.cu:392-441). The states persist inside the handle. */
+typedef struct orc_refbfs orc_refbfs;
+orc_refbfs* orc_refbfs_create(orc_graph* g, const int32_t* level, int32_t source, int64_t bound, int defer);
+void orc_refbfs_free(orc_refbfs* r);
+void orc_refbfs_batch(orc_refbfs* r, const int32_t* uv, const uint8_t* ins, int64_t m, int64_t* stats /* 8 */);
+void orc_refbfs_finish(orc_refbfs* r, int32_t* level);
+
+typedef struct orc_refcc orc_refcc;
+orc_refcc* orc_refcc_create(orc_graph* g, const int32_t* root, int64_t bound, int defer);
+void orc_refcc_free(orc_refcc* r);
+void orc_refcc_batch(orc_refcc* r, const int32_t* uv, const uint8_t* ins, int64_t m, int64_t* stats /* 8 */);
+void orc_refcc_finish(orc_refcc* r, int32_t* label);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,266 @@
+"""scout_amd: Python binding of libscout_hip.so, the MI355X SCOUT repair engine.
+
+The product path is the C ABI declared in include/scout_hip.h; this module loads the
+in-tree library with ctypes and exposes it as an ``Engine`` object whose methods mirror the
+reference's per-batch flow (src/*/scout_*.cu processBatch): stage a batch, classify its
+lines, apply them to the scratchpad, repair, read the solution back.
+
+There is no CPU fallback: if the library is missing the import of ``Engine`` fails loudly.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(ROOT, "libscout_hip.so")
+BIN_DIR = os.path.join(ROOT, "bin")
+
+OK = 0
+STATE_MIS, STATE_COLOR, STATE_LEVEL, STATE_LABEL = 0, 1, 2, 3
+CLASSIFY_CURRENT, CLASSIFY_BASE = 0, 1
+STRATEGY_LOCAL, STRATEGY_RECOMPUTE, STRATEGY_LEARNED = 0, 1, 2
+
+
+class ScoutError(RuntimeError):
+    pass
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [(name, ctypes.c_int64) for name in (
+        "lines", "insertions", "deletions", "noop", "seeds", "touched", "rounds", "fallback",
+        "splits", "conflicts", "sub_batches", "edges_scanned")] + [
+        ("repair_ms", ctypes.c_double), ("apply_ms", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_P = ctypes.c_void_p
+_I32 = ctypes.c_int32
+_I64 = ctypes.c_int64
+
+# name -> argument types (all functions return int status except scout_last_error)
+_SIGNATURES = {
+    "scout_ctx_create": [ctypes.c_int, _P, ctypes.POINTER(_P)],
+    "scout_ctx_destroy": [_P],
+    "scout_device_name": [_P, ctypes.c_char_p, ctypes.c_int],
+    "scout_mem_info": [_P, ctypes.POINTER(_I64), ctypes.POINTER(_I64)],
+    "scout_graph_load": [_P, _I32, _I64, _P, ctypes.c_int, _P],
+    "scout_graph_info": [_P, _P],
+    "scout_graph_download": [_P, _P, _P],
+    "scout_graph_compact": [_P],
+    "scout_graph_degrees": [_P, _P],
+    "scout_state_upload": [_P, ctypes.c_int, _P, _I64],
+    "scout_state_download": [_P, ctypes.c_int, _P, _I64],
+    "scout_set_source": [_P, _I32],
+    "scout_stage_batch": [_P, ctypes.c_int, _P, _I64],
+    "scout_stage_batch_device": [_P, ctypes.c_int, _P, _I64],
+    "scout_classify": [_P, ctypes.c_int, ctypes.c_int, _P],
+    "scout_apply": [_P, ctypes.c_int, ctypes.c_int],
+    "scout_repair_bfs": [_P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(Stats)],
+    "scout_repair_cc": [_P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(Stats)],
+    "scout_repair_color": [_P, ctypes.c_int, ctypes.POINTER(Stats)],
+    "scout_repair_mis": [_P, ctypes.c_int, _I64, ctypes.POINTER(Stats)],
+    "scout_bfs_full": [_P, _I32],
+    "scout_cc_full": [_P],
+    "scout_mis_greedy": [_P],
+    "scout_color_greedy": [_P],
+    "scout_sync": [_P],
+    "scout_gen_rmat": [_P, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                       ctypes.c_uint64],
+}
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Loads libscout_hip.so and sets every exported signature. Raises if it is missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise ScoutError(f"{path} is not built; run __graft_entry__.build() or make -C scout_amd")
+    lib = ctypes.CDLL(path)
+    for name, argtypes in _SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = argtypes
+        fn.restype = ctypes.c_int
+    lib.scout_last_error.argtypes = []
+    lib.scout_last_error.restype = ctypes.c_char_p
+    _lib = lib
+    return lib
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def declared_symbols(header: str) -> list[str]:
+    """Function names declared in a C header of this repository (for export checks)."""
+    import re
+    text = open(header).read()
+    return sorted(set(re.findall(r"^(?:int|const char\*)\s+(scout_\w+)\(", text, re.M)))
+
+
+@dataclass
+class Graph:
+    n: int
+    off: np.ndarray   # int64, n+1
+    col: np.ndarray   # int32
+
+    @staticmethod
+    def read_egr(path: str) -> "Graph":
+        raw = np.fromfile(path, dtype=np.uint8)
+        n, e32 = np.frombuffer(raw[:8].tobytes(), dtype=np.int32)
+        if e32 == -1:
+            e = int(np.frombuffer(raw[8:16].tobytes(), dtype=np.uint64)[0])
+            off = np.frombuffer(raw[16:16 + 8 * (n + 1)].tobytes(), dtype=np.int64)
+            base = 16 + 8 * (n + 1)
+        else:
+            e = int(e32)
+            off = np.frombuffer(raw[8:8 + 4 * (n + 1)].tobytes(), dtype=np.int32).astype(np.int64)
+            base = 8 + 4 * (n + 1)
+        col = np.frombuffer(raw[base:base + 4 * e].tobytes(), dtype=np.int32)
+        return Graph(int(n), np.ascontiguousarray(off), np.ascontiguousarray(col))
+
+
+class Engine:
+    """One device context (scout_ctx) holding a graph and its maintained solutions."""
+
+    def __init__(self, gpu: int = 0, hbm_bytes: int = 0, pinned_bytes: int = 0):
+        self.lib = load_library()
+        budget = (ctypes.c_int64 * 2)(hbm_bytes, pinned_bytes)
+        h = ctypes.c_void_p()
+        self._check(self.lib.scout_ctx_create(gpu, ctypes.cast(budget, ctypes.c_void_p), ctypes.byref(h)),
+                    "ctx_create")
+        self.h = h
+        self.n = 0
+
+    def _check(self, rc: int, what: str):
+        if rc != OK:
+            msg = self.lib.scout_last_error().decode(errors="replace")
+            raise ScoutError(f"{what} failed ({rc}): {msg}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.scout_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # graph
+    def load_graph(self, g: Graph):
+        off = np.ascontiguousarray(g.off, dtype=np.int64)
+        col = np.ascontiguousarray(g.col, dtype=np.int32)
+        self._check(self.lib.scout_graph_load(self.h, g.n, int(off[-1]), _ptr(off), 1, _ptr(col)), "graph_load")
+        self.n = g.n
+
+    def gen_rmat(self, scale: int, edge_factor: int = 16, a=0.57, b=0.19, c=0.19, seed: int = 1):
+        self._check(self.lib.scout_gen_rmat(self.h, scale, edge_factor, a, b, c, seed), "gen_rmat")
+        self.n = 1 << scale
+
+    def graph_info(self) -> dict:
+        out = np.zeros(8, dtype=np.int64)
+        self._check(self.lib.scout_graph_info(self.h, _ptr(out)), "graph_info")
+        return {"n": int(out[0]), "base_entries": int(out[1]), "inserted_entries": int(out[2]),
+                "logical_entries": int(out[3]), "sorted": bool(out[4])}
+
+    def download_graph(self) -> Graph:
+        off = np.zeros(self.n + 1, dtype=np.int64)
+        self._check(self.lib.scout_graph_download(self.h, _ptr(off), None), "graph_download")
+        col = np.zeros(int(off[-1]), dtype=np.int32)
+        self._check(self.lib.scout_graph_download(self.h, _ptr(off), _ptr(col)), "graph_download")
+        return Graph(self.n, off, col)
+
+    def compact(self):
+        self._check(self.lib.scout_graph_compact(self.h), "graph_compact")
+
+    def degrees(self) -> np.ndarray:
+        d = np.zeros(self.n, dtype=np.int64)
+        self._check(self.lib.scout_graph_degrees(self.h, _ptr(d)), "graph_degrees")
+        return d
+
+    # state
+    def upload(self, kind: int, arr: np.ndarray):
+        dt = np.uint8 if kind == STATE_MIS else np.int32
+        a = np.ascontiguousarray(arr, dtype=dt)
+        self._check(self.lib.scout_state_upload(self.h, kind, _ptr(a), self.n), "state_upload")
+
+    def download(self, kind: int) -> np.ndarray:
+        a = np.zeros(self.n, dtype=np.uint8 if kind == STATE_MIS else np.int32)
+        self._check(self.lib.scout_state_download(self.h, kind, _ptr(a), self.n), "state_download")
+        return a
+
+    # batches
+    def stage(self, uv: np.ndarray, channel: int = 0):
+        a = np.ascontiguousarray(uv, dtype=np.int32).reshape(-1)
+        self._check(self.lib.scout_stage_batch(self.h, channel, _ptr(a), a.size // 2), "stage_batch")
+        self._staged = a.size // 2
+
+    def classify(self, rule: int = CLASSIFY_CURRENT, channel: int = 0) -> np.ndarray:
+        out = np.zeros(max(self._staged, 1), dtype=np.uint8)
+        self._check(self.lib.scout_classify(self.h, channel, rule, _ptr(out)), "classify")
+        return out[: self._staged]
+
+    def apply(self, conditional: bool = False, channel: int = 0):
+        self._check(self.lib.scout_apply(self.h, channel, 1 if conditional else 0), "apply")
+
+    def repair_bfs(self, strategy: int = STRATEGY_LOCAL, channel: int = 0) -> Stats:
+        st = Stats()
+        self._check(self.lib.scout_repair_bfs(self.h, channel, strategy, ctypes.byref(st)), "repair_bfs")
+        return st
+
+    def repair_cc(self, strategy: int = STRATEGY_LOCAL, channel: int = 0) -> Stats:
+        st = Stats()
+        self._check(self.lib.scout_repair_cc(self.h, channel, strategy, ctypes.byref(st)), "repair_cc")
+        return st
+
+    def repair_color(self, channel: int = 0) -> Stats:
+        st = Stats()
+        self._check(self.lib.scout_repair_color(self.h, channel, ctypes.byref(st)), "repair_color")
+        return st
+
+    def repair_mis(self, max_csr: int, channel: int = 0) -> Stats:
+        st = Stats()
+        self._check(self.lib.scout_repair_mis(self.h, channel, max_csr, ctypes.byref(st)), "repair_mis")
+        return st
+
+    def bfs_full(self, source: int = 0):
+        self._check(self.lib.scout_bfs_full(self.h, source), "bfs_full")
+
+    def cc_full(self):
+        self._check(self.lib.scout_cc_full(self.h), "cc_full")
+
+    def mis_greedy(self):
+        self._check(self.lib.scout_mis_greedy(self.h), "mis_greedy")
+
+    def color_greedy(self):
+        self._check(self.lib.scout_color_greedy(self.h), "color_greedy")
+
+    def sync(self):
+        self._check(self.lib.scout_sync(self.h), "sync")
+
+
+def mis_max_csr(avg_degree: float, free_bytes: float) -> int:
+    """MAX_CSR_ELEMENTS of src/mis/scout_mis.cu:800-815."""
+    est = int(1048576.0 * 2.0 * avg_degree * avg_degree)
+    mx = min(800000000, est) if est > 0 else 50000000
+    mx = max(mx, 1000000)
+    fit = int(free_bytes / 4.0 / 8.0)
+    if fit > 1000000 and mx > fit:
+        mx = fit
+    return mx

@@ -1,0 +1,163 @@
+// common.cuh: device helpers shared by the repair kernels (wave64 row walks, CAS-min,
+// work-list appends).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "internal.h"
+
+namespace scout {
+
+constexpr int kWave = 64;
+constexpr int kWavesPerBlock = kBlock / kWave;
+// Rows longer than this are split into chunk tasks of this many entries, each walked by
+// one wave, so a hub's row is spread over many CUs.
+constexpr int64_t kChunk = 4096;
+// Grid of the grid-stride work-list kernels: 8 blocks per CU on 256 CUs.
+constexpr unsigned kListGrid = 2048;
+
+struct Row {
+  int64_t b0, b1, a0, a1;
+};
+
+__device__ __forceinline__ Row row_of(const GView& g, int32_t v) {
+  return Row{g.boff[v], g.boff[v + 1], g.aoff[v], g.aoff[v + 1]};
+}
+__device__ __forceinline__ int64_t row_len(const Row& r) { return (r.b1 - r.b0) + (r.a1 - r.a0); }
+// k-th physical entry of the row (base segment first, then the inserted segment). Negative
+// values are not neighbours (deleted base entry, tombstone, hole).
+__device__ __forceinline__ int32_t row_at(const GView& g, const Row& r, int64_t k) {
+  int64_t nb = r.b1 - r.b0;
+  return k < nb ? g.bcol[r.b0 + k] : g.acol[r.a0 + (k - nb)];
+}
+__device__ __forceinline__ int32_t decode(int32_t x) { return x >= 0 ? x : ~x; }
+
+__device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & (kWave - 1)); }
+__device__ __forceinline__ int64_t global_wave() {
+  return ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+}
+__device__ __forceinline__ int64_t total_waves() { return ((int64_t)gridDim.x * blockDim.x) / kWave; }
+
+// Lowers *p to cand if cand is smaller (-1 counts as infinity). True if this call lowered it.
+__device__ __forceinline__ bool cas_min_level(int32_t* p, int32_t cand) {
+  int32_t old = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  while (old == -1 || old > cand) {
+    int32_t prev = atomicCAS(p, old, cand);
+    if (prev == old) return true;
+    old = prev;
+  }
+  return false;
+}
+
+// Wave-aggregated append: every lane with `take` gets a slot in out[]; one atomic per wave.
+__device__ __forceinline__ void wave_append(bool take, int32_t value, int32_t* out, int64_t* count,
+                                            int64_t cap) {
+  unsigned long long mask = __ballot(take);
+  if (mask == 0) return;
+  int leader = __ffsll((long long)mask) - 1;
+  int64_t base = 0;
+  if (lane_id() == leader) base = (int64_t)atomicAdd((unsigned long long*)count, (unsigned long long)__popcll(mask));
+  base = __shfl(base, leader);
+  if (take) {
+    int64_t rank = __popcll(mask & ((1ULL << lane_id()) - 1ULL));
+    int64_t pos = base + rank;
+    if (pos < cap) out[pos] = value;
+  }
+}
+
+// Per-thread append (when lanes of one wave are not converged on the same list).
+__device__ __forceinline__ void thread_append(int32_t value, int32_t* out, int64_t* count, int64_t cap) {
+  int64_t pos = (int64_t)atomicAdd((unsigned long long*)count, 1ULL);
+  if (pos < cap) out[pos] = value;
+}
+
+// Wave walks the physical entries [lo, hi) of row r; f(x) for every live neighbour x.
+template <class F>
+__device__ __forceinline__ void wave_walk(const GView& g, const Row& r, int64_t lo, int64_t hi, F&& f) {
+  for (int64_t k = lo + lane_id(); k < hi; k += kWave) {
+    int32_t x = row_at(g, r, k);
+    if (x >= 0) f(x);
+  }
+}
+
+// Wave searches entries [lo, hi) of row r for a live neighbour satisfying pred. All lanes
+// return the same answer (the neighbour at the smallest index, or -1).
+template <class P>
+__device__ __forceinline__ int32_t wave_find_first(const GView& g, const Row& r, int64_t lo, int64_t hi, P&& pred) {
+  for (int64_t base = lo; base < hi; base += kWave) {
+    int64_t k = base + lane_id();
+    int32_t x = -1;
+    bool hit = false;
+    if (k < hi) {
+      x = row_at(g, r, k);
+      hit = x >= 0 && pred(x);
+    }
+    unsigned long long mask = __ballot(hit);
+    if (mask) {
+      int first = __ffsll((long long)mask) - 1;
+      return __shfl(x, first);
+    }
+  }
+  return -1;
+}
+
+// Range [lo, hi) of base row u whose entries decode to v (sorted rows: binary search).
+// For unsorted rows returns the whole row and callers compare every entry.
+__device__ __forceinline__ void base_range(const GView& g, bool sorted, int32_t u, int32_t v, int64_t& lo,
+                                           int64_t& hi) {
+  int64_t b = g.boff[u], e = g.boff[u + 1];
+  if (!sorted) {
+    lo = b;
+    hi = e;
+    return;
+  }
+  int64_t l = b, h = e;
+  while (l < h) {
+    int64_t mid = l + ((h - l) >> 1);
+    if (decode(g.bcol[mid]) < v) l = mid + 1;
+    else h = mid;
+  }
+  lo = l;
+  int64_t k = l;
+  while (k < e && decode(g.bcol[k]) == v) k++;
+  hi = k;
+}
+
+struct PairState {
+  bool base_live;     // base row u holds a live v
+  bool base_tomb;     // base row u holds ~v
+  bool base_any;      // base row u holds v or ~v
+  bool added_live;    // inserted row u holds v
+  bool added_tomb;    // inserted row u holds the tombstone ~v
+};
+
+__device__ __forceinline__ PairState pair_state(const GView& g, bool sorted, int32_t u, int32_t v) {
+  PairState s{false, false, false, false, false};
+  int64_t lo, hi;
+  base_range(g, sorted, u, v, lo, hi);
+  for (int64_t k = lo; k < hi; k++) {
+    int32_t x = g.bcol[k];
+    if (x == v) s.base_live = true;
+    else if (x == ~v) s.base_tomb = true;
+  }
+  s.base_any = s.base_live || s.base_tomb;
+  int64_t ab = g.aoff[u], ae = g.aoff[u + 1];
+  for (int64_t k = ab; k < ae; k++) {
+    int32_t x = g.acol[k];
+    if (x == v) s.added_live = true;
+    else if (x == ~v) s.added_tomb = true;
+  }
+  return s;
+}
+
+// HybridGraph::isAdjacent (core/FlashGraph.h:356-366): inserted copy -> true; tombstone ->
+// false; else the base row.
+__device__ __forceinline__ bool adjacent_now(const PairState& s) {
+  if (s.added_live) return true;
+  if (s.base_tomb || s.added_tomb) return false;
+  return s.base_live;
+}
+
+}  // namespace scout

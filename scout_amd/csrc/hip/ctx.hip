@@ -1,0 +1,435 @@
+// ctx.hip: context lifetime, staging channels, solution state and the graph entry points of
+// the C ABI (include/scout_hip.h).
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "common.cuh"
+
+namespace scout {
+
+namespace {
+thread_local std::string g_err;
+}
+
+void set_error(const std::string& msg) { g_err = msg; }
+
+int box_read(Ctx& c, int count) {
+  SCOUT_TRY(hipMemcpyAsync(c.h_box, c.d_box, (size_t)count * sizeof(int64_t), hipMemcpyDeviceToHost, c.s));
+  SCOUT_TRY(hipStreamSynchronize(c.s));
+  return SCOUT_OK;
+}
+
+int next_epoch(Ctx& c) {
+  if (c.epoch >= 0x7ffffff0) {
+    SCOUT_TRY(hipMemsetAsync(c.stamp, 0, (size_t)c.g.n * sizeof(int32_t), c.s));
+    c.epoch = 0;
+  }
+  return ++c.epoch;
+}
+
+namespace {
+
+int alloc_n(Ctx& c, void** p, size_t elem) {
+  if (*p) return SCOUT_OK;
+  size_t bytes = ((size_t)c.g.n + 1) * elem;
+  if (c.budget.hbm_bytes > 0 && c.hbm_used + (int64_t)bytes > c.budget.hbm_bytes) {
+    set_error("HBM budget exceeded by solution state");
+    return SCOUT_ERR_BUDGET;
+  }
+  SCOUT_TRY(hipMalloc(p, bytes));
+  c.hbm_used += (int64_t)bytes;
+  return SCOUT_OK;
+}
+
+void free_state(Ctx& c) {
+  void* ps[] = {c.level, c.label, c.color, c.mis, c.stamp, c.aux, c.aux2,
+                c.bfs_mark, c.cc_owner, c.col_marker, c.mis_minw, c.mis_tag};
+  for (void* p : ps)
+    if (p) (void)hipFree(p);
+  c.level = c.label = c.color = nullptr;
+  c.mis = nullptr;
+  c.stamp = c.aux = c.aux2 = nullptr;
+  c.bfs_mark = c.cc_owner = c.col_marker = c.mis_minw = c.mis_tag = nullptr;
+}
+
+int check_ctx(scout_ctx* x) {
+  if (!x) {
+    set_error("null context");
+    return SCOUT_ERR_ARG;
+  }
+  if (hipSetDevice(x->c.dev) != hipSuccess) {
+    set_error("hipSetDevice failed");
+    return SCOUT_ERR_HIP;
+  }
+  return SCOUT_OK;
+}
+
+int check_graph(scout_ctx* x) {
+  SCOUT_CHECK(check_ctx(x));
+  if (!x->c.g.boff) {
+    set_error("no graph loaded");
+    return SCOUT_ERR_STATE;
+  }
+  return SCOUT_OK;
+}
+
+}  // namespace
+
+int ensure_state(Ctx& c, int kind) {
+  switch (kind) {
+    case SCOUT_STATE_MIS: return alloc_n(c, (void**)&c.mis, 1);
+    case SCOUT_STATE_COLOR: return alloc_n(c, (void**)&c.color, 4);
+    case SCOUT_STATE_LEVEL: return alloc_n(c, (void**)&c.level, 4);
+    case SCOUT_STATE_LABEL: return alloc_n(c, (void**)&c.label, 4);
+  }
+  set_error("unknown state kind");
+  return SCOUT_ERR_ARG;
+}
+
+int ensure_scratch(Ctx& c, int32_t** p, int32_t fill) {
+  if (*p) return SCOUT_OK;
+  SCOUT_CHECK(alloc_n(c, (void**)p, 4));
+  // fill with a repeated byte when possible, else a kernel-free host upload
+  uint8_t b = (uint8_t)(fill & 0xff);
+  if (fill == (int32_t)(b * 0x01010101u)) {
+    SCOUT_TRY(hipMemsetAsync(*p, b, ((size_t)c.g.n + 1) * 4, c.s));
+  } else {
+    std::vector<int32_t> h((size_t)c.g.n + 1, fill);
+    SCOUT_TRY(hipMemcpyAsync(*p, h.data(), h.size() * 4, hipMemcpyHostToDevice, c.s));
+    SCOUT_TRY(hipStreamSynchronize(c.s));
+  }
+  return SCOUT_OK;
+}
+
+int ensure_list(DevBuf<int32_t>& l, int64_t count) { return l.reserve(count); }
+
+int Span::begin(hipStream_t s) {
+  if (!a) {
+    SCOUT_TRY(hipEventCreate(&a));
+    SCOUT_TRY(hipEventCreate(&b));
+  }
+  SCOUT_TRY(hipEventRecord(a, s));
+  return SCOUT_OK;
+}
+int Span::end(hipStream_t s) {
+  SCOUT_TRY(hipEventRecord(b, s));
+  return SCOUT_OK;
+}
+double Span::ms() {
+  float t = 0.f;
+  if (hipEventSynchronize(b) != hipSuccess || hipEventElapsedTime(&t, a, b) != hipSuccess) return 0.0;
+  return (double)t;
+}
+Span::~Span() {
+  if (a) (void)hipEventDestroy(a);
+  if (b) (void)hipEventDestroy(b);
+}
+
+Channel* channel_of(Ctx& c, int ch) {
+  if (ch < 0 || ch > 1) {
+    set_error("channel must be 0 or 1");
+    return nullptr;
+  }
+  return &c.ch[ch];
+}
+
+}  // namespace scout
+
+using namespace scout;
+
+extern "C" {
+
+const char* scout_last_error(void) { return g_err.c_str(); }
+
+int scout_ctx_create(int gpu_id, const scout_budget* budget, scout_ctx** out) {
+  if (!out) {
+    set_error("null output pointer");
+    return SCOUT_ERR_ARG;
+  }
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) {
+    set_error("no HIP device available");
+    return SCOUT_ERR_HIP;
+  }
+  if (gpu_id < 0 || gpu_id >= count) {
+    set_error("gpu id out of range");
+    return SCOUT_ERR_ARG;
+  }
+  SCOUT_TRY(hipSetDevice(gpu_id));
+  SCOUT_TRY(hipFree(nullptr));
+  scout_ctx* x = new scout_ctx();
+  Ctx& c = x->c;
+  c.dev = gpu_id;
+  if (budget) c.budget = *budget;
+  if (hipStreamCreateWithFlags(&c.s, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&c.cs, hipStreamNonBlocking) != hipSuccess ||
+      hipHostMalloc((void**)&c.h_box, 64 * sizeof(int64_t), hipHostMallocDefault) != hipSuccess ||
+      hipMalloc((void**)&c.d_box, 64 * sizeof(int64_t)) != hipSuccess) {
+    set_error("context set-up failed");
+    delete x;
+    return SCOUT_ERR_HIP;
+  }
+  for (int k = 0; k < 2; k++) {
+    if (hipEventCreateWithFlags(&c.ch[k].staged, hipEventDisableTiming) != hipSuccess) {
+      set_error("event creation failed");
+      delete x;
+      return SCOUT_ERR_HIP;
+    }
+  }
+  *out = x;
+  return SCOUT_OK;
+}
+
+int scout_ctx_destroy(scout_ctx* x) {
+  if (!x) return SCOUT_OK;
+  Ctx& c = x->c;
+  (void)hipSetDevice(c.dev);
+  (void)hipDeviceSynchronize();
+  graph_free(c);
+  free_state(c);
+  cc_release(c);
+  mis_release(c);
+  for (int k = 0; k < 2; k++) {
+    c.ch[k].h_uv.release();
+    c.ch[k].d_uv.release();
+    c.ch[k].d_ins.release();
+    c.ch[k].d_done.release();
+    if (c.ch[k].staged) (void)hipEventDestroy(c.ch[k].staged);
+  }
+  DevBuf<int32_t>* ls[] = {&c.la, &c.lb, &c.lc, &c.ld, &c.le};
+  for (auto* l : ls) l->release();
+  c.wa.release();
+  c.wb.release();
+  c.tmp.release();
+  if (c.h_box) (void)hipHostFree(c.h_box);
+  if (c.d_box) (void)hipFree(c.d_box);
+  if (c.s) (void)hipStreamDestroy(c.s);
+  if (c.cs) (void)hipStreamDestroy(c.cs);
+  delete x;
+  return SCOUT_OK;
+}
+
+int scout_device_name(scout_ctx* x, char* buf, int len) {
+  SCOUT_CHECK(check_ctx(x));
+  hipDeviceProp_t prop;
+  SCOUT_TRY(hipGetDeviceProperties(&prop, x->c.dev));
+  if (len > 0) {
+    strncpy(buf, prop.name, (size_t)len - 1);
+    buf[len - 1] = 0;
+  }
+  return SCOUT_OK;
+}
+
+int scout_mem_info(scout_ctx* x, int64_t* free_b, int64_t* total_b) {
+  SCOUT_CHECK(check_ctx(x));
+  size_t f = 0, t = 0;
+  SCOUT_TRY(hipMemGetInfo(&f, &t));
+  if (free_b) *free_b = (int64_t)f;
+  if (total_b) *total_b = (int64_t)t;
+  return SCOUT_OK;
+}
+
+int scout_graph_load(scout_ctx* x, int32_t n, int64_t entries, const void* nindex, int wide, const int32_t* nlist) {
+  SCOUT_CHECK(check_ctx(x));
+  Ctx& c = x->c;
+  if (!nindex || (!nlist && entries > 0)) {
+    set_error("graph_load: null array");
+    return SCOUT_ERR_ARG;
+  }
+  int64_t need = ((int64_t)n + 1) * 16 + entries * 4 + (int64_t)n * 12;
+  if (c.budget.hbm_bytes > 0 && need > c.budget.hbm_bytes) {
+    set_error("graph_load: graph exceeds the HBM budget");
+    return SCOUT_ERR_BUDGET;
+  }
+  free_state(c);
+  c.hbm_used = 0;
+  SCOUT_CHECK(graph_upload_egr(c, n, entries, nindex, wide, nlist));
+  c.hbm_used = need;
+  SCOUT_CHECK(alloc_n(c, (void**)&c.stamp, 4));
+  SCOUT_CHECK(alloc_n(c, (void**)&c.aux, 4));
+  SCOUT_CHECK(alloc_n(c, (void**)&c.aux2, 4));
+  SCOUT_TRY(hipMemsetAsync(c.stamp, 0, ((size_t)n + 1) * 4, c.s));
+  SCOUT_TRY(hipMemsetAsync(c.aux, 0xff, ((size_t)n + 1) * 4, c.s));
+  SCOUT_TRY(hipMemsetAsync(c.aux2, 0xff, ((size_t)n + 1) * 4, c.s));
+  c.epoch = 0;
+  SCOUT_TRY(hipStreamSynchronize(c.s));
+  return SCOUT_OK;
+}
+
+int scout_graph_info(scout_ctx* x, int64_t* out) {
+  SCOUT_CHECK(check_graph(x));
+  Ctx& c = x->c;
+  out[0] = c.g.n;
+  out[1] = c.g.E;
+  out[2] = c.g.A;
+  DevBuf<int64_t> deg, sum;
+  SCOUT_CHECK(deg.reserve((int64_t)c.g.n + 1));
+  SCOUT_CHECK(sum.reserve(1));
+  SCOUT_CHECK(graph_logical_degrees(c, deg.p));
+  SCOUT_CHECK(reduce_sum_i64(c, deg.p, c.g.n, sum.p));
+  int64_t total = 0;
+  SCOUT_TRY(hipMemcpyAsync(&total, sum.p, sizeof(int64_t), hipMemcpyDeviceToHost, c.s));
+  SCOUT_TRY(hipStreamSynchronize(c.s));
+  deg.release();
+  sum.release();
+  out[3] = total;
+  out[4] = c.g.sorted ? 1 : 0;
+  return SCOUT_OK;
+}
+
+int scout_graph_download(scout_ctx* x, int64_t* off, int32_t* col) {
+  SCOUT_CHECK(check_graph(x));
+  return graph_download(x->c, off, col);
+}
+
+int scout_graph_compact(scout_ctx* x) {
+  SCOUT_CHECK(check_graph(x));
+  return graph_compact(x->c);
+}
+
+int scout_graph_degrees(scout_ctx* x, int64_t* deg) {
+  SCOUT_CHECK(check_graph(x));
+  Ctx& c = x->c;
+  DevBuf<int64_t> d;
+  SCOUT_CHECK(d.reserve((int64_t)c.g.n + 1));
+  SCOUT_CHECK(graph_logical_degrees(c, d.p));
+  SCOUT_TRY(hipMemcpyAsync(deg, d.p, (size_t)c.g.n * sizeof(int64_t), hipMemcpyDeviceToHost, c.s));
+  SCOUT_TRY(hipStreamSynchronize(c.s));
+  d.release();
+  return SCOUT_OK;
+}
+
+int scout_state_upload(scout_ctx* x, int kind, const void* host, int64_t n) {
+  SCOUT_CHECK(check_graph(x));
+  Ctx& c = x->c;
+  if (n != c.g.n || !host) {
+    set_error("state_upload: size must equal the vertex count");
+    return SCOUT_ERR_ARG;
+  }
+  SCOUT_CHECK(ensure_state(c, kind));
+  void* dst = kind == SCOUT_STATE_MIS ? (void*)c.mis
+              : kind == SCOUT_STATE_COLOR ? (void*)c.color
+              : kind == SCOUT_STATE_LEVEL ? (void*)c.level
+                                          : (void*)c.label;
+  size_t elem = kind == SCOUT_STATE_MIS ? 1 : 4;
+  SCOUT_TRY(hipMemcpyAsync(dst, host, (size_t)n * elem, hipMemcpyHostToDevice, c.s));
+  SCOUT_TRY(hipStreamSynchronize(c.s));
+  return SCOUT_OK;
+}
+
+int scout_state_download(scout_ctx* x, int kind, void* host, int64_t n) {
+  SCOUT_CHECK(check_graph(x));
+  Ctx& c = x->c;
+  if (n != c.g.n || !host) {
+    set_error("state_download: size must equal the vertex count");
+    return SCOUT_ERR_ARG;
+  }
+  const void* src = kind == SCOUT_STATE_MIS ? (const void*)c.mis
+                    : kind == SCOUT_STATE_COLOR ? (const void*)c.color
+                    : kind == SCOUT_STATE_LEVEL ? (const void*)c.level
+                                                : (const void*)c.label;
+  if (!src) {
+    set_error("state_download: state not initialised");
+    return SCOUT_ERR_STATE;
+  }
+  size_t elem = kind == SCOUT_STATE_MIS ? 1 : 4;
+  SCOUT_TRY(hipMemcpyAsync(host, src, (size_t)n * elem, hipMemcpyDeviceToHost, c.s));
+  SCOUT_TRY(hipStreamSynchronize(c.s));
+  return SCOUT_OK;
+}
+
+int scout_set_source(scout_ctx* x, int32_t source) {
+  SCOUT_CHECK(check_graph(x));
+  if (source < 0 || source >= x->c.g.n) {
+    set_error("source out of range");
+    return SCOUT_ERR_ARG;
+  }
+  x->c.source = source;
+  return SCOUT_OK;
+}
+
+int scout_stage_batch(scout_ctx* x, int channel, const int32_t* uv, int64_t m) {
+  SCOUT_CHECK(check_graph(x));
+  Ctx& c = x->c;
+  Channel* ch = channel_of(c, channel);
+  if (!ch || m < 0 || (m > 0 && !uv)) {
+    if (ch) set_error("stage_batch: bad arguments");
+    return SCOUT_ERR_ARG;
+  }
+  if (m > 0x7fffffffLL) {
+    set_error("stage_batch: more than 2^31-1 lines");
+    return SCOUT_ERR_ARG;
+  }
+  // The previous transfer from this channel's pinned buffer must be complete before refill.
+  SCOUT_TRY(hipEventSynchronize(ch->staged));
+  if (c.budget.pinned_bytes > 0 && 8 * m > c.budget.pinned_bytes) {
+    set_error("stage_batch: batch exceeds the pinned-memory budget");
+    return SCOUT_ERR_BUDGET;
+  }
+  SCOUT_CHECK(ch->h_uv.reserve(2 * m + 2));
+  SCOUT_CHECK(ch->d_uv.reserve(2 * m + 2));
+  SCOUT_CHECK(ch->d_ins.reserve(m + 1));
+  SCOUT_CHECK(ch->d_done.reserve(m + 1));
+  if (m > 0) {
+    memcpy(ch->h_uv.p, uv, (size_t)(2 * m) * sizeof(int32_t));
+    SCOUT_TRY(hipMemcpyAsync(ch->d_uv.p, ch->h_uv.p, (size_t)(2 * m) * sizeof(int32_t), hipMemcpyHostToDevice, c.cs));
+  }
+  SCOUT_TRY(hipEventRecord(ch->staged, c.cs));
+  ch->m = m;
+  return SCOUT_OK;
+}
+
+int scout_stage_batch_device(scout_ctx* x, int channel, const int32_t* d_uv, int64_t m) {
+  SCOUT_CHECK(check_graph(x));
+  Ctx& c = x->c;
+  Channel* ch = channel_of(c, channel);
+  if (!ch || m < 0) return SCOUT_ERR_ARG;
+  SCOUT_TRY(hipEventSynchronize(ch->staged));
+  SCOUT_CHECK(ch->d_uv.reserve(2 * m + 2));
+  SCOUT_CHECK(ch->d_ins.reserve(m + 1));
+  SCOUT_CHECK(ch->d_done.reserve(m + 1));
+  if (m > 0)
+    SCOUT_TRY(hipMemcpyAsync(ch->d_uv.p, d_uv, (size_t)(2 * m) * sizeof(int32_t), hipMemcpyDeviceToDevice, c.cs));
+  SCOUT_TRY(hipEventRecord(ch->staged, c.cs));
+  ch->m = m;
+  return SCOUT_OK;
+}
+
+int scout_classify(scout_ctx* x, int channel, int rule, uint8_t* ins) {
+  SCOUT_CHECK(check_graph(x));
+  Ctx& c = x->c;
+  Channel* ch = channel_of(c, channel);
+  if (!ch) return SCOUT_ERR_ARG;
+  if (rule != SCOUT_CLASSIFY_CURRENT && rule != SCOUT_CLASSIFY_BASE) {
+    set_error("classify: unknown rule");
+    return SCOUT_ERR_ARG;
+  }
+  SCOUT_TRY(hipStreamWaitEvent(c.s, ch->staged, 0));
+  SCOUT_CHECK(graph_classify(c, *ch, rule));
+  if (ins && ch->m > 0) {
+    SCOUT_TRY(hipMemcpyAsync(ins, ch->d_ins.p, (size_t)ch->m, hipMemcpyDeviceToHost, c.s));
+    SCOUT_TRY(hipStreamSynchronize(c.s));
+  }
+  return SCOUT_OK;
+}
+
+int scout_apply(scout_ctx* x, int channel, int conditional) {
+  SCOUT_CHECK(check_graph(x));
+  Ctx& c = x->c;
+  Channel* ch = channel_of(c, channel);
+  if (!ch) return SCOUT_ERR_ARG;
+  SCOUT_TRY(hipStreamWaitEvent(c.s, ch->staged, 0));
+  return graph_apply(c, *ch, conditional);
+}
+
+int scout_sync(scout_ctx* x) {
+  SCOUT_CHECK(check_ctx(x));
+  SCOUT_TRY(hipStreamSynchronize(x->c.s));
+  SCOUT_TRY(hipStreamSynchronize(x->c.cs));
+  return SCOUT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,541 @@
+// graph.hip: the HBM-resident dynamic graph -- upload of the base CSR, batch classification,
+// the scratchpad update rules, insertion-ordered rebuild of the inserted rows, logical degrees,
+// download and compaction.
+//
+// The update rules restate DeltaOverlay (core/FlashGraph.h:200-310) on the encoding described
+// in internal.h:
+//   "v in removed[u]"    <=>  base row u holds ~v, or inserted row u holds the tombstone ~v
+//   "v in added[u]" (k)  <=>  inserted row u holds v (k copies, insertion order)
+// Lines of one batch whose unordered pairs are distinct touch disjoint (row, value) slots, so
+// one thread per line applies them in parallel with the sequential result; a batch that
+// repeats a pair is applied as consecutive runs of distinct pairs.
+#include <omp.h>
+
+#include <algorithm>
+#include <cstring>
+#include <unordered_set>
+
+#include "common.cuh"
+
+namespace scout {
+
+namespace {
+
+__global__ void k_widen_offsets(const int32_t* in, int64_t* out, int64_t count) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) out[i] = (int64_t)in[i];
+}
+
+// Wave per vertex: rows non-decreasing by decoded id, ids in range. flags[0] = unsorted rows,
+// flags[1] = out-of-range entries.
+__global__ void k_check_rows(const int64_t* boff, const int32_t* bcol, int32_t n, int64_t* flags) {
+  for (int64_t v = global_wave(); v < n; v += total_waves()) {
+    int64_t b = boff[v], e = boff[v + 1];
+    bool unsorted = false, bad = false;
+    for (int64_t k = b + lane_id(); k < e; k += kWave) {
+      int32_t x = decode(bcol[k]);
+      if (x < 0 || x >= n) bad = true;
+      if (k + 1 < e && decode(bcol[k + 1]) < x) unsorted = true;
+    }
+    if (__ballot(unsorted) && lane_id() == 0) atomicAdd((unsigned long long*)&flags[0], 1ULL);
+    if (__ballot(bad) && lane_id() == 0) atomicAdd((unsigned long long*)&flags[1], 1ULL);
+  }
+}
+
+__device__ __forceinline__ uint64_t pair_key(int32_t u, int32_t v) {
+  uint32_t a = (uint32_t)(u < v ? u : v), b = (uint32_t)(u < v ? v : u);
+  return ((uint64_t)a << 32) | (uint64_t)b;
+}
+
+__device__ __forceinline__ uint64_t hash64(uint64_t k) {
+  k ^= k >> 33;
+  k *= 0xff51afd7ed558ccdULL;
+  k ^= k >> 33;
+  k *= 0xc4ceb9fe1a85ec53ULL;
+  k ^= k >> 33;
+  return k;
+}
+
+constexpr uint64_t kEmpty = ~0ULL;
+
+__global__ void k_dup_insert(const int32_t* uv, int64_t m, uint64_t* hkey, int32_t* hline, uint64_t mask) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  uint64_t key = pair_key(uv[2 * i], uv[2 * i + 1]);
+  uint64_t h = hash64(key) & mask;
+  while (true) {
+    uint64_t prev = atomicCAS((unsigned long long*)&hkey[h], (unsigned long long)kEmpty, (unsigned long long)key);
+    if (prev == kEmpty || prev == key) {
+      atomicMin(&hline[h], (int32_t)i);
+      return;
+    }
+    h = (h + 1) & mask;
+  }
+}
+
+// prev[i] = 1 if an earlier line of the batch has the same unordered pair.
+__global__ void k_dup_mark(const int32_t* uv, int64_t m, const uint64_t* hkey, const int32_t* hline, uint64_t mask,
+                           uint8_t* prev, int64_t* count) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  uint64_t key = pair_key(uv[2 * i], uv[2 * i + 1]);
+  uint64_t h = hash64(key) & mask;
+  while (hkey[h] != key) h = (h + 1) & mask;
+  bool dup = hline[h] < (int32_t)i;
+  prev[i] = dup ? 1 : 0;
+  if (dup) atomicAdd((unsigned long long*)count, 1ULL);
+}
+
+// ins[i] = 1 iff the pair is absent from the rule's graph. rule 0: current logical graph;
+// rule 1: the loaded base graph (decoded, i.e. before any update).
+__global__ void k_classify(GView g, bool sorted, const int32_t* uv, int64_t m, int rule, uint8_t* ins,
+                           int64_t* counts) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  int32_t u = uv[2 * i], v = uv[2 * i + 1];
+  PairState s = pair_state(g, sorted, u, v);
+  bool adj = rule == SCOUT_CLASSIFY_BASE ? s.base_any : adjacent_now(s);
+  ins[i] = adj ? 0 : 1;
+  if (!adj) atomicAdd((unsigned long long*)&counts[0], 1ULL);
+}
+
+__device__ __forceinline__ void flip_base(const GView& g, bool sorted, int32_t u, int32_t v, bool to_tomb) {
+  int64_t lo, hi;
+  base_range(g, sorted, u, v, lo, hi);
+  int32_t* col = const_cast<int32_t*>(g.bcol);
+  int32_t from = to_tomb ? v : ~v, to = to_tomb ? ~v : v;
+  for (int64_t k = lo; k < hi; k++)
+    if (col[k] == from) col[k] = to;
+}
+
+// First inserted-row entry of u equal to x set to HOLE. True if one was found.
+__device__ __forceinline__ bool hole_first(const GView& g, int32_t u, int32_t x, int64_t* holes) {
+  int32_t* col = const_cast<int32_t*>(g.acol);
+  for (int64_t k = g.aoff[u], e = g.aoff[u + 1]; k < e; k++) {
+    if (col[k] == x) {
+      col[k] = HOLE;
+      atomicAdd((unsigned long long*)holes, 1ULL);
+      return true;
+    }
+  }
+  return false;
+}
+
+__device__ __forceinline__ void push_new(int32_t src, int32_t dst, int64_t line, uint64_t* nkey, int32_t* ndst,
+                                         int64_t* ncount, int64_t cap) {
+  int64_t pos = (int64_t)atomicAdd((unsigned long long*)ncount, 1ULL);
+  if (pos < cap) {
+    nkey[pos] = ((uint64_t)(uint32_t)src << 32) | (uint64_t)line;
+    ndst[pos] = dst;
+  }
+}
+
+// One thread per line of [lo, hi): the DeltaOverlay rules. done[i] = 1 if the line was applied.
+// box[0] = new inserted-row entries, box[1] = holes created.
+__global__ void k_apply(GView g, bool sorted, const int32_t* uv, const uint8_t* ins, int64_t lo, int64_t hi,
+                        int conditional, uint8_t* done, uint64_t* nkey, int32_t* ndst, int64_t ncap, int64_t* box) {
+  int64_t i = lo + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= hi) return;
+  int32_t u = uv[2 * i], v = uv[2 * i + 1];
+  bool insertion = ins[i] != 0;
+  PairState su = pair_state(g, sorted, u, v);
+  if (conditional) {
+    bool adj = adjacent_now(su);
+    if (insertion == adj) {
+      done[i] = 0;
+      return;
+    }
+  }
+  done[i] = 1;
+  if (insertion) {
+    // un_tombstone (FlashGraph.h:284-291), else append to both inserted rows.
+    if (su.base_tomb || su.added_tomb) {
+      if (su.base_tomb) flip_base(g, sorted, u, v, false);
+      if (su.added_tomb) hole_first(g, u, ~v, &box[1]);
+      PairState sv = pair_state(g, sorted, v, u);
+      if (sv.base_tomb) flip_base(g, sorted, v, u, false);
+      if (sv.added_tomb) hole_first(g, v, ~u, &box[1]);
+    } else {
+      push_new(u, v, i, nkey, ndst, &box[0], ncap);
+      push_new(v, u, i, nkey, ndst, &box[0], ncap);
+    }
+  } else {
+    // forget_delta_edge (FlashGraph.h:295-310), else tombstone both directions.
+    if (su.added_live) {
+      hole_first(g, u, v, &box[1]);
+      hole_first(g, v, u, &box[1]);
+    } else {
+      if (su.base_live) flip_base(g, sorted, u, v, true);
+      else if (!su.base_any && !su.added_tomb) push_new(u, ~v, i, nkey, ndst, &box[0], ncap);
+      PairState sv = pair_state(g, sorted, v, u);
+      if (sv.base_live) flip_base(g, sorted, v, u, true);
+      else if (!sv.base_any && !sv.added_tomb) push_new(v, ~u, i, nkey, ndst, &box[0], ncap);
+    }
+  }
+}
+
+// cnt_old[v] = entries of inserted row v that are not holes.
+__global__ void k_count_old(const int64_t* aoff, const int32_t* acol, int32_t n, int32_t* cnt_old, int32_t* cnt) {
+  int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v > n) return;
+  if (v == n) {
+    cnt[n] = 0;
+    return;
+  }
+  int32_t c = 0;
+  for (int64_t k = aoff[v], e = aoff[v + 1]; k < e; k++) c += (acol[k] != HOLE);
+  cnt_old[v] = c;
+  cnt[v] = c;
+}
+
+__global__ void k_count_new(const uint64_t* nkey, int64_t cnt_new, int32_t* cnt) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < cnt_new) atomicAdd(&cnt[(int32_t)(nkey[k] >> 32)], 1);
+}
+
+__global__ void k_copy_old(const int64_t* aoff, const int32_t* acol, int32_t n, const int64_t* aoff2, int32_t* acol2) {
+  int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  int64_t o = aoff2[v];
+  for (int64_t k = aoff[v], e = aoff[v + 1]; k < e; k++) {
+    int32_t x = acol[k];
+    if (x != HOLE) acol2[o++] = x;
+  }
+}
+
+// Sorted new entries go after the surviving old entries, in line order.
+__global__ void k_place_new(const uint64_t* nkey, const int32_t* ndst, int64_t cnt_new, const int64_t* aoff2,
+                            const int32_t* cnt_old, int32_t* acol2) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= cnt_new) return;
+  uint64_t key = nkey[k];
+  int32_t src = (int32_t)(key >> 32);
+  uint64_t first = (uint64_t)(uint32_t)src << 32;
+  int64_t l = 0, h = k;
+  while (l < h) {
+    int64_t mid = (l + h) >> 1;
+    if (nkey[mid] < first) l = mid + 1;
+    else h = mid;
+  }
+  acol2[aoff2[src] + cnt_old[src] + (k - l)] = ndst[k];
+}
+
+// Live entries per vertex of the logical graph (wave per vertex).
+__global__ void k_logical_degree(GView g, int64_t* deg) {
+  for (int64_t v = global_wave(); v < g.n; v += total_waves()) {
+    Row r = row_of(g, (int32_t)v);
+    int64_t c = 0;
+    for (int64_t k = lane_id(); k < row_len(r); k += kWave) c += (row_at(g, r, k) >= 0);
+    for (int off = kWave / 2; off > 0; off >>= 1) c += __shfl_xor(c, off);
+    if (lane_id() == 0) deg[v] = c;
+  }
+}
+
+// Live entries of each row copied to the compacted CSR (wave per vertex, order kept).
+__global__ void k_gather_live(GView g, const int64_t* off2, int32_t* col2) {
+  for (int64_t v = global_wave(); v < g.n; v += total_waves()) {
+    Row r = row_of(g, (int32_t)v);
+    int64_t len = row_len(r);
+    int64_t out = off2[v];
+    for (int64_t base = 0; base < len; base += kWave) {
+      int64_t k = base + lane_id();
+      int32_t x = k < len ? row_at(g, r, k) : -1;
+      bool live = x >= 0;
+      unsigned long long mask = __ballot(live);
+      if (live) col2[out + __popcll(mask & ((1ULL << lane_id()) - 1ULL))] = x;
+      out += __popcll(mask);
+    }
+  }
+}
+
+int upload_chunked(Ctx& c, void* dst, const void* src, size_t bytes) {
+  // Stream a (possibly mmap'd, pageable) host array through two pinned buffers on the copy
+  // stream; filling buffer k overlaps the transfer of buffer k^1.
+  const size_t CH = (size_t)64 << 20;
+  if (bytes == 0) return SCOUT_OK;
+  char* pin[2] = {nullptr, nullptr};
+  hipEvent_t ev[2];
+  SCOUT_TRY(hipHostMalloc((void**)&pin[0], CH, hipHostMallocDefault));
+  SCOUT_TRY(hipHostMalloc((void**)&pin[1], CH, hipHostMallocDefault));
+  SCOUT_TRY(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+  SCOUT_TRY(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+  int rc = SCOUT_OK;
+  size_t done = 0;
+  int k = 0;
+  bool used[2] = {false, false};
+  while (done < bytes) {
+    size_t len = std::min(CH, bytes - done);
+    if (used[k]) {
+      if (hipEventSynchronize(ev[k]) != hipSuccess) {
+        rc = SCOUT_ERR_HIP;
+        break;
+      }
+    }
+    const char* s = (const char*)src + done;
+    int nt = omp_get_max_threads();
+    size_t per = (len + (size_t)nt - 1) / (size_t)nt;
+#pragma omp parallel for schedule(static)
+    for (int t = 0; t < nt; t++) {
+      size_t a = (size_t)t * per;
+      if (a < len) memcpy(pin[k] + a, s + a, std::min(per, len - a));
+    }
+    if (hipMemcpyAsync((char*)dst + done, pin[k], len, hipMemcpyHostToDevice, c.cs) != hipSuccess ||
+        hipEventRecord(ev[k], c.cs) != hipSuccess) {
+      rc = SCOUT_ERR_HIP;
+      break;
+    }
+    used[k] = true;
+    done += len;
+    k ^= 1;
+  }
+  (void)hipStreamSynchronize(c.cs);
+  (void)hipEventDestroy(ev[0]);
+  (void)hipEventDestroy(ev[1]);
+  (void)hipHostFree(pin[0]);
+  (void)hipHostFree(pin[1]);
+  if (rc != SCOUT_OK) set_error("upload_chunked: HIP copy failed");
+  return rc;
+}
+
+int rebuild_inserted(Ctx& c, int64_t cnt_new) {
+  DevGraph& g = c.g;
+  int32_t n = g.n;
+  if (cnt_new > 0) {
+    // Stable by (src, line): the new entries of a row keep the batch's line order.
+    SCOUT_CHECK(g.nkey2.reserve(cnt_new));
+    SCOUT_CHECK(g.ndst2.reserve(cnt_new));
+    SCOUT_CHECK(sort_pairs_u64(c, g.nkey.p, g.nkey2.p, g.ndst.p, g.ndst2.p, cnt_new, 64));
+    std::swap(g.nkey, g.nkey2);
+    std::swap(g.ndst, g.ndst2);
+  }
+  SCOUT_CHECK(g.cnt.reserve((int64_t)n + 1));
+  SCOUT_CHECK(g.cnt_old.reserve((int64_t)n + 1));
+  k_count_old<<<grid_for((int64_t)n + 1), kBlock, 0, c.s>>>(g.aoff.p, g.acol.p, n, g.cnt_old.p, g.cnt.p);
+  if (cnt_new > 0) k_count_new<<<grid_for(cnt_new), kBlock, 0, c.s>>>(g.nkey.p, cnt_new, g.cnt.p);
+  SCOUT_CHECK(g.aoff2.reserve((int64_t)n + 1));
+  SCOUT_CHECK(scan_exclusive_i32_to_i64(c, g.cnt.p, g.aoff2.p, (int64_t)n + 1));
+  int64_t total = 0;
+  SCOUT_TRY(hipMemcpyAsync(&total, g.aoff2.p + n, sizeof(int64_t), hipMemcpyDeviceToHost, c.s));
+  SCOUT_TRY(hipStreamSynchronize(c.s));
+  SCOUT_CHECK(g.acol2.reserve(total + 1));
+  k_copy_old<<<grid_for(n), kBlock, 0, c.s>>>(g.aoff.p, g.acol.p, n, g.aoff2.p, g.acol2.p);
+  if (cnt_new > 0)
+    k_place_new<<<grid_for(cnt_new), kBlock, 0, c.s>>>(g.nkey.p, g.ndst.p, cnt_new, g.aoff2.p, g.cnt_old.p, g.acol2.p);
+  std::swap(g.aoff, g.aoff2);
+  std::swap(g.acol, g.acol2);
+  g.A = total;
+  g.holes = 0;
+  return SCOUT_OK;
+}
+
+int apply_range(Ctx& c, Channel& ch, int64_t lo, int64_t hi, int conditional) {
+  DevGraph& g = c.g;
+  int64_t cap = 4 * (hi - lo) + 16;
+  SCOUT_CHECK(g.nkey.reserve(cap));
+  SCOUT_CHECK(g.ndst.reserve(cap));
+  SCOUT_TRY(hipMemsetAsync(c.d_box, 0, 4 * sizeof(int64_t), c.s));
+  if (hi > lo)
+    k_apply<<<grid_for(hi - lo), kBlock, 0, c.s>>>(g.view(), g.sorted, ch.d_uv.p, ch.d_ins.p, lo, hi, conditional,
+                                                   ch.d_done.p, g.nkey.p, g.ndst.p, g.nkey.cap, c.d_box);
+  SCOUT_CHECK(box_read(c, 2));
+  int64_t cnt_new = c.h_box[0];
+  g.holes += c.h_box[1];
+  if (cnt_new > g.nkey.cap) {
+    set_error("apply: inserted-entry staging overflow");
+    return SCOUT_ERR_STATE;
+  }
+  if (cnt_new > 0 || g.holes > (g.A >> 1) + 1024) SCOUT_CHECK(rebuild_inserted(c, cnt_new));
+  return SCOUT_OK;
+}
+
+}  // namespace
+
+int graph_free(Ctx& c) {
+  DevGraph& g = c.g;
+  if (g.boff) (void)hipFree(g.boff);
+  if (g.bcol) (void)hipFree(g.bcol);
+  g.boff = nullptr;
+  g.bcol = nullptr;
+  g.aoff.release();
+  g.aoff2.release();
+  g.acol.release();
+  g.acol2.release();
+  g.nkey.release();
+  g.nkey2.release();
+  g.ndst.release();
+  g.ndst2.release();
+  g.cnt.release();
+  g.cnt_old.release();
+  g.hkey.release();
+  g.hline.release();
+  g.n = 0;
+  g.E = g.A = g.holes = 0;
+  return SCOUT_OK;
+}
+
+int check_rows(Ctx& c) {
+  DevGraph& g = c.g;
+  SCOUT_TRY(hipMemsetAsync(c.d_box, 0, 2 * sizeof(int64_t), c.s));
+  if (g.n > 0) k_check_rows<<<kListGrid, kBlock, 0, c.s>>>(g.boff, g.bcol, g.n, c.d_box);
+  SCOUT_CHECK(box_read(c, 2));
+  if (c.h_box[1] != 0) {
+    set_error("graph has adjacency entries outside [0, n)");
+    return SCOUT_ERR_ARG;
+  }
+  g.sorted = c.h_box[0] == 0;
+  return SCOUT_OK;
+}
+
+int graph_upload_egr(Ctx& c, int32_t n, int64_t E, const void* nindex, int wide, const int32_t* nlist) {
+  if (n < 0 || E < 0) {
+    set_error("graph_load: negative size");
+    return SCOUT_ERR_ARG;
+  }
+  graph_free(c);
+  DevGraph& g = c.g;
+  g.n = n;
+  g.E = E;
+  SCOUT_TRY(hipMalloc((void**)&g.boff, ((size_t)n + 1) * sizeof(int64_t)));
+  SCOUT_TRY(hipMalloc((void**)&g.bcol, (size_t)std::max<int64_t>(E, 1) * sizeof(int32_t)));
+  if (wide) {
+    SCOUT_CHECK(upload_chunked(c, g.boff, nindex, ((size_t)n + 1) * sizeof(int64_t)));
+  } else {
+    int32_t* tmp32 = nullptr;
+    SCOUT_TRY(hipMalloc((void**)&tmp32, ((size_t)n + 1) * sizeof(int32_t)));
+    int rc = upload_chunked(c, tmp32, nindex, ((size_t)n + 1) * sizeof(int32_t));
+    if (rc == SCOUT_OK) {
+      k_widen_offsets<<<grid_for((int64_t)n + 1), kBlock, 0, c.s>>>(tmp32, g.boff, (int64_t)n + 1);
+      (void)hipStreamSynchronize(c.s);
+    }
+    (void)hipFree(tmp32);
+    SCOUT_CHECK(rc);
+  }
+  SCOUT_CHECK(upload_chunked(c, g.bcol, nlist, (size_t)E * sizeof(int32_t)));
+  int64_t first = 0, last = 0;
+  SCOUT_TRY(hipMemcpy(&first, g.boff, sizeof(int64_t), hipMemcpyDeviceToHost));
+  SCOUT_TRY(hipMemcpy(&last, g.boff + n, sizeof(int64_t), hipMemcpyDeviceToHost));
+  if (first != 0 || last != E) {
+    set_error("graph_load: nindex[0] must be 0 and nindex[n] the entry count");
+    return SCOUT_ERR_ARG;
+  }
+  SCOUT_CHECK(g.aoff.reserve((int64_t)n + 1));
+  SCOUT_TRY(hipMemsetAsync(g.aoff.p, 0, ((size_t)n + 1) * sizeof(int64_t), c.s));
+  SCOUT_CHECK(g.acol.reserve(1024));
+  g.A = 0;
+  g.holes = 0;
+  return check_rows(c);
+}
+
+int graph_upload(Ctx& c, int32_t n, int64_t E, const int64_t* off, const int32_t* col) {
+  return graph_upload_egr(c, n, E, off, 1, col);
+}
+
+int graph_classify(Ctx& c, Channel& ch, int rule) {
+  if (!c.g.boff) {
+    set_error("classify: no graph loaded");
+    return SCOUT_ERR_STATE;
+  }
+  SCOUT_TRY(hipMemsetAsync(c.d_box, 0, sizeof(int64_t), c.s));
+  if (ch.m > 0)
+    k_classify<<<grid_for(ch.m), kBlock, 0, c.s>>>(c.g.view(), c.g.sorted, ch.d_uv.p, ch.m, rule, ch.d_ins.p,
+                                                   c.d_box);
+  return SCOUT_OK;
+}
+
+int graph_apply(Ctx& c, Channel& ch, int conditional) {
+  DevGraph& g = c.g;
+  int64_t m = ch.m;
+  if (m == 0) return SCOUT_OK;
+  // Find lines whose pair already occurred earlier in the batch.
+  uint64_t size = 1;
+  while (size < (uint64_t)(4 * m)) size <<= 1;
+  SCOUT_CHECK(g.hkey.reserve((int64_t)size));
+  SCOUT_CHECK(g.hline.reserve((int64_t)size));
+  SCOUT_TRY(hipMemsetAsync(g.hkey.p, 0xff, size * sizeof(uint64_t), c.s));
+  SCOUT_TRY(hipMemsetAsync(g.hline.p, 0x7f, size * sizeof(int32_t), c.s));
+  SCOUT_TRY(hipMemsetAsync(c.d_box, 0, sizeof(int64_t), c.s));
+  k_dup_insert<<<grid_for(m), kBlock, 0, c.s>>>(ch.d_uv.p, m, g.hkey.p, g.hline.p, size - 1);
+  k_dup_mark<<<grid_for(m), kBlock, 0, c.s>>>(ch.d_uv.p, m, g.hkey.p, g.hline.p, size - 1, ch.d_done.p, c.d_box);
+  SCOUT_CHECK(box_read(c, 1));
+  if (c.h_box[0] == 0) return apply_range(c, ch, 0, m, conditional);
+  // Repeated pairs: cut the batch into runs of distinct pairs, applied in order.
+  std::vector<int32_t> uv((size_t)(2 * m));
+  SCOUT_TRY(hipMemcpy(uv.data(), ch.d_uv.p, (size_t)(2 * m) * sizeof(int32_t), hipMemcpyDeviceToHost));
+  std::unordered_set<uint64_t> seen;
+  int64_t start = 0;
+  for (int64_t i = 0; i < m; i++) {
+    uint32_t a = (uint32_t)std::min(uv[2 * i], uv[2 * i + 1]), b = (uint32_t)std::max(uv[2 * i], uv[2 * i + 1]);
+    uint64_t key = ((uint64_t)a << 32) | b;
+    if (seen.count(key)) {
+      SCOUT_CHECK(apply_range(c, ch, start, i, conditional));
+      seen.clear();
+      start = i;
+    }
+    seen.insert(key);
+  }
+  return apply_range(c, ch, start, m, conditional);
+}
+
+int graph_logical_degrees(Ctx& c, int64_t* d_deg) {
+  if (c.g.n > 0) k_logical_degree<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), d_deg);
+  return SCOUT_OK;
+}
+
+// Logical CSR into device buffers off2 (n+1) and col2; returns the entry count in *total.
+int logical_csr_device(Ctx& c, DevBuf<int64_t>& off2, DevBuf<int32_t>& col2, int64_t* total) {
+  DevGraph& g = c.g;
+  int32_t n = g.n;
+  DevBuf<int64_t> deg;
+  SCOUT_CHECK(deg.reserve((int64_t)n + 1));
+  SCOUT_TRY(hipMemsetAsync(deg.p + n, 0, sizeof(int64_t), c.s));
+  SCOUT_CHECK(graph_logical_degrees(c, deg.p));
+  SCOUT_CHECK(off2.reserve((int64_t)n + 1));
+  SCOUT_CHECK(scan_exclusive_i64(c, deg.p, off2.p, (int64_t)n + 1));
+  SCOUT_TRY(hipMemcpyAsync(total, off2.p + n, sizeof(int64_t), hipMemcpyDeviceToHost, c.s));
+  SCOUT_TRY(hipStreamSynchronize(c.s));
+  deg.release();
+  SCOUT_CHECK(col2.reserve(*total + 1));
+  if (n > 0) k_gather_live<<<kListGrid, kBlock, 0, c.s>>>(g.view(), off2.p, col2.p);
+  return SCOUT_OK;
+}
+
+int graph_download(Ctx& c, int64_t* h_off, int32_t* h_col) {
+  DevBuf<int64_t> off2;
+  DevBuf<int32_t> col2;
+  int64_t total = 0;
+  int rc = logical_csr_device(c, off2, col2, &total);
+  if (rc == SCOUT_OK) {
+    if (hipMemcpyAsync(h_off, off2.p, ((size_t)c.g.n + 1) * sizeof(int64_t), hipMemcpyDeviceToHost, c.s) != hipSuccess)
+      rc = SCOUT_ERR_HIP;
+    if (rc == SCOUT_OK && h_col && total > 0 &&
+        hipMemcpyAsync(h_col, col2.p, (size_t)total * sizeof(int32_t), hipMemcpyDeviceToHost, c.s) != hipSuccess)
+      rc = SCOUT_ERR_HIP;
+    if (hipStreamSynchronize(c.s) != hipSuccess) rc = SCOUT_ERR_HIP;
+  }
+  off2.release();
+  col2.release();
+  if (rc == SCOUT_ERR_HIP) set_error("graph_download: copy failed");
+  return rc;
+}
+
+int graph_compact(Ctx& c) {
+  DevGraph& g = c.g;
+  DevBuf<int64_t> off2;
+  DevBuf<int32_t> col2;
+  int64_t total = 0;
+  SCOUT_CHECK(logical_csr_device(c, off2, col2, &total));
+  SCOUT_TRY(hipStreamSynchronize(c.s));
+  (void)hipFree(g.boff);
+  (void)hipFree(g.bcol);
+  g.boff = off2.p;
+  g.bcol = col2.p;
+  off2.p = nullptr;
+  col2.p = nullptr;
+  g.E = total;
+  SCOUT_TRY(hipMemsetAsync(g.aoff.p, 0, ((size_t)g.n + 1) * sizeof(int64_t), c.s));
+  g.A = 0;
+  g.holes = 0;
+  return check_rows(c);
+}
+
+}  // namespace scout

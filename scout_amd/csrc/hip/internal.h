@@ -1,0 +1,229 @@
+// internal.h: shared declarations of libscout_hip (not part of the public C ABI).
+//
+// HBM layout of the dynamic graph (one copy per context):
+//   boff[n+1] int64, bcol[E] int32   base CSR; a deleted base entry keeps its slot and is
+//                                    stored as ~v (negative), so rows never move
+//   aoff[n+1] int64, acol[A] int32   inserted edges per vertex in insertion order; an entry
+//                                    is v (live), ~v (a tombstone on a pair the base row does
+//                                    not hold) or HOLE (a forgotten insertion)
+// The logical row of v is the live entries of its base row, then the live entries of its
+// inserted row -- the order of core/FlashGraph.h:246-265.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../../include/scout_hip.h"
+
+namespace scout {
+
+constexpr int32_t HOLE = INT32_MIN;
+constexpr int kBlock = 256;
+
+#define SCOUT_TRY(expr)                                                              \
+  do {                                                                               \
+    hipError_t _e = (expr);                                                          \
+    if (_e != hipSuccess) {                                                          \
+      ::scout::set_error(std::string(#expr) + ": " + hipGetErrorString(_e));       \
+      return SCOUT_ERR_HIP;                                                          \
+    }                                                                                \
+  } while (0)
+
+#define SCOUT_CHECK(status)              \
+  do {                                   \
+    int _s = (status);                   \
+    if (_s != SCOUT_OK) return _s;       \
+  } while (0)
+
+void set_error(const std::string& msg);
+
+inline unsigned grid_for(int64_t items, int per_block = kBlock) {
+  int64_t g = (items + per_block - 1) / per_block;
+  if (g < 1) g = 1;
+  if (g > 0x7fffffff) g = 0x7fffffff;
+  return (unsigned)g;
+}
+
+// Device buffer with capacity growth (contents are not preserved on growth).
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  int64_t cap = 0;
+  int reserve(int64_t count) {
+    if (count <= cap) return SCOUT_OK;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+    int64_t want = count + count / 4 + 1024;
+    SCOUT_TRY(hipMalloc((void**)&p, (size_t)want * sizeof(T)));
+    cap = want;
+    return SCOUT_OK;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+};
+
+// Pinned host buffer with capacity growth.
+template <typename T>
+struct PinBuf {
+  T* p = nullptr;
+  int64_t cap = 0;
+  int reserve(int64_t count) {
+    if (count <= cap) return SCOUT_OK;
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    cap = 0;
+    int64_t want = count + count / 4 + 1024;
+    SCOUT_TRY(hipHostMalloc((void**)&p, (size_t)want * sizeof(T), hipHostMallocDefault));
+    cap = want;
+    return SCOUT_OK;
+  }
+  void release() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+};
+
+// Read-only view of the logical graph for kernels.
+struct GView {
+  const int64_t* boff;
+  const int32_t* bcol;
+  const int64_t* aoff;
+  const int32_t* acol;
+  int32_t n;
+};
+
+struct DevGraph {
+  int32_t n = 0;
+  int64_t E = 0;        // base entries
+  int64_t A = 0;        // inserted-row entries (aoff[n]), holes included
+  int64_t holes = 0;    // HOLE entries not yet compacted away
+  bool sorted = false;  // every base row ascending (by decoded id)
+  int64_t* boff = nullptr;
+  int32_t* bcol = nullptr;
+  DevBuf<int64_t> aoff, aoff2;
+  DevBuf<int32_t> acol, acol2;
+  // new inserted-row entries of the current batch: key = src << 32 | line
+  DevBuf<uint64_t> nkey, nkey2;
+  DevBuf<int32_t> ndst, ndst2;
+  DevBuf<int32_t> cnt, cnt_old;  // n+1 per-vertex counts
+  // duplicate-pair detection of the current batch
+  DevBuf<uint64_t> hkey;
+  DevBuf<int32_t> hline;
+  GView view() const { return GView{boff, bcol, aoff.p, acol.p, n}; }
+};
+
+// Per-batch edit staging: a channel holds one uploaded batch.
+struct Channel {
+  PinBuf<int32_t> h_uv;
+  DevBuf<int32_t> d_uv;
+  DevBuf<uint8_t> d_ins;    // 1 = insertion label
+  DevBuf<uint8_t> d_done;   // 1 = the edit changed the graph
+  int64_t m = 0;
+  hipEvent_t staged = nullptr;
+};
+
+struct MlpState;  // mlp.hip
+struct CcState;   // cc.hip
+struct MisState;  // mis.hip
+
+struct Ctx {
+  int dev = 0;
+  hipStream_t s = nullptr;     // compute stream
+  hipStream_t cs = nullptr;    // copy stream (fetch / staging)
+  DevGraph g;
+  Channel ch[2];
+  // maintained solutions
+  int32_t* level = nullptr;    // BFS levels, -1 unreached
+  int32_t* label = nullptr;    // CC min-id labels
+  int32_t* color = nullptr;    // colours
+  uint8_t* mis = nullptr;      // MIS membership
+  int32_t source = 0;
+  // n-sized scratch
+  int32_t* stamp = nullptr;    // epoch stamps (dedup of frontiers)
+  int32_t* aux = nullptr;      // per-problem scratch (owner / marker / minWriter)
+  int32_t* aux2 = nullptr;
+  int32_t epoch = 0;
+  // problem-specific n-sized scratch (allocated on first use)
+  int32_t* bfs_mark = nullptr;   // invalidation stamps
+  int32_t* cc_owner = nullptr;   // search group of each claimed vertex (-1 = none)
+  int32_t* col_marker = nullptr; // this round's proposal (-2 = none)
+  int32_t* mis_minw = nullptr;   // smallest speculative writer line
+  int32_t* mis_tag = nullptr;    // row kind stamps
+  // small mailbox read back by the host after a round
+  int64_t* h_box = nullptr;    // pinned
+  int64_t* d_box = nullptr;
+  // generic work lists
+  DevBuf<int32_t> la, lb, lc, ld, le;
+  DevBuf<int64_t> wa, wb;
+  DevBuf<uint8_t> tmp;         // library temp storage (scan/sort)
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> spans;  // timing
+  MlpState* mlp = nullptr;
+  CcState* cc = nullptr;
+  MisState* mis_state = nullptr;
+  scout_budget budget{};
+  int64_t hbm_used = 0;
+  int64_t pinned_used = 0;
+  // last batch's kernel timings
+  float last_repair_ms = 0.f;
+};
+
+// prims.hip (library scans/sorts)
+int scan_exclusive_i64(Ctx& c, const int64_t* in, int64_t* out, int64_t count);
+int scan_exclusive_i32_to_i64(Ctx& c, const int32_t* in, int64_t* out, int64_t count);
+int sort_pairs_u64(Ctx& c, const uint64_t* kin, uint64_t* kout, const int32_t* vin, int32_t* vout, int64_t count,
+                   int end_bit);
+int sort_keys_u64(Ctx& c, const uint64_t* kin, uint64_t* kout, int64_t count, int begin_bit, int end_bit);
+int reduce_max_i32(Ctx& c, const int32_t* in, int64_t count, int32_t* d_out);
+int reduce_sum_i64(Ctx& c, const int64_t* in, int64_t count, int64_t* d_out);
+
+// graph.hip
+int graph_upload(Ctx& c, int32_t n, int64_t E, const int64_t* off, const int32_t* col);
+int graph_upload_egr(Ctx& c, int32_t n, int64_t E, const void* nindex, int wide, const int32_t* nlist);
+int graph_classify(Ctx& c, Channel& ch, int rule);
+int graph_apply(Ctx& c, Channel& ch, int conditional);
+int graph_logical_degrees(Ctx& c, int64_t* d_deg);
+int graph_download(Ctx& c, int64_t* h_off, int32_t* h_col);
+int graph_compact(Ctx& c);
+int graph_free(Ctx& c);
+
+// per-problem modules
+int bfs_full(Ctx& c, int32_t src, int64_t* rounds, int64_t* scanned);
+int bfs_local(Ctx& c, Channel& ch, scout_stats* st);
+int cc_full(Ctx& c, int64_t* scanned);
+int cc_local(Ctx& c, Channel& ch, scout_stats* st);
+void cc_release(Ctx& c);
+void mis_release(Ctx& c);
+
+// helpers in ctx.hip
+int box_read(Ctx& c, int count);  // copies d_box[0..count) into h_box, synchronises c.s
+int next_epoch(Ctx& c);
+int ensure_state(Ctx& c, int kind);
+// n-sized int32 scratch filled with `fill` on first allocation
+int ensure_scratch(Ctx& c, int32_t** p, int32_t fill);
+Channel* channel_of(Ctx& c, int ch);
+int ensure_list(DevBuf<int32_t>& l, int64_t count);
+
+// Device time between two events recorded around a repair.
+struct Span {
+  hipEvent_t a = nullptr, b = nullptr;
+  int begin(hipStream_t s);
+  int end(hipStream_t s);
+  double ms();
+  ~Span();
+};
+
+}  // namespace scout
+
+// The opaque handle of the C ABI.
+struct scout_ctx {
+  scout::Ctx c;
+};

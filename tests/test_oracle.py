@@ -1,0 +1,304 @@
+"""Pins the CPU oracle (oracle/oracle.cpp) before any GPU result is compared with it.
+
+Every oracle function is checked against something it does not compute itself: the
+reference's golden fixtures (datasets/sample), textbook library routines (scipy.sparse.csgraph),
+hand-worked cases of the reference's rules, or invariants that a plausible mistake breaks.
+"""
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+from scipy.sparse.csgraph import connected_components, shortest_path
+
+from oracle_lib import OracleGraph, mis_max_csr, read_batch
+
+
+def csr_from_edges(n, edges):
+    """Symmetric, sorted, duplicate-free CSR from undirected edges."""
+    keys = set()
+    for u, v in edges:
+        if u != v:
+            keys.add((u, v))
+            keys.add((v, u))
+    keys = sorted(keys)
+    off = np.zeros(n + 1, dtype=np.int64)
+    for u, _ in keys:
+        off[u + 1] += 1
+    off = np.cumsum(off)
+    col = np.array([v for _, v in keys], dtype=np.int32)
+    return off, col
+
+
+def random_graph(n, m, rng):
+    edges = set()
+    while len(edges) < m:
+        u, v = rng.integers(0, n, 2)
+        if u != v:
+            edges.add((min(u, v), max(u, v)))
+    return sorted(edges)
+
+
+def scipy_graph(g: OracleGraph):
+    """Adjacency of the logical rows (a batch that repeats a pair can leave a tombstone on a
+    pair the base never held, and then the reference's degree formula, used by its
+    compactor, no longer matches the row; the rows themselves are the graph)."""
+    rows = [g.row(v) for v in range(g.n)]
+    off = np.concatenate([[0], np.cumsum([r.size for r in rows])]).astype(np.int64)
+    col = np.concatenate(rows).astype(np.int32) if off[-1] else np.zeros(0, np.int32)
+    return sp.csr_matrix((np.ones(col.size), col, off), shape=(g.n, g.n))
+
+
+# --- golden fixtures (datasets/sample) ------------------------------------------------
+
+def test_sample_fixtures_are_closed_forms(sample_dir):
+    g = OracleGraph(os.path.join(sample_dir, "sample.egr"))
+    assert g.n == 2000
+    ref = np.loadtxt(os.path.join(sample_dir, "sample_bfs.txt"), dtype=np.int64)
+    want = np.full(g.n, -1, dtype=np.int32)
+    want[ref[:, 0] - 1] = ref[:, 1]
+    assert np.array_equal(g.bfs(0), want)
+    assert np.array_equal(g.cc(), np.loadtxt(os.path.join(sample_dir, "sample_components.txt"), dtype=np.int32))
+    mis = np.zeros(g.n, dtype=np.uint8)
+    mis[np.loadtxt(os.path.join(sample_dir, "sample_mis.txt"), dtype=np.int64) - 1] = 1
+    assert np.array_equal(g.greedy_mis(), mis)
+    colors = np.loadtxt(os.path.join(sample_dir, "sample_colors.txt"), dtype=np.int32)
+    assert g.check_coloring(colors) == 0  # ECL-GC output: proper, 6 colours
+    assert colors.max() + 1 == 6
+
+
+def test_sample_batches_are_deletions_of_base_edges(sample_dir):
+    g = OracleGraph(os.path.join(sample_dir, "sample.egr"))
+    for k in range(1, 6):
+        uv = read_batch(os.path.join(sample_dir, "batches", f"{k}.mtx"), g.n)
+        assert uv.shape == (50, 2)
+        ins = g.classify(uv)
+        assert not ins.any()
+        g.apply(uv, ins)
+        for u, v in uv:
+            assert not g.is_adjacent(u, v) and not g.is_adjacent(v, u)
+
+
+# --- closed forms vs library routines ---------------------------------------------------
+
+@pytest.mark.parametrize("seed", range(6))
+def test_bfs_and_cc_match_scipy_after_random_updates(seed):
+    rng = np.random.default_rng(seed)
+    n = 60
+    off, col = csr_from_edges(n, random_graph(n, 70, rng))
+    g = OracleGraph(n=n, off=off, col=col)
+    for _ in range(4):
+        uv = rng.integers(0, n, (25, 2)).astype(np.int32)
+        uv = uv[uv[:, 0] != uv[:, 1]]
+        g.apply(uv, g.classify(uv))
+        a = scipy_graph(g)
+        d = shortest_path(a, unweighted=True, indices=0)
+        want = np.where(np.isinf(d), -1, d).astype(np.int32)
+        assert np.array_equal(g.bfs(0), want)
+        ncomp, lab = connected_components(a, directed=False)
+        mins = np.full(ncomp, n, dtype=np.int64)
+        np.minimum.at(mins, lab, np.arange(n))
+        assert np.array_equal(g.cc(), mins[lab].astype(np.int32))
+        assert g.check_bfs(g.bfs(0)) == 0
+
+
+def test_checkers_catch_planted_mistakes():
+    rng = np.random.default_rng(7)
+    n = 80
+    off, col = csr_from_edges(n, random_graph(n, 160, rng))
+    g = OracleGraph(n=n, off=off, col=col)
+    lv = g.bfs(0)
+    bad = lv.copy()
+    k = int(np.flatnonzero(lv > 1)[0])
+    bad[k] += 1                      # an off-by-one level
+    assert g.check_bfs(bad) > 0
+    bad = lv.copy()
+    bad[k] -= 1
+    assert g.check_bfs(bad) > 0
+    lab = g.cc()
+    bad = lab.copy()
+    bad[n - 1] = n - 1 if lab[n - 1] != n - 1 else 0
+    assert g.check_cc(bad) > 0
+    colors = g.greedy_color()
+    assert g.check_coloring(colors) == 0
+    u = int(np.flatnonzero(off[1:] - off[:-1])[0])
+    w = int(col[off[u]])
+    bad = colors.copy()
+    bad[w] = bad[u]
+    assert g.check_coloring(bad) > 0
+    mis = g.greedy_mis()
+    assert g.check_mis_independent(mis) == 0 and g.mis_maximality_violations(mis) == 0
+    bad = mis.copy()
+    bad[w] = bad[u] = 1
+    assert g.check_mis_independent(bad) > 0
+
+
+def test_greedy_states_are_first_fit():
+    rng = np.random.default_rng(3)
+    n = 50
+    off, col = csr_from_edges(n, random_graph(n, 120, rng))
+    g = OracleGraph(n=n, off=off, col=col)
+    c = g.greedy_color()
+    for v in range(n):
+        lower = {int(c[w]) for w in col[off[v]:off[v + 1]] if w < v}
+        assert c[v] == min(set(range(len(lower) + 1)) - lower)
+    m = g.greedy_mis()
+    for v in range(n):
+        lower_in = any(m[w] for w in col[off[v]:off[v + 1]] if w < v)
+        assert m[v] == (0 if lower_in else 1)
+
+
+# --- scratchpad semantics (core/FlashGraph.h:200-310) -------------------------------------
+
+def test_scratchpad_row_order_and_toggles():
+    # base: 0-1, 0-2, 0-3 ; rows sorted
+    off, col = csr_from_edges(5, [(0, 1), (0, 2), (0, 3)])
+    g = OracleGraph(n=5, off=off, col=col)
+    uv = np.array([[0, 2], [0, 4], [1, 2], [0, 3]], dtype=np.int32)
+    ins = g.classify(uv)
+    assert ins.tolist() == [0, 1, 1, 0]
+    g.apply(uv, ins)
+    # base row minus deleted entries, then inserted entries in insertion order
+    assert g.row(0).tolist() == [1, 4]
+    assert g.row(2).tolist() == [1]
+    assert g.row(1).tolist() == [0, 2]
+    # re-inserting a deleted base edge restores it in its base position
+    uv2 = np.array([[2, 0], [4, 0]], dtype=np.int32)
+    ins2 = g.classify(uv2)
+    assert ins2.tolist() == [1, 0]
+    g.apply(uv2, ins2)
+    assert g.row(0).tolist() == [1, 2]
+    assert g.row(4).tolist() == []
+    assert g.degree(0) == 2
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_scratchpad_toggles_are_symmetric_and_consistent(seed):
+    rng = np.random.default_rng(100 + seed)
+    n = 30
+    off, col = csr_from_edges(n, random_graph(n, 40, rng))
+    g = OracleGraph(n=n, off=off, col=col)
+    adj = np.zeros((n, n), dtype=bool)
+    adj[np.repeat(np.arange(n), np.diff(off)), col] = True
+    for _ in range(20):
+        uv = rng.integers(0, n, (12, 2)).astype(np.int32)
+        uv = uv[uv[:, 0] != uv[:, 1]]
+        # distinct unordered pairs per batch, as the generators emit
+        _, keep = np.unique(np.sort(uv, axis=1), axis=0, return_index=True)
+        uv = uv[np.sort(keep)]
+        ins = g.classify(uv)
+        assert np.array_equal(ins.astype(bool), ~adj[uv[:, 0], uv[:, 1]])
+        g.apply(uv, ins)
+        adj[uv[:, 0], uv[:, 1]] ^= True
+        adj[uv[:, 1], uv[:, 0]] ^= True
+        for v in range(n):
+            r = g.row(v)
+            assert sorted(r.tolist()) == np.flatnonzero(adj[v]).tolist()
+            assert g.degree(v) == r.size
+
+
+# --- colouring rounds (src/coloring/scout_coloring.cu:46-100,243-277) ----------------------
+
+def test_coloring_rounds_hand_worked():
+    g = OracleGraph(n=4, off=np.zeros(5, dtype=np.int64), col=np.zeros(0, dtype=np.int32))
+    uv = np.array([[0, 1], [1, 2], [2, 3]], dtype=np.int32)
+    ins = g.classify(uv)
+    g.apply(uv, ins)
+    colors, st = g.color_batch(uv, ins, np.zeros(4, dtype=np.int32))
+    # marks {1,2,3}; round 1 proposes 1,1,1 (2 and 3 lose); round 2 proposes 0,0 (3 loses);
+    # round 3 gives 3 colour 1
+    assert colors.tolist() == [0, 1, 0, 1]
+    assert st.tolist() == [3, 3]
+
+
+def test_coloring_path_insert():
+    off, col = csr_from_edges(3, [(0, 1), (1, 2)])
+    g = OracleGraph(n=3, off=off, col=col)
+    uv = np.array([[0, 2]], dtype=np.int32)
+    ins = g.classify(uv)
+    g.apply(uv, ins)
+    colors, st = g.color_batch(uv, ins, np.array([0, 1, 0], dtype=np.int32))
+    assert colors.tolist() == [0, 1, 2] and st.tolist() == [1, 1]
+
+
+@pytest.mark.parametrize("seed", range(5))
+def test_coloring_stays_proper(seed):
+    rng = np.random.default_rng(seed)
+    n = 200
+    off, col = csr_from_edges(n, random_graph(n, 600, rng))
+    g = OracleGraph(n=n, off=off, col=col)
+    colors = g.greedy_color()
+    for _ in range(5):
+        uv = rng.integers(0, n, (80, 2)).astype(np.int32)
+        uv = uv[uv[:, 0] != uv[:, 1]]
+        ins = g.classify(uv)
+        g.apply(uv, ins)
+        colors, _ = g.color_batch(uv, ins, colors)
+        assert g.check_coloring(colors) == 0
+
+
+# --- MIS rules (src/mis/scout_mis.cu:290-362) -----------------------------------------------
+
+BIG = mis_max_csr(16.0, 1e12)
+
+
+def _mis_case(n, edges, members, batch):
+    off, col = csr_from_edges(n, edges)
+    g = OracleGraph(n=n, off=off, col=col)
+    uv = np.array(batch, dtype=np.int32).reshape(-1, 2)
+    ins = g.classify(uv)
+    g.apply(uv, ins, conditional=True)
+    mis = np.zeros(n, dtype=np.uint8)
+    mis[list(members)] = 1
+    out, st = g.mis_batch(uv, ins, mis, BIG)
+    return set(np.flatnonzero(out).tolist()), st
+
+
+def test_mis_deletion_rejected_by_two_hop_member():
+    # path 0-1-2-3, M = {0, 3}; deleting (0,1) leaves 1 undominated but 3 is in its 2-hop
+    # ball, so 1 stays out (the reference does not restore maximality here)
+    out, _ = _mis_case(4, [(0, 1), (1, 2), (2, 3)], {0, 3}, [0, 1])
+    assert out == {0, 3}
+
+
+def test_mis_deletion_accepted():
+    out, _ = _mis_case(2, [(0, 1)], {0}, [0, 1])
+    assert out == {0, 1}
+
+
+def test_mis_insertion_adds_first_addable_in_ball_order():
+    # 1-2 edge, 3-4 edge, M = {0, 1, 3}; insert (0,1): 1 leaves; ball(1) = [1, 2, 0];
+    # 2's hop-1 row [1] holds no member, so 2 joins
+    out, _ = _mis_case(5, [(1, 2), (3, 4)], {0, 1, 3}, [0, 1])
+    assert out == {0, 2, 3}
+
+
+def test_mis_insertion_skips_vertices_without_row():
+    # edges 1-2, 2-3, 2-4; M = {0, 1, 4}; insert (0,1): ball(1) = [1, 2, 0, 3, 4];
+    # 2's row holds 4; 3 is two hops from the endpoints and has no row, so it is skipped
+    out, _ = _mis_case(5, [(1, 2), (2, 3), (2, 4)], {0, 1, 4}, [0, 1])
+    assert out == {0, 4}
+
+
+def test_mis_max_csr_formula():
+    assert mis_max_csr(16.0, 288e9) == min(800000000, int(1048576 * 2 * 256))
+    assert mis_max_csr(0.0, 288e9) == 50000000
+    assert mis_max_csr(100.0, 8e9) == int(8e9 / 32)
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_mis_rules_keep_independence(seed):
+    rng = np.random.default_rng(seed)
+    n = 150
+    off, col = csr_from_edges(n, random_graph(n, 400, rng))
+    g = OracleGraph(n=n, off=off, col=col)
+    mis = g.greedy_mis()
+    for _ in range(5):
+        uv = rng.integers(0, n, (60, 2)).astype(np.int32)
+        uv = uv[uv[:, 0] != uv[:, 1]]
+        _, keep = np.unique(np.sort(uv, axis=1), axis=0, return_index=True)
+        uv = uv[np.sort(keep)]
+        ins = g.classify(uv)
+        g.apply(uv, ins, conditional=True)
+        mis, _ = g.mis_batch(uv, ins, mis, BIG)
+        assert g.check_mis_independent(mis) == 0
