@@ -1,0 +1,212 @@
+#!/usr/bin/env python3
+"""Benchmark: BFS + CC batch repair on an R-MAT stream, on one MI355X.
+
+Workload (the BFS + CC adaptive-repair configuration): R-MAT (a=0.57, b=0.19, c=0.19) of
+`--scale` (default 26: 67.1M vertices, edge factor 16, ~2.1B symmetrised adjacency entries),
+BFS from vertex 0 (int32 levels) and min-id CC labels (int32), batches of `--batch` toggles
+(default 1,000,000): 70% deletions, half of them BFS-tree edges, 30% uniform insertions.
+
+A step is one batch: classify its lines, apply them to the graph, repair BFS levels and CC
+labels, commit (state exact in HBM). The graph, the solutions and every batch are resident in
+HBM before the timed region. The stream is generated on the device by a first pass that runs
+the repair itself (the batches depend on the levels it produces); the state is then restored
+and the W + K batches replayed: W untimed, K timed. Every committed state is exact: the final
+levels and labels are compared with full recomputes on the device.
+
+Prints one JSON line (see the keys below). --gpus N > 1 exits: this build runs on one GPU.
+"""
+import argparse
+import ctypes
+import json
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+import scout_amd as sa  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(eng, n, batches_host, seconds):
+    """The reference's per-edit BFS and CC algorithms (oracle restatement, one thread) on the
+    first lines of the stream, until `seconds` of CPU time are used."""
+    from oracle_lib import OracleGraph, lib  # the checker's library, timed here as the baseline
+    g = eng.download_graph()
+    lv0, lab0 = eng.download(sa.STATE_LEVEL), eng.download(sa.STATE_LABEL)
+    og = OracleGraph(n=n, off=g.off, col=g.col)
+    og_cc = OracleGraph(n=n, off=g.off, col=g.col)
+    del g
+    L = lib()
+    p = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    rb = L.orc_refbfs_create(og.h, p(lv0), 0, 200, 1)
+    rc = L.orc_refcc_create(og_cc.h, p(lab0), 5000, 1)
+    done, t_bfs, t_cc = 0, 0.0, 0.0
+    stats = np.zeros(8, dtype=np.int64)
+    chunk = 2000
+    uv_all = batches_host[0]
+    while done < len(uv_all) and t_bfs + t_cc < seconds:
+        uv = np.ascontiguousarray(uv_all[done:done + chunk])
+        # classification happens outside the timed per-batch work, as in the reference
+        # (parse_batch_file runs before processBatch's timer, src/bfs/scout_bfs.cu:574)
+        ins = np.ascontiguousarray(og.classify(uv))
+        t0 = time.perf_counter()
+        L.orc_refbfs_batch(rb, p(uv), p(ins), len(uv), p(stats))
+        t1 = time.perf_counter()
+        L.orc_refcc_batch(rc, p(uv), p(ins), len(uv), p(stats))
+        t2 = time.perf_counter()
+        t_bfs += t1 - t0
+        t_cc += t2 - t1
+        done += len(uv)
+    L.orc_refbfs_free(rb)
+    L.orc_refcc_free(rc)
+    return {"value": done / max(t_bfs + t_cc, 1e-9), "unit": "edge updates/s", "cores": 1, "kind": "port",
+            "sample": f"first {done} lines of batch 1, reference per-edit BFS (bound 200, deferred fallback) "
+                      f"and CC (bidirectional search, bound 5000, deferred) restated on the CPU; "
+                      f"bfs {t_bfs:.1f}s, cc {t_cc:.1f}s"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--scale", type=int, default=26)
+    ap.add_argument("--batch", type=int, default=1_000_000)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--strategy", choices=["local", "recompute"], default="local")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-verify", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported by this build", file=sys.stderr)
+        sys.exit(2)
+
+    strategy = sa.STRATEGY_LOCAL if args.strategy == "local" else sa.STRATEGY_RECOMPUTE
+    total = args.warmup + args.steps
+    eng = sa.Engine(0)
+    t0 = time.perf_counter()
+    eng.gen_rmat(args.scale, 16, 0.57, 0.19, 0.19, args.seed)
+    info = eng.graph_info()
+    log(f"graph: n={info['n']} entries={info['base_entries']} ({time.perf_counter() - t0:.1f}s)")
+    t0 = time.perf_counter()
+    eng.bfs_full(0)
+    eng.cc_full()
+    eng.sync()
+    log(f"initial BFS + CC on device: {time.perf_counter() - t0:.2f}s")
+    eng.snapshot()
+    lv_init = None
+    if not args.no_cpu:
+        lv_init = (eng.download(sa.STATE_LEVEL), eng.download(sa.STATE_LABEL))
+
+    # Pass 1: generate the stream against the evolving graph and keep every batch in HBM.
+    t0 = time.perf_counter()
+    first_batch_host = None
+    for b in range(total):
+        m = eng.gen_batch(args.batch, 0.7, 0.5, args.seed, expected_total=args.batch * total)
+        eng.batch_store(b)
+        if b == 0 and not args.no_cpu:
+            first_batch_host = eng.batch_download(m)
+        eng.classify()
+        eng.apply()
+        eng.repair_bfs(sa.STRATEGY_LOCAL)
+        eng.repair_cc(sa.STRATEGY_LOCAL)
+    eng.sync()
+    log(f"stream of {total} batches generated in {time.perf_counter() - t0:.1f}s")
+
+    # Pass 2: replay from the initial state; W untimed, K timed.
+    eng.restore()
+    for b in range(args.warmup):
+        eng.batch_load(b)
+        eng.classify()
+        eng.apply()
+        eng.repair_bfs(strategy)
+        eng.repair_cc(strategy)
+    eng.sync()
+    eng.prof_enable(True)
+    lat = []
+    lines = 0
+    eng.sync()
+    t_start = time.perf_counter()
+    for b in range(args.warmup, total):
+        ts = time.perf_counter()
+        eng.batch_load(b)
+        eng.classify()
+        eng.apply()
+        sb = eng.repair_bfs(strategy)
+        sc = eng.repair_cc(strategy)
+        eng.sync()
+        lat.append((time.perf_counter() - ts) * 1e3)
+        lines += sb.lines
+    eng.sync()
+    elapsed = time.perf_counter() - t_start
+    prof = {name: eng.prof_read(i) for i, name in enumerate(sa.PROF_NAMES)}
+    eng.prof_enable(False)
+
+    verified = None
+    if not args.no_verify:
+        lv = eng.download(sa.STATE_LEVEL)
+        lab = eng.download(sa.STATE_LABEL)
+        eng.bfs_full(0)
+        eng.cc_full()
+        verified = bool(np.array_equal(lv, eng.download(sa.STATE_LEVEL)) and
+                        np.array_equal(lab, eng.download(sa.STATE_LABEL)))
+        log(f"final state equals full recompute: {verified}")
+
+    dom = max(prof, key=lambda k: prof[k]["ms"])
+    d = prof[dom]
+    achieved = d["bytes"] / (d["ms"] / 1e3) / 1e9 if d["ms"] > 0 else 0.0
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+                "launches": d["launches"], "avg_launch_ms": round(d["ms"] / max(d["launches"], 1), 4)}
+
+    cpu = None
+    if not args.no_cpu and first_batch_host is not None:
+        eng.restore()
+        t0 = time.perf_counter()
+        cpu = cpu_baseline(eng, info["n"], [first_batch_host], args.cpu_seconds)
+        log(f"cpu baseline: {cpu['value']:.0f} updates/s ({time.perf_counter() - t0:.1f}s)")
+
+    ups = lines / elapsed
+    out = {
+        "metric": "edge updates/s + p99 batch-commit latency, learned vs plain repair (bit-exact)",
+        "value": round(ups, 1),
+        "unit": "edge updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed * 1e3 / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic (R-MAT and update stream generated on the device, fixed seeds)",
+        "config": {"workload": f"BFS(0)+CC batch repair, R-MAT scale {args.scale} ef 16, "
+                               f"{args.batch} toggles/batch (70% deletions, half BFS-tree edges)",
+                   "scale": args.scale, "vertices": info["n"], "adjacency_entries": info["base_entries"],
+                   "batch": args.batch, "strategy": args.strategy + " (plain)"},
+        "p50_ms": round(float(np.percentile(lat, 50)), 3),
+        "p99_ms": round(float(np.percentile(lat, 99)), 3),
+        "verified_exact": verified,
+        "roofline": roofline,
+        "kernels": {k: {"ms": round(v["ms"], 3), "launches": v["launches"],
+                        "GBps": round(v["bytes"] / (v["ms"] / 1e3) / 1e9, 1) if v["ms"] > 0 else 0.0}
+                    for k, v in prof.items() if v["launches"]},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

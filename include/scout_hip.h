@@ -161,6 +161,40 @@ int scout_sync(scout_ctx* ctx);
  * vertices, edge_factor * 2^scale generated edges, symmetrised, self-loops and duplicates
  * removed, rows sorted. Bit-identical to the host generator for the same seed. */
 int scout_gen_rmat(scout_ctx* ctx, int scale, int edge_factor, double a, double b, double c, uint64_t seed);
+/* Generates the next batch of a synthetic stream into a channel, against the current graph
+ * and levels: round(delete_frac*size) deletions of current edges (tree_frac of them BFS-tree
+ * edges, the rest uniform over adjacency entries) and uniform insertions of non-edges, as
+ * canonical u < v pairs never repeated within or across the context's batches (the role of
+ * tools/generate_mixed_batches.py). expected_total sizes the memory of toggled pairs. *m
+ * receives the line count. */
+int scout_gen_batch(scout_ctx* ctx, int channel, int64_t size, double delete_frac, double tree_frac, uint64_t seed,
+                    int64_t expected_total, int64_t* m);
+/* Copies a staged batch back to the host (cap_lines pairs at most). */
+int scout_batch_download(scout_ctx* ctx, int channel, int32_t* uv, int64_t cap_lines);
+/* Keeps a device copy of a channel's staged batch in slot k / stages slot k into a channel
+ * with a device-to-device copy, so a replayed stream is resident in HBM. */
+int scout_batch_store(scout_ctx* ctx, int channel, int slot);
+int scout_batch_load(scout_ctx* ctx, int channel, int slot);
+/* Device copies of the graph and solutions, to replay one stream under several strategies. */
+int scout_snapshot(scout_ctx* ctx);
+int scout_restore(scout_ctx* ctx);
+
+/* Kernel timing for roofline reports: while enabled, launches of the kernel families below
+ * are bracketed by HIP events on the compute stream, and each family accumulates its
+ * algorithmic bytes. scout_prof_read returns the summed device time, the launch count and
+ * the bytes of one family. */
+enum {
+  SCOUT_PROF_BFS_RELAX = 0,
+  SCOUT_PROF_BFS_INVALIDATE = 1,
+  SCOUT_PROF_CC_EXPAND = 2,
+  SCOUT_PROF_APPLY = 3,
+  SCOUT_PROF_CLASSIFY = 4,
+  SCOUT_PROF_COLOR = 5,
+  SCOUT_PROF_MIS_EVAL = 6,
+  SCOUT_PROF_CC_HOOK = 7
+};
+int scout_prof_enable(scout_ctx* ctx, int on);
+int scout_prof_read(scout_ctx* ctx, int id, double* total_ms, int64_t* launches, int64_t* bytes);
 
 #ifdef __cplusplus
 }

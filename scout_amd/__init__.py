@@ -72,7 +72,21 @@ _SIGNATURES = {
     "scout_sync": [_P],
     "scout_gen_rmat": [_P, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                        ctypes.c_uint64],
+    "scout_gen_batch": [_P, ctypes.c_int, _I64, ctypes.c_double, ctypes.c_double, ctypes.c_uint64, _I64,
+                        ctypes.POINTER(_I64)],
+    "scout_batch_download": [_P, ctypes.c_int, _P, _I64],
+    "scout_batch_store": [_P, ctypes.c_int, ctypes.c_int],
+    "scout_batch_load": [_P, ctypes.c_int, ctypes.c_int],
+    "scout_snapshot": [_P],
+    "scout_restore": [_P],
+    "scout_prof_enable": [_P, ctypes.c_int],
+    "scout_prof_read": [_P, ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_I64),
+                        ctypes.POINTER(_I64)],
 }
+
+PROF_BFS_RELAX, PROF_BFS_INVALIDATE, PROF_CC_EXPAND, PROF_APPLY, PROF_CLASSIFY = 0, 1, 2, 3, 4
+PROF_COLOR, PROF_MIS_EVAL, PROF_CC_HOOK = 5, 6, 7
+PROF_NAMES = ["bfs_relax", "bfs_invalidate", "cc_expand", "apply", "classify", "color", "mis_eval", "cc_hook"]
 
 _lib = None
 
@@ -253,6 +267,40 @@ class Engine:
 
     def sync(self):
         self._check(self.lib.scout_sync(self.h), "sync")
+
+    # synthetic streams, snapshots, kernel timing
+    def gen_batch(self, size, delete_frac=0.5, tree_frac=0.0, seed=1, expected_total=0, channel=0) -> int:
+        m = ctypes.c_int64()
+        self._check(self.lib.scout_gen_batch(self.h, channel, size, delete_frac, tree_frac, seed, expected_total,
+                                             ctypes.byref(m)), "gen_batch")
+        self._staged = m.value
+        return m.value
+
+    def batch_download(self, m, channel=0) -> np.ndarray:
+        out = np.zeros((max(m, 1), 2), dtype=np.int32)
+        self._check(self.lib.scout_batch_download(self.h, channel, _ptr(out), m), "batch_download")
+        return out[:m]
+
+    def batch_store(self, slot, channel=0):
+        self._check(self.lib.scout_batch_store(self.h, channel, slot), "batch_store")
+
+    def batch_load(self, slot, channel=0):
+        self._check(self.lib.scout_batch_load(self.h, channel, slot), "batch_load")
+
+    def snapshot(self):
+        self._check(self.lib.scout_snapshot(self.h), "snapshot")
+
+    def restore(self):
+        self._check(self.lib.scout_restore(self.h), "restore")
+
+    def prof_enable(self, on=True):
+        self._check(self.lib.scout_prof_enable(self.h, 1 if on else 0), "prof_enable")
+
+    def prof_read(self, kid: int) -> dict:
+        ms, cnt, by = ctypes.c_double(), ctypes.c_int64(), ctypes.c_int64()
+        self._check(self.lib.scout_prof_read(self.h, kid, ctypes.byref(ms), ctypes.byref(cnt), ctypes.byref(by)),
+                    "prof_read")
+        return {"ms": ms.value, "launches": cnt.value, "bytes": by.value}
 
 
 def mis_max_csr(avg_degree: float, free_bytes: float) -> int:

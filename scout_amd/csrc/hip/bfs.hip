@@ -59,10 +59,10 @@ __global__ void k_relax(GView g, int32_t* level, const int32_t* front, const int
   if (lane_id() == 0 && work) atomicAdd((unsigned long long*)scanned, (unsigned long long)work);
 }
 
-__global__ void k_relax_tasks(GView g, int32_t* level, const int64_t* tasks, const int64_t* task_count, int32_t lvl,
-                              int32_t* next, int64_t* next_count, int64_t cap, int32_t* stamp, int32_t epoch,
-                              int64_t* scanned) {
-  int64_t nt = *task_count;
+__global__ void k_relax_tasks(GView g, int32_t* level, const int64_t* tasks, const int64_t* task_count,
+                              int64_t task_cap, int32_t lvl, int32_t* next, int64_t* next_count, int64_t cap,
+                              int32_t* stamp, int32_t epoch, int64_t* scanned) {
+  int64_t nt = min(*task_count, task_cap);
   int64_t work = 0;
   for (int64_t i = global_wave(); i < nt; i += total_waves()) {
     int64_t t = tasks[i];
@@ -249,18 +249,26 @@ int relax_levels(Ctx& c, int32_t lvl0, int32_t max_seed, const int64_t* seeds, i
   SCOUT_TRY(hipMemsetAsync(bscan, 0, sizeof(int64_t), c.s));
   int32_t lvl = lvl0;
   int64_t r = 0;
+  int64_t scanned_before = 0;
   while (true) {
     if (nseeds > 0 && lvl <= max_seed)
       k_seed_at_level<<<grid_for(nseeds), kBlock, 0, c.s>>>(seeds, nseeds, c.level, lvl, cur, bcur, cap, c.stamp, epoch);
     SCOUT_TRY(hipMemsetAsync(bnext, 0, 2 * sizeof(int64_t), c.s));
+    SCOUT_CHECK(prof_begin(c, PROF_BFS_RELAX));
     k_relax<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), c.level, cur, bcur, lvl, nxt, bnext, cap, c.stamp, epoch,
                                             c.wb.p, btask, task_cap, bscan);
-    k_relax_tasks<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), c.level, c.wb.p, btask, lvl, nxt, bnext, cap, c.stamp,
-                                                  epoch, bscan);
+    k_relax_tasks<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), c.level, c.wb.p, btask, task_cap, lvl, nxt, bnext, cap,
+                                                  c.stamp, epoch, bscan);
+    SCOUT_CHECK(prof_end(c, PROF_BFS_RELAX, 0));
     r++;
     SCOUT_TRY(hipMemcpyAsync(c.h_box + 8, c.d_box + 8, 4 * sizeof(int64_t), hipMemcpyDeviceToHost, c.s));
     SCOUT_TRY(hipStreamSynchronize(c.s));
     int64_t nn = c.h_box[9];
+    // algorithmic bytes: per frontier vertex its id, level and the four row offsets (36 B);
+    // per scanned entry the neighbour id and its level (8 B); per lowered vertex the level
+    // write and its queue entry (8 B)
+    prof_bytes(c, PROF_BFS_RELAX, 36 * c.h_box[8] + 8 * (c.h_box[11] - scanned_before) + 8 * nn);
+    scanned_before = c.h_box[11];
     if (c.h_box[10] > task_cap) {
       set_error("bfs relax: chunk task list overflow");
       return SCOUT_ERR_STATE;
@@ -287,7 +295,8 @@ int ensure_bfs(Ctx& c, int64_t extra) {
   SCOUT_CHECK(c.ld.reserve(n + 1));
   SCOUT_CHECK(c.le.reserve(n + 1));
   SCOUT_CHECK(c.wa.reserve(n + 2 * extra + 1));
-  SCOUT_CHECK(c.wb.reserve(c.g.E / kChunk + n / 8 + 1024));
+  // chunk tasks: at most 2 * entries / kChunk (every split row has more than kChunk entries)
+  SCOUT_CHECK(c.wb.reserve(2 * (c.g.E + c.g.A) / kChunk + 1024));
   return SCOUT_OK;
 }
 

@@ -87,8 +87,8 @@ __global__ void k_cc_hook(GView g, int32_t* parent, int64_t* tasks, int64_t* tas
 }
 
 __global__ void k_cc_hook_tasks(GView g, int32_t* parent, const int64_t* tasks, const int64_t* task_count,
-                                int64_t* scanned) {
-  int64_t nt = *task_count;
+                                int64_t task_cap, int64_t* scanned) {
+  int64_t nt = min(*task_count, task_cap);
   int64_t work = 0;
   for (int64_t i = global_wave(); i < nt; i += total_waves()) {
     int64_t t = tasks[i];
@@ -198,7 +198,10 @@ __global__ void k_cc_expand(GView g, const int32_t* front, const int64_t* nf, in
   int64_t work = 0, exp_count = 0;
   for (int64_t i = global_wave(); i < n; i += total_waves()) {
     int32_t f = front[i];
-    int32_t r = uf_find(gp, owner[f]);
+    // one lane resolves the group root while other waves may be uniting groups
+    int32_t r0 = 0;
+    if (lane_id() == 0) r0 = uf_find(gp, owner[f]);
+    int32_t r = __shfl(r0, 0);
     int32_t comp = label[f];
     if (nalive[comp] <= 1) continue;
     uint32_t lead = 0xffffffffu - (uint32_t)(best[comp] & 0xffffffffULL);
@@ -357,8 +360,8 @@ static int cc_prepare(Ctx& c, CcScratch& s, int64_t m) {
     SCOUT_CHECK(s.next.reserve((int64_t)n + 1));
     SCOUT_CHECK(s.claimed.reserve((int64_t)n + 1));
     SCOUT_CHECK(s.claimed_label.reserve((int64_t)n + 1));
-    SCOUT_CHECK(s.tasks.reserve(c.g.E / kChunk + (int64_t)n / 8 + 1024));
   }
+  SCOUT_CHECK(s.tasks.reserve(2 * (c.g.E + c.g.A) / kChunk + 1024));
   int64_t ng = 2 * m + 2;
   SCOUT_CHECK(s.gp.reserve(ng));
   SCOUT_CHECK(s.gsrc.reserve(ng));
@@ -381,7 +384,8 @@ int cc_full(Ctx& c, int64_t* scanned) {
   k_iota<<<grid_for(n), kBlock, 0, c.s>>>(c.label, n);
   SCOUT_TRY(hipMemsetAsync(c.d_box, 0, 2 * sizeof(int64_t), c.s));
   k_cc_hook<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), c.label, st->s.tasks.p, c.d_box, st->s.tasks.cap, c.d_box + 1);
-  k_cc_hook_tasks<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), c.label, st->s.tasks.p, c.d_box, c.d_box + 1);
+  k_cc_hook_tasks<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), c.label, st->s.tasks.p, c.d_box, st->s.tasks.cap,
+                                                  c.d_box + 1);
   k_cc_flatten<<<grid_for(n), kBlock, 0, c.s>>>(c.label, n);
   SCOUT_CHECK(box_read(c, 2));
   if (c.h_box[0] > st->s.tasks.cap) {
@@ -433,18 +437,29 @@ int cc_local(Ctx& c, Channel& ch, scout_stats* st) {
   if (nsrc > 0) {
     int32_t* front = s.front.p;
     int32_t* nxt = s.next.p;
+    int64_t scan_before = 0, claimed_before = c.h_box[10], front_before = nsrc;
     while (true) {
       SCOUT_TRY(hipMemsetAsync(s.fcnt.p, 0, (size_t)ng * 4, c.s));
       k_cc_count<<<kListGrid, kBlock, 0, c.s>>>(front, bfront, c.cc_owner, s.gp.p, s.fcnt.p);
       k_cc_comp_stats<<<grid_for(ng), kBlock, 0, c.s>>>(ng, s.gp.p, s.gsrc.p, s.fcnt.p, c.label, s.nalive.p, s.best.p);
       SCOUT_TRY(hipMemsetAsync(bnext, 0, sizeof(int64_t), c.s));
       SCOUT_TRY(hipMemsetAsync(bexp, 0, sizeof(int64_t), c.s));
+      SCOUT_CHECK(prof_begin(c, PROF_CC_EXPAND));
       k_cc_expand<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), front, bfront, c.cc_owner, s.gp.p, c.label, s.nalive.p,
                                                   s.best.p, nxt, bnext, s.claimed.p, bclaimed, cap, bexp, bscan);
+      SCOUT_CHECK(prof_end(c, PROF_CC_EXPAND, 0));
       rounds++;
       SCOUT_TRY(hipMemcpyAsync(c.h_box + 8, box + 8, 5 * sizeof(int64_t), hipMemcpyDeviceToHost, c.s));
       SCOUT_TRY(hipStreamSynchronize(c.s));
       int64_t expanded = c.h_box[11];
+      // algorithmic bytes: per frontier vertex id, owner, group root, label, component stats
+      // and row offsets (64 B); per scanned entry the neighbour id and its owner (8 B); per
+      // claim the owner write and the two list entries (12 B)
+      prof_bytes(c, PROF_CC_EXPAND, 64 * front_before + 8 * (c.h_box[12] - scan_before) +
+                                        12 * (c.h_box[10] - claimed_before));
+      scan_before = c.h_box[12];
+      claimed_before = c.h_box[10];
+      front_before = c.h_box[9];
       // per-component stats are rebuilt each round; only the sources' components were set
       k_cc_clear_comp<<<grid_for(ng), kBlock, 0, c.s>>>(ng, s.gsrc.p, c.label, s.nalive.p, s.best.p);
       if (expanded == 0) break;

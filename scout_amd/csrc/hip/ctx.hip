@@ -127,6 +127,71 @@ Span::~Span() {
   if (b) (void)hipEventDestroy(b);
 }
 
+struct ProfState {
+  bool on = false;
+  struct Entry {
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> spans;
+    int64_t bytes = 0;
+    hipEvent_t open = nullptr;
+  } e[PROF_COUNT];
+  std::vector<hipEvent_t> pool;
+  hipEvent_t get() {
+    if (!pool.empty()) {
+      hipEvent_t ev = pool.back();
+      pool.pop_back();
+      return ev;
+    }
+    hipEvent_t ev = nullptr;
+    (void)hipEventCreate(&ev);
+    return ev;
+  }
+  void clear() {
+    for (auto& en : e) {
+      for (auto& sp : en.spans) {
+        pool.push_back(sp.first);
+        pool.push_back(sp.second);
+      }
+      en.spans.clear();
+      en.bytes = 0;
+      if (en.open) pool.push_back(en.open);
+      en.open = nullptr;
+    }
+  }
+};
+
+int prof_begin(Ctx& c, int id) {
+  if (!c.prof || !c.prof->on) return SCOUT_OK;
+  ProfState::Entry& en = c.prof->e[id];
+  en.open = c.prof->get();
+  SCOUT_TRY(hipEventRecord(en.open, c.s));
+  return SCOUT_OK;
+}
+
+int prof_end(Ctx& c, int id, int64_t bytes) {
+  if (!c.prof || !c.prof->on) return SCOUT_OK;
+  ProfState::Entry& en = c.prof->e[id];
+  if (!en.open) return SCOUT_OK;
+  hipEvent_t b = c.prof->get();
+  SCOUT_TRY(hipEventRecord(b, c.s));
+  en.spans.emplace_back(en.open, b);
+  en.open = nullptr;
+  en.bytes += bytes;
+  return SCOUT_OK;
+}
+
+// Adds algorithmic bytes to a kernel family after the counts were read back.
+void prof_bytes(Ctx& c, int id, int64_t bytes) {
+  if (c.prof && c.prof->on) c.prof->e[id].bytes += bytes;
+}
+
+void prof_release(Ctx& c) {
+  if (!c.prof) return;
+  c.prof->clear();
+  for (hipEvent_t ev : c.prof->pool) (void)hipEventDestroy(ev);
+  delete c.prof;
+  c.prof = nullptr;
+}
+
 Channel* channel_of(Ctx& c, int ch) {
   if (ch < 0 || ch > 1) {
     set_error("channel must be 0 or 1");
@@ -192,6 +257,9 @@ int scout_ctx_destroy(scout_ctx* x) {
   free_state(c);
   cc_release(c);
   mis_release(c);
+  stream_release(c);
+  snapshot_release(c);
+  prof_release(c);
   for (int k = 0; k < 2; k++) {
     c.ch[k].h_uv.release();
     c.ch[k].d_uv.release();
@@ -423,6 +491,41 @@ int scout_apply(scout_ctx* x, int channel, int conditional) {
   if (!ch) return SCOUT_ERR_ARG;
   SCOUT_TRY(hipStreamWaitEvent(c.s, ch->staged, 0));
   return graph_apply(c, *ch, conditional);
+}
+
+int scout_prof_enable(scout_ctx* x, int on) {
+  SCOUT_CHECK(check_ctx(x));
+  Ctx& c = x->c;
+  SCOUT_TRY(hipStreamSynchronize(c.s));
+  if (!c.prof) c.prof = new ProfState();
+  c.prof->clear();
+  c.prof->on = on != 0;
+  return SCOUT_OK;
+}
+
+int scout_prof_read(scout_ctx* x, int id, double* total_ms, int64_t* launches, int64_t* bytes) {
+  SCOUT_CHECK(check_ctx(x));
+  Ctx& c = x->c;
+  if (id < 0 || id >= PROF_COUNT) {
+    set_error("prof_read: unknown kernel id");
+    return SCOUT_ERR_ARG;
+  }
+  double ms = 0.0;
+  int64_t cnt = 0, by = 0;
+  if (c.prof) {
+    SCOUT_TRY(hipStreamSynchronize(c.s));
+    for (auto& sp : c.prof->e[id].spans) {
+      float t = 0.f;
+      SCOUT_TRY(hipEventElapsedTime(&t, sp.first, sp.second));
+      ms += t;
+    }
+    cnt = (int64_t)c.prof->e[id].spans.size();
+    by = c.prof->e[id].bytes;
+  }
+  if (total_ms) *total_ms = ms;
+  if (launches) *launches = cnt;
+  if (bytes) *bytes = by;
+  return SCOUT_OK;
 }
 
 int scout_sync(scout_ctx* x) {

@@ -334,9 +334,13 @@ int apply_range(Ctx& c, Channel& ch, int64_t lo, int64_t hi, int conditional) {
   SCOUT_CHECK(g.nkey.reserve(cap));
   SCOUT_CHECK(g.ndst.reserve(cap));
   SCOUT_TRY(hipMemsetAsync(c.d_box, 0, 4 * sizeof(int64_t), c.s));
+  SCOUT_CHECK(prof_begin(c, PROF_APPLY));
   if (hi > lo)
     k_apply<<<grid_for(hi - lo), kBlock, 0, c.s>>>(g.view(), g.sorted, ch.d_uv.p, ch.d_ins.p, lo, hi, conditional,
                                                    ch.d_done.p, g.nkey.p, g.ndst.p, g.nkey.cap, c.d_box);
+  // algorithmic bytes per line: the pair and its label and done flag (10 B), both rows'
+  // base and inserted offsets (32 B) and the entries flipped or appended (8 B)
+  SCOUT_CHECK(prof_end(c, PROF_APPLY, 50 * (hi - lo)));
   SCOUT_CHECK(box_read(c, 2));
   int64_t cnt_new = c.h_box[0];
   g.holes += c.h_box[1];
@@ -436,9 +440,13 @@ int graph_classify(Ctx& c, Channel& ch, int rule) {
     return SCOUT_ERR_STATE;
   }
   SCOUT_TRY(hipMemsetAsync(c.d_box, 0, sizeof(int64_t), c.s));
+  SCOUT_CHECK(prof_begin(c, PROF_CLASSIFY));
   if (ch.m > 0)
     k_classify<<<grid_for(ch.m), kBlock, 0, c.s>>>(c.g.view(), c.g.sorted, ch.d_uv.p, ch.m, rule, ch.d_ins.p,
                                                    c.d_box);
+  // per line: the pair (8 B), row u's base and inserted offsets (16 B), the located entry
+  // (4 B) and the label written (1 B)
+  SCOUT_CHECK(prof_end(c, PROF_CLASSIFY, 29 * ch.m));
   return SCOUT_OK;
 }
 

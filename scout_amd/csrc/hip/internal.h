@@ -134,6 +134,9 @@ struct Channel {
 struct MlpState;  // mlp.hip
 struct CcState;   // cc.hip
 struct MisState;  // mis.hip
+struct StreamState;  // stream.hip
+struct Snapshot;     // stream.hip
+struct ProfState;    // prof in ctx.hip
 
 struct Ctx {
   int dev = 0;
@@ -169,6 +172,9 @@ struct Ctx {
   MlpState* mlp = nullptr;
   CcState* cc = nullptr;
   MisState* mis_state = nullptr;
+  StreamState* stream = nullptr;
+  Snapshot* snap = nullptr;
+  ProfState* prof = nullptr;
   scout_budget budget{};
   int64_t hbm_used = 0;
   int64_t pinned_used = 0;
@@ -202,6 +208,18 @@ int cc_full(Ctx& c, int64_t* scanned);
 int cc_local(Ctx& c, Channel& ch, scout_stats* st);
 void cc_release(Ctx& c);
 void mis_release(Ctx& c);
+void stream_release(Ctx& c);
+void snapshot_release(Ctx& c);
+
+// Kernel timing for the roofline: events recorded on the compute stream around launches of a
+// named kernel family while profiling is on. `bytes` is the algorithmic byte count the caller
+// attributes to that launch.
+int prof_begin(Ctx& c, int id);
+int prof_end(Ctx& c, int id, int64_t bytes);
+void prof_bytes(Ctx& c, int id, int64_t bytes);
+void prof_release(Ctx& c);
+enum ProfId { PROF_BFS_RELAX = 0, PROF_BFS_INVALIDATE, PROF_CC_EXPAND, PROF_APPLY, PROF_CLASSIFY,
+              PROF_COLOR, PROF_MIS_EVAL, PROF_CC_HOOK, PROF_COUNT };
 
 // helpers in ctx.hip
 int box_read(Ctx& c, int count);  // copies d_box[0..count) into h_box, synchronises c.s

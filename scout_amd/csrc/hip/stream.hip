@@ -1,0 +1,395 @@
+// stream.hip: synthetic update batches generated in HBM against the current graph and
+// levels, and device-side snapshots for replaying one stream under several strategies.
+//
+// A batch holds `size` toggles: round(delete_frac * size) deletions of current edges, of which
+// round(tree_frac * deletions) are BFS-tree edges (a reached vertex and a neighbour one level
+// closer to the source) and the rest uniform over current adjacency entries, plus uniform
+// insertions of current non-edges. As in the reference's generators
+// (tools/generate_mixed_batches.py:94-141) every line is a canonical u < v pair and no pair is
+// ever repeated within or across the batches of one context.
+#include <algorithm>
+#include <vector>
+
+#include "../host/rng.h"
+#include "common.cuh"
+
+namespace scout {
+
+namespace {
+
+constexpr uint64_t kEmptyKey = ~0ULL;
+
+__device__ __forceinline__ uint64_t mix(uint64_t k) {
+  k ^= k >> 33;
+  k *= 0xff51afd7ed558ccdULL;
+  k ^= k >> 33;
+  k *= 0xc4ceb9fe1a85ec53ULL;
+  k ^= k >> 33;
+  return k;
+}
+
+// Inserts the canonical pair into the toggled set; true if it was not there.
+__device__ __forceinline__ bool claim_pair(uint64_t* set, uint64_t mask, uint32_t u, uint32_t v) {
+  uint32_t a = u < v ? u : v, b = u < v ? v : u;
+  uint64_t key = ((uint64_t)a << 32) | b;
+  uint64_t h = mix(key) & mask;
+  for (uint64_t probe = 0; probe <= mask; probe++) {
+    uint64_t prev = atomicCAS((unsigned long long*)&set[h], (unsigned long long)kEmptyKey, (unsigned long long)key);
+    if (prev == kEmptyKey) return true;
+    if (prev == key) return false;
+    h = (h + 1) & mask;
+  }
+  return false;
+}
+
+__device__ __forceinline__ void emit(uint32_t u, uint32_t v, int32_t* out, int64_t* count, int64_t limit) {
+  int64_t pos = (int64_t)atomicAdd((unsigned long long*)count, 1ULL);
+  if (pos < limit) {
+    out[2 * pos] = (int32_t)(u < v ? u : v);
+    out[2 * pos + 1] = (int32_t)(u < v ? v : u);
+  }
+}
+
+// Tree deletions: a uniform vertex with level > 0 and its first neighbour one level lower,
+// scanning its row cyclically from a random offset (thread per draw).
+__global__ void k_gen_tree(GView g, const int32_t* level, uint64_t key, uint64_t base, int64_t draws, uint64_t* set,
+                           uint64_t mask, int32_t* out, int64_t* count, int64_t limit) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= draws || *count >= limit) return;
+  uint64_t x = draw(key, base + (uint64_t)i);
+  uint32_t v = uniform_below(x, (uint32_t)g.n);
+  int32_t lv = level[v];
+  if (lv <= 0) return;
+  Row r = row_of(g, (int32_t)v);
+  int64_t len = row_len(r);
+  if (len == 0) return;
+  int64_t start = (int64_t)uniform_below(x << 32 | (x >> 32), (uint32_t)min<int64_t>(len, 0xffffffffLL));
+  for (int64_t t = 0; t < len && t < 256; t++) {
+    int64_t k = start + t;
+    if (k >= len) k -= len;
+    int32_t w = row_at(g, r, k);
+    if (w >= 0 && level[w] == lv - 1) {
+      if (claim_pair(set, mask, v, (uint32_t)w)) emit(v, (uint32_t)w, out, count, limit);
+      return;
+    }
+  }
+}
+
+// Uniform deletions: a uniform physical adjacency position (base, then inserted rows); the
+// draw is kept if the entry is live.
+__global__ void k_gen_del(GView g, int64_t E, int64_t A, uint64_t key, uint64_t base, int64_t draws, uint64_t* set,
+                          uint64_t mask, int32_t* out, int64_t* count, int64_t limit) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= draws || *count >= limit) return;
+  uint64_t x = draw(key, base + (uint64_t)i);
+  uint64_t total = (uint64_t)(E + A);
+  uint64_t pos = __umul64hi(x, total);
+  const int64_t* off = pos < (uint64_t)E ? g.boff : g.aoff;
+  int64_t p = pos < (uint64_t)E ? (int64_t)pos : (int64_t)pos - E;
+  int32_t w = pos < (uint64_t)E ? g.bcol[p] : g.acol[p];
+  if (w < 0) return;
+  int64_t l = 0, h = g.n;  // last vertex whose offset <= p
+  while (l < h) {
+    int64_t mid = (l + h + 1) >> 1;
+    if (off[mid] <= p) l = mid;
+    else h = mid - 1;
+  }
+  uint32_t u = (uint32_t)l;
+  if (u == (uint32_t)w) return;
+  if (claim_pair(set, mask, u, (uint32_t)w)) emit(u, (uint32_t)w, out, count, limit);
+}
+
+// Uniform insertions of current non-edges.
+__global__ void k_gen_ins(GView g, bool sorted, uint64_t key, uint64_t base, int64_t draws, uint64_t* set,
+                          uint64_t mask, int32_t* out, int64_t* count, int64_t limit) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= draws || *count >= limit) return;
+  uint64_t x = draw(key, base + (uint64_t)i);
+  uint32_t u = uniform_below(x, (uint32_t)g.n), v = uniform_below(x << 32 | (x >> 32), (uint32_t)g.n);
+  if (u == v) return;
+  if (adjacent_now(pair_state(g, sorted, (int32_t)u, (int32_t)v))) return;
+  if (claim_pair(set, mask, u, v)) emit(u, v, out, count, limit);
+}
+
+}  // namespace
+
+struct StreamState {
+  DevBuf<uint64_t> set;
+  uint64_t mask = 0;
+  uint64_t draws_used[3] = {0, 0, 0};
+  std::vector<DevBuf<int32_t>> slots;  // stored batches (pairs) for replay
+  std::vector<int64_t> slot_m;
+};
+
+int gen_batch(Ctx& c, Channel& ch, int64_t size, double delete_frac, double tree_frac, uint64_t seed,
+              int64_t expected_total, int64_t* out_m) {
+  if (!c.stream) c.stream = new StreamState();
+  StreamState& s = *c.stream;
+  if (!s.set.p) {
+    uint64_t cap = 1024;
+    while (cap < (uint64_t)(4 * std::max<int64_t>(expected_total, size))) cap <<= 1;
+    SCOUT_CHECK(s.set.reserve((int64_t)cap));
+    SCOUT_TRY(hipMemsetAsync(s.set.p, 0xff, (size_t)s.set.cap * 8, c.s));
+    s.mask = 1;
+    while (s.mask * 2 <= (uint64_t)s.set.cap) s.mask *= 2;
+    s.mask -= 1;
+  }
+  int64_t ndel = (int64_t)llround(delete_frac * (double)size);
+  int64_t ntree = (int64_t)llround(tree_frac * (double)ndel);
+  int64_t nuni = ndel - ntree, nins = size - ndel;
+  SCOUT_CHECK(ch.d_uv.reserve(2 * size + 2));
+  SCOUT_CHECK(ch.d_ins.reserve(size + 1));
+  SCOUT_CHECK(ch.d_done.reserve(size + 1));
+  SCOUT_TRY(hipStreamWaitEvent(c.s, ch.staged, 0));
+  uint64_t keys[3] = {stream_key(seed, RNG_STREAM_TREE), stream_key(seed, RNG_STREAM_DEL), stream_key(seed, RNG_STREAM_INS)};
+  int64_t want[3] = {ntree, nuni, nins};
+  int64_t produced = 0;
+  for (int kind = 0; kind < 3; kind++) {
+    if (want[kind] == 0) continue;
+    if (kind == 0 && !c.level) {
+      set_error("gen_batch: tree deletions need BFS levels");
+      return SCOUT_ERR_STATE;
+    }
+    int32_t* out = ch.d_uv.p + 2 * produced;
+    int64_t got = 0;
+    for (int attempt = 0; attempt < 64 && got < want[kind]; attempt++) {
+      int64_t draws = std::max<int64_t>(2 * (want[kind] - got), 1024);
+      SCOUT_TRY(hipMemcpyAsync(c.d_box, &got, sizeof(int64_t), hipMemcpyHostToDevice, c.s));
+      if (kind == 0)
+        k_gen_tree<<<grid_for(draws), kBlock, 0, c.s>>>(c.g.view(), c.level, keys[0], s.draws_used[0], draws, s.set.p,
+                                                        s.mask, out, c.d_box, want[kind]);
+      else if (kind == 1)
+        k_gen_del<<<grid_for(draws), kBlock, 0, c.s>>>(c.g.view(), c.g.E, c.g.A, keys[1], s.draws_used[1], draws, s.set.p,
+                                                       s.mask, out, c.d_box, want[kind]);
+      else
+        k_gen_ins<<<grid_for(draws), kBlock, 0, c.s>>>(c.g.view(), c.g.sorted, keys[2], s.draws_used[2], draws, s.set.p,
+                                                       s.mask, out, c.d_box, want[kind]);
+      s.draws_used[kind] += (uint64_t)draws;
+      SCOUT_CHECK(box_read(c, 1));
+      got = std::min(c.h_box[0], want[kind]);
+    }
+    // A draw whose pair was claimed after the limit was reached stays claimed: such pairs are
+    // skipped by every later batch, which keeps the no-repeat rule.
+    produced += got;
+  }
+  SCOUT_TRY(hipEventRecord(ch.staged, c.s));
+  ch.m = produced;
+  if (out_m) *out_m = produced;
+  return SCOUT_OK;
+}
+
+// Keeps a device copy of the channel's batch in slot k (batches stay resident in HBM).
+int batch_store(Ctx& c, Channel& ch, int slot) {
+  if (!c.stream) c.stream = new StreamState();
+  StreamState& s = *c.stream;
+  if (slot < 0) {
+    set_error("batch_store: negative slot");
+    return SCOUT_ERR_ARG;
+  }
+  if ((size_t)slot >= s.slots.size()) {
+    s.slots.resize((size_t)slot + 1);
+    s.slot_m.resize((size_t)slot + 1, 0);
+  }
+  SCOUT_CHECK(s.slots[(size_t)slot].reserve(2 * ch.m + 2));
+  SCOUT_TRY(hipStreamWaitEvent(c.s, ch.staged, 0));
+  if (ch.m > 0)
+    SCOUT_TRY(hipMemcpyAsync(s.slots[(size_t)slot].p, ch.d_uv.p, (size_t)(2 * ch.m) * 4, hipMemcpyDeviceToDevice, c.s));
+  s.slot_m[(size_t)slot] = ch.m;
+  return SCOUT_OK;
+}
+
+// Stages stored batch `slot` into the channel with a device-to-device copy on the copy
+// stream (no PCIe transfer).
+int batch_load(Ctx& c, Channel& ch, int slot) {
+  if (!c.stream || slot < 0 || (size_t)slot >= c.stream->slots.size()) {
+    set_error("batch_load: no such slot");
+    return SCOUT_ERR_ARG;
+  }
+  int64_t m = c.stream->slot_m[(size_t)slot];
+  SCOUT_TRY(hipEventSynchronize(ch.staged));
+  SCOUT_CHECK(ch.d_uv.reserve(2 * m + 2));
+  SCOUT_CHECK(ch.d_ins.reserve(m + 1));
+  SCOUT_CHECK(ch.d_done.reserve(m + 1));
+  // the compute stream may still read this channel's previous batch
+  SCOUT_TRY(hipEventRecord(ch.staged, c.s));
+  SCOUT_TRY(hipStreamWaitEvent(c.cs, ch.staged, 0));
+  if (m > 0)
+    SCOUT_TRY(hipMemcpyAsync(ch.d_uv.p, c.stream->slots[(size_t)slot].p, (size_t)(2 * m) * 4, hipMemcpyDeviceToDevice,
+                             c.cs));
+  SCOUT_TRY(hipEventRecord(ch.staged, c.cs));
+  ch.m = m;
+  return SCOUT_OK;
+}
+
+void stream_release(Ctx& c) {
+  if (!c.stream) return;
+  for (auto& b : c.stream->slots) b.release();
+  c.stream->set.release();
+  delete c.stream;
+  c.stream = nullptr;
+}
+
+// Device copies of the graph and every allocated solution, for replaying a stream.
+struct Snapshot {
+  DevBuf<int64_t> boff, aoff;
+  DevBuf<int32_t> bcol, acol;
+  DevBuf<int32_t> level, label, color;
+  DevBuf<uint8_t> mis;
+  int64_t E = 0, A = 0, holes = 0;
+  bool sorted = false;
+  bool has_level = false, has_label = false, has_color = false, has_mis = false;
+};
+
+int snapshot_save(Ctx& c) {
+  if (!c.snap) c.snap = new Snapshot();
+  Snapshot& s = *c.snap;
+  int64_t n = c.g.n;
+  SCOUT_CHECK(s.boff.reserve(n + 1));
+  SCOUT_CHECK(s.aoff.reserve(n + 1));
+  SCOUT_CHECK(s.bcol.reserve(c.g.E + 1));
+  SCOUT_CHECK(s.acol.reserve(c.g.A + 1));
+  SCOUT_TRY(hipMemcpyAsync(s.boff.p, c.g.boff, (size_t)(n + 1) * 8, hipMemcpyDeviceToDevice, c.s));
+  SCOUT_TRY(hipMemcpyAsync(s.aoff.p, c.g.aoff.p, (size_t)(n + 1) * 8, hipMemcpyDeviceToDevice, c.s));
+  SCOUT_TRY(hipMemcpyAsync(s.bcol.p, c.g.bcol, (size_t)c.g.E * 4, hipMemcpyDeviceToDevice, c.s));
+  if (c.g.A > 0) SCOUT_TRY(hipMemcpyAsync(s.acol.p, c.g.acol.p, (size_t)c.g.A * 4, hipMemcpyDeviceToDevice, c.s));
+  s.E = c.g.E;
+  s.A = c.g.A;
+  s.holes = c.g.holes;
+  s.sorted = c.g.sorted;
+  auto save32 = [&](DevBuf<int32_t>& d, const int32_t* src, bool& has) -> int {
+    has = src != nullptr;
+    if (!src) return SCOUT_OK;
+    SCOUT_CHECK(d.reserve(n));
+    SCOUT_TRY(hipMemcpyAsync(d.p, src, (size_t)n * 4, hipMemcpyDeviceToDevice, c.s));
+    return SCOUT_OK;
+  };
+  SCOUT_CHECK(save32(s.level, c.level, s.has_level));
+  SCOUT_CHECK(save32(s.label, c.label, s.has_label));
+  SCOUT_CHECK(save32(s.color, c.color, s.has_color));
+  s.has_mis = c.mis != nullptr;
+  if (c.mis) {
+    SCOUT_CHECK(s.mis.reserve(n));
+    SCOUT_TRY(hipMemcpyAsync(s.mis.p, c.mis, (size_t)n, hipMemcpyDeviceToDevice, c.s));
+  }
+  SCOUT_TRY(hipStreamSynchronize(c.s));
+  return SCOUT_OK;
+}
+
+int snapshot_restore(Ctx& c) {
+  if (!c.snap) {
+    set_error("restore: no snapshot");
+    return SCOUT_ERR_STATE;
+  }
+  Snapshot& s = *c.snap;
+  int64_t n = c.g.n;
+  if (s.E != c.g.E) {
+    // the base was compacted since the snapshot: restore its size
+    (void)hipFree(c.g.bcol);
+    SCOUT_TRY(hipMalloc((void**)&c.g.bcol, (size_t)std::max<int64_t>(s.E, 1) * 4));
+  }
+  SCOUT_TRY(hipMemcpyAsync(c.g.boff, s.boff.p, (size_t)(n + 1) * 8, hipMemcpyDeviceToDevice, c.s));
+  SCOUT_TRY(hipMemcpyAsync(c.g.aoff.p, s.aoff.p, (size_t)(n + 1) * 8, hipMemcpyDeviceToDevice, c.s));
+  SCOUT_TRY(hipMemcpyAsync(c.g.bcol, s.bcol.p, (size_t)s.E * 4, hipMemcpyDeviceToDevice, c.s));
+  SCOUT_CHECK(c.g.acol.reserve(s.A + 1));
+  if (s.A > 0) SCOUT_TRY(hipMemcpyAsync(c.g.acol.p, s.acol.p, (size_t)s.A * 4, hipMemcpyDeviceToDevice, c.s));
+  c.g.E = s.E;
+  c.g.A = s.A;
+  c.g.holes = s.holes;
+  c.g.sorted = s.sorted;
+  if (s.has_level) {
+    SCOUT_CHECK(ensure_state(c, SCOUT_STATE_LEVEL));
+    SCOUT_TRY(hipMemcpyAsync(c.level, s.level.p, (size_t)n * 4, hipMemcpyDeviceToDevice, c.s));
+  }
+  if (s.has_label) {
+    SCOUT_CHECK(ensure_state(c, SCOUT_STATE_LABEL));
+    SCOUT_TRY(hipMemcpyAsync(c.label, s.label.p, (size_t)n * 4, hipMemcpyDeviceToDevice, c.s));
+  }
+  if (s.has_color) {
+    SCOUT_CHECK(ensure_state(c, SCOUT_STATE_COLOR));
+    SCOUT_TRY(hipMemcpyAsync(c.color, s.color.p, (size_t)n * 4, hipMemcpyDeviceToDevice, c.s));
+  }
+  if (s.has_mis) {
+    SCOUT_CHECK(ensure_state(c, SCOUT_STATE_MIS));
+    SCOUT_TRY(hipMemcpyAsync(c.mis, s.mis.p, (size_t)n, hipMemcpyDeviceToDevice, c.s));
+  }
+  SCOUT_TRY(hipStreamSynchronize(c.s));
+  return SCOUT_OK;
+}
+
+void snapshot_release(Ctx& c) {
+  if (!c.snap) return;
+  Snapshot& s = *c.snap;
+  s.boff.release();
+  s.aoff.release();
+  s.bcol.release();
+  s.acol.release();
+  s.level.release();
+  s.label.release();
+  s.color.release();
+  s.mis.release();
+  delete c.snap;
+  c.snap = nullptr;
+}
+
+}  // namespace scout
+
+using namespace scout;
+
+extern "C" {
+
+int scout_gen_batch(scout_ctx* x, int channel, int64_t size, double delete_frac, double tree_frac, uint64_t seed,
+                    int64_t expected_total, int64_t* out_m) {
+  if (!x || size < 0) return SCOUT_ERR_ARG;
+  Ctx& c = x->c;
+  SCOUT_TRY(hipSetDevice(c.dev));
+  Channel* ch = channel_of(c, channel);
+  if (!ch) return SCOUT_ERR_ARG;
+  if (!c.g.boff) {
+    set_error("gen_batch: no graph loaded");
+    return SCOUT_ERR_STATE;
+  }
+  return gen_batch(c, *ch, size, delete_frac, tree_frac, seed, expected_total, out_m);
+}
+
+int scout_batch_download(scout_ctx* x, int channel, int32_t* uv, int64_t cap_lines) {
+  if (!x) return SCOUT_ERR_ARG;
+  Ctx& c = x->c;
+  SCOUT_TRY(hipSetDevice(c.dev));
+  Channel* ch = channel_of(c, channel);
+  if (!ch) return SCOUT_ERR_ARG;
+  int64_t m = std::min(ch->m, cap_lines);
+  SCOUT_TRY(hipStreamWaitEvent(c.s, ch->staged, 0));
+  if (m > 0) SCOUT_TRY(hipMemcpyAsync(uv, ch->d_uv.p, (size_t)(2 * m) * 4, hipMemcpyDeviceToHost, c.s));
+  SCOUT_TRY(hipStreamSynchronize(c.s));
+  return SCOUT_OK;
+}
+
+int scout_batch_store(scout_ctx* x, int channel, int slot) {
+  if (!x) return SCOUT_ERR_ARG;
+  SCOUT_TRY(hipSetDevice(x->c.dev));
+  Channel* ch = channel_of(x->c, channel);
+  if (!ch) return SCOUT_ERR_ARG;
+  return batch_store(x->c, *ch, slot);
+}
+
+int scout_batch_load(scout_ctx* x, int channel, int slot) {
+  if (!x) return SCOUT_ERR_ARG;
+  SCOUT_TRY(hipSetDevice(x->c.dev));
+  Channel* ch = channel_of(x->c, channel);
+  if (!ch) return SCOUT_ERR_ARG;
+  return batch_load(x->c, *ch, slot);
+}
+
+int scout_snapshot(scout_ctx* x) {
+  if (!x) return SCOUT_ERR_ARG;
+  SCOUT_TRY(hipSetDevice(x->c.dev));
+  return snapshot_save(x->c);
+}
+
+int scout_restore(scout_ctx* x) {
+  if (!x) return SCOUT_ERR_ARG;
+  SCOUT_TRY(hipSetDevice(x->c.dev));
+  return snapshot_restore(x->c);
+}
+
+}  // extern "C"

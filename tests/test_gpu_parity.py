@@ -1,0 +1,289 @@
+"""GPU parity: libscout_hip (through its C ABI) against the CPU oracle, batch by batch.
+
+Bit-exact on every batch: classification labels, BFS levels, CC min-id labels, colours, MIS
+membership, and the logical graph (compaction order). Graphs and streams come from the
+host generator (scout_gen) with fixed seeds; sizes stay small enough for the oracle.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import run_gen
+from oracle_lib import OracleGraph, mis_max_csr, read_batch
+import scout_amd as sa
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def data(tmp_path_factory):
+    root = tmp_path_factory.mktemp("gpu_data")
+    d = {}
+    # small ER with a mixed stream
+    er = root / "er.egr"
+    run_gen("er", 3000, 20000, 0, er)
+    run_gen("stream", "mixed", er, root / "er_b", 5, 800, 11)
+    d["er"] = (str(er), str(root / "er_b"), 5)
+    # R-MAT with deletions drawn from BFS-tree edges (70% deletions)
+    rm = root / "rmat.egr"
+    run_gen("rmat", 12, 16, 3, rm)
+    run_gen("stream", "tree", rm, root / "rmat_b", 5, 3000, 12, 0.7, 0.5)
+    d["rmat"] = (str(rm), str(root / "rmat_b"), 5)
+    # degree-biased stream (hub endpoints)
+    run_gen("stream", "degree", rm, root / "deg_b", 4, 2000, 13)
+    d["deg"] = (str(rm), str(root / "deg_b"), 4)
+    # the reference's bundled sample (deletion-only batches)
+    sample = os.path.join(os.path.dirname(__file__), "golden", "sample")
+    d["sample"] = (os.path.join(sample, "sample.egr"), os.path.join(sample, "batches"), 5)
+    return d
+
+
+def batches(bdir, count, n):
+    for k in range(1, count + 1):
+        yield k, read_batch(os.path.join(bdir, f"{k}.mtx"), n)
+
+
+def engine_for(egr):
+    g = sa.Graph.read_egr(egr)
+    eng = sa.Engine(0)
+    eng.load_graph(g)
+    return eng, g
+
+
+@pytest.mark.parametrize("name", ["sample", "er", "rmat", "deg"])
+def test_bfs_cc_exact_every_batch(data, name):
+    egr, bdir, nb = data[name]
+    say(f"[bfs+cc] {name}")
+    eng, g = engine_for(egr)
+    og = OracleGraph(egr)
+    eng.upload(sa.STATE_LEVEL, og.bfs(0))
+    eng.upload(sa.STATE_LABEL, og.cc())
+    for k, uv in batches(bdir, nb, g.n):
+        eng.stage(uv)
+        ins = eng.classify(sa.CLASSIFY_CURRENT)
+        ins_o = og.classify(uv)
+        assert np.array_equal(ins, ins_o), f"classification differs in batch {k}"
+        eng.apply(False)
+        og.apply(uv, ins_o)
+        eng.repair_bfs()
+        eng.repair_cc()
+        lv, lab = eng.download(sa.STATE_LEVEL), eng.download(sa.STATE_LABEL)
+        want_lv, want_lab = og.bfs(0), og.cc()
+        assert np.array_equal(lv, want_lv), f"{name} batch {k}: {np.count_nonzero(lv != want_lv)} levels differ"
+        assert np.array_equal(lab, want_lab), f"{name} batch {k}: {np.count_nonzero(lab != want_lab)} labels differ"
+    off, col = og.logical_csr()
+    dg = eng.download_graph()
+    assert np.array_equal(dg.off, off) and np.array_equal(dg.col, col), "logical graph differs"
+    eng.close()
+
+
+@pytest.mark.parametrize("name", ["er", "rmat"])
+def test_recompute_strategy_equals_local(data, name):
+    egr, bdir, nb = data[name]
+    say(f"[strategies] {name}")
+    og = OracleGraph(egr)
+    lv0, lab0 = og.bfs(0), og.cc()
+    results = []
+    for strat in (sa.STRATEGY_LOCAL, sa.STRATEGY_RECOMPUTE):
+        eng, g = engine_for(egr)
+        eng.upload(sa.STATE_LEVEL, lv0)
+        eng.upload(sa.STATE_LABEL, lab0)
+        out = []
+        for k, uv in batches(bdir, nb, g.n):
+            eng.stage(uv)
+            eng.classify()
+            eng.apply()
+            eng.repair_bfs(strat)
+            eng.repair_cc(strat)
+            out.append((eng.download(sa.STATE_LEVEL), eng.download(sa.STATE_LABEL)))
+        eng.close()
+        results.append(out)
+    for (a1, b1), (a2, b2) in zip(*results):
+        assert np.array_equal(a1, a2) and np.array_equal(b1, b2)
+
+
+@pytest.mark.parametrize("name", ["sample", "er", "rmat", "deg"])
+def test_coloring_exact_every_batch(data, name):
+    egr, bdir, nb = data[name]
+    say(f"[coloring] {name}")
+    eng, g = engine_for(egr)
+    og = OracleGraph(egr)
+    color = og.greedy_color()
+    eng.upload(sa.STATE_COLOR, color)
+    for k, uv in batches(bdir, nb, g.n):
+        eng.stage(uv)
+        ins = eng.classify()
+        eng.apply()
+        og.apply(uv, ins)
+        st = eng.repair_color()
+        color, ost = og.color_batch(uv, ins, color)
+        got = eng.download(sa.STATE_COLOR)
+        assert np.array_equal(got, color), f"{name} batch {k}: {np.count_nonzero(got != color)} colours differ"
+        assert st.conflicts == ost[0] and st.rounds == ost[1]
+        assert og.check_coloring(got) == 0
+
+
+@pytest.mark.parametrize("name", ["sample", "er", "rmat", "deg"])
+def test_mis_exact_every_batch(data, name):
+    egr, bdir, nb = data[name]
+    say(f"[mis] {name}")
+    eng, g = engine_for(egr)
+    og = OracleGraph(egr)
+    base = OracleGraph(egr)       # the initial graph: MIS labels every batch against it
+    mis = og.greedy_mis()
+    eng.upload(sa.STATE_MIS, mis)
+    avg = g.off[-1] / g.n
+    max_csr = mis_max_csr(avg, 288e9)
+    for k, uv in batches(bdir, nb, g.n):
+        eng.stage(uv)
+        ins = eng.classify(sa.CLASSIFY_BASE)
+        assert np.array_equal(ins, base.classify(uv))
+        eng.apply(True)
+        og.apply(uv, ins, conditional=True)
+        st = eng.repair_mis(max_csr)
+        mis, ost = og.mis_batch(uv, ins, mis, max_csr)
+        got = eng.download(sa.STATE_MIS)
+        assert np.array_equal(got, mis), f"{name} batch {k}: {np.count_nonzero(got != mis)} memberships differ"
+        assert st.sub_batches == ost[0]
+        assert og.check_mis_independent(got) == 0
+
+
+def test_mis_small_subbatches_match(data):
+    # a tiny MAX_CSR_ELEMENTS cuts every batch into many sub-batches with different rows
+    egr, bdir, nb = data["er"]
+    say("[mis] sub-batches")
+    eng, g = engine_for(egr)
+    og = OracleGraph(egr)
+    mis = og.greedy_mis()
+    eng.upload(sa.STATE_MIS, mis)
+    for k, uv in batches(bdir, 2, g.n):
+        eng.stage(uv)
+        ins = eng.classify(sa.CLASSIFY_BASE)
+        eng.apply(True)
+        og.apply(uv, ins, conditional=True)
+        st = eng.repair_mis(200000)
+        mis, ost = og.mis_batch(uv, ins, mis, 200000)
+        assert st.sub_batches == ost[0] and st.sub_batches > 1
+        assert np.array_equal(eng.download(sa.STATE_MIS), mis)
+
+
+def test_repeated_pairs_in_one_batch(data):
+    egr, _, _ = data["er"]
+    say("[graph] repeated pairs")
+    eng, g = engine_for(egr)
+    og = OracleGraph(egr)
+    eng.upload(sa.STATE_LEVEL, og.bfs(0))
+    eng.upload(sa.STATE_LABEL, og.cc())
+    rng = np.random.default_rng(5)
+    edges = np.stack([np.repeat(np.arange(g.n), np.diff(g.off)), g.col], 1)
+    pick = edges[rng.integers(0, len(edges), 300)]
+    rand = rng.integers(0, g.n, (300, 2))
+    uv = np.concatenate([pick, rand, pick[:100, ::-1], rand[:100]]).astype(np.int32)
+    uv = uv[uv[:, 0] != uv[:, 1]]
+    for rep in range(3):
+        eng.stage(uv)
+        ins = eng.classify()
+        assert np.array_equal(ins, og.classify(uv))
+        eng.apply()
+        og.apply(uv, ins)
+        eng.repair_bfs()
+        eng.repair_cc()
+        assert np.array_equal(eng.download(sa.STATE_LEVEL), og.bfs(0))
+        assert np.array_equal(eng.download(sa.STATE_LABEL), og.cc())
+        dg = eng.download_graph()
+        rows_o = [og.row(v) for v in range(g.n)]
+        rows_g = [dg.col[dg.off[v]:dg.off[v + 1]] for v in range(g.n)]
+        assert all(np.array_equal(a, b) for a, b in zip(rows_o, rows_g)), "rows differ"
+
+
+def test_device_generator_matches_host(tmp_path):
+    say("[gen] rmat scale 12")
+    p = tmp_path / "h.egr"
+    run_gen("rmat", 12, 16, 21, p)
+    host = sa.Graph.read_egr(str(p))
+    eng = sa.Engine(0)
+    eng.gen_rmat(12, 16, seed=21)
+    dev = eng.download_graph()
+    assert np.array_equal(dev.off, host.off) and np.array_equal(dev.col, host.col)
+    # greedy initial states and full recomputes on the device
+    og = OracleGraph(str(p))
+    eng.mis_greedy()
+    eng.color_greedy()
+    eng.bfs_full(0)
+    eng.cc_full()
+    assert np.array_equal(eng.download(sa.STATE_MIS), og.greedy_mis())
+    assert np.array_equal(eng.download(sa.STATE_COLOR), og.greedy_color())
+    assert np.array_equal(eng.download(sa.STATE_LEVEL), og.bfs(0))
+    assert np.array_equal(eng.download(sa.STATE_LABEL), og.cc())
+    eng.close()
+
+
+def test_compaction_on_device_keeps_logical_graph(data):
+    egr, bdir, nb = data["rmat"]
+    say("[graph] compaction")
+    eng, g = engine_for(egr)
+    og = OracleGraph(egr)
+    for k, uv in batches(bdir, 2, g.n):
+        eng.stage(uv)
+        ins = eng.classify()
+        eng.apply()
+        og.apply(uv, ins)
+    before = eng.download_graph()
+    eng.compact()
+    after = eng.download_graph()
+    off, col = og.logical_csr()
+    assert np.array_equal(after.off, off) and np.array_equal(after.col, col)
+    assert np.array_equal(before.col, after.col)
+    info = eng.graph_info()
+    assert info["inserted_entries"] == 0 and info["base_entries"] == len(col)
+
+
+@pytest.mark.parametrize("prog,init_name", [("bfs", "sample_bfs.txt"), ("cc", "sample_components.txt"),
+                                             ("coloring", "sample_colors.txt"), ("mis", "sample_mis.txt")])
+def test_binaries_on_sample_dump_exact_states(tmp_path, data, prog, init_name):
+    egr, bdir, nb = data["sample"]
+    say(f"[binary] scout_{prog}")
+    work = tmp_path / "w"
+    work.mkdir()
+    local = work / "sample.egr"
+    local.write_bytes(open(egr, "rb").read())
+    dump = work / "dump"
+    env = dict(os.environ, SCOUT_DUMP_STATE=str(dump), SCOUT_SKIP_COMPACTION="1")
+    init = os.path.join(os.path.dirname(egr), init_name)
+    r = subprocess.run([os.path.join(sa.BIN_DIR, f"scout_{prog}"), str(local), init, bdir, "5", "0"],
+                       capture_output=True, text=True, env=env, cwd=work, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert r.stdout.count("Batch ") >= 5 and "=== FINAL SUMMARY ===" in r.stdout
+    og = OracleGraph(egr)
+    base = OracleGraph(egr)
+    n = og.n
+    vals = np.loadtxt(init, dtype=np.int64)
+    if prog == "bfs":
+        state = np.full(n, -1, np.int32)
+        state[vals[:, 0] - 1] = vals[:, 1]
+    elif prog == "cc":
+        state = vals.astype(np.int32)
+    elif prog == "coloring":
+        state = vals.astype(np.int32)
+    else:
+        state = np.zeros(n, np.uint8)
+        state[vals - 1] = 1
+    for k, uv in batches(bdir, 5, n):
+        ins = (base if prog == "mis" else og).classify(uv)
+        og.apply(uv, ins, conditional=(prog == "mis"))
+        if prog == "bfs":
+            state = og.bfs(0)
+        elif prog == "cc":
+            state = og.cc()
+        elif prog == "coloring":
+            state, _ = og.color_batch(uv, ins, state)
+        else:
+            state, _ = og.mis_batch(uv, ins, state, mis_max_csr(8.0, 288e9))
+        got = np.fromfile(dump / f"{prog}_{k}.bin", dtype=np.uint8 if prog == "mis" else np.int32)
+        assert np.array_equal(got, state), f"scout_{prog} batch {k}"
