@@ -234,7 +234,7 @@ __global__ void k_set_one(int32_t* list, int32_t v, int64_t* count) {
 // Level-synchronous relax starting at level `lvl0` with the current list already in `cur`
 // (count at box slot `cur_slot`) and optional seeds; runs until no frontier and no seed
 // above the current level remains. Returns the rounds run via *rounds.
-int relax_levels(Ctx& c, int32_t lvl0, int32_t max_seed, const int64_t* seeds, int64_t nseeds, int32_t epoch,
+int relax_levels(Ctx& c, int32_t* L, int32_t lvl0, int32_t max_seed, const int64_t* seeds, int64_t nseeds, int32_t epoch,
                  int64_t* rounds, int64_t* scanned_total) {
   int32_t n = c.g.n;
   int64_t cap = (int64_t)n + 1;
@@ -252,12 +252,12 @@ int relax_levels(Ctx& c, int32_t lvl0, int32_t max_seed, const int64_t* seeds, i
   int64_t scanned_before = 0;
   while (true) {
     if (nseeds > 0 && lvl <= max_seed)
-      k_seed_at_level<<<grid_for(nseeds), kBlock, 0, c.s>>>(seeds, nseeds, c.level, lvl, cur, bcur, cap, c.stamp, epoch);
+      k_seed_at_level<<<grid_for(nseeds), kBlock, 0, c.s>>>(seeds, nseeds, L, lvl, cur, bcur, cap, c.stamp, epoch);
     SCOUT_TRY(hipMemsetAsync(bnext, 0, 2 * sizeof(int64_t), c.s));
     SCOUT_CHECK(prof_begin(c, PROF_BFS_RELAX));
-    k_relax<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), c.level, cur, bcur, lvl, nxt, bnext, cap, c.stamp, epoch,
+    k_relax<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, cur, bcur, lvl, nxt, bnext, cap, c.stamp, epoch,
                                             c.wb.p, btask, task_cap, bscan);
-    k_relax_tasks<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), c.level, c.wb.p, btask, task_cap, lvl, nxt, bnext, cap,
+    k_relax_tasks<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, c.wb.p, btask, task_cap, lvl, nxt, bnext, cap,
                                                   c.stamp, epoch, bscan);
     SCOUT_CHECK(prof_end(c, PROF_BFS_RELAX, 0));
     r++;
@@ -287,7 +287,6 @@ int relax_levels(Ctx& c, int32_t lvl0, int32_t max_seed, const int64_t* seeds, i
 
 int ensure_bfs(Ctx& c, int64_t extra) {
   int64_t n = c.g.n;
-  SCOUT_CHECK(ensure_state(c, SCOUT_STATE_LEVEL));
   SCOUT_CHECK(ensure_scratch(c, &c.bfs_mark, 0));
   SCOUT_CHECK(c.la.reserve(n + 2 * extra + 1));
   SCOUT_CHECK(c.lb.reserve(n + 1));
@@ -302,21 +301,21 @@ int ensure_bfs(Ctx& c, int64_t extra) {
 
 }  // namespace
 
-int bfs_full(Ctx& c, int32_t src, int64_t* rounds, int64_t* scanned) {
+int bfs_full_on(Ctx& c, int32_t* L, int32_t src, int64_t* rounds, int64_t* scanned) {
   SCOUT_CHECK(ensure_bfs(c, 0));
   int32_t n = c.g.n;
   if (n == 0) return SCOUT_OK;
-  k_fill_level<<<grid_for(n), kBlock, 0, c.s>>>(c.level, n, src);
+  k_fill_level<<<grid_for(n), kBlock, 0, c.s>>>(L, n, src);
   int32_t epoch = next_epoch(c);
   k_set_one<<<1, 1, 0, c.s>>>(c.lb.p, src, c.d_box + 8);
   int64_t r = 0, sc = 0;
-  SCOUT_CHECK(relax_levels(c, 0, -1, nullptr, 0, epoch, &r, &sc));
+  SCOUT_CHECK(relax_levels(c, L, 0, -1, nullptr, 0, epoch, &r, &sc));
   if (rounds) *rounds += r;
   if (scanned) *scanned += sc;
   return SCOUT_OK;
 }
 
-int bfs_local(Ctx& c, Channel& ch, scout_stats* st) {
+int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, scout_stats* st) {
   SCOUT_CHECK(ensure_bfs(c, ch.m));
   int64_t m = ch.m;
   int32_t n = c.g.n;
@@ -327,7 +326,7 @@ int bfs_local(Ctx& c, Channel& ch, scout_stats* st) {
   SCOUT_TRY(hipMemcpyAsync(box, init, sizeof(init), hipMemcpyHostToDevice, c.s));
   SCOUT_TRY(hipMemsetAsync(box + 8, 0, 8 * sizeof(int64_t), c.s));
   if (m > 0)
-    k_del_candidates<<<grid_for(m), kBlock, 0, c.s>>>(ch.d_uv.p, ch.d_ins.p, m, c.level, c.la.p, c.la.cap, box);
+    k_del_candidates<<<grid_for(m), kBlock, 0, c.s>>>(ch.d_uv.p, ch.d_ins.p, m, L, c.la.p, c.la.cap, box);
   SCOUT_CHECK(box_read(c, 8));
   int64_t ncand = c.h_box[0];
   int32_t cmin = (int32_t)c.h_box[1], cmax = (int32_t)c.h_box[2];
@@ -346,10 +345,10 @@ int bfs_local(Ctx& c, Channel& ch, scout_stats* st) {
     SCOUT_TRY(hipMemsetAsync(box + 8, 0, 4 * sizeof(int64_t), c.s));
     for (int32_t lvl = cmin;; lvl++) {
       if (lvl <= cmax)
-        k_cand_at_level<<<grid_for(ncand), kBlock, 0, c.s>>>(c.la.p, ncand, c.level, lvl, cur, bcur, cap, c.stamp,
+        k_cand_at_level<<<grid_for(ncand), kBlock, 0, c.s>>>(c.la.p, ncand, L, lvl, cur, bcur, cap, c.stamp,
                                                               sepoch);
       SCOUT_TRY(hipMemsetAsync(bnext, 0, sizeof(int64_t), c.s));
-      k_invalidate<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), c.level, c.bfs_mark, mepoch, cur, bcur, lvl, c.ld.p,
+      k_invalidate<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, c.bfs_mark, mepoch, cur, bcur, lvl, c.ld.p,
                                                    binval, cap, nxt, bnext, cap, c.stamp, sepoch, bscan);
       rounds++;
       SCOUT_TRY(hipMemcpyAsync(c.h_box + 8, box + 8, 4 * sizeof(int64_t), hipMemcpyDeviceToHost, c.s));
@@ -362,14 +361,14 @@ int bfs_local(Ctx& c, Channel& ch, scout_stats* st) {
     nx = c.h_box[10];
     scanned += c.h_box[11];
     if (nx > 0) {
-      k_reset_levels<<<grid_for(nx), kBlock, 0, c.s>>>(c.ld.p, nx, c.level);
+      k_reset_levels<<<grid_for(nx), kBlock, 0, c.s>>>(c.ld.p, nx, L);
       SCOUT_TRY(hipMemsetAsync(box + 11, 0, sizeof(int64_t), c.s));
-      k_tentative<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), c.level, c.ld.p, nx, c.le.p, box + 11);
-      k_apply_tentative<<<grid_for(nx), kBlock, 0, c.s>>>(c.ld.p, c.le.p, nx, c.level, c.wa.p, c.wa.cap, box);
+      k_tentative<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, c.ld.p, nx, c.le.p, box + 11);
+      k_apply_tentative<<<grid_for(nx), kBlock, 0, c.s>>>(c.ld.p, c.le.p, nx, L, c.wa.p, c.wa.cap, box);
     }
   }
   if (m > 0)
-    k_insert_seeds<<<grid_for(m), kBlock, 0, c.s>>>(c.g.view(), c.g.sorted, ch.d_uv.p, ch.d_ins.p, m, c.level, c.wa.p,
+    k_insert_seeds<<<grid_for(m), kBlock, 0, c.s>>>(c.g.view(), c.g.sorted, ch.d_uv.p, ch.d_ins.p, m, L, c.wa.p,
                                                     c.wa.cap, box);
   SCOUT_CHECK(box_read(c, 12));
   if (nx > 0) scanned += c.h_box[11];
@@ -385,11 +384,24 @@ int bfs_local(Ctx& c, Channel& ch, scout_stats* st) {
     int32_t smin = (int32_t)c.h_box[5], smax = (int32_t)c.h_box[6];
     int32_t epoch = next_epoch(c);
     SCOUT_TRY(hipMemsetAsync(box + 8, 0, sizeof(int64_t), c.s));
-    SCOUT_CHECK(relax_levels(c, smin, smax, c.wa.p, nseeds, epoch, &rounds, &scanned));
+    SCOUT_CHECK(relax_levels(c, L, smin, smax, c.wa.p, nseeds, epoch, &rounds, &scanned));
   }
   st->rounds += rounds;
   st->edges_scanned += scanned;
   st->noop += (ncand == 0 && nseeds == 0) ? m : 0;
+  return SCOUT_OK;
+}
+
+int bfs_full(Ctx& c, int32_t src, int64_t* rounds, int64_t* scanned) {
+  SCOUT_CHECK(ensure_state(c, SCOUT_STATE_LEVEL));
+  SCOUT_CHECK(bfs_full_on(c, c.level, src, rounds, scanned));
+  c.level_seq = c.apply_seq;
+  return SCOUT_OK;
+}
+
+int bfs_local(Ctx& c, Channel& ch, scout_stats* st) {
+  SCOUT_CHECK(bfs_local_on(c, ch, c.level, st));
+  c.level_seq = c.apply_seq;
   return SCOUT_OK;
 }
 

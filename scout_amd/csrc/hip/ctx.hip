@@ -45,9 +45,12 @@ int alloc_n(Ctx& c, void** p, size_t elem) {
 
 void free_state(Ctx& c) {
   void* ps[] = {c.level, c.label, c.color, c.mis, c.stamp, c.aux, c.aux2,
-                c.bfs_mark, c.cc_owner, c.col_marker, c.mis_minw, c.mis_tag};
+                c.bfs_mark, c.cc_owner, c.col_marker, c.mis_minw, c.mis_tag, c.cc_level};
   for (void* p : ps)
     if (p) (void)hipFree(p);
+  c.cc_level = nullptr;
+  c.level_seq = c.cc_level_seq = -1;
+  c.cc_components = -1;
   c.level = c.label = c.color = nullptr;
   c.mis = nullptr;
   c.stamp = c.aux = c.aux2 = nullptr;
@@ -385,6 +388,9 @@ int scout_state_upload(scout_ctx* x, int kind, const void* host, int64_t n) {
   size_t elem = kind == SCOUT_STATE_MIS ? 1 : 4;
   SCOUT_TRY(hipMemcpyAsync(dst, host, (size_t)n * elem, hipMemcpyHostToDevice, c.s));
   SCOUT_TRY(hipStreamSynchronize(c.s));
+  // uploaded levels are taken as exact for the current graph
+  if (kind == SCOUT_STATE_LEVEL) c.level_seq = c.apply_seq;
+  if (kind == SCOUT_STATE_LABEL) c.cc_components = -1;
   return SCOUT_OK;
 }
 

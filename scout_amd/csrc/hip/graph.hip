@@ -453,6 +453,7 @@ int graph_classify(Ctx& c, Channel& ch, int rule) {
 int graph_apply(Ctx& c, Channel& ch, int conditional) {
   DevGraph& g = c.g;
   int64_t m = ch.m;
+  c.apply_seq++;
   if (m == 0) return SCOUT_OK;
   // Find lines whose pair already occurred earlier in the batch.
   uint64_t size = 1;

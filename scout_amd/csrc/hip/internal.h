@@ -161,6 +161,11 @@ struct Ctx {
   int32_t* col_marker = nullptr; // this round's proposal (-2 = none)
   int32_t* mis_minw = nullptr;   // smallest speculative writer line
   int32_t* mis_tag = nullptr;    // row kind stamps
+  int32_t* cc_level = nullptr;   // BFS levels from vertex 0 kept for CC when BFS is not maintained
+  int64_t apply_seq = 0;         // batches applied to the graph
+  int64_t level_seq = -1;        // apply_seq at which `level` was last made exact
+  int64_t cc_level_seq = -1;     // same for cc_level
+  int64_t cc_components = -1;    // component count after the last CC repair
   // small mailbox read back by the host after a round
   int64_t* h_box = nullptr;    // pinned
   int64_t* d_box = nullptr;
@@ -203,6 +208,8 @@ int graph_free(Ctx& c);
 
 // per-problem modules
 int bfs_full(Ctx& c, int32_t src, int64_t* rounds, int64_t* scanned);
+int bfs_full_on(Ctx& c, int32_t* L, int32_t src, int64_t* rounds, int64_t* scanned);
+int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, scout_stats* st);
 int bfs_local(Ctx& c, Channel& ch, scout_stats* st);
 int cc_full(Ctx& c, int64_t* scanned);
 int cc_local(Ctx& c, Channel& ch, scout_stats* st);
