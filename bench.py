@@ -137,6 +137,8 @@ def main():
     eng.prof_enable(True)
     lat = []
     lines = 0
+    work = {"bfs_invalidated": 0, "bfs_seeds": 0, "bfs_rounds": 0, "bfs_entries_scanned": 0,
+            "cc_entries_scanned": 0, "insertions": 0, "deletions": 0}
     eng.sync()
     t_start = time.perf_counter()
     for b in range(args.warmup, total):
@@ -149,6 +151,13 @@ def main():
         eng.sync()
         lat.append((time.perf_counter() - ts) * 1e3)
         lines += sb.lines
+        work["bfs_invalidated"] += sb.touched
+        work["bfs_seeds"] += sb.seeds
+        work["bfs_rounds"] += sb.rounds
+        work["bfs_entries_scanned"] += sb.edges_scanned
+        work["cc_entries_scanned"] += sc.edges_scanned
+        work["insertions"] += sb.insertions
+        work["deletions"] += sb.deletions
     eng.sync()
     elapsed = time.perf_counter() - t_start
     prof = {name: eng.prof_read(i) for i, name in enumerate(sa.PROF_NAMES)}
@@ -203,6 +212,7 @@ def main():
         "kernels": {k: {"ms": round(v["ms"], 3), "launches": v["launches"],
                         "GBps": round(v["bytes"] / (v["ms"] / 1e3) / 1e9, 1) if v["ms"] > 0 else 0.0}
                     for k, v in prof.items() if v["launches"]},
+        "work_per_batch": {k: round(v / args.steps, 1) for k, v in work.items()},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -92,30 +92,95 @@ __global__ void k_relax_tasks(GView g, int32_t* level, const int64_t* tasks, con
 __global__ void k_del_candidates(const int32_t* uv, const uint8_t* ins, int64_t m, const int32_t* level,
                                  int32_t* cand, int64_t cap, int64_t* box) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= m || ins[i]) return;
-  atomicAdd((unsigned long long*)&box[3], 1ULL);
-  int32_t u = uv[2 * i], v = uv[2 * i + 1];
-  int32_t lu = level[u], lv = level[v];
+  bool del = i < m && !ins[i];
   int32_t c = -1, lc = 0;
-  if (lu >= 0 && lv == lu + 1) c = v, lc = lv;
-  else if (lv >= 0 && lu == lv + 1) c = u, lc = lu;
-  if (c < 0) return;
-  thread_append(c, cand, &box[0], cap);
-  atomicMin((long long*)&box[1], (long long)lc);
-  atomicMax((long long*)&box[2], (long long)lc);
+  if (del) {
+    int32_t u = uv[2 * i], v = uv[2 * i + 1];
+    int32_t lu = level[u], lv = level[v];
+    if (lu >= 0 && lv == lu + 1) c = v, lc = lv;
+    else if (lv >= 0 && lu == lv + 1) c = u, lc = lu;
+  }
+  block_append(c >= 0, c, cand, &box[0], cap);
+  block_add(del ? 1 : 0, &box[3]);
+  block_min_max(c >= 0 ? lc : INT64_MAX, c >= 0 ? lc : -1, &box[1], &box[2]);
 }
 
 // Candidates whose level is `lvl`, appended to the current list once (stamp dedup).
 __global__ void k_cand_at_level(const int32_t* cand, int64_t nc, const int32_t* level, int32_t lvl, int32_t* cur,
                                 int64_t* cur_count, int64_t cap, int32_t* stamp, int32_t epoch) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nc) return;
-  int32_t c = cand[i];
-  if (level[c] == lvl && atomicExch(&stamp[c], epoch) != epoch) thread_append(c, cur, cur_count, cap);
+  int32_t c = i < nc ? cand[i] : -1;
+  bool take = c >= 0 && level[c] == lvl && atomicExch(&stamp[c], epoch) != epoch;
+  block_append(take, c, cur, cur_count, cap);
 }
 
-// Wave per candidate at level lvl: keep it if some neighbour at lvl-1 is still valid,
-// otherwise mark it invalid and queue its neighbours at lvl+1.
+// Invalidation of the candidates at level lvl, by row length:
+//   <= kShort  thread per candidate (lanes step through the row together, so appends stay
+//              wave-aggregated);
+//   <= kChunk  wave per candidate (k_invalidate);
+//   longer     chunk tasks: the parent search and the push of dependants are spread over
+//              many waves (k_inval_search, k_inval_decide, k_inval_push).
+// A candidate stays valid if some neighbour at lvl-1 is still valid; otherwise it is marked
+// invalid and its neighbours at lvl+1 are queued once (stamp dedup).
+constexpr int kShort = 16;
+
+__device__ __forceinline__ bool valid_parent(const int32_t* level, const int32_t* mark, int32_t mepoch, int32_t x,
+                                             int32_t lvl) {
+  return level[x] == lvl - 1 && mark[x] != mepoch;
+}
+
+__global__ void k_invalidate_short(GView g, const int32_t* level, int32_t* mark, int32_t mepoch, const int32_t* cur,
+                                   const int64_t* cur_count, int32_t lvl, int32_t* inval, int64_t* inval_count,
+                                   int32_t* next, int64_t* next_count, int64_t cap, int32_t* stamp, int32_t epoch,
+                                   int32_t* medium, int64_t* medium_count, int32_t* longs, int64_t* long_count,
+                                   int64_t* scanned) {
+  int64_t nc = *cur_count;
+  int64_t work = 0;
+  // the trip count is uniform across the wave: every lane loops until the wave is done
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < nc; base += (int64_t)gridDim.x * blockDim.x) {
+    int64_t i = base + threadIdx.x;
+    int32_t c = -1;
+    Row r{0, 0, 0, 0};
+    int64_t len = 0;
+    bool is_short = false;
+    if (i < nc) {
+      c = cur[i];
+      r = row_of(g, c);
+      len = row_len(r);
+      is_short = len <= kShort;
+    }
+    bool invalid = false;
+    if (is_short) {
+      invalid = true;
+      for (int64_t k = 0; k < len; k++) {
+        int32_t x = row_at(g, r, k);
+        if (x >= 0 && valid_parent(level, mark, mepoch, x, lvl)) {
+          invalid = false;
+          work += k + 1;
+          break;
+        }
+      }
+      if (invalid) {
+        work += len;
+        mark[c] = mepoch;
+      }
+    }
+    wave_append(invalid, c, inval, inval_count, cap);
+    wave_append(i < nc && !is_short && len <= kChunk, c, medium, medium_count, cap);
+    wave_append(i < nc && len > kChunk, c, longs, long_count, cap);
+    for (int k = 0; k < kShort; k++) {
+      bool take = false;
+      int32_t w = -1;
+      if (invalid && k < len) {
+        w = row_at(g, r, k);
+        take = w >= 0 && level[w] == lvl + 1 && atomicExch(&stamp[w], epoch) != epoch;
+      }
+      wave_append(take, w, next, next_count, cap);
+    }
+  }
+  block_add(work, scanned);
+}
+
 __global__ void k_invalidate(GView g, const int32_t* level, int32_t* mark, int32_t mepoch, const int32_t* cur,
                              const int64_t* cur_count, int32_t lvl, int32_t* inval, int64_t* inval_count,
                              int64_t inval_cap, int32_t* next, int64_t* next_count, int64_t cap, int32_t* stamp,
@@ -126,7 +191,7 @@ __global__ void k_invalidate(GView g, const int32_t* level, int32_t* mark, int32
     int32_t c = cur[i];
     Row r = row_of(g, c);
     int64_t len = row_len(r);
-    int32_t parent = wave_find_first(g, r, 0, len, [&](int32_t x) { return level[x] == lvl - 1 && mark[x] != mepoch; });
+    int32_t parent = wave_find_first(g, r, 0, len, [&](int32_t x) { return valid_parent(level, mark, mepoch, x, lvl); });
     if (parent >= 0) {
       work += 1;
       continue;
@@ -150,22 +215,144 @@ __global__ void k_invalidate(GView g, const int32_t* level, int32_t* mark, int32
   if (lane_id() == 0 && work) atomicAdd((unsigned long long*)scanned, (unsigned long long)work);
 }
 
+// Chunk tasks (candidate index in `longs` << 32 | chunk) for every long candidate.
+__global__ void k_long_tasks(GView g, const int32_t* longs, const int64_t* long_count, int64_t* tasks,
+                             int64_t* task_count, int64_t task_cap) {
+  int64_t nl = *long_count;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nl; i += (int64_t)gridDim.x * blockDim.x) {
+    int64_t len = row_len(row_of(g, longs[i]));
+    int64_t chunks = (len + kChunk - 1) / kChunk;
+    int64_t pos = (int64_t)atomicAdd((unsigned long long*)task_count, (unsigned long long)chunks);
+    for (int64_t k = 0; k < chunks && pos + k < task_cap; k++) tasks[pos + k] = (i << 32) | k;
+  }
+}
+
+// Wave per chunk: a valid parent anywhere in the chunk marks the candidate supported.
+__global__ void k_inval_search(GView g, const int32_t* level, const int32_t* mark, int32_t mepoch, int32_t lvl,
+                               const int32_t* longs, const int64_t* tasks, const int64_t* task_count, int64_t task_cap,
+                               int32_t* supported, int32_t sepoch) {
+  int64_t nt = min(*task_count, task_cap);
+  for (int64_t t = global_wave(); t < nt; t += total_waves()) {
+    int64_t task = tasks[t];
+    int32_t c = longs[task >> 32];
+    if (supported[c] == sepoch) continue;
+    Row r = row_of(g, c);
+    int64_t lo = (task & 0xffffffffLL) * kChunk, hi = min(lo + kChunk, row_len(r));
+    int32_t parent = wave_find_first(g, r, lo, hi, [&](int32_t x) { return valid_parent(level, mark, mepoch, x, lvl); });
+    if (parent >= 0 && lane_id() == 0) supported[c] = sepoch;
+  }
+}
+
+// Thread per long candidate: unsupported ones become invalid.
+__global__ void k_inval_decide(const int32_t* longs, const int64_t* long_count, const int32_t* supported,
+                               int32_t sepoch, int32_t* mark, int32_t mepoch, int32_t* inval, int64_t* inval_count,
+                               int64_t cap) {
+  int64_t nl = *long_count;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nl; i += (int64_t)gridDim.x * blockDim.x) {
+    int32_t c = longs[i];
+    if (supported[c] != sepoch) {
+      mark[c] = mepoch;
+      thread_append(c, inval, inval_count, cap);
+    }
+  }
+}
+
+// Wave per chunk of an invalid long candidate: queue its neighbours at lvl+1.
+__global__ void k_inval_push(GView g, const int32_t* level, const int32_t* mark, int32_t mepoch, int32_t lvl,
+                             const int32_t* longs, const int64_t* tasks, const int64_t* task_count, int64_t task_cap,
+                             int32_t* next, int64_t* next_count, int64_t cap, int32_t* stamp, int32_t epoch,
+                             int64_t* scanned) {
+  int64_t nt = min(*task_count, task_cap);
+  int64_t work = 0;
+  for (int64_t t = global_wave(); t < nt; t += total_waves()) {
+    int64_t task = tasks[t];
+    int32_t c = longs[task >> 32];
+    if (mark[c] != mepoch) continue;
+    Row r = row_of(g, c);
+    int64_t lo = (task & 0xffffffffLL) * kChunk, hi = min(lo + kChunk, row_len(r));
+    work += hi - lo;
+    for (int64_t base = lo; base < hi; base += kWave) {
+      int64_t k = base + lane_id();
+      bool take = false;
+      int32_t w = -1;
+      if (k < hi) {
+        w = row_at(g, r, k);
+        take = w >= 0 && level[w] == lvl + 1 && atomicExch(&stamp[w], epoch) != epoch;
+      }
+      wave_append(take, w, next, next_count, cap);
+    }
+  }
+  if (lane_id() == 0 && work) atomicAdd((unsigned long long*)scanned, (unsigned long long)work);
+}
+
 __global__ void k_reset_levels(const int32_t* xs, int64_t nx, int32_t* level) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < nx) level[xs[i]] = -1;
 }
 
 // Wave per invalid vertex: 1 + smallest reached neighbour level, or -1.
-__global__ void k_tentative(GView g, const int32_t* level, const int32_t* xs, int64_t nx, int32_t* tent,
-                            int64_t* scanned) {
+// Tentative level of each invalidated vertex: the smallest level among its reached
+// neighbours (tent[i] = INT32_MAX if none). Thread per vertex for rows of at most kShort
+// entries; longer rows go to the wave list (<= kChunk) or to chunk tasks.
+__global__ void k_tentative_short(GView g, const int32_t* level, const int32_t* xs, int64_t nx, int32_t* tent,
+                                  int32_t* medium, int64_t* medium_count, int64_t medium_cap, int64_t* tasks,
+                                  int64_t* task_count, int64_t task_cap, int64_t* scanned) {
   int64_t work = 0;
-  for (int64_t i = global_wave(); i < nx; i += total_waves()) {
-    int32_t x = xs[i];
-    Row r = row_of(g, x);
-    int64_t len = row_len(r);
-    work += len;
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < nx; base += (int64_t)gridDim.x * blockDim.x) {
+    int64_t i = base + threadIdx.x;
+    bool mid = false;
+    if (i < nx) {
+      Row r = row_of(g, xs[i]);
+      int64_t len = row_len(r);
+      int32_t best = INT32_MAX;
+      if (len <= kShort) {
+        for (int64_t k = 0; k < len; k++) {
+          int32_t w = row_at(g, r, k);
+          if (w >= 0) {
+            int32_t lw = level[w];
+            if (lw >= 0 && lw < best) best = lw;
+          }
+        }
+        work += len;
+      } else if (len <= kChunk) {
+        mid = true;
+      } else {
+        int64_t chunks = (len + kChunk - 1) / kChunk;
+        int64_t pos = (int64_t)atomicAdd((unsigned long long*)task_count, (unsigned long long)chunks);
+        for (int64_t k = 0; k < chunks && pos + k < task_cap; k++) tasks[pos + k] = (i << 32) | k;
+      }
+      tent[i] = best;
+    }
+    wave_append(mid, (int32_t)(i < nx ? i : 0), medium, medium_count, medium_cap);
+  }
+  block_add(work, scanned);
+}
+
+// Wave per medium row (index list) or per chunk of a long row; atomicMin into tent.
+__global__ void k_tentative_rows(GView g, const int32_t* level, const int32_t* xs, const int32_t* medium,
+                                 const int64_t* medium_count, const int64_t* tasks, const int64_t* task_count,
+                                 int64_t task_cap, int32_t* tent, int64_t* scanned) {
+  int64_t nm = *medium_count;
+  int64_t nt = min(*task_count, task_cap);
+  int64_t work = 0;
+  for (int64_t t = global_wave(); t < nm + nt; t += total_waves()) {
+    int64_t i, lo, hi;
+    Row r;
+    if (t < nm) {
+      i = medium[t];
+      r = row_of(g, xs[i]);
+      lo = 0;
+      hi = row_len(r);
+    } else {
+      int64_t task = tasks[t - nm];
+      i = task >> 32;
+      r = row_of(g, xs[i]);
+      lo = (task & 0xffffffffLL) * kChunk;
+      hi = min(lo + kChunk, row_len(r));
+    }
+    work += hi - lo;
     int32_t best = INT32_MAX;
-    for (int64_t k = lane_id(); k < len; k += kWave) {
+    for (int64_t k = lo + lane_id(); k < hi; k += kWave) {
       int32_t w = row_at(g, r, k);
       if (w >= 0) {
         int32_t lw = level[w];
@@ -173,52 +360,85 @@ __global__ void k_tentative(GView g, const int32_t* level, const int32_t* xs, in
       }
     }
     for (int off = kWave / 2; off > 0; off >>= 1) best = min(best, __shfl_xor(best, off));
-    if (lane_id() == 0) tent[i] = best == INT32_MAX ? -1 : best + 1;
+    if (lane_id() == 0 && best != INT32_MAX) atomicMin(&tent[i], best);
   }
   if (lane_id() == 0 && work) atomicAdd((unsigned long long*)scanned, (unsigned long long)work);
 }
 
 // Seeds are packed level << 32 | vertex. box[4] = seeds, box[5] = min level, box[6] = max.
-__device__ __forceinline__ void push_seed(int32_t v, int32_t lv, int64_t* seeds, int64_t* box, int64_t cap) {
-  int64_t pos = (int64_t)atomicAdd((unsigned long long*)&box[4], 1ULL);
-  if (pos < cap) seeds[pos] = ((int64_t)lv << 32) | (int64_t)(uint32_t)v;
-  atomicMin((long long*)&box[5], (long long)lv);
-  atomicMax((long long*)&box[6], (long long)lv);
+__device__ __forceinline__ void block_push_seed(bool take, int32_t v, int32_t lv, int64_t* seeds, int64_t* box,
+                                                int64_t cap) {
+  int64_t packed = ((int64_t)lv << 32) | (int64_t)(uint32_t)v;
+  block_append(take, packed, seeds, &box[4], cap);
+  block_min_max(take ? lv : INT64_MAX, take ? lv : -1, &box[5], &box[6]);
 }
 
 __global__ void k_apply_tentative(const int32_t* xs, const int32_t* tent, int64_t nx, int32_t* level, int64_t* seeds,
                                   int64_t cap, int64_t* box) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nx) return;
-  int32_t t = tent[i];
-  level[xs[i]] = t;
-  if (t >= 0) push_seed(xs[i], t, seeds, box, cap);
+  int32_t t = -1, x = 0;
+  if (i < nx) {
+    int32_t best = tent[i];
+    t = best == INT32_MAX ? -1 : best + 1;
+    x = xs[i];
+    level[x] = t;
+  }
+  block_push_seed(t >= 0, x, t, seeds, box, cap);
 }
 
 // Insertion lines whose endpoints are adjacent after the batch: lower the far endpoint.
 __global__ void k_insert_seeds(GView g, bool sorted, const int32_t* uv, const uint8_t* ins, int64_t m,
                                int32_t* level, int64_t* seeds, int64_t cap, int64_t* box) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= m || !ins[i]) return;
-  atomicAdd((unsigned long long*)&box[7], 1ULL);
-  int32_t u = uv[2 * i], v = uv[2 * i + 1];
-  int32_t lu = level[u], lv = level[v];
-  bool maybe = (lu >= 0 && (lv < 0 || lu + 1 < lv)) || (lv >= 0 && (lu < 0 || lv + 1 < lu));
-  if (!maybe) return;
-  if (!adjacent_now(pair_state(g, sorted, u, v))) return;
-  if (lu >= 0 && cas_min_level(&level[v], lu + 1)) push_seed(v, lu + 1, seeds, box, cap);
-  lv = level[v];
-  if (lv >= 0 && cas_min_level(&level[u], lv + 1)) push_seed(u, lv + 1, seeds, box, cap);
+  bool is_ins = i < m && ins[i];
+  bool sv = false, su = false;
+  int32_t u = 0, v = 0, lu = -1, lv = -1;
+  if (is_ins) {
+    u = uv[2 * i];
+    v = uv[2 * i + 1];
+    lu = level[u];
+    lv = level[v];
+    bool maybe = (lu >= 0 && (lv < 0 || lu + 1 < lv)) || (lv >= 0 && (lu < 0 || lv + 1 < lu));
+    if (maybe && adjacent_now(pair_state(g, sorted, u, v))) {
+      if (lu >= 0 && cas_min_level(&level[v], lu + 1)) sv = true;
+      lv = level[v];
+      if (lv >= 0 && cas_min_level(&level[u], lv + 1)) su = true;
+    }
+  }
+  block_add(is_ins ? 1 : 0, &box[7]);
+  block_push_seed(sv, v, lu + 1, seeds, box, cap);
+  block_push_seed(su, u, lv + 1, seeds, box, cap);
 }
 
 // Seeds at `lvl` that still hold that level join the current frontier once.
 __global__ void k_seed_at_level(const int64_t* seeds, int64_t ns, const int32_t* level, int32_t lvl, int32_t* cur,
                                 int64_t* cur_count, int64_t cap, int32_t* stamp, int32_t epoch) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= ns) return;
-  int64_t s = seeds[i];
-  int32_t lv = (int32_t)(s >> 32), v = (int32_t)(s & 0xffffffff);
-  if (lv == lvl && level[v] == lvl && atomicExch(&stamp[v], epoch) != epoch) thread_append(v, cur, cur_count, cap);
+  bool take = false;
+  int32_t v = 0;
+  if (i < ns) {
+    int64_t s = seeds[i];
+    int32_t lv = (int32_t)(s >> 32);
+    v = (int32_t)(s & 0xffffffff);
+    take = lv == lvl && level[v] == lvl && atomicExch(&stamp[v], epoch) != epoch;
+  }
+  block_append(take, v, cur, cur_count, cap);
+}
+
+// Zeroes `count` consecutive counters (round bookkeeping without a copy or fill kernel).
+__global__ void k_zero(int64_t* p, int count) {
+  if ((int)threadIdx.x < count) p[threadIdx.x] = 0;
+}
+
+__global__ void k_zero_pair(int64_t* a, int64_t* b) {
+  *a = 0;
+  *b = 0;
+}
+
+// box[0..7] = {0, INT32_MAX, -1, 0, 0, INT32_MAX, -1, 0}, box[8..15] = 0.
+__global__ void k_init_box(int64_t* box) {
+  int t = (int)threadIdx.x;
+  if (t < 16) box[t] = (t == 1 || t == 5) ? (int64_t)INT32_MAX : (t == 2 || t == 6) ? -1 : 0;
 }
 
 __global__ void k_fill_level(int32_t* level, int32_t n, int32_t src) {
@@ -241,19 +461,22 @@ int relax_levels(Ctx& c, int32_t* L, int32_t lvl0, int32_t max_seed, const int64
   int64_t task_cap = c.wb.cap;
   int32_t* cur = c.lb.p;
   int32_t* nxt = c.lc.p;
-  // box slots: [8] cur count, [9] next count, [10] task count, [11] scanned
-  int64_t* bcur = c.d_box + 8;
-  int64_t* bnext = c.d_box + 9;
+  // box slots: [8]/[9] frontier counts (current and next alternate), [10] task count,
+  // [11] scanned entries; the caller sets the current count in [8]
+  int64_t* cnt[2] = {c.d_box + 8, c.d_box + 9};
   int64_t* btask = c.d_box + 10;
   int64_t* bscan = c.d_box + 11;
-  SCOUT_TRY(hipMemsetAsync(bscan, 0, sizeof(int64_t), c.s));
+  k_zero<<<1, 64, 0, c.s>>>(bscan, 1);
   int32_t lvl = lvl0;
   int64_t r = 0;
   int64_t scanned_before = 0;
+  int p = 0;
   while (true) {
+    int64_t* bcur = cnt[p];
+    int64_t* bnext = cnt[p ^ 1];
     if (nseeds > 0 && lvl <= max_seed)
       k_seed_at_level<<<grid_for(nseeds), kBlock, 0, c.s>>>(seeds, nseeds, L, lvl, cur, bcur, cap, c.stamp, epoch);
-    SCOUT_TRY(hipMemsetAsync(bnext, 0, 2 * sizeof(int64_t), c.s));
+    k_zero_pair<<<1, 1, 0, c.s>>>(bnext, btask);
     SCOUT_CHECK(prof_begin(c, PROF_BFS_RELAX));
     k_relax<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, cur, bcur, lvl, nxt, bnext, cap, c.stamp, epoch,
                                             c.wb.p, btask, task_cap, bscan);
@@ -263,23 +486,21 @@ int relax_levels(Ctx& c, int32_t* L, int32_t lvl0, int32_t max_seed, const int64
     r++;
     SCOUT_TRY(hipMemcpyAsync(c.h_box + 8, c.d_box + 8, 4 * sizeof(int64_t), hipMemcpyDeviceToHost, c.s));
     SCOUT_TRY(hipStreamSynchronize(c.s));
-    int64_t nn = c.h_box[9];
+    int64_t ncur = c.h_box[8 + p], nn = c.h_box[8 + (p ^ 1)];
     // algorithmic bytes: per frontier vertex its id, level and the four row offsets (36 B);
     // per scanned entry the neighbour id and its level (8 B); per lowered vertex the level
     // write and its queue entry (8 B)
-    prof_bytes(c, PROF_BFS_RELAX, 36 * c.h_box[8] + 8 * (c.h_box[11] - scanned_before) + 8 * nn);
+    prof_bytes(c, PROF_BFS_RELAX, 36 * ncur + 8 * (c.h_box[11] - scanned_before) + 8 * nn);
     scanned_before = c.h_box[11];
     if (c.h_box[10] > task_cap) {
       set_error("bfs relax: chunk task list overflow");
       return SCOUT_ERR_STATE;
     }
     lvl++;
-    // next becomes current
     std::swap(cur, nxt);
-    SCOUT_TRY(hipMemcpyAsync(bcur, bnext, sizeof(int64_t), hipMemcpyDeviceToDevice, c.s));
+    p ^= 1;
     if (nn == 0 && (nseeds == 0 || lvl > max_seed)) break;
   }
-  // keep list ownership consistent: cur/nxt are lb/lc in some order, both owned by c.
   *rounds += r;
   *scanned_total += c.h_box[11];
   return SCOUT_OK;
@@ -293,6 +514,7 @@ int ensure_bfs(Ctx& c, int64_t extra) {
   SCOUT_CHECK(c.lc.reserve(n + 1));
   SCOUT_CHECK(c.ld.reserve(n + 1));
   SCOUT_CHECK(c.le.reserve(n + 1));
+  SCOUT_CHECK(c.lf.reserve(n + 1));
   SCOUT_CHECK(c.wa.reserve(n + 2 * extra + 1));
   // chunk tasks: at most 2 * entries / kChunk (every split row has more than kChunk entries)
   SCOUT_CHECK(c.wb.reserve(2 * (c.g.E + c.g.A) / kChunk + 1024));
@@ -322,9 +544,7 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, scout_stats* st) {
   int64_t cap = (int64_t)n + 1;
   int64_t* box = c.d_box;
   // box: [0] cand, [1] min, [2] max, [3] deletions, [4] seeds, [5] seed min, [6] seed max, [7] insertions
-  int64_t init[8] = {0, INT32_MAX, -1, 0, 0, INT32_MAX, -1, 0};
-  SCOUT_TRY(hipMemcpyAsync(box, init, sizeof(init), hipMemcpyHostToDevice, c.s));
-  SCOUT_TRY(hipMemsetAsync(box + 8, 0, 8 * sizeof(int64_t), c.s));
+  k_init_box<<<1, 64, 0, c.s>>>(box);
   if (m > 0)
     k_del_candidates<<<grid_for(m), kBlock, 0, c.s>>>(ch.d_uv.p, ch.d_ins.p, m, L, c.la.p, c.la.cap, box);
   SCOUT_CHECK(box_read(c, 8));
@@ -333,45 +553,79 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, scout_stats* st) {
   st->deletions += c.h_box[3];
   int64_t nx = 0, scanned = 0, rounds = 0;
   if (ncand > 0) {
-    // Invalidation, level by level.
+    // Invalidation, level by level. Counts: [8]/[9] current and next (alternating),
+    // [10] invalid vertices, [11] scanned entries.
     int32_t mepoch = next_epoch(c);
     int32_t sepoch = next_epoch(c);
     int32_t* cur = c.lb.p;
     int32_t* nxt = c.lc.p;
-    int64_t* bcur = box + 8;
-    int64_t* bnext = box + 9;
+    int64_t* cnt[2] = {box + 8, box + 9};
     int64_t* binval = box + 10;
     int64_t* bscan = box + 11;
-    SCOUT_TRY(hipMemsetAsync(box + 8, 0, 4 * sizeof(int64_t), c.s));
+    int64_t inval_scan_before = 0;
+    int p = 0;
     for (int32_t lvl = cmin;; lvl++) {
+      int64_t* bcur = cnt[p];
+      int64_t* bnext = cnt[p ^ 1];
       if (lvl <= cmax)
         k_cand_at_level<<<grid_for(ncand), kBlock, 0, c.s>>>(c.la.p, ncand, L, lvl, cur, bcur, cap, c.stamp,
                                                               sepoch);
-      SCOUT_TRY(hipMemsetAsync(bnext, 0, sizeof(int64_t), c.s));
-      k_invalidate<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, c.bfs_mark, mepoch, cur, bcur, lvl, c.ld.p,
+      k_zero<<<1, 64, 0, c.s>>>(bnext, 1);
+      SCOUT_CHECK(prof_begin(c, PROF_BFS_INVALIDATE));
+      // [12] medium count, [13] long count, [14] chunk tasks
+      k_zero<<<1, 64, 0, c.s>>>(box + 12, 3);
+      k_invalidate_short<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, c.bfs_mark, mepoch, cur, bcur, lvl, c.ld.p,
+                                                         binval, nxt, bnext, cap, c.stamp, sepoch, c.le.p, box + 12,
+                                                         c.lf.p, box + 13, bscan);
+      k_invalidate<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, c.bfs_mark, mepoch, c.le.p, box + 12, lvl, c.ld.p,
                                                    binval, cap, nxt, bnext, cap, c.stamp, sepoch, bscan);
+      k_long_tasks<<<64, kBlock, 0, c.s>>>(c.g.view(), c.lf.p, box + 13, c.wb.p, box + 14, c.wb.cap);
+      int32_t supepoch = next_epoch(c);
+      k_inval_search<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, c.bfs_mark, mepoch, lvl, c.lf.p, c.wb.p, box + 14,
+                                                     c.wb.cap, c.aux, supepoch);
+      k_inval_decide<<<64, kBlock, 0, c.s>>>(c.lf.p, box + 13, c.aux, supepoch, c.bfs_mark, mepoch, c.ld.p, binval,
+                                              cap);
+      k_inval_push<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, c.bfs_mark, mepoch, lvl, c.lf.p, c.wb.p, box + 14,
+                                                   c.wb.cap, nxt, bnext, cap, c.stamp, sepoch, bscan);
+      SCOUT_CHECK(prof_end(c, PROF_BFS_INVALIDATE, 0));
       rounds++;
-      SCOUT_TRY(hipMemcpyAsync(c.h_box + 8, box + 8, 4 * sizeof(int64_t), hipMemcpyDeviceToHost, c.s));
+      SCOUT_TRY(hipMemcpyAsync(c.h_box + 8, box + 8, 7 * sizeof(int64_t), hipMemcpyDeviceToHost, c.s));
       SCOUT_TRY(hipStreamSynchronize(c.s));
-      int64_t nn = c.h_box[9];
+      if (c.h_box[14] > c.wb.cap) {
+        set_error("bfs invalidation: chunk task list overflow");
+        return SCOUT_ERR_STATE;
+      }
+      int64_t nn = c.h_box[8 + (p ^ 1)];
+      // per candidate its id, level and row offsets (36 B); per scanned entry the neighbour,
+      // its level and (for parents) its invalid mark (12 B); per queued vertex 8 B
+      prof_bytes(c, PROF_BFS_INVALIDATE, 36 * c.h_box[8 + p] + 12 * (c.h_box[11] - inval_scan_before) + 8 * nn);
+      inval_scan_before = c.h_box[11];
       std::swap(cur, nxt);
-      SCOUT_TRY(hipMemcpyAsync(bcur, bnext, sizeof(int64_t), hipMemcpyDeviceToDevice, c.s));
+      p ^= 1;
       if (nn == 0 && lvl + 1 > cmax) break;
     }
     nx = c.h_box[10];
     scanned += c.h_box[11];
     if (nx > 0) {
       k_reset_levels<<<grid_for(nx), kBlock, 0, c.s>>>(c.ld.p, nx, L);
-      SCOUT_TRY(hipMemsetAsync(box + 11, 0, sizeof(int64_t), c.s));
-      k_tentative<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, c.ld.p, nx, c.le.p, box + 11);
+      // [11] scanned, [12] medium rows, [13] chunk tasks
+      k_zero<<<1, 64, 0, c.s>>>(box + 11, 3);
+      k_tentative_short<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, c.ld.p, nx, c.le.p, c.lf.p, box + 12,
+                                                        c.lf.cap, c.wb.p, box + 13, c.wb.cap, box + 11);
+      k_tentative_rows<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, c.ld.p, c.lf.p, box + 12, c.wb.p, box + 13,
+                                                       c.wb.cap, c.le.p, box + 11);
       k_apply_tentative<<<grid_for(nx), kBlock, 0, c.s>>>(c.ld.p, c.le.p, nx, L, c.wa.p, c.wa.cap, box);
     }
   }
   if (m > 0)
     k_insert_seeds<<<grid_for(m), kBlock, 0, c.s>>>(c.g.view(), c.g.sorted, ch.d_uv.p, ch.d_ins.p, m, L, c.wa.p,
                                                     c.wa.cap, box);
-  SCOUT_CHECK(box_read(c, 12));
+  SCOUT_CHECK(box_read(c, 14));
   if (nx > 0) scanned += c.h_box[11];
+  if (nx > 0 && c.h_box[13] > c.wb.cap) {
+    set_error("bfs tentative levels: chunk task list overflow");
+    return SCOUT_ERR_STATE;
+  }
   st->insertions += c.h_box[7];
   int64_t nseeds = c.h_box[4];
   if (nseeds > c.wa.cap) {
@@ -383,7 +637,7 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, scout_stats* st) {
   if (nseeds > 0) {
     int32_t smin = (int32_t)c.h_box[5], smax = (int32_t)c.h_box[6];
     int32_t epoch = next_epoch(c);
-    SCOUT_TRY(hipMemsetAsync(box + 8, 0, sizeof(int64_t), c.s));
+    k_zero<<<1, 64, 0, c.s>>>(box + 8, 1);
     SCOUT_CHECK(relax_levels(c, L, smin, smax, c.wa.p, nseeds, epoch, &rounds, &scanned));
   }
   st->rounds += rounds;

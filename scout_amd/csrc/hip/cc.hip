@@ -109,21 +109,35 @@ __global__ void k_cc_flatten(int32_t* parent, int32_t n) {
 
 // --- batch repair ---------------------------------------------------------------------
 
-// Vertices the BFS from vertex 0 does not reach: parent = self; those with a non-empty row
-// are listed for the union pass.
-__global__ void k_cc_unreached(GView g, const int32_t* L, int32_t* par, int32_t* list, int64_t* count, int64_t cap) {
-  int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  bool take = false;
-  if (v < g.n && L[v] < 0) {
-    par[v] = (int32_t)v;
-    take = (g.boff[v + 1] > g.boff[v]) || (g.aoff[v + 1] > g.aoff[v]);
+// Union pass over the unreached subgraph, thread per vertex (grid-stride): vertices the BFS
+// from vertex 0 does not reach unite with their neighbours w > v. Neighbours of unreached
+// vertices are unreached, so the union-find stays inside the unreached subgraph. Rows longer
+// than 32 entries are listed for k_cc_unreached_hook (wave per row). `par` holds the
+// identity for every vertex on entry.
+__global__ void k_cc_unreached(GView g, const int32_t* L, int32_t* par, int32_t* longs, int64_t* nlong, int64_t cap,
+                               int64_t* scanned) {
+  int64_t work = 0;
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < g.n; v += (int64_t)gridDim.x * blockDim.x) {
+    if (L[v] >= 0) continue;
+    Row r = row_of(g, (int32_t)v);
+    int64_t len = row_len(r);
+    if (len == 0) continue;
+    if (len > 32) {
+      thread_append((int32_t)v, longs, nlong, cap);
+      continue;
+    }
+    work += len;
+    for (int64_t k = 0; k < len; k++) {
+      int32_t w = row_at(g, r, k);
+      int32_t h;
+      if (w > v) uf_union(par, (int32_t)v, w, &h);
+    }
   }
-  wave_append(take, (int32_t)v, list, count, cap);
+  block_add(work, scanned);
 }
 
-// Wave per listed vertex v: union(v, w) for every neighbour w > v. Neighbours of unreached
-// vertices are unreached, so the union-find stays inside the unreached subgraph. Rows longer
-// than kChunk become chunk tasks.
+// Wave per listed long row v: union(v, w) for every neighbour w > v; rows longer than
+// kChunk become chunk tasks.
 __global__ void k_cc_unreached_hook(GView g, const int32_t* list, const int64_t* count, int32_t* par, int64_t* tasks,
                                     int64_t* task_count, int64_t task_cap, int64_t* scanned) {
   int64_t nl = *count;
@@ -149,23 +163,22 @@ __global__ void k_cc_unreached_hook(GView g, const int32_t* list, const int64_t*
   if (lane_id() == 0 && work) atomicAdd((unsigned long long*)scanned, (unsigned long long)work);
 }
 
-// Labels: vertex 0's component (the reached set) has minimum id 0; every other vertex takes
-// the root of its union-find set, which is its component's minimum. box[0] counts roots.
+// Labels (grid-stride): vertex 0's component (the reached set) has minimum id 0; every other
+// vertex takes the root of its union-find set, which is its component's minimum. *roots
+// counts the components.
 __global__ void k_cc_final(const int32_t* L, int32_t n, int32_t* par, int32_t* label, int64_t* roots) {
-  int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  bool root = false;
-  if (v < n) {
+  int64_t cnt = 0;
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x) {
     if (L[v] >= 0) {
       label[v] = 0;
-      root = (v == 0);
+      cnt += (v == 0);
     } else {
       int32_t r = uf_find(par, (int32_t)v);
       label[v] = r;
-      root = (r == v);
+      cnt += (r == v);
     }
   }
-  unsigned long long mask = __ballot(root);
-  if (lane_id() == 0 && mask) atomicAdd((unsigned long long*)roots, (unsigned long long)__popcll(mask));
+  block_add(cnt, roots);
 }
 
 }  // namespace
@@ -224,25 +237,24 @@ int cc_local(Ctx& c, Channel& ch, scout_stats* st) {
   int32_t* par = c.cc_owner;
   SCOUT_TRY(hipMemsetAsync(c.d_box, 0, 4 * sizeof(int64_t), c.s));
   SCOUT_CHECK(prof_begin(c, PROF_CC_EXPAND));
-  k_cc_unreached<<<grid_for(n), kBlock, 0, c.s>>>(c.g.view(), L, par, c.la.p, c.d_box, (int64_t)n + 1);
+  k_iota<<<grid_for(n), kBlock, 0, c.s>>>(par, n);
+  k_cc_unreached<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, par, c.la.p, c.d_box, (int64_t)n + 1, c.d_box + 2);
   k_cc_unreached_hook<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), c.la.p, c.d_box, par, c.wb.p, c.d_box + 1,
                                                       c.wb.cap, c.d_box + 2);
   k_cc_hook_tasks<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), par, c.wb.p, c.d_box + 1, c.wb.cap, c.d_box + 2);
-  k_cc_final<<<grid_for(n), kBlock, 0, c.s>>>(L, n, par, c.label, c.d_box + 3);
+  k_cc_final<<<kListGrid, kBlock, 0, c.s>>>(L, n, par, c.label, c.d_box + 3);
   SCOUT_CHECK(prof_end(c, PROF_CC_EXPAND, 0));
   SCOUT_CHECK(box_read(c, 4));
   if (c.h_box[1] > c.wb.cap) {
     set_error("cc: chunk task overflow");
     return SCOUT_ERR_STATE;
   }
-  // per vertex: its level, label write and (unreached) parent (12 B); per unreached vertex
-  // with a row its offsets and list entry (36 B); per scanned entry the neighbour and two
-  // parents (12 B)
-  prof_bytes(c, PROF_CC_EXPAND, 12 * (int64_t)n + 36 * c.h_box[0] + 12 * c.h_box[2]);
+  // per vertex: its level (twice), parent init and label write (16 B); per scanned entry the
+  // neighbour and two parents (12 B)
+  prof_bytes(c, PROF_CC_EXPAND, 16 * (int64_t)n + 12 * c.h_box[2]);
   int64_t comps = c.h_box[3];
   if (c.cc_components >= 0 && comps > c.cc_components) st->splits += comps - c.cc_components;
   c.cc_components = comps;
-  st->touched += c.h_box[0];
   st->edges_scanned += c.h_box[2];
   return SCOUT_OK;
 }

@@ -32,13 +32,20 @@ __device__ __forceinline__ unsigned long long wave_or(unsigned long long x) {
 __global__ void k_color_marks(const int32_t* uv, const uint8_t* ins, int64_t m, const int32_t* color, int32_t* round,
                               int64_t cap, int32_t* stamp, int32_t epoch, int64_t* box) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= m || !ins[i]) return;
-  atomicAdd((unsigned long long*)&box[2], 1ULL);
-  int32_t u = uv[2 * i], v = uv[2 * i + 1];
-  if (color[u] != color[v]) return;
-  atomicAdd((unsigned long long*)&box[1], 1ULL);
-  int32_t loser = u > v ? u : v;
-  if (atomicExch(&stamp[loser], epoch) != epoch) thread_append(loser, round, &box[0], cap);
+  bool is_ins = i < m && ins[i];
+  bool conflict = false, take = false;
+  int32_t loser = 0;
+  if (is_ins) {
+    int32_t u = uv[2 * i], v = uv[2 * i + 1];
+    if (color[u] == color[v]) {
+      conflict = true;
+      loser = u > v ? u : v;
+      take = atomicExch(&stamp[loser], epoch) != epoch;
+    }
+  }
+  block_append(take, loser, round, &box[0], cap);
+  block_add(conflict ? 1 : 0, &box[1]);
+  block_add(is_ins ? 1 : 0, &box[2]);
 }
 
 // Wave per round vertex: first-fit against the snapshot colours.

@@ -67,6 +67,82 @@ __device__ __forceinline__ void wave_append(bool take, int32_t value, int32_t* o
   }
 }
 
+// Block-wide reductions and appends: one global atomic per block instead of one per thread or
+// wave, so O(n) passes do not serialise on a single counter. Every thread of the block must
+// call them (no early return before the call).
+__device__ __forceinline__ int64_t wave_sum(int64_t v) {
+  for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
+__device__ __forceinline__ void block_add(int64_t v, int64_t* out) {
+  __shared__ int64_t part[kWavesPerBlock];
+  v = wave_sum(v);
+  int w = (int)(threadIdx.x / kWave);
+  if (lane_id() == 0) part[w] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int64_t s = 0;
+    for (int i = 0; i < kWavesPerBlock; i++) s += part[i];
+    if (s) atomicAdd((unsigned long long*)out, (unsigned long long)s);
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ void block_min_max(int64_t lo, int64_t hi, int64_t* out_min, int64_t* out_max) {
+  __shared__ int64_t pmin[kWavesPerBlock], pmax[kWavesPerBlock];
+  for (int off = kWave / 2; off > 0; off >>= 1) {
+    lo = min(lo, (int64_t)__shfl_xor(lo, off));
+    hi = max(hi, (int64_t)__shfl_xor(hi, off));
+  }
+  int w = (int)(threadIdx.x / kWave);
+  if (lane_id() == 0) {
+    pmin[w] = lo;
+    pmax[w] = hi;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int64_t a = pmin[0], b = pmax[0];
+    for (int i = 1; i < kWavesPerBlock; i++) {
+      a = min(a, pmin[i]);
+      b = max(b, pmax[i]);
+    }
+    atomicMin((long long*)out_min, (long long)a);
+    atomicMax((long long*)out_max, (long long)b);
+  }
+  __syncthreads();
+}
+
+// Appends `value` for every thread with `take`, reserving the block's slots with one atomic.
+// Returns the slot (or -1).
+template <class T>
+__device__ __forceinline__ int64_t block_append(bool take, T value, T* out, int64_t* count, int64_t cap) {
+  __shared__ int64_t wbase[kWavesPerBlock];
+  __shared__ int64_t gbase;
+  unsigned long long mask = __ballot(take);
+  int w = (int)(threadIdx.x / kWave);
+  if (lane_id() == 0) wbase[w] = __popcll(mask);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int64_t s = 0;
+    for (int i = 0; i < kWavesPerBlock; i++) {
+      int64_t c = wbase[i];
+      wbase[i] = s;
+      s += c;
+    }
+    gbase = s ? (int64_t)atomicAdd((unsigned long long*)count, (unsigned long long)s) : 0;
+  }
+  __syncthreads();
+  int64_t pos = -1;
+  if (take) {
+    pos = gbase + wbase[w] + __popcll(mask & ((1ULL << lane_id()) - 1ULL));
+    if (pos < cap) out[pos] = value;
+    else pos = -1;
+  }
+  __syncthreads();
+  return pos;
+}
+
 // Per-thread append (when lanes of one wave are not converged on the same list).
 __device__ __forceinline__ void thread_append(int32_t value, int32_t* out, int64_t* count, int64_t cap) {
   int64_t pos = (int64_t)atomicAdd((unsigned long long*)count, 1ULL);

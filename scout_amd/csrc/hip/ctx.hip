@@ -23,7 +23,11 @@ int box_read(Ctx& c, int count) {
 
 int next_epoch(Ctx& c) {
   if (c.epoch >= 0x7ffffff0) {
-    SCOUT_TRY(hipMemsetAsync(c.stamp, 0, (size_t)c.g.n * sizeof(int32_t), c.s));
+    // every array compared against epochs starts over (epochs are > 0)
+    size_t bytes = ((size_t)c.g.n + 1) * sizeof(int32_t);
+    SCOUT_TRY(hipMemsetAsync(c.stamp, 0, bytes, c.s));
+    if (c.aux) SCOUT_TRY(hipMemsetAsync(c.aux, 0xff, bytes, c.s));
+    if (c.bfs_mark) SCOUT_TRY(hipMemsetAsync(c.bfs_mark, 0xff, bytes, c.s));
     c.epoch = 0;
   }
   return ++c.epoch;
@@ -270,7 +274,7 @@ int scout_ctx_destroy(scout_ctx* x) {
     c.ch[k].d_done.release();
     if (c.ch[k].staged) (void)hipEventDestroy(c.ch[k].staged);
   }
-  DevBuf<int32_t>* ls[] = {&c.la, &c.lb, &c.lc, &c.ld, &c.le};
+  DevBuf<int32_t>* ls[] = {&c.la, &c.lb, &c.lc, &c.ld, &c.le, &c.lf};
   for (auto* l : ls) l->release();
   c.wa.release();
   c.wb.release();

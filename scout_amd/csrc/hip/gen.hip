@@ -35,17 +35,42 @@ __global__ void k_rmat_count(RmatParams p, uint64_t key, int64_t m, int shift, u
     if (local[i]) atomicAdd(&bins[i], local[i]);
 }
 
-// Pass 2: scatter keys into their bucket's region.
-__global__ void k_rmat_scatter(RmatParams p, uint64_t key, int64_t m, int shift, const int32_t* bin_bucket,
+// Pass 2: scatter keys into their bucket's region. Each block counts its keys per bucket in
+// LDS and reserves every bucket's range with one global atomic, so the few bucket cursors are
+// not contended by every thread.
+__global__ void k_rmat_scatter(RmatParams p, uint64_t key, int64_t m, int shift, const int32_t* bin_bucket, int nb,
                                unsigned long long* cursor, uint64_t* out) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
-    uint64_t e = rmat_edge(p, key, (uint64_t)i);
-    uint32_t u = (uint32_t)(e >> 32), v = (uint32_t)e;
-    if (u == v) continue;
-    unsigned long long a = atomicAdd(&cursor[bin_bucket[u >> shift]], 1ULL);
-    out[a] = ((uint64_t)u << 32) | v;
-    unsigned long long b = atomicAdd(&cursor[bin_bucket[v >> shift]], 1ULL);
-    out[b] = ((uint64_t)v << 32) | u;
+  __shared__ unsigned int lcount[kBins];
+  __shared__ unsigned long long lbase[kBins];
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < m; base += (int64_t)gridDim.x * blockDim.x) {
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) lcount[b] = 0;
+    __syncthreads();
+    int64_t i = base + threadIdx.x;
+    bool valid = false;
+    uint32_t u = 0, v = 0;
+    int bu = 0, bv = 0;
+    unsigned int ou = 0, ov = 0;
+    if (i < m) {
+      uint64_t e = rmat_edge(p, key, (uint64_t)i);
+      u = (uint32_t)(e >> 32);
+      v = (uint32_t)e;
+      valid = u != v;
+      if (valid) {
+        bu = bin_bucket[u >> shift];
+        bv = bin_bucket[v >> shift];
+        ou = atomicAdd(&lcount[bu], 1u);
+        ov = atomicAdd(&lcount[bv], 1u);
+      }
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < nb; b += blockDim.x)
+      lbase[b] = lcount[b] ? atomicAdd(&cursor[b], (unsigned long long)lcount[b]) : 0ULL;
+    __syncthreads();
+    if (valid) {
+      out[lbase[bu] + ou] = ((uint64_t)u << 32) | v;
+      out[lbase[bv] + ov] = ((uint64_t)v << 32) | u;
+    }
+    __syncthreads();
   }
 }
 
@@ -192,7 +217,7 @@ int gen_rmat(Ctx& c, int scale, int ef, double a, double b, double cc, uint64_t 
   std::vector<unsigned long long> cur(bstart.begin(), bstart.end() - 1);
   unsigned long long* d_cur = bins + kBins;
   SCOUT_TRY(hipMemcpyAsync(d_cur, cur.data(), (size_t)nb * 8, hipMemcpyHostToDevice, c.s));
-  k_rmat_scatter<<<4096, kBlock, 0, c.s>>>(p, key, m, shift, d_bin_bucket, d_cur, keys);
+  k_rmat_scatter<<<4096, kBlock, 0, c.s>>>(p, key, m, shift, d_bin_bucket, nb, d_cur, keys);
   // Sort each bucket (source ids of different buckets never interleave).
   for (int i = 0; i < nb; i++) {
     int64_t lo = bstart[(size_t)i], cnt = bstart[(size_t)i + 1] - lo;

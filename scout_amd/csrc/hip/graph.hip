@@ -26,11 +26,37 @@ __global__ void k_widen_offsets(const int32_t* in, int64_t* out, int64_t count) 
   if (i < count) out[i] = (int64_t)in[i];
 }
 
-// Wave per vertex: rows non-decreasing by decoded id, ids in range. flags[0] = unsorted rows,
-// flags[1] = out-of-range entries.
-__global__ void k_check_rows(const int64_t* boff, const int32_t* bcol, int32_t n, int64_t* flags) {
-  for (int64_t v = global_wave(); v < n; v += total_waves()) {
+// Rows non-decreasing by decoded id, ids in range: flags[0] = unsorted rows, flags[1] =
+// out-of-range entries. Thread per vertex for rows up to 64 entries; longer rows are listed
+// for k_check_long_rows (wave per row).
+__global__ void k_check_rows(const int64_t* boff, const int32_t* bcol, int32_t n, int64_t* flags, int32_t* longs,
+                             int64_t* nlong, int64_t cap) {
+  int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  bool unsorted = false, bad = false, is_long = false;
+  if (v < n) {
     int64_t b = boff[v], e = boff[v + 1];
+    if (e - b > 64) {
+      is_long = true;
+    } else {
+      int32_t prev = -1;
+      for (int64_t k = b; k < e; k++) {
+        int32_t x = decode(bcol[k]);
+        if (x < 0 || x >= n) bad = true;
+        if (x < prev) unsorted = true;
+        prev = x;
+      }
+    }
+  }
+  block_append(is_long, (int32_t)v, longs, nlong, cap);
+  block_add(unsorted ? 1 : 0, &flags[0]);
+  block_add(bad ? 1 : 0, &flags[1]);
+}
+
+__global__ void k_check_long_rows(const int64_t* boff, const int32_t* bcol, int32_t n, const int32_t* longs,
+                                  const int64_t* nlong, int64_t* flags) {
+  int64_t nl = *nlong;
+  for (int64_t i = global_wave(); i < nl; i += total_waves()) {
+    int64_t b = boff[longs[i]], e = boff[longs[i] + 1];
     bool unsorted = false, bad = false;
     for (int64_t k = b + lane_id(); k < e; k += kWave) {
       int32_t x = decode(bcol[k]);
@@ -91,12 +117,15 @@ __global__ void k_dup_mark(const int32_t* uv, int64_t m, const uint64_t* hkey, c
 __global__ void k_classify(GView g, bool sorted, const int32_t* uv, int64_t m, int rule, uint8_t* ins,
                            int64_t* counts) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= m) return;
-  int32_t u = uv[2 * i], v = uv[2 * i + 1];
-  PairState s = pair_state(g, sorted, u, v);
-  bool adj = rule == SCOUT_CLASSIFY_BASE ? s.base_any : adjacent_now(s);
-  ins[i] = adj ? 0 : 1;
-  if (!adj) atomicAdd((unsigned long long*)&counts[0], 1ULL);
+  bool insertion = false;
+  if (i < m) {
+    int32_t u = uv[2 * i], v = uv[2 * i + 1];
+    PairState s = pair_state(g, sorted, u, v);
+    bool adj = rule == SCOUT_CLASSIFY_BASE ? s.base_any : adjacent_now(s);
+    ins[i] = adj ? 0 : 1;
+    insertion = !adj;
+  }
+  block_add(insertion ? 1 : 0, &counts[0]);
 }
 
 __device__ __forceinline__ void flip_base(const GView& g, bool sorted, int32_t u, int32_t v, bool to_tomb) {
@@ -109,25 +138,29 @@ __device__ __forceinline__ void flip_base(const GView& g, bool sorted, int32_t u
 }
 
 // First inserted-row entry of u equal to x set to HOLE. True if one was found.
-__device__ __forceinline__ bool hole_first(const GView& g, int32_t u, int32_t x, int64_t* holes) {
+__device__ __forceinline__ bool hole_first(const GView& g, int32_t u, int32_t x) {
   int32_t* col = const_cast<int32_t*>(g.acol);
   for (int64_t k = g.aoff[u], e = g.aoff[u + 1]; k < e; k++) {
     if (col[k] == x) {
       col[k] = HOLE;
-      atomicAdd((unsigned long long*)holes, 1ULL);
       return true;
     }
   }
   return false;
 }
 
-__device__ __forceinline__ void push_new(int32_t src, int32_t dst, int64_t line, uint64_t* nkey, int32_t* ndst,
-                                         int64_t* ncount, int64_t cap) {
-  int64_t pos = (int64_t)atomicAdd((unsigned long long*)ncount, 1ULL);
-  if (pos < cap) {
-    nkey[pos] = ((uint64_t)(uint32_t)src << 32) | (uint64_t)line;
-    ndst[pos] = dst;
-  }
+struct NewEntry {
+  bool take;
+  int32_t src, dst;
+};
+
+// Appends the thread's new inserted-row entry (key src << 32 | line, value dst) with one
+// atomic per block.
+__device__ __forceinline__ void block_push_new(NewEntry e, int64_t line, uint64_t* nkey, int32_t* ndst,
+                                               int64_t* ncount, int64_t cap) {
+  uint64_t key = ((uint64_t)(uint32_t)e.src << 32) | (uint64_t)line;
+  int64_t pos = block_append(e.take, key, nkey, ncount, cap);
+  if (pos >= 0) ndst[pos] = e.dst;
 }
 
 // One thread per line of [lo, hi): the DeltaOverlay rules. done[i] = 1 if the line was applied.
@@ -135,43 +168,43 @@ __device__ __forceinline__ void push_new(int32_t src, int32_t dst, int64_t line,
 __global__ void k_apply(GView g, bool sorted, const int32_t* uv, const uint8_t* ins, int64_t lo, int64_t hi,
                         int conditional, uint8_t* done, uint64_t* nkey, int32_t* ndst, int64_t ncap, int64_t* box) {
   int64_t i = lo + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= hi) return;
-  int32_t u = uv[2 * i], v = uv[2 * i + 1];
-  bool insertion = ins[i] != 0;
-  PairState su = pair_state(g, sorted, u, v);
-  if (conditional) {
-    bool adj = adjacent_now(su);
-    if (insertion == adj) {
-      done[i] = 0;
-      return;
+  NewEntry e1{false, 0, 0}, e2{false, 0, 0};
+  int64_t holes = 0;
+  if (i < hi) {
+    int32_t u = uv[2 * i], v = uv[2 * i + 1];
+    bool insertion = ins[i] != 0;
+    PairState su = pair_state(g, sorted, u, v);
+    bool apply = !conditional || insertion != adjacent_now(su);
+    done[i] = apply ? 1 : 0;
+    if (apply && insertion) {
+      // un_tombstone (FlashGraph.h:284-291), else append to both inserted rows.
+      if (su.base_tomb || su.added_tomb) {
+        if (su.base_tomb) flip_base(g, sorted, u, v, false);
+        if (su.added_tomb) holes += hole_first(g, u, ~v);
+        PairState sv = pair_state(g, sorted, v, u);
+        if (sv.base_tomb) flip_base(g, sorted, v, u, false);
+        if (sv.added_tomb) holes += hole_first(g, v, ~u);
+      } else {
+        e1 = NewEntry{true, u, v};
+        e2 = NewEntry{true, v, u};
+      }
+    } else if (apply) {
+      // forget_delta_edge (FlashGraph.h:295-310), else tombstone both directions.
+      if (su.added_live) {
+        holes += hole_first(g, u, v);
+        holes += hole_first(g, v, u);
+      } else {
+        if (su.base_live) flip_base(g, sorted, u, v, true);
+        else if (!su.base_any && !su.added_tomb) e1 = NewEntry{true, u, ~v};
+        PairState sv = pair_state(g, sorted, v, u);
+        if (sv.base_live) flip_base(g, sorted, v, u, true);
+        else if (!sv.base_any && !sv.added_tomb) e2 = NewEntry{true, v, ~u};
+      }
     }
   }
-  done[i] = 1;
-  if (insertion) {
-    // un_tombstone (FlashGraph.h:284-291), else append to both inserted rows.
-    if (su.base_tomb || su.added_tomb) {
-      if (su.base_tomb) flip_base(g, sorted, u, v, false);
-      if (su.added_tomb) hole_first(g, u, ~v, &box[1]);
-      PairState sv = pair_state(g, sorted, v, u);
-      if (sv.base_tomb) flip_base(g, sorted, v, u, false);
-      if (sv.added_tomb) hole_first(g, v, ~u, &box[1]);
-    } else {
-      push_new(u, v, i, nkey, ndst, &box[0], ncap);
-      push_new(v, u, i, nkey, ndst, &box[0], ncap);
-    }
-  } else {
-    // forget_delta_edge (FlashGraph.h:295-310), else tombstone both directions.
-    if (su.added_live) {
-      hole_first(g, u, v, &box[1]);
-      hole_first(g, v, u, &box[1]);
-    } else {
-      if (su.base_live) flip_base(g, sorted, u, v, true);
-      else if (!su.base_any && !su.added_tomb) push_new(u, ~v, i, nkey, ndst, &box[0], ncap);
-      PairState sv = pair_state(g, sorted, v, u);
-      if (sv.base_live) flip_base(g, sorted, v, u, true);
-      else if (!sv.base_any && !sv.added_tomb) push_new(v, ~u, i, nkey, ndst, &box[0], ncap);
-    }
-  }
+  block_push_new(e1, i, nkey, ndst, &box[0], ncap);
+  block_push_new(e2, i, nkey, ndst, &box[0], ncap);
+  block_add(holes, &box[1]);
 }
 
 // cnt_old[v] = entries of inserted row v that are not holes.
@@ -220,13 +253,33 @@ __global__ void k_place_new(const uint64_t* nkey, const int32_t* ndst, int64_t c
   acol2[aoff2[src] + cnt_old[src] + (k - l)] = ndst[k];
 }
 
-// Live entries per vertex of the logical graph (wave per vertex).
-__global__ void k_logical_degree(GView g, int64_t* deg) {
-  for (int64_t v = global_wave(); v < g.n; v += total_waves()) {
+// Live entries per vertex of the logical graph: thread per vertex for rows up to 64 physical
+// entries, longer rows listed for k_logical_degree_long (wave per row).
+__global__ void k_logical_degree(GView g, int64_t* deg, int32_t* longs, int64_t* nlong, int64_t cap) {
+  int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  bool is_long = false;
+  if (v < g.n) {
     Row r = row_of(g, (int32_t)v);
+    int64_t len = row_len(r);
+    if (len > 64) {
+      is_long = true;
+    } else {
+      int64_t c = 0;
+      for (int64_t k = 0; k < len; k++) c += (row_at(g, r, k) >= 0);
+      deg[v] = c;
+    }
+  }
+  block_append(is_long, (int32_t)v, longs, nlong, cap);
+}
+
+__global__ void k_logical_degree_long(GView g, const int32_t* longs, const int64_t* nlong, int64_t* deg) {
+  int64_t nl = *nlong;
+  for (int64_t i = global_wave(); i < nl; i += total_waves()) {
+    int32_t v = longs[i];
+    Row r = row_of(g, v);
     int64_t c = 0;
     for (int64_t k = lane_id(); k < row_len(r); k += kWave) c += (row_at(g, r, k) >= 0);
-    for (int off = kWave / 2; off > 0; off >>= 1) c += __shfl_xor(c, off);
+    c = wave_sum(c);
     if (lane_id() == 0) deg[v] = c;
   }
 }
@@ -379,8 +432,15 @@ int graph_free(Ctx& c) {
 
 int check_rows(Ctx& c) {
   DevGraph& g = c.g;
-  SCOUT_TRY(hipMemsetAsync(c.d_box, 0, 2 * sizeof(int64_t), c.s));
-  if (g.n > 0) k_check_rows<<<kListGrid, kBlock, 0, c.s>>>(g.boff, g.bcol, g.n, c.d_box);
+  SCOUT_TRY(hipMemsetAsync(c.d_box, 0, 3 * sizeof(int64_t), c.s));
+  DevBuf<int32_t> longs;
+  SCOUT_CHECK(longs.reserve((int64_t)g.n + 1));
+  if (g.n > 0) {
+    k_check_rows<<<grid_for(g.n), kBlock, 0, c.s>>>(g.boff, g.bcol, g.n, c.d_box, longs.p, c.d_box + 2, longs.cap);
+    k_check_long_rows<<<kListGrid, kBlock, 0, c.s>>>(g.boff, g.bcol, g.n, longs.p, c.d_box + 2, c.d_box);
+  }
+  SCOUT_TRY(hipStreamSynchronize(c.s));
+  longs.release();
   SCOUT_CHECK(box_read(c, 2));
   if (c.h_box[1] != 0) {
     set_error("graph has adjacency entries outside [0, n)");
@@ -486,7 +546,14 @@ int graph_apply(Ctx& c, Channel& ch, int conditional) {
 }
 
 int graph_logical_degrees(Ctx& c, int64_t* d_deg) {
-  if (c.g.n > 0) k_logical_degree<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), d_deg);
+  if (c.g.n == 0) return SCOUT_OK;
+  DevBuf<int32_t> longs;
+  SCOUT_CHECK(longs.reserve((int64_t)c.g.n + 1));
+  SCOUT_TRY(hipMemsetAsync(c.d_box + 40, 0, sizeof(int64_t), c.s));
+  k_logical_degree<<<grid_for(c.g.n), kBlock, 0, c.s>>>(c.g.view(), d_deg, longs.p, c.d_box + 40, longs.cap);
+  k_logical_degree_long<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), longs.p, c.d_box + 40, d_deg);
+  SCOUT_TRY(hipStreamSynchronize(c.s));
+  longs.release();
   return SCOUT_OK;
 }
 

@@ -170,7 +170,7 @@ struct Ctx {
   int64_t* h_box = nullptr;    // pinned
   int64_t* d_box = nullptr;
   // generic work lists
-  DevBuf<int32_t> la, lb, lc, ld, le;
+  DevBuf<int32_t> la, lb, lc, ld, le, lf;
   DevBuf<int64_t> wa, wb;
   DevBuf<uint8_t> tmp;         // library temp storage (scan/sort)
   std::vector<std::pair<hipEvent_t, hipEvent_t>> spans;  // timing
