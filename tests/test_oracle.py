@@ -302,3 +302,73 @@ def test_mis_rules_keep_independence(seed):
         g.apply(uv, ins, conditional=True)
         mis, _ = g.mis_batch(uv, ins, mis, BIG)
         assert g.check_mis_independent(mis) == 0
+
+
+def _distinct_stream(og, rng, m, del_frac):
+    uv = []
+    for _ in range(m):
+        u = int(rng.integers(og.n))
+        row = og.row(u)
+        if rng.random() < del_frac and len(row):
+            uv.append((u, int(row[rng.integers(len(row))])))
+        else:
+            v = int(rng.integers(og.n))
+            if u != v:
+                uv.append((u, v))
+    uv = np.array(uv, dtype=np.int32)
+    key = np.minimum(uv[:, 0], uv[:, 1]).astype(np.int64) << 32 | np.maximum(uv[:, 0], uv[:, 1])
+    _, first = np.unique(key, return_index=True)
+    return np.ascontiguousarray(uv[np.sort(first)])
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_per_edit_reference_cc_is_exact_every_batch(sample_dir, seed):
+    """The per-edit CC baseline (bidirectional search, src/cc/scout_cc.cu:300-441) with no
+    bound settles every deletion, so its canonicalised labels equal the closed form after
+    every batch."""
+    from oracle_lib import _p, lib
+    egr = os.path.join(sample_dir, "sample.egr")
+    rng = np.random.default_rng(seed)
+    og, g2 = OracleGraph(egr), OracleGraph(egr)
+    L = lib()
+    rc = L.orc_refcc_create(g2.h, _p(og.cc()), 1 << 40, 0)
+    try:
+        for _ in range(5):
+            uv = _distinct_stream(og, rng, 300, 0.7)
+            ins = og.classify(uv)
+            og.apply(uv, ins)
+            st = np.zeros(8, dtype=np.int64)
+            L.orc_refcc_batch(rc, _p(uv), _p(ins), len(uv), _p(st))
+            lab = np.zeros(og.n, dtype=np.int32)
+            L.orc_refcc_finish(rc, _p(lab))
+            assert np.array_equal(lab, og.cc())
+            assert st[7] == 0  # nothing inconclusive without a bound
+    finally:
+        L.orc_refcc_free(rc)
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_per_edit_reference_bfs_insertions_are_exact(sample_dir, seed):
+    """The per-edit BFS baseline's insertion relax (src/bfs/scout_bfs.cu:460-480) is exact on
+    insertion-only streams; its deletion worklist (min neighbour level + 1) is not exact in
+    general, which is why the baseline is timed and not used as a checker."""
+    from oracle_lib import _p, lib
+    egr = os.path.join(sample_dir, "sample.egr")
+    rng = np.random.default_rng(seed)
+    og, g2 = OracleGraph(egr), OracleGraph(egr)
+    L = lib()
+    rb = L.orc_refbfs_create(g2.h, _p(og.bfs(0)), 0, 1 << 40, 0)
+    try:
+        for _ in range(4):
+            uv = _distinct_stream(og, rng, 200, 0.0)
+            uv = np.ascontiguousarray(uv[og.classify(uv) == 1])  # random pairs that are not edges
+            ins = og.classify(uv)
+            assert ins.all()
+            og.apply(uv, ins)
+            st = np.zeros(8, dtype=np.int64)
+            L.orc_refbfs_batch(rb, _p(uv), _p(ins), len(uv), _p(st))
+            lv = np.zeros(og.n, dtype=np.int32)
+            L.orc_refbfs_finish(rb, _p(lv))
+            assert np.array_equal(lv, og.bfs(0))
+    finally:
+        L.orc_refbfs_free(rb)
