@@ -88,10 +88,22 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-verify", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run 5 plain device copies of MIB MiB each (reference kernel for the "
+                         "FETCH_SIZE/WRITE_SIZE counter scale) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported by this build", file=sys.stderr)
         sys.exit(2)
+    if args.calibrate_copy:
+        import torch
+        a = torch.ones(args.calibrate_copy << 20, dtype=torch.uint8, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes": a.numel(), "copies": 5}), flush=True)
+        return
 
     strategy = sa.STRATEGY_LOCAL if args.strategy == "local" else sa.STRATEGY_RECOMPUTE
     total = args.warmup + args.steps

@@ -120,16 +120,18 @@ __global__ void k_cand_at_level(const int32_t* cand, int64_t nc, const int32_t* 
 //   <= kChunk  wave per candidate (k_invalidate);
 //   longer     chunk tasks: the parent search and the push of dependants are spread over
 //              many waves (k_inval_search, k_inval_decide, k_inval_push).
-// A candidate stays valid if some neighbour at lvl-1 is still valid; otherwise it is marked
-// invalid and its neighbours at lvl+1 are queued once (stamp dedup).
+// A candidate stays valid if some neighbour at lvl-1 is still valid; otherwise its level is
+// set to kInvalid and its neighbours at lvl+1 are queued once (stamp dedup). An invalid vertex
+// thus stops being a parent without a separate mark array; kInvalid is negative, so later
+// passes read it as unreached until the tentative levels overwrite it.
 constexpr int kShort = 16;
+constexpr int32_t kInvalid = -2;
 
-__device__ __forceinline__ bool valid_parent(const int32_t* level, const int32_t* mark, int32_t mepoch, int32_t x,
-                                             int32_t lvl) {
-  return level[x] == lvl - 1 && mark[x] != mepoch;
+__device__ __forceinline__ bool valid_parent(const int32_t* level, int32_t x, int32_t lvl) {
+  return level[x] == lvl - 1;
 }
 
-__global__ void k_invalidate_short(GView g, const int32_t* level, int32_t* mark, int32_t mepoch, const int32_t* cur,
+__global__ void k_invalidate_short(GView g, int32_t* level, const int32_t* cur,
                                    const int64_t* cur_count, int32_t lvl, int32_t* inval, int64_t* inval_count,
                                    int32_t* next, int64_t* next_count, int64_t cap, int32_t* stamp, int32_t epoch,
                                    int32_t* medium, int64_t* medium_count, int32_t* longs, int64_t* long_count,
@@ -154,7 +156,7 @@ __global__ void k_invalidate_short(GView g, const int32_t* level, int32_t* mark,
       invalid = true;
       for (int64_t k = 0; k < len; k++) {
         int32_t x = row_at(g, r, k);
-        if (x >= 0 && valid_parent(level, mark, mepoch, x, lvl)) {
+        if (x >= 0 && valid_parent(level, x, lvl)) {
           invalid = false;
           work += k + 1;
           break;
@@ -162,7 +164,7 @@ __global__ void k_invalidate_short(GView g, const int32_t* level, int32_t* mark,
       }
       if (invalid) {
         work += len;
-        mark[c] = mepoch;
+        level[c] = kInvalid;
       }
     }
     wave_append(invalid, c, inval, inval_count, cap);
@@ -181,7 +183,7 @@ __global__ void k_invalidate_short(GView g, const int32_t* level, int32_t* mark,
   block_add(work, scanned);
 }
 
-__global__ void k_invalidate(GView g, const int32_t* level, int32_t* mark, int32_t mepoch, const int32_t* cur,
+__global__ void k_invalidate(GView g, int32_t* level, const int32_t* cur,
                              const int64_t* cur_count, int32_t lvl, int32_t* inval, int64_t* inval_count,
                              int64_t inval_cap, int32_t* next, int64_t* next_count, int64_t cap, int32_t* stamp,
                              int32_t epoch, int64_t* scanned) {
@@ -191,14 +193,14 @@ __global__ void k_invalidate(GView g, const int32_t* level, int32_t* mark, int32
     int32_t c = cur[i];
     Row r = row_of(g, c);
     int64_t len = row_len(r);
-    int32_t parent = wave_find_first(g, r, 0, len, [&](int32_t x) { return valid_parent(level, mark, mepoch, x, lvl); });
-    if (parent >= 0) {
-      work += 1;
+    int64_t at = wave_find_first(g, r, 0, len, [&](int32_t x) { return valid_parent(level, x, lvl); });
+    if (at >= 0) {
+      work += at + 1;
       continue;
     }
     work += len;
     if (lane_id() == 0) {
-      mark[c] = mepoch;
+      level[c] = kInvalid;
       thread_append(c, inval, inval_count, inval_cap);
     }
     for (int64_t base = 0; base < len; base += kWave) {
@@ -228,37 +230,39 @@ __global__ void k_long_tasks(GView g, const int32_t* longs, const int64_t* long_
 }
 
 // Wave per chunk: a valid parent anywhere in the chunk marks the candidate supported.
-__global__ void k_inval_search(GView g, const int32_t* level, const int32_t* mark, int32_t mepoch, int32_t lvl,
-                               const int32_t* longs, const int64_t* tasks, const int64_t* task_count, int64_t task_cap,
-                               int32_t* supported, int32_t sepoch) {
+__global__ void k_inval_search(GView g, const int32_t* level, int32_t lvl, const int32_t* longs, const int64_t* tasks,
+                               const int64_t* task_count, int64_t task_cap, int32_t* supported, int32_t sepoch,
+                               int64_t* scanned) {
   int64_t nt = min(*task_count, task_cap);
+  int64_t work = 0;
   for (int64_t t = global_wave(); t < nt; t += total_waves()) {
     int64_t task = tasks[t];
     int32_t c = longs[task >> 32];
     if (supported[c] == sepoch) continue;
     Row r = row_of(g, c);
     int64_t lo = (task & 0xffffffffLL) * kChunk, hi = min(lo + kChunk, row_len(r));
-    int32_t parent = wave_find_first(g, r, lo, hi, [&](int32_t x) { return valid_parent(level, mark, mepoch, x, lvl); });
-    if (parent >= 0 && lane_id() == 0) supported[c] = sepoch;
+    int64_t at = wave_find_first(g, r, lo, hi, [&](int32_t x) { return valid_parent(level, x, lvl); });
+    work += (at >= 0 ? at + 1 : hi) - lo;
+    if (at >= 0 && lane_id() == 0) supported[c] = sepoch;
   }
+  if (lane_id() == 0 && work) atomicAdd((unsigned long long*)scanned, (unsigned long long)work);
 }
 
 // Thread per long candidate: unsupported ones become invalid.
 __global__ void k_inval_decide(const int32_t* longs, const int64_t* long_count, const int32_t* supported,
-                               int32_t sepoch, int32_t* mark, int32_t mepoch, int32_t* inval, int64_t* inval_count,
-                               int64_t cap) {
+                               int32_t sepoch, int32_t* level, int32_t* inval, int64_t* inval_count, int64_t cap) {
   int64_t nl = *long_count;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nl; i += (int64_t)gridDim.x * blockDim.x) {
     int32_t c = longs[i];
     if (supported[c] != sepoch) {
-      mark[c] = mepoch;
+      level[c] = kInvalid;
       thread_append(c, inval, inval_count, cap);
     }
   }
 }
 
 // Wave per chunk of an invalid long candidate: queue its neighbours at lvl+1.
-__global__ void k_inval_push(GView g, const int32_t* level, const int32_t* mark, int32_t mepoch, int32_t lvl,
+__global__ void k_inval_push(GView g, const int32_t* level, int32_t lvl,
                              const int32_t* longs, const int64_t* tasks, const int64_t* task_count, int64_t task_cap,
                              int32_t* next, int64_t* next_count, int64_t cap, int32_t* stamp, int32_t epoch,
                              int64_t* scanned) {
@@ -267,7 +271,7 @@ __global__ void k_inval_push(GView g, const int32_t* level, const int32_t* mark,
   for (int64_t t = global_wave(); t < nt; t += total_waves()) {
     int64_t task = tasks[t];
     int32_t c = longs[task >> 32];
-    if (mark[c] != mepoch) continue;
+    if (level[c] != kInvalid) continue;
     Row r = row_of(g, c);
     int64_t lo = (task & 0xffffffffLL) * kChunk, hi = min(lo + kChunk, row_len(r));
     work += hi - lo;
@@ -283,11 +287,6 @@ __global__ void k_inval_push(GView g, const int32_t* level, const int32_t* mark,
     }
   }
   if (lane_id() == 0 && work) atomicAdd((unsigned long long*)scanned, (unsigned long long)work);
-}
-
-__global__ void k_reset_levels(const int32_t* xs, int64_t nx, int32_t* level) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < nx) level[xs[i]] = -1;
 }
 
 // Wave per invalid vertex: 1 + smallest reached neighbour level, or -1.
@@ -430,9 +429,18 @@ __global__ void k_zero(int64_t* p, int count) {
   if ((int)threadIdx.x < count) p[threadIdx.x] = 0;
 }
 
-__global__ void k_zero_pair(int64_t* a, int64_t* b) {
-  *a = 0;
-  *b = 0;
+// Round bookkeeping on the device, so the host can queue several rounds between read-backs:
+// adds the round's list size to *sum (and counts the round in *nonempty if the list is not
+// empty), folds the previous round's chunk-task count into *max_tasks, then zeroes the next
+// list's count and the `zn` counters at `zero` (which may include *tasks).
+__global__ void k_round_begin(const int64_t* cur_count, int64_t* next_count, const int64_t* tasks, int64_t* zero,
+                              int zn, int64_t* sum, int64_t* nonempty, int64_t* max_tasks) {
+  int64_t t = *tasks, nc = *cur_count;
+  if (t > *max_tasks) *max_tasks = t;
+  *sum += nc;
+  if (nc > 0) *nonempty += 1;
+  *next_count = 0;
+  for (int i = 0; i < zn; i++) zero[i] = 0;
 }
 
 // box[0..7] = {0, INT32_MAX, -1, 0, 0, INT32_MAX, -1, 0}, box[8..15] = 0.
@@ -451,9 +459,15 @@ __global__ void k_set_one(int32_t* list, int32_t v, int64_t* count) {
   *count = 1;
 }
 
+// Rounds queued between two read-backs of the round counters: the first group is short (most
+// repairs end within a few levels), later groups double up to kMaxGroup. A round whose list
+// is empty costs a few launches and no work.
+constexpr int kFirstGroup = 2;
+constexpr int kMaxGroup = 16;
+
 // Level-synchronous relax starting at level `lvl0` with the current list already in `cur`
-// (count at box slot `cur_slot`) and optional seeds; runs until no frontier and no seed
-// above the current level remains. Returns the rounds run via *rounds.
+// (count in box slot 8) and optional seeds; runs until no frontier and no seed above the
+// current level remains. Adds the non-empty rounds to *rounds.
 int relax_levels(Ctx& c, int32_t* L, int32_t lvl0, int32_t max_seed, const int64_t* seeds, int64_t nseeds, int32_t epoch,
                  int64_t* rounds, int64_t* scanned_total) {
   int32_t n = c.g.n;
@@ -461,54 +475,52 @@ int relax_levels(Ctx& c, int32_t* L, int32_t lvl0, int32_t max_seed, const int64
   int64_t task_cap = c.wb.cap;
   int32_t* cur = c.lb.p;
   int32_t* nxt = c.lc.p;
-  // box slots: [8]/[9] frontier counts (current and next alternate), [10] task count,
-  // [11] scanned entries; the caller sets the current count in [8]
-  int64_t* cnt[2] = {c.d_box + 8, c.d_box + 9};
-  int64_t* btask = c.d_box + 10;
-  int64_t* bscan = c.d_box + 11;
-  k_zero<<<1, 64, 0, c.s>>>(bscan, 1);
+  int64_t* box = c.d_box;
+  // box slots: [8]/[9] frontier counts (current and next alternate), [10] chunk tasks,
+  // [11] scanned entries, [15] frontier vertices over all rounds, [16] largest task count,
+  // [17] non-empty rounds
+  int64_t* cnt[2] = {box + 8, box + 9};
+  k_zero<<<1, 64, 0, c.s>>>(box + 10, 2);
+  k_zero<<<1, 64, 0, c.s>>>(box + 15, 3);
   int32_t lvl = lvl0;
-  int64_t r = 0;
-  int64_t scanned_before = 0;
   int p = 0;
+  int group = kFirstGroup;
   while (true) {
-    int64_t* bcur = cnt[p];
-    int64_t* bnext = cnt[p ^ 1];
-    if (nseeds > 0 && lvl <= max_seed)
-      k_seed_at_level<<<grid_for(nseeds), kBlock, 0, c.s>>>(seeds, nseeds, L, lvl, cur, bcur, cap, c.stamp, epoch);
-    k_zero_pair<<<1, 1, 0, c.s>>>(bnext, btask);
     SCOUT_CHECK(prof_begin(c, PROF_BFS_RELAX));
-    k_relax<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, cur, bcur, lvl, nxt, bnext, cap, c.stamp, epoch,
-                                            c.wb.p, btask, task_cap, bscan);
-    k_relax_tasks<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, c.wb.p, btask, task_cap, lvl, nxt, bnext, cap,
-                                                  c.stamp, epoch, bscan);
+    for (int k = 0; k < group; k++) {
+      int64_t* bcur = cnt[p];
+      int64_t* bnext = cnt[p ^ 1];
+      if (nseeds > 0 && lvl <= max_seed)
+        k_seed_at_level<<<grid_for(nseeds), kBlock, 0, c.s>>>(seeds, nseeds, L, lvl, cur, bcur, cap, c.stamp, epoch);
+      k_round_begin<<<1, 1, 0, c.s>>>(bcur, bnext, box + 10, box + 10, 1, box + 15, box + 17, box + 16);
+      k_relax<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, cur, bcur, lvl, nxt, bnext, cap, c.stamp, epoch, c.wb.p,
+                                              box + 10, task_cap, box + 11);
+      k_relax_tasks<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, c.wb.p, box + 10, task_cap, lvl, nxt, bnext, cap,
+                                                    c.stamp, epoch, box + 11);
+      lvl++;
+      std::swap(cur, nxt);
+      p ^= 1;
+    }
     SCOUT_CHECK(prof_end(c, PROF_BFS_RELAX, 0));
-    r++;
-    SCOUT_TRY(hipMemcpyAsync(c.h_box + 8, c.d_box + 8, 4 * sizeof(int64_t), hipMemcpyDeviceToHost, c.s));
+    SCOUT_TRY(hipMemcpyAsync(c.h_box + 8, box + 8, 10 * sizeof(int64_t), hipMemcpyDeviceToHost, c.s));
     SCOUT_TRY(hipStreamSynchronize(c.s));
-    int64_t ncur = c.h_box[8 + p], nn = c.h_box[8 + (p ^ 1)];
-    // algorithmic bytes: per frontier vertex its id, level and the four row offsets (36 B);
-    // per scanned entry the neighbour id and its level (8 B); per lowered vertex the level
-    // write and its queue entry (8 B)
-    prof_bytes(c, PROF_BFS_RELAX, 36 * ncur + 8 * (c.h_box[11] - scanned_before) + 8 * nn);
-    scanned_before = c.h_box[11];
-    if (c.h_box[10] > task_cap) {
+    if (std::max(c.h_box[10], c.h_box[16]) > task_cap) {
       set_error("bfs relax: chunk task list overflow");
       return SCOUT_ERR_STATE;
     }
-    lvl++;
-    std::swap(cur, nxt);
-    p ^= 1;
-    if (nn == 0 && (nseeds == 0 || lvl > max_seed)) break;
+    if (c.h_box[8 + p] == 0 && (nseeds == 0 || lvl > max_seed)) break;
+    group = std::min(2 * group, kMaxGroup);
   }
-  *rounds += r;
+  // algorithmic bytes: per frontier vertex its id, level, the four row offsets and its queue
+  // entry (40 B); per scanned entry the neighbour id and its level (8 B)
+  prof_bytes(c, PROF_BFS_RELAX, 40 * c.h_box[15] + 8 * c.h_box[11]);
+  *rounds += c.h_box[17];
   *scanned_total += c.h_box[11];
   return SCOUT_OK;
 }
 
 int ensure_bfs(Ctx& c, int64_t extra) {
   int64_t n = c.g.n;
-  SCOUT_CHECK(ensure_scratch(c, &c.bfs_mark, 0));
   SCOUT_CHECK(c.la.reserve(n + 2 * extra + 1));
   SCOUT_CHECK(c.lb.reserve(n + 1));
   SCOUT_CHECK(c.lc.reserve(n + 1));
@@ -553,61 +565,63 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, scout_stats* st) {
   st->deletions += c.h_box[3];
   int64_t nx = 0, scanned = 0, rounds = 0;
   if (ncand > 0) {
-    // Invalidation, level by level. Counts: [8]/[9] current and next (alternating),
-    // [10] invalid vertices, [11] scanned entries.
-    int32_t mepoch = next_epoch(c);
+    // Invalidation, level by level, queued in groups of rounds. Counts: [8]/[9] candidates at
+    // this level / the next (alternating), [10] invalid vertices, [11] scanned entries,
+    // [12] medium rows, [13] long rows, [14] chunk tasks, [18] candidates over all rounds,
+    // [19] largest task count, [20] non-empty rounds.
     int32_t sepoch = next_epoch(c);
     int32_t* cur = c.lb.p;
     int32_t* nxt = c.lc.p;
     int64_t* cnt[2] = {box + 8, box + 9};
     int64_t* binval = box + 10;
     int64_t* bscan = box + 11;
-    int64_t inval_scan_before = 0;
+    k_zero<<<1, 64, 0, c.s>>>(box + 8, 7);
+    k_zero<<<1, 64, 0, c.s>>>(box + 18, 3);
     int p = 0;
-    for (int32_t lvl = cmin;; lvl++) {
-      int64_t* bcur = cnt[p];
-      int64_t* bnext = cnt[p ^ 1];
-      if (lvl <= cmax)
-        k_cand_at_level<<<grid_for(ncand), kBlock, 0, c.s>>>(c.la.p, ncand, L, lvl, cur, bcur, cap, c.stamp,
-                                                              sepoch);
-      k_zero<<<1, 64, 0, c.s>>>(bnext, 1);
+    int group = kFirstGroup;
+    int32_t lvl = cmin;
+    while (true) {
       SCOUT_CHECK(prof_begin(c, PROF_BFS_INVALIDATE));
-      // [12] medium count, [13] long count, [14] chunk tasks
-      k_zero<<<1, 64, 0, c.s>>>(box + 12, 3);
-      k_invalidate_short<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, c.bfs_mark, mepoch, cur, bcur, lvl, c.ld.p,
-                                                         binval, nxt, bnext, cap, c.stamp, sepoch, c.le.p, box + 12,
-                                                         c.lf.p, box + 13, bscan);
-      k_invalidate<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, c.bfs_mark, mepoch, c.le.p, box + 12, lvl, c.ld.p,
-                                                   binval, cap, nxt, bnext, cap, c.stamp, sepoch, bscan);
-      k_long_tasks<<<64, kBlock, 0, c.s>>>(c.g.view(), c.lf.p, box + 13, c.wb.p, box + 14, c.wb.cap);
-      int32_t supepoch = next_epoch(c);
-      k_inval_search<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, c.bfs_mark, mepoch, lvl, c.lf.p, c.wb.p, box + 14,
-                                                     c.wb.cap, c.aux, supepoch);
-      k_inval_decide<<<64, kBlock, 0, c.s>>>(c.lf.p, box + 13, c.aux, supepoch, c.bfs_mark, mepoch, c.ld.p, binval,
-                                              cap);
-      k_inval_push<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, c.bfs_mark, mepoch, lvl, c.lf.p, c.wb.p, box + 14,
-                                                   c.wb.cap, nxt, bnext, cap, c.stamp, sepoch, bscan);
+      for (int k = 0; k < group; k++) {
+        int64_t* bcur = cnt[p];
+        int64_t* bnext = cnt[p ^ 1];
+        if (lvl <= cmax)
+          k_cand_at_level<<<grid_for(ncand), kBlock, 0, c.s>>>(c.la.p, ncand, L, lvl, cur, bcur, cap, c.stamp,
+                                                                sepoch);
+        k_round_begin<<<1, 1, 0, c.s>>>(bcur, bnext, box + 14, box + 12, 3, box + 18, box + 20, box + 19);
+        k_invalidate_short<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, cur, bcur, lvl, c.ld.p, binval, nxt, bnext,
+                                                           cap, c.stamp, sepoch, c.le.p, box + 12, c.lf.p, box + 13,
+                                                           bscan);
+        k_invalidate<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, c.le.p, box + 12, lvl, c.ld.p, binval, cap, nxt,
+                                                     bnext, cap, c.stamp, sepoch, bscan);
+        k_long_tasks<<<64, kBlock, 0, c.s>>>(c.g.view(), c.lf.p, box + 13, c.wb.p, box + 14, c.wb.cap);
+        int32_t supepoch = next_epoch(c);
+        k_inval_search<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, lvl, c.lf.p, c.wb.p, box + 14, c.wb.cap, c.aux,
+                                                       supepoch, bscan);
+        k_inval_decide<<<64, kBlock, 0, c.s>>>(c.lf.p, box + 13, c.aux, supepoch, L, c.ld.p, binval, cap);
+        k_inval_push<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, lvl, c.lf.p, c.wb.p, box + 14, c.wb.cap, nxt,
+                                                     bnext, cap, c.stamp, sepoch, bscan);
+        lvl++;
+        std::swap(cur, nxt);
+        p ^= 1;
+      }
       SCOUT_CHECK(prof_end(c, PROF_BFS_INVALIDATE, 0));
-      rounds++;
-      SCOUT_TRY(hipMemcpyAsync(c.h_box + 8, box + 8, 7 * sizeof(int64_t), hipMemcpyDeviceToHost, c.s));
+      SCOUT_TRY(hipMemcpyAsync(c.h_box + 8, box + 8, 13 * sizeof(int64_t), hipMemcpyDeviceToHost, c.s));
       SCOUT_TRY(hipStreamSynchronize(c.s));
-      if (c.h_box[14] > c.wb.cap) {
+      if (std::max(c.h_box[14], c.h_box[19]) > c.wb.cap) {
         set_error("bfs invalidation: chunk task list overflow");
         return SCOUT_ERR_STATE;
       }
-      int64_t nn = c.h_box[8 + (p ^ 1)];
-      // per candidate its id, level and row offsets (36 B); per scanned entry the neighbour,
-      // its level and (for parents) its invalid mark (12 B); per queued vertex 8 B
-      prof_bytes(c, PROF_BFS_INVALIDATE, 36 * c.h_box[8 + p] + 12 * (c.h_box[11] - inval_scan_before) + 8 * nn);
-      inval_scan_before = c.h_box[11];
-      std::swap(cur, nxt);
-      p ^= 1;
-      if (nn == 0 && lvl + 1 > cmax) break;
+      if (c.h_box[8 + p] == 0 && lvl > cmax) break;
+      group = std::min(2 * group, kMaxGroup);
     }
+    rounds += c.h_box[20];
+    // per candidate its id, level, row offsets and queue entry (40 B); per scanned entry the
+    // neighbour and its level (8 B)
+    prof_bytes(c, PROF_BFS_INVALIDATE, 40 * c.h_box[18] + 8 * c.h_box[11]);
     nx = c.h_box[10];
     scanned += c.h_box[11];
     if (nx > 0) {
-      k_reset_levels<<<grid_for(nx), kBlock, 0, c.s>>>(c.ld.p, nx, L);
       // [11] scanned, [12] medium rows, [13] chunk tasks
       k_zero<<<1, 64, 0, c.s>>>(box + 11, 3);
       k_tentative_short<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, c.ld.p, nx, c.le.p, c.lf.p, box + 12,

@@ -92,7 +92,7 @@ __global__ void k_color_detect(GView g, const int32_t* round, int64_t nr, const 
     Row r = row_of(g, v);
     int64_t len = row_len(r);
     work += len;
-    int32_t hit = wave_find_first(g, r, 0, len, [&](int32_t w) { return w < v && marker[w] == mine; });
+    int64_t hit = wave_find_first(g, r, 0, len, [&](int32_t w) { return w < v && marker[w] == mine; });
     if (hit >= 0 && lane_id() == 0) thread_append(v, retry, retry_count, cap);
   }
   if (lane_id() == 0 && work) atomicAdd((unsigned long long*)scanned, (unsigned long long)work);

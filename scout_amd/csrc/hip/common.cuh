@@ -159,22 +159,18 @@ __device__ __forceinline__ void wave_walk(const GView& g, const Row& r, int64_t 
 }
 
 // Wave searches entries [lo, hi) of row r for a live neighbour satisfying pred. All lanes
-// return the same answer (the neighbour at the smallest index, or -1).
+// return the same answer: the index of the first such entry, or -1.
 template <class P>
-__device__ __forceinline__ int32_t wave_find_first(const GView& g, const Row& r, int64_t lo, int64_t hi, P&& pred) {
+__device__ __forceinline__ int64_t wave_find_first(const GView& g, const Row& r, int64_t lo, int64_t hi, P&& pred) {
   for (int64_t base = lo; base < hi; base += kWave) {
     int64_t k = base + lane_id();
-    int32_t x = -1;
     bool hit = false;
     if (k < hi) {
-      x = row_at(g, r, k);
+      int32_t x = row_at(g, r, k);
       hit = x >= 0 && pred(x);
     }
     unsigned long long mask = __ballot(hit);
-    if (mask) {
-      int first = __ffsll((long long)mask) - 1;
-      return __shfl(x, first);
-    }
+    if (mask) return base + (__ffsll((long long)mask) - 1);
   }
   return -1;
 }
@@ -207,12 +203,15 @@ struct PairState {
   bool base_any;      // base row u holds v or ~v
   bool added_live;    // inserted row u holds v
   bool added_tomb;    // inserted row u holds the tombstone ~v
+  int64_t lo, hi;     // base entries of row u searched for v (base_range)
 };
 
 __device__ __forceinline__ PairState pair_state(const GView& g, bool sorted, int32_t u, int32_t v) {
-  PairState s{false, false, false, false, false};
+  PairState s{false, false, false, false, false, 0, 0};
   int64_t lo, hi;
   base_range(g, sorted, u, v, lo, hi);
+  s.lo = lo;
+  s.hi = hi;
   for (int64_t k = lo; k < hi; k++) {
     int32_t x = g.bcol[k];
     if (x == v) s.base_live = true;

@@ -27,7 +27,6 @@ int next_epoch(Ctx& c) {
     size_t bytes = ((size_t)c.g.n + 1) * sizeof(int32_t);
     SCOUT_TRY(hipMemsetAsync(c.stamp, 0, bytes, c.s));
     if (c.aux) SCOUT_TRY(hipMemsetAsync(c.aux, 0xff, bytes, c.s));
-    if (c.bfs_mark) SCOUT_TRY(hipMemsetAsync(c.bfs_mark, 0xff, bytes, c.s));
     c.epoch = 0;
   }
   return ++c.epoch;
@@ -49,7 +48,7 @@ int alloc_n(Ctx& c, void** p, size_t elem) {
 
 void free_state(Ctx& c) {
   void* ps[] = {c.level, c.label, c.color, c.mis, c.stamp, c.aux, c.aux2,
-                c.bfs_mark, c.cc_owner, c.col_marker, c.mis_minw, c.mis_tag, c.cc_level};
+                c.cc_owner, c.col_marker, c.mis_minw, c.mis_tag, c.cc_level};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   c.cc_level = nullptr;
@@ -58,7 +57,7 @@ void free_state(Ctx& c) {
   c.level = c.label = c.color = nullptr;
   c.mis = nullptr;
   c.stamp = c.aux = c.aux2 = nullptr;
-  c.bfs_mark = c.cc_owner = c.col_marker = c.mis_minw = c.mis_tag = nullptr;
+  c.cc_owner = c.col_marker = c.mis_minw = c.mis_tag = nullptr;
 }
 
 int check_ctx(scout_ctx* x) {

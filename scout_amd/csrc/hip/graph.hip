@@ -128,12 +128,11 @@ __global__ void k_classify(GView g, bool sorted, const int32_t* uv, int64_t m, i
   block_add(insertion ? 1 : 0, &counts[0]);
 }
 
-__device__ __forceinline__ void flip_base(const GView& g, bool sorted, int32_t u, int32_t v, bool to_tomb) {
-  int64_t lo, hi;
-  base_range(g, sorted, u, v, lo, hi);
+// Flips the base entries of the pair in the range pair_state found (v <-> ~v).
+__device__ __forceinline__ void flip_base(const GView& g, const PairState& s, int32_t v, bool to_tomb) {
   int32_t* col = const_cast<int32_t*>(g.bcol);
   int32_t from = to_tomb ? v : ~v, to = to_tomb ? ~v : v;
-  for (int64_t k = lo; k < hi; k++)
+  for (int64_t k = s.lo; k < s.hi; k++)
     if (col[k] == from) col[k] = to;
 }
 
@@ -179,10 +178,10 @@ __global__ void k_apply(GView g, bool sorted, const int32_t* uv, const uint8_t* 
     if (apply && insertion) {
       // un_tombstone (FlashGraph.h:284-291), else append to both inserted rows.
       if (su.base_tomb || su.added_tomb) {
-        if (su.base_tomb) flip_base(g, sorted, u, v, false);
+        if (su.base_tomb) flip_base(g, su, v, false);
         if (su.added_tomb) holes += hole_first(g, u, ~v);
         PairState sv = pair_state(g, sorted, v, u);
-        if (sv.base_tomb) flip_base(g, sorted, v, u, false);
+        if (sv.base_tomb) flip_base(g, sv, u, false);
         if (sv.added_tomb) holes += hole_first(g, v, ~u);
       } else {
         e1 = NewEntry{true, u, v};
@@ -194,10 +193,10 @@ __global__ void k_apply(GView g, bool sorted, const int32_t* uv, const uint8_t* 
         holes += hole_first(g, u, v);
         holes += hole_first(g, v, u);
       } else {
-        if (su.base_live) flip_base(g, sorted, u, v, true);
+        if (su.base_live) flip_base(g, su, v, true);
         else if (!su.base_any && !su.added_tomb) e1 = NewEntry{true, u, ~v};
         PairState sv = pair_state(g, sorted, v, u);
-        if (sv.base_live) flip_base(g, sorted, v, u, true);
+        if (sv.base_live) flip_base(g, sv, u, true);
         else if (!sv.base_any && !sv.added_tomb) e2 = NewEntry{true, v, ~u};
       }
     }

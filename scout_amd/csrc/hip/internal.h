@@ -156,7 +156,6 @@ struct Ctx {
   int32_t* aux2 = nullptr;
   int32_t epoch = 0;
   // problem-specific n-sized scratch (allocated on first use)
-  int32_t* bfs_mark = nullptr;   // invalidation stamps
   int32_t* cc_owner = nullptr;   // search group of each claimed vertex (-1 = none)
   int32_t* col_marker = nullptr; // this round's proposal (-2 = none)
   int32_t* mis_minw = nullptr;   // smallest speculative writer line
