@@ -76,6 +76,63 @@ def cpu_baseline(eng, n, batches_host, seconds):
                       f"bfs {t_bfs:.1f}s, cc {t_cc:.1f}s"}
 
 
+# Kernels of each profiled family (the launches scout_prof_* brackets on the compute stream).
+FAMILY_KERNELS = {
+    "bfs_invalidate": ["k_cand_at_level", "k_invalidate_short", "k_invalidate", "k_long_tasks", "k_inval_search",
+                       "k_inval_decide", "k_inval_push"],
+    "bfs_relax": ["k_seed_at_level", "k_relax", "k_relax_tasks"],
+    "cc_expand": ["k_iota", "k_cc_unreached", "k_cc_unreached_hook", "k_cc_hook_tasks", "k_cc_final"],
+    "apply": ["k_apply", "k_count_old", "k_count_new", "k_copy_old", "k_place_new"],
+    "classify": ["k_dup_insert", "k_dup_mark", "k_classify"],
+}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_rmat26.json")
+
+
+def _kernel_short_name(name):
+    import re
+    return re.sub(r"\(.*", "", name.replace("scout::(anonymous namespace)::", ""))
+
+
+def summarize_pmc(pmc_dir):
+    """HBM bytes per batch of each kernel family from two rocprofv3 --pmc passes (FETCH_SIZE,
+    WRITE_SIZE, KiB per dispatch) over a bench run, each scaled by what the same counter reads
+    for a plain device copy of known size. Dispatches before the first batch (graph generation,
+    initial BFS/CC) are left out; every batch runs k_classify once."""
+    import csv
+
+    def rows(sub):
+        with open(os.path.join(pmc_dir, sub, "run_counter_collection.csv")) as f:
+            return sorted(csv.DictReader(f), key=lambda r: int(r["Dispatch_Id"]))
+
+    def copy_factor(sub, counter):
+        vals = [float(r["Counter_Value"]) for r in rows(sub) if "copyBuffer" in r["Kernel_Name"]
+                and r["Counter_Name"] == counter]
+        return (1024 * 1024) / (sum(vals) / len(vals))  # 1 GiB copy = 1048576 KiB each way
+
+    out = {"copy_factor": {"FETCH_SIZE": copy_factor("copy_fetch", "FETCH_SIZE"),
+                           "WRITE_SIZE": copy_factor("copy_write", "WRITE_SIZE")}}
+    fam = {k: {"fetch_bytes": 0.0, "write_bytes": 0.0} for k in FAMILY_KERNELS}
+    batches = 0
+    for sub, counter, key in (("bench_fetch", "FETCH_SIZE", "fetch_bytes"), ("bench_write", "WRITE_SIZE", "write_bytes")):
+        started, nb = False, 0
+        for r in rows(sub):
+            name = _kernel_short_name(r["Kernel_Name"])
+            if name == "k_classify":
+                started = True
+                nb += 1
+            if not started or r["Counter_Name"] != counter:
+                continue
+            for f, ks in FAMILY_KERNELS.items():
+                if name in ks:
+                    fam[f][key] += float(r["Counter_Value"]) * 1024 * out["copy_factor"][counter]
+        batches = nb
+    out["batches"] = batches
+    out["per_batch"] = {f: {"fetch_bytes": round(v["fetch_bytes"] / batches), "write_bytes": round(v["write_bytes"] / batches),
+                            "traffic_bytes": round((v["fetch_bytes"] + v["write_bytes"]) / batches)}
+                        for f, v in fam.items()}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -88,6 +145,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-verify", action="store_true")
+    ap.add_argument("--summarize-pmc", metavar="DIR",
+                    help="only summarise rocprofv3 --pmc passes under DIR (copy_fetch, copy_write, bench_fetch, "
+                         "bench_write) into per-batch HBM traffic per kernel family, print it and exit")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 5 plain device copies of MIB MiB each (reference kernel for the "
                          "FETCH_SIZE/WRITE_SIZE counter scale) and exit")
@@ -95,6 +155,9 @@ def main():
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported by this build", file=sys.stderr)
         sys.exit(2)
+    if args.summarize_pmc:
+        print(json.dumps(summarize_pmc(args.summarize_pmc), indent=1))
+        return
     if args.calibrate_copy:
         import torch
         a = torch.ones(args.calibrate_copy << 20, dtype=torch.uint8, device="cuda")
@@ -188,9 +251,19 @@ def main():
     dom = max(prof, key=lambda k: prof[k]["ms"])
     d = prof[dom]
     achieved = d["bytes"] / (d["ms"] / 1e3) / 1e9 if d["ms"] > 0 else 0.0
+    # traffic: HBM bytes per batch of the dominant family, from the committed rocprofv3 --pmc
+    # summary of this workload (scaled FETCH_SIZE + WRITE_SIZE), next to its algorithmic bytes
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE) and args.scale == 26 and args.batch == 1_000_000:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)["per_batch"].get(dom)
+        if t:
+            traffic = t["traffic_bytes"]
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
-                "launches": d["launches"], "avg_launch_ms": round(d["ms"] / max(d["launches"], 1), 4)}
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom,
+                "launch": "one batch's work of the family", "launches": args.steps,
+                "avg_launch_ms": round(d["ms"] / args.steps, 4),
+                "algorithmic_bytes_per_launch": round(d["bytes"] / args.steps)}
 
     cpu = None
     if not args.no_cpu and first_batch_host is not None:

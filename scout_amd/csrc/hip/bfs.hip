@@ -4,74 +4,143 @@
 // rounds, a deletion between consecutive levels starts Jacobi recompute rounds over the
 // child's dependants, and a cascade past b = 200 vertices is deferred to one full BFS after
 // the stream (src/bfs/scout_bfs.cu:436-515). Here one batch is one device job whose result
-// equals textbook BFS on the updated graph:
-//   1. candidates: the deeper endpoint of every deleted consecutive-level pair;
-//   2. invalidation, level by level: a candidate at level l with no valid neighbour at
-//      level l-1 is invalid, and its neighbours at level l+1 become candidates;
-//   3. every invalid vertex restarts at 1 + the smallest valid neighbour level (or
-//      unreached);
+// equals textbook BFS on the updated graph.
+//
+// Besides the levels L, the engine keeps a parent P[v] for every reached vertex other than
+// the source: a neighbour with L[P[v]] = L[v] - 1 (-1 for the source and unreached vertices).
+// A batch then runs:
+//   1. candidates: every vertex whose parent edge was deleted;
+//   2. invalidation, level by level: a candidate at level l that has a neighbour at level
+//      l-1 which is still valid takes it as its new parent; otherwise it is invalid, and its
+//      children (neighbours whose parent it was) become candidates at level l+1;
+//   3. every invalid vertex restarts at 1 + the smallest valid neighbour level (that
+//      neighbour becomes its parent), or unreached;
 //   4. insertions that now join u and v lower the far endpoint by CAS-min (gpu_relax,
 //      src/bfs/scout_bfs.cu:46-54);
 //   5. level-synchronous relax from all lowered vertices, processed in increasing level so
-//      each vertex is expanded once, at its final level.
-// Valid vertices keep a parent chain of valid vertices, so their old level is still an
-// upper bound; relaxing every vertex whose value dropped, in level order, is Dijkstra with
-// unit weights, hence exact.
+//      each vertex is expanded once, at its final level; each successful lowering records
+//      the lowering vertex as the parent.
+// Valid vertices keep a parent chain of valid vertices down to the source, so their old level
+// is still the length of a path and, deletions only lengthening paths, still exact until the
+// insertions are relaxed. Relaxing every vertex whose value dropped, in level order, is
+// Dijkstra with unit weights, hence exact. Only parent edges start work: a deleted edge
+// between consecutive levels that is not a parent edge changes nothing.
+//
+// Rows are walked by length: up to kShort entries by one thread (the entries and their
+// levels are gathered first, so the loads of one row are in flight together), up to kChunk by
+// one wave, longer rows as kChunk-entry tasks spread over many waves.
+#include <algorithm>
+
 #include "common.cuh"
 
 namespace scout {
 
 namespace {
 
-// Wave per frontier vertex: items with row length <= kChunk are walked whole; longer rows
-// are split into (vertex, chunk) tasks for k_relax_tasks. Frontier entries whose level is no
-// longer `lvl` were lowered after being queued and are skipped.
-__global__ void k_relax(GView g, int32_t* level, const int32_t* front, const int64_t* count, int32_t lvl,
-                        int32_t* next, int64_t* next_count, int64_t cap, int32_t* stamp, int32_t epoch,
-                        int64_t* tasks, int64_t* task_count, int64_t task_cap, int64_t* scanned) {
-  int64_t nf = *count;
-  int64_t work = 0;
-  for (int64_t i = global_wave(); i < nf; i += total_waves()) {
-    int32_t f = front[i];
-    if (level[f] != lvl) continue;
-    Row r = row_of(g, f);
-    int64_t len = row_len(r);
-    if (len > kChunk) {
-      if (lane_id() == 0) {
-        int64_t chunks = (len + kChunk - 1) / kChunk;
-        int64_t pos = (int64_t)atomicAdd((unsigned long long*)task_count, (unsigned long long)chunks);
-        for (int64_t k = 0; k < chunks && pos + k < task_cap; k++) tasks[pos + k] = ((int64_t)f << 32) | k;
-      }
-      continue;
-    }
-    work += len;
-    for (int64_t base = 0; base < len; base += kWave) {
-      int64_t k = base + lane_id();
-      bool take = false;
-      int32_t w = -1;
-      if (k < len) {
-        w = row_at(g, r, k);
-        if (w >= 0 && cas_min_level(&level[w], lvl + 1)) take = atomicExch(&stamp[w], epoch) != epoch;
-      }
-      wave_append(take, w, next, next_count, cap);
-    }
-  }
-  if (lane_id() == 0 && work) atomicAdd((unsigned long long*)scanned, (unsigned long long)work);
+constexpr int kShort = 16;
+// Level of a vertex found invalid in this batch until its tentative level is written. It is
+// negative, so it never matches a level and reads as unreached.
+constexpr int32_t kInvalid = -2;
+// Rounds queued between two read-backs of the round counters: the first group is short (most
+// repairs end within a few levels), later groups double up to kMaxGroup. A round whose list
+// is empty costs a few launches and no work.
+constexpr int kFirstGroup = 2;
+constexpr int kMaxGroup = 16;
+
+// Gathers the row entries of a short row (len <= kShort) into registers; -1 past the end.
+__device__ __forceinline__ void gather_short(const GView& g, const Row& r, int64_t len, bool active,
+                                             int32_t (&xs)[kShort]) {
+#pragma unroll
+  for (int k = 0; k < kShort; k++) xs[k] = (active && k < len) ? row_at(g, r, k) : -1;
 }
 
-__global__ void k_relax_tasks(GView g, int32_t* level, const int64_t* tasks, const int64_t* task_count,
-                              int64_t task_cap, int32_t lvl, int32_t* next, int64_t* next_count, int64_t cap,
-                              int32_t* stamp, int32_t epoch, int64_t* scanned) {
+// Appends (f << 32 | chunk) tasks covering a row of `len` entries.
+__device__ __forceinline__ void push_chunks(int64_t f, int64_t len, int64_t* tasks, int64_t* task_count,
+                                            int64_t task_cap) {
+  int64_t chunks = (len + kChunk - 1) / kChunk;
+  int64_t pos = (int64_t)atomicAdd((unsigned long long*)task_count, (unsigned long long)chunks);
+  for (int64_t k = 0; k < chunks && pos + k < task_cap; k++) tasks[pos + k] = (f << 32) | k;
+}
+
+// Lowers w to lvl+1 on behalf of f; on success f becomes w's parent. True if w must be queued
+// (lowered and not queued before in this relax).
+__device__ __forceinline__ bool relax_to(int32_t* level, int32_t* parent, int32_t* stamp, int32_t epoch, int32_t f,
+                                         int32_t w, int32_t lvl) {
+  if (!cas_min_level(&level[w], lvl + 1)) return false;
+  parent[w] = f;
+  return atomicExch(&stamp[w], epoch) != epoch;
+}
+
+// ---------------------------------------------------------------------------------------
+// Relax rounds
+// ---------------------------------------------------------------------------------------
+
+// Thread per frontier vertex still at level lvl. Short rows are relaxed here; rows up to
+// kChunk go to the wave list, longer rows to chunk tasks.
+__global__ void k_relax_short(GView g, int32_t* level, int32_t* parent, const int32_t* front, const int64_t* count,
+                              int32_t lvl, int32_t* next, int64_t* next_count, int64_t cap, int32_t* stamp,
+                              int32_t epoch, int32_t* medium, int64_t* medium_count, int64_t* tasks,
+                              int64_t* task_count, int64_t task_cap, int64_t* scanned) {
+  int64_t nf = *count;
+  int64_t work = 0;
+  // the trip count is uniform across the block, so the wave-wide appends see every lane
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < nf; base += (int64_t)gridDim.x * blockDim.x) {
+    int64_t i = base + threadIdx.x;
+    int32_t f = -1;
+    Row r{0, 0, 0, 0};
+    int64_t len = 0;
+    bool active = false;
+    if (i < nf) {
+      f = front[i];
+      if (level[f] == lvl) {
+        r = row_of(g, f);
+        len = row_len(r);
+        active = true;
+      }
+    }
+    bool is_short = active && len <= kShort;
+    wave_append(active && !is_short && len <= kChunk, f, medium, medium_count, cap);
+    if (active && len > kChunk) push_chunks(f, len, tasks, task_count, task_cap);
+    int32_t xs[kShort];
+    gather_short(g, r, len, is_short, xs);
+    int32_t lv[kShort];
+#pragma unroll
+    for (int k = 0; k < kShort; k++) lv[k] = xs[k] >= 0 ? level[xs[k]] : 0;
+    if (is_short) work += len;
+#pragma unroll
+    for (int k = 0; k < kShort; k++) {
+      bool take = false;
+      if (xs[k] >= 0 && (lv[k] == -1 || lv[k] > lvl + 1)) take = relax_to(level, parent, stamp, epoch, f, xs[k], lvl);
+      wave_append(take, xs[k], next, next_count, cap);
+    }
+  }
+  block_add(work, scanned);
+}
+
+// Wave per medium frontier vertex (listed by k_relax_short) or per chunk task of a long one.
+__global__ void k_relax_rows(GView g, int32_t* level, int32_t* parent, const int32_t* medium,
+                             const int64_t* medium_count, const int64_t* tasks, const int64_t* task_count,
+                             int64_t task_cap, int32_t lvl, int32_t* next, int64_t* next_count, int64_t cap,
+                             int32_t* stamp, int32_t epoch, int64_t* scanned) {
+  int64_t nm = *medium_count;
   int64_t nt = min(*task_count, task_cap);
   int64_t work = 0;
-  for (int64_t i = global_wave(); i < nt; i += total_waves()) {
-    int64_t t = tasks[i];
-    int32_t f = (int32_t)(t >> 32);
-    int64_t chunk = t & 0xffffffffLL;
-    Row r = row_of(g, f);
-    int64_t lo = chunk * kChunk, hi = lo + kChunk;
-    int64_t len = row_len(r);
-    if (hi > len) hi = len;
+  for (int64_t t = global_wave(); t < nm + nt; t += total_waves()) {
+    int32_t f;
+    int64_t lo, hi;
+    Row r;
+    if (t < nm) {
+      f = medium[t];
+      r = row_of(g, f);
+      lo = 0;
+      hi = row_len(r);
+    } else {
+      int64_t task = tasks[t - nm];
+      f = (int32_t)(task >> 32);
+      r = row_of(g, f);
+      lo = (task & 0xffffffffLL) * kChunk;
+      hi = min(lo + kChunk, row_len(r));
+    }
     work += hi - lo;
     for (int64_t base = lo; base < hi; base += kWave) {
       int64_t k = base + lane_id();
@@ -79,7 +148,10 @@ __global__ void k_relax_tasks(GView g, int32_t* level, const int64_t* tasks, con
       int32_t w = -1;
       if (k < hi) {
         w = row_at(g, r, k);
-        if (w >= 0 && cas_min_level(&level[w], lvl + 1)) take = atomicExch(&stamp[w], epoch) != epoch;
+        if (w >= 0) {
+          int32_t lw = level[w];
+          if (lw == -1 || lw > lvl + 1) take = relax_to(level, parent, stamp, epoch, f, w, lvl);
+        }
       }
       wave_append(take, w, next, next_count, cap);
     }
@@ -87,18 +159,22 @@ __global__ void k_relax_tasks(GView g, int32_t* level, const int64_t* tasks, con
   if (lane_id() == 0 && work) atomicAdd((unsigned long long*)scanned, (unsigned long long)work);
 }
 
-// Deeper endpoint of each deleted pair whose levels were consecutive.
+// ---------------------------------------------------------------------------------------
+// Deletion candidates and invalidation
+// ---------------------------------------------------------------------------------------
+
+// Endpoint whose parent edge a deletion line removed.
 // box[0] = candidates, box[1] = min level, box[2] = max level, box[3] = deletion lines.
 __global__ void k_del_candidates(const int32_t* uv, const uint8_t* ins, int64_t m, const int32_t* level,
-                                 int32_t* cand, int64_t cap, int64_t* box) {
+                                 const int32_t* parent, int32_t* cand, int64_t cap, int64_t* box) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   bool del = i < m && !ins[i];
   int32_t c = -1, lc = 0;
   if (del) {
     int32_t u = uv[2 * i], v = uv[2 * i + 1];
     int32_t lu = level[u], lv = level[v];
-    if (lu >= 0 && lv == lu + 1) c = v, lc = lv;
-    else if (lv >= 0 && lu == lv + 1) c = u, lc = lu;
+    if (lu >= 0 && lv == lu + 1 && parent[v] == u) c = v, lc = lv;
+    else if (lv >= 0 && lu == lv + 1 && parent[u] == v) c = u, lc = lu;
   }
   block_append(c >= 0, c, cand, &box[0], cap);
   block_add(del ? 1 : 0, &box[3]);
@@ -114,88 +190,72 @@ __global__ void k_cand_at_level(const int32_t* cand, int64_t nc, const int32_t* 
   block_append(take, c, cur, cur_count, cap);
 }
 
-// Invalidation of the candidates at level lvl, by row length:
-//   <= kShort  thread per candidate (lanes step through the row together, so appends stay
-//              wave-aggregated);
-//   <= kChunk  wave per candidate (k_invalidate);
-//   longer     chunk tasks: the parent search and the push of dependants are spread over
-//              many waves (k_inval_search, k_inval_decide, k_inval_push).
-// A candidate stays valid if some neighbour at lvl-1 is still valid; otherwise its level is
-// set to kInvalid and its neighbours at lvl+1 are queued once (stamp dedup). An invalid vertex
-// thus stops being a parent without a separate mark array; kInvalid is negative, so later
-// passes read it as unreached until the tentative levels overwrite it.
-constexpr int kShort = 16;
-constexpr int32_t kInvalid = -2;
-
-__device__ __forceinline__ bool valid_parent(const int32_t* level, int32_t x, int32_t lvl) {
-  return level[x] == lvl - 1;
-}
-
-__global__ void k_invalidate_short(GView g, int32_t* level, const int32_t* cur,
+// Invalidation of the candidates at level lvl. Thread per candidate; rows longer than kShort
+// go to the wave list (<= kChunk) or the long list (chunk tasks).
+__global__ void k_invalidate_short(GView g, int32_t* level, int32_t* parent, const int32_t* cur,
                                    const int64_t* cur_count, int32_t lvl, int32_t* inval, int64_t* inval_count,
                                    int32_t* next, int64_t* next_count, int64_t cap, int32_t* stamp, int32_t epoch,
                                    int32_t* medium, int64_t* medium_count, int32_t* longs, int64_t* long_count,
                                    int64_t* scanned) {
   int64_t nc = *cur_count;
   int64_t work = 0;
-  // the trip count is uniform across the wave: every lane loops until the wave is done
   for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < nc; base += (int64_t)gridDim.x * blockDim.x) {
     int64_t i = base + threadIdx.x;
     int32_t c = -1;
     Row r{0, 0, 0, 0};
     int64_t len = 0;
-    bool is_short = false;
-    if (i < nc) {
+    bool active = i < nc;
+    if (active) {
       c = cur[i];
       r = row_of(g, c);
       len = row_len(r);
-      is_short = len <= kShort;
     }
-    bool invalid = false;
+    bool is_short = active && len <= kShort;
+    int32_t xs[kShort];
+    gather_short(g, r, len, is_short, xs);
+    int32_t lv[kShort];
+#pragma unroll
+    for (int k = 0; k < kShort; k++) lv[k] = xs[k] >= 0 ? level[xs[k]] : kInvalid;
+    int32_t found = -1;
+#pragma unroll
+    for (int k = 0; k < kShort; k++)
+      if (found < 0 && lv[k] == lvl - 1) found = xs[k];
+    bool invalid = is_short && found < 0;
     if (is_short) {
-      invalid = true;
-      for (int64_t k = 0; k < len; k++) {
-        int32_t x = row_at(g, r, k);
-        if (x >= 0 && valid_parent(level, x, lvl)) {
-          invalid = false;
-          work += k + 1;
-          break;
-        }
-      }
-      if (invalid) {
-        work += len;
-        level[c] = kInvalid;
-      }
+      work += len;
+      if (invalid) level[c] = kInvalid;
+      else parent[c] = found;
     }
     wave_append(invalid, c, inval, inval_count, cap);
-    wave_append(i < nc && !is_short && len <= kChunk, c, medium, medium_count, cap);
-    wave_append(i < nc && len > kChunk, c, longs, long_count, cap);
+    wave_append(active && !is_short && len <= kChunk, c, medium, medium_count, cap);
+    wave_append(active && len > kChunk, c, longs, long_count, cap);
+    // children of an invalid candidate: neighbours whose parent it is
+    int32_t pw[kShort];
+#pragma unroll
+    for (int k = 0; k < kShort; k++) pw[k] = (invalid && xs[k] >= 0) ? parent[xs[k]] : -1;
+#pragma unroll
     for (int k = 0; k < kShort; k++) {
-      bool take = false;
-      int32_t w = -1;
-      if (invalid && k < len) {
-        w = row_at(g, r, k);
-        take = w >= 0 && level[w] == lvl + 1 && atomicExch(&stamp[w], epoch) != epoch;
-      }
-      wave_append(take, w, next, next_count, cap);
+      bool take = pw[k] == c && invalid && level[xs[k]] == lvl + 1 && atomicExch(&stamp[xs[k]], epoch) != epoch;
+      wave_append(take, xs[k], next, next_count, cap);
     }
   }
   block_add(work, scanned);
 }
 
-__global__ void k_invalidate(GView g, int32_t* level, const int32_t* cur,
-                             const int64_t* cur_count, int32_t lvl, int32_t* inval, int64_t* inval_count,
-                             int64_t inval_cap, int32_t* next, int64_t* next_count, int64_t cap, int32_t* stamp,
-                             int32_t epoch, int64_t* scanned) {
+// Wave per medium candidate.
+__global__ void k_invalidate(GView g, int32_t* level, int32_t* parent, const int32_t* cur, const int64_t* cur_count,
+                             int32_t lvl, int32_t* inval, int64_t* inval_count, int64_t inval_cap, int32_t* next,
+                             int64_t* next_count, int64_t cap, int32_t* stamp, int32_t epoch, int64_t* scanned) {
   int64_t nc = *cur_count;
   int64_t work = 0;
   for (int64_t i = global_wave(); i < nc; i += total_waves()) {
     int32_t c = cur[i];
     Row r = row_of(g, c);
     int64_t len = row_len(r);
-    int64_t at = wave_find_first(g, r, 0, len, [&](int32_t x) { return valid_parent(level, x, lvl); });
+    int64_t at = wave_find_first(g, r, 0, len, [&](int32_t x) { return level[x] == lvl - 1; });
     if (at >= 0) {
       work += at + 1;
+      if (lane_id() == 0) parent[c] = row_at(g, r, at);
       continue;
     }
     work += len;
@@ -209,7 +269,7 @@ __global__ void k_invalidate(GView g, int32_t* level, const int32_t* cur,
       int32_t w = -1;
       if (k < len) {
         w = row_at(g, r, k);
-        take = w >= 0 && level[w] == lvl + 1 && atomicExch(&stamp[w], epoch) != epoch;
+        take = w >= 0 && parent[w] == c && level[w] == lvl + 1 && atomicExch(&stamp[w], epoch) != epoch;
       }
       wave_append(take, w, next, next_count, cap);
     }
@@ -217,22 +277,19 @@ __global__ void k_invalidate(GView g, int32_t* level, const int32_t* cur,
   if (lane_id() == 0 && work) atomicAdd((unsigned long long*)scanned, (unsigned long long)work);
 }
 
-// Chunk tasks (candidate index in `longs` << 32 | chunk) for every long candidate.
+// Chunk tasks (index in `longs` << 32 | chunk) for every long candidate.
 __global__ void k_long_tasks(GView g, const int32_t* longs, const int64_t* long_count, int64_t* tasks,
                              int64_t* task_count, int64_t task_cap) {
   int64_t nl = *long_count;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nl; i += (int64_t)gridDim.x * blockDim.x) {
-    int64_t len = row_len(row_of(g, longs[i]));
-    int64_t chunks = (len + kChunk - 1) / kChunk;
-    int64_t pos = (int64_t)atomicAdd((unsigned long long*)task_count, (unsigned long long)chunks);
-    for (int64_t k = 0; k < chunks && pos + k < task_cap; k++) tasks[pos + k] = (i << 32) | k;
-  }
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nl; i += (int64_t)gridDim.x * blockDim.x)
+    push_chunks(i, row_len(row_of(g, longs[i])), tasks, task_count, task_cap);
 }
 
-// Wave per chunk: a valid parent anywhere in the chunk marks the candidate supported.
-__global__ void k_inval_search(GView g, const int32_t* level, int32_t lvl, const int32_t* longs, const int64_t* tasks,
-                               const int64_t* task_count, int64_t task_cap, int32_t* supported, int32_t sepoch,
-                               int64_t* scanned) {
+// Wave per chunk: a neighbour at lvl-1 anywhere in the chunk supports the candidate and
+// becomes its parent (any of them will do).
+__global__ void k_inval_search(GView g, const int32_t* level, int32_t* parent, int32_t lvl, const int32_t* longs,
+                               const int64_t* tasks, const int64_t* task_count, int64_t task_cap, int32_t* supported,
+                               int32_t sepoch, int64_t* scanned) {
   int64_t nt = min(*task_count, task_cap);
   int64_t work = 0;
   for (int64_t t = global_wave(); t < nt; t += total_waves()) {
@@ -241,9 +298,12 @@ __global__ void k_inval_search(GView g, const int32_t* level, int32_t lvl, const
     if (supported[c] == sepoch) continue;
     Row r = row_of(g, c);
     int64_t lo = (task & 0xffffffffLL) * kChunk, hi = min(lo + kChunk, row_len(r));
-    int64_t at = wave_find_first(g, r, lo, hi, [&](int32_t x) { return valid_parent(level, x, lvl); });
+    int64_t at = wave_find_first(g, r, lo, hi, [&](int32_t x) { return level[x] == lvl - 1; });
     work += (at >= 0 ? at + 1 : hi) - lo;
-    if (at >= 0 && lane_id() == 0) supported[c] = sepoch;
+    if (at >= 0 && lane_id() == 0) {
+      parent[c] = row_at(g, r, at);
+      supported[c] = sepoch;
+    }
   }
   if (lane_id() == 0 && work) atomicAdd((unsigned long long*)scanned, (unsigned long long)work);
 }
@@ -261,11 +321,10 @@ __global__ void k_inval_decide(const int32_t* longs, const int64_t* long_count, 
   }
 }
 
-// Wave per chunk of an invalid long candidate: queue its neighbours at lvl+1.
-__global__ void k_inval_push(GView g, const int32_t* level, int32_t lvl,
-                             const int32_t* longs, const int64_t* tasks, const int64_t* task_count, int64_t task_cap,
-                             int32_t* next, int64_t* next_count, int64_t cap, int32_t* stamp, int32_t epoch,
-                             int64_t* scanned) {
+// Wave per chunk of an invalid long candidate: queue its children.
+__global__ void k_inval_push(GView g, const int32_t* level, const int32_t* parent, int32_t lvl, const int32_t* longs,
+                             const int64_t* tasks, const int64_t* task_count, int64_t task_cap, int32_t* next,
+                             int64_t* next_count, int64_t cap, int32_t* stamp, int32_t epoch, int64_t* scanned) {
   int64_t nt = min(*task_count, task_cap);
   int64_t work = 0;
   for (int64_t t = global_wave(); t < nt; t += total_waves()) {
@@ -281,7 +340,7 @@ __global__ void k_inval_push(GView g, const int32_t* level, int32_t lvl,
       int32_t w = -1;
       if (k < hi) {
         w = row_at(g, r, k);
-        take = w >= 0 && level[w] == lvl + 1 && atomicExch(&stamp[w], epoch) != epoch;
+        take = w >= 0 && parent[w] == c && level[w] == lvl + 1 && atomicExch(&stamp[w], epoch) != epoch;
       }
       wave_append(take, w, next, next_count, cap);
     }
@@ -289,48 +348,57 @@ __global__ void k_inval_push(GView g, const int32_t* level, int32_t lvl,
   if (lane_id() == 0 && work) atomicAdd((unsigned long long*)scanned, (unsigned long long)work);
 }
 
-// Wave per invalid vertex: 1 + smallest reached neighbour level, or -1.
-// Tentative level of each invalidated vertex: the smallest level among its reached
-// neighbours (tent[i] = INT32_MAX if none). Thread per vertex for rows of at most kShort
-// entries; longer rows go to the wave list (<= kChunk) or to chunk tasks.
-__global__ void k_tentative_short(GView g, const int32_t* level, const int32_t* xs, int64_t nx, int32_t* tent,
+// ---------------------------------------------------------------------------------------
+// Tentative levels of invalid vertices and insertion seeds
+// ---------------------------------------------------------------------------------------
+
+// Smallest (level << 32 | neighbour) over the reached neighbours of each invalidated vertex
+// (UINT64_MAX if none). Thread per vertex for rows of at most kShort entries; longer rows go
+// to the wave list (<= kChunk) or to chunk tasks.
+__global__ void k_tentative_short(GView g, const int32_t* level, const int32_t* xs, int64_t nx, uint64_t* tent,
                                   int32_t* medium, int64_t* medium_count, int64_t medium_cap, int64_t* tasks,
                                   int64_t* task_count, int64_t task_cap, int64_t* scanned) {
   int64_t work = 0;
   for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < nx; base += (int64_t)gridDim.x * blockDim.x) {
     int64_t i = base + threadIdx.x;
-    bool mid = false;
-    if (i < nx) {
-      Row r = row_of(g, xs[i]);
-      int64_t len = row_len(r);
-      int32_t best = INT32_MAX;
-      if (len <= kShort) {
-        for (int64_t k = 0; k < len; k++) {
-          int32_t w = row_at(g, r, k);
-          if (w >= 0) {
-            int32_t lw = level[w];
-            if (lw >= 0 && lw < best) best = lw;
-          }
-        }
-        work += len;
-      } else if (len <= kChunk) {
-        mid = true;
-      } else {
-        int64_t chunks = (len + kChunk - 1) / kChunk;
-        int64_t pos = (int64_t)atomicAdd((unsigned long long*)task_count, (unsigned long long)chunks);
-        for (int64_t k = 0; k < chunks && pos + k < task_cap; k++) tasks[pos + k] = (i << 32) | k;
-      }
-      tent[i] = best;
+    Row r{0, 0, 0, 0};
+    int64_t len = 0;
+    bool active = i < nx;
+    if (active) {
+      r = row_of(g, xs[i]);
+      len = row_len(r);
     }
-    wave_append(mid, (int32_t)(i < nx ? i : 0), medium, medium_count, medium_cap);
+    bool is_short = active && len <= kShort;
+    int32_t ys[kShort];
+    gather_short(g, r, len, is_short, ys);
+    uint64_t best = UINT64_MAX;
+#pragma unroll
+    for (int k = 0; k < kShort; k++) {
+      int32_t lw = ys[k] >= 0 ? level[ys[k]] : -1;
+      if (lw >= 0) best = min(best, ((uint64_t)(uint32_t)lw << 32) | (uint32_t)ys[k]);
+    }
+    if (is_short) work += len;
+    if (active) {
+      tent[i] = best;
+      if (len > kChunk) push_chunks(i, len, tasks, task_count, task_cap);
+    }
+    wave_append(active && !is_short && len <= kChunk, (int32_t)(active ? i : 0), medium, medium_count, medium_cap);
   }
   block_add(work, scanned);
+}
+
+__device__ __forceinline__ uint64_t wave_min_u64(uint64_t v) {
+  for (int off = kWave / 2; off > 0; off >>= 1) {
+    uint64_t o = ((uint64_t)(uint32_t)__shfl_xor((int)(v >> 32), off) << 32) | (uint32_t)__shfl_xor((int)v, off);
+    v = min(v, o);
+  }
+  return v;
 }
 
 // Wave per medium row (index list) or per chunk of a long row; atomicMin into tent.
 __global__ void k_tentative_rows(GView g, const int32_t* level, const int32_t* xs, const int32_t* medium,
                                  const int64_t* medium_count, const int64_t* tasks, const int64_t* task_count,
-                                 int64_t task_cap, int32_t* tent, int64_t* scanned) {
+                                 int64_t task_cap, uint64_t* tent, int64_t* scanned) {
   int64_t nm = *medium_count;
   int64_t nt = min(*task_count, task_cap);
   int64_t work = 0;
@@ -350,44 +418,50 @@ __global__ void k_tentative_rows(GView g, const int32_t* level, const int32_t* x
       hi = min(lo + kChunk, row_len(r));
     }
     work += hi - lo;
-    int32_t best = INT32_MAX;
+    uint64_t best = UINT64_MAX;
     for (int64_t k = lo + lane_id(); k < hi; k += kWave) {
       int32_t w = row_at(g, r, k);
       if (w >= 0) {
         int32_t lw = level[w];
-        if (lw >= 0 && lw < best) best = lw;
+        if (lw >= 0) best = min(best, ((uint64_t)(uint32_t)lw << 32) | (uint32_t)w);
       }
     }
-    for (int off = kWave / 2; off > 0; off >>= 1) best = min(best, __shfl_xor(best, off));
-    if (lane_id() == 0 && best != INT32_MAX) atomicMin(&tent[i], best);
+    best = wave_min_u64(best);
+    if (lane_id() == 0 && best != UINT64_MAX) atomicMin((unsigned long long*)&tent[i], (unsigned long long)best);
   }
   if (lane_id() == 0 && work) atomicAdd((unsigned long long*)scanned, (unsigned long long)work);
 }
 
-// Seeds are packed level << 32 | vertex. box[4] = seeds, box[5] = min level, box[6] = max.
-__device__ __forceinline__ void block_push_seed(bool take, int32_t v, int32_t lv, int64_t* seeds, int64_t* box,
-                                                int64_t cap) {
+// Seeds are packed level << 32 | vertex, with the vertex that lowered them in seed_par.
+// box[4] = seeds, box[5] = min level, box[6] = max.
+__device__ __forceinline__ void block_push_seed(bool take, int32_t v, int32_t lv, int32_t par, int64_t* seeds,
+                                                int32_t* seed_par, int64_t* box, int64_t cap) {
   int64_t packed = ((int64_t)lv << 32) | (int64_t)(uint32_t)v;
-  block_append(take, packed, seeds, &box[4], cap);
+  int64_t pos = block_append(take, packed, seeds, &box[4], cap);
+  if (pos >= 0) seed_par[pos] = par;
   block_min_max(take ? lv : INT64_MAX, take ? lv : -1, &box[5], &box[6]);
 }
 
-__global__ void k_apply_tentative(const int32_t* xs, const int32_t* tent, int64_t nx, int32_t* level, int64_t* seeds,
-                                  int64_t cap, int64_t* box) {
+__global__ void k_apply_tentative(const int32_t* xs, const uint64_t* tent, int64_t nx, int32_t* level,
+                                  int32_t* parent, int64_t* seeds, int32_t* seed_par, int64_t cap, int64_t* box) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  int32_t t = -1, x = 0;
+  int32_t t = -1, x = 0, p = -1;
   if (i < nx) {
-    int32_t best = tent[i];
-    t = best == INT32_MAX ? -1 : best + 1;
+    uint64_t best = tent[i];
+    if (best != UINT64_MAX) {
+      t = (int32_t)(best >> 32) + 1;
+      p = (int32_t)(best & 0xffffffffu);
+    }
     x = xs[i];
     level[x] = t;
+    parent[x] = p;
   }
-  block_push_seed(t >= 0, x, t, seeds, box, cap);
+  block_push_seed(t >= 0, x, t, p, seeds, seed_par, box, cap);
 }
 
 // Insertion lines whose endpoints are adjacent after the batch: lower the far endpoint.
 __global__ void k_insert_seeds(GView g, bool sorted, const int32_t* uv, const uint8_t* ins, int64_t m,
-                               int32_t* level, int64_t* seeds, int64_t cap, int64_t* box) {
+                               int32_t* level, int64_t* seeds, int32_t* seed_par, int64_t cap, int64_t* box) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   bool is_ins = i < m && ins[i];
   bool sv = false, su = false;
@@ -405,8 +479,19 @@ __global__ void k_insert_seeds(GView g, bool sorted, const int32_t* uv, const ui
     }
   }
   block_add(is_ins ? 1 : 0, &box[7]);
-  block_push_seed(sv, v, lu + 1, seeds, box, cap);
-  block_push_seed(su, u, lv + 1, seeds, box, cap);
+  block_push_seed(sv, v, lu + 1, u, seeds, seed_par, box, cap);
+  block_push_seed(su, u, lv + 1, v, seeds, seed_par, box, cap);
+}
+
+// A seed that still holds the level it was lowered to takes the vertex that lowered it as
+// its parent (several seeds of one vertex at one level are all valid parents).
+__global__ void k_seed_parents(const int64_t* seeds, const int32_t* seed_par, int64_t ns, const int32_t* level,
+                               int32_t* parent) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ns) return;
+  int64_t s = seeds[i];
+  int32_t lv = (int32_t)(s >> 32), v = (int32_t)(s & 0xffffffff);
+  if (level[v] == lv) parent[v] = seed_par[i];
 }
 
 // Seeds at `lvl` that still hold that level join the current frontier once.
@@ -423,6 +508,10 @@ __global__ void k_seed_at_level(const int64_t* seeds, int64_t ns, const int32_t*
   }
   block_append(take, v, cur, cur_count, cap);
 }
+
+// ---------------------------------------------------------------------------------------
+// Bookkeeping
+// ---------------------------------------------------------------------------------------
 
 // Zeroes `count` consecutive counters (round bookkeeping without a copy or fill kernel).
 __global__ void k_zero(int64_t* p, int count) {
@@ -449,9 +538,12 @@ __global__ void k_init_box(int64_t* box) {
   if (t < 16) box[t] = (t == 1 || t == 5) ? (int64_t)INT32_MAX : (t == 2 || t == 6) ? -1 : 0;
 }
 
-__global__ void k_fill_level(int32_t* level, int32_t n, int32_t src) {
+__global__ void k_fill_level(int32_t* level, int32_t* parent, int32_t n, int32_t src) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) level[i] = (i == src) ? 0 : -1;
+  if (i < n) {
+    level[i] = (i == src) ? 0 : -1;
+    parent[i] = -1;
+  }
 }
 
 __global__ void k_set_one(int32_t* list, int32_t v, int64_t* count) {
@@ -459,17 +551,27 @@ __global__ void k_set_one(int32_t* list, int32_t v, int64_t* count) {
   *count = 1;
 }
 
-// Rounds queued between two read-backs of the round counters: the first group is short (most
-// repairs end within a few levels), later groups double up to kMaxGroup. A round whose list
-// is empty costs a few launches and no work.
-constexpr int kFirstGroup = 2;
-constexpr int kMaxGroup = 16;
+// Wave per vertex: a parent for every reached vertex of a level array that came from outside
+// (an init file or a download); *missing counts reached vertices with no neighbour one level up.
+__global__ void k_find_parents(GView g, const int32_t* level, int32_t* parent, int32_t n, int64_t* missing) {
+  for (int64_t v = global_wave(); v < n; v += total_waves()) {
+    int32_t lv = level[v];
+    int32_t p = -1;
+    if (lv > 0) {
+      Row r = row_of(g, (int32_t)v);
+      int64_t at = wave_find_first(g, r, 0, row_len(r), [&](int32_t x) { return level[x] == lv - 1; });
+      if (at >= 0) p = row_at(g, r, at);
+      else if (lane_id() == 0) atomicAdd((unsigned long long*)missing, 1ULL);
+    }
+    if (lane_id() == 0) parent[v] = p;
+  }
+}
 
 // Level-synchronous relax starting at level `lvl0` with the current list already in `cur`
 // (count in box slot 8) and optional seeds; runs until no frontier and no seed above the
 // current level remains. Adds the non-empty rounds to *rounds.
-int relax_levels(Ctx& c, int32_t* L, int32_t lvl0, int32_t max_seed, const int64_t* seeds, int64_t nseeds, int32_t epoch,
-                 int64_t* rounds, int64_t* scanned_total) {
+int relax_levels(Ctx& c, int32_t* L, int32_t* P, int32_t lvl0, int32_t max_seed, const int64_t* seeds, int64_t nseeds,
+                 int32_t epoch, int64_t* rounds, int64_t* scanned_total) {
   int32_t n = c.g.n;
   int64_t cap = (int64_t)n + 1;
   int64_t task_cap = c.wb.cap;
@@ -477,10 +579,10 @@ int relax_levels(Ctx& c, int32_t* L, int32_t lvl0, int32_t max_seed, const int64
   int32_t* nxt = c.lc.p;
   int64_t* box = c.d_box;
   // box slots: [8]/[9] frontier counts (current and next alternate), [10] chunk tasks,
-  // [11] scanned entries, [15] frontier vertices over all rounds, [16] largest task count,
-  // [17] non-empty rounds
+  // [11] medium rows, [12] scanned entries, [15] frontier vertices over all rounds,
+  // [16] largest task count, [17] non-empty rounds
   int64_t* cnt[2] = {box + 8, box + 9};
-  k_zero<<<1, 64, 0, c.s>>>(box + 10, 2);
+  k_zero<<<1, 64, 0, c.s>>>(box + 10, 3);
   k_zero<<<1, 64, 0, c.s>>>(box + 15, 3);
   int32_t lvl = lvl0;
   int p = 0;
@@ -492,11 +594,11 @@ int relax_levels(Ctx& c, int32_t* L, int32_t lvl0, int32_t max_seed, const int64
       int64_t* bnext = cnt[p ^ 1];
       if (nseeds > 0 && lvl <= max_seed)
         k_seed_at_level<<<grid_for(nseeds), kBlock, 0, c.s>>>(seeds, nseeds, L, lvl, cur, bcur, cap, c.stamp, epoch);
-      k_round_begin<<<1, 1, 0, c.s>>>(bcur, bnext, box + 10, box + 10, 1, box + 15, box + 17, box + 16);
-      k_relax<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, cur, bcur, lvl, nxt, bnext, cap, c.stamp, epoch, c.wb.p,
-                                              box + 10, task_cap, box + 11);
-      k_relax_tasks<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, c.wb.p, box + 10, task_cap, lvl, nxt, bnext, cap,
-                                                    c.stamp, epoch, box + 11);
+      k_round_begin<<<1, 1, 0, c.s>>>(bcur, bnext, box + 10, box + 10, 2, box + 15, box + 17, box + 16);
+      k_relax_short<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, P, cur, bcur, lvl, nxt, bnext, cap, c.stamp, epoch,
+                                                    c.le.p, box + 11, c.wb.p, box + 10, task_cap, box + 12);
+      k_relax_rows<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, P, c.le.p, box + 11, c.wb.p, box + 10, task_cap, lvl,
+                                                   nxt, bnext, cap, c.stamp, epoch, box + 12);
       lvl++;
       std::swap(cur, nxt);
       p ^= 1;
@@ -513,9 +615,9 @@ int relax_levels(Ctx& c, int32_t* L, int32_t lvl0, int32_t max_seed, const int64
   }
   // algorithmic bytes: per frontier vertex its id, level, the four row offsets and its queue
   // entry (40 B); per scanned entry the neighbour id and its level (8 B)
-  prof_bytes(c, PROF_BFS_RELAX, 40 * c.h_box[15] + 8 * c.h_box[11]);
+  prof_bytes(c, PROF_BFS_RELAX, 40 * c.h_box[15] + 8 * c.h_box[12]);
   *rounds += c.h_box[17];
-  *scanned_total += c.h_box[11];
+  *scanned_total += c.h_box[12];
   return SCOUT_OK;
 }
 
@@ -527,7 +629,9 @@ int ensure_bfs(Ctx& c, int64_t extra) {
   SCOUT_CHECK(c.ld.reserve(n + 1));
   SCOUT_CHECK(c.le.reserve(n + 1));
   SCOUT_CHECK(c.lf.reserve(n + 1));
+  SCOUT_CHECK(c.lg.reserve(n + 2 * extra + 1));
   SCOUT_CHECK(c.wa.reserve(n + 2 * extra + 1));
+  SCOUT_CHECK(c.wc.reserve(n + 1));
   // chunk tasks: at most 2 * entries / kChunk (every split row has more than kChunk entries)
   SCOUT_CHECK(c.wb.reserve(2 * (c.g.E + c.g.A) / kChunk + 1024));
   return SCOUT_OK;
@@ -535,30 +639,45 @@ int ensure_bfs(Ctx& c, int64_t extra) {
 
 }  // namespace
 
-int bfs_full_on(Ctx& c, int32_t* L, int32_t src, int64_t* rounds, int64_t* scanned) {
+int compute_parents(Ctx& c, const int32_t* L, int32_t* P) {
+  int32_t n = c.g.n;
+  if (n == 0) return SCOUT_OK;
+  k_zero<<<1, 64, 0, c.s>>>(c.d_box, 1);
+  k_find_parents<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, P, n, c.d_box);
+  SCOUT_CHECK(box_read(c, 1));
+  if (c.h_box[0] != 0) {
+    set_error("levels are not BFS levels of the graph: " + std::to_string(c.h_box[0]) +
+              " reached vertices have no neighbour one level up");
+    return SCOUT_ERR_ARG;
+  }
+  return SCOUT_OK;
+}
+
+int bfs_full_on(Ctx& c, int32_t* L, int32_t* P, int32_t src, int64_t* rounds, int64_t* scanned) {
   SCOUT_CHECK(ensure_bfs(c, 0));
   int32_t n = c.g.n;
   if (n == 0) return SCOUT_OK;
-  k_fill_level<<<grid_for(n), kBlock, 0, c.s>>>(L, n, src);
+  k_fill_level<<<grid_for(n), kBlock, 0, c.s>>>(L, P, n, src);
   int32_t epoch = next_epoch(c);
   k_set_one<<<1, 1, 0, c.s>>>(c.lb.p, src, c.d_box + 8);
   int64_t r = 0, sc = 0;
-  SCOUT_CHECK(relax_levels(c, L, 0, -1, nullptr, 0, epoch, &r, &sc));
+  SCOUT_CHECK(relax_levels(c, L, P, 0, -1, nullptr, 0, epoch, &r, &sc));
   if (rounds) *rounds += r;
   if (scanned) *scanned += sc;
   return SCOUT_OK;
 }
 
-int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, scout_stats* st) {
+int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, int32_t* P, scout_stats* st) {
   SCOUT_CHECK(ensure_bfs(c, ch.m));
   int64_t m = ch.m;
   int32_t n = c.g.n;
   int64_t cap = (int64_t)n + 1;
+  int64_t seed_cap = std::min(c.wa.cap, c.lg.cap);  // seeds and their parents
   int64_t* box = c.d_box;
   // box: [0] cand, [1] min, [2] max, [3] deletions, [4] seeds, [5] seed min, [6] seed max, [7] insertions
   k_init_box<<<1, 64, 0, c.s>>>(box);
   if (m > 0)
-    k_del_candidates<<<grid_for(m), kBlock, 0, c.s>>>(ch.d_uv.p, ch.d_ins.p, m, L, c.la.p, c.la.cap, box);
+    k_del_candidates<<<grid_for(m), kBlock, 0, c.s>>>(ch.d_uv.p, ch.d_ins.p, m, L, P, c.la.p, c.la.cap, box);
   SCOUT_CHECK(box_read(c, 8));
   int64_t ncand = c.h_box[0];
   int32_t cmin = (int32_t)c.h_box[1], cmax = (int32_t)c.h_box[2];
@@ -589,17 +708,17 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, scout_stats* st) {
           k_cand_at_level<<<grid_for(ncand), kBlock, 0, c.s>>>(c.la.p, ncand, L, lvl, cur, bcur, cap, c.stamp,
                                                                 sepoch);
         k_round_begin<<<1, 1, 0, c.s>>>(bcur, bnext, box + 14, box + 12, 3, box + 18, box + 20, box + 19);
-        k_invalidate_short<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, cur, bcur, lvl, c.ld.p, binval, nxt, bnext,
-                                                           cap, c.stamp, sepoch, c.le.p, box + 12, c.lf.p, box + 13,
-                                                           bscan);
-        k_invalidate<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, c.le.p, box + 12, lvl, c.ld.p, binval, cap, nxt,
-                                                     bnext, cap, c.stamp, sepoch, bscan);
+        k_invalidate_short<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, P, cur, bcur, lvl, c.ld.p, binval, nxt,
+                                                           bnext, cap, c.stamp, sepoch, c.le.p, box + 12, c.lf.p,
+                                                           box + 13, bscan);
+        k_invalidate<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, P, c.le.p, box + 12, lvl, c.ld.p, binval, cap,
+                                                     nxt, bnext, cap, c.stamp, sepoch, bscan);
         k_long_tasks<<<64, kBlock, 0, c.s>>>(c.g.view(), c.lf.p, box + 13, c.wb.p, box + 14, c.wb.cap);
         int32_t supepoch = next_epoch(c);
-        k_inval_search<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, lvl, c.lf.p, c.wb.p, box + 14, c.wb.cap, c.aux,
-                                                       supepoch, bscan);
+        k_inval_search<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, P, lvl, c.lf.p, c.wb.p, box + 14, c.wb.cap,
+                                                       c.aux, supepoch, bscan);
         k_inval_decide<<<64, kBlock, 0, c.s>>>(c.lf.p, box + 13, c.aux, supepoch, L, c.ld.p, binval, cap);
-        k_inval_push<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, lvl, c.lf.p, c.wb.p, box + 14, c.wb.cap, nxt,
+        k_inval_push<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, P, lvl, c.lf.p, c.wb.p, box + 14, c.wb.cap, nxt,
                                                      bnext, cap, c.stamp, sepoch, bscan);
         lvl++;
         std::swap(cur, nxt);
@@ -624,16 +743,17 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, scout_stats* st) {
     if (nx > 0) {
       // [11] scanned, [12] medium rows, [13] chunk tasks
       k_zero<<<1, 64, 0, c.s>>>(box + 11, 3);
-      k_tentative_short<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, c.ld.p, nx, c.le.p, c.lf.p, box + 12,
-                                                        c.lf.cap, c.wb.p, box + 13, c.wb.cap, box + 11);
+      uint64_t* tent = reinterpret_cast<uint64_t*>(c.wc.p);
+      k_tentative_short<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, c.ld.p, nx, tent, c.lf.p, box + 12, c.lf.cap,
+                                                        c.wb.p, box + 13, c.wb.cap, box + 11);
       k_tentative_rows<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, c.ld.p, c.lf.p, box + 12, c.wb.p, box + 13,
-                                                       c.wb.cap, c.le.p, box + 11);
-      k_apply_tentative<<<grid_for(nx), kBlock, 0, c.s>>>(c.ld.p, c.le.p, nx, L, c.wa.p, c.wa.cap, box);
+                                                       c.wb.cap, tent, box + 11);
+      k_apply_tentative<<<grid_for(nx), kBlock, 0, c.s>>>(c.ld.p, tent, nx, L, P, c.wa.p, c.lg.p, seed_cap, box);
     }
   }
   if (m > 0)
     k_insert_seeds<<<grid_for(m), kBlock, 0, c.s>>>(c.g.view(), c.g.sorted, ch.d_uv.p, ch.d_ins.p, m, L, c.wa.p,
-                                                    c.wa.cap, box);
+                                                    c.lg.p, seed_cap, box);
   SCOUT_CHECK(box_read(c, 14));
   if (nx > 0) scanned += c.h_box[11];
   if (nx > 0 && c.h_box[13] > c.wb.cap) {
@@ -642,7 +762,7 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, scout_stats* st) {
   }
   st->insertions += c.h_box[7];
   int64_t nseeds = c.h_box[4];
-  if (nseeds > c.wa.cap) {
+  if (nseeds > seed_cap) {
     set_error("bfs: seed list overflow");
     return SCOUT_ERR_STATE;
   }
@@ -650,9 +770,10 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, scout_stats* st) {
   st->touched += nx;
   if (nseeds > 0) {
     int32_t smin = (int32_t)c.h_box[5], smax = (int32_t)c.h_box[6];
+    k_seed_parents<<<grid_for(nseeds), kBlock, 0, c.s>>>(c.wa.p, c.lg.p, nseeds, L, P);
     int32_t epoch = next_epoch(c);
     k_zero<<<1, 64, 0, c.s>>>(box + 8, 1);
-    SCOUT_CHECK(relax_levels(c, L, smin, smax, c.wa.p, nseeds, epoch, &rounds, &scanned));
+    SCOUT_CHECK(relax_levels(c, L, P, smin, smax, c.wa.p, nseeds, epoch, &rounds, &scanned));
   }
   st->rounds += rounds;
   st->edges_scanned += scanned;
@@ -662,13 +783,13 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, scout_stats* st) {
 
 int bfs_full(Ctx& c, int32_t src, int64_t* rounds, int64_t* scanned) {
   SCOUT_CHECK(ensure_state(c, SCOUT_STATE_LEVEL));
-  SCOUT_CHECK(bfs_full_on(c, c.level, src, rounds, scanned));
+  SCOUT_CHECK(bfs_full_on(c, c.level, c.parent, src, rounds, scanned));
   c.level_seq = c.apply_seq;
   return SCOUT_OK;
 }
 
 int bfs_local(Ctx& c, Channel& ch, scout_stats* st) {
-  SCOUT_CHECK(bfs_local_on(c, ch, c.level, st));
+  SCOUT_CHECK(bfs_local_on(c, ch, c.level, c.parent, st));
   c.level_seq = c.apply_seq;
   return SCOUT_OK;
 }
@@ -711,7 +832,9 @@ int scout_repair_bfs(scout_ctx* x, int channel, int strategy, scout_stats* st) {
   SCOUT_CHECK(span.begin(c.s));
   st->lines += ch->m;
   int rc;
-  if (strategy == SCOUT_STRATEGY_RECOMPUTE) {
+  // a local repair needs levels that were exact for the graph before this batch
+  bool stale = c.level_seq != c.apply_seq - 1;
+  if (strategy == SCOUT_STRATEGY_RECOMPUTE || stale) {
     st->fallback = 1;
     rc = bfs_full(c, c.source, &st->rounds, &st->edges_scanned);
   } else {

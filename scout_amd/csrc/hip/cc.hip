@@ -216,11 +216,12 @@ int cc_local(Ctx& c, Channel& ch, scout_stats* st) {
     L = c.level;  // this batch's BFS repair from vertex 0 already ran on this context
   } else {
     SCOUT_CHECK(ensure_scratch(c, &c.cc_level, -1));
+    SCOUT_CHECK(ensure_scratch(c, &c.cc_parent, -1));
     scout_stats bst{};
     if (c.cc_level_seq >= 0 && c.cc_level_seq == c.apply_seq - 1) {
-      SCOUT_CHECK(bfs_local_on(c, ch, c.cc_level, &bst));
+      SCOUT_CHECK(bfs_local_on(c, ch, c.cc_level, c.cc_parent, &bst));
     } else {
-      SCOUT_CHECK(bfs_full_on(c, c.cc_level, 0, &bst.rounds, &bst.edges_scanned));
+      SCOUT_CHECK(bfs_full_on(c, c.cc_level, c.cc_parent, 0, &bst.rounds, &bst.edges_scanned));
     }
     c.cc_level_seq = c.apply_seq;
     st->rounds += bst.rounds;

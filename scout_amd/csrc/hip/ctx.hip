@@ -47,11 +47,11 @@ int alloc_n(Ctx& c, void** p, size_t elem) {
 }
 
 void free_state(Ctx& c) {
-  void* ps[] = {c.level, c.label, c.color, c.mis, c.stamp, c.aux, c.aux2,
-                c.cc_owner, c.col_marker, c.mis_minw, c.mis_tag, c.cc_level};
+  void* ps[] = {c.level, c.parent, c.label, c.color, c.mis, c.stamp, c.aux, c.aux2,
+                c.cc_owner, c.col_marker, c.mis_minw, c.mis_tag, c.cc_level, c.cc_parent};
   for (void* p : ps)
     if (p) (void)hipFree(p);
-  c.cc_level = nullptr;
+  c.cc_level = c.cc_parent = c.parent = nullptr;
   c.level_seq = c.cc_level_seq = -1;
   c.cc_components = -1;
   c.level = c.label = c.color = nullptr;
@@ -87,7 +87,9 @@ int ensure_state(Ctx& c, int kind) {
   switch (kind) {
     case SCOUT_STATE_MIS: return alloc_n(c, (void**)&c.mis, 1);
     case SCOUT_STATE_COLOR: return alloc_n(c, (void**)&c.color, 4);
-    case SCOUT_STATE_LEVEL: return alloc_n(c, (void**)&c.level, 4);
+    case SCOUT_STATE_LEVEL:
+      SCOUT_CHECK(alloc_n(c, (void**)&c.level, 4));
+      return alloc_n(c, (void**)&c.parent, 4);
     case SCOUT_STATE_LABEL: return alloc_n(c, (void**)&c.label, 4);
   }
   set_error("unknown state kind");
@@ -273,10 +275,11 @@ int scout_ctx_destroy(scout_ctx* x) {
     c.ch[k].d_done.release();
     if (c.ch[k].staged) (void)hipEventDestroy(c.ch[k].staged);
   }
-  DevBuf<int32_t>* ls[] = {&c.la, &c.lb, &c.lc, &c.ld, &c.le, &c.lf};
+  DevBuf<int32_t>* ls[] = {&c.la, &c.lb, &c.lc, &c.ld, &c.le, &c.lf, &c.lg};
   for (auto* l : ls) l->release();
   c.wa.release();
   c.wb.release();
+  c.wc.release();
   c.tmp.release();
   if (c.h_box) (void)hipHostFree(c.h_box);
   if (c.d_box) (void)hipFree(c.d_box);
@@ -391,8 +394,11 @@ int scout_state_upload(scout_ctx* x, int kind, const void* host, int64_t n) {
   size_t elem = kind == SCOUT_STATE_MIS ? 1 : 4;
   SCOUT_TRY(hipMemcpyAsync(dst, host, (size_t)n * elem, hipMemcpyHostToDevice, c.s));
   SCOUT_TRY(hipStreamSynchronize(c.s));
-  // uploaded levels are taken as exact for the current graph
-  if (kind == SCOUT_STATE_LEVEL) c.level_seq = c.apply_seq;
+  // uploaded levels are taken as exact for the current graph; their parents are derived
+  if (kind == SCOUT_STATE_LEVEL) {
+    SCOUT_CHECK(compute_parents(c, c.level, c.parent));
+    c.level_seq = c.apply_seq;
+  }
   if (kind == SCOUT_STATE_LABEL) c.cc_components = -1;
   return SCOUT_OK;
 }

@@ -307,11 +307,11 @@ int scout_gen_rmat(scout_ctx* x, int scale, int edge_factor, double a, double b,
   SCOUT_CHECK(gen_rmat(ctx, scale, edge_factor, a, b, c, seed));
   // fresh per-vertex scratch for the new vertex count
   int32_t n = ctx.g.n;
-  void* ps[] = {ctx.level, ctx.label, ctx.color, ctx.mis, ctx.stamp, ctx.aux, ctx.aux2,
-                ctx.cc_owner, ctx.col_marker, ctx.mis_minw, ctx.mis_tag, ctx.cc_level};
+  void* ps[] = {ctx.level, ctx.parent, ctx.label, ctx.color, ctx.mis, ctx.stamp, ctx.aux, ctx.aux2,
+                ctx.cc_owner, ctx.col_marker, ctx.mis_minw, ctx.mis_tag, ctx.cc_level, ctx.cc_parent};
   for (void* p : ps)
     if (p) (void)hipFree(p);
-  ctx.cc_level = nullptr;
+  ctx.cc_level = ctx.cc_parent = ctx.parent = nullptr;
   ctx.level_seq = ctx.cc_level_seq = ctx.cc_components = -1;
   ctx.level = ctx.label = ctx.color = nullptr;
   ctx.mis = nullptr;

@@ -146,6 +146,7 @@ struct Ctx {
   Channel ch[2];
   // maintained solutions
   int32_t* level = nullptr;    // BFS levels, -1 unreached
+  int32_t* parent = nullptr;   // BFS parent (a neighbour one level up), -1 for source/unreached
   int32_t* label = nullptr;    // CC min-id labels
   int32_t* color = nullptr;    // colours
   uint8_t* mis = nullptr;      // MIS membership
@@ -161,6 +162,7 @@ struct Ctx {
   int32_t* mis_minw = nullptr;   // smallest speculative writer line
   int32_t* mis_tag = nullptr;    // row kind stamps
   int32_t* cc_level = nullptr;   // BFS levels from vertex 0 kept for CC when BFS is not maintained
+  int32_t* cc_parent = nullptr;  // parents for cc_level
   int64_t apply_seq = 0;         // batches applied to the graph
   int64_t level_seq = -1;        // apply_seq at which `level` was last made exact
   int64_t cc_level_seq = -1;     // same for cc_level
@@ -169,8 +171,8 @@ struct Ctx {
   int64_t* h_box = nullptr;    // pinned
   int64_t* d_box = nullptr;
   // generic work lists
-  DevBuf<int32_t> la, lb, lc, ld, le, lf;
-  DevBuf<int64_t> wa, wb;
+  DevBuf<int32_t> la, lb, lc, ld, le, lf, lg;
+  DevBuf<int64_t> wa, wb, wc;
   DevBuf<uint8_t> tmp;         // library temp storage (scan/sort)
   std::vector<std::pair<hipEvent_t, hipEvent_t>> spans;  // timing
   MlpState* mlp = nullptr;
@@ -207,8 +209,11 @@ int graph_free(Ctx& c);
 
 // per-problem modules
 int bfs_full(Ctx& c, int32_t src, int64_t* rounds, int64_t* scanned);
-int bfs_full_on(Ctx& c, int32_t* L, int32_t src, int64_t* rounds, int64_t* scanned);
-int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, scout_stats* st);
+int bfs_full_on(Ctx& c, int32_t* L, int32_t* P, int32_t src, int64_t* rounds, int64_t* scanned);
+int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, int32_t* P, scout_stats* st);
+// Parents for levels that came from outside (init file, upload); fails if L is not a BFS level
+// array of the current graph.
+int compute_parents(Ctx& c, const int32_t* L, int32_t* P);
 int bfs_local(Ctx& c, Channel& ch, scout_stats* st);
 int cc_full(Ctx& c, int64_t* scanned);
 int cc_local(Ctx& c, Channel& ch, scout_stats* st);

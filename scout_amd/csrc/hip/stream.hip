@@ -233,7 +233,7 @@ void stream_release(Ctx& c) {
 struct Snapshot {
   DevBuf<int64_t> boff, aoff;
   DevBuf<int32_t> bcol, acol;
-  DevBuf<int32_t> level, label, color;
+  DevBuf<int32_t> level, parent, label, color;
   DevBuf<uint8_t> mis;
   int64_t E = 0, A = 0, holes = 0;
   bool sorted = false;
@@ -264,6 +264,8 @@ int snapshot_save(Ctx& c) {
     return SCOUT_OK;
   };
   SCOUT_CHECK(save32(s.level, c.level, s.has_level));
+  bool has_parent = false;
+  SCOUT_CHECK(save32(s.parent, c.parent, has_parent));
   SCOUT_CHECK(save32(s.label, c.label, s.has_label));
   SCOUT_CHECK(save32(s.color, c.color, s.has_color));
   s.has_mis = c.mis != nullptr;
@@ -299,6 +301,7 @@ int snapshot_restore(Ctx& c) {
   if (s.has_level) {
     SCOUT_CHECK(ensure_state(c, SCOUT_STATE_LEVEL));
     SCOUT_TRY(hipMemcpyAsync(c.level, s.level.p, (size_t)n * 4, hipMemcpyDeviceToDevice, c.s));
+    SCOUT_TRY(hipMemcpyAsync(c.parent, s.parent.p, (size_t)n * 4, hipMemcpyDeviceToDevice, c.s));
   }
   if (s.has_label) {
     SCOUT_CHECK(ensure_state(c, SCOUT_STATE_LABEL));
@@ -324,6 +327,7 @@ void snapshot_release(Ctx& c) {
   s.bcol.release();
   s.acol.release();
   s.level.release();
+  s.parent.release();
   s.label.release();
   s.color.release();
   s.mis.release();
