@@ -74,6 +74,7 @@ typedef struct {
   int64_t edges_scanned;/* adjacency entries read by the repair kernels */
   double repair_ms;     /* device time of the repair, from HIP events on the compute stream */
   double apply_ms;      /* device time of classification + graph update */
+  int64_t readbacks;    /* host read-backs of device counters (each waits for the stream) */
 } scout_stats;
 
 const char* scout_last_error(void);
@@ -172,9 +173,10 @@ int scout_gen_batch(scout_ctx* ctx, int channel, int64_t size, double delete_fra
 /* Copies a staged batch back to the host (cap_lines pairs at most). */
 int scout_batch_download(scout_ctx* ctx, int channel, int32_t* uv, int64_t cap_lines);
 /* Keeps a device copy of a channel's staged batch in slot k / stages slot k into a channel
- * with a device-to-device copy, so a replayed stream is resident in HBM. */
+ * with a device-to-device copy, so a replayed stream is resident in HBM. *m (may be NULL)
+ * receives the loaded batch's line count, the size scout_classify's ins must have. */
 int scout_batch_store(scout_ctx* ctx, int channel, int slot);
-int scout_batch_load(scout_ctx* ctx, int channel, int slot);
+int scout_batch_load(scout_ctx* ctx, int channel, int slot, int64_t* m);
 /* Device copies of the graph and solutions, to replay one stream under several strategies. */
 int scout_snapshot(scout_ctx* ctx);
 int scout_restore(scout_ctx* ctx);
@@ -195,6 +197,37 @@ enum {
 };
 int scout_prof_enable(scout_ctx* ctx, int on);
 int scout_prof_read(scout_ctx* ctx, int id, double* total_ms, int64_t* launches, int64_t* bytes);
+
+/* Learned scheduler. No counterpart in the reference: it adds a small MLP (12 features, two
+ * hidden layers of 64, 3 outputs; bf16 weights, fp32 accumulation) trained on the device on
+ * samples logged by the plain BFS repair. Per deleted parent edge it predicts whether the
+ * child ends unreached (a split of vertex 0's component), log1p of the cascade size and the
+ * levels it spans. SCOUT_STRATEGY_LEARNED uses the predictions only to size work (rounds
+ * queued before the first read-back, local repair versus full recompute); results equal the
+ * plain strategy's bit for bit. Parameter layout: W1[12][64], b1[64], W2[64][64], b2[64],
+ * W3[64][3], b3[3] (row-major, input index first). */
+enum { SCOUT_MLP_FEATURES = 12, SCOUT_MLP_OUTPUTS = 3, SCOUT_MLP_PARAMS = 5187 };
+/* He-uniform weights from the seed (counter-based, reproducible); not yet trained. */
+int scout_mlp_init(scout_ctx* ctx, uint64_t seed);
+/* Sets the fp32 master weights (SCOUT_MLP_PARAMS floats); the forward pass uses their
+ * round-to-nearest-even bf16 values. Marks the network trained. */
+int scout_mlp_set_params(scout_ctx* ctx, const float* params);
+/* Master weights (rounded = 0) or the bf16 weights as floats (rounded = 1). */
+int scout_mlp_get_params(scout_ctx* ctx, float* params, int rounded);
+/* Outputs for n feature rows: out[n][3] = {split logit, log1p size, levels}. */
+int scout_mlp_forward(scout_ctx* ctx, const float* feats, int64_t n, float* out);
+/* Gradient of the mean loss over n <= 4096 samples (targets[n][3] = {split 0/1, log1p size,
+ * levels}) with respect to the parameters, evaluated at the bf16 weights: BCE on the split
+ * logit plus half squared errors on the two regressions. */
+int scout_mlp_grad(scout_ctx* ctx, const float* feats, const float* targets, int64_t n, float* grad, float* loss);
+/* While on, every local BFS repair of the maintained levels logs features and outcomes of up
+ * to 16384 deletion candidates (at most 2^22 samples are kept). */
+int scout_mlp_log(scout_ctx* ctx, int on);
+/* Logged sample count, and up to cap rows of 15 floats (features, targets) if rows != NULL. */
+int scout_mlp_samples(scout_ctx* ctx, float* rows, int64_t cap, int64_t* count);
+/* Adam over the logged samples (batches of 4096, a seeded permutation); mean loss of the first
+ * and the last epoch. Marks the network trained. */
+int scout_mlp_train(scout_ctx* ctx, int epochs, float lr, uint64_t seed, float* first_loss, float* last_loss);
 
 #ifdef __cplusplus
 }

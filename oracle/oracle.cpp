@@ -20,6 +20,7 @@
 #include "oracle.h"
 
 #include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
@@ -907,3 +908,89 @@ void orc_refcc_finish(orc_refcc* r, int32_t* label) {
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------------------
+// MLP of the learned scheduler (double precision). Pinned by central finite differences of
+// its own loss and by an independent numpy forward pass (tests/test_oracle.py).
+// ---------------------------------------------------------------------------------------
+namespace {
+constexpr int kIn = 12, kH = 64, kOut = 3;
+constexpr int oW1 = 0, oB1 = oW1 + kIn * kH, oW2 = oB1 + kH, oB2 = oW2 + kH * kH, oW3 = oB2 + kH,
+              oB3 = oW3 + kH * kOut, kParams = oB3 + kOut;
+
+void mlp_fwd(const float* p, const float* x, double* z1, double* h1, double* z2, double* h2, double* out) {
+  for (int j = 0; j < kH; j++) {
+    double a = p[oB1 + j];
+    for (int i = 0; i < kIn; i++) a += (double)x[i] * p[oW1 + i * kH + j];
+    z1[j] = a;
+    h1[j] = a > 0 ? a : 0;
+  }
+  for (int j = 0; j < kH; j++) {
+    double a = p[oB2 + j];
+    for (int i = 0; i < kH; i++) a += h1[i] * p[oW2 + i * kH + j];
+    z2[j] = a;
+    h2[j] = a > 0 ? a : 0;
+  }
+  for (int k = 0; k < kOut; k++) {
+    double a = p[oB3 + k];
+    for (int i = 0; i < kH; i++) a += h2[i] * p[oW3 + i * kOut + k];
+    out[k] = a;
+  }
+}
+}  // namespace
+
+void orc_mlp_forward(const float* params, const float* feats, int64_t n, double* out) {
+  double z1[kH], h1[kH], z2[kH], h2[kH];
+  for (int64_t s = 0; s < n; s++) mlp_fwd(params, feats + s * kIn, z1, h1, z2, h2, out + s * kOut);
+}
+
+double orc_mlp_loss_grad(const float* params, const float* feats, const float* targets, int64_t n, double* grad) {
+  if (grad) std::fill(grad, grad + kParams, 0.0);
+  double z1[kH], h1[kH], z2[kH], h2[kH], out[kOut], d3[kOut], d2[kH], d1[kH];
+  double total = 0;
+  for (int64_t s = 0; s < n; s++) {
+    const float* x = feats + s * kIn;
+    const float* t = targets + s * kOut;
+    mlp_fwd(params, x, z1, h1, z2, h2, out);
+    double z = out[0];
+    total += std::max(z, 0.0) - z * t[0] + std::log1p(std::exp(-std::fabs(z)));
+    double e1 = out[1] - t[1], e2 = out[2] - t[2];
+    total += 0.5 * (e1 * e1 + e2 * e2);
+    if (!grad) continue;
+    d3[0] = (1.0 / (1.0 + std::exp(-z)) - t[0]) / (double)n;
+    d3[1] = e1 / (double)n;
+    d3[2] = e2 / (double)n;
+    for (int i = 0; i < kH; i++) {
+      double a = 0;
+      for (int k = 0; k < kOut; k++) a += (double)params[oW3 + i * kOut + k] * d3[k];
+      d2[i] = z2[i] > 0 ? a : 0;
+    }
+    for (int i = 0; i < kH; i++) {
+      double a = 0;
+      for (int j = 0; j < kH; j++) a += (double)params[oW2 + i * kH + j] * d2[j];
+      d1[i] = z1[i] > 0 ? a : 0;
+    }
+    for (int i = 0; i < kIn; i++)
+      for (int j = 0; j < kH; j++) grad[oW1 + i * kH + j] += (double)x[i] * d1[j];
+    for (int j = 0; j < kH; j++) grad[oB1 + j] += d1[j];
+    for (int i = 0; i < kH; i++)
+      for (int j = 0; j < kH; j++) grad[oW2 + i * kH + j] += h1[i] * d2[j];
+    for (int j = 0; j < kH; j++) grad[oB2 + j] += d2[j];
+    for (int i = 0; i < kH; i++)
+      for (int k = 0; k < kOut; k++) grad[oW3 + i * kOut + k] += h2[i] * d3[k];
+    for (int k = 0; k < kOut; k++) grad[oB3 + k] += d3[k];
+  }
+  return n > 0 ? total / (double)n : 0.0;
+}
+
+float orc_bf16_round(float x) {
+  uint32_t u;
+  std::memcpy(&u, &x, 4);
+  if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x7fffffu)) return x;  // NaN
+  uint32_t lsb = (u >> 16) & 1u;
+  u = (u + 0x7fffu + lsb) & 0xffff0000u;
+  float r;
+  std::memcpy(&r, &u, 4);
+  return r;
+}
+

@@ -79,6 +79,16 @@ void orc_refcc_free(orc_refcc* r);
 void orc_refcc_batch(orc_refcc* r, const int32_t* uv, const uint8_t* ins, int64_t m, int64_t* stats /* 8 */);
 void orc_refcc_finish(orc_refcc* r, int32_t* label);
 
+/* The learned scheduler's MLP (no reference counterpart): 12 -> 64 -> 64 -> 3, ReLU,
+ * parameters W1[12][64], b1[64], W2[64][64], b2[64], W3[64][3], b3[3]. Restated in double
+ * precision with the parameters taken as given (pass the bf16-rounded values the device
+ * uses). Loss per sample: BCE(out0, t0) + 0.5 (out1 - t1)^2 + 0.5 (out2 - t2)^2; the mean over
+ * n samples is returned and its gradient written to grad (5187 doubles) when grad != NULL. */
+double orc_mlp_loss_grad(const float* params, const float* feats, const float* targets, int64_t n, double* grad);
+void orc_mlp_forward(const float* params, const float* feats, int64_t n, double* out);
+/* fp32 -> bf16 (round to nearest, ties to even) -> fp32. */
+float orc_bf16_round(float x);
+
 #ifdef __cplusplus
 }
 #endif

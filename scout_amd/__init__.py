@@ -33,7 +33,7 @@ class Stats(ctypes.Structure):
     _fields_ = [(name, ctypes.c_int64) for name in (
         "lines", "insertions", "deletions", "noop", "seeds", "touched", "rounds", "fallback",
         "splits", "conflicts", "sub_batches", "edges_scanned")] + [
-        ("repair_ms", ctypes.c_double), ("apply_ms", ctypes.c_double)]
+        ("repair_ms", ctypes.c_double), ("apply_ms", ctypes.c_double), ("readbacks", ctypes.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -76,13 +76,24 @@ _SIGNATURES = {
                         ctypes.POINTER(_I64)],
     "scout_batch_download": [_P, ctypes.c_int, _P, _I64],
     "scout_batch_store": [_P, ctypes.c_int, ctypes.c_int],
-    "scout_batch_load": [_P, ctypes.c_int, ctypes.c_int],
+    "scout_batch_load": [_P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_I64)],
     "scout_snapshot": [_P],
     "scout_restore": [_P],
     "scout_prof_enable": [_P, ctypes.c_int],
     "scout_prof_read": [_P, ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_I64),
                         ctypes.POINTER(_I64)],
+    "scout_mlp_init": [_P, ctypes.c_uint64],
+    "scout_mlp_set_params": [_P, _P],
+    "scout_mlp_get_params": [_P, _P, ctypes.c_int],
+    "scout_mlp_forward": [_P, _P, _I64, _P],
+    "scout_mlp_grad": [_P, _P, _P, _I64, _P, ctypes.POINTER(ctypes.c_float)],
+    "scout_mlp_log": [_P, ctypes.c_int],
+    "scout_mlp_samples": [_P, _P, _I64, ctypes.POINTER(_I64)],
+    "scout_mlp_train": [_P, ctypes.c_int, ctypes.c_float, ctypes.c_uint64, ctypes.POINTER(ctypes.c_float),
+                        ctypes.POINTER(ctypes.c_float)],
 }
+
+MLP_FEATURES, MLP_OUTPUTS, MLP_PARAMS = 12, 3, 5187
 
 PROF_BFS_RELAX, PROF_BFS_INVALIDATE, PROF_CC_EXPAND, PROF_APPLY, PROF_CLASSIFY = 0, 1, 2, 3, 4
 PROF_COLOR, PROF_MIS_EVAL, PROF_CC_HOOK = 5, 6, 7
@@ -153,6 +164,7 @@ class Engine:
                     "ctx_create")
         self.h = h
         self.n = 0
+        self._staged = {0: 0, 1: 0}  # lines staged per channel (sizes classify's output)
 
     def _check(self, rc: int, what: str):
         if rc != OK:
@@ -223,12 +235,13 @@ class Engine:
     def stage(self, uv: np.ndarray, channel: int = 0):
         a = np.ascontiguousarray(uv, dtype=np.int32).reshape(-1)
         self._check(self.lib.scout_stage_batch(self.h, channel, _ptr(a), a.size // 2), "stage_batch")
-        self._staged = a.size // 2
+        self._staged[channel] = a.size // 2
 
     def classify(self, rule: int = CLASSIFY_CURRENT, channel: int = 0) -> np.ndarray:
-        out = np.zeros(max(self._staged, 1), dtype=np.uint8)
+        m = self._staged[channel]
+        out = np.zeros(max(m, 1), dtype=np.uint8)
         self._check(self.lib.scout_classify(self.h, channel, rule, _ptr(out)), "classify")
-        return out[: self._staged]
+        return out[:m]
 
     def apply(self, conditional: bool = False, channel: int = 0):
         self._check(self.lib.scout_apply(self.h, channel, 1 if conditional else 0), "apply")
@@ -273,7 +286,7 @@ class Engine:
         m = ctypes.c_int64()
         self._check(self.lib.scout_gen_batch(self.h, channel, size, delete_frac, tree_frac, seed, expected_total,
                                              ctypes.byref(m)), "gen_batch")
-        self._staged = m.value
+        self._staged[channel] = m.value
         return m.value
 
     def batch_download(self, m, channel=0) -> np.ndarray:
@@ -285,7 +298,10 @@ class Engine:
         self._check(self.lib.scout_batch_store(self.h, channel, slot), "batch_store")
 
     def batch_load(self, slot, channel=0):
-        self._check(self.lib.scout_batch_load(self.h, channel, slot), "batch_load")
+        m = ctypes.c_int64()
+        self._check(self.lib.scout_batch_load(self.h, channel, slot, ctypes.byref(m)), "batch_load")
+        self._staged[channel] = m.value
+        return m.value
 
     def snapshot(self):
         self._check(self.lib.scout_snapshot(self.h), "snapshot")
@@ -301,6 +317,53 @@ class Engine:
         self._check(self.lib.scout_prof_read(self.h, kid, ctypes.byref(ms), ctypes.byref(cnt), ctypes.byref(by)),
                     "prof_read")
         return {"ms": ms.value, "launches": cnt.value, "bytes": by.value}
+
+
+    # --- learned scheduler -------------------------------------------------------------
+    def mlp_init(self, seed: int = 1):
+        self._check(self.lib.scout_mlp_init(self.h, seed), "mlp_init")
+
+    def mlp_set_params(self, params):
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        assert p.size == MLP_PARAMS
+        self._check(self.lib.scout_mlp_set_params(self.h, _ptr(p)), "mlp_set_params")
+
+    def mlp_get_params(self, rounded: bool = False) -> np.ndarray:
+        p = np.zeros(MLP_PARAMS, dtype=np.float32)
+        self._check(self.lib.scout_mlp_get_params(self.h, _ptr(p), 1 if rounded else 0), "mlp_get_params")
+        return p
+
+    def mlp_forward(self, feats) -> np.ndarray:
+        f = np.ascontiguousarray(feats, dtype=np.float32).reshape(-1, MLP_FEATURES)
+        out = np.zeros((len(f), MLP_OUTPUTS), dtype=np.float32)
+        self._check(self.lib.scout_mlp_forward(self.h, _ptr(f), len(f), _ptr(out)), "mlp_forward")
+        return out
+
+    def mlp_grad(self, feats, targets):
+        f = np.ascontiguousarray(feats, dtype=np.float32).reshape(-1, MLP_FEATURES)
+        t = np.ascontiguousarray(targets, dtype=np.float32).reshape(-1, MLP_OUTPUTS)
+        g = np.zeros(MLP_PARAMS, dtype=np.float32)
+        loss = ctypes.c_float()
+        self._check(self.lib.scout_mlp_grad(self.h, _ptr(f), _ptr(t), len(f), _ptr(g), ctypes.byref(loss)),
+                    "mlp_grad")
+        return g, loss.value
+
+    def mlp_log(self, on: bool = True):
+        self._check(self.lib.scout_mlp_log(self.h, 1 if on else 0), "mlp_log")
+
+    def mlp_samples(self, download: bool = False):
+        cnt = ctypes.c_int64()
+        self._check(self.lib.scout_mlp_samples(self.h, None, 0, ctypes.byref(cnt)), "mlp_samples")
+        if not download:
+            return cnt.value
+        rows = np.zeros((max(cnt.value, 1), MLP_FEATURES + MLP_OUTPUTS), dtype=np.float32)
+        self._check(self.lib.scout_mlp_samples(self.h, _ptr(rows), cnt.value, ctypes.byref(cnt)), "mlp_samples")
+        return rows[: cnt.value]
+
+    def mlp_train(self, epochs: int = 4, lr: float = 1e-3, seed: int = 1):
+        a, b = ctypes.c_float(), ctypes.c_float()
+        self._check(self.lib.scout_mlp_train(self.h, epochs, lr, seed, ctypes.byref(a), ctypes.byref(b)), "mlp_train")
+        return a.value, b.value
 
 
 def mis_max_csr(avg_degree: float, free_bytes: float) -> int:

@@ -46,6 +46,9 @@ constexpr int32_t kInvalid = -2;
 // is empty costs a few launches and no work.
 constexpr int kFirstGroup = 2;
 constexpr int kMaxGroup = 16;
+// The learned scheduler switches a batch to a full recompute when the predicted cascades
+// cover more than this fraction of the vertices.
+constexpr double kRecomputeFraction = 0.25;
 
 // Gathers the row entries of a short row (len <= kShort) into registers; -1 past the end.
 __device__ __forceinline__ void gather_short(const GView& g, const Row& r, int64_t len, bool active,
@@ -60,6 +63,28 @@ __device__ __forceinline__ void push_chunks(int64_t f, int64_t len, int64_t* tas
   int64_t chunks = (len + kChunk - 1) / kChunk;
   int64_t pos = (int64_t)atomicAdd((unsigned long long*)task_count, (unsigned long long)chunks);
   for (int64_t k = 0; k < chunks && pos + k < task_cap; k++) tasks[pos + k] = (f << 32) | k;
+}
+
+// Cascade attribution while the learned scheduler logs training data: every invalid vertex
+// is charged to the deletion candidate its cascade started from (root), for the first
+// kMlpSample candidates. root == nullptr when not logging.
+struct Attrib {
+  int32_t* root;
+  int32_t* subtree;
+  int32_t* maxlvl;
+};
+
+__device__ __forceinline__ void attrib_invalid(const Attrib& a, int32_t c, int32_t lvl) {
+  if (!a.root) return;
+  int32_t rt = a.root[c];
+  if (rt >= 0 && rt < kMlpSample) {
+    atomicAdd(&a.subtree[rt], 1);
+    atomicMax(&a.maxlvl[rt], lvl);
+  }
+}
+
+__device__ __forceinline__ void attrib_child(const Attrib& a, int32_t c, int32_t w) {
+  if (a.root) a.root[w] = a.root[c];
 }
 
 // Lowers w to lvl+1 on behalf of f; on success f becomes w's parent. True if w must be queued
@@ -166,7 +191,7 @@ __global__ void k_relax_rows(GView g, int32_t* level, int32_t* parent, const int
 // Endpoint whose parent edge a deletion line removed.
 // box[0] = candidates, box[1] = min level, box[2] = max level, box[3] = deletion lines.
 __global__ void k_del_candidates(const int32_t* uv, const uint8_t* ins, int64_t m, const int32_t* level,
-                                 const int32_t* parent, int32_t* cand, int64_t cap, int64_t* box) {
+                                 const int32_t* parent, int32_t* cand, int64_t cap, int64_t* box, int32_t* root) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   bool del = i < m && !ins[i];
   int32_t c = -1, lc = 0;
@@ -176,7 +201,8 @@ __global__ void k_del_candidates(const int32_t* uv, const uint8_t* ins, int64_t 
     if (lu >= 0 && lv == lu + 1 && parent[v] == u) c = v, lc = lv;
     else if (lv >= 0 && lu == lv + 1 && parent[u] == v) c = u, lc = lu;
   }
-  block_append(c >= 0, c, cand, &box[0], cap);
+  int64_t pos = block_append(c >= 0, c, cand, &box[0], cap);
+  if (root && c >= 0) root[c] = pos < kMlpSample ? (int32_t)pos : -1;
   block_add(del ? 1 : 0, &box[3]);
   block_min_max(c >= 0 ? lc : INT64_MAX, c >= 0 ? lc : -1, &box[1], &box[2]);
 }
@@ -196,7 +222,7 @@ __global__ void k_invalidate_short(GView g, int32_t* level, int32_t* parent, con
                                    const int64_t* cur_count, int32_t lvl, int32_t* inval, int64_t* inval_count,
                                    int32_t* next, int64_t* next_count, int64_t cap, int32_t* stamp, int32_t epoch,
                                    int32_t* medium, int64_t* medium_count, int32_t* longs, int64_t* long_count,
-                                   int64_t* scanned) {
+                                   int64_t* scanned, Attrib at) {
   int64_t nc = *cur_count;
   int64_t work = 0;
   for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < nc; base += (int64_t)gridDim.x * blockDim.x) {
@@ -223,8 +249,12 @@ __global__ void k_invalidate_short(GView g, int32_t* level, int32_t* parent, con
     bool invalid = is_short && found < 0;
     if (is_short) {
       work += len;
-      if (invalid) level[c] = kInvalid;
-      else parent[c] = found;
+      if (invalid) {
+        level[c] = kInvalid;
+        attrib_invalid(at, c, lvl);
+      } else {
+        parent[c] = found;
+      }
     }
     wave_append(invalid, c, inval, inval_count, cap);
     wave_append(active && !is_short && len <= kChunk, c, medium, medium_count, cap);
@@ -236,6 +266,7 @@ __global__ void k_invalidate_short(GView g, int32_t* level, int32_t* parent, con
 #pragma unroll
     for (int k = 0; k < kShort; k++) {
       bool take = pw[k] == c && invalid && level[xs[k]] == lvl + 1 && atomicExch(&stamp[xs[k]], epoch) != epoch;
+      if (take) attrib_child(at, c, xs[k]);
       wave_append(take, xs[k], next, next_count, cap);
     }
   }
@@ -245,7 +276,8 @@ __global__ void k_invalidate_short(GView g, int32_t* level, int32_t* parent, con
 // Wave per medium candidate.
 __global__ void k_invalidate(GView g, int32_t* level, int32_t* parent, const int32_t* cur, const int64_t* cur_count,
                              int32_t lvl, int32_t* inval, int64_t* inval_count, int64_t inval_cap, int32_t* next,
-                             int64_t* next_count, int64_t cap, int32_t* stamp, int32_t epoch, int64_t* scanned) {
+                             int64_t* next_count, int64_t cap, int32_t* stamp, int32_t epoch, int64_t* scanned,
+                             Attrib ab) {
   int64_t nc = *cur_count;
   int64_t work = 0;
   for (int64_t i = global_wave(); i < nc; i += total_waves()) {
@@ -261,6 +293,7 @@ __global__ void k_invalidate(GView g, int32_t* level, int32_t* parent, const int
     work += len;
     if (lane_id() == 0) {
       level[c] = kInvalid;
+      attrib_invalid(ab, c, lvl);
       thread_append(c, inval, inval_count, inval_cap);
     }
     for (int64_t base = 0; base < len; base += kWave) {
@@ -270,6 +303,7 @@ __global__ void k_invalidate(GView g, int32_t* level, int32_t* parent, const int
       if (k < len) {
         w = row_at(g, r, k);
         take = w >= 0 && parent[w] == c && level[w] == lvl + 1 && atomicExch(&stamp[w], epoch) != epoch;
+        if (take) attrib_child(ab, c, w);
       }
       wave_append(take, w, next, next_count, cap);
     }
@@ -310,12 +344,14 @@ __global__ void k_inval_search(GView g, const int32_t* level, int32_t* parent, i
 
 // Thread per long candidate: unsupported ones become invalid.
 __global__ void k_inval_decide(const int32_t* longs, const int64_t* long_count, const int32_t* supported,
-                               int32_t sepoch, int32_t* level, int32_t* inval, int64_t* inval_count, int64_t cap) {
+                               int32_t sepoch, int32_t* level, int32_t* inval, int64_t* inval_count, int64_t cap,
+                               int32_t lvl, Attrib ab) {
   int64_t nl = *long_count;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nl; i += (int64_t)gridDim.x * blockDim.x) {
     int32_t c = longs[i];
     if (supported[c] != sepoch) {
       level[c] = kInvalid;
+      attrib_invalid(ab, c, lvl);
       thread_append(c, inval, inval_count, cap);
     }
   }
@@ -324,7 +360,8 @@ __global__ void k_inval_decide(const int32_t* longs, const int64_t* long_count, 
 // Wave per chunk of an invalid long candidate: queue its children.
 __global__ void k_inval_push(GView g, const int32_t* level, const int32_t* parent, int32_t lvl, const int32_t* longs,
                              const int64_t* tasks, const int64_t* task_count, int64_t task_cap, int32_t* next,
-                             int64_t* next_count, int64_t cap, int32_t* stamp, int32_t epoch, int64_t* scanned) {
+                             int64_t* next_count, int64_t cap, int32_t* stamp, int32_t epoch, int64_t* scanned,
+                             Attrib ab) {
   int64_t nt = min(*task_count, task_cap);
   int64_t work = 0;
   for (int64_t t = global_wave(); t < nt; t += total_waves()) {
@@ -341,6 +378,7 @@ __global__ void k_inval_push(GView g, const int32_t* level, const int32_t* paren
       if (k < hi) {
         w = row_at(g, r, k);
         take = w >= 0 && parent[w] == c && level[w] == lvl + 1 && atomicExch(&stamp[w], epoch) != epoch;
+        if (take) attrib_child(ab, c, w);
       }
       wave_append(take, w, next, next_count, cap);
     }
@@ -606,6 +644,7 @@ int relax_levels(Ctx& c, int32_t* L, int32_t* P, int32_t lvl0, int32_t max_seed,
     SCOUT_CHECK(prof_end(c, PROF_BFS_RELAX, 0));
     SCOUT_TRY(hipMemcpyAsync(c.h_box + 8, box + 8, 10 * sizeof(int64_t), hipMemcpyDeviceToHost, c.s));
     SCOUT_TRY(hipStreamSynchronize(c.s));
+    c.readbacks++;
     if (std::max(c.h_box[10], c.h_box[16]) > task_cap) {
       set_error("bfs relax: chunk task list overflow");
       return SCOUT_ERR_STATE;
@@ -667,7 +706,7 @@ int bfs_full_on(Ctx& c, int32_t* L, int32_t* P, int32_t src, int64_t* rounds, in
   return SCOUT_OK;
 }
 
-int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, int32_t* P, scout_stats* st) {
+int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, int32_t* P, int strategy, scout_stats* st) {
   SCOUT_CHECK(ensure_bfs(c, ch.m));
   int64_t m = ch.m;
   int32_t n = c.g.n;
@@ -676,13 +715,35 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, int32_t* P, scout_stats* st) {
   int64_t* box = c.d_box;
   // box: [0] cand, [1] min, [2] max, [3] deletions, [4] seeds, [5] seed min, [6] seed max, [7] insertions
   k_init_box<<<1, 64, 0, c.s>>>(box);
+  // the learned scheduler logs from, and predicts for, the maintained levels only
+  bool logging = L == c.level && mlp_logging(c);
+  bool learned = L == c.level && strategy == SCOUT_STRATEGY_LEARNED && mlp_ready(c);
+  Attrib ab{nullptr, nullptr, nullptr};
+  if (logging) ab = Attrib{mlp_root(c), mlp_subtree(c), mlp_maxlvl(c)};
   if (m > 0)
-    k_del_candidates<<<grid_for(m), kBlock, 0, c.s>>>(ch.d_uv.p, ch.d_ins.p, m, L, P, c.la.p, c.la.cap, box);
-  SCOUT_CHECK(box_read(c, 8));
+    k_del_candidates<<<grid_for(m), kBlock, 0, c.s>>>(ch.d_uv.p, ch.d_ins.p, m, L, P, c.la.p, c.la.cap, box, ab.root);
+  // features and predictions are computed from the device-side counts and come back with the
+  // same read-back as the candidate counts ([24] predicted cascade sizes, [25] last level)
+  if (logging || learned) SCOUT_CHECK(mlp_begin_batch(c, c.la.p, box, L, P));
+  if (learned) SCOUT_CHECK(mlp_predict_async(c, box, box + 24));
+  SCOUT_CHECK(box_read(c, learned ? 26 : 8));
   int64_t ncand = c.h_box[0];
   int32_t cmin = (int32_t)c.h_box[1], cmax = (int32_t)c.h_box[2];
   st->deletions += c.h_box[3];
   int64_t nx = 0, scanned = 0, rounds = 0;
+  int64_t nsample = std::min<int64_t>(ncand, kMlpSample);
+  int first_group = kFirstGroup;
+  if (learned && ncand > 0) {
+    // predictions size and order the work; the plain search below decides every level
+    double predicted = (double)c.h_box[24] * (double)ncand / (double)nsample;
+    if (predicted > kRecomputeFraction * (double)n) {
+      // cascades over a large part of the graph: one full BFS costs less than local repair
+      st->fallback = 1;
+      return bfs_full_on(c, L, P, c.source, &st->rounds, &st->edges_scanned);
+    }
+    int64_t last = c.h_box[25];
+    if (last >= cmin) first_group = (int)std::min<int64_t>(std::max<int64_t>(last - cmin + 2, kFirstGroup), kMaxGroup);
+  }
   if (ncand > 0) {
     // Invalidation, level by level, queued in groups of rounds. Counts: [8]/[9] candidates at
     // this level / the next (alternating), [10] invalid vertices, [11] scanned entries,
@@ -697,7 +758,7 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, int32_t* P, scout_stats* st) {
     k_zero<<<1, 64, 0, c.s>>>(box + 8, 7);
     k_zero<<<1, 64, 0, c.s>>>(box + 18, 3);
     int p = 0;
-    int group = kFirstGroup;
+    int group = first_group;
     int32_t lvl = cmin;
     while (true) {
       SCOUT_CHECK(prof_begin(c, PROF_BFS_INVALIDATE));
@@ -710,16 +771,16 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, int32_t* P, scout_stats* st) {
         k_round_begin<<<1, 1, 0, c.s>>>(bcur, bnext, box + 14, box + 12, 3, box + 18, box + 20, box + 19);
         k_invalidate_short<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, P, cur, bcur, lvl, c.ld.p, binval, nxt,
                                                            bnext, cap, c.stamp, sepoch, c.le.p, box + 12, c.lf.p,
-                                                           box + 13, bscan);
+                                                           box + 13, bscan, ab);
         k_invalidate<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, P, c.le.p, box + 12, lvl, c.ld.p, binval, cap,
-                                                     nxt, bnext, cap, c.stamp, sepoch, bscan);
+                                                     nxt, bnext, cap, c.stamp, sepoch, bscan, ab);
         k_long_tasks<<<64, kBlock, 0, c.s>>>(c.g.view(), c.lf.p, box + 13, c.wb.p, box + 14, c.wb.cap);
         int32_t supepoch = next_epoch(c);
         k_inval_search<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, P, lvl, c.lf.p, c.wb.p, box + 14, c.wb.cap,
                                                        c.aux, supepoch, bscan);
-        k_inval_decide<<<64, kBlock, 0, c.s>>>(c.lf.p, box + 13, c.aux, supepoch, L, c.ld.p, binval, cap);
+        k_inval_decide<<<64, kBlock, 0, c.s>>>(c.lf.p, box + 13, c.aux, supepoch, L, c.ld.p, binval, cap, lvl, ab);
         k_inval_push<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, P, lvl, c.lf.p, c.wb.p, box + 14, c.wb.cap, nxt,
-                                                     bnext, cap, c.stamp, sepoch, bscan);
+                                                     bnext, cap, c.stamp, sepoch, bscan, ab);
         lvl++;
         std::swap(cur, nxt);
         p ^= 1;
@@ -727,6 +788,7 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, int32_t* P, scout_stats* st) {
       SCOUT_CHECK(prof_end(c, PROF_BFS_INVALIDATE, 0));
       SCOUT_TRY(hipMemcpyAsync(c.h_box + 8, box + 8, 13 * sizeof(int64_t), hipMemcpyDeviceToHost, c.s));
       SCOUT_TRY(hipStreamSynchronize(c.s));
+      c.readbacks++;
       if (std::max(c.h_box[14], c.h_box[19]) > c.wb.cap) {
         set_error("bfs invalidation: chunk task list overflow");
         return SCOUT_ERR_STATE;
@@ -778,6 +840,7 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, int32_t* P, scout_stats* st) {
   st->rounds += rounds;
   st->edges_scanned += scanned;
   st->noop += (ncand == 0 && nseeds == 0) ? m : 0;
+  if (logging) SCOUT_CHECK(mlp_end_batch(c, c.la.p, nsample, L));
   return SCOUT_OK;
 }
 
@@ -788,8 +851,8 @@ int bfs_full(Ctx& c, int32_t src, int64_t* rounds, int64_t* scanned) {
   return SCOUT_OK;
 }
 
-int bfs_local(Ctx& c, Channel& ch, scout_stats* st) {
-  SCOUT_CHECK(bfs_local_on(c, ch, c.level, c.parent, st));
+int bfs_local(Ctx& c, Channel& ch, int strategy, scout_stats* st) {
+  SCOUT_CHECK(bfs_local_on(c, ch, c.level, c.parent, strategy, st));
   c.level_seq = c.apply_seq;
   return SCOUT_OK;
 }
@@ -831,6 +894,7 @@ int scout_repair_bfs(scout_ctx* x, int channel, int strategy, scout_stats* st) {
   Span span;
   SCOUT_CHECK(span.begin(c.s));
   st->lines += ch->m;
+  int64_t rb0 = c.readbacks;
   int rc;
   // a local repair needs levels that were exact for the graph before this batch
   bool stale = c.level_seq != c.apply_seq - 1;
@@ -838,11 +902,12 @@ int scout_repair_bfs(scout_ctx* x, int channel, int strategy, scout_stats* st) {
     st->fallback = 1;
     rc = bfs_full(c, c.source, &st->rounds, &st->edges_scanned);
   } else {
-    rc = bfs_local(c, *ch, st);
+    rc = bfs_local(c, *ch, strategy, st);
   }
   SCOUT_CHECK(rc);
   SCOUT_CHECK(span.end(c.s));
   st->repair_ms += span.ms();
+  st->readbacks += c.readbacks - rb0;
   return SCOUT_OK;
 }
 

@@ -219,7 +219,7 @@ int cc_local(Ctx& c, Channel& ch, scout_stats* st) {
     SCOUT_CHECK(ensure_scratch(c, &c.cc_parent, -1));
     scout_stats bst{};
     if (c.cc_level_seq >= 0 && c.cc_level_seq == c.apply_seq - 1) {
-      SCOUT_CHECK(bfs_local_on(c, ch, c.cc_level, c.cc_parent, &bst));
+      SCOUT_CHECK(bfs_local_on(c, ch, c.cc_level, c.cc_parent, SCOUT_STRATEGY_LOCAL, &bst));
     } else {
       SCOUT_CHECK(bfs_full_on(c, c.cc_level, c.cc_parent, 0, &bst.rounds, &bst.edges_scanned));
     }
@@ -295,6 +295,7 @@ int scout_repair_cc(scout_ctx* x, int channel, int strategy, scout_stats* st) {
   Span span;
   SCOUT_CHECK(span.begin(c.s));
   st->lines += ch->m;
+  int64_t rb0 = c.readbacks;
   if (strategy == SCOUT_STRATEGY_RECOMPUTE) {
     st->fallback = 1;
     SCOUT_CHECK(cc_full(c, &st->edges_scanned));
@@ -303,6 +304,7 @@ int scout_repair_cc(scout_ctx* x, int channel, int strategy, scout_stats* st) {
   }
   SCOUT_CHECK(span.end(c.s));
   st->repair_ms += span.ms();
+  st->readbacks += c.readbacks - rb0;
   return SCOUT_OK;
 }
 

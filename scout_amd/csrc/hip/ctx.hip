@@ -18,6 +18,7 @@ void set_error(const std::string& msg) { g_err = msg; }
 int box_read(Ctx& c, int count) {
   SCOUT_TRY(hipMemcpyAsync(c.h_box, c.d_box, (size_t)count * sizeof(int64_t), hipMemcpyDeviceToHost, c.s));
   SCOUT_TRY(hipStreamSynchronize(c.s));
+  c.readbacks++;
   return SCOUT_OK;
 }
 
@@ -264,6 +265,7 @@ int scout_ctx_destroy(scout_ctx* x) {
   graph_free(c);
   free_state(c);
   cc_release(c);
+  mlp_release(c);
   mis_release(c);
   stream_release(c);
   snapshot_release(c);

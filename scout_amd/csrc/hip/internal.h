@@ -186,6 +186,7 @@ struct Ctx {
   int64_t pinned_used = 0;
   // last batch's kernel timings
   float last_repair_ms = 0.f;
+  int64_t readbacks = 0;         // host read-backs of device counters so far
 };
 
 // prims.hip (library scans/sorts)
@@ -210,17 +211,39 @@ int graph_free(Ctx& c);
 // per-problem modules
 int bfs_full(Ctx& c, int32_t src, int64_t* rounds, int64_t* scanned);
 int bfs_full_on(Ctx& c, int32_t* L, int32_t* P, int32_t src, int64_t* rounds, int64_t* scanned);
-int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, int32_t* P, scout_stats* st);
+int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, int32_t* P, int strategy, scout_stats* st);
 // Parents for levels that came from outside (init file, upload); fails if L is not a BFS level
 // array of the current graph.
 int compute_parents(Ctx& c, const int32_t* L, int32_t* P);
-int bfs_local(Ctx& c, Channel& ch, scout_stats* st);
+int bfs_local(Ctx& c, Channel& ch, int strategy, scout_stats* st);
 int cc_full(Ctx& c, int64_t* scanned);
 int cc_local(Ctx& c, Channel& ch, scout_stats* st);
 void cc_release(Ctx& c);
 void mis_release(Ctx& c);
 void stream_release(Ctx& c);
 void snapshot_release(Ctx& c);
+
+// mlp.hip: the learned scheduler (12 features -> 64 -> 64 -> 3 outputs, bf16 weights).
+constexpr int kMlpIn = 12, kMlpHidden = 64, kMlpOut = 3;
+constexpr int kMlpParams = kMlpIn * kMlpHidden + kMlpHidden + kMlpHidden * kMlpHidden + kMlpHidden +
+                           kMlpHidden * kMlpOut + kMlpOut;
+constexpr int64_t kMlpSample = 16384;  // candidates per batch whose features are computed
+bool mlp_logging(Ctx& c);
+bool mlp_ready(Ctx& c);
+// Features of the first min(ncand, kMlpSample) candidates (state before the repair); the
+// counts are read on the device from the candidate pass's mailbox d_box ([0] candidates,
+// [1] smallest level, [3] deletion lines).
+int mlp_begin_batch(Ctx& c, const int32_t* cand, const int64_t* d_box, const int32_t* L, const int32_t* P);
+// Root attribution arrays while logging (nullptr otherwise).
+int32_t* mlp_root(Ctx& c);
+int32_t* mlp_subtree(Ctx& c);
+int32_t* mlp_maxlvl(Ctx& c);
+// Predictions over the sample, aggregated on the device: d_out[0] = predicted cascade sizes
+// summed over the sample, d_out[1] = the largest predicted last level.
+int mlp_predict_async(Ctx& c, const int64_t* d_ncand, int64_t* d_out);
+// Appends the sample's features and outcomes (state after the repair) to the training store.
+int mlp_end_batch(Ctx& c, const int32_t* cand, int64_t ns, const int32_t* L);
+void mlp_release(Ctx& c);
 
 // Kernel timing for the roofline: events recorded on the compute stream around launches of a
 // named kernel family while profiling is on. `bytes` is the algorithmic byte count the caller

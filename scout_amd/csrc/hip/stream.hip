@@ -376,12 +376,14 @@ int scout_batch_store(scout_ctx* x, int channel, int slot) {
   return batch_store(x->c, *ch, slot);
 }
 
-int scout_batch_load(scout_ctx* x, int channel, int slot) {
+int scout_batch_load(scout_ctx* x, int channel, int slot, int64_t* m) {
   if (!x) return SCOUT_ERR_ARG;
   SCOUT_TRY(hipSetDevice(x->c.dev));
   Channel* ch = channel_of(x->c, channel);
   if (!ch) return SCOUT_ERR_ARG;
-  return batch_load(x->c, *ch, slot);
+  SCOUT_CHECK(batch_load(x->c, *ch, slot));
+  if (m) *m = ch->m;
+  return SCOUT_OK;
 }
 
 int scout_snapshot(scout_ctx* x) {

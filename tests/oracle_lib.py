@@ -46,6 +46,9 @@ _SIG = {
     "orc_refcc_free": ([_P], None),
     "orc_refcc_batch": ([_P, _P, _P, _I64, _P], None),
     "orc_refcc_finish": ([_P, _P], None),
+    "orc_mlp_loss_grad": ([_P, _P, _P, _I64, _P], ctypes.c_double),
+    "orc_mlp_forward": ([_P, _P, _I64, _P], None),
+    "orc_bf16_round": ([ctypes.c_float], ctypes.c_float),
 }
 
 _lib = None
@@ -196,3 +199,26 @@ def read_batch(path, n):
             if 0 <= u < n and 0 <= v < n and u != v:
                 out.append((u, v))
     return np.array(out, dtype=np.int32).reshape(-1, 2)
+
+
+def mlp_loss_grad(params, feats, targets, with_grad=True):
+    """Mean loss (double) and its gradient from the oracle's double-precision MLP."""
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    f = np.ascontiguousarray(feats, dtype=np.float32)
+    t = np.ascontiguousarray(targets, dtype=np.float32)
+    g = np.zeros(p.size, dtype=np.float64) if with_grad else None
+    loss = lib().orc_mlp_loss_grad(_p(p), _p(f), _p(t), len(f), _p(g) if with_grad else None)
+    return loss, g
+
+
+def mlp_forward(params, feats):
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    f = np.ascontiguousarray(feats, dtype=np.float32)
+    out = np.zeros((len(f), 3), dtype=np.float64)
+    lib().orc_mlp_forward(_p(p), _p(f), len(f), _p(out))
+    return out
+
+
+def bf16_round(x):
+    return np.array([lib().orc_bf16_round(float(v)) for v in np.ravel(x)], dtype=np.float32).reshape(np.shape(x))
+

@@ -1,0 +1,166 @@
+"""GPU tests of the learned scheduler: the HIP forward and backward kernels against the CPU
+oracle's double-precision MLP, training on samples logged by the plain repair, and learned
+repair == plain repair bit for bit, with what the network changes in the work measured.
+
+Tolerances (fp32 device arithmetic against the float64 oracle, same bf16 weights): forward
+outputs within 1e-4 absolute + 1e-4 relative; gradients within 2e-3 of the largest gradient
+entry; losses within 1e-4 relative.
+"""
+import numpy as np
+import pytest
+
+import scout_amd as sa
+from oracle_lib import OracleGraph, bf16_round, mlp_forward, mlp_loss_grad
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+def _case(seed, n):
+    rng = np.random.default_rng(seed)
+    feats = rng.standard_normal((n, 12)).astype(np.float32)
+    targets = np.stack([rng.integers(0, 2, n), rng.random(n) * 3, rng.random(n) * 4], 1).astype(np.float32)
+    return feats, targets
+
+
+def test_bf16_weights_are_round_to_nearest_even():
+    say("mlp: bf16 rounding of the master weights")
+    with sa.Engine(0) as eng:
+        eng.mlp_init(11)
+        master = eng.mlp_get_params(rounded=False)
+        w16 = eng.mlp_get_params(rounded=True)
+    assert np.array_equal(w16, bf16_round(master))
+    assert not np.array_equal(w16, master)
+
+
+def test_forward_matches_oracle():
+    say("mlp: forward vs oracle")
+    feats, _ = _case(1, 777)
+    with sa.Engine(0) as eng:
+        eng.mlp_init(12)
+        w = eng.mlp_get_params(rounded=True)
+        out = eng.mlp_forward(feats)
+    ref = mlp_forward(w, feats)
+    assert np.allclose(out, ref, rtol=1e-4, atol=1e-4), np.abs(out - ref).max()
+
+
+@pytest.mark.parametrize("n", [1, 100, 4096])
+def test_gradient_matches_oracle(n):
+    say(f"mlp: gradient vs oracle, n={n}")
+    feats, targets = _case(2 + n, n)
+    with sa.Engine(0) as eng:
+        eng.mlp_init(13)
+        w = eng.mlp_get_params(rounded=True)
+        g, loss = eng.mlp_grad(feats, targets)
+    ref_loss, ref_g = mlp_loss_grad(w, feats, targets)
+    assert abs(loss - ref_loss) <= 1e-4 * abs(ref_loss)
+    scale = np.abs(ref_g).max()
+    assert np.abs(g - ref_g).max() <= 2e-3 * scale, (np.abs(g - ref_g).max(), scale)
+
+
+def _stream_engine(scale, seed):
+    eng = sa.Engine(0)
+    eng.gen_rmat(scale, 16, 0.57, 0.19, 0.19, seed)
+    eng.bfs_full(0)
+    eng.cc_full()
+    return eng
+
+
+def test_training_on_logged_repairs_and_learned_equals_plain():
+    """Logs 20 plain batches, trains on the device, then replays 6 more batches with the plain
+    and the learned strategy: levels and labels identical after every batch; the learned
+    repair queues enough rounds up front to need no more host read-backs than the plain one."""
+    scale, bsize, ntrain, ntest = 15, 20000, 20, 6
+    say(f"mlp: logged training, R-MAT scale {scale}, {ntrain} + {ntest} batches of {bsize}")
+    eng = _stream_engine(scale, 5)
+    try:
+        eng.mlp_log(True)
+        total = ntrain + ntest
+        eng.snapshot()
+        for b in range(total):
+            eng.gen_batch(bsize, 0.7, 0.5, 9, expected_total=bsize * total)
+            eng.batch_store(b)
+            eng.classify()
+            eng.apply()
+            eng.repair_bfs(sa.STRATEGY_LOCAL)
+            eng.repair_cc(sa.STRATEGY_LOCAL)
+            if b == ntrain - 1:
+                eng.mlp_log(False)
+        nsamp = eng.mlp_samples()
+        assert nsamp >= 4096
+        rows = eng.mlp_samples(download=True)
+        assert np.isfinite(rows).all()
+        assert set(np.unique(rows[:, 12])) <= {0.0, 1.0}
+        eng.mlp_init(3)
+        first, last = eng.mlp_train(epochs=8, lr=2e-3, seed=1)
+        say(f"mlp: {nsamp} samples, loss {first:.4f} -> {last:.4f}")
+        assert last < first
+
+        def replay(strategy):
+            eng.restore()
+            for b in range(ntrain):  # bring the graph to the state after the training batches
+                eng.batch_load(b)
+                eng.classify()
+                eng.apply()
+                eng.repair_bfs(sa.STRATEGY_LOCAL)
+                eng.repair_cc(sa.STRATEGY_LOCAL)
+            states, readbacks = [], 0
+            for b in range(ntrain, total):
+                eng.batch_load(b)
+                eng.classify()
+                eng.apply()
+                st = eng.repair_bfs(strategy)
+                eng.repair_cc(strategy)
+                readbacks += st.readbacks
+                states.append((eng.download(sa.STATE_LEVEL), eng.download(sa.STATE_LABEL)))
+            return states, readbacks
+
+        plain, rb_plain = replay(sa.STRATEGY_LOCAL)
+        learned, rb_learned = replay(sa.STRATEGY_LEARNED)
+        say(f"mlp: BFS read-backs plain {rb_plain}, learned {rb_learned}")
+        for (lp, cp), (ll, cl) in zip(plain, learned):
+            assert np.array_equal(lp, ll)
+            assert np.array_equal(cp, cl)
+        assert rb_learned <= rb_plain
+        # and both equal a full recompute
+        eng.bfs_full(0)
+        eng.cc_full()
+        assert np.array_equal(eng.download(sa.STATE_LEVEL), learned[-1][0])
+        assert np.array_equal(eng.download(sa.STATE_LABEL), learned[-1][1])
+    finally:
+        eng.close()
+
+
+def test_learned_recompute_switch_keeps_results_exact(tmp_path):
+    """A network whose size head predicts huge cascades switches every batch to a full
+    recompute; the answers stay those of the oracle."""
+    from conftest import run_gen
+    egr = tmp_path / "g.egr"
+    run_gen("rmat", 11, 16, 4, egr)
+    run_gen("stream", "tree", egr, tmp_path / "b", 3, 1500, 5, 0.7, 0.5)
+    g = sa.Graph.read_egr(str(egr))
+    og = OracleGraph(str(egr))
+    with sa.Engine(0) as eng:
+        eng.load_graph(g)
+        eng.upload(sa.STATE_LEVEL, og.bfs(0))
+        eng.upload(sa.STATE_LABEL, og.cc())
+        p = np.zeros(sa.MLP_PARAMS, dtype=np.float32)
+        p[5184 + 1] = 20.0  # b3[1]: log1p(size) = 20 for every candidate
+        eng.mlp_set_params(p)
+        from oracle_lib import read_batch
+        import os
+        for k in range(1, 4):
+            uv = read_batch(os.path.join(tmp_path / "b", f"{k}.mtx"), g.n)
+            eng.stage(uv)
+            ins = eng.classify()
+            eng.apply()
+            og.apply(uv, og.classify(uv))
+            st = eng.repair_bfs(sa.STRATEGY_LEARNED)
+            eng.repair_cc(sa.STRATEGY_LEARNED)
+            assert st.fallback == 1
+            assert np.array_equal(eng.download(sa.STATE_LEVEL), og.bfs(0))
+            assert np.array_equal(eng.download(sa.STATE_LABEL), og.cc())
+            assert ins is not None

@@ -372,3 +372,108 @@ def test_per_edit_reference_bfs_insertions_are_exact(sample_dir, seed):
             assert np.array_equal(lv, og.bfs(0))
     finally:
         L.orc_refbfs_free(rb)
+
+
+# --- learned scheduler's MLP (oracle side) ---------------------------------------------
+
+def _mlp_case(seed, n=40):
+    rng = np.random.default_rng(seed)
+    params = (rng.standard_normal(5187) * 0.2).astype(np.float32)
+    feats = rng.standard_normal((n, 12)).astype(np.float32)
+    targets = np.stack([rng.integers(0, 2, n), rng.random(n) * 3, rng.random(n) * 4], 1).astype(np.float32)
+    return params, feats, targets
+
+
+def _numpy_forward(p, x):
+    """Independent forward pass (float64 numpy) from the documented parameter layout."""
+    p = p.astype(np.float64)
+    w1 = p[0:768].reshape(12, 64); b1 = p[768:832]
+    w2 = p[832:4928].reshape(64, 64); b2 = p[4928:4992]
+    w3 = p[4992:5184].reshape(64, 3); b3 = p[5184:5187]
+    h1 = np.maximum(x.astype(np.float64) @ w1 + b1, 0)
+    h2 = np.maximum(h1 @ w2 + b2, 0)
+    return h2 @ w3 + b3
+
+
+def test_mlp_oracle_forward_matches_numpy():
+    from oracle_lib import mlp_forward
+    p, x, _ = _mlp_case(3)
+    assert np.allclose(mlp_forward(p, x), _numpy_forward(p, x), rtol=1e-12, atol=1e-12)
+
+
+def test_mlp_oracle_loss_matches_numpy_closed_form():
+    from oracle_lib import mlp_loss_grad
+    p, x, t = _mlp_case(4)
+    out = _numpy_forward(p, x)
+    z = out[:, 0]
+    bce = np.maximum(z, 0) - z * t[:, 0] + np.log1p(np.exp(-np.abs(z)))
+    expect = np.mean(bce + 0.5 * ((out[:, 1] - t[:, 1]) ** 2 + (out[:, 2] - t[:, 2]) ** 2))
+    loss, _ = mlp_loss_grad(p, x, t, with_grad=False)
+    assert abs(loss - expect) < 1e-10
+
+
+@pytest.mark.parametrize("seed", [5, 6])
+def test_mlp_oracle_gradient_matches_finite_differences(seed):
+    """Central differences of the oracle's own loss in float64 pin its backward pass; every
+    parameter block (W1, b1, W2, b2, W3, b3) is probed."""
+    from oracle_lib import mlp_loss_grad
+    p, x, t = _mlp_case(seed)
+    _, g = mlp_loss_grad(p, x, t)
+    rng = np.random.default_rng(seed)
+    blocks = [(0, 768), (768, 832), (832, 4928), (4928, 4992), (4992, 5184), (5184, 5187)]
+    idx = np.concatenate([rng.integers(lo, hi, size=min(8, hi - lo)) for lo, hi in blocks])
+    # the parameters are float32: step in float64 via a scaled copy of the loss
+    pd = p.astype(np.float64)
+    for i in idx:
+        h = 1e-6
+        up, dn = pd.copy(), pd.copy()
+        up[i] += h
+        dn[i] -= h
+        # evaluate with the numpy closed form in float64 so the step is exact
+        def loss_of(q):
+            out = _numpy_forward(q, x)
+            z = out[:, 0]
+            bce = np.maximum(z, 0) - z * t[:, 0] + np.log1p(np.exp(-np.abs(z)))
+            return np.mean(bce + 0.5 * ((out[:, 1] - t[:, 1]) ** 2 + (out[:, 2] - t[:, 2]) ** 2))
+        fd = (loss_of(up) - loss_of(dn)) / (2 * h)
+        assert abs(fd - g[i]) <= 1e-6 + 1e-4 * abs(fd), (i, fd, g[i])
+
+
+def test_mlp_oracle_gradient_catches_a_planted_mistake():
+    """A backward pass that drops the layer-1 ReLU mask gives a W1 gradient far from the
+    oracle's (which the finite differences above pin), so that mistake cannot pass."""
+    from oracle_lib import mlp_loss_grad
+    p, x, t = _mlp_case(7)
+    _, g = mlp_loss_grad(p, x, t)
+    q = p.astype(np.float64)
+    w1, b1 = q[0:768].reshape(12, 64), q[768:832]
+    w2, b2 = q[832:4928].reshape(64, 64), q[4928:4992]
+    w3, b3 = q[4992:5184].reshape(64, 3), q[5184:5187]
+    xd = x.astype(np.float64)
+    z1 = xd @ w1 + b1
+    h1 = np.maximum(z1, 0)
+    z2 = h1 @ w2 + b2
+    out = np.maximum(z2, 0) @ w3 + b3
+    d3 = np.stack([1 / (1 + np.exp(-out[:, 0])) - t[:, 0], out[:, 1] - t[:, 1], out[:, 2] - t[:, 2]], 1) / len(x)
+    d2 = (d3 @ w3.T) * (z2 > 0)
+    right = xd.T @ ((d2 @ w2.T) * (z1 > 0))
+    planted = xd.T @ (d2 @ w2.T)
+    assert np.allclose(right.ravel(), g[0:768], rtol=1e-9, atol=1e-12)
+    assert np.abs(planted.ravel() - g[0:768]).max() > 1e-2 * np.abs(g[0:768]).max()
+
+
+def test_bf16_round_is_nearest_even():
+    from oracle_lib import bf16_round
+    def rne(v):
+        u = np.array([v], dtype=np.float32).view(np.uint32)[0].item()
+        lsb = (u >> 16) & 1
+        r = ((u + 0x7FFF + lsb) & 0xFFFF0000) & 0xFFFFFFFF
+        return np.array([r], dtype=np.uint32).view(np.float32)[0]
+    one = np.float32(1.0)
+    ulp = np.float32(2.0 ** -7)  # bf16 spacing at 1.0
+    cases = [one, one + ulp / 2, one + ulp + ulp / 2, one + ulp / 4, -(one + 3 * ulp / 2), np.float32(3.14159),
+             np.float32(1e-30), np.float32(-7.5e5)]
+    for v in cases:
+        assert bf16_round(v) == rne(v)
+    assert bf16_round(one + ulp / 2) == one                     # tie to even (down)
+    assert bf16_round(one + ulp + ulp / 2) == one + 2 * ulp     # tie to even (up)
