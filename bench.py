@@ -141,7 +141,11 @@ def main():
     ap.add_argument("--scale", type=int, default=26)
     ap.add_argument("--batch", type=int, default=1_000_000)
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--strategy", choices=["local", "recompute"], default="local")
+    ap.add_argument("--strategy", choices=["local", "recompute"], default="local",
+                    help="plain strategy the learned scheduler is compared with")
+    ap.add_argument("--train-batches", type=int, default=20,
+                    help="plain batches whose repairs log the learned scheduler's training data (0: plain only)")
+    ap.add_argument("--epochs", type=int, default=4)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-verify", action="store_true")
@@ -168,8 +172,8 @@ def main():
         print(json.dumps({"calibrate_copy_bytes": a.numel(), "copies": 5}), flush=True)
         return
 
-    strategy = sa.STRATEGY_LOCAL if args.strategy == "local" else sa.STRATEGY_RECOMPUTE
-    total = args.warmup + args.steps
+    ntrain = args.train_batches
+    total = ntrain + args.warmup + args.steps
     eng = sa.Engine(0)
     t0 = time.perf_counter()
     eng.gen_rmat(args.scale, 16, 0.57, 0.19, 0.19, args.seed)
@@ -181,72 +185,105 @@ def main():
     eng.sync()
     log(f"initial BFS + CC on device: {time.perf_counter() - t0:.2f}s")
     eng.snapshot()
-    lv_init = None
-    if not args.no_cpu:
-        lv_init = (eng.download(sa.STATE_LEVEL), eng.download(sa.STATE_LABEL))
 
-    # Pass 1: generate the stream against the evolving graph and keep every batch in HBM.
+    # Pass 1: generate the stream against the evolving graph and keep every batch in HBM; the
+    # plain repair of the first `ntrain` batches logs the learned scheduler's training data.
     t0 = time.perf_counter()
     first_batch_host = None
+    if ntrain > 0:
+        eng.mlp_log(True)
     for b in range(total):
         m = eng.gen_batch(args.batch, 0.7, 0.5, args.seed, expected_total=args.batch * total)
         eng.batch_store(b)
-        if b == 0 and not args.no_cpu:
+        if b == ntrain + args.warmup and not args.no_cpu:
             first_batch_host = eng.batch_download(m)
         eng.classify()
         eng.apply()
         eng.repair_bfs(sa.STRATEGY_LOCAL)
         eng.repair_cc(sa.STRATEGY_LOCAL)
+        if b == ntrain - 1:
+            eng.mlp_log(False)
     eng.sync()
     log(f"stream of {total} batches generated in {time.perf_counter() - t0:.1f}s")
-
-    # Pass 2: replay from the initial state; W untimed, K timed.
-    eng.restore()
-    for b in range(args.warmup):
-        eng.batch_load(b)
-        eng.classify()
-        eng.apply()
-        eng.repair_bfs(strategy)
-        eng.repair_cc(strategy)
-    eng.sync()
-    eng.prof_enable(True)
-    lat = []
-    lines = 0
-    work = {"bfs_invalidated": 0, "bfs_seeds": 0, "bfs_rounds": 0, "bfs_entries_scanned": 0,
-            "cc_entries_scanned": 0, "insertions": 0, "deletions": 0}
-    eng.sync()
-    t_start = time.perf_counter()
-    for b in range(args.warmup, total):
-        ts = time.perf_counter()
-        eng.batch_load(b)
-        eng.classify()
-        eng.apply()
-        sb = eng.repair_bfs(strategy)
-        sc = eng.repair_cc(strategy)
+    train = None
+    if ntrain > 0:
+        t0 = time.perf_counter()
+        eng.mlp_init(args.seed)
+        first, last = eng.mlp_train(epochs=args.epochs, lr=2e-3, seed=args.seed)
         eng.sync()
-        lat.append((time.perf_counter() - ts) * 1e3)
-        lines += sb.lines
-        work["bfs_invalidated"] += sb.touched
-        work["bfs_seeds"] += sb.seeds
-        work["bfs_rounds"] += sb.rounds
-        work["bfs_entries_scanned"] += sb.edges_scanned
-        work["cc_entries_scanned"] += sc.edges_scanned
-        work["insertions"] += sb.insertions
-        work["deletions"] += sb.deletions
-    eng.sync()
-    elapsed = time.perf_counter() - t_start
-    prof = {name: eng.prof_read(i) for i, name in enumerate(sa.PROF_NAMES)}
-    eng.prof_enable(False)
+        train = {"samples": eng.mlp_samples(), "batches": ntrain, "epochs": args.epochs,
+                 "loss_first_epoch": round(first, 4), "loss_last_epoch": round(last, 4),
+                 "seconds": round(time.perf_counter() - t0, 2)}
+        log(f"learned scheduler: {train}")
+
+    def run(strategy, profile):
+        """Replays the stream from the snapshot: training and warmup batches untimed, then the
+        timed steps. Returns throughput, latencies, work counters and kernel profile."""
+        eng.restore()
+        for b in range(ntrain + args.warmup):
+            eng.batch_load(b)
+            eng.classify()
+            eng.apply()
+            eng.repair_bfs(sa.STRATEGY_LOCAL if b < ntrain else strategy)
+            eng.repair_cc(sa.STRATEGY_LOCAL if b < ntrain else strategy)
+        eng.sync()
+        if profile:
+            eng.prof_enable(True)
+        lat = []
+        lines = 0
+        work = {"bfs_invalidated": 0, "bfs_seeds": 0, "bfs_rounds": 0, "bfs_entries_scanned": 0,
+                "cc_entries_scanned": 0, "readbacks": 0, "fallbacks": 0, "insertions": 0, "deletions": 0}
+        eng.sync()
+        t_start = time.perf_counter()
+        for b in range(ntrain + args.warmup, total):
+            ts = time.perf_counter()
+            eng.batch_load(b)
+            eng.classify()
+            eng.apply()
+            sb = eng.repair_bfs(strategy)
+            sc = eng.repair_cc(strategy)
+            eng.sync()
+            lat.append((time.perf_counter() - ts) * 1e3)
+            lines += sb.lines
+            work["bfs_invalidated"] += sb.touched
+            work["bfs_seeds"] += sb.seeds
+            work["bfs_rounds"] += sb.rounds
+            work["bfs_entries_scanned"] += sb.edges_scanned
+            work["cc_entries_scanned"] += sc.edges_scanned
+            work["readbacks"] += sb.readbacks + sc.readbacks
+            work["fallbacks"] += sb.fallback
+            work["insertions"] += sb.insertions
+            work["deletions"] += sb.deletions
+        eng.sync()
+        elapsed = time.perf_counter() - t_start
+        prof = None
+        if profile:
+            prof = {name: eng.prof_read(i) for i, name in enumerate(sa.PROF_NAMES)}
+            eng.prof_enable(False)
+        return {"ups": lines / elapsed, "elapsed": elapsed, "lat": lat, "work": work, "prof": prof,
+                "state": (eng.download(sa.STATE_LEVEL), eng.download(sa.STATE_LABEL))}
+
+    strategy = {"local": sa.STRATEGY_LOCAL, "recompute": sa.STRATEGY_RECOMPUTE}[args.strategy]
+    plain = run(strategy, profile=ntrain == 0)
+    log(f"plain ({args.strategy}): {plain['ups']:.4g} updates/s, p50 {np.percentile(plain['lat'], 50):.3f} ms")
+    head = plain
+    learned = None
+    if ntrain > 0:
+        learned = run(sa.STRATEGY_LEARNED, profile=True)
+        log(f"learned: {learned['ups']:.4g} updates/s, p50 {np.percentile(learned['lat'], 50):.3f} ms")
+        head = learned
+    prof = head["prof"]
 
     verified = None
     if not args.no_verify:
-        lv = eng.download(sa.STATE_LEVEL)
-        lab = eng.download(sa.STATE_LABEL)
+        lv, lab = head["state"]
         eng.bfs_full(0)
         eng.cc_full()
         verified = bool(np.array_equal(lv, eng.download(sa.STATE_LEVEL)) and
                         np.array_equal(lab, eng.download(sa.STATE_LABEL)))
-        log(f"final state equals full recompute: {verified}")
+        if learned is not None:
+            verified = verified and all(np.array_equal(x, y) for x, y in zip(plain["state"], learned["state"]))
+        log(f"final state equals full recompute{' and the plain repair' if learned else ''}: {verified}")
 
     dom = max(prof, key=lambda k: prof[k]["ms"])
     d = prof[dom]
@@ -267,20 +304,28 @@ def main():
 
     cpu = None
     if not args.no_cpu and first_batch_host is not None:
+        # the state the first timed batch starts from
         eng.restore()
+        for b in range(ntrain + args.warmup):
+            eng.batch_load(b)
+            eng.classify()
+            eng.apply()
+            eng.repair_bfs(sa.STRATEGY_LOCAL)
+            eng.repair_cc(sa.STRATEGY_LOCAL)
+        eng.sync()
         t0 = time.perf_counter()
         cpu = cpu_baseline(eng, info["n"], [first_batch_host], args.cpu_seconds)
         log(f"cpu baseline: {cpu['value']:.0f} updates/s ({time.perf_counter() - t0:.1f}s)")
 
-    ups = lines / elapsed
+    steps = args.steps
     out = {
         "metric": "edge updates/s + p99 batch-commit latency, learned vs plain repair (bit-exact)",
-        "value": round(ups, 1),
+        "value": round(head["ups"], 1),
         "unit": "edge updates/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": steps,
         "warmup": args.warmup,
-        "ms_per_step": round(elapsed * 1e3 / args.steps, 3),
+        "ms_per_step": round(head["elapsed"] * 1e3 / steps, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -289,15 +334,22 @@ def main():
         "config": {"workload": f"BFS(0)+CC batch repair, R-MAT scale {args.scale} ef 16, "
                                f"{args.batch} toggles/batch (70% deletions, half BFS-tree edges)",
                    "scale": args.scale, "vertices": info["n"], "adjacency_entries": info["base_entries"],
-                   "batch": args.batch, "strategy": args.strategy + " (plain)"},
-        "p50_ms": round(float(np.percentile(lat, 50)), 3),
-        "p99_ms": round(float(np.percentile(lat, 99)), 3),
+                   "batch": args.batch,
+                   "strategy": "learned" if learned else args.strategy + " (plain)",
+                   "train_batches": ntrain},
+        "p50_ms": round(float(np.percentile(head["lat"], 50)), 3),
+        "p99_ms": round(float(np.percentile(head["lat"], 99)), 3),
+        "plain": {"value": round(plain["ups"], 1), "p50_ms": round(float(np.percentile(plain["lat"], 50)), 3),
+                  "p99_ms": round(float(np.percentile(plain["lat"], 99)), 3),
+                  "work_per_batch": {k: round(v / steps, 1) for k, v in plain["work"].items()}},
+        "learned_over_plain": round(learned["ups"] / plain["ups"], 4) if learned else None,
+        "learned_training": train,
         "verified_exact": verified,
         "roofline": roofline,
         "kernels": {k: {"ms": round(v["ms"], 3), "launches": v["launches"],
                         "GBps": round(v["bytes"] / (v["ms"] / 1e3) / 1e9, 1) if v["ms"] > 0 else 0.0}
                     for k, v in prof.items() if v["launches"]},
-        "work_per_batch": {k: round(v / args.steps, 1) for k, v in work.items()},
+        "work_per_batch": {k: round(v / steps, 1) for k, v in head["work"].items()},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

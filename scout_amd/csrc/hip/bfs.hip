@@ -609,7 +609,7 @@ __global__ void k_find_parents(GView g, const int32_t* level, int32_t* parent, i
 // (count in box slot 8) and optional seeds; runs until no frontier and no seed above the
 // current level remains. Adds the non-empty rounds to *rounds.
 int relax_levels(Ctx& c, int32_t* L, int32_t* P, int32_t lvl0, int32_t max_seed, const int64_t* seeds, int64_t nseeds,
-                 int32_t epoch, int64_t* rounds, int64_t* scanned_total) {
+                 int32_t epoch, int64_t* rounds, int64_t* scanned_total, int first_group = kFirstGroup) {
   int32_t n = c.g.n;
   int64_t cap = (int64_t)n + 1;
   int64_t task_cap = c.wb.cap;
@@ -624,7 +624,7 @@ int relax_levels(Ctx& c, int32_t* L, int32_t* P, int32_t lvl0, int32_t max_seed,
   k_zero<<<1, 64, 0, c.s>>>(box + 15, 3);
   int32_t lvl = lvl0;
   int p = 0;
-  int group = kFirstGroup;
+  int group = first_group;
   while (true) {
     SCOUT_CHECK(prof_begin(c, PROF_BFS_RELAX));
     for (int k = 0; k < group; k++) {
@@ -724,15 +724,17 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, int32_t* P, int strategy, scou
     k_del_candidates<<<grid_for(m), kBlock, 0, c.s>>>(ch.d_uv.p, ch.d_ins.p, m, L, P, c.la.p, c.la.cap, box, ab.root);
   // features and predictions are computed from the device-side counts and come back with the
   // same read-back as the candidate counts ([24] predicted cascade sizes, [25] last level)
-  if (logging || learned) SCOUT_CHECK(mlp_begin_batch(c, c.la.p, box, L, P));
-  if (learned) SCOUT_CHECK(mlp_predict_async(c, box, box + 24));
+  int64_t sample = logging ? kMlpSample : kMlpInfer;
+  if (logging || learned) SCOUT_CHECK(mlp_begin_batch(c, c.la.p, box, L, P, sample));
+  if (learned) SCOUT_CHECK(mlp_predict_async(c, box, sample, box + 24));
   SCOUT_CHECK(box_read(c, learned ? 26 : 8));
   int64_t ncand = c.h_box[0];
   int32_t cmin = (int32_t)c.h_box[1], cmax = (int32_t)c.h_box[2];
   st->deletions += c.h_box[3];
   int64_t nx = 0, scanned = 0, rounds = 0;
-  int64_t nsample = std::min<int64_t>(ncand, kMlpSample);
+  int64_t nsample = std::min<int64_t>(ncand, sample);
   int first_group = kFirstGroup;
+  int64_t predicted_last = -1;
   if (learned && ncand > 0) {
     // predictions size and order the work; the plain search below decides every level
     double predicted = (double)c.h_box[24] * (double)ncand / (double)nsample;
@@ -741,8 +743,9 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, int32_t* P, int strategy, scou
       st->fallback = 1;
       return bfs_full_on(c, L, P, c.source, &st->rounds, &st->edges_scanned);
     }
-    int64_t last = c.h_box[25];
-    if (last >= cmin) first_group = (int)std::min<int64_t>(std::max<int64_t>(last - cmin + 2, kFirstGroup), kMaxGroup);
+    predicted_last = c.h_box[25];
+    if (predicted_last >= cmin)
+      first_group = (int)std::min<int64_t>(std::max<int64_t>(predicted_last - cmin + 2, kFirstGroup), kMaxGroup);
   }
   if (ncand > 0) {
     // Invalidation, level by level, queued in groups of rounds. Counts: [8]/[9] candidates at
@@ -835,7 +838,11 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, int32_t* P, int strategy, scou
     k_seed_parents<<<grid_for(nseeds), kBlock, 0, c.s>>>(c.wa.p, c.lg.p, nseeds, L, P);
     int32_t epoch = next_epoch(c);
     k_zero<<<1, 64, 0, c.s>>>(box + 8, 1);
-    SCOUT_CHECK(relax_levels(c, L, P, smin, smax, c.wa.p, nseeds, epoch, &rounds, &scanned));
+    // the learned scheduler's predicted last level bounds the relax as well
+    int relax_group = kFirstGroup;
+    if (learned && predicted_last >= smin)
+      relax_group = (int)std::min<int64_t>(std::max<int64_t>(predicted_last - smin + 2, kFirstGroup), kMaxGroup);
+    SCOUT_CHECK(relax_levels(c, L, P, smin, smax, c.wa.p, nseeds, epoch, &rounds, &scanned, relax_group));
   }
   st->rounds += rounds;
   st->edges_scanned += scanned;

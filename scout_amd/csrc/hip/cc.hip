@@ -11,8 +11,13 @@
 //      mark it; 0 is the smallest id, so every reached vertex has label 0. A deletion that
 //      splits a piece off this component shows up as that piece becoming unreached;
 //   2. every other component lies wholly in the unreached subgraph (a neighbour of an
-//      unreached vertex is unreached), which a lock-free union-find over its rows relabels
-//      from scratch, hooking larger roots under smaller so every root is its set's minimum.
+//      unreached vertex is unreached). Only the components the batch can have changed are
+//      relabelled: those holding an unreached endpoint of a batch line (edges inserted or
+//      deleted inside or between them), plus the vertices that left vertex 0's component.
+//      Every other unreached component kept all its edges and all its vertices, so it is
+//      still a component with the same minimum. The affected vertices are closed under
+//      adjacency, and a lock-free union-find over their rows relabels them, hooking larger
+//      roots under smaller so every root is its set's minimum.
 // When the context also maintains BFS from vertex 0 and this batch's BFS repair already ran,
 // its levels are used directly; otherwise the CC state keeps its own level array.
 #include "common.cuh"
@@ -109,33 +114,6 @@ __global__ void k_cc_flatten(int32_t* parent, int32_t n) {
 
 // --- batch repair ---------------------------------------------------------------------
 
-// Union pass over the unreached subgraph, thread per vertex (grid-stride): vertices the BFS
-// from vertex 0 does not reach unite with their neighbours w > v. Neighbours of unreached
-// vertices are unreached, so the union-find stays inside the unreached subgraph. Rows longer
-// than 32 entries are listed for k_cc_unreached_hook (wave per row). `par` holds the
-// identity for every vertex on entry.
-__global__ void k_cc_unreached(GView g, const int32_t* L, int32_t* par, int32_t* longs, int64_t* nlong, int64_t cap,
-                               int64_t* scanned) {
-  int64_t work = 0;
-  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < g.n; v += (int64_t)gridDim.x * blockDim.x) {
-    if (L[v] >= 0) continue;
-    Row r = row_of(g, (int32_t)v);
-    int64_t len = row_len(r);
-    if (len == 0) continue;
-    if (len > 32) {
-      thread_append((int32_t)v, longs, nlong, cap);
-      continue;
-    }
-    work += len;
-    for (int64_t k = 0; k < len; k++) {
-      int32_t w = row_at(g, r, k);
-      int32_t h;
-      if (w > v) uf_union(par, (int32_t)v, w, &h);
-    }
-  }
-  block_add(work, scanned);
-}
-
 // Wave per listed long row v: union(v, w) for every neighbour w > v; rows longer than
 // kChunk become chunk tasks.
 __global__ void k_cc_unreached_hook(GView g, const int32_t* list, const int64_t* count, int32_t* par, int64_t* tasks,
@@ -163,19 +141,101 @@ __global__ void k_cc_unreached_hook(GView g, const int32_t* list, const int64_t*
   if (lane_id() == 0 && work) atomicAdd((unsigned long long*)scanned, (unsigned long long)work);
 }
 
-// Labels (grid-stride): vertex 0's component (the reached set) has minimum id 0; every other
-// vertex takes the root of its union-find set, which is its component's minimum. *roots
-// counts the components.
-__global__ void k_cc_final(const int32_t* L, int32_t n, int32_t* par, int32_t* label, int64_t* roots) {
+// Marks (bitmap over label ids) the old label of every endpoint of a batch line that is
+// unreached after the batch. Label 0 (vertex 0's old component) needs no mark: its unreached
+// vertices are relabelled anyway. A bit already set is not written again, so the endpoints of
+// one large component do not serialise on one word.
+// A coarse bitmap (one bit per kCoarse labels, small enough to stay in L2) filters the
+// lookups of the collect pass, so most unreached vertices never touch the fine bitmap.
+constexpr int kCoarseShift = 4;
+
+__global__ void k_cc_mark(const int32_t* uv, int64_t m, const int32_t* L, const int32_t* label, uint32_t* bits,
+                          uint32_t* coarse) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 2 * m) return;
+  int32_t x = uv[i];
+  if (L[x] >= 0) return;
+  int32_t l = label[x];
+  if (l == 0) return;
+  uint32_t bit = 1u << (l & 31);
+  if (!(bits[l >> 5] & bit)) atomicOr(&bits[l >> 5], bit);
+  int32_t cl = l >> kCoarseShift;
+  uint32_t cbit = 1u << (cl & 31);
+  if (!(coarse[cl >> 5] & cbit)) atomicOr(&coarse[cl >> 5], cbit);
+}
+
+__global__ void k_zero_u32(uint32_t* p, int64_t count) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) p[i] = 0;
+}
+
+// Grid-stride pass over all vertices: newly reached vertices take label 0; unreached vertices
+// whose old component is marked, or that were in vertex 0's component (old label 0), are
+// listed for relabelling with par[v] = v. kept counts the roots of untouched components.
+__global__ void k_cc_collect(const int32_t* L, int32_t* label, const uint32_t* bits, const uint32_t* coarse, int32_t n,
+                             int32_t* par, int32_t* list, int64_t* count, int64_t cap, int64_t* kept) {
+  int64_t roots = 0;
+  // listed vertices are sparse and spread over the whole range: staged per wave in LDS
+  constexpr int kSeg = 512;
+  __shared__ int32_t seg[kWavesPerBlock * kSeg];
+  WaveStage<kSeg> ws{seg + (threadIdx.x / kWave) * kSeg, 0};
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n; base += (int64_t)gridDim.x * blockDim.x) {
+    int64_t v = base + threadIdx.x;
+    bool take = false;
+    if (v < n) {
+      int32_t l = label[v];
+      if (L[v] >= 0) {
+        if (l != 0) label[v] = 0;
+      } else if (l == 0 || (((coarse[l >> (5 + kCoarseShift)] >> ((l >> kCoarseShift) & 31)) & 1u) &&
+                            ((bits[l >> 5] >> (l & 31)) & 1u))) {
+        take = true;
+        par[v] = (int32_t)v;
+      } else {
+        roots += l == v;
+      }
+    }
+    stage_push(ws, take, (int32_t)v, list, count, cap);
+  }
+  stage_finish(ws, list, count, cap);
+  block_add(roots, kept);
+}
+
+// Thread per listed vertex: union with neighbours w > v for rows up to 32 entries; longer
+// rows are listed for k_cc_unreached_hook.
+__global__ void k_cc_union_list(GView g, const int32_t* list, const int64_t* count, int32_t* par, int32_t* longs,
+                                int64_t* nlong, int64_t cap, int64_t* scanned) {
+  int64_t nl = *count;
+  int64_t work = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nl; i += (int64_t)gridDim.x * blockDim.x) {
+    int32_t v = list[i];
+    Row r = row_of(g, v);
+    int64_t len = row_len(r);
+    if (len > 32) {
+      thread_append(v, longs, nlong, cap);
+      continue;
+    }
+    work += len;
+    for (int64_t k = 0; k < len; k++) {
+      int32_t w = row_at(g, r, k);
+      int32_t h;
+      if (w > v) uf_union(par, v, w, &h);
+    }
+  }
+  block_add(work, scanned);
+}
+
+// Thread per listed vertex: its label is its set's root (the component minimum).
+__global__ void k_cc_final_list(const int32_t* list, const int64_t* count, int32_t* par, int32_t* label,
+                                int64_t* roots) {
+  int64_t nl = *count;
   int64_t cnt = 0;
-  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x) {
-    if (L[v] >= 0) {
-      label[v] = 0;
-      cnt += (v == 0);
-    } else {
-      int32_t r = uf_find(par, (int32_t)v);
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < nl; base += (int64_t)gridDim.x * blockDim.x) {
+    int64_t i = base + threadIdx.x;
+    if (i < nl) {
+      int32_t v = list[i];
+      int32_t r = uf_find(par, v);
       label[v] = r;
-      cnt += (r == v);
+      cnt += r == v;
     }
   }
   block_add(cnt, roots);
@@ -203,6 +263,8 @@ int cc_full(Ctx& c, int64_t* scanned) {
   // parent and final label (40 B)
   prof_bytes(c, PROF_CC_HOOK, 12 * c.h_box[1] + 40 * (int64_t)n);
   if (scanned) *scanned += c.h_box[1];
+  c.label_seq = c.apply_seq;
+  c.cc_components = -1;
   return SCOUT_OK;
 }
 
@@ -234,29 +296,43 @@ int cc_local(Ctx& c, Channel& ch, scout_stats* st) {
   }
   SCOUT_CHECK(ensure_scratch(c, &c.cc_owner, -1));
   SCOUT_CHECK(c.la.reserve((int64_t)n + 1));
+  SCOUT_CHECK(c.lb.reserve((int64_t)n + 1));
   SCOUT_CHECK(c.wb.reserve(2 * (c.g.E + c.g.A) / kChunk + 1024));
+  int64_t words = ((int64_t)n + 31) / 32, cwords = (((int64_t)n >> kCoarseShift) + 32) / 32;
+  SCOUT_CHECK(c.cc_bits.reserve(words + cwords + 1));
+  uint32_t* bits = c.cc_bits.p;
+  uint32_t* coarse = c.cc_bits.p + words;
   int32_t* par = c.cc_owner;
-  SCOUT_TRY(hipMemsetAsync(c.d_box, 0, 4 * sizeof(int64_t), c.s));
+  int64_t* box = c.d_box;
+  // box: [0] listed vertices, [1] chunk tasks, [2] scanned entries, [3] roots among the listed,
+  // [4] roots of untouched components, [5] long rows
+  SCOUT_TRY(hipMemsetAsync(box, 0, 6 * sizeof(int64_t), c.s));
   SCOUT_CHECK(prof_begin(c, PROF_CC_EXPAND));
-  k_iota<<<grid_for(n), kBlock, 0, c.s>>>(par, n);
-  k_cc_unreached<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, par, c.la.p, c.d_box, (int64_t)n + 1, c.d_box + 2);
-  k_cc_unreached_hook<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), c.la.p, c.d_box, par, c.wb.p, c.d_box + 1,
-                                                      c.wb.cap, c.d_box + 2);
-  k_cc_hook_tasks<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), par, c.wb.p, c.d_box + 1, c.wb.cap, c.d_box + 2);
-  k_cc_final<<<kListGrid, kBlock, 0, c.s>>>(L, n, par, c.label, c.d_box + 3);
+  k_zero_u32<<<grid_for(words + cwords), kBlock, 0, c.s>>>(bits, words + cwords);
+  if (ch.m > 0) k_cc_mark<<<grid_for(2 * ch.m), kBlock, 0, c.s>>>(ch.d_uv.p, ch.m, L, c.label, bits, coarse);
+  k_cc_collect<<<kListGrid, kBlock, 0, c.s>>>(L, c.label, bits, coarse, n, par, c.la.p, box, (int64_t)n + 1, box + 4);
+  k_cc_union_list<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), c.la.p, box, par, c.lb.p, box + 5, (int64_t)n + 1,
+                                                  box + 2);
+  k_cc_unreached_hook<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), c.lb.p, box + 5, par, c.wb.p, box + 1, c.wb.cap,
+                                                      box + 2);
+  k_cc_hook_tasks<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), par, c.wb.p, box + 1, c.wb.cap, box + 2);
+  k_cc_final_list<<<kListGrid, kBlock, 0, c.s>>>(c.la.p, box, par, c.label, box + 3);
   SCOUT_CHECK(prof_end(c, PROF_CC_EXPAND, 0));
-  SCOUT_CHECK(box_read(c, 4));
+  SCOUT_CHECK(box_read(c, 6));
   if (c.h_box[1] > c.wb.cap) {
     set_error("cc: chunk task overflow");
     return SCOUT_ERR_STATE;
   }
-  // per vertex: its level (twice), parent init and label write (16 B); per scanned entry the
-  // neighbour and two parents (12 B)
-  prof_bytes(c, PROF_CC_EXPAND, 16 * (int64_t)n + 12 * c.h_box[2]);
-  int64_t comps = c.h_box[3];
+  // per vertex its level and old label (8 B); per listed vertex its list entry, parent and
+  // label (12 B); per scanned entry the neighbour and two parents (12 B); per batch line
+  // endpoint its level, label and mark (12 B)
+  prof_bytes(c, PROF_CC_EXPAND, 8 * (int64_t)n + 12 * c.h_box[0] + 12 * c.h_box[2] + 24 * ch.m);
+  int64_t comps = 1 + c.h_box[4] + c.h_box[3];  // vertex 0's, untouched ones, relabelled ones
   if (c.cc_components >= 0 && comps > c.cc_components) st->splits += comps - c.cc_components;
   c.cc_components = comps;
   st->edges_scanned += c.h_box[2];
+  st->touched += c.h_box[0];
+  c.label_seq = c.apply_seq;
   return SCOUT_OK;
 }
 
@@ -296,7 +372,9 @@ int scout_repair_cc(scout_ctx* x, int channel, int strategy, scout_stats* st) {
   SCOUT_CHECK(span.begin(c.s));
   st->lines += ch->m;
   int64_t rb0 = c.readbacks;
-  if (strategy == SCOUT_STRATEGY_RECOMPUTE) {
+  // relabelling only the touched components needs labels exact for the graph before this batch
+  bool stale = c.label_seq != c.apply_seq - 1;
+  if (strategy == SCOUT_STRATEGY_RECOMPUTE || stale) {
     st->fallback = 1;
     SCOUT_CHECK(cc_full(c, &st->edges_scanned));
   } else {

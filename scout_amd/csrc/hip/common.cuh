@@ -143,6 +143,65 @@ __device__ __forceinline__ int64_t block_append(bool take, T value, T* out, int6
   return pos;
 }
 
+// Sparse appends over a long grid-stride loop: each wave stages its takes in an LDS segment
+// of kSeg entries and global slots are reserved once per block at the end (or once per wave
+// when its segment fills). A sparse list spread over many waves would otherwise cost one
+// global atomic on the same counter per wave step, which serialises at one L2 channel.
+// Every thread of the block must call stage_finish once, after the loop.
+template <int kSeg>
+struct WaveStage {
+  int32_t* seg;  // this wave's segment in LDS
+  int n;         // entries staged (uniform across the wave)
+};
+
+template <int kSeg>
+__device__ __forceinline__ void stage_flush(WaveStage<kSeg>& ws, int32_t* out, int64_t* count, int64_t cap) {
+  int64_t base = 0;
+  if (lane_id() == 0) base = (int64_t)atomicAdd((unsigned long long*)count, (unsigned long long)ws.n);
+  base = __shfl(base, 0);
+  for (int k = lane_id(); k < ws.n; k += kWave)
+    if (base + k < cap) out[base + k] = ws.seg[k];
+  ws.n = 0;
+}
+
+template <int kSeg>
+__device__ __forceinline__ void stage_push(WaveStage<kSeg>& ws, bool take, int32_t value, int32_t* out,
+                                           int64_t* count, int64_t cap) {
+  unsigned long long mask = __ballot(take);
+  if (mask == 0) return;
+  int cnt = __popcll(mask);
+  if (ws.n + cnt > kSeg) {
+    __threadfence_block();
+    stage_flush(ws, out, count, cap);
+  }
+  if (take) ws.seg[ws.n + __popcll(mask & ((1ULL << lane_id()) - 1ULL))] = value;
+  ws.n += cnt;
+}
+
+template <int kSeg>
+__device__ __forceinline__ void stage_finish(WaveStage<kSeg>& ws, int32_t* out, int64_t* count, int64_t cap) {
+  __shared__ int64_t wn[kWavesPerBlock];
+  __shared__ int64_t gbase;
+  __threadfence_block();
+  int w = (int)(threadIdx.x / kWave);
+  if (lane_id() == 0) wn[w] = ws.n;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int64_t t = 0;
+    for (int i = 0; i < kWavesPerBlock; i++) {
+      int64_t c = wn[i];
+      wn[i] = t;
+      t += c;
+    }
+    gbase = t ? (int64_t)atomicAdd((unsigned long long*)count, (unsigned long long)t) : 0;
+  }
+  __syncthreads();
+  int64_t base = gbase + wn[w];
+  for (int k = lane_id(); k < ws.n; k += kWave)
+    if (base + k < cap) out[base + k] = ws.seg[k];
+  __syncthreads();
+}
+
 // Per-thread append (when lanes of one wave are not converged on the same list).
 __device__ __forceinline__ void thread_append(int32_t value, int32_t* out, int64_t* count, int64_t cap) {
   int64_t pos = (int64_t)atomicAdd((unsigned long long*)count, 1ULL);

@@ -53,7 +53,7 @@ void free_state(Ctx& c) {
   for (void* p : ps)
     if (p) (void)hipFree(p);
   c.cc_level = c.cc_parent = c.parent = nullptr;
-  c.level_seq = c.cc_level_seq = -1;
+  c.level_seq = c.cc_level_seq = c.label_seq = -1;
   c.cc_components = -1;
   c.level = c.label = c.color = nullptr;
   c.mis = nullptr;
@@ -282,6 +282,7 @@ int scout_ctx_destroy(scout_ctx* x) {
   c.wa.release();
   c.wb.release();
   c.wc.release();
+  c.cc_bits.release();
   c.tmp.release();
   if (c.h_box) (void)hipHostFree(c.h_box);
   if (c.d_box) (void)hipFree(c.d_box);
@@ -401,7 +402,10 @@ int scout_state_upload(scout_ctx* x, int kind, const void* host, int64_t n) {
     SCOUT_CHECK(compute_parents(c, c.level, c.parent));
     c.level_seq = c.apply_seq;
   }
-  if (kind == SCOUT_STATE_LABEL) c.cc_components = -1;
+  if (kind == SCOUT_STATE_LABEL) {
+    c.cc_components = -1;
+    c.label_seq = c.apply_seq;
+  }
   return SCOUT_OK;
 }
 

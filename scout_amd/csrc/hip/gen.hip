@@ -312,7 +312,7 @@ int scout_gen_rmat(scout_ctx* x, int scale, int edge_factor, double a, double b,
   for (void* p : ps)
     if (p) (void)hipFree(p);
   ctx.cc_level = ctx.cc_parent = ctx.parent = nullptr;
-  ctx.level_seq = ctx.cc_level_seq = ctx.cc_components = -1;
+  ctx.level_seq = ctx.cc_level_seq = ctx.cc_components = ctx.label_seq = -1;
   ctx.level = ctx.label = ctx.color = nullptr;
   ctx.mis = nullptr;
   ctx.cc_owner = ctx.col_marker = ctx.mis_minw = ctx.mis_tag = nullptr;

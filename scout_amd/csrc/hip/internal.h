@@ -167,6 +167,8 @@ struct Ctx {
   int64_t level_seq = -1;        // apply_seq at which `level` was last made exact
   int64_t cc_level_seq = -1;     // same for cc_level
   int64_t cc_components = -1;    // component count after the last CC repair
+  int64_t label_seq = -1;        // apply_seq at which `label` was last made exact
+  DevBuf<uint32_t> cc_bits;      // CC: bitmap of labels touched by the batch
   // small mailbox read back by the host after a round
   int64_t* h_box = nullptr;    // pinned
   int64_t* d_box = nullptr;
@@ -227,20 +229,22 @@ void snapshot_release(Ctx& c);
 constexpr int kMlpIn = 12, kMlpHidden = 64, kMlpOut = 3;
 constexpr int kMlpParams = kMlpIn * kMlpHidden + kMlpHidden + kMlpHidden * kMlpHidden + kMlpHidden +
                            kMlpHidden * kMlpOut + kMlpOut;
-constexpr int64_t kMlpSample = 16384;  // candidates per batch whose features are computed
+constexpr int64_t kMlpSample = 16384;  // candidates per batch logged for training
+constexpr int64_t kMlpInfer = 2048;    // candidates per batch the scheduler predicts for
 bool mlp_logging(Ctx& c);
 bool mlp_ready(Ctx& c);
 // Features of the first min(ncand, kMlpSample) candidates (state before the repair); the
 // counts are read on the device from the candidate pass's mailbox d_box ([0] candidates,
 // [1] smallest level, [3] deletion lines).
-int mlp_begin_batch(Ctx& c, const int32_t* cand, const int64_t* d_box, const int32_t* L, const int32_t* P);
+int mlp_begin_batch(Ctx& c, const int32_t* cand, const int64_t* d_box, const int32_t* L, const int32_t* P,
+                    int64_t sample);
 // Root attribution arrays while logging (nullptr otherwise).
 int32_t* mlp_root(Ctx& c);
 int32_t* mlp_subtree(Ctx& c);
 int32_t* mlp_maxlvl(Ctx& c);
 // Predictions over the sample, aggregated on the device: d_out[0] = predicted cascade sizes
 // summed over the sample, d_out[1] = the largest predicted last level.
-int mlp_predict_async(Ctx& c, const int64_t* d_ncand, int64_t* d_out);
+int mlp_predict_async(Ctx& c, const int64_t* d_ncand, int64_t sample, int64_t* d_out);
 // Appends the sample's features and outcomes (state after the repair) to the training store.
 int mlp_end_batch(Ctx& c, const int32_t* cand, int64_t ns, const int32_t* L);
 void mlp_release(Ctx& c);

@@ -272,11 +272,11 @@ constexpr int kScan = 16;
 
 // Counts come from the candidate pass's mailbox: box[0] candidates, box[1] smallest candidate
 // level, box[3] deletion lines (no read-back needed before the features).
-__global__ void k_mlp_features(GView g, const int32_t* cand, const int64_t* box, const int32_t* level,
+__global__ void k_mlp_features(GView g, const int32_t* cand, const int64_t* box, int64_t sample, const int32_t* level,
                                const int32_t* parent, float* feats, int32_t* lvl0, int32_t* subtree, int32_t* maxlvl) {
   int64_t ncand = box[0], ndel = box[3];
   int32_t cmin = (int32_t)box[1];
-  int64_t ns = ncand < kMlpSample ? ncand : kMlpSample;
+  int64_t ns = ncand < sample ? ncand : sample;
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= ns) return;
   if (subtree) {
@@ -338,8 +338,9 @@ __global__ void k_pred_init(int64_t* box) {
 
 // Aggregates of the predictions: box[0] = sum of predicted cascade sizes (rounded), box[1] =
 // the largest predicted last level L[c] + depth - 1.
-__global__ void k_mlp_aggregate(const float* out, const int32_t* lvl0, const int64_t* ncand, int64_t* box) {
-  int64_t ns = *ncand < kMlpSample ? *ncand : kMlpSample;
+__global__ void k_mlp_aggregate(const float* out, const int32_t* lvl0, const int64_t* ncand, int64_t sample,
+                                int64_t* box) {
+  int64_t ns = *ncand < sample ? *ncand : sample;
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   int64_t size = 0, last = -1;
   if (i < ns) {
@@ -437,12 +438,13 @@ int64_t coprime_stride(int64_t n, uint64_t seed) {
 bool mlp_logging(Ctx& c) { return c.mlp && c.mlp->logging; }
 bool mlp_ready(Ctx& c) { return c.mlp && c.mlp->trained; }
 
-int mlp_begin_batch(Ctx& c, const int32_t* cand, const int64_t* d_box, const int32_t* L, const int32_t* P) {
+int mlp_begin_batch(Ctx& c, const int32_t* cand, const int64_t* d_box, const int32_t* L, const int32_t* P,
+                    int64_t sample) {
   SCOUT_CHECK(ensure_mlp(c));
   MlpState& s = *c.mlp;
   bool log = s.logging;
-  k_mlp_features<<<grid_for(kMlpSample), kBlock, 0, c.s>>>(c.g.view(), cand, d_box, L, P, s.feat_tmp.p, s.lvl0.p,
-                                                           log ? s.subtree.p : nullptr, log ? s.maxlvl.p : nullptr);
+  k_mlp_features<<<grid_for(sample), kBlock, 0, c.s>>>(c.g.view(), cand, d_box, sample, L, P, s.feat_tmp.p, s.lvl0.p,
+                                                       log ? s.subtree.p : nullptr, log ? s.maxlvl.p : nullptr);
   return SCOUT_OK;
 }
 
@@ -455,12 +457,13 @@ int32_t* mlp_root(Ctx& c) {
 int32_t* mlp_subtree(Ctx& c) { return mlp_logging(c) ? c.mlp->subtree.p : nullptr; }
 int32_t* mlp_maxlvl(Ctx& c) { return mlp_logging(c) ? c.mlp->maxlvl.p : nullptr; }
 
-int mlp_predict_async(Ctx& c, const int64_t* d_ncand, int64_t* d_out) {
+int mlp_predict_async(Ctx& c, const int64_t* d_ncand, int64_t sample, int64_t* d_out) {
   MlpState& s = *c.mlp;
-  k_mlp_forward<<<kMlpSample * kWave / kBlock, kBlock, 0, c.s>>>(s.w16.p, s.feat_tmp.p, kMlpSample, d_ncand,
-                                                                  s.out_tmp.p);
+  // 8 samples per wave: the lane weights are loaded once per wave
+  k_mlp_forward<<<(unsigned)std::max<int64_t>(1, sample / 8 * kWave / kBlock), kBlock, 0, c.s>>>(
+      s.w16.p, s.feat_tmp.p, sample, d_ncand, s.out_tmp.p);
   k_pred_init<<<1, 1, 0, c.s>>>(d_out);
-  k_mlp_aggregate<<<grid_for(kMlpSample), kBlock, 0, c.s>>>(s.out_tmp.p, s.lvl0.p, d_ncand, d_out);
+  k_mlp_aggregate<<<grid_for(sample), kBlock, 0, c.s>>>(s.out_tmp.p, s.lvl0.p, d_ncand, sample, d_out);
   return SCOUT_OK;
 }
 

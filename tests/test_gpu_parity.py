@@ -38,6 +38,10 @@ def data(tmp_path_factory):
     # degree-biased stream (hub endpoints)
     run_gen("stream", "degree", rm, root / "deg_b", 4, 2000, 13)
     d["deg"] = (str(rm), str(root / "deg_b"), 4)
+    # heavy deletions: many pieces split off vertex 0's component, later batches delete inside
+    # and insert between the unreached pieces
+    run_gen("stream", "tree", rm, root / "heavy_b", 8, 6000, 14, 0.9, 0.6)
+    d["heavy"] = (str(rm), str(root / "heavy_b"), 8)
     # the reference's bundled sample (deletion-only batches)
     sample = os.path.join(os.path.dirname(__file__), "golden", "sample")
     d["sample"] = (os.path.join(sample, "sample.egr"), os.path.join(sample, "batches"), 5)
@@ -56,7 +60,7 @@ def engine_for(egr):
     return eng, g
 
 
-@pytest.mark.parametrize("name", ["sample", "er", "rmat", "deg"])
+@pytest.mark.parametrize("name", ["sample", "er", "rmat", "deg", "heavy"])
 def test_bfs_cc_exact_every_batch(data, name):
     egr, bdir, nb = data[name]
     say(f"[bfs+cc] {name}")
@@ -80,6 +84,26 @@ def test_bfs_cc_exact_every_batch(data, name):
     off, col = og.logical_csr()
     dg = eng.download_graph()
     assert np.array_equal(dg.off, off) and np.array_equal(dg.col, col), "logical graph differs"
+    eng.close()
+
+
+@pytest.mark.parametrize("name", ["rmat", "heavy"])
+def test_cc_alone_exact_every_batch(data, name):
+    """CC repaired without BFS maintained on the context: the CC state keeps its own levels
+    from vertex 0 (and their parents) and repairs them batch by batch."""
+    egr, bdir, nb = data[name]
+    say(f"[cc alone] {name}")
+    eng, g = engine_for(egr)
+    og = OracleGraph(egr)
+    eng.upload(sa.STATE_LABEL, og.cc())
+    for k, uv in batches(bdir, nb, g.n):
+        eng.stage(uv)
+        ins = eng.classify(sa.CLASSIFY_CURRENT)
+        eng.apply(False)
+        og.apply(uv, og.classify(uv))
+        st = eng.repair_cc()
+        assert np.array_equal(eng.download(sa.STATE_LABEL), og.cc()), f"{name} batch {k}"
+        assert st.lines == len(ins)
     eng.close()
 
 
