@@ -23,7 +23,8 @@ struct Row {
 };
 
 __device__ __forceinline__ Row row_of(const GView& g, int32_t v) {
-  return Row{g.boff[v], g.boff[v + 1], g.aoff[v], g.aoff[v + 1]};
+  int64_t a0 = g.aoff[v];
+  return Row{g.boff[v], g.boff[v + 1], a0, a0 + g.alen[v]};
 }
 __device__ __forceinline__ int64_t row_len(const Row& r) { return (r.b1 - r.b0) + (r.a1 - r.a0); }
 // k-th physical entry of the row (base segment first, then the inserted segment). Negative
@@ -202,6 +203,35 @@ __device__ __forceinline__ void stage_finish(WaveStage<kSeg>& ws, int32_t* out, 
   __syncthreads();
 }
 
+// Block-wide reservation: every thread asks for `need` slots of a shared counter; returns the
+// start of this thread's slots. One global atomic per block.
+__device__ __forceinline__ int64_t block_reserve(int64_t need, int64_t* top) {
+  __shared__ int64_t wsum[kWavesPerBlock];
+  __shared__ int64_t gbase;
+  // inclusive scan within the wave
+  int64_t x = need;
+  for (int off = 1; off < kWave; off <<= 1) {
+    int64_t y = __shfl_up(x, off);
+    if (lane_id() >= off) x += y;
+  }
+  int w = (int)(threadIdx.x / kWave);
+  if (lane_id() == kWave - 1) wsum[w] = x;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int64_t t = 0;
+    for (int i = 0; i < kWavesPerBlock; i++) {
+      int64_t c = wsum[i];
+      wsum[i] = t;
+      t += c;
+    }
+    gbase = t ? (int64_t)atomicAdd((unsigned long long*)top, (unsigned long long)t) : 0;
+  }
+  __syncthreads();
+  int64_t pos = gbase + wsum[w] + x - need;
+  __syncthreads();
+  return pos;
+}
+
 // Per-thread append (when lanes of one wave are not converged on the same list).
 __device__ __forceinline__ void thread_append(int32_t value, int32_t* out, int64_t* count, int64_t cap) {
   int64_t pos = (int64_t)atomicAdd((unsigned long long*)count, 1ULL);
@@ -277,7 +307,7 @@ __device__ __forceinline__ PairState pair_state(const GView& g, bool sorted, int
     else if (x == ~v) s.base_tomb = true;
   }
   s.base_any = s.base_live || s.base_tomb;
-  int64_t ab = g.aoff[u], ae = g.aoff[u + 1];
+  int64_t ab = g.aoff[u], ae = ab + g.alen[u];
   for (int64_t k = ab; k < ae; k++) {
     int32_t x = g.acol[k];
     if (x == v) s.added_live = true;

@@ -344,7 +344,7 @@ int scout_graph_info(scout_ctx* x, int64_t* out) {
   Ctx& c = x->c;
   out[0] = c.g.n;
   out[1] = c.g.E;
-  out[2] = c.g.A;
+  out[2] = c.g.live;
   DevBuf<int64_t> deg, sum;
   SCOUT_CHECK(deg.reserve((int64_t)c.g.n + 1));
   SCOUT_CHECK(sum.reserve(1));

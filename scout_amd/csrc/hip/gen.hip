@@ -255,11 +255,7 @@ int gen_rmat(Ctx& c, int scale, int ef, double a, double b, double cc, uint64_t 
   (void)hipFree(bins);
   (void)hipFree(d_bin_bucket);
   g.E = E;
-  SCOUT_CHECK(g.aoff.reserve((int64_t)n + 1));
-  SCOUT_TRY(hipMemsetAsync(g.aoff.p, 0, ((size_t)n + 1) * 8, c.s));
-  SCOUT_CHECK(g.acol.reserve(1024));
-  g.A = 0;
-  g.holes = 0;
+  SCOUT_CHECK(reset_inserted(c));
   return check_rows(c);
 }
 

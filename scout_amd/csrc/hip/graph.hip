@@ -139,7 +139,7 @@ __device__ __forceinline__ void flip_base(const GView& g, const PairState& s, in
 // First inserted-row entry of u equal to x set to HOLE. True if one was found.
 __device__ __forceinline__ bool hole_first(const GView& g, int32_t u, int32_t x) {
   int32_t* col = const_cast<int32_t*>(g.acol);
-  for (int64_t k = g.aoff[u], e = g.aoff[u + 1]; k < e; k++) {
+  for (int64_t k = g.aoff[u], e = k + g.alen[u]; k < e; k++) {
     if (col[k] == x) {
       col[k] = HOLE;
       return true;
@@ -206,50 +206,142 @@ __global__ void k_apply(GView g, bool sorted, const int32_t* uv, const uint8_t* 
   block_add(holes, &box[1]);
 }
 
-// cnt_old[v] = entries of inserted row v that are not holes.
-__global__ void k_count_old(const int64_t* aoff, const int32_t* acol, int32_t n, int32_t* cnt_old, int32_t* cnt) {
-  int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (v > n) return;
-  if (v == n) {
-    cnt[n] = 0;
-    return;
-  }
-  int32_t c = 0;
-  for (int64_t k = aoff[v], e = aoff[v + 1]; k < e; k++) c += (acol[k] != HOLE);
-  cnt_old[v] = c;
-  cnt[v] = c;
+// --- inserted rows: appends into the slack pool ---------------------------------------
+
+// Group heads of the sorted new entries: a group is the run of one source vertex.
+__global__ void k_group_heads(const uint64_t* nkey, int64_t cnt, int32_t* heads, int64_t* nheads, int64_t cap) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  bool head = i < cnt && (i == 0 || (nkey[i] >> 32) != (nkey[i - 1] >> 32));
+  block_append(head, (int32_t)i, heads, nheads, cap);
 }
 
-__global__ void k_count_new(const uint64_t* nkey, int64_t cnt_new, int32_t* cnt) {
-  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < cnt_new) atomicAdd(&cnt[(int32_t)(nkey[k] >> 32)], 1);
+__device__ __forceinline__ int64_t group_end(const uint64_t* nkey, int64_t cnt, int64_t s) {
+  uint64_t src = nkey[s] >> 32;
+  int64_t e = s + 1;
+  while (e < cnt && (nkey[e] >> 32) == src) e++;
+  return e;
 }
 
-__global__ void k_copy_old(const int64_t* aoff, const int32_t* acol, int32_t n, const int64_t* aoff2, int32_t* acol2) {
-  int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= n) return;
-  int64_t o = aoff2[v];
-  for (int64_t k = aoff[v], e = aoff[v + 1]; k < e; k++) {
-    int32_t x = acol[k];
-    if (x != HOLE) acol2[o++] = x;
+__device__ __forceinline__ int32_t grown_capacity(int64_t len) {
+  int64_t c = 2 * len;
+  if (c < 4) c = 4;
+  return (int32_t)((c + 3) & ~3LL);
+}
+
+// Rows moved by one thread at most this long; longer ones by k_move_rows (wave per row).
+constexpr int64_t kMoveThread = 64;
+
+// Thread per group (source u, sorted new entries [s, e)): the new entries go after u's row
+// when its span has room; otherwise the row moves to fresh pool space of twice the needed
+// size (reserved with one atomic per block). The pool has room for the worst case (the host
+// checks before the launch); *overflow counts reservations past it all the same.
+__global__ void k_append_groups(int64_t* aoff, int32_t* alen, int32_t* acap, int32_t* acol, int32_t* aown,
+                                const uint64_t* nkey, const int32_t* ndst, int64_t cnt, const int32_t* heads,
+                                const int64_t* nheads, int64_t* top, int64_t pool_cap, int32_t* moves,
+                                int64_t* nmoves, int64_t* move_pos, int64_t* overflow) {
+  int64_t gi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t ng = *nheads;
+  int64_t need = 0, s = 0, e = 0;
+  int32_t u = -1, len = 0;
+  if (gi < ng) {
+    s = heads[gi];
+    e = group_end(nkey, cnt, s);
+    u = (int32_t)(nkey[s] >> 32);
+    len = alen[u];
+    if (len + (e - s) > acap[u]) need = grown_capacity(len + (e - s));
+  }
+  int64_t pos = block_reserve(need, top);
+  if (gi >= ng) return;
+  int64_t k = e - s;
+  if (need == 0) {
+    int64_t at = aoff[u] + len;
+    for (int64_t j = 0; j < k; j++) {
+      acol[at + j] = ndst[s + j];
+      aown[at + j] = u;
+    }
+    alen[u] = (int32_t)(len + k);
+  } else if (pos + need > pool_cap) {
+    atomicAdd((unsigned long long*)overflow, 1ULL);
+  } else if (len <= kMoveThread) {
+    int64_t old = aoff[u];
+    for (int64_t j = 0; j < len; j++) {
+      acol[pos + j] = acol[old + j];
+      aown[pos + j] = u;
+    }
+    for (int64_t j = 0; j < k; j++) {
+      acol[pos + len + j] = ndst[s + j];
+      aown[pos + len + j] = u;
+    }
+    aoff[u] = pos;
+    acap[u] = (int32_t)need;
+    alen[u] = (int32_t)(len + k);
+  } else {
+    int64_t slot = (int64_t)atomicAdd((unsigned long long*)nmoves, 1ULL);
+    moves[slot] = (int32_t)gi;
+    move_pos[gi] = pos;
   }
 }
 
-// Sorted new entries go after the surviving old entries, in line order.
-__global__ void k_place_new(const uint64_t* nkey, const int32_t* ndst, int64_t cnt_new, const int64_t* aoff2,
-                            const int32_t* cnt_old, int32_t* acol2) {
-  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= cnt_new) return;
-  uint64_t key = nkey[k];
-  int32_t src = (int32_t)(key >> 32);
-  uint64_t first = (uint64_t)(uint32_t)src << 32;
-  int64_t l = 0, h = k;
-  while (l < h) {
-    int64_t mid = (l + h) >> 1;
-    if (nkey[mid] < first) l = mid + 1;
-    else h = mid;
+// Wave per long row that moves: old entries, then the new ones, into the reserved span.
+__global__ void k_move_rows(int64_t* aoff, int32_t* alen, int32_t* acap, int32_t* acol, int32_t* aown,
+                            const uint64_t* nkey, const int32_t* ndst, int64_t cnt, const int32_t* heads,
+                            const int32_t* moves, const int64_t* nmoves, const int64_t* move_pos) {
+  int64_t nm = *nmoves;
+  for (int64_t i = global_wave(); i < nm; i += total_waves()) {
+    int32_t gi = moves[i];
+    int64_t s = heads[gi];
+    int64_t e = group_end(nkey, cnt, s);
+    int32_t u = (int32_t)(nkey[s] >> 32);
+    int64_t len = alen[u], k = e - s, old = aoff[u], pos = move_pos[gi];
+    for (int64_t j = lane_id(); j < len; j += kWave) {
+      acol[pos + j] = acol[old + j];
+      aown[pos + j] = u;
+    }
+    for (int64_t j = lane_id(); j < k; j += kWave) {
+      acol[pos + len + j] = ndst[s + j];
+      aown[pos + len + j] = u;
+    }
+    if (lane_id() == 0) {
+      aoff[u] = pos;
+      acap[u] = grown_capacity(len + k);
+      alen[u] = (int32_t)(len + k);
+    }
   }
-  acol2[aoff2[src] + cnt_old[src] + (k - l)] = ndst[k];
+}
+
+// --- inserted rows: full re-pack (holes dropped, fresh slack) --------------------------
+
+// Non-hole entries of every row; capacities with half again as slack. *live sums the counts.
+__global__ void k_repack_count(GView g, int32_t* cnt, int32_t* cap2, int64_t* live) {
+  int64_t c = 0;
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base <= g.n; base += (int64_t)gridDim.x * blockDim.x) {
+    int64_t v = base + threadIdx.x;
+    if (v < g.n) {
+      int64_t a0 = g.aoff[v], a1 = a0 + g.alen[v];
+      int32_t k = 0;
+      for (int64_t j = a0; j < a1; j++) k += g.acol[j] != HOLE;
+      cnt[v] = k;
+      cap2[v] = k ? (int32_t)((k + k / 2 + 4) & ~3) : 0;
+      c += k;
+    } else if (v == g.n) {
+      cap2[v] = 0;
+    }
+  }
+  block_add(c, live);
+}
+
+__global__ void k_repack_copy(GView g, const int64_t* aoff2, int32_t* acol2, int32_t* aown2) {
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < g.n; v += (int64_t)gridDim.x * blockDim.x) {
+    int64_t a0 = g.aoff[v], a1 = a0 + g.alen[v], o = aoff2[v];
+    for (int64_t j = a0; j < a1; j++) {
+      int32_t x = g.acol[j];
+      if (x != HOLE) {
+        acol2[o] = x;
+        aown2[o] = (int32_t)v;
+        o++;
+      }
+    }
+  }
 }
 
 // Live entries per vertex of the logical graph: thread per vertex for rows up to 64 physical
@@ -349,34 +441,80 @@ int upload_chunked(Ctx& c, void* dst, const void* src, size_t bytes) {
   return rc;
 }
 
-int rebuild_inserted(Ctx& c, int64_t cnt_new) {
+// Re-packs the inserted rows into a fresh pool: holes dropped, each row with half again its
+// size as slack, and room for `headroom` more slots. O(n + pool); runs when holes or moved-away
+// spans waste the pool, or the pool is full.
+int repack_inserted(Ctx& c, int64_t headroom) {
   DevGraph& g = c.g;
   int32_t n = g.n;
-  if (cnt_new > 0) {
-    // Stable by (src, line): the new entries of a row keep the batch's line order.
-    SCOUT_CHECK(g.nkey2.reserve(cnt_new));
-    SCOUT_CHECK(g.ndst2.reserve(cnt_new));
-    SCOUT_CHECK(sort_pairs_u64(c, g.nkey.p, g.nkey2.p, g.ndst.p, g.ndst2.p, cnt_new, 64));
-    std::swap(g.nkey, g.nkey2);
-    std::swap(g.ndst, g.ndst2);
-  }
-  SCOUT_CHECK(g.cnt.reserve((int64_t)n + 1));
-  SCOUT_CHECK(g.cnt_old.reserve((int64_t)n + 1));
-  k_count_old<<<grid_for((int64_t)n + 1), kBlock, 0, c.s>>>(g.aoff.p, g.acol.p, n, g.cnt_old.p, g.cnt.p);
-  if (cnt_new > 0) k_count_new<<<grid_for(cnt_new), kBlock, 0, c.s>>>(g.nkey.p, cnt_new, g.cnt.p);
+  SCOUT_CHECK(g.acap2.reserve((int64_t)n + 1));
   SCOUT_CHECK(g.aoff2.reserve((int64_t)n + 1));
-  SCOUT_CHECK(scan_exclusive_i32_to_i64(c, g.cnt.p, g.aoff2.p, (int64_t)n + 1));
+  SCOUT_TRY(hipMemsetAsync(c.d_box + 32, 0, sizeof(int64_t), c.s));
+  // counts land in alen (read by the copy through the old aoff... so stage them in moves)
+  SCOUT_CHECK(g.moves.reserve((int64_t)n + 1));
+  k_repack_count<<<kListGrid, kBlock, 0, c.s>>>(g.view(), g.moves.p, g.acap2.p, c.d_box + 32);
+  SCOUT_CHECK(scan_exclusive_i32_to_i64(c, g.acap2.p, g.aoff2.p, (int64_t)n + 1));
   int64_t total = 0;
   SCOUT_TRY(hipMemcpyAsync(&total, g.aoff2.p + n, sizeof(int64_t), hipMemcpyDeviceToHost, c.s));
+  SCOUT_TRY(hipMemcpyAsync(c.h_box + 32, c.d_box + 32, sizeof(int64_t), hipMemcpyDeviceToHost, c.s));
   SCOUT_TRY(hipStreamSynchronize(c.s));
-  SCOUT_CHECK(g.acol2.reserve(total + 1));
-  k_copy_old<<<grid_for(n), kBlock, 0, c.s>>>(g.aoff.p, g.acol.p, n, g.aoff2.p, g.acol2.p);
-  if (cnt_new > 0)
-    k_place_new<<<grid_for(cnt_new), kBlock, 0, c.s>>>(g.nkey.p, g.ndst.p, cnt_new, g.aoff2.p, g.cnt_old.p, g.acol2.p);
+  c.readbacks++;
+  SCOUT_CHECK(g.acol2.reserve(total + headroom + 1024));
+  SCOUT_CHECK(g.aown2.reserve(total + headroom + 1024));
+  k_repack_copy<<<kListGrid, kBlock, 0, c.s>>>(g.view(), g.aoff2.p, g.acol2.p, g.aown2.p);
+  SCOUT_TRY(hipMemcpyAsync(g.alen.p, g.moves.p, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToDevice, c.s));
   std::swap(g.aoff, g.aoff2);
+  std::swap(g.acap, g.acap2);
   std::swap(g.acol, g.acol2);
+  std::swap(g.aown, g.aown2);
   g.A = total;
+  g.live = c.h_box[32];
   g.holes = 0;
+  return SCOUT_OK;
+}
+
+// Slots usable in the pool: both the entry and the owner arrays hold them.
+int64_t pool_cap(const DevGraph& g) { return std::min(g.acol.cap, g.aown.cap); }
+
+// Appends this batch's new entries (nkey/ndst, cnt_new of them) to their rows.
+int insert_new(Ctx& c, int64_t cnt_new) {
+  DevGraph& g = c.g;
+  // Stable by (src, line): the new entries of a row keep the batch's line order.
+  SCOUT_CHECK(g.nkey2.reserve(cnt_new));
+  SCOUT_CHECK(g.ndst2.reserve(cnt_new));
+  SCOUT_CHECK(sort_pairs_u64(c, g.nkey.p, g.nkey2.p, g.ndst.p, g.ndst2.p, cnt_new, 64));
+  std::swap(g.nkey, g.nkey2);
+  std::swap(g.ndst, g.ndst2);
+  // Worst case: every touched row moves to twice its new size, and the rows hold at most the
+  // pool's used part: 2 * (A + cnt_new) + 4 per group <= 2 * A + 6 * cnt_new.
+  int64_t worst = 2 * g.A + 6 * cnt_new + 1024;
+  if (g.A + worst > pool_cap(g)) {
+    if (g.A > 2 * g.live + (1 << 20)) SCOUT_CHECK(repack_inserted(c, 2 * g.live + 6 * cnt_new));
+    worst = 2 * g.A + 6 * cnt_new + 1024;
+    if (g.A + worst > pool_cap(g)) SCOUT_CHECK(repack_inserted(c, 2 * (g.A + worst)));
+  }
+  SCOUT_CHECK(g.heads.reserve(cnt_new));
+  SCOUT_CHECK(g.moves.reserve(cnt_new));
+  SCOUT_CHECK(g.move_pos.reserve(cnt_new));
+  // box: [2] groups, [3] moved rows, [4] pool top, [5] overflow
+  int64_t init[4] = {0, 0, g.A, 0};
+  SCOUT_TRY(hipMemcpyAsync(c.d_box + 2, init, sizeof(init), hipMemcpyHostToDevice, c.s));
+  k_group_heads<<<grid_for(cnt_new), kBlock, 0, c.s>>>(g.nkey.p, cnt_new, g.heads.p, c.d_box + 2, g.heads.cap);
+  k_append_groups<<<grid_for(cnt_new), kBlock, 0, c.s>>>(g.aoff.p, g.alen.p, g.acap.p, g.acol.p, g.aown.p, g.nkey.p,
+                                                         g.ndst.p, cnt_new, g.heads.p, c.d_box + 2, c.d_box + 4,
+                                                         pool_cap(g), g.moves.p, c.d_box + 3, g.move_pos.p,
+                                                         c.d_box + 5);
+  k_move_rows<<<kListGrid, kBlock, 0, c.s>>>(g.aoff.p, g.alen.p, g.acap.p, g.acol.p, g.aown.p, g.nkey.p, g.ndst.p,
+                                             cnt_new, g.heads.p, g.moves.p, c.d_box + 3, g.move_pos.p);
+  SCOUT_TRY(hipMemcpyAsync(c.h_box + 2, c.d_box + 2, 4 * sizeof(int64_t), hipMemcpyDeviceToHost, c.s));
+  SCOUT_TRY(hipStreamSynchronize(c.s));
+  c.readbacks++;
+  if (c.h_box[5] != 0) {
+    set_error("apply: inserted-row pool overflow");
+    return SCOUT_ERR_STATE;
+  }
+  g.A = c.h_box[4];
+  g.live += cnt_new;
   return SCOUT_OK;
 }
 
@@ -400,7 +538,8 @@ int apply_range(Ctx& c, Channel& ch, int64_t lo, int64_t hi, int conditional) {
     set_error("apply: inserted-entry staging overflow");
     return SCOUT_ERR_STATE;
   }
-  if (cnt_new > 0 || g.holes > (g.A >> 1) + 1024) SCOUT_CHECK(rebuild_inserted(c, cnt_new));
+  if (cnt_new > 0) SCOUT_CHECK(insert_new(c, cnt_new));
+  if (g.holes > (g.live >> 1) + (1 << 20)) SCOUT_CHECK(repack_inserted(c, 2 * g.live + (1 << 20)));
   return SCOUT_OK;
 }
 
@@ -414,18 +553,39 @@ int graph_free(Ctx& c) {
   g.bcol = nullptr;
   g.aoff.release();
   g.aoff2.release();
+  g.alen.release();
+  g.acap.release();
+  g.acap2.release();
   g.acol.release();
   g.acol2.release();
+  g.aown.release();
+  g.aown2.release();
+  g.heads.release();
+  g.moves.release();
+  g.move_pos.release();
   g.nkey.release();
   g.nkey2.release();
   g.ndst.release();
   g.ndst2.release();
-  g.cnt.release();
-  g.cnt_old.release();
   g.hkey.release();
   g.hline.release();
   g.n = 0;
-  g.E = g.A = g.holes = 0;
+  g.E = g.A = g.live = g.holes = 0;
+  return SCOUT_OK;
+}
+
+int reset_inserted(Ctx& c) {
+  DevGraph& g = c.g;
+  size_t n1 = (size_t)g.n + 1;
+  SCOUT_CHECK(g.aoff.reserve((int64_t)n1));
+  SCOUT_CHECK(g.alen.reserve((int64_t)n1));
+  SCOUT_CHECK(g.acap.reserve((int64_t)n1));
+  SCOUT_TRY(hipMemsetAsync(g.aoff.p, 0, n1 * sizeof(int64_t), c.s));
+  SCOUT_TRY(hipMemsetAsync(g.alen.p, 0, n1 * sizeof(int32_t), c.s));
+  SCOUT_TRY(hipMemsetAsync(g.acap.p, 0, n1 * sizeof(int32_t), c.s));
+  SCOUT_CHECK(g.acol.reserve(1 << 16));
+  SCOUT_CHECK(g.aown.reserve(g.acol.cap));
+  g.A = g.live = g.holes = 0;
   return SCOUT_OK;
 }
 
@@ -481,11 +641,7 @@ int graph_upload_egr(Ctx& c, int32_t n, int64_t E, const void* nindex, int wide,
     set_error("graph_load: nindex[0] must be 0 and nindex[n] the entry count");
     return SCOUT_ERR_ARG;
   }
-  SCOUT_CHECK(g.aoff.reserve((int64_t)n + 1));
-  SCOUT_TRY(hipMemsetAsync(g.aoff.p, 0, ((size_t)n + 1) * sizeof(int64_t), c.s));
-  SCOUT_CHECK(g.acol.reserve(1024));
-  g.A = 0;
-  g.holes = 0;
+  SCOUT_CHECK(reset_inserted(c));
   return check_rows(c);
 }
 
@@ -607,9 +763,7 @@ int graph_compact(Ctx& c) {
   off2.p = nullptr;
   col2.p = nullptr;
   g.E = total;
-  SCOUT_TRY(hipMemsetAsync(g.aoff.p, 0, ((size_t)g.n + 1) * sizeof(int64_t), c.s));
-  g.A = 0;
-  g.holes = 0;
+  SCOUT_CHECK(reset_inserted(c));
   return check_rows(c);
 }
 

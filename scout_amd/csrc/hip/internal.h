@@ -3,9 +3,15 @@
 // HBM layout of the dynamic graph (one copy per context):
 //   boff[n+1] int64, bcol[E] int32   base CSR; a deleted base entry keeps its slot and is
 //                                    stored as ~v (negative), so rows never move
-//   aoff[n+1] int64, acol[A] int32   inserted edges per vertex in insertion order; an entry
-//                                    is v (live), ~v (a tombstone on a pair the base row does
-//                                    not hold) or HOLE (a forgotten insertion)
+//   aoff[n] int64, alen/acap[n] int32, acol/aown[pool] int32
+//                                    inserted edges of v in insertion order at
+//                                    acol[aoff[v] .. aoff[v] + alen[v]), inside a reserved
+//                                    span of acap[v] slots of a bump-allocated pool; aown
+//                                    names the vertex a pool slot was written for. An entry is
+//                                    v (live), ~v (a tombstone on a pair the base row does not
+//                                    hold) or HOLE (a forgotten insertion). A row that
+//                                    outgrows its span moves to fresh pool space of twice the
+//                                    size, so a batch costs O(its edits), not O(n).
 // The logical row of v is the live entries of its base row, then the live entries of its
 // inserted row -- the order of core/FlashGraph.h:246-265.
 #pragma once
@@ -97,6 +103,7 @@ struct GView {
   const int64_t* boff;
   const int32_t* bcol;
   const int64_t* aoff;
+  const int32_t* alen;
   const int32_t* acol;
   int32_t n;
 };
@@ -104,21 +111,25 @@ struct GView {
 struct DevGraph {
   int32_t n = 0;
   int64_t E = 0;        // base entries
-  int64_t A = 0;        // inserted-row entries (aoff[n]), holes included
-  int64_t holes = 0;    // HOLE entries not yet compacted away
+  int64_t A = 0;        // pool slots handed out (rows, their slack and moved-away spans)
+  int64_t live = 0;     // entries inside rows (sum of alen), holes included
+  int64_t holes = 0;    // HOLE entries inside rows
   bool sorted = false;  // every base row ascending (by decoded id)
   int64_t* boff = nullptr;
   int32_t* bcol = nullptr;
   DevBuf<int64_t> aoff, aoff2;
-  DevBuf<int32_t> acol, acol2;
+  DevBuf<int32_t> alen, acap, acap2;
+  DevBuf<int32_t> acol, acol2, aown, aown2;
+  // groups of this batch's new entries (one per source vertex) and moved rows
+  DevBuf<int32_t> heads, moves;
+  DevBuf<int64_t> move_pos;
   // new inserted-row entries of the current batch: key = src << 32 | line
   DevBuf<uint64_t> nkey, nkey2;
   DevBuf<int32_t> ndst, ndst2;
-  DevBuf<int32_t> cnt, cnt_old;  // n+1 per-vertex counts
   // duplicate-pair detection of the current batch
   DevBuf<uint64_t> hkey;
   DevBuf<int32_t> hline;
-  GView view() const { return GView{boff, bcol, aoff.p, acol.p, n}; }
+  GView view() const { return GView{boff, bcol, aoff.p, alen.p, acol.p, n}; }
 };
 
 // Per-batch edit staging: a channel holds one uploaded batch.
@@ -209,6 +220,8 @@ int graph_logical_degrees(Ctx& c, int64_t* d_deg);
 int graph_download(Ctx& c, int64_t* h_off, int32_t* h_col);
 int graph_compact(Ctx& c);
 int graph_free(Ctx& c);
+// Empties the inserted rows (after a load, a generated graph or a compaction).
+int reset_inserted(Ctx& c);
 
 // per-problem modules
 int bfs_full(Ctx& c, int32_t src, int64_t* rounds, int64_t* scanned);

@@ -253,7 +253,8 @@ __global__ void k_mis_clear_minw(int64_t p, int64_t hi, const int32_t* w0s, cons
 __global__ void k_mis_degrees(GView g, const int32_t* uv, int64_t m, int64_t* deg) {
   for (int64_t w = global_wave(); w < 2 * m; w += total_waves()) {
     int32_t x = uv[w];
-    int64_t b0 = g.boff[x], b1 = g.boff[x + 1], a0 = g.aoff[x], a1 = g.aoff[x + 1];
+    Row row = row_of(g, x);
+    int64_t b0 = row.b0, b1 = row.b1, a0 = row.a0, a1 = row.a1;
     int64_t removed = 0, added = 0;
     for (int64_t k = b0 + lane_id(); k < b1; k += kWave) removed += (g.bcol[k] < 0);
     for (int64_t k = a0 + lane_id(); k < a1; k += kWave) {

@@ -75,26 +75,37 @@ __global__ void k_gen_tree(GView g, const int32_t* level, uint64_t key, uint64_t
   }
 }
 
-// Uniform deletions: a uniform physical adjacency position (base, then inserted rows); the
-// draw is kept if the entry is live.
-__global__ void k_gen_del(GView g, int64_t E, int64_t A, uint64_t key, uint64_t base, int64_t draws, uint64_t* set,
-                          uint64_t mask, int32_t* out, int64_t* count, int64_t limit) {
+// Uniform deletions: a uniform position over the base entries and the inserted-row pool; the
+// draw is kept if it holds a live entry of a row (rejection keeps the choice uniform over the
+// live adjacency entries).
+__global__ void k_gen_del(GView g, int64_t E, int64_t A, const int32_t* aown, uint64_t key, uint64_t base,
+                          int64_t draws, uint64_t* set, uint64_t mask, int32_t* out, int64_t* count, int64_t limit) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= draws || *count >= limit) return;
   uint64_t x = draw(key, base + (uint64_t)i);
   uint64_t total = (uint64_t)(E + A);
   uint64_t pos = __umul64hi(x, total);
-  const int64_t* off = pos < (uint64_t)E ? g.boff : g.aoff;
-  int64_t p = pos < (uint64_t)E ? (int64_t)pos : (int64_t)pos - E;
-  int32_t w = pos < (uint64_t)E ? g.bcol[p] : g.acol[p];
-  if (w < 0) return;
-  int64_t l = 0, h = g.n;  // last vertex whose offset <= p
-  while (l < h) {
-    int64_t mid = (l + h + 1) >> 1;
-    if (off[mid] <= p) l = mid;
-    else h = mid - 1;
+  int32_t w;
+  uint32_t u;
+  if (pos < (uint64_t)E) {
+    int64_t p = (int64_t)pos;
+    w = g.bcol[p];
+    if (w < 0) return;
+    int64_t l = 0, h = g.n;  // last vertex whose offset <= p
+    while (l < h) {
+      int64_t mid = (l + h + 1) >> 1;
+      if (g.boff[mid] <= p) l = mid;
+      else h = mid - 1;
+    }
+    u = (uint32_t)l;
+  } else {
+    int64_t p = (int64_t)pos - E;
+    w = g.acol[p];
+    int32_t o = aown[p];
+    if (w < 0 || o < 0 || o >= g.n) return;
+    if (p < g.aoff[o] || p >= g.aoff[o] + g.alen[o]) return;  // slack or a moved-away span
+    u = (uint32_t)o;
   }
-  uint32_t u = (uint32_t)l;
   if (u == (uint32_t)w) return;
   if (claim_pair(set, mask, u, (uint32_t)w)) emit(u, (uint32_t)w, out, count, limit);
 }
@@ -159,8 +170,9 @@ int gen_batch(Ctx& c, Channel& ch, int64_t size, double delete_frac, double tree
         k_gen_tree<<<grid_for(draws), kBlock, 0, c.s>>>(c.g.view(), c.level, keys[0], s.draws_used[0], draws, s.set.p,
                                                         s.mask, out, c.d_box, want[kind]);
       else if (kind == 1)
-        k_gen_del<<<grid_for(draws), kBlock, 0, c.s>>>(c.g.view(), c.g.E, c.g.A, keys[1], s.draws_used[1], draws, s.set.p,
-                                                       s.mask, out, c.d_box, want[kind]);
+        k_gen_del<<<grid_for(draws), kBlock, 0, c.s>>>(c.g.view(), c.g.E, c.g.A, c.g.aown.p, keys[1],
+                                                       s.draws_used[1], draws, s.set.p, s.mask, out, c.d_box,
+                                                       want[kind]);
       else
         k_gen_ins<<<grid_for(draws), kBlock, 0, c.s>>>(c.g.view(), c.g.sorted, keys[2], s.draws_used[2], draws, s.set.p,
                                                        s.mask, out, c.d_box, want[kind]);
@@ -232,7 +244,8 @@ void stream_release(Ctx& c) {
 // Device copies of the graph and every allocated solution, for replaying a stream.
 struct Snapshot {
   DevBuf<int64_t> boff, aoff;
-  DevBuf<int32_t> bcol, acol;
+  DevBuf<int32_t> bcol, acol, aown, alen, acap;
+  int64_t live = 0;
   DevBuf<int32_t> level, parent, label, color;
   DevBuf<uint8_t> mis;
   int64_t E = 0, A = 0, holes = 0;
@@ -248,12 +261,21 @@ int snapshot_save(Ctx& c) {
   SCOUT_CHECK(s.aoff.reserve(n + 1));
   SCOUT_CHECK(s.bcol.reserve(c.g.E + 1));
   SCOUT_CHECK(s.acol.reserve(c.g.A + 1));
+  SCOUT_CHECK(s.aown.reserve(c.g.A + 1));
+  SCOUT_CHECK(s.alen.reserve(n + 1));
+  SCOUT_CHECK(s.acap.reserve(n + 1));
   SCOUT_TRY(hipMemcpyAsync(s.boff.p, c.g.boff, (size_t)(n + 1) * 8, hipMemcpyDeviceToDevice, c.s));
   SCOUT_TRY(hipMemcpyAsync(s.aoff.p, c.g.aoff.p, (size_t)(n + 1) * 8, hipMemcpyDeviceToDevice, c.s));
+  SCOUT_TRY(hipMemcpyAsync(s.alen.p, c.g.alen.p, (size_t)(n + 1) * 4, hipMemcpyDeviceToDevice, c.s));
+  SCOUT_TRY(hipMemcpyAsync(s.acap.p, c.g.acap.p, (size_t)(n + 1) * 4, hipMemcpyDeviceToDevice, c.s));
   SCOUT_TRY(hipMemcpyAsync(s.bcol.p, c.g.bcol, (size_t)c.g.E * 4, hipMemcpyDeviceToDevice, c.s));
-  if (c.g.A > 0) SCOUT_TRY(hipMemcpyAsync(s.acol.p, c.g.acol.p, (size_t)c.g.A * 4, hipMemcpyDeviceToDevice, c.s));
+  if (c.g.A > 0) {
+    SCOUT_TRY(hipMemcpyAsync(s.acol.p, c.g.acol.p, (size_t)c.g.A * 4, hipMemcpyDeviceToDevice, c.s));
+    SCOUT_TRY(hipMemcpyAsync(s.aown.p, c.g.aown.p, (size_t)c.g.A * 4, hipMemcpyDeviceToDevice, c.s));
+  }
   s.E = c.g.E;
   s.A = c.g.A;
+  s.live = c.g.live;
   s.holes = c.g.holes;
   s.sorted = c.g.sorted;
   auto save32 = [&](DevBuf<int32_t>& d, const int32_t* src, bool& has) -> int {
@@ -292,10 +314,17 @@ int snapshot_restore(Ctx& c) {
   SCOUT_TRY(hipMemcpyAsync(c.g.boff, s.boff.p, (size_t)(n + 1) * 8, hipMemcpyDeviceToDevice, c.s));
   SCOUT_TRY(hipMemcpyAsync(c.g.aoff.p, s.aoff.p, (size_t)(n + 1) * 8, hipMemcpyDeviceToDevice, c.s));
   SCOUT_TRY(hipMemcpyAsync(c.g.bcol, s.bcol.p, (size_t)s.E * 4, hipMemcpyDeviceToDevice, c.s));
+  SCOUT_TRY(hipMemcpyAsync(c.g.alen.p, s.alen.p, (size_t)(n + 1) * 4, hipMemcpyDeviceToDevice, c.s));
+  SCOUT_TRY(hipMemcpyAsync(c.g.acap.p, s.acap.p, (size_t)(n + 1) * 4, hipMemcpyDeviceToDevice, c.s));
   SCOUT_CHECK(c.g.acol.reserve(s.A + 1));
-  if (s.A > 0) SCOUT_TRY(hipMemcpyAsync(c.g.acol.p, s.acol.p, (size_t)s.A * 4, hipMemcpyDeviceToDevice, c.s));
+  SCOUT_CHECK(c.g.aown.reserve(c.g.acol.cap));
+  if (s.A > 0) {
+    SCOUT_TRY(hipMemcpyAsync(c.g.acol.p, s.acol.p, (size_t)s.A * 4, hipMemcpyDeviceToDevice, c.s));
+    SCOUT_TRY(hipMemcpyAsync(c.g.aown.p, s.aown.p, (size_t)s.A * 4, hipMemcpyDeviceToDevice, c.s));
+  }
   c.g.E = s.E;
   c.g.A = s.A;
+  c.g.live = s.live;
   c.g.holes = s.holes;
   c.g.sorted = s.sorted;
   if (s.has_level) {
@@ -326,6 +355,9 @@ void snapshot_release(Ctx& c) {
   s.aoff.release();
   s.bcol.release();
   s.acol.release();
+  s.aown.release();
+  s.alen.release();
+  s.acap.release();
   s.level.release();
   s.parent.release();
   s.label.release();
