@@ -177,6 +177,18 @@ int scout_batch_download(scout_ctx* ctx, int channel, int32_t* uv, int64_t cap_l
  * receives the loaded batch's line count, the size scout_classify's ins must have. */
 int scout_batch_store(scout_ctx* ctx, int channel, int slot);
 int scout_batch_load(scout_ctx* ctx, int channel, int slot, int64_t* m);
+/* Appends stored batch `slot` after the lines already staged in the channel (a serving step
+ * that takes several queued requests at once). *m (may be NULL) receives the total. No
+ * counterpart in the reference, which processes one batch file at a time. */
+int scout_batch_append(scout_ctx* ctx, int channel, int slot, int64_t* m);
+/* Classifies and applies the staged lines as nseg consecutive segments [seg_off[k],
+ * seg_off[k+1]) in order, each against the graph the earlier segments left: the same graph
+ * and labels as staging the segments as separate batches (parse_batch_file +
+ * processBatch's apply loop, core/cli_setup.h:87-103, per batch). The repairs that follow
+ * treat all segments as one batch, which gives the same solution as repairing after each.
+ * ins (may be NULL) receives every line's label. */
+int scout_apply_segments(scout_ctx* ctx, int channel, int32_t nseg, const int64_t* seg_off, int rule,
+                         int conditional, uint8_t* ins);
 /* Device copies of the graph and solutions, to replay one stream under several strategies. */
 int scout_snapshot(scout_ctx* ctx);
 int scout_restore(scout_ctx* ctx);

@@ -82,6 +82,8 @@ _SIGNATURES = {
     "scout_prof_enable": [_P, ctypes.c_int],
     "scout_prof_read": [_P, ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_I64),
                         ctypes.POINTER(_I64)],
+    "scout_batch_append": [_P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_I64)],
+    "scout_apply_segments": [_P, ctypes.c_int, _I32, _P, ctypes.c_int, ctypes.c_int, _P],
     "scout_mlp_init": [_P, ctypes.c_uint64],
     "scout_mlp_set_params": [_P, _P],
     "scout_mlp_get_params": [_P, _P, ctypes.c_int],
@@ -292,6 +294,22 @@ class Engine:
     def batch_download(self, m, channel=0) -> np.ndarray:
         out = np.zeros((max(m, 1), 2), dtype=np.int32)
         self._check(self.lib.scout_batch_download(self.h, channel, _ptr(out), m), "batch_download")
+        return out[:m]
+
+    def batch_append(self, slot, channel=0):
+        m = ctypes.c_int64()
+        self._check(self.lib.scout_batch_append(self.h, channel, slot, ctypes.byref(m)), "batch_append")
+        self._staged[channel] = m.value
+        return m.value
+
+    def apply_segments(self, seg_off, rule: int = CLASSIFY_CURRENT, conditional: bool = False, channel: int = 0):
+        """Classifies and applies the staged lines as consecutive segments, each against the
+        graph the earlier ones left; returns every line's label."""
+        off = np.ascontiguousarray(seg_off, dtype=np.int64)
+        m = self._staged[channel]
+        out = np.zeros(max(m, 1), dtype=np.uint8)
+        self._check(self.lib.scout_apply_segments(self.h, channel, len(off) - 1, _ptr(off), rule,
+                                                  1 if conditional else 0, _ptr(out)), "apply_segments")
         return out[:m]
 
     def batch_store(self, slot, channel=0):

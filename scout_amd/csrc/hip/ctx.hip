@@ -514,6 +514,38 @@ int scout_apply(scout_ctx* x, int channel, int conditional) {
   return graph_apply(c, *ch, conditional);
 }
 
+int scout_apply_segments(scout_ctx* x, int channel, int32_t nseg, const int64_t* seg_off, int rule,
+                         int conditional, uint8_t* ins) {
+  SCOUT_CHECK(check_graph(x));
+  Ctx& c = x->c;
+  Channel* ch = channel_of(c, channel);
+  if (!ch || nseg < 0 || (nseg > 0 && !seg_off)) return SCOUT_ERR_ARG;
+  if (rule != SCOUT_CLASSIFY_CURRENT && rule != SCOUT_CLASSIFY_BASE) {
+    set_error("apply_segments: unknown rule");
+    return SCOUT_ERR_ARG;
+  }
+  if (nseg > 0 && (seg_off[0] != 0 || seg_off[nseg] != ch->m)) {
+    set_error("apply_segments: segments must cover the staged lines from 0 to the line count");
+    return SCOUT_ERR_ARG;
+  }
+  for (int32_t k = 0; k < nseg; k++)
+    if (seg_off[k] > seg_off[k + 1]) {
+      set_error("apply_segments: segment offsets must not decrease");
+      return SCOUT_ERR_ARG;
+    }
+  SCOUT_TRY(hipStreamWaitEvent(c.s, ch->staged, 0));
+  c.apply_seq++;  // one commit: the repairs that follow see one batch of all segments
+  for (int32_t k = 0; k < nseg; k++) {
+    SCOUT_CHECK(graph_classify_lines(c, *ch, seg_off[k], seg_off[k + 1], rule));
+    SCOUT_CHECK(graph_apply_lines(c, *ch, seg_off[k], seg_off[k + 1], conditional));
+  }
+  if (ins && ch->m > 0) {
+    SCOUT_TRY(hipMemcpyAsync(ins, ch->d_ins.p, (size_t)ch->m, hipMemcpyDeviceToHost, c.s));
+    SCOUT_TRY(hipStreamSynchronize(c.s));
+  }
+  return SCOUT_OK;
+}
+
 int scout_prof_enable(scout_ctx* x, int on) {
   SCOUT_CHECK(check_ctx(x));
   Ctx& c = x->c;

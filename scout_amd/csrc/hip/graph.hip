@@ -649,28 +649,32 @@ int graph_upload(Ctx& c, int32_t n, int64_t E, const int64_t* off, const int32_t
   return graph_upload_egr(c, n, E, off, 1, col);
 }
 
-int graph_classify(Ctx& c, Channel& ch, int rule) {
+int graph_classify_lines(Ctx& c, Channel& ch, int64_t lo, int64_t hi, int rule) {
   if (!c.g.boff) {
     set_error("classify: no graph loaded");
     return SCOUT_ERR_STATE;
   }
+  int64_t m = hi - lo;
   SCOUT_TRY(hipMemsetAsync(c.d_box, 0, sizeof(int64_t), c.s));
   SCOUT_CHECK(prof_begin(c, PROF_CLASSIFY));
-  if (ch.m > 0)
-    k_classify<<<grid_for(ch.m), kBlock, 0, c.s>>>(c.g.view(), c.g.sorted, ch.d_uv.p, ch.m, rule, ch.d_ins.p,
-                                                   c.d_box);
+  if (m > 0)
+    k_classify<<<grid_for(m), kBlock, 0, c.s>>>(c.g.view(), c.g.sorted, ch.d_uv.p + 2 * lo, m, rule, ch.d_ins.p + lo,
+                                                c.d_box);
   // per line: the pair (8 B), row u's base and inserted offsets (16 B), the located entry
   // (4 B) and the label written (1 B)
-  SCOUT_CHECK(prof_end(c, PROF_CLASSIFY, 29 * ch.m));
+  SCOUT_CHECK(prof_end(c, PROF_CLASSIFY, 29 * m));
   return SCOUT_OK;
 }
 
-int graph_apply(Ctx& c, Channel& ch, int conditional) {
+int graph_classify(Ctx& c, Channel& ch, int rule) { return graph_classify_lines(c, ch, 0, ch.m, rule); }
+
+// Applies lines [lo, hi) of the staged batch; pairs repeated inside the range are applied as
+// runs of distinct pairs, in line order.
+int graph_apply_lines(Ctx& c, Channel& ch, int64_t lo, int64_t hi, int conditional) {
   DevGraph& g = c.g;
-  int64_t m = ch.m;
-  c.apply_seq++;
-  if (m == 0) return SCOUT_OK;
-  // Find lines whose pair already occurred earlier in the batch.
+  int64_t m = hi - lo;
+  if (m <= 0) return SCOUT_OK;
+  // Find lines whose pair already occurred earlier in the range.
   uint64_t size = 1;
   while (size < (uint64_t)(4 * m)) size <<= 1;
   SCOUT_CHECK(g.hkey.reserve((int64_t)size));
@@ -678,26 +682,32 @@ int graph_apply(Ctx& c, Channel& ch, int conditional) {
   SCOUT_TRY(hipMemsetAsync(g.hkey.p, 0xff, size * sizeof(uint64_t), c.s));
   SCOUT_TRY(hipMemsetAsync(g.hline.p, 0x7f, size * sizeof(int32_t), c.s));
   SCOUT_TRY(hipMemsetAsync(c.d_box, 0, sizeof(int64_t), c.s));
-  k_dup_insert<<<grid_for(m), kBlock, 0, c.s>>>(ch.d_uv.p, m, g.hkey.p, g.hline.p, size - 1);
-  k_dup_mark<<<grid_for(m), kBlock, 0, c.s>>>(ch.d_uv.p, m, g.hkey.p, g.hline.p, size - 1, ch.d_done.p, c.d_box);
+  k_dup_insert<<<grid_for(m), kBlock, 0, c.s>>>(ch.d_uv.p + 2 * lo, m, g.hkey.p, g.hline.p, size - 1);
+  k_dup_mark<<<grid_for(m), kBlock, 0, c.s>>>(ch.d_uv.p + 2 * lo, m, g.hkey.p, g.hline.p, size - 1, ch.d_done.p + lo,
+                                              c.d_box);
   SCOUT_CHECK(box_read(c, 1));
-  if (c.h_box[0] == 0) return apply_range(c, ch, 0, m, conditional);
-  // Repeated pairs: cut the batch into runs of distinct pairs, applied in order.
+  if (c.h_box[0] == 0) return apply_range(c, ch, lo, hi, conditional);
+  // Repeated pairs: cut the range into runs of distinct pairs, applied in order.
   std::vector<int32_t> uv((size_t)(2 * m));
-  SCOUT_TRY(hipMemcpy(uv.data(), ch.d_uv.p, (size_t)(2 * m) * sizeof(int32_t), hipMemcpyDeviceToHost));
+  SCOUT_TRY(hipMemcpy(uv.data(), ch.d_uv.p + 2 * lo, (size_t)(2 * m) * sizeof(int32_t), hipMemcpyDeviceToHost));
   std::unordered_set<uint64_t> seen;
-  int64_t start = 0;
+  int64_t start = lo;
   for (int64_t i = 0; i < m; i++) {
     uint32_t a = (uint32_t)std::min(uv[2 * i], uv[2 * i + 1]), b = (uint32_t)std::max(uv[2 * i], uv[2 * i + 1]);
     uint64_t key = ((uint64_t)a << 32) | b;
     if (seen.count(key)) {
-      SCOUT_CHECK(apply_range(c, ch, start, i, conditional));
+      SCOUT_CHECK(apply_range(c, ch, start, lo + i, conditional));
       seen.clear();
-      start = i;
+      start = lo + i;
     }
     seen.insert(key);
   }
-  return apply_range(c, ch, start, m, conditional);
+  return apply_range(c, ch, start, hi, conditional);
+}
+
+int graph_apply(Ctx& c, Channel& ch, int conditional) {
+  c.apply_seq++;
+  return graph_apply_lines(c, ch, 0, ch.m, conditional);
 }
 
 int graph_logical_degrees(Ctx& c, int64_t* d_deg) {

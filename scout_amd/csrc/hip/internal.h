@@ -216,6 +216,8 @@ int graph_upload(Ctx& c, int32_t n, int64_t E, const int64_t* off, const int32_t
 int graph_upload_egr(Ctx& c, int32_t n, int64_t E, const void* nindex, int wide, const int32_t* nlist);
 int graph_classify(Ctx& c, Channel& ch, int rule);
 int graph_apply(Ctx& c, Channel& ch, int conditional);
+int graph_classify_lines(Ctx& c, Channel& ch, int64_t lo, int64_t hi, int rule);
+int graph_apply_lines(Ctx& c, Channel& ch, int64_t lo, int64_t hi, int conditional);
 int graph_logical_degrees(Ctx& c, int64_t* d_deg);
 int graph_download(Ctx& c, int64_t* h_off, int32_t* h_col);
 int graph_compact(Ctx& c);

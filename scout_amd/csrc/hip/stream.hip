@@ -233,6 +233,43 @@ int batch_load(Ctx& c, Channel& ch, int slot) {
   return SCOUT_OK;
 }
 
+// Grows a device buffer to `count` elements keeping its first `keep` elements (copied on the
+// compute stream, which the caller has ordered after every user of the old buffer).
+template <typename T>
+int grow_keep(Ctx& c, DevBuf<T>& b, int64_t count, int64_t keep) {
+  if (count <= b.cap) return SCOUT_OK;
+  DevBuf<T> nb;
+  SCOUT_CHECK(nb.reserve(count));
+  if (keep > 0) SCOUT_TRY(hipMemcpyAsync(nb.p, b.p, (size_t)keep * sizeof(T), hipMemcpyDeviceToDevice, c.s));
+  SCOUT_TRY(hipStreamSynchronize(c.s));
+  b.release();
+  b = nb;
+  return SCOUT_OK;
+}
+
+// Appends stored batch `slot` after the lines already staged in the channel (a serving step
+// that takes several queued requests at once).
+int batch_append(Ctx& c, Channel& ch, int slot) {
+  if (!c.stream || slot < 0 || (size_t)slot >= c.stream->slots.size()) {
+    set_error("batch_append: no such slot");
+    return SCOUT_ERR_ARG;
+  }
+  int64_t add = c.stream->slot_m[(size_t)slot], old = ch.m;
+  SCOUT_TRY(hipEventSynchronize(ch.staged));
+  SCOUT_TRY(hipStreamSynchronize(c.s));
+  SCOUT_CHECK(grow_keep(c, ch.d_uv, 2 * (old + add) + 2, 2 * old));
+  SCOUT_CHECK(grow_keep(c, ch.d_ins, old + add + 1, old));
+  SCOUT_CHECK(grow_keep(c, ch.d_done, old + add + 1, 0));
+  SCOUT_TRY(hipEventRecord(ch.staged, c.s));
+  SCOUT_TRY(hipStreamWaitEvent(c.cs, ch.staged, 0));
+  if (add > 0)
+    SCOUT_TRY(hipMemcpyAsync(ch.d_uv.p + 2 * old, c.stream->slots[(size_t)slot].p, (size_t)(2 * add) * 4,
+                             hipMemcpyDeviceToDevice, c.cs));
+  SCOUT_TRY(hipEventRecord(ch.staged, c.cs));
+  ch.m = old + add;
+  return SCOUT_OK;
+}
+
 void stream_release(Ctx& c) {
   if (!c.stream) return;
   for (auto& b : c.stream->slots) b.release();
@@ -414,6 +451,16 @@ int scout_batch_load(scout_ctx* x, int channel, int slot, int64_t* m) {
   Channel* ch = channel_of(x->c, channel);
   if (!ch) return SCOUT_ERR_ARG;
   SCOUT_CHECK(batch_load(x->c, *ch, slot));
+  if (m) *m = ch->m;
+  return SCOUT_OK;
+}
+
+int scout_batch_append(scout_ctx* x, int channel, int slot, int64_t* m) {
+  if (!x) return SCOUT_ERR_ARG;
+  SCOUT_TRY(hipSetDevice(x->c.dev));
+  Channel* ch = channel_of(x->c, channel);
+  if (!ch) return SCOUT_ERR_ARG;
+  SCOUT_CHECK(batch_append(x->c, *ch, slot));
   if (m) *m = ch->m;
   return SCOUT_OK;
 }

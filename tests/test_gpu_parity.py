@@ -87,6 +87,56 @@ def test_bfs_cc_exact_every_batch(data, name):
     eng.close()
 
 
+def test_segments_match_sequential_batches(data):
+    """Three batches staged back to back and applied as segments (one serving step) give the
+    labels, the logical graph and the BFS/CC solutions of applying them one by one, also when
+    later batches toggle pairs of earlier ones."""
+    egr, bdir, _ = data["rmat"]
+    say("[segments] rmat")
+    eng, g = engine_for(egr)
+    og = OracleGraph(egr)
+    eng.upload(sa.STATE_LEVEL, og.bfs(0))
+    eng.upload(sa.STATE_LABEL, og.cc())
+    parts = [uv for _, uv in batches(bdir, 3, g.n)]
+    rng = np.random.default_rng(3)
+    # later segments toggle some pairs of earlier ones again
+    parts[1] = np.concatenate([parts[1], parts[0][rng.choice(len(parts[0]), 200, replace=False)][:, ::-1]])
+    parts[2] = np.concatenate([parts[2], parts[1][rng.choice(len(parts[1]), 200, replace=False)]])
+    allv = np.ascontiguousarray(np.concatenate(parts).astype(np.int32))
+    off = np.cumsum([0] + [len(p) for p in parts])
+    eng.stage(allv)
+    ins = eng.apply_segments(off)
+    want = []
+    for p in parts:
+        i = og.classify(p)
+        og.apply(p, i)
+        want.append(i)
+    assert np.array_equal(ins, np.concatenate(want))
+    eng.repair_bfs()
+    eng.repair_cc()
+    assert np.array_equal(eng.download(sa.STATE_LEVEL), og.bfs(0))
+    assert np.array_equal(eng.download(sa.STATE_LABEL), og.cc())
+    off_o, col_o = og.logical_csr()
+    dg = eng.download_graph()
+    assert np.array_equal(dg.off, off_o) and np.array_equal(dg.col, col_o)
+    eng.close()
+
+
+def test_serving_loop_mix_equals_alone():
+    """The serving loop at a rate that queues requests: steps take several requests, and the
+    final levels and labels equal serving every request alone and a full recompute."""
+    import json
+    say("[serve] scale 14")
+    exe = os.path.join(os.path.dirname(sa.LIB_PATH), "bin", "scout_serve")
+    r = subprocess.run([exe, "--scale", "14", "--rate", "2e8", "--min", "200", "--max", "6000", "--requests", "80",
+                        "--budget-lines", "30000", "--seed", "3"], capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stderr
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    say(f"[serve] {out['steps']} steps for {out['requests']} requests, p99 {out['p99_ms']} ms")
+    assert out["same_as_alone"] and out["verified_exact"]
+    assert out["max_requests_per_step"] > 1
+
+
 @pytest.mark.parametrize("name", ["rmat", "heavy"])
 def test_cc_alone_exact_every_batch(data, name):
     """CC repaired without BFS maintained on the context: the CC state keeps its own levels
