@@ -144,6 +144,27 @@ int scout_repair_color(scout_ctx* ctx, int channel, scout_stats* st);
 /* MIS: replaces launchBatch + syncAndCollect (src/mis/scout_mis.cu:500-720); max_csr is the
  * reference's MAX_CSR_ELEMENTS sub-batch budget (src/mis/scout_mis.cu:800-815). */
 int scout_repair_mis(scout_ctx* ctx, int channel, int64_t max_csr, scout_stats* st);
+/* Multi-instance MIS: no counterpart in the reference, which runs one graph per process. The
+ * staged lines are grouped by instance (instances own disjoint vertex ranges of the loaded
+ * graph): lines [line_off[i], line_off[i+1]) belong to instance i, line_off[0] = 0 and
+ * line_off[ninst] = the staged line count. Each instance gets the sub-batch cuts and the result
+ * scout_repair_mis gives it alone; all instances' rounds run in the same launches. */
+int scout_repair_mis_multi(scout_ctx* ctx, int channel, int32_t ninst, const int64_t* line_off, int64_t max_csr,
+                           scout_stats* st);
+/* Launch settings of the multi-instance MIS rounds: grid blocks, rounds per replay of the
+ * captured hipGraph, graph on/off. No counterpart in the reference. */
+int scout_mis_config(scout_ctx* ctx, int grid, int group_rounds, int use_graph);
+/* Colouring repair settings: rows of at most thread_max entries are handled per thread, rows
+ * longer than block_min per workgroup, the rest per wavefront; grid = blocks of the round
+ * kernels; use_graph = replay captured round pairs. The reference fixes one thread per vertex
+ * (src/coloring/scout_coloring.cu:46-93); results do not depend on these settings. */
+int scout_color_config(scout_ctx* ctx, int thread_max, int block_min, int grid, int use_graph);
+/* out4 = {thread_max, block_min, grid, use_graph}. */
+int scout_color_get_config(scout_ctx* ctx, int* out4);
+/* Autotuner: times the colouring repair of the staged, already applied batch under each
+ * candidate setting (colours restored between trials) and keeps the fastest. No counterpart
+ * in the reference. */
+int scout_color_autotune(scout_ctx* ctx, int channel, double* best_ms);
 
 /* Full recomputes. BFS from `source` replaces gpu_full_bfs (src/bfs/scout_bfs.cu:274-323);
  * CC replaces full_recompute_cc_gpu (src/cc/scout_cc.cu:118-278). */

@@ -65,6 +65,11 @@ _SIGNATURES = {
     "scout_repair_cc": [_P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(Stats)],
     "scout_repair_color": [_P, ctypes.c_int, ctypes.POINTER(Stats)],
     "scout_repair_mis": [_P, ctypes.c_int, _I64, ctypes.POINTER(Stats)],
+    "scout_repair_mis_multi": [_P, ctypes.c_int, _I32, _P, _I64, ctypes.POINTER(Stats)],
+    "scout_mis_config": [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int],
+    "scout_color_config": [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int],
+    "scout_color_get_config": [_P, _P],
+    "scout_color_autotune": [_P, ctypes.c_int, ctypes.POINTER(ctypes.c_double)],
     "scout_bfs_full": [_P, _I32],
     "scout_cc_full": [_P],
     "scout_mis_greedy": [_P],
@@ -267,6 +272,35 @@ class Engine:
         st = Stats()
         self._check(self.lib.scout_repair_mis(self.h, channel, max_csr, ctypes.byref(st)), "repair_mis")
         return st
+
+    def repair_mis_multi(self, line_off, max_csr: int, channel: int = 0) -> Stats:
+        """MIS repair of a staged batch whose lines are grouped by instance: lines
+        [line_off[i], line_off[i+1]) belong to instance i (disjoint vertex sets)."""
+        off = np.ascontiguousarray(line_off, dtype=np.int64)
+        st = Stats()
+        self._check(self.lib.scout_repair_mis_multi(self.h, channel, len(off) - 1, _ptr(off), max_csr,
+                                                    ctypes.byref(st)), "repair_mis_multi")
+        return st
+
+    def mis_config(self, grid: int = 1024, group_rounds: int = 4, use_graph: bool = True):
+        self._check(self.lib.scout_mis_config(self.h, grid, group_rounds, 1 if use_graph else 0), "mis_config")
+
+    def color_config(self, thread_max: int = 8, block_min: int = 4096, grid: int = 1024, use_graph: bool = True):
+        self._check(self.lib.scout_color_config(self.h, thread_max, block_min, grid, 1 if use_graph else 0),
+                    "color_config")
+
+    def color_get_config(self):
+        out = np.zeros(4, dtype=np.int32)
+        self._check(self.lib.scout_color_get_config(self.h, _ptr(out)), "color_get_config")
+        return {"thread_max": int(out[0]), "block_min": int(out[1]), "grid": int(out[2]),
+                "use_graph": bool(out[3])}
+
+    def color_autotune(self, channel: int = 0) -> float:
+        """Times the colouring repair of the staged, applied batch under each candidate setting
+        and keeps the fastest; the colours are left unchanged. Returns the best time in ms."""
+        best = ctypes.c_double()
+        self._check(self.lib.scout_color_autotune(self.h, channel, ctypes.byref(best)), "color_autotune")
+        return best.value
 
     def bfs_full(self, source: int = 0):
         self._check(self.lib.scout_bfs_full(self.h, source), "bfs_full")

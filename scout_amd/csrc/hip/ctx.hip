@@ -267,6 +267,7 @@ int scout_ctx_destroy(scout_ctx* x) {
   cc_release(c);
   mlp_release(c);
   mis_release(c);
+  color_release(c);
   stream_release(c);
   snapshot_release(c);
   prof_release(c);

@@ -145,6 +145,7 @@ struct Channel {
 struct MlpState;  // mlp.hip
 struct CcState;   // cc.hip
 struct MisState;  // mis.hip
+struct ColorState;  // color.hip
 struct StreamState;  // stream.hip
 struct Snapshot;     // stream.hip
 struct ProfState;    // prof in ctx.hip
@@ -191,6 +192,7 @@ struct Ctx {
   MlpState* mlp = nullptr;
   CcState* cc = nullptr;
   MisState* mis_state = nullptr;
+  ColorState* color_state = nullptr;
   StreamState* stream = nullptr;
   Snapshot* snap = nullptr;
   ProfState* prof = nullptr;
@@ -263,6 +265,7 @@ int mlp_predict_async(Ctx& c, const int64_t* d_ncand, int64_t sample, int64_t* d
 // Appends the sample's features and outcomes (state after the repair) to the training store.
 int mlp_end_batch(Ctx& c, const int32_t* cand, int64_t ns, const int32_t* L);
 void mlp_release(Ctx& c);
+void color_release(Ctx& c);
 
 // Kernel timing for the roofline: events recorded on the compute stream around launches of a
 // named kernel family while profiling is on. `bytes` is the algorithmic byte count the caller

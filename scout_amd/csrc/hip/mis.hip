@@ -270,6 +270,111 @@ __global__ void k_mis_degrees(GView g, const int32_t* uv, int64_t m, int64_t* de
   }
 }
 
+// --- multi-instance repair --------------------------------------------------------------
+// The staged lines are permuted so that every phase (the k-th sub-batch of every instance) is
+// one contiguous range, each instance's lines in their original order. A phase holds one
+// segment per instance; the rounds commit a conflict-free prefix of every segment at once.
+// Device header of the segment table (int64): [0] segments S, [1] phase begin, [2] phase end,
+// [3] tag epoch, [4] segments still pending, [5] non-empty rounds, [6] writes; then pending
+// starts p[0..stride), ends hi[stride..2 stride), first conflicts k0[2 stride..3 stride).
+constexpr int kSegHdr = 8;
+
+__global__ void k_mm_gather(const int32_t* uv, const uint8_t* ins, const int32_t* perm, int64_t m, int32_t* uv2,
+                            uint8_t* ins2) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= m) return;
+  int64_t k = perm[t];
+  uv2[2 * t] = uv[2 * k];
+  uv2[2 * t + 1] = uv[2 * k + 1];
+  ins2[t] = ins[k];
+}
+
+__global__ void k_mm_eval(GView g, const int32_t* uv, const uint8_t* ins, const int32_t* lseg, const int64_t* seg,
+                          int64_t stride, const uint8_t* mis, const int32_t* tag, int32_t* minw, int32_t* w0s,
+                          int32_t* w1s) {
+  int64_t plo = seg[1], phi = seg[2];
+  int32_t ep = (int32_t)seg[3];
+  const int64_t* p = seg + kSegHdr;
+  for (int64_t k = plo + global_wave(); k < phi; k += total_waves()) {
+    if (k < p[lseg[k]]) continue;
+    MisView s{g, mis, tag, ep, minw, (int32_t)k, false, -1};
+    int32_t w0, w1;
+    eval_line(s, uv[2 * k], uv[2 * k + 1], ins[k] != 0, &w0, &w1);
+    if (lane_id() == 0) {
+      w0s[k] = w0;
+      w1s[k] = w1;
+      if (w0 >= 0) atomicMin(&minw[w0], (int32_t)k);
+      if (w1 >= 0) atomicMin(&minw[w1], (int32_t)k);
+    }
+  }
+}
+
+__global__ void k_mm_check(GView g, const int32_t* uv, const uint8_t* ins, const int32_t* lseg, int64_t* seg,
+                           int64_t stride, const uint8_t* mis, const int32_t* tag, int32_t* minw) {
+  int64_t plo = seg[1], phi = seg[2];
+  int32_t ep = (int32_t)seg[3];
+  const int64_t* p = seg + kSegHdr;
+  int64_t* k0 = seg + kSegHdr + 2 * stride;
+  for (int64_t k = plo + global_wave(); k < phi; k += total_waves()) {
+    int32_t sg = lseg[k];
+    if (k <= p[sg]) continue;  // committed, or the first pending line (never conflicts)
+    if (k >= __hip_atomic_load(&k0[sg], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) continue;
+    MisView s{g, mis, tag, ep, minw, (int32_t)k, true, -1};
+    int32_t w0, w1;
+    bool ok = eval_line(s, uv[2 * k], uv[2 * k + 1], ins[k] != 0, &w0, &w1);
+    if (!ok && lane_id() == 0) atomicMin((long long*)&k0[sg], (long long)k);
+  }
+}
+
+// Commits each segment's prefix [p, k0) and clears minw of every pending line's writes.
+__global__ void k_mm_commit(const uint8_t* ins, const int32_t* lseg, int64_t* seg, int64_t stride, const int32_t* w0s,
+                            const int32_t* w1s, uint8_t* mis, int32_t* minw) {
+  int64_t plo = seg[1], phi = seg[2];
+  const int64_t* p = seg + kSegHdr;
+  const int64_t* k0 = seg + kSegHdr + 2 * stride;
+  int64_t writes = 0;
+  for (int64_t k = plo + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < phi;
+       k += (int64_t)gridDim.x * blockDim.x) {
+    int32_t sg = lseg[k];
+    if (k < p[sg]) continue;
+    int32_t w0 = w0s[k], w1 = w1s[k];
+    if (k < k0[sg]) {
+      if (w0 >= 0) mis[w0] = ins[k] ? 0 : 1;
+      if (w1 >= 0) mis[w1] = 1;
+      writes += w0 >= 0;
+    }
+    if (w0 >= 0) minw[w0] = kNoWriter;
+    if (w1 >= 0) minw[w1] = kNoWriter;
+  }
+  writes = wave_sum(writes);
+  if (lane_id() == 0 && writes) atomicAdd((unsigned long long*)&seg[6], (unsigned long long)writes);
+}
+
+// One block: p = k0, k0 = hi; counts the segments still pending and the non-empty rounds.
+__global__ void k_mm_round_end(int64_t* seg, int64_t stride) {
+  __shared__ int64_t active, pending;
+  if (threadIdx.x == 0) active = pending = 0;
+  __syncthreads();
+  int64_t S = seg[0];
+  int64_t* p = seg + kSegHdr;
+  const int64_t* hi = seg + kSegHdr + stride;
+  int64_t* k0 = seg + kSegHdr + 2 * stride;
+  int64_t a = 0, q = 0;
+  for (int64_t i = threadIdx.x; i < S; i += blockDim.x) {
+    a += p[i] < hi[i];
+    p[i] = k0[i];
+    k0[i] = hi[i];
+    q += p[i] < hi[i];
+  }
+  if (a) atomicAdd((unsigned long long*)&active, (unsigned long long)a);
+  if (q) atomicAdd((unsigned long long*)&pending, (unsigned long long)q);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    seg[4] = pending;
+    if (active) seg[5] += 1;
+  }
+}
+
 }  // namespace
 
 struct MisState {
@@ -277,6 +382,15 @@ struct MisState {
   DevBuf<int64_t> deg;
   std::vector<int64_t> h_deg;
   int32_t tag_ep = 0;
+  // multi-instance path
+  DevBuf<int32_t> uv2, perm, lseg;
+  DevBuf<uint8_t> ins2;
+  DevBuf<int64_t> seg;
+  unsigned grid = kListGrid;
+  bool use_graph = true;
+  int group_rounds = 4;
+  hipGraphExec_t exec = nullptr;
+  std::vector<const void*> key;
 };
 
 void mis_release(Ctx& c) {
@@ -284,8 +398,40 @@ void mis_release(Ctx& c) {
   c.mis_state->w0.release();
   c.mis_state->w1.release();
   c.mis_state->deg.release();
+  c.mis_state->uv2.release();
+  c.mis_state->perm.release();
+  c.mis_state->lseg.release();
+  c.mis_state->ins2.release();
+  c.mis_state->seg.release();
+  if (c.mis_state->exec) (void)hipGraphExecDestroy(c.mis_state->exec);
   delete c.mis_state;
   c.mis_state = nullptr;
+}
+
+// Sub-batch cuts of lines [lo, hi) by the reference's estimate deg * 1000 per new endpoint
+// against max_csr, at most 2^20 lines each (src/mis/scout_mis.cu:519-542).
+void mis_cuts(const std::vector<int32_t>& uv, const std::vector<int64_t>& deg, int64_t lo, int64_t hi,
+              int64_t max_csr, std::vector<std::pair<int64_t, int64_t>>& subs) {
+  const int64_t MAX_BATCH = 1048576;
+  int64_t idx = lo;
+  std::unordered_set<int32_t> uniq;
+  while (idx < hi) {
+    uniq.clear();
+    int64_t cur = 0, se = 0, start = idx;
+    while (idx < hi && se < MAX_BATCH) {
+      int32_t a = uv[(size_t)(2 * idx)], b = uv[(size_t)(2 * idx + 1)];
+      int64_t est = 0;
+      if (!uniq.count(a)) est += deg[(size_t)(2 * idx)] * 1000;
+      if (!uniq.count(b)) est += deg[(size_t)(2 * idx + 1)] * 1000;
+      if (se > 0 && cur + est >= max_csr) break;
+      cur += est;
+      uniq.insert(a);
+      uniq.insert(b);
+      se++;
+      idx++;
+    }
+    subs.emplace_back(start, idx);
+  }
 }
 
 int mis_repair(Ctx& c, Channel& ch, int64_t max_csr, scout_stats* st) {
@@ -310,28 +456,7 @@ int mis_repair(Ctx& c, Channel& ch, int64_t max_csr, scout_stats* st) {
   SCOUT_TRY(hipMemcpyAsync(ins.data(), ch.d_ins.p, (size_t)m, hipMemcpyDeviceToHost, c.s));
   SCOUT_TRY(hipStreamSynchronize(c.s));
   std::vector<std::pair<int64_t, int64_t>> subs;
-  {
-    const int64_t MAX_BATCH = 1048576;
-    int64_t idx = 0;
-    std::unordered_set<int32_t> uniq;
-    while (idx < m) {
-      uniq.clear();
-      int64_t cur = 0, se = 0, start = idx;
-      while (idx < m && se < MAX_BATCH) {
-        int32_t a = uv[(size_t)(2 * idx)], b = uv[(size_t)(2 * idx + 1)];
-        int64_t est = 0;
-        if (!uniq.count(a)) est += ms.h_deg[(size_t)(2 * idx)] * 1000;
-        if (!uniq.count(b)) est += ms.h_deg[(size_t)(2 * idx + 1)] * 1000;
-        if (se > 0 && cur + est >= max_csr) break;
-        cur += est;
-        uniq.insert(a);
-        uniq.insert(b);
-        se++;
-        idx++;
-      }
-      subs.emplace_back(start, idx);
-    }
-  }
+  mis_cuts(uv, ms.h_deg, 0, m, max_csr, subs);
   for (int64_t k = 0; k < m; k++) (ins[(size_t)k] ? st->insertions : st->deletions)++;
   st->sub_batches += (int64_t)subs.size();
   int64_t rounds = 0;
@@ -391,5 +516,213 @@ extern "C" int scout_repair_mis(scout_ctx* x, int channel, int64_t max_csr, scou
   SCOUT_CHECK(mis_repair(c, *ch, max_csr, st));
   SCOUT_CHECK(span.end(c.s));
   st->repair_ms += span.ms();
+  return SCOUT_OK;
+}
+
+namespace scout {
+
+namespace {
+
+void mm_launch_round(Ctx& c, MisState& ms, int64_t stride) {
+  GView g = c.g.view();
+  k_mm_eval<<<ms.grid, kBlock, 0, c.s>>>(g, ms.uv2.p, ms.ins2.p, ms.lseg.p, ms.seg.p, stride, c.mis, c.mis_tag,
+                                         c.mis_minw, ms.w0.p, ms.w1.p);
+  k_mm_check<<<ms.grid, kBlock, 0, c.s>>>(g, ms.uv2.p, ms.ins2.p, ms.lseg.p, ms.seg.p, stride, c.mis, c.mis_tag,
+                                          c.mis_minw);
+  k_mm_commit<<<ms.grid, kBlock, 0, c.s>>>(ms.ins2.p, ms.lseg.p, ms.seg.p, stride, ms.w0.p, ms.w1.p, c.mis,
+                                           c.mis_minw);
+  k_mm_round_end<<<1, kBlock, 0, c.s>>>(ms.seg.p, stride);
+}
+
+void mm_launch_group(Ctx& c, MisState& ms, int64_t stride) {
+  for (int r = 0; r < ms.group_rounds; r++) mm_launch_round(c, ms, stride);
+}
+
+// Captures one group of rounds, unless the graph in hand holds the same buffers and settings.
+int mm_graph(Ctx& c, MisState& ms, int64_t stride) {
+  GView g = c.g.view();
+  std::vector<const void*> key = {ms.uv2.p, ms.ins2.p, ms.lseg.p, ms.seg.p, ms.w0.p, ms.w1.p, c.mis, c.mis_tag,
+                                  c.mis_minw, g.boff, g.bcol, g.aoff, g.alen, g.acol,
+                                  (const void*)(intptr_t)g.n, (const void*)(intptr_t)stride,
+                                  (const void*)(intptr_t)ms.grid, (const void*)(intptr_t)ms.group_rounds};
+  if (ms.exec && key == ms.key) return SCOUT_OK;
+  if (ms.exec) (void)hipGraphExecDestroy(ms.exec);
+  ms.exec = nullptr;
+  ms.key.clear();
+  hipGraph_t graph = nullptr;
+  SCOUT_TRY(hipStreamBeginCapture(c.s, hipStreamCaptureModeThreadLocal));
+  mm_launch_group(c, ms, stride);
+  hipError_t e = hipStreamEndCapture(c.s, &graph);
+  if (e != hipSuccess) {
+    set_error(std::string("mis: graph capture failed: ") + hipGetErrorString(e));
+    return SCOUT_ERR_HIP;
+  }
+  e = hipGraphInstantiate(&ms.exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) {
+    ms.exec = nullptr;
+    set_error(std::string("mis: graph instantiation failed: ") + hipGetErrorString(e));
+    return SCOUT_ERR_HIP;
+  }
+  ms.key = key;
+  return SCOUT_OK;
+}
+
+}  // namespace
+
+// Instances own disjoint vertex sets and their lines sit in the staged batch instance after
+// instance: lines [off[i], off[i+1]) belong to instance i. Every instance gets the sub-batch
+// cuts it would get alone; phase k runs the k-th sub-batch of every instance; a round commits
+// the conflict-free prefix of every segment. The result for each instance is the one its
+// lines give when repaired alone.
+int mis_repair_multi(Ctx& c, Channel& ch, int32_t ninst, const int64_t* off, int64_t max_csr, scout_stats* st) {
+  if (!c.mis_state) c.mis_state = new MisState();
+  MisState& ms = *c.mis_state;
+  int64_t m = ch.m;
+  int32_t n = c.g.n;
+  SCOUT_CHECK(ensure_state(c, SCOUT_STATE_MIS));
+  SCOUT_CHECK(ensure_scratch(c, &c.mis_minw, kNoWriter));
+  SCOUT_CHECK(ensure_scratch(c, &c.mis_tag, -1));
+  if (m == 0) return SCOUT_OK;
+  SCOUT_CHECK(ms.w0.reserve(m));
+  SCOUT_CHECK(ms.w1.reserve(m));
+  SCOUT_CHECK(ms.deg.reserve(2 * m));
+  SCOUT_CHECK(ms.uv2.reserve(2 * m));
+  SCOUT_CHECK(ms.ins2.reserve(m));
+  SCOUT_CHECK(ms.perm.reserve(m));
+  SCOUT_CHECK(ms.lseg.reserve(m));
+  int64_t stride = ninst;
+  SCOUT_CHECK(ms.seg.reserve(kSegHdr + 3 * stride));
+  k_mis_degrees<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), ch.d_uv.p, m, ms.deg.p);
+  ms.h_deg.resize((size_t)(2 * m));
+  std::vector<int32_t> uv((size_t)(2 * m));
+  std::vector<uint8_t> ins((size_t)m);
+  SCOUT_TRY(hipMemcpyAsync(ms.h_deg.data(), ms.deg.p, (size_t)(2 * m) * 8, hipMemcpyDeviceToHost, c.s));
+  SCOUT_TRY(hipMemcpyAsync(uv.data(), ch.d_uv.p, (size_t)(2 * m) * 4, hipMemcpyDeviceToHost, c.s));
+  SCOUT_TRY(hipMemcpyAsync(ins.data(), ch.d_ins.p, (size_t)m, hipMemcpyDeviceToHost, c.s));
+  SCOUT_TRY(hipStreamSynchronize(c.s));
+  // per instance cuts, then phases
+  std::vector<std::vector<std::pair<int64_t, int64_t>>> cuts((size_t)ninst);
+  size_t nphase = 0;
+  for (int32_t i = 0; i < ninst; i++) {
+    mis_cuts(uv, ms.h_deg, off[i], off[i + 1], max_csr, cuts[(size_t)i]);
+    nphase = std::max(nphase, cuts[(size_t)i].size());
+    st->sub_batches += (int64_t)cuts[(size_t)i].size();
+  }
+  for (int64_t k = 0; k < m; k++) (ins[(size_t)k] ? st->insertions : st->deletions)++;
+  std::vector<int32_t> perm((size_t)m), lseg((size_t)m);
+  struct Phase {
+    int64_t lo, hi;
+    std::vector<int64_t> seg_lo, seg_hi;
+  };
+  std::vector<Phase> phases(nphase);
+  int64_t t = 0;
+  for (size_t ph = 0; ph < nphase; ph++) {
+    Phase& P = phases[ph];
+    P.lo = t;
+    for (int32_t i = 0; i < ninst; i++) {
+      if (ph >= cuts[(size_t)i].size()) continue;
+      auto sb = cuts[(size_t)i][ph];
+      int32_t sg = (int32_t)P.seg_lo.size();
+      P.seg_lo.push_back(t);
+      for (int64_t k = sb.first; k < sb.second; k++, t++) {
+        perm[(size_t)t] = (int32_t)k;
+        lseg[(size_t)t] = sg;
+      }
+      P.seg_hi.push_back(t);
+    }
+    P.hi = t;
+  }
+  SCOUT_TRY(hipMemcpyAsync(ms.perm.p, perm.data(), (size_t)m * 4, hipMemcpyHostToDevice, c.s));
+  SCOUT_TRY(hipMemcpyAsync(ms.lseg.p, lseg.data(), (size_t)m * 4, hipMemcpyHostToDevice, c.s));
+  k_mm_gather<<<grid_for(m), kBlock, 0, c.s>>>(ch.d_uv.p, ch.d_ins.p, ms.perm.p, m, ms.uv2.p, ms.ins2.p);
+  if (ms.use_graph) SCOUT_CHECK(mm_graph(c, ms, stride));
+  std::vector<int64_t> hdr((size_t)(kSegHdr + 3 * stride), 0);
+  int64_t rounds = 0, writes = 0;
+  for (Phase& P : phases) {
+    if (ms.tag_ep >= (1 << 29)) {
+      SCOUT_TRY(hipMemsetAsync(c.mis_tag, 0xff, ((size_t)n + 1) * 4, c.s));
+      ms.tag_ep = 0;
+    }
+    int32_t ep = ++ms.tag_ep;
+    k_mis_tag_endpoints<<<grid_for(P.hi - P.lo), kBlock, 0, c.s>>>(ms.uv2.p, P.lo, P.hi, c.mis_tag, ep);
+    k_mis_tag_hop1<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), ms.uv2.p, P.lo, P.hi, c.mis_tag, ep);
+    int64_t S = (int64_t)P.seg_lo.size();
+    std::fill(hdr.begin(), hdr.end(), 0);
+    hdr[0] = S;
+    hdr[1] = P.lo;
+    hdr[2] = P.hi;
+    hdr[3] = ep;
+    for (int64_t sg = 0; sg < S; sg++) {
+      hdr[(size_t)(kSegHdr + sg)] = P.seg_lo[(size_t)sg];
+      hdr[(size_t)(kSegHdr + stride + sg)] = P.seg_hi[(size_t)sg];
+      hdr[(size_t)(kSegHdr + 2 * stride + sg)] = P.seg_hi[(size_t)sg];
+    }
+    // synchronous copy: hdr is reused by the next phase
+    SCOUT_TRY(hipMemcpyAsync(ms.seg.p, hdr.data(), hdr.size() * 8, hipMemcpyHostToDevice, c.s));
+    SCOUT_TRY(hipStreamSynchronize(c.s));
+    while (true) {
+      if (ms.use_graph) SCOUT_TRY(hipGraphLaunch(ms.exec, c.s));
+      else mm_launch_group(c, ms, stride);
+      SCOUT_TRY(hipMemcpyAsync(c.h_box, ms.seg.p, kSegHdr * sizeof(int64_t), hipMemcpyDeviceToHost, c.s));
+      SCOUT_TRY(hipStreamSynchronize(c.s));
+      c.readbacks++;
+      if (c.h_box[4] == 0) break;
+    }
+    rounds += c.h_box[5];
+    writes += c.h_box[6];
+  }
+  st->rounds += rounds;
+  st->conflicts += writes;
+  return SCOUT_OK;
+}
+
+}  // namespace scout
+
+extern "C" int scout_repair_mis_multi(scout_ctx* x, int channel, int32_t ninst, const int64_t* line_off,
+                                      int64_t max_csr, scout_stats* st) {
+  if (!x || !st || !line_off || ninst < 1) return SCOUT_ERR_ARG;
+  Ctx& c = x->c;
+  SCOUT_TRY(hipSetDevice(c.dev));
+  Channel* ch = channel_of(c, channel);
+  if (!ch) return SCOUT_ERR_ARG;
+  if (!c.g.boff || !c.mis) {
+    set_error("repair_mis_multi: graph and MIS must be loaded first");
+    return SCOUT_ERR_STATE;
+  }
+  if (max_csr <= 0) {
+    set_error("repair_mis_multi: max_csr must be positive");
+    return SCOUT_ERR_ARG;
+  }
+  if (line_off[0] != 0 || line_off[ninst] != ch->m) {
+    set_error("repair_mis_multi: line offsets must run from 0 to the staged line count");
+    return SCOUT_ERR_ARG;
+  }
+  for (int32_t i = 0; i < ninst; i++)
+    if (line_off[i + 1] < line_off[i]) {
+      set_error("repair_mis_multi: line offsets must not decrease");
+      return SCOUT_ERR_ARG;
+    }
+  Span span;
+  SCOUT_CHECK(span.begin(c.s));
+  st->lines += ch->m;
+  int64_t rb0 = c.readbacks;
+  SCOUT_CHECK(mis_repair_multi(c, *ch, ninst, line_off, max_csr, st));
+  SCOUT_CHECK(span.end(c.s));
+  st->repair_ms += span.ms();
+  st->readbacks += c.readbacks - rb0;
+  return SCOUT_OK;
+}
+
+extern "C" int scout_mis_config(scout_ctx* x, int grid, int group_rounds, int use_graph) {
+  if (!x || grid < 1 || grid > 65535 || group_rounds < 1 || group_rounds > 64) {
+    set_error("mis_config: need 1 <= grid <= 65535 and 1 <= group_rounds <= 64");
+    return SCOUT_ERR_ARG;
+  }
+  Ctx& c = x->c;
+  if (!c.mis_state) c.mis_state = new MisState();
+  c.mis_state->grid = (unsigned)grid;
+  c.mis_state->group_rounds = group_rounds;
+  c.mis_state->use_graph = use_graph != 0;
   return SCOUT_OK;
 }

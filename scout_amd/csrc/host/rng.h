@@ -44,6 +44,7 @@ enum : uint64_t {
   RNG_STREAM_SIZE = 7,
   RNG_MLP_INIT = 8,
   RNG_ARRIVAL = 9,
+  RNG_MULTI = 10,
 };
 
 // R-MAT: one 32-bit uniform per recursion level picks a quadrant with probabilities
