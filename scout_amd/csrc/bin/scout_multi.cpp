@@ -18,7 +18,7 @@
 //
 //   scout_multi [--instances 64] [--min-n 1e4] [--max-n 2e6] [--m 8] [--batches 10]
 //               [--min-lines 1000] [--max-lines 100000] [--seed 1] [--no-alone] [--no-tune]
-//               [--gpu 0]
+//               [--no-graph] [--gpu 0]
 //
 // Prints one JSON line. The reference maintains one graph per process; this driver is new.
 #include <omp.h>
@@ -103,7 +103,7 @@ int main(int argc, char** argv) {
   int ninst = 64, mba = 8, batches = 10, gpu = 0;
   double min_n = 1e4, max_n = 2e6, min_lines = 1000, max_lines = 100000;
   uint64_t seed = 1;
-  bool alone = true, tune = true;
+  bool alone = true, tune = true, graphs = true;
   for (int i = 1; i < argc; i++) {
     std::string k = argv[i];
     if (k == "--no-alone") {
@@ -112,6 +112,10 @@ int main(int argc, char** argv) {
     }
     if (k == "--no-tune") {
       tune = false;
+      continue;
+    }
+    if (k == "--no-graph") {
+      graphs = false;
       continue;
     }
     if (i + 1 >= argc) k = "?";
@@ -127,7 +131,7 @@ int main(int argc, char** argv) {
     else if (k == "--gpu") gpu = atoi(v);
     else {
       fprintf(stderr, "usage: scout_multi [--instances N] [--min-n A] [--max-n B] [--m M] [--batches K] "
-                      "[--min-lines L] [--max-lines L] [--seed S] [--no-alone] [--no-tune] [--gpu G]\n");
+                      "[--min-lines L] [--max-lines L] [--seed S] [--no-alone] [--no-tune] [--no-graph] [--gpu G]\n");
       return 1;
     }
   }
@@ -208,6 +212,13 @@ int main(int argc, char** argv) {
   if (fit > 1000000LL && mx > fit) mx = fit;
   int64_t max_csr = mx;
   CHECK(scout_snapshot(ctx));
+  if (!graphs) {
+    // the same kernels launched one by one instead of replayed from captured graphs
+    int cur[4];
+    CHECK(scout_color_get_config(ctx, cur));
+    CHECK(scout_color_config(ctx, cur[0], cur[1], cur[2], 0));
+    CHECK(scout_mis_config(ctx, 1024, 8, 0));
+  }
 
   int cfg[4] = {0, 0, 0, 0};
   double tune_ms = 0;

@@ -14,12 +14,19 @@
 // The reference runs these per cluster with racing threads; this file computes the result of
 // running them one line after another in ascending line order.
 //
-// Sequential equivalence by speculative prefix commit: every pending line is evaluated
-// against the committed state; minw[x] records the smallest pending line that would write x.
-// A line whose reads include a vertex with minw < its own index may depend on an earlier
-// line; the first such line k0 ends the round. Every line before k0 read nothing an earlier
-// pending line writes, so its speculative outcome is its sequential outcome, and its writes
-// commit. The first pending line never conflicts, so each round commits at least one line.
+// Sequential equivalence by speculative prefix commit: the pending lines of a window are
+// evaluated against the committed state; minw[x] records the smallest pending line that would
+// write x. A line whose reads include a vertex with minw < its own index may depend on an
+// earlier line; the first such line k0 ends the round. Every line before k0 read nothing an
+// earlier pending line writes, so its speculative outcome is its sequential outcome, and its
+// writes commit. The first pending line never conflicts, so each round commits at least one
+// line. Lines past the window are not evaluated (they would be evaluated again after the
+// conflict anyway): the window doubles while a round commits all of it and shrinks to twice
+// the committed prefix after a conflict, so a round's work stays near twice its commits.
+//
+// Segments: the lines of a sub-batch form one segment. Several segments (the sub-batches of
+// independent instances, which share no vertex) run their rounds in the same launches, each
+// with its own window and first conflict. A single graph is the case of one instance.
 //
 // The candidate order of row(r) is reproduced without building the ball: the sequence
 // [r] + N(r) + N(y1) + N(y2) + ... (y_i in N(r) order) lists the ball's vertices in the same
@@ -199,54 +206,6 @@ __global__ void k_mis_tag_hop1(GView g, const int32_t* uv, int64_t lo, int64_t h
   }
 }
 
-// Pass A: wave per pending line. Writes are recorded in w0/w1; minw[x] = smallest writer.
-__global__ void k_mis_eval(GView g, const int32_t* uv, const uint8_t* ins, int64_t p, int64_t hi, const uint8_t* mis,
-                           const int32_t* tag, int32_t ep, int32_t* minw, int32_t* w0s, int32_t* w1s) {
-  for (int64_t k = p + global_wave(); k < hi; k += total_waves()) {
-    MisView s{g, mis, tag, ep, minw, (int32_t)k, false, -1};
-    int32_t w0, w1;
-    eval_line(s, uv[2 * k], uv[2 * k + 1], ins[k] != 0, &w0, &w1);
-    if (lane_id() == 0) {
-      w0s[k] = w0;
-      w1s[k] = w1;
-      if (w0 >= 0) atomicMin(&minw[w0], (int32_t)k);
-      if (w1 >= 0) atomicMin(&minw[w1], (int32_t)k);
-    }
-  }
-}
-
-// Pass B: the first pending line whose reads meet an earlier pending writer.
-__global__ void k_mis_check(GView g, const int32_t* uv, const uint8_t* ins, int64_t p, int64_t hi, const uint8_t* mis,
-                            const int32_t* tag, int32_t ep, int32_t* minw, int64_t* first_conflict) {
-  for (int64_t k = p + 1 + global_wave(); k < hi; k += total_waves()) {
-    if (k >= __hip_atomic_load(first_conflict, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) continue;
-    MisView s{g, mis, tag, ep, minw, (int32_t)k, true, -1};
-    int32_t w0, w1;
-    bool ok = eval_line(s, uv[2 * k], uv[2 * k + 1], ins[k] != 0, &w0, &w1);
-    if (!ok && lane_id() == 0) atomicMin((long long*)first_conflict, (long long)k);
-  }
-}
-
-// Commit lines [p, k0); clear minw of every pending line's writes.
-__global__ void k_mis_commit(const uint8_t* ins, int64_t p, int64_t k0, int64_t hi, const int32_t* w0s,
-                             const int32_t* w1s, uint8_t* mis, int32_t* minw, int64_t* writes) {
-  int64_t k = p + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= hi) return;
-  int32_t w0 = w0s[k], w1 = w1s[k];
-  if (k < k0) {
-    if (w0 >= 0) mis[w0] = ins[k] ? 0 : 1;
-    if (w1 >= 0) mis[w1] = 1;
-    if (w0 >= 0) atomicAdd((unsigned long long*)writes, 1ULL);
-  }
-}
-
-__global__ void k_mis_clear_minw(int64_t p, int64_t hi, const int32_t* w0s, const int32_t* w1s, int32_t* minw) {
-  int64_t k = p + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= hi) return;
-  if (w0s[k] >= 0) minw[w0s[k]] = kNoWriter;
-  if (w1s[k] >= 0) minw[w1s[k]] = kNoWriter;
-}
-
 // Reference degree estimate per endpoint: base row length - |removed set| + |added list|
 // (HybridGraph::degree, core/FlashGraph.h:267-274); removed entries are the deleted base
 // entries plus the tombstones held in the inserted row.
@@ -270,14 +229,40 @@ __global__ void k_mis_degrees(GView g, const int32_t* uv, int64_t m, int64_t* de
   }
 }
 
-// --- multi-instance repair --------------------------------------------------------------
-// The staged lines are permuted so that every phase (the k-th sub-batch of every instance) is
-// one contiguous range, each instance's lines in their original order. A phase holds one
-// segment per instance; the rounds commit a conflict-free prefix of every segment at once.
-// Device header of the segment table (int64): [0] segments S, [1] phase begin, [2] phase end,
-// [3] tag epoch, [4] segments still pending, [5] non-empty rounds, [6] writes; then pending
-// starts p[0..stride), ends hi[stride..2 stride), first conflicts k0[2 stride..3 stride).
+// --- round engine over segments ---------------------------------------------------------
+// Segment table (int64, on the device): header [0] segments S, [1] slots (window lines over
+// all segments), [2] tag epoch, [3] segments still pending, [4] non-empty rounds, [5] writes,
+// [6] line evaluations; then per segment: pending start p, end hi, first conflict k0, window
+// size win, and woff (exclusive prefix of the current window sizes, S + 1 entries).
 constexpr int kSegHdr = 8;
+constexpr int64_t kMinWindow = 64;
+constexpr int64_t kMaxWindow = 1 << 20;
+
+struct SegView {
+  int64_t* t;
+  int64_t stride;
+  __device__ int64_t& S() const { return t[0]; }
+  __device__ int64_t* p() const { return t + kSegHdr; }
+  __device__ int64_t* hi() const { return t + kSegHdr + stride; }
+  __device__ int64_t* k0() const { return t + kSegHdr + 2 * stride; }
+  __device__ int64_t* win() const { return t + kSegHdr + 3 * stride; }
+  __device__ int64_t* woff() const { return t + kSegHdr + 4 * stride; }
+};
+
+inline int64_t seg_table_size(int64_t stride) { return kSegHdr + 5 * stride + 1; }
+
+// Window slot -> (segment, line): binary search over woff.
+__device__ __forceinline__ int64_t slot_line(const SegView& sv, int64_t slot, int32_t* sg_out) {
+  const int64_t* woff = sv.woff();
+  int64_t lo = 0, hi = sv.t[0];  // find the last s with woff[s] <= slot
+  while (hi - lo > 1) {
+    int64_t mid = (lo + hi) >> 1;
+    if (woff[mid] <= slot) lo = mid;
+    else hi = mid;
+  }
+  *sg_out = (int32_t)lo;
+  return sv.p()[lo] + (slot - woff[lo]);
+}
 
 __global__ void k_mm_gather(const int32_t* uv, const uint8_t* ins, const int32_t* perm, int64_t m, int32_t* uv2,
                             uint8_t* ins2) {
@@ -289,14 +274,14 @@ __global__ void k_mm_gather(const int32_t* uv, const uint8_t* ins, const int32_t
   ins2[t] = ins[k];
 }
 
-__global__ void k_mm_eval(GView g, const int32_t* uv, const uint8_t* ins, const int32_t* lseg, const int64_t* seg,
-                          int64_t stride, const uint8_t* mis, const int32_t* tag, int32_t* minw, int32_t* w0s,
-                          int32_t* w1s) {
-  int64_t plo = seg[1], phi = seg[2];
-  int32_t ep = (int32_t)seg[3];
-  const int64_t* p = seg + kSegHdr;
-  for (int64_t k = plo + global_wave(); k < phi; k += total_waves()) {
-    if (k < p[lseg[k]]) continue;
+// Pass A: wave per window line. Writes go to w0/w1; minw[x] = smallest writer.
+__global__ void k_mm_eval(GView g, const int32_t* uv, const uint8_t* ins, SegView sv, const uint8_t* mis,
+                          const int32_t* tag, int32_t* minw, int32_t* w0s, int32_t* w1s) {
+  int64_t slots = sv.t[1];
+  int32_t ep = (int32_t)sv.t[2];
+  for (int64_t t = global_wave(); t < slots; t += total_waves()) {
+    int32_t sg;
+    int64_t k = slot_line(sv, t, &sg);
     MisView s{g, mis, tag, ep, minw, (int32_t)k, false, -1};
     int32_t w0, w1;
     eval_line(s, uv[2 * k], uv[2 * k + 1], ins[k] != 0, &w0, &w1);
@@ -309,15 +294,16 @@ __global__ void k_mm_eval(GView g, const int32_t* uv, const uint8_t* ins, const 
   }
 }
 
-__global__ void k_mm_check(GView g, const int32_t* uv, const uint8_t* ins, const int32_t* lseg, int64_t* seg,
-                           int64_t stride, const uint8_t* mis, const int32_t* tag, int32_t* minw) {
-  int64_t plo = seg[1], phi = seg[2];
-  int32_t ep = (int32_t)seg[3];
-  const int64_t* p = seg + kSegHdr;
-  int64_t* k0 = seg + kSegHdr + 2 * stride;
-  for (int64_t k = plo + global_wave(); k < phi; k += total_waves()) {
-    int32_t sg = lseg[k];
-    if (k <= p[sg]) continue;  // committed, or the first pending line (never conflicts)
+// Pass B: per segment, the first window line whose reads meet an earlier pending writer.
+__global__ void k_mm_check(GView g, const int32_t* uv, const uint8_t* ins, SegView sv, const uint8_t* mis,
+                           const int32_t* tag, int32_t* minw) {
+  int64_t slots = sv.t[1];
+  int32_t ep = (int32_t)sv.t[2];
+  int64_t* k0 = sv.k0();
+  for (int64_t t = global_wave(); t < slots; t += total_waves()) {
+    int32_t sg;
+    int64_t k = slot_line(sv, t, &sg);
+    if (k == sv.p()[sg]) continue;  // the first pending line never conflicts
     if (k >= __hip_atomic_load(&k0[sg], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) continue;
     MisView s{g, mis, tag, ep, minw, (int32_t)k, true, -1};
     int32_t w0, w1;
@@ -326,90 +312,114 @@ __global__ void k_mm_check(GView g, const int32_t* uv, const uint8_t* ins, const
   }
 }
 
-// Commits each segment's prefix [p, k0) and clears minw of every pending line's writes.
-__global__ void k_mm_commit(const uint8_t* ins, const int32_t* lseg, int64_t* seg, int64_t stride, const int32_t* w0s,
-                            const int32_t* w1s, uint8_t* mis, int32_t* minw) {
-  int64_t plo = seg[1], phi = seg[2];
-  const int64_t* p = seg + kSegHdr;
-  const int64_t* k0 = seg + kSegHdr + 2 * stride;
+// Commits each segment's prefix [p, k0) and clears minw of every window line's writes.
+__global__ void k_mm_commit(const uint8_t* ins, SegView sv, const int32_t* w0s, const int32_t* w1s, uint8_t* mis,
+                            int32_t* minw) {
+  int64_t slots = sv.t[1];
+  const int64_t* k0 = sv.k0();
   int64_t writes = 0;
-  for (int64_t k = plo + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < phi;
-       k += (int64_t)gridDim.x * blockDim.x) {
-    int32_t sg = lseg[k];
-    if (k < p[sg]) continue;
-    int32_t w0 = w0s[k], w1 = w1s[k];
-    if (k < k0[sg]) {
-      if (w0 >= 0) mis[w0] = ins[k] ? 0 : 1;
-      if (w1 >= 0) mis[w1] = 1;
-      writes += w0 >= 0;
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < slots; base += (int64_t)gridDim.x * blockDim.x) {
+    int64_t t = base + threadIdx.x;
+    if (t < slots) {
+      int32_t sg;
+      int64_t k = slot_line(sv, t, &sg);
+      int32_t w0 = w0s[k], w1 = w1s[k];
+      if (k < k0[sg]) {
+        if (w0 >= 0) mis[w0] = ins[k] ? 0 : 1;
+        if (w1 >= 0) mis[w1] = 1;
+        writes += w0 >= 0;
+      }
+      if (w0 >= 0) minw[w0] = kNoWriter;
+      if (w1 >= 0) minw[w1] = kNoWriter;
     }
-    if (w0 >= 0) minw[w0] = kNoWriter;
-    if (w1 >= 0) minw[w1] = kNoWriter;
   }
   writes = wave_sum(writes);
-  if (lane_id() == 0 && writes) atomicAdd((unsigned long long*)&seg[6], (unsigned long long)writes);
+  if (lane_id() == 0 && writes) atomicAdd((unsigned long long*)&sv.t[5], (unsigned long long)writes);
 }
 
-// One block: p = k0, k0 = hi; counts the segments still pending and the non-empty rounds.
-__global__ void k_mm_round_end(int64_t* seg, int64_t stride) {
-  __shared__ int64_t active, pending;
-  if (threadIdx.x == 0) active = pending = 0;
+// One block: advances every segment (p = k0), resizes its window, sets k0 to the window end
+// and rebuilds woff and the slot count; counts pending segments and non-empty rounds.
+__global__ void k_mm_round_end(SegView sv) {
+  __shared__ int64_t scan[kBlock];
+  __shared__ int64_t carry_s, active_s, pending_s;
+  int64_t S = sv.t[0];
+  int64_t *p = sv.p(), *hi = sv.hi(), *k0 = sv.k0(), *win = sv.win(), *woff = sv.woff();
+  if (threadIdx.x == 0) carry_s = active_s = pending_s = 0;
   __syncthreads();
-  int64_t S = seg[0];
-  int64_t* p = seg + kSegHdr;
-  const int64_t* hi = seg + kSegHdr + stride;
-  int64_t* k0 = seg + kSegHdr + 2 * stride;
-  int64_t a = 0, q = 0;
-  for (int64_t i = threadIdx.x; i < S; i += blockDim.x) {
-    a += p[i] < hi[i];
-    p[i] = k0[i];
-    k0[i] = hi[i];
-    q += p[i] < hi[i];
+  for (int64_t base = 0; base < S; base += blockDim.x) {
+    int64_t s = base + threadIdx.x;
+    int64_t size = 0;
+    if (s < S) {
+      int64_t ps = p[s], hs = hi[s], ks = k0[s], ws = win[s];
+      if (ps < hs) {
+        atomicAdd((unsigned long long*)&active_s, 1ULL);
+        int64_t done = ks - ps;
+        ws = (ks == ps + min(ws, hs - ps)) ? min(2 * ws, kMaxWindow) : max(kMinWindow, 2 * done);
+      }
+      ps = ks;
+      size = min(ws, hs - ps);
+      if (ps < hs) atomicAdd((unsigned long long*)&pending_s, 1ULL);
+      p[s] = ps;
+      win[s] = ws;
+      k0[s] = ps + size;
+    }
+    scan[threadIdx.x] = size;
+    __syncthreads();
+    for (int off = 1; off < (int)blockDim.x; off <<= 1) {
+      int64_t y = threadIdx.x >= (unsigned)off ? scan[threadIdx.x - off] : 0;
+      __syncthreads();
+      scan[threadIdx.x] += y;
+      __syncthreads();
+    }
+    if (s < S) woff[s] = carry_s + scan[threadIdx.x] - size;
+    __syncthreads();
+    if (threadIdx.x == blockDim.x - 1) carry_s += scan[threadIdx.x];
+    __syncthreads();
   }
-  if (a) atomicAdd((unsigned long long*)&active, (unsigned long long)a);
-  if (q) atomicAdd((unsigned long long*)&pending, (unsigned long long)q);
-  __syncthreads();
   if (threadIdx.x == 0) {
-    seg[4] = pending;
-    if (active) seg[5] += 1;
+    woff[S] = carry_s;
+    sv.t[1] = carry_s;
+    sv.t[3] = pending_s;
+    if (active_s) sv.t[4] += 1;
+    sv.t[6] += carry_s;
   }
 }
 
 }  // namespace
 
 struct MisState {
-  DevBuf<int32_t> w0, w1;
-  DevBuf<int64_t> deg;
+  DevBuf<int32_t> w0, w1, uv2, perm;
+  DevBuf<uint8_t> ins2;
+  DevBuf<int64_t> deg, seg;
   std::vector<int64_t> h_deg;
   int32_t tag_ep = 0;
-  // multi-instance path
-  DevBuf<int32_t> uv2, perm, lseg;
-  DevBuf<uint8_t> ins2;
-  DevBuf<int64_t> seg;
   unsigned grid = kListGrid;
   bool use_graph = true;
-  int group_rounds = 4;
+  int group_rounds = 8;
+  int64_t window0 = 256;
   hipGraphExec_t exec = nullptr;
   std::vector<const void*> key;
 };
 
 void mis_release(Ctx& c) {
   if (!c.mis_state) return;
-  c.mis_state->w0.release();
-  c.mis_state->w1.release();
-  c.mis_state->deg.release();
-  c.mis_state->uv2.release();
-  c.mis_state->perm.release();
-  c.mis_state->lseg.release();
-  c.mis_state->ins2.release();
-  c.mis_state->seg.release();
-  if (c.mis_state->exec) (void)hipGraphExecDestroy(c.mis_state->exec);
+  MisState& ms = *c.mis_state;
+  ms.w0.release();
+  ms.w1.release();
+  ms.uv2.release();
+  ms.perm.release();
+  ms.ins2.release();
+  ms.deg.release();
+  ms.seg.release();
+  if (ms.exec) (void)hipGraphExecDestroy(ms.exec);
   delete c.mis_state;
   c.mis_state = nullptr;
 }
 
+namespace {
+
 // Sub-batch cuts of lines [lo, hi) by the reference's estimate deg * 1000 per new endpoint
-// against max_csr, at most 2^20 lines each (src/mis/scout_mis.cu:519-542).
+// against max_csr, at most 2^20 endpoints' worth of lines each (src/mis/scout_mis.cu:519-542).
 void mis_cuts(const std::vector<int32_t>& uv, const std::vector<int64_t>& deg, int64_t lo, int64_t hi,
               int64_t max_csr, std::vector<std::pair<int64_t, int64_t>>& subs) {
   const int64_t MAX_BATCH = 1048576;
@@ -434,124 +444,33 @@ void mis_cuts(const std::vector<int32_t>& uv, const std::vector<int64_t>& deg, i
   }
 }
 
-int mis_repair(Ctx& c, Channel& ch, int64_t max_csr, scout_stats* st) {
-  if (!c.mis_state) c.mis_state = new MisState();
-  MisState& ms = *c.mis_state;
-  int64_t m = ch.m;
-  int32_t n = c.g.n;
-  SCOUT_CHECK(ensure_state(c, SCOUT_STATE_MIS));
-  SCOUT_CHECK(ensure_scratch(c, &c.mis_minw, kNoWriter));
-  SCOUT_CHECK(ensure_scratch(c, &c.mis_tag, -1));
-  if (m == 0) return SCOUT_OK;
-  SCOUT_CHECK(ms.w0.reserve(m));
-  SCOUT_CHECK(ms.w1.reserve(m));
-  SCOUT_CHECK(ms.deg.reserve(2 * m));
-  // Sub-batch cuts by the reference's estimate deg * 1000 against max_csr (:519-542).
-  k_mis_degrees<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), ch.d_uv.p, m, ms.deg.p);
-  ms.h_deg.resize((size_t)(2 * m));
-  std::vector<int32_t> uv((size_t)(2 * m));
-  std::vector<uint8_t> ins((size_t)m);
-  SCOUT_TRY(hipMemcpyAsync(ms.h_deg.data(), ms.deg.p, (size_t)(2 * m) * 8, hipMemcpyDeviceToHost, c.s));
-  SCOUT_TRY(hipMemcpyAsync(uv.data(), ch.d_uv.p, (size_t)(2 * m) * 4, hipMemcpyDeviceToHost, c.s));
-  SCOUT_TRY(hipMemcpyAsync(ins.data(), ch.d_ins.p, (size_t)m, hipMemcpyDeviceToHost, c.s));
-  SCOUT_TRY(hipStreamSynchronize(c.s));
-  std::vector<std::pair<int64_t, int64_t>> subs;
-  mis_cuts(uv, ms.h_deg, 0, m, max_csr, subs);
-  for (int64_t k = 0; k < m; k++) (ins[(size_t)k] ? st->insertions : st->deletions)++;
-  st->sub_batches += (int64_t)subs.size();
-  int64_t rounds = 0;
-  for (auto& sb : subs) {
-    int64_t lo = sb.first, hi = sb.second;
-    if (ms.tag_ep >= (1 << 29)) {
-      SCOUT_TRY(hipMemsetAsync(c.mis_tag, 0xff, ((size_t)n + 1) * 4, c.s));
-      ms.tag_ep = 0;
-    }
-    int32_t ep = ++ms.tag_ep;
-    k_mis_tag_endpoints<<<grid_for(hi - lo), kBlock, 0, c.s>>>(ch.d_uv.p, lo, hi, c.mis_tag, ep);
-    k_mis_tag_hop1<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), ch.d_uv.p, lo, hi, c.mis_tag, ep);
-    int64_t p = lo;
-    while (p < hi) {
-      rounds++;
-      k_mis_eval<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), ch.d_uv.p, ch.d_ins.p, p, hi, c.mis, c.mis_tag, ep,
-                                                c.mis_minw, ms.w0.p, ms.w1.p);
-      int64_t init[2] = {hi, 0};
-      SCOUT_TRY(hipMemcpyAsync(c.d_box, init, sizeof(init), hipMemcpyHostToDevice, c.s));
-      k_mis_check<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), ch.d_uv.p, ch.d_ins.p, p, hi, c.mis, c.mis_tag, ep,
-                                                 c.mis_minw, c.d_box);
-      SCOUT_CHECK(box_read(c, 1));
-      int64_t k0 = c.h_box[0];
-      k_mis_commit<<<grid_for(hi - p), kBlock, 0, c.s>>>(ch.d_ins.p, p, k0, hi, ms.w0.p, ms.w1.p, c.mis, c.mis_minw,
-                                                         c.d_box + 1);
-      k_mis_clear_minw<<<grid_for(hi - p), kBlock, 0, c.s>>>(p, hi, ms.w0.p, ms.w1.p, c.mis_minw);
-      SCOUT_CHECK(box_read(c, 2));
-      st->conflicts += c.h_box[1];
-      p = k0;
-    }
-  }
-  st->rounds += rounds;
-  return SCOUT_OK;
-}
-
-}  // namespace scout
-
-using namespace scout;
-
-extern "C" int scout_repair_mis(scout_ctx* x, int channel, int64_t max_csr, scout_stats* st) {
-  if (!x || !st) return SCOUT_ERR_ARG;
-  Ctx& c = x->c;
-  SCOUT_TRY(hipSetDevice(c.dev));
-  Channel* ch = channel_of(c, channel);
-  if (!ch) return SCOUT_ERR_ARG;
-  if (!c.g.boff || !c.mis) {
-    set_error("repair_mis: graph and MIS must be loaded first");
-    return SCOUT_ERR_STATE;
-  }
-  if (max_csr <= 0) {
-    set_error("repair_mis: max_csr must be positive");
-    return SCOUT_ERR_ARG;
-  }
-  Span span;
-  SCOUT_CHECK(span.begin(c.s));
-  st->lines += ch->m;
-  SCOUT_CHECK(mis_repair(c, *ch, max_csr, st));
-  SCOUT_CHECK(span.end(c.s));
-  st->repair_ms += span.ms();
-  return SCOUT_OK;
-}
-
-namespace scout {
-
-namespace {
-
-void mm_launch_round(Ctx& c, MisState& ms, int64_t stride) {
+void mm_launch_round(Ctx& c, MisState& ms, SegView sv) {
   GView g = c.g.view();
-  k_mm_eval<<<ms.grid, kBlock, 0, c.s>>>(g, ms.uv2.p, ms.ins2.p, ms.lseg.p, ms.seg.p, stride, c.mis, c.mis_tag,
-                                         c.mis_minw, ms.w0.p, ms.w1.p);
-  k_mm_check<<<ms.grid, kBlock, 0, c.s>>>(g, ms.uv2.p, ms.ins2.p, ms.lseg.p, ms.seg.p, stride, c.mis, c.mis_tag,
-                                          c.mis_minw);
-  k_mm_commit<<<ms.grid, kBlock, 0, c.s>>>(ms.ins2.p, ms.lseg.p, ms.seg.p, stride, ms.w0.p, ms.w1.p, c.mis,
-                                           c.mis_minw);
-  k_mm_round_end<<<1, kBlock, 0, c.s>>>(ms.seg.p, stride);
+  k_mm_eval<<<ms.grid, kBlock, 0, c.s>>>(g, ms.uv2.p, ms.ins2.p, sv, c.mis, c.mis_tag, c.mis_minw, ms.w0.p,
+                                         ms.w1.p);
+  k_mm_check<<<ms.grid, kBlock, 0, c.s>>>(g, ms.uv2.p, ms.ins2.p, sv, c.mis, c.mis_tag, c.mis_minw);
+  k_mm_commit<<<ms.grid, kBlock, 0, c.s>>>(ms.ins2.p, sv, ms.w0.p, ms.w1.p, c.mis, c.mis_minw);
+  k_mm_round_end<<<1, kBlock, 0, c.s>>>(sv);
 }
 
-void mm_launch_group(Ctx& c, MisState& ms, int64_t stride) {
-  for (int r = 0; r < ms.group_rounds; r++) mm_launch_round(c, ms, stride);
+void mm_launch_group(Ctx& c, MisState& ms, SegView sv) {
+  for (int r = 0; r < ms.group_rounds; r++) mm_launch_round(c, ms, sv);
 }
 
 // Captures one group of rounds, unless the graph in hand holds the same buffers and settings.
-int mm_graph(Ctx& c, MisState& ms, int64_t stride) {
+int mm_graph(Ctx& c, MisState& ms, SegView sv) {
   GView g = c.g.view();
-  std::vector<const void*> key = {ms.uv2.p, ms.ins2.p, ms.lseg.p, ms.seg.p, ms.w0.p, ms.w1.p, c.mis, c.mis_tag,
-                                  c.mis_minw, g.boff, g.bcol, g.aoff, g.alen, g.acol,
-                                  (const void*)(intptr_t)g.n, (const void*)(intptr_t)stride,
-                                  (const void*)(intptr_t)ms.grid, (const void*)(intptr_t)ms.group_rounds};
+  std::vector<const void*> key = {ms.uv2.p, ms.ins2.p, sv.t, ms.w0.p, ms.w1.p, c.mis, c.mis_tag, c.mis_minw,
+                                  g.boff, g.bcol, g.aoff, g.alen, g.acol, (const void*)(intptr_t)g.n,
+                                  (const void*)(intptr_t)sv.stride, (const void*)(intptr_t)ms.grid,
+                                  (const void*)(intptr_t)ms.group_rounds};
   if (ms.exec && key == ms.key) return SCOUT_OK;
   if (ms.exec) (void)hipGraphExecDestroy(ms.exec);
   ms.exec = nullptr;
   ms.key.clear();
   hipGraph_t graph = nullptr;
   SCOUT_TRY(hipStreamBeginCapture(c.s, hipStreamCaptureModeThreadLocal));
-  mm_launch_group(c, ms, stride);
+  mm_launch_group(c, ms, sv);
   hipError_t e = hipStreamEndCapture(c.s, &graph);
   if (e != hipSuccess) {
     set_error(std::string("mis: graph capture failed: ") + hipGetErrorString(e));
@@ -572,9 +491,8 @@ int mm_graph(Ctx& c, MisState& ms, int64_t stride) {
 
 // Instances own disjoint vertex sets and their lines sit in the staged batch instance after
 // instance: lines [off[i], off[i+1]) belong to instance i. Every instance gets the sub-batch
-// cuts it would get alone; phase k runs the k-th sub-batch of every instance; a round commits
-// the conflict-free prefix of every segment. The result for each instance is the one its
-// lines give when repaired alone.
+// cuts it would get alone; phase k runs the k-th sub-batch of every instance as one segment
+// each. The result for each instance is the one its lines give when repaired alone.
 int mis_repair_multi(Ctx& c, Channel& ch, int32_t ninst, const int64_t* off, int64_t max_csr, scout_stats* st) {
   if (!c.mis_state) c.mis_state = new MisState();
   MisState& ms = *c.mis_state;
@@ -590,9 +508,11 @@ int mis_repair_multi(Ctx& c, Channel& ch, int32_t ninst, const int64_t* off, int
   SCOUT_CHECK(ms.uv2.reserve(2 * m));
   SCOUT_CHECK(ms.ins2.reserve(m));
   SCOUT_CHECK(ms.perm.reserve(m));
-  SCOUT_CHECK(ms.lseg.reserve(m));
-  int64_t stride = ninst;
-  SCOUT_CHECK(ms.seg.reserve(kSegHdr + 3 * stride));
+  int64_t stride = std::max<int64_t>(ninst, 64);
+  SCOUT_CHECK(ms.seg.reserve(seg_table_size(stride)));
+  // the table stride is part of the captured graph: keep the largest seen
+  stride = (ms.seg.cap - kSegHdr - 1) / 5;
+  SegView sv{ms.seg.p, stride};
   k_mis_degrees<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), ch.d_uv.p, m, ms.deg.p);
   ms.h_deg.resize((size_t)(2 * m));
   std::vector<int32_t> uv((size_t)(2 * m));
@@ -601,7 +521,6 @@ int mis_repair_multi(Ctx& c, Channel& ch, int32_t ninst, const int64_t* off, int
   SCOUT_TRY(hipMemcpyAsync(uv.data(), ch.d_uv.p, (size_t)(2 * m) * 4, hipMemcpyDeviceToHost, c.s));
   SCOUT_TRY(hipMemcpyAsync(ins.data(), ch.d_ins.p, (size_t)m, hipMemcpyDeviceToHost, c.s));
   SCOUT_TRY(hipStreamSynchronize(c.s));
-  // per instance cuts, then phases
   std::vector<std::vector<std::pair<int64_t, int64_t>>> cuts((size_t)ninst);
   size_t nphase = 0;
   for (int32_t i = 0; i < ninst; i++) {
@@ -610,35 +529,40 @@ int mis_repair_multi(Ctx& c, Channel& ch, int32_t ninst, const int64_t* off, int
     st->sub_batches += (int64_t)cuts[(size_t)i].size();
   }
   for (int64_t k = 0; k < m; k++) (ins[(size_t)k] ? st->insertions : st->deletions)++;
-  std::vector<int32_t> perm((size_t)m), lseg((size_t)m);
+  // permutation: phase after phase, segment after segment, lines in their order
+  std::vector<int32_t> perm((size_t)m);
   struct Phase {
     int64_t lo, hi;
     std::vector<int64_t> seg_lo, seg_hi;
   };
   std::vector<Phase> phases(nphase);
   int64_t t = 0;
+  bool identity = true;
   for (size_t ph = 0; ph < nphase; ph++) {
     Phase& P = phases[ph];
     P.lo = t;
     for (int32_t i = 0; i < ninst; i++) {
       if (ph >= cuts[(size_t)i].size()) continue;
       auto sb = cuts[(size_t)i][ph];
-      int32_t sg = (int32_t)P.seg_lo.size();
       P.seg_lo.push_back(t);
       for (int64_t k = sb.first; k < sb.second; k++, t++) {
         perm[(size_t)t] = (int32_t)k;
-        lseg[(size_t)t] = sg;
+        identity = identity && k == t;
       }
       P.seg_hi.push_back(t);
     }
     P.hi = t;
   }
-  SCOUT_TRY(hipMemcpyAsync(ms.perm.p, perm.data(), (size_t)m * 4, hipMemcpyHostToDevice, c.s));
-  SCOUT_TRY(hipMemcpyAsync(ms.lseg.p, lseg.data(), (size_t)m * 4, hipMemcpyHostToDevice, c.s));
-  k_mm_gather<<<grid_for(m), kBlock, 0, c.s>>>(ch.d_uv.p, ch.d_ins.p, ms.perm.p, m, ms.uv2.p, ms.ins2.p);
-  if (ms.use_graph) SCOUT_CHECK(mm_graph(c, ms, stride));
-  std::vector<int64_t> hdr((size_t)(kSegHdr + 3 * stride), 0);
-  int64_t rounds = 0, writes = 0;
+  if (!identity) {
+    SCOUT_TRY(hipMemcpyAsync(ms.perm.p, perm.data(), (size_t)m * 4, hipMemcpyHostToDevice, c.s));
+    k_mm_gather<<<grid_for(m), kBlock, 0, c.s>>>(ch.d_uv.p, ch.d_ins.p, ms.perm.p, m, ms.uv2.p, ms.ins2.p);
+  } else {
+    SCOUT_TRY(hipMemcpyAsync(ms.uv2.p, ch.d_uv.p, (size_t)(2 * m) * 4, hipMemcpyDeviceToDevice, c.s));
+    SCOUT_TRY(hipMemcpyAsync(ms.ins2.p, ch.d_ins.p, (size_t)m, hipMemcpyDeviceToDevice, c.s));
+  }
+  if (ms.use_graph) SCOUT_CHECK(mm_graph(c, ms, sv));
+  std::vector<int64_t> tab((size_t)seg_table_size(stride), 0);
+  int64_t rounds = 0, writes = 0, evals = 0;
   for (Phase& P : phases) {
     if (ms.tag_ep >= (1 << 29)) {
       SCOUT_TRY(hipMemsetAsync(c.mis_tag, 0xff, ((size_t)n + 1) * 4, c.s));
@@ -647,60 +571,81 @@ int mis_repair_multi(Ctx& c, Channel& ch, int32_t ninst, const int64_t* off, int
     int32_t ep = ++ms.tag_ep;
     k_mis_tag_endpoints<<<grid_for(P.hi - P.lo), kBlock, 0, c.s>>>(ms.uv2.p, P.lo, P.hi, c.mis_tag, ep);
     k_mis_tag_hop1<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), ms.uv2.p, P.lo, P.hi, c.mis_tag, ep);
-    int64_t S = (int64_t)P.seg_lo.size();
-    std::fill(hdr.begin(), hdr.end(), 0);
-    hdr[0] = S;
-    hdr[1] = P.lo;
-    hdr[2] = P.hi;
-    hdr[3] = ep;
-    for (int64_t sg = 0; sg < S; sg++) {
-      hdr[(size_t)(kSegHdr + sg)] = P.seg_lo[(size_t)sg];
-      hdr[(size_t)(kSegHdr + stride + sg)] = P.seg_hi[(size_t)sg];
-      hdr[(size_t)(kSegHdr + 2 * stride + sg)] = P.seg_hi[(size_t)sg];
-    }
-    // synchronous copy: hdr is reused by the next phase
-    SCOUT_TRY(hipMemcpyAsync(ms.seg.p, hdr.data(), hdr.size() * 8, hipMemcpyHostToDevice, c.s));
-    SCOUT_TRY(hipStreamSynchronize(c.s));
-    while (true) {
-      if (ms.use_graph) SCOUT_TRY(hipGraphLaunch(ms.exec, c.s));
-      else mm_launch_group(c, ms, stride);
-      SCOUT_TRY(hipMemcpyAsync(c.h_box, ms.seg.p, kSegHdr * sizeof(int64_t), hipMemcpyDeviceToHost, c.s));
+    for (size_t b = 0; b < P.seg_lo.size(); b += (size_t)stride) {
+      // at most `stride` segments per pass (more instances than table rows run in passes)
+      size_t e = std::min(P.seg_lo.size(), b + (size_t)stride);
+      int64_t S = (int64_t)(e - b);
+      std::fill(tab.begin(), tab.end(), 0);
+      tab[0] = S;
+      tab[2] = ep;
+      int64_t slots = 0;
+      for (int64_t sg = 0; sg < S; sg++) {
+        int64_t lo = P.seg_lo[b + (size_t)sg], hi = P.seg_hi[b + (size_t)sg];
+        int64_t size = std::min(ms.window0, hi - lo);
+        tab[(size_t)(kSegHdr + sg)] = lo;
+        tab[(size_t)(kSegHdr + stride + sg)] = hi;
+        tab[(size_t)(kSegHdr + 2 * stride + sg)] = lo + size;
+        tab[(size_t)(kSegHdr + 3 * stride + sg)] = ms.window0;
+        tab[(size_t)(kSegHdr + 4 * stride + sg)] = slots;
+        slots += size;
+      }
+      tab[(size_t)(kSegHdr + 4 * stride + S)] = slots;
+      tab[1] = slots;
+      tab[3] = S;
+      // synchronous: the host table is reused by the next pass
+      SCOUT_TRY(hipMemcpyAsync(ms.seg.p, tab.data(), tab.size() * 8, hipMemcpyHostToDevice, c.s));
       SCOUT_TRY(hipStreamSynchronize(c.s));
-      c.readbacks++;
-      if (c.h_box[4] == 0) break;
+      while (true) {
+        if (ms.use_graph) SCOUT_TRY(hipGraphLaunch(ms.exec, c.s));
+        else mm_launch_group(c, ms, sv);
+        SCOUT_TRY(hipMemcpyAsync(c.h_box, ms.seg.p, kSegHdr * sizeof(int64_t), hipMemcpyDeviceToHost, c.s));
+        SCOUT_TRY(hipStreamSynchronize(c.s));
+        c.readbacks++;
+        if (c.h_box[3] == 0) break;
+      }
+      rounds += c.h_box[4];
+      writes += c.h_box[5];
+      evals += c.h_box[6];
     }
-    rounds += c.h_box[5];
-    writes += c.h_box[6];
   }
   st->rounds += rounds;
   st->conflicts += writes;
+  st->touched += evals;  // line evaluations of pass A over all rounds
   return SCOUT_OK;
 }
 
 }  // namespace scout
 
-extern "C" int scout_repair_mis_multi(scout_ctx* x, int channel, int32_t ninst, const int64_t* line_off,
-                                      int64_t max_csr, scout_stats* st) {
+using namespace scout;
+
+namespace {
+
+int mis_entry(scout_ctx* x, int channel, int32_t ninst, const int64_t* line_off, int64_t max_csr, scout_stats* st,
+              const char* name) {
   if (!x || !st || !line_off || ninst < 1) return SCOUT_ERR_ARG;
   Ctx& c = x->c;
   SCOUT_TRY(hipSetDevice(c.dev));
   Channel* ch = channel_of(c, channel);
   if (!ch) return SCOUT_ERR_ARG;
   if (!c.g.boff || !c.mis) {
-    set_error("repair_mis_multi: graph and MIS must be loaded first");
+    set_error(std::string(name) + ": graph and MIS must be loaded first");
     return SCOUT_ERR_STATE;
   }
   if (max_csr <= 0) {
-    set_error("repair_mis_multi: max_csr must be positive");
+    set_error(std::string(name) + ": max_csr must be positive");
+    return SCOUT_ERR_ARG;
+  }
+  if (ch->m >= INT32_MAX) {
+    set_error(std::string(name) + ": at most 2^31 - 2 lines per batch");
     return SCOUT_ERR_ARG;
   }
   if (line_off[0] != 0 || line_off[ninst] != ch->m) {
-    set_error("repair_mis_multi: line offsets must run from 0 to the staged line count");
+    set_error(std::string(name) + ": line offsets must run from 0 to the staged line count");
     return SCOUT_ERR_ARG;
   }
   for (int32_t i = 0; i < ninst; i++)
     if (line_off[i + 1] < line_off[i]) {
-      set_error("repair_mis_multi: line offsets must not decrease");
+      set_error(std::string(name) + ": line offsets must not decrease");
       return SCOUT_ERR_ARG;
     }
   Span span;
@@ -712,6 +657,21 @@ extern "C" int scout_repair_mis_multi(scout_ctx* x, int channel, int32_t ninst, 
   st->repair_ms += span.ms();
   st->readbacks += c.readbacks - rb0;
   return SCOUT_OK;
+}
+
+}  // namespace
+
+extern "C" int scout_repair_mis(scout_ctx* x, int channel, int64_t max_csr, scout_stats* st) {
+  if (!x) return SCOUT_ERR_ARG;
+  Channel* ch = channel_of(x->c, channel);
+  if (!ch) return SCOUT_ERR_ARG;
+  int64_t off[2] = {0, ch->m};
+  return mis_entry(x, channel, 1, off, max_csr, st, "repair_mis");
+}
+
+extern "C" int scout_repair_mis_multi(scout_ctx* x, int channel, int32_t ninst, const int64_t* line_off,
+                                      int64_t max_csr, scout_stats* st) {
+  return mis_entry(x, channel, ninst, line_off, max_csr, st, "repair_mis_multi");
 }
 
 extern "C" int scout_mis_config(scout_ctx* x, int grid, int group_rounds, int use_graph) {
