@@ -86,6 +86,10 @@ FAMILY_KERNELS = {
     "classify": ["k_dup_insert", "k_dup_mark", "k_classify"],
 }
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_rmat26.json")
+# Single kernels timed launch by launch (HIP events on the compute stream) with their
+# algorithmic bytes: profiling id -> kernel name. The roofline reports the one with the most
+# device time over the timed batches.
+ROOFLINE_KERNELS = {"k_invalidate": "bfs_invalidate_kernel", "k_apply": "apply"}
 
 
 def _kernel_short_name(name):
@@ -127,19 +131,237 @@ def summarize_pmc(pmc_dir):
                     fam[f][key] += float(r["Counter_Value"]) * 1024 * out["copy_factor"][counter]
         batches = nb
     out["batches"] = batches
+    # per dispatch of the roofline kernels (every dispatch after the first batch started)
+    per_disp = {}
+    for kname in ROOFLINE_KERNELS:
+        d = {}
+        for sub, counter, key in (("bench_fetch", "FETCH_SIZE", "fetch_bytes"),
+                                  ("bench_write", "WRITE_SIZE", "write_bytes")):
+            started, vals = False, []
+            for r in rows(sub):
+                name = _kernel_short_name(r["Kernel_Name"])
+                if name == "k_classify":
+                    started = True
+                if started and name == kname and r["Counter_Name"] == counter:
+                    vals.append(float(r["Counter_Value"]) * 1024 * out["copy_factor"][counter])
+            d[key] = round(sum(vals) / len(vals)) if vals else None
+            d["dispatches"] = len(vals)
+        if d["fetch_bytes"] is not None and d["write_bytes"] is not None:
+            d["traffic_bytes"] = d["fetch_bytes"] + d["write_bytes"]
+        per_disp[kname] = d
+    out["per_dispatch"] = per_disp
     out["per_batch"] = {f: {"fetch_bytes": round(v["fetch_bytes"] / batches), "write_bytes": round(v["write_bytes"] / batches),
                             "traffic_bytes": round((v["fetch_bytes"] + v["write_bytes"]) / batches)}
                         for f, v in fam.items()}
     return out
 
 
+def mis_max_csr(avg_degree, free_bytes):
+    """The MIS sub-batch budget as the reference sets it (src/mis/scout_mis.cu:800-815): about
+    2^21 * avg_degree^2 entries, at most 8e8, at least 1e6, and within a quarter of free memory
+    in 8-byte entries."""
+    est = int(1048576.0 * 2.0 * avg_degree * avg_degree)
+    mx = min(800_000_000, est) if est > 0 else 50_000_000
+    mx = max(mx, 1_000_000)
+    fit = int(free_bytes / 4.0 / 8.0)
+    if fit > 1_000_000 and mx > fit:
+        mx = fit
+    return mx
+
+
+def check_color_mis(eng, n):
+    """Proper colouring and independent MIS on the final logical graph (host, vectorised)."""
+    g = eng.download_graph()
+    col, mis = eng.download(sa.STATE_COLOR), eng.download(sa.STATE_MIS)
+    bad_c = bad_m = 0
+    step = 1 << 22
+    for lo in range(0, n, step):
+        hi = min(n, lo + step)
+        a, b = int(g.off[lo]), int(g.off[hi])
+        src = np.repeat(np.arange(lo, hi, dtype=np.int64), np.diff(g.off[lo:hi + 1]))
+        dst = g.col[a:b]
+        bad_c += int(np.count_nonzero(col[src] == col[dst]))
+        bad_m += int(np.count_nonzero(mis[src].astype(bool) & mis[dst].astype(bool)))
+    return bad_c == 0, bad_m == 0
+
+
+def cpu_baseline_color_mis(eng, n, uv_host, max_csr, seconds):
+    """The reference's colouring and MIS batch repair (oracle restatement, one thread) on chunks
+    of the first timed batch, until `seconds` of CPU time are used."""
+    from oracle_lib import OracleGraph  # the checker's library, timed here as the baseline
+    g = eng.download_graph()
+    col, mis = eng.download(sa.STATE_COLOR), eng.download(sa.STATE_MIS)
+    og = OracleGraph(n=n, off=g.off, col=g.col)
+    del g
+    done, t_c, t_m = 0, 0.0, 0.0
+    chunk = 500
+    while done < len(uv_host) and t_c + t_m < seconds:
+        uv = np.ascontiguousarray(uv_host[done:done + chunk])
+        ins = og.classify(uv)
+        og.apply(uv, ins)
+        t0 = time.perf_counter()
+        col, _ = og.color_batch(uv, ins, col)
+        t1 = time.perf_counter()
+        mis, _ = og.mis_batch(uv, ins, mis, max_csr)
+        t2 = time.perf_counter()
+        t_c += t1 - t0
+        t_m += t2 - t1
+        done += len(uv)
+    return {"value": done / max(t_c + t_m, 1e-9), "unit": "edge updates/s", "cores": 1, "kind": "port",
+            "sample": f"first {done} lines of the first timed batch in chunks of {chunk}, the reference's colouring "
+                      f"(speculative first-fit rounds) and MIS (2-hop rules, sub-batch cuts) restated on the CPU; "
+                      f"colour {t_c:.1f}s, mis {t_m:.1f}s"}
+
+
+def run_color_mis(args):
+    """Colouring (1-hop) + MIS (2-hop) repair on an R-MAT stream with degree-biased endpoints."""
+    scale = args.scale or 24
+    bsize = args.batch or 100_000
+    total = args.warmup + args.steps
+    eng = sa.Engine(0)
+    t0 = time.perf_counter()
+    eng.gen_rmat(scale, 16, 0.57, 0.19, 0.19, args.seed)
+    info = eng.graph_info()
+    n = info["n"]
+    eng.color_greedy()
+    eng.mis_greedy()
+    eng.sync()
+    log(f"graph: n={n} entries={info['base_entries']}, greedy colouring + MIS ({time.perf_counter() - t0:.1f}s)")
+    free, _ = eng.mem_info()
+    max_csr = mis_max_csr(info["base_entries"] / n, free)
+    if args.no_graph:
+        # the same kernels launched one by one (for kernel traces)
+        cfg = eng.color_get_config()
+        eng.color_config(cfg["thread_max"], cfg["block_min"], cfg["grid"], use_graph=False)
+        eng.mis_config(grid=1024, group_rounds=16, use_graph=False)
+    eng.gen_mode(sa.GEN_DEGREE)
+    eng.snapshot()
+    t0 = time.perf_counter()
+    for b in range(total):  # the stream depends only on the graph
+        eng.gen_batch(bsize, 0.5, 0.0, args.seed, expected_total=bsize * total)
+        eng.batch_store(b)
+        eng.classify()
+        eng.apply()
+    eng.sync()
+    log(f"stream of {total} batches generated in {time.perf_counter() - t0:.1f}s")
+
+    def replay(profile):
+        eng.restore()
+        for b in range(args.warmup):
+            eng.batch_load(b)
+            eng.classify()
+            eng.apply()
+            sc = eng.repair_color()
+            sm = eng.repair_mis(max_csr)
+            log(f"warmup batch {b}: colour {sc.repair_ms:.1f} ms, MIS {sm.repair_ms:.1f} ms ({sm.rounds} rounds)")
+        eng.sync()
+        if profile:
+            eng.prof_enable(True)
+        lat, lines = [], 0
+        work = {"color_rounds": 0, "color_recoloured": 0, "mis_rounds": 0, "mis_sub_batches": 0,
+                "mis_line_evaluations": 0, "readbacks": 0}
+        t_start = time.perf_counter()
+        for b in range(args.warmup, total):
+            ts = time.perf_counter()
+            eng.batch_load(b)
+            eng.classify()
+            eng.apply()
+            sc = eng.repair_color()
+            sm = eng.repair_mis(max_csr)
+            eng.sync()
+            lat.append((time.perf_counter() - ts) * 1e3)
+            lines += sc.lines
+            work["color_rounds"] += sc.rounds
+            work["color_recoloured"] += sc.touched
+            work["mis_rounds"] += sm.rounds
+            work["mis_sub_batches"] += sm.sub_batches
+            work["mis_line_evaluations"] += sm.touched
+            work["readbacks"] += sc.readbacks + sm.readbacks
+            log(f"batch {b}: {lat[-1]:.1f} ms (colour {sc.repair_ms:.1f} ms, {sc.rounds} rounds; MIS "
+                f"{sm.repair_ms:.1f} ms, {sm.rounds} rounds, {sm.sub_batches} sub-batches)")
+        elapsed = time.perf_counter() - t_start
+        prof = None
+        if profile:
+            prof = {name: eng.prof_read(i) for i, name in enumerate(sa.PROF_NAMES)}
+            eng.prof_enable(False)
+        return {"ups": lines / elapsed, "elapsed": elapsed, "lat": lat, "work": work, "prof": prof,
+                "state": (eng.download(sa.STATE_COLOR), eng.download(sa.STATE_MIS))}
+
+    head = replay(profile=True)
+    log(f"colour + MIS: {head['ups']:.4g} updates/s, p50 {np.percentile(head['lat'], 50):.3f} ms")
+    verified = None
+    if not args.no_verify:
+        proper, independent = check_color_mis(eng, n)
+        # the same stream with the colouring split differently between threads, waves and
+        # blocks and both round loops launched kernel by kernel: identical states
+        eng.color_config(thread_max=0, block_min=64, grid=512, use_graph=False)
+        eng.mis_config(grid=512, group_rounds=3, use_graph=False)
+        other = replay(profile=False)
+        same = all(np.array_equal(x, y) for x, y in zip(head["state"], other["state"]))
+        verified = bool(proper and independent and same)
+        log(f"proper colouring {proper}, independent MIS {independent}, launch settings agree {same}")
+    prof = head["prof"]
+    cpu = None
+    if not args.no_cpu:
+        eng.restore()
+        for b in range(args.warmup):
+            eng.batch_load(b)
+            eng.classify()
+            eng.apply()
+            eng.repair_color()
+            eng.repair_mis(max_csr)
+        m = eng.batch_load(args.warmup)
+        uv_host = eng.batch_download(m)
+        cpu = cpu_baseline_color_mis(eng, n, uv_host, max_csr, args.cpu_seconds)
+        log(f"cpu baseline: {cpu['value']:.0f} updates/s")
+    steps = args.steps
+    fam = {k: {"ms_per_batch": round(v["ms"] / steps, 4), "launches": v["launches"],
+               "GBps": round(v["bytes"] / (v["ms"] / 1e3) / 1e9, 1) if v["ms"] > 0 else 0.0}
+           for k, v in prof.items() if v["launches"]}
+    d = prof["color"]
+    nl = max(d["launches"], 1)
+    achieved = (d["bytes"] / nl) / (d["ms"] / 1e3 / nl) / 1e9 if d["ms"] > 0 else 0.0
+    out = {
+        "metric": "edge updates/s + p99 batch-commit latency, colouring + MIS repair (bit-exact)",
+        "value": round(head["ups"], 1),
+        "unit": "edge updates/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(head["elapsed"] * 1e3 / steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic (R-MAT and degree-biased update stream generated on the device, fixed seeds)",
+        "config": {"workload": f"colouring (1-hop) + MIS (2-hop) batch repair, R-MAT scale {scale} ef 16, {bsize} "
+                               f"toggles/batch (50% deletions, endpoints proportional to degree + 1)",
+                   "scale": scale, "vertices": n, "adjacency_entries": info["base_entries"], "batch": bsize,
+                   "max_csr": max_csr},
+        "p50_ms": round(float(np.percentile(head["lat"], 50)), 3),
+        "p99_ms": round(float(np.percentile(head["lat"], 99)), 3),
+        "verified_exact": verified,
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "colouring rounds (one repair's launches)", "launches": d["launches"],
+                     "avg_launch_ms": round(d["ms"] / nl, 4),
+                     "algorithmic_bytes_per_launch": round(d["bytes"] / nl), "batch_families": fam},
+        "work_per_batch": {k: round(v / steps, 1) for k, v in head["work"].items()},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--workload", choices=["bfs-cc", "color-mis"], default="bfs-cc",
+                    help="bfs-cc: BFS(0)+CC at R-MAT scale 26 (the default line); color-mis: colouring + MIS "
+                         "at R-MAT scale 24, 100k degree-biased toggles per batch")
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--scale", type=int, default=26)
-    ap.add_argument("--batch", type=int, default=1_000_000)
+    ap.add_argument("--scale", type=int, default=None, help="R-MAT scale (26 for bfs-cc, 24 for color-mis)")
+    ap.add_argument("--batch", type=int, default=None, help="toggles per batch (1M for bfs-cc, 100k for color-mis)")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--strategy", choices=["local", "recompute"], default="local",
                     help="plain strategy the learned scheduler is compared with")
@@ -149,6 +371,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-verify", action="store_true")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="color-mis: launch the repair rounds kernel by kernel instead of replaying captured graphs")
     ap.add_argument("--summarize-pmc", metavar="DIR",
                     help="only summarise rocprofv3 --pmc passes under DIR (copy_fetch, copy_write, bench_fetch, "
                          "bench_write) into per-batch HBM traffic per kernel family, print it and exit")
@@ -162,6 +386,11 @@ def main():
     if args.summarize_pmc:
         print(json.dumps(summarize_pmc(args.summarize_pmc), indent=1))
         return
+    if args.workload == "color-mis":
+        run_color_mis(args)
+        return
+    args.scale = args.scale or 26
+    args.batch = args.batch or 1_000_000
     if args.calibrate_copy:
         import torch
         a = torch.ones(args.calibrate_copy << 20, dtype=torch.uint8, device="cuda")
@@ -285,22 +514,29 @@ def main():
             verified = verified and all(np.array_equal(x, y) for x, y in zip(plain["state"], learned["state"]))
         log(f"final state equals full recompute{' and the plain repair' if learned else ''}: {verified}")
 
-    dom = max(prof, key=lambda k: prof[k]["ms"])
-    d = prof[dom]
-    achieved = d["bytes"] / (d["ms"] / 1e3) / 1e9 if d["ms"] > 0 else 0.0
-    # traffic: HBM bytes per batch of the dominant family, from the committed rocprofv3 --pmc
-    # summary of this workload (scaled FETCH_SIZE + WRITE_SIZE), next to its algorithmic bytes
+    # the dominant single kernel: each of its launches bracketed by HIP events on the compute
+    # stream; achieved = algorithmic bytes per launch / average launch duration
+    kname = max(ROOFLINE_KERNELS, key=lambda k: prof[ROOFLINE_KERNELS[k]]["ms"])
+    d = prof[ROOFLINE_KERNELS[kname]]
+    nl = max(d["launches"], 1)
+    avg_s = d["ms"] / 1e3 / nl
+    bytes_per_launch = d["bytes"] / nl
+    achieved = bytes_per_launch / avg_s / 1e9 if avg_s > 0 else 0.0
+    # traffic: HBM bytes per dispatch of that kernel from the committed rocprofv3 --pmc summary
+    # of this workload (FETCH_SIZE + WRITE_SIZE, scaled by a plain copy's readings)
     traffic = None
     if os.path.exists(TRAFFIC_FILE) and args.scale == 26 and args.batch == 1_000_000:
         with open(TRAFFIC_FILE) as f:
-            t = json.load(f)["per_batch"].get(dom)
-        if t:
+            t = json.load(f).get("per_dispatch", {}).get(kname)
+        if t and t.get("traffic_bytes") is not None:
             traffic = t["traffic_bytes"]
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom,
-                "launch": "one batch's work of the family", "launches": args.steps,
-                "avg_launch_ms": round(d["ms"] / args.steps, 4),
-                "algorithmic_bytes_per_launch": round(d["bytes"] / args.steps)}
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": kname,
+                "launches": d["launches"], "avg_launch_ms": round(d["ms"] / nl, 4),
+                "algorithmic_bytes_per_launch": round(bytes_per_launch),
+                "batch_families": {k: {"ms_per_batch": round(v["ms"] / args.steps, 4),
+                                       "GBps": round(v["bytes"] / (v["ms"] / 1e3) / 1e9, 1) if v["ms"] > 0 else 0.0}
+                                   for k, v in prof.items() if v["launches"]}}
 
     cpu = None
     if not args.no_cpu and first_batch_host is not None:

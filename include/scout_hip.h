@@ -191,6 +191,13 @@ int scout_gen_rmat(scout_ctx* ctx, int scale, int edge_factor, double a, double 
  * receives the line count. */
 int scout_gen_batch(scout_ctx* ctx, int channel, int64_t size, double delete_frac, double tree_frac, uint64_t seed,
                     int64_t expected_total, int64_t* m);
+/* Endpoint distribution of the generated deletions and insertions (tree deletions are not
+ * affected): SCOUT_GEN_UNIFORM = deletions uniform over adjacency entries, insertions uniform
+ * pairs; SCOUT_GEN_DEGREE = every endpoint drawn with probability proportional to degree + 1
+ * (a deletion takes a uniform live neighbour of its first endpoint), the convention of the
+ * "degree" streams of the host generator. */
+enum { SCOUT_GEN_UNIFORM = 0, SCOUT_GEN_DEGREE = 1 };
+int scout_gen_mode(scout_ctx* ctx, int mode);
 /* Copies a staged batch back to the host (cap_lines pairs at most). */
 int scout_batch_download(scout_ctx* ctx, int channel, int32_t* uv, int64_t cap_lines);
 /* Keeps a device copy of a channel's staged batch in slot k / stages slot k into a channel
@@ -226,7 +233,9 @@ enum {
   SCOUT_PROF_CLASSIFY = 4,
   SCOUT_PROF_COLOR = 5,
   SCOUT_PROF_MIS_EVAL = 6,
-  SCOUT_PROF_CC_HOOK = 7
+  SCOUT_PROF_CC_HOOK = 7,
+  /* the k_invalidate launches alone (wave per medium-length candidate row) */
+  SCOUT_PROF_BFS_INVALIDATE_KERNEL = 8
 };
 int scout_prof_enable(scout_ctx* ctx, int on);
 int scout_prof_read(scout_ctx* ctx, int id, double* total_ms, int64_t* launches, int64_t* bytes);

@@ -80,6 +80,7 @@ _SIGNATURES = {
     "scout_gen_batch": [_P, ctypes.c_int, _I64, ctypes.c_double, ctypes.c_double, ctypes.c_uint64, _I64,
                         ctypes.POINTER(_I64)],
     "scout_batch_download": [_P, ctypes.c_int, _P, _I64],
+    "scout_gen_mode": [_P, ctypes.c_int],
     "scout_batch_store": [_P, ctypes.c_int, ctypes.c_int],
     "scout_batch_load": [_P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_I64)],
     "scout_snapshot": [_P],
@@ -101,10 +102,12 @@ _SIGNATURES = {
 }
 
 MLP_FEATURES, MLP_OUTPUTS, MLP_PARAMS = 12, 3, 5187
+GEN_UNIFORM, GEN_DEGREE = 0, 1
 
 PROF_BFS_RELAX, PROF_BFS_INVALIDATE, PROF_CC_EXPAND, PROF_APPLY, PROF_CLASSIFY = 0, 1, 2, 3, 4
-PROF_COLOR, PROF_MIS_EVAL, PROF_CC_HOOK = 5, 6, 7
-PROF_NAMES = ["bfs_relax", "bfs_invalidate", "cc_expand", "apply", "classify", "color", "mis_eval", "cc_hook"]
+PROF_COLOR, PROF_MIS_EVAL, PROF_CC_HOOK, PROF_BFS_INVALIDATE_KERNEL = 5, 6, 7, 8
+PROF_NAMES = ["bfs_relax", "bfs_invalidate", "cc_expand", "apply", "classify", "color", "mis_eval", "cc_hook",
+              "bfs_invalidate_kernel"]
 
 _lib = None
 
@@ -205,6 +208,12 @@ class Engine:
     def gen_rmat(self, scale: int, edge_factor: int = 16, a=0.57, b=0.19, c=0.19, seed: int = 1):
         self._check(self.lib.scout_gen_rmat(self.h, scale, edge_factor, a, b, c, seed), "gen_rmat")
         self.n = 1 << scale
+
+    def mem_info(self):
+        """(free, total) device memory in bytes."""
+        free, total = ctypes.c_int64(), ctypes.c_int64()
+        self._check(self.lib.scout_mem_info(self.h, ctypes.byref(free), ctypes.byref(total)), "mem_info")
+        return free.value, total.value
 
     def graph_info(self) -> dict:
         out = np.zeros(8, dtype=np.int64)
@@ -324,6 +333,10 @@ class Engine:
                                              ctypes.byref(m)), "gen_batch")
         self._staged[channel] = m.value
         return m.value
+
+    def gen_mode(self, mode: int):
+        """GEN_UNIFORM (default) or GEN_DEGREE: endpoints proportional to degree + 1."""
+        self._check(self.lib.scout_gen_mode(self.h, mode), "gen_mode")
 
     def batch_download(self, m, channel=0) -> np.ndarray:
         out = np.zeros((max(m, 1), 2), dtype=np.int32)

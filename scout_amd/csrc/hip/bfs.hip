@@ -274,23 +274,28 @@ __global__ void k_invalidate_short(GView g, int32_t* level, int32_t* parent, con
 }
 
 // Wave per medium candidate.
+// own[0] += entries read (a second pass over the row of an invalid candidate counts again),
+// own[1] += rows.
 __global__ void k_invalidate(GView g, int32_t* level, int32_t* parent, const int32_t* cur, const int64_t* cur_count,
                              int32_t lvl, int32_t* inval, int64_t* inval_count, int64_t inval_cap, int32_t* next,
                              int64_t* next_count, int64_t cap, int32_t* stamp, int32_t epoch, int64_t* scanned,
-                             Attrib ab) {
+                             int64_t* own, Attrib ab) {
   int64_t nc = *cur_count;
-  int64_t work = 0;
+  int64_t work = 0, reads = 0, rows = 0;
   for (int64_t i = global_wave(); i < nc; i += total_waves()) {
     int32_t c = cur[i];
     Row r = row_of(g, c);
     int64_t len = row_len(r);
     int64_t at = wave_find_first(g, r, 0, len, [&](int32_t x) { return level[x] == lvl - 1; });
+    rows++;
     if (at >= 0) {
       work += at + 1;
+      reads += at + 1;
       if (lane_id() == 0) parent[c] = row_at(g, r, at);
       continue;
     }
     work += len;
+    reads += 2 * len;
     if (lane_id() == 0) {
       level[c] = kInvalid;
       attrib_invalid(ab, c, lvl);
@@ -308,7 +313,11 @@ __global__ void k_invalidate(GView g, int32_t* level, int32_t* parent, const int
       wave_append(take, w, next, next_count, cap);
     }
   }
-  if (lane_id() == 0 && work) atomicAdd((unsigned long long*)scanned, (unsigned long long)work);
+  if (lane_id() == 0 && work) {
+    atomicAdd((unsigned long long*)scanned, (unsigned long long)work);
+    atomicAdd((unsigned long long*)&own[0], (unsigned long long)reads);
+    atomicAdd((unsigned long long*)&own[1], (unsigned long long)rows);
+  }
 }
 
 // Chunk tasks (index in `longs` << 32 | chunk) for every long candidate.
@@ -759,7 +768,7 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, int32_t* P, int strategy, scou
     int64_t* binval = box + 10;
     int64_t* bscan = box + 11;
     k_zero<<<1, 64, 0, c.s>>>(box + 8, 7);
-    k_zero<<<1, 64, 0, c.s>>>(box + 18, 3);
+    k_zero<<<1, 64, 0, c.s>>>(box + 18, 5);
     int p = 0;
     int group = first_group;
     int32_t lvl = cmin;
@@ -775,8 +784,10 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, int32_t* P, int strategy, scou
         k_invalidate_short<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, P, cur, bcur, lvl, c.ld.p, binval, nxt,
                                                            bnext, cap, c.stamp, sepoch, c.le.p, box + 12, c.lf.p,
                                                            box + 13, bscan, ab);
+        SCOUT_CHECK(prof_begin(c, PROF_BFS_INVALIDATE_KERNEL));
         k_invalidate<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, P, c.le.p, box + 12, lvl, c.ld.p, binval, cap,
-                                                     nxt, bnext, cap, c.stamp, sepoch, bscan, ab);
+                                                     nxt, bnext, cap, c.stamp, sepoch, bscan, box + 21, ab);
+        SCOUT_CHECK(prof_end(c, PROF_BFS_INVALIDATE_KERNEL, 0));
         k_long_tasks<<<64, kBlock, 0, c.s>>>(c.g.view(), c.lf.p, box + 13, c.wb.p, box + 14, c.wb.cap);
         int32_t supepoch = next_epoch(c);
         k_inval_search<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, P, lvl, c.lf.p, c.wb.p, box + 14, c.wb.cap,
@@ -789,7 +800,7 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, int32_t* P, int strategy, scou
         p ^= 1;
       }
       SCOUT_CHECK(prof_end(c, PROF_BFS_INVALIDATE, 0));
-      SCOUT_TRY(hipMemcpyAsync(c.h_box + 8, box + 8, 13 * sizeof(int64_t), hipMemcpyDeviceToHost, c.s));
+      SCOUT_TRY(hipMemcpyAsync(c.h_box + 8, box + 8, 15 * sizeof(int64_t), hipMemcpyDeviceToHost, c.s));
       SCOUT_TRY(hipStreamSynchronize(c.s));
       c.readbacks++;
       if (std::max(c.h_box[14], c.h_box[19]) > c.wb.cap) {
@@ -803,6 +814,9 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, int32_t* P, int strategy, scou
     // per candidate its id, level, row offsets and queue entry (40 B); per scanned entry the
     // neighbour and its level (8 B)
     prof_bytes(c, PROF_BFS_INVALIDATE, 40 * c.h_box[18] + 8 * c.h_box[11]);
+    // k_invalidate alone: per row its id, offsets, level and parent write (36 B); per entry
+    // read the neighbour and its level, or on the children pass its parent and level (8 B)
+    prof_bytes(c, PROF_BFS_INVALIDATE_KERNEL, 36 * c.h_box[22] + 8 * c.h_box[21]);
     nx = c.h_box[10];
     scanned += c.h_box[11];
     if (nx > 0) {

@@ -247,6 +247,23 @@ __device__ __forceinline__ void wave_walk(const GView& g, const Row& r, int64_t 
   }
 }
 
+// wave_walk with kU entries per lane loaded before any is used (kU loads in flight per lane
+// on long rows).
+template <int kU, class F>
+__device__ __forceinline__ void wave_walk_wide(const GView& g, const Row& r, int64_t lo, int64_t hi, F&& f) {
+  for (int64_t base = lo; base < hi; base += (int64_t)kU * kWave) {
+    int32_t xs[kU];
+#pragma unroll
+    for (int j = 0; j < kU; j++) {
+      int64_t k = base + (int64_t)j * kWave + lane_id();
+      xs[j] = k < hi ? row_at(g, r, k) : -1;
+    }
+#pragma unroll
+    for (int j = 0; j < kU; j++)
+      if (xs[j] >= 0) f(xs[j]);
+  }
+}
+
 // Wave searches entries [lo, hi) of row r for a live neighbour satisfying pred. All lanes
 // return the same answer: the index of the first such entry, or -1.
 template <class P>
@@ -294,6 +311,26 @@ struct PairState {
   bool added_tomb;    // inserted row u holds the tombstone ~v
   int64_t lo, hi;     // base entries of row u searched for v (base_range)
 };
+
+// pair_state over a base range [lo, hi) of row u already located.
+__device__ __forceinline__ PairState pair_state_in(const GView& g, int32_t u, int32_t v, int64_t lo, int64_t hi) {
+  PairState s{false, false, false, false, false, 0, 0};
+  s.lo = lo;
+  s.hi = hi;
+  for (int64_t k = lo; k < hi; k++) {
+    int32_t x = g.bcol[k];
+    if (x == v) s.base_live = true;
+    else if (x == ~v) s.base_tomb = true;
+  }
+  s.base_any = s.base_live || s.base_tomb;
+  int64_t ab = g.aoff[u], ae = ab + g.alen[u];
+  for (int64_t k = ab; k < ae; k++) {
+    int32_t x = g.acol[k];
+    if (x == v) s.added_live = true;
+    else if (x == ~v) s.added_tomb = true;
+  }
+  return s;
+}
 
 __device__ __forceinline__ PairState pair_state(const GView& g, bool sorted, int32_t u, int32_t v) {
   PairState s{false, false, false, false, false, 0, 0};

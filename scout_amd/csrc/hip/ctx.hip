@@ -396,6 +396,7 @@ int scout_state_upload(scout_ctx* x, int kind, const void* host, int64_t n) {
               : kind == SCOUT_STATE_LEVEL ? (void*)c.level
                                           : (void*)c.label;
   size_t elem = kind == SCOUT_STATE_MIS ? 1 : 4;
+  if (kind == SCOUT_STATE_MIS) c.mis_version++;
   SCOUT_TRY(hipMemcpyAsync(dst, host, (size_t)n * elem, hipMemcpyHostToDevice, c.s));
   SCOUT_TRY(hipStreamSynchronize(c.s));
   // uploaded levels are taken as exact for the current graph; their parents are derived
@@ -468,7 +469,7 @@ int scout_stage_batch(scout_ctx* x, int channel, const int32_t* uv, int64_t m) {
     SCOUT_TRY(hipMemcpyAsync(ch->d_uv.p, ch->h_uv.p, (size_t)(2 * m) * sizeof(int32_t), hipMemcpyHostToDevice, c.cs));
   }
   SCOUT_TRY(hipEventRecord(ch->staged, c.cs));
-  ch->m = m;
+  channel_staged(*ch, m);
   return SCOUT_OK;
 }
 
@@ -484,7 +485,7 @@ int scout_stage_batch_device(scout_ctx* x, int channel, const int32_t* d_uv, int
   if (m > 0)
     SCOUT_TRY(hipMemcpyAsync(ch->d_uv.p, d_uv, (size_t)(2 * m) * sizeof(int32_t), hipMemcpyDeviceToDevice, c.cs));
   SCOUT_TRY(hipEventRecord(ch->staged, c.cs));
-  ch->m = m;
+  channel_staged(*ch, m);
   return SCOUT_OK;
 }
 

@@ -1,8 +1,8 @@
 // gen.hip: inputs built in HBM -- R-MAT graphs bit-identical to the host generator
 // (csrc/host/gen.h), and greedy-ascending initial MIS / colouring (the closed forms of
-// tools/compute_initial_mis.py:32-43 and tools/greedy_color.cpp:37-52) computed by rounds of
-// Jones-Plassmann with vertex-id priority: a vertex decides once every lower-id neighbour
-// has, which is exactly the ascending greedy order.
+// tools/compute_initial_mis.py:32-43 and tools/greedy_color.cpp:37-52) computed in rounds
+// with vertex-id priority: a vertex decides once every lower-id neighbour has, which is
+// exactly the ascending greedy order.
 #include <algorithm>
 #include <vector>
 
@@ -96,74 +96,95 @@ __global__ void k_place_by_prefix(const uint64_t* keys, int64_t count, const int
   }
 }
 
-// Greedy-ascending MIS by rounds: state 0 undecided, 1 member, 2 not member.
-__global__ void k_greedy_mis_round(GView g, uint8_t* state, int64_t* undecided) {
-  int64_t left = 0;
-  for (int64_t v = global_wave(); v < g.n; v += total_waves()) {
-    if (state[v] != 0) continue;
-    Row r = row_of(g, (int32_t)v);
-    int64_t len = row_len(r);
-    bool in_lower = false, open_lower = false;
-    for (int64_t k = lane_id(); k < len; k += kWave) {
-      int32_t w = row_at(g, r, k);
-      if (w >= 0 && w < v) {
-        uint8_t s = state[w];
-        in_lower |= (s == 1);
-        open_lower |= (s == 0);
+// Greedy-ascending MIS and colouring in topological order of the id-ordered DAG (Kahn): a
+// vertex becomes ready once all its lower-id neighbours are decided (a count of undecided
+// lower neighbours reaches zero); ready vertices decide together, then release their higher-id
+// neighbours. Rounds = the longest increasing path; work = O(entries) in total.
+
+// cnt[v] = live entries w < v of v's row; v with none is ready at once.
+__global__ void k_lower_count(GView g, int32_t* cnt, int32_t* ready, int64_t* ready_count, int64_t cap) {
+  for (int64_t base = (int64_t)blockIdx.x * kWavesPerBlock; base < g.n; base += (int64_t)gridDim.x * kWavesPerBlock) {
+    int64_t v = base + threadIdx.x / kWave;
+    int64_t c = 0;
+    if (v < g.n) {
+      Row r = row_of(g, (int32_t)v);
+      int64_t len = row_len(r);
+      for (int64_t k = lane_id(); k < len; k += kWave) {
+        int32_t w = row_at(g, r, k);
+        c += (w >= 0 && w < v);
       }
     }
-    in_lower = __ballot(in_lower) != 0;
-    open_lower = __ballot(open_lower) != 0;
-    if (lane_id() == 0) {
-      if (in_lower) state[v] = 2;
-      else if (!open_lower) state[v] = 1;
-      else left++;
+    c = wave_sum(c);
+    bool take = v < g.n && c == 0 && lane_id() == 0;
+    if (v < g.n && lane_id() == 0) cnt[v] = (int32_t)c;
+    block_append(take, (int32_t)v, ready, ready_count, cap);
+  }
+}
+
+// Wave per ready vertex: MIS member iff no lower-id neighbour is a member (state 1 member,
+// 2 not); colouring: first fit over the lower-id neighbours' colours.
+__global__ void k_greedy_decide(GView g, const int32_t* ready, const int64_t* ready_count, uint8_t* state,
+                                int32_t* color) {
+  int64_t nr = *ready_count;
+  for (int64_t i = global_wave(); i < nr; i += total_waves()) {
+    int32_t v = ready[i];
+    Row r = row_of(g, v);
+    int64_t len = row_len(r);
+    if (state) {
+      bool in_lower = false;
+      for (int64_t k = lane_id(); k < len && !in_lower; k += kWave) {
+        int32_t w = row_at(g, r, k);
+        in_lower = w >= 0 && w < v && state[w] == 1;
+        in_lower = __ballot(in_lower) != 0;
+      }
+      if (lane_id() == 0) state[v] = in_lower ? 2 : 1;
+    } else {
+      int32_t cbase = 0;
+      while (true) {
+        unsigned long long mask = 0;
+        for (int64_t k = lane_id(); k < len; k += kWave) {
+          int32_t w = row_at(g, r, k);
+          if (w >= 0 && w < v) {
+            int32_t cw = color[w] - cbase;
+            if (cw >= 0 && cw < 64) mask |= 1ULL << cw;
+          }
+        }
+        for (int off = kWave / 2; off > 0; off >>= 1) mask |= __shfl_xor(mask, off);
+        if (~mask != 0ULL) {
+          if (lane_id() == 0) color[v] = cbase + (__ffsll((long long)~mask) - 1);
+          break;
+        }
+        cbase += 64;
+      }
     }
   }
-  if (lane_id() == 0 && left) atomicAdd((unsigned long long*)undecided, (unsigned long long)left);
+}
+
+// Wave per decided vertex: every higher-id neighbour loses one undecided lower neighbour; the
+// ones reaching zero are ready next round.
+__global__ void k_greedy_release(GView g, const int32_t* ready, const int64_t* ready_count, int32_t* cnt,
+                                 int32_t* next, int64_t* next_count, int64_t cap) {
+  int64_t nr = *ready_count;
+  for (int64_t i = global_wave(); i < nr; i += total_waves()) {
+    int32_t v = ready[i];
+    Row r = row_of(g, v);
+    int64_t len = row_len(r);
+    for (int64_t base = 0; base < len; base += kWave) {
+      int64_t k = base + lane_id();
+      bool take = false;
+      int32_t w = -1;
+      if (k < len) {
+        w = row_at(g, r, k);
+        take = w > v && atomicSub(&cnt[w], 1) == 1;
+      }
+      wave_append(take, w, next, next_count, cap);
+    }
+  }
 }
 
 __global__ void k_mis_from_state(const uint8_t* state, int32_t n, uint8_t* mis) {
   int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (v < n) mis[v] = state[v] == 1 ? 1 : 0;
-}
-
-// Greedy-ascending colouring by rounds: colour v once every lower-id neighbour has a colour,
-// with the smallest colour none of them uses (wave bitmap over windows of 64 colours).
-__global__ void k_greedy_color_round(GView g, int32_t* color, int64_t* undecided) {
-  int64_t left = 0;
-  for (int64_t v = global_wave(); v < g.n; v += total_waves()) {
-    if (color[v] >= 0) continue;
-    Row r = row_of(g, (int32_t)v);
-    int64_t len = row_len(r);
-    bool open = false;
-    for (int64_t k = lane_id(); k < len; k += kWave) {
-      int32_t w = row_at(g, r, k);
-      if (w >= 0 && w < v && color[w] < 0) open = true;
-    }
-    if (__ballot(open)) {
-      if (lane_id() == 0) left++;
-      continue;
-    }
-    int32_t base = 0;
-    while (true) {
-      unsigned long long mask = 0;
-      for (int64_t k = lane_id(); k < len; k += kWave) {
-        int32_t w = row_at(g, r, k);
-        if (w >= 0 && w < v) {
-          int32_t cw = color[w] - base;
-          if (cw >= 0 && cw < 64) mask |= 1ULL << cw;
-        }
-      }
-      for (int off = kWave / 2; off > 0; off >>= 1) mask |= __shfl_xor(mask, off);
-      if (~mask != 0ULL) {
-        if (lane_id() == 0) color[v] = base + (__ffsll((long long)~mask) - 1);
-        break;
-      }
-      base += 64;
-    }
-  }
-  if (lane_id() == 0 && left) atomicAdd((unsigned long long*)undecided, (unsigned long long)left);
 }
 
 __global__ void k_fill_i32(int32_t* p, int64_t count, int32_t value) {
@@ -255,9 +276,44 @@ int gen_rmat(Ctx& c, int scale, int ef, double a, double b, double cc, uint64_t 
   (void)hipFree(bins);
   (void)hipFree(d_bin_bucket);
   g.E = E;
+  g.base_gen++;
   SCOUT_CHECK(reset_inserted(c));
   return check_rows(c);
 }
+
+namespace {
+
+// Kahn rounds over the id-ordered DAG; state (MIS) or color (colouring) receives the result.
+int greedy_rounds(Ctx& c, uint8_t* state, int32_t* color) {
+  int32_t n = c.g.n;
+  int64_t cap = (int64_t)n + 1;
+  DevBuf<int32_t> cnt, la, lb;
+  SCOUT_CHECK(cnt.reserve(cap));
+  SCOUT_CHECK(la.reserve(cap));
+  SCOUT_CHECK(lb.reserve(cap));
+  int64_t* box = c.d_box;
+  SCOUT_TRY(hipMemsetAsync(box, 0, 2 * sizeof(int64_t), c.s));
+  k_lower_count<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), cnt.p, la.p, box, cap);
+  int32_t* cur = la.p;
+  int32_t* nxt = lb.p;
+  int p = 0;
+  while (true) {
+    SCOUT_CHECK(box_read(c, 2));
+    if (c.h_box[p] == 0) break;
+    SCOUT_TRY(hipMemsetAsync(box + (p ^ 1), 0, sizeof(int64_t), c.s));
+    k_greedy_decide<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), cur, box + p, state, color);
+    k_greedy_release<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), cur, box + p, cnt.p, nxt, box + (p ^ 1), cap);
+    std::swap(cur, nxt);
+    p ^= 1;
+  }
+  SCOUT_TRY(hipStreamSynchronize(c.s));
+  cnt.release();
+  la.release();
+  lb.release();
+  return SCOUT_OK;
+}
+
+}  // namespace
 
 int greedy_mis(Ctx& c) {
   int32_t n = c.g.n;
@@ -265,12 +321,8 @@ int greedy_mis(Ctx& c) {
   DevBuf<uint8_t> state;
   SCOUT_CHECK(state.reserve((int64_t)n + 1));
   SCOUT_TRY(hipMemsetAsync(state.p, 0, (size_t)n + 1, c.s));
-  while (true) {
-    SCOUT_TRY(hipMemsetAsync(c.d_box, 0, 8, c.s));
-    k_greedy_mis_round<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), state.p, c.d_box);
-    SCOUT_CHECK(box_read(c, 1));
-    if (c.h_box[0] == 0) break;
-  }
+  SCOUT_CHECK(greedy_rounds(c, state.p, nullptr));
+  c.mis_version++;
   k_mis_from_state<<<grid_for(n), kBlock, 0, c.s>>>(state.p, n, c.mis);
   SCOUT_TRY(hipStreamSynchronize(c.s));
   state.release();
@@ -281,13 +333,7 @@ int greedy_color(Ctx& c) {
   int32_t n = c.g.n;
   SCOUT_CHECK(ensure_state(c, SCOUT_STATE_COLOR));
   k_fill_i32<<<grid_for(n), kBlock, 0, c.s>>>(c.color, n, -1);
-  while (true) {
-    SCOUT_TRY(hipMemsetAsync(c.d_box, 0, 8, c.s));
-    k_greedy_color_round<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), c.color, c.d_box);
-    SCOUT_CHECK(box_read(c, 1));
-    if (c.h_box[0] == 0) break;
-  }
-  return SCOUT_OK;
+  return greedy_rounds(c, nullptr, c.color);
 }
 
 }  // namespace scout

@@ -114,13 +114,18 @@ __global__ void k_dup_mark(const int32_t* uv, int64_t m, const uint64_t* hkey, c
 
 // ins[i] = 1 iff the pair is absent from the rule's graph. rule 0: current logical graph;
 // rule 1: the loaded base graph (decoded, i.e. before any update).
+// pos (optional): the base range of row u found for each line, reused by k_apply.
 __global__ void k_classify(GView g, bool sorted, const int32_t* uv, int64_t m, int rule, uint8_t* ins,
-                           int64_t* counts) {
+                           int64_t* pos, int64_t* counts) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   bool insertion = false;
   if (i < m) {
     int32_t u = uv[2 * i], v = uv[2 * i + 1];
     PairState s = pair_state(g, sorted, u, v);
+    if (pos) {
+      pos[2 * i] = s.lo;
+      pos[2 * i + 1] = s.hi;
+    }
     bool adj = rule == SCOUT_CLASSIFY_BASE ? s.base_any : adjacent_now(s);
     ins[i] = adj ? 0 : 1;
     insertion = !adj;
@@ -164,15 +169,18 @@ __device__ __forceinline__ void block_push_new(NewEntry e, int64_t line, uint64_
 
 // One thread per line of [lo, hi): the DeltaOverlay rules. done[i] = 1 if the line was applied.
 // box[0] = new inserted-row entries, box[1] = holes created.
-__global__ void k_apply(GView g, bool sorted, const int32_t* uv, const uint8_t* ins, int64_t lo, int64_t hi,
-                        int conditional, uint8_t* done, uint64_t* nkey, int32_t* ndst, int64_t ncap, int64_t* box) {
+// pos (optional): base ranges of row u located by k_classify (the base rows' entries do not
+// move within a batch; only their deletion marks flip).
+__global__ void k_apply(GView g, bool sorted, const int32_t* uv, const uint8_t* ins, const int64_t* pos, int64_t lo,
+                        int64_t hi, int conditional, uint8_t* done, uint64_t* nkey, int32_t* ndst, int64_t ncap,
+                        int64_t* box) {
   int64_t i = lo + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   NewEntry e1{false, 0, 0}, e2{false, 0, 0};
   int64_t holes = 0;
   if (i < hi) {
     int32_t u = uv[2 * i], v = uv[2 * i + 1];
     bool insertion = ins[i] != 0;
-    PairState su = pair_state(g, sorted, u, v);
+    PairState su = pos ? pair_state_in(g, u, v, pos[2 * i], pos[2 * i + 1]) : pair_state(g, sorted, u, v);
     bool apply = !conditional || insertion != adjacent_now(su);
     done[i] = apply ? 1 : 0;
     if (apply && insertion) {
@@ -525,9 +533,10 @@ int apply_range(Ctx& c, Channel& ch, int64_t lo, int64_t hi, int conditional) {
   SCOUT_CHECK(g.ndst.reserve(cap));
   SCOUT_TRY(hipMemsetAsync(c.d_box, 0, 4 * sizeof(int64_t), c.s));
   SCOUT_CHECK(prof_begin(c, PROF_APPLY));
+  const int64_t* pos = (ch.pos_gen == g.base_gen && ch.pos_lo <= lo && hi <= ch.pos_hi) ? ch.d_pos.p : nullptr;
   if (hi > lo)
-    k_apply<<<grid_for(hi - lo), kBlock, 0, c.s>>>(g.view(), g.sorted, ch.d_uv.p, ch.d_ins.p, lo, hi, conditional,
-                                                   ch.d_done.p, g.nkey.p, g.ndst.p, g.nkey.cap, c.d_box);
+    k_apply<<<grid_for(hi - lo), kBlock, 0, c.s>>>(g.view(), g.sorted, ch.d_uv.p, ch.d_ins.p, pos, lo, hi,
+                                                   conditional, ch.d_done.p, g.nkey.p, g.ndst.p, g.nkey.cap, c.d_box);
   // algorithmic bytes per line: the pair and its label and done flag (10 B), both rows'
   // base and inserted offsets (32 B) and the entries flipped or appended (8 B)
   SCOUT_CHECK(prof_end(c, PROF_APPLY, 50 * (hi - lo)));
@@ -571,6 +580,7 @@ int graph_free(Ctx& c) {
   g.hline.release();
   g.n = 0;
   g.E = g.A = g.live = g.holes = 0;
+  g.base_gen++;
   return SCOUT_OK;
 }
 
@@ -618,6 +628,7 @@ int graph_upload_egr(Ctx& c, int32_t n, int64_t E, const void* nindex, int wide,
   DevGraph& g = c.g;
   g.n = n;
   g.E = E;
+  g.base_gen++;
   SCOUT_TRY(hipMalloc((void**)&g.boff, ((size_t)n + 1) * sizeof(int64_t)));
   SCOUT_TRY(hipMalloc((void**)&g.bcol, (size_t)std::max<int64_t>(E, 1) * sizeof(int32_t)));
   if (wide) {
@@ -655,11 +666,15 @@ int graph_classify_lines(Ctx& c, Channel& ch, int64_t lo, int64_t hi, int rule) 
     return SCOUT_ERR_STATE;
   }
   int64_t m = hi - lo;
+  SCOUT_CHECK(ch.d_pos.reserve(2 * ch.m));
+  if (ch.pos_gen != c.g.base_gen || ch.pos_hi != lo) ch.pos_lo = lo;  // else extend the valid range
+  ch.pos_hi = hi;
+  ch.pos_gen = c.g.base_gen;
   SCOUT_TRY(hipMemsetAsync(c.d_box, 0, sizeof(int64_t), c.s));
   SCOUT_CHECK(prof_begin(c, PROF_CLASSIFY));
   if (m > 0)
     k_classify<<<grid_for(m), kBlock, 0, c.s>>>(c.g.view(), c.g.sorted, ch.d_uv.p + 2 * lo, m, rule, ch.d_ins.p + lo,
-                                                c.d_box);
+                                                ch.d_pos.p + 2 * lo, c.d_box);
   // per line: the pair (8 B), row u's base and inserted offsets (16 B), the located entry
   // (4 B) and the label written (1 B)
   SCOUT_CHECK(prof_end(c, PROF_CLASSIFY, 29 * m));
@@ -773,6 +788,7 @@ int graph_compact(Ctx& c) {
   off2.p = nullptr;
   col2.p = nullptr;
   g.E = total;
+  g.base_gen++;
   SCOUT_CHECK(reset_inserted(c));
   return check_rows(c);
 }

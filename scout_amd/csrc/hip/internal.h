@@ -115,6 +115,7 @@ struct DevGraph {
   int64_t live = 0;     // entries inside rows (sum of alen), holes included
   int64_t holes = 0;    // HOLE entries inside rows
   bool sorted = false;  // every base row ascending (by decoded id)
+  int64_t base_gen = 0; // bumped whenever the base arrays are replaced or restored
   int64_t* boff = nullptr;
   int32_t* bcol = nullptr;
   DevBuf<int64_t> aoff, aoff2;
@@ -138,9 +139,19 @@ struct Channel {
   DevBuf<int32_t> d_uv;
   DevBuf<uint8_t> d_ins;    // 1 = insertion label
   DevBuf<uint8_t> d_done;   // 1 = the edit changed the graph
+  // base range of row u searched by classify for line (u, v): lo, hi per line; valid for
+  // lines [pos_lo, pos_hi) while the base generation is pos_gen
+  DevBuf<int64_t> d_pos;
+  int64_t pos_gen = -1, pos_lo = 0, pos_hi = 0;
   int64_t m = 0;
   hipEvent_t staged = nullptr;
 };
+
+// Sets the staged line count; base ranges found by an earlier classify no longer apply.
+inline void channel_staged(Channel& ch, int64_t m) {
+  ch.m = m;
+  ch.pos_lo = ch.pos_hi = 0;
+}
 
 struct MlpState;  // mlp.hip
 struct CcState;   // cc.hip
@@ -180,6 +191,7 @@ struct Ctx {
   int64_t cc_level_seq = -1;     // same for cc_level
   int64_t cc_components = -1;    // component count after the last CC repair
   int64_t label_seq = -1;        // apply_seq at which `label` was last made exact
+  int64_t mis_version = 0;       // bumped when `mis` is overwritten outside the MIS repair
   DevBuf<uint32_t> cc_bits;      // CC: bitmap of labels touched by the batch
   // small mailbox read back by the host after a round
   int64_t* h_box = nullptr;    // pinned
@@ -275,7 +287,7 @@ int prof_end(Ctx& c, int id, int64_t bytes);
 void prof_bytes(Ctx& c, int id, int64_t bytes);
 void prof_release(Ctx& c);
 enum ProfId { PROF_BFS_RELAX = 0, PROF_BFS_INVALIDATE, PROF_CC_EXPAND, PROF_APPLY, PROF_CLASSIFY,
-              PROF_COLOR, PROF_MIS_EVAL, PROF_CC_HOOK, PROF_COUNT };
+              PROF_COLOR, PROF_MIS_EVAL, PROF_CC_HOOK, PROF_BFS_INVALIDATE_KERNEL, PROF_COUNT };
 
 // helpers in ctx.hip
 int box_read(Ctx& c, int count);  // copies d_box[0..count) into h_box, synchronises c.s

@@ -46,13 +46,17 @@ constexpr int32_t kNoWriter = INT32_MAX;
 
 struct MisView {
   GView g;
+  bool sorted;
   const uint8_t* mis;
   const int32_t* tag;
   int32_t tag_ep;        // endpoint tag = 2*ep+1, hop-1 tag = 2*ep
-  int32_t* minw;
+  const int32_t* mcount; // members among each vertex's neighbours (committed state)
+  const int32_t* minw;   // smallest pending line writing the vertex
+  const int32_t* minwn;  // smallest pending line writing one of the vertex's neighbours
   int32_t line;          // index of the line being evaluated
   bool check;            // pass B: report reads of vertices written by an earlier line
   int32_t dropped;       // vertex whose membership reads as 0 (the insertion's r), or -1
+  int64_t* scanned;      // row entries read by this wave (counted on every lane alike)
 };
 
 // Membership as seen by the evaluating line. In check mode a read of a vertex with an
@@ -63,28 +67,48 @@ __device__ __forceinline__ bool member(const MisView& s, int32_t x, bool* confli
   return s.mis[x] != 0;
 }
 
-// Wave: is some vertex of row segment r[lo, hi) other than `skip` a member? (Any conflict in
-// check mode also stops the scan.)
-__device__ __forceinline__ bool wave_any_member(const MisView& s, const Row& r, int32_t skip, bool* conflict) {
-  int64_t len = row_len(r);
-  for (int64_t base = 0; base < len; base += kWave) {
-    int64_t k = base + lane_id();
-    bool hit = false, conf = false;
-    if (k < len) {
-      int32_t x = row_at(s.g, r, k);
-      if (x >= 0 && x != skip) hit = member(s, x, &conf);
-    }
-    if (__ballot(conf)) {
-      *conflict = true;
-      return true;
-    }
-    if (__ballot(hit)) return true;
+// Live entries equal to y in row x (base entries not marked deleted, inserted entries); the
+// shorter of the two rows is searched (the adjacency is symmetric).
+__device__ __forceinline__ int32_t live_count(const MisView& s, int32_t x, int32_t y) {
+  int64_t lx = (s.g.boff[x + 1] - s.g.boff[x]) + s.g.alen[x];
+  int64_t ly = (s.g.boff[y + 1] - s.g.boff[y]) + s.g.alen[y];
+  if (ly < lx) {
+    int32_t tmp = x;
+    x = y;
+    y = tmp;
   }
-  return false;
+  int64_t lo, hi;
+  base_range(s.g, s.sorted, x, y, lo, hi);
+  int32_t c = 0;
+  for (int64_t k = lo; k < hi; k++) c += s.g.bcol[k] == y;
+  for (int64_t k = s.g.aoff[x], e = k + s.g.alen[x]; k < e; k++) c += s.g.acol[k] == y;
+  return c;
+}
+
+// Does N(x) hold a member other than `skip`? The committed member-neighbour count of x,
+// less the entries of skip when it is a member, less those of the dropped vertex (a member
+// that reads as 0). Reads every neighbour's membership at once: a conflict if a pending
+// earlier line writes any of them (or skip or the dropped vertex).
+__device__ __forceinline__ bool nbr_has_member(const MisView& s, int32_t x, int32_t skip, bool* conflict) {
+  if (s.check && s.minwn[x] < s.line) *conflict = true;
+  int32_t c = s.mcount[x];
+  if (c == 0) return false;
+  // a row holds a neighbour at most twice (an inserted copy next to a deleted base entry is
+  // not live), so counts above 2 stay positive without the adjacency tests
+  if (skip >= 0) {
+    if (s.check && s.minw[skip] < s.line) *conflict = true;
+    if (c <= 2 && s.mis[skip]) c -= live_count(s, x, skip);
+  }
+  if (s.dropped >= 0 && s.dropped != skip && c > 0) {
+    if (s.check && s.minw[s.dropped] < s.line) *conflict = true;
+    if (c <= 2 && s.mis[s.dropped]) c -= live_count(s, x, s.dropped);
+  }
+  return c > 0;
 }
 
 // Any member (other than skip) in the vertex's row: 2-hop ball for endpoints, N(x) for
-// hop-1 vertices.
+// hop-1 vertices. The hop-2 part takes y in N(x) order, 64 at a time, one lane each; the
+// first y in order whose test conflicts or finds a member decides.
 __device__ bool row_has_member(const MisView& s, int32_t x, bool two_hop, int32_t skip, bool* conflict) {
   if (x != skip) {
     bool conf = false;
@@ -95,14 +119,40 @@ __device__ bool row_has_member(const MisView& s, int32_t x, bool two_hop, int32_
     }
     if (m) return true;
   }
-  Row r = row_of(s.g, x);
-  if (wave_any_member(s, r, skip, conflict)) return true;
+  {
+    bool conf = false;
+    bool has = nbr_has_member(s, x, skip, &conf);
+    if (conf) {
+      *conflict = true;
+      return true;
+    }
+    if (has) return true;
+  }
   if (!two_hop) return false;
+  Row r = row_of(s.g, x);
   int64_t len = row_len(r);
-  for (int64_t k = 0; k < len; k++) {
-    int32_t y = row_at(s.g, r, k);
-    if (y < 0) continue;
-    if (wave_any_member(s, row_of(s.g, y), skip, conflict)) return true;
+  *s.scanned += len;
+  // four chunks of 64 y's in flight per iteration, decided chunk by chunk in order
+  constexpr int kU = 4;
+  for (int64_t g0 = 0; g0 < len; g0 += kU * kWave) {
+    bool conf[kU], has[kU];
+#pragma unroll
+    for (int j = 0; j < kU; j++) {
+      int64_t k = g0 + j * kWave + lane_id();
+      int32_t y = k < len ? row_at(s.g, r, k) : -1;
+      conf[j] = false;
+      has[j] = y >= 0 && nbr_has_member(s, y, skip, &conf[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < kU; j++) {
+      unsigned long long cm = __ballot(conf[j]), hm = __ballot(has[j]);
+      unsigned long long any = cm | hm;
+      if (any) {
+        int b = __ffsll((long long)any) - 1;
+        if ((cm >> b) & 1ULL) *conflict = true;
+        return true;
+      }
+    }
   }
   return false;
 }
@@ -114,18 +164,57 @@ __device__ __forceinline__ int row_kind(const MisView& s, int32_t x) {
   return 0;
 }
 
-// Candidate c of the insertion rule: not a member, has a row, no member in its row. Returns
-// 1 = accept, 0 = reject, -1 = conflict (check mode).
-__device__ int try_candidate(const MisView& s, int32_t c) {
+// Candidate c of the insertion rule (not a member, has a row, no member in its row), one
+// lane: 1 = accept, 0 = reject, -1 = conflict (check mode), 3 = an endpoint (its row is its
+// 2-hop ball, evaluated by the wave).
+__device__ __forceinline__ int candidate_hop1(const MisView& s, int32_t c) {
   bool conf = false;
   bool m = member(s, c, &conf);
   if (conf) return -1;
   if (m) return 0;
   int kind = row_kind(s, c);
   if (kind == 0) return 0;
-  bool has = row_has_member(s, c, kind == 2, -1, &conf);
+  if (kind == 2) return 3;
+  bool has = nbr_has_member(s, c, -1, &conf);
   if (conf) return -1;
   return has ? 0 : 1;
+}
+
+// An endpoint candidate, by the wave.
+__device__ int try_endpoint(const MisView& s, int32_t c) {
+  bool conf = false;
+  bool m = member(s, c, &conf);
+  if (conf) return -1;
+  if (m) return 0;
+  bool has = row_has_member(s, c, true, -1, &conf);
+  if (conf) return -1;
+  return has ? 0 : 1;
+}
+
+// Candidates of row r in order, 64 at a time (one lane each; endpoints go through the wave
+// when their turn comes). Returns 1 with *accepted, 0 if none is accepted, -1 on conflict.
+__device__ int first_candidate(const MisView& s, const Row& r, int32_t* accepted) {
+  int64_t len = row_len(r);
+  *s.scanned += len;
+  for (int64_t g0 = 0; g0 < len; g0 += kWave) {
+    int64_t k = g0 + lane_id();
+    int32_t c = k < len ? row_at(s.g, r, k) : -1;
+    int res = c >= 0 ? candidate_hop1(s, c) : 0;
+    unsigned long long pending = __ballot(res != 0);
+    while (pending) {
+      int j = __ffsll((long long)pending) - 1;
+      pending &= pending - 1;
+      int rj = __shfl(res, j);
+      int32_t cj = __shfl(c, j);
+      if (rj == 3) rj = try_endpoint(s, cj);
+      if (rj == -1) return -1;
+      if (rj == 1) {
+        *accepted = cj;
+        return 1;
+      }
+    }
+  }
+  return 0;
 }
 
 // Evaluates one line. Outputs the vertices it writes (w0 -> 0 or 1 by rule, w1 -> 1) or
@@ -142,39 +231,30 @@ __device__ bool eval_line(MisView s, int32_t u, int32_t v, bool ins, int32_t* w0
     int32_t r = u > v ? u : v;
     *w0 = r;
     s.dropped = r;
-    // candidate sequence [r] + N(r) + N(y) for y in N(r)
-    int res = try_candidate(s, r);
+    // candidate sequence [r] + N(r) + N(y) for y in N(r); r is an endpoint
+    int res = try_endpoint(s, r);
     if (res < 0) return false;
     if (res > 0) {
       *w1 = r;
       return true;
     }
     Row rr = row_of(s.g, r);
-    int64_t len = row_len(rr);
-    for (int64_t k = 0; k < len; k++) {
-      int32_t c = row_at(s.g, rr, k);
-      if (c < 0) continue;
-      res = try_candidate(s, c);
-      if (res < 0) return false;
-      if (res > 0) {
-        *w1 = c;
-        return true;
-      }
+    int32_t acc = -1;
+    res = first_candidate(s, rr, &acc);
+    if (res < 0) return false;
+    if (res > 0) {
+      *w1 = acc;
+      return true;
     }
+    int64_t len = row_len(rr);
     for (int64_t k = 0; k < len; k++) {
       int32_t y = row_at(s.g, rr, k);
       if (y < 0) continue;
-      Row ry = row_of(s.g, y);
-      int64_t ly = row_len(ry);
-      for (int64_t j = 0; j < ly; j++) {
-        int32_t c = row_at(s.g, ry, j);
-        if (c < 0) continue;
-        res = try_candidate(s, c);
-        if (res < 0) return false;
-        if (res > 0) {
-          *w1 = c;
-          return true;
-        }
+      res = first_candidate(s, row_of(s.g, y), &acc);
+      if (res < 0) return false;
+      if (res > 0) {
+        *w1 = acc;
+        return true;
       }
     }
     return true;
@@ -190,19 +270,29 @@ __device__ bool eval_line(MisView s, int32_t u, int32_t v, bool ins, int32_t* w0
   return true;
 }
 
-// Endpoint and hop-1 tags of a sub-batch.
-__global__ void k_mis_tag_endpoints(const int32_t* uv, int64_t lo, int64_t hi, int32_t* tag, int32_t ep) {
-  int64_t i = lo + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= hi) return;
-  atomicMax(&tag[uv[2 * i]], 2 * ep + 1);
-  atomicMax(&tag[uv[2 * i + 1]], 2 * ep + 1);
+// Endpoint and hop-1 tags of the phase that starts this round (nothing otherwise).
+__global__ void k_mis_tag_endpoints(const int32_t* uv, const int64_t* t, const int64_t* ph, int64_t pcap,
+                                    int32_t* tag) {
+  if (!t[10]) return;
+  int64_t f = t[8];
+  int64_t lo = ph[f], hi = ph[pcap + f];
+  int32_t ep = (int32_t)t[2];
+  for (int64_t i = lo + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < hi; i += (int64_t)gridDim.x * blockDim.x) {
+    atomicMax(&tag[uv[2 * i]], 2 * ep + 1);
+    atomicMax(&tag[uv[2 * i + 1]], 2 * ep + 1);
+  }
 }
 
-__global__ void k_mis_tag_hop1(GView g, const int32_t* uv, int64_t lo, int64_t hi, int32_t* tag, int32_t ep) {
+__global__ void k_mis_tag_hop1(GView g, const int32_t* uv, const int64_t* t, const int64_t* ph, int64_t pcap,
+                               int32_t* tag) {
+  if (!t[10]) return;
+  int64_t f = t[8];
+  int64_t lo = ph[f], hi = ph[pcap + f];
+  int32_t ep = (int32_t)t[2];
   for (int64_t w = global_wave(); w < 2 * (hi - lo); w += total_waves()) {
     int32_t x = uv[2 * lo + w];
     Row r = row_of(g, x);
-    wave_walk(g, r, 0, row_len(r), [&](int32_t y) { atomicMax(&tag[y], 2 * ep); });
+    wave_walk_wide<8>(g, r, 0, row_len(r), [&](int32_t y) { atomicMax(&tag[y], 2 * ep); });
   }
 }
 
@@ -230,26 +320,38 @@ __global__ void k_mis_degrees(GView g, const int32_t* uv, int64_t m, int64_t* de
 }
 
 // --- round engine over segments ---------------------------------------------------------
-// Segment table (int64, on the device): header [0] segments S, [1] slots (window lines over
-// all segments), [2] tag epoch, [3] segments still pending, [4] non-empty rounds, [5] writes,
-// [6] line evaluations; then per segment: pending start p, end hi, first conflict k0, window
-// size win, and woff (exclusive prefix of the current window sizes, S + 1 entries).
-constexpr int kSegHdr = 8;
+// Segment table (int64, on the device): header [0] segments S of the current phase, [1] slots
+// (window lines over all segments), [2] tag epoch, [3] segments still pending, [4] non-empty
+// rounds, [5] writes, [6] line evaluations, [7] row entries read by passes A and B, [8]
+// current phase, [9] phases, [10] 1 = the phase starts this round (tags to set), [11] first
+// tag epoch, [12] initial window; then per segment: pending start p, end hi, first conflict
+// k0, window size win, and woff (exclusive prefix of the current window sizes, S + 1).
+// Phase table (int64): lines lo[P] and hi[P] of each phase, first segment seg0[P + 1], then
+// the segments' line ranges slo[NS] and shi[NS]; its layout sizes are in the SegView.
+constexpr int kSegHdr = 16;
 constexpr int64_t kMinWindow = 64;
 constexpr int64_t kMaxWindow = 1 << 20;
 
 struct SegView {
   int64_t* t;
   int64_t stride;
+  const int64_t* ph;  // phase table
+  int64_t pcap, scap; // its phase and segment capacities
   __device__ int64_t& S() const { return t[0]; }
   __device__ int64_t* p() const { return t + kSegHdr; }
   __device__ int64_t* hi() const { return t + kSegHdr + stride; }
   __device__ int64_t* k0() const { return t + kSegHdr + 2 * stride; }
   __device__ int64_t* win() const { return t + kSegHdr + 3 * stride; }
   __device__ int64_t* woff() const { return t + kSegHdr + 4 * stride; }
+  __device__ int64_t ph_lo(int64_t f) const { return ph[f]; }
+  __device__ int64_t ph_hi(int64_t f) const { return ph[pcap + f]; }
+  __device__ int64_t ph_seg0(int64_t f) const { return ph[2 * pcap + f]; }
+  __device__ int64_t seg_lo(int64_t i) const { return ph[3 * pcap + 1 + i]; }
+  __device__ int64_t seg_hi(int64_t i) const { return ph[3 * pcap + 1 + scap + i]; }
 };
 
 inline int64_t seg_table_size(int64_t stride) { return kSegHdr + 5 * stride + 1; }
+inline int64_t phase_table_size(int64_t pcap, int64_t scap) { return 3 * pcap + 1 + 2 * scap; }
 
 // Window slot -> (segment, line): binary search over woff.
 __device__ __forceinline__ int64_t slot_line(const SegView& sv, int64_t slot, int32_t* sg_out) {
@@ -274,95 +376,144 @@ __global__ void k_mm_gather(const int32_t* uv, const uint8_t* ins, const int32_t
   ins2[t] = ins[k];
 }
 
-// Pass A: wave per window line. Writes go to w0/w1; minw[x] = smallest writer.
-__global__ void k_mm_eval(GView g, const int32_t* uv, const uint8_t* ins, SegView sv, const uint8_t* mis,
-                          const int32_t* tag, int32_t* minw, int32_t* w0s, int32_t* w1s) {
+// Device arrays of one MIS repair.
+struct MisDev {
+  GView g;
+  bool sorted;
+  const int32_t* uv;
+  const uint8_t* ins;
+  uint8_t* mis;
+  const int32_t* tag;
+  int32_t* mcount;
+  int32_t* minw;
+  int32_t* minwn;
+  int32_t* w0s;
+  int32_t* w1s;
+};
+
+__device__ __forceinline__ MisView view_of(const MisDev& d, int32_t ep, int64_t k, bool check, int64_t* scanned) {
+  return MisView{d.g, d.sorted, d.mis, d.tag, ep, d.mcount, d.minw, d.minwn, (int32_t)k, check, -1, scanned};
+}
+
+// Pass A: wave per window line. Writes go to w0/w1; minw[w] = smallest writer of w and
+// minwn[z] = smallest writer of a neighbour of z.
+__global__ void k_mm_eval(MisDev d, SegView sv) {
   int64_t slots = sv.t[1];
   int32_t ep = (int32_t)sv.t[2];
+  int64_t scanned = 0;
   for (int64_t t = global_wave(); t < slots; t += total_waves()) {
     int32_t sg;
     int64_t k = slot_line(sv, t, &sg);
-    MisView s{g, mis, tag, ep, minw, (int32_t)k, false, -1};
+    MisView s = view_of(d, ep, k, false, &scanned);
     int32_t w0, w1;
-    eval_line(s, uv[2 * k], uv[2 * k + 1], ins[k] != 0, &w0, &w1);
+    eval_line(s, d.uv[2 * k], d.uv[2 * k + 1], d.ins[k] != 0, &w0, &w1);
     if (lane_id() == 0) {
-      w0s[k] = w0;
-      w1s[k] = w1;
-      if (w0 >= 0) atomicMin(&minw[w0], (int32_t)k);
-      if (w1 >= 0) atomicMin(&minw[w1], (int32_t)k);
+      d.w0s[k] = w0;
+      d.w1s[k] = w1;
+      if (w0 >= 0) atomicMin(&d.minw[w0], (int32_t)k);
+      if (w1 >= 0) atomicMin(&d.minw[w1], (int32_t)k);
+    }
+    for (int j = 0; j < 2; j++) {
+      int32_t w = j == 0 ? w0 : w1;
+      if (w < 0) continue;
+      Row r = row_of(d.g, w);
+      wave_walk_wide<8>(d.g, r, 0, row_len(r), [&](int32_t z) { atomicMin(&d.minwn[z], (int32_t)k); });
     }
   }
+  if (lane_id() == 0 && scanned) atomicAdd((unsigned long long*)&sv.t[7], (unsigned long long)scanned);
 }
 
 // Pass B: per segment, the first window line whose reads meet an earlier pending writer.
-__global__ void k_mm_check(GView g, const int32_t* uv, const uint8_t* ins, SegView sv, const uint8_t* mis,
-                           const int32_t* tag, int32_t* minw) {
+__global__ void k_mm_check(MisDev d, SegView sv) {
   int64_t slots = sv.t[1];
   int32_t ep = (int32_t)sv.t[2];
   int64_t* k0 = sv.k0();
+  int64_t scanned = 0;
   for (int64_t t = global_wave(); t < slots; t += total_waves()) {
     int32_t sg;
     int64_t k = slot_line(sv, t, &sg);
     if (k == sv.p()[sg]) continue;  // the first pending line never conflicts
     if (k >= __hip_atomic_load(&k0[sg], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) continue;
-    MisView s{g, mis, tag, ep, minw, (int32_t)k, true, -1};
+    MisView s = view_of(d, ep, k, true, &scanned);
     int32_t w0, w1;
-    bool ok = eval_line(s, uv[2 * k], uv[2 * k + 1], ins[k] != 0, &w0, &w1);
+    bool ok = eval_line(s, d.uv[2 * k], d.uv[2 * k + 1], d.ins[k] != 0, &w0, &w1);
     if (!ok && lane_id() == 0) atomicMin((long long*)&k0[sg], (long long)k);
   }
+  if (lane_id() == 0 && scanned) atomicAdd((unsigned long long*)&sv.t[7], (unsigned long long)scanned);
 }
 
-// Commits each segment's prefix [p, k0) and clears minw of every window line's writes.
-__global__ void k_mm_commit(const uint8_t* ins, SegView sv, const int32_t* w0s, const int32_t* w1s, uint8_t* mis,
-                            int32_t* minw) {
+// Wave per window line: commits each segment's prefix [p, k0) (memberships and the member
+// counts of the written vertices' neighbours) and clears the pending marks of every window
+// line's writes. Committed lines write distinct vertices (a second writer of a vertex reads
+// it and conflicts), so the count updates do not overlap.
+__global__ void k_mm_commit(MisDev d, SegView sv) {
   int64_t slots = sv.t[1];
   const int64_t* k0 = sv.k0();
   int64_t writes = 0;
-  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < slots; base += (int64_t)gridDim.x * blockDim.x) {
-    int64_t t = base + threadIdx.x;
-    if (t < slots) {
-      int32_t sg;
-      int64_t k = slot_line(sv, t, &sg);
-      int32_t w0 = w0s[k], w1 = w1s[k];
-      if (k < k0[sg]) {
-        if (w0 >= 0) mis[w0] = ins[k] ? 0 : 1;
-        if (w1 >= 0) mis[w1] = 1;
-        writes += w0 >= 0;
-      }
-      if (w0 >= 0) minw[w0] = kNoWriter;
-      if (w1 >= 0) minw[w1] = kNoWriter;
+  for (int64_t t = global_wave(); t < slots; t += total_waves()) {
+    int32_t sg;
+    int64_t k = slot_line(sv, t, &sg);
+    int32_t w0 = d.w0s[k], w1 = d.w1s[k];
+    bool commit = k < k0[sg];
+    bool ins = d.ins[k] != 0;
+    if (commit && lane_id() == 0) {
+      if (w0 >= 0) d.mis[w0] = ins ? 0 : 1;
+      if (w1 >= 0) d.mis[w1] = 1;
+      writes += w0 >= 0;
+    }
+    if (lane_id() == 0) {
+      if (w0 >= 0) d.minw[w0] = kNoWriter;
+      if (w1 >= 0) d.minw[w1] = kNoWriter;
+    }
+    for (int j = 0; j < 2; j++) {
+      int32_t w = j == 0 ? w0 : w1;
+      if (w < 0) continue;
+      int32_t delta = commit ? ((j == 0 && ins) ? -1 : 1) : 0;
+      Row r = row_of(d.g, w);
+      wave_walk_wide<8>(d.g, r, 0, row_len(r), [&](int32_t z) {
+        d.minwn[z] = kNoWriter;
+        if (delta) atomicAdd(&d.mcount[z], delta);
+      });
     }
   }
-  writes = wave_sum(writes);
   if (lane_id() == 0 && writes) atomicAdd((unsigned long long*)&sv.t[5], (unsigned long long)writes);
 }
 
-// One block: advances every segment (p = k0), resizes its window, sets k0 to the window end
-// and rebuilds woff and the slot count; counts pending segments and non-empty rounds.
-__global__ void k_mm_round_end(SegView sv) {
-  __shared__ int64_t scan[kBlock];
-  __shared__ int64_t carry_s, active_s, pending_s;
-  int64_t S = sv.t[0];
-  int64_t *p = sv.p(), *hi = sv.hi(), *k0 = sv.k0(), *win = sv.win(), *woff = sv.woff();
-  if (threadIdx.x == 0) carry_s = active_s = pending_s = 0;
+// Member neighbours of every vertex (wave per vertex).
+__global__ void k_mcount_init(GView g, const uint8_t* mis, int32_t* mcount) {
+  for (int64_t v = global_wave(); v < g.n; v += total_waves()) {
+    Row r = row_of(g, (int32_t)v);
+    int64_t len = row_len(r);
+    int64_t c = 0;
+    for (int64_t k = lane_id(); k < len; k += kWave) {
+      int32_t z = row_at(g, r, k);
+      c += z >= 0 && mis[z];
+    }
+    c = wave_sum(c);
+    if (lane_id() == 0) mcount[v] = (int32_t)c;
+  }
+}
+
+// Member counts after one applied batch: every line that changed the graph adds (insertion)
+// or removes (deletion) the other endpoint's membership.
+__global__ void k_mcount_adjust(const int32_t* uv, const uint8_t* ins, const uint8_t* done, int64_t m,
+                                const uint8_t* mis, int32_t* mcount) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m || !done[i]) return;
+  int32_t u = uv[2 * i], v = uv[2 * i + 1];
+  int32_t sgn = ins[i] ? 1 : -1;
+  if (mis[v]) atomicAdd(&mcount[u], sgn);
+  if (mis[u]) atomicAdd(&mcount[v], sgn);
+}
+
+// Block-wide exclusive scan of sizes[s] (s < S) into woff, in chunks of the block size;
+// returns the total on every thread.
+__device__ int64_t block_woff(int64_t S, int64_t* woff, SegView sv, int64_t* scan, int64_t* carry_s) {
+  if (threadIdx.x == 0) *carry_s = 0;
   __syncthreads();
   for (int64_t base = 0; base < S; base += blockDim.x) {
     int64_t s = base + threadIdx.x;
-    int64_t size = 0;
-    if (s < S) {
-      int64_t ps = p[s], hs = hi[s], ks = k0[s], ws = win[s];
-      if (ps < hs) {
-        atomicAdd((unsigned long long*)&active_s, 1ULL);
-        int64_t done = ks - ps;
-        ws = (ks == ps + min(ws, hs - ps)) ? min(2 * ws, kMaxWindow) : max(kMinWindow, 2 * done);
-      }
-      ps = ks;
-      size = min(ws, hs - ps);
-      if (ps < hs) atomicAdd((unsigned long long*)&pending_s, 1ULL);
-      p[s] = ps;
-      win[s] = ws;
-      k0[s] = ps + size;
-    }
+    int64_t size = s < S ? sv.k0()[s] - sv.p()[s] : 0;
     scan[threadIdx.x] = size;
     __syncthreads();
     for (int off = 1; off < (int)blockDim.x; off <<= 1) {
@@ -371,17 +522,82 @@ __global__ void k_mm_round_end(SegView sv) {
       scan[threadIdx.x] += y;
       __syncthreads();
     }
-    if (s < S) woff[s] = carry_s + scan[threadIdx.x] - size;
+    if (s < S) woff[s] = *carry_s + scan[threadIdx.x] - size;
     __syncthreads();
-    if (threadIdx.x == blockDim.x - 1) carry_s += scan[threadIdx.x];
+    if (threadIdx.x == blockDim.x - 1) *carry_s += scan[threadIdx.x];
     __syncthreads();
   }
+  int64_t total = *carry_s;
+  __syncthreads();
+  return total;
+}
+
+// One block: advances every segment (p = k0) and resizes its window (k0 = window end); when
+// every segment of the phase is done, loads the next phase's segments and flags its start.
+// Rebuilds woff and the slot count; counts non-empty rounds.
+__global__ void k_mm_round_end(SegView sv) {
+  __shared__ int64_t scan[kBlock];
+  __shared__ int64_t carry_s, active_s, pending_s;
+  int64_t* t = sv.t;
+  int64_t S = t[0];
+  int64_t *p = sv.p(), *hi = sv.hi(), *k0 = sv.k0(), *win = sv.win();
   if (threadIdx.x == 0) {
-    woff[S] = carry_s;
-    sv.t[1] = carry_s;
-    sv.t[3] = pending_s;
-    if (active_s) sv.t[4] += 1;
-    sv.t[6] += carry_s;
+    active_s = pending_s = 0;
+    t[10] = 0;  // this round's tags are set
+  }
+  __syncthreads();
+  for (int64_t s = threadIdx.x; s < S; s += blockDim.x) {
+    int64_t ps = p[s], hs = hi[s], ks = k0[s], ws = win[s];
+    if (ps < hs) {
+      atomicAdd((unsigned long long*)&active_s, 1ULL);
+      int64_t done = ks - ps;
+      ws = (ks == ps + min(ws, hs - ps)) ? min(2 * ws, kMaxWindow) : max(kMinWindow, 2 * done);
+    }
+    ps = ks;
+    if (ps < hs) atomicAdd((unsigned long long*)&pending_s, 1ULL);
+    p[s] = ps;
+    win[s] = ws;
+    k0[s] = ps + min(ws, hs - ps);
+  }
+  __syncthreads();
+  bool next_phase = pending_s == 0 && t[8] < t[9];
+  if (next_phase) {
+    int64_t f = t[8] + 1;
+    __syncthreads();
+    if (threadIdx.x == 0) t[8] = f;
+    if (f < t[9]) {
+      int64_t s0 = sv.ph_seg0(f), s1 = sv.ph_seg0(f + 1);
+      S = s1 - s0;
+      int64_t w0 = t[12];
+      for (int64_t s = threadIdx.x; s < S; s += blockDim.x) {
+        int64_t lo = sv.seg_lo(s0 + s), h = sv.seg_hi(s0 + s);
+        p[s] = lo;
+        hi[s] = h;
+        win[s] = w0;
+        k0[s] = lo + min(w0, h - lo);
+      }
+      if (threadIdx.x == 0) {
+        t[0] = S;
+        t[2] = t[11] + f;
+        t[3] = S;
+        t[10] = 1;
+      }
+    } else {
+      S = 0;
+      if (threadIdx.x == 0) {
+        t[0] = 0;
+        t[3] = 0;
+      }
+    }
+    __syncthreads();
+  }
+  int64_t total = block_woff(S, sv.woff(), sv, scan, &carry_s);
+  if (threadIdx.x == 0) {
+    sv.woff()[S] = total;
+    t[1] = total;
+    if (!next_phase) t[3] = pending_s;
+    if (active_s) t[4] += 1;
+    t[6] += total;
   }
 }
 
@@ -395,8 +611,12 @@ struct MisState {
   int32_t tag_ep = 0;
   unsigned grid = kListGrid;
   bool use_graph = true;
-  int group_rounds = 8;
+  int group_rounds = 16;
   int64_t window0 = 256;
+  DevBuf<int32_t> mcount, minwn;
+  int64_t mc_version = -1, mc_apply = -1, mc_gen = -1;  // what mcount was computed for
+  DevBuf<int64_t> phase;  // phase table
+  int64_t pcap = 0, scap = 0;
   hipGraphExec_t exec = nullptr;
   std::vector<const void*> key;
 };
@@ -411,6 +631,9 @@ void mis_release(Ctx& c) {
   ms.ins2.release();
   ms.deg.release();
   ms.seg.release();
+  ms.mcount.release();
+  ms.minwn.release();
+  ms.phase.release();
   if (ms.exec) (void)hipGraphExecDestroy(ms.exec);
   delete c.mis_state;
   c.mis_state = nullptr;
@@ -444,12 +667,18 @@ void mis_cuts(const std::vector<int32_t>& uv, const std::vector<int64_t>& deg, i
   }
 }
 
+MisDev dev_of(Ctx& c, MisState& ms) {
+  return MisDev{c.g.view(), c.g.sorted, ms.uv2.p, ms.ins2.p, c.mis, c.mis_tag, ms.mcount.p, c.mis_minw, ms.minwn.p,
+                ms.w0.p, ms.w1.p};
+}
+
 void mm_launch_round(Ctx& c, MisState& ms, SegView sv) {
-  GView g = c.g.view();
-  k_mm_eval<<<ms.grid, kBlock, 0, c.s>>>(g, ms.uv2.p, ms.ins2.p, sv, c.mis, c.mis_tag, c.mis_minw, ms.w0.p,
-                                         ms.w1.p);
-  k_mm_check<<<ms.grid, kBlock, 0, c.s>>>(g, ms.uv2.p, ms.ins2.p, sv, c.mis, c.mis_tag, c.mis_minw);
-  k_mm_commit<<<ms.grid, kBlock, 0, c.s>>>(ms.ins2.p, sv, ms.w0.p, ms.w1.p, c.mis, c.mis_minw);
+  MisDev d = dev_of(c, ms);
+  k_mis_tag_endpoints<<<kListGrid, kBlock, 0, c.s>>>(ms.uv2.p, sv.t, sv.ph, sv.pcap, c.mis_tag);
+  k_mis_tag_hop1<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), ms.uv2.p, sv.t, sv.ph, sv.pcap, c.mis_tag);
+  k_mm_eval<<<ms.grid, kBlock, 0, c.s>>>(d, sv);
+  k_mm_check<<<ms.grid, kBlock, 0, c.s>>>(d, sv);
+  k_mm_commit<<<ms.grid, kBlock, 0, c.s>>>(d, sv);
   k_mm_round_end<<<1, kBlock, 0, c.s>>>(sv);
 }
 
@@ -461,6 +690,8 @@ void mm_launch_group(Ctx& c, MisState& ms, SegView sv) {
 int mm_graph(Ctx& c, MisState& ms, SegView sv) {
   GView g = c.g.view();
   std::vector<const void*> key = {ms.uv2.p, ms.ins2.p, sv.t, ms.w0.p, ms.w1.p, c.mis, c.mis_tag, c.mis_minw,
+                                  ms.mcount.p, ms.minwn.p, (const void*)(intptr_t)c.g.sorted, sv.ph,
+                                  (const void*)(intptr_t)sv.pcap, (const void*)(intptr_t)sv.scap,
                                   g.boff, g.bcol, g.aoff, g.alen, g.acol, (const void*)(intptr_t)g.n,
                                   (const void*)(intptr_t)sv.stride, (const void*)(intptr_t)ms.grid,
                                   (const void*)(intptr_t)ms.group_rounds};
@@ -512,7 +743,21 @@ int mis_repair_multi(Ctx& c, Channel& ch, int32_t ninst, const int64_t* off, int
   SCOUT_CHECK(ms.seg.reserve(seg_table_size(stride)));
   // the table stride is part of the captured graph: keep the largest seen
   stride = (ms.seg.cap - kSegHdr - 1) / 5;
-  SegView sv{ms.seg.p, stride};
+  // member counts for the graph as applied: one batch on from the last repair, or recomputed
+  bool fresh = ms.mcount.cap >= (int64_t)n + 1 && ms.mc_version == c.mis_version && ms.mc_gen == c.g.base_gen;
+  SCOUT_CHECK(ms.mcount.reserve((int64_t)n + 1));
+  if (!ms.minwn.p) {
+    SCOUT_CHECK(ms.minwn.reserve((int64_t)n + 1));
+    SCOUT_TRY(hipMemsetAsync(ms.minwn.p, 0x7f, ((size_t)n + 1) * 4, c.s));  // 0x7f7f7f7f > any line
+  }
+  if (fresh && ms.mc_apply == c.apply_seq - 1) {
+    k_mcount_adjust<<<grid_for(m), kBlock, 0, c.s>>>(ch.d_uv.p, ch.d_ins.p, ch.d_done.p, m, c.mis, ms.mcount.p);
+  } else if (!(fresh && ms.mc_apply == c.apply_seq)) {
+    k_mcount_init<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), c.mis, ms.mcount.p);
+  }
+  ms.mc_version = c.mis_version;
+  ms.mc_gen = c.g.base_gen;
+  ms.mc_apply = c.apply_seq;
   k_mis_degrees<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), ch.d_uv.p, m, ms.deg.p);
   ms.h_deg.resize((size_t)(2 * m));
   std::vector<int32_t> uv((size_t)(2 * m));
@@ -560,54 +805,82 @@ int mis_repair_multi(Ctx& c, Channel& ch, int32_t ninst, const int64_t* off, int
     SCOUT_TRY(hipMemcpyAsync(ms.uv2.p, ch.d_uv.p, (size_t)(2 * m) * 4, hipMemcpyDeviceToDevice, c.s));
     SCOUT_TRY(hipMemcpyAsync(ms.ins2.p, ch.d_ins.p, (size_t)m, hipMemcpyDeviceToDevice, c.s));
   }
-  if (ms.use_graph) SCOUT_CHECK(mm_graph(c, ms, sv));
-  std::vector<int64_t> tab((size_t)seg_table_size(stride), 0);
-  int64_t rounds = 0, writes = 0, evals = 0;
-  for (Phase& P : phases) {
-    if (ms.tag_ep >= (1 << 29)) {
-      SCOUT_TRY(hipMemsetAsync(c.mis_tag, 0xff, ((size_t)n + 1) * 4, c.s));
-      ms.tag_ep = 0;
-    }
-    int32_t ep = ++ms.tag_ep;
-    k_mis_tag_endpoints<<<grid_for(P.hi - P.lo), kBlock, 0, c.s>>>(ms.uv2.p, P.lo, P.hi, c.mis_tag, ep);
-    k_mis_tag_hop1<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), ms.uv2.p, P.lo, P.hi, c.mis_tag, ep);
-    for (size_t b = 0; b < P.seg_lo.size(); b += (size_t)stride) {
-      // at most `stride` segments per pass (more instances than table rows run in passes)
-      size_t e = std::min(P.seg_lo.size(), b + (size_t)stride);
-      int64_t S = (int64_t)(e - b);
-      std::fill(tab.begin(), tab.end(), 0);
-      tab[0] = S;
-      tab[2] = ep;
-      int64_t slots = 0;
-      for (int64_t sg = 0; sg < S; sg++) {
-        int64_t lo = P.seg_lo[b + (size_t)sg], hi = P.seg_hi[b + (size_t)sg];
-        int64_t size = std::min(ms.window0, hi - lo);
-        tab[(size_t)(kSegHdr + sg)] = lo;
-        tab[(size_t)(kSegHdr + stride + sg)] = hi;
-        tab[(size_t)(kSegHdr + 2 * stride + sg)] = lo + size;
-        tab[(size_t)(kSegHdr + 3 * stride + sg)] = ms.window0;
-        tab[(size_t)(kSegHdr + 4 * stride + sg)] = slots;
-        slots += size;
-      }
-      tab[(size_t)(kSegHdr + 4 * stride + S)] = slots;
-      tab[1] = slots;
-      tab[3] = S;
-      // synchronous: the host table is reused by the next pass
-      SCOUT_TRY(hipMemcpyAsync(ms.seg.p, tab.data(), tab.size() * 8, hipMemcpyHostToDevice, c.s));
-      SCOUT_TRY(hipStreamSynchronize(c.s));
-      while (true) {
-        if (ms.use_graph) SCOUT_TRY(hipGraphLaunch(ms.exec, c.s));
-        else mm_launch_group(c, ms, sv);
-        SCOUT_TRY(hipMemcpyAsync(c.h_box, ms.seg.p, kSegHdr * sizeof(int64_t), hipMemcpyDeviceToHost, c.s));
-        SCOUT_TRY(hipStreamSynchronize(c.s));
-        c.readbacks++;
-        if (c.h_box[3] == 0) break;
-      }
-      rounds += c.h_box[4];
-      writes += c.h_box[5];
-      evals += c.h_box[6];
-    }
+  // phase table: capacities grow by doubling so the captured graph survives most batches
+  int64_t P = (int64_t)phases.size(), NS = 0;
+  for (Phase& F : phases) NS += (int64_t)F.seg_lo.size();
+  if (P > ms.pcap || NS > ms.scap || !ms.phase.p) {
+    while (ms.pcap < P) ms.pcap = std::max<int64_t>(2 * ms.pcap, 64);
+    while (ms.scap < NS) ms.scap = std::max<int64_t>(2 * ms.scap, 256);
+    ms.phase.release();
+    SCOUT_CHECK(ms.phase.reserve(phase_table_size(ms.pcap, ms.scap)));
   }
+  std::vector<int64_t> ph((size_t)phase_table_size(ms.pcap, ms.scap), 0);
+  {
+    int64_t sidx = 0;
+    for (int64_t f = 0; f < P; f++) {
+      Phase& F = phases[(size_t)f];
+      ph[(size_t)f] = F.lo;
+      ph[(size_t)(ms.pcap + f)] = F.hi;
+      ph[(size_t)(2 * ms.pcap + f)] = sidx;
+      for (size_t j = 0; j < F.seg_lo.size(); j++, sidx++) {
+        ph[(size_t)(3 * ms.pcap + 1 + sidx)] = F.seg_lo[j];
+        ph[(size_t)(3 * ms.pcap + 1 + ms.scap + sidx)] = F.seg_hi[j];
+      }
+    }
+    ph[(size_t)(2 * ms.pcap + P)] = sidx;
+  }
+  SegView sv{ms.seg.p, stride, ms.phase.p, ms.pcap, ms.scap};
+  // tag epochs: one per phase
+  if (ms.tag_ep + P + 1 >= (1 << 29)) {
+    SCOUT_TRY(hipMemsetAsync(c.mis_tag, 0xff, ((size_t)n + 1) * 4, c.s));
+    ms.tag_ep = 0;
+  }
+  int64_t ep0 = ms.tag_ep + 1;
+  ms.tag_ep += P;
+  // segment table with phase 0 loaded
+  std::vector<int64_t> tab((size_t)seg_table_size(stride), 0);
+  int64_t slots0 = 0;
+  {
+    Phase& F = phases[0];
+    int64_t S = (int64_t)F.seg_lo.size();
+    for (int64_t sg = 0; sg < S; sg++) {
+      int64_t lo = F.seg_lo[(size_t)sg], hi = F.seg_hi[(size_t)sg];
+      int64_t size = std::min(ms.window0, hi - lo);
+      tab[(size_t)(kSegHdr + sg)] = lo;
+      tab[(size_t)(kSegHdr + stride + sg)] = hi;
+      tab[(size_t)(kSegHdr + 2 * stride + sg)] = lo + size;
+      tab[(size_t)(kSegHdr + 3 * stride + sg)] = ms.window0;
+      tab[(size_t)(kSegHdr + 4 * stride + sg)] = slots0;
+      slots0 += size;
+    }
+    tab[(size_t)(kSegHdr + 4 * stride + S)] = slots0;
+    tab[0] = S;
+    tab[1] = slots0;
+    tab[2] = ep0;
+    tab[3] = S;
+    tab[8] = 0;
+    tab[9] = P;
+    tab[10] = 1;
+    tab[11] = ep0;
+    tab[12] = ms.window0;
+  }
+  SCOUT_TRY(hipMemcpyAsync(ms.phase.p, ph.data(), ph.size() * 8, hipMemcpyHostToDevice, c.s));
+  SCOUT_TRY(hipMemcpyAsync(ms.seg.p, tab.data(), tab.size() * 8, hipMemcpyHostToDevice, c.s));
+  SCOUT_TRY(hipStreamSynchronize(c.s));  // the host tables are freed on return
+  if (ms.use_graph) SCOUT_CHECK(mm_graph(c, ms, sv));
+  SCOUT_CHECK(prof_begin(c, PROF_MIS_EVAL));
+  while (true) {
+    if (ms.use_graph) SCOUT_TRY(hipGraphLaunch(ms.exec, c.s));
+    else mm_launch_group(c, ms, sv);
+    SCOUT_TRY(hipMemcpyAsync(c.h_box, ms.seg.p, kSegHdr * sizeof(int64_t), hipMemcpyDeviceToHost, c.s));
+    SCOUT_TRY(hipStreamSynchronize(c.s));
+    c.readbacks++;
+    if (c.h_box[8] >= P) break;
+  }
+  int64_t rounds = c.h_box[4], writes = c.h_box[5], evals = c.h_box[6] + slots0, entries = c.h_box[7];
+  // per line evaluation: the pair, label, row offsets of both endpoints, memberships and the
+  // write slots (~48 B); per row entry read: the entry and the membership byte (5 B)
+  SCOUT_CHECK(prof_end(c, PROF_MIS_EVAL, 48 * evals + 5 * entries));
   st->rounds += rounds;
   st->conflicts += writes;
   st->touched += evals;  // line evaluations of pass A over all rounds

@@ -122,9 +122,90 @@ __global__ void k_gen_ins(GView g, bool sorted, uint64_t key, uint64_t base, int
   if (claim_pair(set, mask, u, v)) emit(u, v, out, count, limit);
 }
 
+// A vertex drawn with probability proportional to its live degree + 1: a uniform position
+// over the base entries, the inserted-row pool and one slot per vertex; an entry position
+// gives its row's vertex if the entry is live, else the draw is rejected (false).
+__device__ __forceinline__ bool pick_degree_plus_one(const GView& g, int64_t E, int64_t A, const int32_t* aown,
+                                                     uint64_t x, uint32_t* out) {
+  uint64_t total = (uint64_t)(E + A + g.n);
+  uint64_t pos = __umul64hi(x, total);
+  if (pos < (uint64_t)E) {
+    int64_t p = (int64_t)pos;
+    if (g.bcol[p] < 0) return false;
+    int64_t l = 0, h = g.n;  // last vertex whose offset <= p
+    while (l < h) {
+      int64_t mid = (l + h + 1) >> 1;
+      if (g.boff[mid] <= p) l = mid;
+      else h = mid - 1;
+    }
+    *out = (uint32_t)l;
+    return true;
+  }
+  if (pos < (uint64_t)(E + A)) {
+    int64_t p = (int64_t)pos - E;
+    int32_t w = g.acol[p];
+    int32_t o = aown[p];
+    if (w < 0 || o < 0 || o >= g.n) return false;
+    if (p < g.aoff[o] || p >= g.aoff[o] + g.alen[o]) return false;
+    *out = (uint32_t)o;
+    return true;
+  }
+  *out = (uint32_t)(pos - (uint64_t)(E + A));
+  return true;
+}
+
+// Degree-biased deletions: u with probability proportional to degree + 1, then a uniform live
+// entry of u's row (rejection over its base and inserted entries).
+__global__ void k_gen_del_deg(GView g, int64_t E, int64_t A, const int32_t* aown, uint64_t key, uint64_t base,
+                              int64_t draws, uint64_t* set, uint64_t mask, int32_t* out, int64_t* count,
+                              int64_t limit) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= draws || *count >= limit) return;
+  uint64_t x = draw(key, base + 2 * (uint64_t)i);
+  uint32_t u;
+  if (!pick_degree_plus_one(g, E, A, aown, x, &u)) return;
+  Row r = row_of(g, (int32_t)u);
+  int64_t len = row_len(r);
+  if (len == 0 || len > 0xffffffffLL) return;
+  int32_t w = row_at(g, r, (int64_t)uniform_below(draw(key, base + 2 * (uint64_t)i + 1), (uint32_t)len));
+  if (w < 0 || (uint32_t)w == u) return;
+  if (claim_pair(set, mask, u, (uint32_t)w)) emit(u, (uint32_t)w, out, count, limit);
+}
+
+// Degree-biased insertions: both endpoints with probability proportional to degree + 1, kept
+// if they are distinct and not adjacent.
+__global__ void k_gen_ins_deg(GView g, bool sorted, int64_t E, int64_t A, const int32_t* aown, uint64_t key,
+                              uint64_t base, int64_t draws, uint64_t* set, uint64_t mask, int32_t* out,
+                              int64_t* count, int64_t limit) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= draws || *count >= limit) return;
+  uint32_t u, v;
+  if (!pick_degree_plus_one(g, E, A, aown, draw(key, base + 2 * (uint64_t)i), &u)) return;
+  if (!pick_degree_plus_one(g, E, A, aown, draw(key, base + 2 * (uint64_t)i + 1), &v)) return;
+  if (u == v) return;
+  if (adjacent_now(pair_state(g, sorted, (int32_t)u, (int32_t)v))) return;
+  if (claim_pair(set, mask, u, v)) emit(u, v, out, count, limit);
+}
+
+// Shuffle of a generated batch: key = (hash of the line index under the batch key) << 32 | index.
+__global__ void k_shuffle_keys(uint64_t key, int64_t m, uint64_t* keys) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < m) keys[i] = (draw(key, (uint64_t)i) & 0xffffffff00000000ULL) | (uint64_t)i;
+}
+
+__global__ void k_gather_lines(const uint64_t* keys, int64_t m, const int32_t* in, int32_t* out) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  int64_t k = (int64_t)(keys[i] & 0xffffffffULL);
+  out[2 * i] = in[2 * k];
+  out[2 * i + 1] = in[2 * k + 1];
+}
+
 }  // namespace
 
 struct StreamState {
+  int mode = SCOUT_GEN_UNIFORM;
+  uint64_t batches = 0;  // batches generated so far
   DevBuf<uint64_t> set;
   uint64_t mask = 0;
   uint64_t draws_used[3] = {0, 0, 0};
@@ -169,14 +250,23 @@ int gen_batch(Ctx& c, Channel& ch, int64_t size, double delete_frac, double tree
       if (kind == 0)
         k_gen_tree<<<grid_for(draws), kBlock, 0, c.s>>>(c.g.view(), c.level, keys[0], s.draws_used[0], draws, s.set.p,
                                                         s.mask, out, c.d_box, want[kind]);
+      else if (kind == 1 && s.mode == SCOUT_GEN_DEGREE)
+        k_gen_del_deg<<<grid_for(draws), kBlock, 0, c.s>>>(c.g.view(), c.g.E, c.g.A, c.g.aown.p, keys[1],
+                                                           s.draws_used[1], draws, s.set.p, s.mask, out, c.d_box,
+                                                           want[kind]);
       else if (kind == 1)
         k_gen_del<<<grid_for(draws), kBlock, 0, c.s>>>(c.g.view(), c.g.E, c.g.A, c.g.aown.p, keys[1],
                                                        s.draws_used[1], draws, s.set.p, s.mask, out, c.d_box,
                                                        want[kind]);
+      else if (s.mode == SCOUT_GEN_DEGREE)
+        k_gen_ins_deg<<<grid_for(draws), kBlock, 0, c.s>>>(c.g.view(), c.g.sorted, c.g.E, c.g.A, c.g.aown.p, keys[2],
+                                                           s.draws_used[2], draws, s.set.p, s.mask, out, c.d_box,
+                                                           want[kind]);
       else
         k_gen_ins<<<grid_for(draws), kBlock, 0, c.s>>>(c.g.view(), c.g.sorted, keys[2], s.draws_used[2], draws, s.set.p,
                                                        s.mask, out, c.d_box, want[kind]);
-      s.draws_used[kind] += (uint64_t)draws;
+      // the degree-biased kernels use two draws per attempt
+      s.draws_used[kind] += (uint64_t)draws * ((kind > 0 && s.mode == SCOUT_GEN_DEGREE) ? 2 : 1);
       SCOUT_CHECK(box_read(c, 1));
       got = std::min(c.h_box[0], want[kind]);
     }
@@ -184,8 +274,26 @@ int gen_batch(Ctx& c, Channel& ch, int64_t size, double delete_frac, double tree
     // skipped by every later batch, which keeps the no-repeat rule.
     produced += got;
   }
+  if (s.mode == SCOUT_GEN_DEGREE && produced > 1) {
+    // interleave deletions and insertions (deterministic shuffle)
+    DevBuf<uint64_t> k1, k2;
+    DevBuf<int32_t> tmp;
+    SCOUT_CHECK(k1.reserve(produced));
+    SCOUT_CHECK(k2.reserve(produced));
+    SCOUT_CHECK(tmp.reserve(2 * produced));
+    k_shuffle_keys<<<grid_for(produced), kBlock, 0, c.s>>>(stream_key(seed, RNG_STREAM_SIZE) + s.batches, produced,
+                                                           k1.p);
+    SCOUT_CHECK(sort_keys_u64(c, k1.p, k2.p, produced, 0, 64));
+    SCOUT_TRY(hipMemcpyAsync(tmp.p, ch.d_uv.p, (size_t)(2 * produced) * 4, hipMemcpyDeviceToDevice, c.s));
+    k_gather_lines<<<grid_for(produced), kBlock, 0, c.s>>>(k2.p, produced, tmp.p, ch.d_uv.p);
+    SCOUT_TRY(hipStreamSynchronize(c.s));
+    k1.release();
+    k2.release();
+    tmp.release();
+  }
+  s.batches++;
   SCOUT_TRY(hipEventRecord(ch.staged, c.s));
-  ch.m = produced;
+  channel_staged(ch, produced);
   if (out_m) *out_m = produced;
   return SCOUT_OK;
 }
@@ -229,7 +337,7 @@ int batch_load(Ctx& c, Channel& ch, int slot) {
     SCOUT_TRY(hipMemcpyAsync(ch.d_uv.p, c.stream->slots[(size_t)slot].p, (size_t)(2 * m) * 4, hipMemcpyDeviceToDevice,
                              c.cs));
   SCOUT_TRY(hipEventRecord(ch.staged, c.cs));
-  ch.m = m;
+  channel_staged(ch, m);
   return SCOUT_OK;
 }
 
@@ -266,7 +374,7 @@ int batch_append(Ctx& c, Channel& ch, int slot) {
     SCOUT_TRY(hipMemcpyAsync(ch.d_uv.p + 2 * old, c.stream->slots[(size_t)slot].p, (size_t)(2 * add) * 4,
                              hipMemcpyDeviceToDevice, c.cs));
   SCOUT_TRY(hipEventRecord(ch.staged, c.cs));
-  ch.m = old + add;
+  channel_staged(ch, old + add);
   return SCOUT_OK;
 }
 
@@ -360,6 +468,7 @@ int snapshot_restore(Ctx& c) {
     SCOUT_TRY(hipMemcpyAsync(c.g.aown.p, s.aown.p, (size_t)s.A * 4, hipMemcpyDeviceToDevice, c.s));
   }
   c.g.E = s.E;
+  c.g.base_gen++;
   c.g.A = s.A;
   c.g.live = s.live;
   c.g.holes = s.holes;
@@ -380,6 +489,7 @@ int snapshot_restore(Ctx& c) {
   if (s.has_mis) {
     SCOUT_CHECK(ensure_state(c, SCOUT_STATE_MIS));
     SCOUT_TRY(hipMemcpyAsync(c.mis, s.mis.p, (size_t)n, hipMemcpyDeviceToDevice, c.s));
+    c.mis_version++;
   }
   SCOUT_TRY(hipStreamSynchronize(c.s));
   return SCOUT_OK;
@@ -422,6 +532,17 @@ int scout_gen_batch(scout_ctx* x, int channel, int64_t size, double delete_frac,
     return SCOUT_ERR_STATE;
   }
   return gen_batch(c, *ch, size, delete_frac, tree_frac, seed, expected_total, out_m);
+}
+
+int scout_gen_mode(scout_ctx* x, int mode) {
+  if (!x || (mode != SCOUT_GEN_UNIFORM && mode != SCOUT_GEN_DEGREE)) {
+    set_error("gen_mode: unknown mode");
+    return SCOUT_ERR_ARG;
+  }
+  Ctx& c = x->c;
+  if (!c.stream) c.stream = new StreamState();
+  c.stream->mode = mode;
+  return SCOUT_OK;
 }
 
 int scout_batch_download(scout_ctx* x, int channel, int32_t* uv, int64_t cap_lines) {

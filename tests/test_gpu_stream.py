@@ -1,0 +1,42 @@
+"""GPU tests of the degree-biased stream generator (GEN_DEGREE): every line is a canonical
+u < v pair, no pair repeats within or across batches, deletions are current edges and
+insertions current non-edges (labelled by the oracle on the same graph), deletions and
+insertions are interleaved, and endpoints lean towards high degree."""
+import numpy as np
+import pytest
+
+import scout_amd as sa
+from oracle_lib import OracleGraph
+
+pytestmark = pytest.mark.gpu
+
+
+def test_degree_stream_is_valid_and_biased():
+    print("stream: degree-biased batches on R-MAT scale 12", flush=True)
+    with sa.Engine(0) as eng:
+        eng.gen_rmat(12, 16, seed=3)
+        g = eng.download_graph()
+        og = OracleGraph(n=g.n, off=g.off, col=g.col)
+        deg = np.diff(g.off)
+        eng.gen_mode(sa.GEN_DEGREE)
+        seen = set()
+        ins_deg, uni_deg = [], float(deg.mean())
+        for b in range(3):
+            m = eng.gen_batch(2000, 0.5, 0.0, seed=5, expected_total=6000)
+            uv = eng.batch_download(m)
+            assert m == 2000
+            assert np.all(uv[:, 0] < uv[:, 1])
+            pairs = set(map(tuple, uv.tolist()))
+            assert len(pairs) == m and not (pairs & seen)
+            seen |= pairs
+            ins = og.classify(uv)
+            assert int(ins.sum()) == 1000  # the 1000 insertions are exactly the non-edges
+            # interleaved: insertions are not one block at the end
+            first_ins = int(np.argmax(ins))
+            assert first_ins < 100 and ins[-100:].sum() < 100
+            ins_deg.append(deg[uv[ins == 1]].mean())
+            eng.classify()
+            eng.apply()
+            og.apply(uv, ins)
+        # endpoints proportional to degree + 1 have a mean degree far above the vertex mean
+        assert min(ins_deg) > 3 * uni_deg
