@@ -191,26 +191,36 @@ __device__ int try_endpoint(const MisView& s, int32_t c) {
   return has ? 0 : 1;
 }
 
-// Candidates of row r in order, 64 at a time (one lane each; endpoints go through the wave
-// when their turn comes). Returns 1 with *accepted, 0 if none is accepted, -1 on conflict.
+// Candidates of row r in order, 256 at a time (one lane each for four chunks of 64, taken
+// chunk by chunk in order; endpoints go through the wave when their turn comes). Returns 1
+// with *accepted, 0 if none is accepted, -1 on conflict.
 __device__ int first_candidate(const MisView& s, const Row& r, int32_t* accepted) {
+  constexpr int kU = 4;
   int64_t len = row_len(r);
   *s.scanned += len;
-  for (int64_t g0 = 0; g0 < len; g0 += kWave) {
-    int64_t k = g0 + lane_id();
-    int32_t c = k < len ? row_at(s.g, r, k) : -1;
-    int res = c >= 0 ? candidate_hop1(s, c) : 0;
-    unsigned long long pending = __ballot(res != 0);
-    while (pending) {
-      int j = __ffsll((long long)pending) - 1;
-      pending &= pending - 1;
-      int rj = __shfl(res, j);
-      int32_t cj = __shfl(c, j);
-      if (rj == 3) rj = try_endpoint(s, cj);
-      if (rj == -1) return -1;
-      if (rj == 1) {
-        *accepted = cj;
-        return 1;
+  for (int64_t g0 = 0; g0 < len; g0 += kU * kWave) {
+    int32_t cs[kU];
+    int res[kU];
+#pragma unroll
+    for (int u = 0; u < kU; u++) {
+      int64_t k = g0 + u * kWave + lane_id();
+      cs[u] = k < len ? row_at(s.g, r, k) : -1;
+    }
+#pragma unroll
+    for (int u = 0; u < kU; u++) res[u] = cs[u] >= 0 ? candidate_hop1(s, cs[u]) : 0;
+    for (int u = 0; u < kU; u++) {
+      unsigned long long pending = __ballot(res[u] != 0);
+      while (pending) {
+        int j = __ffsll((long long)pending) - 1;
+        pending &= pending - 1;
+        int rj = __shfl(res[u], j);
+        int32_t cj = __shfl(cs[u], j);
+        if (rj == 3) rj = try_endpoint(s, cj);
+        if (rj == -1) return -1;
+        if (rj == 1) {
+          *accepted = cj;
+          return 1;
+        }
       }
     }
   }
@@ -270,6 +280,8 @@ __device__ bool eval_line(MisView s, int32_t u, int32_t v, bool ins, int32_t* w0
   return true;
 }
 
+constexpr int64_t kTagChunk = 4096;  // row entries per hop-1 tag task
+
 // Endpoint and hop-1 tags of the phase that starts this round (nothing otherwise).
 __global__ void k_mis_tag_endpoints(const int32_t* uv, const int64_t* t, const int64_t* ph, int64_t pcap,
                                     int32_t* tag) {
@@ -283,16 +295,21 @@ __global__ void k_mis_tag_endpoints(const int32_t* uv, const int64_t* t, const i
   }
 }
 
+// Hop-1 tags: wave per tag task (a chunk of an endpoint's row), so hub rows spread over
+// many waves.
 __global__ void k_mis_tag_hop1(GView g, const int32_t* uv, const int64_t* t, const int64_t* ph, int64_t pcap,
-                               int32_t* tag) {
+                               int64_t scap, int32_t* tag) {
   if (!t[10]) return;
   int64_t f = t[8];
-  int64_t lo = ph[f], hi = ph[pcap + f];
+  const int64_t* task0 = ph + 3 * pcap + 1 + 2 * scap;
+  const int64_t* tasks = ph + 4 * pcap + 2 + 2 * scap;
   int32_t ep = (int32_t)t[2];
-  for (int64_t w = global_wave(); w < 2 * (hi - lo); w += total_waves()) {
-    int32_t x = uv[2 * lo + w];
+  for (int64_t i = task0[f] + global_wave(); i < task0[f + 1]; i += total_waves()) {
+    int64_t task = tasks[i];
+    int32_t x = uv[task >> 20];
     Row r = row_of(g, x);
-    wave_walk_wide<8>(g, r, 0, row_len(r), [&](int32_t y) { atomicMax(&tag[y], 2 * ep); });
+    int64_t lo = (task & 0xfffff) * kTagChunk, hi = min(lo + kTagChunk, row_len(r));
+    wave_walk_wide<8>(g, r, lo, hi, [&](int32_t y) { atomicMax(&tag[y], 2 * ep); });
   }
 }
 
@@ -315,7 +332,9 @@ __global__ void k_mis_degrees(GView g, const int32_t* uv, int64_t m, int64_t* de
       removed += __shfl_xor(removed, off);
       added += __shfl_xor(added, off);
     }
-    if (lane_id() == 0) deg[w] = (b1 - b0) - removed + added;
+    // low 32 bits: the degree estimate; high 32 bits: the row's entries (tombstones and holes
+    // included), which the hop-1 tag tasks cover
+    if (lane_id() == 0) deg[w] = ((b1 - b0) - removed + added) | (((b1 - b0) + (a1 - a0)) << 32);
   }
 }
 
@@ -326,8 +345,9 @@ __global__ void k_mis_degrees(GView g, const int32_t* uv, int64_t m, int64_t* de
 // current phase, [9] phases, [10] 1 = the phase starts this round (tags to set), [11] first
 // tag epoch, [12] initial window; then per segment: pending start p, end hi, first conflict
 // k0, window size win, and woff (exclusive prefix of the current window sizes, S + 1).
-// Phase table (int64): lines lo[P] and hi[P] of each phase, first segment seg0[P + 1], then
-// the segments' line ranges slo[NS] and shi[NS]; its layout sizes are in the SegView.
+// Phase table (int64): lines lo[P] and hi[P] of each phase, first segment seg0[P + 1], the
+// segments' line ranges slo[NS] and shi[NS], first tag task task0[P + 1] and the tasks
+// (endpoint slot << 20 | chunk of kTagChunk row entries); its capacities are in the SegView.
 constexpr int kSegHdr = 16;
 constexpr int64_t kMinWindow = 64;
 constexpr int64_t kMaxWindow = 1 << 20;
@@ -348,10 +368,15 @@ struct SegView {
   __device__ int64_t ph_seg0(int64_t f) const { return ph[2 * pcap + f]; }
   __device__ int64_t seg_lo(int64_t i) const { return ph[3 * pcap + 1 + i]; }
   __device__ int64_t seg_hi(int64_t i) const { return ph[3 * pcap + 1 + scap + i]; }
+  __device__ int64_t task0(int64_t f) const { return ph[3 * pcap + 1 + 2 * scap + f]; }
+  __device__ int64_t task(int64_t i) const { return ph[4 * pcap + 2 + 2 * scap + i]; }
 };
 
 inline int64_t seg_table_size(int64_t stride) { return kSegHdr + 5 * stride + 1; }
-inline int64_t phase_table_size(int64_t pcap, int64_t scap) { return 3 * pcap + 1 + 2 * scap; }
+inline int64_t phase_table_size(int64_t pcap, int64_t scap, int64_t tcap) {
+  return 4 * pcap + 2 + 2 * scap + tcap;
+}
+
 
 // Window slot -> (segment, line): binary search over woff.
 __device__ __forceinline__ int64_t slot_line(const SegView& sv, int64_t slot, int32_t* sg_out) {
@@ -616,7 +641,7 @@ struct MisState {
   DevBuf<int32_t> mcount, minwn;
   int64_t mc_version = -1, mc_apply = -1, mc_gen = -1;  // what mcount was computed for
   DevBuf<int64_t> phase;  // phase table
-  int64_t pcap = 0, scap = 0;
+  int64_t pcap = 0, scap = 0, tcap = 0;
   hipGraphExec_t exec = nullptr;
   std::vector<const void*> key;
 };
@@ -675,7 +700,7 @@ MisDev dev_of(Ctx& c, MisState& ms) {
 void mm_launch_round(Ctx& c, MisState& ms, SegView sv) {
   MisDev d = dev_of(c, ms);
   k_mis_tag_endpoints<<<kListGrid, kBlock, 0, c.s>>>(ms.uv2.p, sv.t, sv.ph, sv.pcap, c.mis_tag);
-  k_mis_tag_hop1<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), ms.uv2.p, sv.t, sv.ph, sv.pcap, c.mis_tag);
+  k_mis_tag_hop1<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), ms.uv2.p, sv.t, sv.ph, sv.pcap, sv.scap, c.mis_tag);
   k_mm_eval<<<ms.grid, kBlock, 0, c.s>>>(d, sv);
   k_mm_check<<<ms.grid, kBlock, 0, c.s>>>(d, sv);
   k_mm_commit<<<ms.grid, kBlock, 0, c.s>>>(d, sv);
@@ -692,6 +717,7 @@ int mm_graph(Ctx& c, MisState& ms, SegView sv) {
   std::vector<const void*> key = {ms.uv2.p, ms.ins2.p, sv.t, ms.w0.p, ms.w1.p, c.mis, c.mis_tag, c.mis_minw,
                                   ms.mcount.p, ms.minwn.p, (const void*)(intptr_t)c.g.sorted, sv.ph,
                                   (const void*)(intptr_t)sv.pcap, (const void*)(intptr_t)sv.scap,
+                                  (const void*)(intptr_t)ms.tcap,
                                   g.boff, g.bcol, g.aoff, g.alen, g.acol, (const void*)(intptr_t)g.n,
                                   (const void*)(intptr_t)sv.stride, (const void*)(intptr_t)ms.grid,
                                   (const void*)(intptr_t)ms.group_rounds};
@@ -766,6 +792,11 @@ int mis_repair_multi(Ctx& c, Channel& ch, int32_t ninst, const int64_t* off, int
   SCOUT_TRY(hipMemcpyAsync(uv.data(), ch.d_uv.p, (size_t)(2 * m) * 4, hipMemcpyDeviceToHost, c.s));
   SCOUT_TRY(hipMemcpyAsync(ins.data(), ch.d_ins.p, (size_t)m, hipMemcpyDeviceToHost, c.s));
   SCOUT_TRY(hipStreamSynchronize(c.s));
+  std::vector<int64_t> row_entries((size_t)(2 * m));
+  for (size_t i = 0; i < (size_t)(2 * m); i++) {
+    row_entries[i] = ms.h_deg[i] >> 32;
+    ms.h_deg[i] &= 0xffffffffLL;
+  }
   std::vector<std::vector<std::pair<int64_t, int64_t>>> cuts((size_t)ninst);
   size_t nphase = 0;
   for (int32_t i = 0; i < ninst; i++) {
@@ -806,17 +837,21 @@ int mis_repair_multi(Ctx& c, Channel& ch, int32_t ninst, const int64_t* off, int
     SCOUT_TRY(hipMemcpyAsync(ms.ins2.p, ch.d_ins.p, (size_t)m, hipMemcpyDeviceToDevice, c.s));
   }
   // phase table: capacities grow by doubling so the captured graph survives most batches
-  int64_t P = (int64_t)phases.size(), NS = 0;
+  int64_t P = (int64_t)phases.size(), NS = 0, NT = 0;
   for (Phase& F : phases) NS += (int64_t)F.seg_lo.size();
-  if (P > ms.pcap || NS > ms.scap || !ms.phase.p) {
+  for (int64_t w = 0; w < 2 * m; w++) NT += (row_entries[(size_t)w] + kTagChunk - 1) / kTagChunk;
+  if (P > ms.pcap || NS > ms.scap || NT > ms.tcap || !ms.phase.p) {
     while (ms.pcap < P) ms.pcap = std::max<int64_t>(2 * ms.pcap, 64);
     while (ms.scap < NS) ms.scap = std::max<int64_t>(2 * ms.scap, 256);
+    while (ms.tcap < NT) ms.tcap = std::max<int64_t>(2 * ms.tcap, 4096);
     ms.phase.release();
-    SCOUT_CHECK(ms.phase.reserve(phase_table_size(ms.pcap, ms.scap)));
+    SCOUT_CHECK(ms.phase.reserve(phase_table_size(ms.pcap, ms.scap, ms.tcap)));
   }
-  std::vector<int64_t> ph((size_t)phase_table_size(ms.pcap, ms.scap), 0);
+  std::vector<int64_t> ph((size_t)phase_table_size(ms.pcap, ms.scap, ms.tcap), 0);
   {
-    int64_t sidx = 0;
+    int64_t sidx = 0, tidx = 0;
+    int64_t* task0 = ph.data() + 3 * ms.pcap + 1 + 2 * ms.scap;
+    int64_t* tasks = ph.data() + 4 * ms.pcap + 2 + 2 * ms.scap;
     for (int64_t f = 0; f < P; f++) {
       Phase& F = phases[(size_t)f];
       ph[(size_t)f] = F.lo;
@@ -826,8 +861,16 @@ int mis_repair_multi(Ctx& c, Channel& ch, int32_t ninst, const int64_t* off, int
         ph[(size_t)(3 * ms.pcap + 1 + sidx)] = F.seg_lo[j];
         ph[(size_t)(3 * ms.pcap + 1 + ms.scap + sidx)] = F.seg_hi[j];
       }
+      task0[f] = tidx;
+      for (int64_t tl = F.lo; tl < F.hi; tl++)
+        for (int side = 0; side < 2; side++) {
+          int64_t slot = 2 * tl + side;  // endpoint slot in the permuted batch
+          int64_t len = row_entries[(size_t)(2 * (int64_t)perm[(size_t)tl] + side)];
+          for (int64_t ck = 0; ck * kTagChunk < len; ck++) tasks[tidx++] = (slot << 20) | ck;
+        }
     }
     ph[(size_t)(2 * ms.pcap + P)] = sidx;
+    task0[P] = tidx;
   }
   SegView sv{ms.seg.p, stride, ms.phase.p, ms.pcap, ms.scap};
   // tag epochs: one per phase
