@@ -318,7 +318,9 @@ def run_color_mis(args):
     fam = {k: {"ms_per_batch": round(v["ms"] / steps, 4), "launches": v["launches"],
                "GBps": round(v["bytes"] / (v["ms"] / 1e3) / 1e9, 1) if v["ms"] > 0 else 0.0}
            for k, v in prof.items() if v["launches"]}
-    d = prof["color"]
+    # the dominant work is the MIS round loop (replayed from a captured graph, so it is timed as
+    # one span per repair with HIP events; its algorithmic bytes are counted on the device)
+    d = prof["mis_eval"]
     nl = max(d["launches"], 1)
     achieved = (d["bytes"] / nl) / (d["ms"] / 1e3 / nl) / 1e9 if d["ms"] > 0 else 0.0
     out = {
@@ -343,7 +345,8 @@ def run_color_mis(args):
         "verified_exact": verified,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "colouring rounds (one repair's launches)", "launches": d["launches"],
+                     "kernel": "MIS round loop (k_mm_eval, k_mm_check, k_mm_commit, k_mm_round_end, tags), "
+                               "one repair per launch", "launches": d["launches"],
                      "avg_launch_ms": round(d["ms"] / nl, 4),
                      "algorithmic_bytes_per_launch": round(d["bytes"] / nl), "batch_families": fam},
         "work_per_batch": {k: round(v / steps, 1) for k, v in head["work"].items()},
