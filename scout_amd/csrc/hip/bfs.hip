@@ -506,8 +506,11 @@ __global__ void k_apply_tentative(const int32_t* xs, const uint64_t* tent, int64
   block_push_seed(t >= 0, x, t, p, seeds, seed_par, box, cap);
 }
 
-// Insertion lines whose endpoints are adjacent after the batch: lower the far endpoint.
-__global__ void k_insert_seeds(GView g, bool sorted, const int32_t* uv, const uint8_t* ins, int64_t m,
+// Insertion lines whose endpoints are adjacent after the batch: lower the far endpoint. With
+// distinct pairs every insertion line's pair is adjacent after the batch (applied, or skipped
+// by a conditional apply because it already was), so the row search is needed only when a
+// pair repeats.
+__global__ void k_insert_seeds(GView g, bool sorted, bool distinct, const int32_t* uv, const uint8_t* ins, int64_t m,
                                int32_t* level, int64_t* seeds, int32_t* seed_par, int64_t cap, int64_t* box) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   bool is_ins = i < m && ins[i];
@@ -519,7 +522,7 @@ __global__ void k_insert_seeds(GView g, bool sorted, const int32_t* uv, const ui
     lu = level[u];
     lv = level[v];
     bool maybe = (lu >= 0 && (lv < 0 || lu + 1 < lv)) || (lv >= 0 && (lu < 0 || lv + 1 < lu));
-    if (maybe && adjacent_now(pair_state(g, sorted, u, v))) {
+    if (maybe && (distinct || adjacent_now(pair_state(g, sorted, u, v)))) {
       if (lu >= 0 && cas_min_level(&level[v], lu + 1)) sv = true;
       lv = level[v];
       if (lv >= 0 && cas_min_level(&level[u], lv + 1)) su = true;
@@ -752,9 +755,10 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, int32_t* P, int strategy, scou
       st->fallback = 1;
       return bfs_full_on(c, L, P, c.source, &st->rounds, &st->edges_scanned);
     }
-    predicted_last = c.h_box[25];
-    if (predicted_last >= cmin)
-      first_group = (int)std::min<int64_t>(std::max<int64_t>(predicted_last - cmin + 2, kFirstGroup), kMaxGroup);
+    // rounds cmin .. max(predicted last level, deepest candidate): one read-back when the
+    // prediction holds; an empty extra round costs as much as a read-back, so no slack
+    predicted_last = std::max<int64_t>(c.h_box[25], cmax);
+    first_group = (int)std::min<int64_t>(std::max<int64_t>(predicted_last - cmin + 1, kFirstGroup), kMaxGroup);
   }
   if (ncand > 0) {
     // Invalidation, level by level, queued in groups of rounds. Counts: [8]/[9] candidates at
@@ -831,8 +835,8 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, int32_t* P, int strategy, scou
     }
   }
   if (m > 0)
-    k_insert_seeds<<<grid_for(m), kBlock, 0, c.s>>>(c.g.view(), c.g.sorted, ch.d_uv.p, ch.d_ins.p, m, L, c.wa.p,
-                                                    c.lg.p, seed_cap, box);
+    k_insert_seeds<<<grid_for(m), kBlock, 0, c.s>>>(c.g.view(), c.g.sorted, ch.pairs_distinct, ch.d_uv.p, ch.d_ins.p,
+                                                    m, L, c.wa.p, c.lg.p, seed_cap, box);
   SCOUT_CHECK(box_read(c, 14));
   if (nx > 0) scanned += c.h_box[11];
   if (nx > 0 && c.h_box[13] > c.wb.cap) {
@@ -854,8 +858,9 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, int32_t* P, int strategy, scou
     k_zero<<<1, 64, 0, c.s>>>(box + 8, 1);
     // the learned scheduler's predicted last level bounds the relax as well
     int relax_group = kFirstGroup;
-    if (learned && predicted_last >= smin)
-      relax_group = (int)std::min<int64_t>(std::max<int64_t>(predicted_last - smin + 2, kFirstGroup), kMaxGroup);
+    if (learned && predicted_last >= 0)
+      relax_group = (int)std::min<int64_t>(
+          std::max<int64_t>(std::max<int64_t>(predicted_last, smax) - smin + 1, kFirstGroup), kMaxGroup);
     SCOUT_CHECK(relax_levels(c, L, P, smin, smax, c.wa.p, nseeds, epoch, &rounds, &scanned, relax_group));
   }
   st->rounds += rounds;

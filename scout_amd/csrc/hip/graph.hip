@@ -701,6 +701,7 @@ int graph_apply_lines(Ctx& c, Channel& ch, int64_t lo, int64_t hi, int condition
   k_dup_mark<<<grid_for(m), kBlock, 0, c.s>>>(ch.d_uv.p + 2 * lo, m, g.hkey.p, g.hline.p, size - 1, ch.d_done.p + lo,
                                               c.d_box);
   SCOUT_CHECK(box_read(c, 1));
+  ch.pairs_distinct = lo == 0 && hi == ch.m && c.h_box[0] == 0;
   if (c.h_box[0] == 0) return apply_range(c, ch, lo, hi, conditional);
   // Repeated pairs: cut the range into runs of distinct pairs, applied in order.
   std::vector<int32_t> uv((size_t)(2 * m));

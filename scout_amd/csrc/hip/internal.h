@@ -143,6 +143,9 @@ struct Channel {
   // lines [pos_lo, pos_hi) while the base generation is pos_gen
   DevBuf<int64_t> d_pos;
   int64_t pos_gen = -1, pos_lo = 0, pos_hi = 0;
+  // the last apply covered every staged line and no pair repeated: an insertion line's pair
+  // is then adjacent after the batch
+  bool pairs_distinct = false;
   int64_t m = 0;
   hipEvent_t staged = nullptr;
 };
@@ -151,6 +154,7 @@ struct Channel {
 inline void channel_staged(Channel& ch, int64_t m) {
   ch.m = m;
   ch.pos_lo = ch.pos_hi = 0;
+  ch.pairs_distinct = false;
 }
 
 struct MlpState;  // mlp.hip
@@ -259,7 +263,7 @@ constexpr int kMlpIn = 12, kMlpHidden = 64, kMlpOut = 3;
 constexpr int kMlpParams = kMlpIn * kMlpHidden + kMlpHidden + kMlpHidden * kMlpHidden + kMlpHidden +
                            kMlpHidden * kMlpOut + kMlpOut;
 constexpr int64_t kMlpSample = 16384;  // candidates per batch logged for training
-constexpr int64_t kMlpInfer = 2048;    // candidates per batch the scheduler predicts for
+constexpr int64_t kMlpInfer = 512;     // candidates per batch the scheduler predicts for
 bool mlp_logging(Ctx& c);
 bool mlp_ready(Ctx& c);
 // Features of the first min(ncand, kMlpSample) candidates (state before the repair); the
