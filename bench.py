@@ -345,8 +345,8 @@ def run_color_mis(args):
         "verified_exact": verified,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "MIS round loop (k_mm_eval, k_mm_check, k_mm_commit, k_mm_round_end, tags), "
-                               "one repair per launch", "launches": d["launches"],
+                     "kernel": "MIS round loop (tags, pass A, hop-2 plan/expand/scan/resume, pass B, commit, "
+                               "round end), one repair per launch", "launches": d["launches"],
                      "avg_launch_ms": round(d["ms"] / nl, 4),
                      "algorithmic_bytes_per_launch": round(d["bytes"] / nl), "batch_families": fam},
         "work_per_batch": {k: round(v / steps, 1) for k, v in head["work"].items()},

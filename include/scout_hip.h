@@ -64,7 +64,7 @@ typedef struct {
   int64_t insertions;   /* lines labelled insertion */
   int64_t deletions;    /* lines labelled deletion */
   int64_t noop;         /* lines that needed no repair work */
-  int64_t seeds;        /* vertices the repair started from */
+  int64_t seeds;        /* vertices the repair started from; mis: searches reaching hop-2 rows */
   int64_t touched;      /* vertices whose rows were scanned */
   int64_t rounds;       /* device rounds */
   int64_t fallback;     /* 1 if the batch was settled by a full recompute */

@@ -167,13 +167,16 @@ __device__ __forceinline__ int row_kind(const MisView& s, int32_t x) {
 // Candidate c of the insertion rule (not a member, has a row, no member in its row), one
 // lane: 1 = accept, 0 = reject, -1 = conflict (check mode), 3 = an endpoint (its row is its
 // 2-hop ball, evaluated by the wave).
+//
+// A candidate without a row is rejected whatever its membership, so its membership is not
+// read (and cannot conflict): the row kind (static within a sub-batch) is tested first.
 __device__ __forceinline__ int candidate_hop1(const MisView& s, int32_t c) {
+  int kind = row_kind(s, c);
+  if (kind == 0) return 0;
   bool conf = false;
   bool m = member(s, c, &conf);
   if (conf) return -1;
   if (m) return 0;
-  int kind = row_kind(s, c);
-  if (kind == 0) return 0;
   if (kind == 2) return 3;
   bool has = nbr_has_member(s, c, -1, &conf);
   if (conf) return -1;
@@ -229,9 +232,15 @@ __device__ int first_candidate(const MisView& s, const Row& r, int32_t* accepted
 
 // Evaluates one line. Outputs the vertices it writes (w0 -> 0 or 1 by rule, w1 -> 1) or
 // reports a conflict (check mode). Returns false on conflict.
-__device__ bool eval_line(MisView s, int32_t u, int32_t v, bool ins, int32_t* w0, int32_t* w1) {
+//
+// With defer_hop2, an insertion whose candidates [r] + N(r) are all rejected stops there with
+// *hop2 = true (w0 = r known, w1 open): the hop-2 candidates N(y), y in N(r), are then
+// searched by chunk tasks spread over many waves (k_mm_hop2_*).
+__device__ bool eval_line(MisView s, int32_t u, int32_t v, bool ins, int32_t* w0, int32_t* w1, bool defer_hop2,
+                          bool* hop2) {
   *w0 = -1;
   *w1 = -1;
+  *hop2 = false;
   bool conf = false;
   bool um = member(s, u, &conf);
   bool vm = member(s, v, &conf);
@@ -254,6 +263,10 @@ __device__ bool eval_line(MisView s, int32_t u, int32_t v, bool ins, int32_t* w0
     if (res < 0) return false;
     if (res > 0) {
       *w1 = acc;
+      return true;
+    }
+    if (defer_hop2) {
+      *hop2 = true;
       return true;
     }
     int64_t len = row_len(rr);
@@ -343,12 +356,14 @@ __global__ void k_mis_degrees(GView g, const int32_t* uv, int64_t m, int64_t* de
 // (window lines over all segments), [2] tag epoch, [3] segments still pending, [4] non-empty
 // rounds, [5] writes, [6] line evaluations, [7] row entries read by passes A and B, [8]
 // current phase, [9] phases, [10] 1 = the phase starts this round (tags to set), [11] first
-// tag epoch, [12] initial window; then per segment: pending start p, end hi, first conflict
+// tag epoch, [12] initial window, [13]/[14] hop-2 searches requested in passes A/B this round,
+// [15] hop-2 chunk tasks of the current pass, [16] hop-2 searches of pass A over all rounds;
+// then per segment: pending start p, end hi, first conflict
 // k0, window size win, and woff (exclusive prefix of the current window sizes, S + 1).
 // Phase table (int64): lines lo[P] and hi[P] of each phase, first segment seg0[P + 1], the
 // segments' line ranges slo[NS] and shi[NS], first tag task task0[P + 1] and the tasks
 // (endpoint slot << 20 | chunk of kTagChunk row entries); its capacities are in the SegView.
-constexpr int kSegHdr = 16;
+constexpr int kSegHdr = 20;
 constexpr int64_t kMinWindow = 64;
 constexpr int64_t kMaxWindow = 1 << 20;
 
@@ -414,14 +429,54 @@ struct MisDev {
   int32_t* minwn;
   int32_t* w0s;
   int32_t* w1s;
+  // hop-2 candidate searches of one pass: requests (line, segment), their best result, the
+  // y-block prefix, overflow flags, and the chunk tasks
+  int32_t* h2_line;
+  int32_t* h2_seg;
+  unsigned long long* h2_best;
+  int64_t* h2_off;
+  int32_t* h2_flag;
+  uint64_t* h2_task;
+  int64_t h2_rcap, h2_tcap;
 };
 
 __device__ __forceinline__ MisView view_of(const MisDev& d, int32_t ep, int64_t k, bool check, int64_t* scanned) {
   return MisView{d.g, d.sorted, d.mis, d.tag, ep, d.mcount, d.minw, d.minwn, (int32_t)k, check, -1, scanned};
 }
 
+constexpr int32_t kHop2Pending = -3;  // w1 of a line whose hop-2 search is still running
+constexpr int64_t kHop2Requests = 65536;   // deferred hop-2 searches per pass (16-bit task field)
+constexpr int64_t kHop2Tasks = 1 << 22;    // hop-2 chunk tasks per pass
+constexpr int64_t kHop2Chunk = 1024;  // row entries of N(y) per hop-2 task
+constexpr unsigned long long kNoResult = ~0ULL;
+
+// Marks the line's write w: minw[w] and, over w's row, minwn.
+__device__ __forceinline__ void mark_write(const MisDev& d, int32_t w, int64_t k) {
+  if (w < 0) return;
+  if (lane_id() == 0) atomicMin(&d.minw[w], (int32_t)k);
+  Row r = row_of(d.g, w);
+  wave_walk_wide<8>(d.g, r, 0, row_len(r), [&](int32_t z) { atomicMin(&d.minwn[z], (int32_t)k); });
+}
+
+// Records a deferred hop-2 search of line k (segment sg) in the pass's request list; false if
+// the list is full (the caller then searches serially).
+__device__ __forceinline__ bool hop2_request(const MisDev& d, int64_t* count, int64_t k, int32_t sg) {
+  int64_t idx = 0;
+  if (lane_id() == 0) {
+    idx = (int64_t)atomicAdd((unsigned long long*)count, 1ULL);
+    if (idx < d.h2_rcap) {
+      d.h2_line[idx] = (int32_t)k;
+      d.h2_seg[idx] = sg;
+      d.h2_best[idx] = kNoResult;
+      d.h2_flag[idx] = 0;
+    }
+  }
+  idx = __shfl(idx, 0);
+  return idx < d.h2_rcap;
+}
+
 // Pass A: wave per window line. Writes go to w0/w1; minw[w] = smallest writer of w and
-// minwn[z] = smallest writer of a neighbour of z.
+// minwn[z] = smallest writer of a neighbour of z. A deferred hop-2 search leaves w1 pending.
 __global__ void k_mm_eval(MisDev d, SegView sv) {
   int64_t slots = sv.t[1];
   int32_t ep = (int32_t)sv.t[2];
@@ -431,19 +486,18 @@ __global__ void k_mm_eval(MisDev d, SegView sv) {
     int64_t k = slot_line(sv, t, &sg);
     MisView s = view_of(d, ep, k, false, &scanned);
     int32_t w0, w1;
-    eval_line(s, d.uv[2 * k], d.uv[2 * k + 1], d.ins[k] != 0, &w0, &w1);
+    bool hop2 = false;
+    eval_line(s, d.uv[2 * k], d.uv[2 * k + 1], d.ins[k] != 0, &w0, &w1, true, &hop2);
+    if (hop2) {
+      if (hop2_request(d, &sv.t[13], k, sg)) w1 = kHop2Pending;
+      else eval_line(s, d.uv[2 * k], d.uv[2 * k + 1], d.ins[k] != 0, &w0, &w1, false, &hop2);
+    }
     if (lane_id() == 0) {
       d.w0s[k] = w0;
       d.w1s[k] = w1;
-      if (w0 >= 0) atomicMin(&d.minw[w0], (int32_t)k);
-      if (w1 >= 0) atomicMin(&d.minw[w1], (int32_t)k);
     }
-    for (int j = 0; j < 2; j++) {
-      int32_t w = j == 0 ? w0 : w1;
-      if (w < 0) continue;
-      Row r = row_of(d.g, w);
-      wave_walk_wide<8>(d.g, r, 0, row_len(r), [&](int32_t z) { atomicMin(&d.minwn[z], (int32_t)k); });
-    }
+    mark_write(d, w0, k);
+    mark_write(d, w1, k);
   }
   if (lane_id() == 0 && scanned) atomicAdd((unsigned long long*)&sv.t[7], (unsigned long long)scanned);
 }
@@ -461,8 +515,181 @@ __global__ void k_mm_check(MisDev d, SegView sv) {
     if (k >= __hip_atomic_load(&k0[sg], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) continue;
     MisView s = view_of(d, ep, k, true, &scanned);
     int32_t w0, w1;
-    bool ok = eval_line(s, d.uv[2 * k], d.uv[2 * k + 1], d.ins[k] != 0, &w0, &w1);
+    bool hop2 = false;
+    bool ok = eval_line(s, d.uv[2 * k], d.uv[2 * k + 1], d.ins[k] != 0, &w0, &w1, true, &hop2);
+    if (ok && hop2 && !hop2_request(d, &sv.t[14], k, sg))
+      ok = eval_line(s, d.uv[2 * k], d.uv[2 * k + 1], d.ins[k] != 0, &w0, &w1, false, &hop2);
     if (!ok && lane_id() == 0) atomicMin((long long*)&k0[sg], (long long)k);
+  }
+  if (lane_id() == 0 && scanned) atomicAdd((unsigned long long*)&sv.t[7], (unsigned long long)scanned);
+}
+
+// The insertion's r of line k.
+__device__ __forceinline__ int32_t line_r(const MisDev& d, int64_t k) {
+  int32_t u = d.uv[2 * k], v = d.uv[2 * k + 1];
+  return u > v ? u : v;
+}
+
+// Hop-2 plan (one block): y-blocks of 256 entries of N(r) per request, prefix in h2_off; the
+// task counter starts from 0.
+__global__ void k_mm_hop2_plan(MisDev d, SegView sv, int pass) {
+  __shared__ int64_t scan[kBlock];
+  __shared__ int64_t carry;
+  int64_t n = min(sv.t[pass == 0 ? 13 : 14], d.h2_rcap);
+  if (threadIdx.x == 0) {
+    carry = 0;
+    sv.t[15] = 0;
+    if (pass == 0) sv.t[16] += n;
+  }
+  __syncthreads();
+  for (int64_t base = 0; base < n; base += blockDim.x) {
+    int64_t i = base + threadIdx.x;
+    int64_t nb = 0;
+    if (i < n) nb = (row_len(row_of(d.g, line_r(d, d.h2_line[i]))) + kBlock - 1) / kBlock;
+    scan[threadIdx.x] = nb;
+    __syncthreads();
+    for (int off = 1; off < (int)blockDim.x; off <<= 1) {
+      int64_t y = threadIdx.x >= (unsigned)off ? scan[threadIdx.x - off] : 0;
+      __syncthreads();
+      scan[threadIdx.x] += y;
+      __syncthreads();
+    }
+    if (i < n) d.h2_off[i] = carry + scan[threadIdx.x] - nb;
+    __syncthreads();
+    if (threadIdx.x == blockDim.x - 1) carry += scan[threadIdx.x];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) d.h2_off[n] = carry;
+}
+
+// Hop-2 expand: block per y-block; thread per y in N(r): one task per kHop2Chunk entries of
+// N(y). A request whose tasks do not fit is flagged and searched serially by the resume.
+__global__ void k_mm_hop2_expand(MisDev d, SegView sv, int pass) {
+  __shared__ int64_t scan[kBlock];
+  __shared__ int64_t base_s;
+  int64_t n = min(sv.t[pass == 0 ? 13 : 14], d.h2_rcap);
+  int64_t nblocks = n > 0 ? d.h2_off[n] : 0;
+  for (int64_t b = blockIdx.x; b < nblocks; b += gridDim.x) {
+    int64_t lo = 0, hi = n;  // last request i with h2_off[i] <= b
+    while (hi - lo > 1) {
+      int64_t mid = (lo + hi) >> 1;
+      if (d.h2_off[mid] <= b) lo = mid;
+      else hi = mid;
+    }
+    int64_t i = lo;
+    int32_t r = line_r(d, d.h2_line[i]);
+    Row rr = row_of(d.g, r);
+    int64_t yi = (b - d.h2_off[i]) * kBlock + threadIdx.x;
+    int64_t nch = 0;
+    if (yi < row_len(rr)) {
+      int32_t y = row_at(d.g, rr, yi);
+      if (y >= 0) nch = (row_len(row_of(d.g, y)) + kHop2Chunk - 1) / kHop2Chunk;
+    }
+    scan[threadIdx.x] = nch;
+    __syncthreads();
+    for (int off = 1; off < (int)blockDim.x; off <<= 1) {
+      int64_t y = threadIdx.x >= (unsigned)off ? scan[threadIdx.x - off] : 0;
+      __syncthreads();
+      scan[threadIdx.x] += y;
+      __syncthreads();
+    }
+    if (threadIdx.x == blockDim.x - 1) base_s = (int64_t)atomicAdd((unsigned long long*)&sv.t[15], (unsigned long long)scan[threadIdx.x]);
+    __syncthreads();
+    int64_t at = base_s + scan[threadIdx.x] - nch;
+    if (at + nch > d.h2_tcap || nch > 0x1ffff) {
+      if (nch > 0) d.h2_flag[i] = 1;
+    } else {
+      for (int64_t ch = 0; ch < nch; ch++) d.h2_task[at + ch] = ((uint64_t)i << 48) | ((uint64_t)yi << 17) | (uint64_t)ch;
+    }
+    __syncthreads();
+  }
+}
+
+__device__ __forceinline__ unsigned long long hop2_key(int64_t yi, int64_t pos, int code) {
+  return ((unsigned long long)yi << 33) | ((unsigned long long)pos << 2) | (unsigned long long)code;
+}
+
+// Hop-2 scan: wave per task, the candidates of one chunk of N(y) in order; the first decisive
+// one (accept, or a conflict in pass B) lowers the request's best key (y index, position).
+__global__ void k_mm_hop2_scan(MisDev d, SegView sv, int pass) {
+  int64_t ntask = min(sv.t[15], d.h2_tcap);
+  int32_t ep = (int32_t)sv.t[2];
+  int64_t scanned = 0;
+  for (int64_t t = global_wave(); t < ntask; t += total_waves()) {
+    uint64_t task = d.h2_task[t];
+    int64_t i = (int64_t)(task >> 48), yi = (int64_t)((task >> 17) & 0x7fffffffULL), ch = (int64_t)(task & 0x1ffffULL);
+    if (d.h2_flag[i]) continue;
+    int64_t c0 = ch * kHop2Chunk;
+    unsigned long long* best = &d.h2_best[i];
+    if (__shfl(__hip_atomic_load(best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), 0) <= hop2_key(yi, c0, 0)) continue;
+    int64_t k = d.h2_line[i];
+    int32_t r = line_r(d, k);
+    int32_t y = row_at(d.g, row_of(d.g, r), yi);
+    Row ry = row_of(d.g, y);
+    int64_t c1 = min(c0 + kHop2Chunk, row_len(ry));
+    MisView s = view_of(d, ep, k, pass == 1, &scanned);
+    s.dropped = r;
+    scanned += c1 - c0;
+    bool done = false;
+    for (int64_t g0 = c0; g0 < c1 && !done; g0 += kWave) {
+      if (__shfl(__hip_atomic_load(best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), 0) <= hop2_key(yi, g0, 0)) break;
+      int64_t q = g0 + lane_id();
+      int32_t c = q < c1 ? row_at(d.g, ry, q) : -1;
+      int res = c >= 0 ? candidate_hop1(s, c) : 0;
+      unsigned long long pending = __ballot(res != 0);
+      while (pending) {
+        int j = __ffsll((long long)pending) - 1;
+        pending &= pending - 1;
+        int rj = __shfl(res, j);
+        int32_t cj = __shfl(c, j);
+        if (rj == 3) rj = try_endpoint(s, cj);
+        if (rj == -1 || rj == 1) {
+          if (lane_id() == 0) atomicMin(best, hop2_key(yi, g0 + j, rj == 1 ? 1 : 0));
+          done = true;
+          break;
+        }
+      }
+    }
+  }
+  if (lane_id() == 0 && scanned) atomicAdd((unsigned long long*)&sv.t[7], (unsigned long long)scanned);
+}
+
+// Hop-2 resume: wave per request. Pass A: the accepted candidate becomes w1 and is marked.
+// Pass B: a conflict before any acceptance ends the segment's prefix there. A flagged request
+// is searched serially.
+__global__ void k_mm_hop2_resume(MisDev d, SegView sv, int pass) {
+  int64_t n = min(sv.t[pass == 0 ? 13 : 14], d.h2_rcap);
+  int32_t ep = (int32_t)sv.t[2];
+  int64_t scanned = 0;
+  for (int64_t i = global_wave(); i < n; i += total_waves()) {
+    int64_t k = d.h2_line[i];
+    int32_t sg = d.h2_seg[i];
+    int32_t r = line_r(d, k);
+    bool conflict = false;
+    int32_t w1 = -1;
+    if (d.h2_flag[i]) {
+      MisView s = view_of(d, ep, k, pass == 1, &scanned);
+      int32_t w0;
+      bool hop2 = false;
+      conflict = !eval_line(s, d.uv[2 * k], d.uv[2 * k + 1], d.ins[k] != 0, &w0, &w1, false, &hop2);
+    } else {
+      unsigned long long key = d.h2_best[i];
+      if (key != kNoResult) {
+        if ((key & 3ULL) == 0) {
+          conflict = true;
+        } else {
+          int64_t yi = (int64_t)(key >> 33), pos = (int64_t)((key >> 2) & 0x7fffffffULL);
+          int32_t y = row_at(d.g, row_of(d.g, r), yi);
+          w1 = row_at(d.g, row_of(d.g, y), pos);
+        }
+      }
+    }
+    if (pass == 0) {
+      if (lane_id() == 0) d.w1s[k] = w1;
+      mark_write(d, w1, k);
+    } else if (conflict && lane_id() == 0) {
+      atomicMin((long long*)&sv.k0()[sg], (long long)k);
+    }
   }
   if (lane_id() == 0 && scanned) atomicAdd((unsigned long long*)&sv.t[7], (unsigned long long)scanned);
 }
@@ -569,6 +796,8 @@ __global__ void k_mm_round_end(SegView sv) {
   if (threadIdx.x == 0) {
     active_s = pending_s = 0;
     t[10] = 0;  // this round's tags are set
+    t[13] = 0;  // hop-2 requests of passes A and B
+    t[14] = 0;
   }
   __syncthreads();
   for (int64_t s = threadIdx.x; s < S; s += blockDim.x) {
@@ -641,6 +870,10 @@ struct MisState {
   DevBuf<int32_t> mcount, minwn;
   int64_t mc_version = -1, mc_apply = -1, mc_gen = -1;  // what mcount was computed for
   DevBuf<int64_t> phase;  // phase table
+  DevBuf<int32_t> h2_line, h2_seg, h2_flag;
+  DevBuf<unsigned long long> h2_best;
+  DevBuf<int64_t> h2_off;
+  DevBuf<uint64_t> h2_task;
   int64_t pcap = 0, scap = 0, tcap = 0;
   hipGraphExec_t exec = nullptr;
   std::vector<const void*> key;
@@ -659,6 +892,12 @@ void mis_release(Ctx& c) {
   ms.mcount.release();
   ms.minwn.release();
   ms.phase.release();
+  ms.h2_line.release();
+  ms.h2_seg.release();
+  ms.h2_flag.release();
+  ms.h2_best.release();
+  ms.h2_off.release();
+  ms.h2_task.release();
   if (ms.exec) (void)hipGraphExecDestroy(ms.exec);
   delete c.mis_state;
   c.mis_state = nullptr;
@@ -694,15 +933,23 @@ void mis_cuts(const std::vector<int32_t>& uv, const std::vector<int64_t>& deg, i
 
 MisDev dev_of(Ctx& c, MisState& ms) {
   return MisDev{c.g.view(), c.g.sorted, ms.uv2.p, ms.ins2.p, c.mis, c.mis_tag, ms.mcount.p, c.mis_minw, ms.minwn.p,
-                ms.w0.p, ms.w1.p};
+                ms.w0.p, ms.w1.p, ms.h2_line.p, ms.h2_seg.p, ms.h2_best.p, ms.h2_off.p, ms.h2_flag.p, ms.h2_task.p,
+                kHop2Requests, kHop2Tasks};
 }
 
 void mm_launch_round(Ctx& c, MisState& ms, SegView sv) {
   MisDev d = dev_of(c, ms);
+  // tags, pass A, its hop-2 searches, pass B, its hop-2 searches, commit, round end
   k_mis_tag_endpoints<<<kListGrid, kBlock, 0, c.s>>>(ms.uv2.p, sv.t, sv.ph, sv.pcap, c.mis_tag);
   k_mis_tag_hop1<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), ms.uv2.p, sv.t, sv.ph, sv.pcap, sv.scap, c.mis_tag);
   k_mm_eval<<<ms.grid, kBlock, 0, c.s>>>(d, sv);
-  k_mm_check<<<ms.grid, kBlock, 0, c.s>>>(d, sv);
+  for (int pass = 0; pass < 2; pass++) {
+    if (pass == 1) k_mm_check<<<ms.grid, kBlock, 0, c.s>>>(d, sv);
+    k_mm_hop2_plan<<<1, kBlock, 0, c.s>>>(d, sv, pass);
+    k_mm_hop2_expand<<<ms.grid, kBlock, 0, c.s>>>(d, sv, pass);
+    k_mm_hop2_scan<<<ms.grid, kBlock, 0, c.s>>>(d, sv, pass);
+    k_mm_hop2_resume<<<ms.grid, kBlock, 0, c.s>>>(d, sv, pass);
+  }
   k_mm_commit<<<ms.grid, kBlock, 0, c.s>>>(d, sv);
   k_mm_round_end<<<1, kBlock, 0, c.s>>>(sv);
 }
@@ -715,7 +962,8 @@ void mm_launch_group(Ctx& c, MisState& ms, SegView sv) {
 int mm_graph(Ctx& c, MisState& ms, SegView sv) {
   GView g = c.g.view();
   std::vector<const void*> key = {ms.uv2.p, ms.ins2.p, sv.t, ms.w0.p, ms.w1.p, c.mis, c.mis_tag, c.mis_minw,
-                                  ms.mcount.p, ms.minwn.p, (const void*)(intptr_t)c.g.sorted, sv.ph,
+                                  ms.mcount.p, ms.minwn.p, (const void*)(intptr_t)c.g.sorted, sv.ph, ms.h2_line.p,
+                                  ms.h2_seg.p, ms.h2_flag.p, ms.h2_best.p, ms.h2_off.p, ms.h2_task.p,
                                   (const void*)(intptr_t)sv.pcap, (const void*)(intptr_t)sv.scap,
                                   (const void*)(intptr_t)ms.tcap,
                                   g.boff, g.bcol, g.aoff, g.alen, g.acol, (const void*)(intptr_t)g.n,
@@ -765,6 +1013,12 @@ int mis_repair_multi(Ctx& c, Channel& ch, int32_t ninst, const int64_t* off, int
   SCOUT_CHECK(ms.uv2.reserve(2 * m));
   SCOUT_CHECK(ms.ins2.reserve(m));
   SCOUT_CHECK(ms.perm.reserve(m));
+  SCOUT_CHECK(ms.h2_line.reserve(kHop2Requests));
+  SCOUT_CHECK(ms.h2_seg.reserve(kHop2Requests));
+  SCOUT_CHECK(ms.h2_flag.reserve(kHop2Requests));
+  SCOUT_CHECK(ms.h2_best.reserve(kHop2Requests));
+  SCOUT_CHECK(ms.h2_off.reserve(kHop2Requests + 1));
+  SCOUT_CHECK(ms.h2_task.reserve(kHop2Tasks));
   int64_t stride = std::max<int64_t>(ninst, 64);
   SCOUT_CHECK(ms.seg.reserve(seg_table_size(stride)));
   // the table stride is part of the captured graph: keep the largest seen
@@ -921,6 +1175,7 @@ int mis_repair_multi(Ctx& c, Channel& ch, int32_t ninst, const int64_t* off, int
     if (c.h_box[8] >= P) break;
   }
   int64_t rounds = c.h_box[4], writes = c.h_box[5], evals = c.h_box[6] + slots0, entries = c.h_box[7];
+  st->seeds += c.h_box[16];  // insertions whose candidate search reached the hop-2 rows
   // per line evaluation: the pair, label, row offsets of both endpoints, memberships and the
   // write slots (~48 B); per row entry read: the entry and the membership byte (5 B)
   SCOUT_CHECK(prof_end(c, PROF_MIS_EVAL, 48 * evals + 5 * entries));

@@ -213,3 +213,37 @@ def test_autotune_keeps_colours_and_picks_a_setting(tmp_path):
         assert np.array_equal(eng.download(sa.STATE_COLOR), before)
         eng.repair_color()
         assert np.array_equal(eng.download(sa.STATE_COLOR), want)
+
+
+@pytest.mark.parametrize("max_csr", [300000, 10 ** 9])
+def test_mis_hop2_searches_equal_oracle(max_csr):
+    """Degree-biased toggles on an R-MAT graph make many insertions between two members whose
+    candidates in [r] + N(r) are all rejected: their hop-2 searches run as chunk tasks over
+    many waves. MIS and colours equal the oracle after every batch, and the hop-2 path ran."""
+    say(f"mis hop-2: R-MAT scale 12, degree-biased stream, max_csr {max_csr}")
+    with sa.Engine(0) as eng:
+        eng.gen_rmat(12, 16, seed=4)
+        g = eng.download_graph()
+        og = OracleGraph(n=g.n, off=g.off, col=g.col)
+        eng.mis_greedy()
+        eng.color_greedy()
+        mis, col = og.greedy_mis(), og.greedy_color()
+        assert np.array_equal(eng.download(sa.STATE_MIS), mis)
+        eng.gen_mode(sa.GEN_DEGREE)
+        hop2 = 0
+        for b in range(6):
+            m = eng.gen_batch(3000, 0.5, 0.0, seed=7, expected_total=18000)
+            uv = eng.batch_download(m)
+            eng.classify()
+            eng.apply()
+            st = eng.repair_mis(max_csr)
+            eng.repair_color()
+            hop2 += st.seeds
+            ins = og.classify(uv)
+            og.apply(uv, ins)
+            mis, _ = og.mis_batch(uv, ins, mis, max_csr)
+            col, _ = og.color_batch(uv, ins, col)
+            assert np.array_equal(eng.download(sa.STATE_MIS), mis), f"batch {b}: MIS differs"
+            assert np.array_equal(eng.download(sa.STATE_COLOR), col), f"batch {b}: colours differ"
+        say(f"mis hop-2: {hop2} hop-2 searches")
+        assert hop2 > 0
