@@ -322,7 +322,11 @@ __global__ void k_mis_tag_hop1(GView g, const int32_t* uv, const int64_t* t, con
     int32_t x = uv[task >> 20];
     Row r = row_of(g, x);
     int64_t lo = (task & 0xfffff) * kTagChunk, hi = min(lo + kTagChunk, row_len(r));
-    wave_walk_wide<8>(g, r, lo, hi, [&](int32_t y) { atomicMax(&tag[y], 2 * ep); });
+    // endpoint tags (2 ep + 1) were set by the previous kernel; every writer here stores the
+    // same value, so a plain store after the check is enough
+    wave_walk_wide<8>(g, r, lo, hi, [&](int32_t y) {
+      if (tag[y] < 2 * ep) tag[y] = 2 * ep;
+    });
   }
 }
 
