@@ -154,6 +154,10 @@ int scout_repair_mis_multi(scout_ctx* ctx, int channel, int32_t ninst, const int
 /* Launch settings of the multi-instance MIS rounds: grid blocks, rounds per replay of the
  * captured hipGraph, graph on/off. No counterpart in the reference. */
 int scout_mis_config(scout_ctx* ctx, int grid, int group_rounds, int use_graph);
+/* Capacities of the MIS hop-2 search lists per pass: deferred searches (at most 65536) and
+ * chunk tasks. A search that does not fit runs serially in one wave; results do not depend on
+ * them. No counterpart in the reference. */
+int scout_mis_limits(scout_ctx* ctx, int64_t hop2_requests, int64_t hop2_tasks);
 /* Colouring repair settings: rows of at most thread_max entries are handled per thread, rows
  * longer than block_min per workgroup, the rest per wavefront; grid = blocks of the round
  * kernels; use_graph = replay captured round pairs. The reference fixes one thread per vertex

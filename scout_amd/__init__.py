@@ -67,6 +67,7 @@ _SIGNATURES = {
     "scout_repair_mis": [_P, ctypes.c_int, _I64, ctypes.POINTER(Stats)],
     "scout_repair_mis_multi": [_P, ctypes.c_int, _I32, _P, _I64, ctypes.POINTER(Stats)],
     "scout_mis_config": [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int],
+    "scout_mis_limits": [_P, _I64, _I64],
     "scout_color_config": [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int],
     "scout_color_get_config": [_P, _P],
     "scout_color_autotune": [_P, ctypes.c_int, ctypes.POINTER(ctypes.c_double)],
@@ -293,6 +294,9 @@ class Engine:
 
     def mis_config(self, grid: int = 1024, group_rounds: int = 4, use_graph: bool = True):
         self._check(self.lib.scout_mis_config(self.h, grid, group_rounds, 1 if use_graph else 0), "mis_config")
+
+    def mis_limits(self, hop2_requests: int = 65536, hop2_tasks: int = 1 << 22):
+        self._check(self.lib.scout_mis_limits(self.h, hop2_requests, hop2_tasks), "mis_limits")
 
     def color_config(self, thread_max: int = 8, block_min: int = 4096, grid: int = 1024, use_graph: bool = True):
         self._check(self.lib.scout_color_config(self.h, thread_max, block_min, grid, 1 if use_graph else 0),

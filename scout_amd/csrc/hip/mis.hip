@@ -878,6 +878,7 @@ struct MisState {
   DevBuf<unsigned long long> h2_best;
   DevBuf<int64_t> h2_off;
   DevBuf<uint64_t> h2_task;
+  int64_t h2_rcap = kHop2Requests, h2_tcap = kHop2Tasks;  // list capacities (fallback: serial)
   int64_t pcap = 0, scap = 0, tcap = 0;
   hipGraphExec_t exec = nullptr;
   std::vector<const void*> key;
@@ -938,7 +939,7 @@ void mis_cuts(const std::vector<int32_t>& uv, const std::vector<int64_t>& deg, i
 MisDev dev_of(Ctx& c, MisState& ms) {
   return MisDev{c.g.view(), c.g.sorted, ms.uv2.p, ms.ins2.p, c.mis, c.mis_tag, ms.mcount.p, c.mis_minw, ms.minwn.p,
                 ms.w0.p, ms.w1.p, ms.h2_line.p, ms.h2_seg.p, ms.h2_best.p, ms.h2_off.p, ms.h2_flag.p, ms.h2_task.p,
-                kHop2Requests, kHop2Tasks};
+                ms.h2_rcap, ms.h2_tcap};
 }
 
 void mm_launch_round(Ctx& c, MisState& ms, SegView sv) {
@@ -968,6 +969,7 @@ int mm_graph(Ctx& c, MisState& ms, SegView sv) {
   std::vector<const void*> key = {ms.uv2.p, ms.ins2.p, sv.t, ms.w0.p, ms.w1.p, c.mis, c.mis_tag, c.mis_minw,
                                   ms.mcount.p, ms.minwn.p, (const void*)(intptr_t)c.g.sorted, sv.ph, ms.h2_line.p,
                                   ms.h2_seg.p, ms.h2_flag.p, ms.h2_best.p, ms.h2_off.p, ms.h2_task.p,
+                                  (const void*)(intptr_t)ms.h2_rcap, (const void*)(intptr_t)ms.h2_tcap,
                                   (const void*)(intptr_t)sv.pcap, (const void*)(intptr_t)sv.scap,
                                   (const void*)(intptr_t)ms.tcap,
                                   g.boff, g.bcol, g.aoff, g.alen, g.acol, (const void*)(intptr_t)g.n,
@@ -1017,12 +1019,12 @@ int mis_repair_multi(Ctx& c, Channel& ch, int32_t ninst, const int64_t* off, int
   SCOUT_CHECK(ms.uv2.reserve(2 * m));
   SCOUT_CHECK(ms.ins2.reserve(m));
   SCOUT_CHECK(ms.perm.reserve(m));
-  SCOUT_CHECK(ms.h2_line.reserve(kHop2Requests));
-  SCOUT_CHECK(ms.h2_seg.reserve(kHop2Requests));
-  SCOUT_CHECK(ms.h2_flag.reserve(kHop2Requests));
-  SCOUT_CHECK(ms.h2_best.reserve(kHop2Requests));
-  SCOUT_CHECK(ms.h2_off.reserve(kHop2Requests + 1));
-  SCOUT_CHECK(ms.h2_task.reserve(kHop2Tasks));
+  SCOUT_CHECK(ms.h2_line.reserve(ms.h2_rcap));
+  SCOUT_CHECK(ms.h2_seg.reserve(ms.h2_rcap));
+  SCOUT_CHECK(ms.h2_flag.reserve(ms.h2_rcap));
+  SCOUT_CHECK(ms.h2_best.reserve(ms.h2_rcap));
+  SCOUT_CHECK(ms.h2_off.reserve(ms.h2_rcap + 1));
+  SCOUT_CHECK(ms.h2_task.reserve(ms.h2_tcap));
   int64_t stride = std::max<int64_t>(ninst, 64);
   SCOUT_CHECK(ms.seg.reserve(seg_table_size(stride)));
   // the table stride is part of the captured graph: keep the largest seen
@@ -1259,5 +1261,17 @@ extern "C" int scout_mis_config(scout_ctx* x, int grid, int group_rounds, int us
   c.mis_state->grid = (unsigned)grid;
   c.mis_state->group_rounds = group_rounds;
   c.mis_state->use_graph = use_graph != 0;
+  return SCOUT_OK;
+}
+
+extern "C" int scout_mis_limits(scout_ctx* x, int64_t hop2_requests, int64_t hop2_tasks) {
+  if (!x || hop2_requests < 1 || hop2_requests > kHop2Requests || hop2_tasks < 1 || hop2_tasks > (int64_t)1 << 30) {
+    set_error("mis_limits: need 1 <= hop2_requests <= 65536 and 1 <= hop2_tasks <= 2^30");
+    return SCOUT_ERR_ARG;
+  }
+  Ctx& c = x->c;
+  if (!c.mis_state) c.mis_state = new MisState();
+  c.mis_state->h2_rcap = hop2_requests;
+  c.mis_state->h2_tcap = hop2_tasks;
   return SCOUT_OK;
 }

@@ -215,13 +215,17 @@ def test_autotune_keeps_colours_and_picks_a_setting(tmp_path):
         assert np.array_equal(eng.download(sa.STATE_COLOR), want)
 
 
-@pytest.mark.parametrize("max_csr", [300000, 10 ** 9])
-def test_mis_hop2_searches_equal_oracle(max_csr):
+@pytest.mark.parametrize("max_csr,limits", [(300000, None), (10 ** 9, None), (10 ** 9, (2, 16))])
+def test_mis_hop2_searches_equal_oracle(max_csr, limits):
     """Degree-biased toggles on an R-MAT graph make many insertions between two members whose
     candidates in [r] + N(r) are all rejected: their hop-2 searches run as chunk tasks over
-    many waves. MIS and colours equal the oracle after every batch, and the hop-2 path ran."""
-    say(f"mis hop-2: R-MAT scale 12, degree-biased stream, max_csr {max_csr}")
+    many waves, or serially when the request or task list is full. MIS and colours equal the
+    oracle after every batch, and the hop-2 path ran."""
+    say(f"mis hop-2: R-MAT scale 12, degree-biased stream, max_csr {max_csr}, list limits {limits}")
     with sa.Engine(0) as eng:
+        if limits:
+            # tiny request and task lists: most searches take the serial fallback
+            eng.mis_limits(*limits)
         eng.gen_rmat(12, 16, seed=4)
         g = eng.download_graph()
         og = OracleGraph(n=g.n, off=g.off, col=g.col)
