@@ -361,3 +361,36 @@ def test_binaries_on_sample_dump_exact_states(tmp_path, data, prog, init_name):
             state, _ = og.mis_batch(uv, ins, state, mis_max_csr(8.0, 288e9))
         got = np.fromfile(dump / f"{prog}_{k}.bin", dtype=np.uint8 if prog == "mis" else np.int32)
         assert np.array_equal(got, state), f"scout_{prog} batch {k}"
+
+
+def test_mis_er_100k_full_size_exact_every_batch(tmp_path):
+    """The MIS configuration at its full size: ER n=100,000 with 800,000 edges (seed 0),
+    greedy-ascending initial MIS, 10 batches of 10,000 toggles (half insertions of non-edges,
+    half deletions of edges). Every batch's MIS equals the oracle's and is independent; the
+    maximality violations (which the reference's rules allow) are reported."""
+    say("[mis] ER 100k / 800k edges, 10 x 10k toggles")
+    egr = tmp_path / "er100k.egr"
+    run_gen("er", 100000, 800000, 0, egr)
+    run_gen("stream", "mixed", egr, tmp_path / "b", 10, 10000, 1)
+    eng, g = engine_for(str(egr))
+    og = OracleGraph(str(egr))
+    base = OracleGraph(str(egr))
+    eng.mis_greedy()
+    mis = og.greedy_mis()
+    assert np.array_equal(eng.download(sa.STATE_MIS), mis)
+    max_csr = mis_max_csr(g.off[-1] / g.n, 288e9)
+    for k, uv in batches(str(tmp_path / "b"), 10, g.n):
+        eng.stage(uv)
+        ins = eng.classify(sa.CLASSIFY_BASE)
+        assert np.array_equal(ins, base.classify(uv))
+        eng.apply(True)
+        og.apply(uv, ins, conditional=True)
+        st = eng.repair_mis(max_csr)
+        mis, ost = og.mis_batch(uv, ins, mis, max_csr)
+        got = eng.download(sa.STATE_MIS)
+        assert np.array_equal(got, mis), f"batch {k}: {np.count_nonzero(got != mis)} memberships differ"
+        assert st.sub_batches == ost[0]
+        assert og.check_mis_independent(got) == 0
+        say(f"[mis] batch {k}: {int(got.sum())} members, {og.mis_maximality_violations(got)} maximality "
+            f"violations, {st.rounds} rounds, {st.repair_ms:.2f} ms")
+    eng.close()
