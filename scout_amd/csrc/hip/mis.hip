@@ -28,6 +28,19 @@
 // independent instances, which share no vertex) run their rounds in the same launches, each
 // with its own window and first conflict. A single graph is the case of one instance.
 //
+// Counts instead of scans: mcount[x] = members among x's neighbours, kept exact across
+// batches. "N(x) holds a member other than skip" is read from it in O(1) (less skip and the
+// dropped vertex when they are member neighbours). Such a read depends on every neighbour's
+// membership, so pass B treats it as a conflict when minwn[x] (the smallest pending writer of
+// a neighbour of x) is earlier. A 2-hop ball then costs one pass over N(x).
+//
+// Hop-2 candidates N(y), y in N(r), are searched by chunk tasks over many waves; the first
+// decisive candidate in sequence order is the smallest (y position, entry position) key.
+//
+// Phases: the k-th sub-batches of all instances form phase k. All phases of a repair are
+// listed in one device table; the round-end kernel loads the next phase itself, so one
+// captured graph of rounds runs across phase boundaries.
+//
 // The candidate order of row(r) is reproduced without building the ball: the sequence
 // [r] + N(r) + N(y1) + N(y2) + ... (y_i in N(r) order) lists the ball's vertices in the same
 // first-occurrence order, with repeats; a repeated candidate was already rejected at its first
