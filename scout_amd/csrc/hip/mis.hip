@@ -183,8 +183,11 @@ __device__ __forceinline__ int row_kind(const MisView& s, int32_t x) {
 //
 // A candidate without a row is rejected whatever its membership, so its membership is not
 // read (and cannot conflict): the row kind (static within a sub-batch) is tested first.
-__device__ __forceinline__ int candidate_hop1(const MisView& s, int32_t c) {
+// near_ep: c is known to neighbour an endpoint (c in N(r)), so it has a row even when the
+// phase's hop-1 tags are not set.
+__device__ __forceinline__ int candidate_hop1(const MisView& s, int32_t c, bool near_ep) {
   int kind = row_kind(s, c);
+  if (kind == 0 && near_ep) kind = 1;
   if (kind == 0) return 0;
   bool conf = false;
   bool m = member(s, c, &conf);
@@ -209,8 +212,8 @@ __device__ int try_endpoint(const MisView& s, int32_t c) {
 
 // Candidates of row r in order, 256 at a time (one lane each for four chunks of 64, taken
 // chunk by chunk in order; endpoints go through the wave when their turn comes). Returns 1
-// with *accepted, 0 if none is accepted, -1 on conflict.
-__device__ int first_candidate(const MisView& s, const Row& r, int32_t* accepted) {
+// with *accepted, 0 if none is accepted, -1 on conflict. near_ep: r is the row of an endpoint.
+__device__ int first_candidate(const MisView& s, const Row& r, int32_t* accepted, bool near_ep) {
   constexpr int kU = 4;
   int64_t len = row_len(r);
   *s.scanned += len;
@@ -223,7 +226,7 @@ __device__ int first_candidate(const MisView& s, const Row& r, int32_t* accepted
       cs[u] = k < len ? row_at(s.g, r, k) : -1;
     }
 #pragma unroll
-    for (int u = 0; u < kU; u++) res[u] = cs[u] >= 0 ? candidate_hop1(s, cs[u]) : 0;
+    for (int u = 0; u < kU; u++) res[u] = cs[u] >= 0 ? candidate_hop1(s, cs[u], near_ep) : 0;
     for (int u = 0; u < kU; u++) {
       unsigned long long pending = __ballot(res[u] != 0);
       while (pending) {
@@ -272,7 +275,7 @@ __device__ bool eval_line(MisView s, int32_t u, int32_t v, bool ins, int32_t* w0
     }
     Row rr = row_of(s.g, r);
     int32_t acc = -1;
-    res = first_candidate(s, rr, &acc);
+    res = first_candidate(s, rr, &acc, true);
     if (res < 0) return false;
     if (res > 0) {
       *w1 = acc;
@@ -286,7 +289,7 @@ __device__ bool eval_line(MisView s, int32_t u, int32_t v, bool ins, int32_t* w0
     for (int64_t k = 0; k < len; k++) {
       int32_t y = row_at(s.g, rr, k);
       if (y < 0) continue;
-      res = first_candidate(s, row_of(s.g, y), &acc);
+      res = first_candidate(s, row_of(s.g, y), &acc, false);
       if (res < 0) return false;
       if (res > 0) {
         *w1 = acc;
@@ -308,7 +311,7 @@ __device__ bool eval_line(MisView s, int32_t u, int32_t v, bool ins, int32_t* w0
 
 constexpr int64_t kTagChunk = 4096;  // row entries per hop-1 tag task
 
-// Endpoint and hop-1 tags of the phase that starts this round (nothing otherwise).
+// Endpoint tags of the phase that starts this round (nothing otherwise).
 __global__ void k_mis_tag_endpoints(const int32_t* uv, const int64_t* t, const int64_t* ph, int64_t pcap,
                                     int32_t* tag) {
   if (!t[10]) return;
@@ -322,10 +325,11 @@ __global__ void k_mis_tag_endpoints(const int32_t* uv, const int64_t* t, const i
 }
 
 // Hop-1 tags: wave per tag task (a chunk of an endpoint's row), so hub rows spread over
-// many waves.
+// many waves. Only hop-2 candidates need them (a candidate in N(r) neighbours the endpoint
+// r), so they are set on demand: in the round after the phase's first hop-2 search.
 __global__ void k_mis_tag_hop1(GView g, const int32_t* uv, const int64_t* t, const int64_t* ph, int64_t pcap,
                                int64_t scap, int32_t* tag) {
-  if (!t[10]) return;
+  if (t[18] != 1) return;
   int64_t f = t[8];
   const int64_t* task0 = ph + 3 * pcap + 1 + 2 * scap;
   const int64_t* tasks = ph + 4 * pcap + 2 + 2 * scap;
@@ -374,8 +378,9 @@ __global__ void k_mis_degrees(GView g, const int32_t* uv, int64_t m, int64_t* de
 // rounds, [5] writes, [6] line evaluations, [7] row entries read by passes A and B, [8]
 // current phase, [9] phases, [10] 1 = the phase starts this round (tags to set), [11] first
 // tag epoch, [12] initial window, [13]/[14] hop-2 searches requested in passes A/B this round,
-// [15] hop-2 chunk tasks of the current pass, [16] hop-2 searches of pass A over all rounds;
-// then per segment: pending start p, end hi, first conflict
+// [15] hop-2 chunk tasks of the current pass, [16] hop-2 searches of pass A over all rounds,
+// [17] 1 = a line of this round needs the phase's hop-1 tags, [18] hop-1 tags of the phase:
+// 0 = not set, 1 = set at the start of this round, 2 = set, [19] phases tagged; then per segment: pending start p, end hi, first conflict
 // k0, window size win, and woff (exclusive prefix of the current window sizes, S + 1).
 // Phase table (int64): lines lo[P] and hi[P] of each phase, first segment seg0[P + 1], the
 // segments' line ranges slo[NS] and shi[NS], first tag task task0[P + 1] and the tasks
@@ -494,9 +499,13 @@ __device__ __forceinline__ bool hop2_request(const MisDev& d, int64_t* count, in
 
 // Pass A: wave per window line. Writes go to w0/w1; minw[w] = smallest writer of w and
 // minwn[z] = smallest writer of a neighbour of z. A deferred hop-2 search leaves w1 pending.
+// Before the phase's hop-1 tags are set, a line that needs a hop-2 search ends its segment's
+// prefix (it writes nothing, and lines after it do not commit) and asks for the tags; it is
+// evaluated again in the next round, when they are set.
 __global__ void k_mm_eval(MisDev d, SegView sv) {
   int64_t slots = sv.t[1];
   int32_t ep = (int32_t)sv.t[2];
+  bool tagged = sv.t[18] != 0;
   int64_t scanned = 0;
   for (int64_t t = global_wave(); t < slots; t += total_waves()) {
     int32_t sg;
@@ -505,7 +514,13 @@ __global__ void k_mm_eval(MisDev d, SegView sv) {
     int32_t w0, w1;
     bool hop2 = false;
     eval_line(s, d.uv[2 * k], d.uv[2 * k + 1], d.ins[k] != 0, &w0, &w1, true, &hop2);
-    if (hop2) {
+    if (hop2 && !tagged) {
+      w0 = w1 = -1;
+      if (lane_id() == 0) {
+        atomicMin((long long*)&sv.k0()[sg], (long long)k);
+        sv.t[17] = 1;
+      }
+    } else if (hop2) {
       if (hop2_request(d, &sv.t[13], k, sg)) w1 = kHop2Pending;
       else eval_line(s, d.uv[2 * k], d.uv[2 * k + 1], d.ins[k] != 0, &w0, &w1, false, &hop2);
     }
@@ -534,7 +549,10 @@ __global__ void k_mm_check(MisDev d, SegView sv) {
     int32_t w0, w1;
     bool hop2 = false;
     bool ok = eval_line(s, d.uv[2 * k], d.uv[2 * k + 1], d.ins[k] != 0, &w0, &w1, true, &hop2);
-    if (ok && hop2 && !hop2_request(d, &sv.t[14], k, sg))
+    // without tags pass A has ended the prefix at the first line that reaches hop 2 (this
+    // evaluation reads the same state, so it would reach hop 2 only past that line)
+    if (ok && hop2 && sv.t[18] == 0) ok = false;
+    else if (ok && hop2 && !hop2_request(d, &sv.t[14], k, sg))
       ok = eval_line(s, d.uv[2 * k], d.uv[2 * k + 1], d.ins[k] != 0, &w0, &w1, false, &hop2);
     if (!ok && lane_id() == 0) atomicMin((long long*)&k0[sg], (long long)k);
   }
@@ -652,7 +670,7 @@ __global__ void k_mm_hop2_scan(MisDev d, SegView sv, int pass) {
       if (__shfl(__hip_atomic_load(best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), 0) <= hop2_key(yi, g0, 0)) break;
       int64_t q = g0 + lane_id();
       int32_t c = q < c1 ? row_at(d.g, ry, q) : -1;
-      int res = c >= 0 ? candidate_hop1(s, c) : 0;
+      int res = c >= 0 ? candidate_hop1(s, c, false) : 0;
       unsigned long long pending = __ballot(res != 0);
       while (pending) {
         int j = __ffsll((long long)pending) - 1;
@@ -803,7 +821,8 @@ __device__ int64_t block_woff(int64_t S, int64_t* woff, SegView sv, int64_t* sca
 
 // One block: advances every segment (p = k0) and resizes its window (k0 = window end); when
 // every segment of the phase is done, loads the next phase's segments and flags its start.
-// Rebuilds woff and the slot count; counts non-empty rounds.
+// Schedules the phase's hop-1 tags for the next round when a line asked for them. Rebuilds
+// woff and the slot count; counts non-empty rounds.
 __global__ void k_mm_round_end(SegView sv) {
   __shared__ int64_t scan[kBlock];
   __shared__ int64_t carry_s, active_s, pending_s;
@@ -815,6 +834,13 @@ __global__ void k_mm_round_end(SegView sv) {
     t[10] = 0;  // this round's tags are set
     t[13] = 0;  // hop-2 requests of passes A and B
     t[14] = 0;
+    if (t[17]) {
+      t[18] = 1;  // tagged at the start of the next round, before any line reads the tags
+      t[19] += 1;
+    } else if (t[18] == 1) {
+      t[18] = 2;
+    }
+    t[17] = 0;
   }
   __syncthreads();
   for (int64_t s = threadIdx.x; s < S; s += blockDim.x) {
@@ -852,6 +878,7 @@ __global__ void k_mm_round_end(SegView sv) {
         t[2] = t[11] + f;
         t[3] = S;
         t[10] = 1;
+        t[18] = 0;
       }
     } else {
       S = 0;
@@ -1195,6 +1222,7 @@ int mis_repair_multi(Ctx& c, Channel& ch, int32_t ninst, const int64_t* off, int
   }
   int64_t rounds = c.h_box[4], writes = c.h_box[5], evals = c.h_box[6] + slots0, entries = c.h_box[7];
   st->seeds += c.h_box[16];  // insertions whose candidate search reached the hop-2 rows
+  st->splits += c.h_box[19];  // phases whose hop-1 rows were tagged
   // per line evaluation: the pair, label, row offsets of both endpoints, memberships and the
   // write slots (~48 B); per row entry read: the entry and the membership byte (5 B)
   SCOUT_CHECK(prof_end(c, PROF_MIS_EVAL, 48 * evals + 5 * entries));
