@@ -259,7 +259,7 @@ def run_color_mis(args):
             eng.prof_enable(True)
         lat, lines = [], 0
         work = {"color_rounds": 0, "color_recoloured": 0, "mis_rounds": 0, "mis_sub_batches": 0,
-                "mis_line_evaluations": 0, "mis_hop2_searches": 0, "mis_tagged_sub_batches": 0, "readbacks": 0}
+                "mis_line_evaluations": 0, "mis_hop2_searches": 0, "readbacks": 0}
         t_start = time.perf_counter()
         for b in range(args.warmup, total):
             ts = time.perf_counter()
@@ -277,11 +277,10 @@ def run_color_mis(args):
             work["mis_sub_batches"] += sm.sub_batches
             work["mis_line_evaluations"] += sm.touched
             work["mis_hop2_searches"] += sm.seeds
-            work["mis_tagged_sub_batches"] += sm.splits
             work["readbacks"] += sc.readbacks + sm.readbacks
             log(f"batch {b}: {lat[-1]:.1f} ms (colour {sc.repair_ms:.1f} ms, {sc.rounds} rounds; MIS "
                 f"{sm.repair_ms:.1f} ms, {sm.rounds} rounds, {sm.sub_batches} sub-batches, "
-                f"{sm.splits} with hop-2 searches)")
+                f"{sm.seeds} hop-2 searches)")
         elapsed = time.perf_counter() - t_start
         prof = None
         if profile:

@@ -68,8 +68,7 @@ typedef struct {
   int64_t touched;      /* vertices whose rows were scanned */
   int64_t rounds;       /* device rounds */
   int64_t fallback;     /* 1 if the batch was settled by a full recompute */
-  int64_t splits;       /* CC: new components created by deletions; mis: sub-batches whose
-                           hop-1 rows were tagged (those with a hop-2 search) */
+  int64_t splits;       /* CC: new components created by deletions */
   int64_t conflicts;    /* coloring: insertions with equal colours; mis: edits that wrote */
   int64_t sub_batches;  /* mis: sub-batches */
   int64_t edges_scanned;/* adjacency entries read by the repair kernels */

@@ -49,7 +49,7 @@ int alloc_n(Ctx& c, void** p, size_t elem) {
 
 void free_state(Ctx& c) {
   void* ps[] = {c.level, c.parent, c.label, c.color, c.mis, c.stamp, c.aux, c.aux2,
-                c.cc_owner, c.col_marker, c.mis_minw, c.mis_tag, c.cc_level, c.cc_parent};
+                c.cc_owner, c.col_marker, c.mis_minw, c.cc_level, c.cc_parent};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   c.cc_level = c.cc_parent = c.parent = nullptr;
@@ -58,7 +58,7 @@ void free_state(Ctx& c) {
   c.level = c.label = c.color = nullptr;
   c.mis = nullptr;
   c.stamp = c.aux = c.aux2 = nullptr;
-  c.cc_owner = c.col_marker = c.mis_minw = c.mis_tag = nullptr;
+  c.cc_owner = c.col_marker = c.mis_minw = nullptr;
 }
 
 int check_ctx(scout_ctx* x) {

@@ -350,14 +350,14 @@ int scout_gen_rmat(scout_ctx* x, int scale, int edge_factor, double a, double b,
   // fresh per-vertex scratch for the new vertex count
   int32_t n = ctx.g.n;
   void* ps[] = {ctx.level, ctx.parent, ctx.label, ctx.color, ctx.mis, ctx.stamp, ctx.aux, ctx.aux2,
-                ctx.cc_owner, ctx.col_marker, ctx.mis_minw, ctx.mis_tag, ctx.cc_level, ctx.cc_parent};
+                ctx.cc_owner, ctx.col_marker, ctx.mis_minw, ctx.cc_level, ctx.cc_parent};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   ctx.cc_level = ctx.cc_parent = ctx.parent = nullptr;
   ctx.level_seq = ctx.cc_level_seq = ctx.cc_components = ctx.label_seq = -1;
   ctx.level = ctx.label = ctx.color = nullptr;
   ctx.mis = nullptr;
-  ctx.cc_owner = ctx.col_marker = ctx.mis_minw = ctx.mis_tag = nullptr;
+  ctx.cc_owner = ctx.col_marker = ctx.mis_minw = nullptr;
   cc_release(ctx);
   mis_release(ctx);
   SCOUT_TRY(hipMalloc((void**)&ctx.stamp, ((size_t)n + 1) * 4));

@@ -187,7 +187,6 @@ struct Ctx {
   int32_t* cc_owner = nullptr;   // search group of each claimed vertex (-1 = none)
   int32_t* col_marker = nullptr; // this round's proposal (-2 = none)
   int32_t* mis_minw = nullptr;   // smallest speculative writer line
-  int32_t* mis_tag = nullptr;    // row kind stamps
   int32_t* cc_level = nullptr;   // BFS levels from vertex 0 kept for CC when BFS is not maintained
   int32_t* cc_parent = nullptr;  // parents for cc_level
   int64_t apply_seq = 0;         // batches applied to the graph

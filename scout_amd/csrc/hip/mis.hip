@@ -24,9 +24,16 @@
 // conflict anyway): the window doubles while a round commits all of it and shrinks to twice
 // the committed prefix after a conflict, so a round's work stays near twice its commits.
 //
-// Segments: the lines of a sub-batch form one segment. Several segments (the sub-batches of
-// independent instances, which share no vertex) run their rounds in the same launches, each
-// with its own window and first conflict. A single graph is the case of one instance.
+// Sub-batches: the reference cuts a batch into sub-batches and builds rows per sub-batch, so a
+// vertex's row kind depends on the line's sub-batch. It is read per line: the endpoints of
+// every sub-batch sit in one hash set of (vertex, sub-batch) keys, and a vertex has a row for
+// a line iff it is an endpoint of the line's sub-batch or neighbours one. The graph does not
+// change during the repair, so the lines of all sub-batches form one sequence, and one
+// speculative window runs across sub-batch boundaries.
+//
+// Segments: the lines of an instance form one segment. Several segments (independent
+// instances, which share no vertex) run their rounds in the same launches, each with its own
+// window and first conflict. A single graph is the case of one instance.
 //
 // Counts instead of scans: mcount[x] = members among x's neighbours, kept exact across
 // batches. "N(x) holds a member other than skip" is read from it in O(1) (less skip and the
@@ -36,10 +43,6 @@
 //
 // Hop-2 candidates N(y), y in N(r), are searched by chunk tasks over many waves; the first
 // decisive candidate in sequence order is the smallest (y position, entry position) key.
-//
-// Phases: the k-th sub-batches of all instances form phase k. All phases of a repair are
-// listed in one device table; the round-end kernel loads the next phase itself, so one
-// captured graph of rounds runs across phase boundaries.
 //
 // The candidate order of row(r) is reproduced without building the ball: the sequence
 // [r] + N(r) + N(y1) + N(y2) + ... (y_i in N(r) order) lists the ball's vertices in the same
@@ -61,8 +64,9 @@ struct MisView {
   GView g;
   bool sorted;
   const uint8_t* mis;
-  const int32_t* tag;
-  int32_t tag_ep;        // endpoint tag = 2*ep+1, hop-1 tag = 2*ep
+  const unsigned long long* eps;  // endpoint set: (vertex << 32 | sub-batch) keys, open addressing
+  int64_t emask;                  // its capacity - 1 (a power of two)
+  int32_t sb;                     // sub-batch of the line (its first line)
   const int32_t* mcount; // members among each vertex's neighbours (committed state)
   const int32_t* minw;   // smallest pending line writing the vertex
   const int32_t* minwn;  // smallest pending line writing one of the vertex's neighbours
@@ -170,34 +174,63 @@ __device__ bool row_has_member(const MisView& s, int32_t x, bool two_hop, int32_
   return false;
 }
 
-__device__ __forceinline__ int row_kind(const MisView& s, int32_t x) {
-  int32_t t = s.tag[x];
-  if (t == 2 * s.tag_ep + 1) return 2;
-  if (t == 2 * s.tag_ep) return 1;
-  return 0;
+constexpr unsigned long long kNoKey = ~0ULL;
+
+__device__ __forceinline__ unsigned long long ep_key(int32_t x, int32_t sb) {
+  return ((unsigned long long)(uint32_t)x << 32) | (uint32_t)sb;
+}
+
+__device__ __forceinline__ int64_t ep_slot(unsigned long long key, int64_t mask) {
+  unsigned long long h = key * 0x9E3779B97F4A7C15ULL;
+  return (int64_t)((h ^ (h >> 31)) & (unsigned long long)mask);
+}
+
+// Is x an endpoint of the line's sub-batch? (The set is at most a quarter full, so every
+// probe sequence meets an empty slot.)
+__device__ __forceinline__ bool is_endpoint(const MisView& s, int32_t x) {
+  unsigned long long key = ep_key(x, s.sb);
+  for (int64_t h = ep_slot(key, s.emask);; h = (h + 1) & s.emask) {
+    unsigned long long k = s.eps[h];
+    if (k == key) return true;
+    if (k == kNoKey) return false;
+  }
+}
+
+// Does the non-endpoint c have a row for the line, i.e. does it neighbour an endpoint of the
+// line's sub-batch? By the wave.
+__device__ bool near_endpoint(const MisView& s, int32_t c) {
+  Row r = row_of(s.g, c);
+  int64_t len = row_len(r);
+  *s.scanned += len;
+  for (int64_t g0 = 0; g0 < len; g0 += kWave) {
+    int64_t k = g0 + lane_id();
+    int32_t z = k < len ? row_at(s.g, r, k) : -1;
+    if (__ballot(z >= 0 && is_endpoint(s, z))) return true;
+  }
+  return false;
 }
 
 // Candidate c of the insertion rule (not a member, has a row, no member in its row), one
 // lane: 1 = accept, 0 = reject, -1 = conflict (check mode), 3 = an endpoint (its row is its
-// 2-hop ball, evaluated by the wave).
+// 2-hop ball, evaluated by the wave), 4 / 5 = accept / conflict if c has a row (the wave
+// tests that with near_endpoint).
 //
-// A candidate without a row is rejected whatever its membership, so its membership is not
-// read (and cannot conflict): the row kind (static within a sub-batch) is tested first.
-// near_ep: c is known to neighbour an endpoint (c in N(r)), so it has a row even when the
-// phase's hop-1 tags are not set.
+// near_ep: c is known to neighbour an endpoint (c in N(r)). Otherwise whether c has a row
+// costs a pass over N(c), so it is tested only when the outcome depends on it: a candidate
+// without a row is rejected whatever it reads, so it neither accepts nor conflicts.
 __device__ __forceinline__ int candidate_hop1(const MisView& s, int32_t c, bool near_ep) {
-  int kind = row_kind(s, c);
-  if (kind == 0 && near_ep) kind = 1;
-  if (kind == 0) return 0;
   bool conf = false;
   bool m = member(s, c, &conf);
-  if (conf) return -1;
-  if (m) return 0;
-  if (kind == 2) return 3;
+  if (is_endpoint(s, c)) return conf ? -1 : m ? 0 : 3;
+  if (!conf && m) return 0;
   bool has = nbr_has_member(s, c, -1, &conf);
-  if (conf) return -1;
-  return has ? 0 : 1;
+  if (!conf && has) return 0;
+  if (near_ep) return conf ? -1 : 1;
+  return conf ? 5 : 4;
 }
+
+// Outcome of a candidate code by the wave (codes 3 to 5 of candidate_hop1).
+__device__ __forceinline__ int resolve_candidate(const MisView& s, int code, int32_t c);
 
 // An endpoint candidate, by the wave.
 __device__ int try_endpoint(const MisView& s, int32_t c) {
@@ -208,6 +241,12 @@ __device__ int try_endpoint(const MisView& s, int32_t c) {
   bool has = row_has_member(s, c, true, -1, &conf);
   if (conf) return -1;
   return has ? 0 : 1;
+}
+
+__device__ __forceinline__ int resolve_candidate(const MisView& s, int code, int32_t c) {
+  if (code == 3) return try_endpoint(s, c);
+  if (code >= 4) return near_endpoint(s, c) ? (code == 4 ? 1 : -1) : 0;
+  return code;
 }
 
 // Candidates of row r in order, 256 at a time (one lane each for four chunks of 64, taken
@@ -234,7 +273,7 @@ __device__ int first_candidate(const MisView& s, const Row& r, int32_t* accepted
         pending &= pending - 1;
         int rj = __shfl(res[u], j);
         int32_t cj = __shfl(cs[u], j);
-        if (rj == 3) rj = try_endpoint(s, cj);
+        rj = resolve_candidate(s, rj, cj);
         if (rj == -1) return -1;
         if (rj == 1) {
           *accepted = cj;
@@ -309,44 +348,6 @@ __device__ bool eval_line(MisView s, int32_t u, int32_t v, bool ins, int32_t* w0
   return true;
 }
 
-constexpr int64_t kTagChunk = 4096;  // row entries per hop-1 tag task
-
-// Endpoint tags of the phase that starts this round (nothing otherwise).
-__global__ void k_mis_tag_endpoints(const int32_t* uv, const int64_t* t, const int64_t* ph, int64_t pcap,
-                                    int32_t* tag) {
-  if (!t[10]) return;
-  int64_t f = t[8];
-  int64_t lo = ph[f], hi = ph[pcap + f];
-  int32_t ep = (int32_t)t[2];
-  for (int64_t i = lo + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < hi; i += (int64_t)gridDim.x * blockDim.x) {
-    atomicMax(&tag[uv[2 * i]], 2 * ep + 1);
-    atomicMax(&tag[uv[2 * i + 1]], 2 * ep + 1);
-  }
-}
-
-// Hop-1 tags: wave per tag task (a chunk of an endpoint's row), so hub rows spread over
-// many waves. Only hop-2 candidates need them (a candidate in N(r) neighbours the endpoint
-// r), so they are set on demand: in the round after the phase's first hop-2 search.
-__global__ void k_mis_tag_hop1(GView g, const int32_t* uv, const int64_t* t, const int64_t* ph, int64_t pcap,
-                               int64_t scap, int32_t* tag) {
-  if (t[18] != 1) return;
-  int64_t f = t[8];
-  const int64_t* task0 = ph + 3 * pcap + 1 + 2 * scap;
-  const int64_t* tasks = ph + 4 * pcap + 2 + 2 * scap;
-  int32_t ep = (int32_t)t[2];
-  for (int64_t i = task0[f] + global_wave(); i < task0[f + 1]; i += total_waves()) {
-    int64_t task = tasks[i];
-    int32_t x = uv[task >> 20];
-    Row r = row_of(g, x);
-    int64_t lo = (task & 0xfffff) * kTagChunk, hi = min(lo + kTagChunk, row_len(r));
-    // endpoint tags (2 ep + 1) were set by the previous kernel; every writer here stores the
-    // same value, so a plain store after the check is enough
-    wave_walk_wide<8>(g, r, lo, hi, [&](int32_t y) {
-      if (tag[y] < 2 * ep) tag[y] = 2 * ep;
-    });
-  }
-}
-
 // Reference degree estimate per endpoint: base row length - |removed set| + |added list|
 // (HybridGraph::degree, core/FlashGraph.h:267-274); removed entries are the deleted base
 // entries plus the tombstones held in the inserted row.
@@ -366,59 +367,60 @@ __global__ void k_mis_degrees(GView g, const int32_t* uv, int64_t m, int64_t* de
       removed += __shfl_xor(removed, off);
       added += __shfl_xor(added, off);
     }
-    // low 32 bits: the degree estimate; high 32 bits: the row's entries (tombstones and holes
-    // included), which the hop-1 tag tasks cover
-    if (lane_id() == 0) deg[w] = ((b1 - b0) - removed + added) | (((b1 - b0) + (a1 - a0)) << 32);
+    if (lane_id() == 0) deg[w] = (b1 - b0) - removed + added;
+  }
+}
+
+// Endpoint set: key (x, sub-batch) for both endpoints of every line (linear probing; the
+// capacity is at least four times the keys, so every insertion finds a slot).
+__global__ void k_mis_ep_insert(const int32_t* uv, const int32_t* lsb, int64_t m, unsigned long long* eps,
+                                int64_t mask) {
+  for (int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; w < 2 * m; w += (int64_t)gridDim.x * blockDim.x) {
+    unsigned long long key = ep_key(uv[w], lsb[w >> 1]);
+    for (int64_t h = ep_slot(key, mask);; h = (h + 1) & mask) {
+      unsigned long long prev = atomicCAS(&eps[h], kNoKey, key);
+      if (prev == kNoKey || prev == key) break;
+    }
   }
 }
 
 // --- round engine over segments ---------------------------------------------------------
-// Segment table (int64, on the device): header [0] segments S of the current phase, [1] slots
-// (window lines over all segments), [2] tag epoch, [3] segments still pending, [4] non-empty
-// rounds, [5] writes, [6] line evaluations, [7] row entries read by passes A and B, [8]
-// current phase, [9] phases, [10] 1 = the phase starts this round (tags to set), [11] first
-// tag epoch, [12] initial window, [13]/[14] hop-2 searches requested in passes A/B this round,
-// [15] hop-2 chunk tasks of the current pass, [16] hop-2 searches of pass A over all rounds,
-// [17] 1 = a line of this round needs the phase's hop-1 tags, [18] hop-1 tags of the phase:
-// 0 = not set, 1 = set at the start of this round, 2 = set, [19] phases tagged; then per segment: pending start p, end hi, first conflict
-// k0, window size win, and woff (exclusive prefix of the current window sizes, S + 1).
-// Phase table (int64): lines lo[P] and hi[P] of each phase, first segment seg0[P + 1], the
-// segments' line ranges slo[NS] and shi[NS], first tag task task0[P + 1] and the tasks
-// (endpoint slot << 20 | chunk of kTagChunk row entries); its capacities are in the SegView.
-constexpr int kSegHdr = 20;
+// Segment table (int64, on the device): a header (slots below), then per segment: pending
+// start p, end hi, first conflict k0, window size win, and woff (exclusive prefix of the
+// current window sizes, S + 1).
+enum : int {
+  kHSegs = 0,      // segments S
+  kHSlots = 1,     // window lines over all segments
+  kHPending = 2,   // segments with lines left
+  kHRounds = 3,    // non-empty rounds
+  kHWrites = 4,    // committed lines that wrote
+  kHEvals = 5,     // line evaluations of pass A
+  kHEntries = 6,   // row entries read
+  kHReqA = 7,      // hop-2 searches requested in pass A this round
+  kHReqB = 8,      // ... and in pass B
+  kHTasks = 9,     // hop-2 chunk tasks of the current pass
+  kHHop2 = 10,     // hop-2 searches of pass A over all rounds
+  kSegHdr = 12
+};
 constexpr int64_t kMinWindow = 64;
 constexpr int64_t kMaxWindow = 1 << 20;
 
 struct SegView {
   int64_t* t;
   int64_t stride;
-  const int64_t* ph;  // phase table
-  int64_t pcap, scap; // its phase and segment capacities
-  __device__ int64_t& S() const { return t[0]; }
   __device__ int64_t* p() const { return t + kSegHdr; }
   __device__ int64_t* hi() const { return t + kSegHdr + stride; }
   __device__ int64_t* k0() const { return t + kSegHdr + 2 * stride; }
   __device__ int64_t* win() const { return t + kSegHdr + 3 * stride; }
   __device__ int64_t* woff() const { return t + kSegHdr + 4 * stride; }
-  __device__ int64_t ph_lo(int64_t f) const { return ph[f]; }
-  __device__ int64_t ph_hi(int64_t f) const { return ph[pcap + f]; }
-  __device__ int64_t ph_seg0(int64_t f) const { return ph[2 * pcap + f]; }
-  __device__ int64_t seg_lo(int64_t i) const { return ph[3 * pcap + 1 + i]; }
-  __device__ int64_t seg_hi(int64_t i) const { return ph[3 * pcap + 1 + scap + i]; }
-  __device__ int64_t task0(int64_t f) const { return ph[3 * pcap + 1 + 2 * scap + f]; }
-  __device__ int64_t task(int64_t i) const { return ph[4 * pcap + 2 + 2 * scap + i]; }
 };
 
 inline int64_t seg_table_size(int64_t stride) { return kSegHdr + 5 * stride + 1; }
-inline int64_t phase_table_size(int64_t pcap, int64_t scap, int64_t tcap) {
-  return 4 * pcap + 2 + 2 * scap + tcap;
-}
-
 
 // Window slot -> (segment, line): binary search over woff.
 __device__ __forceinline__ int64_t slot_line(const SegView& sv, int64_t slot, int32_t* sg_out) {
   const int64_t* woff = sv.woff();
-  int64_t lo = 0, hi = sv.t[0];  // find the last s with woff[s] <= slot
+  int64_t lo = 0, hi = sv.t[kHSegs];  // find the last s with woff[s] <= slot
   while (hi - lo > 1) {
     int64_t mid = (lo + hi) >> 1;
     if (woff[mid] <= slot) lo = mid;
@@ -428,16 +430,6 @@ __device__ __forceinline__ int64_t slot_line(const SegView& sv, int64_t slot, in
   return sv.p()[lo] + (slot - woff[lo]);
 }
 
-__global__ void k_mm_gather(const int32_t* uv, const uint8_t* ins, const int32_t* perm, int64_t m, int32_t* uv2,
-                            uint8_t* ins2) {
-  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= m) return;
-  int64_t k = perm[t];
-  uv2[2 * t] = uv[2 * k];
-  uv2[2 * t + 1] = uv[2 * k + 1];
-  ins2[t] = ins[k];
-}
-
 // Device arrays of one MIS repair.
 struct MisDev {
   GView g;
@@ -445,7 +437,9 @@ struct MisDev {
   const int32_t* uv;
   const uint8_t* ins;
   uint8_t* mis;
-  const int32_t* tag;
+  const int32_t* lsb;              // sub-batch of each line (its first line)
+  const unsigned long long* eps;   // endpoint set of all sub-batches
+  int64_t emask;
   int32_t* mcount;
   int32_t* minw;
   int32_t* minwn;
@@ -462,8 +456,9 @@ struct MisDev {
   int64_t h2_rcap, h2_tcap;
 };
 
-__device__ __forceinline__ MisView view_of(const MisDev& d, int32_t ep, int64_t k, bool check, int64_t* scanned) {
-  return MisView{d.g, d.sorted, d.mis, d.tag, ep, d.mcount, d.minw, d.minwn, (int32_t)k, check, -1, scanned};
+__device__ __forceinline__ MisView view_of(const MisDev& d, int64_t k, bool check, int64_t* scanned) {
+  return MisView{d.g,       d.sorted, d.mis,         d.eps,  d.emask, d.lsb[k], d.mcount,
+                 d.minw,    d.minwn,  (int32_t)k,    check,  -1,      scanned};
 }
 
 constexpr int32_t kHop2Pending = -3;  // w1 of a line whose hop-2 search is still running
@@ -499,29 +494,18 @@ __device__ __forceinline__ bool hop2_request(const MisDev& d, int64_t* count, in
 
 // Pass A: wave per window line. Writes go to w0/w1; minw[w] = smallest writer of w and
 // minwn[z] = smallest writer of a neighbour of z. A deferred hop-2 search leaves w1 pending.
-// Before the phase's hop-1 tags are set, a line that needs a hop-2 search ends its segment's
-// prefix (it writes nothing, and lines after it do not commit) and asks for the tags; it is
-// evaluated again in the next round, when they are set.
 __global__ void k_mm_eval(MisDev d, SegView sv) {
-  int64_t slots = sv.t[1];
-  int32_t ep = (int32_t)sv.t[2];
-  bool tagged = sv.t[18] != 0;
+  int64_t slots = sv.t[kHSlots];
   int64_t scanned = 0;
   for (int64_t t = global_wave(); t < slots; t += total_waves()) {
     int32_t sg;
     int64_t k = slot_line(sv, t, &sg);
-    MisView s = view_of(d, ep, k, false, &scanned);
+    MisView s = view_of(d, k, false, &scanned);
     int32_t w0, w1;
     bool hop2 = false;
     eval_line(s, d.uv[2 * k], d.uv[2 * k + 1], d.ins[k] != 0, &w0, &w1, true, &hop2);
-    if (hop2 && !tagged) {
-      w0 = w1 = -1;
-      if (lane_id() == 0) {
-        atomicMin((long long*)&sv.k0()[sg], (long long)k);
-        sv.t[17] = 1;
-      }
-    } else if (hop2) {
-      if (hop2_request(d, &sv.t[13], k, sg)) w1 = kHop2Pending;
+    if (hop2) {
+      if (hop2_request(d, &sv.t[kHReqA], k, sg)) w1 = kHop2Pending;
       else eval_line(s, d.uv[2 * k], d.uv[2 * k + 1], d.ins[k] != 0, &w0, &w1, false, &hop2);
     }
     if (lane_id() == 0) {
@@ -531,13 +515,12 @@ __global__ void k_mm_eval(MisDev d, SegView sv) {
     mark_write(d, w0, k);
     mark_write(d, w1, k);
   }
-  if (lane_id() == 0 && scanned) atomicAdd((unsigned long long*)&sv.t[7], (unsigned long long)scanned);
+  if (lane_id() == 0 && scanned) atomicAdd((unsigned long long*)&sv.t[kHEntries], (unsigned long long)scanned);
 }
 
 // Pass B: per segment, the first window line whose reads meet an earlier pending writer.
 __global__ void k_mm_check(MisDev d, SegView sv) {
-  int64_t slots = sv.t[1];
-  int32_t ep = (int32_t)sv.t[2];
+  int64_t slots = sv.t[kHSlots];
   int64_t* k0 = sv.k0();
   int64_t scanned = 0;
   for (int64_t t = global_wave(); t < slots; t += total_waves()) {
@@ -545,18 +528,15 @@ __global__ void k_mm_check(MisDev d, SegView sv) {
     int64_t k = slot_line(sv, t, &sg);
     if (k == sv.p()[sg]) continue;  // the first pending line never conflicts
     if (k >= __hip_atomic_load(&k0[sg], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) continue;
-    MisView s = view_of(d, ep, k, true, &scanned);
+    MisView s = view_of(d, k, true, &scanned);
     int32_t w0, w1;
     bool hop2 = false;
     bool ok = eval_line(s, d.uv[2 * k], d.uv[2 * k + 1], d.ins[k] != 0, &w0, &w1, true, &hop2);
-    // without tags pass A has ended the prefix at the first line that reaches hop 2 (this
-    // evaluation reads the same state, so it would reach hop 2 only past that line)
-    if (ok && hop2 && sv.t[18] == 0) ok = false;
-    else if (ok && hop2 && !hop2_request(d, &sv.t[14], k, sg))
+    if (ok && hop2 && !hop2_request(d, &sv.t[kHReqB], k, sg))
       ok = eval_line(s, d.uv[2 * k], d.uv[2 * k + 1], d.ins[k] != 0, &w0, &w1, false, &hop2);
     if (!ok && lane_id() == 0) atomicMin((long long*)&k0[sg], (long long)k);
   }
-  if (lane_id() == 0 && scanned) atomicAdd((unsigned long long*)&sv.t[7], (unsigned long long)scanned);
+  if (lane_id() == 0 && scanned) atomicAdd((unsigned long long*)&sv.t[kHEntries], (unsigned long long)scanned);
 }
 
 // The insertion's r of line k.
@@ -570,11 +550,11 @@ __device__ __forceinline__ int32_t line_r(const MisDev& d, int64_t k) {
 __global__ void k_mm_hop2_plan(MisDev d, SegView sv, int pass) {
   __shared__ int64_t scan[kBlock];
   __shared__ int64_t carry;
-  int64_t n = min(sv.t[pass == 0 ? 13 : 14], d.h2_rcap);
+  int64_t n = min(sv.t[pass == 0 ? kHReqA : kHReqB], d.h2_rcap);
   if (threadIdx.x == 0) {
     carry = 0;
-    sv.t[15] = 0;
-    if (pass == 0) sv.t[16] += n;
+    sv.t[kHTasks] = 0;
+    if (pass == 0) sv.t[kHHop2] += n;
   }
   __syncthreads();
   for (int64_t base = 0; base < n; base += blockDim.x) {
@@ -602,7 +582,7 @@ __global__ void k_mm_hop2_plan(MisDev d, SegView sv, int pass) {
 __global__ void k_mm_hop2_expand(MisDev d, SegView sv, int pass) {
   __shared__ int64_t scan[kBlock];
   __shared__ int64_t base_s;
-  int64_t n = min(sv.t[pass == 0 ? 13 : 14], d.h2_rcap);
+  int64_t n = min(sv.t[pass == 0 ? kHReqA : kHReqB], d.h2_rcap);
   int64_t nblocks = n > 0 ? d.h2_off[n] : 0;
   for (int64_t b = blockIdx.x; b < nblocks; b += gridDim.x) {
     int64_t lo = 0, hi = n;  // last request i with h2_off[i] <= b
@@ -628,7 +608,7 @@ __global__ void k_mm_hop2_expand(MisDev d, SegView sv, int pass) {
       scan[threadIdx.x] += y;
       __syncthreads();
     }
-    if (threadIdx.x == blockDim.x - 1) base_s = (int64_t)atomicAdd((unsigned long long*)&sv.t[15], (unsigned long long)scan[threadIdx.x]);
+    if (threadIdx.x == blockDim.x - 1) base_s = (int64_t)atomicAdd((unsigned long long*)&sv.t[kHTasks], (unsigned long long)scan[threadIdx.x]);
     __syncthreads();
     int64_t at = base_s + scan[threadIdx.x] - nch;
     if (at + nch > d.h2_tcap || nch > 0x1ffff) {
@@ -647,8 +627,7 @@ __device__ __forceinline__ unsigned long long hop2_key(int64_t yi, int64_t pos, 
 // Hop-2 scan: wave per task, the candidates of one chunk of N(y) in order; the first decisive
 // one (accept, or a conflict in pass B) lowers the request's best key (y index, position).
 __global__ void k_mm_hop2_scan(MisDev d, SegView sv, int pass) {
-  int64_t ntask = min(sv.t[15], d.h2_tcap);
-  int32_t ep = (int32_t)sv.t[2];
+  int64_t ntask = min(sv.t[kHTasks], d.h2_tcap);
   int64_t scanned = 0;
   for (int64_t t = global_wave(); t < ntask; t += total_waves()) {
     uint64_t task = d.h2_task[t];
@@ -662,7 +641,7 @@ __global__ void k_mm_hop2_scan(MisDev d, SegView sv, int pass) {
     int32_t y = row_at(d.g, row_of(d.g, r), yi);
     Row ry = row_of(d.g, y);
     int64_t c1 = min(c0 + kHop2Chunk, row_len(ry));
-    MisView s = view_of(d, ep, k, pass == 1, &scanned);
+    MisView s = view_of(d, k, pass == 1, &scanned);
     s.dropped = r;
     scanned += c1 - c0;
     bool done = false;
@@ -677,7 +656,7 @@ __global__ void k_mm_hop2_scan(MisDev d, SegView sv, int pass) {
         pending &= pending - 1;
         int rj = __shfl(res, j);
         int32_t cj = __shfl(c, j);
-        if (rj == 3) rj = try_endpoint(s, cj);
+        rj = resolve_candidate(s, rj, cj);
         if (rj == -1 || rj == 1) {
           if (lane_id() == 0) atomicMin(best, hop2_key(yi, g0 + j, rj == 1 ? 1 : 0));
           done = true;
@@ -686,15 +665,14 @@ __global__ void k_mm_hop2_scan(MisDev d, SegView sv, int pass) {
       }
     }
   }
-  if (lane_id() == 0 && scanned) atomicAdd((unsigned long long*)&sv.t[7], (unsigned long long)scanned);
+  if (lane_id() == 0 && scanned) atomicAdd((unsigned long long*)&sv.t[kHEntries], (unsigned long long)scanned);
 }
 
 // Hop-2 resume: wave per request. Pass A: the accepted candidate becomes w1 and is marked.
 // Pass B: a conflict before any acceptance ends the segment's prefix there. A flagged request
 // is searched serially.
 __global__ void k_mm_hop2_resume(MisDev d, SegView sv, int pass) {
-  int64_t n = min(sv.t[pass == 0 ? 13 : 14], d.h2_rcap);
-  int32_t ep = (int32_t)sv.t[2];
+  int64_t n = min(sv.t[pass == 0 ? kHReqA : kHReqB], d.h2_rcap);
   int64_t scanned = 0;
   for (int64_t i = global_wave(); i < n; i += total_waves()) {
     int64_t k = d.h2_line[i];
@@ -703,7 +681,7 @@ __global__ void k_mm_hop2_resume(MisDev d, SegView sv, int pass) {
     bool conflict = false;
     int32_t w1 = -1;
     if (d.h2_flag[i]) {
-      MisView s = view_of(d, ep, k, pass == 1, &scanned);
+      MisView s = view_of(d, k, pass == 1, &scanned);
       int32_t w0;
       bool hop2 = false;
       conflict = !eval_line(s, d.uv[2 * k], d.uv[2 * k + 1], d.ins[k] != 0, &w0, &w1, false, &hop2);
@@ -726,7 +704,7 @@ __global__ void k_mm_hop2_resume(MisDev d, SegView sv, int pass) {
       atomicMin((long long*)&sv.k0()[sg], (long long)k);
     }
   }
-  if (lane_id() == 0 && scanned) atomicAdd((unsigned long long*)&sv.t[7], (unsigned long long)scanned);
+  if (lane_id() == 0 && scanned) atomicAdd((unsigned long long*)&sv.t[kHEntries], (unsigned long long)scanned);
 }
 
 // Wave per window line: commits each segment's prefix [p, k0) (memberships and the member
@@ -734,7 +712,7 @@ __global__ void k_mm_hop2_resume(MisDev d, SegView sv, int pass) {
 // line's writes. Committed lines write distinct vertices (a second writer of a vertex reads
 // it and conflicts), so the count updates do not overlap.
 __global__ void k_mm_commit(MisDev d, SegView sv) {
-  int64_t slots = sv.t[1];
+  int64_t slots = sv.t[kHSlots];
   const int64_t* k0 = sv.k0();
   int64_t writes = 0;
   for (int64_t t = global_wave(); t < slots; t += total_waves()) {
@@ -763,7 +741,7 @@ __global__ void k_mm_commit(MisDev d, SegView sv) {
       });
     }
   }
-  if (lane_id() == 0 && writes) atomicAdd((unsigned long long*)&sv.t[5], (unsigned long long)writes);
+  if (lane_id() == 0 && writes) atomicAdd((unsigned long long*)&sv.t[kHWrites], (unsigned long long)writes);
 }
 
 // Member neighbours of every vertex (wave per vertex).
@@ -819,28 +797,18 @@ __device__ int64_t block_woff(int64_t S, int64_t* woff, SegView sv, int64_t* sca
   return total;
 }
 
-// One block: advances every segment (p = k0) and resizes its window (k0 = window end); when
-// every segment of the phase is done, loads the next phase's segments and flags its start.
-// Schedules the phase's hop-1 tags for the next round when a line asked for them. Rebuilds
-// woff and the slot count; counts non-empty rounds.
+// One block: advances every segment (p = k0), resizes its window (k0 = window end), rebuilds
+// woff and the slot count, and counts pending segments and non-empty rounds.
 __global__ void k_mm_round_end(SegView sv) {
   __shared__ int64_t scan[kBlock];
   __shared__ int64_t carry_s, active_s, pending_s;
   int64_t* t = sv.t;
-  int64_t S = t[0];
+  int64_t S = t[kHSegs];
   int64_t *p = sv.p(), *hi = sv.hi(), *k0 = sv.k0(), *win = sv.win();
   if (threadIdx.x == 0) {
     active_s = pending_s = 0;
-    t[10] = 0;  // this round's tags are set
-    t[13] = 0;  // hop-2 requests of passes A and B
-    t[14] = 0;
-    if (t[17]) {
-      t[18] = 1;  // tagged at the start of the next round, before any line reads the tags
-      t[19] += 1;
-    } else if (t[18] == 1) {
-      t[18] = 2;
-    }
-    t[17] = 0;
+    t[kHReqA] = 0;
+    t[kHReqB] = 0;
   }
   __syncthreads();
   for (int64_t s = threadIdx.x; s < S; s += blockDim.x) {
@@ -857,69 +825,37 @@ __global__ void k_mm_round_end(SegView sv) {
     k0[s] = ps + min(ws, hs - ps);
   }
   __syncthreads();
-  bool next_phase = pending_s == 0 && t[8] < t[9];
-  if (next_phase) {
-    int64_t f = t[8] + 1;
-    __syncthreads();
-    if (threadIdx.x == 0) t[8] = f;
-    if (f < t[9]) {
-      int64_t s0 = sv.ph_seg0(f), s1 = sv.ph_seg0(f + 1);
-      S = s1 - s0;
-      int64_t w0 = t[12];
-      for (int64_t s = threadIdx.x; s < S; s += blockDim.x) {
-        int64_t lo = sv.seg_lo(s0 + s), h = sv.seg_hi(s0 + s);
-        p[s] = lo;
-        hi[s] = h;
-        win[s] = w0;
-        k0[s] = lo + min(w0, h - lo);
-      }
-      if (threadIdx.x == 0) {
-        t[0] = S;
-        t[2] = t[11] + f;
-        t[3] = S;
-        t[10] = 1;
-        t[18] = 0;
-      }
-    } else {
-      S = 0;
-      if (threadIdx.x == 0) {
-        t[0] = 0;
-        t[3] = 0;
-      }
-    }
-    __syncthreads();
-  }
   int64_t total = block_woff(S, sv.woff(), sv, scan, &carry_s);
   if (threadIdx.x == 0) {
     sv.woff()[S] = total;
-    t[1] = total;
-    if (!next_phase) t[3] = pending_s;
-    if (active_s) t[4] += 1;
-    t[6] += total;
+    t[kHSlots] = total;
+    t[kHPending] = pending_s;
+    if (active_s) t[kHRounds] += 1;
+    t[kHEvals] += total;
   }
 }
 
 }  // namespace
 
 struct MisState {
-  DevBuf<int32_t> w0, w1, uv2, perm;
+  DevBuf<int32_t> w0, w1, uv2, lsb;
   DevBuf<uint8_t> ins2;
   DevBuf<int64_t> deg, seg;
+  DevBuf<unsigned long long> eps;  // endpoint set
+  int64_t ecap = 0;                // its slots (a power of two, at least 4 per key)
   std::vector<int64_t> h_deg;
-  int32_t tag_ep = 0;
+  std::vector<int32_t> h_lsb;
   unsigned grid = kListGrid;
   bool use_graph = true;
   int group_rounds = 16;
   int64_t window0 = 256;
   DevBuf<int32_t> mcount, minwn;
   int64_t mc_version = -1, mc_apply = -1, mc_gen = -1;  // what mcount was computed for
-  DevBuf<int64_t> phase;  // phase table
   DevBuf<int32_t> h2_line, h2_seg, h2_flag;
   DevBuf<unsigned long long> h2_best;
   DevBuf<int64_t> h2_off;
   DevBuf<uint64_t> h2_task;
   int64_t h2_rcap = kHop2Requests, h2_tcap = kHop2Tasks;  // list capacities (fallback: serial)
-  int64_t pcap = 0, scap = 0, tcap = 0;
   hipGraphExec_t exec = nullptr;
   std::vector<const void*> key;
 };
@@ -930,13 +866,13 @@ void mis_release(Ctx& c) {
   ms.w0.release();
   ms.w1.release();
   ms.uv2.release();
-  ms.perm.release();
+  ms.lsb.release();
+  ms.eps.release();
   ms.ins2.release();
   ms.deg.release();
   ms.seg.release();
   ms.mcount.release();
   ms.minwn.release();
-  ms.phase.release();
   ms.h2_line.release();
   ms.h2_seg.release();
   ms.h2_flag.release();
@@ -950,43 +886,81 @@ void mis_release(Ctx& c) {
 
 namespace {
 
+// Set of vertex ids, emptied in O(1) by a new stamp (open addressing, linear probing).
+struct StampSet {
+  std::vector<int32_t> key;
+  std::vector<uint32_t> stamp;
+  uint32_t now = 0;
+  uint64_t mask = 0;
+  void init(int64_t keys) {
+    uint64_t cap = 1024;
+    while (cap < (uint64_t)(4 * keys)) cap <<= 1;
+    if (cap > key.size()) {
+      key.assign(cap, 0);
+      stamp.assign(cap, 0);
+      now = 0;
+    }
+    mask = key.size() - 1;
+  }
+  void clear() {
+    if (++now == 0) {
+      std::fill(stamp.begin(), stamp.end(), 0);
+      now = 1;
+    }
+  }
+  // slot of x, or the empty slot where it would go
+  uint64_t find(int32_t x) const {
+    uint64_t h = ((uint64_t)(uint32_t)x * 0x9E3779B97F4A7C15ULL) >> 20;
+    for (h &= mask;; h = (h + 1) & mask)
+      if (stamp[h] != now || key[h] == x) return h;
+  }
+  bool has(int32_t x) const { return stamp[find(x)] == now; }
+  void insert(int32_t x) {
+    uint64_t h = find(x);
+    stamp[h] = now;
+    key[h] = x;
+  }
+};
+
 // Sub-batch cuts of lines [lo, hi) by the reference's estimate deg * 1000 per new endpoint
-// against max_csr, at most 2^20 endpoints' worth of lines each (src/mis/scout_mis.cu:519-542).
-void mis_cuts(const std::vector<int32_t>& uv, const std::vector<int64_t>& deg, int64_t lo, int64_t hi,
-              int64_t max_csr, std::vector<std::pair<int64_t, int64_t>>& subs) {
+// against max_csr, at most 2^20 lines each (src/mis/scout_mis.cu:519-542). lsb[k] = the
+// first line of k's sub-batch; returns the number of sub-batches.
+int64_t mis_cuts(const std::vector<int32_t>& uv, const std::vector<int64_t>& deg, int64_t lo, int64_t hi,
+                 int64_t max_csr, StampSet& uniq, std::vector<int32_t>& lsb) {
   const int64_t MAX_BATCH = 1048576;
-  int64_t idx = lo;
-  std::unordered_set<int32_t> uniq;
+  uniq.init(2 * std::min(hi - lo, MAX_BATCH));
+  int64_t idx = lo, subs = 0;
   while (idx < hi) {
     uniq.clear();
     int64_t cur = 0, se = 0, start = idx;
     while (idx < hi && se < MAX_BATCH) {
       int32_t a = uv[(size_t)(2 * idx)], b = uv[(size_t)(2 * idx + 1)];
       int64_t est = 0;
-      if (!uniq.count(a)) est += deg[(size_t)(2 * idx)] * 1000;
-      if (!uniq.count(b)) est += deg[(size_t)(2 * idx + 1)] * 1000;
+      if (!uniq.has(a)) est += deg[(size_t)(2 * idx)] * 1000;
+      if (!uniq.has(b)) est += deg[(size_t)(2 * idx + 1)] * 1000;
       if (se > 0 && cur + est >= max_csr) break;
       cur += est;
       uniq.insert(a);
       uniq.insert(b);
+      lsb[(size_t)idx] = (int32_t)start;
       se++;
       idx++;
     }
-    subs.emplace_back(start, idx);
+    subs++;
   }
+  return subs;
 }
 
 MisDev dev_of(Ctx& c, MisState& ms) {
-  return MisDev{c.g.view(), c.g.sorted, ms.uv2.p, ms.ins2.p, c.mis, c.mis_tag, ms.mcount.p, c.mis_minw, ms.minwn.p,
-                ms.w0.p, ms.w1.p, ms.h2_line.p, ms.h2_seg.p, ms.h2_best.p, ms.h2_off.p, ms.h2_flag.p, ms.h2_task.p,
-                ms.h2_rcap, ms.h2_tcap};
+  return MisDev{c.g.view(), c.g.sorted,   ms.uv2.p,     ms.ins2.p,    c.mis,        ms.lsb.p,
+                ms.eps.p,   ms.ecap - 1,  ms.mcount.p, c.mis_minw,   ms.minwn.p,   ms.w0.p,
+                ms.w1.p,    ms.h2_line.p, ms.h2_seg.p,  ms.h2_best.p, ms.h2_off.p,  ms.h2_flag.p,
+                ms.h2_task.p, ms.h2_rcap, ms.h2_tcap};
 }
 
 void mm_launch_round(Ctx& c, MisState& ms, SegView sv) {
   MisDev d = dev_of(c, ms);
-  // tags, pass A, its hop-2 searches, pass B, its hop-2 searches, commit, round end
-  k_mis_tag_endpoints<<<kListGrid, kBlock, 0, c.s>>>(ms.uv2.p, sv.t, sv.ph, sv.pcap, c.mis_tag);
-  k_mis_tag_hop1<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), ms.uv2.p, sv.t, sv.ph, sv.pcap, sv.scap, c.mis_tag);
+  // pass A, its hop-2 searches, pass B, its hop-2 searches, commit, round end
   k_mm_eval<<<ms.grid, kBlock, 0, c.s>>>(d, sv);
   for (int pass = 0; pass < 2; pass++) {
     if (pass == 1) k_mm_check<<<ms.grid, kBlock, 0, c.s>>>(d, sv);
@@ -1006,13 +980,11 @@ void mm_launch_group(Ctx& c, MisState& ms, SegView sv) {
 // Captures one group of rounds, unless the graph in hand holds the same buffers and settings.
 int mm_graph(Ctx& c, MisState& ms, SegView sv) {
   GView g = c.g.view();
-  std::vector<const void*> key = {ms.uv2.p, ms.ins2.p, sv.t, ms.w0.p, ms.w1.p, c.mis, c.mis_tag, c.mis_minw,
-                                  ms.mcount.p, ms.minwn.p, (const void*)(intptr_t)c.g.sorted, sv.ph, ms.h2_line.p,
-                                  ms.h2_seg.p, ms.h2_flag.p, ms.h2_best.p, ms.h2_off.p, ms.h2_task.p,
-                                  (const void*)(intptr_t)ms.h2_rcap, (const void*)(intptr_t)ms.h2_tcap,
-                                  (const void*)(intptr_t)sv.pcap, (const void*)(intptr_t)sv.scap,
-                                  (const void*)(intptr_t)ms.tcap,
-                                  g.boff, g.bcol, g.aoff, g.alen, g.acol, (const void*)(intptr_t)g.n,
+  std::vector<const void*> key = {ms.uv2.p, ms.ins2.p, sv.t, ms.w0.p, ms.w1.p, c.mis, ms.lsb.p, ms.eps.p,
+                                  (const void*)(intptr_t)ms.ecap, c.mis_minw, ms.mcount.p, ms.minwn.p,
+                                  (const void*)(intptr_t)c.g.sorted, ms.h2_line.p, ms.h2_seg.p, ms.h2_flag.p,
+                                  ms.h2_best.p, ms.h2_off.p, ms.h2_task.p, (const void*)(intptr_t)ms.h2_rcap,
+                                  (const void*)(intptr_t)ms.h2_tcap, g.boff, g.bcol, g.aoff, g.alen, g.acol, (const void*)(intptr_t)g.n,
                                   (const void*)(intptr_t)sv.stride, (const void*)(intptr_t)ms.grid,
                                   (const void*)(intptr_t)ms.group_rounds};
   if (ms.exec && key == ms.key) return SCOUT_OK;
@@ -1041,9 +1013,9 @@ int mm_graph(Ctx& c, MisState& ms, SegView sv) {
 }  // namespace
 
 // Instances own disjoint vertex sets and their lines sit in the staged batch instance after
-// instance: lines [off[i], off[i+1]) belong to instance i. Every instance gets the sub-batch
-// cuts it would get alone; phase k runs the k-th sub-batch of every instance as one segment
-// each. The result for each instance is the one its lines give when repaired alone.
+// instance: lines [off[i], off[i+1]) belong to instance i and form segment i. Every instance
+// gets the sub-batch cuts it would get alone, so the result for each instance is the one its
+// lines give when repaired alone.
 int mis_repair_multi(Ctx& c, Channel& ch, int32_t ninst, const int64_t* off, int64_t max_csr, scout_stats* st) {
   if (!c.mis_state) c.mis_state = new MisState();
   MisState& ms = *c.mis_state;
@@ -1051,20 +1023,26 @@ int mis_repair_multi(Ctx& c, Channel& ch, int32_t ninst, const int64_t* off, int
   int32_t n = c.g.n;
   SCOUT_CHECK(ensure_state(c, SCOUT_STATE_MIS));
   SCOUT_CHECK(ensure_scratch(c, &c.mis_minw, kNoWriter));
-  SCOUT_CHECK(ensure_scratch(c, &c.mis_tag, -1));
   if (m == 0) return SCOUT_OK;
   SCOUT_CHECK(ms.w0.reserve(m));
   SCOUT_CHECK(ms.w1.reserve(m));
   SCOUT_CHECK(ms.deg.reserve(2 * m));
   SCOUT_CHECK(ms.uv2.reserve(2 * m));
   SCOUT_CHECK(ms.ins2.reserve(m));
-  SCOUT_CHECK(ms.perm.reserve(m));
+  SCOUT_CHECK(ms.lsb.reserve(m));
   SCOUT_CHECK(ms.h2_line.reserve(ms.h2_rcap));
   SCOUT_CHECK(ms.h2_seg.reserve(ms.h2_rcap));
   SCOUT_CHECK(ms.h2_flag.reserve(ms.h2_rcap));
   SCOUT_CHECK(ms.h2_best.reserve(ms.h2_rcap));
   SCOUT_CHECK(ms.h2_off.reserve(ms.h2_rcap + 1));
   SCOUT_CHECK(ms.h2_task.reserve(ms.h2_tcap));
+  // endpoint set: at least 4 slots per key; grows by doubling (its size is in the graph key)
+  if (ms.ecap < 8 * m || !ms.eps.p) {
+    if (ms.ecap < 1024) ms.ecap = 1024;
+    while (ms.ecap < 8 * m) ms.ecap *= 2;
+    ms.eps.release();
+    SCOUT_CHECK(ms.eps.reserve(ms.ecap));
+  }
   int64_t stride = std::max<int64_t>(ninst, 64);
   SCOUT_CHECK(ms.seg.reserve(seg_table_size(stride)));
   // the table stride is part of the captured graph: keep the largest seen
@@ -1085,6 +1063,8 @@ int mis_repair_multi(Ctx& c, Channel& ch, int32_t ninst, const int64_t* off, int
   ms.mc_gen = c.g.base_gen;
   ms.mc_apply = c.apply_seq;
   k_mis_degrees<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), ch.d_uv.p, m, ms.deg.p);
+  SCOUT_TRY(hipMemcpyAsync(ms.uv2.p, ch.d_uv.p, (size_t)(2 * m) * 4, hipMemcpyDeviceToDevice, c.s));
+  SCOUT_TRY(hipMemcpyAsync(ms.ins2.p, ch.d_ins.p, (size_t)m, hipMemcpyDeviceToDevice, c.s));
   ms.h_deg.resize((size_t)(2 * m));
   std::vector<int32_t> uv((size_t)(2 * m));
   std::vector<uint8_t> ins((size_t)m);
@@ -1092,122 +1072,34 @@ int mis_repair_multi(Ctx& c, Channel& ch, int32_t ninst, const int64_t* off, int
   SCOUT_TRY(hipMemcpyAsync(uv.data(), ch.d_uv.p, (size_t)(2 * m) * 4, hipMemcpyDeviceToHost, c.s));
   SCOUT_TRY(hipMemcpyAsync(ins.data(), ch.d_ins.p, (size_t)m, hipMemcpyDeviceToHost, c.s));
   SCOUT_TRY(hipStreamSynchronize(c.s));
-  std::vector<int64_t> row_entries((size_t)(2 * m));
-  for (size_t i = 0; i < (size_t)(2 * m); i++) {
-    row_entries[i] = ms.h_deg[i] >> 32;
-    ms.h_deg[i] &= 0xffffffffLL;
-  }
-  std::vector<std::vector<std::pair<int64_t, int64_t>>> cuts((size_t)ninst);
-  size_t nphase = 0;
-  for (int32_t i = 0; i < ninst; i++) {
-    mis_cuts(uv, ms.h_deg, off[i], off[i + 1], max_csr, cuts[(size_t)i]);
-    nphase = std::max(nphase, cuts[(size_t)i].size());
-    st->sub_batches += (int64_t)cuts[(size_t)i].size();
-  }
   for (int64_t k = 0; k < m; k++) (ins[(size_t)k] ? st->insertions : st->deletions)++;
-  // permutation: phase after phase, segment after segment, lines in their order
-  std::vector<int32_t> perm((size_t)m);
-  struct Phase {
-    int64_t lo, hi;
-    std::vector<int64_t> seg_lo, seg_hi;
-  };
-  std::vector<Phase> phases(nphase);
-  int64_t t = 0;
-  bool identity = true;
-  for (size_t ph = 0; ph < nphase; ph++) {
-    Phase& P = phases[ph];
-    P.lo = t;
-    for (int32_t i = 0; i < ninst; i++) {
-      if (ph >= cuts[(size_t)i].size()) continue;
-      auto sb = cuts[(size_t)i][ph];
-      P.seg_lo.push_back(t);
-      for (int64_t k = sb.first; k < sb.second; k++, t++) {
-        perm[(size_t)t] = (int32_t)k;
-        identity = identity && k == t;
-      }
-      P.seg_hi.push_back(t);
-    }
-    P.hi = t;
-  }
-  if (!identity) {
-    SCOUT_TRY(hipMemcpyAsync(ms.perm.p, perm.data(), (size_t)m * 4, hipMemcpyHostToDevice, c.s));
-    k_mm_gather<<<grid_for(m), kBlock, 0, c.s>>>(ch.d_uv.p, ch.d_ins.p, ms.perm.p, m, ms.uv2.p, ms.ins2.p);
-  } else {
-    SCOUT_TRY(hipMemcpyAsync(ms.uv2.p, ch.d_uv.p, (size_t)(2 * m) * 4, hipMemcpyDeviceToDevice, c.s));
-    SCOUT_TRY(hipMemcpyAsync(ms.ins2.p, ch.d_ins.p, (size_t)m, hipMemcpyDeviceToDevice, c.s));
-  }
-  // phase table: capacities grow by doubling so the captured graph survives most batches
-  int64_t P = (int64_t)phases.size(), NS = 0, NT = 0;
-  for (Phase& F : phases) NS += (int64_t)F.seg_lo.size();
-  for (int64_t w = 0; w < 2 * m; w++) NT += (row_entries[(size_t)w] + kTagChunk - 1) / kTagChunk;
-  if (P > ms.pcap || NS > ms.scap || NT > ms.tcap || !ms.phase.p) {
-    while (ms.pcap < P) ms.pcap = std::max<int64_t>(2 * ms.pcap, 64);
-    while (ms.scap < NS) ms.scap = std::max<int64_t>(2 * ms.scap, 256);
-    while (ms.tcap < NT) ms.tcap = std::max<int64_t>(2 * ms.tcap, 4096);
-    ms.phase.release();
-    SCOUT_CHECK(ms.phase.reserve(phase_table_size(ms.pcap, ms.scap, ms.tcap)));
-  }
-  std::vector<int64_t> ph((size_t)phase_table_size(ms.pcap, ms.scap, ms.tcap), 0);
-  {
-    int64_t sidx = 0, tidx = 0;
-    int64_t* task0 = ph.data() + 3 * ms.pcap + 1 + 2 * ms.scap;
-    int64_t* tasks = ph.data() + 4 * ms.pcap + 2 + 2 * ms.scap;
-    for (int64_t f = 0; f < P; f++) {
-      Phase& F = phases[(size_t)f];
-      ph[(size_t)f] = F.lo;
-      ph[(size_t)(ms.pcap + f)] = F.hi;
-      ph[(size_t)(2 * ms.pcap + f)] = sidx;
-      for (size_t j = 0; j < F.seg_lo.size(); j++, sidx++) {
-        ph[(size_t)(3 * ms.pcap + 1 + sidx)] = F.seg_lo[j];
-        ph[(size_t)(3 * ms.pcap + 1 + ms.scap + sidx)] = F.seg_hi[j];
-      }
-      task0[f] = tidx;
-      for (int64_t tl = F.lo; tl < F.hi; tl++)
-        for (int side = 0; side < 2; side++) {
-          int64_t slot = 2 * tl + side;  // endpoint slot in the permuted batch
-          int64_t len = row_entries[(size_t)(2 * (int64_t)perm[(size_t)tl] + side)];
-          for (int64_t ck = 0; ck * kTagChunk < len; ck++) tasks[tidx++] = (slot << 20) | ck;
-        }
-    }
-    ph[(size_t)(2 * ms.pcap + P)] = sidx;
-    task0[P] = tidx;
-  }
-  SegView sv{ms.seg.p, stride, ms.phase.p, ms.pcap, ms.scap};
-  // tag epochs: one per phase
-  if (ms.tag_ep + P + 1 >= (1 << 29)) {
-    SCOUT_TRY(hipMemsetAsync(c.mis_tag, 0xff, ((size_t)n + 1) * 4, c.s));
-    ms.tag_ep = 0;
-  }
-  int64_t ep0 = ms.tag_ep + 1;
-  ms.tag_ep += P;
-  // segment table with phase 0 loaded
+  ms.h_lsb.resize((size_t)m);
+  StampSet uniq;
+  for (int32_t i = 0; i < ninst; i++)
+    st->sub_batches += mis_cuts(uv, ms.h_deg, off[i], off[i + 1], max_csr, uniq, ms.h_lsb);
+  SCOUT_TRY(hipMemcpyAsync(ms.lsb.p, ms.h_lsb.data(), (size_t)m * 4, hipMemcpyHostToDevice, c.s));
+  SCOUT_TRY(hipMemsetAsync(ms.eps.p, 0xff, (size_t)ms.ecap * 8, c.s));
+  k_mis_ep_insert<<<grid_for(2 * m), kBlock, 0, c.s>>>(ms.uv2.p, ms.lsb.p, m, ms.eps.p, ms.ecap - 1);
+  SegView sv{ms.seg.p, stride};
+  // segment table: one segment per instance with lines
   std::vector<int64_t> tab((size_t)seg_table_size(stride), 0);
-  int64_t slots0 = 0;
-  {
-    Phase& F = phases[0];
-    int64_t S = (int64_t)F.seg_lo.size();
-    for (int64_t sg = 0; sg < S; sg++) {
-      int64_t lo = F.seg_lo[(size_t)sg], hi = F.seg_hi[(size_t)sg];
-      int64_t size = std::min(ms.window0, hi - lo);
-      tab[(size_t)(kSegHdr + sg)] = lo;
-      tab[(size_t)(kSegHdr + stride + sg)] = hi;
-      tab[(size_t)(kSegHdr + 2 * stride + sg)] = lo + size;
-      tab[(size_t)(kSegHdr + 3 * stride + sg)] = ms.window0;
-      tab[(size_t)(kSegHdr + 4 * stride + sg)] = slots0;
-      slots0 += size;
-    }
+  int64_t S = 0, slots0 = 0;
+  for (int32_t i = 0; i < ninst; i++) {
+    int64_t lo = off[i], hi = off[i + 1];
+    if (lo == hi) continue;
+    int64_t size = std::min(ms.window0, hi - lo);
+    tab[(size_t)(kSegHdr + S)] = lo;
+    tab[(size_t)(kSegHdr + stride + S)] = hi;
+    tab[(size_t)(kSegHdr + 2 * stride + S)] = lo + size;
+    tab[(size_t)(kSegHdr + 3 * stride + S)] = ms.window0;
     tab[(size_t)(kSegHdr + 4 * stride + S)] = slots0;
-    tab[0] = S;
-    tab[1] = slots0;
-    tab[2] = ep0;
-    tab[3] = S;
-    tab[8] = 0;
-    tab[9] = P;
-    tab[10] = 1;
-    tab[11] = ep0;
-    tab[12] = ms.window0;
+    slots0 += size;
+    S++;
   }
-  SCOUT_TRY(hipMemcpyAsync(ms.phase.p, ph.data(), ph.size() * 8, hipMemcpyHostToDevice, c.s));
+  tab[(size_t)(kSegHdr + 4 * stride + S)] = slots0;
+  tab[kHSegs] = S;
+  tab[kHSlots] = slots0;
+  tab[kHPending] = S;
   SCOUT_TRY(hipMemcpyAsync(ms.seg.p, tab.data(), tab.size() * 8, hipMemcpyHostToDevice, c.s));
   SCOUT_TRY(hipStreamSynchronize(c.s));  // the host tables are freed on return
   if (ms.use_graph) SCOUT_CHECK(mm_graph(c, ms, sv));
@@ -1218,11 +1110,11 @@ int mis_repair_multi(Ctx& c, Channel& ch, int32_t ninst, const int64_t* off, int
     SCOUT_TRY(hipMemcpyAsync(c.h_box, ms.seg.p, kSegHdr * sizeof(int64_t), hipMemcpyDeviceToHost, c.s));
     SCOUT_TRY(hipStreamSynchronize(c.s));
     c.readbacks++;
-    if (c.h_box[8] >= P) break;
+    if (c.h_box[kHPending] == 0) break;
   }
-  int64_t rounds = c.h_box[4], writes = c.h_box[5], evals = c.h_box[6] + slots0, entries = c.h_box[7];
-  st->seeds += c.h_box[16];  // insertions whose candidate search reached the hop-2 rows
-  st->splits += c.h_box[19];  // phases whose hop-1 rows were tagged
+  int64_t rounds = c.h_box[kHRounds], writes = c.h_box[kHWrites];
+  int64_t evals = c.h_box[kHEvals] + slots0, entries = c.h_box[kHEntries];
+  st->seeds += c.h_box[kHHop2];  // insertions whose candidate search reached the hop-2 rows
   // per line evaluation: the pair, label, row offsets of both endpoints, memberships and the
   // write slots (~48 B); per row entry read: the entry and the membership byte (5 B)
   SCOUT_CHECK(prof_end(c, PROF_MIS_EVAL, 48 * evals + 5 * entries));
