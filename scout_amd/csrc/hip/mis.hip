@@ -215,16 +215,22 @@ __device__ bool near_endpoint(const MisView& s, int32_t c) {
 // 2-hop ball, evaluated by the wave), 4 / 5 = accept / conflict if c has a row (the wave
 // tests that with near_endpoint).
 //
+// Any candidate with a member neighbour (other than the dropped r) is rejected, whatever its
+// kind (an endpoint's 2-hop ball contains N(c)), and so is a member; a reject needs only that
+// its own read is settled (no earlier pending writer). The member-neighbour count is tested
+// first: in a near-maximal set it rejects most candidates with one read.
+//
 // near_ep: c is known to neighbour an endpoint (c in N(r)). Otherwise whether c has a row
 // costs a pass over N(c), so it is tested only when the outcome depends on it: a candidate
 // without a row is rejected whatever it reads, so it neither accepts nor conflicts.
 __device__ __forceinline__ int candidate_hop1(const MisView& s, int32_t c, bool near_ep) {
-  bool conf = false;
-  bool m = member(s, c, &conf);
-  if (is_endpoint(s, c)) return conf ? -1 : m ? 0 : 3;
-  if (!conf && m) return 0;
-  bool has = nbr_has_member(s, c, -1, &conf);
-  if (!conf && has) return 0;
+  bool conf_n = false, conf_m = false;
+  bool has = nbr_has_member(s, c, -1, &conf_n);
+  if (!conf_n && has) return 0;
+  bool m = member(s, c, &conf_m);
+  if (!conf_m && m) return 0;
+  bool conf = conf_n || conf_m;
+  if (is_endpoint(s, c)) return conf ? -1 : 3;
   if (near_ep) return conf ? -1 : 1;
   return conf ? 5 : 4;
 }
@@ -350,10 +356,14 @@ __device__ bool eval_line(MisView s, int32_t u, int32_t v, bool ins, int32_t* w0
 
 // Reference degree estimate per endpoint: base row length - |removed set| + |added list|
 // (HybridGraph::degree, core/FlashGraph.h:267-274); removed entries are the deleted base
-// entries plus the tombstones held in the inserted row.
-__global__ void k_mis_degrees(GView g, const int32_t* uv, int64_t m, int64_t* deg) {
+// entries plus the tombstones held in the inserted row. Computed once per distinct endpoint:
+// the wave that first stamps x with this repair's generation writes val[x].
+__global__ void k_mis_degrees(GView g, const int32_t* uv, int64_t m, int32_t* stamp, int32_t gen, int32_t* val) {
   for (int64_t w = global_wave(); w < 2 * m; w += total_waves()) {
     int32_t x = uv[w];
+    int32_t old = 0;
+    if (lane_id() == 0) old = atomicMax(&stamp[x], gen);
+    if (__shfl(old, 0) == gen) continue;
     Row row = row_of(g, x);
     int64_t b0 = row.b0, b1 = row.b1, a0 = row.a0, a1 = row.a1;
     int64_t removed = 0, added = 0;
@@ -367,8 +377,13 @@ __global__ void k_mis_degrees(GView g, const int32_t* uv, int64_t m, int64_t* de
       removed += __shfl_xor(removed, off);
       added += __shfl_xor(added, off);
     }
-    if (lane_id() == 0) deg[w] = (b1 - b0) - removed + added;
+    if (lane_id() == 0) val[x] = (int32_t)((b1 - b0) - removed + added);
   }
+}
+
+__global__ void k_mis_deg_gather(const int32_t* uv, int64_t m, const int32_t* val, int64_t* deg) {
+  int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (w < 2 * m) deg[w] = val[uv[w]];
 }
 
 // Endpoint set: key (x, sub-batch) for both endpoints of every line (linear probing; the
@@ -453,6 +468,7 @@ struct MisDev {
   int64_t* h2_off;
   int32_t* h2_flag;
   uint64_t* h2_task;
+  unsigned long long* h2_keyA;  // per line: the first accepted candidate pass A found at hop 2
   int64_t h2_rcap, h2_tcap;
 };
 
@@ -476,15 +492,18 @@ __device__ __forceinline__ void mark_write(const MisDev& d, int32_t w, int64_t k
 }
 
 // Records a deferred hop-2 search of line k (segment sg) in the pass's request list; false if
-// the list is full (the caller then searches serially).
-__device__ __forceinline__ bool hop2_request(const MisDev& d, int64_t* count, int64_t k, int32_t sg) {
+// the list is full (the caller then searches serially). best starts at `seed`: pass B needs
+// only the candidates up to the one pass A accepted (the same reads in check mode end at a
+// conflict or at that acceptance).
+__device__ __forceinline__ bool hop2_request(const MisDev& d, int64_t* count, int64_t k, int32_t sg,
+                                             unsigned long long seed) {
   int64_t idx = 0;
   if (lane_id() == 0) {
     idx = (int64_t)atomicAdd((unsigned long long*)count, 1ULL);
     if (idx < d.h2_rcap) {
       d.h2_line[idx] = (int32_t)k;
       d.h2_seg[idx] = sg;
-      d.h2_best[idx] = kNoResult;
+      d.h2_best[idx] = seed;
       d.h2_flag[idx] = 0;
     }
   }
@@ -505,8 +524,12 @@ __global__ void k_mm_eval(MisDev d, SegView sv) {
     bool hop2 = false;
     eval_line(s, d.uv[2 * k], d.uv[2 * k + 1], d.ins[k] != 0, &w0, &w1, true, &hop2);
     if (hop2) {
-      if (hop2_request(d, &sv.t[kHReqA], k, sg)) w1 = kHop2Pending;
-      else eval_line(s, d.uv[2 * k], d.uv[2 * k + 1], d.ins[k] != 0, &w0, &w1, false, &hop2);
+      if (hop2_request(d, &sv.t[kHReqA], k, sg, kNoResult)) {
+        w1 = kHop2Pending;
+      } else {
+        eval_line(s, d.uv[2 * k], d.uv[2 * k + 1], d.ins[k] != 0, &w0, &w1, false, &hop2);
+        if (lane_id() == 0) d.h2_keyA[k] = kNoResult;
+      }
     }
     if (lane_id() == 0) {
       d.w0s[k] = w0;
@@ -532,7 +555,7 @@ __global__ void k_mm_check(MisDev d, SegView sv) {
     int32_t w0, w1;
     bool hop2 = false;
     bool ok = eval_line(s, d.uv[2 * k], d.uv[2 * k + 1], d.ins[k] != 0, &w0, &w1, true, &hop2);
-    if (ok && hop2 && !hop2_request(d, &sv.t[kHReqB], k, sg))
+    if (ok && hop2 && !hop2_request(d, &sv.t[kHReqB], k, sg, d.h2_keyA[k]))
       ok = eval_line(s, d.uv[2 * k], d.uv[2 * k + 1], d.ins[k] != 0, &w0, &w1, false, &hop2);
     if (!ok && lane_id() == 0) atomicMin((long long*)&k0[sg], (long long)k);
   }
@@ -698,7 +721,10 @@ __global__ void k_mm_hop2_resume(MisDev d, SegView sv, int pass) {
       }
     }
     if (pass == 0) {
-      if (lane_id() == 0) d.w1s[k] = w1;
+      if (lane_id() == 0) {
+        d.w1s[k] = w1;
+        d.h2_keyA[k] = d.h2_flag[i] ? kNoResult : d.h2_best[i];
+      }
       mark_write(d, w1, k);
     } else if (conflict && lane_id() == 0) {
       atomicMin((long long*)&sv.k0()[sg], (long long)k);
@@ -841,6 +867,8 @@ struct MisState {
   DevBuf<int32_t> w0, w1, uv2, lsb;
   DevBuf<uint8_t> ins2;
   DevBuf<int64_t> deg, seg;
+  DevBuf<int32_t> dstamp, dval;  // per-vertex degree stamps and values
+  int32_t dgen = 0;
   DevBuf<unsigned long long> eps;  // endpoint set
   int64_t ecap = 0;                // its slots (a power of two, at least 4 per key)
   std::vector<int64_t> h_deg;
@@ -855,6 +883,7 @@ struct MisState {
   DevBuf<unsigned long long> h2_best;
   DevBuf<int64_t> h2_off;
   DevBuf<uint64_t> h2_task;
+  DevBuf<unsigned long long> h2_keyA;
   int64_t h2_rcap = kHop2Requests, h2_tcap = kHop2Tasks;  // list capacities (fallback: serial)
   hipGraphExec_t exec = nullptr;
   std::vector<const void*> key;
@@ -870,6 +899,8 @@ void mis_release(Ctx& c) {
   ms.eps.release();
   ms.ins2.release();
   ms.deg.release();
+  ms.dstamp.release();
+  ms.dval.release();
   ms.seg.release();
   ms.mcount.release();
   ms.minwn.release();
@@ -879,6 +910,7 @@ void mis_release(Ctx& c) {
   ms.h2_best.release();
   ms.h2_off.release();
   ms.h2_task.release();
+  ms.h2_keyA.release();
   if (ms.exec) (void)hipGraphExecDestroy(ms.exec);
   delete c.mis_state;
   c.mis_state = nullptr;
@@ -955,7 +987,7 @@ MisDev dev_of(Ctx& c, MisState& ms) {
   return MisDev{c.g.view(), c.g.sorted,   ms.uv2.p,     ms.ins2.p,    c.mis,        ms.lsb.p,
                 ms.eps.p,   ms.ecap - 1,  ms.mcount.p, c.mis_minw,   ms.minwn.p,   ms.w0.p,
                 ms.w1.p,    ms.h2_line.p, ms.h2_seg.p,  ms.h2_best.p, ms.h2_off.p,  ms.h2_flag.p,
-                ms.h2_task.p, ms.h2_rcap, ms.h2_tcap};
+                ms.h2_task.p, ms.h2_keyA.p, ms.h2_rcap, ms.h2_tcap};
 }
 
 void mm_launch_round(Ctx& c, MisState& ms, SegView sv) {
@@ -983,7 +1015,7 @@ int mm_graph(Ctx& c, MisState& ms, SegView sv) {
   std::vector<const void*> key = {ms.uv2.p, ms.ins2.p, sv.t, ms.w0.p, ms.w1.p, c.mis, ms.lsb.p, ms.eps.p,
                                   (const void*)(intptr_t)ms.ecap, c.mis_minw, ms.mcount.p, ms.minwn.p,
                                   (const void*)(intptr_t)c.g.sorted, ms.h2_line.p, ms.h2_seg.p, ms.h2_flag.p,
-                                  ms.h2_best.p, ms.h2_off.p, ms.h2_task.p, (const void*)(intptr_t)ms.h2_rcap,
+                                  ms.h2_best.p, ms.h2_off.p, ms.h2_task.p, ms.h2_keyA.p, (const void*)(intptr_t)ms.h2_rcap,
                                   (const void*)(intptr_t)ms.h2_tcap, g.boff, g.bcol, g.aoff, g.alen, g.acol, (const void*)(intptr_t)g.n,
                                   (const void*)(intptr_t)sv.stride, (const void*)(intptr_t)ms.grid,
                                   (const void*)(intptr_t)ms.group_rounds};
@@ -1030,6 +1062,7 @@ int mis_repair_multi(Ctx& c, Channel& ch, int32_t ninst, const int64_t* off, int
   SCOUT_CHECK(ms.uv2.reserve(2 * m));
   SCOUT_CHECK(ms.ins2.reserve(m));
   SCOUT_CHECK(ms.lsb.reserve(m));
+  SCOUT_CHECK(ms.h2_keyA.reserve(m));
   SCOUT_CHECK(ms.h2_line.reserve(ms.h2_rcap));
   SCOUT_CHECK(ms.h2_seg.reserve(ms.h2_rcap));
   SCOUT_CHECK(ms.h2_flag.reserve(ms.h2_rcap));
@@ -1062,7 +1095,17 @@ int mis_repair_multi(Ctx& c, Channel& ch, int32_t ninst, const int64_t* off, int
   ms.mc_version = c.mis_version;
   ms.mc_gen = c.g.base_gen;
   ms.mc_apply = c.apply_seq;
-  k_mis_degrees<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), ch.d_uv.p, m, ms.deg.p);
+  if (!ms.dstamp.p || ms.dstamp.cap < (int64_t)n + 1 || ms.dgen == INT32_MAX) {
+    ms.dstamp.release();
+    ms.dval.release();
+    SCOUT_CHECK(ms.dstamp.reserve((int64_t)n + 1));
+    SCOUT_CHECK(ms.dval.reserve((int64_t)n + 1));
+    SCOUT_TRY(hipMemsetAsync(ms.dstamp.p, 0, (size_t)ms.dstamp.cap * 4, c.s));
+    ms.dgen = 0;
+  }
+  ms.dgen++;
+  k_mis_degrees<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), ch.d_uv.p, m, ms.dstamp.p, ms.dgen, ms.dval.p);
+  k_mis_deg_gather<<<grid_for(2 * m), kBlock, 0, c.s>>>(ch.d_uv.p, m, ms.dval.p, ms.deg.p);
   SCOUT_TRY(hipMemcpyAsync(ms.uv2.p, ch.d_uv.p, (size_t)(2 * m) * 4, hipMemcpyDeviceToDevice, c.s));
   SCOUT_TRY(hipMemcpyAsync(ms.ins2.p, ch.d_ins.p, (size_t)m, hipMemcpyDeviceToDevice, c.s));
   ms.h_deg.resize((size_t)(2 * m));
