@@ -235,7 +235,8 @@ __device__ __forceinline__ int candidate_hop1(const MisView& s, int32_t c, bool 
   return conf ? 5 : 4;
 }
 
-// Outcome of a candidate code by the wave (codes 3 to 5 of candidate_hop1).
+// Outcome of a candidate code by the wave (codes 3 to 5 of candidate_hop1; 5 = a conflict
+// if the candidate has a row, endpoint or not).
 __device__ __forceinline__ int resolve_candidate(const MisView& s, int code, int32_t c);
 
 // An endpoint candidate, by the wave.
@@ -251,7 +252,8 @@ __device__ int try_endpoint(const MisView& s, int32_t c) {
 
 __device__ __forceinline__ int resolve_candidate(const MisView& s, int code, int32_t c) {
   if (code == 3) return try_endpoint(s, c);
-  if (code >= 4) return near_endpoint(s, c) ? (code == 4 ? 1 : -1) : 0;
+  if (code == 4) return near_endpoint(s, c) ? 1 : 0;
+  if (code == 5) return (is_endpoint(s, c) || near_endpoint(s, c)) ? -1 : 0;
   return code;
 }
 
@@ -468,6 +470,8 @@ struct MisDev {
   int64_t* h2_off;
   int32_t* h2_flag;
   uint64_t* h2_task;
+  int32_t* h2_y;                // per task: its y
+  int64_t* h2_rsb;              // per request: r << 32 | sub-batch of the line
   unsigned long long* h2_keyA;  // per line: the first accepted candidate pass A found at hop 2
   int64_t h2_rcap, h2_tcap;
 };
@@ -480,8 +484,14 @@ __device__ __forceinline__ MisView view_of(const MisDev& d, int64_t k, bool chec
 constexpr int32_t kHop2Pending = -3;  // w1 of a line whose hop-2 search is still running
 constexpr int64_t kHop2Requests = 65536;   // deferred hop-2 searches per pass (16-bit task field)
 constexpr int64_t kHop2Tasks = 1 << 22;    // hop-2 chunk tasks per pass
-constexpr int64_t kHop2Chunk = 1024;  // row entries of N(y) per hop-2 task
+constexpr int64_t kHop2Chunk = 512;  // row entries of N(y) per hop-2 task
 constexpr unsigned long long kNoResult = ~0ULL;
+
+// The insertion's r of line k.
+__device__ __forceinline__ int32_t line_r(const MisDev& d, int64_t k) {
+  int32_t u = d.uv[2 * k], v = d.uv[2 * k + 1];
+  return u > v ? u : v;
+}
 
 // Marks the line's write w: minw[w] and, over w's row, minwn.
 __device__ __forceinline__ void mark_write(const MisDev& d, int32_t w, int64_t k) {
@@ -503,6 +513,7 @@ __device__ __forceinline__ bool hop2_request(const MisDev& d, int64_t* count, in
     if (idx < d.h2_rcap) {
       d.h2_line[idx] = (int32_t)k;
       d.h2_seg[idx] = sg;
+      d.h2_rsb[idx] = ((int64_t)line_r(d, k) << 32) | (uint32_t)d.lsb[k];
       d.h2_best[idx] = seed;
       d.h2_flag[idx] = 0;
     }
@@ -562,12 +573,6 @@ __global__ void k_mm_check(MisDev d, SegView sv) {
   if (lane_id() == 0 && scanned) atomicAdd((unsigned long long*)&sv.t[kHEntries], (unsigned long long)scanned);
 }
 
-// The insertion's r of line k.
-__device__ __forceinline__ int32_t line_r(const MisDev& d, int64_t k) {
-  int32_t u = d.uv[2 * k], v = d.uv[2 * k + 1];
-  return u > v ? u : v;
-}
-
 // Hop-2 plan (one block): y-blocks of 256 entries of N(r) per request, prefix in h2_off; the
 // task counter starts from 0.
 __global__ void k_mm_hop2_plan(MisDev d, SegView sv, int pass) {
@@ -619,8 +624,9 @@ __global__ void k_mm_hop2_expand(MisDev d, SegView sv, int pass) {
     Row rr = row_of(d.g, r);
     int64_t yi = (b - d.h2_off[i]) * kBlock + threadIdx.x;
     int64_t nch = 0;
+    int32_t y = -1;
     if (yi < row_len(rr)) {
-      int32_t y = row_at(d.g, rr, yi);
+      y = row_at(d.g, rr, yi);
       if (y >= 0) nch = (row_len(row_of(d.g, y)) + kHop2Chunk - 1) / kHop2Chunk;
     }
     scan[threadIdx.x] = nch;
@@ -637,7 +643,10 @@ __global__ void k_mm_hop2_expand(MisDev d, SegView sv, int pass) {
     if (at + nch > d.h2_tcap || nch > 0x1ffff) {
       if (nch > 0) d.h2_flag[i] = 1;
     } else {
-      for (int64_t ch = 0; ch < nch; ch++) d.h2_task[at + ch] = ((uint64_t)i << 48) | ((uint64_t)yi << 17) | (uint64_t)ch;
+      for (int64_t ch = 0; ch < nch; ch++) {
+        d.h2_task[at + ch] = ((uint64_t)i << 48) | ((uint64_t)yi << 17) | (uint64_t)ch;
+        d.h2_y[at + ch] = y;
+      }
     }
     __syncthreads();
   }
@@ -647,43 +656,69 @@ __device__ __forceinline__ unsigned long long hop2_key(int64_t yi, int64_t pos, 
   return ((unsigned long long)yi << 33) | ((unsigned long long)pos << 2) | (unsigned long long)code;
 }
 
-// Hop-2 scan: wave per task, the candidates of one chunk of N(y) in order; the first decisive
-// one (accept, or a conflict in pass B) lowers the request's best key (y index, position).
+// Hop-2 scan: wave per task, the candidates of one chunk of N(y) in order, 256 at a time
+// (four per lane, loaded together); the first decisive one (accept, or a conflict in pass B)
+// lowers the request's best key (y index, position).
+//
+// A hop-2 candidate c with a member neighbour is rejected without the adjustments for r:
+// if c is not in N(r), r is not among its member neighbours; if it is, c was a hop-1
+// candidate, rejected there by the same reads (the search reached hop 2). The read is
+// settled unless an earlier pending line writes a neighbour of c (then a conflict, if c has
+// a row).
 __global__ void k_mm_hop2_scan(MisDev d, SegView sv, int pass) {
+  constexpr int kU = 4;
   int64_t ntask = min(sv.t[kHTasks], d.h2_tcap);
+  bool check = pass == 1;
   int64_t scanned = 0;
   for (int64_t t = global_wave(); t < ntask; t += total_waves()) {
     uint64_t task = d.h2_task[t];
+    int32_t y = d.h2_y[t];
     int64_t i = (int64_t)(task >> 48), yi = (int64_t)((task >> 17) & 0x7fffffffULL), ch = (int64_t)(task & 0x1ffffULL);
     if (d.h2_flag[i]) continue;
     int64_t c0 = ch * kHop2Chunk;
     unsigned long long* best = &d.h2_best[i];
     if (__shfl(__hip_atomic_load(best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), 0) <= hop2_key(yi, c0, 0)) continue;
     int64_t k = d.h2_line[i];
-    int32_t r = line_r(d, k);
-    int32_t y = row_at(d.g, row_of(d.g, r), yi);
+    int64_t rsb = d.h2_rsb[i];
     Row ry = row_of(d.g, y);
     int64_t c1 = min(c0 + kHop2Chunk, row_len(ry));
-    MisView s = view_of(d, k, pass == 1, &scanned);
-    s.dropped = r;
+    MisView s{d.g,     d.sorted, d.mis,      d.eps, d.emask, (int32_t)(uint32_t)rsb, d.mcount,
+              d.minw,  d.minwn,  (int32_t)k, check, (int32_t)(rsb >> 32),          &scanned};
     scanned += c1 - c0;
     bool done = false;
-    for (int64_t g0 = c0; g0 < c1 && !done; g0 += kWave) {
+    for (int64_t g0 = c0; g0 < c1 && !done; g0 += kU * kWave) {
       if (__shfl(__hip_atomic_load(best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), 0) <= hop2_key(yi, g0, 0)) break;
-      int64_t q = g0 + lane_id();
-      int32_t c = q < c1 ? row_at(d.g, ry, q) : -1;
-      int res = c >= 0 ? candidate_hop1(s, c, false) : 0;
-      unsigned long long pending = __ballot(res != 0);
-      while (pending) {
-        int j = __ffsll((long long)pending) - 1;
-        pending &= pending - 1;
-        int rj = __shfl(res, j);
-        int32_t cj = __shfl(c, j);
-        rj = resolve_candidate(s, rj, cj);
-        if (rj == -1 || rj == 1) {
-          if (lane_id() == 0) atomicMin(best, hop2_key(yi, g0 + j, rj == 1 ? 1 : 0));
-          done = true;
-          break;
+      int32_t cs[kU], mc[kU], mn[kU];
+      int res[kU];
+#pragma unroll
+      for (int u = 0; u < kU; u++) {
+        int64_t q = g0 + u * kWave + lane_id();
+        cs[u] = q < c1 ? row_at(d.g, ry, q) : -1;
+      }
+#pragma unroll
+      for (int u = 0; u < kU; u++) {
+        mc[u] = cs[u] >= 0 ? d.mcount[cs[u]] : 1;
+        mn[u] = (check && cs[u] >= 0) ? d.minwn[cs[u]] : kNoWriter;
+      }
+#pragma unroll
+      for (int u = 0; u < kU; u++) {
+        if (cs[u] < 0) res[u] = 0;
+        else if (mc[u] > 0) res[u] = mn[u] < k ? 5 : 0;
+        else res[u] = candidate_hop1(s, cs[u], false);
+      }
+      for (int u = 0; u < kU && !done; u++) {
+        unsigned long long pending = __ballot(res[u] != 0);
+        while (pending) {
+          int j = __ffsll((long long)pending) - 1;
+          pending &= pending - 1;
+          int rj = __shfl(res[u], j);
+          int32_t cj = __shfl(cs[u], j);
+          rj = resolve_candidate(s, rj, cj);
+          if (rj == -1 || rj == 1) {
+            if (lane_id() == 0) atomicMin(best, hop2_key(yi, g0 + u * kWave + j, rj == 1 ? 1 : 0));
+            done = true;
+            break;
+          }
         }
       }
     }
@@ -883,6 +918,8 @@ struct MisState {
   DevBuf<unsigned long long> h2_best;
   DevBuf<int64_t> h2_off;
   DevBuf<uint64_t> h2_task;
+  DevBuf<int32_t> h2_y;
+  DevBuf<int64_t> h2_rsb;
   DevBuf<unsigned long long> h2_keyA;
   int64_t h2_rcap = kHop2Requests, h2_tcap = kHop2Tasks;  // list capacities (fallback: serial)
   hipGraphExec_t exec = nullptr;
@@ -910,6 +947,8 @@ void mis_release(Ctx& c) {
   ms.h2_best.release();
   ms.h2_off.release();
   ms.h2_task.release();
+  ms.h2_y.release();
+  ms.h2_rsb.release();
   ms.h2_keyA.release();
   if (ms.exec) (void)hipGraphExecDestroy(ms.exec);
   delete c.mis_state;
@@ -987,7 +1026,7 @@ MisDev dev_of(Ctx& c, MisState& ms) {
   return MisDev{c.g.view(), c.g.sorted,   ms.uv2.p,     ms.ins2.p,    c.mis,        ms.lsb.p,
                 ms.eps.p,   ms.ecap - 1,  ms.mcount.p, c.mis_minw,   ms.minwn.p,   ms.w0.p,
                 ms.w1.p,    ms.h2_line.p, ms.h2_seg.p,  ms.h2_best.p, ms.h2_off.p,  ms.h2_flag.p,
-                ms.h2_task.p, ms.h2_keyA.p, ms.h2_rcap, ms.h2_tcap};
+                ms.h2_task.p, ms.h2_y.p,    ms.h2_rsb.p,  ms.h2_keyA.p, ms.h2_rcap,   ms.h2_tcap};
 }
 
 void mm_launch_round(Ctx& c, MisState& ms, SegView sv) {
@@ -1015,7 +1054,8 @@ int mm_graph(Ctx& c, MisState& ms, SegView sv) {
   std::vector<const void*> key = {ms.uv2.p, ms.ins2.p, sv.t, ms.w0.p, ms.w1.p, c.mis, ms.lsb.p, ms.eps.p,
                                   (const void*)(intptr_t)ms.ecap, c.mis_minw, ms.mcount.p, ms.minwn.p,
                                   (const void*)(intptr_t)c.g.sorted, ms.h2_line.p, ms.h2_seg.p, ms.h2_flag.p,
-                                  ms.h2_best.p, ms.h2_off.p, ms.h2_task.p, ms.h2_keyA.p, (const void*)(intptr_t)ms.h2_rcap,
+                                  ms.h2_best.p, ms.h2_off.p, ms.h2_task.p, ms.h2_y.p, ms.h2_rsb.p, ms.h2_keyA.p,
+                                  (const void*)(intptr_t)ms.h2_rcap,
                                   (const void*)(intptr_t)ms.h2_tcap, g.boff, g.bcol, g.aoff, g.alen, g.acol, (const void*)(intptr_t)g.n,
                                   (const void*)(intptr_t)sv.stride, (const void*)(intptr_t)ms.grid,
                                   (const void*)(intptr_t)ms.group_rounds};
@@ -1069,6 +1109,8 @@ int mis_repair_multi(Ctx& c, Channel& ch, int32_t ninst, const int64_t* off, int
   SCOUT_CHECK(ms.h2_best.reserve(ms.h2_rcap));
   SCOUT_CHECK(ms.h2_off.reserve(ms.h2_rcap + 1));
   SCOUT_CHECK(ms.h2_task.reserve(ms.h2_tcap));
+  SCOUT_CHECK(ms.h2_y.reserve(ms.h2_tcap));
+  SCOUT_CHECK(ms.h2_rsb.reserve(ms.h2_rcap));
   // endpoint set: at least 4 slots per key; grows by doubling (its size is in the graph key)
   if (ms.ecap < 8 * m || !ms.eps.p) {
     if (ms.ecap < 1024) ms.ecap = 1024;
