@@ -37,9 +37,12 @@
 //
 // Counts instead of scans: mcount[x] = members among x's neighbours, kept exact across
 // batches. "N(x) holds a member other than skip" is read from it in O(1) (less skip and the
-// dropped vertex when they are member neighbours). Such a read depends on every neighbour's
-// membership, so pass B treats it as a conflict when minwn[x] (the smallest pending writer of
-// a neighbour of x) is earlier. A 2-hop ball then costs one pass over N(x).
+// dropped vertex when they are member neighbours). Such a read depends on the neighbours'
+// memberships. Pending lines mark, per vertex, the smallest line that adds a member
+// neighbour (wn.x), the smallest that removes one (wn.y) and how many removals are pending
+// (wncnt). Pass B takes "no member" as a conflict if an earlier line adds a neighbour, and
+// "has a member" as a conflict only if an earlier line removes one and the pending removals
+// could leave none. A 2-hop ball then costs one pass over N(x).
 //
 // Hop-2 candidates N(y), y in N(r), are searched by chunk tasks over many waves; the first
 // decisive candidate in sequence order is the smallest (y position, entry position) key.
@@ -69,7 +72,8 @@ struct MisView {
   int32_t sb;                     // sub-batch of the line (its first line)
   const int32_t* mcount; // members among each vertex's neighbours (committed state)
   const int32_t* minw;   // smallest pending line writing the vertex
-  const int32_t* minwn;  // smallest pending line writing one of the vertex's neighbours
+  const int2* wn;        // smallest pending line adding (x) / removing (y) a member neighbour
+  const int32_t* wncnt;  // pending removals of member neighbours (any window line)
   int32_t line;          // index of the line being evaluated
   bool check;            // pass B: report reads of vertices written by an earlier line
   int32_t dropped;       // vertex whose membership reads as 0 (the insertion's r), or -1
@@ -104,23 +108,26 @@ __device__ __forceinline__ int32_t live_count(const MisView& s, int32_t x, int32
 
 // Does N(x) hold a member other than `skip`? The committed member-neighbour count of x,
 // less the entries of skip when it is a member, less those of the dropped vertex (a member
-// that reads as 0). Reads every neighbour's membership at once: a conflict if a pending
-// earlier line writes any of them (or skip or the dropped vertex).
+// that reads as 0). In check mode, a conflict if pending earlier lines could change the
+// answer (or write skip or the dropped vertex).
 __device__ __forceinline__ bool nbr_has_member(const MisView& s, int32_t x, int32_t skip, bool* conflict) {
-  if (s.check && s.minwn[x] < s.line) *conflict = true;
   int32_t c = s.mcount[x];
-  if (c == 0) return false;
+  int2 wn = s.check ? s.wn[x] : make_int2(kNoWriter, kNoWriter);
+  // removals an earlier line may make (all pending removals: an upper bound)
+  int32_t rem = (c > 0 && wn.y < s.line) ? s.wncnt[x] : 0;
   // a row holds a neighbour at most twice (an inserted copy next to a deleted base entry is
-  // not live), so counts above 2 stay positive without the adjacency tests
+  // not live), so counts above 2 + rem stay positive without the adjacency tests
   if (skip >= 0) {
     if (s.check && s.minw[skip] < s.line) *conflict = true;
-    if (c <= 2 && s.mis[skip]) c -= live_count(s, x, skip);
+    if (c > 0 && c <= 2 + rem && s.mis[skip]) c -= live_count(s, x, skip);
   }
-  if (s.dropped >= 0 && s.dropped != skip && c > 0) {
+  if (s.dropped >= 0 && s.dropped != skip) {
     if (s.check && s.minw[s.dropped] < s.line) *conflict = true;
-    if (c <= 2 && s.mis[s.dropped]) c -= live_count(s, x, s.dropped);
+    if (c > 0 && c <= 2 + rem && s.mis[s.dropped]) c -= live_count(s, x, s.dropped);
   }
-  return c > 0;
+  bool has = c > 0;
+  if (s.check && (has ? c <= rem : wn.x < s.line)) *conflict = true;
+  return has;
 }
 
 // Any member (other than skip) in the vertex's row: 2-hop ball for endpoints, N(x) for
@@ -459,7 +466,8 @@ struct MisDev {
   int64_t emask;
   int32_t* mcount;
   int32_t* minw;
-  int32_t* minwn;
+  int2* wn;
+  int32_t* wncnt;
   int32_t* w0s;
   int32_t* w1s;
   // hop-2 candidate searches of one pass: requests (line, segment), their best result, the
@@ -478,7 +486,7 @@ struct MisDev {
 
 __device__ __forceinline__ MisView view_of(const MisDev& d, int64_t k, bool check, int64_t* scanned) {
   return MisView{d.g,       d.sorted, d.mis,         d.eps,  d.emask, d.lsb[k], d.mcount,
-                 d.minw,    d.minwn,  (int32_t)k,    check,  -1,      scanned};
+                 d.minw,    d.wn,     d.wncnt,       (int32_t)k, check,  -1,       scanned};
 }
 
 constexpr int32_t kHop2Pending = -3;  // w1 of a line whose hop-2 search is still running
@@ -493,12 +501,20 @@ __device__ __forceinline__ int32_t line_r(const MisDev& d, int64_t k) {
   return u > v ? u : v;
 }
 
-// Marks the line's write w: minw[w] and, over w's row, minwn.
-__device__ __forceinline__ void mark_write(const MisDev& d, int32_t w, int64_t k) {
+// Marks the line's write w: minw[w] and, over w's row, the neighbour marks (a removal when
+// w leaves the set).
+__device__ __forceinline__ void mark_write(const MisDev& d, int32_t w, int64_t k, bool removal) {
   if (w < 0) return;
   if (lane_id() == 0) atomicMin(&d.minw[w], (int32_t)k);
   Row r = row_of(d.g, w);
-  wave_walk_wide<8>(d.g, r, 0, row_len(r), [&](int32_t z) { atomicMin(&d.minwn[z], (int32_t)k); });
+  if (removal) {
+    wave_walk_wide<8>(d.g, r, 0, row_len(r), [&](int32_t z) {
+      atomicMin(&d.wn[z].y, (int32_t)k);
+      atomicAdd(&d.wncnt[z], 1);
+    });
+  } else {
+    wave_walk_wide<8>(d.g, r, 0, row_len(r), [&](int32_t z) { atomicMin(&d.wn[z].x, (int32_t)k); });
+  }
 }
 
 // Records a deferred hop-2 search of line k (segment sg) in the pass's request list; false if
@@ -522,8 +538,8 @@ __device__ __forceinline__ bool hop2_request(const MisDev& d, int64_t* count, in
   return idx < d.h2_rcap;
 }
 
-// Pass A: wave per window line. Writes go to w0/w1; minw[w] = smallest writer of w and
-// minwn[z] = smallest writer of a neighbour of z. A deferred hop-2 search leaves w1 pending.
+// Pass A: wave per window line. Writes go to w0/w1 and are marked (minw[w] = smallest writer
+// of w, and the neighbour marks of w's row). A deferred hop-2 search leaves w1 pending.
 __global__ void k_mm_eval(MisDev d, SegView sv) {
   int64_t slots = sv.t[kHSlots];
   int64_t scanned = 0;
@@ -546,8 +562,8 @@ __global__ void k_mm_eval(MisDev d, SegView sv) {
       d.w0s[k] = w0;
       d.w1s[k] = w1;
     }
-    mark_write(d, w0, k);
-    mark_write(d, w1, k);
+    mark_write(d, w0, k, d.ins[k] != 0);
+    mark_write(d, w1, k, false);
   }
   if (lane_id() == 0 && scanned) atomicAdd((unsigned long long*)&sv.t[kHEntries], (unsigned long long)scanned);
 }
@@ -663,8 +679,8 @@ __device__ __forceinline__ unsigned long long hop2_key(int64_t yi, int64_t pos, 
 // A hop-2 candidate c with a member neighbour is rejected without the adjustments for r:
 // if c is not in N(r), r is not among its member neighbours; if it is, c was a hop-1
 // candidate, rejected there by the same reads (the search reached hop 2). The read is
-// settled unless an earlier pending line writes a neighbour of c (then a conflict, if c has
-// a row).
+// settled unless pending removals by earlier lines could leave c without member neighbours
+// (then a conflict, if c has a row).
 __global__ void k_mm_hop2_scan(MisDev d, SegView sv, int pass) {
   constexpr int kU = 4;
   int64_t ntask = min(sv.t[kHTasks], d.h2_tcap);
@@ -683,12 +699,13 @@ __global__ void k_mm_hop2_scan(MisDev d, SegView sv, int pass) {
     Row ry = row_of(d.g, y);
     int64_t c1 = min(c0 + kHop2Chunk, row_len(ry));
     MisView s{d.g,     d.sorted, d.mis,      d.eps, d.emask, (int32_t)(uint32_t)rsb, d.mcount,
-              d.minw,  d.minwn,  (int32_t)k, check, (int32_t)(rsb >> 32),          &scanned};
+              d.minw,  d.wn,     d.wncnt,    (int32_t)k, check, (int32_t)(rsb >> 32),  &scanned};
     scanned += c1 - c0;
     bool done = false;
     for (int64_t g0 = c0; g0 < c1 && !done; g0 += kU * kWave) {
       if (__shfl(__hip_atomic_load(best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), 0) <= hop2_key(yi, g0, 0)) break;
-      int32_t cs[kU], mc[kU], mn[kU];
+      int32_t cs[kU], mc[kU];
+      int2 wn[kU];
       int res[kU];
 #pragma unroll
       for (int u = 0; u < kU; u++) {
@@ -698,12 +715,12 @@ __global__ void k_mm_hop2_scan(MisDev d, SegView sv, int pass) {
 #pragma unroll
       for (int u = 0; u < kU; u++) {
         mc[u] = cs[u] >= 0 ? d.mcount[cs[u]] : 1;
-        mn[u] = (check && cs[u] >= 0) ? d.minwn[cs[u]] : kNoWriter;
+        wn[u] = (check && cs[u] >= 0) ? d.wn[cs[u]] : make_int2(kNoWriter, kNoWriter);
       }
 #pragma unroll
       for (int u = 0; u < kU; u++) {
         if (cs[u] < 0) res[u] = 0;
-        else if (mc[u] > 0) res[u] = mn[u] < k ? 5 : 0;
+        else if (mc[u] > 0) res[u] = (wn[u].y < k && mc[u] <= d.wncnt[cs[u]]) ? 5 : 0;
         else res[u] = candidate_hop1(s, cs[u], false);
       }
       for (int u = 0; u < kU && !done; u++) {
@@ -760,7 +777,7 @@ __global__ void k_mm_hop2_resume(MisDev d, SegView sv, int pass) {
         d.w1s[k] = w1;
         d.h2_keyA[k] = d.h2_flag[i] ? kNoResult : d.h2_best[i];
       }
-      mark_write(d, w1, k);
+      mark_write(d, w1, k, false);
     } else if (conflict && lane_id() == 0) {
       atomicMin((long long*)&sv.k0()[sg], (long long)k);
     }
@@ -795,9 +812,11 @@ __global__ void k_mm_commit(MisDev d, SegView sv) {
       int32_t w = j == 0 ? w0 : w1;
       if (w < 0) continue;
       int32_t delta = commit ? ((j == 0 && ins) ? -1 : 1) : 0;
+      bool removal = j == 0 && ins;
       Row r = row_of(d.g, w);
       wave_walk_wide<8>(d.g, r, 0, row_len(r), [&](int32_t z) {
-        d.minwn[z] = kNoWriter;
+        d.wn[z] = make_int2(kNoWriter, kNoWriter);
+        if (removal) d.wncnt[z] = 0;
         if (delta) atomicAdd(&d.mcount[z], delta);
       });
     }
@@ -912,7 +931,8 @@ struct MisState {
   bool use_graph = true;
   int group_rounds = 16;
   int64_t window0 = 256;
-  DevBuf<int32_t> mcount, minwn;
+  DevBuf<int32_t> mcount, wncnt;
+  DevBuf<int2> wn;
   int64_t mc_version = -1, mc_apply = -1, mc_gen = -1;  // what mcount was computed for
   DevBuf<int32_t> h2_line, h2_seg, h2_flag;
   DevBuf<unsigned long long> h2_best;
@@ -940,7 +960,8 @@ void mis_release(Ctx& c) {
   ms.dval.release();
   ms.seg.release();
   ms.mcount.release();
-  ms.minwn.release();
+  ms.wn.release();
+  ms.wncnt.release();
   ms.h2_line.release();
   ms.h2_seg.release();
   ms.h2_flag.release();
@@ -1024,7 +1045,7 @@ int64_t mis_cuts(const std::vector<int32_t>& uv, const std::vector<int64_t>& deg
 
 MisDev dev_of(Ctx& c, MisState& ms) {
   return MisDev{c.g.view(), c.g.sorted,   ms.uv2.p,     ms.ins2.p,    c.mis,        ms.lsb.p,
-                ms.eps.p,   ms.ecap - 1,  ms.mcount.p, c.mis_minw,   ms.minwn.p,   ms.w0.p,
+                ms.eps.p,   ms.ecap - 1,  ms.mcount.p, c.mis_minw,   ms.wn.p,      ms.wncnt.p,   ms.w0.p,
                 ms.w1.p,    ms.h2_line.p, ms.h2_seg.p,  ms.h2_best.p, ms.h2_off.p,  ms.h2_flag.p,
                 ms.h2_task.p, ms.h2_y.p,    ms.h2_rsb.p,  ms.h2_keyA.p, ms.h2_rcap,   ms.h2_tcap};
 }
@@ -1052,7 +1073,7 @@ void mm_launch_group(Ctx& c, MisState& ms, SegView sv) {
 int mm_graph(Ctx& c, MisState& ms, SegView sv) {
   GView g = c.g.view();
   std::vector<const void*> key = {ms.uv2.p, ms.ins2.p, sv.t, ms.w0.p, ms.w1.p, c.mis, ms.lsb.p, ms.eps.p,
-                                  (const void*)(intptr_t)ms.ecap, c.mis_minw, ms.mcount.p, ms.minwn.p,
+                                  (const void*)(intptr_t)ms.ecap, c.mis_minw, ms.mcount.p, ms.wn.p, ms.wncnt.p,
                                   (const void*)(intptr_t)c.g.sorted, ms.h2_line.p, ms.h2_seg.p, ms.h2_flag.p,
                                   ms.h2_best.p, ms.h2_off.p, ms.h2_task.p, ms.h2_y.p, ms.h2_rsb.p, ms.h2_keyA.p,
                                   (const void*)(intptr_t)ms.h2_rcap,
@@ -1125,9 +1146,13 @@ int mis_repair_multi(Ctx& c, Channel& ch, int32_t ninst, const int64_t* off, int
   // member counts for the graph as applied: one batch on from the last repair, or recomputed
   bool fresh = ms.mcount.cap >= (int64_t)n + 1 && ms.mc_version == c.mis_version && ms.mc_gen == c.g.base_gen;
   SCOUT_CHECK(ms.mcount.reserve((int64_t)n + 1));
-  if (!ms.minwn.p) {
-    SCOUT_CHECK(ms.minwn.reserve((int64_t)n + 1));
-    SCOUT_TRY(hipMemsetAsync(ms.minwn.p, 0x7f, ((size_t)n + 1) * 4, c.s));  // 0x7f7f7f7f > any line
+  if (ms.wn.cap < (int64_t)n + 1) {
+    ms.wn.release();
+    ms.wncnt.release();
+    SCOUT_CHECK(ms.wn.reserve((int64_t)n + 1));
+    SCOUT_CHECK(ms.wncnt.reserve((int64_t)n + 1));
+    SCOUT_TRY(hipMemsetAsync(ms.wn.p, 0x7f, (size_t)ms.wn.cap * 8, c.s));  // 0x7f7f7f7f > any line
+    SCOUT_TRY(hipMemsetAsync(ms.wncnt.p, 0, (size_t)ms.wncnt.cap * 4, c.s));
   }
   if (fresh && ms.mc_apply == c.apply_seq - 1) {
     k_mcount_adjust<<<grid_for(m), kBlock, 0, c.s>>>(ch.d_uv.p, ch.d_ins.p, ch.d_done.p, m, c.mis, ms.mcount.p);
