@@ -363,34 +363,44 @@ __device__ bool eval_line(MisView s, int32_t u, int32_t v, bool ins, int32_t* w0
   return true;
 }
 
-// Reference degree estimate per endpoint: base row length - |removed set| + |added list|
-// (HybridGraph::degree, core/FlashGraph.h:267-274); removed entries are the deleted base
-// entries plus the tombstones held in the inserted row. Computed once per distinct endpoint:
-// the wave that first stamps x with this repair's generation writes val[x].
-__global__ void k_mis_degrees(GView g, const int32_t* uv, int64_t m, int32_t* stamp, int32_t gen, int32_t* val) {
-  for (int64_t w = global_wave(); w < 2 * m; w += total_waves()) {
-    int32_t x = uv[w];
-    int32_t old = 0;
-    if (lane_id() == 0) old = atomicMax(&stamp[x], gen);
-    if (__shfl(old, 0) == gen) continue;
+// Distinct endpoints: the first slot to stamp x with this repair's generation appends x.
+__global__ void k_mis_deg_claim(const int32_t* uv, int64_t m, int32_t* stamp, int32_t gen, int32_t* list,
+                                int64_t* count) {
+  int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= 2 * m) return;
+  int32_t x = uv[w];
+  if (atomicMax(&stamp[x], gen) != gen) list[atomicAdd((unsigned long long*)count, 1ULL)] = x;
+}
+
+// Reference degree estimate per distinct endpoint: base row length - |removed set| + |added
+// list| (HybridGraph::degree, core/FlashGraph.h:267-274); removed entries are the deleted
+// base entries plus the tombstones held in the inserted row. Block per vertex, so a hub's row
+// is read by 256 threads.
+__global__ void k_mis_degrees(GView g, const int32_t* list, const int64_t* count, int32_t* val) {
+  __shared__ int32_t part[kBlock / kWave];
+  int64_t nx = *count;
+  for (int64_t i = blockIdx.x; i < nx; i += gridDim.x) {
+    int32_t x = list[i];
     Row row = row_of(g, x);
-    int64_t b0 = row.b0, b1 = row.b1, a0 = row.a0, a1 = row.a1;
-    int64_t removed = 0, added = 0;
-    for (int64_t k = b0 + lane_id(); k < b1; k += kWave) removed += (g.bcol[k] < 0);
-    for (int64_t k = a0 + lane_id(); k < a1; k += kWave) {
+    int32_t net = 0;  // added - removed
+    for (int64_t k = row.b0 + threadIdx.x; k < row.b1; k += blockDim.x) net -= (g.bcol[k] < 0);
+    for (int64_t k = row.a0 + threadIdx.x; k < row.a1; k += blockDim.x) {
       int32_t y = g.acol[k];
-      if (y >= 0) added++;
-      else if (y != HOLE) removed++;
+      net += y >= 0 ? 1 : (y != HOLE ? -1 : 0);
     }
-    for (int off = kWave / 2; off > 0; off >>= 1) {
-      removed += __shfl_xor(removed, off);
-      added += __shfl_xor(added, off);
+    net = (int32_t)wave_sum(net);
+    if (lane_id() == 0) part[threadIdx.x / kWave] = net;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int32_t t = 0;
+      for (int j = 0; j < (int)(blockDim.x / kWave); j++) t += part[j];
+      val[x] = (int32_t)(row.b1 - row.b0) + t;
     }
-    if (lane_id() == 0) val[x] = (int32_t)((b1 - b0) - removed + added);
+    __syncthreads();
   }
 }
 
-__global__ void k_mis_deg_gather(const int32_t* uv, int64_t m, const int32_t* val, int64_t* deg) {
+__global__ void k_mis_deg_gather(const int32_t* uv, int64_t m, const int32_t* val, int32_t* deg) {
   int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (w < 2 * m) deg[w] = val[uv[w]];
 }
@@ -917,16 +927,59 @@ __global__ void k_mm_round_end(SegView sv) {
 
 }  // namespace
 
+// Set of vertex ids for one sub-batch: open addressing with linear probing, at most a quarter
+// full (it doubles as it fills), emptied in O(size). Small sub-batches stay in cache.
+struct VertexSet {
+  std::vector<int32_t> slot;  // -1 = empty
+  std::vector<uint64_t> used; // occupied slots
+  uint64_t mask = 0;
+  VertexSet() : slot(1024, -1), mask(1023) {}
+  uint64_t find(int32_t x) const {
+    uint64_t h = ((uint64_t)(uint32_t)x * 0x9E3779B97F4A7C15ULL) >> 20;
+    for (h &= mask;; h = (h + 1) & mask)
+      if (slot[h] == -1 || slot[h] == x) return h;
+  }
+  bool has(int32_t x) const { return slot[find(x)] == x; }
+  void insert(int32_t x) {
+    uint64_t h = find(x);
+    if (slot[h] == x) return;
+    slot[h] = x;
+    used.push_back(h);
+    if (used.size() * 4 > slot.size()) grow();
+  }
+  void grow() {
+    std::vector<int32_t> keys;
+    keys.reserve(used.size());
+    for (uint64_t h : used) keys.push_back(slot[h]);
+    slot.assign(slot.size() * 2, -1);
+    mask = slot.size() - 1;
+    used.clear();
+    for (int32_t x : keys) {
+      uint64_t h = find(x);
+      slot[h] = x;
+      used.push_back(h);
+    }
+  }
+  void clear() {
+    for (uint64_t h : used) slot[h] = -1;
+    used.clear();
+  }
+};
+
 struct MisState {
   DevBuf<int32_t> w0, w1, uv2, lsb;
   DevBuf<uint8_t> ins2;
-  DevBuf<int64_t> deg, seg;
+  DevBuf<int32_t> deg;
+  DevBuf<int64_t> seg;
   DevBuf<int32_t> dstamp, dval;  // per-vertex degree stamps and values
+  DevBuf<int32_t> dlist;          // distinct endpoints of the batch
+  DevBuf<int64_t> dcount;         // their number
   int32_t dgen = 0;
   DevBuf<unsigned long long> eps;  // endpoint set
   int64_t ecap = 0;                // its slots (a power of two, at least 4 per key)
-  std::vector<int64_t> h_deg;
-  std::vector<int32_t> h_lsb;
+  PinBuf<int32_t> h_deg, h_uv, h_lsb;  // host copies for the sub-batch cuts
+  PinBuf<uint8_t> h_ins;
+  VertexSet uniq;
   unsigned grid = kListGrid;
   bool use_graph = true;
   int group_rounds = 16;
@@ -956,8 +1009,14 @@ void mis_release(Ctx& c) {
   ms.eps.release();
   ms.ins2.release();
   ms.deg.release();
+  ms.h_deg.release();
+  ms.h_uv.release();
+  ms.h_lsb.release();
+  ms.h_ins.release();
   ms.dstamp.release();
   ms.dval.release();
+  ms.dlist.release();
+  ms.dcount.release();
   ms.seg.release();
   ms.mcount.release();
   ms.wn.release();
@@ -978,63 +1037,26 @@ void mis_release(Ctx& c) {
 
 namespace {
 
-// Set of vertex ids, emptied in O(1) by a new stamp (open addressing, linear probing).
-struct StampSet {
-  std::vector<int32_t> key;
-  std::vector<uint32_t> stamp;
-  uint32_t now = 0;
-  uint64_t mask = 0;
-  void init(int64_t keys) {
-    uint64_t cap = 1024;
-    while (cap < (uint64_t)(4 * keys)) cap <<= 1;
-    if (cap > key.size()) {
-      key.assign(cap, 0);
-      stamp.assign(cap, 0);
-      now = 0;
-    }
-    mask = key.size() - 1;
-  }
-  void clear() {
-    if (++now == 0) {
-      std::fill(stamp.begin(), stamp.end(), 0);
-      now = 1;
-    }
-  }
-  // slot of x, or the empty slot where it would go
-  uint64_t find(int32_t x) const {
-    uint64_t h = ((uint64_t)(uint32_t)x * 0x9E3779B97F4A7C15ULL) >> 20;
-    for (h &= mask;; h = (h + 1) & mask)
-      if (stamp[h] != now || key[h] == x) return h;
-  }
-  bool has(int32_t x) const { return stamp[find(x)] == now; }
-  void insert(int32_t x) {
-    uint64_t h = find(x);
-    stamp[h] = now;
-    key[h] = x;
-  }
-};
-
 // Sub-batch cuts of lines [lo, hi) by the reference's estimate deg * 1000 per new endpoint
 // against max_csr, at most 2^20 lines each (src/mis/scout_mis.cu:519-542). lsb[k] = the
 // first line of k's sub-batch; returns the number of sub-batches.
-int64_t mis_cuts(const std::vector<int32_t>& uv, const std::vector<int64_t>& deg, int64_t lo, int64_t hi,
-                 int64_t max_csr, StampSet& uniq, std::vector<int32_t>& lsb) {
+int64_t mis_cuts(const int32_t* uv, const int32_t* deg, int64_t lo, int64_t hi, int64_t max_csr, VertexSet& uniq,
+                 int32_t* lsb) {
   const int64_t MAX_BATCH = 1048576;
-  uniq.init(2 * std::min(hi - lo, MAX_BATCH));
   int64_t idx = lo, subs = 0;
   while (idx < hi) {
     uniq.clear();
     int64_t cur = 0, se = 0, start = idx;
     while (idx < hi && se < MAX_BATCH) {
-      int32_t a = uv[(size_t)(2 * idx)], b = uv[(size_t)(2 * idx + 1)];
+      int32_t a = uv[2 * idx], b = uv[2 * idx + 1];
       int64_t est = 0;
-      if (!uniq.has(a)) est += deg[(size_t)(2 * idx)] * 1000;
-      if (!uniq.has(b)) est += deg[(size_t)(2 * idx + 1)] * 1000;
+      if (!uniq.has(a)) est += (int64_t)deg[2 * idx] * 1000;
+      if (!uniq.has(b)) est += (int64_t)deg[2 * idx + 1] * 1000;
       if (se > 0 && cur + est >= max_csr) break;
       cur += est;
       uniq.insert(a);
       uniq.insert(b);
-      lsb[(size_t)idx] = (int32_t)start;
+      lsb[idx] = (int32_t)start;
       se++;
       idx++;
     }
@@ -1171,23 +1193,29 @@ int mis_repair_multi(Ctx& c, Channel& ch, int32_t ninst, const int64_t* off, int
     ms.dgen = 0;
   }
   ms.dgen++;
-  k_mis_degrees<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), ch.d_uv.p, m, ms.dstamp.p, ms.dgen, ms.dval.p);
+  SCOUT_CHECK(ms.dlist.reserve(2 * m));
+  SCOUT_CHECK(ms.dcount.reserve(1));
+  SCOUT_TRY(hipMemsetAsync(ms.dcount.p, 0, 8, c.s));
+  k_mis_deg_claim<<<grid_for(2 * m), kBlock, 0, c.s>>>(ch.d_uv.p, m, ms.dstamp.p, ms.dgen, ms.dlist.p, ms.dcount.p);
+  k_mis_degrees<<<4 * kListGrid, kBlock, 0, c.s>>>(c.g.view(), ms.dlist.p, ms.dcount.p, ms.dval.p);
   k_mis_deg_gather<<<grid_for(2 * m), kBlock, 0, c.s>>>(ch.d_uv.p, m, ms.dval.p, ms.deg.p);
   SCOUT_TRY(hipMemcpyAsync(ms.uv2.p, ch.d_uv.p, (size_t)(2 * m) * 4, hipMemcpyDeviceToDevice, c.s));
   SCOUT_TRY(hipMemcpyAsync(ms.ins2.p, ch.d_ins.p, (size_t)m, hipMemcpyDeviceToDevice, c.s));
-  ms.h_deg.resize((size_t)(2 * m));
-  std::vector<int32_t> uv((size_t)(2 * m));
-  std::vector<uint8_t> ins((size_t)m);
-  SCOUT_TRY(hipMemcpyAsync(ms.h_deg.data(), ms.deg.p, (size_t)(2 * m) * 8, hipMemcpyDeviceToHost, c.s));
-  SCOUT_TRY(hipMemcpyAsync(uv.data(), ch.d_uv.p, (size_t)(2 * m) * 4, hipMemcpyDeviceToHost, c.s));
-  SCOUT_TRY(hipMemcpyAsync(ins.data(), ch.d_ins.p, (size_t)m, hipMemcpyDeviceToHost, c.s));
+  SCOUT_CHECK(ms.h_deg.reserve(2 * m));
+  SCOUT_CHECK(ms.h_uv.reserve(2 * m));
+  SCOUT_CHECK(ms.h_ins.reserve(m));
+  SCOUT_CHECK(ms.h_lsb.reserve(m));
+  SCOUT_TRY(hipMemcpyAsync(ms.h_deg.p, ms.deg.p, (size_t)(2 * m) * 4, hipMemcpyDeviceToHost, c.s));
+  SCOUT_TRY(hipMemcpyAsync(ms.h_uv.p, ch.d_uv.p, (size_t)(2 * m) * 4, hipMemcpyDeviceToHost, c.s));
+  SCOUT_TRY(hipMemcpyAsync(ms.h_ins.p, ch.d_ins.p, (size_t)m, hipMemcpyDeviceToHost, c.s));
   SCOUT_TRY(hipStreamSynchronize(c.s));
-  for (int64_t k = 0; k < m; k++) (ins[(size_t)k] ? st->insertions : st->deletions)++;
-  ms.h_lsb.resize((size_t)m);
-  StampSet uniq;
+  int64_t nins = 0;
+  for (int64_t k = 0; k < m; k++) nins += ms.h_ins.p[k] != 0;
+  st->insertions += nins;
+  st->deletions += m - nins;
   for (int32_t i = 0; i < ninst; i++)
-    st->sub_batches += mis_cuts(uv, ms.h_deg, off[i], off[i + 1], max_csr, uniq, ms.h_lsb);
-  SCOUT_TRY(hipMemcpyAsync(ms.lsb.p, ms.h_lsb.data(), (size_t)m * 4, hipMemcpyHostToDevice, c.s));
+    st->sub_batches += mis_cuts(ms.h_uv.p, ms.h_deg.p, off[i], off[i + 1], max_csr, ms.uniq, ms.h_lsb.p);
+  SCOUT_TRY(hipMemcpyAsync(ms.lsb.p, ms.h_lsb.p, (size_t)m * 4, hipMemcpyHostToDevice, c.s));
   SCOUT_TRY(hipMemsetAsync(ms.eps.p, 0xff, (size_t)ms.ecap * 8, c.s));
   k_mis_ep_insert<<<grid_for(2 * m), kBlock, 0, c.s>>>(ms.uv2.p, ms.lsb.p, m, ms.eps.p, ms.ecap - 1);
   SegView sv{ms.seg.p, stride};
