@@ -599,6 +599,17 @@ __global__ void k_mm_check(MisDev d, SegView sv) {
   if (lane_id() == 0 && scanned) atomicAdd((unsigned long long*)&sv.t[kHEntries], (unsigned long long)scanned);
 }
 
+// Positions of N(r) that request i searches. Pass B skips a request whose line is already
+// past its segment's first conflict, and stops at the y of the candidate pass A accepted.
+__device__ __forceinline__ int64_t hop2_y_limit(const MisDev& d, const SegView& sv, int64_t i, int pass) {
+  int64_t k = d.h2_line[i];
+  int64_t len = row_len(row_of(d.g, line_r(d, k)));
+  if (pass == 0) return len;
+  if (k >= sv.k0()[d.h2_seg[i]]) return 0;
+  unsigned long long seed = d.h2_best[i];
+  return seed == kNoResult ? len : min(len, (int64_t)(seed >> 33) + 1);
+}
+
 // Hop-2 plan (one block): y-blocks of 256 entries of N(r) per request, prefix in h2_off; the
 // task counter starts from 0.
 __global__ void k_mm_hop2_plan(MisDev d, SegView sv, int pass) {
@@ -614,7 +625,7 @@ __global__ void k_mm_hop2_plan(MisDev d, SegView sv, int pass) {
   for (int64_t base = 0; base < n; base += blockDim.x) {
     int64_t i = base + threadIdx.x;
     int64_t nb = 0;
-    if (i < n) nb = (row_len(row_of(d.g, line_r(d, d.h2_line[i]))) + kBlock - 1) / kBlock;
+    if (i < n) nb = (hop2_y_limit(d, sv, i, pass) + kBlock - 1) / kBlock;
     scan[threadIdx.x] = nb;
     __syncthreads();
     for (int off = 1; off < (int)blockDim.x; off <<= 1) {
@@ -632,7 +643,8 @@ __global__ void k_mm_hop2_plan(MisDev d, SegView sv, int pass) {
 }
 
 // Hop-2 expand: block per y-block; thread per y in N(r): one task per kHop2Chunk entries of
-// N(y). A request whose tasks do not fit is flagged and searched serially by the resume.
+// N(y) (none when y is a settled member). A request whose tasks do not fit is flagged and
+// searched serially by the resume.
 __global__ void k_mm_hop2_expand(MisDev d, SegView sv, int pass) {
   __shared__ int64_t scan[kBlock];
   __shared__ int64_t base_s;
@@ -651,9 +663,12 @@ __global__ void k_mm_hop2_expand(MisDev d, SegView sv, int pass) {
     int64_t yi = (b - d.h2_off[i]) * kBlock + threadIdx.x;
     int64_t nch = 0;
     int32_t y = -1;
-    if (yi < row_len(rr)) {
+    if (yi < hop2_y_limit(d, sv, i, pass)) {
       y = row_at(d.g, rr, yi);
-      if (y >= 0) nch = (row_len(row_of(d.g, y)) + kHop2Chunk - 1) / kHop2Chunk;
+      // a member y whose membership no earlier pending line changes gives every c in N(y) a
+      // settled member neighbour: all of N(y) is rejected without reading it
+      bool settled = y >= 0 && d.mis[y] && (pass == 0 || d.minw[y] >= d.h2_line[i]);
+      if (y >= 0 && !settled) nch = (row_len(row_of(d.g, y)) + kHop2Chunk - 1) / kHop2Chunk;
     }
     scan[threadIdx.x] = nch;
     __syncthreads();

@@ -28,34 +28,88 @@ __device__ __forceinline__ uint64_t mix(uint64_t k) {
   return k;
 }
 
-// Inserts the canonical pair into the toggled set; true if it was not there.
-__device__ __forceinline__ bool claim_pair(uint64_t* set, uint64_t mask, uint32_t u, uint32_t v) {
+// A draw proposes a canonical pair (smaller id first) in its own slot; kEmptyKey = no pair.
+// Generation is deterministic: the draws of one attempt propose in parallel, a pair goes to
+// the earliest draw that proposed it (and only if no earlier batch or attempt toggled it),
+// and accepted pairs keep draw order.
+__device__ __forceinline__ void propose(uint64_t* cand, int64_t i, uint32_t u, uint32_t v) {
   uint32_t a = u < v ? u : v, b = u < v ? v : u;
-  uint64_t key = ((uint64_t)a << 32) | b;
+  cand[i] = ((uint64_t)a << 32) | b;
+}
+
+// Toggled-pair set: open addressing over canonical keys; owner[h] = the smallest draw tag
+// that proposed the key (tags grow across attempts and batches).
+__global__ void k_gen_claim(const uint64_t* cand, int64_t draws, uint64_t* set, uint64_t* owner, uint64_t mask,
+                            uint64_t tag0, uint64_t* cslot) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= draws) return;
+  uint64_t key = cand[i];
+  if (key == kEmptyKey) return;
   uint64_t h = mix(key) & mask;
   for (uint64_t probe = 0; probe <= mask; probe++) {
     uint64_t prev = atomicCAS((unsigned long long*)&set[h], (unsigned long long)kEmptyKey, (unsigned long long)key);
-    if (prev == kEmptyKey) return true;
-    if (prev == key) return false;
+    if (prev == kEmptyKey || prev == key) {
+      atomicMin((unsigned long long*)&owner[h], (unsigned long long)(tag0 + (uint64_t)i));
+      cslot[i] = h;
+      return;
+    }
     h = (h + 1) & mask;
   }
-  return false;
+  cslot[i] = kEmptyKey;  // set full: the draw is dropped
 }
 
-__device__ __forceinline__ void emit(uint32_t u, uint32_t v, int32_t* out, int64_t* count, int64_t limit) {
-  int64_t pos = (int64_t)atomicAdd((unsigned long long*)count, 1ULL);
-  if (pos < limit) {
-    out[2 * pos] = (int32_t)(u < v ? u : v);
-    out[2 * pos + 1] = (int32_t)(u < v ? v : u);
-  }
+__global__ void k_gen_accept(const uint64_t* cand, const uint64_t* cslot, const uint64_t* owner, uint64_t tag0,
+                             int64_t draws, int32_t* acc) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= draws) return;
+  acc[i] = cand[i] != kEmptyKey && cslot[i] != kEmptyKey && owner[cslot[i]] == tag0 + (uint64_t)i;
+}
+
+// Accepted pairs in draw order, at most `room` of them; count = accepted pairs of the attempt.
+__global__ void k_gen_emit(const uint64_t* cand, const int32_t* acc, const int64_t* pos, int64_t draws, int32_t* out,
+                           int64_t room, int64_t* count) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= draws) return;
+  if (i == draws - 1) *count = pos[i] + acc[i];
+  if (!acc[i] || pos[i] >= room) return;
+  out[2 * pos[i]] = (int32_t)(cand[i] >> 32);
+  out[2 * pos[i] + 1] = (int32_t)(cand[i] & 0xffffffffULL);
+}
+
+// The inserted-row pool in logical order, independent of where rows sit in the pool: one key
+// (owner row << 32 | entry index in the row) per slot inside a row's span, ~0 for slack and
+// moved-away spans; sorted, the keys list every row's inserted entries, rows by id.
+__global__ void k_pool_keys(GView g, const int32_t* aown, int64_t A, uint64_t* keys) {
+  int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= A) return;
+  int32_t o = aown[p];
+  bool in = o >= 0 && o < g.n && p >= g.aoff[o] && p < g.aoff[o] + g.alen[o];
+  keys[p] = in ? (((uint64_t)(uint32_t)o << 32) | (uint64_t)(p - g.aoff[o])) : kEmptyKey;
+}
+
+__global__ void k_count_keys(const uint64_t* keys, int64_t A, int64_t* count) {
+  int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  bool last = p < A && keys[p] != kEmptyKey && (p + 1 == A || keys[p + 1] == kEmptyKey);
+  if (last) *count = p + 1;
+  if (p == 0 && keys[0] == kEmptyKey) *count = 0;
+}
+
+// Inserted entry j of the logical pool order: its row and value; false if not live.
+__device__ __forceinline__ bool pool_entry(const GView& g, const uint64_t* pidx, int64_t j, uint32_t* owner,
+                                           int32_t* w) {
+  uint64_t key = pidx[j];
+  int32_t o = (int32_t)(key >> 32);
+  *owner = (uint32_t)o;
+  *w = g.acol[g.aoff[o] + (int64_t)(key & 0xffffffffULL)];
+  return *w >= 0;
 }
 
 // Tree deletions: a uniform vertex with level > 0 and its first neighbour one level lower,
 // scanning its row cyclically from a random offset (thread per draw).
-__global__ void k_gen_tree(GView g, const int32_t* level, uint64_t key, uint64_t base, int64_t draws, uint64_t* set,
-                           uint64_t mask, int32_t* out, int64_t* count, int64_t limit) {
+__global__ void k_gen_tree(GView g, const int32_t* level, uint64_t key, uint64_t base, int64_t draws, uint64_t* cand) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= draws || *count >= limit) return;
+  if (i >= draws) return;
+  cand[i] = kEmptyKey;
   uint64_t x = draw(key, base + (uint64_t)i);
   uint32_t v = uniform_below(x, (uint32_t)g.n);
   int32_t lv = level[v];
@@ -69,21 +123,22 @@ __global__ void k_gen_tree(GView g, const int32_t* level, uint64_t key, uint64_t
     if (k >= len) k -= len;
     int32_t w = row_at(g, r, k);
     if (w >= 0 && level[w] == lv - 1) {
-      if (claim_pair(set, mask, v, (uint32_t)w)) emit(v, (uint32_t)w, out, count, limit);
+      propose(cand, i, v, (uint32_t)w);
       return;
     }
   }
 }
 
-// Uniform deletions: a uniform position over the base entries and the inserted-row pool; the
-// draw is kept if it holds a live entry of a row (rejection keeps the choice uniform over the
-// live adjacency entries).
-__global__ void k_gen_del(GView g, int64_t E, int64_t A, const int32_t* aown, uint64_t key, uint64_t base,
-                          int64_t draws, uint64_t* set, uint64_t mask, int32_t* out, int64_t* count, int64_t limit) {
+// Uniform deletions: a uniform position over the base entries and the inserted entries (in
+// logical pool order); the draw is kept if it holds a live entry (rejection keeps the choice
+// uniform over the live adjacency entries).
+__global__ void k_gen_del(GView g, int64_t E, int64_t V, const uint64_t* pidx, uint64_t key, uint64_t base,
+                          int64_t draws, uint64_t* cand) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= draws || *count >= limit) return;
+  if (i >= draws) return;
+  cand[i] = kEmptyKey;
   uint64_t x = draw(key, base + (uint64_t)i);
-  uint64_t total = (uint64_t)(E + A);
+  uint64_t total = (uint64_t)(E + V);
   uint64_t pos = __umul64hi(x, total);
   int32_t w;
   uint32_t u;
@@ -98,36 +153,31 @@ __global__ void k_gen_del(GView g, int64_t E, int64_t A, const int32_t* aown, ui
       else h = mid - 1;
     }
     u = (uint32_t)l;
-  } else {
-    int64_t p = (int64_t)pos - E;
-    w = g.acol[p];
-    int32_t o = aown[p];
-    if (w < 0 || o < 0 || o >= g.n) return;
-    if (p < g.aoff[o] || p >= g.aoff[o] + g.alen[o]) return;  // slack or a moved-away span
-    u = (uint32_t)o;
+  } else if (!pool_entry(g, pidx, (int64_t)pos - E, &u, &w)) {
+    return;
   }
   if (u == (uint32_t)w) return;
-  if (claim_pair(set, mask, u, (uint32_t)w)) emit(u, (uint32_t)w, out, count, limit);
+  propose(cand, i, u, (uint32_t)w);
 }
 
 // Uniform insertions of current non-edges.
-__global__ void k_gen_ins(GView g, bool sorted, uint64_t key, uint64_t base, int64_t draws, uint64_t* set,
-                          uint64_t mask, int32_t* out, int64_t* count, int64_t limit) {
+__global__ void k_gen_ins(GView g, bool sorted, uint64_t key, uint64_t base, int64_t draws, uint64_t* cand) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= draws || *count >= limit) return;
+  if (i >= draws) return;
+  cand[i] = kEmptyKey;
   uint64_t x = draw(key, base + (uint64_t)i);
   uint32_t u = uniform_below(x, (uint32_t)g.n), v = uniform_below(x << 32 | (x >> 32), (uint32_t)g.n);
   if (u == v) return;
   if (adjacent_now(pair_state(g, sorted, (int32_t)u, (int32_t)v))) return;
-  if (claim_pair(set, mask, u, v)) emit(u, v, out, count, limit);
+  propose(cand, i, u, v);
 }
 
 // A vertex drawn with probability proportional to its live degree + 1: a uniform position
-// over the base entries, the inserted-row pool and one slot per vertex; an entry position
-// gives its row's vertex if the entry is live, else the draw is rejected (false).
-__device__ __forceinline__ bool pick_degree_plus_one(const GView& g, int64_t E, int64_t A, const int32_t* aown,
+// over the base entries, the inserted entries (logical pool order) and one slot per vertex;
+// an entry position gives its row's vertex if the entry is live, else the draw is rejected.
+__device__ __forceinline__ bool pick_degree_plus_one(const GView& g, int64_t E, int64_t V, const uint64_t* pidx,
                                                      uint64_t x, uint32_t* out) {
-  uint64_t total = (uint64_t)(E + A + g.n);
+  uint64_t total = (uint64_t)(E + V + g.n);
   uint64_t pos = __umul64hi(x, total);
   if (pos < (uint64_t)E) {
     int64_t p = (int64_t)pos;
@@ -141,50 +191,45 @@ __device__ __forceinline__ bool pick_degree_plus_one(const GView& g, int64_t E, 
     *out = (uint32_t)l;
     return true;
   }
-  if (pos < (uint64_t)(E + A)) {
-    int64_t p = (int64_t)pos - E;
-    int32_t w = g.acol[p];
-    int32_t o = aown[p];
-    if (w < 0 || o < 0 || o >= g.n) return false;
-    if (p < g.aoff[o] || p >= g.aoff[o] + g.alen[o]) return false;
-    *out = (uint32_t)o;
-    return true;
+  if (pos < (uint64_t)(E + V)) {
+    int32_t w;
+    return pool_entry(g, pidx, (int64_t)pos - E, out, &w);
   }
-  *out = (uint32_t)(pos - (uint64_t)(E + A));
+  *out = (uint32_t)(pos - (uint64_t)(E + V));
   return true;
 }
 
 // Degree-biased deletions: u with probability proportional to degree + 1, then a uniform live
 // entry of u's row (rejection over its base and inserted entries).
-__global__ void k_gen_del_deg(GView g, int64_t E, int64_t A, const int32_t* aown, uint64_t key, uint64_t base,
-                              int64_t draws, uint64_t* set, uint64_t mask, int32_t* out, int64_t* count,
-                              int64_t limit) {
+__global__ void k_gen_del_deg(GView g, int64_t E, int64_t V, const uint64_t* pidx, uint64_t key, uint64_t base,
+                              int64_t draws, uint64_t* cand) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= draws || *count >= limit) return;
+  if (i >= draws) return;
+  cand[i] = kEmptyKey;
   uint64_t x = draw(key, base + 2 * (uint64_t)i);
   uint32_t u;
-  if (!pick_degree_plus_one(g, E, A, aown, x, &u)) return;
+  if (!pick_degree_plus_one(g, E, V, pidx, x, &u)) return;
   Row r = row_of(g, (int32_t)u);
   int64_t len = row_len(r);
   if (len == 0 || len > 0xffffffffLL) return;
   int32_t w = row_at(g, r, (int64_t)uniform_below(draw(key, base + 2 * (uint64_t)i + 1), (uint32_t)len));
   if (w < 0 || (uint32_t)w == u) return;
-  if (claim_pair(set, mask, u, (uint32_t)w)) emit(u, (uint32_t)w, out, count, limit);
+  propose(cand, i, u, (uint32_t)w);
 }
 
 // Degree-biased insertions: both endpoints with probability proportional to degree + 1, kept
 // if they are distinct and not adjacent.
-__global__ void k_gen_ins_deg(GView g, bool sorted, int64_t E, int64_t A, const int32_t* aown, uint64_t key,
-                              uint64_t base, int64_t draws, uint64_t* set, uint64_t mask, int32_t* out,
-                              int64_t* count, int64_t limit) {
+__global__ void k_gen_ins_deg(GView g, bool sorted, int64_t E, int64_t V, const uint64_t* pidx, uint64_t key,
+                              uint64_t base, int64_t draws, uint64_t* cand) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= draws || *count >= limit) return;
+  if (i >= draws) return;
+  cand[i] = kEmptyKey;
   uint32_t u, v;
-  if (!pick_degree_plus_one(g, E, A, aown, draw(key, base + 2 * (uint64_t)i), &u)) return;
-  if (!pick_degree_plus_one(g, E, A, aown, draw(key, base + 2 * (uint64_t)i + 1), &v)) return;
+  if (!pick_degree_plus_one(g, E, V, pidx, draw(key, base + 2 * (uint64_t)i), &u)) return;
+  if (!pick_degree_plus_one(g, E, V, pidx, draw(key, base + 2 * (uint64_t)i + 1), &v)) return;
   if (u == v) return;
   if (adjacent_now(pair_state(g, sorted, (int32_t)u, (int32_t)v))) return;
-  if (claim_pair(set, mask, u, v)) emit(u, v, out, count, limit);
+  propose(cand, i, u, v);
 }
 
 // Shuffle of a generated batch: key = (hash of the line index under the batch key) << 32 | index.
@@ -206,9 +251,14 @@ __global__ void k_gather_lines(const uint64_t* keys, int64_t m, const int32_t* i
 struct StreamState {
   int mode = SCOUT_GEN_UNIFORM;
   uint64_t batches = 0;  // batches generated so far
-  DevBuf<uint64_t> set;
+  DevBuf<uint64_t> set, owner;  // toggled pairs and the draw tag that owns each
   uint64_t mask = 0;
   uint64_t draws_used[3] = {0, 0, 0};
+  uint64_t next_tag = 0;           // draw tags, increasing over attempts and batches
+  DevBuf<uint64_t> cand, cslot;    // per draw: proposed pair, its set slot
+  DevBuf<int32_t> acc;             // per draw: accepted
+  DevBuf<int64_t> pos;             // per draw: position among the accepted
+  DevBuf<uint64_t> pidx;           // inserted entries in logical order
   std::vector<DevBuf<int32_t>> slots;  // stored batches (pairs) for replay
   std::vector<int64_t> slot_m;
 };
@@ -221,7 +271,9 @@ int gen_batch(Ctx& c, Channel& ch, int64_t size, double delete_frac, double tree
     uint64_t cap = 1024;
     while (cap < (uint64_t)(4 * std::max<int64_t>(expected_total, size))) cap <<= 1;
     SCOUT_CHECK(s.set.reserve((int64_t)cap));
+    SCOUT_CHECK(s.owner.reserve((int64_t)cap));
     SCOUT_TRY(hipMemsetAsync(s.set.p, 0xff, (size_t)s.set.cap * 8, c.s));
+    SCOUT_TRY(hipMemsetAsync(s.owner.p, 0xff, (size_t)s.owner.cap * 8, c.s));
     s.mask = 1;
     while (s.mask * 2 <= (uint64_t)s.set.cap) s.mask *= 2;
     s.mask -= 1;
@@ -236,6 +288,19 @@ int gen_batch(Ctx& c, Channel& ch, int64_t size, double delete_frac, double tree
   uint64_t keys[3] = {stream_key(seed, RNG_STREAM_TREE), stream_key(seed, RNG_STREAM_DEL), stream_key(seed, RNG_STREAM_INS)};
   int64_t want[3] = {ntree, nuni, nins};
   int64_t produced = 0;
+  // the inserted entries in logical order, for the draws over adjacency entries
+  int64_t V = 0;
+  if (c.g.A > 0) {
+    DevBuf<uint64_t> raw;
+    SCOUT_CHECK(raw.reserve(c.g.A));
+    SCOUT_CHECK(s.pidx.reserve(c.g.A));
+    k_pool_keys<<<grid_for(c.g.A), kBlock, 0, c.s>>>(c.g.view(), c.g.aown.p, c.g.A, raw.p);
+    SCOUT_CHECK(sort_keys_u64(c, raw.p, s.pidx.p, c.g.A, 0, 64));
+    k_count_keys<<<grid_for(c.g.A), kBlock, 0, c.s>>>(s.pidx.p, c.g.A, c.d_box);
+    SCOUT_CHECK(box_read(c, 1));
+    V = c.h_box[0];
+    raw.release();
+  }
   for (int kind = 0; kind < 3; kind++) {
     if (want[kind] == 0) continue;
     if (kind == 0 && !c.level) {
@@ -246,32 +311,38 @@ int gen_batch(Ctx& c, Channel& ch, int64_t size, double delete_frac, double tree
     int64_t got = 0;
     for (int attempt = 0; attempt < 64 && got < want[kind]; attempt++) {
       int64_t draws = std::max<int64_t>(2 * (want[kind] - got), 1024);
-      SCOUT_TRY(hipMemcpyAsync(c.d_box, &got, sizeof(int64_t), hipMemcpyHostToDevice, c.s));
+      SCOUT_CHECK(s.cand.reserve(draws));
+      SCOUT_CHECK(s.cslot.reserve(draws));
+      SCOUT_CHECK(s.acc.reserve(draws));
+      SCOUT_CHECK(s.pos.reserve(draws));
+      unsigned grid = grid_for(draws);
       if (kind == 0)
-        k_gen_tree<<<grid_for(draws), kBlock, 0, c.s>>>(c.g.view(), c.level, keys[0], s.draws_used[0], draws, s.set.p,
-                                                        s.mask, out, c.d_box, want[kind]);
+        k_gen_tree<<<grid, kBlock, 0, c.s>>>(c.g.view(), c.level, keys[0], s.draws_used[0], draws, s.cand.p);
       else if (kind == 1 && s.mode == SCOUT_GEN_DEGREE)
-        k_gen_del_deg<<<grid_for(draws), kBlock, 0, c.s>>>(c.g.view(), c.g.E, c.g.A, c.g.aown.p, keys[1],
-                                                           s.draws_used[1], draws, s.set.p, s.mask, out, c.d_box,
-                                                           want[kind]);
+        k_gen_del_deg<<<grid, kBlock, 0, c.s>>>(c.g.view(), c.g.E, V, s.pidx.p, keys[1], s.draws_used[1], draws,
+                                                s.cand.p);
       else if (kind == 1)
-        k_gen_del<<<grid_for(draws), kBlock, 0, c.s>>>(c.g.view(), c.g.E, c.g.A, c.g.aown.p, keys[1],
-                                                       s.draws_used[1], draws, s.set.p, s.mask, out, c.d_box,
-                                                       want[kind]);
+        k_gen_del<<<grid, kBlock, 0, c.s>>>(c.g.view(), c.g.E, V, s.pidx.p, keys[1], s.draws_used[1], draws,
+                                            s.cand.p);
       else if (s.mode == SCOUT_GEN_DEGREE)
-        k_gen_ins_deg<<<grid_for(draws), kBlock, 0, c.s>>>(c.g.view(), c.g.sorted, c.g.E, c.g.A, c.g.aown.p, keys[2],
-                                                           s.draws_used[2], draws, s.set.p, s.mask, out, c.d_box,
-                                                           want[kind]);
+        k_gen_ins_deg<<<grid, kBlock, 0, c.s>>>(c.g.view(), c.g.sorted, c.g.E, V, s.pidx.p, keys[2],
+                                                s.draws_used[2], draws, s.cand.p);
       else
-        k_gen_ins<<<grid_for(draws), kBlock, 0, c.s>>>(c.g.view(), c.g.sorted, keys[2], s.draws_used[2], draws, s.set.p,
-                                                       s.mask, out, c.d_box, want[kind]);
+        k_gen_ins<<<grid, kBlock, 0, c.s>>>(c.g.view(), c.g.sorted, keys[2], s.draws_used[2], draws, s.cand.p);
       // the degree-biased kernels use two draws per attempt
       s.draws_used[kind] += (uint64_t)draws * ((kind > 0 && s.mode == SCOUT_GEN_DEGREE) ? 2 : 1);
+      uint64_t tag0 = s.next_tag;
+      s.next_tag += (uint64_t)draws;
+      k_gen_claim<<<grid, kBlock, 0, c.s>>>(s.cand.p, draws, s.set.p, s.owner.p, s.mask, tag0, s.cslot.p);
+      k_gen_accept<<<grid, kBlock, 0, c.s>>>(s.cand.p, s.cslot.p, s.owner.p, tag0, draws, s.acc.p);
+      SCOUT_CHECK(scan_exclusive_i32_to_i64(c, s.acc.p, s.pos.p, draws));
+      k_gen_emit<<<grid, kBlock, 0, c.s>>>(s.cand.p, s.acc.p, s.pos.p, draws, out + 2 * got, want[kind] - got,
+                                           c.d_box);
       SCOUT_CHECK(box_read(c, 1));
-      got = std::min(c.h_box[0], want[kind]);
+      got += std::min(c.h_box[0], want[kind] - got);
     }
-    // A draw whose pair was claimed after the limit was reached stays claimed: such pairs are
-    // skipped by every later batch, which keeps the no-repeat rule.
+    // A pair accepted past the batch's room stays claimed: such pairs are skipped by every
+    // later batch, which keeps the no-repeat rule.
     produced += got;
   }
   if (s.mode == SCOUT_GEN_DEGREE && produced > 1) {
@@ -382,6 +453,12 @@ void stream_release(Ctx& c) {
   if (!c.stream) return;
   for (auto& b : c.stream->slots) b.release();
   c.stream->set.release();
+  c.stream->owner.release();
+  c.stream->cand.release();
+  c.stream->cslot.release();
+  c.stream->acc.release();
+  c.stream->pos.release();
+  c.stream->pidx.release();
   delete c.stream;
   c.stream = nullptr;
 }

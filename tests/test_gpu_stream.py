@@ -1,7 +1,8 @@
-"""GPU tests of the degree-biased stream generator (GEN_DEGREE): every line is a canonical
-u < v pair, no pair repeats within or across batches, deletions are current edges and
-insertions current non-edges (labelled by the oracle on the same graph), deletions and
-insertions are interleaved, and endpoints lean towards high degree."""
+"""GPU tests of the stream generators. Degree-biased streams (GEN_DEGREE): every line is a
+canonical u < v pair, no pair repeats within or across batches, deletions are current edges
+and insertions current non-edges (labelled by the oracle on the same graph), deletions and
+insertions are interleaved, and endpoints lean towards high degree. Both modes: the same
+graph and seed give the same batches, line for line."""
 import numpy as np
 import pytest
 
@@ -40,3 +41,29 @@ def test_degree_stream_is_valid_and_biased():
             og.apply(uv, ins)
         # endpoints proportional to degree + 1 have a mean degree far above the vertex mean
         assert min(ins_deg) > 3 * uni_deg
+
+
+@pytest.mark.parametrize("mode", ["uniform", "degree"])
+def test_streams_are_reproducible(mode):
+    """Two contexts with the same graph and seed generate the same batches, line for line
+    (the generator's draws run in parallel, but a pair goes to its earliest draw and accepted
+    pairs keep draw order)."""
+    print(f"stream: {mode} batches generated twice on R-MAT scale 12", flush=True)
+    runs = []
+    for _ in range(2):
+        with sa.Engine(0) as eng:
+            eng.gen_rmat(12, 16, seed=3)
+            eng.bfs_full(0)
+            if mode == "degree":
+                eng.gen_mode(sa.GEN_DEGREE)
+            out = []
+            for b in range(3):
+                m = eng.gen_batch(3000, 0.7, 0.5 if mode == "uniform" else 0.0, seed=9, expected_total=9000)
+                out.append(eng.batch_download(m).copy())
+                eng.classify()
+                eng.apply()
+                if mode == "uniform":
+                    eng.repair_bfs(sa.STRATEGY_LOCAL)
+            runs.append(out)
+    for a, b in zip(*runs):
+        assert a.shape == b.shape and np.array_equal(a, b)
