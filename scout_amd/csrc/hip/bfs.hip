@@ -662,7 +662,9 @@ int relax_levels(Ctx& c, int32_t* L, int32_t* P, int32_t lvl0, int32_t max_seed,
       return SCOUT_ERR_STATE;
     }
     if (c.h_box[8 + p] == 0 && (nseeds == 0 || lvl > max_seed)) break;
-    group = std::min(2 * group, kMaxGroup);
+    // after a predicted first group the repair is near its end: start again from a short group
+    group = group == first_group && first_group != kFirstGroup ? kFirstGroup : std::min(2 * group, kMaxGroup);
+    first_group = kFirstGroup;
   }
   // algorithmic bytes: per frontier vertex its id, level, the four row offsets and its queue
   // entry (40 B); per scanned entry the neighbour id and its level (8 B)
@@ -756,8 +758,10 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, int32_t* P, int strategy, scou
       return bfs_full_on(c, L, P, c.source, &st->rounds, &st->edges_scanned);
     }
     // rounds cmin .. max(predicted last level, deepest candidate): one read-back when the
-    // prediction holds; an empty extra round costs as much as a read-back, so no slack
-    predicted_last = std::max<int64_t>(c.h_box[25], cmax);
+    // prediction holds; an empty extra round costs as much as a read-back, so no slack. The
+    // prediction is a maximum over sampled candidates, which overshoots: it is lowered by the
+    // running mean of its overshoot on earlier batches.
+    predicted_last = std::max<int64_t>(c.h_box[25] - (int64_t)std::floor(c.pred_over), cmax);
     first_group = (int)std::min<int64_t>(std::max<int64_t>(predicted_last - cmin + 1, kFirstGroup), kMaxGroup);
   }
   if (ncand > 0) {
@@ -812,9 +816,16 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, int32_t* P, int strategy, scou
         return SCOUT_ERR_STATE;
       }
       if (c.h_box[8 + p] == 0 && lvl > cmax) break;
-      group = std::min(2 * group, kMaxGroup);
+      // after a predicted first group the repair is near its end: start again from a short group
+      group = group == first_group && first_group != kFirstGroup ? kFirstGroup : std::min(2 * group, kMaxGroup);
+      first_group = kFirstGroup;
     }
     rounds += c.h_box[20];
+    if (predicted_last >= 0) {
+      int64_t last = cmin + c.h_box[20] - 1;  // deepest invalidation level
+      double over = std::max<double>(0.0, (double)(c.h_box[25] - last));
+      c.pred_over = 0.75 * c.pred_over + 0.25 * over;
+    }
     // per candidate its id, level, row offsets and queue entry (40 B); per scanned entry the
     // neighbour and its level (8 B)
     prof_bytes(c, PROF_BFS_INVALIDATE, 40 * c.h_box[18] + 8 * c.h_box[11]);

@@ -217,6 +217,7 @@ struct Ctx {
   // last batch's kernel timings
   float last_repair_ms = 0.f;
   int64_t readbacks = 0;         // host read-backs of device counters so far
+  double pred_over = 0;          // running mean of how far the predicted last level overshoots
 };
 
 // prims.hip (library scans/sorts)
