@@ -80,9 +80,10 @@ def cpu_baseline(eng, n, batches_host, seconds):
 FAMILY_KERNELS = {
     "bfs_invalidate": ["k_cand_at_level", "k_invalidate_short", "k_invalidate", "k_long_tasks", "k_inval_search",
                        "k_inval_decide", "k_inval_push"],
-    "bfs_relax": ["k_seed_at_level", "k_relax", "k_relax_tasks"],
-    "cc_expand": ["k_iota", "k_cc_unreached", "k_cc_unreached_hook", "k_cc_hook_tasks", "k_cc_final"],
-    "apply": ["k_apply", "k_count_old", "k_count_new", "k_copy_old", "k_place_new"],
+    "bfs_relax": ["k_seed_at_level", "k_relax_short", "k_relax_rows"],
+    "cc_expand": ["k_zero_u32", "k_cc_mark", "k_cc_collect", "k_cc_union_list", "k_cc_unreached_hook",
+                  "k_cc_hook_tasks", "k_cc_final_list"],
+    "apply": ["k_apply", "k_group_heads", "k_append_groups", "k_move_rows"],
     "classify": ["k_dup_insert", "k_dup_mark", "k_classify"],
 }
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_rmat26.json")
@@ -97,11 +98,13 @@ def _kernel_short_name(name):
     return re.sub(r"\(.*", "", name.replace("scout::(anonymous namespace)::", ""))
 
 
-def summarize_pmc(pmc_dir):
+def summarize_pmc(pmc_dir, last_batches=0):
     """HBM bytes per batch of each kernel family from two rocprofv3 --pmc passes (FETCH_SIZE,
     WRITE_SIZE, KiB per dispatch) over a bench run, each scaled by what the same counter reads
     for a plain device copy of known size. Dispatches before the first batch (graph generation,
-    initial BFS/CC) are left out; every batch runs k_classify once."""
+    initial BFS/CC) are left out; every batch runs k_classify once. The per-dispatch figures of
+    the roofline kernels cover the last `last_batches` batches (the run's final, headline
+    replay), or every batch when 0."""
     import csv
 
     def rows(sub):
@@ -137,8 +140,13 @@ def summarize_pmc(pmc_dir):
         d = {}
         for sub, counter, key in (("bench_fetch", "FETCH_SIZE", "fetch_bytes"),
                                   ("bench_write", "WRITE_SIZE", "write_bytes")):
+            rs = rows(sub)
+            first = 0  # the dispatch that starts the counted batches
+            if last_batches > 0:
+                cls = [i for i, r in enumerate(rs) if _kernel_short_name(r["Kernel_Name"]) == "k_classify"]
+                first = cls[-last_batches] if len(cls) >= last_batches else 0
             started, vals = False, []
-            for r in rows(sub):
+            for r in rs[first:]:
                 name = _kernel_short_name(r["Kernel_Name"])
                 if name == "k_classify":
                     started = True
@@ -380,7 +388,8 @@ def main():
                     help="color-mis: launch the repair rounds kernel by kernel instead of replaying captured graphs")
     ap.add_argument("--summarize-pmc", metavar="DIR",
                     help="only summarise rocprofv3 --pmc passes under DIR (copy_fetch, copy_write, bench_fetch, "
-                         "bench_write) into per-batch HBM traffic per kernel family, print it and exit")
+                         "bench_write) into per-batch HBM traffic per kernel family, print it and exit; the "
+                         "per-dispatch figures cover the last --steps batches (the headline replay)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 5 plain device copies of MIB MiB each (reference kernel for the "
                          "FETCH_SIZE/WRITE_SIZE counter scale) and exit")
@@ -389,7 +398,7 @@ def main():
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported by this build", file=sys.stderr)
         sys.exit(2)
     if args.summarize_pmc:
-        print(json.dumps(summarize_pmc(args.summarize_pmc), indent=1))
+        print(json.dumps(summarize_pmc(args.summarize_pmc, args.steps), indent=1))
         return
     if args.workload == "color-mis":
         run_color_mis(args)

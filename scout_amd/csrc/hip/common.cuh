@@ -303,6 +303,38 @@ __device__ __forceinline__ void base_range(const GView& g, bool sorted, int32_t 
   hi = k;
 }
 
+// base_range through the entry samples when the row is long: the search first narrows to one
+// sample stride, so a hub row costs a few cached sample reads and one stride of the row. (Kept
+// apart from base_range, whose smaller code suits the kernels that inline it many times.)
+__device__ __forceinline__ void base_range_sampled(const GView& g, bool sorted, int32_t u, int32_t v, int64_t& lo,
+                                                   int64_t& hi) {
+  int64_t b = g.boff[u], e = g.boff[u + 1];
+  if (!sorted || !g.bsamp || e - b <= 4 * kSampleStride) {
+    base_range(g, sorted, u, v, lo, hi);
+    return;
+  }
+  // first sample position p = kSampleStride * k in [b, e) whose entry is >= v: the lower
+  // bound lies after the sample before it and at or before p (or after the last sample)
+  int64_t k0 = (b + kSampleStride - 1) / kSampleStride, k1 = (e - 1) / kSampleStride;
+  int64_t sl = k0, sh = k1 + 1;
+  while (sl < sh) {
+    int64_t mid = sl + ((sh - sl) >> 1);
+    if (g.bsamp[mid] < v) sl = mid + 1;
+    else sh = mid;
+  }
+  int64_t l = sl > k0 ? (sl - 1) * kSampleStride + 1 : b;
+  int64_t h = sl <= k1 ? sl * kSampleStride : e;
+  while (l < h) {
+    int64_t mid = l + ((h - l) >> 1);
+    if (decode(g.bcol[mid]) < v) l = mid + 1;
+    else h = mid;
+  }
+  lo = l;
+  int64_t k = l;
+  while (k < e && decode(g.bcol[k]) == v) k++;
+  hi = k;
+}
+
 struct PairState {
   bool base_live;     // base row u holds a live v
   bool base_tomb;     // base row u holds ~v
@@ -335,7 +367,7 @@ __device__ __forceinline__ PairState pair_state_in(const GView& g, int32_t u, in
 __device__ __forceinline__ PairState pair_state(const GView& g, bool sorted, int32_t u, int32_t v) {
   PairState s{false, false, false, false, false, 0, 0};
   int64_t lo, hi;
-  base_range(g, sorted, u, v, lo, hi);
+  base_range_sampled(g, sorted, u, v, lo, hi);
   s.lo = lo;
   s.hi = hi;
   for (int64_t k = lo; k < hi; k++) {

@@ -114,6 +114,12 @@ __global__ void k_dup_mark(const int32_t* uv, int64_t m, const uint64_t* hkey, c
 
 // ins[i] = 1 iff the pair is absent from the rule's graph. rule 0: current logical graph;
 // rule 1: the loaded base graph (decoded, i.e. before any update).
+// Every kSampleStride-th base entry, decoded (deleted entries keep their position and id).
+__global__ void k_base_samples(const int32_t* bcol, int64_t E, int32_t* samp) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k * kSampleStride < E) samp[k] = decode(bcol[k * kSampleStride]);
+}
+
 // pos (optional): the base range of row u found for each line, reused by k_apply.
 __global__ void k_classify(GView g, bool sorted, const int32_t* uv, int64_t m, int rule, uint8_t* ins,
                            int64_t* pos, int64_t* counts) {
@@ -578,6 +584,7 @@ int graph_free(Ctx& c) {
   g.ndst2.release();
   g.hkey.release();
   g.hline.release();
+  g.bsamp.release();
   g.n = 0;
   g.E = g.A = g.live = g.holes = 0;
   g.base_gen++;
@@ -660,11 +667,24 @@ int graph_upload(Ctx& c, int32_t n, int64_t E, const int64_t* off, const int32_t
   return graph_upload_egr(c, n, E, off, 1, col);
 }
 
+// Samples of the base entries for the searches of long sorted rows (taken again whenever the
+// base arrays change; a view without them searches the rows directly).
+int graph_samples(Ctx& c) {
+  DevGraph& g = c.g;
+  if (!g.sorted || g.E == 0 || g.bsamp_gen == g.base_gen) return SCOUT_OK;
+  int64_t ns = (g.E + kSampleStride - 1) / kSampleStride;
+  SCOUT_CHECK(g.bsamp.reserve(ns));
+  k_base_samples<<<grid_for(ns), kBlock, 0, c.s>>>(g.bcol, g.E, g.bsamp.p);
+  g.bsamp_gen = g.base_gen;
+  return SCOUT_OK;
+}
+
 int graph_classify_lines(Ctx& c, Channel& ch, int64_t lo, int64_t hi, int rule) {
   if (!c.g.boff) {
     set_error("classify: no graph loaded");
     return SCOUT_ERR_STATE;
   }
+  SCOUT_CHECK(graph_samples(c));
   int64_t m = hi - lo;
   SCOUT_CHECK(ch.d_pos.reserve(2 * ch.m));
   if (ch.pos_gen != c.g.base_gen || ch.pos_hi != lo) ch.pos_lo = lo;  // else extend the valid range

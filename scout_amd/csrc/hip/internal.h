@@ -106,7 +106,13 @@ struct GView {
   const int32_t* alen;
   const int32_t* acol;
   int32_t n;
+  const int32_t* bsamp;  // decoded bcol[kSampleStride * k], or nullptr (rows searched directly)
 };
+
+// Every kSampleStride-th base entry is sampled (decoded): a search in a long sorted row first
+// narrows to one stride of the row through the samples, which stay cached for the hub rows
+// many lines search.
+constexpr int64_t kSampleStride = 64;
 
 struct DevGraph {
   int32_t n = 0;
@@ -130,7 +136,12 @@ struct DevGraph {
   // duplicate-pair detection of the current batch
   DevBuf<uint64_t> hkey;
   DevBuf<int32_t> hline;
-  GView view() const { return GView{boff, bcol, aoff.p, alen.p, acol.p, n}; }
+  // base entry samples and the base generation they were taken from
+  DevBuf<int32_t> bsamp;
+  int64_t bsamp_gen = -1;
+  GView view() const {
+    return GView{boff, bcol, aoff.p, alen.p, acol.p, n, (sorted && bsamp_gen == base_gen) ? bsamp.p : nullptr};
+  }
 };
 
 // Per-batch edit staging: a channel holds one uploaded batch.

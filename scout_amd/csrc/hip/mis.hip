@@ -1114,7 +1114,7 @@ int mm_graph(Ctx& c, MisState& ms, SegView sv) {
                                   (const void*)(intptr_t)c.g.sorted, ms.h2_line.p, ms.h2_seg.p, ms.h2_flag.p,
                                   ms.h2_best.p, ms.h2_off.p, ms.h2_task.p, ms.h2_y.p, ms.h2_rsb.p, ms.h2_keyA.p,
                                   (const void*)(intptr_t)ms.h2_rcap,
-                                  (const void*)(intptr_t)ms.h2_tcap, g.boff, g.bcol, g.aoff, g.alen, g.acol, (const void*)(intptr_t)g.n,
+                                  (const void*)(intptr_t)ms.h2_tcap, g.boff, g.bcol, g.aoff, g.alen, g.acol, g.bsamp, (const void*)(intptr_t)g.n,
                                   (const void*)(intptr_t)sv.stride, (const void*)(intptr_t)ms.grid,
                                   (const void*)(intptr_t)ms.group_rounds};
   if (ms.exec && key == ms.key) return SCOUT_OK;
