@@ -248,7 +248,7 @@ def run_color_mis(args):
     for b in range(total):  # the stream depends only on the graph
         eng.gen_batch(bsize, 0.5, 0.0, args.seed, expected_total=bsize * total)
         eng.batch_store(b)
-        eng.classify()
+        eng.classify(download=False)
         eng.apply()
     eng.sync()
     log(f"stream of {total} batches generated in {time.perf_counter() - t0:.1f}s")
@@ -257,7 +257,7 @@ def run_color_mis(args):
         eng.restore()
         for b in range(args.warmup):
             eng.batch_load(b)
-            eng.classify()
+            eng.classify(download=False)
             eng.apply()
             sc = eng.repair_color()
             sm = eng.repair_mis(max_csr)
@@ -272,7 +272,7 @@ def run_color_mis(args):
         for b in range(args.warmup, total):
             ts = time.perf_counter()
             eng.batch_load(b)
-            eng.classify()
+            eng.classify(download=False)
             eng.apply()
             sc = eng.repair_color()
             sm = eng.repair_mis(max_csr)
@@ -316,7 +316,7 @@ def run_color_mis(args):
         eng.restore()
         for b in range(args.warmup):
             eng.batch_load(b)
-            eng.classify()
+            eng.classify(download=False)
             eng.apply()
             eng.repair_color()
             eng.repair_mis(max_csr)
@@ -440,7 +440,7 @@ def main():
         eng.batch_store(b)
         if b == ntrain + args.warmup and not args.no_cpu:
             first_batch_host = eng.batch_download(m)
-        eng.classify()
+        eng.classify(download=False)
         eng.apply()
         eng.repair_bfs(sa.STRATEGY_LOCAL)
         eng.repair_cc(sa.STRATEGY_LOCAL)
@@ -465,7 +465,7 @@ def main():
         eng.restore()
         for b in range(ntrain + args.warmup):
             eng.batch_load(b)
-            eng.classify()
+            eng.classify(download=False)
             eng.apply()
             eng.repair_bfs(sa.STRATEGY_LOCAL if b < ntrain else strategy)
             eng.repair_cc(sa.STRATEGY_LOCAL if b < ntrain else strategy)
@@ -481,7 +481,7 @@ def main():
         for b in range(ntrain + args.warmup, total):
             ts = time.perf_counter()
             eng.batch_load(b)
-            eng.classify()
+            eng.classify(download=False)
             eng.apply()
             sb = eng.repair_bfs(strategy)
             sc = eng.repair_cc(strategy)
@@ -563,7 +563,7 @@ def main():
         eng.restore()
         for b in range(ntrain + args.warmup):
             eng.batch_load(b)
-            eng.classify()
+            eng.classify(download=False)
             eng.apply()
             eng.repair_bfs(sa.STRATEGY_LOCAL)
             eng.repair_cc(sa.STRATEGY_LOCAL)

@@ -254,7 +254,12 @@ class Engine:
         self._check(self.lib.scout_stage_batch(self.h, channel, _ptr(a), a.size // 2), "stage_batch")
         self._staged[channel] = a.size // 2
 
-    def classify(self, rule: int = CLASSIFY_CURRENT, channel: int = 0) -> np.ndarray:
+    def classify(self, rule: int = CLASSIFY_CURRENT, channel: int = 0, download: bool = True):
+        """Labels the staged lines (1 = insertion). Returns the labels, or None with
+        download=False (they stay on the device for apply and the repairs)."""
+        if not download:
+            self._check(self.lib.scout_classify(self.h, channel, rule, None), "classify")
+            return None
         m = self._staged[channel]
         out = np.zeros(max(m, 1), dtype=np.uint8)
         self._check(self.lib.scout_classify(self.h, channel, rule, _ptr(out)), "classify")
