@@ -76,31 +76,24 @@ __global__ void k_gen_emit(const uint64_t* cand, const int32_t* acc, const int64
   out[2 * pos[i] + 1] = (int32_t)(cand[i] & 0xffffffffULL);
 }
 
-// The inserted-row pool in logical order, independent of where rows sit in the pool: one key
-// (owner row << 32 | entry index in the row) per slot inside a row's span, ~0 for slack and
-// moved-away spans; sorted, the keys list every row's inserted entries, rows by id.
-__global__ void k_pool_keys(GView g, const int32_t* aown, int64_t A, uint64_t* keys) {
-  int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= A) return;
-  int32_t o = aown[p];
-  bool in = o >= 0 && o < g.n && p >= g.aoff[o] && p < g.aoff[o] + g.alen[o];
-  keys[p] = in ? (((uint64_t)(uint32_t)o << 32) | (uint64_t)(p - g.aoff[o])) : kEmptyKey;
+// The inserted entries in logical order (rows by id, entries in row order), independent of
+// where rows sit in the pool: rowpre[u] = inserted-row slots of the rows before u
+// (rowpre[n] = all of them), so entry j lies in the last row u with rowpre[u] <= j.
+__global__ void k_rowpre_total(const int32_t* alen, int32_t n, int64_t* rowpre) {
+  rowpre[n] = n > 0 ? rowpre[n - 1] + alen[n - 1] : 0;
 }
 
-__global__ void k_count_keys(const uint64_t* keys, int64_t A, int64_t* count) {
-  int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  bool last = p < A && keys[p] != kEmptyKey && (p + 1 == A || keys[p + 1] == kEmptyKey);
-  if (last) *count = p + 1;
-  if (p == 0 && keys[0] == kEmptyKey) *count = 0;
-}
-
-// Inserted entry j of the logical pool order: its row and value; false if not live.
-__device__ __forceinline__ bool pool_entry(const GView& g, const uint64_t* pidx, int64_t j, uint32_t* owner,
+// Inserted entry j of the logical order: its row and value; false if not live.
+__device__ __forceinline__ bool pool_entry(const GView& g, const int64_t* rowpre, int64_t j, uint32_t* owner,
                                            int32_t* w) {
-  uint64_t key = pidx[j];
-  int32_t o = (int32_t)(key >> 32);
-  *owner = (uint32_t)o;
-  *w = g.acol[g.aoff[o] + (int64_t)(key & 0xffffffffULL)];
+  int64_t lo = 0, hi = g.n;  // last u with rowpre[u] <= j
+  while (lo < hi) {
+    int64_t mid = (lo + hi + 1) >> 1;
+    if (rowpre[mid] <= j) lo = mid;
+    else hi = mid - 1;
+  }
+  *owner = (uint32_t)lo;
+  *w = g.acol[g.aoff[lo] + (j - rowpre[lo])];
   return *w >= 0;
 }
 
@@ -132,7 +125,7 @@ __global__ void k_gen_tree(GView g, const int32_t* level, uint64_t key, uint64_t
 // Uniform deletions: a uniform position over the base entries and the inserted entries (in
 // logical pool order); the draw is kept if it holds a live entry (rejection keeps the choice
 // uniform over the live adjacency entries).
-__global__ void k_gen_del(GView g, int64_t E, int64_t V, const uint64_t* pidx, uint64_t key, uint64_t base,
+__global__ void k_gen_del(GView g, int64_t E, int64_t V, const int64_t* rowpre, uint64_t key, uint64_t base,
                           int64_t draws, uint64_t* cand) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= draws) return;
@@ -153,7 +146,7 @@ __global__ void k_gen_del(GView g, int64_t E, int64_t V, const uint64_t* pidx, u
       else h = mid - 1;
     }
     u = (uint32_t)l;
-  } else if (!pool_entry(g, pidx, (int64_t)pos - E, &u, &w)) {
+  } else if (!pool_entry(g, rowpre, (int64_t)pos - E, &u, &w)) {
     return;
   }
   if (u == (uint32_t)w) return;
@@ -175,7 +168,7 @@ __global__ void k_gen_ins(GView g, bool sorted, uint64_t key, uint64_t base, int
 // A vertex drawn with probability proportional to its live degree + 1: a uniform position
 // over the base entries, the inserted entries (logical pool order) and one slot per vertex;
 // an entry position gives its row's vertex if the entry is live, else the draw is rejected.
-__device__ __forceinline__ bool pick_degree_plus_one(const GView& g, int64_t E, int64_t V, const uint64_t* pidx,
+__device__ __forceinline__ bool pick_degree_plus_one(const GView& g, int64_t E, int64_t V, const int64_t* rowpre,
                                                      uint64_t x, uint32_t* out) {
   uint64_t total = (uint64_t)(E + V + g.n);
   uint64_t pos = __umul64hi(x, total);
@@ -193,7 +186,7 @@ __device__ __forceinline__ bool pick_degree_plus_one(const GView& g, int64_t E, 
   }
   if (pos < (uint64_t)(E + V)) {
     int32_t w;
-    return pool_entry(g, pidx, (int64_t)pos - E, out, &w);
+    return pool_entry(g, rowpre, (int64_t)pos - E, out, &w);
   }
   *out = (uint32_t)(pos - (uint64_t)(E + V));
   return true;
@@ -201,14 +194,14 @@ __device__ __forceinline__ bool pick_degree_plus_one(const GView& g, int64_t E, 
 
 // Degree-biased deletions: u with probability proportional to degree + 1, then a uniform live
 // entry of u's row (rejection over its base and inserted entries).
-__global__ void k_gen_del_deg(GView g, int64_t E, int64_t V, const uint64_t* pidx, uint64_t key, uint64_t base,
+__global__ void k_gen_del_deg(GView g, int64_t E, int64_t V, const int64_t* rowpre, uint64_t key, uint64_t base,
                               int64_t draws, uint64_t* cand) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= draws) return;
   cand[i] = kEmptyKey;
   uint64_t x = draw(key, base + 2 * (uint64_t)i);
   uint32_t u;
-  if (!pick_degree_plus_one(g, E, V, pidx, x, &u)) return;
+  if (!pick_degree_plus_one(g, E, V, rowpre, x, &u)) return;
   Row r = row_of(g, (int32_t)u);
   int64_t len = row_len(r);
   if (len == 0 || len > 0xffffffffLL) return;
@@ -219,14 +212,14 @@ __global__ void k_gen_del_deg(GView g, int64_t E, int64_t V, const uint64_t* pid
 
 // Degree-biased insertions: both endpoints with probability proportional to degree + 1, kept
 // if they are distinct and not adjacent.
-__global__ void k_gen_ins_deg(GView g, bool sorted, int64_t E, int64_t V, const uint64_t* pidx, uint64_t key,
+__global__ void k_gen_ins_deg(GView g, bool sorted, int64_t E, int64_t V, const int64_t* rowpre, uint64_t key,
                               uint64_t base, int64_t draws, uint64_t* cand) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= draws) return;
   cand[i] = kEmptyKey;
   uint32_t u, v;
-  if (!pick_degree_plus_one(g, E, V, pidx, draw(key, base + 2 * (uint64_t)i), &u)) return;
-  if (!pick_degree_plus_one(g, E, V, pidx, draw(key, base + 2 * (uint64_t)i + 1), &v)) return;
+  if (!pick_degree_plus_one(g, E, V, rowpre, draw(key, base + 2 * (uint64_t)i), &u)) return;
+  if (!pick_degree_plus_one(g, E, V, rowpre, draw(key, base + 2 * (uint64_t)i + 1), &v)) return;
   if (u == v) return;
   if (adjacent_now(pair_state(g, sorted, (int32_t)u, (int32_t)v))) return;
   propose(cand, i, u, v);
@@ -258,7 +251,7 @@ struct StreamState {
   DevBuf<uint64_t> cand, cslot;    // per draw: proposed pair, its set slot
   DevBuf<int32_t> acc;             // per draw: accepted
   DevBuf<int64_t> pos;             // per draw: position among the accepted
-  DevBuf<uint64_t> pidx;           // inserted entries in logical order
+  DevBuf<int64_t> rowpre;          // inserted-row slots before each row (n + 1)
   std::vector<DevBuf<int32_t>> slots;  // stored batches (pairs) for replay
   std::vector<int64_t> slot_m;
 };
@@ -290,16 +283,14 @@ int gen_batch(Ctx& c, Channel& ch, int64_t size, double delete_frac, double tree
   int64_t produced = 0;
   // the inserted entries in logical order, for the draws over adjacency entries
   int64_t V = 0;
+  SCOUT_CHECK(s.rowpre.reserve((int64_t)c.g.n + 1));
   if (c.g.A > 0) {
-    DevBuf<uint64_t> raw;
-    SCOUT_CHECK(raw.reserve(c.g.A));
-    SCOUT_CHECK(s.pidx.reserve(c.g.A));
-    k_pool_keys<<<grid_for(c.g.A), kBlock, 0, c.s>>>(c.g.view(), c.g.aown.p, c.g.A, raw.p);
-    SCOUT_CHECK(sort_keys_u64(c, raw.p, s.pidx.p, c.g.A, 0, 64));
-    k_count_keys<<<grid_for(c.g.A), kBlock, 0, c.s>>>(s.pidx.p, c.g.A, c.d_box);
-    SCOUT_CHECK(box_read(c, 1));
+    SCOUT_CHECK(scan_exclusive_i32_to_i64(c, c.g.alen.p, s.rowpre.p, c.g.n));
+    k_rowpre_total<<<1, 1, 0, c.s>>>(c.g.alen.p, c.g.n, s.rowpre.p);
+    SCOUT_TRY(hipMemcpyAsync(c.h_box, s.rowpre.p + c.g.n, sizeof(int64_t), hipMemcpyDeviceToHost, c.s));
+    SCOUT_TRY(hipStreamSynchronize(c.s));
+    c.readbacks++;
     V = c.h_box[0];
-    raw.release();
   }
   for (int kind = 0; kind < 3; kind++) {
     if (want[kind] == 0) continue;
@@ -319,13 +310,13 @@ int gen_batch(Ctx& c, Channel& ch, int64_t size, double delete_frac, double tree
       if (kind == 0)
         k_gen_tree<<<grid, kBlock, 0, c.s>>>(c.g.view(), c.level, keys[0], s.draws_used[0], draws, s.cand.p);
       else if (kind == 1 && s.mode == SCOUT_GEN_DEGREE)
-        k_gen_del_deg<<<grid, kBlock, 0, c.s>>>(c.g.view(), c.g.E, V, s.pidx.p, keys[1], s.draws_used[1], draws,
+        k_gen_del_deg<<<grid, kBlock, 0, c.s>>>(c.g.view(), c.g.E, V, s.rowpre.p, keys[1], s.draws_used[1], draws,
                                                 s.cand.p);
       else if (kind == 1)
-        k_gen_del<<<grid, kBlock, 0, c.s>>>(c.g.view(), c.g.E, V, s.pidx.p, keys[1], s.draws_used[1], draws,
+        k_gen_del<<<grid, kBlock, 0, c.s>>>(c.g.view(), c.g.E, V, s.rowpre.p, keys[1], s.draws_used[1], draws,
                                             s.cand.p);
       else if (s.mode == SCOUT_GEN_DEGREE)
-        k_gen_ins_deg<<<grid, kBlock, 0, c.s>>>(c.g.view(), c.g.sorted, c.g.E, V, s.pidx.p, keys[2],
+        k_gen_ins_deg<<<grid, kBlock, 0, c.s>>>(c.g.view(), c.g.sorted, c.g.E, V, s.rowpre.p, keys[2],
                                                 s.draws_used[2], draws, s.cand.p);
       else
         k_gen_ins<<<grid, kBlock, 0, c.s>>>(c.g.view(), c.g.sorted, keys[2], s.draws_used[2], draws, s.cand.p);
@@ -458,7 +449,7 @@ void stream_release(Ctx& c) {
   c.stream->cslot.release();
   c.stream->acc.release();
   c.stream->pos.release();
-  c.stream->pidx.release();
+  c.stream->rowpre.release();
   delete c.stream;
   c.stream = nullptr;
 }
