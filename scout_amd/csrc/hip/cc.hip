@@ -172,6 +172,7 @@ __global__ void k_zero_u32(uint32_t* p, int64_t count) {
 // Grid-stride pass over all vertices: newly reached vertices take label 0; unreached vertices
 // whose old component is marked, or that were in vertex 0's component (old label 0), are
 // listed for relabelling with par[v] = v. kept counts the roots of untouched components.
+// Four consecutive vertices per thread (one 16-byte load of levels and of labels).
 __global__ void k_cc_collect(const int32_t* L, int32_t* label, const uint32_t* bits, const uint32_t* coarse, int32_t n,
                              int32_t* par, int32_t* list, int64_t* count, int64_t cap, int64_t* kept) {
   int64_t roots = 0;
@@ -179,22 +180,37 @@ __global__ void k_cc_collect(const int32_t* L, int32_t* label, const uint32_t* b
   constexpr int kSeg = 512;
   __shared__ int32_t seg[kWavesPerBlock * kSeg];
   WaveStage<kSeg> ws{seg + (threadIdx.x / kWave) * kSeg, 0};
-  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n; base += (int64_t)gridDim.x * blockDim.x) {
-    int64_t v = base + threadIdx.x;
-    bool take = false;
-    if (v < n) {
-      int32_t l = label[v];
-      if (L[v] >= 0) {
-        if (l != 0) label[v] = 0;
-      } else if (l == 0 || (((coarse[l >> (5 + kCoarseShift)] >> ((l >> kCoarseShift) & 31)) & 1u) &&
-                            ((bits[l >> 5] >> (l & 31)) & 1u))) {
-        take = true;
-        par[v] = (int32_t)v;
+  int64_t n4 = ((int64_t)n + 3) / 4;
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n4; base += (int64_t)gridDim.x * blockDim.x) {
+    int64_t q = base + threadIdx.x;
+    int32_t ls[4] = {-1, -1, -1, -1}, lab[4] = {0, 0, 0, 0};
+    if (q < n4) {
+      if (4 * q + 4 <= n) {
+        int4 a = reinterpret_cast<const int4*>(L)[q], b = reinterpret_cast<const int4*>(label)[q];
+        ls[0] = a.x, ls[1] = a.y, ls[2] = a.z, ls[3] = a.w;
+        lab[0] = b.x, lab[1] = b.y, lab[2] = b.z, lab[3] = b.w;
       } else {
-        roots += l == v;
+        for (int k = 0; k < 4 && 4 * q + k < n; k++) ls[k] = L[4 * q + k], lab[k] = label[4 * q + k];
       }
     }
-    stage_push(ws, take, (int32_t)v, list, count, cap);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      int64_t v = 4 * q + k;
+      bool take = false;
+      if (q < n4 && v < n) {
+        int32_t l = lab[k];
+        if (ls[k] >= 0) {
+          if (l != 0) label[v] = 0;
+        } else if (l == 0 || (((coarse[l >> (5 + kCoarseShift)] >> ((l >> kCoarseShift) & 31)) & 1u) &&
+                              ((bits[l >> 5] >> (l & 31)) & 1u))) {
+          take = true;
+          par[v] = (int32_t)v;
+        } else {
+          roots += l == v;
+        }
+      }
+      stage_push(ws, take, (int32_t)v, list, count, cap);
+    }
   }
   stage_finish(ws, list, count, cap);
   block_add(roots, kept);
