@@ -273,20 +273,41 @@ __global__ void k_invalidate_short(GView g, int32_t* level, int32_t* parent, con
   block_add(work, scanned);
 }
 
-// Wave per medium candidate.
-// own[0] += entries read (a second pass over the row of an invalid candidate counts again),
+// Wave per medium candidate: a neighbour one level up supports it (and becomes its parent);
+// otherwise it is invalid and its children (neighbours one level down whose parent it is)
+// join the next level. The support search keeps the neighbours one level down in LDS, so an
+// invalid candidate then reads only their parents. own[0] += entries and parents read,
 // own[1] += rows.
 __global__ void k_invalidate(GView g, int32_t* level, int32_t* parent, const int32_t* cur, const int64_t* cur_count,
                              int32_t lvl, int32_t* inval, int64_t* inval_count, int64_t inval_cap, int32_t* next,
                              int64_t* next_count, int64_t cap, int32_t* stamp, int32_t epoch, int64_t* scanned,
                              int64_t* own, Attrib ab) {
+  // neighbours one level down met while searching for support: the only possible children
+  constexpr int kKids = 1024;
+  __shared__ int32_t kids_all[kWavesPerBlock * kKids];
+  int32_t* kids = kids_all + (threadIdx.x / kWave) * kKids;
   int64_t nc = *cur_count;
   int64_t work = 0, reads = 0, rows = 0;
   for (int64_t i = global_wave(); i < nc; i += total_waves()) {
     int32_t c = cur[i];
     Row r = row_of(g, c);
     int64_t len = row_len(r);
-    int64_t at = wave_find_first(g, r, 0, len, [&](int32_t x) { return level[x] == lvl - 1; });
+    int64_t at = -1, nk = 0;
+    for (int64_t base = 0; base < len; base += kWave) {
+      int64_t k = base + lane_id();
+      int32_t x = k < len ? row_at(g, r, k) : -1;
+      int32_t lx = x >= 0 ? level[x] : INT32_MIN;
+      unsigned long long sup = __ballot(x >= 0 && lx == lvl - 1);
+      if (sup) {
+        at = base + (__ffsll((long long)sup) - 1);
+        break;
+      }
+      bool kid = x >= 0 && lx == lvl + 1;
+      unsigned long long km = __ballot(kid);
+      int64_t slot = nk + __popcll(km & ((1ULL << lane_id()) - 1ULL));
+      if (kid && slot < kKids) kids[slot] = x;
+      nk += __popcll(km);
+    }
     rows++;
     if (at >= 0) {
       work += at + 1;
@@ -295,22 +316,35 @@ __global__ void k_invalidate(GView g, int32_t* level, int32_t* parent, const int
       continue;
     }
     work += len;
-    reads += 2 * len;
     if (lane_id() == 0) {
       level[c] = kInvalid;
       attrib_invalid(ab, c, lvl);
       thread_append(c, inval, inval_count, inval_cap);
     }
-    for (int64_t base = 0; base < len; base += kWave) {
-      int64_t k = base + lane_id();
-      bool take = false;
-      int32_t w = -1;
-      if (k < len) {
-        w = row_at(g, r, k);
-        take = w >= 0 && parent[w] == c && level[w] == lvl + 1 && atomicExch(&stamp[w], epoch) != epoch;
+    if (nk <= kKids) {
+      // children among the buffered neighbours (their levels were read above)
+      __threadfence_block();
+      reads += len + nk;
+      for (int64_t base = 0; base < nk; base += kWave) {
+        int64_t j = base + lane_id();
+        int32_t w = j < nk ? kids[j] : -1;
+        bool take = w >= 0 && parent[w] == c && atomicExch(&stamp[w], epoch) != epoch;
         if (take) attrib_child(ab, c, w);
+        wave_append(take, w, next, next_count, cap);
       }
-      wave_append(take, w, next, next_count, cap);
+    } else {
+      reads += 2 * len;
+      for (int64_t base = 0; base < len; base += kWave) {
+        int64_t k = base + lane_id();
+        bool take = false;
+        int32_t w = -1;
+        if (k < len) {
+          w = row_at(g, r, k);
+          take = w >= 0 && parent[w] == c && level[w] == lvl + 1 && atomicExch(&stamp[w], epoch) != epoch;
+          if (take) attrib_child(ab, c, w);
+        }
+        wave_append(take, w, next, next_count, cap);
+      }
     }
   }
   if (lane_id() == 0 && work) {
@@ -830,7 +864,7 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, int32_t* P, int strategy, scou
     // neighbour and its level (8 B)
     prof_bytes(c, PROF_BFS_INVALIDATE, 40 * c.h_box[18] + 8 * c.h_box[11]);
     // k_invalidate alone: per row its id, offsets, level and parent write (36 B); per entry
-    // read the neighbour and its level, or on the children pass its parent and level (8 B)
+    // read the neighbour and its level, per buffered child candidate its entry and parent (8 B)
     prof_bytes(c, PROF_BFS_INVALIDATE_KERNEL, 36 * c.h_box[22] + 8 * c.h_box[21]);
     nx = c.h_box[10];
     scanned += c.h_box[11];
