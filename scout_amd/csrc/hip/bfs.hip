@@ -259,13 +259,14 @@ __global__ void k_invalidate_short(GView g, int32_t* level, int32_t* parent, con
     wave_append(invalid, c, inval, inval_count, cap);
     wave_append(active && !is_short && len <= kChunk, c, medium, medium_count, cap);
     wave_append(active && len > kChunk, c, longs, long_count, cap);
-    // children of an invalid candidate: neighbours whose parent it is
+    // children of an invalid candidate: neighbours one level down (levels read above; no
+    // vertex of that level changes in this kernel) whose parent it is
     int32_t pw[kShort];
 #pragma unroll
-    for (int k = 0; k < kShort; k++) pw[k] = (invalid && xs[k] >= 0) ? parent[xs[k]] : -1;
+    for (int k = 0; k < kShort; k++) pw[k] = (invalid && xs[k] >= 0 && lv[k] == lvl + 1) ? parent[xs[k]] : -1;
 #pragma unroll
     for (int k = 0; k < kShort; k++) {
-      bool take = pw[k] == c && invalid && level[xs[k]] == lvl + 1 && atomicExch(&stamp[xs[k]], epoch) != epoch;
+      bool take = pw[k] == c && invalid && atomicExch(&stamp[xs[k]], epoch) != epoch;
       if (take) attrib_child(at, c, xs[k]);
       wave_append(take, xs[k], next, next_count, cap);
     }
