@@ -108,8 +108,11 @@ __device__ __forceinline__ void block_min_max(int64_t lo, int64_t hi, int64_t* o
       a = min(a, pmin[i]);
       b = max(b, pmax[i]);
     }
-    atomicMin((long long*)out_min, (long long)a);
-    atomicMax((long long*)out_max, (long long)b);
+    // only improvements go to the shared words (most blocks bring none once a few have)
+    if (a < __hip_atomic_load(out_min, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+      atomicMin((long long*)out_min, (long long)a);
+    if (b > __hip_atomic_load(out_max, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+      atomicMax((long long*)out_max, (long long)b);
   }
   __syncthreads();
 }
