@@ -45,6 +45,9 @@ constexpr int32_t kInvalid = -2;
 // repairs end within a few levels), later groups double up to kMaxGroup. A round whose list
 // is empty costs a few launches and no work.
 constexpr int kFirstGroup = 2;
+// Batch lines per thread in the per-line kernels: fewer blocks, so fewer block-wide
+// reductions and shared-counter atomics per batch.
+constexpr int kLinesPerThread = 4;
 constexpr int kMaxGroup = 16;
 // The learned scheduler switches a batch to a full recompute when the predicted cascades
 // cover more than this fraction of the vertices.
@@ -192,19 +195,41 @@ __global__ void k_relax_rows(GView g, int32_t* level, int32_t* parent, const int
 // box[0] = candidates, box[1] = min level, box[2] = max level, box[3] = deletion lines.
 __global__ void k_del_candidates(const int32_t* uv, const uint8_t* ins, int64_t m, const int32_t* level,
                                  const int32_t* parent, int32_t* cand, int64_t cap, int64_t* box, int32_t* root) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  bool del = i < m && !ins[i];
-  int32_t c = -1, lc = 0;
-  if (del) {
-    int32_t u = uv[2 * i], v = uv[2 * i + 1];
-    int32_t lu = level[u], lv = level[v];
-    if (lu >= 0 && lv == lu + 1 && parent[v] == u) c = v, lc = lv;
-    else if (lv >= 0 && lu == lv + 1 && parent[u] == v) c = u, lc = lu;
+  // kLinesPerThread lines per thread (each pass coalesced over the block), one append and one
+  // reduction per block
+  constexpr int K = kLinesPerThread;
+  int64_t first = (int64_t)blockIdx.x * blockDim.x * K + threadIdx.x;
+  bool take[K];
+  int32_t cs[K];
+  int64_t dels = 0, lmin = INT64_MAX, lmax = -1;
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    int64_t i = first + (int64_t)k * blockDim.x;
+    bool del = i < m && !ins[i];
+    int32_t c = -1, lc = 0;
+    if (del) {
+      int32_t u = uv[2 * i], v = uv[2 * i + 1];
+      int32_t lu = level[u], lv = level[v];
+      if (lu >= 0 && lv == lu + 1 && parent[v] == u) c = v, lc = lv;
+      else if (lv >= 0 && lu == lv + 1 && parent[u] == v) c = u, lc = lu;
+    }
+    take[k] = c >= 0;
+    cs[k] = c;
+    dels += del;
+    if (c >= 0) {
+      lmin = min(lmin, (int64_t)lc);
+      lmax = max(lmax, (int64_t)lc);
+    }
   }
-  int64_t pos = block_append(c >= 0, c, cand, &box[0], cap);
-  if (root && c >= 0) root[c] = pos < kMlpSample ? (int32_t)pos : -1;
-  block_add(del ? 1 : 0, &box[3]);
-  block_min_max(c >= 0 ? lc : INT64_MAX, c >= 0 ? lc : -1, &box[1], &box[2]);
+  int64_t pos[K];
+  block_append_k(take, cs, cand, &box[0], cap, pos);
+  if (root) {
+#pragma unroll
+    for (int k = 0; k < K; k++)
+      if (take[k]) root[cs[k]] = pos[k] >= 0 && pos[k] < kMlpSample ? (int32_t)pos[k] : -1;
+  }
+  block_add(dels, &box[3]);
+  block_min_max(lmin, lmax, &box[1], &box[2]);
 }
 
 // Candidates whose level is `lvl`, appended to the current list once (stamp dedup).
@@ -770,7 +795,8 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, int32_t* P, int strategy, scou
   Attrib ab{nullptr, nullptr, nullptr};
   if (logging) ab = Attrib{mlp_root(c), mlp_subtree(c), mlp_maxlvl(c)};
   if (m > 0)
-    k_del_candidates<<<grid_for(m), kBlock, 0, c.s>>>(ch.d_uv.p, ch.d_ins.p, m, L, P, c.la.p, c.la.cap, box, ab.root);
+    k_del_candidates<<<grid_for(m, kBlock * kLinesPerThread), kBlock, 0, c.s>>>(ch.d_uv.p, ch.d_ins.p, m, L, P,
+                                                                              c.la.p, c.la.cap, box, ab.root);
   // features and predictions are computed from the device-side counts and come back with the
   // same read-back as the candidate counts ([24] predicted cascade sizes, [25] last level)
   int64_t sample = logging ? kMlpSample : kMlpInfer;

@@ -90,6 +90,49 @@ __device__ __forceinline__ void block_add(int64_t v, int64_t* out) {
   __syncthreads();
 }
 
+// Block-wide append of up to K values per thread (value k kept if take[k]) with one atomic
+// per block; pos[k] = its position in out, or -1 (not kept, or past cap).
+template <int K, typename T>
+__device__ __forceinline__ void block_append_k(const bool (&take)[K], const T (&val)[K], T* out, int64_t* count,
+                                               int64_t cap, int64_t (&pos)[K]) {
+  __shared__ int64_t wsum[kWavesPerBlock];
+  __shared__ int64_t gbase;
+  int64_t n = 0;
+#pragma unroll
+  for (int k = 0; k < K; k++) n += take[k];
+  int64_t x = n;  // inclusive scan within the wave
+  for (int off = 1; off < kWave; off <<= 1) {
+    int64_t y = __shfl_up(x, off);
+    if (lane_id() >= off) x += y;
+  }
+  int w = (int)(threadIdx.x / kWave);
+  if (lane_id() == kWave - 1) wsum[w] = x;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int64_t t = 0;
+    for (int i = 0; i < kWavesPerBlock; i++) {
+      int64_t c = wsum[i];
+      wsum[i] = t;
+      t += c;
+    }
+    gbase = t ? (int64_t)atomicAdd((unsigned long long*)count, (unsigned long long)t) : 0;
+  }
+  __syncthreads();
+  int64_t p = gbase + wsum[w] + x - n;
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    pos[k] = -1;
+    if (take[k]) {
+      if (p < cap) {
+        out[p] = val[k];
+        pos[k] = p;
+      }
+      p++;
+    }
+  }
+  __syncthreads();
+}
+
 __device__ __forceinline__ void block_min_max(int64_t lo, int64_t hi, int64_t* out_min, int64_t* out_max) {
   __shared__ int64_t pmin[kWavesPerBlock], pmax[kWavesPerBlock];
   for (int off = kWave / 2; off > 0; off >>= 1) {
