@@ -572,25 +572,50 @@ __global__ void k_apply_tentative(const int32_t* xs, const uint64_t* tent, int64
 // pair repeats.
 __global__ void k_insert_seeds(GView g, bool sorted, bool distinct, const int32_t* uv, const uint8_t* ins, int64_t m,
                                int32_t* level, int64_t* seeds, int32_t* seed_par, int64_t cap, int64_t* box) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  bool is_ins = i < m && ins[i];
-  bool sv = false, su = false;
-  int32_t u = 0, v = 0, lu = -1, lv = -1;
-  if (is_ins) {
-    u = uv[2 * i];
-    v = uv[2 * i + 1];
-    lu = level[u];
-    lv = level[v];
-    bool maybe = (lu >= 0 && (lv < 0 || lu + 1 < lv)) || (lv >= 0 && (lu < 0 || lv + 1 < lu));
-    if (maybe && (distinct || adjacent_now(pair_state(g, sorted, u, v)))) {
-      if (lu >= 0 && cas_min_level(&level[v], lu + 1)) sv = true;
+  // kLinesPerThread lines per thread (each pass coalesced over the block): one append of the
+  // seeds and one reduction of their levels per block
+  constexpr int K = kLinesPerThread;
+  int64_t first = (int64_t)blockIdx.x * blockDim.x * K + threadIdx.x;
+  bool take[2 * K];
+  int64_t packed[2 * K];
+  int32_t par[2 * K];
+  int64_t nins = 0, lmin = INT64_MAX, lmax = -1;
+#pragma unroll
+  for (int q = 0; q < K; q++) {
+    int64_t i = first + (int64_t)q * blockDim.x;
+    bool is_ins = i < m && ins[i];
+    bool sv = false, su = false;
+    int32_t u = 0, v = 0, lu = -1, lv = -1;
+    if (is_ins) {
+      u = uv[2 * i];
+      v = uv[2 * i + 1];
+      lu = level[u];
       lv = level[v];
-      if (lv >= 0 && cas_min_level(&level[u], lv + 1)) su = true;
+      bool maybe = (lu >= 0 && (lv < 0 || lu + 1 < lv)) || (lv >= 0 && (lu < 0 || lv + 1 < lu));
+      if (maybe && (distinct || adjacent_now(pair_state(g, sorted, u, v)))) {
+        if (lu >= 0 && cas_min_level(&level[v], lu + 1)) sv = true;
+        lv = level[v];
+        if (lv >= 0 && cas_min_level(&level[u], lv + 1)) su = true;
+      }
     }
+    nins += is_ins;
+    // seed = (level << 32 | vertex) with the vertex that lowered it as its parent candidate
+    take[2 * q] = sv;
+    packed[2 * q] = ((int64_t)(lu + 1) << 32) | (int64_t)(uint32_t)v;
+    par[2 * q] = u;
+    take[2 * q + 1] = su;
+    packed[2 * q + 1] = ((int64_t)(lv + 1) << 32) | (int64_t)(uint32_t)u;
+    par[2 * q + 1] = v;
+    if (sv) lmin = min(lmin, (int64_t)(lu + 1)), lmax = max(lmax, (int64_t)(lu + 1));
+    if (su) lmin = min(lmin, (int64_t)(lv + 1)), lmax = max(lmax, (int64_t)(lv + 1));
   }
-  block_add(is_ins ? 1 : 0, &box[7]);
-  block_push_seed(sv, v, lu + 1, u, seeds, seed_par, box, cap);
-  block_push_seed(su, u, lv + 1, v, seeds, seed_par, box, cap);
+  int64_t at[2 * K];
+  block_append_k(take, packed, seeds, &box[4], cap, at);
+#pragma unroll
+  for (int q = 0; q < 2 * K; q++)
+    if (at[q] >= 0) seed_par[at[q]] = par[q];
+  block_add(nins, &box[7]);
+  block_min_max(lmin, lmax, &box[5], &box[6]);
 }
 
 // A seed that still holds the level it was lowered to takes the vertex that lowered it as
@@ -907,8 +932,8 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, int32_t* P, int strategy, scou
     }
   }
   if (m > 0)
-    k_insert_seeds<<<grid_for(m), kBlock, 0, c.s>>>(c.g.view(), c.g.sorted, ch.pairs_distinct, ch.d_uv.p, ch.d_ins.p,
-                                                    m, L, c.wa.p, c.lg.p, seed_cap, box);
+    k_insert_seeds<<<grid_for(m, kBlock * kLinesPerThread), kBlock, 0, c.s>>>(
+        c.g.view(), c.g.sorted, ch.pairs_distinct, ch.d_uv.p, ch.d_ins.p, m, L, c.wa.p, c.lg.p, seed_cap, box);
   SCOUT_CHECK(box_read(c, 14));
   if (nx > 0) scanned += c.h_box[11];
   if (nx > 0 && c.h_box[13] > c.wb.cap) {

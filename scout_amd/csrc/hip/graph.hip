@@ -122,9 +122,8 @@ __global__ void k_base_samples(const int32_t* bcol, int64_t E, int32_t* samp) {
 
 // pos (optional): the base range of row u found for each line, reused by k_apply.
 __global__ void k_classify(GView g, bool sorted, const int32_t* uv, int64_t m, int rule, uint8_t* ins,
-                           int64_t* pos, int64_t* counts) {
+                           int64_t* pos) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  bool insertion = false;
   if (i < m) {
     int32_t u = uv[2 * i], v = uv[2 * i + 1];
     PairState s = pair_state(g, sorted, u, v);
@@ -134,9 +133,7 @@ __global__ void k_classify(GView g, bool sorted, const int32_t* uv, int64_t m, i
     }
     bool adj = rule == SCOUT_CLASSIFY_BASE ? s.base_any : adjacent_now(s);
     ins[i] = adj ? 0 : 1;
-    insertion = !adj;
   }
-  block_add(insertion ? 1 : 0, &counts[0]);
 }
 
 // Flips the base entries of the pair in the range pair_state found (v <-> ~v).
@@ -164,59 +161,73 @@ struct NewEntry {
   int32_t src, dst;
 };
 
-// Appends the thread's new inserted-row entry (key src << 32 | line, value dst) with one
-// atomic per block.
-__device__ __forceinline__ void block_push_new(NewEntry e, int64_t line, uint64_t* nkey, int32_t* ndst,
-                                               int64_t* ncount, int64_t cap) {
-  uint64_t key = ((uint64_t)(uint32_t)e.src << 32) | (uint64_t)line;
-  int64_t pos = block_append(e.take, key, nkey, ncount, cap);
-  if (pos >= 0) ndst[pos] = e.dst;
-}
+// Batch lines per thread in k_apply: fewer blocks, so fewer block-wide appends.
+constexpr int kApplyLines = 4;
 
-// One thread per line of [lo, hi): the DeltaOverlay rules. done[i] = 1 if the line was applied.
-// box[0] = new inserted-row entries, box[1] = holes created.
+// The DeltaOverlay rules for the lines of [lo, hi). done[i] = 1 if the line was applied.
+// New inserted-row entries (key src << 32 | line, value dst) are appended with one atomic
+// per block. box[0] = new inserted-row entries, box[1] = holes created.
 // pos (optional): base ranges of row u located by k_classify (the base rows' entries do not
 // move within a batch; only their deletion marks flip).
 __global__ void k_apply(GView g, bool sorted, const int32_t* uv, const uint8_t* ins, const int64_t* pos, int64_t lo,
                         int64_t hi, int conditional, uint8_t* done, uint64_t* nkey, int32_t* ndst, int64_t ncap,
                         int64_t* box) {
-  int64_t i = lo + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  NewEntry e1{false, 0, 0}, e2{false, 0, 0};
+  // kApplyLines lines per thread (each pass coalesced over the block): one block-wide append
+  // of the new entries per block
+  constexpr int K = kApplyLines;
+  int64_t first = lo + (int64_t)blockIdx.x * blockDim.x * K + threadIdx.x;
+  bool take[2 * K];
+  uint64_t keys[2 * K];
+  int32_t dsts[2 * K];
   int64_t holes = 0;
-  if (i < hi) {
-    int32_t u = uv[2 * i], v = uv[2 * i + 1];
-    bool insertion = ins[i] != 0;
-    PairState su = pos ? pair_state_in(g, u, v, pos[2 * i], pos[2 * i + 1]) : pair_state(g, sorted, u, v);
-    bool apply = !conditional || insertion != adjacent_now(su);
-    done[i] = apply ? 1 : 0;
-    if (apply && insertion) {
-      // un_tombstone (FlashGraph.h:284-291), else append to both inserted rows.
-      if (su.base_tomb || su.added_tomb) {
-        if (su.base_tomb) flip_base(g, su, v, false);
-        if (su.added_tomb) holes += hole_first(g, u, ~v);
-        PairState sv = pair_state(g, sorted, v, u);
-        if (sv.base_tomb) flip_base(g, sv, u, false);
-        if (sv.added_tomb) holes += hole_first(g, v, ~u);
-      } else {
-        e1 = NewEntry{true, u, v};
-        e2 = NewEntry{true, v, u};
-      }
-    } else if (apply) {
-      // forget_delta_edge (FlashGraph.h:295-310), else tombstone both directions.
-      if (su.added_live) {
-        holes += hole_first(g, u, v);
-        holes += hole_first(g, v, u);
-      } else {
-        if (su.base_live) flip_base(g, su, v, true);
-        else if (!su.base_any && !su.added_tomb) e1 = NewEntry{true, u, ~v};
-        PairState sv = pair_state(g, sorted, v, u);
-        if (sv.base_live) flip_base(g, sv, u, true);
-        else if (!sv.base_any && !sv.added_tomb) e2 = NewEntry{true, v, ~u};
+#pragma unroll
+  for (int q = 0; q < K; q++) {
+    int64_t i = first + (int64_t)q * blockDim.x;
+    NewEntry e1{false, 0, 0}, e2{false, 0, 0};
+    if (i < hi) {
+      int32_t u = uv[2 * i], v = uv[2 * i + 1];
+      bool insertion = ins[i] != 0;
+      PairState su = pos ? pair_state_in(g, u, v, pos[2 * i], pos[2 * i + 1]) : pair_state(g, sorted, u, v);
+      bool apply = !conditional || insertion != adjacent_now(su);
+      done[i] = apply ? 1 : 0;
+      if (apply && insertion) {
+        // un_tombstone (FlashGraph.h:284-291), else append to both inserted rows.
+        if (su.base_tomb || su.added_tomb) {
+          if (su.base_tomb) flip_base(g, su, v, false);
+          if (su.added_tomb) holes += hole_first(g, u, ~v);
+          PairState sv = pair_state(g, sorted, v, u);
+          if (sv.base_tomb) flip_base(g, sv, u, false);
+          if (sv.added_tomb) holes += hole_first(g, v, ~u);
+        } else {
+          e1 = NewEntry{true, u, v};
+          e2 = NewEntry{true, v, u};
+        }
+      } else if (apply) {
+        // forget_delta_edge (FlashGraph.h:295-310), else tombstone both directions.
+        if (su.added_live) {
+          holes += hole_first(g, u, v);
+          holes += hole_first(g, v, u);
+        } else {
+          if (su.base_live) flip_base(g, su, v, true);
+          else if (!su.base_any && !su.added_tomb) e1 = NewEntry{true, u, ~v};
+          PairState sv = pair_state(g, sorted, v, u);
+          if (sv.base_live) flip_base(g, sv, u, true);
+          else if (!sv.base_any && !sv.added_tomb) e2 = NewEntry{true, v, ~u};
+        }
       }
     }
+    take[2 * q] = e1.take;
+    keys[2 * q] = ((uint64_t)(uint32_t)e1.src << 32) | (uint64_t)i;
+    dsts[2 * q] = e1.dst;
+    take[2 * q + 1] = e2.take;
+    keys[2 * q + 1] = ((uint64_t)(uint32_t)e2.src << 32) | (uint64_t)i;
+    dsts[2 * q + 1] = e2.dst;
   }
-  block_push_new(e1, i, nkey, ndst, &box[0], ncap);
-  block_push_new(e2, i, nkey, ndst, &box[0], ncap);
+  int64_t at[2 * K];
+  block_append_k(take, keys, nkey, &box[0], ncap, at);
+#pragma unroll
+  for (int q = 0; q < 2 * K; q++)
+    if (at[q] >= 0) ndst[at[q]] = dsts[q];
   block_add(holes, &box[1]);
 }
 
@@ -541,8 +552,9 @@ int apply_range(Ctx& c, Channel& ch, int64_t lo, int64_t hi, int conditional) {
   SCOUT_CHECK(prof_begin(c, PROF_APPLY));
   const int64_t* pos = (ch.pos_gen == g.base_gen && ch.pos_lo <= lo && hi <= ch.pos_hi) ? ch.d_pos.p : nullptr;
   if (hi > lo)
-    k_apply<<<grid_for(hi - lo), kBlock, 0, c.s>>>(g.view(), g.sorted, ch.d_uv.p, ch.d_ins.p, pos, lo, hi,
-                                                   conditional, ch.d_done.p, g.nkey.p, g.ndst.p, g.nkey.cap, c.d_box);
+    k_apply<<<grid_for(hi - lo, kBlock * kApplyLines), kBlock, 0, c.s>>>(g.view(), g.sorted, ch.d_uv.p, ch.d_ins.p,
+                                                                         pos, lo, hi, conditional, ch.d_done.p,
+                                                                         g.nkey.p, g.ndst.p, g.nkey.cap, c.d_box);
   // algorithmic bytes per line: the pair and its label and done flag (10 B), both rows'
   // base and inserted offsets (32 B) and the entries flipped or appended (8 B)
   SCOUT_CHECK(prof_end(c, PROF_APPLY, 50 * (hi - lo)));
@@ -690,11 +702,10 @@ int graph_classify_lines(Ctx& c, Channel& ch, int64_t lo, int64_t hi, int rule) 
   if (ch.pos_gen != c.g.base_gen || ch.pos_hi != lo) ch.pos_lo = lo;  // else extend the valid range
   ch.pos_hi = hi;
   ch.pos_gen = c.g.base_gen;
-  SCOUT_TRY(hipMemsetAsync(c.d_box, 0, sizeof(int64_t), c.s));
   SCOUT_CHECK(prof_begin(c, PROF_CLASSIFY));
   if (m > 0)
     k_classify<<<grid_for(m), kBlock, 0, c.s>>>(c.g.view(), c.g.sorted, ch.d_uv.p + 2 * lo, m, rule, ch.d_ins.p + lo,
-                                                ch.d_pos.p + 2 * lo, c.d_box);
+                                                ch.d_pos.p + 2 * lo);
   // per line: the pair (8 B), row u's base and inserted offsets (16 B), the located entry
   // (4 B) and the label written (1 B)
   SCOUT_CHECK(prof_end(c, PROF_CLASSIFY, 29 * m));
