@@ -235,10 +235,18 @@ __global__ void k_del_candidates(const int32_t* uv, const uint8_t* ins, int64_t 
 // Candidates whose level is `lvl`, appended to the current list once (stamp dedup).
 __global__ void k_cand_at_level(const int32_t* cand, int64_t nc, const int32_t* level, int32_t lvl, int32_t* cur,
                                 int64_t* cur_count, int64_t cap, int32_t* stamp, int32_t epoch) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  int32_t c = i < nc ? cand[i] : -1;
-  bool take = c >= 0 && level[c] == lvl && atomicExch(&stamp[c], epoch) != epoch;
-  block_append(take, c, cur, cur_count, cap);
+  constexpr int K = kLinesPerThread;
+  int64_t first = (int64_t)blockIdx.x * blockDim.x * K + threadIdx.x;
+  bool take[K];
+  int32_t cs[K];
+#pragma unroll
+  for (int q = 0; q < K; q++) {
+    int64_t i = first + (int64_t)q * blockDim.x;
+    cs[q] = i < nc ? cand[i] : -1;
+    take[q] = cs[q] >= 0 && level[cs[q]] == lvl && atomicExch(&stamp[cs[q]], epoch) != epoch;
+  }
+  int64_t at[K];
+  block_append_k(take, cs, cur, cur_count, cap, at);
 }
 
 // Invalidation of the candidates at level lvl. Thread per candidate; rows longer than kShort
@@ -539,31 +547,39 @@ __global__ void k_tentative_rows(GView g, const int32_t* level, const int32_t* x
   if (lane_id() == 0 && work) atomicAdd((unsigned long long*)scanned, (unsigned long long)work);
 }
 
-// Seeds are packed level << 32 | vertex, with the vertex that lowered them in seed_par.
-// box[4] = seeds, box[5] = min level, box[6] = max.
-__device__ __forceinline__ void block_push_seed(bool take, int32_t v, int32_t lv, int32_t par, int64_t* seeds,
-                                                int32_t* seed_par, int64_t* box, int64_t cap) {
-  int64_t packed = ((int64_t)lv << 32) | (int64_t)(uint32_t)v;
-  int64_t pos = block_append(take, packed, seeds, &box[4], cap);
-  if (pos >= 0) seed_par[pos] = par;
-  block_min_max(take ? lv : INT64_MAX, take ? lv : -1, &box[5], &box[6]);
-}
-
 __global__ void k_apply_tentative(const int32_t* xs, const uint64_t* tent, int64_t nx, int32_t* level,
                                   int32_t* parent, int64_t* seeds, int32_t* seed_par, int64_t cap, int64_t* box) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  int32_t t = -1, x = 0, p = -1;
-  if (i < nx) {
-    uint64_t best = tent[i];
-    if (best != UINT64_MAX) {
-      t = (int32_t)(best >> 32) + 1;
-      p = (int32_t)(best & 0xffffffffu);
+  constexpr int K = kLinesPerThread;
+  int64_t first = (int64_t)blockIdx.x * blockDim.x * K + threadIdx.x;
+  bool take[K];
+  int64_t packed[K];
+  int32_t par[K];
+  int64_t lmin = INT64_MAX, lmax = -1;
+#pragma unroll
+  for (int q = 0; q < K; q++) {
+    int64_t i = first + (int64_t)q * blockDim.x;
+    int32_t t = -1, x = 0, p = -1;
+    if (i < nx) {
+      uint64_t best = tent[i];
+      if (best != UINT64_MAX) {
+        t = (int32_t)(best >> 32) + 1;
+        p = (int32_t)(best & 0xffffffffu);
+      }
+      x = xs[i];
+      level[x] = t;
+      parent[x] = p;
     }
-    x = xs[i];
-    level[x] = t;
-    parent[x] = p;
+    take[q] = t >= 0;
+    packed[q] = ((int64_t)t << 32) | (int64_t)(uint32_t)x;
+    par[q] = p;
+    if (t >= 0) lmin = min(lmin, (int64_t)t), lmax = max(lmax, (int64_t)t);
   }
-  block_push_seed(t >= 0, x, t, p, seeds, seed_par, box, cap);
+  int64_t at[K];
+  block_append_k(take, packed, seeds, &box[4], cap, at);
+#pragma unroll
+  for (int q = 0; q < K; q++)
+    if (at[q] >= 0) seed_par[at[q]] = par[q];
+  block_min_max(lmin, lmax, &box[5], &box[6]);
 }
 
 // Insertion lines whose endpoints are adjacent after the batch: lower the far endpoint. With
@@ -573,7 +589,9 @@ __global__ void k_apply_tentative(const int32_t* xs, const uint64_t* tent, int64
 __global__ void k_insert_seeds(GView g, bool sorted, bool distinct, const int32_t* uv, const uint8_t* ins, int64_t m,
                                int32_t* level, int64_t* seeds, int32_t* seed_par, int64_t cap, int64_t* box) {
   // kLinesPerThread lines per thread (each pass coalesced over the block): one append of the
-  // seeds and one reduction of their levels per block
+  // seeds and one reduction of their levels per block. Seeds are packed level << 32 | vertex,
+  // with the vertex that lowered them in seed_par; box[4] = seeds, box[5] / box[6] = their
+  // smallest / largest level, box[7] = insertion lines.
   constexpr int K = kLinesPerThread;
   int64_t first = (int64_t)blockIdx.x * blockDim.x * K + threadIdx.x;
   bool take[2 * K];
@@ -632,16 +650,24 @@ __global__ void k_seed_parents(const int64_t* seeds, const int32_t* seed_par, in
 // Seeds at `lvl` that still hold that level join the current frontier once.
 __global__ void k_seed_at_level(const int64_t* seeds, int64_t ns, const int32_t* level, int32_t lvl, int32_t* cur,
                                 int64_t* cur_count, int64_t cap, int32_t* stamp, int32_t epoch) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  bool take = false;
-  int32_t v = 0;
-  if (i < ns) {
-    int64_t s = seeds[i];
-    int32_t lv = (int32_t)(s >> 32);
-    v = (int32_t)(s & 0xffffffff);
-    take = lv == lvl && level[v] == lvl && atomicExch(&stamp[v], epoch) != epoch;
+  constexpr int K = kLinesPerThread;
+  int64_t first = (int64_t)blockIdx.x * blockDim.x * K + threadIdx.x;
+  bool take[K];
+  int32_t vs[K];
+#pragma unroll
+  for (int q = 0; q < K; q++) {
+    int64_t i = first + (int64_t)q * blockDim.x;
+    take[q] = false;
+    vs[q] = 0;
+    if (i < ns) {
+      int64_t sd = seeds[i];
+      int32_t lv = (int32_t)(sd >> 32);
+      vs[q] = (int32_t)(sd & 0xffffffff);
+      take[q] = lv == lvl && level[vs[q]] == lvl && atomicExch(&stamp[vs[q]], epoch) != epoch;
+    }
   }
-  block_append(take, v, cur, cur_count, cap);
+  int64_t at[K];
+  block_append_k(take, vs, cur, cur_count, cap, at);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -728,7 +754,8 @@ int relax_levels(Ctx& c, int32_t* L, int32_t* P, int32_t lvl0, int32_t max_seed,
       int64_t* bcur = cnt[p];
       int64_t* bnext = cnt[p ^ 1];
       if (nseeds > 0 && lvl <= max_seed)
-        k_seed_at_level<<<grid_for(nseeds), kBlock, 0, c.s>>>(seeds, nseeds, L, lvl, cur, bcur, cap, c.stamp, epoch);
+        k_seed_at_level<<<grid_for(nseeds, kBlock * kLinesPerThread), kBlock, 0, c.s>>>(seeds, nseeds, L, lvl, cur,
+                                                                                         bcur, cap, c.stamp, epoch);
       k_round_begin<<<1, 1, 0, c.s>>>(bcur, bnext, box + 10, box + 10, 2, box + 15, box + 17, box + 16);
       k_relax_short<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, P, cur, bcur, lvl, nxt, bnext, cap, c.stamp, epoch,
                                                     c.le.p, box + 11, c.wb.p, box + 10, task_cap, box + 12);
@@ -872,8 +899,8 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, int32_t* P, int strategy, scou
         int64_t* bcur = cnt[p];
         int64_t* bnext = cnt[p ^ 1];
         if (lvl <= cmax)
-          k_cand_at_level<<<grid_for(ncand), kBlock, 0, c.s>>>(c.la.p, ncand, L, lvl, cur, bcur, cap, c.stamp,
-                                                                sepoch);
+          k_cand_at_level<<<grid_for(ncand, kBlock * kLinesPerThread), kBlock, 0, c.s>>>(c.la.p, ncand, L, lvl, cur,
+                                                                                           bcur, cap, c.stamp, sepoch);
         k_round_begin<<<1, 1, 0, c.s>>>(bcur, bnext, box + 14, box + 12, 3, box + 18, box + 20, box + 19);
         k_invalidate_short<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, P, cur, bcur, lvl, c.ld.p, binval, nxt,
                                                            bnext, cap, c.stamp, sepoch, c.le.p, box + 12, c.lf.p,
@@ -928,7 +955,8 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, int32_t* P, int strategy, scou
                                                         c.wb.p, box + 13, c.wb.cap, box + 11);
       k_tentative_rows<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, c.ld.p, c.lf.p, box + 12, c.wb.p, box + 13,
                                                        c.wb.cap, tent, box + 11);
-      k_apply_tentative<<<grid_for(nx), kBlock, 0, c.s>>>(c.ld.p, tent, nx, L, P, c.wa.p, c.lg.p, seed_cap, box);
+      k_apply_tentative<<<grid_for(nx, kBlock * kLinesPerThread), kBlock, 0, c.s>>>(c.ld.p, tent, nx, L, P, c.wa.p,
+                                                                                    c.lg.p, seed_cap, box);
     }
   }
   if (m > 0)
