@@ -135,12 +135,11 @@ __global__ void k_relax_short(GView g, int32_t* level, int32_t* parent, const in
 #pragma unroll
     for (int k = 0; k < kShort; k++) lv[k] = xs[k] >= 0 ? level[xs[k]] : 0;
     if (is_short) work += len;
+    bool take[kShort];
 #pragma unroll
-    for (int k = 0; k < kShort; k++) {
-      bool take = false;
-      if (xs[k] >= 0 && (lv[k] == -1 || lv[k] > lvl + 1)) take = relax_to(level, parent, stamp, epoch, f, xs[k], lvl);
-      wave_append(take, xs[k], next, next_count, cap);
-    }
+    for (int k = 0; k < kShort; k++)
+      take[k] = xs[k] >= 0 && (lv[k] == -1 || lv[k] > lvl + 1) && relax_to(level, parent, stamp, epoch, f, xs[k], lvl);
+    wave_append_k(take, xs, next, next_count, cap);
   }
   block_add(work, scanned);
 }
@@ -297,12 +296,13 @@ __global__ void k_invalidate_short(GView g, int32_t* level, int32_t* parent, con
     int32_t pw[kShort];
 #pragma unroll
     for (int k = 0; k < kShort; k++) pw[k] = (invalid && xs[k] >= 0 && lv[k] == lvl + 1) ? parent[xs[k]] : -1;
+    bool take[kShort];
 #pragma unroll
     for (int k = 0; k < kShort; k++) {
-      bool take = pw[k] == c && invalid && atomicExch(&stamp[xs[k]], epoch) != epoch;
-      if (take) attrib_child(at, c, xs[k]);
-      wave_append(take, xs[k], next, next_count, cap);
+      take[k] = pw[k] == c && invalid && atomicExch(&stamp[xs[k]], epoch) != epoch;
+      if (take[k]) attrib_child(at, c, xs[k]);
     }
+    wave_append_k(take, xs, next, next_count, cap);
   }
   block_add(work, scanned);
 }

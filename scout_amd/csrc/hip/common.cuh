@@ -68,6 +68,31 @@ __device__ __forceinline__ void wave_append(bool take, int32_t value, int32_t* o
   }
 }
 
+// Wave-wide append of up to K values per lane (value k kept if take[k]) with one atomic.
+template <int K>
+__device__ __forceinline__ void wave_append_k(const bool (&take)[K], const int32_t (&val)[K], int32_t* out,
+                                              int64_t* count, int64_t cap) {
+  int n = 0;
+#pragma unroll
+  for (int k = 0; k < K; k++) n += take[k];
+  int x = n;  // inclusive scan across the wave
+  for (int off = 1; off < kWave; off <<= 1) {
+    int y = __shfl_up(x, off);
+    if (lane_id() >= off) x += y;
+  }
+  int total = __shfl(x, kWave - 1);
+  if (total == 0) return;
+  int64_t base = 0;
+  if (lane_id() == 0) base = (int64_t)atomicAdd((unsigned long long*)count, (unsigned long long)total);
+  int64_t p = __shfl(base, 0) + (x - n);
+#pragma unroll
+  for (int k = 0; k < K; k++)
+    if (take[k]) {
+      if (p < cap) out[p] = val[k];
+      p++;
+    }
+}
+
 // Block-wide reductions and appends: one global atomic per block instead of one per thread or
 // wave, so O(n) passes do not serialise on a single counter. Every thread of the block must
 // call them (no early return before the call).
