@@ -27,8 +27,9 @@
 // between consecutive levels that is not a parent edge changes nothing.
 //
 // Rows are walked by length: up to kShort entries by one thread (the entries and their
-// levels are gathered first, so the loads of one row are in flight together), up to kChunk by
-// one wave, longer rows as kChunk-entry tasks spread over many waves.
+// levels are gathered first, so the loads of one row are in flight together), up to kTask
+// (relax, tentative levels) or kChunk (invalidation, which stops at the first support) by one
+// wave, longer rows as kTask-entry tasks spread over many waves.
 #include <algorithm>
 
 #include "common.cuh"
@@ -38,6 +39,12 @@ namespace scout {
 namespace {
 
 constexpr int kShort = 16;
+// Entries per chunk task of a split row, and the longest row the relax and tentative passes
+// give one wave: a wave walks at most kTask entries (kTask / (kU * kWave) dependent steps),
+// so one long row no longer sets a round's duration.
+constexpr int64_t kTask = 512;
+// Row steps per iteration of the wave walks: kU entries per lane loaded before any is used.
+constexpr int kU = 4;
 // Level of a vertex found invalid in this batch until its tentative level is written. It is
 // negative, so it never matches a level and reads as unreached.
 constexpr int32_t kInvalid = -2;
@@ -60,12 +67,38 @@ __device__ __forceinline__ void gather_short(const GView& g, const Row& r, int64
   for (int k = 0; k < kShort; k++) xs[k] = (active && k < len) ? row_at(g, r, k) : -1;
 }
 
-// Appends (f << 32 | chunk) tasks covering a row of `len` entries.
+// Appends (f << 32 | chunk) tasks of kTask entries covering a row of `len` entries.
 __device__ __forceinline__ void push_chunks(int64_t f, int64_t len, int64_t* tasks, int64_t* task_count,
                                             int64_t task_cap) {
-  int64_t chunks = (len + kChunk - 1) / kChunk;
+  int64_t chunks = (len + kTask - 1) / kTask;
   int64_t pos = (int64_t)atomicAdd((unsigned long long*)task_count, (unsigned long long)chunks);
   for (int64_t k = 0; k < chunks && pos + k < task_cap; k++) tasks[pos + k] = (f << 32) | k;
+}
+
+// push_chunks by a whole wave (all lanes call it with the same f and len).
+__device__ __forceinline__ void wave_push_chunks(int64_t f, int64_t len, int64_t* tasks, int64_t* task_count,
+                                                 int64_t task_cap) {
+  int64_t chunks = (len + kTask - 1) / kTask;
+  int64_t pos = 0;
+  if (lane_id() == 0) pos = (int64_t)atomicAdd((unsigned long long*)task_count, (unsigned long long)chunks);
+  pos = __shfl(pos, 0);
+  for (int64_t k = lane_id(); k < chunks && pos + k < task_cap; k += kWave) tasks[pos + k] = (f << 32) | k;
+}
+
+// Bounds of chunk task `task` of row r.
+__device__ __forceinline__ void task_range(const Row& r, int64_t task, int64_t& lo, int64_t& hi) {
+  lo = (task & 0xffffffffLL) * kTask;
+  hi = min(lo + kTask, row_len(r));
+}
+
+// Loads entries base + j * kWave + lane (j < kU) of row r, -1 at or past hi.
+__device__ __forceinline__ void load_steps(const GView& g, const Row& r, int64_t base, int64_t hi,
+                                           int32_t (&xs)[kU]) {
+#pragma unroll
+  for (int j = 0; j < kU; j++) {
+    int64_t k = base + (int64_t)j * kWave + lane_id();
+    xs[j] = k < hi ? row_at(g, r, k) : -1;
+  }
 }
 
 // Cascade attribution while the learned scheduler logs training data: every invalid vertex
@@ -104,7 +137,7 @@ __device__ __forceinline__ bool relax_to(int32_t* level, int32_t* parent, int32_
 // ---------------------------------------------------------------------------------------
 
 // Thread per frontier vertex still at level lvl. Short rows are relaxed here; rows up to
-// kChunk go to the wave list, longer rows to chunk tasks.
+// kTask go to the wave list, longer rows to chunk tasks.
 __global__ void k_relax_short(GView g, int32_t* level, int32_t* parent, const int32_t* front, const int64_t* count,
                               int32_t lvl, int32_t* next, int64_t* next_count, int64_t cap, int32_t* stamp,
                               int32_t epoch, int32_t* medium, int64_t* medium_count, int64_t* tasks,
@@ -127,8 +160,8 @@ __global__ void k_relax_short(GView g, int32_t* level, int32_t* parent, const in
       }
     }
     bool is_short = active && len <= kShort;
-    wave_append(active && !is_short && len <= kChunk, f, medium, medium_count, cap);
-    if (active && len > kChunk) push_chunks(f, len, tasks, task_count, task_cap);
+    wave_append(active && !is_short && len <= kTask, f, medium, medium_count, cap);
+    if (active && len > kTask) push_chunks(f, len, tasks, task_count, task_cap);
     int32_t xs[kShort];
     gather_short(g, r, len, is_short, xs);
     int32_t lv[kShort];
@@ -165,22 +198,20 @@ __global__ void k_relax_rows(GView g, int32_t* level, int32_t* parent, const int
       int64_t task = tasks[t - nm];
       f = (int32_t)(task >> 32);
       r = row_of(g, f);
-      lo = (task & 0xffffffffLL) * kChunk;
-      hi = min(lo + kChunk, row_len(r));
+      task_range(r, task, lo, hi);
     }
     work += hi - lo;
-    for (int64_t base = lo; base < hi; base += kWave) {
-      int64_t k = base + lane_id();
-      bool take = false;
-      int32_t w = -1;
-      if (k < hi) {
-        w = row_at(g, r, k);
-        if (w >= 0) {
-          int32_t lw = level[w];
-          if (lw == -1 || lw > lvl + 1) take = relax_to(level, parent, stamp, epoch, f, w, lvl);
-        }
-      }
-      wave_append(take, w, next, next_count, cap);
+    for (int64_t base = lo; base < hi; base += (int64_t)kU * kWave) {
+      int32_t ws[kU];
+      load_steps(g, r, base, hi, ws);
+      int32_t lw[kU];
+#pragma unroll
+      for (int j = 0; j < kU; j++) lw[j] = ws[j] >= 0 ? level[ws[j]] : 0;
+      bool take[kU];
+#pragma unroll
+      for (int j = 0; j < kU; j++)
+        take[j] = ws[j] >= 0 && (lw[j] == -1 || lw[j] > lvl + 1) && relax_to(level, parent, stamp, epoch, f, ws[j], lvl);
+      wave_append_k(take, ws, next, next_count, cap);
     }
   }
   if (lane_id() == 0 && work) atomicAdd((unsigned long long*)scanned, (unsigned long long)work);
@@ -327,20 +358,27 @@ __global__ void k_invalidate(GView g, int32_t* level, int32_t* parent, const int
     Row r = row_of(g, c);
     int64_t len = row_len(r);
     int64_t at = -1, nk = 0;
-    for (int64_t base = 0; base < len; base += kWave) {
-      int64_t k = base + lane_id();
-      int32_t x = k < len ? row_at(g, r, k) : -1;
-      int32_t lx = x >= 0 ? level[x] : INT32_MIN;
-      unsigned long long sup = __ballot(x >= 0 && lx == lvl - 1);
-      if (sup) {
-        at = base + (__ffsll((long long)sup) - 1);
-        break;
+    // kU steps of the row in flight per iteration; `at` is uniform across the wave
+    for (int64_t base = 0; base < len && at < 0; base += (int64_t)kU * kWave) {
+      int32_t xs[kU];
+      load_steps(g, r, base, len, xs);
+      int32_t lx[kU];
+#pragma unroll
+      for (int j = 0; j < kU; j++) lx[j] = xs[j] >= 0 ? level[xs[j]] : INT32_MIN;
+#pragma unroll
+      for (int j = 0; j < kU; j++) {
+        if (at >= 0) break;
+        unsigned long long sup = __ballot(xs[j] >= 0 && lx[j] == lvl - 1);
+        if (sup) {
+          at = base + (int64_t)j * kWave + (__ffsll((long long)sup) - 1);
+          break;
+        }
+        bool kid = xs[j] >= 0 && lx[j] == lvl + 1;
+        unsigned long long km = __ballot(kid);
+        int64_t slot = nk + __popcll(km & ((1ULL << lane_id()) - 1ULL));
+        if (kid && slot < kKids) kids[slot] = xs[j];
+        nk += __popcll(km);
       }
-      bool kid = x >= 0 && lx == lvl + 1;
-      unsigned long long km = __ballot(kid);
-      int64_t slot = nk + __popcll(km & ((1ULL << lane_id()) - 1ULL));
-      if (kid && slot < kKids) kids[slot] = x;
-      nk += __popcll(km);
     }
     rows++;
     if (at >= 0) {
@@ -359,25 +397,39 @@ __global__ void k_invalidate(GView g, int32_t* level, int32_t* parent, const int
       // children among the buffered neighbours (their levels were read above)
       __threadfence_block();
       reads += len + nk;
-      for (int64_t base = 0; base < nk; base += kWave) {
-        int64_t j = base + lane_id();
-        int32_t w = j < nk ? kids[j] : -1;
-        bool take = w >= 0 && parent[w] == c && atomicExch(&stamp[w], epoch) != epoch;
-        if (take) attrib_child(ab, c, w);
-        wave_append(take, w, next, next_count, cap);
+      for (int64_t base = 0; base < nk; base += (int64_t)kU * kWave) {
+        int32_t ws[kU];
+#pragma unroll
+        for (int j = 0; j < kU; j++) {
+          int64_t q = base + (int64_t)j * kWave + lane_id();
+          ws[j] = q < nk ? kids[q] : -1;
+        }
+        int32_t pw[kU];
+#pragma unroll
+        for (int j = 0; j < kU; j++) pw[j] = ws[j] >= 0 ? parent[ws[j]] : -1;
+        bool take[kU];
+#pragma unroll
+        for (int j = 0; j < kU; j++) {
+          take[j] = pw[j] == c && atomicExch(&stamp[ws[j]], epoch) != epoch;
+          if (take[j]) attrib_child(ab, c, ws[j]);
+        }
+        wave_append_k(take, ws, next, next_count, cap);
       }
     } else {
       reads += 2 * len;
-      for (int64_t base = 0; base < len; base += kWave) {
-        int64_t k = base + lane_id();
-        bool take = false;
-        int32_t w = -1;
-        if (k < len) {
-          w = row_at(g, r, k);
-          take = w >= 0 && parent[w] == c && level[w] == lvl + 1 && atomicExch(&stamp[w], epoch) != epoch;
-          if (take) attrib_child(ab, c, w);
+      for (int64_t base = 0; base < len; base += (int64_t)kU * kWave) {
+        int32_t ws[kU];
+        load_steps(g, r, base, len, ws);
+        int32_t pw[kU];
+#pragma unroll
+        for (int j = 0; j < kU; j++) pw[j] = ws[j] >= 0 ? parent[ws[j]] : -1;
+        bool take[kU];
+#pragma unroll
+        for (int j = 0; j < kU; j++) {
+          take[j] = pw[j] == c && level[ws[j]] == lvl + 1 && atomicExch(&stamp[ws[j]], epoch) != epoch;
+          if (take[j]) attrib_child(ab, c, ws[j]);
         }
-        wave_append(take, w, next, next_count, cap);
+        wave_append_k(take, ws, next, next_count, cap);
       }
     }
   }
@@ -388,12 +440,12 @@ __global__ void k_invalidate(GView g, int32_t* level, int32_t* parent, const int
   }
 }
 
-// Chunk tasks (index in `longs` << 32 | chunk) for every long candidate.
+// Chunk tasks (index in `longs` << 32 | chunk) for every long candidate, a wave per row.
 __global__ void k_long_tasks(GView g, const int32_t* longs, const int64_t* long_count, int64_t* tasks,
                              int64_t* task_count, int64_t task_cap) {
   int64_t nl = *long_count;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nl; i += (int64_t)gridDim.x * blockDim.x)
-    push_chunks(i, row_len(row_of(g, longs[i])), tasks, task_count, task_cap);
+  for (int64_t i = global_wave(); i < nl; i += total_waves())
+    wave_push_chunks(i, row_len(row_of(g, longs[i])), tasks, task_count, task_cap);
 }
 
 // Wave per chunk: a neighbour at lvl-1 anywhere in the chunk supports the candidate and
@@ -408,8 +460,21 @@ __global__ void k_inval_search(GView g, const int32_t* level, int32_t* parent, i
     int32_t c = longs[task >> 32];
     if (supported[c] == sepoch) continue;
     Row r = row_of(g, c);
-    int64_t lo = (task & 0xffffffffLL) * kChunk, hi = min(lo + kChunk, row_len(r));
-    int64_t at = wave_find_first(g, r, lo, hi, [&](int32_t x) { return level[x] == lvl - 1; });
+    int64_t lo, hi;
+    task_range(r, task, lo, hi);
+    int64_t at = -1;
+    for (int64_t base = lo; base < hi && at < 0; base += (int64_t)kU * kWave) {
+      int32_t xs[kU];
+      load_steps(g, r, base, hi, xs);
+      int32_t lx[kU];
+#pragma unroll
+      for (int j = 0; j < kU; j++) lx[j] = xs[j] >= 0 ? level[xs[j]] : INT32_MIN;
+#pragma unroll
+      for (int j = 0; j < kU; j++) {
+        unsigned long long sup = __ballot(xs[j] >= 0 && lx[j] == lvl - 1);
+        if (sup && at < 0) at = base + (int64_t)j * kWave + (__ffsll((long long)sup) - 1);
+      }
+    }
     work += (at >= 0 ? at + 1 : hi) - lo;
     if (at >= 0 && lane_id() == 0) {
       parent[c] = row_at(g, r, at);
@@ -446,18 +511,22 @@ __global__ void k_inval_push(GView g, const int32_t* level, const int32_t* paren
     int32_t c = longs[task >> 32];
     if (level[c] != kInvalid) continue;
     Row r = row_of(g, c);
-    int64_t lo = (task & 0xffffffffLL) * kChunk, hi = min(lo + kChunk, row_len(r));
+    int64_t lo, hi;
+    task_range(r, task, lo, hi);
     work += hi - lo;
-    for (int64_t base = lo; base < hi; base += kWave) {
-      int64_t k = base + lane_id();
-      bool take = false;
-      int32_t w = -1;
-      if (k < hi) {
-        w = row_at(g, r, k);
-        take = w >= 0 && parent[w] == c && level[w] == lvl + 1 && atomicExch(&stamp[w], epoch) != epoch;
-        if (take) attrib_child(ab, c, w);
+    for (int64_t base = lo; base < hi; base += (int64_t)kU * kWave) {
+      int32_t ws[kU];
+      load_steps(g, r, base, hi, ws);
+      int32_t pw[kU];
+#pragma unroll
+      for (int j = 0; j < kU; j++) pw[j] = ws[j] >= 0 ? parent[ws[j]] : -1;
+      bool take[kU];
+#pragma unroll
+      for (int j = 0; j < kU; j++) {
+        take[j] = pw[j] == c && level[ws[j]] == lvl + 1 && atomicExch(&stamp[ws[j]], epoch) != epoch;
+        if (take[j]) attrib_child(ab, c, ws[j]);
       }
-      wave_append(take, w, next, next_count, cap);
+      wave_append_k(take, ws, next, next_count, cap);
     }
   }
   if (lane_id() == 0 && work) atomicAdd((unsigned long long*)scanned, (unsigned long long)work);
@@ -469,7 +538,7 @@ __global__ void k_inval_push(GView g, const int32_t* level, const int32_t* paren
 
 // Smallest (level << 32 | neighbour) over the reached neighbours of each invalidated vertex
 // (UINT64_MAX if none). Thread per vertex for rows of at most kShort entries; longer rows go
-// to the wave list (<= kChunk) or to chunk tasks.
+// to the wave list (<= kTask) or to chunk tasks.
 __global__ void k_tentative_short(GView g, const int32_t* level, const int32_t* xs, int64_t nx, uint64_t* tent,
                                   int32_t* medium, int64_t* medium_count, int64_t medium_cap, int64_t* tasks,
                                   int64_t* task_count, int64_t task_cap, int64_t* scanned) {
@@ -495,9 +564,9 @@ __global__ void k_tentative_short(GView g, const int32_t* level, const int32_t* 
     if (is_short) work += len;
     if (active) {
       tent[i] = best;
-      if (len > kChunk) push_chunks(i, len, tasks, task_count, task_cap);
+      if (len > kTask) push_chunks(i, len, tasks, task_count, task_cap);
     }
-    wave_append(active && !is_short && len <= kChunk, (int32_t)(active ? i : 0), medium, medium_count, medium_cap);
+    wave_append(active && !is_short && len <= kTask, (int32_t)(active ? i : 0), medium, medium_count, medium_cap);
   }
   block_add(work, scanned);
 }
@@ -529,17 +598,19 @@ __global__ void k_tentative_rows(GView g, const int32_t* level, const int32_t* x
       int64_t task = tasks[t - nm];
       i = task >> 32;
       r = row_of(g, xs[i]);
-      lo = (task & 0xffffffffLL) * kChunk;
-      hi = min(lo + kChunk, row_len(r));
+      task_range(r, task, lo, hi);
     }
     work += hi - lo;
     uint64_t best = UINT64_MAX;
-    for (int64_t k = lo + lane_id(); k < hi; k += kWave) {
-      int32_t w = row_at(g, r, k);
-      if (w >= 0) {
-        int32_t lw = level[w];
-        if (lw >= 0) best = min(best, ((uint64_t)(uint32_t)lw << 32) | (uint32_t)w);
-      }
+    for (int64_t base = lo; base < hi; base += (int64_t)kU * kWave) {
+      int32_t ws[kU];
+      load_steps(g, r, base, hi, ws);
+      int32_t lw[kU];
+#pragma unroll
+      for (int j = 0; j < kU; j++) lw[j] = ws[j] >= 0 ? level[ws[j]] : -1;
+#pragma unroll
+      for (int j = 0; j < kU; j++)
+        if (lw[j] >= 0) best = min(best, ((uint64_t)(uint32_t)lw[j] << 32) | (uint32_t)ws[j]);
     }
     best = wave_min_u64(best);
     if (lane_id() == 0 && best != UINT64_MAX) atomicMin((unsigned long long*)&tent[i], (unsigned long long)best);
@@ -797,8 +868,8 @@ int ensure_bfs(Ctx& c, int64_t extra) {
   SCOUT_CHECK(c.lg.reserve(n + 2 * extra + 1));
   SCOUT_CHECK(c.wa.reserve(n + 2 * extra + 1));
   SCOUT_CHECK(c.wc.reserve(n + 1));
-  // chunk tasks: at most 2 * entries / kChunk (every split row has more than kChunk entries)
-  SCOUT_CHECK(c.wb.reserve(2 * (c.g.E + c.g.A) / kChunk + 1024));
+  // chunk tasks: at most 2 * entries / kTask (every split row has more than kTask entries)
+  SCOUT_CHECK(c.wb.reserve(2 * (c.g.E + c.g.A) / kTask + 1024));
   return SCOUT_OK;
 }
 
