@@ -45,6 +45,10 @@ constexpr int kShort = 16;
 constexpr int64_t kTask = 512;
 // Row steps per iteration of the wave walks: kU entries per lane loaded before any is used.
 constexpr int kU = 4;
+// LDS staging of a wave's appends to the next frontier (entries per wave): the list counter
+// is then taken once per block, or once per wave when the segment fills, instead of once per
+// row step of every wave.
+constexpr int kNextSeg = 256;
 // Level of a vertex found invalid in this batch until its tentative level is written. It is
 // negative, so it never matches a level and reads as unreached.
 constexpr int32_t kInvalid = -2;
@@ -182,6 +186,8 @@ __global__ void k_relax_rows(GView g, int32_t* level, int32_t* parent, const int
                              const int64_t* medium_count, const int64_t* tasks, const int64_t* task_count,
                              int64_t task_cap, int32_t lvl, int32_t* next, int64_t* next_count, int64_t cap,
                              int32_t* stamp, int32_t epoch, int64_t* scanned) {
+  __shared__ int32_t seg_all[kWavesPerBlock * kNextSeg];
+  WaveStage<kNextSeg> st{seg_all + (threadIdx.x / kWave) * kNextSeg, 0};
   int64_t nm = *medium_count;
   int64_t nt = min(*task_count, task_cap);
   int64_t work = 0;
@@ -211,10 +217,12 @@ __global__ void k_relax_rows(GView g, int32_t* level, int32_t* parent, const int
 #pragma unroll
       for (int j = 0; j < kU; j++)
         take[j] = ws[j] >= 0 && (lw[j] == -1 || lw[j] > lvl + 1) && relax_to(level, parent, stamp, epoch, f, ws[j], lvl);
-      wave_append_k(take, ws, next, next_count, cap);
+#pragma unroll
+      for (int j = 0; j < kU; j++) stage_push(st, take[j], ws[j], next, next_count, cap);
     }
   }
-  if (lane_id() == 0 && work) atomicAdd((unsigned long long*)scanned, (unsigned long long)work);
+  stage_finish(st, next, next_count, cap);
+  block_add(lane_id() == 0 ? work : 0, scanned);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -349,8 +357,13 @@ __global__ void k_invalidate(GView g, int32_t* level, int32_t* parent, const int
                              int64_t* own, Attrib ab) {
   // neighbours one level down met while searching for support: the only possible children
   constexpr int kKids = 1024;
+  constexpr int kInvSeg = 64;
   __shared__ int32_t kids_all[kWavesPerBlock * kKids];
+  __shared__ int32_t seg_all[kWavesPerBlock * kNextSeg];
+  __shared__ int32_t inv_all[kWavesPerBlock * kInvSeg];
   int32_t* kids = kids_all + (threadIdx.x / kWave) * kKids;
+  WaveStage<kNextSeg> st{seg_all + (threadIdx.x / kWave) * kNextSeg, 0};
+  WaveStage<kInvSeg> sinv{inv_all + (threadIdx.x / kWave) * kInvSeg, 0};
   int64_t nc = *cur_count;
   int64_t work = 0, reads = 0, rows = 0;
   for (int64_t i = global_wave(); i < nc; i += total_waves()) {
@@ -391,8 +404,8 @@ __global__ void k_invalidate(GView g, int32_t* level, int32_t* parent, const int
     if (lane_id() == 0) {
       level[c] = kInvalid;
       attrib_invalid(ab, c, lvl);
-      thread_append(c, inval, inval_count, inval_cap);
     }
+    stage_push(sinv, lane_id() == 0, c, inval, inval_count, inval_cap);
     if (nk <= kKids) {
       // children among the buffered neighbours (their levels were read above)
       __threadfence_block();
@@ -413,7 +426,8 @@ __global__ void k_invalidate(GView g, int32_t* level, int32_t* parent, const int
           take[j] = pw[j] == c && atomicExch(&stamp[ws[j]], epoch) != epoch;
           if (take[j]) attrib_child(ab, c, ws[j]);
         }
-        wave_append_k(take, ws, next, next_count, cap);
+#pragma unroll
+        for (int j = 0; j < kU; j++) stage_push(st, take[j], ws[j], next, next_count, cap);
       }
     } else {
       reads += 2 * len;
@@ -429,15 +443,18 @@ __global__ void k_invalidate(GView g, int32_t* level, int32_t* parent, const int
           take[j] = pw[j] == c && level[ws[j]] == lvl + 1 && atomicExch(&stamp[ws[j]], epoch) != epoch;
           if (take[j]) attrib_child(ab, c, ws[j]);
         }
-        wave_append_k(take, ws, next, next_count, cap);
+#pragma unroll
+        for (int j = 0; j < kU; j++) stage_push(st, take[j], ws[j], next, next_count, cap);
       }
     }
   }
-  if (lane_id() == 0 && work) {
-    atomicAdd((unsigned long long*)scanned, (unsigned long long)work);
-    atomicAdd((unsigned long long*)&own[0], (unsigned long long)reads);
-    atomicAdd((unsigned long long*)&own[1], (unsigned long long)rows);
-  }
+  stage_finish(st, next, next_count, cap);
+  stage_finish(sinv, inval, inval_count, inval_cap);
+  // work, reads and rows are uniform across the wave: counted once per wave
+  bool l0 = lane_id() == 0;
+  block_add(l0 ? work : 0, scanned);
+  block_add(l0 ? reads : 0, &own[0]);
+  block_add(l0 ? rows : 0, &own[1]);
 }
 
 // Chunk tasks (index in `longs` << 32 | chunk) for every long candidate, a wave per row.
@@ -481,7 +498,7 @@ __global__ void k_inval_search(GView g, const int32_t* level, int32_t* parent, i
       supported[c] = sepoch;
     }
   }
-  if (lane_id() == 0 && work) atomicAdd((unsigned long long*)scanned, (unsigned long long)work);
+  block_add(lane_id() == 0 ? work : 0, scanned);
 }
 
 // Thread per long candidate: unsupported ones become invalid.
@@ -504,6 +521,8 @@ __global__ void k_inval_push(GView g, const int32_t* level, const int32_t* paren
                              const int64_t* tasks, const int64_t* task_count, int64_t task_cap, int32_t* next,
                              int64_t* next_count, int64_t cap, int32_t* stamp, int32_t epoch, int64_t* scanned,
                              Attrib ab) {
+  __shared__ int32_t seg_all[kWavesPerBlock * kNextSeg];
+  WaveStage<kNextSeg> st{seg_all + (threadIdx.x / kWave) * kNextSeg, 0};
   int64_t nt = min(*task_count, task_cap);
   int64_t work = 0;
   for (int64_t t = global_wave(); t < nt; t += total_waves()) {
@@ -526,10 +545,12 @@ __global__ void k_inval_push(GView g, const int32_t* level, const int32_t* paren
         take[j] = pw[j] == c && level[ws[j]] == lvl + 1 && atomicExch(&stamp[ws[j]], epoch) != epoch;
         if (take[j]) attrib_child(ab, c, ws[j]);
       }
-      wave_append_k(take, ws, next, next_count, cap);
+#pragma unroll
+      for (int j = 0; j < kU; j++) stage_push(st, take[j], ws[j], next, next_count, cap);
     }
   }
-  if (lane_id() == 0 && work) atomicAdd((unsigned long long*)scanned, (unsigned long long)work);
+  stage_finish(st, next, next_count, cap);
+  block_add(lane_id() == 0 ? work : 0, scanned);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -615,7 +636,7 @@ __global__ void k_tentative_rows(GView g, const int32_t* level, const int32_t* x
     best = wave_min_u64(best);
     if (lane_id() == 0 && best != UINT64_MAX) atomicMin((unsigned long long*)&tent[i], (unsigned long long)best);
   }
-  if (lane_id() == 0 && work) atomicAdd((unsigned long long*)scanned, (unsigned long long)work);
+  block_add(lane_id() == 0 ? work : 0, scanned);
 }
 
 __global__ void k_apply_tentative(const int32_t* xs, const uint64_t* tent, int64_t nx, int32_t* level,
