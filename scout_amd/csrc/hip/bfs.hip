@@ -71,15 +71,17 @@ __device__ __forceinline__ void gather_short(const GView& g, const Row& r, int64
   for (int k = 0; k < kShort; k++) xs[k] = (active && k < len) ? row_at(g, r, k) : -1;
 }
 
-// Appends (f << 32 | chunk) tasks of kTask entries covering a row of `len` entries.
-__device__ __forceinline__ void push_chunks(int64_t f, int64_t len, int64_t* tasks, int64_t* task_count,
-                                            int64_t task_cap) {
-  int64_t chunks = (len + kTask - 1) / kTask;
-  int64_t pos = (int64_t)atomicAdd((unsigned long long*)task_count, (unsigned long long)chunks);
+// Appends (f << 32 | chunk) tasks of kTask entries covering a row of `len` entries for every
+// thread of the block with `take`, the block's slots reserved with one atomic (every thread of
+// the block calls it).
+__device__ __forceinline__ void block_push_chunks(bool take, int64_t f, int64_t len, int64_t* tasks,
+                                                  int64_t* task_count, int64_t task_cap) {
+  int64_t chunks = take ? (len + kTask - 1) / kTask : 0;
+  int64_t pos = block_reserve(chunks, task_count);
   for (int64_t k = 0; k < chunks && pos + k < task_cap; k++) tasks[pos + k] = (f << 32) | k;
 }
 
-// push_chunks by a whole wave (all lanes call it with the same f and len).
+// The same for one row by a whole wave (all lanes call it with the same f and len).
 __device__ __forceinline__ void wave_push_chunks(int64_t f, int64_t len, int64_t* tasks, int64_t* task_count,
                                                  int64_t task_cap) {
   int64_t chunks = (len + kTask - 1) / kTask;
@@ -148,7 +150,7 @@ __global__ void k_relax_short(GView g, int32_t* level, int32_t* parent, const in
                               int64_t* task_count, int64_t task_cap, int64_t* scanned) {
   int64_t nf = *count;
   int64_t work = 0;
-  // the trip count is uniform across the block, so the wave-wide appends see every lane
+  // the trip count is uniform across the block, so the block-wide appends see every thread
   for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < nf; base += (int64_t)gridDim.x * blockDim.x) {
     int64_t i = base + threadIdx.x;
     int32_t f = -1;
@@ -164,8 +166,8 @@ __global__ void k_relax_short(GView g, int32_t* level, int32_t* parent, const in
       }
     }
     bool is_short = active && len <= kShort;
-    wave_append(active && !is_short && len <= kTask, f, medium, medium_count, cap);
-    if (active && len > kTask) push_chunks(f, len, tasks, task_count, task_cap);
+    block_append(active && !is_short && len <= kTask, f, medium, medium_count, cap);
+    block_push_chunks(active && len > kTask, f, len, tasks, task_count, task_cap);
     int32_t xs[kShort];
     gather_short(g, r, len, is_short, xs);
     int32_t lv[kShort];
@@ -176,7 +178,8 @@ __global__ void k_relax_short(GView g, int32_t* level, int32_t* parent, const in
 #pragma unroll
     for (int k = 0; k < kShort; k++)
       take[k] = xs[k] >= 0 && (lv[k] == -1 || lv[k] > lvl + 1) && relax_to(level, parent, stamp, epoch, f, xs[k], lvl);
-    wave_append_k(take, xs, next, next_count, cap);
+    int64_t at[kShort];
+    block_append_k(take, xs, next, next_count, cap, at);
   }
   block_add(work, scanned);
 }
@@ -327,9 +330,9 @@ __global__ void k_invalidate_short(GView g, int32_t* level, int32_t* parent, con
         parent[c] = found;
       }
     }
-    wave_append(invalid, c, inval, inval_count, cap);
-    wave_append(active && !is_short && len <= kChunk, c, medium, medium_count, cap);
-    wave_append(active && len > kChunk, c, longs, long_count, cap);
+    block_append(invalid, c, inval, inval_count, cap);
+    block_append(active && !is_short && len <= kChunk, c, medium, medium_count, cap);
+    block_append(active && len > kChunk, c, longs, long_count, cap);
     // children of an invalid candidate: neighbours one level down (levels read above; no
     // vertex of that level changes in this kernel) whose parent it is
     int32_t pw[kShort];
@@ -341,7 +344,8 @@ __global__ void k_invalidate_short(GView g, int32_t* level, int32_t* parent, con
       take[k] = pw[k] == c && invalid && atomicExch(&stamp[xs[k]], epoch) != epoch;
       if (take[k]) attrib_child(at, c, xs[k]);
     }
-    wave_append_k(take, xs, next, next_count, cap);
+    int64_t at[kShort];
+    block_append_k(take, xs, next, next_count, cap, at);
   }
   block_add(work, scanned);
 }
@@ -585,9 +589,9 @@ __global__ void k_tentative_short(GView g, const int32_t* level, const int32_t* 
     if (is_short) work += len;
     if (active) {
       tent[i] = best;
-      if (len > kTask) push_chunks(i, len, tasks, task_count, task_cap);
     }
-    wave_append(active && !is_short && len <= kTask, (int32_t)(active ? i : 0), medium, medium_count, medium_cap);
+    block_push_chunks(active && len > kTask, i, len, tasks, task_count, task_cap);
+    block_append(active && !is_short && len <= kTask, (int32_t)(active ? i : 0), medium, medium_count, medium_cap);
   }
   block_add(work, scanned);
 }
