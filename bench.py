@@ -84,7 +84,7 @@ FAMILY_KERNELS = {
     "cc_expand": ["k_zero_u32", "k_cc_mark", "k_cc_collect", "k_cc_union_list", "k_cc_unreached_hook",
                   "k_cc_hook_tasks", "k_cc_final_list"],
     "apply": ["k_apply", "k_group_heads", "k_append_groups", "k_move_rows"],
-    "classify": ["k_dup_insert", "k_dup_mark", "k_classify"],
+    "classify": ["k_dup_count", "k_classify"],
 }
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_rmat26.json")
 # Single kernels timed launch by launch (HIP events on the compute stream) with their

@@ -84,32 +84,22 @@ __device__ __forceinline__ uint64_t hash64(uint64_t k) {
 
 constexpr uint64_t kEmpty = ~0ULL;
 
-__global__ void k_dup_insert(const int32_t* uv, int64_t m, uint64_t* hkey, int32_t* hline, uint64_t mask) {
+// Inserts every line's unordered pair into an open-addressing set; *count += the lines whose
+// pair another line claimed first (m minus the distinct pairs).
+__global__ void k_dup_count(const int32_t* uv, int64_t m, uint64_t* hkey, uint64_t mask, int64_t* count) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m) return;
   uint64_t key = pair_key(uv[2 * i], uv[2 * i + 1]);
   uint64_t h = hash64(key) & mask;
   while (true) {
     uint64_t prev = atomicCAS((unsigned long long*)&hkey[h], (unsigned long long)kEmpty, (unsigned long long)key);
-    if (prev == kEmpty || prev == key) {
-      atomicMin(&hline[h], (int32_t)i);
+    if (prev == kEmpty) return;
+    if (prev == key) {
+      atomicAdd((unsigned long long*)count, 1ULL);
       return;
     }
     h = (h + 1) & mask;
   }
-}
-
-// prev[i] = 1 if an earlier line of the batch has the same unordered pair.
-__global__ void k_dup_mark(const int32_t* uv, int64_t m, const uint64_t* hkey, const int32_t* hline, uint64_t mask,
-                           uint8_t* prev, int64_t* count) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= m) return;
-  uint64_t key = pair_key(uv[2 * i], uv[2 * i + 1]);
-  uint64_t h = hash64(key) & mask;
-  while (hkey[h] != key) h = (h + 1) & mask;
-  bool dup = hline[h] < (int32_t)i;
-  prev[i] = dup ? 1 : 0;
-  if (dup) atomicAdd((unsigned long long*)count, 1ULL);
 }
 
 // ins[i] = 1 iff the pair is absent from the rule's graph. rule 0: current logical graph;
@@ -165,13 +155,14 @@ struct NewEntry {
 constexpr int kApplyLines = 4;
 
 // The DeltaOverlay rules for the lines of [lo, hi). done[i] = 1 if the line was applied.
-// New inserted-row entries (key src << 32 | line, value dst) are appended with one atomic
+// New inserted-row entries (key src << ks | line - lo, value dst; ks bits hold every line
+// offset, so the sort needs only the key's low ks + log2(n) bits) are appended with one atomic
 // per block. box[0] = new inserted-row entries, box[1] = holes created.
 // pos (optional): base ranges of row u located by k_classify (the base rows' entries do not
 // move within a batch; only their deletion marks flip).
 __global__ void k_apply(GView g, bool sorted, const int32_t* uv, const uint8_t* ins, const int64_t* pos, int64_t lo,
                         int64_t hi, int conditional, uint8_t* done, uint64_t* nkey, int32_t* ndst, int64_t ncap,
-                        int64_t* box) {
+                        int ks, int64_t* box) {
   // kApplyLines lines per thread (each pass coalesced over the block): one block-wide append
   // of the new entries per block
   constexpr int K = kApplyLines;
@@ -217,10 +208,10 @@ __global__ void k_apply(GView g, bool sorted, const int32_t* uv, const uint8_t* 
       }
     }
     take[2 * q] = e1.take;
-    keys[2 * q] = ((uint64_t)(uint32_t)e1.src << 32) | (uint64_t)i;
+    keys[2 * q] = ((uint64_t)(uint32_t)e1.src << ks) | (uint64_t)(i - lo);
     dsts[2 * q] = e1.dst;
     take[2 * q + 1] = e2.take;
-    keys[2 * q + 1] = ((uint64_t)(uint32_t)e2.src << 32) | (uint64_t)i;
+    keys[2 * q + 1] = ((uint64_t)(uint32_t)e2.src << ks) | (uint64_t)(i - lo);
     dsts[2 * q + 1] = e2.dst;
   }
   int64_t at[2 * K];
@@ -234,16 +225,17 @@ __global__ void k_apply(GView g, bool sorted, const int32_t* uv, const uint8_t* 
 // --- inserted rows: appends into the slack pool ---------------------------------------
 
 // Group heads of the sorted new entries: a group is the run of one source vertex.
-__global__ void k_group_heads(const uint64_t* nkey, int64_t cnt, int32_t* heads, int64_t* nheads, int64_t cap) {
+__global__ void k_group_heads(const uint64_t* nkey, int64_t cnt, int ks, int32_t* heads, int64_t* nheads,
+                              int64_t cap) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  bool head = i < cnt && (i == 0 || (nkey[i] >> 32) != (nkey[i - 1] >> 32));
+  bool head = i < cnt && (i == 0 || (nkey[i] >> ks) != (nkey[i - 1] >> ks));
   block_append(head, (int32_t)i, heads, nheads, cap);
 }
 
-__device__ __forceinline__ int64_t group_end(const uint64_t* nkey, int64_t cnt, int64_t s) {
-  uint64_t src = nkey[s] >> 32;
+__device__ __forceinline__ int64_t group_end(const uint64_t* nkey, int64_t cnt, int ks, int64_t s) {
+  uint64_t src = nkey[s] >> ks;
   int64_t e = s + 1;
-  while (e < cnt && (nkey[e] >> 32) == src) e++;
+  while (e < cnt && (nkey[e] >> ks) == src) e++;
   return e;
 }
 
@@ -261,7 +253,7 @@ constexpr int64_t kMoveThread = 64;
 // size (reserved with one atomic per block). The pool has room for the worst case (the host
 // checks before the launch); *overflow counts reservations past it all the same.
 __global__ void k_append_groups(int64_t* aoff, int32_t* alen, int32_t* acap, int32_t* acol, int32_t* aown,
-                                const uint64_t* nkey, const int32_t* ndst, int64_t cnt, const int32_t* heads,
+                                const uint64_t* nkey, const int32_t* ndst, int64_t cnt, int ks, const int32_t* heads,
                                 const int64_t* nheads, int64_t* top, int64_t pool_cap, int32_t* moves,
                                 int64_t* nmoves, int64_t* move_pos, int64_t* overflow) {
   int64_t gi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -270,8 +262,8 @@ __global__ void k_append_groups(int64_t* aoff, int32_t* alen, int32_t* acap, int
   int32_t u = -1, len = 0;
   if (gi < ng) {
     s = heads[gi];
-    e = group_end(nkey, cnt, s);
-    u = (int32_t)(nkey[s] >> 32);
+    e = group_end(nkey, cnt, ks, s);
+    u = (int32_t)(nkey[s] >> ks);
     len = alen[u];
     if (len + (e - s) > acap[u]) need = grown_capacity(len + (e - s));
   }
@@ -309,14 +301,14 @@ __global__ void k_append_groups(int64_t* aoff, int32_t* alen, int32_t* acap, int
 
 // Wave per long row that moves: old entries, then the new ones, into the reserved span.
 __global__ void k_move_rows(int64_t* aoff, int32_t* alen, int32_t* acap, int32_t* acol, int32_t* aown,
-                            const uint64_t* nkey, const int32_t* ndst, int64_t cnt, const int32_t* heads,
+                            const uint64_t* nkey, const int32_t* ndst, int64_t cnt, int ks, const int32_t* heads,
                             const int32_t* moves, const int64_t* nmoves, const int64_t* move_pos) {
   int64_t nm = *nmoves;
   for (int64_t i = global_wave(); i < nm; i += total_waves()) {
     int32_t gi = moves[i];
     int64_t s = heads[gi];
-    int64_t e = group_end(nkey, cnt, s);
-    int32_t u = (int32_t)(nkey[s] >> 32);
+    int64_t e = group_end(nkey, cnt, ks, s);
+    int32_t u = (int32_t)(nkey[s] >> ks);
     int64_t len = alen[u], k = e - s, old = aoff[u], pos = move_pos[gi];
     for (int64_t j = lane_id(); j < len; j += kWave) {
       acol[pos + j] = acol[old + j];
@@ -502,12 +494,21 @@ int repack_inserted(Ctx& c, int64_t headroom) {
 int64_t pool_cap(const DevGraph& g) { return std::min(g.acol.cap, g.aown.cap); }
 
 // Appends this batch's new entries (nkey/ndst, cnt_new of them) to their rows.
-int insert_new(Ctx& c, int64_t cnt_new) {
+// Bits that hold every value in [0, x] (at least 1).
+int bits_for(int64_t x) {
+  int b = 1;
+  while (b < 63 && (x >> b) != 0) b++;
+  return b;
+}
+
+int insert_new(Ctx& c, int64_t cnt_new, int ks) {
   DevGraph& g = c.g;
-  // Stable by (src, line): the new entries of a row keep the batch's line order.
+  // Stable by (src, line): the new entries of a row keep the batch's line order. Keys are
+  // src << ks | line offset, so only their low ks + bits(n - 1) bits are sorted.
   SCOUT_CHECK(g.nkey2.reserve(cnt_new));
   SCOUT_CHECK(g.ndst2.reserve(cnt_new));
-  SCOUT_CHECK(sort_pairs_u64(c, g.nkey.p, g.nkey2.p, g.ndst.p, g.ndst2.p, cnt_new, 64));
+  SCOUT_CHECK(sort_pairs_u64(c, g.nkey.p, g.nkey2.p, g.ndst.p, g.ndst2.p, cnt_new,
+                             ks + bits_for(std::max<int64_t>(g.n - 1, 0))));
   std::swap(g.nkey, g.nkey2);
   std::swap(g.ndst, g.ndst2);
   // Worst case: every touched row moves to twice its new size, and the rows hold at most the
@@ -524,13 +525,13 @@ int insert_new(Ctx& c, int64_t cnt_new) {
   // box: [2] groups, [3] moved rows, [4] pool top, [5] overflow
   int64_t init[4] = {0, 0, g.A, 0};
   SCOUT_TRY(hipMemcpyAsync(c.d_box + 2, init, sizeof(init), hipMemcpyHostToDevice, c.s));
-  k_group_heads<<<grid_for(cnt_new), kBlock, 0, c.s>>>(g.nkey.p, cnt_new, g.heads.p, c.d_box + 2, g.heads.cap);
+  k_group_heads<<<grid_for(cnt_new), kBlock, 0, c.s>>>(g.nkey.p, cnt_new, ks, g.heads.p, c.d_box + 2, g.heads.cap);
   k_append_groups<<<grid_for(cnt_new), kBlock, 0, c.s>>>(g.aoff.p, g.alen.p, g.acap.p, g.acol.p, g.aown.p, g.nkey.p,
-                                                         g.ndst.p, cnt_new, g.heads.p, c.d_box + 2, c.d_box + 4,
+                                                         g.ndst.p, cnt_new, ks, g.heads.p, c.d_box + 2, c.d_box + 4,
                                                          pool_cap(g), g.moves.p, c.d_box + 3, g.move_pos.p,
                                                          c.d_box + 5);
   k_move_rows<<<kListGrid, kBlock, 0, c.s>>>(g.aoff.p, g.alen.p, g.acap.p, g.acol.p, g.aown.p, g.nkey.p, g.ndst.p,
-                                             cnt_new, g.heads.p, g.moves.p, c.d_box + 3, g.move_pos.p);
+                                             cnt_new, ks, g.heads.p, g.moves.p, c.d_box + 3, g.move_pos.p);
   SCOUT_TRY(hipMemcpyAsync(c.h_box + 2, c.d_box + 2, 4 * sizeof(int64_t), hipMemcpyDeviceToHost, c.s));
   SCOUT_TRY(hipStreamSynchronize(c.s));
   c.readbacks++;
@@ -551,10 +552,11 @@ int apply_range(Ctx& c, Channel& ch, int64_t lo, int64_t hi, int conditional) {
   SCOUT_TRY(hipMemsetAsync(c.d_box, 0, 4 * sizeof(int64_t), c.s));
   SCOUT_CHECK(prof_begin(c, PROF_APPLY));
   const int64_t* pos = (ch.pos_gen == g.base_gen && ch.pos_lo <= lo && hi <= ch.pos_hi) ? ch.d_pos.p : nullptr;
+  int ks = bits_for(std::max<int64_t>(hi - lo - 1, 0));  // key bits of a line offset
   if (hi > lo)
     k_apply<<<grid_for(hi - lo, kBlock * kApplyLines), kBlock, 0, c.s>>>(g.view(), g.sorted, ch.d_uv.p, ch.d_ins.p,
                                                                          pos, lo, hi, conditional, ch.d_done.p,
-                                                                         g.nkey.p, g.ndst.p, g.nkey.cap, c.d_box);
+                                                                         g.nkey.p, g.ndst.p, g.nkey.cap, ks, c.d_box);
   // algorithmic bytes per line: the pair and its label and done flag (10 B), both rows'
   // base and inserted offsets (32 B) and the entries flipped or appended (8 B)
   SCOUT_CHECK(prof_end(c, PROF_APPLY, 50 * (hi - lo)));
@@ -565,7 +567,7 @@ int apply_range(Ctx& c, Channel& ch, int64_t lo, int64_t hi, int conditional) {
     set_error("apply: inserted-entry staging overflow");
     return SCOUT_ERR_STATE;
   }
-  if (cnt_new > 0) SCOUT_CHECK(insert_new(c, cnt_new));
+  if (cnt_new > 0) SCOUT_CHECK(insert_new(c, cnt_new, ks));
   if (g.holes > (g.live >> 1) + (1 << 20)) SCOUT_CHECK(repack_inserted(c, 2 * g.live + (1 << 20)));
   return SCOUT_OK;
 }
@@ -595,7 +597,6 @@ int graph_free(Ctx& c) {
   g.ndst.release();
   g.ndst2.release();
   g.hkey.release();
-  g.hline.release();
   g.bsamp.release();
   g.n = 0;
   g.E = g.A = g.live = g.holes = 0;
@@ -720,17 +721,13 @@ int graph_apply_lines(Ctx& c, Channel& ch, int64_t lo, int64_t hi, int condition
   DevGraph& g = c.g;
   int64_t m = hi - lo;
   if (m <= 0) return SCOUT_OK;
-  // Find lines whose pair already occurred earlier in the range.
+  // Count the lines whose pair occurs more than once in the range.
   uint64_t size = 1;
   while (size < (uint64_t)(4 * m)) size <<= 1;
   SCOUT_CHECK(g.hkey.reserve((int64_t)size));
-  SCOUT_CHECK(g.hline.reserve((int64_t)size));
   SCOUT_TRY(hipMemsetAsync(g.hkey.p, 0xff, size * sizeof(uint64_t), c.s));
-  SCOUT_TRY(hipMemsetAsync(g.hline.p, 0x7f, size * sizeof(int32_t), c.s));
   SCOUT_TRY(hipMemsetAsync(c.d_box, 0, sizeof(int64_t), c.s));
-  k_dup_insert<<<grid_for(m), kBlock, 0, c.s>>>(ch.d_uv.p + 2 * lo, m, g.hkey.p, g.hline.p, size - 1);
-  k_dup_mark<<<grid_for(m), kBlock, 0, c.s>>>(ch.d_uv.p + 2 * lo, m, g.hkey.p, g.hline.p, size - 1, ch.d_done.p + lo,
-                                              c.d_box);
+  k_dup_count<<<grid_for(m), kBlock, 0, c.s>>>(ch.d_uv.p + 2 * lo, m, g.hkey.p, size - 1, c.d_box);
   SCOUT_CHECK(box_read(c, 1));
   ch.pairs_distinct = lo == 0 && hi == ch.m && c.h_box[0] == 0;
   if (c.h_box[0] == 0) return apply_range(c, ch, lo, hi, conditional);

@@ -135,7 +135,6 @@ struct DevGraph {
   DevBuf<int32_t> ndst, ndst2;
   // duplicate-pair detection of the current batch
   DevBuf<uint64_t> hkey;
-  DevBuf<int32_t> hline;
   // base entry samples and the base generation they were taken from
   DevBuf<int32_t> bsamp;
   int64_t bsamp_gen = -1;
