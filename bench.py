@@ -78,9 +78,8 @@ def cpu_baseline(eng, n, batches_host, seconds):
 
 # Kernels of each profiled family (the launches scout_prof_* brackets on the compute stream).
 FAMILY_KERNELS = {
-    "bfs_invalidate": ["k_cand_at_level", "k_invalidate_short", "k_invalidate", "k_long_tasks", "k_inval_search",
-                       "k_inval_decide", "k_inval_push"],
-    "bfs_relax": ["k_seed_at_level", "k_relax_short", "k_relax_rows"],
+    "bfs_invalidate": ["k_invalidate_short", "k_invalidate", "k_inval_push"],
+    "bfs_relax": ["k_relax_short", "k_relax_rows"],
     "cc_expand": ["k_zero_u32", "k_cc_mark", "k_cc_collect", "k_cc_union_list", "k_cc_unreached_hook",
                   "k_cc_hook_tasks", "k_cc_final_list"],
     "apply": ["k_apply", "k_group_heads", "k_append_groups", "k_move_rows"],

@@ -81,16 +81,6 @@ __device__ __forceinline__ void block_push_chunks(bool take, int64_t f, int64_t 
   for (int64_t k = 0; k < chunks && pos + k < task_cap; k++) tasks[pos + k] = (f << 32) | k;
 }
 
-// The same for one row by a whole wave (all lanes call it with the same f and len).
-__device__ __forceinline__ void wave_push_chunks(int64_t f, int64_t len, int64_t* tasks, int64_t* task_count,
-                                                 int64_t task_cap) {
-  int64_t chunks = (len + kTask - 1) / kTask;
-  int64_t pos = 0;
-  if (lane_id() == 0) pos = (int64_t)atomicAdd((unsigned long long*)task_count, (unsigned long long)chunks);
-  pos = __shfl(pos, 0);
-  for (int64_t k = lane_id(); k < chunks && pos + k < task_cap; k += kWave) tasks[pos + k] = (f << 32) | k;
-}
-
 // Bounds of chunk task `task` of row r.
 __device__ __forceinline__ void task_range(const Row& r, int64_t task, int64_t& lo, int64_t& hi) {
   lo = (task & 0xffffffffLL) * kTask;
@@ -142,16 +132,20 @@ __device__ __forceinline__ bool relax_to(int32_t* level, int32_t* parent, int32_
 // Relax rounds
 // ---------------------------------------------------------------------------------------
 
-// Thread per frontier vertex still at level lvl. Short rows are relaxed here; rows up to
-// kTask go to the wave list, longer rows to chunk tasks.
+// Thread per frontier vertex still at level lvl, and per seed (level << 32 | vertex) of this
+// level not queued before (seeds[0..ns); ns = 0 in rounds past the deepest seed). Short rows
+// are relaxed here; rows up to kTask go to the wave list, longer rows to chunk tasks.
+// *seeded += the seeds taken.
 __global__ void k_relax_short(GView g, int32_t* level, int32_t* parent, const int32_t* front, const int64_t* count,
-                              int32_t lvl, int32_t* next, int64_t* next_count, int64_t cap, int32_t* stamp,
-                              int32_t epoch, int32_t* medium, int64_t* medium_count, int64_t* tasks,
-                              int64_t* task_count, int64_t task_cap, int64_t* scanned) {
+                              const int64_t* seeds, int64_t ns, int64_t* seeded, int32_t lvl, int32_t* next,
+                              int64_t* next_count, int64_t cap, int32_t* stamp, int32_t epoch, int32_t* medium,
+                              int64_t* medium_count, int64_t* tasks, int64_t* task_count, int64_t task_cap,
+                              int64_t* scanned) {
   int64_t nf = *count;
-  int64_t work = 0;
+  int64_t total = nf + ns;
+  int64_t work = 0, taken = 0;
   // the trip count is uniform across the block, so the block-wide appends see every thread
-  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < nf; base += (int64_t)gridDim.x * blockDim.x) {
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < total; base += (int64_t)gridDim.x * blockDim.x) {
     int64_t i = base + threadIdx.x;
     int32_t f = -1;
     Row r{0, 0, 0, 0};
@@ -159,11 +153,16 @@ __global__ void k_relax_short(GView g, int32_t* level, int32_t* parent, const in
     bool active = false;
     if (i < nf) {
       f = front[i];
-      if (level[f] == lvl) {
-        r = row_of(g, f);
-        len = row_len(r);
-        active = true;
-      }
+      active = level[f] == lvl;
+    } else if (i < total) {
+      int64_t sd = seeds[i - nf];
+      f = (int32_t)(sd & 0xffffffff);
+      active = (int32_t)(sd >> 32) == lvl && level[f] == lvl && atomicExch(&stamp[f], epoch) != epoch;
+      taken += active;
+    }
+    if (active) {
+      r = row_of(g, f);
+      len = row_len(r);
     }
     bool is_short = active && len <= kShort;
     block_append(active && !is_short && len <= kTask, f, medium, medium_count, cap);
@@ -182,6 +181,7 @@ __global__ void k_relax_short(GView g, int32_t* level, int32_t* parent, const in
     block_append_k(take, xs, next, next_count, cap, at);
   }
   block_add(work, scanned);
+  block_add(taken, seeded);
 }
 
 // Wave per medium frontier vertex (listed by k_relax_short) or per chunk task of a long one.
@@ -273,33 +273,19 @@ __global__ void k_del_candidates(const int32_t* uv, const uint8_t* ins, int64_t 
   block_min_max(lmin, lmax, &box[1], &box[2]);
 }
 
-// Candidates whose level is `lvl`, appended to the current list once (stamp dedup).
-__global__ void k_cand_at_level(const int32_t* cand, int64_t nc, const int32_t* level, int32_t lvl, int32_t* cur,
-                                int64_t* cur_count, int64_t cap, int32_t* stamp, int32_t epoch) {
-  constexpr int K = kLinesPerThread;
-  int64_t first = (int64_t)blockIdx.x * blockDim.x * K + threadIdx.x;
-  bool take[K];
-  int32_t cs[K];
-#pragma unroll
-  for (int q = 0; q < K; q++) {
-    int64_t i = first + (int64_t)q * blockDim.x;
-    cs[q] = i < nc ? cand[i] : -1;
-    take[q] = cs[q] >= 0 && level[cs[q]] == lvl && atomicExch(&stamp[cs[q]], epoch) != epoch;
-  }
-  int64_t at[K];
-  block_append_k(take, cs, cur, cur_count, cap, at);
-}
-
 // Invalidation of the candidates at level lvl. Thread per candidate; rows longer than kShort
 // go to the wave list (<= kChunk) or the long list (chunk tasks).
 __global__ void k_invalidate_short(GView g, int32_t* level, int32_t* parent, const int32_t* cur,
-                                   const int64_t* cur_count, int32_t lvl, int32_t* inval, int64_t* inval_count,
-                                   int32_t* next, int64_t* next_count, int64_t cap, int32_t* stamp, int32_t epoch,
-                                   int32_t* medium, int64_t* medium_count, int32_t* longs, int64_t* long_count,
-                                   int64_t* scanned, Attrib at) {
+                                   const int64_t* cur_count, const int32_t* cand, int64_t ncand, int64_t* listed,
+                                   int32_t lvl, int32_t* inval, int64_t* inval_count, int32_t* next,
+                                   int64_t* next_count, int64_t cap, int32_t* stamp, int32_t epoch, int32_t* medium,
+                                   int64_t* medium_count, int32_t* longs, int64_t* long_count, int64_t* tasks,
+                                   int64_t* task_count, int64_t task_cap, int64_t* scanned, Attrib at) {
   int64_t nc = *cur_count;
-  int64_t work = 0;
-  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < nc; base += (int64_t)gridDim.x * blockDim.x) {
+  int64_t total = nc + ncand;
+  int64_t work = 0, taken = 0;
+  // the trip count is uniform across the block, so the block-wide appends see every thread
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < total; base += (int64_t)gridDim.x * blockDim.x) {
     int64_t i = base + threadIdx.x;
     int32_t c = -1;
     Row r{0, 0, 0, 0};
@@ -307,6 +293,14 @@ __global__ void k_invalidate_short(GView g, int32_t* level, int32_t* parent, con
     bool active = i < nc;
     if (active) {
       c = cur[i];
+    } else if (i < total) {
+      // a deletion candidate whose level is this round's, listed once (stamp)
+      int32_t x = cand[i - nc];
+      active = level[x] == lvl && atomicExch(&stamp[x], epoch) != epoch;
+      if (active) c = x;
+      taken += active;
+    }
+    if (active) {
       r = row_of(g, c);
       len = row_len(r);
     }
@@ -332,7 +326,9 @@ __global__ void k_invalidate_short(GView g, int32_t* level, int32_t* parent, con
     }
     block_append(invalid, c, inval, inval_count, cap);
     block_append(active && !is_short && len <= kChunk, c, medium, medium_count, cap);
-    block_append(active && len > kChunk, c, longs, long_count, cap);
+    // a long row: listed, and its chunk tasks (index in longs << 32 | chunk) pushed
+    int64_t lpos = block_append(active && len > kChunk, c, longs, long_count, cap);
+    block_push_chunks(lpos >= 0, lpos, len, tasks, task_count, task_cap);
     // children of an invalid candidate: neighbours one level down (levels read above; no
     // vertex of that level changes in this kernel) whose parent it is
     int32_t pw[kShort];
@@ -348,17 +344,22 @@ __global__ void k_invalidate_short(GView g, int32_t* level, int32_t* parent, con
     block_append_k(take, xs, next, next_count, cap, at);
   }
   block_add(work, scanned);
+  block_add(taken, listed);
 }
 
 // Wave per medium candidate: a neighbour one level up supports it (and becomes its parent);
 // otherwise it is invalid and its children (neighbours one level down whose parent it is)
 // join the next level. The support search keeps the neighbours one level down in LDS, so an
-// invalid candidate then reads only their parents. own[0] += entries and parents read,
-// own[1] += rows.
+// invalid candidate then reads only their parents.
+// Then wave per chunk task of a long candidate: a neighbour one level up anywhere in the
+// chunk supports the candidate and becomes its parent (any of them will do); supported[c] =
+// supepoch records it for k_inval_push.
+// own[0] += entries and parents read, own[1] += rows and tasks.
 __global__ void k_invalidate(GView g, int32_t* level, int32_t* parent, const int32_t* cur, const int64_t* cur_count,
-                             int32_t lvl, int32_t* inval, int64_t* inval_count, int64_t inval_cap, int32_t* next,
-                             int64_t* next_count, int64_t cap, int32_t* stamp, int32_t epoch, int64_t* scanned,
-                             int64_t* own, Attrib ab) {
+                             const int32_t* longs, const int64_t* tasks, const int64_t* task_count, int64_t task_cap,
+                             int32_t* supported, int32_t supepoch, int32_t lvl, int32_t* inval, int64_t* inval_count,
+                             int64_t inval_cap, int32_t* next, int64_t* next_count, int64_t cap, int32_t* stamp,
+                             int32_t epoch, int64_t* scanned, int64_t* own, Attrib ab) {
   // neighbours one level down met while searching for support: the only possible children
   constexpr int kKids = 1024;
   constexpr int kInvSeg = 64;
@@ -369,8 +370,39 @@ __global__ void k_invalidate(GView g, int32_t* level, int32_t* parent, const int
   WaveStage<kNextSeg> st{seg_all + (threadIdx.x / kWave) * kNextSeg, 0};
   WaveStage<kInvSeg> sinv{inv_all + (threadIdx.x / kWave) * kInvSeg, 0};
   int64_t nc = *cur_count;
+  int64_t nt = min(*task_count, task_cap);
   int64_t work = 0, reads = 0, rows = 0;
-  for (int64_t i = global_wave(); i < nc; i += total_waves()) {
+  for (int64_t i = global_wave(); i < nc + nt; i += total_waves()) {
+    if (i >= nc) {
+      int64_t task = tasks[i - nc];
+      int32_t c = longs[task >> 32];
+      if (supported[c] == supepoch) continue;
+      Row r = row_of(g, c);
+      int64_t lo, hi;
+      task_range(r, task, lo, hi);
+      int64_t at = -1;
+      for (int64_t base = lo; base < hi && at < 0; base += (int64_t)kU * kWave) {
+        int32_t xs[kU];
+        load_steps(g, r, base, hi, xs);
+        int32_t lx[kU];
+#pragma unroll
+        for (int j = 0; j < kU; j++) lx[j] = xs[j] >= 0 ? level[xs[j]] : INT32_MIN;
+#pragma unroll
+        for (int j = 0; j < kU; j++) {
+          unsigned long long sup = __ballot(xs[j] >= 0 && lx[j] == lvl - 1);
+          if (sup && at < 0) at = base + (int64_t)j * kWave + (__ffsll((long long)sup) - 1);
+        }
+      }
+      int64_t nread = (at >= 0 ? at + 1 : hi) - lo;
+      work += nread;
+      reads += nread;
+      rows++;
+      if (at >= 0 && lane_id() == 0) {
+        parent[c] = row_at(g, r, at);
+        supported[c] = supepoch;
+      }
+      continue;
+    }
     int32_t c = cur[i];
     Row r = row_of(g, c);
     int64_t len = row_len(r);
@@ -461,78 +493,30 @@ __global__ void k_invalidate(GView g, int32_t* level, int32_t* parent, const int
   block_add(l0 ? rows : 0, &own[1]);
 }
 
-// Chunk tasks (index in `longs` << 32 | chunk) for every long candidate, a wave per row.
-__global__ void k_long_tasks(GView g, const int32_t* longs, const int64_t* long_count, int64_t* tasks,
-                             int64_t* task_count, int64_t task_cap) {
-  int64_t nl = *long_count;
-  for (int64_t i = global_wave(); i < nl; i += total_waves())
-    wave_push_chunks(i, row_len(row_of(g, longs[i])), tasks, task_count, task_cap);
-}
-
-// Wave per chunk: a neighbour at lvl-1 anywhere in the chunk supports the candidate and
-// becomes its parent (any of them will do).
-__global__ void k_inval_search(GView g, const int32_t* level, int32_t* parent, int32_t lvl, const int32_t* longs,
-                               const int64_t* tasks, const int64_t* task_count, int64_t task_cap, int32_t* supported,
-                               int32_t sepoch, int64_t* scanned) {
+// Wave per chunk of a long candidate that no chunk supported: it is invalid (recorded by its
+// chunk 0), and every chunk queues its children.
+__global__ void k_inval_push(GView g, int32_t* level, const int32_t* parent, int32_t lvl, const int32_t* longs,
+                             const int64_t* tasks, const int64_t* task_count, int64_t task_cap,
+                             const int32_t* supported, int32_t supepoch, int32_t* inval, int64_t* inval_count,
+                             int64_t inval_cap, int32_t* next, int64_t* next_count, int64_t cap, int32_t* stamp,
+                             int32_t epoch, int64_t* scanned, Attrib ab) {
+  constexpr int kInvSeg = 64;
+  __shared__ int32_t seg_all[kWavesPerBlock * kNextSeg];
+  __shared__ int32_t inv_all[kWavesPerBlock * kInvSeg];
+  WaveStage<kNextSeg> st{seg_all + (threadIdx.x / kWave) * kNextSeg, 0};
+  WaveStage<kInvSeg> sinv{inv_all + (threadIdx.x / kWave) * kInvSeg, 0};
   int64_t nt = min(*task_count, task_cap);
   int64_t work = 0;
   for (int64_t t = global_wave(); t < nt; t += total_waves()) {
     int64_t task = tasks[t];
     int32_t c = longs[task >> 32];
-    if (supported[c] == sepoch) continue;
-    Row r = row_of(g, c);
-    int64_t lo, hi;
-    task_range(r, task, lo, hi);
-    int64_t at = -1;
-    for (int64_t base = lo; base < hi && at < 0; base += (int64_t)kU * kWave) {
-      int32_t xs[kU];
-      load_steps(g, r, base, hi, xs);
-      int32_t lx[kU];
-#pragma unroll
-      for (int j = 0; j < kU; j++) lx[j] = xs[j] >= 0 ? level[xs[j]] : INT32_MIN;
-#pragma unroll
-      for (int j = 0; j < kU; j++) {
-        unsigned long long sup = __ballot(xs[j] >= 0 && lx[j] == lvl - 1);
-        if (sup && at < 0) at = base + (int64_t)j * kWave + (__ffsll((long long)sup) - 1);
-      }
-    }
-    work += (at >= 0 ? at + 1 : hi) - lo;
-    if (at >= 0 && lane_id() == 0) {
-      parent[c] = row_at(g, r, at);
-      supported[c] = sepoch;
-    }
-  }
-  block_add(lane_id() == 0 ? work : 0, scanned);
-}
-
-// Thread per long candidate: unsupported ones become invalid.
-__global__ void k_inval_decide(const int32_t* longs, const int64_t* long_count, const int32_t* supported,
-                               int32_t sepoch, int32_t* level, int32_t* inval, int64_t* inval_count, int64_t cap,
-                               int32_t lvl, Attrib ab) {
-  int64_t nl = *long_count;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nl; i += (int64_t)gridDim.x * blockDim.x) {
-    int32_t c = longs[i];
-    if (supported[c] != sepoch) {
+    if (supported[c] == supepoch) continue;
+    bool first = (task & 0xffffffffLL) == 0;
+    if (first && lane_id() == 0) {
       level[c] = kInvalid;
       attrib_invalid(ab, c, lvl);
-      thread_append(c, inval, inval_count, cap);
     }
-  }
-}
-
-// Wave per chunk of an invalid long candidate: queue its children.
-__global__ void k_inval_push(GView g, const int32_t* level, const int32_t* parent, int32_t lvl, const int32_t* longs,
-                             const int64_t* tasks, const int64_t* task_count, int64_t task_cap, int32_t* next,
-                             int64_t* next_count, int64_t cap, int32_t* stamp, int32_t epoch, int64_t* scanned,
-                             Attrib ab) {
-  __shared__ int32_t seg_all[kWavesPerBlock * kNextSeg];
-  WaveStage<kNextSeg> st{seg_all + (threadIdx.x / kWave) * kNextSeg, 0};
-  int64_t nt = min(*task_count, task_cap);
-  int64_t work = 0;
-  for (int64_t t = global_wave(); t < nt; t += total_waves()) {
-    int64_t task = tasks[t];
-    int32_t c = longs[task >> 32];
-    if (level[c] != kInvalid) continue;
+    stage_push(sinv, first && lane_id() == 0, c, inval, inval_count, inval_cap);
     Row r = row_of(g, c);
     int64_t lo, hi;
     task_range(r, task, lo, hi);
@@ -553,6 +537,7 @@ __global__ void k_inval_push(GView g, const int32_t* level, const int32_t* paren
       for (int j = 0; j < kU; j++) stage_push(st, take[j], ws[j], next, next_count, cap);
     }
   }
+  stage_finish(sinv, inval, inval_count, inval_cap);
   stage_finish(st, next, next_count, cap);
   block_add(lane_id() == 0 ? work : 0, scanned);
 }
@@ -743,29 +728,6 @@ __global__ void k_seed_parents(const int64_t* seeds, const int32_t* seed_par, in
   if (level[v] == lv) parent[v] = seed_par[i];
 }
 
-// Seeds at `lvl` that still hold that level join the current frontier once.
-__global__ void k_seed_at_level(const int64_t* seeds, int64_t ns, const int32_t* level, int32_t lvl, int32_t* cur,
-                                int64_t* cur_count, int64_t cap, int32_t* stamp, int32_t epoch) {
-  constexpr int K = kLinesPerThread;
-  int64_t first = (int64_t)blockIdx.x * blockDim.x * K + threadIdx.x;
-  bool take[K];
-  int32_t vs[K];
-#pragma unroll
-  for (int q = 0; q < K; q++) {
-    int64_t i = first + (int64_t)q * blockDim.x;
-    take[q] = false;
-    vs[q] = 0;
-    if (i < ns) {
-      int64_t sd = seeds[i];
-      int32_t lv = (int32_t)(sd >> 32);
-      vs[q] = (int32_t)(sd & 0xffffffff);
-      take[q] = lv == lvl && level[vs[q]] == lvl && atomicExch(&stamp[vs[q]], epoch) != epoch;
-    }
-  }
-  int64_t at[K];
-  block_append_k(take, vs, cur, cur_count, cap, at);
-}
-
 // ---------------------------------------------------------------------------------------
 // Bookkeeping
 // ---------------------------------------------------------------------------------------
@@ -777,14 +739,15 @@ __global__ void k_zero(int64_t* p, int count) {
 
 // Round bookkeeping on the device, so the host can queue several rounds between read-backs:
 // adds the round's list size to *sum (and counts the round in *nonempty if the list is not
-// empty), folds the previous round's chunk-task count into *max_tasks, then zeroes the next
-// list's count and the `zn` counters at `zero` (which may include *tasks).
+// empty or `side` says that side-list items of the round's level may exist), folds the
+// previous round's chunk-task count into *max_tasks, then zeroes the next list's count and
+// the `zn` counters at `zero` (which may include *tasks).
 __global__ void k_round_begin(const int64_t* cur_count, int64_t* next_count, const int64_t* tasks, int64_t* zero,
-                              int zn, int64_t* sum, int64_t* nonempty, int64_t* max_tasks) {
+                              int zn, int64_t* sum, int64_t* nonempty, int64_t* max_tasks, int side) {
   int64_t t = *tasks, nc = *cur_count;
   if (t > *max_tasks) *max_tasks = t;
   *sum += nc;
-  if (nc > 0) *nonempty += 1;
+  if (nc > 0 || side) *nonempty += 1;
   *next_count = 0;
   for (int i = 0; i < zn; i++) zero[i] = 0;
 }
@@ -849,12 +812,12 @@ int relax_levels(Ctx& c, int32_t* L, int32_t* P, int32_t lvl0, int32_t max_seed,
     for (int k = 0; k < group; k++) {
       int64_t* bcur = cnt[p];
       int64_t* bnext = cnt[p ^ 1];
-      if (nseeds > 0 && lvl <= max_seed)
-        k_seed_at_level<<<grid_for(nseeds, kBlock * kLinesPerThread), kBlock, 0, c.s>>>(seeds, nseeds, L, lvl, cur,
-                                                                                         bcur, cap, c.stamp, epoch);
-      k_round_begin<<<1, 1, 0, c.s>>>(bcur, bnext, box + 10, box + 10, 2, box + 15, box + 17, box + 16);
-      k_relax_short<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, P, cur, bcur, lvl, nxt, bnext, cap, c.stamp, epoch,
-                                                    c.le.p, box + 11, c.wb.p, box + 10, task_cap, box + 12);
+      // seeds of this level are taken by the round's first kernel
+      int64_t ns = (nseeds > 0 && lvl <= max_seed) ? nseeds : 0;
+      k_round_begin<<<1, 1, 0, c.s>>>(bcur, bnext, box + 10, box + 10, 2, box + 15, box + 17, box + 16, ns > 0);
+      k_relax_short<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, P, cur, bcur, seeds, ns, box + 15, lvl, nxt, bnext,
+                                                    cap, c.stamp, epoch, c.le.p, box + 11, c.wb.p, box + 10, task_cap,
+                                                    box + 12);
       k_relax_rows<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, P, c.le.p, box + 11, c.wb.p, box + 10, task_cap, lvl,
                                                    nxt, bnext, cap, c.stamp, epoch, box + 12);
       lvl++;
@@ -994,24 +957,25 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, int32_t* P, int strategy, scou
       for (int k = 0; k < group; k++) {
         int64_t* bcur = cnt[p];
         int64_t* bnext = cnt[p ^ 1];
-        if (lvl <= cmax)
-          k_cand_at_level<<<grid_for(ncand, kBlock * kLinesPerThread), kBlock, 0, c.s>>>(c.la.p, ncand, L, lvl, cur,
-                                                                                           bcur, cap, c.stamp, sepoch);
-        k_round_begin<<<1, 1, 0, c.s>>>(bcur, bnext, box + 14, box + 12, 3, box + 18, box + 20, box + 19);
-        k_invalidate_short<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, P, cur, bcur, lvl, c.ld.p, binval, nxt,
-                                                           bnext, cap, c.stamp, sepoch, c.le.p, box + 12, c.lf.p,
-                                                           box + 13, bscan, ab);
-        SCOUT_CHECK(prof_begin(c, PROF_BFS_INVALIDATE_KERNEL));
-        k_invalidate<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, P, c.le.p, box + 12, lvl, c.ld.p, binval, cap,
-                                                     nxt, bnext, cap, c.stamp, sepoch, bscan, box + 21, ab);
-        SCOUT_CHECK(prof_end(c, PROF_BFS_INVALIDATE_KERNEL, 0));
-        k_long_tasks<<<64, kBlock, 0, c.s>>>(c.g.view(), c.lf.p, box + 13, c.wb.p, box + 14, c.wb.cap);
+        // candidates of this level are taken by the round's first kernel, which also lists the
+        // long rows and their chunk tasks; k_invalidate searches the medium rows and the
+        // tasks; k_inval_push settles the long rows no task supported
+        int64_t nlevel = lvl <= cmax ? ncand : 0;
+        k_round_begin<<<1, 1, 0, c.s>>>(bcur, bnext, box + 14, box + 12, 3, box + 18, box + 20, box + 19,
+                                        nlevel > 0);
+        k_invalidate_short<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, P, cur, bcur, c.la.p, nlevel, box + 18, lvl,
+                                                           c.ld.p, binval, nxt, bnext, cap, c.stamp, sepoch, c.le.p,
+                                                           box + 12, c.lf.p, box + 13, c.wb.p, box + 14, c.wb.cap,
+                                                           bscan, ab);
         int32_t supepoch = next_epoch(c);
-        k_inval_search<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, P, lvl, c.lf.p, c.wb.p, box + 14, c.wb.cap,
-                                                       c.aux, supepoch, bscan);
-        k_inval_decide<<<64, kBlock, 0, c.s>>>(c.lf.p, box + 13, c.aux, supepoch, L, c.ld.p, binval, cap, lvl, ab);
-        k_inval_push<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, P, lvl, c.lf.p, c.wb.p, box + 14, c.wb.cap, nxt,
-                                                     bnext, cap, c.stamp, sepoch, bscan, ab);
+        SCOUT_CHECK(prof_begin(c, PROF_BFS_INVALIDATE_KERNEL));
+        k_invalidate<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, P, c.le.p, box + 12, c.lf.p, c.wb.p, box + 14,
+                                                     c.wb.cap, c.aux, supepoch, lvl, c.ld.p, binval, cap, nxt, bnext,
+                                                     cap, c.stamp, sepoch, bscan, box + 21, ab);
+        SCOUT_CHECK(prof_end(c, PROF_BFS_INVALIDATE_KERNEL, 0));
+        k_inval_push<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, P, lvl, c.lf.p, c.wb.p, box + 14, c.wb.cap, c.aux,
+                                                     supepoch, c.ld.p, binval, cap, nxt, bnext, cap, c.stamp, sepoch,
+                                                     bscan, ab);
         lvl++;
         std::swap(cur, nxt);
         p ^= 1;
