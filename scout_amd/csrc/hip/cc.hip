@@ -181,35 +181,47 @@ __global__ void k_cc_collect(const int32_t* L, int32_t* label, const uint32_t* b
   __shared__ int32_t seg[kWavesPerBlock * kSeg];
   WaveStage<kSeg> ws{seg + (threadIdx.x / kWave) * kSeg, 0};
   int64_t n4 = ((int64_t)n + 3) / 4;
-  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n4; base += (int64_t)gridDim.x * blockDim.x) {
-    int64_t q = base + threadIdx.x;
-    int32_t ls[4] = {-1, -1, -1, -1}, lab[4] = {0, 0, 0, 0};
-    if (q < n4) {
-      if (4 * q + 4 <= n) {
-        int4 a = reinterpret_cast<const int4*>(L)[q], b = reinterpret_cast<const int4*>(label)[q];
-        ls[0] = a.x, ls[1] = a.y, ls[2] = a.z, ls[3] = a.w;
-        lab[0] = b.x, lab[1] = b.y, lab[2] = b.z, lab[3] = b.w;
-      } else {
-        for (int k = 0; k < 4 && 4 * q + k < n; k++) ls[k] = L[4 * q + k], lab[k] = label[4 * q + k];
+  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  // two groups of four vertices per thread and iteration: both groups' loads are in flight
+  // before either is used
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n4; base += 2 * stride) {
+    int32_t ls[2][4], lab[2][4];
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      int64_t q = base + h * stride + threadIdx.x;
+#pragma unroll
+      for (int k = 0; k < 4; k++) ls[h][k] = -1, lab[h][k] = 0;
+      if (q < n4) {
+        if (4 * q + 4 <= n) {
+          int4 a = reinterpret_cast<const int4*>(L)[q], b = reinterpret_cast<const int4*>(label)[q];
+          ls[h][0] = a.x, ls[h][1] = a.y, ls[h][2] = a.z, ls[h][3] = a.w;
+          lab[h][0] = b.x, lab[h][1] = b.y, lab[h][2] = b.z, lab[h][3] = b.w;
+        } else {
+          for (int k = 0; k < 4 && 4 * q + k < n; k++) ls[h][k] = L[4 * q + k], lab[h][k] = label[4 * q + k];
+        }
       }
     }
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
-      int64_t v = 4 * q + k;
-      bool take = false;
-      if (q < n4 && v < n) {
-        int32_t l = lab[k];
-        if (ls[k] >= 0) {
-          if (l != 0) label[v] = 0;
-        } else if (l == 0 || (((coarse[l >> (5 + kCoarseShift)] >> ((l >> kCoarseShift) & 31)) & 1u) &&
-                              ((bits[l >> 5] >> (l & 31)) & 1u))) {
-          take = true;
-          par[v] = (int32_t)v;
-        } else {
-          roots += l == v;
+    for (int h = 0; h < 2; h++) {
+      int64_t q = base + h * stride + threadIdx.x;
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        int64_t v = 4 * q + k;
+        bool take = false;
+        if (q < n4 && v < n) {
+          int32_t l = lab[h][k];
+          if (ls[h][k] >= 0) {
+            if (l != 0) label[v] = 0;
+          } else if (l == 0 || (((coarse[l >> (5 + kCoarseShift)] >> ((l >> kCoarseShift) & 31)) & 1u) &&
+                                ((bits[l >> 5] >> (l & 31)) & 1u))) {
+            take = true;
+            par[v] = (int32_t)v;
+          } else {
+            roots += l == v;
+          }
         }
+        stage_push(ws, take, (int32_t)v, list, count, cap);
       }
-      stage_push(ws, take, (int32_t)v, list, count, cap);
     }
   }
   stage_finish(ws, list, count, cap);
