@@ -41,7 +41,7 @@ namespace {
 constexpr int kShort = 16;
 // Entries per chunk task of a split row, and the longest row the relax and tentative passes
 // give one wave: a wave walks at most kTask entries (kTask / (kU * kWave) dependent steps),
-// so one long row no longer sets a round's duration.
+// so one long row does not set a round's duration.
 constexpr int64_t kTask = 512;
 // Row steps per iteration of the wave walks: kU entries per lane loaded before any is used.
 constexpr int kU = 4;
@@ -273,8 +273,11 @@ __global__ void k_del_candidates(const int32_t* uv, const uint8_t* ins, int64_t 
   block_min_max(lmin, lmax, &box[1], &box[2]);
 }
 
-// Invalidation of the candidates at level lvl. Thread per candidate; rows longer than kShort
-// go to the wave list (<= kChunk) or the long list (chunk tasks).
+// Invalidation of the candidates at level lvl: the children queued by the previous round
+// (cur) and the batch's deletion candidates of this level (cand[0..ncand), filtered by level
+// and stamp; ncand = 0 past the deepest candidate; *listed += those taken). Thread per
+// candidate; rows longer than kShort go to the wave list (<= kChunk) or the long list, whose
+// chunk tasks are pushed here.
 __global__ void k_invalidate_short(GView g, int32_t* level, int32_t* parent, const int32_t* cur,
                                    const int64_t* cur_count, const int32_t* cand, int64_t ncand, int64_t* listed,
                                    int32_t lvl, int32_t* inval, int64_t* inval_count, int32_t* next,
