@@ -244,6 +244,21 @@ enum {
 int scout_prof_enable(scout_ctx* ctx, int on);
 int scout_prof_read(scout_ctx* ctx, int id, double* total_ms, int64_t* launches, int64_t* bytes);
 
+/* Cumulative counts, since the context was created, of the long-row code paths the repairs
+ * took (rows longer than one wave's share are split into chunk tasks walked by many waves).
+ * No counterpart in the reference, whose kernels give every row to one thread; the counts let
+ * tests show that a workload reached these paths. out[0..count), count <= SCOUT_PATH_COUNT. */
+enum {
+  SCOUT_PATH_INVAL_TASKS = 0,        /* BFS invalidation: chunk tasks searched for support */
+  SCOUT_PATH_INVAL_PUSH_TASKS = 1,   /* BFS invalidation: chunk tasks of unsupported long rows */
+  SCOUT_PATH_INVAL_KIDS_OVERFLOW = 2,/* BFS invalidation: invalid rows whose children were re-read */
+  SCOUT_PATH_CC_TASKS = 3,           /* CC union-find: chunk tasks of long rows */
+  SCOUT_PATH_RELAX_TASKS = 4,        /* BFS relax: chunk tasks of long rows */
+  SCOUT_PATH_TENTATIVE_TASKS = 5,    /* BFS tentative levels: chunk tasks of long rows */
+  SCOUT_PATH_COUNT = 6
+};
+int scout_path_counts(scout_ctx* ctx, int64_t* out, int count);
+
 /* Learned scheduler. No counterpart in the reference: it adds a small MLP (12 features, two
  * hidden layers of 64, 3 outputs; bf16 weights, fp32 accumulation) trained on the device on
  * samples logged by the plain BFS repair. Per deleted parent edge it predicts whether the

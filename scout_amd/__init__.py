@@ -90,6 +90,7 @@ _SIGNATURES = {
     "scout_prof_read": [_P, ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_I64),
                         ctypes.POINTER(_I64)],
     "scout_batch_append": [_P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_I64)],
+    "scout_path_counts": [_P, _P, ctypes.c_int],
     "scout_apply_segments": [_P, ctypes.c_int, _I32, _P, ctypes.c_int, ctypes.c_int, _P],
     "scout_mlp_init": [_P, ctypes.c_uint64],
     "scout_mlp_set_params": [_P, _P],
@@ -109,6 +110,8 @@ PROF_BFS_RELAX, PROF_BFS_INVALIDATE, PROF_CC_EXPAND, PROF_APPLY, PROF_CLASSIFY =
 PROF_COLOR, PROF_MIS_EVAL, PROF_CC_HOOK, PROF_BFS_INVALIDATE_KERNEL = 5, 6, 7, 8
 PROF_NAMES = ["bfs_relax", "bfs_invalidate", "cc_expand", "apply", "classify", "color", "mis_eval", "cc_hook",
               "bfs_invalidate_kernel"]
+PATH_NAMES = ["inval_tasks", "inval_push_tasks", "inval_kids_overflow", "cc_tasks", "relax_tasks",
+              "tentative_tasks"]
 
 _lib = None
 
@@ -391,6 +394,12 @@ class Engine:
         self._check(self.lib.scout_prof_read(self.h, kid, ctypes.byref(ms), ctypes.byref(cnt), ctypes.byref(by)),
                     "prof_read")
         return {"ms": ms.value, "launches": cnt.value, "bytes": by.value}
+
+    def path_counts(self) -> dict:
+        """Cumulative counts of the long-row code paths (chunk tasks per kernel family)."""
+        out = np.zeros(len(PATH_NAMES), dtype=np.int64)
+        self._check(self.lib.scout_path_counts(self.h, _ptr(out), len(PATH_NAMES)), "path_counts")
+        return {k: int(v) for k, v in zip(PATH_NAMES, out)}
 
 
     # --- learned scheduler -------------------------------------------------------------

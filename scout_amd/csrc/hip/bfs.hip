@@ -188,12 +188,12 @@ __global__ void k_relax_short(GView g, int32_t* level, int32_t* parent, const in
 __global__ void k_relax_rows(GView g, int32_t* level, int32_t* parent, const int32_t* medium,
                              const int64_t* medium_count, const int64_t* tasks, const int64_t* task_count,
                              int64_t task_cap, int32_t lvl, int32_t* next, int64_t* next_count, int64_t cap,
-                             int32_t* stamp, int32_t epoch, int64_t* scanned) {
+                             int32_t* stamp, int32_t epoch, int64_t* scanned, int64_t* paths) {
   __shared__ int32_t seg_all[kWavesPerBlock * kNextSeg];
   WaveStage<kNextSeg> st{seg_all + (threadIdx.x / kWave) * kNextSeg, 0};
   int64_t nm = *medium_count;
   int64_t nt = min(*task_count, task_cap);
-  int64_t work = 0;
+  int64_t work = 0, ntask = 0;
   for (int64_t t = global_wave(); t < nm + nt; t += total_waves()) {
     int32_t f;
     int64_t lo, hi;
@@ -208,6 +208,7 @@ __global__ void k_relax_rows(GView g, int32_t* level, int32_t* parent, const int
       f = (int32_t)(task >> 32);
       r = row_of(g, f);
       task_range(r, task, lo, hi);
+      ntask++;
     }
     work += hi - lo;
     for (int64_t base = lo; base < hi; base += (int64_t)kU * kWave) {
@@ -225,6 +226,7 @@ __global__ void k_relax_rows(GView g, int32_t* level, int32_t* parent, const int
     }
   }
   stage_finish(st, next, next_count, cap);
+  count_path(ntask, &paths[PATH_RELAX_TASKS]);
   block_add(lane_id() == 0 ? work : 0, scanned);
 }
 
@@ -362,7 +364,7 @@ __global__ void k_invalidate(GView g, int32_t* level, int32_t* parent, const int
                              const int32_t* longs, const int64_t* tasks, const int64_t* task_count, int64_t task_cap,
                              int32_t* supported, int32_t supepoch, int32_t lvl, int32_t* inval, int64_t* inval_count,
                              int64_t inval_cap, int32_t* next, int64_t* next_count, int64_t cap, int32_t* stamp,
-                             int32_t epoch, int64_t* scanned, int64_t* own, Attrib ab) {
+                             int32_t epoch, int64_t* scanned, int64_t* own, int64_t* paths, Attrib ab) {
   // neighbours one level down met while searching for support: the only possible children
   constexpr int kKids = 1024;
   constexpr int kInvSeg = 64;
@@ -374,12 +376,13 @@ __global__ void k_invalidate(GView g, int32_t* level, int32_t* parent, const int
   WaveStage<kInvSeg> sinv{inv_all + (threadIdx.x / kWave) * kInvSeg, 0};
   int64_t nc = *cur_count;
   int64_t nt = min(*task_count, task_cap);
-  int64_t work = 0, reads = 0, rows = 0;
+  int64_t work = 0, reads = 0, rows = 0, ntask = 0, overflow = 0;
   for (int64_t i = global_wave(); i < nc + nt; i += total_waves()) {
     if (i >= nc) {
       int64_t task = tasks[i - nc];
       int32_t c = longs[task >> 32];
       if (supported[c] == supepoch) continue;
+      ntask++;
       Row r = row_of(g, c);
       int64_t lo, hi;
       task_range(r, task, lo, hi);
@@ -470,6 +473,7 @@ __global__ void k_invalidate(GView g, int32_t* level, int32_t* parent, const int
       }
     } else {
       reads += 2 * len;
+      overflow++;
       for (int64_t base = 0; base < len; base += (int64_t)kU * kWave) {
         int32_t ws[kU];
         load_steps(g, r, base, len, ws);
@@ -489,6 +493,8 @@ __global__ void k_invalidate(GView g, int32_t* level, int32_t* parent, const int
   }
   stage_finish(st, next, next_count, cap);
   stage_finish(sinv, inval, inval_count, inval_cap);
+  count_path(ntask, &paths[PATH_INVAL_TASKS]);
+  count_path(overflow, &paths[PATH_INVAL_KIDS_OVERFLOW]);
   // work, reads and rows are uniform across the wave: counted once per wave
   bool l0 = lane_id() == 0;
   block_add(l0 ? work : 0, scanned);
@@ -502,18 +508,19 @@ __global__ void k_inval_push(GView g, int32_t* level, const int32_t* parent, int
                              const int64_t* tasks, const int64_t* task_count, int64_t task_cap,
                              const int32_t* supported, int32_t supepoch, int32_t* inval, int64_t* inval_count,
                              int64_t inval_cap, int32_t* next, int64_t* next_count, int64_t cap, int32_t* stamp,
-                             int32_t epoch, int64_t* scanned, Attrib ab) {
+                             int32_t epoch, int64_t* scanned, int64_t* paths, Attrib ab) {
   constexpr int kInvSeg = 64;
   __shared__ int32_t seg_all[kWavesPerBlock * kNextSeg];
   __shared__ int32_t inv_all[kWavesPerBlock * kInvSeg];
   WaveStage<kNextSeg> st{seg_all + (threadIdx.x / kWave) * kNextSeg, 0};
   WaveStage<kInvSeg> sinv{inv_all + (threadIdx.x / kWave) * kInvSeg, 0};
   int64_t nt = min(*task_count, task_cap);
-  int64_t work = 0;
+  int64_t work = 0, ntask = 0;
   for (int64_t t = global_wave(); t < nt; t += total_waves()) {
     int64_t task = tasks[t];
     int32_t c = longs[task >> 32];
     if (supported[c] == supepoch) continue;
+    ntask++;
     bool first = (task & 0xffffffffLL) == 0;
     if (first && lane_id() == 0) {
       level[c] = kInvalid;
@@ -542,6 +549,7 @@ __global__ void k_inval_push(GView g, int32_t* level, const int32_t* parent, int
   }
   stage_finish(sinv, inval, inval_count, inval_cap);
   stage_finish(st, next, next_count, cap);
+  count_path(ntask, &paths[PATH_INVAL_PUSH_TASKS]);
   block_add(lane_id() == 0 ? work : 0, scanned);
 }
 
@@ -595,10 +603,10 @@ __device__ __forceinline__ uint64_t wave_min_u64(uint64_t v) {
 // Wave per medium row (index list) or per chunk of a long row; atomicMin into tent.
 __global__ void k_tentative_rows(GView g, const int32_t* level, const int32_t* xs, const int32_t* medium,
                                  const int64_t* medium_count, const int64_t* tasks, const int64_t* task_count,
-                                 int64_t task_cap, uint64_t* tent, int64_t* scanned) {
+                                 int64_t task_cap, uint64_t* tent, int64_t* scanned, int64_t* paths) {
   int64_t nm = *medium_count;
   int64_t nt = min(*task_count, task_cap);
-  int64_t work = 0;
+  int64_t work = 0, ntask = 0;
   for (int64_t t = global_wave(); t < nm + nt; t += total_waves()) {
     int64_t i, lo, hi;
     Row r;
@@ -612,6 +620,7 @@ __global__ void k_tentative_rows(GView g, const int32_t* level, const int32_t* x
       i = task >> 32;
       r = row_of(g, xs[i]);
       task_range(r, task, lo, hi);
+      ntask++;
     }
     work += hi - lo;
     uint64_t best = UINT64_MAX;
@@ -628,6 +637,7 @@ __global__ void k_tentative_rows(GView g, const int32_t* level, const int32_t* x
     best = wave_min_u64(best);
     if (lane_id() == 0 && best != UINT64_MAX) atomicMin((unsigned long long*)&tent[i], (unsigned long long)best);
   }
+  count_path(ntask, &paths[PATH_TENTATIVE_TASKS]);
   block_add(lane_id() == 0 ? work : 0, scanned);
 }
 
@@ -822,7 +832,7 @@ int relax_levels(Ctx& c, int32_t* L, int32_t* P, int32_t lvl0, int32_t max_seed,
                                                     cap, c.stamp, epoch, c.le.p, box + 11, c.wb.p, box + 10, task_cap,
                                                     box + 12);
       k_relax_rows<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, P, c.le.p, box + 11, c.wb.p, box + 10, task_cap, lvl,
-                                                   nxt, bnext, cap, c.stamp, epoch, box + 12);
+                                                   nxt, bnext, cap, c.stamp, epoch, box + 12, c.d_paths);
       lvl++;
       std::swap(cur, nxt);
       p ^= 1;
@@ -974,11 +984,11 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, int32_t* P, int strategy, scou
         SCOUT_CHECK(prof_begin(c, PROF_BFS_INVALIDATE_KERNEL));
         k_invalidate<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, P, c.le.p, box + 12, c.lf.p, c.wb.p, box + 14,
                                                      c.wb.cap, c.aux, supepoch, lvl, c.ld.p, binval, cap, nxt, bnext,
-                                                     cap, c.stamp, sepoch, bscan, box + 21, ab);
+                                                     cap, c.stamp, sepoch, bscan, box + 21, c.d_paths, ab);
         SCOUT_CHECK(prof_end(c, PROF_BFS_INVALIDATE_KERNEL, 0));
         k_inval_push<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, P, lvl, c.lf.p, c.wb.p, box + 14, c.wb.cap, c.aux,
                                                      supepoch, c.ld.p, binval, cap, nxt, bnext, cap, c.stamp, sepoch,
-                                                     bscan, ab);
+                                                     bscan, c.d_paths, ab);
         lvl++;
         std::swap(cur, nxt);
         p ^= 1;
@@ -1017,7 +1027,7 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, int32_t* P, int strategy, scou
       k_tentative_short<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, c.ld.p, nx, tent, c.lf.p, box + 12, c.lf.cap,
                                                         c.wb.p, box + 13, c.wb.cap, box + 11);
       k_tentative_rows<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, c.ld.p, c.lf.p, box + 12, c.wb.p, box + 13,
-                                                       c.wb.cap, tent, box + 11);
+                                                       c.wb.cap, tent, box + 11, c.d_paths);
       k_apply_tentative<<<grid_for(nx, kBlock * kLinesPerThread), kBlock, 0, c.s>>>(c.ld.p, tent, nx, L, P, c.wa.p,
                                                                                     c.lg.p, seed_cap, box);
     }

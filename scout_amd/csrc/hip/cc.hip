@@ -88,10 +88,11 @@ __global__ void k_cc_hook(GView g, int32_t* parent, int64_t* tasks, int64_t* tas
 }
 
 __global__ void k_cc_hook_tasks(GView g, int32_t* parent, const int64_t* tasks, const int64_t* task_count,
-                                int64_t task_cap, int64_t* scanned) {
+                                int64_t task_cap, int64_t* scanned, int64_t* paths) {
   int64_t nt = min(*task_count, task_cap);
-  int64_t work = 0;
+  int64_t work = 0, ntask = 0;
   for (int64_t i = global_wave(); i < nt; i += total_waves()) {
+    ntask++;
     int64_t t = tasks[i];
     int32_t v = (int32_t)(t >> 32);
     int64_t chunk = t & 0xffffffffLL;
@@ -104,6 +105,7 @@ __global__ void k_cc_hook_tasks(GView g, int32_t* parent, const int64_t* tasks, 
       if (w > v) uf_union(parent, v, w, &h);
     });
   }
+  count_path(ntask, &paths[PATH_CC_TASKS]);
   if (lane_id() == 0 && work) atomicAdd((unsigned long long*)scanned, (unsigned long long)work);
 }
 
@@ -279,7 +281,8 @@ int cc_full(Ctx& c, int64_t* scanned) {
   SCOUT_TRY(hipMemsetAsync(c.d_box, 0, 2 * sizeof(int64_t), c.s));
   SCOUT_CHECK(prof_begin(c, PROF_CC_HOOK));
   k_cc_hook<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), c.label, c.wb.p, c.d_box, c.wb.cap, c.d_box + 1);
-  k_cc_hook_tasks<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), c.label, c.wb.p, c.d_box, c.wb.cap, c.d_box + 1);
+  k_cc_hook_tasks<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), c.label, c.wb.p, c.d_box, c.wb.cap, c.d_box + 1,
+                                                  c.d_paths);
   k_cc_flatten<<<grid_for(n), kBlock, 0, c.s>>>(c.label, n);
   SCOUT_CHECK(prof_end(c, PROF_CC_HOOK, 0));
   SCOUT_CHECK(box_read(c, 2));
@@ -343,7 +346,7 @@ int cc_local(Ctx& c, Channel& ch, scout_stats* st) {
                                                   box + 2);
   k_cc_unreached_hook<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), c.lb.p, box + 5, par, c.wb.p, box + 1, c.wb.cap,
                                                       box + 2);
-  k_cc_hook_tasks<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), par, c.wb.p, box + 1, c.wb.cap, box + 2);
+  k_cc_hook_tasks<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), par, c.wb.p, box + 1, c.wb.cap, box + 2, c.d_paths);
   k_cc_final_list<<<kListGrid, kBlock, 0, c.s>>>(c.la.p, box, par, c.label, box + 3);
   SCOUT_CHECK(prof_end(c, PROF_CC_EXPAND, 0));
   SCOUT_CHECK(box_read(c, 6));

@@ -41,6 +41,12 @@ __device__ __forceinline__ int64_t global_wave() {
 }
 __device__ __forceinline__ int64_t total_waves() { return ((int64_t)gridDim.x * blockDim.x) / kWave; }
 
+// Adds a wave's count of a long-row code path to its device counter (one atomic per wave, none
+// when the wave never took the path). cnt must be uniform across the wave.
+__device__ __forceinline__ void count_path(int64_t cnt, int64_t* p) {
+  if (lane_id() == 0 && cnt) atomicAdd((unsigned long long*)p, (unsigned long long)cnt);
+}
+
 // Lowers *p to cand if cand is smaller (-1 counts as infinity). True if this call lowered it.
 __device__ __forceinline__ bool cas_min_level(int32_t* p, int32_t cand) {
   int32_t old = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);

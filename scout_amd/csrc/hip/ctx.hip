@@ -240,12 +240,15 @@ int scout_ctx_create(int gpu_id, const scout_budget* budget, scout_ctx** out) {
   if (budget) c.budget = *budget;
   if (hipStreamCreateWithFlags(&c.s, hipStreamNonBlocking) != hipSuccess ||
       hipStreamCreateWithFlags(&c.cs, hipStreamNonBlocking) != hipSuccess ||
-      hipHostMalloc((void**)&c.h_box, 64 * sizeof(int64_t), hipHostMallocDefault) != hipSuccess ||
-      hipMalloc((void**)&c.d_box, 64 * sizeof(int64_t)) != hipSuccess) {
+      hipHostMalloc((void**)&c.h_box, (kBoxSlots + kPathCount) * sizeof(int64_t), hipHostMallocDefault) !=
+          hipSuccess ||
+      hipMalloc((void**)&c.d_box, (kBoxSlots + kPathCount) * sizeof(int64_t)) != hipSuccess ||
+      hipMemset(c.d_box, 0, (kBoxSlots + kPathCount) * sizeof(int64_t)) != hipSuccess) {
     set_error("context set-up failed");
     delete x;
     return SCOUT_ERR_HIP;
   }
+  c.d_paths = c.d_box + kBoxSlots;
   for (int k = 0; k < 2; k++) {
     if (hipEventCreateWithFlags(&c.ch[k].staged, hipEventDisableTiming) != hipSuccess) {
       set_error("event creation failed");
@@ -580,6 +583,18 @@ int scout_prof_read(scout_ctx* x, int id, double* total_ms, int64_t* launches, i
   if (total_ms) *total_ms = ms;
   if (launches) *launches = cnt;
   if (bytes) *bytes = by;
+  return SCOUT_OK;
+}
+
+int scout_path_counts(scout_ctx* x, int64_t* out, int count) {
+  SCOUT_CHECK(check_ctx(x));
+  Ctx& c = x->c;
+  if (!out || count < 0 || count > kPathCount) {
+    set_error("path_counts: count must be 0.." + std::to_string(kPathCount));
+    return SCOUT_ERR_ARG;
+  }
+  SCOUT_TRY(hipStreamSynchronize(c.s));
+  if (count > 0) SCOUT_TRY(hipMemcpy(out, c.d_paths, (size_t)count * sizeof(int64_t), hipMemcpyDeviceToHost));
   return SCOUT_OK;
 }
 

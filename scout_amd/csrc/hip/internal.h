@@ -206,9 +206,11 @@ struct Ctx {
   int64_t label_seq = -1;        // apply_seq at which `label` was last made exact
   int64_t mis_version = 0;       // bumped when `mis` is overwritten outside the MIS repair
   DevBuf<uint32_t> cc_bits;      // CC: bitmap of labels touched by the batch
-  // small mailbox read back by the host after a round
+  // small mailbox read back by the host after a round (kBoxSlots entries)
   int64_t* h_box = nullptr;    // pinned
   int64_t* d_box = nullptr;
+  // cumulative counts of the long-row code paths (kPathCount entries after the mailbox)
+  int64_t* d_paths = nullptr;
   // generic work lists
   DevBuf<int32_t> la, lb, lc, ld, le, lf, lg;
   DevBuf<int64_t> wa, wb, wc;
@@ -229,6 +231,12 @@ struct Ctx {
   int64_t readbacks = 0;         // host read-backs of device counters so far
   double pred_over = 0;          // running mean of how far the predicted last level overshoots
 };
+
+constexpr int kBoxSlots = 64;
+// Long-row code paths counted on the device (scout_path_counts): chunk tasks of split rows
+// per kernel family, and invalid rows whose children did not fit the LDS list.
+enum PathId { PATH_INVAL_TASKS = 0, PATH_INVAL_PUSH_TASKS, PATH_INVAL_KIDS_OVERFLOW, PATH_CC_TASKS,
+              PATH_RELAX_TASKS, PATH_TENTATIVE_TASKS, kPathCount };
 
 // prims.hip (library scans/sorts)
 int scan_exclusive_i64(Ctx& c, const int64_t* in, int64_t* out, int64_t count);
