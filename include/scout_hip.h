@@ -174,6 +174,15 @@ int scout_color_autotune(scout_ctx* ctx, int channel, double* best_ms);
  * CC replaces full_recompute_cc_gpu (src/cc/scout_cc.cu:118-278). */
 int scout_bfs_full(scout_ctx* ctx, int32_t source);
 int scout_cc_full(scout_ctx* ctx);
+/* scout_cc_full without any allocation: the min-id labels of the context's current graph go
+ * to d_label, a caller-owned device array of n int32 on the context's GPU; the context's own
+ * labels are left as they are. No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under stream capture, serving loops, harnesses that forbid
+ * hipMalloc). Chunk tasks and counters live in a fixed 16 MiB device array per GPU (2^21
+ * tasks: graphs of up to about 2^32 adjacency entries; larger ones return SCOUT_ERR_ARG).
+ * Every call on one GPU shares that array, so calls that overlap on one GPU (two contexts,
+ * two threads) race; calls from one thread in sequence are safe. */
+int scout_cc_full_noalloc(scout_ctx* ctx, int32_t* d_label);
 /* Greedy-ascending initial MIS and colouring (tools/compute_initial_mis.py:32-43,
  * tools/greedy_color.cpp:37-52), computed on the device. */
 int scout_mis_greedy(scout_ctx* ctx);

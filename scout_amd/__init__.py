@@ -73,6 +73,7 @@ _SIGNATURES = {
     "scout_color_autotune": [_P, ctypes.c_int, ctypes.POINTER(ctypes.c_double)],
     "scout_bfs_full": [_P, _I32],
     "scout_cc_full": [_P],
+    "scout_cc_full_noalloc": [_P, _P],
     "scout_mis_greedy": [_P],
     "scout_color_greedy": [_P],
     "scout_sync": [_P],
@@ -328,6 +329,11 @@ class Engine:
 
     def cc_full(self):
         self._check(self.lib.scout_cc_full(self.h), "cc_full")
+
+    def cc_full_noalloc(self, d_label: int):
+        """Full CC recompute into a caller-owned device array (address d_label, n int32) with
+        no device allocation; the context's own labels are left unchanged."""
+        self._check(self.lib.scout_cc_full_noalloc(self.h, ctypes.c_void_p(d_label)), "cc_full_noalloc")
 
     def mis_greedy(self):
         self._check(self.lib.scout_mis_greedy(self.h), "mis_greedy")

@@ -109,6 +109,11 @@ __global__ void k_cc_hook_tasks(GView g, int32_t* parent, const int64_t* tasks, 
   if (lane_id() == 0 && work) atomicAdd((unsigned long long*)scanned, (unsigned long long)work);
 }
 
+// Zeroes `count` counters with a kernel (no memset node, so a captured stream replays it).
+__global__ void k_cc_zero_counters(int64_t* p, int count) {
+  if ((int)threadIdx.x < count) p[threadIdx.x] = 0;
+}
+
 __global__ void k_cc_flatten(int32_t* parent, int32_t n) {
   int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (v < n) parent[v] = uf_find(parent, (int32_t)v);
@@ -273,17 +278,26 @@ __global__ void k_cc_final_list(const int32_t* list, const int64_t* count, int32
 
 }  // namespace
 
+int64_t cc_full_task_bound(const Ctx& c) { return 2 * (c.g.E + c.g.A) / kChunk + 1024; }
+
+int cc_full_into(Ctx& c, int32_t* label, int64_t* tasks, int64_t task_cap, int64_t* counters) {
+  int32_t n = c.g.n;
+  k_iota<<<grid_for(n), kBlock, 0, c.s>>>(label, n);
+  k_cc_zero_counters<<<1, 64, 0, c.s>>>(counters, 2);
+  k_cc_hook<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), label, tasks, counters, task_cap, counters + 1);
+  k_cc_hook_tasks<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), label, tasks, counters, task_cap, counters + 1,
+                                                  c.d_paths);
+  k_cc_flatten<<<grid_for(n), kBlock, 0, c.s>>>(label, n);
+  SCOUT_TRY(hipGetLastError());
+  return SCOUT_OK;
+}
+
 int cc_full(Ctx& c, int64_t* scanned) {
   int32_t n = c.g.n;
   if (n == 0) return SCOUT_OK;
-  SCOUT_CHECK(c.wb.reserve(2 * (c.g.E + c.g.A) / kChunk + 1024));
-  k_iota<<<grid_for(n), kBlock, 0, c.s>>>(c.label, n);
-  SCOUT_TRY(hipMemsetAsync(c.d_box, 0, 2 * sizeof(int64_t), c.s));
+  SCOUT_CHECK(c.wb.reserve(cc_full_task_bound(c)));
   SCOUT_CHECK(prof_begin(c, PROF_CC_HOOK));
-  k_cc_hook<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), c.label, c.wb.p, c.d_box, c.wb.cap, c.d_box + 1);
-  k_cc_hook_tasks<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), c.label, c.wb.p, c.d_box, c.wb.cap, c.d_box + 1,
-                                                  c.d_paths);
-  k_cc_flatten<<<grid_for(n), kBlock, 0, c.s>>>(c.label, n);
+  SCOUT_CHECK(cc_full_into(c, c.label, c.wb.p, c.wb.cap, c.d_box));
   SCOUT_CHECK(prof_end(c, PROF_CC_HOOK, 0));
   SCOUT_CHECK(box_read(c, 2));
   if (c.h_box[0] > c.wb.cap) {

@@ -234,6 +234,7 @@ int scout_ctx_create(int gpu_id, const scout_budget* budget, scout_ctx** out) {
   }
   SCOUT_TRY(hipSetDevice(gpu_id));
   SCOUT_TRY(hipFree(nullptr));
+  SCOUT_CHECK(noalloc_init(gpu_id));
   scout_ctx* x = new scout_ctx();
   Ctx& c = x->c;
   c.dev = gpu_id;

@@ -270,6 +270,14 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, int32_t* P, int strategy, scou
 int compute_parents(Ctx& c, const int32_t* L, int32_t* P);
 int bfs_local(Ctx& c, Channel& ch, int strategy, scout_stats* st);
 int cc_full(Ctx& c, int64_t* scanned);
+// Upper bound of the chunk tasks of a full CC recompute of the current graph.
+int64_t cc_full_task_bound(const Ctx& c);
+// Launches the full CC recompute into `label` on the compute stream with caller-provided
+// chunk-task space and two counters ([0] tasks, [1] scanned entries); allocates nothing.
+int cc_full_into(Ctx& c, int32_t* label, int64_t* tasks, int64_t task_cap, int64_t* counters);
+// noalloc.hip: resolves the device address of the file-scope scratch array on device `dev`
+// (once per device; called by scout_ctx_create with `dev` current).
+int noalloc_init(int dev);
 int cc_local(Ctx& c, Channel& ch, scout_stats* st);
 void cc_release(Ctx& c);
 void mis_release(Ctx& c);
