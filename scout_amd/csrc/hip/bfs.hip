@@ -129,19 +129,152 @@ __device__ __forceinline__ bool relax_to(int32_t* level, int32_t* parent, int32_
 }
 
 // ---------------------------------------------------------------------------------------
+// Level buckets: the batch's deletion candidates and relax seeds grouped by level once, so
+// each round reads only its own level's items instead of filtering the whole list.
+// ---------------------------------------------------------------------------------------
+
+// Levels spanned by one bucketing (LDS counters per block); wider spans keep the unbucketed
+// per-round filter.
+constexpr int kMaxBuckets = 2048;
+// Items per thread of the bucketing kernels (a block handles kBlock * kBucketItems items).
+constexpr int kBucketItems = 8;
+
+// Level of item i: a vertex list keyed by its current level, or packed (level << 32 | vertex)
+// seeds keyed by their level.
+struct VertexKey {
+  const int32_t* items;
+  const int32_t* level;
+  __device__ int32_t key(int64_t i) const { return level[items[i]]; }
+  __device__ int64_t value(int64_t i) const { return items[i]; }
+};
+struct SeedKey {
+  const int64_t* items;
+  __device__ int32_t key(int64_t i) const { return (int32_t)(items[i] >> 32); }
+  __device__ int64_t value(int64_t i) const { return items[i]; }
+};
+
+// Zeroes start[0..nb] (the per-level counts land at start[b + 1]).
+__global__ void k_bucket_clear(int64_t* start, int nb) {
+  for (int i = (int)threadIdx.x; i <= nb; i += (int)blockDim.x) start[i] = 0;
+}
+
+// Per-level counts: LDS counters per block, one global atomic per (block, level).
+template <class K>
+__global__ void k_bucket_count(K src, int64_t count, int32_t lmin, int nb, int64_t* start) {
+  __shared__ int32_t h[kMaxBuckets];
+  for (int i = (int)threadIdx.x; i < nb; i += (int)blockDim.x) h[i] = 0;
+  __syncthreads();
+  int64_t base = (int64_t)blockIdx.x * blockDim.x * kBucketItems;
+#pragma unroll
+  for (int k = 0; k < kBucketItems; k++) {
+    int64_t i = base + (int64_t)k * blockDim.x + threadIdx.x;
+    if (i < count) {
+      int32_t b = src.key(i) - lmin;
+      if (b >= 0 && b < nb) atomicAdd(&h[b], 1);  // always in range: lmin..lmin+nb-1 span the keys
+    }
+  }
+  __syncthreads();
+  for (int i = (int)threadIdx.x; i < nb; i += (int)blockDim.x)
+    if (h[i]) atomicAdd((unsigned long long*)&start[i + 1], (unsigned long long)h[i]);
+}
+
+// Counts -> exclusive offsets start[0..nb], and cursor[b] = start[b] (one block).
+__global__ void k_bucket_scan(int64_t* start, int64_t* cursor, int nb) {
+  __shared__ int64_t part[kBlock];
+  // each thread sums a contiguous run of counts, the runs are scanned, then written back
+  int per = (nb + kBlock - 1) / kBlock;
+  int lo = (int)threadIdx.x * per, hi = min(lo + per, nb);
+  int64_t s = 0;
+  for (int b = lo; b < hi; b++) s += start[b + 1];
+  part[threadIdx.x] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int64_t t = 0;
+    for (int i = 0; i < kBlock; i++) {
+      int64_t v = part[i];
+      part[i] = t;
+      t += v;
+    }
+  }
+  __syncthreads();
+  // offsets into cursor first (the counts in start are still read), then copied over them
+  int64_t run = part[threadIdx.x];
+  for (int b = lo; b < hi; b++) {
+    int64_t cnt = start[b + 1];
+    cursor[b] = run;
+    run += cnt;
+  }
+  __syncthreads();
+  if (threadIdx.x == kBlock - 1) start[nb] = run;
+  for (int b = lo; b < hi; b++) start[b] = cursor[b];
+}
+
+// Scatter into level order: ranks inside the block from LDS counters, one global atomic per
+// (block, level) reserves the block's slots.
+template <class K, class T>
+__global__ void k_bucket_scatter(K src, int64_t count, int32_t lmin, int nb, int64_t* cursor, T* out) {
+  __shared__ int32_t h[kMaxBuckets];
+  __shared__ int64_t at[kMaxBuckets];
+  for (int i = (int)threadIdx.x; i < nb; i += (int)blockDim.x) h[i] = 0;
+  __syncthreads();
+  int64_t base = (int64_t)blockIdx.x * blockDim.x * kBucketItems;
+  int32_t bk[kBucketItems], rank[kBucketItems];
+#pragma unroll
+  for (int k = 0; k < kBucketItems; k++) {
+    int64_t i = base + (int64_t)k * blockDim.x + threadIdx.x;
+    bk[k] = -1;
+    rank[k] = 0;
+    if (i < count) {
+      int32_t b = src.key(i) - lmin;
+      if (b >= 0 && b < nb) {
+        bk[k] = b;
+        rank[k] = atomicAdd(&h[b], 1);
+      }
+    }
+  }
+  __syncthreads();
+  for (int i = (int)threadIdx.x; i < nb; i += (int)blockDim.x)
+    if (h[i]) at[i] = (int64_t)atomicAdd((unsigned long long*)&cursor[i], (unsigned long long)h[i]);
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < kBucketItems; k++) {
+    int64_t i = base + (int64_t)k * blockDim.x + threadIdx.x;
+    if (bk[k] >= 0) out[at[bk[k]] + rank[k]] = (T)src.value(i);
+  }
+}
+
+// Groups count items of src by level into out; start[0..nb] receives each level's range
+// (lmin + b -> [start[b], start[b + 1])), cursor is nb entries of scratch. Every key must lie
+// in [lmin, lmin + nb) and nb <= kMaxBuckets.
+template <class K, class T>
+void bucket_by_level(hipStream_t s, K src, int64_t count, int32_t lmin, int nb, int64_t* start, int64_t* cursor,
+                     T* out) {
+  unsigned grid = grid_for(count, kBlock * kBucketItems);
+  k_bucket_clear<<<1, kBlock, 0, s>>>(start, nb);
+  k_bucket_count<<<grid, kBlock, 0, s>>>(src, count, lmin, nb, start);
+  k_bucket_scan<<<1, kBlock, 0, s>>>(start, cursor, nb);
+  k_bucket_scatter<<<grid, kBlock, 0, s>>>(src, count, lmin, nb, cursor, out);
+}
+
+// ---------------------------------------------------------------------------------------
 // Relax rounds
 // ---------------------------------------------------------------------------------------
 
 // Thread per frontier vertex still at level lvl, and per seed (level << 32 | vertex) of this
-// level not queued before (seeds[0..ns); ns = 0 in rounds past the deepest seed). Short rows
-// are relaxed here; rows up to kTask go to the wave list, longer rows to chunk tasks.
+// level not queued before: seeds[0..ns) filtered by level, or, when the seeds are grouped by
+// level, seeds[sb[0]..sb[1]) (ns = 0 in rounds past the deepest seed). Short rows are relaxed
+// here; rows up to kTask go to the wave list, longer rows to chunk tasks.
 // *seeded += the seeds taken.
 __global__ void k_relax_short(GView g, int32_t* level, int32_t* parent, const int32_t* front, const int64_t* count,
-                              const int64_t* seeds, int64_t ns, int64_t* seeded, int32_t lvl, int32_t* next,
-                              int64_t* next_count, int64_t cap, int32_t* stamp, int32_t epoch, int32_t* medium,
-                              int64_t* medium_count, int64_t* tasks, int64_t* task_count, int64_t task_cap,
-                              int64_t* scanned) {
+                              const int64_t* seeds, int64_t ns, const int64_t* sb, int64_t* seeded, int32_t lvl,
+                              int32_t* next, int64_t* next_count, int64_t cap, int32_t* stamp, int32_t epoch,
+                              int32_t* medium, int64_t* medium_count, int64_t* tasks, int64_t* task_count,
+                              int64_t task_cap, int64_t* scanned) {
   int64_t nf = *count;
+  if (sb && ns > 0) {
+    seeds += sb[0];
+    ns = sb[1] - sb[0];
+  }
   int64_t total = nf + ns;
   int64_t work = 0, taken = 0;
   // the trip count is uniform across the block, so the block-wide appends see every thread
@@ -276,17 +409,21 @@ __global__ void k_del_candidates(const int32_t* uv, const uint8_t* ins, int64_t 
 }
 
 // Invalidation of the candidates at level lvl: the children queued by the previous round
-// (cur) and the batch's deletion candidates of this level (cand[0..ncand), filtered by level
-// and stamp; ncand = 0 past the deepest candidate; *listed += those taken). Thread per
-// candidate; rows longer than kShort go to the wave list (<= kChunk) or the long list, whose
-// chunk tasks are pushed here.
+// (cur) and the batch's deletion candidates of this level (cand[0..ncand), or cand[cb[0]..cb[1])
+// when the candidates are grouped by level; filtered by level and stamp; ncand = 0 past the
+// deepest candidate; *listed += those taken). Thread per candidate; rows longer than kShort go
+// to the wave list (<= kChunk) or the long list, whose chunk tasks are pushed here.
 __global__ void k_invalidate_short(GView g, int32_t* level, int32_t* parent, const int32_t* cur,
-                                   const int64_t* cur_count, const int32_t* cand, int64_t ncand, int64_t* listed,
-                                   int32_t lvl, int32_t* inval, int64_t* inval_count, int32_t* next,
+                                   const int64_t* cur_count, const int32_t* cand, int64_t ncand, const int64_t* cb,
+                                   int64_t* listed, int32_t lvl, int32_t* inval, int64_t* inval_count, int32_t* next,
                                    int64_t* next_count, int64_t cap, int32_t* stamp, int32_t epoch, int32_t* medium,
                                    int64_t* medium_count, int32_t* longs, int64_t* long_count, int64_t* tasks,
                                    int64_t* task_count, int64_t task_cap, int64_t* scanned, Attrib at) {
   int64_t nc = *cur_count;
+  if (cb && ncand > 0) {
+    cand += cb[0];
+    ncand = cb[1] - cb[0];
+  }
   int64_t total = nc + ncand;
   int64_t work = 0, taken = 0;
   // the trip count is uniform across the block, so the block-wide appends see every thread
@@ -817,6 +954,16 @@ int relax_levels(Ctx& c, int32_t* L, int32_t* P, int32_t lvl0, int32_t max_seed,
   int64_t* cnt[2] = {box + 8, box + 9};
   k_zero<<<1, 64, 0, c.s>>>(box + 10, 3);
   k_zero<<<1, 64, 0, c.s>>>(box + 15, 3);
+  // seeds grouped by level once (each round then reads only its own), when the levels they
+  // span fit the bucket counters and the list fits the scratch
+  const int64_t* sbuckets = nullptr;
+  int64_t span = (int64_t)max_seed - lvl0 + 1;
+  if (nseeds > 0 && span >= 1 && span <= kMaxBuckets && c.wc.cap >= nseeds) {
+    bucket_by_level(c.s, SeedKey{seeds}, nseeds, lvl0, (int)span, c.lvlb.p, c.lvlb.p + kMaxBuckets + 1,
+                    reinterpret_cast<int64_t*>(c.wc.p));
+    seeds = reinterpret_cast<const int64_t*>(c.wc.p);
+    sbuckets = c.lvlb.p;
+  }
   int32_t lvl = lvl0;
   int p = 0;
   int group = first_group;
@@ -827,10 +974,11 @@ int relax_levels(Ctx& c, int32_t* L, int32_t* P, int32_t lvl0, int32_t max_seed,
       int64_t* bnext = cnt[p ^ 1];
       // seeds of this level are taken by the round's first kernel
       int64_t ns = (nseeds > 0 && lvl <= max_seed) ? nseeds : 0;
+      const int64_t* sb = (sbuckets && ns > 0) ? sbuckets + (lvl - lvl0) : nullptr;
       k_round_begin<<<1, 1, 0, c.s>>>(bcur, bnext, box + 10, box + 10, 2, box + 15, box + 17, box + 16, ns > 0);
-      k_relax_short<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, P, cur, bcur, seeds, ns, box + 15, lvl, nxt, bnext,
-                                                    cap, c.stamp, epoch, c.le.p, box + 11, c.wb.p, box + 10, task_cap,
-                                                    box + 12);
+      k_relax_short<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, P, cur, bcur, seeds, ns, sb, box + 15, lvl, nxt,
+                                                    bnext, cap, c.stamp, epoch, c.le.p, box + 11, c.wb.p, box + 10,
+                                                    task_cap, box + 12);
       k_relax_rows<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, P, c.le.p, box + 11, c.wb.p, box + 10, task_cap, lvl,
                                                    nxt, bnext, cap, c.stamp, epoch, box + 12, c.d_paths);
       lvl++;
@@ -868,7 +1016,8 @@ int ensure_bfs(Ctx& c, int64_t extra) {
   SCOUT_CHECK(c.lf.reserve(n + 1));
   SCOUT_CHECK(c.lg.reserve(n + 2 * extra + 1));
   SCOUT_CHECK(c.wa.reserve(n + 2 * extra + 1));
-  SCOUT_CHECK(c.wc.reserve(n + 1));
+  SCOUT_CHECK(c.wc.reserve(n + 2 * extra + 1));
+  SCOUT_CHECK(c.lvlb.reserve(2 * kMaxBuckets + 2));
   // chunk tasks: at most 2 * entries / kTask (every split row has more than kTask entries)
   SCOUT_CHECK(c.wb.reserve(2 * (c.g.E + c.g.A) / kTask + 1024));
   return SCOUT_OK;
@@ -957,6 +1106,16 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, int32_t* P, int strategy, scou
     int32_t sepoch = next_epoch(c);
     int32_t* cur = c.lb.p;
     int32_t* nxt = c.lc.p;
+    // candidates grouped by level once (each round then reads only its own level's)
+    const int32_t* cand = c.la.p;
+    const int64_t* cbuckets = nullptr;
+    int64_t cspan = (int64_t)cmax - cmin + 1;
+    if (cspan >= 1 && cspan <= kMaxBuckets) {
+      bucket_by_level(c.s, VertexKey{c.la.p, L}, ncand, cmin, (int)cspan, c.lvlb.p, c.lvlb.p + kMaxBuckets + 1,
+                      c.lg.p);
+      cand = c.lg.p;
+      cbuckets = c.lvlb.p;
+    }
     int64_t* cnt[2] = {box + 8, box + 9};
     int64_t* binval = box + 10;
     int64_t* bscan = box + 11;
@@ -976,10 +1135,11 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, int32_t* P, int strategy, scou
         int64_t nlevel = lvl <= cmax ? ncand : 0;
         k_round_begin<<<1, 1, 0, c.s>>>(bcur, bnext, box + 14, box + 12, 3, box + 18, box + 20, box + 19,
                                         nlevel > 0);
-        k_invalidate_short<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, P, cur, bcur, c.la.p, nlevel, box + 18, lvl,
-                                                           c.ld.p, binval, nxt, bnext, cap, c.stamp, sepoch, c.le.p,
-                                                           box + 12, c.lf.p, box + 13, c.wb.p, box + 14, c.wb.cap,
-                                                           bscan, ab);
+        const int64_t* cb = (cbuckets && nlevel > 0) ? cbuckets + (lvl - cmin) : nullptr;
+        k_invalidate_short<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, P, cur, bcur, cand, nlevel, cb, box + 18,
+                                                           lvl, c.ld.p, binval, nxt, bnext, cap, c.stamp, sepoch,
+                                                           c.le.p, box + 12, c.lf.p, box + 13, c.wb.p, box + 14,
+                                                           c.wb.cap, bscan, ab);
         int32_t supepoch = next_epoch(c);
         SCOUT_CHECK(prof_begin(c, PROF_BFS_INVALIDATE_KERNEL));
         k_invalidate<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, P, c.le.p, box + 12, c.lf.p, c.wb.p, box + 14,

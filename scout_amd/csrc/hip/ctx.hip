@@ -287,6 +287,7 @@ int scout_ctx_destroy(scout_ctx* x) {
   c.wa.release();
   c.wb.release();
   c.wc.release();
+  c.lvlb.release();
   c.cc_bits.release();
   c.tmp.release();
   if (c.h_box) (void)hipHostFree(c.h_box);

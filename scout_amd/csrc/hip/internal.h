@@ -214,6 +214,7 @@ struct Ctx {
   // generic work lists
   DevBuf<int32_t> la, lb, lc, ld, le, lf, lg;
   DevBuf<int64_t> wa, wb, wc;
+  DevBuf<int64_t> lvlb;        // BFS: per-level bucket offsets and cursors of candidates / seeds
   DevBuf<uint8_t> tmp;         // library temp storage (scan/sort)
   std::vector<std::pair<hipEvent_t, hipEvent_t>> spans;  // timing
   MlpState* mlp = nullptr;
