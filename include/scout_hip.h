@@ -121,6 +121,13 @@ int scout_set_source(scout_ctx* ctx, int32_t source);
 int scout_stage_batch(scout_ctx* ctx, int channel, const int32_t* uv, int64_t m);
 /* Same, from a device pointer already in HBM (no PCIe transfer). */
 int scout_stage_batch_device(scout_ctx* ctx, int channel, const int32_t* d_uv, int64_t m);
+/* The channel's last upload: host time copying the lines into pinned staging (the reference's
+ * seed pack, src/mis/scout_mis.cu:614-637), device time of the host-to-device copy on the copy
+ * stream (HIP events; the reference's h2d time, core/reporting.h:46-51) and bytes sent. Waits
+ * for the copy. Any pointer may be NULL. scout_stage_batch on one channel may run on a second
+ * host thread while the driving thread works on the other channel (a serving loop stages the
+ * next step while the current one repairs); every other call on a context must not overlap. */
+int scout_stage_info(scout_ctx* ctx, int channel, double* pack_ms, double* h2d_ms, int64_t* bytes);
 
 /* Labels every staged line insertion (1) or deletion (0) against the rule's graph
  * (parse_batch_file, core/cli_setup.h:87-103). ins may be NULL. */
@@ -136,6 +143,11 @@ int scout_apply(scout_ctx* ctx, int channel, int conditional);
 /* BFS levels: replaces processBatch + gpu_relax_round + gpu_deletion_round
  * (src/bfs/scout_bfs.cu:177-272,325-515). */
 int scout_repair_bfs(scout_ctx* ctx, int channel, int strategy, scout_stats* st);
+/* The cascade bound b of the reference (SCOUT_BFS_BOUND, src/bfs/scout_bfs.cu:475-488): a
+ * batch whose invalidated vertices exceed b per deleted parent edge is finished by a full BFS
+ * (st->fallback = 1) instead of the local tentative levels and relax. The answer is exact either
+ * way, before the batch commits; 0 (the default) never falls back. */
+int scout_bfs_bound(scout_ctx* ctx, int64_t bound);
 /* CC min-id labels: replaces processBatch + bidirectional_reachability_check
  * (src/cc/scout_cc.cu:300-441). */
 int scout_repair_cc(scout_ctx* ctx, int channel, int strategy, scout_stats* st);
@@ -267,6 +279,14 @@ enum {
   SCOUT_PATH_COUNT = 6
 };
 int scout_path_counts(scout_ctx* ctx, int64_t* out, int count);
+
+/* Rows read by the repairs, for the page-bound evaluation of SCOUT_DUMP_TOUCHED
+ * (core/FlashGraph.h:318-329,376-385, where every vertex whose adjacency list is fetched is
+ * recorded). While enabled, every repair marks each vertex whose row it reads (enabling clears
+ * the set). scout_touched_download writes the marked ids in ascending order (at most cap) and
+ * their count. */
+int scout_touched_enable(scout_ctx* ctx, int on);
+int scout_touched_download(scout_ctx* ctx, int32_t* ids, int64_t cap, int64_t* count);
 
 /* Learned scheduler. No counterpart in the reference: it adds a small MLP (12 features, two
  * hidden layers of 64, 3 outputs; bf16 weights, fp32 accumulation) trained on the device on

@@ -92,6 +92,11 @@ _SIGNATURES = {
                         ctypes.POINTER(_I64)],
     "scout_batch_append": [_P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_I64)],
     "scout_path_counts": [_P, _P, ctypes.c_int],
+    "scout_stage_info": [_P, ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                         ctypes.POINTER(_I64)],
+    "scout_bfs_bound": [_P, _I64],
+    "scout_touched_enable": [_P, ctypes.c_int],
+    "scout_touched_download": [_P, _P, _I64, ctypes.POINTER(_I64)],
     "scout_apply_segments": [_P, ctypes.c_int, _I32, _P, ctypes.c_int, ctypes.c_int, _P],
     "scout_mlp_init": [_P, ctypes.c_uint64],
     "scout_mlp_set_params": [_P, _P],
@@ -400,6 +405,27 @@ class Engine:
         self._check(self.lib.scout_prof_read(self.h, kid, ctypes.byref(ms), ctypes.byref(cnt), ctypes.byref(by)),
                     "prof_read")
         return {"ms": ms.value, "launches": cnt.value, "bytes": by.value}
+
+    def stage_info(self, channel: int = 0) -> dict:
+        """The channel's last upload: host pack time, copy-stream H2D time (ms) and bytes."""
+        pack, h2d, by = ctypes.c_double(), ctypes.c_double(), ctypes.c_int64()
+        self._check(self.lib.scout_stage_info(self.h, channel, ctypes.byref(pack), ctypes.byref(h2d),
+                                              ctypes.byref(by)), "stage_info")
+        return {"pack_ms": pack.value, "h2d_ms": h2d.value, "bytes": by.value}
+
+    def bfs_bound(self, bound: int):
+        self._check(self.lib.scout_bfs_bound(self.h, bound), "bfs_bound")
+
+    def touched_enable(self, on: bool = True):
+        self._check(self.lib.scout_touched_enable(self.h, 1 if on else 0), "touched_enable")
+
+    def touched(self) -> np.ndarray:
+        cnt = ctypes.c_int64()
+        self._check(self.lib.scout_touched_download(self.h, None, 0, ctypes.byref(cnt)), "touched_download")
+        out = np.zeros(max(cnt.value, 1), dtype=np.int32)
+        self._check(self.lib.scout_touched_download(self.h, _ptr(out), cnt.value, ctypes.byref(cnt)),
+                    "touched_download")
+        return out[:cnt.value]
 
     def path_counts(self) -> dict:
         """Cumulative counts of the long-row code paths (chunk tasks per kernel family)."""

@@ -7,9 +7,20 @@
 // is on the device (SCOUT_NO_OVERLAP disables it), the role of the reference's 2-channel
 // overlap (src/mis/scout_mis.cu:853-912).
 //
-// Environment: SCOUT_NO_OVERLAP, SCOUT_NO_MADV_RANDOM, SCOUT_SKIP_COMPACTION=1 (mis) as in the
-// reference; SCOUT_DUMP_STATE=<dir> writes the solution after every batch as raw
-// little-endian arrays (<dir>/<problem>_<batch>.bin: uint8 membership / int32 otherwise);
+// Environment, as in the reference:
+//   SCOUT_NO_OVERLAP, SCOUT_NO_MADV_RANDOM, SCOUT_SKIP_COMPACTION=1 (mis);
+//   SCOUT_DUMP_TOUCHED=<file>: the ids (int32) of every vertex whose row a repair read, written
+//     at exit (core/FlashGraph.h:318-329);
+//   SCOUT_BFS_BOUND=<b> (bfs, and the levels cc keeps; default 200, src/bfs/scout_bfs.cu:476): a
+//     batch whose invalidated vertices exceed b per deleted parent edge is finished by a full
+//     BFS (fallback=1), still before the batch commits;
+//   SCOUT_DEFER_FALLBACK (core/local_repair.h:22-26): read and reported; every fallback runs
+//     once per batch before the batch commits, so "deferred" and "immediate" coincide here;
+//   SCOUT_NO_MEMO, SCOUT_FORCE_COPY: accepted with no effect (rows are resident in HBM, so there
+//     is no fetch memo, and every transfer to the discrete MI355X is an explicit copy); a note
+//     goes to stderr.
+// Added: SCOUT_DUMP_STATE=<dir> writes the solution after every batch as raw little-endian
+// arrays (<dir>/<problem>_<batch>.bin: uint8 membership / int32 otherwise);
 // SCOUT_STRATEGY=local|recompute|learned (bfs, cc).
 #pragma once
 
@@ -73,10 +84,37 @@ inline int run_driver(int argc, char** argv, Problem prob) {
     else if (v == "learned") strategy = SCOUT_STRATEGY_LEARNED;
   }
 
+  const char* touched_path = std::getenv("SCOUT_DUMP_TOUCHED");
+  long long bfs_bound = 200;
+  if (const char* b = std::getenv("SCOUT_BFS_BOUND")) bfs_bound = std::max(0LL, atoll(b));
+  bool defer = true;
+  if (const char* d = std::getenv("SCOUT_DEFER_FALLBACK")) defer = std::string(d) != "0";
+  for (const char* v : {"SCOUT_NO_MEMO", "SCOUT_FORCE_COPY"})
+    if (std::getenv(v))
+      std::cerr << v << ": no effect on this engine (rows are resident in HBM and every transfer is an explicit copy)"
+                << std::endl;
+
   scout_ctx* ctx = nullptr;
   DRV_CHECK(scout_ctx_create(args.gpu_id, nullptr, &ctx));
   char gpu_name[256] = {0};
   DRV_CHECK(scout_device_name(ctx, gpu_name, sizeof(gpu_name)));
+  // device memory in use before this program's own allocations (src/mis/scout_mis.cu:51-72,789)
+  int64_t mem_free0 = 0, mem_total = 0;
+  DRV_CHECK(scout_mem_info(ctx, &mem_free0, &mem_total));
+  double peak_delta_mb = 0.0;
+  auto sample_memory = [&] {
+    int64_t f = 0, t = 0;
+    if (scout_mem_info(ctx, &f, &t) != SCOUT_OK) return;
+    peak_delta_mb = std::max(peak_delta_mb, (double)(mem_free0 - f) / (1024.0 * 1024.0));
+  };
+  if (prob == Problem::BFS || prob == Problem::CC) {
+    DRV_CHECK(scout_bfs_bound(ctx, bfs_bound));
+    std::cout << (defer ? "[Deferred fallback: ON (default)] Batches whose cascades pass the safety bound are "
+                          "settled by one full recompute each, before the batch commits."
+                        : "[Deferred fallback: OFF (SCOUT_DEFER_FALLBACK=0)] Batches whose cascades pass the "
+                          "safety bound are settled by one full recompute each, before the batch commits.")
+              << std::endl;
+  }
 
   auto setup_t0 = std::chrono::steady_clock::now();
   EgrFile egr;
@@ -88,9 +126,8 @@ inline int run_driver(int argc, char** argv, Problem prob) {
     std::cout << "======================================= SCOUT MIS =========================================" << std::endl;
   }
   print_run_header(algo, args.egr_path, n, (long long)egr.edges, avg_degree, gpu_name);
-  if (prob == Problem::BFS || prob == Problem::CC)
-    std::cout << "[Exact per-batch repair] every batch commits the exact solution; no deferred fallback." << std::endl;
   DRV_CHECK(scout_graph_load(ctx, n, (int64_t)egr.edges, egr.nindex_raw(), egr.wide ? 1 : 0, egr.nlist()));
+  if (touched_path) DRV_CHECK(scout_touched_enable(ctx, 1));
 
   // Initial state.
   std::vector<long long> vals = read_ints(args.context_path);
@@ -163,6 +200,8 @@ inline int run_driver(int argc, char** argv, Problem prob) {
     auto t0 = std::chrono::steady_clock::now();
     scout_stats st{};
     DRV_CHECK(scout_stage_batch(ctx, ch, uv.data(), m));
+    double pack_ms = 0.0, h2d_ms = 0.0;
+    DRV_CHECK(scout_stage_info(ctx, ch, &pack_ms, &h2d_ms, nullptr));
     auto tc = std::chrono::steady_clock::now();
     DRV_CHECK(scout_classify(ctx, ch, prob == Problem::MIS ? SCOUT_CLASSIFY_BASE : SCOUT_CLASSIFY_CURRENT, nullptr));
     DRV_CHECK(scout_apply(ctx, ch, prob == Problem::MIS ? 1 : 0));
@@ -181,7 +220,8 @@ inline int run_driver(int argc, char** argv, Problem prob) {
     lines_total += m;
     std::vector<KV> kv;
     if (prob == Problem::MIS) {
-      kv = {{"clusters", std::to_string(st.rounds)}};
+      // repair regions: the lines whose rule wrote the membership
+      kv = {{"clusters", std::to_string(st.conflicts)}};
     } else if (prob == Problem::COLORING) {
       kv = {{"insertions", std::to_string(st.insertions)}, {"conflicts", std::to_string(st.conflicts)},
             {"rounds", std::to_string(st.rounds)}};
@@ -190,8 +230,12 @@ inline int run_driver(int argc, char** argv, Problem prob) {
             {"fallback", std::to_string(st.fallback)}};
     }
     print_batch_result(k + 1, total, m, this_ms, kv);
-    double h2d_ms = 0.0;  // the staged copy is folded into apply_ms (copy stream, waited on by classify)
-    print_stage_times(k + 1, fetch_ms, 0.0, h2d_ms, st.repair_ms + apply_ms, std::max(0.0, this_ms - st.repair_ms - apply_ms));
+    // fetch: waiting for the batch's lines; seed pack: copying them into pinned staging; h2d:
+    // the copy-stream transfer; repair: classify + apply + repair; commit: the rest (read-backs
+    // and the final wait)
+    double repair_ms = st.repair_ms + apply_ms;
+    print_stage_times(k + 1, fetch_ms, pack_ms, h2d_ms, repair_ms, std::max(0.0, this_ms - pack_ms - h2d_ms - repair_ms));
+    sample_memory();
     sum.insertions += st.insertions;
     sum.deletions += st.deletions;
     sum.rounds += st.rounds;
@@ -249,6 +293,11 @@ inline int run_driver(int argc, char** argv, Problem prob) {
     std::cout << "  Number of batches:       " << total << std::endl;
     std::cout << "\nEnd-of-cycle compaction time (NOT counted in per-batch numbers above): " << std::fixed
               << std::setprecision(3) << compact_ms << " ms" << std::endl;
+    sample_memory();
+    double total_mb = (double)mem_total / (1024.0 * 1024.0);
+    double peak_pct = total_mb > 0.0 ? peak_delta_mb / total_mb * 100.0 : 0.0;
+    std::cout << "Peak program GPU memory: " << std::setprecision(2) << peak_delta_mb << " MB (" << std::setprecision(1)
+              << peak_pct << "% of " << std::setprecision(2) << total_mb << " MB total)" << std::endl;
   } else if (prob == Problem::COLORING) {
     DRV_CHECK(scout_state_download(ctx, SCOUT_STATE_COLOR, st32.data(), n));
     int32_t mx = -1;
@@ -274,6 +323,15 @@ inline int run_driver(int argc, char** argv, Problem prob) {
   print_final_summary(setup_ms, processing_ms, setup_ms + processing_ms + compact_ms, label, initial_value, label,
                       final_value);
   print_throughput(lines_total, processing_ms, batch_ms);
+  if (touched_path) {
+    int64_t cnt = 0;
+    DRV_CHECK(scout_touched_download(ctx, nullptr, 0, &cnt));
+    std::vector<int32_t> ids((size_t)cnt);
+    DRV_CHECK(scout_touched_download(ctx, ids.data(), cnt, &cnt));
+    FILE* f = fopen(touched_path, "wb");
+    if (!f || fwrite(ids.data(), sizeof(int32_t), ids.size(), f) != ids.size()) die(std::string("cannot write ") + touched_path);
+    fclose(f);
+  }
   std::cout.flush();
   fflush(stdout);
   std::_Exit(0);

@@ -5,6 +5,7 @@
 //   scout_gen ba    <n> <m> <seed> <out.egr>
 //   scout_gen init  <mis|coloring|bfs|cc> <graph.egr> <out.txt>
 //   scout_gen stream <mixed|tree|degree> <graph.egr> <out_dir> <batches> <size> <seed> [delete_frac] [tree_frac]
+//   scout_gen convert <in.mtx> <out.egr>     (the reference's .mtx -> wide .egr converter)
 #include <cstdio>
 #include <cstdlib>
 #include <filesystem>
@@ -12,6 +13,7 @@
 
 #include "../host/cli.h"
 #include "../host/gen.h"
+#include "../host/mtx.h"
 
 using namespace scout;
 
@@ -19,7 +21,8 @@ static int usage() {
   fprintf(stderr,
           "usage:\n  scout_gen er <n> <m> <seed> <out.egr>\n  scout_gen rmat <scale> <edge_factor> <seed> <out.egr> [a b c]\n"
           "  scout_gen ba <n> <m> <seed> <out.egr>\n  scout_gen init <mis|coloring|bfs|cc> <graph.egr> <out.txt>\n"
-          "  scout_gen stream <mixed|tree|degree> <graph.egr> <out_dir> <batches> <size> <seed> [delete_frac] [tree_frac]\n");
+          "  scout_gen stream <mixed|tree|degree> <graph.egr> <out_dir> <batches> <size> <seed> [delete_frac] [tree_frac]\n"
+          "  scout_gen convert <in.mtx> <out.egr>\n");
   return 1;
 }
 
@@ -74,6 +77,8 @@ int main(int argc, char** argv) {
     std::error_code ec;
     std::filesystem::create_directories(argv[4], ec);
     err = gen_stream(g, argv[4], s);
+  } else if (cmd == "convert" && argc >= 4) {
+    err = mtx_to_egr(argv[2], argv[3]);
   } else if (cmd == "parse" && argc >= 4) {
     // Prints the 0-indexed pairs the binaries read from a batch file (cli.h rules).
     std::vector<int32_t> uv;

@@ -1180,6 +1180,17 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, int32_t* P, int strategy, scou
     prof_bytes(c, PROF_BFS_INVALIDATE_KERNEL, 36 * c.h_box[22] + 8 * c.h_box[21]);
     nx = c.h_box[10];
     scanned += c.h_box[11];
+    if (c.bfs_bound > 0 && nx > c.bfs_bound * ncand) {
+      // the cascades passed the bound (src/bfs/scout_bfs.cu:475-488 per deletion, here over the
+      // batch's candidates): the batch is settled by a full BFS before it commits
+      st->fallback = 1;
+      st->touched += nx;
+      st->rounds += rounds;
+      st->edges_scanned += scanned;
+      SCOUT_CHECK(bfs_full_on(c, L, P, c.source, &st->rounds, &st->edges_scanned));
+      if (logging) SCOUT_CHECK(mlp_end_batch(c, c.la.p, nsample, L));
+      return SCOUT_OK;
+    }
     if (nx > 0) {
       // [11] scanned, [12] medium rows, [13] chunk tasks
       k_zero<<<1, 64, 0, c.s>>>(box + 11, 3);
@@ -1265,6 +1276,15 @@ int scout_bfs_full(scout_ctx* x, int32_t source) {
   return SCOUT_OK;
 }
 
+int scout_bfs_bound(scout_ctx* x, int64_t bound) {
+  if (!x || bound < 0) {
+    set_error("bfs_bound: bound must be >= 0");
+    return SCOUT_ERR_ARG;
+  }
+  x->c.bfs_bound = bound;
+  return SCOUT_OK;
+}
+
 int scout_repair_bfs(scout_ctx* x, int channel, int strategy, scout_stats* st) {
   if (!x || !st) return SCOUT_ERR_ARG;
   Ctx& c = x->c;
@@ -1275,6 +1295,7 @@ int scout_repair_bfs(scout_ctx* x, int channel, int strategy, scout_stats* st) {
     set_error("repair_bfs: graph and levels must be loaded first");
     return SCOUT_ERR_STATE;
   }
+  TouchScope touch(c);
   Span span;
   SCOUT_CHECK(span.begin(c.s));
   st->lines += ch->m;
@@ -1292,7 +1313,7 @@ int scout_repair_bfs(scout_ctx* x, int channel, int strategy, scout_stats* st) {
   SCOUT_CHECK(span.end(c.s));
   st->repair_ms += span.ms();
   st->readbacks += c.readbacks - rb0;
-  return SCOUT_OK;
+  return channel_used(c.s, *ch);
 }
 
 }  // extern "C"

@@ -337,6 +337,7 @@ int cc_local(Ctx& c, Channel& ch, scout_stats* st) {
     st->touched += bst.touched;
     st->insertions += bst.insertions;
     st->deletions += bst.deletions;
+    st->fallback += bst.fallback;
     L = c.cc_level;
   }
   SCOUT_CHECK(ensure_scratch(c, &c.cc_owner, -1));
@@ -413,6 +414,7 @@ int scout_repair_cc(scout_ctx* x, int channel, int strategy, scout_stats* st) {
     set_error("repair_cc: graph and labels must be loaded first");
     return SCOUT_ERR_STATE;
   }
+  TouchScope touch(c);
   Span span;
   SCOUT_CHECK(span.begin(c.s));
   st->lines += ch->m;
@@ -428,7 +430,7 @@ int scout_repair_cc(scout_ctx* x, int channel, int strategy, scout_stats* st) {
   SCOUT_CHECK(span.end(c.s));
   st->repair_ms += span.ms();
   st->readbacks += c.readbacks - rb0;
-  return SCOUT_OK;
+  return channel_used(c.s, *ch);
 }
 
 }  // extern "C"

@@ -317,7 +317,7 @@ void launch_group(Ctx& c, ColorState& s, const RoundBufs& b) {
 std::vector<const void*> graph_key(Ctx& c, ColorState& s, const RoundBufs& b) {
   GView g = c.g.view();
   return {b.la, b.lb, b.prop, (const void*)(intptr_t)b.cap, c.color, c.col_marker, c.d_box, s.wave_list.p,
-          s.block_list.p, g.boff, g.bcol, g.aoff, g.alen, g.acol, g.bsamp, (const void*)(intptr_t)g.n,
+          s.block_list.p, g.boff, g.bcol, g.aoff, g.alen, g.acol, g.bsamp, g.touched, (const void*)(intptr_t)g.n,
           (const void*)(intptr_t)s.thread_max, (const void*)(intptr_t)s.block_min, (const void*)(intptr_t)s.grid,
           (const void*)(intptr_t)s.group_rounds};
 }
@@ -423,6 +423,7 @@ extern "C" int scout_repair_color(scout_ctx* x, int channel, scout_stats* st) {
     set_error("repair_color: graph and colours must be loaded first");
     return SCOUT_ERR_STATE;
   }
+  TouchScope touch(c);
   Span span;
   SCOUT_CHECK(span.begin(c.s));
   st->lines += ch->m;
@@ -431,7 +432,7 @@ extern "C" int scout_repair_color(scout_ctx* x, int channel, scout_stats* st) {
   SCOUT_CHECK(span.end(c.s));
   st->repair_ms += span.ms();
   st->readbacks += c.readbacks - rb0;
-  return SCOUT_OK;
+  return channel_used(c.s, *ch);
 }
 
 extern "C" int scout_color_config(scout_ctx* x, int thread_max, int block_min, int grid, int use_graph) {

@@ -23,6 +23,10 @@ struct Row {
 };
 
 __device__ __forceinline__ Row row_of(const GView& g, int32_t v) {
+  if (g.touched) {
+    uint32_t bit = 1u << (v & 31);
+    if (!(g.touched[v >> 5] & bit)) atomicOr(&g.touched[v >> 5], bit);
+  }
   int64_t a0 = g.aoff[v];
   return Row{g.boff[v], g.boff[v + 1], a0, a0 + g.alen[v]};
 }

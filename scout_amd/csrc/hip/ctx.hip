@@ -1,5 +1,6 @@
 // ctx.hip: context lifetime, staging channels, solution state and the graph entry points of
 // the C ABI (include/scout_hip.h).
+#include <chrono>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -251,7 +252,10 @@ int scout_ctx_create(int gpu_id, const scout_budget* budget, scout_ctx** out) {
   }
   c.d_paths = c.d_box + kBoxSlots;
   for (int k = 0; k < 2; k++) {
-    if (hipEventCreateWithFlags(&c.ch[k].staged, hipEventDisableTiming) != hipSuccess) {
+    Channel& ch = c.ch[k];
+    if (hipEventCreateWithFlags(&ch.staged, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ch.consumed, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreate(&ch.h2d0) != hipSuccess || hipEventCreate(&ch.h2d1) != hipSuccess) {
       set_error("event creation failed");
       delete x;
       return SCOUT_ERR_HIP;
@@ -280,7 +284,8 @@ int scout_ctx_destroy(scout_ctx* x) {
     c.ch[k].d_uv.release();
     c.ch[k].d_ins.release();
     c.ch[k].d_done.release();
-    if (c.ch[k].staged) (void)hipEventDestroy(c.ch[k].staged);
+    for (hipEvent_t ev : {c.ch[k].staged, c.ch[k].consumed, c.ch[k].h2d0, c.ch[k].h2d1})
+      if (ev) (void)hipEventDestroy(ev);
   }
   DevBuf<int32_t>* ls[] = {&c.la, &c.lb, &c.lc, &c.ld, &c.le, &c.lf, &c.lg};
   for (auto* l : ls) l->release();
@@ -469,12 +474,32 @@ int scout_stage_batch(scout_ctx* x, int channel, const int32_t* uv, int64_t m) {
   SCOUT_CHECK(ch->d_uv.reserve(2 * m + 2));
   SCOUT_CHECK(ch->d_ins.reserve(m + 1));
   SCOUT_CHECK(ch->d_done.reserve(m + 1));
-  if (m > 0) {
-    memcpy(ch->h_uv.p, uv, (size_t)(2 * m) * sizeof(int32_t));
+  auto t0 = std::chrono::steady_clock::now();
+  if (m > 0) memcpy(ch->h_uv.p, uv, (size_t)(2 * m) * sizeof(int32_t));
+  ch->pack_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  // the compute stream may still read this channel's previous lines
+  SCOUT_TRY(hipStreamWaitEvent(c.cs, ch->consumed, 0));
+  SCOUT_TRY(hipEventRecord(ch->h2d0, c.cs));
+  if (m > 0)
     SCOUT_TRY(hipMemcpyAsync(ch->d_uv.p, ch->h_uv.p, (size_t)(2 * m) * sizeof(int32_t), hipMemcpyHostToDevice, c.cs));
-  }
+  SCOUT_TRY(hipEventRecord(ch->h2d1, c.cs));
   SCOUT_TRY(hipEventRecord(ch->staged, c.cs));
+  ch->h2d_bytes = 8 * m;
   channel_staged(*ch, m);
+  return SCOUT_OK;
+}
+
+int scout_stage_info(scout_ctx* x, int channel, double* pack_ms, double* h2d_ms, int64_t* bytes) {
+  SCOUT_CHECK(check_ctx(x));
+  Ctx& c = x->c;
+  Channel* ch = channel_of(c, channel);
+  if (!ch) return SCOUT_ERR_ARG;
+  SCOUT_TRY(hipEventSynchronize(ch->h2d1));
+  float t = 0.f;
+  SCOUT_TRY(hipEventElapsedTime(&t, ch->h2d0, ch->h2d1));
+  if (pack_ms) *pack_ms = ch->pack_ms;
+  if (h2d_ms) *h2d_ms = (double)t;
+  if (bytes) *bytes = ch->h2d_bytes;
   return SCOUT_OK;
 }
 
@@ -487,9 +512,14 @@ int scout_stage_batch_device(scout_ctx* x, int channel, const int32_t* d_uv, int
   SCOUT_CHECK(ch->d_uv.reserve(2 * m + 2));
   SCOUT_CHECK(ch->d_ins.reserve(m + 1));
   SCOUT_CHECK(ch->d_done.reserve(m + 1));
+  SCOUT_TRY(hipStreamWaitEvent(c.cs, ch->consumed, 0));
+  SCOUT_TRY(hipEventRecord(ch->h2d0, c.cs));
   if (m > 0)
     SCOUT_TRY(hipMemcpyAsync(ch->d_uv.p, d_uv, (size_t)(2 * m) * sizeof(int32_t), hipMemcpyDeviceToDevice, c.cs));
+  SCOUT_TRY(hipEventRecord(ch->h2d1, c.cs));
   SCOUT_TRY(hipEventRecord(ch->staged, c.cs));
+  ch->pack_ms = 0.0;
+  ch->h2d_bytes = 0;
   channel_staged(*ch, m);
   return SCOUT_OK;
 }
@@ -505,6 +535,7 @@ int scout_classify(scout_ctx* x, int channel, int rule, uint8_t* ins) {
   }
   SCOUT_TRY(hipStreamWaitEvent(c.s, ch->staged, 0));
   SCOUT_CHECK(graph_classify(c, *ch, rule));
+  SCOUT_CHECK(channel_used(c.s, *ch));
   if (ins && ch->m > 0) {
     SCOUT_TRY(hipMemcpyAsync(ins, ch->d_ins.p, (size_t)ch->m, hipMemcpyDeviceToHost, c.s));
     SCOUT_TRY(hipStreamSynchronize(c.s));
@@ -518,7 +549,8 @@ int scout_apply(scout_ctx* x, int channel, int conditional) {
   Channel* ch = channel_of(c, channel);
   if (!ch) return SCOUT_ERR_ARG;
   SCOUT_TRY(hipStreamWaitEvent(c.s, ch->staged, 0));
-  return graph_apply(c, *ch, conditional);
+  SCOUT_CHECK(graph_apply(c, *ch, conditional));
+  return channel_used(c.s, *ch);
 }
 
 int scout_apply_segments(scout_ctx* x, int channel, int32_t nseg, const int64_t* seg_off, int rule,
@@ -546,6 +578,7 @@ int scout_apply_segments(scout_ctx* x, int channel, int32_t nseg, const int64_t*
     SCOUT_CHECK(graph_classify_lines(c, *ch, seg_off[k], seg_off[k + 1], rule));
     SCOUT_CHECK(graph_apply_lines(c, *ch, seg_off[k], seg_off[k + 1], conditional));
   }
+  SCOUT_CHECK(channel_used(c.s, *ch));
   if (ins && ch->m > 0) {
     SCOUT_TRY(hipMemcpyAsync(ins, ch->d_ins.p, (size_t)ch->m, hipMemcpyDeviceToHost, c.s));
     SCOUT_TRY(hipStreamSynchronize(c.s));
@@ -585,6 +618,43 @@ int scout_prof_read(scout_ctx* x, int id, double* total_ms, int64_t* launches, i
   if (total_ms) *total_ms = ms;
   if (launches) *launches = cnt;
   if (bytes) *bytes = by;
+  return SCOUT_OK;
+}
+
+int scout_touched_enable(scout_ctx* x, int on) {
+  SCOUT_CHECK(check_graph(x));
+  Ctx& c = x->c;
+  if (on) {
+    int64_t words = ((int64_t)c.g.n + 31) / 32;
+    SCOUT_CHECK(c.touched_bits.reserve(words));
+    SCOUT_TRY(hipMemsetAsync(c.touched_bits.p, 0, (size_t)words * sizeof(uint32_t), c.s));
+    SCOUT_TRY(hipStreamSynchronize(c.s));
+  }
+  c.touched_on = on != 0;
+  return SCOUT_OK;
+}
+
+int scout_touched_download(scout_ctx* x, int32_t* ids, int64_t cap, int64_t* count) {
+  SCOUT_CHECK(check_graph(x));
+  Ctx& c = x->c;
+  if (!count || (cap > 0 && !ids)) {
+    set_error("touched_download: null output");
+    return SCOUT_ERR_ARG;
+  }
+  *count = 0;
+  if (!c.touched_bits.p) return SCOUT_OK;
+  int64_t words = ((int64_t)c.g.n + 31) / 32;
+  std::vector<uint32_t> bits((size_t)words);
+  SCOUT_TRY(hipStreamSynchronize(c.s));
+  SCOUT_TRY(hipMemcpy(bits.data(), c.touched_bits.p, (size_t)words * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  int64_t k = 0;
+  for (int64_t w = 0; w < words; w++)
+    for (uint32_t b = bits[(size_t)w]; b; b &= b - 1) {
+      int64_t v = 32 * w + __builtin_ctz(b);
+      if (k < cap) ids[k] = (int32_t)v;
+      k++;
+    }
+  *count = k;
   return SCOUT_OK;
 }
 

@@ -107,6 +107,7 @@ struct GView {
   const int32_t* acol;
   int32_t n;
   const int32_t* bsamp;  // decoded bcol[kSampleStride * k], or nullptr (rows searched directly)
+  uint32_t* touched;     // bitmap of vertices whose rows a repair read, or nullptr (not tracked)
 };
 
 // Every kSampleStride-th base entry is sampled (decoded): a search in a long sorted row first
@@ -138,8 +139,11 @@ struct DevGraph {
   // base entry samples and the base generation they were taken from
   DevBuf<int32_t> bsamp;
   int64_t bsamp_gen = -1;
+  // set only while a repair runs with row tracking on (scout_touched_enable)
+  uint32_t* touched = nullptr;
   GView view() const {
-    return GView{boff, bcol, aoff.p, alen.p, acol.p, n, (sorted && bsamp_gen == base_gen) ? bsamp.p : nullptr};
+    return GView{boff, bcol, aoff.p, alen.p, acol.p, n, (sorted && bsamp_gen == base_gen) ? bsamp.p : nullptr,
+                 touched};
   }
 };
 
@@ -158,7 +162,19 @@ struct Channel {
   bool pairs_distinct = false;
   int64_t m = 0;
   hipEvent_t staged = nullptr;
+  // recorded on the compute stream after the last work that reads this channel's lines: the
+  // next upload into the channel waits for it (write after read)
+  hipEvent_t consumed = nullptr;
+  // the last upload: host copy into pinned staging (ms) and its copy-stream span (events)
+  hipEvent_t h2d0 = nullptr, h2d1 = nullptr;
+  double pack_ms = 0.0;
+  int64_t h2d_bytes = 0;
 };
+
+// Marks the channel's lines as read by the work queued on the compute stream so far.
+inline int channel_used(hipStream_t s, Channel& ch) {
+  return hipEventRecord(ch.consumed, s) == hipSuccess ? SCOUT_OK : SCOUT_ERR_HIP;
+}
 
 // Sets the staged line count; base ranges found by an earlier classify no longer apply.
 inline void channel_staged(Channel& ch, int64_t m) {
@@ -200,6 +216,8 @@ struct Ctx {
   int32_t* cc_level = nullptr;   // BFS levels from vertex 0 kept for CC when BFS is not maintained
   int32_t* cc_parent = nullptr;  // parents for cc_level
   int64_t apply_seq = 0;         // batches applied to the graph
+  int64_t bfs_bound = 0;         // scout_bfs_bound: invalidated vertices per deletion candidate past which a
+                                 // batch's BFS repair is finished by a full recompute (0 = never)
   int64_t level_seq = -1;        // apply_seq at which `level` was last made exact
   int64_t cc_level_seq = -1;     // same for cc_level
   int64_t cc_components = -1;    // component count after the last CC repair
@@ -215,6 +233,9 @@ struct Ctx {
   DevBuf<int32_t> la, lb, lc, ld, le, lf, lg;
   DevBuf<int64_t> wa, wb, wc;
   DevBuf<int64_t> lvlb;        // BFS: per-level bucket offsets and cursors of candidates / seeds
+  // rows read by repairs (SCOUT_DUMP_TOUCHED): one bit per vertex while tracking is on
+  DevBuf<uint32_t> touched_bits;
+  bool touched_on = false;
   DevBuf<uint8_t> tmp;         // library temp storage (scan/sort)
   std::vector<std::pair<hipEvent_t, hipEvent_t>> spans;  // timing
   MlpState* mlp = nullptr;
@@ -328,6 +349,14 @@ int ensure_state(Ctx& c, int kind);
 int ensure_scratch(Ctx& c, int32_t** p, int32_t fill);
 Channel* channel_of(Ctx& c, int ch);
 int ensure_list(DevBuf<int32_t>& l, int64_t count);
+
+// Row tracking for the duration of one repair entry point: the graph view carries the touched
+// bitmap while tracking is on.
+struct TouchScope {
+  DevGraph& g;
+  explicit TouchScope(Ctx& c) : g(c.g) { g.touched = c.touched_on ? c.touched_bits.p : nullptr; }
+  ~TouchScope() { g.touched = nullptr; }
+};
 
 // Device time between two events recorded around a repair.
 struct Span {

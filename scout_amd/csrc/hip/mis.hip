@@ -1114,7 +1114,8 @@ int mm_graph(Ctx& c, MisState& ms, SegView sv) {
                                   (const void*)(intptr_t)c.g.sorted, ms.h2_line.p, ms.h2_seg.p, ms.h2_flag.p,
                                   ms.h2_best.p, ms.h2_off.p, ms.h2_task.p, ms.h2_y.p, ms.h2_rsb.p, ms.h2_keyA.p,
                                   (const void*)(intptr_t)ms.h2_rcap,
-                                  (const void*)(intptr_t)ms.h2_tcap, g.boff, g.bcol, g.aoff, g.alen, g.acol, g.bsamp, (const void*)(intptr_t)g.n,
+                                  (const void*)(intptr_t)ms.h2_tcap, g.boff, g.bcol, g.aoff, g.alen, g.acol, g.bsamp, g.touched,
+                                  (const void*)(intptr_t)g.n,
                                   (const void*)(intptr_t)sv.stride, (const void*)(intptr_t)ms.grid,
                                   (const void*)(intptr_t)ms.group_rounds};
   if (ms.exec && key == ms.key) return SCOUT_OK;
@@ -1311,6 +1312,7 @@ int mis_entry(scout_ctx* x, int channel, int32_t ninst, const int64_t* line_off,
       set_error(std::string(name) + ": line offsets must not decrease");
       return SCOUT_ERR_ARG;
     }
+  TouchScope touch(c);
   Span span;
   SCOUT_CHECK(span.begin(c.s));
   st->lines += ch->m;
@@ -1319,7 +1321,7 @@ int mis_entry(scout_ctx* x, int channel, int32_t ninst, const int64_t* line_off,
   SCOUT_CHECK(span.end(c.s));
   st->repair_ms += span.ms();
   st->readbacks += c.readbacks - rb0;
-  return SCOUT_OK;
+  return channel_used(c.s, *ch);
 }
 
 }  // namespace

@@ -599,7 +599,8 @@ int scout_gen_batch(scout_ctx* x, int channel, int64_t size, double delete_frac,
     set_error("gen_batch: no graph loaded");
     return SCOUT_ERR_STATE;
   }
-  return gen_batch(c, *ch, size, delete_frac, tree_frac, seed, expected_total, out_m);
+  SCOUT_CHECK(gen_batch(c, *ch, size, delete_frac, tree_frac, seed, expected_total, out_m));
+  return channel_used(c.s, *ch);
 }
 
 int scout_gen_mode(scout_ctx* x, int mode) {
@@ -631,7 +632,8 @@ int scout_batch_store(scout_ctx* x, int channel, int slot) {
   SCOUT_TRY(hipSetDevice(x->c.dev));
   Channel* ch = channel_of(x->c, channel);
   if (!ch) return SCOUT_ERR_ARG;
-  return batch_store(x->c, *ch, slot);
+  SCOUT_CHECK(batch_store(x->c, *ch, slot));
+  return channel_used(x->c.s, *ch);
 }
 
 int scout_batch_load(scout_ctx* x, int channel, int slot, int64_t* m) {

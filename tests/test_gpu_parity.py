@@ -394,3 +394,80 @@ def test_mis_er_100k_full_size_exact_every_batch(tmp_path):
         say(f"[mis] batch {k}: {int(got.sum())} members, {og.mis_maximality_violations(got)} maximality "
             f"violations, {st.rounds} rounds, {st.repair_ms:.2f} ms")
     eng.close()
+
+
+def run_binary(prog, egr, init, bdir, nb, cwd, **env):
+    e = dict(os.environ, **{k: str(v) for k, v in env.items()})
+    r = subprocess.run([os.path.join(sa.BIN_DIR, f"scout_{prog}"), str(egr), init, bdir, str(nb), "0"],
+                       capture_output=True, text=True, env=e, cwd=cwd, timeout=120)
+    assert r.returncode == 0, r.stderr
+    return r.stdout
+
+
+def test_binaries_read_the_narrow_egr(tmp_path, data):
+    """scout_bfs and scout_cc on the sample written in the narrow layout dump the same states
+    as on the wide file."""
+    egr, bdir, nb = data["sample"]
+    say("[binary] narrow .egr")
+    g = sa.Graph.read_egr(egr)
+    narrow = tmp_path / "narrow.egr"
+    with open(narrow, "wb") as f:
+        f.write(np.array([g.n, g.off[-1]], np.int32).tobytes())
+        f.write(g.off.astype(np.int32).tobytes())
+        f.write(g.col.astype(np.int32).tobytes())
+    sample = os.path.dirname(egr)
+    for prog, init in (("bfs", "sample_bfs.txt"), ("cc", "sample_components.txt")):
+        for name, path in (("wide", egr), ("narrow", narrow)):
+            run_binary(prog, path, os.path.join(sample, init), bdir, nb, tmp_path, SCOUT_DUMP_STATE=tmp_path / name)
+        for k in range(1, nb + 1):
+            a = (tmp_path / "wide" / f"{prog}_{k}.bin").read_bytes()
+            assert a == (tmp_path / "narrow" / f"{prog}_{k}.bin").read_bytes(), f"{prog} batch {k}"
+
+
+def test_mis_binary_compacts_the_egr_in_place(tmp_path, data):
+    """Without SCOUT_SKIP_COMPACTION, scout_mis rewrites its .egr at the end of the run
+    (src/mis/scout_mis.cu:949-965, core/flash_compactor.h:23-119): the file must equal the
+    oracle's compaction of the logical graph after the batches, applied with MIS's rules
+    (classification against the initial graph, conditional apply)."""
+    egr, bdir, nb = data["sample"]
+    say("[binary] scout_mis compaction")
+    local = tmp_path / "sample.egr"
+    local.write_bytes(open(egr, "rb").read())
+    out = run_binary("mis", local, os.path.join(os.path.dirname(egr), "sample_mis.txt"), bdir, nb, tmp_path)
+    assert "End-of-cycle compaction time" in out and "Peak program GPU memory:" in out
+    og, base = OracleGraph(egr), OracleGraph(egr)
+    for k, uv in batches(bdir, nb, og.n):
+        og.apply(uv, base.classify(uv), conditional=True)
+    want = tmp_path / "oracle.egr"
+    from oracle_lib import lib
+    assert lib().orc_compact_to(og.h, str(want).encode()) == 0
+    assert local.read_bytes() == want.read_bytes()
+    # the run wrote its final MIS where the reference does
+    assert (tmp_path / "datasets" / "Final_MISs" / "sample_flash_final_mis.txt").exists()
+
+
+def test_binary_switches_and_stage_lines(tmp_path, data):
+    """SCOUT_BFS_BOUND is a scheduling switch: with b = 0 no batch falls back, with b = 1 the
+    batches whose cascades pass one vertex per deleted parent edge are settled by a full BFS
+    (fallback=1); the dumped levels are identical either way. [STAGE] lines carry the measured
+    pack and copy times, and SCOUT_DUMP_TOUCHED writes the ascending ids of the rows read."""
+    egr, bdir, nb = data["heavy"]
+    init = str(tmp_path / "bfs_init.txt")
+    run_gen("init", "bfs", egr, init)
+    say("[binary] bound, stage lines, touched rows")
+    outs = {}
+    for b in (0, 1):
+        outs[b] = run_binary("bfs", egr, init, bdir, nb, tmp_path, SCOUT_BFS_BOUND=b, SCOUT_DUMP_STATE=tmp_path / f"b{b}",
+                             SCOUT_DUMP_TOUCHED=tmp_path / f"t{b}.bin", SCOUT_NO_MEMO=1)
+        assert "[Deferred fallback: ON (default)]" in outs[b]
+    fb = {b: sum(int(t.split("=")[1]) for t in o.split() if t.startswith("fallback=")) for b, o in outs.items()}
+    assert fb[0] == 0 and fb[1] > 0, fb
+    for k in range(1, nb + 1):
+        assert (tmp_path / "b0" / f"bfs_{k}.bin").read_bytes() == (tmp_path / "b1" / f"bfs_{k}.bin").read_bytes()
+    stages = [ln for ln in outs[0].splitlines() if ln.startswith("[STAGE]")]
+    assert len(stages) == nb
+    for ln in stages:
+        f = dict(kv.split("=") for kv in ln.split()[1:])
+        assert float(f["h2d_ms"]) > 0 and float(f["seed_pack_ms"]) >= 0 and float(f["repair_ms"]) > 0
+    ids = np.fromfile(tmp_path / "t0.bin", dtype=np.int32)
+    assert len(ids) > 0 and np.all(np.diff(ids) > 0) and ids[0] >= 0 and ids[-1] < sa.Graph.read_egr(egr).n

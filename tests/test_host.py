@@ -131,3 +131,69 @@ def test_batch_line_parser_rules(tmp_path):
     # 1-indexed -> 0-indexed; self-loop, out-of-range and malformed lines dropped;
     # text after the second integer ignored
     assert pairs == [(0, 1), (3, 4), (6, 1), (5, 0)]
+
+
+def _write_mtx(path, n, pairs, comment=True):
+    with open(path, "w") as f:
+        if comment:
+            f.write("%%MatrixMarket matrix coordinate pattern general\n% written by a test\n")
+        f.write(f"{n} {n} {len(pairs)}\n")
+        for u, v in pairs:
+            f.write(f"{u + 1} {v + 1}\n")
+
+
+def test_mtx_converter_reproduces_the_reference_sample_egr(tmp_path, sample_dir):
+    """The reference's sample graph (wide, sorted, symmetric) listed row by row as a .mtx
+    converts back to the identical file (tools/efficient_converter.py writes wide output with
+    rows in input order)."""
+    egr = os.path.join(sample_dir, "sample.egr")
+    g = Graph.read_egr(egr)
+    src = np.repeat(np.arange(g.n), np.diff(g.off))
+    mtx, out = tmp_path / "s.mtx", tmp_path / "s.egr"
+    _write_mtx(mtx, g.n, list(zip(src.tolist(), g.col.tolist())))
+    run_gen("convert", mtx, out)
+    assert out.read_bytes() == open(egr, "rb").read()
+
+
+def test_mtx_converter_keeps_input_order_and_skips_rows_past_the_count(tmp_path):
+    rng = np.random.default_rng(4)
+    n = 300
+    pairs = [(int(u), int(v)) for u, v in rng.integers(0, n, (2000, 2))]
+    mtx, out = tmp_path / "r.mtx", tmp_path / "r.egr"
+    # a line whose row is past the vertex count is skipped, one with a single field too
+    _write_mtx(mtx, n, pairs)
+    with open(mtx, "a") as f:
+        f.write(f"{n + 5} 1\n7\n")
+    run_gen("convert", mtx, out)
+    raw = out.read_bytes()
+    assert np.frombuffer(raw[4:8], np.int32)[0] == -1, "always wide"
+    g = Graph.read_egr(str(out))
+    assert g.n == n and g.off[-1] == len(pairs)
+    for v in range(n):
+        assert g.col[g.off[v]:g.off[v + 1]].tolist() == [b for a, b in pairs if a == v], f"row {v}"
+
+
+def test_mtx_converter_rejects_malformed_entries(tmp_path):
+    mtx = tmp_path / "bad.mtx"
+    mtx.write_text("4 4 2\n1 2\nx 3\n")
+    r = subprocess.run([GEN, "convert", str(mtx), str(tmp_path / "bad.egr")], capture_output=True, text=True)
+    assert r.returncode != 0 and "line 3" in r.stderr
+
+
+def test_narrow_egr_reads_like_the_wide_file(tmp_path, sample_dir):
+    """The narrow layout (int32 entry count and int32 offsets, core/FlashGraph.h:130-192) is read
+    to the same rows as the wide one, by the host reader the binaries and the oracle share and
+    by the Python reader."""
+    egr = os.path.join(sample_dir, "sample.egr")
+    g = Graph.read_egr(egr)
+    narrow = tmp_path / "narrow.egr"
+    with open(narrow, "wb") as f:
+        f.write(np.array([g.n, g.off[-1]], np.int32).tobytes())
+        f.write(g.off.astype(np.int32).tobytes())
+        f.write(g.col.astype(np.int32).tobytes())
+    gn = Graph.read_egr(str(narrow))
+    assert np.array_equal(gn.off, g.off) and np.array_equal(gn.col, g.col)
+    ow, on = OracleGraph(egr), OracleGraph(str(narrow))
+    assert on.n == ow.n
+    for v in range(0, g.n, 7):
+        assert np.array_equal(on.row(v), ow.row(v))
