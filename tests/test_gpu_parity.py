@@ -122,19 +122,38 @@ def test_segments_match_sequential_batches(data):
     eng.close()
 
 
-def test_serving_loop_mix_equals_alone():
-    """The serving loop at a rate that queues requests: steps take several requests, and the
-    final levels and labels equal serving every request alone and a full recompute."""
+def test_serving_loop_steps_equal_the_oracle(tmp_path):
+    """The serving loop (Poisson arrivals, log-uniform request sizes, 60% deletions over
+    adjacency entries) at a rate that queues requests: every committed step's levels and labels
+    equal the oracle after the step's requests are applied in order, each against the graph the
+    earlier ones left; the final state equals serving every request alone and a full
+    recompute; steps are staged on one channel while the device repairs the other."""
     import json
-    say("[serve] scale 14")
-    exe = os.path.join(os.path.dirname(sa.LIB_PATH), "bin", "scout_serve")
-    r = subprocess.run([exe, "--scale", "14", "--rate", "2e8", "--min", "200", "--max", "6000", "--requests", "80",
-                        "--budget-lines", "30000", "--seed", "3"], capture_output=True, text=True, timeout=240)
+    say("[serve] R-MAT scale 12, 60 requests")
+    exe = os.path.join(sa.BIN_DIR, "scout_serve")
+    dump = tmp_path / "serve"
+    r = subprocess.run([exe, "--scale", "12", "--rate", "3e6", "--min", "50", "--max", "4000", "--requests", "60",
+                        "--pinned-mb", "1", "--hbm-mb", "8", "--seed", "3", "--dump", str(dump)],
+                       capture_output=True, text=True, timeout=240)
     assert r.returncode == 0, r.stderr
     out = json.loads(r.stdout.strip().splitlines()[-1])
-    say(f"[serve] {out['steps']} steps for {out['requests']} requests, p99 {out['p99_ms']} ms")
+    say(f"[serve] {out['steps']} steps, at most {out['max_requests_per_step']} requests per step, "
+        f"{out['steps_staged_while_device_busy']} staged while busy, PCIe {out['pcie_h2d_GBps']} GB/s")
     assert out["same_as_alone"] and out["verified_exact"]
-    assert out["max_requests_per_step"] > 1
+    assert out["max_requests_per_step"] > 1 and out["steps"] > 2
+    egr = tmp_path / "g.egr"
+    run_gen("rmat", 12, 16, 3, egr)
+    og = OracleGraph(str(egr))
+    steps = [tuple(map(int, ln.split())) for ln in (dump / "steps.txt").read_text().split("\n") if ln]
+    assert steps[0][0] == 0 and steps[-1][1] == 60 and all(a[1] == b[0] for a, b in zip(steps, steps[1:]))
+    for k, (first, last) in enumerate(steps):
+        for q in range(first, last):
+            uv = np.fromfile(dump / f"req_{q}.bin", dtype=np.int32).reshape(-1, 2)
+            og.apply(uv, og.classify(uv))
+        lv = np.fromfile(dump / f"lv_{k}.bin", dtype=np.int32)
+        lab = np.fromfile(dump / f"lab_{k}.bin", dtype=np.int32)
+        assert np.array_equal(lv, og.bfs(0)), f"step {k}: levels differ"
+        assert np.array_equal(lab, og.cc()), f"step {k}: labels differ"
 
 
 @pytest.mark.parametrize("name", ["rmat", "heavy"])
