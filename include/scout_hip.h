@@ -75,6 +75,8 @@ typedef struct {
   double repair_ms;     /* device time of the repair, from HIP events on the compute stream */
   double apply_ms;      /* device time of classification + graph update */
   int64_t readbacks;    /* host read-backs of device counters (each waits for the stream) */
+  int64_t inval_rounds; /* bfs: invalidation levels with work */
+  int64_t relax_rounds; /* bfs: relax levels with work */
 } scout_stats;
 
 const char* scout_last_error(void);
@@ -148,6 +150,13 @@ int scout_repair_bfs(scout_ctx* ctx, int channel, int strategy, scout_stats* st)
  * (st->fallback = 1) instead of the local tentative levels and relax. The answer is exact either
  * way, before the batch commits; 0 (the default) never falls back. */
 int scout_bfs_bound(scout_ctx* ctx, int64_t bound);
+/* Round groups of the next BFS repair only: its first invalidation group queues inval_rounds
+ * levels and its first relax group relax_rounds levels before the first read-back of each
+ * (at most 64; 0 keeps the strategy's own sizing). A measurement aid with no counterpart in the
+ * reference: fed with the rounds a plain repair of the same batch reported (inval_rounds,
+ * relax_rounds), it gives the schedule a perfect predictor would choose, the ceiling of what
+ * any learned schedule can gain. Results do not depend on it. */
+int scout_bfs_schedule(scout_ctx* ctx, int inval_rounds, int relax_rounds);
 /* CC min-id labels: replaces processBatch + bidirectional_reachability_check
  * (src/cc/scout_cc.cu:300-441). */
 int scout_repair_cc(scout_ctx* ctx, int channel, int strategy, scout_stats* st);
@@ -260,7 +269,13 @@ enum {
   SCOUT_PROF_MIS_EVAL = 6,
   SCOUT_PROF_CC_HOOK = 7,
   /* the k_invalidate launches alone (wave per medium-length candidate row) */
-  SCOUT_PROF_BFS_INVALIDATE_KERNEL = 8
+  SCOUT_PROF_BFS_INVALIDATE_KERNEL = 8,
+  /* single kernels of the BFS and CC repairs, timed launch by launch */
+  SCOUT_PROF_K_RELAX_SHORT = 9,
+  SCOUT_PROF_K_RELAX_ROWS = 10,
+  SCOUT_PROF_K_INVALIDATE_SHORT = 11,
+  SCOUT_PROF_K_INVAL_PUSH = 12,
+  SCOUT_PROF_K_CC_COLLECT = 13
 };
 int scout_prof_enable(scout_ctx* ctx, int on);
 int scout_prof_read(scout_ctx* ctx, int id, double* total_ms, int64_t* launches, int64_t* bytes);

@@ -31,6 +31,8 @@
 #include <unordered_set>
 #include <vector>
 
+#include <omp.h>
+
 #include "../scout_amd/csrc/host/egr.h"
 
 namespace {
@@ -613,6 +615,15 @@ struct orc_refbfs {
   int defer;
   int64_t deferred = 0;
   std::vector<int32_t> row;
+  std::vector<std::vector<int32_t>> rows;
+
+  // The round's row fetch: OpenMP threads gather the rows, as the reference's host fetch does
+  // before each GPU round (src/bfs/scout_bfs.cu:199,347); small rounds stay on one thread.
+  void fetch(const std::vector<int32_t>& vs, int64_t k) {
+    if ((int64_t)rows.size() < k) rows.resize((size_t)k);
+#pragma omp parallel for schedule(dynamic, 64) if (k >= 256)
+    for (int64_t i = 0; i < k; i++) g->neighbors(vs[(size_t)i], rows[(size_t)i]);
+  }
 
   // gpu_relax_round (src/bfs/scout_bfs.cu:177-272) on the CPU: admitted prefix only.
   std::vector<int32_t> relax_round(const std::vector<int32_t>& frontier) {
@@ -620,8 +631,9 @@ struct orc_refbfs {
     std::vector<int32_t> next;
     std::vector<int32_t> lv((size_t)k);
     for (int64_t i = 0; i < k; i++) lv[(size_t)i] = level[(size_t)frontier[(size_t)i]];
+    fetch(frontier, k);
     for (int64_t i = 0; i < k; i++) {
-      g->neighbors(frontier[(size_t)i], row);
+      const std::vector<int32_t>& row = rows[(size_t)i];
       int32_t cand = lv[(size_t)i] + 1;
       for (int32_t u : row) {
         int32_t old = level[(size_t)u];
@@ -637,10 +649,11 @@ struct orc_refbfs {
   std::vector<int32_t> deletion_round(const std::vector<int32_t>& work) {
     int64_t k = bound_prefix(*g, work, MAX_ROUND);
     std::vector<int32_t> next, nl((size_t)k);
+    fetch(work, k);
     for (int64_t i = 0; i < k; i++) {
       int32_t w = work[(size_t)i];
       int32_t old = level[(size_t)w];
-      g->neighbors(w, row);
+      const std::vector<int32_t>& row = rows[(size_t)i];
       int32_t best = -1;
       for (int32_t x : row) {
         int32_t l = level[(size_t)x];
@@ -745,23 +758,43 @@ struct orc_refcc {
     else side.assign(vv.begin(), vv.end());
     return 1;
   }
+  // The full recompute on the host, as full_recompute_cc_streamed does when the graph does not
+  // fit the device (src/cc/scout_cc.cu:76-116): OpenMP threads hook roots by relaxed CAS, the
+  // lower root winning, so every root ends as its component's smallest id.
+  static int32_t find_atomic(int32_t* p, int32_t x) {
+    while (true) {
+      int32_t q = __atomic_load_n(&p[x], __ATOMIC_RELAXED);
+      if (q == x) return x;
+      int32_t r = __atomic_load_n(&p[q], __ATOMIC_RELAXED);
+      if (r != q) __atomic_compare_exchange_n(&p[x], &q, r, false, __ATOMIC_RELAXED, __ATOMIC_RELAXED);
+      x = r;  // path halving
+    }
+  }
   void recompute() {
-    std::vector<int32_t> roots(label.size());
-    Snapshot s(*g);
-    std::vector<int32_t> q;
-    std::fill(roots.begin(), roots.end(), -1);
-    for (int32_t r = 0; r < g->n; r++) {
-      if (roots[(size_t)r] >= 0) continue;
-      roots[(size_t)r] = r;
-      q.assign(1, r);
-      for (size_t h = 0; h < q.size(); h++)
-        for (int64_t i = s.b(q[h]); i < s.e(q[h]); i++) {
-          int32_t y = s.col[(size_t)i];
-          if (roots[(size_t)y] < 0) {
-            roots[(size_t)y] = r;
-            q.push_back(y);
+    std::vector<int32_t> roots((size_t)g->n);
+    int32_t* p = roots.data();
+    for (int32_t v = 0; v < g->n; v++) p[v] = v;
+#pragma omp parallel
+    {
+      std::vector<int32_t> nb;
+#pragma omp for schedule(dynamic, 4096)
+      for (int32_t v = 0; v < g->n; v++) {
+        g->neighbors(v, nb);
+        for (int32_t w : nb) {
+          if (w <= v) continue;
+          int32_t a = v, b = w;
+          while (true) {
+            a = find_atomic(p, a);
+            b = find_atomic(p, b);
+            if (a == b) break;
+            if (a < b) std::swap(a, b);  // hook the larger root under the smaller
+            int32_t expect = a;
+            if (__atomic_compare_exchange_n(&p[a], &expect, b, false, __ATOMIC_RELAXED, __ATOMIC_RELAXED)) break;
           }
         }
+      }
+#pragma omp for schedule(static)
+      for (int32_t v = 0; v < g->n; v++) p[v] = find_atomic(p, v);
     }
     reset_from_roots(roots);
   }
@@ -838,6 +871,8 @@ void orc_refbfs_batch(orc_refbfs* r, const int32_t* uv, const uint8_t* ins, int6
     }
   }
 }
+
+int orc_threads(void) { return omp_get_max_threads(); }
 
 void orc_refbfs_finish(orc_refbfs* r, int32_t* level) {
   if (r->defer && r->deferred > 0) {

@@ -72,6 +72,8 @@ orc_refbfs* orc_refbfs_create(orc_graph* g, const int32_t* level, int32_t source
 void orc_refbfs_free(orc_refbfs* r);
 void orc_refbfs_batch(orc_refbfs* r, const int32_t* uv, const uint8_t* ins, int64_t m, int64_t* stats /* 8 */);
 void orc_refbfs_finish(orc_refbfs* r, int32_t* level);
+/* OpenMP threads the per-edit baselines use for row fetches and full recomputes. */
+int orc_threads(void);
 
 typedef struct orc_refcc orc_refcc;
 orc_refcc* orc_refcc_create(orc_graph* g, const int32_t* root, int64_t bound, int defer);

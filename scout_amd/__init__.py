@@ -33,7 +33,8 @@ class Stats(ctypes.Structure):
     _fields_ = [(name, ctypes.c_int64) for name in (
         "lines", "insertions", "deletions", "noop", "seeds", "touched", "rounds", "fallback",
         "splits", "conflicts", "sub_batches", "edges_scanned")] + [
-        ("repair_ms", ctypes.c_double), ("apply_ms", ctypes.c_double), ("readbacks", ctypes.c_int64)]
+        ("repair_ms", ctypes.c_double), ("apply_ms", ctypes.c_double), ("readbacks", ctypes.c_int64),
+        ("inval_rounds", ctypes.c_int64), ("relax_rounds", ctypes.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -95,6 +96,7 @@ _SIGNATURES = {
     "scout_stage_info": [_P, ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
                          ctypes.POINTER(_I64)],
     "scout_bfs_bound": [_P, _I64],
+    "scout_bfs_schedule": [_P, ctypes.c_int, ctypes.c_int],
     "scout_touched_enable": [_P, ctypes.c_int],
     "scout_touched_download": [_P, _P, _I64, ctypes.POINTER(_I64)],
     "scout_apply_segments": [_P, ctypes.c_int, _I32, _P, ctypes.c_int, ctypes.c_int, _P],
@@ -115,7 +117,8 @@ GEN_UNIFORM, GEN_DEGREE = 0, 1
 PROF_BFS_RELAX, PROF_BFS_INVALIDATE, PROF_CC_EXPAND, PROF_APPLY, PROF_CLASSIFY = 0, 1, 2, 3, 4
 PROF_COLOR, PROF_MIS_EVAL, PROF_CC_HOOK, PROF_BFS_INVALIDATE_KERNEL = 5, 6, 7, 8
 PROF_NAMES = ["bfs_relax", "bfs_invalidate", "cc_expand", "apply", "classify", "color", "mis_eval", "cc_hook",
-              "bfs_invalidate_kernel"]
+              "bfs_invalidate_kernel", "k_relax_short", "k_relax_rows", "k_invalidate_short", "k_inval_push",
+              "k_cc_collect"]
 PATH_NAMES = ["inval_tasks", "inval_push_tasks", "inval_kids_overflow", "cc_tasks", "relax_tasks",
               "tentative_tasks"]
 
@@ -412,6 +415,10 @@ class Engine:
         self._check(self.lib.scout_stage_info(self.h, channel, ctypes.byref(pack), ctypes.byref(h2d),
                                               ctypes.byref(by)), "stage_info")
         return {"pack_ms": pack.value, "h2d_ms": h2d.value, "bytes": by.value}
+
+    def bfs_schedule(self, inval_rounds: int, relax_rounds: int):
+        """Round groups of the next BFS repair only (a perfect predictor's schedule)."""
+        self._check(self.lib.scout_bfs_schedule(self.h, inval_rounds, relax_rounds), "bfs_schedule")
 
     def bfs_bound(self, bound: int):
         self._check(self.lib.scout_bfs_bound(self.h, bound), "bfs_bound")

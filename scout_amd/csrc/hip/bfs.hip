@@ -524,9 +524,10 @@ __global__ void k_invalidate(GView g, int32_t* level, int32_t* parent, const int
       int64_t lo, hi;
       task_range(r, task, lo, hi);
       int64_t at = -1;
-      for (int64_t base = lo; base < hi && at < 0; base += (int64_t)kU * kWave) {
+      // one row step first (support is usually among the first entries), then kU at a time
+      for (int64_t base = lo, step = kWave; base < hi && at < 0; base += step, step = (int64_t)kU * kWave) {
         int32_t xs[kU];
-        load_steps(g, r, base, hi, xs);
+        load_steps(g, r, base, min(hi, base + step), xs);
         int32_t lx[kU];
 #pragma unroll
         for (int j = 0; j < kU; j++) lx[j] = xs[j] >= 0 ? level[xs[j]] : INT32_MIN;
@@ -550,10 +551,11 @@ __global__ void k_invalidate(GView g, int32_t* level, int32_t* parent, const int
     Row r = row_of(g, c);
     int64_t len = row_len(r);
     int64_t at = -1, nk = 0;
-    // kU steps of the row in flight per iteration; `at` is uniform across the wave
-    for (int64_t base = 0; base < len && at < 0; base += (int64_t)kU * kWave) {
+    // one row step first (support is usually among the first entries of a row, whose hubs
+    // come first), then kU steps in flight per iteration; `at` is uniform across the wave
+    for (int64_t base = 0, step = kWave; base < len && at < 0; base += step, step = (int64_t)kU * kWave) {
       int32_t xs[kU];
-      load_steps(g, r, base, len, xs);
+      load_steps(g, r, base, min(len, base + step), xs);
       int32_t lx[kU];
 #pragma unroll
       for (int j = 0; j < kU; j++) lx[j] = xs[j] >= 0 ? level[xs[j]] : INT32_MIN;
@@ -952,7 +954,7 @@ int relax_levels(Ctx& c, int32_t* L, int32_t* P, int32_t lvl0, int32_t max_seed,
   // [11] medium rows, [12] scanned entries, [15] frontier vertices over all rounds,
   // [16] largest task count, [17] non-empty rounds
   int64_t* cnt[2] = {box + 8, box + 9};
-  k_zero<<<1, 64, 0, c.s>>>(box + 10, 3);
+  k_zero<<<1, 64, 0, c.s>>>(box + 10, 4);
   k_zero<<<1, 64, 0, c.s>>>(box + 15, 3);
   // seeds grouped by level once (each round then reads only its own), when the levels they
   // span fit the bucket counters and the list fits the scratch
@@ -976,11 +978,15 @@ int relax_levels(Ctx& c, int32_t* L, int32_t* P, int32_t lvl0, int32_t max_seed,
       int64_t ns = (nseeds > 0 && lvl <= max_seed) ? nseeds : 0;
       const int64_t* sb = (sbuckets && ns > 0) ? sbuckets + (lvl - lvl0) : nullptr;
       k_round_begin<<<1, 1, 0, c.s>>>(bcur, bnext, box + 10, box + 10, 2, box + 15, box + 17, box + 16, ns > 0);
+      SCOUT_CHECK(prof_begin(c, PROF_K_RELAX_SHORT));
       k_relax_short<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, P, cur, bcur, seeds, ns, sb, box + 15, lvl, nxt,
                                                     bnext, cap, c.stamp, epoch, c.le.p, box + 11, c.wb.p, box + 10,
-                                                    task_cap, box + 12);
+                                                    task_cap, box + 13);
+      SCOUT_CHECK(prof_end(c, PROF_K_RELAX_SHORT, 0));
+      SCOUT_CHECK(prof_begin(c, PROF_K_RELAX_ROWS));
       k_relax_rows<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, P, c.le.p, box + 11, c.wb.p, box + 10, task_cap, lvl,
                                                    nxt, bnext, cap, c.stamp, epoch, box + 12, c.d_paths);
+      SCOUT_CHECK(prof_end(c, PROF_K_RELAX_ROWS, 0));
       lvl++;
       std::swap(cur, nxt);
       p ^= 1;
@@ -999,10 +1005,13 @@ int relax_levels(Ctx& c, int32_t* L, int32_t* P, int32_t lvl0, int32_t max_seed,
     first_group = kFirstGroup;
   }
   // algorithmic bytes: per frontier vertex its id, level, the four row offsets and its queue
-  // entry (40 B); per scanned entry the neighbour id and its level (8 B)
-  prof_bytes(c, PROF_BFS_RELAX, 40 * c.h_box[15] + 8 * c.h_box[12]);
+  // entry (40 B); per scanned entry the neighbour id and its level (8 B). k_relax_short reads
+  // every frontier vertex and the short rows ([13]), k_relax_rows the longer rows ([12]).
+  prof_bytes(c, PROF_BFS_RELAX, 40 * c.h_box[15] + 8 * (c.h_box[12] + c.h_box[13]));
+  prof_bytes(c, PROF_K_RELAX_SHORT, 40 * c.h_box[15] + 8 * c.h_box[13]);
+  prof_bytes(c, PROF_K_RELAX_ROWS, 8 * c.h_box[12]);
   *rounds += c.h_box[17];
-  *scanned_total += c.h_box[12];
+  *scanned_total += c.h_box[12] + c.h_box[13];
   return SCOUT_OK;
 }
 
@@ -1098,6 +1107,10 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, int32_t* P, int strategy, scou
     predicted_last = std::max<int64_t>(c.h_box[25] - (int64_t)std::floor(c.pred_over), cmax);
     first_group = (int)std::min<int64_t>(std::max<int64_t>(predicted_last - cmin + 1, kFirstGroup), kMaxGroup);
   }
+  // a schedule handed in for this repair (the rounds a plain repair of the batch needed)
+  int hint_inval = c.hint_inval, hint_relax = c.hint_relax;
+  c.hint_inval = c.hint_relax = 0;
+  if (L == c.level && hint_inval > 0) first_group = hint_inval;
   if (ncand > 0) {
     // Invalidation, level by level, queued in groups of rounds. Counts: [8]/[9] candidates at
     // this level / the next (alternating), [10] invalid vertices, [11] scanned entries,
@@ -1119,8 +1132,9 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, int32_t* P, int strategy, scou
     int64_t* cnt[2] = {box + 8, box + 9};
     int64_t* binval = box + 10;
     int64_t* bscan = box + 11;
-    k_zero<<<1, 64, 0, c.s>>>(box + 8, 7);
-    k_zero<<<1, 64, 0, c.s>>>(box + 18, 5);
+    // [15] entries read by k_inval_push, [23] entries of short rows read by k_invalidate_short
+    k_zero<<<1, 64, 0, c.s>>>(box + 8, 8);
+    k_zero<<<1, 64, 0, c.s>>>(box + 18, 6);
     int p = 0;
     int group = first_group;
     int32_t lvl = cmin;
@@ -1136,25 +1150,29 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, int32_t* P, int strategy, scou
         k_round_begin<<<1, 1, 0, c.s>>>(bcur, bnext, box + 14, box + 12, 3, box + 18, box + 20, box + 19,
                                         nlevel > 0);
         const int64_t* cb = (cbuckets && nlevel > 0) ? cbuckets + (lvl - cmin) : nullptr;
+        SCOUT_CHECK(prof_begin(c, PROF_K_INVALIDATE_SHORT));
         k_invalidate_short<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, P, cur, bcur, cand, nlevel, cb, box + 18,
                                                            lvl, c.ld.p, binval, nxt, bnext, cap, c.stamp, sepoch,
                                                            c.le.p, box + 12, c.lf.p, box + 13, c.wb.p, box + 14,
-                                                           c.wb.cap, bscan, ab);
+                                                           c.wb.cap, box + 23, ab);
+        SCOUT_CHECK(prof_end(c, PROF_K_INVALIDATE_SHORT, 0));
         int32_t supepoch = next_epoch(c);
         SCOUT_CHECK(prof_begin(c, PROF_BFS_INVALIDATE_KERNEL));
         k_invalidate<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, P, c.le.p, box + 12, c.lf.p, c.wb.p, box + 14,
                                                      c.wb.cap, c.aux, supepoch, lvl, c.ld.p, binval, cap, nxt, bnext,
                                                      cap, c.stamp, sepoch, bscan, box + 21, c.d_paths, ab);
         SCOUT_CHECK(prof_end(c, PROF_BFS_INVALIDATE_KERNEL, 0));
+        SCOUT_CHECK(prof_begin(c, PROF_K_INVAL_PUSH));
         k_inval_push<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, P, lvl, c.lf.p, c.wb.p, box + 14, c.wb.cap, c.aux,
                                                      supepoch, c.ld.p, binval, cap, nxt, bnext, cap, c.stamp, sepoch,
-                                                     bscan, c.d_paths, ab);
+                                                     box + 15, c.d_paths, ab);
+        SCOUT_CHECK(prof_end(c, PROF_K_INVAL_PUSH, 0));
         lvl++;
         std::swap(cur, nxt);
         p ^= 1;
       }
       SCOUT_CHECK(prof_end(c, PROF_BFS_INVALIDATE, 0));
-      SCOUT_TRY(hipMemcpyAsync(c.h_box + 8, box + 8, 15 * sizeof(int64_t), hipMemcpyDeviceToHost, c.s));
+      SCOUT_TRY(hipMemcpyAsync(c.h_box + 8, box + 8, 16 * sizeof(int64_t), hipMemcpyDeviceToHost, c.s));
       SCOUT_TRY(hipStreamSynchronize(c.s));
       c.readbacks++;
       if (std::max(c.h_box[14], c.h_box[19]) > c.wb.cap) {
@@ -1167,6 +1185,7 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, int32_t* P, int strategy, scou
       first_group = kFirstGroup;
     }
     rounds += c.h_box[20];
+    st->inval_rounds += c.h_box[20];
     if (predicted_last >= 0) {
       int64_t last = cmin + c.h_box[20] - 1;  // deepest invalidation level
       double over = std::max<double>(0.0, (double)(c.h_box[25] - last));
@@ -1174,12 +1193,17 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, int32_t* P, int strategy, scou
     }
     // per candidate its id, level, row offsets and queue entry (40 B); per scanned entry the
     // neighbour and its level (8 B)
-    prof_bytes(c, PROF_BFS_INVALIDATE, 40 * c.h_box[18] + 8 * c.h_box[11]);
+    int64_t inval_scanned = c.h_box[11] + c.h_box[15] + c.h_box[23];
+    prof_bytes(c, PROF_BFS_INVALIDATE, 40 * c.h_box[18] + 8 * inval_scanned);
+    // k_invalidate_short: every candidate (40 B) and the entries of the short rows (8 B)
+    prof_bytes(c, PROF_K_INVALIDATE_SHORT, 40 * c.h_box[18] + 8 * c.h_box[23]);
     // k_invalidate alone: per row its id, offsets, level and parent write (36 B); per entry
     // read the neighbour and its level, per buffered child candidate its entry and parent (8 B)
     prof_bytes(c, PROF_BFS_INVALIDATE_KERNEL, 36 * c.h_box[22] + 8 * c.h_box[21]);
+    // k_inval_push: per entry of an unsupported long row the neighbour and its parent (8 B)
+    prof_bytes(c, PROF_K_INVAL_PUSH, 8 * c.h_box[15]);
     nx = c.h_box[10];
-    scanned += c.h_box[11];
+    scanned += inval_scanned;
     if (c.bfs_bound > 0 && nx > c.bfs_bound * ncand) {
       // the cascades passed the bound (src/bfs/scout_bfs.cu:475-488 per deletion, here over the
       // batch's candidates): the batch is settled by a full BFS before it commits
@@ -1230,7 +1254,10 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, int32_t* P, int strategy, scou
     if (learned && predicted_last >= 0)
       relax_group = (int)std::min<int64_t>(
           std::max<int64_t>(std::max<int64_t>(predicted_last, smax) - smin + 1, kFirstGroup), kMaxGroup);
+    if (L == c.level && hint_relax > 0) relax_group = hint_relax;
+    int64_t r0 = rounds;
     SCOUT_CHECK(relax_levels(c, L, P, smin, smax, c.wa.p, nseeds, epoch, &rounds, &scanned, relax_group));
+    st->relax_rounds += rounds - r0;
   }
   st->rounds += rounds;
   st->edges_scanned += scanned;
@@ -1273,6 +1300,16 @@ int scout_bfs_full(scout_ctx* x, int32_t source) {
   c.source = source;
   SCOUT_CHECK(bfs_full(c, source, nullptr, nullptr));
   SCOUT_TRY(hipStreamSynchronize(c.s));
+  return SCOUT_OK;
+}
+
+int scout_bfs_schedule(scout_ctx* x, int inval_rounds, int relax_rounds) {
+  if (!x || inval_rounds < 0 || relax_rounds < 0 || inval_rounds > 64 || relax_rounds > 64) {
+    set_error("bfs_schedule: rounds must be 0..64");
+    return SCOUT_ERR_ARG;
+  }
+  x->c.hint_inval = inval_rounds;
+  x->c.hint_relax = relax_rounds;
   return SCOUT_OK;
 }
 

@@ -356,7 +356,9 @@ int cc_local(Ctx& c, Channel& ch, scout_stats* st) {
   SCOUT_CHECK(prof_begin(c, PROF_CC_EXPAND));
   k_zero_u32<<<grid_for(words + cwords), kBlock, 0, c.s>>>(bits, words + cwords);
   if (ch.m > 0) k_cc_mark<<<grid_for(2 * ch.m), kBlock, 0, c.s>>>(ch.d_uv.p, ch.m, L, c.label, bits, coarse);
+  SCOUT_CHECK(prof_begin(c, PROF_K_CC_COLLECT));
   k_cc_collect<<<kListGrid, kBlock, 0, c.s>>>(L, c.label, bits, coarse, n, par, c.la.p, box, (int64_t)n + 1, box + 4);
+  SCOUT_CHECK(prof_end(c, PROF_K_CC_COLLECT, 0));
   k_cc_union_list<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), c.la.p, box, par, c.lb.p, box + 5, (int64_t)n + 1,
                                                   box + 2);
   k_cc_unreached_hook<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), c.lb.p, box + 5, par, c.wb.p, box + 1, c.wb.cap,
@@ -373,6 +375,9 @@ int cc_local(Ctx& c, Channel& ch, scout_stats* st) {
   // label (12 B); per scanned entry the neighbour and two parents (12 B); per batch line
   // endpoint its level, label and mark (12 B)
   prof_bytes(c, PROF_CC_EXPAND, 8 * (int64_t)n + 12 * c.h_box[0] + 12 * c.h_box[2] + 24 * ch.m);
+  // k_cc_collect: every vertex's level and label (8 B), per listed vertex its parent and list
+  // entry (8 B)
+  prof_bytes(c, PROF_K_CC_COLLECT, 8 * (int64_t)n + 8 * c.h_box[0]);
   int64_t comps = 1 + c.h_box[4] + c.h_box[3];  // vertex 0's, untouched ones, relabelled ones
   if (c.cc_components >= 0 && comps > c.cc_components) st->splits += comps - c.cc_components;
   c.cc_components = comps;

@@ -252,6 +252,7 @@ struct Ctx {
   float last_repair_ms = 0.f;
   int64_t readbacks = 0;         // host read-backs of device counters so far
   double pred_over = 0;          // running mean of how far the predicted last level overshoots
+  int hint_inval = 0, hint_relax = 0;  // scout_bfs_schedule: first group sizes of the next repair
 };
 
 constexpr int kBoxSlots = 64;
@@ -339,7 +340,8 @@ int prof_end(Ctx& c, int id, int64_t bytes);
 void prof_bytes(Ctx& c, int id, int64_t bytes);
 void prof_release(Ctx& c);
 enum ProfId { PROF_BFS_RELAX = 0, PROF_BFS_INVALIDATE, PROF_CC_EXPAND, PROF_APPLY, PROF_CLASSIFY,
-              PROF_COLOR, PROF_MIS_EVAL, PROF_CC_HOOK, PROF_BFS_INVALIDATE_KERNEL, PROF_COUNT };
+              PROF_COLOR, PROF_MIS_EVAL, PROF_CC_HOOK, PROF_BFS_INVALIDATE_KERNEL, PROF_K_RELAX_SHORT,
+              PROF_K_RELAX_ROWS, PROF_K_INVALIDATE_SHORT, PROF_K_INVAL_PUSH, PROF_K_CC_COLLECT, PROF_COUNT };
 
 // helpers in ctx.hip
 int box_read(Ctx& c, int count);  // copies d_box[0..count) into h_box, synchronises c.s

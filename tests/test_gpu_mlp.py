@@ -71,8 +71,11 @@ def _stream_engine(scale, seed):
 
 def test_training_on_logged_repairs_and_learned_equals_plain():
     """Logs 20 plain batches, trains on the device, then replays 6 more batches with the plain
-    and the learned strategy: levels and labels identical after every batch; the learned
-    repair queues enough rounds up front to need no more host read-backs than the plain one."""
+    strategy, the learned one and the schedule a perfect predictor would choose (each batch's
+    first invalidation and relax groups sized to the rounds the plain repair needed). Levels and
+    labels equal the oracle after every learned batch and are identical across the three; the
+    learned schedule needs fewer host read-backs than the plain one, and the perfect schedule
+    bounds what any learned schedule can save."""
     scale, bsize, ntrain, ntest = 15, 20000, 20, 6
     say(f"mlp: logged training, R-MAT scale {scale}, {ntrain} + {ntest} batches of {bsize}")
     eng = _stream_engine(scale, 5)
@@ -99,7 +102,7 @@ def test_training_on_logged_repairs_and_learned_equals_plain():
         say(f"mlp: {nsamp} samples, loss {first:.4f} -> {last:.4f}")
         assert last < first
 
-        def replay(strategy):
+        def replay(strategy, schedule=None, oracle=False):
             eng.restore()
             for b in range(ntrain):  # bring the graph to the state after the training batches
                 eng.batch_load(b)
@@ -107,25 +110,40 @@ def test_training_on_logged_repairs_and_learned_equals_plain():
                 eng.apply()
                 eng.repair_bfs(sa.STRATEGY_LOCAL)
                 eng.repair_cc(sa.STRATEGY_LOCAL)
-            states, readbacks = [], 0
-            for b in range(ntrain, total):
-                eng.batch_load(b)
+            og = None
+            if oracle:
+                g = eng.download_graph()
+                og = OracleGraph(n=g.n, off=g.off, col=g.col)
+            states, readbacks, rounds = [], 0, []
+            for i, b in enumerate(range(ntrain, total)):
+                m = eng.batch_load(b)
                 eng.classify()
                 eng.apply()
+                if schedule:
+                    eng.bfs_schedule(*schedule[i])
                 st = eng.repair_bfs(strategy)
                 eng.repair_cc(strategy)
                 readbacks += st.readbacks
-                states.append((eng.download(sa.STATE_LEVEL), eng.download(sa.STATE_LABEL)))
-            return states, readbacks
+                rounds.append((st.inval_rounds, st.relax_rounds))
+                lv, lab = eng.download(sa.STATE_LEVEL), eng.download(sa.STATE_LABEL)
+                if og is not None:
+                    uv = eng.batch_download(m)
+                    og.apply(uv, og.classify(uv))
+                    assert np.array_equal(lv, og.bfs(0)), f"learned batch {b}: levels differ from the oracle"
+                    assert np.array_equal(lab, og.cc()), f"learned batch {b}: labels differ from the oracle"
+                states.append((lv, lab))
+            return states, readbacks, rounds
 
-        plain, rb_plain = replay(sa.STRATEGY_LOCAL)
-        learned, rb_learned = replay(sa.STRATEGY_LEARNED)
-        say(f"mlp: BFS read-backs plain {rb_plain}, learned {rb_learned}")
-        for (lp, cp), (ll, cl) in zip(plain, learned):
-            assert np.array_equal(lp, ll)
-            assert np.array_equal(cp, cl)
-        assert rb_learned <= rb_plain
-        # and both equal a full recompute
+        plain, rb_plain, rounds = replay(sa.STRATEGY_LOCAL)
+        learned, rb_learned, _ = replay(sa.STRATEGY_LEARNED, oracle=True)
+        perfect, rb_perfect, _ = replay(sa.STRATEGY_LOCAL, schedule=[(max(a, 1), max(b, 1)) for a, b in rounds])
+        say(f"mlp: BFS read-backs over {ntest} batches: plain {rb_plain}, learned {rb_learned}, "
+            f"perfect schedule {rb_perfect}; rounds per batch {rounds}")
+        for (lp, cp), (ll, cl), (lq, cq) in zip(plain, learned, perfect):
+            assert np.array_equal(lp, ll) and np.array_equal(lp, lq)
+            assert np.array_equal(cp, cl) and np.array_equal(cp, cq)
+        assert rb_perfect <= rb_learned < rb_plain
+        # and all equal a full recompute
         eng.bfs_full(0)
         eng.cc_full()
         assert np.array_equal(eng.download(sa.STATE_LEVEL), learned[-1][0])
