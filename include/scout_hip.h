@@ -77,6 +77,7 @@ typedef struct {
   int64_t readbacks;    /* host read-backs of device counters (each waits for the stream) */
   int64_t inval_rounds; /* bfs: invalidation levels with work */
   int64_t relax_rounds; /* bfs: relax levels with work */
+  int64_t completed;    /* mis complete mode: vertices the completion added */
 } scout_stats;
 
 const char* scout_last_error(void);
@@ -172,6 +173,14 @@ int scout_repair_mis(scout_ctx* ctx, int channel, int64_t max_csr, scout_stats* 
  * scout_repair_mis gives it alone; all instances' rounds run in the same launches. */
 int scout_repair_mis_multi(scout_ctx* ctx, int channel, int32_t ninst, const int64_t* line_off, int64_t max_csr,
                            scout_stats* st);
+/* MIS mode. SCOUT_MIS_RULES (the default) applies the reference's per-line rules, which do not
+ * restore maximality (a deletion's candidate is rejected by any member within two hops,
+ * src/mis/scout_mis.cu:332-344; an insertion re-adds at most one vertex, :326).
+ * SCOUT_MIS_COMPLETE then completes the set: every vertex within two hops of an endpoint of the
+ * batch, in ascending id, joins unless it or a neighbour is a member, so a maximal set stays
+ * maximal. No counterpart in the reference. */
+enum { SCOUT_MIS_RULES = 0, SCOUT_MIS_COMPLETE = 1 };
+int scout_mis_mode(scout_ctx* ctx, int mode);
 /* Launch settings of the multi-instance MIS rounds: grid blocks, rounds per replay of the
  * captured hipGraph, graph on/off. No counterpart in the reference. */
 int scout_mis_config(scout_ctx* ctx, int grid, int group_rounds, int use_graph);

@@ -542,6 +542,53 @@ int64_t orc_check_mis_independent(orc_graph* h, const uint8_t* mis) {
   return bad;
 }
 
+// The completion of the MIS "complete" mode: after the batch's rules, every vertex within two
+// hops of an endpoint of the batch (on the graph after the batch), in ascending id, joins the
+// set unless it or a neighbour is already a member. Only vertices near the batch can have
+// lost their last member neighbour (rule writes and deleted edges touch endpoints and their
+// neighbours), so a maximal set before the batch is maximal again after it. Restated directly
+// from that definition (a sequential scan); returns the vertices added.
+int64_t orc_mis_complete(orc_graph* h, const int32_t* uv, int64_t m, uint8_t* mis) {
+  const LogicalGraph& g = h->g;
+  std::vector<uint8_t> in((size_t)g.n, 0);
+  std::vector<int32_t> region, row;
+  for (int64_t i = 0; i < 2 * m; i++)
+    if (!in[(size_t)uv[i]]) {
+      in[(size_t)uv[i]] = 1;
+      region.push_back(uv[i]);
+    }
+  size_t begin = 0;
+  for (int hop = 0; hop < 2; hop++) {  // expand the newest layer each hop
+    size_t end = region.size();
+    for (size_t k = begin; k < end; k++) {
+      g.neighbors(region[k], row);
+      for (int32_t w : row)
+        if (!in[(size_t)w]) {
+          in[(size_t)w] = 1;
+          region.push_back(w);
+        }
+    }
+    begin = end;
+  }
+  std::sort(region.begin(), region.end());
+  int64_t added = 0;
+  for (int32_t v : region) {
+    if (mis[v]) continue;
+    g.neighbors(v, row);
+    bool dominated = false;
+    for (int32_t w : row)
+      if (mis[w]) {
+        dominated = true;
+        break;
+      }
+    if (!dominated) {
+      mis[v] = 1;
+      added++;
+    }
+  }
+  return added;
+}
+
 int64_t orc_mis_maximality_violations(orc_graph* h, const uint8_t* mis) {
   Snapshot s(h->g);
   int64_t bad = 0;

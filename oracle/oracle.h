@@ -58,6 +58,9 @@ int64_t orc_check_cc(orc_graph* g, const int32_t* label);
 int64_t orc_check_coloring(orc_graph* g, const int32_t* color);
 int64_t orc_check_mis_independent(orc_graph* g, const uint8_t* mis);
 int64_t orc_mis_maximality_violations(orc_graph* g, const uint8_t* mis);
+/* MIS complete mode: greedy ascending completion over the batch's 2-hop region; returns the
+ * vertices added. */
+int64_t orc_mis_complete(orc_graph* g, const int32_t* uv, int64_t m, uint8_t* mis);
 
 /* Initial states by greedy ascending id (tools/compute_initial_mis.py:32-43,
  * tools/greedy_color.cpp:37-52). */

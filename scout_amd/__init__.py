@@ -23,6 +23,7 @@ OK = 0
 STATE_MIS, STATE_COLOR, STATE_LEVEL, STATE_LABEL = 0, 1, 2, 3
 CLASSIFY_CURRENT, CLASSIFY_BASE = 0, 1
 STRATEGY_LOCAL, STRATEGY_RECOMPUTE, STRATEGY_LEARNED = 0, 1, 2
+MIS_RULES, MIS_COMPLETE = 0, 1
 
 
 class ScoutError(RuntimeError):
@@ -34,7 +35,7 @@ class Stats(ctypes.Structure):
         "lines", "insertions", "deletions", "noop", "seeds", "touched", "rounds", "fallback",
         "splits", "conflicts", "sub_batches", "edges_scanned")] + [
         ("repair_ms", ctypes.c_double), ("apply_ms", ctypes.c_double), ("readbacks", ctypes.c_int64),
-        ("inval_rounds", ctypes.c_int64), ("relax_rounds", ctypes.c_int64)]
+        ("inval_rounds", ctypes.c_int64), ("relax_rounds", ctypes.c_int64), ("completed", ctypes.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -68,6 +69,7 @@ _SIGNATURES = {
     "scout_repair_mis": [_P, ctypes.c_int, _I64, ctypes.POINTER(Stats)],
     "scout_repair_mis_multi": [_P, ctypes.c_int, _I32, _P, _I64, ctypes.POINTER(Stats)],
     "scout_mis_config": [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int],
+    "scout_mis_mode": [_P, ctypes.c_int],
     "scout_mis_limits": [_P, _I64, _I64],
     "scout_color_config": [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int],
     "scout_color_get_config": [_P, _P],
@@ -308,6 +310,11 @@ class Engine:
         self._check(self.lib.scout_repair_mis_multi(self.h, channel, len(off) - 1, _ptr(off), max_csr,
                                                     ctypes.byref(st)), "repair_mis_multi")
         return st
+
+    def mis_mode(self, mode: int):
+        """MIS_RULES (the reference's rules) or MIS_COMPLETE (rules, then greedy completion over
+        the batch's 2-hop region, which keeps the set maximal)."""
+        self._check(self.lib.scout_mis_mode(self.h, mode), "mis_mode")
 
     def mis_config(self, grid: int = 1024, group_rounds: int = 4, use_graph: bool = True):
         self._check(self.lib.scout_mis_config(self.h, grid, group_rounds, 1 if use_graph else 0), "mis_config")

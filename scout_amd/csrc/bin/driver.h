@@ -19,7 +19,8 @@
 //   SCOUT_NO_MEMO, SCOUT_FORCE_COPY: accepted with no effect (rows are resident in HBM, so there
 //     is no fetch memo, and every transfer to the discrete MI355X is an explicit copy); a note
 //     goes to stderr.
-// Added: SCOUT_DUMP_STATE=<dir> writes the solution after every batch as raw little-endian
+// Added: SCOUT_MIS_COMPLETE=1 (mis) completes the set after the reference's rules, keeping it
+// maximal (scout_mis_mode); SCOUT_DUMP_STATE=<dir> writes the solution after every batch as raw little-endian
 // arrays (<dir>/<problem>_<batch>.bin: uint8 membership / int32 otherwise);
 // SCOUT_STRATEGY=local|recompute|learned (bfs, cc).
 #pragma once
@@ -169,6 +170,8 @@ inline int run_driver(int argc, char** argv, Problem prob) {
     if (fit > 1000000LL && mx > fit) mx = fit;
     max_csr = mx;
     std::cout << "MAX_CSR_ELEMENTS set to: " << max_csr << std::endl;
+    if (const char* cm = std::getenv("SCOUT_MIS_COMPLETE"))
+      if (std::string(cm) == "1") DRV_CHECK(scout_mis_mode(ctx, SCOUT_MIS_COMPLETE));
   }
   double setup_ms = ms_since(setup_t0);
 

@@ -1012,11 +1012,18 @@ struct MisState {
   int64_t h2_rcap = kHop2Requests, h2_tcap = kHop2Tasks;  // list capacities (fallback: serial)
   hipGraphExec_t exec = nullptr;
   std::vector<const void*> key;
+  // complete mode (scout_mis_mode): per-vertex completion state (0 none, 1 undecided
+  // candidate, 2 joins, 3 stays out) and counts of undecided lower-id candidate neighbours
+  bool complete = false;
+  DevBuf<uint8_t> cst;
+  DevBuf<int32_t> ccnt;
 };
 
 void mis_release(Ctx& c) {
   if (!c.mis_state) return;
   MisState& ms = *c.mis_state;
+  ms.cst.release();
+  ms.ccnt.release();
   ms.w0.release();
   ms.w1.release();
   ms.uv2.release();
@@ -1138,6 +1145,198 @@ int mm_graph(Ctx& c, MisState& ms, SegView sv) {
     return SCOUT_ERR_HIP;
   }
   ms.key = key;
+  return SCOUT_OK;
+}
+
+// --- complete mode ----------------------------------------------------------------------
+// After the rules, every vertex within two hops of a batch endpoint joins the set in ascending
+// id unless it or a neighbour is a member (the completion the oracle restates as a sequential
+// scan). Only undominated vertices of the region (no member neighbour: mcount == 0) can join,
+// and the ascending scan over them is the greedy MIS of the subgraph they induce: it runs in
+// Kahn rounds over the id-ordered DAG, a candidate deciding once every lower-id candidate
+// neighbour has decided.
+
+__global__ void k_zero_i64(int64_t* p, int count) {
+  if ((int)threadIdx.x < count) p[threadIdx.x] = 0;
+}
+
+// The batch endpoints, listed once (stamp epoch).
+__global__ void k_cm_endpoints(const int32_t* uv, int64_t m, int32_t* stamp, int32_t ep, int32_t* list, int64_t* count,
+                               int64_t cap) {
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < 2 * m; base += (int64_t)gridDim.x * blockDim.x) {
+    int64_t i = base + threadIdx.x;
+    bool take = false;
+    int32_t x = 0;
+    if (i < 2 * m) {
+      x = uv[i];
+      take = atomicExch(&stamp[x], ep) != ep;
+    }
+    block_append(take, x, list, count, cap);
+  }
+}
+
+// Wave per listed vertex of list[lo, hi): its neighbours not listed before are appended.
+__global__ void k_cm_expand(GView g, int32_t* list, int64_t lo, int64_t hi, int32_t* stamp, int32_t ep,
+                            int64_t* count, int64_t cap) {
+  for (int64_t i = lo + global_wave(); i < hi; i += total_waves()) {
+    Row r = row_of(g, list[i]);
+    int64_t len = row_len(r);
+    for (int64_t base = 0; base < len; base += kWave) {
+      int64_t k = base + lane_id();
+      int32_t w = k < len ? row_at(g, r, k) : -1;
+      bool take = w >= 0 && stamp[w] != ep && atomicExch(&stamp[w], ep) != ep;
+      wave_append(take, w, list, count, cap);
+    }
+  }
+}
+
+// Region vertices that are neither members nor next to one become candidates.
+__global__ void k_cm_candidates(const int32_t* list, int64_t nl, const uint8_t* mis, const int32_t* mcount,
+                                uint8_t* cst, int32_t* cand, int64_t* count, int64_t cap) {
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < nl; base += (int64_t)gridDim.x * blockDim.x) {
+    int64_t i = base + threadIdx.x;
+    bool take = false;
+    int32_t v = 0;
+    if (i < nl) {
+      v = list[i];
+      take = !mis[v] && mcount[v] == 0;
+      if (take) cst[v] = 1;
+    }
+    block_append(take, v, cand, count, cap);
+  }
+}
+
+// Wave per candidate: its lower-id candidate neighbours; none makes it ready at once.
+__global__ void k_cm_lower(GView g, const int32_t* cand, int64_t nc, const uint8_t* cst, int32_t* ccnt, int32_t* ready,
+                           int64_t* ready_count, int64_t cap) {
+  for (int64_t i = global_wave(); i < nc; i += total_waves()) {
+    int32_t v = cand[i];
+    Row r = row_of(g, v);
+    int64_t len = row_len(r), cnt = 0;
+    for (int64_t k = lane_id(); k < len; k += kWave) {
+      int32_t w = row_at(g, r, k);
+      cnt += w >= 0 && w < v && cst[w] == 1;
+    }
+    cnt = wave_sum(cnt);
+    if (lane_id() == 0) ccnt[v] = (int32_t)cnt;
+    wave_append(lane_id() == 0 && cnt == 0, v, ready, ready_count, cap);
+  }
+}
+
+// Wave per ready candidate: it joins unless a lower-id candidate neighbour joined.
+__global__ void k_cm_decide(GView g, const int32_t* ready, const int64_t* ready_count, uint8_t* cst) {
+  int64_t nr = *ready_count;
+  for (int64_t i = global_wave(); i < nr; i += total_waves()) {
+    int32_t v = ready[i];
+    Row r = row_of(g, v);
+    int64_t len = row_len(r);
+    bool blocked = false;
+    for (int64_t base = 0; base < len && !blocked; base += kWave) {
+      int64_t k = base + lane_id();
+      int32_t w = k < len ? row_at(g, r, k) : -1;
+      blocked = __ballot(w >= 0 && w < v && cst[w] == 2) != 0;
+    }
+    if (lane_id() == 0) cst[v] = blocked ? 3 : 2;
+  }
+}
+
+// Wave per decided candidate: each higher-id undecided candidate neighbour has one lower
+// neighbour fewer to wait for; those reaching zero are ready next round.
+__global__ void k_cm_release(GView g, const int32_t* ready, const int64_t* ready_count, const uint8_t* cst,
+                             int32_t* ccnt, int32_t* next, int64_t* next_count, int64_t cap) {
+  int64_t nr = *ready_count;
+  for (int64_t i = global_wave(); i < nr; i += total_waves()) {
+    int32_t v = ready[i];
+    Row r = row_of(g, v);
+    int64_t len = row_len(r);
+    for (int64_t base = 0; base < len; base += kWave) {
+      int64_t k = base + lane_id();
+      int32_t w = k < len ? row_at(g, r, k) : -1;
+      bool take = w > v && cst[w] == 1 && atomicSub(&ccnt[w], 1) == 1;
+      wave_append(take, w, next, next_count, cap);
+    }
+  }
+}
+
+// Wave per candidate: joiners become members and raise their neighbours' member counts; the
+// completion state is cleared for the next batch. *added += joiners.
+__global__ void k_cm_commit(GView g, const int32_t* cand, int64_t nc, uint8_t* cst, uint8_t* mis, int32_t* mcount,
+                            int64_t* added) {
+  int64_t joined = 0;
+  for (int64_t i = global_wave(); i < nc; i += total_waves()) {
+    int32_t v = cand[i];
+    bool join = cst[v] == 2;
+    if (join) {
+      Row r = row_of(g, v);
+      int64_t len = row_len(r);
+      for (int64_t k = lane_id(); k < len; k += kWave) {
+        int32_t w = row_at(g, r, k);
+        if (w >= 0) atomicAdd(&mcount[w], 1);
+      }
+      joined++;
+    }
+    if (lane_id() == 0) {
+      if (join) mis[v] = 1;
+      cst[v] = 0;
+    }
+  }
+  block_add(lane_id() == 0 ? joined : 0, added);
+}
+
+int mis_complete(Ctx& c, Channel& ch, MisState& ms, scout_stats* st) {
+  int64_t m = ch.m;
+  int32_t n = c.g.n;
+  if (m == 0) return SCOUT_OK;
+  int64_t cap = (int64_t)n + 1;
+  SCOUT_CHECK(c.la.reserve(cap));
+  SCOUT_CHECK(c.lb.reserve(cap));
+  SCOUT_CHECK(c.lc.reserve(cap));
+  SCOUT_CHECK(c.ld.reserve(cap));
+  if (ms.cst.cap < cap) {
+    ms.cst.release();
+    SCOUT_CHECK(ms.cst.reserve(cap));
+    SCOUT_TRY(hipMemsetAsync(ms.cst.p, 0, (size_t)ms.cst.cap, c.s));
+  }
+  SCOUT_CHECK(ms.ccnt.reserve(cap));
+  int32_t ep = next_epoch(c);
+  int64_t* box = c.d_box;
+  // box: [0] region size, [1] candidates, [2]/[3] ready lists, [4] joiners
+  SCOUT_TRY(hipMemsetAsync(box, 0, 5 * sizeof(int64_t), c.s));
+  k_cm_endpoints<<<grid_for(2 * m), kBlock, 0, c.s>>>(ch.d_uv.p, m, c.stamp, ep, c.la.p, box, cap);
+  int64_t lo = 0;
+  for (int hop = 0; hop < 2; hop++) {
+    SCOUT_CHECK(box_read(c, 1));
+    int64_t hi = std::min(c.h_box[0], cap);
+    k_cm_expand<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), c.la.p, lo, hi, c.stamp, ep, box, cap);
+    lo = hi;
+  }
+  SCOUT_CHECK(box_read(c, 1));
+  int64_t nr = std::min(c.h_box[0], cap);
+  k_cm_candidates<<<grid_for(nr), kBlock, 0, c.s>>>(c.la.p, nr, c.mis, ms.mcount.p, ms.cst.p, c.lb.p, box + 1, cap);
+  SCOUT_CHECK(box_read(c, 2));
+  int64_t nc = c.h_box[1];
+  int64_t rounds = 0;
+  if (nc > 0) {
+    k_cm_lower<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), c.lb.p, nc, ms.cst.p, ms.ccnt.p, c.lc.p, box + 2, cap);
+    int32_t* cur = c.lc.p;
+    int32_t* nxt = c.ld.p;
+    int p = 0;
+    while (true) {
+      SCOUT_CHECK(box_read(c, 4));
+      if (c.h_box[2 + p] == 0) break;
+      rounds++;
+      k_zero_i64<<<1, 64, 0, c.s>>>(box + 2 + (p ^ 1), 1);
+      k_cm_decide<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), cur, box + 2 + p, ms.cst.p);
+      k_cm_release<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), cur, box + 2 + p, ms.cst.p, ms.ccnt.p, nxt,
+                                                   box + 2 + (p ^ 1), cap);
+      std::swap(cur, nxt);
+      p ^= 1;
+    }
+    k_cm_commit<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), c.lb.p, nc, ms.cst.p, c.mis, ms.mcount.p, box + 4);
+    SCOUT_CHECK(box_read(c, 5));
+    st->completed += c.h_box[4];
+  }
+  st->rounds += rounds;
   return SCOUT_OK;
 }
 
@@ -1275,6 +1474,7 @@ int mis_repair_multi(Ctx& c, Channel& ch, int32_t ninst, const int64_t* off, int
   st->rounds += rounds;
   st->conflicts += writes;
   st->touched += evals;  // line evaluations of pass A over all rounds
+  if (ms.complete) SCOUT_CHECK(mis_complete(c, ch, ms, st));
   return SCOUT_OK;
 }
 
@@ -1337,6 +1537,17 @@ extern "C" int scout_repair_mis(scout_ctx* x, int channel, int64_t max_csr, scou
 extern "C" int scout_repair_mis_multi(scout_ctx* x, int channel, int32_t ninst, const int64_t* line_off,
                                       int64_t max_csr, scout_stats* st) {
   return mis_entry(x, channel, ninst, line_off, max_csr, st, "repair_mis_multi");
+}
+
+extern "C" int scout_mis_mode(scout_ctx* x, int mode) {
+  if (!x || (mode != SCOUT_MIS_RULES && mode != SCOUT_MIS_COMPLETE)) {
+    set_error("mis_mode: unknown mode");
+    return SCOUT_ERR_ARG;
+  }
+  Ctx& c = x->c;
+  if (!c.mis_state) c.mis_state = new MisState();
+  c.mis_state->complete = mode == SCOUT_MIS_COMPLETE;
+  return SCOUT_OK;
 }
 
 extern "C" int scout_mis_config(scout_ctx* x, int grid, int group_rounds, int use_graph) {

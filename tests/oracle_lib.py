@@ -36,6 +36,7 @@ _SIG = {
     "orc_check_coloring": ([_P, _P], _I64),
     "orc_check_mis_independent": ([_P, _P], _I64),
     "orc_mis_maximality_violations": ([_P, _P], _I64),
+    "orc_mis_complete": ([_P, _P, _I64, _P], _I64),
     "orc_greedy_mis": ([_P, _P], None),
     "orc_greedy_color": ([_P, _P], None),
     "orc_refbfs_create": ([_P, _P, _I32, _I64, ctypes.c_int], _P),
@@ -165,6 +166,13 @@ class OracleGraph:
 
     def check_mis_independent(self, mis):
         return int(lib().orc_check_mis_independent(self.h, _p(np.ascontiguousarray(mis, dtype=np.uint8))))
+
+    def mis_complete(self, uv, mis):
+        """MIS complete mode: greedy ascending completion over the batch's 2-hop region."""
+        uv = np.ascontiguousarray(uv, dtype=np.int32).reshape(-1)
+        mis = np.ascontiguousarray(mis, dtype=np.uint8).copy()
+        added = lib().orc_mis_complete(self.h, _p(uv), uv.size // 2, _p(mis))
+        return mis, int(added)
 
     def mis_maximality_violations(self, mis):
         return int(lib().orc_mis_maximality_violations(self.h, _p(np.ascontiguousarray(mis, dtype=np.uint8))))

@@ -247,6 +247,34 @@ def test_mis_exact_every_batch(data, name):
         assert og.check_mis_independent(got) == 0
 
 
+@pytest.mark.parametrize("name", ["sample", "er", "rmat", "deg"])
+def test_mis_complete_mode_exact_and_maximal_every_batch(data, name):
+    """Complete mode: the reference's rules, then the greedy completion over the batch's 2-hop
+    region. Equal to the oracle's sequential completion after every batch, independent, and
+    maximal on the whole graph (the greedy start is maximal)."""
+    egr, bdir, nb = data[name]
+    say(f"[mis complete] {name}")
+    eng, g = engine_for(egr)
+    eng.mis_mode(sa.MIS_COMPLETE)
+    og, base = OracleGraph(egr), OracleGraph(egr)
+    mis = og.greedy_mis()
+    eng.upload(sa.STATE_MIS, mis)
+    max_csr = mis_max_csr(g.off[-1] / g.n, 288e9)
+    for k, uv in batches(bdir, nb, g.n):
+        eng.stage(uv)
+        ins = eng.classify(sa.CLASSIFY_BASE)
+        eng.apply(True)
+        og.apply(uv, ins, conditional=True)
+        st = eng.repair_mis(max_csr)
+        mis, _ = og.mis_batch(uv, ins, mis, max_csr)
+        mis, added = og.mis_complete(uv, mis)
+        got = eng.download(sa.STATE_MIS)
+        assert np.array_equal(got, mis), f"{name} batch {k}: {np.count_nonzero(got != mis)} memberships differ"
+        assert st.completed == added
+        assert og.check_mis_independent(got) == 0 and og.mis_maximality_violations(got) == 0
+    eng.close()
+
+
 def test_mis_small_subbatches_match(data):
     # a tiny MAX_CSR_ELEMENTS cuts every batch into many sub-batches with different rows
     egr, bdir, nb = data["er"]
@@ -412,6 +440,27 @@ def test_mis_er_100k_full_size_exact_every_batch(tmp_path):
         assert og.check_mis_independent(got) == 0
         say(f"[mis] batch {k}: {int(got.sum())} members, {og.mis_maximality_violations(got)} maximality "
             f"violations, {st.rounds} rounds, {st.repair_ms:.2f} ms")
+    eng.close()
+    # complete mode on the same stream: equal to the oracle's completion, independent and maximal
+    say("[mis] ER 100k, complete mode")
+    eng, g = engine_for(str(egr))
+    eng.mis_mode(sa.MIS_COMPLETE)
+    og, base = OracleGraph(str(egr)), OracleGraph(str(egr))
+    mis = og.greedy_mis()
+    eng.upload(sa.STATE_MIS, mis)
+    for k, uv in batches(str(tmp_path / "b"), 10, g.n):
+        eng.stage(uv)
+        ins = eng.classify(sa.CLASSIFY_BASE)
+        eng.apply(True)
+        og.apply(uv, ins, conditional=True)
+        st = eng.repair_mis(max_csr)
+        mis, _ = og.mis_batch(uv, ins, mis, max_csr)
+        mis, added = og.mis_complete(uv, mis)
+        got = eng.download(sa.STATE_MIS)
+        assert np.array_equal(got, mis), f"complete batch {k}: {np.count_nonzero(got != mis)} memberships differ"
+        assert og.check_mis_independent(got) == 0 and og.mis_maximality_violations(got) == 0
+        say(f"[mis] complete batch {k}: {int(got.sum())} members, {added} added by the completion, "
+            f"{st.repair_ms:.2f} ms")
     eng.close()
 
 

@@ -280,6 +280,88 @@ def test_mis_insertion_skips_vertices_without_row():
     assert out == {0, 4}
 
 
+def _mis_case_csr(n, edges, members, batch, max_csr):
+    off, col = csr_from_edges(n, edges)
+    g = OracleGraph(n=n, off=off, col=col)
+    uv = np.array(batch, dtype=np.int32).reshape(-1, 2)
+    ins = g.classify(uv)
+    g.apply(uv, ins, conditional=True)
+    mis = np.zeros(n, dtype=np.uint8)
+    mis[list(members)] = 1
+    out, st = g.mis_batch(uv, ins, mis, max_csr)
+    return set(np.flatnonzero(out).tolist()), int(st[0])
+
+
+def test_mis_sub_batch_cut_decides_the_row_kinds():
+    """The deg*1000 estimate against MAX_CSR_ELEMENTS (src/mis/scout_mis.cu:519-542) cuts the
+    batch, and the cut changes the answer, because a vertex's row is its 2-hop ball only when it
+    is an endpoint of a line in the same sub-batch. Path 1-2-3-4 plus vertices 0 and 5,
+    M = {0, 1, 4}; lines: insert (0,1), insert (2,5). Degrees after the batch: 0:1, 1:2, 2:3,
+    5:1, so the first line estimates 3000 and the second 4000.
+      * max_csr 7001 (3000 + 4000 < 7001): one sub-batch. 1 leaves; ball(1) = [1, 2, 0, 3, 5];
+        2 is an endpoint, its ball holds members 0 and 4; 3's adjacency row holds 4; 5's ball
+        [5, 2, 1, 3] holds none, so 5 joins: {0, 4, 5}.
+      * max_csr 7000 (3000 + 4000 >= 7000 cuts): line 1 alone; 2 is only next to an endpoint,
+        its adjacency row [1, 3, 5] holds no member, so 2 joins; line 2 then has one member
+        endpoint on an insertion and changes nothing: {0, 2, 4}.
+    Degrees taken before the batch (0:0, 1:1, 2:2, 5:0) would keep one sub-batch at 7000."""
+    edges = [(1, 2), (2, 3), (3, 4)]
+    batch = [0, 1, 2, 5]
+    out, subs = _mis_case_csr(6, edges, {0, 1, 4}, batch, 7001)
+    assert subs == 1 and out == {0, 4, 5}
+    out, subs = _mis_case_csr(6, edges, {0, 1, 4}, batch, 7000)
+    assert subs == 2 and out == {0, 2, 4}
+
+
+def test_mis_complete_restores_maximality_on_a_hand_case():
+    # path 0-1-2-3, M = {0, 3}; deleting (0,1) leaves 1 undominated (the reference's rule keeps
+    # it out: 3 is in its 2-hop ball); the completion scans the region {0, 1, 2, 3} in id
+    # order and adds 1 (no member neighbour left), after which 2 stays dominated
+    n = 4
+    off, col = csr_from_edges(n, [(0, 1), (1, 2), (2, 3)])
+    g = OracleGraph(n=n, off=off, col=col)
+    uv = np.array([[0, 1]], dtype=np.int32)
+    ins = g.classify(uv)
+    g.apply(uv, ins, conditional=True)
+    mis = np.array([1, 0, 0, 1], dtype=np.uint8)
+    mis, _ = g.mis_batch(uv, ins, mis, BIG)
+    assert g.mis_maximality_violations(mis) == 1
+    done, added = g.mis_complete(uv, mis)
+    assert added == 1 and np.flatnonzero(done).tolist() == [0, 1, 3]
+    assert g.mis_maximality_violations(done) == 0
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_mis_complete_keeps_the_set_maximal_and_touches_only_the_region(seed):
+    """Rules, then completion, batch after batch from the greedy set: independent and maximal
+    on the whole graph after every batch; members are never removed by the completion, and
+    vertices more than two hops from every endpoint keep their membership."""
+    rng = np.random.default_rng(seed)
+    n = 300
+    off, col = csr_from_edges(n, random_graph(n, 700, rng))
+    g = OracleGraph(n=n, off=off, col=col)
+    mis = g.greedy_mis()
+    for _ in range(6):
+        uv = rng.integers(0, n, (40, 2)).astype(np.int32)
+        uv = uv[uv[:, 0] != uv[:, 1]]
+        _, keep = np.unique(np.sort(uv, axis=1), axis=0, return_index=True)
+        uv = uv[np.sort(keep)]
+        ins = g.classify(uv)
+        g.apply(uv, ins, conditional=True)
+        mis, _ = g.mis_batch(uv, ins, mis, BIG)
+        done, added = g.mis_complete(uv, mis)
+        assert added == int(done.sum()) - int(mis.sum()) and np.all(done >= mis)
+        # the 2-hop region on the graph after the batch
+        region = set(uv.ravel().tolist())
+        for _hop in range(2):
+            region |= {int(w) for v in list(region) for w in g.row(v)}
+        changed = set(np.flatnonzero(done != mis).tolist())
+        assert changed <= region
+        assert g.check_mis_independent(done) == 0
+        assert g.mis_maximality_violations(done) == 0
+        mis = done
+
+
 def test_mis_max_csr_formula():
     assert mis_max_csr(16.0, 288e9) == min(800000000, int(1048576 * 2 * 256))
     assert mis_max_csr(0.0, 288e9) == 50000000
