@@ -95,10 +95,18 @@ int scout_mem_info(scout_ctx* ctx, int64_t* free_bytes, int64_t* total_bytes);
 /* Loads the base graph into HBM, streaming the (mmap'd) .egr arrays through pinned staging
  * on the copy stream. nindex is int32 (wide = 0) or uint64 (wide = 1), n+1 entries.
  * Replaces the per-round row fetch HybridGraph::neighbors_into (core/FlashGraph.h:376-385)
- * and the chunked nlist upload of full_recompute_cc_gpu (src/cc/scout_cc.cu:157-215). */
+ * and the chunked nlist upload of full_recompute_cc_gpu (src/cc/scout_cc.cu:157-215).
+ * Storage-resident mode: when the context's HBM budget cannot hold the base entries
+ * (4 B each), they are copied by OpenMP threads into pinned host memory that the kernels read
+ * over PCIe, and only the offsets, the inserted rows and the per-vertex state go to HBM; the
+ * full CC recompute then streams the entries in chunks through two device buffers, the copy of
+ * chunk k+1 overlapping the hooking of chunk k (src/cc/scout_cc.cu:86-215). Results are the
+ * same in both modes. Snapshots and on-device compaction are not available in this mode. */
 int scout_graph_load(scout_ctx* ctx, int32_t n, int64_t entries, const void* nindex, int wide,
                      const int32_t* nlist);
-/* out[0..5] = n, base entries, inserted-row entries, logical entries, rows sorted (0/1). */
+/* out[0..7] = n, base entries, inserted-row entries, logical entries, rows sorted (0/1),
+ * base entries host-resident (0/1), bytes streamed from host memory by full recomputes, and
+ * the copy-stream time of those transfers in microseconds. */
 int scout_graph_info(scout_ctx* ctx, int64_t* out);
 /* The logical graph as a CSR in scratchpad-compaction row order (base rows minus deleted
  * entries, then inserted entries in insertion order): core/flash_compactor.h:23-95.

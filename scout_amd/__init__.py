@@ -234,7 +234,8 @@ class Engine:
         out = np.zeros(8, dtype=np.int64)
         self._check(self.lib.scout_graph_info(self.h, _ptr(out)), "graph_info")
         return {"n": int(out[0]), "base_entries": int(out[1]), "inserted_entries": int(out[2]),
-                "logical_entries": int(out[3]), "sorted": bool(out[4])}
+                "logical_entries": int(out[3]), "sorted": bool(out[4]), "host_resident": bool(out[5]),
+                "fetch_bytes": int(out[6]), "fetch_us": int(out[7])}
 
     def download_graph(self) -> Graph:
         off = np.zeros(self.n + 1, dtype=np.int64)

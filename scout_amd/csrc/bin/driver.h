@@ -19,7 +19,8 @@
 //   SCOUT_NO_MEMO, SCOUT_FORCE_COPY: accepted with no effect (rows are resident in HBM, so there
 //     is no fetch memo, and every transfer to the discrete MI355X is an explicit copy); a note
 //     goes to stderr.
-// Added: SCOUT_MIS_COMPLETE=1 (mis) completes the set after the reference's rules, keeping it
+// Added: SCOUT_HBM_BUDGET=<bytes> caps the context's HBM (a graph whose base entries do not fit
+// stays in pinned host memory); SCOUT_MIS_COMPLETE=1 (mis) completes the set after the reference's rules, keeping it
 // maximal (scout_mis_mode); SCOUT_DUMP_STATE=<dir> writes the solution after every batch as raw little-endian
 // arrays (<dir>/<problem>_<batch>.bin: uint8 membership / int32 otherwise);
 // SCOUT_STRATEGY=local|recompute|learned (bfs, cc).
@@ -95,8 +96,12 @@ inline int run_driver(int argc, char** argv, Problem prob) {
       std::cerr << v << ": no effect on this engine (rows are resident in HBM and every transfer is an explicit copy)"
                 << std::endl;
 
+  // SCOUT_HBM_BUDGET=<bytes>: a graph whose base entries do not fit stays storage-resident
+  // (pinned host memory read over PCIe; full recomputes stream it in chunks)
+  scout_budget budget{0, 0};
+  if (const char* b = std::getenv("SCOUT_HBM_BUDGET")) budget.hbm_bytes = std::max(0LL, atoll(b));
   scout_ctx* ctx = nullptr;
-  DRV_CHECK(scout_ctx_create(args.gpu_id, nullptr, &ctx));
+  DRV_CHECK(scout_ctx_create(args.gpu_id, &budget, &ctx));
   char gpu_name[256] = {0};
   DRV_CHECK(scout_device_name(ctx, gpu_name, sizeof(gpu_name)));
   // device memory in use before this program's own allocations (src/mis/scout_mis.cu:51-72,789)

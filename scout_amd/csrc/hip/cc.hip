@@ -109,6 +109,30 @@ __global__ void k_cc_hook_tasks(GView g, int32_t* parent, const int64_t* tasks, 
   if (lane_id() == 0 && work) atomicAdd((unsigned long long*)scanned, (unsigned long long)work);
 }
 
+// k_cc_hook over the vertex range [v0, v1) only (one streamed chunk of the base entries).
+__global__ void k_cc_hook_range(GView g, int32_t v0, int32_t v1, int32_t* parent, int64_t* tasks, int64_t* task_count,
+                                int64_t task_cap, int64_t* scanned) {
+  int64_t work = 0;
+  for (int64_t v = v0 + global_wave(); v < v1; v += total_waves()) {
+    Row r = row_of(g, (int32_t)v);
+    int64_t len = row_len(r);
+    if (len > kChunk) {
+      if (lane_id() == 0) {
+        int64_t chunks = (len + kChunk - 1) / kChunk;
+        int64_t pos = (int64_t)atomicAdd((unsigned long long*)task_count, (unsigned long long)chunks);
+        for (int64_t k = 0; k < chunks && pos + k < task_cap; k++) tasks[pos + k] = (v << 32) | k;
+      }
+      continue;
+    }
+    work += len;
+    wave_walk(g, r, 0, len, [&](int32_t w) {
+      int32_t h;
+      if (w > v) uf_union(parent, (int32_t)v, w, &h);
+    });
+  }
+  if (lane_id() == 0 && work) atomicAdd((unsigned long long*)scanned, (unsigned long long)work);
+}
+
 // Zeroes `count` counters with a kernel (no memset node, so a captured stream replays it).
 __global__ void k_cc_zero_counters(int64_t* p, int count) {
   if ((int)threadIdx.x < count) p[threadIdx.x] = 0;
@@ -292,9 +316,86 @@ int cc_full_into(Ctx& c, int32_t* label, int64_t* tasks, int64_t task_cap, int64
   return SCOUT_OK;
 }
 
+// The full recompute of a storage-resident graph: the base entries are copied from pinned host
+// memory in chunks of whole rows through two device buffers on the copy stream, the copy of
+// chunk k+1 overlapping the hooking of chunk k on the compute stream (the chunked, two-stream
+// upload of full_recompute_cc_gpu, src/cc/scout_cc.cu:157-215). The inserted rows are in HBM.
+int cc_full_streamed(Ctx& c, int64_t* scanned) {
+  DevGraph& g = c.g;
+  int32_t n = g.n;
+  const std::vector<int64_t>& off = g.hoff;
+  // chunks of whole rows, at most kStreamEntries entries unless one row is longer
+  constexpr int64_t kStreamEntries = int64_t(1) << 25;  // 128 MiB of entries per buffer
+  std::vector<std::pair<int32_t, int32_t>> chunks;
+  int64_t longest = 0;
+  for (int32_t v0 = 0; v0 < n;) {
+    int32_t v1 = v0 + 1;
+    while (v1 < n && off[(size_t)v1 + 1] - off[(size_t)v0] <= kStreamEntries) v1++;
+    chunks.emplace_back(v0, v1);
+    longest = std::max(longest, off[(size_t)v1] - off[(size_t)v0]);
+    v0 = v1;
+  }
+  int64_t cap_entries = std::max<int64_t>(longest, 1);
+  int64_t task_cap = 2 * cap_entries / kChunk + 1024;
+  DevBuf<int32_t> buf[2];
+  DevBuf<int64_t> tasks;
+  SCOUT_CHECK(buf[0].reserve(cap_entries));
+  SCOUT_CHECK(buf[1].reserve(cap_entries));
+  SCOUT_CHECK(tasks.reserve(task_cap));
+  size_t nc = chunks.size();
+  std::vector<hipEvent_t> ev(4 * nc + 4, nullptr);
+  for (auto& e : ev) SCOUT_TRY(hipEventCreate(&e));
+  // ev[4k] / ev[4k+1]: copy start / end of chunk k; ev[4k+2]: chunk k hooked
+  k_iota<<<grid_for(n), kBlock, 0, c.s>>>(c.label, n);
+  k_cc_zero_counters<<<1, 64, 0, c.s>>>(c.d_box, 2);
+  SCOUT_TRY(hipEventRecord(ev[4 * nc], c.s));
+  SCOUT_TRY(hipStreamWaitEvent(c.cs, ev[4 * nc], 0));  // the labels and counters are set first
+  int64_t bytes = 0;
+  for (size_t k = 0; k < nc; k++) {
+    int32_t v0 = chunks[k].first, v1 = chunks[k].second;
+    int b = (int)(k & 1);
+    int64_t e0 = off[(size_t)v0], len = off[(size_t)v1] - e0;
+    if (k >= 2) SCOUT_TRY(hipStreamWaitEvent(c.cs, ev[4 * (k - 2) + 2], 0));  // buffer b is free again
+    SCOUT_TRY(hipEventRecord(ev[4 * k], c.cs));
+    if (len > 0)
+      SCOUT_TRY(hipMemcpyAsync(buf[b].p, g.hbcol + e0, (size_t)len * 4, hipMemcpyHostToDevice, c.cs));
+    SCOUT_TRY(hipEventRecord(ev[4 * k + 1], c.cs));
+    bytes += len * 4;
+    SCOUT_TRY(hipStreamWaitEvent(c.s, ev[4 * k + 1], 0));
+    // the chunk's rows read their base entries from the buffer: entry e of the graph sits at
+    // buf[e - e0]
+    GView gv = g.view();
+    gv.bcol = reinterpret_cast<const int32_t*>(reinterpret_cast<uintptr_t>(buf[b].p) - (uintptr_t)e0 * 4);
+    gv.bsamp = nullptr;
+    k_cc_zero_counters<<<1, 64, 0, c.s>>>(c.d_box, 1);
+    k_cc_hook_range<<<kListGrid, kBlock, 0, c.s>>>(gv, v0, v1, c.label, tasks.p, c.d_box, task_cap, c.d_box + 1);
+    k_cc_hook_tasks<<<kListGrid, kBlock, 0, c.s>>>(gv, c.label, tasks.p, c.d_box, task_cap, c.d_box + 1, c.d_paths);
+    SCOUT_TRY(hipEventRecord(ev[4 * k + 2], c.s));
+  }
+  k_cc_flatten<<<grid_for(n), kBlock, 0, c.s>>>(c.label, n);
+  SCOUT_CHECK(box_read(c, 2));
+  double ms = 0;
+  for (size_t k = 0; k < nc; k++) {
+    float t = 0.f;
+    SCOUT_TRY(hipEventElapsedTime(&t, ev[4 * k], ev[4 * k + 1]));
+    ms += t;
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  c.fetch_ms += ms;
+  c.fetch_bytes += bytes;
+  if (scanned) *scanned += c.h_box[1];
+  return SCOUT_OK;
+}
+
 int cc_full(Ctx& c, int64_t* scanned) {
   int32_t n = c.g.n;
   if (n == 0) return SCOUT_OK;
+  if (c.g.host_bcol) {
+    SCOUT_CHECK(cc_full_streamed(c, scanned));
+    c.label_seq = c.apply_seq;
+    c.cc_components = -1;
+    return SCOUT_OK;
+  }
   SCOUT_CHECK(c.wb.reserve(cc_full_task_bound(c)));
   SCOUT_CHECK(prof_begin(c, PROF_CC_HOOK));
   SCOUT_CHECK(cc_full_into(c, c.label, c.wb.p, c.wb.cap, c.d_box));

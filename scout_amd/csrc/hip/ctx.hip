@@ -330,15 +330,25 @@ int scout_graph_load(scout_ctx* x, int32_t n, int64_t entries, const void* ninde
     set_error("graph_load: null array");
     return SCOUT_ERR_ARG;
   }
-  int64_t need = ((int64_t)n + 1) * 16 + entries * 4 + (int64_t)n * 12;
-  if (c.budget.hbm_bytes > 0 && need > c.budget.hbm_bytes) {
-    set_error("graph_load: graph exceeds the HBM budget");
+  // offsets, inserted-row heads and the n-sized scratch stay in HBM; the base entries too when
+  // the budget holds them, else they stay in pinned host memory (storage-resident mode)
+  int64_t need_small = ((int64_t)n + 1) * 16 + (int64_t)n * 12;
+  int64_t need = need_small + entries * 4;
+  bool host_resident = c.budget.hbm_bytes > 0 && need > c.budget.hbm_bytes;
+  if (host_resident && need_small > c.budget.hbm_bytes) {
+    set_error("graph_load: the offsets and per-vertex state exceed the HBM budget even with the base entries "
+              "kept on the host");
+    return SCOUT_ERR_BUDGET;
+  }
+  if (host_resident && c.budget.pinned_bytes > 0 && entries * 4 > c.budget.pinned_bytes) {
+    set_error("graph_load: the base entries exceed both the HBM and the pinned-memory budget");
     return SCOUT_ERR_BUDGET;
   }
   free_state(c);
   c.hbm_used = 0;
-  SCOUT_CHECK(graph_upload_egr(c, n, entries, nindex, wide, nlist));
-  c.hbm_used = need;
+  SCOUT_CHECK(graph_upload_egr(c, n, entries, nindex, wide, nlist, host_resident));
+  // the n-sized scratch below counts itself as it is allocated
+  c.hbm_used = ((int64_t)n + 1) * 16 + (host_resident ? 0 : entries * 4);
   SCOUT_CHECK(alloc_n(c, (void**)&c.stamp, 4));
   SCOUT_CHECK(alloc_n(c, (void**)&c.aux, 4));
   SCOUT_CHECK(alloc_n(c, (void**)&c.aux2, 4));
@@ -368,6 +378,9 @@ int scout_graph_info(scout_ctx* x, int64_t* out) {
   sum.release();
   out[3] = total;
   out[4] = c.g.sorted ? 1 : 0;
+  out[5] = c.g.host_bcol ? 1 : 0;
+  out[6] = c.fetch_bytes;
+  out[7] = (int64_t)(c.fetch_ms * 1e3);  // microseconds
   return SCOUT_OK;
 }
 

@@ -577,9 +577,17 @@ int apply_range(Ctx& c, Channel& ch, int64_t lo, int64_t hi, int conditional) {
 int graph_free(Ctx& c) {
   DevGraph& g = c.g;
   if (g.boff) (void)hipFree(g.boff);
-  if (g.bcol) (void)hipFree(g.bcol);
+  if (g.host_bcol) {
+    if (g.hbcol) (void)hipHostFree(g.hbcol);
+  } else if (g.bcol) {
+    (void)hipFree(g.bcol);
+  }
   g.boff = nullptr;
   g.bcol = nullptr;
+  g.hbcol = nullptr;
+  g.host_bcol = false;
+  g.hoff.clear();
+  g.hoff.shrink_to_fit();
   g.aoff.release();
   g.aoff2.release();
   g.alen.release();
@@ -639,7 +647,8 @@ int check_rows(Ctx& c) {
   return SCOUT_OK;
 }
 
-int graph_upload_egr(Ctx& c, int32_t n, int64_t E, const void* nindex, int wide, const int32_t* nlist) {
+int graph_upload_egr(Ctx& c, int32_t n, int64_t E, const void* nindex, int wide, const int32_t* nlist,
+                     bool host_resident) {
   if (n < 0 || E < 0) {
     set_error("graph_load: negative size");
     return SCOUT_ERR_ARG;
@@ -650,7 +659,29 @@ int graph_upload_egr(Ctx& c, int32_t n, int64_t E, const void* nindex, int wide,
   g.E = E;
   g.base_gen++;
   SCOUT_TRY(hipMalloc((void**)&g.boff, ((size_t)n + 1) * sizeof(int64_t)));
-  SCOUT_TRY(hipMalloc((void**)&g.bcol, (size_t)std::max<int64_t>(E, 1) * sizeof(int32_t)));
+  if (host_resident) {
+    // the base entries stay in pinned host memory, filled by OpenMP threads from the (mmap'd)
+    // file; kernels read it over PCIe through its mapped device address
+    size_t bytes = (size_t)std::max<int64_t>(E, 1) * sizeof(int32_t);
+    SCOUT_TRY(hipHostMalloc((void**)&g.hbcol, bytes, hipHostMallocMapped | hipHostMallocNonCoherent));
+    g.host_bcol = true;
+    const char* src = reinterpret_cast<const char*>(nlist);
+    char* dst = reinterpret_cast<char*>(g.hbcol);
+    size_t total = (size_t)E * sizeof(int32_t), per = (size_t)64 << 20;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t k = 0; k < (int64_t)((total + per - 1) / per); k++) {
+      size_t a = (size_t)k * per;
+      memcpy(dst + a, src + a, std::min(per, total - a));
+    }
+    void* dp = nullptr;
+    SCOUT_TRY(hipHostGetDevicePointer(&dp, g.hbcol, 0));
+    g.bcol = static_cast<int32_t*>(dp);
+    g.hoff.resize((size_t)n + 1);
+    for (size_t v = 0; v <= (size_t)n; v++)
+      g.hoff[v] = wide ? (int64_t)((const uint64_t*)nindex)[v] : (int64_t)((const int32_t*)nindex)[v];
+  } else {
+    SCOUT_TRY(hipMalloc((void**)&g.bcol, (size_t)std::max<int64_t>(E, 1) * sizeof(int32_t)));
+  }
   if (wide) {
     SCOUT_CHECK(upload_chunked(c, g.boff, nindex, ((size_t)n + 1) * sizeof(int64_t)));
   } else {
@@ -664,7 +695,7 @@ int graph_upload_egr(Ctx& c, int32_t n, int64_t E, const void* nindex, int wide,
     (void)hipFree(tmp32);
     SCOUT_CHECK(rc);
   }
-  SCOUT_CHECK(upload_chunked(c, g.bcol, nlist, (size_t)E * sizeof(int32_t)));
+  if (!host_resident) SCOUT_CHECK(upload_chunked(c, g.bcol, nlist, (size_t)E * sizeof(int32_t)));
   int64_t first = 0, last = 0;
   SCOUT_TRY(hipMemcpy(&first, g.boff, sizeof(int64_t), hipMemcpyDeviceToHost));
   SCOUT_TRY(hipMemcpy(&last, g.boff + n, sizeof(int64_t), hipMemcpyDeviceToHost));
@@ -805,6 +836,11 @@ int graph_download(Ctx& c, int64_t* h_off, int32_t* h_col) {
 
 int graph_compact(Ctx& c) {
   DevGraph& g = c.g;
+  if (g.host_bcol) {
+    set_error("graph_compact: the base entries are host-resident; download the logical graph and write it on the "
+              "host (scout_graph_download + the .egr writer)");
+    return SCOUT_ERR_UNSUPPORTED;
+  }
   DevBuf<int64_t> off2;
   DevBuf<int32_t> col2;
   int64_t total = 0;

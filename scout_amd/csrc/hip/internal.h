@@ -124,7 +124,12 @@ struct DevGraph {
   bool sorted = false;  // every base row ascending (by decoded id)
   int64_t base_gen = 0; // bumped whenever the base arrays are replaced or restored
   int64_t* boff = nullptr;
-  int32_t* bcol = nullptr;
+  int32_t* bcol = nullptr;      // device address of the base entries (HBM, or mapped host memory)
+  // storage-resident mode: the base entries stay in pinned host memory that kernels read over
+  // PCIe (hbcol is its host address, hoff a host copy of the offsets for streamed passes)
+  bool host_bcol = false;
+  int32_t* hbcol = nullptr;
+  std::vector<int64_t> hoff;
   DevBuf<int64_t> aoff, aoff2;
   DevBuf<int32_t> alen, acap, acap2;
   DevBuf<int32_t> acol, acol2, aown, aown2;
@@ -216,7 +221,9 @@ struct Ctx {
   int32_t* cc_level = nullptr;   // BFS levels from vertex 0 kept for CC when BFS is not maintained
   int32_t* cc_parent = nullptr;  // parents for cc_level
   int64_t apply_seq = 0;         // batches applied to the graph
-  int64_t bfs_bound = 0;         // scout_bfs_bound: invalidated vertices per deletion candidate past which a
+  int64_t bfs_bound = 0;
+  double fetch_ms = 0;           // copy-stream time of streamed base-entry chunks (storage-resident mode)
+  int64_t fetch_bytes = 0;         // scout_bfs_bound: invalidated vertices per deletion candidate past which a
                                  // batch's BFS repair is finished by a full recompute (0 = never)
   int64_t level_seq = -1;        // apply_seq at which `level` was last made exact
   int64_t cc_level_seq = -1;     // same for cc_level
@@ -272,7 +279,8 @@ int reduce_sum_i64(Ctx& c, const int64_t* in, int64_t count, int64_t* d_out);
 
 // graph.hip
 int graph_upload(Ctx& c, int32_t n, int64_t E, const int64_t* off, const int32_t* col);
-int graph_upload_egr(Ctx& c, int32_t n, int64_t E, const void* nindex, int wide, const int32_t* nlist);
+int graph_upload_egr(Ctx& c, int32_t n, int64_t E, const void* nindex, int wide, const int32_t* nlist,
+                     bool host_resident = false);
 int graph_classify(Ctx& c, Channel& ch, int rule);
 int graph_apply(Ctx& c, Channel& ch, int conditional);
 int graph_classify_lines(Ctx& c, Channel& ch, int64_t lo, int64_t hi, int rule);

@@ -467,6 +467,10 @@ struct Snapshot {
 };
 
 int snapshot_save(Ctx& c) {
+  if (c.g.host_bcol) {
+    set_error("snapshot: not available for a host-resident graph");
+    return SCOUT_ERR_UNSUPPORTED;
+  }
   if (!c.snap) c.snap = new Snapshot();
   Snapshot& s = *c.snap;
   int64_t n = c.g.n;
@@ -519,6 +523,10 @@ int snapshot_restore(Ctx& c) {
   }
   Snapshot& s = *c.snap;
   int64_t n = c.g.n;
+  if (c.g.host_bcol) {
+    set_error("restore: not available for a host-resident graph");
+    return SCOUT_ERR_UNSUPPORTED;
+  }
   if (s.E != c.g.E) {
     // the base was compacted since the snapshot: restore its size
     (void)hipFree(c.g.bcol);
