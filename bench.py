@@ -100,8 +100,8 @@ def cpu_baseline(eng, n, batches_host, seconds):
 FAMILY_KERNELS = {
     "bfs_invalidate": ["k_invalidate_short", "k_invalidate", "k_inval_push"],
     "bfs_relax": ["k_relax_short", "k_relax_rows"],
-    "cc_expand": ["k_zero_u32", "k_cc_mark", "k_cc_collect", "k_cc_union_list", "k_cc_unreached_hook",
-                  "k_cc_hook_tasks", "k_cc_final_list"],
+    "cc_expand": ["k_cc_zero", "k_cc_seed", "k_cc_first_ranges", "k_cc_explore", "k_zero_u32", "k_cc_mark",
+                  "k_cc_collect", "k_cc_union_list", "k_cc_unreached_hook", "k_cc_hook_tasks", "k_cc_final_list"],
     "apply": ["k_apply", "k_group_heads", "k_append_groups", "k_move_rows"],
     "classify": ["k_dup_count", "k_classify"],
 }
@@ -112,7 +112,7 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_rmat26.json")
 ROOFLINE_KERNELS = {"k_invalidate": "bfs_invalidate_kernel", "k_apply": "apply", "k_classify": "classify",
                     "k_relax_short": "k_relax_short", "k_relax_rows": "k_relax_rows",
                     "k_invalidate_short": "k_invalidate_short", "k_inval_push": "k_inval_push",
-                    "k_cc_collect": "k_cc_collect"}
+                    "k_cc_explore": "k_cc_explore", "k_cc_collect": "k_cc_collect"}
 
 
 def _kernel_short_name(name):
@@ -387,6 +387,64 @@ def run_color_mis(args):
     print(json.dumps(out), flush=True)
 
 
+def run_cc_crossover(args):
+    """CC repair cost per batch in walk and scan mode over a range of batch sizes on the same
+    graph and batches (each size: the same stored batches replayed from the snapshot in both
+    modes, final labels compared). The crossover sets SCOUT_CC_AUTO's batch-size rule."""
+    scale = args.scale or 26
+    sizes = [int(x) for x in args.cc_crossover.split(",")]
+    eng = sa.Engine(0)
+    eng.gen_rmat(scale, 16, 0.57, 0.19, 0.19, args.seed)
+    n = eng.graph_info()["n"]
+    eng.bfs_full(0)
+    eng.cc_full()
+    eng.sync()
+    eng.snapshot()
+    k = args.steps
+    rows = []
+    for size in sizes:
+        eng.restore()
+        for b in range(args.warmup + k):
+            eng.gen_batch(size, 0.7, 0.5, args.seed + size, expected_total=size * (args.warmup + k))
+            eng.batch_store(b)
+            eng.classify(download=False)
+            eng.apply()
+            eng.repair_bfs(sa.STRATEGY_LOCAL)
+            eng.repair_cc(sa.STRATEGY_LOCAL)
+        row = {"batch": size, "endpoints_per_vertex": round(2 * size / n, 5)}
+        labels = []
+        for name, mode in (("walk", sa.CC_WALK), ("scan", sa.CC_SCAN)):
+            eng.restore()
+            eng.cc_mode(mode)
+            rounds = scanned = 0
+            for b in range(args.warmup + k):
+                eng.batch_load(b)
+                eng.classify(download=False)
+                eng.apply()
+                eng.repair_bfs(sa.STRATEGY_LOCAL)
+                if b == args.warmup:
+                    eng.sync()
+                    eng.prof_enable(True)
+                st = eng.repair_cc(sa.STRATEGY_LOCAL)
+                if b >= args.warmup:
+                    rounds += st.rounds
+                    scanned += st.edges_scanned
+            fam = eng.prof_read(sa.PROF_CC_EXPAND)
+            eng.prof_enable(False)
+            row[f"{name}_ms"] = round(fam["ms"] / k, 4)
+            row[f"{name}_rounds"] = round(rounds / k, 1)
+            row[f"{name}_entries"] = round(scanned / k)
+            labels.append(eng.download(sa.STATE_LABEL))
+        row["equal"] = bool(np.array_equal(labels[0], labels[1]))
+        row["auto"] = "walk" if 2 * size * 256 <= n else "scan"
+        log(f"cc crossover: {row}")
+        rows.append(row)
+    eng.cc_mode(sa.CC_AUTO)
+    eng.close()
+    print(json.dumps({"cc_crossover": rows, "scale": scale, "vertices": n, "steps": k, "warmup": args.warmup}),
+          flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -413,6 +471,9 @@ def main():
                     help="only summarise rocprofv3 --pmc passes under DIR (copy_fetch, copy_write, bench_fetch, "
                          "bench_write) into per-batch HBM traffic per kernel family, print it and exit; the "
                          "per-dispatch figures cover the last --steps batches (the headline replay)")
+    ap.add_argument("--cc-crossover", metavar="SIZES", default=None,
+                    help="only measure the CC repair per batch in walk and scan mode for these comma-separated "
+                         "batch sizes and exit")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 5 plain device copies of MIB MiB each (reference kernel for the "
                          "FETCH_SIZE/WRITE_SIZE counter scale) and exit")
@@ -425,6 +486,9 @@ def main():
         return
     if args.workload == "color-mis":
         run_color_mis(args)
+        return
+    if args.cc_crossover:
+        run_cc_crossover(args)
         return
     args.scale = args.scale or 26
     args.batch = args.batch or 1_000_000

@@ -169,6 +169,14 @@ int scout_bfs_schedule(scout_ctx* ctx, int inval_rounds, int relax_rounds);
 /* CC min-id labels: replaces processBatch + bidirectional_reachability_check
  * (src/cc/scout_cc.cu:300-441). */
 int scout_repair_cc(scout_ctx* ctx, int channel, int strategy, scout_stats* st);
+/* How the local CC repair finds the vertices a batch can relabel. SCOUT_CC_WALK walks from the
+ * batch's endpoints over the affected components (work proportional to them, one device round
+ * per level); SCOUT_CC_SCAN makes one pass over every vertex (fixed cost, no rounds);
+ * SCOUT_CC_AUTO (the default) walks when the batch has at most n/256 endpoints and scans
+ * otherwise. Labels are the same in every mode. No counterpart in the reference, whose deletion
+ * check is a bounded host BFS per line (src/cc/scout_cc.cu:380-441). */
+enum { SCOUT_CC_AUTO = 0, SCOUT_CC_WALK = 1, SCOUT_CC_SCAN = 2 };
+int scout_cc_mode(scout_ctx* ctx, int mode);
 /* Colouring: replaces processBatch + gpu_color_round (src/coloring/scout_coloring.cu:172-277). */
 int scout_repair_color(scout_ctx* ctx, int channel, scout_stats* st);
 /* MIS: replaces launchBatch + syncAndCollect (src/mis/scout_mis.cu:500-720); max_csr is the
@@ -292,7 +300,8 @@ enum {
   SCOUT_PROF_K_RELAX_ROWS = 10,
   SCOUT_PROF_K_INVALIDATE_SHORT = 11,
   SCOUT_PROF_K_INVAL_PUSH = 12,
-  SCOUT_PROF_K_CC_COLLECT = 13
+  SCOUT_PROF_K_CC_EXPLORE = 13, /* CC walk mode: the walks from the batch endpoints */
+  SCOUT_PROF_K_CC_COLLECT = 14  /* CC scan mode: the pass over every vertex */
 };
 int scout_prof_enable(scout_ctx* ctx, int on);
 int scout_prof_read(scout_ctx* ctx, int id, double* total_ms, int64_t* launches, int64_t* bytes);
@@ -308,7 +317,8 @@ enum {
   SCOUT_PATH_CC_TASKS = 3,           /* CC union-find: chunk tasks of long rows */
   SCOUT_PATH_RELAX_TASKS = 4,        /* BFS relax: chunk tasks of long rows */
   SCOUT_PATH_TENTATIVE_TASKS = 5,    /* BFS tentative levels: chunk tasks of long rows */
-  SCOUT_PATH_COUNT = 6
+  SCOUT_PATH_CC_WALK_ROWS = 6,       /* CC walk mode: rows walked by a whole wave */
+  SCOUT_PATH_COUNT = 7
 };
 int scout_path_counts(scout_ctx* ctx, int64_t* out, int count);
 

@@ -24,6 +24,7 @@ STATE_MIS, STATE_COLOR, STATE_LEVEL, STATE_LABEL = 0, 1, 2, 3
 CLASSIFY_CURRENT, CLASSIFY_BASE = 0, 1
 STRATEGY_LOCAL, STRATEGY_RECOMPUTE, STRATEGY_LEARNED = 0, 1, 2
 MIS_RULES, MIS_COMPLETE = 0, 1
+CC_AUTO, CC_WALK, CC_SCAN = 0, 1, 2
 
 
 class ScoutError(RuntimeError):
@@ -70,6 +71,7 @@ _SIGNATURES = {
     "scout_repair_mis_multi": [_P, ctypes.c_int, _I32, _P, _I64, ctypes.POINTER(Stats)],
     "scout_mis_config": [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int],
     "scout_mis_mode": [_P, ctypes.c_int],
+    "scout_cc_mode": [_P, ctypes.c_int],
     "scout_mis_limits": [_P, _I64, _I64],
     "scout_color_config": [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int],
     "scout_color_get_config": [_P, _P],
@@ -120,9 +122,9 @@ PROF_BFS_RELAX, PROF_BFS_INVALIDATE, PROF_CC_EXPAND, PROF_APPLY, PROF_CLASSIFY =
 PROF_COLOR, PROF_MIS_EVAL, PROF_CC_HOOK, PROF_BFS_INVALIDATE_KERNEL = 5, 6, 7, 8
 PROF_NAMES = ["bfs_relax", "bfs_invalidate", "cc_expand", "apply", "classify", "color", "mis_eval", "cc_hook",
               "bfs_invalidate_kernel", "k_relax_short", "k_relax_rows", "k_invalidate_short", "k_inval_push",
-              "k_cc_collect"]
+              "k_cc_explore", "k_cc_collect"]
 PATH_NAMES = ["inval_tasks", "inval_push_tasks", "inval_kids_overflow", "cc_tasks", "relax_tasks",
-              "tentative_tasks"]
+              "tentative_tasks", "cc_walk_rows"]
 
 _lib = None
 
@@ -316,6 +318,11 @@ class Engine:
         """MIS_RULES (the reference's rules) or MIS_COMPLETE (rules, then greedy completion over
         the batch's 2-hop region, which keeps the set maximal)."""
         self._check(self.lib.scout_mis_mode(self.h, mode), "mis_mode")
+
+    def cc_mode(self, mode: int):
+        """CC_AUTO (by batch size), CC_WALK (walk from the batch endpoints) or CC_SCAN (one pass
+        over every vertex): how the local CC repair finds the vertices to relabel."""
+        self._check(self.lib.scout_cc_mode(self.h, mode), "cc_mode")
 
     def mis_config(self, grid: int = 1024, group_rounds: int = 4, use_graph: bool = True):
         self._check(self.lib.scout_mis_config(self.h, grid, group_rounds, 1 if use_graph else 0), "mis_config")

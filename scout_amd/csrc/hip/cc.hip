@@ -12,12 +12,13 @@
 //      splits a piece off this component shows up as that piece becoming unreached;
 //   2. every other component lies wholly in the unreached subgraph (a neighbour of an
 //      unreached vertex is unreached). Only the components the batch can have changed are
-//      relabelled: those holding an unreached endpoint of a batch line (edges inserted or
-//      deleted inside or between them), plus the vertices that left vertex 0's component.
-//      Every other unreached component kept all its edges and all its vertices, so it is
-//      still a component with the same minimum. The affected vertices are closed under
-//      adjacency, and a lock-free union-find over their rows relabels them, hooking larger
-//      roots under smaller so every root is its set's minimum.
+//      relabelled: they are exactly the components of the unreached endpoints of its lines
+//      (a piece cut off vertex 0's component or an edited component holds such an endpoint;
+//      one without kept all its edges and vertices, so its label stands). A walk from those
+//      endpoints lists them, and a lock-free union-find over their rows relabels them,
+//      hooking larger roots under smaller so every root is its set's minimum. A second walk
+//      from the reached endpoints with a non-zero label gives label 0 to the vertices that
+//      joined vertex 0's component. The work follows the affected components, not n.
 // When the context also maintains BFS from vertex 0 and this batch's BFS repair already ran,
 // its levels are used directly; otherwise the CC state keeps its own level array.
 #include "common.cuh"
@@ -171,6 +172,178 @@ __global__ void k_cc_unreached_hook(GView g, const int32_t* list, const int64_t*
   }
   if (lane_id() == 0 && work) atomicAdd((unsigned long long*)scanned, (unsigned long long)work);
 }
+
+// --- walk mode -----------------------------------------------------------------------
+
+// Batch repair by exploration from the batch's endpoints (work proportional to the affected
+// components, not to n):
+//  * reached side: an endpoint that is reached after the batch but still has a non-zero label
+//    was in a component that the batch joined to vertex 0's; it and every reached vertex with a
+//    non-zero label connected to it take label 0. Every newly reached vertex v is found: on a
+//    path from v to vertex 0 the first vertex with an old label of 0 is entered over an inserted
+//    edge, whose other endpoint is such a seed, and the path up to it has non-zero labels;
+//  * unreached side: every component the batch can have changed among the unreached vertices
+//    (a piece cut off vertex 0's component, a component edited inside, merged or split) holds
+//    an unreached endpoint of a batch line -- a component without one kept every edge, so it
+//    is an unchanged old component. The walk from the unreached endpoints lists exactly those
+//    components (closed under adjacency); a union-find over their rows relabels them.
+// The label write is the visited mark of the reached side; the unreached side uses stamps.
+
+__global__ void k_cc_zero(int64_t* p, int count) {
+  if ((int)threadIdx.x < count) p[threadIdx.x] = 0;
+}
+
+// The first round covers every seed: ranges [0, count) of both lists.
+__global__ void k_cc_first_ranges(int64_t* box) {
+  box[6] = 0;
+  box[7] = box[0];
+  box[9] = 0;
+  box[10] = box[8];
+}
+
+// Seeds of both sides from the batch endpoints. roots_r / roots_u += old labels that were their
+// own vertex (old component roots) among the seeds of each side.
+__global__ void k_cc_seed(const int32_t* uv, int64_t m, const int32_t* L, int32_t* label, int32_t* stamp, int32_t ep,
+                          int32_t* par, int32_t* rlist, int64_t* rcount, int32_t* ulist, int64_t* ucount, int64_t cap,
+                          int64_t* roots_r, int64_t* roots_u) {
+  int64_t rr = 0, ru = 0;
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < 2 * m; base += (int64_t)gridDim.x * blockDim.x) {
+    int64_t i = base + threadIdx.x;
+    bool tr = false, tu = false;
+    int32_t x = 0;
+    if (i < 2 * m) {
+      x = uv[i];
+      if (L[x] >= 0) {
+        if (label[x] != 0) {
+          int32_t old = atomicExch(&label[x], 0);
+          tr = old != 0;
+          rr += tr && old == x;
+        }
+      } else if (atomicExch(&stamp[x], ep) != ep) {
+        tu = true;
+        par[x] = x;
+        ru += label[x] == x;
+      }
+    }
+    block_append(tr, x, rlist, rcount, cap);
+    block_append(tu, x, ulist, ucount, cap);
+  }
+  block_add(rr, roots_r);
+  block_add(ru, roots_u);
+}
+
+// Rows up to this many entries are scanned by the thread that holds the vertex; longer rows by
+// the whole wave.
+constexpr int kShortRow = 16;
+// Blocks of a walk launch: rounds hold tens of thousands of vertices, and every block ends on
+// one counter.
+constexpr unsigned kExploreGrid = 512;
+// SCOUT_CC_AUTO walks while the batch's endpoints number at most n / kWalkRatio and scans
+// beyond. Measured on R-MAT scale 26 (n = 67M, 70% deletions): the two cost the same at about
+// 100K lines per batch (200K endpoints, n / 335); 1K lines walk in 0.10 ms against the scan's
+// 0.16 ms, 1M lines take 0.45 ms walking and 0.31 ms scanning.
+constexpr int64_t kWalkRatio = 256;
+
+// Unreached side: w joins the list once per batch (the stamp is the visited mark), as its own
+// union-find root. ru counts old component roots among the listed vertices.
+__device__ __forceinline__ bool cc_take_unreached(int32_t w, const int32_t* label, int32_t* stamp, int32_t ep,
+                                                  int32_t* par, int64_t& ru) {
+  if (w < 0 || stamp[w] == ep || atomicExch(&stamp[w], ep) == ep) return false;
+  par[w] = w;
+  ru += label[w] == w;
+  return true;
+}
+
+// Reached side: a reached w with a non-zero label takes label 0 and joins the list (the label
+// write is the visited mark). rr counts old component roots among them.
+__device__ __forceinline__ bool cc_take_reached(int32_t w, const int32_t* L, int32_t* label, int64_t& rr) {
+  if (w < 0 || L[w] < 0 || label[w] == 0) return false;
+  int32_t old = atomicExch(&label[w], 0);
+  rr += old != 0 && old == w;
+  return old != 0;
+}
+
+// One level of both walks in one launch. box[6..7] and box[9..10] hold the unreached and
+// reached lists' last-round ranges; a thread per listed vertex scans its row when it has at
+// most kShortRow entries, and the wave walks the longer rows together, one at a time. The
+// last block to finish (box[12] counts finished blocks) publishes the next round's ranges, so
+// a round is a single launch.
+__global__ void k_cc_explore(GView g, const int32_t* L, int32_t* label, int32_t* stamp, int32_t ep, int32_t* par,
+                             int32_t* ulist, int32_t* rlist, int64_t* box, int64_t cap, int64_t* paths) {
+  const int64_t ulo = box[6], uhi = box[7], rlo = box[9], rhi = box[10];
+  const int64_t nu = uhi - ulo, total = nu + (rhi - rlo);
+  int64_t work = 0, ru = 0, rr = 0;
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < total; base += (int64_t)gridDim.x * blockDim.x) {
+    int64_t t = base + threadIdx.x;
+    bool have = t < total;
+    bool unr = t < nu;
+    Row r{0, 0, 0, 0};
+    int64_t len = 0;
+    if (have) {
+      r = row_of(g, unr ? ulist[ulo + t] : rlist[rlo + (t - nu)]);
+      len = row_len(r);
+    }
+    work += len;
+    int steps = len <= kShortRow ? (int)len : 0;
+    int smax = steps;
+    for (int off = kWave / 2; off > 0; off >>= 1) smax = max(smax, __shfl_xor(smax, off));
+    for (int k = 0; k < smax; k++) {
+      int32_t w = k < steps ? row_at(g, r, k) : -1;
+      bool tu = unr && cc_take_unreached(w, label, stamp, ep, par, ru);
+      bool tr = !unr && cc_take_reached(w, L, label, rr);
+      wave_append(tu, w, ulist, box + 0, cap);
+      wave_append(tr, w, rlist, box + 8, cap);
+    }
+    unsigned long long longs = __ballot(len > kShortRow);
+    count_path(__popcll(longs), paths + PATH_CC_WALK_ROWS);
+    while (longs) {
+      int j = __ffsll((long long)longs) - 1;
+      longs &= longs - 1;
+      Row lr{__shfl(r.b0, j), __shfl(r.b1, j), __shfl(r.a0, j), __shfl(r.a1, j)};
+      bool lunr = __shfl((int)unr, j) != 0;
+      int64_t llen = row_len(lr);
+      for (int64_t b0 = 0; b0 < llen; b0 += kWave) {
+        int64_t k = b0 + lane_id();
+        int32_t w = k < llen ? row_at(g, lr, k) : -1;
+        bool tu = lunr && cc_take_unreached(w, label, stamp, ep, par, ru);
+        bool tr = !lunr && cc_take_reached(w, L, label, rr);
+        wave_append(tu, w, ulist, box + 0, cap);
+        wave_append(tr, w, rlist, box + 8, cap);
+      }
+    }
+  }
+  block_add(work, box + 2);
+  block_add(ru, box + 4);
+  block_add(rr, box + 11);
+  __shared__ bool last;
+  if (threadIdx.x == 0) {
+    __threadfence();
+    last = atomicAdd((unsigned long long*)(box + 12), 1ULL) == (unsigned long long)gridDim.x - 1;
+  }
+  __syncthreads();
+  if (last && threadIdx.x == 0) {
+    __threadfence();
+    int64_t uc = __hip_atomic_load(box + 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    int64_t rc = __hip_atomic_load(box + 8, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    box[6] = uhi;
+    box[7] = uc < cap ? uc : cap;
+    box[9] = rhi;
+    box[10] = rc < cap ? rc : cap;
+    box[12] = 0;
+  }
+}
+
+// Roots of the labels (vertices labelled with their own id): the component count.
+__global__ void k_cc_count_roots(const int32_t* label, int32_t n, int64_t* roots) {
+  int64_t cnt = 0;
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n; base += (int64_t)gridDim.x * blockDim.x) {
+    int64_t v = base + threadIdx.x;
+    cnt += v < n && label[v] == v;
+  }
+  block_add(cnt, roots);
+}
+
+// --- scan mode -----------------------------------------------------------------------
 
 // Marks (bitmap over label ids) the old label of every endpoint of a batch line that is
 // unreached after the batch. Label 0 (vertex 0's old component) needs no mark: its unreached
@@ -414,33 +587,87 @@ int cc_full(Ctx& c, int64_t* scanned) {
   return SCOUT_OK;
 }
 
-// Exact labels after the batch: the BFS levels from vertex 0 decide vertex 0's component;
-// the rest is a union-find over the unreached subgraph.
-int cc_local(Ctx& c, Channel& ch, scout_stats* st) {
+// Walk mode: lists the affected vertices by walking from the batch endpoints (work follows the
+// affected components).
+int cc_walk(Ctx& c, Channel& ch, const int32_t* L, scout_stats* st) {
   int32_t n = c.g.n;
-  if (n == 0) return SCOUT_OK;
-  const int32_t* L = nullptr;
-  if (c.level && c.source == 0 && c.level_seq == c.apply_seq) {
-    L = c.level;  // this batch's BFS repair from vertex 0 already ran on this context
-  } else {
-    SCOUT_CHECK(ensure_scratch(c, &c.cc_level, -1));
-    SCOUT_CHECK(ensure_scratch(c, &c.cc_parent, -1));
-    scout_stats bst{};
-    if (c.cc_level_seq >= 0 && c.cc_level_seq == c.apply_seq - 1) {
-      SCOUT_CHECK(bfs_local_on(c, ch, c.cc_level, c.cc_parent, SCOUT_STRATEGY_LOCAL, &bst));
-    } else {
-      SCOUT_CHECK(bfs_full_on(c, c.cc_level, c.cc_parent, 0, &bst.rounds, &bst.edges_scanned));
+  SCOUT_CHECK(ensure_scratch(c, &c.cc_owner, -1));
+  int64_t cap = (int64_t)n + 1;
+  SCOUT_CHECK(c.la.reserve(cap));
+  SCOUT_CHECK(c.lb.reserve(cap));
+  SCOUT_CHECK(c.lc.reserve(cap));
+  SCOUT_CHECK(c.wb.reserve(2 * (c.g.E + c.g.A) / kChunk + 1024));
+  int32_t* par = c.cc_owner;
+  int64_t* box = c.d_box;
+  // box: [0] unreached list, [1] chunk tasks, [2] scanned entries, [3] new roots among the
+  // unreached list, [4] old roots among it, [5] long rows, [6..7] its round range,
+  // [8] reached list, [9..10] its round range, [11] old roots among it, [12] finished blocks
+  k_cc_zero<<<1, 64, 0, c.s>>>(box, 13);
+  int32_t ep = next_epoch(c);
+  SCOUT_CHECK(prof_begin(c, PROF_CC_EXPAND));
+  if (ch.m > 0)
+    k_cc_seed<<<grid_for(2 * ch.m, kBlock * 4), kBlock, 0, c.s>>>(ch.d_uv.p, ch.m, L, c.label, c.stamp, ep, par,
+                                                                  c.lc.p, box + 8, c.la.p, box, cap, box + 11, box + 4);
+  k_cc_first_ranges<<<1, 1, 0, c.s>>>(box);
+  // both walks advance one level per launch, queued in groups between read-backs; the first
+  // group is the previous batch's round count
+  int group = std::max(2, std::min(c.hint_cc, 32));
+  int64_t rounds = 0;
+  while (true) {
+    for (int k = 0; k < group; k++) {
+      SCOUT_CHECK(prof_begin(c, PROF_K_CC_EXPLORE));
+      k_cc_explore<<<kExploreGrid, kBlock, 0, c.s>>>(c.g.view(), L, c.label, c.stamp, ep, par, c.la.p, c.lc.p, box,
+                                                      cap, c.d_paths);
+      SCOUT_CHECK(prof_end(c, PROF_K_CC_EXPLORE, 0));
     }
-    c.cc_level_seq = c.apply_seq;
-    st->rounds += bst.rounds;
-    st->edges_scanned += bst.edges_scanned;
-    st->seeds += bst.seeds;
-    st->touched += bst.touched;
-    st->insertions += bst.insertions;
-    st->deletions += bst.deletions;
-    st->fallback += bst.fallback;
-    L = c.cc_level;
+    rounds += group;
+    SCOUT_CHECK(box_read(c, 13));
+    if (c.h_box[6] == c.h_box[7] && c.h_box[9] == c.h_box[10]) break;
+    group = std::min(2 * group, 16);
   }
+  c.hint_cc = (int)rounds;
+  // union-find over the unreached components' rows, then their roots as labels
+  k_cc_union_list<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), c.la.p, box, par, c.lb.p, box + 5, cap, box + 2);
+  k_cc_unreached_hook<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), c.lb.p, box + 5, par, c.wb.p, box + 1, c.wb.cap,
+                                                      box + 2);
+  k_cc_hook_tasks<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), par, c.wb.p, box + 1, c.wb.cap, box + 2, c.d_paths);
+  k_cc_final_list<<<kListGrid, kBlock, 0, c.s>>>(c.la.p, box, par, c.label, box + 3);
+  SCOUT_CHECK(prof_end(c, PROF_CC_EXPAND, 0));
+  SCOUT_CHECK(box_read(c, 12));
+  if (c.h_box[1] > c.wb.cap) {
+    set_error("cc: chunk task overflow");
+    return SCOUT_ERR_STATE;
+  }
+  int64_t nu = c.h_box[0], nr = c.h_box[8], scanned_e = c.h_box[2];
+  // per listed vertex of either side its entry, row offsets and label or parent (32 B); per
+  // scanned entry the neighbour and its level, label, stamp or parents (12 B); per batch line
+  // endpoint its level, label and stamp (24 B per line)
+  prof_bytes(c, PROF_CC_EXPAND, 32 * (nu + nr) + 12 * scanned_e + 24 * ch.m);
+  // the walks alone: per listed vertex its entry and row offsets (32 B) and its take (stamp or
+  // label, parent and list slot: 12 B); per row entry the neighbour and its stamp or level (8 B)
+  prof_bytes(c, PROF_K_CC_EXPLORE, 44 * (nu + nr) + 8 * scanned_e);
+  // components: old roots that joined vertex 0's or were relabelled go, new roots come
+  if (c.cc_components >= 0) {
+    int64_t comps = c.cc_components - c.h_box[11] - c.h_box[4] + c.h_box[3];
+    if (comps > c.cc_components) st->splits += comps - c.cc_components;
+    c.cc_components = comps;
+  } else {
+    k_cc_zero<<<1, 64, 0, c.s>>>(box, 1);
+    k_cc_count_roots<<<kListGrid, kBlock, 0, c.s>>>(c.label, n, box);
+    SCOUT_CHECK(box_read(c, 1));
+    c.cc_components = c.h_box[0];
+  }
+  st->rounds += rounds;
+  st->edges_scanned += scanned_e;
+  st->touched += nu + nr;
+  c.label_seq = c.apply_seq;
+  return SCOUT_OK;
+}
+
+// Scan mode: one pass over every vertex lists the affected ones (cheaper than the walk's
+// dependent rounds when the batch touches a large share of the graph).
+int cc_scan(Ctx& c, Channel& ch, const int32_t* L, scout_stats* st) {
+  int32_t n = c.g.n;
   SCOUT_CHECK(ensure_scratch(c, &c.cc_owner, -1));
   SCOUT_CHECK(c.la.reserve((int64_t)n + 1));
   SCOUT_CHECK(c.lb.reserve((int64_t)n + 1));
@@ -453,7 +680,7 @@ int cc_local(Ctx& c, Channel& ch, scout_stats* st) {
   int64_t* box = c.d_box;
   // box: [0] listed vertices, [1] chunk tasks, [2] scanned entries, [3] roots among the listed,
   // [4] roots of untouched components, [5] long rows
-  SCOUT_TRY(hipMemsetAsync(box, 0, 6 * sizeof(int64_t), c.s));
+  k_cc_zero<<<1, 64, 0, c.s>>>(box, 6);
   SCOUT_CHECK(prof_begin(c, PROF_CC_EXPAND));
   k_zero_u32<<<grid_for(words + cwords), kBlock, 0, c.s>>>(bits, words + cwords);
   if (ch.m > 0) k_cc_mark<<<grid_for(2 * ch.m), kBlock, 0, c.s>>>(ch.d_uv.p, ch.m, L, c.label, bits, coarse);
@@ -488,6 +715,37 @@ int cc_local(Ctx& c, Channel& ch, scout_stats* st) {
   return SCOUT_OK;
 }
 
+// Exact labels after the batch: the BFS levels from vertex 0 decide vertex 0's component;
+// the rest is a union-find over the unreached subgraph.
+int cc_local(Ctx& c, Channel& ch, scout_stats* st) {
+  int32_t n = c.g.n;
+  if (n == 0) return SCOUT_OK;
+  const int32_t* L = nullptr;
+  if (c.level && c.source == 0 && c.level_seq == c.apply_seq) {
+    L = c.level;  // this batch's BFS repair from vertex 0 already ran on this context
+  } else {
+    SCOUT_CHECK(ensure_scratch(c, &c.cc_level, -1));
+    SCOUT_CHECK(ensure_scratch(c, &c.cc_parent, -1));
+    scout_stats bst{};
+    if (c.cc_level_seq >= 0 && c.cc_level_seq == c.apply_seq - 1) {
+      SCOUT_CHECK(bfs_local_on(c, ch, c.cc_level, c.cc_parent, SCOUT_STRATEGY_LOCAL, &bst));
+    } else {
+      SCOUT_CHECK(bfs_full_on(c, c.cc_level, c.cc_parent, 0, &bst.rounds, &bst.edges_scanned));
+    }
+    c.cc_level_seq = c.apply_seq;
+    st->rounds += bst.rounds;
+    st->edges_scanned += bst.edges_scanned;
+    st->seeds += bst.seeds;
+    st->touched += bst.touched;
+    st->insertions += bst.insertions;
+    st->deletions += bst.deletions;
+    st->fallback += bst.fallback;
+    L = c.cc_level;
+  }
+  bool walk = c.cc_mode == SCOUT_CC_WALK || (c.cc_mode == SCOUT_CC_AUTO && 2 * ch.m * kWalkRatio <= (int64_t)n);
+  return walk ? cc_walk(c, ch, L, st) : cc_scan(c, ch, L, st);
+}
+
 void cc_release(Ctx& c) { (void)c; }
 
 }  // namespace scout
@@ -507,6 +765,16 @@ int scout_cc_full(scout_ctx* x) {
   SCOUT_CHECK(ensure_state(c, SCOUT_STATE_LABEL));
   SCOUT_CHECK(cc_full(c, nullptr));
   SCOUT_TRY(hipStreamSynchronize(c.s));
+  return SCOUT_OK;
+}
+
+int scout_cc_mode(scout_ctx* x, int mode) {
+  if (!x) return SCOUT_ERR_ARG;
+  if (mode != SCOUT_CC_AUTO && mode != SCOUT_CC_WALK && mode != SCOUT_CC_SCAN) {
+    set_error("cc mode must be SCOUT_CC_AUTO, SCOUT_CC_WALK or SCOUT_CC_SCAN");
+    return SCOUT_ERR_ARG;
+  }
+  x->c.cc_mode = mode;
   return SCOUT_OK;
 }
 

@@ -230,7 +230,6 @@ struct Ctx {
   int64_t cc_components = -1;    // component count after the last CC repair
   int64_t label_seq = -1;        // apply_seq at which `label` was last made exact
   int64_t mis_version = 0;       // bumped when `mis` is overwritten outside the MIS repair
-  DevBuf<uint32_t> cc_bits;      // CC: bitmap of labels touched by the batch
   // small mailbox read back by the host after a round (kBoxSlots entries)
   int64_t* h_box = nullptr;    // pinned
   int64_t* d_box = nullptr;
@@ -260,13 +259,16 @@ struct Ctx {
   int64_t readbacks = 0;         // host read-backs of device counters so far
   double pred_over = 0;          // running mean of how far the predicted last level overshoots
   int hint_inval = 0, hint_relax = 0;  // scout_bfs_schedule: first group sizes of the next repair
+  int hint_cc = 0;                     // CC repair: walk rounds of the previous batch
+  int cc_mode = SCOUT_CC_AUTO;         // CC repair: walk, scan or chosen by batch size
+  DevBuf<uint32_t> cc_bits;            // CC scan mode: bitmaps of the labels the batch touched
 };
 
 constexpr int kBoxSlots = 64;
 // Long-row code paths counted on the device (scout_path_counts): chunk tasks of split rows
 // per kernel family, and invalid rows whose children did not fit the LDS list.
 enum PathId { PATH_INVAL_TASKS = 0, PATH_INVAL_PUSH_TASKS, PATH_INVAL_KIDS_OVERFLOW, PATH_CC_TASKS,
-              PATH_RELAX_TASKS, PATH_TENTATIVE_TASKS, kPathCount };
+              PATH_RELAX_TASKS, PATH_TENTATIVE_TASKS, PATH_CC_WALK_ROWS, kPathCount };
 
 // prims.hip (library scans/sorts)
 int scan_exclusive_i64(Ctx& c, const int64_t* in, int64_t* out, int64_t count);
@@ -349,7 +351,7 @@ void prof_bytes(Ctx& c, int id, int64_t bytes);
 void prof_release(Ctx& c);
 enum ProfId { PROF_BFS_RELAX = 0, PROF_BFS_INVALIDATE, PROF_CC_EXPAND, PROF_APPLY, PROF_CLASSIFY,
               PROF_COLOR, PROF_MIS_EVAL, PROF_CC_HOOK, PROF_BFS_INVALIDATE_KERNEL, PROF_K_RELAX_SHORT,
-              PROF_K_RELAX_ROWS, PROF_K_INVALIDATE_SHORT, PROF_K_INVAL_PUSH, PROF_K_CC_COLLECT, PROF_COUNT };
+              PROF_K_RELAX_ROWS, PROF_K_INVALIDATE_SHORT, PROF_K_INVAL_PUSH, PROF_K_CC_EXPLORE, PROF_K_CC_COLLECT, PROF_COUNT };
 
 // helpers in ctx.hip
 int box_read(Ctx& c, int count);  // copies d_box[0..count) into h_box, synchronises c.s

@@ -18,6 +18,9 @@ namespace scout {
 namespace {
 
 constexpr uint64_t kEmptyKey = ~0ULL;
+// Longest probe sequence of a claim. The set is kept at most half full, where linear probing
+// runs far shorter; the bound only caps the cost of a draw.
+constexpr uint64_t kMaxProbe = 4096;
 
 __device__ __forceinline__ uint64_t mix(uint64_t k) {
   k ^= k >> 33;
@@ -46,7 +49,7 @@ __global__ void k_gen_claim(const uint64_t* cand, int64_t draws, uint64_t* set, 
   uint64_t key = cand[i];
   if (key == kEmptyKey) return;
   uint64_t h = mix(key) & mask;
-  for (uint64_t probe = 0; probe <= mask; probe++) {
+  for (uint64_t probe = 0; probe <= mask && probe < kMaxProbe; probe++) {
     uint64_t prev = atomicCAS((unsigned long long*)&set[h], (unsigned long long)kEmptyKey, (unsigned long long)key);
     if (prev == kEmptyKey || prev == key) {
       atomicMin((unsigned long long*)&owner[h], (unsigned long long)(tag0 + (uint64_t)i));
@@ -55,7 +58,25 @@ __global__ void k_gen_claim(const uint64_t* cand, int64_t draws, uint64_t* set, 
     }
     h = (h + 1) & mask;
   }
-  cslot[i] = kEmptyKey;  // set full: the draw is dropped
+  cslot[i] = kEmptyKey;  // no free slot within kMaxProbe: the draw is dropped
+}
+
+// Moves every pair of the old toggle set into a larger one. Their owner tag becomes 0, below
+// every later draw's tag, so they are never accepted again.
+__global__ void k_gen_rehash(const uint64_t* old, uint64_t old_slots, uint64_t* set, uint64_t* owner, uint64_t mask) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= old_slots) return;
+  uint64_t key = old[i];
+  if (key == kEmptyKey) return;
+  uint64_t h = mix(key) & mask;
+  for (uint64_t probe = 0; probe <= mask; probe++) {
+    if (atomicCAS((unsigned long long*)&set[h], (unsigned long long)kEmptyKey, (unsigned long long)key) ==
+        kEmptyKey) {
+      owner[h] = 0;
+      return;
+    }
+    h = (h + 1) & mask;
+  }
 }
 
 __global__ void k_gen_accept(const uint64_t* cand, const uint64_t* cslot, const uint64_t* owner, uint64_t tag0,
@@ -246,6 +267,7 @@ struct StreamState {
   uint64_t batches = 0;  // batches generated so far
   DevBuf<uint64_t> set, owner;  // toggled pairs and the draw tag that owns each
   uint64_t mask = 0;
+  int64_t used = 0;  // pairs in the set
   uint64_t draws_used[3] = {0, 0, 0};
   uint64_t next_tag = 0;           // draw tags, increasing over attempts and batches
   DevBuf<uint64_t> cand, cslot;    // per draw: proposed pair, its set slot
@@ -256,20 +278,40 @@ struct StreamState {
   std::vector<int64_t> slot_m;
 };
 
+// Replaces the toggle set by one of at least `slots` slots holding the same pairs.
+static int grow_set(Ctx& c, StreamState& s, uint64_t slots) {
+  DevBuf<uint64_t> set2, own2;
+  SCOUT_CHECK(set2.reserve((int64_t)slots));
+  SCOUT_CHECK(own2.reserve((int64_t)slots));
+  SCOUT_TRY(hipMemsetAsync(set2.p, 0xff, (size_t)set2.cap * 8, c.s));
+  SCOUT_TRY(hipMemsetAsync(own2.p, 0xff, (size_t)own2.cap * 8, c.s));
+  uint64_t mask = 1;
+  while (mask * 2 <= (uint64_t)set2.cap) mask *= 2;
+  mask -= 1;
+  if (s.set.p) {
+    k_gen_rehash<<<grid_for((int64_t)(s.mask + 1)), kBlock, 0, c.s>>>(s.set.p, s.mask + 1, set2.p, own2.p, mask);
+    SCOUT_TRY(hipStreamSynchronize(c.s));
+  }
+  s.set.release();
+  s.owner.release();
+  s.set = set2;
+  s.owner = own2;
+  s.mask = mask;
+  return SCOUT_OK;
+}
+
 int gen_batch(Ctx& c, Channel& ch, int64_t size, double delete_frac, double tree_frac, uint64_t seed,
               int64_t expected_total, int64_t* out_m) {
   if (!c.stream) c.stream = new StreamState();
   StreamState& s = *c.stream;
-  if (!s.set.p) {
-    uint64_t cap = 1024;
-    while (cap < (uint64_t)(4 * std::max<int64_t>(expected_total, size))) cap <<= 1;
-    SCOUT_CHECK(s.set.reserve((int64_t)cap));
-    SCOUT_CHECK(s.owner.reserve((int64_t)cap));
-    SCOUT_TRY(hipMemsetAsync(s.set.p, 0xff, (size_t)s.set.cap * 8, c.s));
-    SCOUT_TRY(hipMemsetAsync(s.owner.p, 0xff, (size_t)s.owner.cap * 8, c.s));
-    s.mask = 1;
-    while (s.mask * 2 <= (uint64_t)s.set.cap) s.mask *= 2;
-    s.mask -= 1;
+  // a batch fills at most about 3 * size slots (draws accepted past its room stay claimed);
+  // the set stays at most half full
+  uint64_t need = (uint64_t)(s.used + 4 * size);
+  if (!s.set.p || 2 * need > s.mask + 1) {
+    uint64_t want = std::max<uint64_t>((uint64_t)(4 * std::max<int64_t>(expected_total, size)), 4 * need);
+    uint64_t slots = 1024;
+    while (slots < want) slots <<= 1;
+    SCOUT_CHECK(grow_set(c, s, slots));
   }
   int64_t ndel = (int64_t)llround(delete_frac * (double)size);
   int64_t ntree = (int64_t)llround(tree_frac * (double)ndel);
@@ -330,6 +372,7 @@ int gen_batch(Ctx& c, Channel& ch, int64_t size, double delete_frac, double tree
       k_gen_emit<<<grid, kBlock, 0, c.s>>>(s.cand.p, s.acc.p, s.pos.p, draws, out + 2 * got, want[kind] - got,
                                            c.d_box);
       SCOUT_CHECK(box_read(c, 1));
+      s.used += c.h_box[0];
       got += std::min(c.h_box[0], want[kind] - got);
     }
     // A pair accepted past the batch's room stays claimed: such pairs are skipped by every

@@ -109,8 +109,10 @@ def test_hub_rows_bfs_cc_exact_every_batch(tmp_path):
     g = sa.Graph(n, off, col)
     eng = sa.Engine(0)
     eng.load_graph(g)
+    eng.cc_mode(sa.CC_WALK)
     cc_only = sa.Engine(0)  # CC without BFS maintained: its own levels from vertex 0
     cc_only.load_graph(g)
+    cc_only.cc_mode(sa.CC_SCAN)
     lv0 = og.bfs(0)
     eng.upload(sa.STATE_LEVEL, lv0)
     eng.upload(sa.STATE_LABEL, og.cc())
@@ -165,6 +167,6 @@ def test_hub_rows_bfs_cc_exact_every_batch(tmp_path):
     say(f"[hubs] path counts {paths}")
     for name in sa.PATH_NAMES:
         assert paths[name] > 0, f"the {name} path never ran"
-    assert cc_only.path_counts()["cc_tasks"] > 0
+    assert cc_only.path_counts()["cc_tasks"] > 0 and cc_only.path_counts()["cc_walk_rows"] == 0
     eng.close()
     cc_only.close()

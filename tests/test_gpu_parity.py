@@ -60,15 +60,28 @@ def engine_for(egr):
     return eng, g
 
 
+CC_MODES = {"walk": [sa.CC_WALK], "scan": [sa.CC_SCAN], "mixed": [sa.CC_WALK, sa.CC_SCAN, sa.CC_AUTO]}
+
+
+def components(lab):
+    return int(np.count_nonzero(lab == np.arange(len(lab))))
+
+
+@pytest.mark.parametrize("mode", list(CC_MODES))
 @pytest.mark.parametrize("name", ["sample", "er", "rmat", "deg", "heavy"])
-def test_bfs_cc_exact_every_batch(data, name):
+def test_bfs_cc_exact_every_batch(data, name, mode):
+    """Levels and labels equal the oracle's after every batch, in each CC mode (walk from the
+    endpoints, scan of every vertex, or the modes in turn), and the split count equals the rise
+    in the oracle's component count."""
     egr, bdir, nb = data[name]
-    say(f"[bfs+cc] {name}")
+    say(f"[bfs+cc] {name} {mode}")
     eng, g = engine_for(egr)
     og = OracleGraph(egr)
     eng.upload(sa.STATE_LEVEL, og.bfs(0))
     eng.upload(sa.STATE_LABEL, og.cc())
+    comps = components(og.cc())
     for k, uv in batches(bdir, nb, g.n):
+        eng.cc_mode(CC_MODES[mode][k % len(CC_MODES[mode])])
         eng.stage(uv)
         ins = eng.classify(sa.CLASSIFY_CURRENT)
         ins_o = og.classify(uv)
@@ -76,11 +89,15 @@ def test_bfs_cc_exact_every_batch(data, name):
         eng.apply(False)
         og.apply(uv, ins_o)
         eng.repair_bfs()
-        eng.repair_cc()
+        st = eng.repair_cc()
         lv, lab = eng.download(sa.STATE_LEVEL), eng.download(sa.STATE_LABEL)
         want_lv, want_lab = og.bfs(0), og.cc()
         assert np.array_equal(lv, want_lv), f"{name} batch {k}: {np.count_nonzero(lv != want_lv)} levels differ"
         assert np.array_equal(lab, want_lab), f"{name} batch {k}: {np.count_nonzero(lab != want_lab)} labels differ"
+        now = components(want_lab)
+        if k >= 2:  # the first repair after an upload counts the components without a delta
+            assert st.splits == max(0, now - comps), f"{name} batch {k}: splits {st.splits}, want {now - comps}"
+        comps = now
     off, col = og.logical_csr()
     dg = eng.download_graph()
     assert np.array_equal(dg.off, off) and np.array_equal(dg.col, col), "logical graph differs"
@@ -156,13 +173,15 @@ def test_serving_loop_steps_equal_the_oracle(tmp_path):
         assert np.array_equal(lab, og.cc()), f"step {k}: labels differ"
 
 
+@pytest.mark.parametrize("mode", ["walk", "scan"])
 @pytest.mark.parametrize("name", ["rmat", "heavy"])
-def test_cc_alone_exact_every_batch(data, name):
+def test_cc_alone_exact_every_batch(data, name, mode):
     """CC repaired without BFS maintained on the context: the CC state keeps its own levels
     from vertex 0 (and their parents) and repairs them batch by batch."""
     egr, bdir, nb = data[name]
-    say(f"[cc alone] {name}")
+    say(f"[cc alone] {name} {mode}")
     eng, g = engine_for(egr)
+    eng.cc_mode(CC_MODES[mode][0])
     og = OracleGraph(egr)
     eng.upload(sa.STATE_LABEL, og.cc())
     for k, uv in batches(bdir, nb, g.n):

@@ -67,3 +67,30 @@ def test_streams_are_reproducible(mode):
             runs.append(out)
     for a, b in zip(*runs):
         assert a.shape == b.shape and np.array_equal(a, b)
+
+
+def test_toggle_set_grows_and_keeps_the_stream():
+    """A context whose toggle set was sized for a short stream grows it as batches accumulate:
+    its batches equal those of a context sized for the whole stream up front, line for line,
+    and no pair repeats across batches."""
+    print("stream: toggle set sized for 1 line vs 1M lines, 8 batches of 3000", flush=True)
+    runs = []
+    for expected in (1, 1_000_000):
+        with sa.Engine(0) as eng:
+            eng.gen_rmat(12, 16, seed=3)
+            eng.bfs_full(0)
+            out = []
+            for b in range(8):
+                m = eng.gen_batch(3000, 0.7, 0.0, seed=9, expected_total=expected)
+                assert m == 3000, f"batch {b}: {m} lines"
+                out.append(eng.batch_download(m).copy())
+                eng.classify()
+                eng.apply()
+                eng.repair_bfs(sa.STRATEGY_LOCAL)
+            runs.append(out)
+    seen = set()
+    for a, b in zip(*runs):
+        assert np.array_equal(a, b)
+        pairs = set(map(tuple, np.sort(a, axis=1).tolist()))
+        assert len(pairs) == len(a) and not (pairs & seen)
+        seen |= pairs
