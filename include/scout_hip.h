@@ -78,6 +78,8 @@ typedef struct {
   int64_t inval_rounds; /* bfs: invalidation levels with work */
   int64_t relax_rounds; /* bfs: relax levels with work */
   int64_t completed;    /* mis complete mode: vertices the completion added */
+  int64_t cc_mode;      /* cc: SCOUT_CC_WALK or SCOUT_CC_SCAN, how the local repair listed the
+                           vertices to relabel (0 after a full recompute) */
 } scout_stats;
 
 const char* scout_last_error(void);
@@ -173,7 +175,9 @@ int scout_repair_cc(scout_ctx* ctx, int channel, int strategy, scout_stats* st);
  * batch's endpoints over the affected components (work proportional to them, one device round
  * per level); SCOUT_CC_SCAN makes one pass over every vertex (fixed cost, no rounds);
  * SCOUT_CC_AUTO (the default) walks when the batch has at most n/256 endpoints and scans
- * otherwise. Labels are the same in every mode. No counterpart in the reference, whose deletion
+ * otherwise; under SCOUT_STRATEGY_LEARNED the count adds the vertices the scheduler's split head
+ * expects the batch to cut off vertex 0's component (when this batch's BFS repair ran learned
+ * on the same context). Labels are the same in every mode. No counterpart in the reference, whose deletion
  * check is a bounded host BFS per line (src/cc/scout_cc.cu:380-441). */
 enum { SCOUT_CC_AUTO = 0, SCOUT_CC_WALK = 1, SCOUT_CC_SCAN = 2 };
 int scout_cc_mode(scout_ctx* ctx, int mode);

@@ -36,7 +36,8 @@ class Stats(ctypes.Structure):
         "lines", "insertions", "deletions", "noop", "seeds", "touched", "rounds", "fallback",
         "splits", "conflicts", "sub_batches", "edges_scanned")] + [
         ("repair_ms", ctypes.c_double), ("apply_ms", ctypes.c_double), ("readbacks", ctypes.c_int64),
-        ("inval_rounds", ctypes.c_int64), ("relax_rounds", ctypes.c_int64), ("completed", ctypes.c_int64)]
+        ("inval_rounds", ctypes.c_int64), ("relax_rounds", ctypes.c_int64), ("completed", ctypes.c_int64),
+        ("cc_mode", ctypes.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -1080,11 +1080,12 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, int32_t* P, int strategy, scou
     k_del_candidates<<<grid_for(m, kBlock * kLinesPerThread), kBlock, 0, c.s>>>(ch.d_uv.p, ch.d_ins.p, m, L, P,
                                                                               c.la.p, c.la.cap, box, ab.root);
   // features and predictions are computed from the device-side counts and come back with the
-  // same read-back as the candidate counts ([24] predicted cascade sizes, [25] last level)
+  // same read-back as the candidate counts ([24] predicted cascade sizes, [25] last level,
+  // [27] expected vertices cut off vertex 0's component, in sixteenths)
   int64_t sample = logging ? kMlpSample : kMlpInfer;
   if (logging || learned) SCOUT_CHECK(mlp_begin_batch(c, c.la.p, box, L, P, sample));
   if (learned) SCOUT_CHECK(mlp_predict_async(c, box, sample, box + 24));
-  SCOUT_CHECK(box_read(c, learned ? 26 : 8));
+  SCOUT_CHECK(box_read(c, learned ? 28 : 8));
   int64_t ncand = c.h_box[0];
   int32_t cmin = (int32_t)c.h_box[1], cmax = (int32_t)c.h_box[2];
   st->deletions += c.h_box[3];
@@ -1095,6 +1096,9 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, int32_t* P, int strategy, scou
   if (learned && ncand > 0) {
     // predictions size and order the work; the plain search below decides every level
     double predicted = (double)c.h_box[24] * (double)ncand / (double)nsample;
+    // the split head's expected cut-off, kept for this batch's CC repair
+    c.pred_cut = (int64_t)((double)c.h_box[27] / 16.0 * (double)ncand / (double)nsample);
+    c.pred_cut_seq = c.apply_seq;
     if (predicted > kRecomputeFraction * (double)n) {
       // cascades over a large part of the graph: one full BFS costs less than local repair
       st->fallback = 1;

@@ -717,7 +717,7 @@ int cc_scan(Ctx& c, Channel& ch, const int32_t* L, scout_stats* st) {
 
 // Exact labels after the batch: the BFS levels from vertex 0 decide vertex 0's component;
 // the rest is a union-find over the unreached subgraph.
-int cc_local(Ctx& c, Channel& ch, scout_stats* st) {
+int cc_local(Ctx& c, Channel& ch, int strategy, scout_stats* st) {
   int32_t n = c.g.n;
   if (n == 0) return SCOUT_OK;
   const int32_t* L = nullptr;
@@ -742,7 +742,13 @@ int cc_local(Ctx& c, Channel& ch, scout_stats* st) {
     st->fallback += bst.fallback;
     L = c.cc_level;
   }
-  bool walk = c.cc_mode == SCOUT_CC_WALK || (c.cc_mode == SCOUT_CC_AUTO && 2 * ch.m * kWalkRatio <= (int64_t)n);
+  // the walk lists the endpoints' components; under the learned strategy the split head adds
+  // the vertices it expects the batch to cut off vertex 0's component (the walk must list them
+  // all, the scan's cost does not depend on them)
+  int64_t est = 2 * ch.m;
+  if (strategy == SCOUT_STRATEGY_LEARNED && c.pred_cut_seq == c.apply_seq && L == c.level) est += c.pred_cut;
+  bool walk = c.cc_mode == SCOUT_CC_WALK || (c.cc_mode == SCOUT_CC_AUTO && est * kWalkRatio <= (int64_t)n);
+  st->cc_mode = walk ? SCOUT_CC_WALK : SCOUT_CC_SCAN;
   return walk ? cc_walk(c, ch, L, st) : cc_scan(c, ch, L, st);
 }
 
@@ -799,7 +805,7 @@ int scout_repair_cc(scout_ctx* x, int channel, int strategy, scout_stats* st) {
     st->fallback = 1;
     SCOUT_CHECK(cc_full(c, &st->edges_scanned));
   } else {
-    SCOUT_CHECK(cc_local(c, *ch, st));
+    SCOUT_CHECK(cc_local(c, *ch, strategy, st));
   }
   SCOUT_CHECK(span.end(c.s));
   st->repair_ms += span.ms();

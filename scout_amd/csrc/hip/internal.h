@@ -258,6 +258,8 @@ struct Ctx {
   float last_repair_ms = 0.f;
   int64_t readbacks = 0;         // host read-backs of device counters so far
   double pred_over = 0;          // running mean of how far the predicted last level overshoots
+  int64_t pred_cut = 0;          // split head: expected vertices the batch cuts off vertex 0's component
+  int64_t pred_cut_seq = -1;     // the batch (apply_seq) pred_cut was predicted for
   int hint_inval = 0, hint_relax = 0;  // scout_bfs_schedule: first group sizes of the next repair
   int hint_cc = 0;                     // CC repair: walk rounds of the previous batch
   int cc_mode = SCOUT_CC_AUTO;         // CC repair: walk, scan or chosen by batch size
@@ -311,7 +313,7 @@ int cc_full_into(Ctx& c, int32_t* label, int64_t* tasks, int64_t task_cap, int64
 // noalloc.hip: resolves the device address of the file-scope scratch array on device `dev`
 // (once per device; called by scout_ctx_create with `dev` current).
 int noalloc_init(int dev);
-int cc_local(Ctx& c, Channel& ch, scout_stats* st);
+int cc_local(Ctx& c, Channel& ch, int strategy, scout_stats* st);
 void cc_release(Ctx& c);
 void mis_release(Ctx& c);
 void stream_release(Ctx& c);

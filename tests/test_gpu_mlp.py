@@ -182,3 +182,51 @@ def test_learned_recompute_switch_keeps_results_exact(tmp_path):
             assert np.array_equal(eng.download(sa.STATE_LEVEL), og.bfs(0))
             assert np.array_equal(eng.download(sa.STATE_LABEL), og.cc())
             assert ins is not None
+
+
+@pytest.mark.parametrize("logit,want", [(-20.0, "walk"), (20.0, "scan")])
+def test_split_head_chooses_the_cc_listing(tmp_path, logit, want):
+    """The split head's expected cut-off decides how the learned CC repair lists the vertices to
+    relabel: a network sure that no deletion splits vertex 0's component leaves small batches to
+    the walk, one sure that every deletion cuts off its predicted cascade moves them to the scan.
+    Each batch deletes 10 edges to a vertex's only neighbour one level up (a parent edge in every
+    BFS tree, so the network sees candidates). Labels equal the oracle's either way."""
+    from conftest import run_gen
+    egr = tmp_path / "g.egr"
+    run_gen("rmat", 14, 16, 4, egr)
+    g = sa.Graph.read_egr(str(egr))
+    og = OracleGraph(str(egr))
+    rng = np.random.default_rng(17)
+    print(f"mlp: split head logit {logit}, n={g.n}, 10-line batches of parent-edge deletions", flush=True)
+    with sa.Engine(0) as eng:
+        eng.load_graph(g)
+        eng.upload(sa.STATE_LEVEL, og.bfs(0))
+        eng.upload(sa.STATE_LABEL, og.cc())
+        p = np.zeros(sa.MLP_PARAMS, dtype=np.float32)
+        p[5184 + 0] = logit  # b3[0]: logit of "ends unreached"
+        p[5184 + 1] = 5.0    # b3[1]: log1p(cascade size) = 5, far below the recompute switch
+        eng.mlp_set_params(p)
+        for k in range(1, 4):
+            lv = og.bfs(0)
+            lines = []
+            for v in rng.permutation(g.n):
+                if lv[v] <= 0:
+                    continue
+                r = og.row(int(v))
+                up = r[lv[r] == lv[v] - 1]
+                if len(up) == 1:
+                    lines.append((int(up[0]), int(v)))
+                if len(lines) == 10:
+                    break
+            uv = np.array(lines, dtype=np.int32)
+            eng.stage(uv)
+            eng.classify()
+            eng.apply()
+            og.apply(uv, og.classify(uv))
+            sb = eng.repair_bfs(sa.STRATEGY_LEARNED)
+            sc = eng.repair_cc(sa.STRATEGY_LEARNED)
+            assert sb.fallback == 0
+            assert 2 * len(uv) * 256 <= g.n  # by size alone the batch is walked
+            assert sc.cc_mode == (sa.CC_WALK if want == "walk" else sa.CC_SCAN), f"batch {k}"
+            assert np.array_equal(eng.download(sa.STATE_LEVEL), og.bfs(0))
+            assert np.array_equal(eng.download(sa.STATE_LABEL), og.cc())
