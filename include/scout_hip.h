@@ -109,6 +109,7 @@ int scout_graph_load(scout_ctx* ctx, int32_t n, int64_t entries, const void* nin
 /* out[0..7] = n, base entries, inserted-row entries, logical entries, rows sorted (0/1),
  * base entries host-resident (0/1), bytes streamed from host memory by full recomputes, and
  * the copy-stream time of those transfers in microseconds. */
+enum { SCOUT_GRAPH_INFO_COUNT = 8 }; /* entries scout_graph_info writes */
 int scout_graph_info(scout_ctx* ctx, int64_t* out);
 /* The logical graph as a CSR in scratchpad-compaction row order (base rows minus deleted
  * entries, then inserted entries in insertion order): core/flash_compactor.h:23-95.
@@ -204,6 +205,16 @@ int scout_mis_mode(scout_ctx* ctx, int mode);
 /* Launch settings of the multi-instance MIS rounds: grid blocks, rounds per replay of the
  * captured hipGraph, graph on/off. No counterpart in the reference. */
 int scout_mis_config(scout_ctx* ctx, int grid, int group_rounds, int use_graph);
+/* out[0..2] = the MIS round settings in use: grid, rounds per replay, graph on/off. */
+int scout_mis_get_config(scout_ctx* ctx, int* out);
+/* Times the MIS repair of the staged, applied batch (lines grouped by instance as for
+ * scout_repair_mis_multi) under grids {512, 1024, 2048, 4096} x round groups {4, 8, 16, 32}
+ * and keeps the fastest. The membership and member counts are put back before each trial and
+ * at the end, so the batch is then repaired as usual; a setting whose membership differs from
+ * the first trial's is an error (SCOUT_ERR_STATE). best_ms (may be NULL) = the kept setting's
+ * repair time. No counterpart in the reference. */
+int scout_mis_autotune(scout_ctx* ctx, int channel, int32_t ninst, const int64_t* line_off, int64_t max_csr,
+                       double* best_ms);
 /* Capacities of the MIS hop-2 search lists per pass: deferred searches (at most 65536) and
  * chunk tasks. A search that does not fit runs serially in one wave; results do not depend on
  * them. No counterpart in the reference. */

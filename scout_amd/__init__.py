@@ -25,6 +25,7 @@ CLASSIFY_CURRENT, CLASSIFY_BASE = 0, 1
 STRATEGY_LOCAL, STRATEGY_RECOMPUTE, STRATEGY_LEARNED = 0, 1, 2
 MIS_RULES, MIS_COMPLETE = 0, 1
 CC_AUTO, CC_WALK, CC_SCAN = 0, 1, 2
+GRAPH_INFO_COUNT = 8  # entries scout_graph_info writes
 
 
 class ScoutError(RuntimeError):
@@ -74,6 +75,8 @@ _SIGNATURES = {
     "scout_mis_mode": [_P, ctypes.c_int],
     "scout_cc_mode": [_P, ctypes.c_int],
     "scout_mis_limits": [_P, _I64, _I64],
+    "scout_mis_get_config": [_P, _P],
+    "scout_mis_autotune": [_P, ctypes.c_int, _I32, _P, _I64, ctypes.POINTER(ctypes.c_double)],
     "scout_color_config": [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int],
     "scout_color_get_config": [_P, _P],
     "scout_color_autotune": [_P, ctypes.c_int, ctypes.POINTER(ctypes.c_double)],
@@ -234,7 +237,7 @@ class Engine:
         return free.value, total.value
 
     def graph_info(self) -> dict:
-        out = np.zeros(8, dtype=np.int64)
+        out = np.zeros(GRAPH_INFO_COUNT, dtype=np.int64)
         self._check(self.lib.scout_graph_info(self.h, _ptr(out)), "graph_info")
         return {"n": int(out[0]), "base_entries": int(out[1]), "inserted_entries": int(out[2]),
                 "logical_entries": int(out[3]), "sorted": bool(out[4]), "host_resident": bool(out[5]),
@@ -327,6 +330,21 @@ class Engine:
 
     def mis_config(self, grid: int = 1024, group_rounds: int = 4, use_graph: bool = True):
         self._check(self.lib.scout_mis_config(self.h, grid, group_rounds, 1 if use_graph else 0), "mis_config")
+
+    def mis_get_config(self):
+        out = np.zeros(3, dtype=np.int32)
+        self._check(self.lib.scout_mis_get_config(self.h, _ptr(out)), "mis_get_config")
+        return {"grid": int(out[0]), "group_rounds": int(out[1]), "use_graph": bool(out[2])}
+
+    def mis_autotune(self, line_off, max_csr: int, channel: int = 0) -> float:
+        """Times the MIS repair of the staged, applied batch (lines grouped by instance) under
+        each grid and round-group setting and keeps the fastest; the membership is left as it
+        was. Returns the best time in ms."""
+        off = np.ascontiguousarray(line_off, dtype=np.int64)
+        best = ctypes.c_double()
+        self._check(self.lib.scout_mis_autotune(self.h, channel, len(off) - 1, _ptr(off), max_csr,
+                                                ctypes.byref(best)), "mis_autotune")
+        return best.value
 
     def mis_limits(self, hop2_requests: int = 65536, hop2_tasks: int = 1 << 22):
         self._check(self.lib.scout_mis_limits(self.h, hop2_requests, hop2_tasks), "mis_limits")

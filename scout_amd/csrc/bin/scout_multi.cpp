@@ -10,7 +10,8 @@
 //      launches, round groups replayed from a captured hipGraph)
 //   -> scout_repair_color (the union's colouring repair is each instance's, since no edge
 //      crosses instances and the lower-id rule keeps its order under the id offset)
-// The colouring launch settings come from scout_color_autotune on the first batch.
+// The colouring and MIS launch settings come from scout_color_autotune and scout_mis_autotune
+// on the first batch.
 //
 // Checks: the same stream served instance by instance, each instance alone in its own steps,
 // must give identical MIS bits and colours ("alone == in the mix"); the final colouring must be
@@ -220,17 +221,19 @@ int main(int argc, char** argv) {
     CHECK(scout_mis_config(ctx, 1024, 8, 0));
   }
 
-  int cfg[4] = {0, 0, 0, 0};
-  double tune_ms = 0;
+  int cfg[4] = {0, 0, 0, 0}, mis_cfg[3] = {0, 0, 0};
+  double tune_ms = 0, mis_tune_ms = 0;
   if (tune) {
     const auto& uv = batch_uv[0];
     CHECK(scout_stage_batch(ctx, 0, uv.data(), (int64_t)uv.size() / 2));
     CHECK(scout_classify(ctx, 0, SCOUT_CLASSIFY_CURRENT, nullptr));
     CHECK(scout_apply(ctx, 0, 0));
     CHECK(scout_color_autotune(ctx, 0, &tune_ms));
+    CHECK(scout_mis_autotune(ctx, 0, ninst, batch_off[0].data(), max_csr, &mis_tune_ms));
     CHECK(scout_restore(ctx));
   }
   CHECK(scout_color_get_config(ctx, cfg));
+  CHECK(scout_mis_get_config(ctx, mis_cfg));
 
   // the mix: one step per batch for all instances; the first pass warms up (graph capture,
   // buffer growth) and is repeated from the snapshot, the second is timed
@@ -263,7 +266,7 @@ int main(int argc, char** argv) {
   CHECK(scout_state_download(ctx, SCOUT_STATE_COLOR, col_mix.data(), n));
 
   // checks on the final graph
-  int64_t info[6];
+  int64_t info[SCOUT_GRAPH_INFO_COUNT];
   CHECK(scout_graph_info(ctx, info));
   std::vector<int64_t> loff((size_t)n + 1);
   std::vector<int32_t> lcol((size_t)info[3]);
@@ -314,14 +317,16 @@ int main(int argc, char** argv) {
          "\"updates_per_s\": %.1f, \"step_ms_p50\": %.3f, \"step_ms_max\": %.3f, "
          "\"mis_rounds\": %lld, \"mis_sub_batches\": %lld, \"mis_readbacks\": %lld, \"color_rounds\": %lld, "
          "\"color_readbacks\": %lld, \"color_config\": {\"thread_max\": %d, \"block_min\": %d, \"grid\": %d, "
-         "\"graph\": %s}, \"autotune_best_ms\": %.3f, "
+         "\"graph\": %s}, \"autotune_best_ms\": %.3f, \"mis_config\": {\"grid\": %d, \"group_rounds\": %d, "
+         "\"graph\": %s}, \"mis_autotune_best_ms\": %.3f, "
          "\"alone_updates_per_s\": %s, \"speedup_vs_alone\": %s, \"same_as_alone\": %s, "
          "\"proper_coloring\": %s, \"independent_mis\": %s}\n",
          ninst, mba, min_n, max_n, (unsigned long long)seed, batches, min_lines, max_lines, n, (long long)entries,
          (long long)total_lines, gen_s, (long long)max_csr, (double)total_lines / mix_s, pct(step_ms, 50),
          pct(step_ms, 100), (long long)mis_st.rounds, (long long)mis_st.sub_batches, (long long)mis_st.readbacks,
          (long long)col_st.rounds, (long long)col_st.readbacks, cfg[0], cfg[1], cfg[2], cfg[3] ? "true" : "false",
-         tune_ms, alone ? std::to_string((double)total_lines / alone_s).c_str() : "null",
+         tune_ms, mis_cfg[0], mis_cfg[1], mis_cfg[2] ? "true" : "false", mis_tune_ms,
+         alone ? std::to_string((double)total_lines / alone_s).c_str() : "null",
          alone ? std::to_string(alone_s / mix_s).c_str() : "null", alone ? (same ? "true" : "false") : "null",
          bad_color == 0 ? "true" : "false", bad_mis == 0 ? "true" : "false");
   fflush(stdout);

@@ -130,7 +130,7 @@ int main(int argc, char** argv) {
   scout_ctx* ctx = nullptr;
   CHECK(scout_ctx_create(gpu, &budget, &ctx));
   CHECK(scout_gen_rmat(ctx, scale, 16, 0.57, 0.19, 0.19, seed));
-  int64_t info[6];
+  int64_t info[SCOUT_GRAPH_INFO_COUNT];
   CHECK(scout_graph_info(ctx, info));
   int32_t n = (int32_t)info[0];
   CHECK(scout_bfs_full(ctx, 0));

@@ -1563,6 +1563,104 @@ extern "C" int scout_mis_config(scout_ctx* x, int grid, int group_rounds, int us
   return SCOUT_OK;
 }
 
+extern "C" int scout_mis_get_config(scout_ctx* x, int* out) {
+  if (!x || !out) return SCOUT_ERR_ARG;
+  Ctx& c = x->c;
+  if (!c.mis_state) c.mis_state = new MisState();
+  out[0] = (int)c.mis_state->grid;
+  out[1] = c.mis_state->group_rounds;
+  out[2] = c.mis_state->use_graph ? 1 : 0;
+  return SCOUT_OK;
+}
+
+// Times the staged, applied batch's MIS repair under each grid and round-group setting (the
+// graph replay setting is kept) and keeps the fastest. Every trial starts from the same state:
+// the membership, the member counts and what those counts were computed for are put back before
+// it, and again at the end. Each trial's membership must equal the first one's.
+extern "C" int scout_mis_autotune(scout_ctx* x, int channel, int32_t ninst, const int64_t* line_off, int64_t max_csr,
+                                  double* best_ms) {
+  if (!x) return SCOUT_ERR_ARG;
+  Ctx& c = x->c;
+  SCOUT_TRY(hipSetDevice(c.dev));
+  Channel* ch = channel_of(c, channel);
+  if (!ch) return SCOUT_ERR_ARG;
+  if (!c.g.boff || !c.mis) {
+    set_error("mis_autotune: graph, MIS and a staged batch are needed");
+    return SCOUT_ERR_STATE;
+  }
+  if (!c.mis_state) c.mis_state = new MisState();
+  MisState& ms = *c.mis_state;
+  int32_t n = c.g.n;
+  bool had_counts = ms.mcount.p && ms.mcount.cap >= (int64_t)n + 1;
+  int64_t v0 = ms.mc_version, a0 = ms.mc_apply, g0 = ms.mc_gen;
+  DevBuf<uint8_t> saved_mis;
+  DevBuf<int32_t> saved_counts;
+  SCOUT_CHECK(saved_mis.reserve((int64_t)n + 1));
+  SCOUT_TRY(hipMemcpyAsync(saved_mis.p, c.mis, (size_t)n, hipMemcpyDeviceToDevice, c.s));
+  if (had_counts) {
+    SCOUT_CHECK(saved_counts.reserve((int64_t)n + 1));
+    SCOUT_TRY(hipMemcpyAsync(saved_counts.p, ms.mcount.p, ((size_t)n + 1) * 4, hipMemcpyDeviceToDevice, c.s));
+  }
+  auto put_back = [&]() -> int {
+    SCOUT_TRY(hipMemcpyAsync(c.mis, saved_mis.p, (size_t)n, hipMemcpyDeviceToDevice, c.s));
+    if (had_counts)
+      SCOUT_TRY(hipMemcpyAsync(ms.mcount.p, saved_counts.p, ((size_t)n + 1) * 4, hipMemcpyDeviceToDevice, c.s));
+    ms.mc_version = v0;
+    ms.mc_apply = a0;
+    ms.mc_gen = g0;
+    return SCOUT_OK;
+  };
+  const unsigned grids[] = {512, 1024, 2048, 4096};
+  const int groups[] = {4, 8, 16, 32};
+  unsigned keep_grid = ms.grid;
+  int keep_group = ms.group_rounds;
+  double best = 1e30;
+  std::vector<uint8_t> first((size_t)n), got((size_t)n);
+  bool have_first = false;
+  int rc = SCOUT_OK;
+  for (unsigned gr : grids) {
+    for (int gp : groups) {
+      ms.grid = gr;
+      ms.group_rounds = gp;
+      // the first run of a setting captures its graph; the second is timed
+      for (int rep = 0; rep < 2 && rc == SCOUT_OK; rep++) {
+        rc = put_back();
+        scout_stats st{};
+        if (rc == SCOUT_OK) rc = mis_entry(x, channel, ninst, line_off, max_csr, &st, "mis_autotune");
+        if (rc != SCOUT_OK) break;
+        if (rep == 1 && st.repair_ms < best) {
+          best = st.repair_ms;
+          keep_grid = gr;
+          keep_group = gp;
+        }
+      }
+      if (rc != SCOUT_OK) break;
+      if (hipMemcpy(have_first ? got.data() : first.data(), c.mis, (size_t)n, hipMemcpyDeviceToHost) != hipSuccess) {
+        set_error("mis_autotune: membership read-back failed");
+        rc = SCOUT_ERR_HIP;
+        break;
+      }
+      if (have_first && got != first) {
+        set_error("mis_autotune: a launch setting changed the membership");
+        rc = SCOUT_ERR_STATE;
+        break;
+      }
+      have_first = true;
+    }
+    if (rc != SCOUT_OK) break;
+  }
+  ms.grid = keep_grid;
+  ms.group_rounds = keep_group;
+  int rc2 = put_back();
+  if (hipStreamSynchronize(c.s) != hipSuccess && rc == SCOUT_OK) rc = SCOUT_ERR_HIP;
+  saved_mis.release();
+  saved_counts.release();
+  if (rc != SCOUT_OK) return rc;
+  if (rc2 != SCOUT_OK) return rc2;
+  if (best_ms) *best_ms = best;
+  return SCOUT_OK;
+}
+
 extern "C" int scout_mis_limits(scout_ctx* x, int64_t hop2_requests, int64_t hop2_tasks) {
   if (!x || hop2_requests < 1 || hop2_requests > kHop2Requests || hop2_tasks < 1 || hop2_tasks > (int64_t)1 << 30) {
     set_error("mis_limits: need 1 <= hop2_requests <= 65536 and 1 <= hop2_tasks <= 2^30");

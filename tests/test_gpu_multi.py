@@ -215,6 +215,44 @@ def test_autotune_keeps_colours_and_picks_a_setting(tmp_path):
         assert np.array_equal(eng.download(sa.STATE_COLOR), want)
 
 
+def test_mis_autotune_keeps_membership_and_picks_a_setting(tmp_path):
+    """The MIS autotuner times the staged multi-instance batch under each launch setting, leaves
+    the membership as it was, and the repair that follows (with the kept setting) equals the
+    oracle's, on this batch and the next."""
+    sizes = [300, 1200, 2500, 700]
+    say(f"mis autotune: {len(sizes)} BA instances {sizes}")
+    graphs = _instances(tmp_path, sizes, 4, 5)
+    union, base = _union(graphs)
+    oracles = [OracleGraph(n=g.n, off=g.off, col=g.col) for g in graphs]
+    mis_o = [og.greedy_mis() for og in oracles]
+    rng = np.random.default_rng(8)
+    edges = _edge_sets(graphs)
+    max_csr = 150000
+    with sa.Engine(0) as eng:
+        eng.load_graph(union)
+        eng.mis_greedy()
+        for b in range(2):
+            per = [_toggles(rng, g.n, edges[i], int(rng.integers(20, 400))) for i, g in enumerate(graphs)]
+            uv = np.concatenate([l + base[i] for i, l in enumerate(per)])
+            off = np.cumsum([0] + [len(l) for l in per])
+            eng.stage(uv)
+            eng.classify()
+            eng.apply()
+            if b == 0:
+                before = eng.download(sa.STATE_MIS)
+                best = eng.mis_autotune(off, max_csr)
+                cfg = eng.mis_get_config()
+                say(f"mis autotune: {cfg}, {best:.3f} ms")
+                assert best > 0 and cfg["grid"] in (512, 1024, 2048, 4096) and cfg["group_rounds"] in (4, 8, 16, 32)
+                assert np.array_equal(eng.download(sa.STATE_MIS), before)
+            eng.repair_mis_multi(off, max_csr)
+            for i, l in enumerate(per):
+                ins = oracles[i].classify(l)
+                oracles[i].apply(l, ins)
+                mis_o[i], _ = oracles[i].mis_batch(l, ins, mis_o[i], max_csr)
+            assert np.array_equal(eng.download(sa.STATE_MIS), np.concatenate(mis_o)), f"batch {b}"
+
+
 @pytest.mark.parametrize("max_csr,limits", [(300000, None), (10 ** 9, None), (10 ** 9, (2, 16))])
 def test_mis_hop2_searches_equal_oracle(max_csr, limits):
     """Degree-biased toggles on an R-MAT graph make many insertions between two members whose

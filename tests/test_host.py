@@ -20,6 +20,22 @@ def test_library_exports_every_declared_symbol():
     assert missing == []
 
 
+def test_graph_info_callers_size_their_buffer_from_the_header():
+    """scout_graph_info writes SCOUT_GRAPH_INFO_COUNT entries: the binding and every binary that
+    calls it size their buffer from that count (a shorter array is overwritten past its end)."""
+    import re
+    import scout_amd
+    header = open(os.path.join(REPO, "include", "scout_hip.h")).read()
+    count = int(re.search(r"SCOUT_GRAPH_INFO_COUNT = (\d+)", header).group(1))
+    assert scout_amd.GRAPH_INFO_COUNT == count
+    bins = os.path.join(REPO, "scout_amd", "csrc", "bin")
+    for name in sorted(os.listdir(bins)):
+        text = open(os.path.join(bins, name)).read()
+        for m in re.finditer(r"scout_graph_info\(ctx, (\w+)\)", text):
+            decl = re.search(r"int64_t " + m.group(1) + r"\[([^\]]+)\]", text)
+            assert decl and decl.group(1) == "SCOUT_GRAPH_INFO_COUNT", f"{name}: {m.group(0)}"
+
+
 @pytest.mark.parametrize("prog", ["scout_bfs", "scout_cc", "scout_coloring", "scout_mis"])
 def test_binaries_print_usage_with_too_few_arguments(prog):
     r = subprocess.run([os.path.join(BIN_DIR, prog), "a", "b"], capture_output=True, text=True)
