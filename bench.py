@@ -445,6 +445,82 @@ def run_cc_crossover(args):
           flush=True)
 
 
+def run_storage(args):
+    """The same BFS(0)+CC stream repaired on one R-MAT graph twice: base entries in HBM, then
+    held in pinned host memory (storage-resident mode, forced by an HBM budget that holds only
+    the offsets and the per-vertex state), where the repair kernels read rows over PCIe and the
+    full CC recompute streams the entries through two device buffers. Reports both rates, the
+    streamed recompute's PCIe rate and whether both runs end in the states of the generating
+    run."""
+    scale = args.scale or 24
+    size = args.batch or 100_000
+    k, warm = args.steps, args.warmup
+    dev = sa.Engine(0)
+    t0 = time.perf_counter()
+    dev.gen_rmat(scale, 16, 0.57, 0.19, 0.19, args.seed)
+    g = dev.download_graph()
+    n = g.n
+    log(f"graph: n={n} entries={len(g.col)} ({time.perf_counter() - t0:.1f}s)")
+    dev.bfs_full(0)
+    dev.cc_full()
+    batches = []
+    for b in range(warm + k):
+        m = dev.gen_batch(size, 0.7, 0.5, args.seed, expected_total=size * (warm + k))
+        batches.append(dev.batch_download(m).copy())
+        dev.classify(download=False)
+        dev.apply()
+        dev.repair_bfs(sa.STRATEGY_LOCAL)
+        dev.repair_cc(sa.STRATEGY_LOCAL)
+    want = (dev.download(sa.STATE_LEVEL), dev.download(sa.STATE_LABEL))
+    dev.close()
+    out = {}
+    for mode, budget in (("hbm", 0), ("host", 80 * (n + 1))):
+        eng = sa.Engine(0, hbm_bytes=budget)
+        t0 = time.perf_counter()
+        eng.load_graph(g)
+        eng.sync()
+        load_s = time.perf_counter() - t0
+        info = eng.graph_info()
+        assert info["host_resident"] == (mode == "host"), info
+        t0 = time.perf_counter()
+        eng.bfs_full(0)
+        eng.cc_full()
+        eng.sync()
+        init_s = time.perf_counter() - t0
+        info = eng.graph_info()
+        lines = 0
+        for b, uv in enumerate(batches):
+            if b == warm:
+                eng.sync()
+                t0 = time.perf_counter()
+            eng.stage(uv)
+            eng.classify(download=False)
+            eng.apply()
+            eng.repair_bfs(sa.STRATEGY_LOCAL)
+            eng.repair_cc(sa.STRATEGY_LOCAL)
+            if b >= warm:
+                lines += len(uv)
+        eng.sync()
+        elapsed = time.perf_counter() - t0
+        st = eng.stage_info()
+        equal = bool(np.array_equal(eng.download(sa.STATE_LEVEL), want[0]) and
+                     np.array_equal(eng.download(sa.STATE_LABEL), want[1]))
+        row = {"updates_per_s": round(lines / elapsed, 1), "ms_per_batch": round(elapsed / k * 1e3, 3),
+               "load_s": round(load_s, 2), "initial_bfs_cc_s": round(init_s, 3), "equal": equal,
+               "last_batch_h2d": st}
+        if mode == "host":
+            row["cc_full_streamed_bytes"] = info["fetch_bytes"]
+            row["cc_full_stream_us"] = info["fetch_us"]
+            row["cc_full_pcie_GBps"] = round(info["fetch_bytes"] / max(info["fetch_us"], 1) / 1e3, 2)
+        log(f"{mode}: {row}")
+        out[mode] = row
+        eng.close()
+    print(json.dumps({"storage_resident": out, "scale": scale, "vertices": n, "adjacency_entries": len(g.col),
+                      "batch": size, "steps": k, "warmup": warm,
+                      "host_over_hbm": round(out["host"]["updates_per_s"] / out["hbm"]["updates_per_s"], 4)}),
+          flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -471,6 +547,9 @@ def main():
                     help="only summarise rocprofv3 --pmc passes under DIR (copy_fetch, copy_write, bench_fetch, "
                          "bench_write) into per-batch HBM traffic per kernel family, print it and exit; the "
                          "per-dispatch figures cover the last --steps batches (the headline replay)")
+    ap.add_argument("--storage", action="store_true",
+                    help="only compare HBM-resident and host-resident (storage mode) base entries on one stream "
+                         "and exit (R-MAT scale 24, 100k toggles per batch by default)")
     ap.add_argument("--cc-crossover", metavar="SIZES", default=None,
                     help="only measure the CC repair per batch in walk and scan mode for these comma-separated "
                          "batch sizes and exit")
@@ -489,6 +568,9 @@ def main():
         return
     if args.cc_crossover:
         run_cc_crossover(args)
+        return
+    if args.storage:
+        run_storage(args)
         return
     args.scale = args.scale or 26
     args.batch = args.batch or 1_000_000
