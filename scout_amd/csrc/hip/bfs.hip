@@ -978,15 +978,12 @@ int relax_levels(Ctx& c, int32_t* L, int32_t* P, int32_t lvl0, int32_t max_seed,
       int64_t ns = (nseeds > 0 && lvl <= max_seed) ? nseeds : 0;
       const int64_t* sb = (sbuckets && ns > 0) ? sbuckets + (lvl - lvl0) : nullptr;
       k_round_begin<<<1, 1, 0, c.s>>>(bcur, bnext, box + 10, box + 10, 2, box + 15, box + 17, box + 16, ns > 0);
-      SCOUT_CHECK(prof_begin(c, PROF_K_RELAX_SHORT));
-      k_relax_short<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, P, cur, bcur, seeds, ns, sb, box + 15, lvl, nxt,
-                                                    bnext, cap, c.stamp, epoch, c.le.p, box + 11, c.wb.p, box + 10,
-                                                    task_cap, box + 13);
-      SCOUT_CHECK(prof_end(c, PROF_K_RELAX_SHORT, 0));
-      SCOUT_CHECK(prof_begin(c, PROF_K_RELAX_ROWS));
-      k_relax_rows<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, P, c.le.p, box + 11, c.wb.p, box + 10, task_cap, lvl,
-                                                   nxt, bnext, cap, c.stamp, epoch, box + 12, c.d_paths);
-      SCOUT_CHECK(prof_end(c, PROF_K_RELAX_ROWS, 0));
+      SCOUT_CHECK(prof_launch(c, PROF_K_RELAX_SHORT, k_relax_short, dim3(kListGrid), dim3(kBlock), c.g.view(), L, P,
+                              cur, bcur, seeds, ns, sb, box + 15, lvl, nxt, bnext, cap, c.stamp, epoch, c.le.p,
+                              box + 11, c.wb.p, box + 10, task_cap, box + 13));
+      SCOUT_CHECK(prof_launch(c, PROF_K_RELAX_ROWS, k_relax_rows, dim3(kListGrid), dim3(kBlock), c.g.view(), L, P,
+                              c.le.p, box + 11, c.wb.p, box + 10, task_cap, lvl, nxt, bnext, cap, c.stamp, epoch,
+                              box + 12, c.d_paths));
       lvl++;
       std::swap(cur, nxt);
       p ^= 1;
@@ -1154,23 +1151,18 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, int32_t* P, int strategy, scou
         k_round_begin<<<1, 1, 0, c.s>>>(bcur, bnext, box + 14, box + 12, 3, box + 18, box + 20, box + 19,
                                         nlevel > 0);
         const int64_t* cb = (cbuckets && nlevel > 0) ? cbuckets + (lvl - cmin) : nullptr;
-        SCOUT_CHECK(prof_begin(c, PROF_K_INVALIDATE_SHORT));
-        k_invalidate_short<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, P, cur, bcur, cand, nlevel, cb, box + 18,
-                                                           lvl, c.ld.p, binval, nxt, bnext, cap, c.stamp, sepoch,
-                                                           c.le.p, box + 12, c.lf.p, box + 13, c.wb.p, box + 14,
-                                                           c.wb.cap, box + 23, ab);
-        SCOUT_CHECK(prof_end(c, PROF_K_INVALIDATE_SHORT, 0));
+        SCOUT_CHECK(prof_launch(c, PROF_K_INVALIDATE_SHORT, k_invalidate_short, dim3(kListGrid), dim3(kBlock),
+                                c.g.view(), L, P, cur, bcur, cand, nlevel, cb, box + 18, lvl, c.ld.p, binval, nxt,
+                                bnext, cap, c.stamp, sepoch, c.le.p, box + 12, c.lf.p, box + 13, c.wb.p, box + 14,
+                                c.wb.cap, box + 23, ab));
         int32_t supepoch = next_epoch(c);
-        SCOUT_CHECK(prof_begin(c, PROF_BFS_INVALIDATE_KERNEL));
-        k_invalidate<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, P, c.le.p, box + 12, c.lf.p, c.wb.p, box + 14,
-                                                     c.wb.cap, c.aux, supepoch, lvl, c.ld.p, binval, cap, nxt, bnext,
-                                                     cap, c.stamp, sepoch, bscan, box + 21, c.d_paths, ab);
-        SCOUT_CHECK(prof_end(c, PROF_BFS_INVALIDATE_KERNEL, 0));
-        SCOUT_CHECK(prof_begin(c, PROF_K_INVAL_PUSH));
-        k_inval_push<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, P, lvl, c.lf.p, c.wb.p, box + 14, c.wb.cap, c.aux,
-                                                     supepoch, c.ld.p, binval, cap, nxt, bnext, cap, c.stamp, sepoch,
-                                                     box + 15, c.d_paths, ab);
-        SCOUT_CHECK(prof_end(c, PROF_K_INVAL_PUSH, 0));
+        SCOUT_CHECK(prof_launch(c, PROF_BFS_INVALIDATE_KERNEL, k_invalidate, dim3(kListGrid), dim3(kBlock),
+                                c.g.view(), L, P, c.le.p, box + 12, c.lf.p, c.wb.p, box + 14, c.wb.cap, c.aux,
+                                supepoch, lvl, c.ld.p, binval, cap, nxt, bnext, cap, c.stamp, sepoch, bscan,
+                                box + 21, c.d_paths, ab));
+        SCOUT_CHECK(prof_launch(c, PROF_K_INVAL_PUSH, k_inval_push, dim3(kListGrid), dim3(kBlock), c.g.view(), L, P,
+                                lvl, c.lf.p, c.wb.p, box + 14, c.wb.cap, c.aux, supepoch, c.ld.p, binval, cap, nxt,
+                                bnext, cap, c.stamp, sepoch, box + 15, c.d_paths, ab));
         lvl++;
         std::swap(cur, nxt);
         p ^= 1;

@@ -615,10 +615,8 @@ int cc_walk(Ctx& c, Channel& ch, const int32_t* L, scout_stats* st) {
   int64_t rounds = 0;
   while (true) {
     for (int k = 0; k < group; k++) {
-      SCOUT_CHECK(prof_begin(c, PROF_K_CC_EXPLORE));
-      k_cc_explore<<<kExploreGrid, kBlock, 0, c.s>>>(c.g.view(), L, c.label, c.stamp, ep, par, c.la.p, c.lc.p, box,
-                                                      cap, c.d_paths);
-      SCOUT_CHECK(prof_end(c, PROF_K_CC_EXPLORE, 0));
+      SCOUT_CHECK(prof_launch(c, PROF_K_CC_EXPLORE, k_cc_explore, dim3(kExploreGrid), dim3(kBlock), c.g.view(), L,
+                              c.label, c.stamp, ep, par, c.la.p, c.lc.p, box, cap, c.d_paths));
     }
     rounds += group;
     SCOUT_CHECK(box_read(c, 13));
@@ -684,9 +682,8 @@ int cc_scan(Ctx& c, Channel& ch, const int32_t* L, scout_stats* st) {
   SCOUT_CHECK(prof_begin(c, PROF_CC_EXPAND));
   k_zero_u32<<<grid_for(words + cwords), kBlock, 0, c.s>>>(bits, words + cwords);
   if (ch.m > 0) k_cc_mark<<<grid_for(2 * ch.m), kBlock, 0, c.s>>>(ch.d_uv.p, ch.m, L, c.label, bits, coarse);
-  SCOUT_CHECK(prof_begin(c, PROF_K_CC_COLLECT));
-  k_cc_collect<<<kListGrid, kBlock, 0, c.s>>>(L, c.label, bits, coarse, n, par, c.la.p, box, (int64_t)n + 1, box + 4);
-  SCOUT_CHECK(prof_end(c, PROF_K_CC_COLLECT, 0));
+  SCOUT_CHECK(prof_launch(c, PROF_K_CC_COLLECT, k_cc_collect, dim3(kListGrid), dim3(kBlock), L, c.label, bits, coarse,
+                          n, par, c.la.p, box, (int64_t)n + 1, box + 4));
   k_cc_union_list<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), c.la.p, box, par, c.lb.p, box + 5, (int64_t)n + 1,
                                                   box + 2);
   k_cc_unreached_hook<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), c.lb.p, box + 5, par, c.wb.p, box + 1, c.wb.cap,

@@ -189,6 +189,16 @@ int prof_end(Ctx& c, int id, int64_t bytes) {
   return SCOUT_OK;
 }
 
+bool prof_kernel_events(Ctx& c, int id, hipEvent_t* start, hipEvent_t* stop) {
+  if (!c.prof || !c.prof->on) return false;
+  hipEvent_t a = c.prof->get(), b = c.prof->get();
+  if (!a || !b) return false;
+  c.prof->e[id].spans.emplace_back(a, b);
+  *start = a;
+  *stop = b;
+  return true;
+}
+
 // Adds algorithmic bytes to a kernel family after the counts were read back.
 void prof_bytes(Ctx& c, int id, int64_t bytes) {
   if (c.prof && c.prof->on) c.prof->e[id].bytes += bytes;

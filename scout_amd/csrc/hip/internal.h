@@ -16,6 +16,7 @@
 // inserted row -- the order of core/FlashGraph.h:246-265.
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -350,6 +351,20 @@ void color_release(Ctx& c);
 int prof_begin(Ctx& c, int id);
 int prof_end(Ctx& c, int id, int64_t bytes);
 void prof_bytes(Ctx& c, int id, int64_t bytes);
+// A span for one kernel launch: two events stamped by the dispatch itself (through
+// hipExtLaunchKernelGGL), so the span is the kernel's execution and not the gap before it.
+// False (no events) when profiling is off.
+bool prof_kernel_events(Ctx& c, int id, hipEvent_t* start, hipEvent_t* stop);
+// Launches kernel<<<grid, block, 0, c.s>>>(args...), timed as span `id` while profiling.
+template <typename F, typename... Args>
+int prof_launch(Ctx& c, int id, F kernel, dim3 grid, dim3 block, Args... args) {
+  hipEvent_t a = nullptr, b = nullptr;
+  if (prof_kernel_events(c, id, &a, &b))
+    hipExtLaunchKernelGGL(kernel, grid, block, 0, c.s, a, b, 0, args...);
+  else
+    kernel<<<grid, block, 0, c.s>>>(args...);
+  return SCOUT_OK;
+}
 void prof_release(Ctx& c);
 enum ProfId { PROF_BFS_RELAX = 0, PROF_BFS_INVALIDATE, PROF_CC_EXPAND, PROF_APPLY, PROF_CLASSIFY,
               PROF_COLOR, PROF_MIS_EVAL, PROF_CC_HOOK, PROF_BFS_INVALIDATE_KERNEL, PROF_K_RELAX_SHORT,
