@@ -103,7 +103,11 @@ int scout_mem_info(scout_ctx* ctx, int64_t* free_bytes, int64_t* total_bytes);
  * over PCIe, and only the offsets, the inserted rows and the per-vertex state go to HBM; the
  * full CC recompute then streams the entries in chunks through two device buffers, the copy of
  * chunk k+1 overlapping the hooking of chunk k (src/cc/scout_cc.cu:86-215). Results are the
- * same in both modes. Snapshots and on-device compaction are not available in this mode. */
+ * same in both modes. Snapshots and on-device compaction are not available in this mode.
+ * Precondition: the CSR is symmetric (every entry u -> v has its v -> u), as the reference's
+ * generated graphs are; the repairs read one endpoint's row per pair and the MIS member counts
+ * assume both directions, so a one-direction graph gives answers that differ from the
+ * reference's. */
 int scout_graph_load(scout_ctx* ctx, int32_t n, int64_t entries, const void* nindex, int wide,
                      const int32_t* nlist);
 /* out[0..7] = n, base entries, inserted-row entries, logical entries, rows sorted (0/1),
