@@ -292,7 +292,8 @@ int scout_batch_append(scout_ctx* ctx, int channel, int slot, int64_t* m);
  * and labels as staging the segments as separate batches (parse_batch_file +
  * processBatch's apply loop, core/cli_setup.h:87-103, per batch). The repairs that follow
  * treat all segments as one batch, which gives the same solution as repairing after each.
- * ins (may be NULL) receives every line's label. */
+ * When no pair occurs twice among all the lines, the segments are classified and applied in
+ * one pass, with the same result. ins (may be NULL) receives every line's label. */
 int scout_apply_segments(scout_ctx* ctx, int channel, int32_t nseg, const int64_t* seg_off, int rule,
                          int conditional, uint8_t* ins);
 /* Device copies of the graph and solutions, to replay one stream under several strategies. */

@@ -597,9 +597,20 @@ int scout_apply_segments(scout_ctx* x, int channel, int32_t nseg, const int64_t*
     }
   SCOUT_TRY(hipStreamWaitEvent(c.s, ch->staged, 0));
   c.apply_seq++;  // one commit: the repairs that follow see one batch of all segments
-  for (int32_t k = 0; k < nseg; k++) {
-    SCOUT_CHECK(graph_classify_lines(c, *ch, seg_off[k], seg_off[k + 1], rule));
-    SCOUT_CHECK(graph_apply_lines(c, *ch, seg_off[k], seg_off[k + 1], conditional));
+  // When no pair occurs twice in the whole range, no segment touches a pair of an earlier one:
+  // each line's pair stands in the graph as it stood before the segments, so one
+  // classification and one application of all lines (inserted entries in line order) equal
+  // the segments applied one by one. Otherwise the segments go one by one.
+  int64_t repeated = 1;
+  if (nseg > 1) SCOUT_CHECK(graph_repeated_pairs(c, *ch, 0, ch->m, &repeated));
+  if (nseg > 1 && repeated == 0) {
+    SCOUT_CHECK(graph_classify_lines(c, *ch, 0, ch->m, rule));
+    SCOUT_CHECK(graph_apply_lines(c, *ch, 0, ch->m, conditional));
+  } else {
+    for (int32_t k = 0; k < nseg; k++) {
+      SCOUT_CHECK(graph_classify_lines(c, *ch, seg_off[k], seg_off[k + 1], rule));
+      SCOUT_CHECK(graph_apply_lines(c, *ch, seg_off[k], seg_off[k + 1], conditional));
+    }
   }
   SCOUT_CHECK(channel_used(c.s, *ch));
   if (ins && ch->m > 0) {

@@ -746,13 +746,12 @@ int graph_classify_lines(Ctx& c, Channel& ch, int64_t lo, int64_t hi, int rule) 
 
 int graph_classify(Ctx& c, Channel& ch, int rule) { return graph_classify_lines(c, ch, 0, ch.m, rule); }
 
-// Applies lines [lo, hi) of the staged batch; pairs repeated inside the range are applied as
-// runs of distinct pairs, in line order.
-int graph_apply_lines(Ctx& c, Channel& ch, int64_t lo, int64_t hi, int conditional) {
+// *repeated = the lines of [lo, hi) whose unordered pair an earlier line of the range holds.
+int graph_repeated_pairs(Ctx& c, Channel& ch, int64_t lo, int64_t hi, int64_t* repeated) {
   DevGraph& g = c.g;
   int64_t m = hi - lo;
+  *repeated = 0;
   if (m <= 0) return SCOUT_OK;
-  // Count the lines whose pair occurs more than once in the range.
   uint64_t size = 1;
   while (size < (uint64_t)(4 * m)) size <<= 1;
   SCOUT_CHECK(g.hkey.reserve((int64_t)size));
@@ -760,6 +759,19 @@ int graph_apply_lines(Ctx& c, Channel& ch, int64_t lo, int64_t hi, int condition
   SCOUT_TRY(hipMemsetAsync(c.d_box, 0, sizeof(int64_t), c.s));
   k_dup_count<<<grid_for(m), kBlock, 0, c.s>>>(ch.d_uv.p + 2 * lo, m, g.hkey.p, size - 1, c.d_box);
   SCOUT_CHECK(box_read(c, 1));
+  *repeated = c.h_box[0];
+  return SCOUT_OK;
+}
+
+// Applies lines [lo, hi) of the staged batch; pairs repeated inside the range are applied as
+// runs of distinct pairs, in line order.
+int graph_apply_lines(Ctx& c, Channel& ch, int64_t lo, int64_t hi, int conditional) {
+  int64_t m = hi - lo;
+  if (m <= 0) return SCOUT_OK;
+  // Count the lines whose pair occurs more than once in the range.
+  int64_t repeated = 0;
+  SCOUT_CHECK(graph_repeated_pairs(c, ch, lo, hi, &repeated));
+  c.h_box[0] = repeated;
   ch.pairs_distinct = lo == 0 && hi == ch.m && c.h_box[0] == 0;
   if (c.h_box[0] == 0) return apply_range(c, ch, lo, hi, conditional);
   // Repeated pairs: cut the range into runs of distinct pairs, applied in order.

@@ -290,6 +290,7 @@ int graph_classify(Ctx& c, Channel& ch, int rule);
 int graph_apply(Ctx& c, Channel& ch, int conditional);
 int graph_classify_lines(Ctx& c, Channel& ch, int64_t lo, int64_t hi, int rule);
 int graph_apply_lines(Ctx& c, Channel& ch, int64_t lo, int64_t hi, int conditional);
+int graph_repeated_pairs(Ctx& c, Channel& ch, int64_t lo, int64_t hi, int64_t* repeated);
 int graph_logical_degrees(Ctx& c, int64_t* d_deg);
 int graph_download(Ctx& c, int64_t* h_off, int32_t* h_col);
 int graph_compact(Ctx& c);
