@@ -162,7 +162,10 @@ constexpr int kApplyLines = 4;
 // move within a batch; only their deletion marks flip).
 __global__ void k_apply(GView g, bool sorted, const int32_t* uv, const uint8_t* ins, const int64_t* pos, int64_t lo,
                         int64_t hi, int conditional, uint8_t* done, uint64_t* nkey, int32_t* ndst, int64_t ncap,
-                        int ks, int64_t* box) {
+                        int ks, int64_t* box, const int64_t* skip) {
+  // the range repeats a pair (*skip, counted by k_dup_count just before): nothing is applied
+  // here, and the host applies the range as runs of distinct pairs instead
+  if (skip && *skip != 0) return;
   // kApplyLines lines per thread (each pass coalesced over the block): one block-wide append
   // of the new entries per block
   constexpr int K = kApplyLines;
@@ -544,7 +547,13 @@ int insert_new(Ctx& c, int64_t cnt_new, int ks) {
   return SCOUT_OK;
 }
 
-int apply_range(Ctx& c, Channel& ch, int64_t lo, int64_t hi, int conditional) {
+// Mailbox slot of the repeated-pair count that k_apply checks on the device.
+constexpr int kRepeatSlot = 48;
+
+// Applies lines [lo, hi). With `skip` (a device count of repeated pairs in the range, read back
+// with the new-entry count), nothing is applied when the count is not zero, and *skipped says so.
+int apply_range(Ctx& c, Channel& ch, int64_t lo, int64_t hi, int conditional, const int64_t* skip = nullptr,
+                bool* skipped = nullptr) {
   DevGraph& g = c.g;
   int64_t cap = 4 * (hi - lo) + 16;
   SCOUT_CHECK(g.nkey.reserve(cap));
@@ -556,11 +565,16 @@ int apply_range(Ctx& c, Channel& ch, int64_t lo, int64_t hi, int conditional) {
   if (hi > lo)
     k_apply<<<grid_for(hi - lo, kBlock * kApplyLines), kBlock, 0, c.s>>>(g.view(), g.sorted, ch.d_uv.p, ch.d_ins.p,
                                                                          pos, lo, hi, conditional, ch.d_done.p,
-                                                                         g.nkey.p, g.ndst.p, g.nkey.cap, ks, c.d_box);
+                                                                         g.nkey.p, g.ndst.p, g.nkey.cap, ks, c.d_box,
+                                                                         skip);
   // algorithmic bytes per line: the pair and its label and done flag (10 B), both rows'
   // base and inserted offsets (32 B) and the entries flipped or appended (8 B)
   SCOUT_CHECK(prof_end(c, PROF_APPLY, 50 * (hi - lo)));
-  SCOUT_CHECK(box_read(c, 2));
+  SCOUT_CHECK(box_read(c, skip ? kRepeatSlot + 1 : 2));
+  if (skip && c.h_box[kRepeatSlot] != 0) {
+    *skipped = true;
+    return SCOUT_OK;
+  }
   int64_t cnt_new = c.h_box[0];
   g.holes += c.h_box[1];
   if (cnt_new > g.nkey.cap) {
@@ -746,18 +760,25 @@ int graph_classify_lines(Ctx& c, Channel& ch, int64_t lo, int64_t hi, int rule) 
 
 int graph_classify(Ctx& c, Channel& ch, int rule) { return graph_classify_lines(c, ch, 0, ch.m, rule); }
 
-// *repeated = the lines of [lo, hi) whose unordered pair an earlier line of the range holds.
-int graph_repeated_pairs(Ctx& c, Channel& ch, int64_t lo, int64_t hi, int64_t* repeated) {
+// Queues the count of the lines of [lo, hi) whose unordered pair an earlier line of the range
+// holds into *out (device).
+int queue_repeat_count(Ctx& c, Channel& ch, int64_t lo, int64_t hi, int64_t* out) {
   DevGraph& g = c.g;
   int64_t m = hi - lo;
-  *repeated = 0;
-  if (m <= 0) return SCOUT_OK;
   uint64_t size = 1;
   while (size < (uint64_t)(4 * m)) size <<= 1;
   SCOUT_CHECK(g.hkey.reserve((int64_t)size));
   SCOUT_TRY(hipMemsetAsync(g.hkey.p, 0xff, size * sizeof(uint64_t), c.s));
-  SCOUT_TRY(hipMemsetAsync(c.d_box, 0, sizeof(int64_t), c.s));
-  k_dup_count<<<grid_for(m), kBlock, 0, c.s>>>(ch.d_uv.p + 2 * lo, m, g.hkey.p, size - 1, c.d_box);
+  SCOUT_TRY(hipMemsetAsync(out, 0, sizeof(int64_t), c.s));
+  if (m > 0) k_dup_count<<<grid_for(m), kBlock, 0, c.s>>>(ch.d_uv.p + 2 * lo, m, g.hkey.p, size - 1, out);
+  return SCOUT_OK;
+}
+
+// *repeated = the lines of [lo, hi) whose unordered pair an earlier line of the range holds.
+int graph_repeated_pairs(Ctx& c, Channel& ch, int64_t lo, int64_t hi, int64_t* repeated) {
+  *repeated = 0;
+  if (hi <= lo) return SCOUT_OK;
+  SCOUT_CHECK(queue_repeat_count(c, ch, lo, hi, c.d_box));
   SCOUT_CHECK(box_read(c, 1));
   *repeated = c.h_box[0];
   return SCOUT_OK;
@@ -768,12 +789,13 @@ int graph_repeated_pairs(Ctx& c, Channel& ch, int64_t lo, int64_t hi, int64_t* r
 int graph_apply_lines(Ctx& c, Channel& ch, int64_t lo, int64_t hi, int conditional) {
   int64_t m = hi - lo;
   if (m <= 0) return SCOUT_OK;
-  // Count the lines whose pair occurs more than once in the range.
-  int64_t repeated = 0;
-  SCOUT_CHECK(graph_repeated_pairs(c, ch, lo, hi, &repeated));
-  c.h_box[0] = repeated;
-  ch.pairs_distinct = lo == 0 && hi == ch.m && c.h_box[0] == 0;
-  if (c.h_box[0] == 0) return apply_range(c, ch, lo, hi, conditional);
+  // The repeated-pair count goes to the device ahead of k_apply, which applies the range only
+  // when no pair repeats; the count comes back with the new-entry count (one read-back).
+  SCOUT_CHECK(queue_repeat_count(c, ch, lo, hi, c.d_box + kRepeatSlot));
+  bool skipped = false;
+  SCOUT_CHECK(apply_range(c, ch, lo, hi, conditional, c.d_box + kRepeatSlot, &skipped));
+  ch.pairs_distinct = !skipped && lo == 0 && hi == ch.m;
+  if (!skipped) return SCOUT_OK;
   // Repeated pairs: cut the range into runs of distinct pairs, applied in order.
   std::vector<int32_t> uv((size_t)(2 * m));
   SCOUT_TRY(hipMemcpy(uv.data(), ch.d_uv.p + 2 * lo, (size_t)(2 * m) * sizeof(int32_t), hipMemcpyDeviceToHost));
