@@ -173,6 +173,55 @@ def test_serving_loop_steps_equal_the_oracle(tmp_path):
         assert np.array_equal(lab, og.cc()), f"step {k}: labels differ"
 
 
+@pytest.mark.parametrize("size", [2000, 20000])
+def test_cc_listings_equal_the_oracle_at_scale_18(size):
+    """R-MAT scale 18 (262k vertices, about 8M entries) with device-generated batches of `size`
+    toggles (70% deletions, half of them BFS-tree edges): the walk listing and the scan listing
+    give the oracle's levels and labels after every batch."""
+    gen = sa.Engine(0)
+    gen.gen_rmat(18, 16, seed=21)
+    gen.bfs_full(0)
+    gen.cc_full()
+    g = gen.download_graph()
+    batches = []
+    for _ in range(4):
+        m = gen.gen_batch(size, 0.7, 0.5, seed=22, expected_total=4 * size)
+        batches.append(gen.batch_download(m).copy())
+        gen.classify(download=False)
+        gen.apply()
+        gen.repair_bfs()
+        gen.repair_cc()
+    gen.close()
+    say(f"[cc scale 18] n={g.n} entries={len(g.col)} batches of {size}")
+    og = OracleGraph(n=g.n, off=g.off, col=g.col)
+    lv0, lab0 = og.bfs(0), og.cc()
+    engines = {}
+    for mode in ("walk", "scan"):
+        e = sa.Engine(0)
+        e.load_graph(g)
+        e.cc_mode(CC_MODES[mode][0])
+        e.upload(sa.STATE_LEVEL, lv0)
+        e.upload(sa.STATE_LABEL, lab0)
+        engines[mode] = e
+    try:
+        for k, uv in enumerate(batches):
+            ins = og.classify(uv)
+            og.apply(uv, ins)
+            want_lv, want_lab = og.bfs(0), og.cc()
+            for mode, e in engines.items():
+                e.stage(uv)
+                e.classify(download=False)
+                e.apply()
+                e.repair_bfs()
+                st = e.repair_cc()
+                assert st.cc_mode == CC_MODES[mode][0]
+                assert np.array_equal(e.download(sa.STATE_LEVEL), want_lv), f"{mode} batch {k}: levels"
+                assert np.array_equal(e.download(sa.STATE_LABEL), want_lab), f"{mode} batch {k}: labels"
+    finally:
+        for e in engines.values():
+            e.close()
+
+
 @pytest.mark.parametrize("mode", ["walk", "scan"])
 @pytest.mark.parametrize("name", ["rmat", "heavy"])
 def test_cc_alone_exact_every_batch(data, name, mode):
