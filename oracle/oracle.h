@@ -4,8 +4,37 @@
  * cpu_baseline leg load it. The product path (libscout_hip.so and the scout_* binaries)
  * never links, loads or calls it.
  *
- * What each function restates (reference file:line) and what pins it is written next to
- * the function in oracle.cpp. Everything is integer arithmetic.
+ * What each function restates (reference file:line) is written next to it below and in
+ * oracle.cpp. Everything is integer arithmetic except the MLP. What pins each function, so
+ * that a plausible mistake fails a check other than a retyped formula (tests/):
+ *   - graph overlay, classify, apply, rows, degrees, logical CSR, compaction: equal to the
+ *     reference's own HybridGraph and compact_flash_graph compiled from its headers
+ *     (test_ref_core.py: test_scratchpad_restatement_equals_reference_hybrid_graph,
+ *     test_compaction_is_byte_identical_to_reference,
+ *     test_inconsistent_delta_fails_compaction_like_reference), plus
+ *     test_scratchpad_row_order_and_toggles, test_scratchpad_toggles_are_symmetric_and_consistent;
+ *   - orc_bfs, orc_cc: scipy shortest_path / connected_components after random updates
+ *     (test_bfs_and_cc_match_scipy_after_random_updates); the sample fixtures
+ *     (test_sample_fixtures_are_closed_forms);
+ *   - orc_check_*: planted mistakes (test_checkers_catch_planted_mistakes);
+ *   - orc_greedy_mis, orc_greedy_color: the sample fixtures and brute-force first fit
+ *     (test_greedy_states_are_first_fit);
+ *   - orc_color_batch: hand-worked rounds (test_coloring_rounds_hand_worked,
+ *     test_coloring_path_insert) and properness on random streams;
+ *   - orc_mis_batch: hand-worked rule cases (test_mis_deletion_*, test_mis_insertion_*), the
+ *     multi-sub-batch cut case (test_mis_sub_batch_cut_decides_the_row_kinds) and
+ *     independence on random streams. The line-order serialisation itself is "parity
+ *     unpinned": the reference's own result is racy, so no reference output pins it;
+ *   - orc_mis_max_csr: the formula with its clamps (test_mis_max_csr_formula);
+ *   - orc_mis_complete: a hand case and maximality / region invariants on random streams
+ *     (test_mis_complete_*);
+ *   - orc_refcc_*: canonical labels equal the closed form every batch with no bound
+ *     (test_per_edit_reference_cc_is_exact_every_batch);
+ *   - orc_refbfs_*: its insertion relax equals the closed form on insertion-only streams
+ *     (test_per_edit_reference_bfs_insertions_are_exact); its deletion worklist is "parity
+ *     unpinned" beyond its code citation (the reference algorithm itself is not exact there);
+ *   - orc_mlp_*, orc_bf16_round: numpy closed forms, central finite differences, a planted
+ *     mistake and round-to-nearest-even cases (test_mlp_oracle_*, test_bf16_round_is_nearest_even).
  */
 #ifndef SCOUT_ORACLE_H
 #define SCOUT_ORACLE_H
