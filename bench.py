@@ -44,7 +44,7 @@ def cpu_baseline(eng, n, batches_host, seconds):
     and full recomputes use OpenMP threads, as the reference's host code does; the per-edit
     logic is sequential, as in the reference. The deferred fallbacks are then settled once, as
     the reference does after its stream, and that time is reported beside the rate."""
-    from oracle_lib import OracleGraph, lib  # the checker's library, timed here as the baseline
+    from oracle_lib import OracleGraph, lib  # the test oracle's library, timed here as the baseline
     g = eng.download_graph()
     lv0, lab0 = eng.download(sa.STATE_LEVEL), eng.download(sa.STATE_LABEL)
     og = OracleGraph(n=n, off=g.off, col=g.col)
@@ -218,7 +218,7 @@ def check_color_mis(eng, n):
 def cpu_baseline_color_mis(eng, n, uv_host, max_csr, seconds):
     """The reference's colouring and MIS batch repair (oracle restatement, one thread) on chunks
     of the first timed batch, until `seconds` of CPU time are used."""
-    from oracle_lib import OracleGraph  # the checker's library, timed here as the baseline
+    from oracle_lib import OracleGraph  # the test oracle's library, timed here as the baseline
     g = eng.download_graph()
     col, mis = eng.download(sa.STATE_COLOR), eng.download(sa.STATE_MIS)
     og = OracleGraph(n=n, off=g.off, col=g.col)

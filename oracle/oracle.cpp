@@ -1,5 +1,5 @@
 // oracle.cpp -- CPU restatement of the SCOUT reference's hot path. TEST INFRASTRUCTURE ONLY:
-// loaded by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg as the checker,
+// loaded by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg as the test oracle,
 // never by the product path.
 //
 // Pinning (what fixes each function independently of this file):

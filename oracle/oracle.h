@@ -1,4 +1,4 @@
-/* oracle.h -- CPU restatement of the SCOUT reference's hot path, used ONLY as the checker.
+/* oracle.h -- CPU restatement of the SCOUT reference's hot path, used ONLY as the test oracle.
  *
  * This is test infrastructure. Only tests/, __graft_entry__.smoke() and bench.py's
  * cpu_baseline leg load it. The product path (libscout_hip.so and the scout_* binaries)
@@ -16,7 +16,7 @@
  *   - orc_bfs, orc_cc: scipy shortest_path / connected_components after random updates
  *     (test_bfs_and_cc_match_scipy_after_random_updates); the sample fixtures
  *     (test_sample_fixtures_are_closed_forms);
- *   - orc_check_*: planted mistakes (test_checkers_catch_planted_mistakes);
+ *   - orc_check_*: planted mistakes (test_validity_checks_catch_planted_mistakes);
  *   - orc_greedy_mis, orc_greedy_color: the sample fixtures and brute-force first fit
  *     (test_greedy_states_are_first_fit);
  *   - orc_color_batch: hand-worked rounds (test_coloring_rounds_hand_worked,

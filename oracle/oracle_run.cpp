@@ -1,4 +1,4 @@
-// oracle_run: the CPU oracle driven over a graph, init file and batch folder (checker and CPU
+// oracle_run: the CPU oracle driven over a graph, init file and batch folder (test oracle and CPU
 // baseline; test infrastructure, never part of the product path).
 //
 //   oracle_run <mis|coloring|bfs|cc> <graph.egr> <init.txt> <batch_dir> <num_batches>

@@ -1,4 +1,4 @@
-"""ctypes binding of oracle/liboracle.so (the CPU restatement used as the checker)."""
+"""ctypes binding of oracle/liboracle.so (the CPU restatement used as the test oracle)."""
 from __future__ import annotations
 
 import ctypes

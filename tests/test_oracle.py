@@ -102,7 +102,7 @@ def test_bfs_and_cc_match_scipy_after_random_updates(seed):
         assert g.check_bfs(g.bfs(0)) == 0
 
 
-def test_checkers_catch_planted_mistakes():
+def test_validity_checks_catch_planted_mistakes():
     rng = np.random.default_rng(7)
     n = 80
     off, col = csr_from_edges(n, random_graph(n, 160, rng))
@@ -433,7 +433,7 @@ def test_per_edit_reference_cc_is_exact_every_batch(sample_dir, seed):
 def test_per_edit_reference_bfs_insertions_are_exact(sample_dir, seed):
     """The per-edit BFS baseline's insertion relax (src/bfs/scout_bfs.cu:460-480) is exact on
     insertion-only streams; its deletion worklist (min neighbour level + 1) is not exact in
-    general, which is why the baseline is timed and not used as a checker."""
+    general, which is why the baseline is timed and not used as a test oracle."""
     from oracle_lib import _p, lib
     egr = os.path.join(sample_dir, "sample.egr")
     rng = np.random.default_rng(seed)
