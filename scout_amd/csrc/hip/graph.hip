@@ -228,6 +228,43 @@ __global__ void k_apply(GView g, bool sorted, const int32_t* uv, const uint8_t* 
 // --- inserted rows: appends into the slack pool ---------------------------------------
 
 // Group heads of the sorted new entries: a group is the run of one source vertex.
+// Counts the new entries of every source (ncnt, zero before).
+__global__ void k_src_claim(const uint64_t* nkey, const int64_t* count, int64_t cap, int ks, int32_t* ncnt) {
+  int64_t cnt = min(*count, cap);
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += (int64_t)gridDim.x * blockDim.x)
+    atomicAdd(&ncnt[nkey[i] >> ks], 1);
+}
+
+// Splits the new entries: a source with one new entry is a group of its own, in line order
+// trivially, and goes to the single list; the entries of sources with several go to the multi
+// list, which is sorted by (source, line) before they are placed. Order inside each list does
+// not matter (the multi keys carry the line).
+__global__ void k_src_split(const uint64_t* nkey, const int32_t* ndst, const int64_t* count, int64_t cap, int ks,
+                            const int32_t* ncnt, uint64_t* skey, int32_t* sdst, int64_t* nsingle, uint64_t* mkey,
+                            int32_t* mdst, int64_t* nmulti) {
+  int64_t cnt = min(*count, cap);
+  // the trip count is uniform across the block, so the block-wide appends see every thread
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < cnt; base += (int64_t)gridDim.x * blockDim.x) {
+    int64_t i = base + threadIdx.x;
+    bool have = i < cnt;
+    uint64_t key = have ? nkey[i] : 0;
+    int32_t dst = have ? ndst[i] : 0;
+    bool single = have && ncnt[key >> ks] == 1;
+    int64_t ps = block_append(single, key, skey, nsingle, cap);
+    if (ps >= 0) sdst[ps] = dst;
+    int64_t pm = block_append(have && !single, key, mkey, nmulti, cap);
+    if (pm >= 0) mdst[pm] = dst;
+  }
+}
+
+__global__ void k_zero_slot(int64_t* p) { *p = 0; }
+
+// Zeroes the counts k_src_claim left (for the next batch).
+__global__ void k_src_release(const uint64_t* nkey, int64_t cnt, int ks, int32_t* ncnt) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < cnt) ncnt[nkey[i] >> ks] = 0;
+}
+
 __global__ void k_group_heads(const uint64_t* nkey, int64_t cnt, int ks, int32_t* heads, int64_t* nheads,
                               int64_t cap) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -264,8 +301,8 @@ __global__ void k_append_groups(int64_t* aoff, int32_t* alen, int32_t* acap, int
   int64_t need = 0, s = 0, e = 0;
   int32_t u = -1, len = 0;
   if (gi < ng) {
-    s = heads[gi];
-    e = group_end(nkey, cnt, ks, s);
+    s = heads ? heads[gi] : gi;  // no head list: every entry is a group of its own
+    e = heads ? group_end(nkey, cnt, ks, s) : s + 1;
     u = (int32_t)(nkey[s] >> ks);
     len = alen[u];
     if (len + (e - s) > acap[u]) need = grown_capacity(len + (e - s));
@@ -309,8 +346,8 @@ __global__ void k_move_rows(int64_t* aoff, int32_t* alen, int32_t* acap, int32_t
   int64_t nm = *nmoves;
   for (int64_t i = global_wave(); i < nm; i += total_waves()) {
     int32_t gi = moves[i];
-    int64_t s = heads[gi];
-    int64_t e = group_end(nkey, cnt, ks, s);
+    int64_t s = heads ? heads[gi] : gi;
+    int64_t e = heads ? group_end(nkey, cnt, ks, s) : s + 1;
     int32_t u = (int32_t)(nkey[s] >> ks);
     int64_t len = alen[u], k = e - s, old = aoff[u], pos = move_pos[gi];
     for (int64_t j = lane_id(); j < len; j += kWave) {
@@ -504,16 +541,21 @@ int bits_for(int64_t x) {
   return b;
 }
 
-int insert_new(Ctx& c, int64_t cnt_new, int ks) {
+// Mailbox slots of the repeated-pair count that k_apply checks on the device, and of the
+// single and multi entry counts of k_src_split.
+constexpr int kRepeatSlot = 48, kSingleSlot = 50, kMultiSlot = 51;
+
+// Places the batch's new inserted-row entries as split by k_src_split: ns whose source has no
+// other (nkey2/ndst2, counted in box[kSingleSlot]) and nm whose source has several (mkey/mdst).
+int insert_new(Ctx& c, int64_t cnt_new, int ks, int64_t ns, int64_t nm) {
   DevGraph& g = c.g;
-  // Stable by (src, line): the new entries of a row keep the batch's line order. Keys are
-  // src << ks | line offset, so only their low ks + bits(n - 1) bits are sorted.
-  SCOUT_CHECK(g.nkey2.reserve(cnt_new));
-  SCOUT_CHECK(g.ndst2.reserve(cnt_new));
-  SCOUT_CHECK(sort_pairs_u64(c, g.nkey.p, g.nkey2.p, g.ndst.p, g.ndst2.p, cnt_new,
-                             ks + bits_for(std::max<int64_t>(g.n - 1, 0))));
-  std::swap(g.nkey, g.nkey2);
-  std::swap(g.ndst, g.ndst2);
+  if (nm > 0) {
+    // Stable by (src, line): the new entries of a row keep the batch's line order. Keys are
+    // src << ks | line offset, so only their low ks + bits(n - 1) bits are sorted. The sorted
+    // entries go to nkey/ndst, whose unsplit contents are no longer needed.
+    SCOUT_CHECK(sort_pairs_u64(c, g.mkey.p, g.nkey.p, g.mdst.p, g.ndst.p, nm,
+                               ks + bits_for(std::max<int64_t>(g.n - 1, 0))));
+  }
   // Worst case: every touched row moves to twice its new size, and the rows hold at most the
   // pool's used part: 2 * (A + cnt_new) + 4 per group <= 2 * A + 6 * cnt_new.
   int64_t worst = 2 * g.A + 6 * cnt_new + 1024;
@@ -528,13 +570,27 @@ int insert_new(Ctx& c, int64_t cnt_new, int ks) {
   // box: [2] groups, [3] moved rows, [4] pool top, [5] overflow
   int64_t init[4] = {0, 0, g.A, 0};
   SCOUT_TRY(hipMemcpyAsync(c.d_box + 2, init, sizeof(init), hipMemcpyHostToDevice, c.s));
-  k_group_heads<<<grid_for(cnt_new), kBlock, 0, c.s>>>(g.nkey.p, cnt_new, ks, g.heads.p, c.d_box + 2, g.heads.cap);
-  k_append_groups<<<grid_for(cnt_new), kBlock, 0, c.s>>>(g.aoff.p, g.alen.p, g.acap.p, g.acol.p, g.aown.p, g.nkey.p,
-                                                         g.ndst.p, cnt_new, ks, g.heads.p, c.d_box + 2, c.d_box + 4,
-                                                         pool_cap(g), g.moves.p, c.d_box + 3, g.move_pos.p,
-                                                         c.d_box + 5);
-  k_move_rows<<<kListGrid, kBlock, 0, c.s>>>(g.aoff.p, g.alen.p, g.acap.p, g.acol.p, g.aown.p, g.nkey.p, g.ndst.p,
-                                             cnt_new, ks, g.heads.p, g.moves.p, c.d_box + 3, g.move_pos.p);
+  // the single entries: each one a group of its own (no head list)
+  if (ns > 0) {
+    k_append_groups<<<grid_for(ns), kBlock, 0, c.s>>>(g.aoff.p, g.alen.p, g.acap.p, g.acol.p, g.aown.p, g.nkey2.p,
+                                                      g.ndst2.p, ns, ks, nullptr, c.d_box + kSingleSlot, c.d_box + 4,
+                                                      pool_cap(g), g.moves.p, c.d_box + 3, g.move_pos.p, c.d_box + 5);
+    k_move_rows<<<kListGrid, kBlock, 0, c.s>>>(g.aoff.p, g.alen.p, g.acap.p, g.acol.p, g.aown.p, g.nkey2.p,
+                                               g.ndst2.p, ns, ks, nullptr, g.moves.p, c.d_box + 3, g.move_pos.p);
+  }
+  // the sorted multi entries: one group per source (their sources differ from the singles')
+  if (nm > 0) {
+    k_zero_slot<<<1, 1, 0, c.s>>>(c.d_box + 3);
+    k_group_heads<<<grid_for(nm), kBlock, 0, c.s>>>(g.nkey.p, nm, ks, g.heads.p, c.d_box + 2, g.heads.cap);
+    k_append_groups<<<grid_for(nm), kBlock, 0, c.s>>>(g.aoff.p, g.alen.p, g.acap.p, g.acol.p, g.aown.p, g.nkey.p,
+                                                      g.ndst.p, nm, ks, g.heads.p, c.d_box + 2, c.d_box + 4,
+                                                      pool_cap(g), g.moves.p, c.d_box + 3, g.move_pos.p, c.d_box + 5);
+    k_move_rows<<<kListGrid, kBlock, 0, c.s>>>(g.aoff.p, g.alen.p, g.acap.p, g.acol.p, g.aown.p, g.nkey.p,
+                                               g.ndst.p, nm, ks, g.heads.p, g.moves.p, c.d_box + 3, g.move_pos.p);
+  }
+  // the counts of k_src_claim go back to zero for the next batch
+  if (ns > 0) k_src_release<<<grid_for(ns), kBlock, 0, c.s>>>(g.nkey2.p, ns, ks, g.ncnt.p);
+  if (nm > 0) k_src_release<<<grid_for(nm), kBlock, 0, c.s>>>(g.nkey.p, nm, ks, g.ncnt.p);
   SCOUT_TRY(hipMemcpyAsync(c.h_box + 2, c.d_box + 2, 4 * sizeof(int64_t), hipMemcpyDeviceToHost, c.s));
   SCOUT_TRY(hipStreamSynchronize(c.s));
   c.readbacks++;
@@ -546,9 +602,6 @@ int insert_new(Ctx& c, int64_t cnt_new, int ks) {
   g.live += cnt_new;
   return SCOUT_OK;
 }
-
-// Mailbox slot of the repeated-pair count that k_apply checks on the device.
-constexpr int kRepeatSlot = 48;
 
 // Applies lines [lo, hi). With `skip` (a device count of repeated pairs in the range, read back
 // with the new-entry count), nothing is applied when the count is not zero, and *skipped says so.
@@ -570,7 +623,22 @@ int apply_range(Ctx& c, Channel& ch, int64_t lo, int64_t hi, int conditional, co
   // algorithmic bytes per line: the pair and its label and done flag (10 B), both rows'
   // base and inserted offsets (32 B) and the entries flipped or appended (8 B)
   SCOUT_CHECK(prof_end(c, PROF_APPLY, 50 * (hi - lo)));
-  SCOUT_CHECK(box_read(c, skip ? kRepeatSlot + 1 : 2));
+  if (!g.ncnt.p || g.ncnt.cap < (int64_t)g.n + 1) {
+    g.ncnt.release();
+    SCOUT_CHECK(g.ncnt.reserve((int64_t)g.n + 1));
+    SCOUT_TRY(hipMemsetAsync(g.ncnt.p, 0, (size_t)g.ncnt.cap * sizeof(int32_t), c.s));
+  }
+  SCOUT_CHECK(g.nkey2.reserve(g.nkey.cap));
+  SCOUT_CHECK(g.ndst2.reserve(g.nkey.cap));
+  SCOUT_CHECK(g.mkey.reserve(g.nkey.cap));
+  SCOUT_CHECK(g.mdst.reserve(g.nkey.cap));
+  k_zero_slot<<<1, 1, 0, c.s>>>(c.d_box + kSingleSlot);
+  k_zero_slot<<<1, 1, 0, c.s>>>(c.d_box + kMultiSlot);
+  k_src_claim<<<kListGrid, kBlock, 0, c.s>>>(g.nkey.p, c.d_box, g.nkey.cap, ks, g.ncnt.p);
+  k_src_split<<<kListGrid, kBlock, 0, c.s>>>(g.nkey.p, g.ndst.p, c.d_box, g.nkey.cap, ks, g.ncnt.p, g.nkey2.p,
+                                              g.ndst2.p, c.d_box + kSingleSlot, g.mkey.p, g.mdst.p,
+                                              c.d_box + kMultiSlot);
+  SCOUT_CHECK(box_read(c, kMultiSlot + 1));
   if (skip && c.h_box[kRepeatSlot] != 0) {
     *skipped = true;
     return SCOUT_OK;
@@ -581,7 +649,7 @@ int apply_range(Ctx& c, Channel& ch, int64_t lo, int64_t hi, int conditional, co
     set_error("apply: inserted-entry staging overflow");
     return SCOUT_ERR_STATE;
   }
-  if (cnt_new > 0) SCOUT_CHECK(insert_new(c, cnt_new, ks));
+  if (cnt_new > 0) SCOUT_CHECK(insert_new(c, cnt_new, ks, c.h_box[kSingleSlot], c.h_box[kMultiSlot]));
   if (g.holes > (g.live >> 1) + (1 << 20)) SCOUT_CHECK(repack_inserted(c, 2 * g.live + (1 << 20)));
   return SCOUT_OK;
 }
@@ -619,6 +687,9 @@ int graph_free(Ctx& c) {
   g.ndst.release();
   g.ndst2.release();
   g.hkey.release();
+  g.ncnt.release();
+  g.mkey.release();
+  g.mdst.release();
   g.bsamp.release();
   g.n = 0;
   g.E = g.A = g.live = g.holes = 0;

@@ -138,10 +138,11 @@ struct DevGraph {
   DevBuf<int32_t> heads, moves;
   DevBuf<int64_t> move_pos;
   // new inserted-row entries of the current batch: key = src << 32 | line
-  DevBuf<uint64_t> nkey, nkey2;
-  DevBuf<int32_t> ndst, ndst2;
+  DevBuf<uint64_t> nkey, nkey2, mkey;
+  DevBuf<int32_t> ndst, ndst2, mdst;
   // duplicate-pair detection of the current batch
   DevBuf<uint64_t> hkey;
+  DevBuf<int32_t> ncnt;  // per vertex: new inserted-row entries of the batch being applied (0 between batches)
   // base entry samples and the base generation they were taken from
   DevBuf<int32_t> bsamp;
   int64_t bsamp_gen = -1;
