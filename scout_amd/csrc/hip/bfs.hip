@@ -71,14 +71,16 @@ __device__ __forceinline__ void gather_short(const GView& g, const Row& r, int64
   for (int k = 0; k < kShort; k++) xs[k] = (active && k < len) ? row_at(g, r, k) : -1;
 }
 
-// Appends (f << 32 | chunk) tasks of kTask entries covering a row of `len` entries for every
-// thread of the block with `take`, the block's slots reserved with one atomic (every thread of
-// the block calls it).
-__device__ __forceinline__ void block_push_chunks(bool take, int64_t f, int64_t len, int64_t* tasks,
-                                                  int64_t* task_count, int64_t task_cap) {
-  int64_t chunks = take ? (len + kTask - 1) / kTask : 0;
-  int64_t pos = block_reserve(chunks, task_count);
-  for (int64_t k = 0; k < chunks && pos + k < task_cap; k++) tasks[pos + k] = (f << 32) | k;
+// Writes xs[k] for every take[k], in k order, from slot p on (slots at or past cap dropped).
+__device__ __forceinline__ void write_taken(const bool (&take)[kShort], const int32_t (&xs)[kShort], int32_t* out,
+                                            int64_t p, int64_t cap) {
+#pragma unroll
+  for (int k = 0; k < kShort; k++) {
+    if (take[k]) {
+      if (p < cap) out[p] = xs[k];
+      p++;
+    }
+  }
 }
 
 // Bounds of chunk task `task` of row r.
@@ -298,8 +300,8 @@ __global__ void k_relax_short(GView g, int32_t* level, int32_t* parent, const in
       len = row_len(r);
     }
     bool is_short = active && len <= kShort;
-    block_append(active && !is_short && len <= kTask, f, medium, medium_count, cap);
-    block_push_chunks(active && len > kTask, f, len, tasks, task_count, task_cap);
+    bool is_medium = active && !is_short && len <= kTask;
+    bool is_long = active && len > kTask;
     int32_t xs[kShort];
     gather_short(g, r, len, is_short, xs);
     int32_t lv[kShort];
@@ -307,11 +309,19 @@ __global__ void k_relax_short(GView g, int32_t* level, int32_t* parent, const in
     for (int k = 0; k < kShort; k++) lv[k] = xs[k] >= 0 ? level[xs[k]] : 0;
     if (is_short) work += len;
     bool take[kShort];
+    int32_t nnext = 0;
 #pragma unroll
-    for (int k = 0; k < kShort; k++)
+    for (int k = 0; k < kShort; k++) {
       take[k] = xs[k] >= 0 && (lv[k] == -1 || lv[k] > lvl + 1) && relax_to(level, parent, stamp, epoch, f, xs[k], lvl);
-    int64_t at[kShort];
-    block_append_k(take, xs, next, next_count, cap, at);
+      nnext += take[k];
+    }
+    // one reservation for the block's medium rows, chunk tasks and lowered neighbours
+    int32_t need[3] = {is_medium, is_long ? (int32_t)((len + kTask - 1) / kTask) : 0, nnext};
+    int64_t pos[3];
+    block_reserve_n(need, pos, medium_count, task_count, next_count);
+    if (is_medium && pos[0] < cap) medium[pos[0]] = f;
+    for (int64_t k = 0; k < need[1] && pos[1] + k < task_cap; k++) tasks[pos[1] + k] = ((int64_t)f << 32) | k;
+    write_taken(take, xs, next, pos[2], cap);
   }
   block_add(work, scanned);
   block_add(taken, seeded);
@@ -466,24 +476,34 @@ __global__ void k_invalidate_short(GView g, int32_t* level, int32_t* parent, con
         parent[c] = found;
       }
     }
-    block_append(invalid, c, inval, inval_count, cap);
-    block_append(active && !is_short && len <= kChunk, c, medium, medium_count, cap);
-    // a long row: listed, and its chunk tasks (index in longs << 32 | chunk) pushed
-    int64_t lpos = block_append(active && len > kChunk, c, longs, long_count, cap);
-    block_push_chunks(lpos >= 0, lpos, len, tasks, task_count, task_cap);
+    bool is_medium = active && !is_short && len <= kChunk;
+    bool is_long = active && len > kChunk;
     // children of an invalid candidate: neighbours one level down (levels read above; no
     // vertex of that level changes in this kernel) whose parent it is
     int32_t pw[kShort];
 #pragma unroll
     for (int k = 0; k < kShort; k++) pw[k] = (invalid && xs[k] >= 0 && lv[k] == lvl + 1) ? parent[xs[k]] : -1;
     bool take[kShort];
+    int32_t nkids = 0;
 #pragma unroll
     for (int k = 0; k < kShort; k++) {
       take[k] = pw[k] == c && invalid && atomicExch(&stamp[xs[k]], epoch) != epoch;
       if (take[k]) attrib_child(at, c, xs[k]);
+      nkids += take[k];
     }
-    int64_t at[kShort];
-    block_append_k(take, xs, next, next_count, cap, at);
+    // one reservation for the block's invalid, medium and long rows, the long rows' chunk
+    // tasks (index in longs << 32 | chunk) and the children; a candidate is listed once per
+    // round, so the lists (n + 1 slots) never fill
+    int32_t need[5] = {invalid, is_medium, is_long, is_long ? (int32_t)((len + kTask - 1) / kTask) : 0, nkids};
+    int64_t pos[5];
+    block_reserve_n(need, pos, inval_count, medium_count, long_count, task_count, next_count);
+    if (invalid && pos[0] < cap) inval[pos[0]] = c;
+    if (is_medium && pos[1] < cap) medium[pos[1]] = c;
+    if (is_long && pos[2] < cap) {
+      longs[pos[2]] = c;
+      for (int64_t k = 0; k < need[3] && pos[3] + k < task_cap; k++) tasks[pos[3] + k] = (pos[2] << 32) | k;
+    }
+    write_taken(take, xs, next, pos[4], cap);
   }
   block_add(work, scanned);
   block_add(taken, listed);
@@ -725,8 +745,13 @@ __global__ void k_tentative_short(GView g, const int32_t* level, const int32_t* 
     if (active) {
       tent[i] = best;
     }
-    block_push_chunks(active && len > kTask, i, len, tasks, task_count, task_cap);
-    block_append(active && !is_short && len <= kTask, (int32_t)(active ? i : 0), medium, medium_count, medium_cap);
+    bool is_medium = active && !is_short && len <= kTask;
+    bool is_long = active && len > kTask;
+    int32_t need[2] = {is_medium, is_long ? (int32_t)((len + kTask - 1) / kTask) : 0};
+    int64_t pos[2];
+    block_reserve_n(need, pos, medium_count, task_count);
+    if (is_medium && pos[0] < medium_cap) medium[pos[0]] = (int32_t)i;
+    for (int64_t k = 0; k < need[1] && pos[1] + k < task_cap; k++) tasks[pos[1] + k] = (i << 32) | k;
   }
   block_add(work, scanned);
 }

@@ -168,6 +168,52 @@ __device__ __forceinline__ void block_append_k(const bool (&take)[K], const T (&
   __syncthreads();
 }
 
+// Block-wide reservation on N counters at once: each thread asks need[k] (< 2^31) slots of
+// the k-th counter and gets its first slot in pos[k]. Lanes 0..N-1 of the first wave issue the
+// N atomics together, so the block waits for one round trip instead of N in a row. Every
+// thread of the block calls it. The counters come as separate arguments, picked per lane by a
+// chain of selects: an array of them indexed by lane would live in memory.
+template <int N, typename... Ctr>
+__device__ __forceinline__ void block_reserve_n(const int32_t (&need)[N], int64_t (&pos)[N], Ctr... count) {
+  static_assert(sizeof...(Ctr) == N, "one counter per need");
+  __shared__ int64_t wsum[N * kWavesPerBlock];
+  __shared__ int64_t gbase[N];
+  int w = (int)(threadIdx.x / kWave);
+  int32_t x[N];
+#pragma unroll
+  for (int k = 0; k < N; k++) x[k] = need[k];
+  // inclusive scans within the wave, the N of them interleaved
+  for (int off = 1; off < kWave; off <<= 1) {
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+      int32_t y = __shfl_up(x[k], off);
+      if (lane_id() >= off) x[k] += y;
+    }
+  }
+  if (lane_id() == kWave - 1) {
+#pragma unroll
+    for (int k = 0; k < N; k++) wsum[k * kWavesPerBlock + w] = x[k];
+  }
+  __syncthreads();
+  if (threadIdx.x < N) {
+    int k = (int)threadIdx.x;
+    int64_t t = 0;
+    for (int i = 0; i < kWavesPerBlock; i++) {
+      int64_t c = wsum[k * kWavesPerBlock + i];
+      wsum[k * kWavesPerBlock + i] = t;
+      t += c;
+    }
+    int64_t* ctr = nullptr;
+    int j = 0;
+    ((ctr = (k == j++) ? count : ctr), ...);
+    gbase[k] = t ? (int64_t)atomicAdd((unsigned long long*)ctr, (unsigned long long)t) : 0;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < N; k++) pos[k] = gbase[k] + wsum[k * kWavesPerBlock + w] + x[k] - need[k];
+  __syncthreads();
+}
+
 __device__ __forceinline__ void block_min_max(int64_t lo, int64_t hi, int64_t* out_min, int64_t* out_max) {
   __shared__ int64_t pmin[kWavesPerBlock], pmax[kWavesPerBlock];
   for (int off = kWave / 2; off > 0; off >>= 1) {
