@@ -83,6 +83,36 @@ __device__ __forceinline__ void write_taken(const bool (&take)[kShort], const in
   }
 }
 
+// Round bookkeeping on the device, so the host can queue several rounds between read-backs.
+// Each round's list counters come in two sets used by alternate rounds. The last kernel of a
+// round calls round_book: one thread of its last block (whose share of the work list is the
+// smallest) adds the round's list size to *sum, counts the round in *nonempty when its list
+// was not empty or `side` says that side-list items of its level may exist, folds the other
+// set's chunk-task count (the previous round's) into *max_tasks and zeroes that set for the
+// next round, and zeroes the round's list count, which the next round appends to.
+struct RoundBook {
+  int64_t* fcur;         // this round's list count
+  int64_t* sum;
+  int64_t* nonempty;
+  int64_t* max_tasks;
+  int64_t* other_tasks;  // the other set: chunk tasks, then up to two more counters (or null)
+  int64_t* other_a;
+  int64_t* other_b;
+  int side;
+};
+
+__device__ __forceinline__ void round_book(const RoundBook& b) {
+  if (blockIdx.x != gridDim.x - 1 || threadIdx.x != 0) return;
+  int64_t nc = *b.fcur, t = *b.other_tasks;
+  *b.sum += nc;
+  if (nc > 0 || b.side) *b.nonempty += 1;
+  if (t > *b.max_tasks) *b.max_tasks = t;
+  *b.other_tasks = 0;
+  if (b.other_a) *b.other_a = 0;
+  if (b.other_b) *b.other_b = 0;
+  *b.fcur = 0;
+}
+
 // Bounds of chunk task `task` of row r.
 __device__ __forceinline__ void task_range(const Row& r, int64_t task, int64_t& lo, int64_t& hi) {
   lo = (task & 0xffffffffLL) * kTask;
@@ -331,7 +361,7 @@ __global__ void k_relax_short(GView g, int32_t* level, int32_t* parent, const in
 __global__ void k_relax_rows(GView g, int32_t* level, int32_t* parent, const int32_t* medium,
                              const int64_t* medium_count, const int64_t* tasks, const int64_t* task_count,
                              int64_t task_cap, int32_t lvl, int32_t* next, int64_t* next_count, int64_t cap,
-                             int32_t* stamp, int32_t epoch, int64_t* scanned, int64_t* paths) {
+                             int32_t* stamp, int32_t epoch, int64_t* scanned, int64_t* paths, RoundBook book) {
   __shared__ int32_t seg_all[kWavesPerBlock * kNextSeg];
   WaveStage<kNextSeg> st{seg_all + (threadIdx.x / kWave) * kNextSeg, 0};
   int64_t nm = *medium_count;
@@ -371,6 +401,7 @@ __global__ void k_relax_rows(GView g, int32_t* level, int32_t* parent, const int
   stage_finish(st, next, next_count, cap);
   count_path(ntask, &paths[PATH_RELAX_TASKS]);
   block_add(lane_id() == 0 ? work : 0, scanned);
+  round_book(book);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -667,7 +698,7 @@ __global__ void k_inval_push(GView g, int32_t* level, const int32_t* parent, int
                              const int64_t* tasks, const int64_t* task_count, int64_t task_cap,
                              const int32_t* supported, int32_t supepoch, int32_t* inval, int64_t* inval_count,
                              int64_t inval_cap, int32_t* next, int64_t* next_count, int64_t cap, int32_t* stamp,
-                             int32_t epoch, int64_t* scanned, int64_t* paths, Attrib ab) {
+                             int32_t epoch, int64_t* scanned, int64_t* paths, Attrib ab, RoundBook book) {
   constexpr int kInvSeg = 64;
   __shared__ int32_t seg_all[kWavesPerBlock * kNextSeg];
   __shared__ int32_t inv_all[kWavesPerBlock * kInvSeg];
@@ -710,6 +741,7 @@ __global__ void k_inval_push(GView g, int32_t* level, const int32_t* parent, int
   stage_finish(st, next, next_count, cap);
   count_path(ntask, &paths[PATH_INVAL_PUSH_TASKS]);
   block_add(lane_id() == 0 ? work : 0, scanned);
+  round_book(book);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -914,19 +946,11 @@ __global__ void k_zero(int64_t* p, int count) {
   if ((int)threadIdx.x < count) p[threadIdx.x] = 0;
 }
 
-// Round bookkeeping on the device, so the host can queue several rounds between read-backs:
-// adds the round's list size to *sum (and counts the round in *nonempty if the list is not
-// empty or `side` says that side-list items of the round's level may exist), folds the
-// previous round's chunk-task count into *max_tasks, then zeroes the next list's count and
-// the `zn` counters at `zero` (which may include *tasks).
-__global__ void k_round_begin(const int64_t* cur_count, int64_t* next_count, const int64_t* tasks, int64_t* zero,
-                              int zn, int64_t* sum, int64_t* nonempty, int64_t* max_tasks, int side) {
-  int64_t t = *tasks, nc = *cur_count;
-  if (t > *max_tasks) *max_tasks = t;
-  *sum += nc;
-  if (nc > 0 || side) *nonempty += 1;
-  *next_count = 0;
-  for (int i = 0; i < zn; i++) zero[i] = 0;
+// Zeroes two runs of counters.
+__global__ void k_zero2(int64_t* a, int na, int64_t* b, int nb) {
+  int t = (int)threadIdx.x;
+  if (t < na) a[t] = 0;
+  if (t < nb) b[t] = 0;
 }
 
 // box[0..7] = {0, INT32_MAX, -1, 0, 0, INT32_MAX, -1, 0}, box[8..15] = 0.
@@ -975,12 +999,14 @@ int relax_levels(Ctx& c, int32_t* L, int32_t* P, int32_t lvl0, int32_t max_seed,
   int32_t* cur = c.lb.p;
   int32_t* nxt = c.lc.p;
   int64_t* box = c.d_box;
-  // box slots: [8]/[9] frontier counts (current and next alternate), [10] chunk tasks,
-  // [11] medium rows, [12] scanned entries, [15] frontier vertices over all rounds,
-  // [16] largest task count, [17] non-empty rounds
+  // box slots: [8]/[9] frontier counts (current and next alternate), [10]/[11] and [18]/[19]
+  // chunk tasks and medium rows (the counter sets of alternate rounds), [12] scanned entries,
+  // [13] entries of short rows, [15] frontier vertices over all rounds, [16] largest task
+  // count, [17] non-empty rounds
   int64_t* cnt[2] = {box + 8, box + 9};
-  k_zero<<<1, 64, 0, c.s>>>(box + 10, 4);
-  k_zero<<<1, 64, 0, c.s>>>(box + 15, 3);
+  int64_t* tasks_of[2] = {box + 10, box + 18};
+  int64_t* medium_of[2] = {box + 11, box + 19};
+  k_zero2<<<1, 64, 0, c.s>>>(box + 9, 5, box + 15, 5);
   // seeds grouped by level once (each round then reads only its own), when the levels they
   // span fit the bucket counters and the list fits the scratch
   const int64_t* sbuckets = nullptr;
@@ -1002,22 +1028,22 @@ int relax_levels(Ctx& c, int32_t* L, int32_t* P, int32_t lvl0, int32_t max_seed,
       // seeds of this level are taken by the round's first kernel
       int64_t ns = (nseeds > 0 && lvl <= max_seed) ? nseeds : 0;
       const int64_t* sb = (sbuckets && ns > 0) ? sbuckets + (lvl - lvl0) : nullptr;
-      k_round_begin<<<1, 1, 0, c.s>>>(bcur, bnext, box + 10, box + 10, 2, box + 15, box + 17, box + 16, ns > 0);
+      RoundBook book{bcur, box + 15, box + 17, box + 16, tasks_of[p ^ 1], medium_of[p ^ 1], nullptr, ns > 0};
       SCOUT_CHECK(prof_launch(c, PROF_K_RELAX_SHORT, k_relax_short, dim3(kListGrid), dim3(kBlock), c.g.view(), L, P,
                               cur, bcur, seeds, ns, sb, box + 15, lvl, nxt, bnext, cap, c.stamp, epoch, c.le.p,
-                              box + 11, c.wb.p, box + 10, task_cap, box + 13));
+                              medium_of[p], c.wb.p, tasks_of[p], task_cap, box + 13));
       SCOUT_CHECK(prof_launch(c, PROF_K_RELAX_ROWS, k_relax_rows, dim3(kListGrid), dim3(kBlock), c.g.view(), L, P,
-                              c.le.p, box + 11, c.wb.p, box + 10, task_cap, lvl, nxt, bnext, cap, c.stamp, epoch,
-                              box + 12, c.d_paths));
+                              c.le.p, medium_of[p], c.wb.p, tasks_of[p], task_cap, lvl, nxt, bnext, cap, c.stamp,
+                              epoch, box + 12, c.d_paths, book));
       lvl++;
       std::swap(cur, nxt);
       p ^= 1;
     }
     SCOUT_CHECK(prof_end(c, PROF_BFS_RELAX, 0));
-    SCOUT_TRY(hipMemcpyAsync(c.h_box + 8, box + 8, 10 * sizeof(int64_t), hipMemcpyDeviceToHost, c.s));
+    SCOUT_TRY(hipMemcpyAsync(c.h_box + 8, box + 8, 12 * sizeof(int64_t), hipMemcpyDeviceToHost, c.s));
     SCOUT_TRY(hipStreamSynchronize(c.s));
     c.readbacks++;
-    if (std::max(c.h_box[10], c.h_box[16]) > task_cap) {
+    if (std::max({c.h_box[10], c.h_box[16], c.h_box[18]}) > task_cap) {
       set_error("bfs relax: chunk task list overflow");
       return SCOUT_ERR_STATE;
     }
@@ -1140,8 +1166,8 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, int32_t* P, int strategy, scou
   if (ncand > 0) {
     // Invalidation, level by level, queued in groups of rounds. Counts: [8]/[9] candidates at
     // this level / the next (alternating), [10] invalid vertices, [11] scanned entries,
-    // [12] medium rows, [13] long rows, [14] chunk tasks, [18] candidates over all rounds,
-    // [19] largest task count, [20] non-empty rounds.
+    // [12..14] / [28..30] medium rows, long rows and chunk tasks (alternating), [18]
+    // candidates over all rounds, [19] largest task count, [20] non-empty rounds.
     int32_t sepoch = next_epoch(c);
     int32_t* cur = c.lb.p;
     int32_t* nxt = c.lc.p;
@@ -1158,9 +1184,13 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, int32_t* P, int strategy, scou
     int64_t* cnt[2] = {box + 8, box + 9};
     int64_t* binval = box + 10;
     int64_t* bscan = box + 11;
+    // the counter sets of alternate rounds: medium rows, long rows and chunk tasks at
+    // [12..14] and [28..30]
+    int64_t* medium_of[2] = {box + 12, box + 28};
+    int64_t* longs_of[2] = {box + 13, box + 29};
+    int64_t* tasks_of[2] = {box + 14, box + 30};
     // [15] entries read by k_inval_push, [23] entries of short rows read by k_invalidate_short
-    k_zero<<<1, 64, 0, c.s>>>(box + 8, 8);
-    k_zero<<<1, 64, 0, c.s>>>(box + 18, 6);
+    k_zero2<<<1, 64, 0, c.s>>>(box + 8, 16, box + 28, 3);
     int p = 0;
     int group = first_group;
     int32_t lvl = cmin;
@@ -1173,30 +1203,31 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, int32_t* P, int strategy, scou
         // long rows and their chunk tasks; k_invalidate searches the medium rows and the
         // tasks; k_inval_push settles the long rows no task supported
         int64_t nlevel = lvl <= cmax ? ncand : 0;
-        k_round_begin<<<1, 1, 0, c.s>>>(bcur, bnext, box + 14, box + 12, 3, box + 18, box + 20, box + 19,
-                                        nlevel > 0);
+        RoundBook book{bcur,           box + 18,          box + 20,        box + 19,
+                       tasks_of[p ^ 1], medium_of[p ^ 1], longs_of[p ^ 1], nlevel > 0};
         const int64_t* cb = (cbuckets && nlevel > 0) ? cbuckets + (lvl - cmin) : nullptr;
         SCOUT_CHECK(prof_launch(c, PROF_K_INVALIDATE_SHORT, k_invalidate_short, dim3(kListGrid), dim3(kBlock),
                                 c.g.view(), L, P, cur, bcur, cand, nlevel, cb, box + 18, lvl, c.ld.p, binval, nxt,
-                                bnext, cap, c.stamp, sepoch, c.le.p, box + 12, c.lf.p, box + 13, c.wb.p, box + 14,
-                                c.wb.cap, box + 23, ab));
+                                bnext, cap, c.stamp, sepoch, c.le.p, medium_of[p], c.lf.p, longs_of[p], c.wb.p,
+                                tasks_of[p], c.wb.cap, box + 23, ab));
         int32_t supepoch = next_epoch(c);
         SCOUT_CHECK(prof_launch(c, PROF_BFS_INVALIDATE_KERNEL, k_invalidate, dim3(kListGrid), dim3(kBlock),
-                                c.g.view(), L, P, c.le.p, box + 12, c.lf.p, c.wb.p, box + 14, c.wb.cap, c.aux,
-                                supepoch, lvl, c.ld.p, binval, cap, nxt, bnext, cap, c.stamp, sepoch, bscan,
+                                c.g.view(), L, P, c.le.p, medium_of[p], c.lf.p, c.wb.p, tasks_of[p], c.wb.cap,
+                                c.aux, supepoch, lvl, c.ld.p, binval, cap, nxt, bnext, cap, c.stamp, sepoch, bscan,
                                 box + 21, c.d_paths, ab));
         SCOUT_CHECK(prof_launch(c, PROF_K_INVAL_PUSH, k_inval_push, dim3(kListGrid), dim3(kBlock), c.g.view(), L, P,
-                                lvl, c.lf.p, c.wb.p, box + 14, c.wb.cap, c.aux, supepoch, c.ld.p, binval, cap, nxt,
-                                bnext, cap, c.stamp, sepoch, box + 15, c.d_paths, ab));
+                                lvl, c.lf.p, c.wb.p, tasks_of[p], c.wb.cap, c.aux, supepoch, c.ld.p, binval, cap,
+                                nxt, bnext, cap, c.stamp, sepoch, box + 15, c.d_paths, ab, book));
         lvl++;
         std::swap(cur, nxt);
         p ^= 1;
       }
       SCOUT_CHECK(prof_end(c, PROF_BFS_INVALIDATE, 0));
-      SCOUT_TRY(hipMemcpyAsync(c.h_box + 8, box + 8, 16 * sizeof(int64_t), hipMemcpyDeviceToHost, c.s));
+      // [8..30]: the prediction slots [24..27] come back unchanged
+      SCOUT_TRY(hipMemcpyAsync(c.h_box + 8, box + 8, 23 * sizeof(int64_t), hipMemcpyDeviceToHost, c.s));
       SCOUT_TRY(hipStreamSynchronize(c.s));
       c.readbacks++;
-      if (std::max(c.h_box[14], c.h_box[19]) > c.wb.cap) {
+      if (std::max({c.h_box[14], c.h_box[19], c.h_box[30]}) > c.wb.cap) {
         set_error("bfs invalidation: chunk task list overflow");
         return SCOUT_ERR_STATE;
       }
