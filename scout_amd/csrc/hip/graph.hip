@@ -84,12 +84,11 @@ __device__ __forceinline__ uint64_t hash64(uint64_t k) {
 
 constexpr uint64_t kEmpty = ~0ULL;
 
-// Inserts every line's unordered pair into an open-addressing set; *count += the lines whose
-// pair another line claimed first (m minus the distinct pairs).
-__global__ void k_dup_count(const int32_t* uv, int64_t m, uint64_t* hkey, uint64_t mask, int64_t* count) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= m) return;
-  uint64_t key = pair_key(uv[2 * i], uv[2 * i + 1]);
+// Inserts the unordered pair {u, v} into an open-addressing set (at most a quarter full);
+// *count += 1 when another line claimed it first.
+__device__ __forceinline__ void pair_set_insert(int32_t u, int32_t v, uint64_t* hkey, uint64_t mask,
+                                                int64_t* count) {
+  uint64_t key = pair_key(u, v);
   uint64_t h = hash64(key) & mask;
   while (true) {
     uint64_t prev = atomicCAS((unsigned long long*)&hkey[h], (unsigned long long)kEmpty, (unsigned long long)key);
@@ -102,6 +101,13 @@ __global__ void k_dup_count(const int32_t* uv, int64_t m, uint64_t* hkey, uint64
   }
 }
 
+// *count += the lines whose unordered pair an earlier-claiming line holds (m minus the
+// distinct pairs).
+__global__ void k_dup_count(const int32_t* uv, int64_t m, uint64_t* hkey, uint64_t mask, int64_t* count) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < m) pair_set_insert(uv[2 * i], uv[2 * i + 1], hkey, mask, count);
+}
+
 // ins[i] = 1 iff the pair is absent from the rule's graph. rule 0: current logical graph;
 // rule 1: the loaded base graph (decoded, i.e. before any update).
 // Every kSampleStride-th base entry, decoded (deleted entries keep their position and id).
@@ -111,8 +117,9 @@ __global__ void k_base_samples(const int32_t* bcol, int64_t E, int32_t* samp) {
 }
 
 // pos (optional): the base range of row u found for each line, reused by k_apply.
+// hkey (optional): the repeated-pair count of k_dup_count is taken here too, into *repeated.
 __global__ void k_classify(GView g, bool sorted, const int32_t* uv, int64_t m, int rule, uint8_t* ins,
-                           int64_t* pos) {
+                           int64_t* pos, uint64_t* hkey, uint64_t mask, int64_t* repeated) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < m) {
     int32_t u = uv[2 * i], v = uv[2 * i + 1];
@@ -123,6 +130,7 @@ __global__ void k_classify(GView g, bool sorted, const int32_t* uv, int64_t m, i
     }
     bool adj = rule == SCOUT_CLASSIFY_BASE ? s.base_any : adjacent_now(s);
     ins[i] = adj ? 0 : 1;
+    if (hkey) pair_set_insert(u, v, hkey, mask, repeated);
   }
 }
 
@@ -258,6 +266,9 @@ __global__ void k_src_split(const uint64_t* nkey, const int32_t* ndst, const int
 }
 
 __global__ void k_zero_slot(int64_t* p) { *p = 0; }
+__global__ void k_zero_slots(int64_t* p, int n) {
+  if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
+}
 
 // Zeroes the counts k_src_claim left (for the next batch).
 __global__ void k_src_release(const uint64_t* nkey, int64_t cnt, int ks, int32_t* ncnt) {
@@ -632,14 +643,14 @@ int apply_range(Ctx& c, Channel& ch, int64_t lo, int64_t hi, int conditional, co
   SCOUT_CHECK(g.ndst2.reserve(g.nkey.cap));
   SCOUT_CHECK(g.mkey.reserve(g.nkey.cap));
   SCOUT_CHECK(g.mdst.reserve(g.nkey.cap));
-  k_zero_slot<<<1, 1, 0, c.s>>>(c.d_box + kSingleSlot);
-  k_zero_slot<<<1, 1, 0, c.s>>>(c.d_box + kMultiSlot);
+  static_assert(kMultiSlot == kSingleSlot + 1, "split counts in adjacent slots");
+  k_zero_slots<<<1, 64, 0, c.s>>>(c.d_box + kSingleSlot, 2);
   k_src_claim<<<kListGrid, kBlock, 0, c.s>>>(g.nkey.p, c.d_box, g.nkey.cap, ks, g.ncnt.p);
   k_src_split<<<kListGrid, kBlock, 0, c.s>>>(g.nkey.p, g.ndst.p, c.d_box, g.nkey.cap, ks, g.ncnt.p, g.nkey2.p,
                                               g.ndst2.p, c.d_box + kSingleSlot, g.mkey.p, g.mdst.p,
                                               c.d_box + kMultiSlot);
   SCOUT_CHECK(box_read(c, kMultiSlot + 1));
-  if (skip && c.h_box[kRepeatSlot] != 0) {
+  if (skip && c.h_box[skip - c.d_box] != 0) {
     *skipped = true;
     return SCOUT_OK;
   }
@@ -651,6 +662,17 @@ int apply_range(Ctx& c, Channel& ch, int64_t lo, int64_t hi, int conditional, co
   }
   if (cnt_new > 0) SCOUT_CHECK(insert_new(c, cnt_new, ks, c.h_box[kSingleSlot], c.h_box[kMultiSlot]));
   if (g.holes > (g.live >> 1) + (1 << 20)) SCOUT_CHECK(repack_inserted(c, 2 * g.live + (1 << 20)));
+  return SCOUT_OK;
+}
+
+// Clears the pair set of k_dup_count and k_classify for m lines: a power of two of at least
+// 4m slots, so it is at most a quarter full. *size = its slot count.
+int pair_set_clear(Ctx& c, int64_t m, uint64_t* size) {
+  uint64_t s = 1;
+  while (s < (uint64_t)(4 * m)) s <<= 1;
+  SCOUT_CHECK(c.g.hkey.reserve((int64_t)s));
+  SCOUT_TRY(hipMemsetAsync(c.g.hkey.p, 0xff, s * sizeof(uint64_t), c.s));
+  *size = s;
   return SCOUT_OK;
 }
 
@@ -819,13 +841,25 @@ int graph_classify_lines(Ctx& c, Channel& ch, int64_t lo, int64_t hi, int rule) 
   if (ch.pos_gen != c.g.base_gen || ch.pos_hi != lo) ch.pos_lo = lo;  // else extend the valid range
   ch.pos_hi = hi;
   ch.pos_gen = c.g.base_gen;
+  // A classify of every staged line also counts the repeated pairs for the apply that follows
+  // (its skip test), into the channel's mailbox slot.
+  bool count_repeats = lo == 0 && hi == ch.m && m > 0;
+  uint64_t* hkey = nullptr;
+  uint64_t size = 0;
+  int64_t* repeated = c.d_box + kChannelRepeatSlot + (&ch - c.ch);
+  if (count_repeats) {
+    SCOUT_CHECK(pair_set_clear(c, m, &size));
+    hkey = c.g.hkey.p;
+    SCOUT_TRY(hipMemsetAsync(repeated, 0, sizeof(int64_t), c.s));
+  }
   SCOUT_CHECK(prof_begin(c, PROF_CLASSIFY));
   if (m > 0)
     k_classify<<<grid_for(m), kBlock, 0, c.s>>>(c.g.view(), c.g.sorted, ch.d_uv.p + 2 * lo, m, rule, ch.d_ins.p + lo,
-                                                ch.d_pos.p + 2 * lo);
+                                                ch.d_pos.p + 2 * lo, hkey, size - 1, repeated);
   // per line: the pair (8 B), row u's base and inserted offsets (16 B), the located entry
-  // (4 B) and the label written (1 B)
-  SCOUT_CHECK(prof_end(c, PROF_CLASSIFY, 29 * m));
+  // (4 B), the label written (1 B) and, when counting repeats, the pair's set slot (8 B)
+  SCOUT_CHECK(prof_end(c, PROF_CLASSIFY, (count_repeats ? 37 : 29) * m));
+  ch.rep_hi = count_repeats ? hi : ch.rep_hi;
   return SCOUT_OK;
 }
 
@@ -836,10 +870,8 @@ int graph_classify(Ctx& c, Channel& ch, int rule) { return graph_classify_lines(
 int queue_repeat_count(Ctx& c, Channel& ch, int64_t lo, int64_t hi, int64_t* out) {
   DevGraph& g = c.g;
   int64_t m = hi - lo;
-  uint64_t size = 1;
-  while (size < (uint64_t)(4 * m)) size <<= 1;
-  SCOUT_CHECK(g.hkey.reserve((int64_t)size));
-  SCOUT_TRY(hipMemsetAsync(g.hkey.p, 0xff, size * sizeof(uint64_t), c.s));
+  uint64_t size = 0;
+  SCOUT_CHECK(pair_set_clear(c, m, &size));
   SCOUT_TRY(hipMemsetAsync(out, 0, sizeof(int64_t), c.s));
   if (m > 0) k_dup_count<<<grid_for(m), kBlock, 0, c.s>>>(ch.d_uv.p + 2 * lo, m, g.hkey.p, size - 1, out);
   return SCOUT_OK;
@@ -861,10 +893,16 @@ int graph_apply_lines(Ctx& c, Channel& ch, int64_t lo, int64_t hi, int condition
   int64_t m = hi - lo;
   if (m <= 0) return SCOUT_OK;
   // The repeated-pair count goes to the device ahead of k_apply, which applies the range only
-  // when no pair repeats; the count comes back with the new-entry count (one read-back).
-  SCOUT_CHECK(queue_repeat_count(c, ch, lo, hi, c.d_box + kRepeatSlot));
+  // when no pair repeats; the count comes back with the new-entry count (one read-back). A
+  // classify of every staged line has counted them already.
+  int slot = kRepeatSlot;
+  if (lo == 0 && hi == ch.m && ch.rep_hi == hi) {
+    slot = kChannelRepeatSlot + (int)(&ch - c.ch);
+  } else {
+    SCOUT_CHECK(queue_repeat_count(c, ch, lo, hi, c.d_box + kRepeatSlot));
+  }
   bool skipped = false;
-  SCOUT_CHECK(apply_range(c, ch, lo, hi, conditional, c.d_box + kRepeatSlot, &skipped));
+  SCOUT_CHECK(apply_range(c, ch, lo, hi, conditional, c.d_box + slot, &skipped));
   ch.pairs_distinct = !skipped && lo == 0 && hi == ch.m;
   if (!skipped) return SCOUT_OK;
   // Repeated pairs: cut the range into runs of distinct pairs, applied in order.

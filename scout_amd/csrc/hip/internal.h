@@ -167,6 +167,9 @@ struct Channel {
   // the last apply covered every staged line and no pair repeated: an insertion line's pair
   // is then adjacent after the batch
   bool pairs_distinct = false;
+  // a classify of lines [0, rep_hi) also counted the lines whose unordered pair an earlier line
+  // holds, into the mailbox slot kChannelRepeatSlot + channel index (-1: no such count)
+  int64_t rep_hi = -1;
   int64_t m = 0;
   hipEvent_t staged = nullptr;
   // recorded on the compute stream after the last work that reads this channel's lines: the
@@ -188,6 +191,7 @@ inline void channel_staged(Channel& ch, int64_t m) {
   ch.m = m;
   ch.pos_lo = ch.pos_hi = 0;
   ch.pairs_distinct = false;
+  ch.rep_hi = -1;
 }
 
 struct MlpState;  // mlp.hip
@@ -269,6 +273,8 @@ struct Ctx {
 };
 
 constexpr int kBoxSlots = 64;
+// Mailbox slots 44 and 45: each channel's repeated-pair count from its last full classify.
+constexpr int kChannelRepeatSlot = 44;
 // Long-row code paths counted on the device (scout_path_counts): chunk tasks of split rows
 // per kernel family, and invalid rows whose children did not fit the LDS list.
 enum PathId { PATH_INVAL_TASKS = 0, PATH_INVAL_PUSH_TASKS, PATH_INVAL_KIDS_OVERFLOW, PATH_CC_TASKS,
