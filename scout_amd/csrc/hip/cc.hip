@@ -405,19 +405,36 @@ __global__ void k_cc_collect(const int32_t* L, int32_t* label, const uint32_t* b
         }
       }
     }
+    // the mark reads of an unreached vertex's old component, in two steps whose reads are all
+    // in flight together: the coarse bitmap for every such vertex, then the fine bitmap where
+    // the coarse bit is set (vertices past n read as reached)
+    uint32_t cw[2][4], fw[2][4];
+#pragma unroll
+    for (int h = 0; h < 2; h++)
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        int32_t l = lab[h][k];
+        cw[h][k] = (ls[h][k] < 0 && l != 0) ? coarse[l >> (5 + kCoarseShift)] : 0u;
+      }
+#pragma unroll
+    for (int h = 0; h < 2; h++)
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        int32_t l = lab[h][k];
+        fw[h][k] = ((cw[h][k] >> ((l >> kCoarseShift) & 31)) & 1u) ? bits[l >> 5] : 0u;
+      }
 #pragma unroll
     for (int h = 0; h < 2; h++) {
       int64_t q = base + h * stride + threadIdx.x;
 #pragma unroll
       for (int k = 0; k < 4; k++) {
         int64_t v = 4 * q + k;
+        int32_t l = lab[h][k];
         bool take = false;
-        if (q < n4 && v < n) {
-          int32_t l = lab[h][k];
-          if (ls[h][k] >= 0) {
-            if (l != 0) label[v] = 0;
-          } else if (l == 0 || (((coarse[l >> (5 + kCoarseShift)] >> ((l >> kCoarseShift) & 31)) & 1u) &&
-                                ((bits[l >> 5] >> (l & 31)) & 1u))) {
+        if (ls[h][k] >= 0) {
+          if (l != 0) label[v] = 0;
+        } else if (q < n4 && v < n) {
+          if (l == 0 || ((fw[h][k] >> (l & 31)) & 1u)) {
             take = true;
             par[v] = (int32_t)v;
           } else {
