@@ -538,6 +538,9 @@ def main():
                     help="plain batches whose repairs log the learned scheduler's training data (0: plain only)")
     ap.add_argument("--epochs", type=int, default=4)
     ap.add_argument("--repeats", type=int, default=3, help="alternating plain / learned replays (medians reported)")
+    ap.add_argument("--bfs-mode", choices=["async", "level"], default="async",
+                    help="BFS round order: async (every level's candidates and seeds in the first rounds) or "
+                         "level (one level per round); the levels are the same")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-verify", action="store_true")
@@ -587,6 +590,7 @@ def main():
     ntrain = args.train_batches
     total = ntrain + args.warmup + args.steps
     eng = sa.Engine(0)
+    eng.bfs_mode(sa.BFS_ASYNC if args.bfs_mode == "async" else sa.BFS_LEVEL_SYNC)
     t0 = time.perf_counter()
     eng.gen_rmat(args.scale, 16, 0.57, 0.19, 0.19, args.seed)
     info = eng.graph_info()
@@ -786,7 +790,7 @@ def main():
                    "scale": args.scale, "vertices": info["n"], "adjacency_entries": info["base_entries"],
                    "batch": args.batch,
                    "strategy": "learned" if learned else args.strategy + " (plain)",
-                   "train_batches": ntrain},
+                   "train_batches": ntrain, "bfs_rounds": args.bfs_mode},
         "p50_ms": round(float(np.percentile(head["lat"], 50)), 3),
         "p99_ms": round(float(np.percentile(head["lat"], 99)), 3),
         "plain": {"value": round(plain["ups"], 1), "p50_ms": round(float(np.percentile(plain["lat"], 50)), 3),

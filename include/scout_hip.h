@@ -166,6 +166,15 @@ int scout_repair_bfs(scout_ctx* ctx, int channel, int strategy, scout_stats* st)
  * (st->fallback = 1) instead of the local tentative levels and relax. The answer is exact either
  * way, before the batch commits; 0 (the default) never falls back. */
 int scout_bfs_bound(scout_ctx* ctx, int64_t bound);
+/* How the BFS repair orders its rounds; levels are the same in both. SCOUT_BFS_ASYNC (the
+ * default) decides the deletion candidates of every level in the first invalidation round and
+ * relaxes every insertion seed in the first relax round: a vertex whose possible support is
+ * still being decided waits one round, and a vertex lowered again is relaxed again, so the
+ * rounds follow the longest cascade instead of the span of levels the batch touches.
+ * SCOUT_BFS_LEVEL_SYNC runs one level per round, in increasing level order. No counterpart in the
+ * reference, which repairs one edit at a time (src/bfs/scout_bfs.cu:436-515). */
+enum { SCOUT_BFS_ASYNC = 0, SCOUT_BFS_LEVEL_SYNC = 1 };
+int scout_bfs_mode(scout_ctx* ctx, int mode);
 /* Round groups of the next BFS repair only: its first invalidation group queues inval_rounds
  * levels and its first relax group relax_rounds levels before the first read-back of each
  * (at most 64; 0 keeps the strategy's own sizing). A measurement aid with no counterpart in the
@@ -338,7 +347,9 @@ enum {
   SCOUT_PATH_RELAX_TASKS = 4,        /* BFS relax: chunk tasks of long rows */
   SCOUT_PATH_TENTATIVE_TASKS = 5,    /* BFS tentative levels: chunk tasks of long rows */
   SCOUT_PATH_CC_WALK_ROWS = 6,       /* CC walk mode: rows walked by a whole wave */
-  SCOUT_PATH_COUNT = 7
+  SCOUT_PATH_INVAL_DEFERRED = 7,     /* asynchronous BFS invalidation: vertices that waited a round
+                                        for a pending neighbour one level up */
+  SCOUT_PATH_COUNT = 8
 };
 int scout_path_counts(scout_ctx* ctx, int64_t* out, int count);
 

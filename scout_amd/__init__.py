@@ -25,6 +25,7 @@ CLASSIFY_CURRENT, CLASSIFY_BASE = 0, 1
 STRATEGY_LOCAL, STRATEGY_RECOMPUTE, STRATEGY_LEARNED = 0, 1, 2
 MIS_RULES, MIS_COMPLETE = 0, 1
 CC_AUTO, CC_WALK, CC_SCAN = 0, 1, 2
+BFS_ASYNC, BFS_LEVEL_SYNC = 0, 1
 GRAPH_INFO_COUNT = 8  # entries scout_graph_info writes
 
 
@@ -105,6 +106,7 @@ _SIGNATURES = {
                          ctypes.POINTER(_I64)],
     "scout_bfs_bound": [_P, _I64],
     "scout_bfs_schedule": [_P, ctypes.c_int, ctypes.c_int],
+    "scout_bfs_mode": [_P, ctypes.c_int],
     "scout_touched_enable": [_P, ctypes.c_int],
     "scout_touched_download": [_P, _P, _I64, ctypes.POINTER(_I64)],
     "scout_apply_segments": [_P, ctypes.c_int, _I32, _P, ctypes.c_int, ctypes.c_int, _P],
@@ -128,7 +130,8 @@ PROF_NAMES = ["bfs_relax", "bfs_invalidate", "cc_expand", "apply", "classify", "
               "bfs_invalidate_kernel", "k_relax_short", "k_relax_rows", "k_invalidate_short", "k_inval_push",
               "k_cc_explore", "k_cc_collect"]
 PATH_NAMES = ["inval_tasks", "inval_push_tasks", "inval_kids_overflow", "cc_tasks", "relax_tasks",
-              "tentative_tasks", "cc_walk_rows"]
+              "tentative_tasks", "cc_walk_rows", "inval_deferred"]
+LONG_ROW_PATHS = PATH_NAMES[:7]
 
 _lib = None
 
@@ -454,6 +457,11 @@ class Engine:
         """Round groups of the next BFS repair only (a perfect predictor's schedule)."""
         self._check(self.lib.scout_bfs_schedule(self.h, inval_rounds, relax_rounds), "bfs_schedule")
 
+    def bfs_mode(self, mode: int):
+        """BFS_ASYNC (every level's candidates and seeds in the first round) or BFS_LEVEL_SYNC (one
+        level per round); the levels are the same."""
+        self._check(self.lib.scout_bfs_mode(self.h, mode), "bfs_mode")
+
     def bfs_bound(self, bound: int):
         self._check(self.lib.scout_bfs_bound(self.h, bound), "bfs_bound")
 
@@ -469,7 +477,8 @@ class Engine:
         return out[:cnt.value]
 
     def path_counts(self) -> dict:
-        """Cumulative counts of the long-row code paths (chunk tasks per kernel family)."""
+        """Cumulative counts of the long-row code paths (chunk tasks per kernel family) and of
+        the vertices the asynchronous BFS invalidation made wait a round."""
         out = np.zeros(len(PATH_NAMES), dtype=np.int64)
         self._check(self.lib.scout_path_counts(self.h, _ptr(out), len(PATH_NAMES)), "path_counts")
         return {k: int(v) for k, v in zip(PATH_NAMES, out)}

@@ -52,6 +52,12 @@ constexpr int kNextSeg = 256;
 // Level of a vertex found invalid in this batch until its tentative level is written. It is
 // negative, so it never matches a level and reads as unreached.
 constexpr int32_t kInvalid = -2;
+// Asynchronous invalidation (SCOUT_BFS_ASYNC): while a round decides its members, each member
+// at level l is stored as kPend - l, so that no vertex of the same round takes it as support
+// before it is decided (a vertex seen pending waits for the next round instead). Levels
+// kPend - l are all below kInvalid and read as unreached everywhere else.
+constexpr int32_t kPend = -3;
+__device__ __forceinline__ int32_t pend_of(int32_t l) { return kPend - l; }
 // Rounds queued between two read-backs of the round counters: the first group is short (most
 // repairs end within a few levels), later groups double up to kMaxGroup. A round whose list
 // is empty costs a few launches and no work.
@@ -152,7 +158,8 @@ __device__ __forceinline__ void attrib_child(const Attrib& a, int32_t c, int32_t
 }
 
 // Lowers w to lvl+1 on behalf of f; on success f becomes w's parent. True if w must be queued
-// (lowered and not queued before in this relax).
+// (lowered and not queued before under this epoch: the whole relax when it is
+// level-synchronous, the next round when it is asynchronous).
 __device__ __forceinline__ bool relax_to(int32_t* level, int32_t* parent, int32_t* stamp, int32_t epoch, int32_t f,
                                          int32_t w, int32_t lvl) {
   if (!cas_min_level(&level[w], lvl + 1)) return false;
@@ -297,13 +304,20 @@ void bucket_by_level(hipStream_t s, K src, int64_t count, int32_t lmin, int nb, 
 // level, seeds[sb[0]..sb[1]) (ns = 0 in rounds past the deepest seed). Short rows are relaxed
 // here; rows up to kTask go to the wave list, longer rows to chunk tasks.
 // *seeded += the seeds taken.
+// Asynchronous (A): every frontier vertex relaxes from its current level, whatever the round;
+// the first round takes every seed (seeds[0..ns), once each under seed_epoch). A vertex may be
+// lowered in several rounds, and each lowering queues it for the next round (epoch is that
+// round's). A frontier vertex also takes a neighbour one level up as its parent: concurrent
+// lowerings of one vertex in one round may leave the parent of a level that lost the race,
+// and the round after its last lowering sets it right (its last lowerer is such a neighbour).
+template <bool A>
 __global__ void k_relax_short(GView g, int32_t* level, int32_t* parent, const int32_t* front, const int64_t* count,
                               const int64_t* seeds, int64_t ns, const int64_t* sb, int64_t* seeded, int32_t lvl,
                               int32_t* next, int64_t* next_count, int64_t cap, int32_t* stamp, int32_t epoch,
-                              int32_t* medium, int64_t* medium_count, int64_t* tasks, int64_t* task_count,
-                              int64_t task_cap, int64_t* scanned) {
+                              int32_t seed_epoch, int32_t* medium, int64_t* medium_count, int64_t* tasks,
+                              int64_t* task_count, int64_t task_cap, int64_t* scanned) {
   int64_t nf = *count;
-  if (sb && ns > 0) {
+  if (!A && sb && ns > 0) {
     seeds += sb[0];
     ns = sb[1] - sb[0];
   }
@@ -312,17 +326,27 @@ __global__ void k_relax_short(GView g, int32_t* level, int32_t* parent, const in
   // the trip count is uniform across the block, so the block-wide appends see every thread
   for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < total; base += (int64_t)gridDim.x * blockDim.x) {
     int64_t i = base + threadIdx.x;
-    int32_t f = -1;
+    int32_t f = -1, lf = lvl;
     Row r{0, 0, 0, 0};
     int64_t len = 0;
     bool active = false;
     if (i < nf) {
       f = front[i];
-      active = level[f] == lvl;
+      if constexpr (A) {
+        lf = level[f];
+        active = lf >= 0;
+      } else {
+        active = level[f] == lvl;
+      }
     } else if (i < total) {
       int64_t sd = seeds[i - nf];
       f = (int32_t)(sd & 0xffffffff);
-      active = (int32_t)(sd >> 32) == lvl && level[f] == lvl && atomicExch(&stamp[f], epoch) != epoch;
+      if constexpr (A) {
+        lf = level[f];
+        active = lf >= 0 && atomicExch(&stamp[f], seed_epoch) != seed_epoch;
+      } else {
+        active = (int32_t)(sd >> 32) == lvl && level[f] == lvl && atomicExch(&stamp[f], epoch) != epoch;
+      }
       taken += active;
     }
     if (active) {
@@ -338,11 +362,18 @@ __global__ void k_relax_short(GView g, int32_t* level, int32_t* parent, const in
 #pragma unroll
     for (int k = 0; k < kShort; k++) lv[k] = xs[k] >= 0 ? level[xs[k]] : 0;
     if (is_short) work += len;
+    if constexpr (A) {
+      int32_t up = -1;
+#pragma unroll
+      for (int k = 0; k < kShort; k++)
+        if (up < 0 && xs[k] >= 0 && lv[k] == lf - 1) up = xs[k];
+      if (is_short && lf > 0 && up >= 0) parent[f] = up;
+    }
     bool take[kShort];
     int32_t nnext = 0;
 #pragma unroll
     for (int k = 0; k < kShort; k++) {
-      take[k] = xs[k] >= 0 && (lv[k] == -1 || lv[k] > lvl + 1) && relax_to(level, parent, stamp, epoch, f, xs[k], lvl);
+      take[k] = xs[k] >= 0 && (lv[k] == -1 || lv[k] > lf + 1) && relax_to(level, parent, stamp, epoch, f, xs[k], lf);
       nnext += take[k];
     }
     // one reservation for the block's medium rows, chunk tasks and lowered neighbours
@@ -358,6 +389,9 @@ __global__ void k_relax_short(GView g, int32_t* level, int32_t* parent, const in
 }
 
 // Wave per medium frontier vertex (listed by k_relax_short) or per chunk task of a long one.
+// Asynchronous (A): relaxes from the vertex's current level and sets its parent to a
+// neighbour one level up, as k_relax_short does.
+template <bool A>
 __global__ void k_relax_rows(GView g, int32_t* level, int32_t* parent, const int32_t* medium,
                              const int64_t* medium_count, const int64_t* tasks, const int64_t* task_count,
                              int64_t task_cap, int32_t lvl, int32_t* next, int64_t* next_count, int64_t cap,
@@ -383,6 +417,9 @@ __global__ void k_relax_rows(GView g, int32_t* level, int32_t* parent, const int
       task_range(r, task, lo, hi);
       ntask++;
     }
+    int32_t lf = lvl;
+    if constexpr (A) lf = level[f];
+    bool fixed = !A || lf <= 0;
     work += hi - lo;
     for (int64_t base = lo; base < hi; base += (int64_t)kU * kWave) {
       int32_t ws[kU];
@@ -390,10 +427,20 @@ __global__ void k_relax_rows(GView g, int32_t* level, int32_t* parent, const int
       int32_t lw[kU];
 #pragma unroll
       for (int j = 0; j < kU; j++) lw[j] = ws[j] >= 0 ? level[ws[j]] : 0;
+      if constexpr (A) {
+#pragma unroll
+        for (int j = 0; j < kU; j++) {
+          unsigned long long up = __ballot(ws[j] >= 0 && lw[j] == lf - 1);
+          if (!fixed && up) {
+            if (lane_id() == __ffsll((long long)up) - 1) parent[f] = ws[j];
+            fixed = true;
+          }
+        }
+      }
       bool take[kU];
 #pragma unroll
       for (int j = 0; j < kU; j++)
-        take[j] = ws[j] >= 0 && (lw[j] == -1 || lw[j] > lvl + 1) && relax_to(level, parent, stamp, epoch, f, ws[j], lvl);
+        take[j] = ws[j] >= 0 && (lw[j] == -1 || lw[j] > lf + 1) && relax_to(level, parent, stamp, epoch, f, ws[j], lf);
 #pragma unroll
       for (int j = 0; j < kU; j++) stage_push(st, take[j], ws[j], next, next_count, cap);
     }
@@ -449,33 +496,72 @@ __global__ void k_del_candidates(const int32_t* uv, const uint8_t* ins, int64_t 
   block_min_max(lmin, lmax, &box[1], &box[2]);
 }
 
+// Asynchronous invalidation: the batch's deletion candidates (cand[0..*ncand)) become the
+// first round's list, each marked pending (stored as pend_of(level)); a vertex listed twice
+// is kept once. Runs after every reader of the plain candidate levels (the candidate pass and
+// the scheduler's features).
+__global__ void k_mark_pending(const int32_t* cand, const int64_t* ncand, int32_t* level, int32_t* out,
+                               int64_t* out_count, int64_t cap) {
+  constexpr int K = kLinesPerThread;
+  int64_t n = *ncand;
+  int64_t first = (int64_t)blockIdx.x * blockDim.x * K + threadIdx.x;
+  bool take[K];
+  int32_t cs[K];
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    int64_t i = first + (int64_t)k * blockDim.x;
+    int32_t c = i < n ? cand[i] : -1;
+    int32_t lc = c >= 0 ? level[c] : -1;
+    take[k] = lc > 0 && atomicCAS(&level[c], lc, pend_of(lc)) == lc;
+    cs[k] = c;
+  }
+  int64_t pos[K];
+  block_append_k(take, cs, out, out_count, cap, pos);
+}
+
 // Invalidation of the candidates at level lvl: the children queued by the previous round
 // (cur) and the batch's deletion candidates of this level (cand[0..ncand), or cand[cb[0]..cb[1])
 // when the candidates are grouped by level; filtered by level and stamp; ncand = 0 past the
 // deepest candidate; *listed += those taken). Thread per candidate; rows longer than kShort go
 // to the wave list (<= kChunk) or the long list, whose chunk tasks are pushed here.
+// Asynchronous (A): the round's list (cur) holds pending vertices of any level, each decided
+// at its own level l = kPend - level: a neighbour at l - 1 that is not pending supports it; if
+// none does but one is pending, it waits for the next round (queued again, still pending);
+// otherwise it is invalid and its children at l + 1 are marked pending and queued. An early
+// "invalid" is final (during the invalidation support only disappears), and a supported
+// vertex whose parent is later found invalid is queued again by that parent's child scan, so
+// the result is the level-synchronous one in as many rounds as the longest cascade needs.
+// A long row's level goes to long_lvl beside its entry in longs.
+template <bool A>
 __global__ void k_invalidate_short(GView g, int32_t* level, int32_t* parent, const int32_t* cur,
                                    const int64_t* cur_count, const int32_t* cand, int64_t ncand, const int64_t* cb,
                                    int64_t* listed, int32_t lvl, int32_t* inval, int64_t* inval_count, int32_t* next,
                                    int64_t* next_count, int64_t cap, int32_t* stamp, int32_t epoch, int32_t* medium,
-                                   int64_t* medium_count, int32_t* longs, int64_t* long_count, int64_t* tasks,
-                                   int64_t* task_count, int64_t task_cap, int64_t* scanned, Attrib at) {
+                                   int64_t* medium_count, int32_t* longs, int32_t* long_lvl, int64_t* long_count,
+                                   int64_t* tasks, int64_t* task_count, int64_t task_cap, int64_t* scanned,
+                                   int64_t* paths, Attrib at) {
   int64_t nc = *cur_count;
-  if (cb && ncand > 0) {
+  if (A) ncand = 0;
+  if (!A && cb && ncand > 0) {
     cand += cb[0];
     ncand = cb[1] - cb[0];
   }
   int64_t total = nc + ncand;
-  int64_t work = 0, taken = 0;
+  int64_t work = 0, taken = 0, deferred = 0;
   // the trip count is uniform across the block, so the block-wide appends see every thread
   for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < total; base += (int64_t)gridDim.x * blockDim.x) {
     int64_t i = base + threadIdx.x;
-    int32_t c = -1;
+    int32_t c = -1, lm = lvl;
     Row r{0, 0, 0, 0};
     int64_t len = 0;
     bool active = i < nc;
     if (active) {
       c = cur[i];
+      if constexpr (A) {
+        int32_t e = level[c];
+        active = e <= kPend;
+        lm = kPend - e;
+      }
     } else if (i < total) {
       // a deletion candidate whose level is this round's, listed once (stamp)
       int32_t x = cand[i - nc];
@@ -494,50 +580,63 @@ __global__ void k_invalidate_short(GView g, int32_t* level, int32_t* parent, con
 #pragma unroll
     for (int k = 0; k < kShort; k++) lv[k] = xs[k] >= 0 ? level[xs[k]] : kInvalid;
     int32_t found = -1;
+    bool pending = false;
 #pragma unroll
-    for (int k = 0; k < kShort; k++)
-      if (found < 0 && lv[k] == lvl - 1) found = xs[k];
-    bool invalid = is_short && found < 0;
+    for (int k = 0; k < kShort; k++) {
+      if (found < 0 && lv[k] == lm - 1) found = xs[k];
+      if (A) pending |= xs[k] >= 0 && lv[k] == pend_of(lm - 1);
+    }
+    bool defer = A && is_short && found < 0 && pending;
+    bool invalid = is_short && found < 0 && !defer;
+    deferred += defer;
     if (is_short) {
       work += len;
       if (invalid) {
         level[c] = kInvalid;
-        attrib_invalid(at, c, lvl);
-      } else {
+        attrib_invalid(at, c, lm);
+      } else if (!defer) {
         parent[c] = found;
+        if (A) level[c] = lm;
       }
     }
     bool is_medium = active && !is_short && len <= kChunk;
     bool is_long = active && len > kChunk;
     // children of an invalid candidate: neighbours one level down (levels read above; no
-    // vertex of that level changes in this kernel) whose parent it is
+    // vertex of that level changes in this kernel, except to pending) whose parent it is
     int32_t pw[kShort];
 #pragma unroll
-    for (int k = 0; k < kShort; k++) pw[k] = (invalid && xs[k] >= 0 && lv[k] == lvl + 1) ? parent[xs[k]] : -1;
+    for (int k = 0; k < kShort; k++) pw[k] = (invalid && xs[k] >= 0 && lv[k] == lm + 1) ? parent[xs[k]] : -1;
     bool take[kShort];
     int32_t nkids = 0;
 #pragma unroll
     for (int k = 0; k < kShort; k++) {
-      take[k] = pw[k] == c && invalid && atomicExch(&stamp[xs[k]], epoch) != epoch;
+      if constexpr (A)
+        take[k] = pw[k] == c && invalid && atomicCAS(&level[xs[k]], lm + 1, pend_of(lm + 1)) == lm + 1;
+      else
+        take[k] = pw[k] == c && invalid && atomicExch(&stamp[xs[k]], epoch) != epoch;
       if (take[k]) attrib_child(at, c, xs[k]);
       nkids += take[k];
     }
     // one reservation for the block's invalid, medium and long rows, the long rows' chunk
-    // tasks (index in longs << 32 | chunk) and the children; a candidate is listed once per
-    // round, so the lists (n + 1 slots) never fill
-    int32_t need[5] = {invalid, is_medium, is_long, is_long ? (int32_t)((len + kTask - 1) / kTask) : 0, nkids};
+    // tasks (index in longs << 32 | chunk) and the children (and a deferred vertex); a vertex
+    // is listed once per round, so the lists (n + 1 slots) never fill
+    int32_t need[5] = {invalid, is_medium, is_long, is_long ? (int32_t)((len + kTask - 1) / kTask) : 0,
+                       nkids + (int32_t)defer};
     int64_t pos[5];
     block_reserve_n(need, pos, inval_count, medium_count, long_count, task_count, next_count);
     if (invalid && pos[0] < cap) inval[pos[0]] = c;
     if (is_medium && pos[1] < cap) medium[pos[1]] = c;
     if (is_long && pos[2] < cap) {
       longs[pos[2]] = c;
+      if (A) long_lvl[pos[2]] = lm;
       for (int64_t k = 0; k < need[3] && pos[3] + k < task_cap; k++) tasks[pos[3] + k] = (pos[2] << 32) | k;
     }
     write_taken(take, xs, next, pos[4], cap);
+    if (defer && pos[4] < cap) next[pos[4]] = c;
   }
   block_add(work, scanned);
   block_add(taken, listed);
+  if (A) block_add(deferred, &paths[PATH_INVAL_DEFERRED]);
 }
 
 // Wave per medium candidate: a neighbour one level up supports it (and becomes its parent);
@@ -547,12 +646,17 @@ __global__ void k_invalidate_short(GView g, int32_t* level, int32_t* parent, con
 // Then wave per chunk task of a long candidate: a neighbour one level up anywhere in the
 // chunk supports the candidate and becomes its parent (any of them will do); supported[c] =
 // supepoch records it for k_inval_push.
+// Asynchronous (A): levels are the candidates' own (as in k_invalidate_short); a medium row
+// with no support but a pending neighbour one level up waits for the next round, and a chunk
+// that sees such a neighbour raises supported[c] to supepoch - 1 (atomicMax: support wins).
 // own[0] += entries and parents read, own[1] += rows and tasks.
+template <bool A>
 __global__ void k_invalidate(GView g, int32_t* level, int32_t* parent, const int32_t* cur, const int64_t* cur_count,
-                             const int32_t* longs, const int64_t* tasks, const int64_t* task_count, int64_t task_cap,
-                             int32_t* supported, int32_t supepoch, int32_t lvl, int32_t* inval, int64_t* inval_count,
-                             int64_t inval_cap, int32_t* next, int64_t* next_count, int64_t cap, int32_t* stamp,
-                             int32_t epoch, int64_t* scanned, int64_t* own, int64_t* paths, Attrib ab) {
+                             const int32_t* longs, const int32_t* long_lvl, const int64_t* tasks,
+                             const int64_t* task_count, int64_t task_cap, int32_t* supported, int32_t supepoch,
+                             int32_t lvl, int32_t* inval, int64_t* inval_count, int64_t inval_cap, int32_t* next,
+                             int64_t* next_count, int64_t cap, int32_t* stamp, int32_t epoch, int64_t* scanned,
+                             int64_t* own, int64_t* paths, Attrib ab) {
   // neighbours one level down met while searching for support: the only possible children
   constexpr int kKids = 1024;
   constexpr int kInvSeg = 64;
@@ -564,17 +668,20 @@ __global__ void k_invalidate(GView g, int32_t* level, int32_t* parent, const int
   WaveStage<kInvSeg> sinv{inv_all + (threadIdx.x / kWave) * kInvSeg, 0};
   int64_t nc = *cur_count;
   int64_t nt = min(*task_count, task_cap);
-  int64_t work = 0, reads = 0, rows = 0, ntask = 0, overflow = 0;
+  int64_t work = 0, reads = 0, rows = 0, ntask = 0, overflow = 0, deferred = 0;
   for (int64_t i = global_wave(); i < nc + nt; i += total_waves()) {
     if (i >= nc) {
       int64_t task = tasks[i - nc];
       int32_t c = longs[task >> 32];
+      int32_t lm = lvl;
+      if constexpr (A) lm = long_lvl[task >> 32];
       if (supported[c] == supepoch) continue;
       ntask++;
       Row r = row_of(g, c);
       int64_t lo, hi;
       task_range(r, task, lo, hi);
       int64_t at = -1;
+      bool pending = false;
       // one row step first (support is usually among the first entries), then kU at a time
       for (int64_t base = lo, step = kWave; base < hi && at < 0; base += step, step = (int64_t)kU * kWave) {
         int32_t xs[kU];
@@ -584,8 +691,9 @@ __global__ void k_invalidate(GView g, int32_t* level, int32_t* parent, const int
         for (int j = 0; j < kU; j++) lx[j] = xs[j] >= 0 ? level[xs[j]] : INT32_MIN;
 #pragma unroll
         for (int j = 0; j < kU; j++) {
-          unsigned long long sup = __ballot(xs[j] >= 0 && lx[j] == lvl - 1);
+          unsigned long long sup = __ballot(xs[j] >= 0 && lx[j] == lm - 1);
           if (sup && at < 0) at = base + (int64_t)j * kWave + (__ffsll((long long)sup) - 1);
+          if (A) pending |= __ballot(xs[j] >= 0 && lx[j] == pend_of(lm - 1)) != 0;
         }
       }
       int64_t nread = (at >= 0 ? at + 1 : hi) - lo;
@@ -594,14 +702,24 @@ __global__ void k_invalidate(GView g, int32_t* level, int32_t* parent, const int
       rows++;
       if (at >= 0 && lane_id() == 0) {
         parent[c] = row_at(g, r, at);
-        supported[c] = supepoch;
+        if constexpr (A) {
+          level[c] = lm;
+          atomicMax(&supported[c], supepoch);
+        } else {
+          supported[c] = supepoch;
+        }
+      } else if (A && at < 0 && pending && lane_id() == 0) {
+        atomicMax(&supported[c], supepoch - 1);
       }
       continue;
     }
     int32_t c = cur[i];
     Row r = row_of(g, c);
+    int32_t lm = lvl;
+    if constexpr (A) lm = kPend - level[c];
     int64_t len = row_len(r);
     int64_t at = -1, nk = 0;
+    bool pending = false;
     // one row step first (support is usually among the first entries of a row, whose hubs
     // come first), then kU steps in flight per iteration; `at` is uniform across the wave
     for (int64_t base = 0, step = kWave; base < len && at < 0; base += step, step = (int64_t)kU * kWave) {
@@ -613,12 +731,13 @@ __global__ void k_invalidate(GView g, int32_t* level, int32_t* parent, const int
 #pragma unroll
       for (int j = 0; j < kU; j++) {
         if (at >= 0) break;
-        unsigned long long sup = __ballot(xs[j] >= 0 && lx[j] == lvl - 1);
+        unsigned long long sup = __ballot(xs[j] >= 0 && lx[j] == lm - 1);
         if (sup) {
           at = base + (int64_t)j * kWave + (__ffsll((long long)sup) - 1);
           break;
         }
-        bool kid = xs[j] >= 0 && lx[j] == lvl + 1;
+        if (A) pending |= __ballot(xs[j] >= 0 && lx[j] == pend_of(lm - 1)) != 0;
+        bool kid = xs[j] >= 0 && lx[j] == lm + 1;
         unsigned long long km = __ballot(kid);
         int64_t slot = nk + __popcll(km & ((1ULL << lane_id()) - 1ULL));
         if (kid && slot < kKids) kids[slot] = xs[j];
@@ -629,13 +748,23 @@ __global__ void k_invalidate(GView g, int32_t* level, int32_t* parent, const int
     if (at >= 0) {
       work += at + 1;
       reads += at + 1;
-      if (lane_id() == 0) parent[c] = row_at(g, r, at);
+      if (lane_id() == 0) {
+        parent[c] = row_at(g, r, at);
+        if (A) level[c] = lm;
+      }
       continue;
     }
     work += len;
+    if (A && pending) {
+      // a pending neighbour one level up may still support it: decided next round
+      reads += len;
+      deferred++;
+      stage_push(st, lane_id() == 0, c, next, next_count, cap);
+      continue;
+    }
     if (lane_id() == 0) {
       level[c] = kInvalid;
-      attrib_invalid(ab, c, lvl);
+      attrib_invalid(ab, c, lm);
     }
     stage_push(sinv, lane_id() == 0, c, inval, inval_count, inval_cap);
     if (nk <= kKids) {
@@ -655,7 +784,10 @@ __global__ void k_invalidate(GView g, int32_t* level, int32_t* parent, const int
         bool take[kU];
 #pragma unroll
         for (int j = 0; j < kU; j++) {
-          take[j] = pw[j] == c && atomicExch(&stamp[ws[j]], epoch) != epoch;
+          if constexpr (A)
+            take[j] = pw[j] == c && atomicCAS(&level[ws[j]], lm + 1, pend_of(lm + 1)) == lm + 1;
+          else
+            take[j] = pw[j] == c && atomicExch(&stamp[ws[j]], epoch) != epoch;
           if (take[j]) attrib_child(ab, c, ws[j]);
         }
 #pragma unroll
@@ -673,7 +805,10 @@ __global__ void k_invalidate(GView g, int32_t* level, int32_t* parent, const int
         bool take[kU];
 #pragma unroll
         for (int j = 0; j < kU; j++) {
-          take[j] = pw[j] == c && level[ws[j]] == lvl + 1 && atomicExch(&stamp[ws[j]], epoch) != epoch;
+          if constexpr (A)
+            take[j] = pw[j] == c && atomicCAS(&level[ws[j]], lm + 1, pend_of(lm + 1)) == lm + 1;
+          else
+            take[j] = pw[j] == c && level[ws[j]] == lvl + 1 && atomicExch(&stamp[ws[j]], epoch) != epoch;
           if (take[j]) attrib_child(ab, c, ws[j]);
         }
 #pragma unroll
@@ -685,6 +820,7 @@ __global__ void k_invalidate(GView g, int32_t* level, int32_t* parent, const int
   stage_finish(sinv, inval, inval_count, inval_cap);
   count_path(ntask, &paths[PATH_INVAL_TASKS]);
   count_path(overflow, &paths[PATH_INVAL_KIDS_OVERFLOW]);
+  count_path(deferred, &paths[PATH_INVAL_DEFERRED]);
   // work, reads and rows are uniform across the wave: counted once per wave
   bool l0 = lane_id() == 0;
   block_add(l0 ? work : 0, scanned);
@@ -694,27 +830,40 @@ __global__ void k_invalidate(GView g, int32_t* level, int32_t* parent, const int
 
 // Wave per chunk of a long candidate that no chunk supported: it is invalid (recorded by its
 // chunk 0), and every chunk queues its children.
+// Asynchronous (A): a long row whose chunks saw a pending neighbour one level up (supported
+// = supepoch - 1) waits for the next round instead (queued by its chunk 0, still pending);
+// its level comes from long_lvl.
+template <bool A>
 __global__ void k_inval_push(GView g, int32_t* level, const int32_t* parent, int32_t lvl, const int32_t* longs,
-                             const int64_t* tasks, const int64_t* task_count, int64_t task_cap,
-                             const int32_t* supported, int32_t supepoch, int32_t* inval, int64_t* inval_count,
-                             int64_t inval_cap, int32_t* next, int64_t* next_count, int64_t cap, int32_t* stamp,
-                             int32_t epoch, int64_t* scanned, int64_t* paths, Attrib ab, RoundBook book) {
+                             const int32_t* long_lvl, const int64_t* tasks, const int64_t* task_count,
+                             int64_t task_cap, const int32_t* supported, int32_t supepoch, int32_t* inval,
+                             int64_t* inval_count, int64_t inval_cap, int32_t* next, int64_t* next_count, int64_t cap,
+                             int32_t* stamp, int32_t epoch, int64_t* scanned, int64_t* paths, Attrib ab,
+                             RoundBook book) {
   constexpr int kInvSeg = 64;
   __shared__ int32_t seg_all[kWavesPerBlock * kNextSeg];
   __shared__ int32_t inv_all[kWavesPerBlock * kInvSeg];
   WaveStage<kNextSeg> st{seg_all + (threadIdx.x / kWave) * kNextSeg, 0};
   WaveStage<kInvSeg> sinv{inv_all + (threadIdx.x / kWave) * kInvSeg, 0};
   int64_t nt = min(*task_count, task_cap);
-  int64_t work = 0, ntask = 0;
+  int64_t work = 0, ntask = 0, deferred = 0;
   for (int64_t t = global_wave(); t < nt; t += total_waves()) {
     int64_t task = tasks[t];
     int32_t c = longs[task >> 32];
-    if (supported[c] == supepoch) continue;
-    ntask++;
+    int32_t sup = supported[c];
+    if (sup == supepoch) continue;
     bool first = (task & 0xffffffffLL) == 0;
+    if (A && sup == supepoch - 1) {
+      deferred += first;
+      stage_push(st, first && lane_id() == 0, c, next, next_count, cap);
+      continue;
+    }
+    int32_t lm = lvl;
+    if constexpr (A) lm = long_lvl[task >> 32];
+    ntask++;
     if (first && lane_id() == 0) {
       level[c] = kInvalid;
-      attrib_invalid(ab, c, lvl);
+      attrib_invalid(ab, c, lm);
     }
     stage_push(sinv, first && lane_id() == 0, c, inval, inval_count, inval_cap);
     Row r = row_of(g, c);
@@ -730,7 +879,10 @@ __global__ void k_inval_push(GView g, int32_t* level, const int32_t* parent, int
       bool take[kU];
 #pragma unroll
       for (int j = 0; j < kU; j++) {
-        take[j] = pw[j] == c && level[ws[j]] == lvl + 1 && atomicExch(&stamp[ws[j]], epoch) != epoch;
+        if constexpr (A)
+          take[j] = pw[j] == c && atomicCAS(&level[ws[j]], lm + 1, pend_of(lm + 1)) == lm + 1;
+        else
+          take[j] = pw[j] == c && level[ws[j]] == lvl + 1 && atomicExch(&stamp[ws[j]], epoch) != epoch;
         if (take[j]) attrib_child(ab, c, ws[j]);
       }
 #pragma unroll
@@ -740,6 +892,7 @@ __global__ void k_inval_push(GView g, int32_t* level, const int32_t* parent, int
   stage_finish(sinv, inval, inval_count, inval_cap);
   stage_finish(st, next, next_count, cap);
   count_path(ntask, &paths[PATH_INVAL_PUSH_TASKS]);
+  count_path(deferred, &paths[PATH_INVAL_DEFERRED]);
   block_add(lane_id() == 0 ? work : 0, scanned);
   round_book(book);
 }
@@ -988,11 +1141,14 @@ __global__ void k_find_parents(GView g, const int32_t* level, int32_t* parent, i
   }
 }
 
-// Level-synchronous relax starting at level `lvl0` with the current list already in `cur`
-// (count in box slot 8) and optional seeds; runs until no frontier and no seed above the
-// current level remains. Adds the non-empty rounds to *rounds.
+// Relax with the current list already in `cur` (count in box slot 8) and optional seeds.
+// Level-synchronous (async = false): starts at level `lvl0` and runs until no frontier and no
+// seed above the current level remains. Asynchronous: the first round takes every seed and
+// later rounds their lowered neighbours, whatever their levels, until a round lowers nothing.
+// Adds the non-empty rounds to *rounds.
 int relax_levels(Ctx& c, int32_t* L, int32_t* P, int32_t lvl0, int32_t max_seed, const int64_t* seeds, int64_t nseeds,
-                 int32_t epoch, int64_t* rounds, int64_t* scanned_total, int first_group = kFirstGroup) {
+                 int32_t epoch, int64_t* rounds, int64_t* scanned_total, int first_group = kFirstGroup,
+                 bool async = false) {
   int32_t n = c.g.n;
   int64_t cap = (int64_t)n + 1;
   int64_t task_cap = c.wb.cap;
@@ -1011,7 +1167,7 @@ int relax_levels(Ctx& c, int32_t* L, int32_t* P, int32_t lvl0, int32_t max_seed,
   // span fit the bucket counters and the list fits the scratch
   const int64_t* sbuckets = nullptr;
   int64_t span = (int64_t)max_seed - lvl0 + 1;
-  if (nseeds > 0 && span >= 1 && span <= kMaxBuckets && c.wc.cap >= nseeds) {
+  if (!async && nseeds > 0 && span >= 1 && span <= kMaxBuckets && c.wc.cap >= nseeds) {
     bucket_by_level(c.s, SeedKey{seeds}, nseeds, lvl0, (int)span, c.lvlb.p, c.lvlb.p + kMaxBuckets + 1,
                     reinterpret_cast<int64_t*>(c.wc.p));
     seeds = reinterpret_cast<const int64_t*>(c.wc.p);
@@ -1020,21 +1176,33 @@ int relax_levels(Ctx& c, int32_t* L, int32_t* P, int32_t lvl0, int32_t max_seed,
   int32_t lvl = lvl0;
   int p = 0;
   int group = first_group;
+  int64_t done = 0;  // rounds queued so far
   while (true) {
     SCOUT_CHECK(prof_begin(c, PROF_BFS_RELAX));
-    for (int k = 0; k < group; k++) {
+    for (int k = 0; k < group; k++, done++) {
       int64_t* bcur = cnt[p];
       int64_t* bnext = cnt[p ^ 1];
-      // seeds of this level are taken by the round's first kernel
-      int64_t ns = (nseeds > 0 && lvl <= max_seed) ? nseeds : 0;
-      const int64_t* sb = (sbuckets && ns > 0) ? sbuckets + (lvl - lvl0) : nullptr;
+      // level-synchronous: the seeds of this level are taken by the round's first kernel;
+      // asynchronous: all of them by the first round, each round queueing under its own epoch
+      int64_t ns = async ? (done == 0 ? nseeds : 0) : (nseeds > 0 && lvl <= max_seed) ? nseeds : 0;
+      const int64_t* sb = (!async && sbuckets && ns > 0) ? sbuckets + (lvl - lvl0) : nullptr;
+      int32_t ep = async ? next_epoch(c) : epoch;
       RoundBook book{bcur, box + 15, box + 17, box + 16, tasks_of[p ^ 1], medium_of[p ^ 1], nullptr, ns > 0};
-      SCOUT_CHECK(prof_launch(c, PROF_K_RELAX_SHORT, k_relax_short, dim3(kListGrid), dim3(kBlock), c.g.view(), L, P,
-                              cur, bcur, seeds, ns, sb, box + 15, lvl, nxt, bnext, cap, c.stamp, epoch, c.le.p,
-                              medium_of[p], c.wb.p, tasks_of[p], task_cap, box + 13));
-      SCOUT_CHECK(prof_launch(c, PROF_K_RELAX_ROWS, k_relax_rows, dim3(kListGrid), dim3(kBlock), c.g.view(), L, P,
-                              c.le.p, medium_of[p], c.wb.p, tasks_of[p], task_cap, lvl, nxt, bnext, cap, c.stamp,
-                              epoch, box + 12, c.d_paths, book));
+      if (async) {
+        SCOUT_CHECK(prof_launch(c, PROF_K_RELAX_SHORT, k_relax_short<true>, dim3(kListGrid), dim3(kBlock),
+                                c.g.view(), L, P, cur, bcur, seeds, ns, sb, box + 15, lvl, nxt, bnext, cap, c.stamp,
+                                ep, epoch, c.le.p, medium_of[p], c.wb.p, tasks_of[p], task_cap, box + 13));
+        SCOUT_CHECK(prof_launch(c, PROF_K_RELAX_ROWS, k_relax_rows<true>, dim3(kListGrid), dim3(kBlock), c.g.view(),
+                                L, P, c.le.p, medium_of[p], c.wb.p, tasks_of[p], task_cap, lvl, nxt, bnext, cap,
+                                c.stamp, ep, box + 12, c.d_paths, book));
+      } else {
+        SCOUT_CHECK(prof_launch(c, PROF_K_RELAX_SHORT, k_relax_short<false>, dim3(kListGrid), dim3(kBlock),
+                                c.g.view(), L, P, cur, bcur, seeds, ns, sb, box + 15, lvl, nxt, bnext, cap, c.stamp,
+                                ep, epoch, c.le.p, medium_of[p], c.wb.p, tasks_of[p], task_cap, box + 13));
+        SCOUT_CHECK(prof_launch(c, PROF_K_RELAX_ROWS, k_relax_rows<false>, dim3(kListGrid), dim3(kBlock),
+                                c.g.view(), L, P, c.le.p, medium_of[p], c.wb.p, tasks_of[p], task_cap, lvl, nxt,
+                                bnext, cap, c.stamp, ep, box + 12, c.d_paths, book));
+      }
       lvl++;
       std::swap(cur, nxt);
       p ^= 1;
@@ -1047,7 +1215,7 @@ int relax_levels(Ctx& c, int32_t* L, int32_t* P, int32_t lvl0, int32_t max_seed,
       set_error("bfs relax: chunk task list overflow");
       return SCOUT_ERR_STATE;
     }
-    if (c.h_box[8 + p] == 0 && (nseeds == 0 || lvl > max_seed)) break;
+    if (c.h_box[8 + p] == 0 && (async || nseeds == 0 || lvl > max_seed)) break;
     // after a predicted first group the repair is near its end: start again from a short group
     group = group == first_group && first_group != kFirstGroup ? kFirstGroup : std::min(2 * group, kMaxGroup);
     first_group = kFirstGroup;
@@ -1137,10 +1305,10 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, int32_t* P, int strategy, scou
   int64_t ncand = c.h_box[0];
   int32_t cmin = (int32_t)c.h_box[1], cmax = (int32_t)c.h_box[2];
   st->deletions += c.h_box[3];
-  int64_t nx = 0, scanned = 0, rounds = 0;
+  int64_t nx = 0, scanned = 0, rounds = 0, st_inval_rounds = 0;
   int64_t nsample = std::min<int64_t>(ncand, sample);
   int first_group = kFirstGroup;
-  int64_t predicted_last = -1;
+  int64_t predicted_last = -1, predicted_depth = -1;
   if (learned && ncand > 0) {
     // predictions size and order the work; the plain search below decides every level
     double predicted = (double)c.h_box[24] * (double)ncand / (double)nsample;
@@ -1156,8 +1324,19 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, int32_t* P, int strategy, scou
     // prediction holds; an empty extra round costs as much as a read-back, so no slack. The
     // prediction is a maximum over sampled candidates, which overshoots: it is lowered by the
     // running mean of its overshoot on earlier batches.
-    predicted_last = std::max<int64_t>(c.h_box[25] - (int64_t)std::floor(c.pred_over), cmax);
-    first_group = (int)std::min<int64_t>(std::max<int64_t>(predicted_last - cmin + 1, kFirstGroup), kMaxGroup);
+    if (c.bfs_async) {
+      // asynchronous rounds: one per level of the deepest cascade, plus the round that finds
+      // its last children supported. [26] holds minus the largest predicted depth over the
+      // sampled candidates; the deepest cascade of the whole batch is rarely among them, so
+      // the prediction is corrected by the running mean of its error on earlier batches
+      // (rounded up: an empty extra round costs less than a second read-back).
+      predicted_depth = std::max<int64_t>(-c.h_box[26], 1) + 1;
+      int64_t want = predicted_depth + (int64_t)std::ceil(c.pred_bias);
+      first_group = (int)std::min<int64_t>(std::max<int64_t>(want, kFirstGroup), kMaxGroup);
+    } else {
+      predicted_last = std::max<int64_t>(c.h_box[25] - (int64_t)std::floor(c.pred_over), cmax);
+      first_group = (int)std::min<int64_t>(std::max<int64_t>(predicted_last - cmin + 1, kFirstGroup), kMaxGroup);
+    }
   }
   // a schedule handed in for this repair (the rounds a plain repair of the batch needed)
   int hint_inval = c.hint_inval, hint_relax = c.hint_relax;
@@ -1168,6 +1347,9 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, int32_t* P, int strategy, scou
     // this level / the next (alternating), [10] invalid vertices, [11] scanned entries,
     // [12..14] / [28..30] medium rows, long rows and chunk tasks (alternating), [18]
     // candidates over all rounds, [19] largest task count, [20] non-empty rounds.
+    // Asynchronous (c.bfs_async): the first round holds every candidate, marked pending, and
+    // the rounds run until one queues nothing.
+    bool async = c.bfs_async;
     int32_t sepoch = next_epoch(c);
     int32_t* cur = c.lb.p;
     int32_t* nxt = c.lc.p;
@@ -1175,7 +1357,7 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, int32_t* P, int strategy, scou
     const int32_t* cand = c.la.p;
     const int64_t* cbuckets = nullptr;
     int64_t cspan = (int64_t)cmax - cmin + 1;
-    if (cspan >= 1 && cspan <= kMaxBuckets) {
+    if (!async && cspan >= 1 && cspan <= kMaxBuckets) {
       bucket_by_level(c.s, VertexKey{c.la.p, L}, ncand, cmin, (int)cspan, c.lvlb.p, c.lvlb.p + kMaxBuckets + 1,
                       c.lg.p);
       cand = c.lg.p;
@@ -1191,6 +1373,9 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, int32_t* P, int strategy, scou
     int64_t* tasks_of[2] = {box + 14, box + 30};
     // [15] entries read by k_inval_push, [23] entries of short rows read by k_invalidate_short
     k_zero2<<<1, 64, 0, c.s>>>(box + 8, 16, box + 28, 3);
+    if (async)
+      k_mark_pending<<<grid_for(ncand, kBlock * kLinesPerThread), kBlock, 0, c.s>>>(c.la.p, box, L, cur, box + 8,
+                                                                                    cap);
     int p = 0;
     int group = first_group;
     int32_t lvl = cmin;
@@ -1202,22 +1387,41 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, int32_t* P, int strategy, scou
         // candidates of this level are taken by the round's first kernel, which also lists the
         // long rows and their chunk tasks; k_invalidate searches the medium rows and the
         // tasks; k_inval_push settles the long rows no task supported
-        int64_t nlevel = lvl <= cmax ? ncand : 0;
+        int64_t nlevel = !async && lvl <= cmax ? ncand : 0;
         RoundBook book{bcur,           box + 18,          box + 20,        box + 19,
                        tasks_of[p ^ 1], medium_of[p ^ 1], longs_of[p ^ 1], nlevel > 0};
         const int64_t* cb = (cbuckets && nlevel > 0) ? cbuckets + (lvl - cmin) : nullptr;
-        SCOUT_CHECK(prof_launch(c, PROF_K_INVALIDATE_SHORT, k_invalidate_short, dim3(kListGrid), dim3(kBlock),
-                                c.g.view(), L, P, cur, bcur, cand, nlevel, cb, box + 18, lvl, c.ld.p, binval, nxt,
-                                bnext, cap, c.stamp, sepoch, c.le.p, medium_of[p], c.lf.p, longs_of[p], c.wb.p,
-                                tasks_of[p], c.wb.cap, box + 23, ab));
+        // two fresh consecutive epochs: supepoch marks support, supepoch - 1 a pending neighbour
         int32_t supepoch = next_epoch(c);
-        SCOUT_CHECK(prof_launch(c, PROF_BFS_INVALIDATE_KERNEL, k_invalidate, dim3(kListGrid), dim3(kBlock),
-                                c.g.view(), L, P, c.le.p, medium_of[p], c.lf.p, c.wb.p, tasks_of[p], c.wb.cap,
-                                c.aux, supepoch, lvl, c.ld.p, binval, cap, nxt, bnext, cap, c.stamp, sepoch, bscan,
-                                box + 21, c.d_paths, ab));
-        SCOUT_CHECK(prof_launch(c, PROF_K_INVAL_PUSH, k_inval_push, dim3(kListGrid), dim3(kBlock), c.g.view(), L, P,
-                                lvl, c.lf.p, c.wb.p, tasks_of[p], c.wb.cap, c.aux, supepoch, c.ld.p, binval, cap,
-                                nxt, bnext, cap, c.stamp, sepoch, box + 15, c.d_paths, ab, book));
+        if (async) {
+          supepoch = next_epoch(c);
+          if (supepoch == 1) supepoch = next_epoch(c);  // the counter started over
+          SCOUT_CHECK(prof_launch(c, PROF_K_INVALIDATE_SHORT, k_invalidate_short<true>, dim3(kListGrid),
+                                  dim3(kBlock), c.g.view(), L, P, cur, bcur, cand, nlevel, cb, box + 18, lvl, c.ld.p,
+                                  binval, nxt, bnext, cap, c.stamp, sepoch, c.le.p, medium_of[p], c.lf.p, c.lg.p,
+                                  longs_of[p], c.wb.p, tasks_of[p], c.wb.cap, box + 23, c.d_paths, ab));
+          SCOUT_CHECK(prof_launch(c, PROF_BFS_INVALIDATE_KERNEL, k_invalidate<true>, dim3(kListGrid), dim3(kBlock),
+                                  c.g.view(), L, P, c.le.p, medium_of[p], c.lf.p, c.lg.p, c.wb.p, tasks_of[p],
+                                  c.wb.cap, c.aux, supepoch, lvl, c.ld.p, binval, cap, nxt, bnext, cap, c.stamp,
+                                  sepoch, bscan, box + 21, c.d_paths, ab));
+          SCOUT_CHECK(prof_launch(c, PROF_K_INVAL_PUSH, k_inval_push<true>, dim3(kListGrid), dim3(kBlock),
+                                  c.g.view(), L, P, lvl, c.lf.p, c.lg.p, c.wb.p, tasks_of[p], c.wb.cap, c.aux,
+                                  supepoch, c.ld.p, binval, cap, nxt, bnext, cap, c.stamp, sepoch, box + 15,
+                                  c.d_paths, ab, book));
+        } else {
+          SCOUT_CHECK(prof_launch(c, PROF_K_INVALIDATE_SHORT, k_invalidate_short<false>, dim3(kListGrid),
+                                  dim3(kBlock), c.g.view(), L, P, cur, bcur, cand, nlevel, cb, box + 18, lvl, c.ld.p,
+                                  binval, nxt, bnext, cap, c.stamp, sepoch, c.le.p, medium_of[p], c.lf.p, c.lg.p,
+                                  longs_of[p], c.wb.p, tasks_of[p], c.wb.cap, box + 23, c.d_paths, ab));
+          SCOUT_CHECK(prof_launch(c, PROF_BFS_INVALIDATE_KERNEL, k_invalidate<false>, dim3(kListGrid),
+                                  dim3(kBlock), c.g.view(), L, P, c.le.p, medium_of[p], c.lf.p, c.lg.p, c.wb.p,
+                                  tasks_of[p], c.wb.cap, c.aux, supepoch, lvl, c.ld.p, binval, cap, nxt, bnext, cap,
+                                  c.stamp, sepoch, bscan, box + 21, c.d_paths, ab));
+          SCOUT_CHECK(prof_launch(c, PROF_K_INVAL_PUSH, k_inval_push<false>, dim3(kListGrid), dim3(kBlock),
+                                  c.g.view(), L, P, lvl, c.lf.p, c.lg.p, c.wb.p, tasks_of[p], c.wb.cap, c.aux,
+                                  supepoch, c.ld.p, binval, cap, nxt, bnext, cap, c.stamp, sepoch, box + 15,
+                                  c.d_paths, ab, book));
+        }
         lvl++;
         std::swap(cur, nxt);
         p ^= 1;
@@ -1231,14 +1435,17 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, int32_t* P, int strategy, scou
         set_error("bfs invalidation: chunk task list overflow");
         return SCOUT_ERR_STATE;
       }
-      if (c.h_box[8 + p] == 0 && lvl > cmax) break;
+      if (c.h_box[8 + p] == 0 && (async || lvl > cmax)) break;
       // after a predicted first group the repair is near its end: start again from a short group
       group = group == first_group && first_group != kFirstGroup ? kFirstGroup : std::min(2 * group, kMaxGroup);
       first_group = kFirstGroup;
     }
     rounds += c.h_box[20];
     st->inval_rounds += c.h_box[20];
-    if (predicted_last >= 0) {
+    st_inval_rounds = c.h_box[20];
+    if (predicted_depth >= 0) {
+      c.pred_bias = 0.75 * c.pred_bias + 0.25 * (double)(c.h_box[20] - predicted_depth);
+    } else if (predicted_last >= 0) {
       int64_t last = cmin + c.h_box[20] - 1;  // deepest invalidation level
       double over = std::max<double>(0.0, (double)(c.h_box[25] - last));
       c.pred_over = 0.75 * c.pred_over + 0.25 * over;
@@ -1306,9 +1513,14 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, int32_t* P, int strategy, scou
     if (learned && predicted_last >= 0)
       relax_group = (int)std::min<int64_t>(
           std::max<int64_t>(std::max<int64_t>(predicted_last, smax) - smin + 1, kFirstGroup), kMaxGroup);
+    // asynchronous: the lowered regions are the invalidated cascades and the insertions' far
+    // sides, as deep as the invalidation rounds that ran
+    if (learned && c.bfs_async && predicted_depth >= 0)
+      relax_group = (int)std::min<int64_t>(std::max<int64_t>(st_inval_rounds + 1, kFirstGroup), kMaxGroup);
     if (L == c.level && hint_relax > 0) relax_group = hint_relax;
     int64_t r0 = rounds;
-    SCOUT_CHECK(relax_levels(c, L, P, smin, smax, c.wa.p, nseeds, epoch, &rounds, &scanned, relax_group));
+    SCOUT_CHECK(relax_levels(c, L, P, smin, smax, c.wa.p, nseeds, epoch, &rounds, &scanned, relax_group,
+                             c.bfs_async));
     st->relax_rounds += rounds - r0;
   }
   st->rounds += rounds;
@@ -1371,6 +1583,15 @@ int scout_bfs_bound(scout_ctx* x, int64_t bound) {
     return SCOUT_ERR_ARG;
   }
   x->c.bfs_bound = bound;
+  return SCOUT_OK;
+}
+
+int scout_bfs_mode(scout_ctx* x, int mode) {
+  if (!x || (mode != SCOUT_BFS_ASYNC && mode != SCOUT_BFS_LEVEL_SYNC)) {
+    set_error("bfs_mode: mode must be SCOUT_BFS_ASYNC or SCOUT_BFS_LEVEL_SYNC");
+    return SCOUT_ERR_ARG;
+  }
+  x->c.bfs_async = mode == SCOUT_BFS_ASYNC;
   return SCOUT_OK;
 }
 

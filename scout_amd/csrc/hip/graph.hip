@@ -170,7 +170,7 @@ constexpr int kApplyLines = 4;
 // move within a batch; only their deletion marks flip).
 __global__ void k_apply(GView g, bool sorted, const int32_t* uv, const uint8_t* ins, const int64_t* pos, int64_t lo,
                         int64_t hi, int conditional, uint8_t* done, uint64_t* nkey, int32_t* ndst, int64_t ncap,
-                        int ks, int64_t* box, const int64_t* skip) {
+                        int ks, int64_t* box, const int64_t* skip, int32_t* ncnt) {
   // the range repeats a pair (*skip, counted by k_dup_count just before): nothing is applied
   // here, and the host applies the range as runs of distinct pairs instead
   if (skip && *skip != 0) return;
@@ -218,6 +218,9 @@ __global__ void k_apply(GView g, bool sorted, const int32_t* uv, const uint8_t* 
         }
       }
     }
+    // each source's new entries are counted here for their placement (k_place_single)
+    if (e1.take) atomicAdd(&ncnt[e1.src], 1);
+    if (e2.take) atomicAdd(&ncnt[e2.src], 1);
     take[2 * q] = e1.take;
     keys[2 * q] = ((uint64_t)(uint32_t)e1.src << ks) | (uint64_t)(i - lo);
     dsts[2 * q] = e1.dst;
@@ -234,61 +237,14 @@ __global__ void k_apply(GView g, bool sorted, const int32_t* uv, const uint8_t* 
 }
 
 // --- inserted rows: appends into the slack pool ---------------------------------------
-
-// Group heads of the sorted new entries: a group is the run of one source vertex.
-// Counts the new entries of every source (ncnt, zero before).
-__global__ void k_src_claim(const uint64_t* nkey, const int64_t* count, int64_t cap, int ks, int32_t* ncnt) {
-  int64_t cnt = min(*count, cap);
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += (int64_t)gridDim.x * blockDim.x)
-    atomicAdd(&ncnt[nkey[i] >> ks], 1);
-}
-
-// Splits the new entries: a source with one new entry is a group of its own, in line order
-// trivially, and goes to the single list; the entries of sources with several go to the multi
-// list, which is sorted by (source, line) before they are placed. Order inside each list does
-// not matter (the multi keys carry the line).
-__global__ void k_src_split(const uint64_t* nkey, const int32_t* ndst, const int64_t* count, int64_t cap, int ks,
-                            const int32_t* ncnt, uint64_t* skey, int32_t* sdst, int64_t* nsingle, uint64_t* mkey,
-                            int32_t* mdst, int64_t* nmulti) {
-  int64_t cnt = min(*count, cap);
-  // the trip count is uniform across the block, so the block-wide appends see every thread
-  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < cnt; base += (int64_t)gridDim.x * blockDim.x) {
-    int64_t i = base + threadIdx.x;
-    bool have = i < cnt;
-    uint64_t key = have ? nkey[i] : 0;
-    int32_t dst = have ? ndst[i] : 0;
-    bool single = have && ncnt[key >> ks] == 1;
-    int64_t ps = block_append(single, key, skey, nsingle, cap);
-    if (ps >= 0) sdst[ps] = dst;
-    int64_t pm = block_append(have && !single, key, mkey, nmulti, cap);
-    if (pm >= 0) mdst[pm] = dst;
-  }
-}
-
-__global__ void k_zero_slot(int64_t* p) { *p = 0; }
-__global__ void k_zero_slots(int64_t* p, int n) {
-  if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
-}
-
-// Zeroes the counts k_src_claim left (for the next batch).
-__global__ void k_src_release(const uint64_t* nkey, int64_t cnt, int ks, int32_t* ncnt) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < cnt) ncnt[nkey[i] >> ks] = 0;
-}
-
-__global__ void k_group_heads(const uint64_t* nkey, int64_t cnt, int ks, int32_t* heads, int64_t* nheads,
-                              int64_t cap) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  bool head = i < cnt && (i == 0 || (nkey[i] >> ks) != (nkey[i - 1] >> ks));
-  block_append(head, (int32_t)i, heads, nheads, cap);
-}
-
-__device__ __forceinline__ int64_t group_end(const uint64_t* nkey, int64_t cnt, int ks, int64_t s) {
-  uint64_t src = nkey[s] >> ks;
-  int64_t e = s + 1;
-  while (e < cnt && (nkey[e] >> ks) == src) e++;
-  return e;
-}
+//
+// A batch's new entries (key src << ks | line offset, value dst) are placed after k_apply
+// without a host round trip: k_apply counts each source's new entries (ncnt); k_place_single
+// places every entry whose source has no other (a group of one, in line order trivially) and
+// lists the others; k_group_small groups those by source in one block, each source's run in
+// line order (so a row keeps the batch's line order); k_place_multi places each run; k_move_rows copies the long rows that
+// outgrew their span. Counts of the lists live in the mailbox, so every launch sizes itself on
+// the device.
 
 __device__ __forceinline__ int32_t grown_capacity(int64_t len) {
   int64_t c = 2 * len;
@@ -299,32 +255,27 @@ __device__ __forceinline__ int32_t grown_capacity(int64_t len) {
 // Rows moved by one thread at most this long; longer ones by k_move_rows (wave per row).
 constexpr int64_t kMoveThread = 64;
 
-// Thread per group (source u, sorted new entries [s, e)): the new entries go after u's row
-// when its span has room; otherwise the row moves to fresh pool space of twice the needed
-// size (reserved with one atomic per block). The pool has room for the worst case (the host
-// checks before the launch); *overflow counts reservations past it all the same.
-__global__ void k_append_groups(int64_t* aoff, int32_t* alen, int32_t* acap, int32_t* acol, int32_t* aown,
-                                const uint64_t* nkey, const int32_t* ndst, int64_t cnt, int ks, const int32_t* heads,
-                                const int64_t* nheads, int64_t* top, int64_t pool_cap, int32_t* moves,
-                                int64_t* nmoves, int64_t* move_pos, int64_t* overflow) {
-  int64_t gi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  int64_t ng = *nheads;
-  int64_t need = 0, s = 0, e = 0;
-  int32_t u = -1, len = 0;
-  if (gi < ng) {
-    s = heads ? heads[gi] : gi;  // no head list: every entry is a group of its own
-    e = heads ? group_end(nkey, cnt, ks, s) : s + 1;
-    u = (int32_t)(nkey[s] >> ks);
-    len = alen[u];
-    if (len + (e - s) > acap[u]) need = grown_capacity(len + (e - s));
-  }
-  int64_t pos = block_reserve(need, top);
-  if (gi >= ng) return;
-  int64_t k = e - s;
+// Rows moving to fresh pool space that one thread does not copy: row u, its new span at pos,
+// and its k new entries at dst[s..s + k) of the single entries (src = -1 - s, k = 1) or of the
+// grouped multi entries (src = s).
+struct MoveList {
+  int32_t* u;
+  int64_t* pos;
+  int64_t* src;
+  int32_t* k;
+  int64_t* count;
+};
+
+// Places k new entries of row u (dst[0..k)) after its len entries, or moves the row to the
+// span of `need` slots reserved at pos (need = 0: the span has room).
+__device__ __forceinline__ void place_group(int64_t* aoff, int32_t* alen, int32_t* acap, int32_t* acol, int32_t* aown,
+                                            const int32_t* dst, int64_t k, int32_t u, int32_t len, int64_t need,
+                                            int64_t pos, int64_t pool_cap, int64_t src, const MoveList& mv,
+                                            int64_t* overflow) {
   if (need == 0) {
     int64_t at = aoff[u] + len;
     for (int64_t j = 0; j < k; j++) {
-      acol[at + j] = ndst[s + j];
+      acol[at + j] = dst[j];
       aown[at + j] = u;
     }
     alen[u] = (int32_t)(len + k);
@@ -337,36 +288,217 @@ __global__ void k_append_groups(int64_t* aoff, int32_t* alen, int32_t* acap, int
       aown[pos + j] = u;
     }
     for (int64_t j = 0; j < k; j++) {
-      acol[pos + len + j] = ndst[s + j];
+      acol[pos + len + j] = dst[j];
       aown[pos + len + j] = u;
     }
     aoff[u] = pos;
     acap[u] = (int32_t)need;
     alen[u] = (int32_t)(len + k);
   } else {
-    int64_t slot = (int64_t)atomicAdd((unsigned long long*)nmoves, 1ULL);
-    moves[slot] = (int32_t)gi;
-    move_pos[gi] = pos;
+    int64_t slot = (int64_t)atomicAdd((unsigned long long*)mv.count, 1ULL);
+    mv.u[slot] = u;
+    mv.pos[slot] = pos;
+    mv.src[slot] = src;
+    mv.k[slot] = (int32_t)k;
   }
 }
 
-// Wave per long row that moves: old entries, then the new ones, into the reserved span.
+// Thread per new entry (nkey/ndst, *count of them): an entry whose source has no other new
+// entry is placed here and its source's count goes back to zero; the others go to the multi
+// list (mkey/mdst, *nmulti). The span of a row that must move is reserved on *top with one
+// atomic per block; the pool has room for the worst case (checked on the host before the
+// batch), and *overflow counts reservations past it all the same.
+__global__ void k_place_single(int64_t* aoff, int32_t* alen, int32_t* acap, int32_t* acol, int32_t* aown,
+                               const uint64_t* nkey, const int32_t* ndst, const int64_t* count, int64_t cap, int ks,
+                               int32_t* ncnt, uint64_t* mkey, int32_t* mdst, int64_t* nmulti, int64_t* top,
+                               int64_t pool_cap, MoveList mv, int64_t* overflow) {
+  int64_t cnt = min(*count, cap);
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  bool have = i < cnt;
+  uint64_t key = have ? nkey[i] : 0;
+  int32_t dst = have ? ndst[i] : 0;
+  int32_t u = (int32_t)(key >> ks);
+  bool single = have && ncnt[u] == 1;
+  int64_t pm = block_append(have && !single, key, mkey, nmulti, cap);
+  if (pm >= 0) mdst[pm] = dst;
+  int32_t len = 0;
+  int64_t need = 0;
+  if (single) {
+    len = alen[u];
+    if (len + 1 > acap[u]) need = grown_capacity(len + 1);
+  }
+  int64_t pos = block_reserve(need, top);
+  if (!single) return;
+  ncnt[u] = 0;
+  place_group(aoff, alen, acap, acol, aown, &ndst[i], 1, u, len, need, pos, pool_cap, -1 - i, mv, overflow);
+}
+
+// Multi entries one block groups (at most kSmallSort / 2 sources, each with two or more
+// entries, so the LDS table of kGroupSlots sources is at most half full), and the largest run
+// it ranks by counting (a run of k costs k reads per entry).
+constexpr int kSmallSort = 8192;
+constexpr int kGroupSlots = 8192;
+static_assert(kGroupSlots == 1 << 13, "the hash keeps the top 13 bits");
+constexpr int kGroupMaxRun = 256;
+constexpr int kSortThreads = 1024;
+constexpr int kSortPerThread = kSmallSort / kSortThreads;
+
+// One block: groups the *count multi entries of kin/vin by source into kout/vout, each
+// source's run in line order (the order a sort by key gives inside each run; runs come in any
+// order). Sources go to an LDS hash table (slot, run size, run start); each entry is written
+// into its run at a cursor, then moved to run start + the number of its run's keys below its
+// own. More than kSmallSort entries, or a run longer than kGroupMaxRun: sets *too_many and
+// writes nothing (the host sorts them then). tmp_key holds the unordered runs.
+__global__ __launch_bounds__(kSortThreads) void k_group_small(const uint64_t* kin, const int32_t* vin,
+                                                              const int64_t* count, int64_t cap, int ks,
+                                                              uint64_t* tmp_key, uint64_t* kout,
+                                                              int32_t* vout, int64_t* too_many) {
+  __shared__ int32_t hsrc[kGroupSlots];
+  __shared__ int32_t hcnt[kGroupSlots];
+  __shared__ int32_t hoff[kGroupSlots];
+  __shared__ int32_t wsum[kSortThreads / kWave];
+  __shared__ int32_t longest;
+  int64_t n = min(*count, cap);
+  if (n > kSmallSort) {
+    if (threadIdx.x == 0) *too_many = 1;
+    return;
+  }
+  int t = (int)threadIdx.x;
+  for (int h = t; h < kGroupSlots; h += kSortThreads) {
+    hsrc[h] = -1;
+    hcnt[h] = 0;
+  }
+  if (t == 0) longest = 0;
+  __syncthreads();
+  uint64_t key[kSortPerThread];
+  int32_t dst[kSortPerThread], slot[kSortPerThread];
+#pragma unroll
+  for (int r = 0; r < kSortPerThread; r++) {
+    int64_t i = (int64_t)r * kSortThreads + t;
+    slot[r] = -1;
+    if (i < n) {
+      key[r] = kin[i];
+      dst[r] = vin[i];
+      int32_t src = (int32_t)(key[r] >> ks);
+      uint32_t h = ((uint32_t)src * 2654435761u) >> (32 - 13);  // kGroupSlots = 2^13
+      while (true) {
+        int32_t prev = atomicCAS(&hsrc[h], -1, src);
+        if (prev == -1 || prev == src) break;
+        h = (h + 1) & (kGroupSlots - 1);
+      }
+      slot[r] = (int32_t)h;
+      atomicAdd(&hcnt[h], 1);
+    }
+  }
+  __syncthreads();
+  // run starts: exclusive scan of the run sizes (contiguous slots per thread, then the waves)
+  constexpr int kPer = kGroupSlots / kSortThreads;
+  int run = 0, big = 0;
+#pragma unroll
+  for (int q = 0; q < kPer; q++) {
+    int c = hcnt[t * kPer + q];
+    run += c;
+    big = max(big, c);
+  }
+  if (big > 0) atomicMax(&longest, big);
+  int x = run;
+  for (int off = 1; off < kWave; off <<= 1) {
+    int y = __shfl_up(x, off);
+    if (lane_id() >= off) x += y;
+  }
+  if (lane_id() == kWave - 1) wsum[t / kWave] = x;
+  __syncthreads();
+  if (t == 0) {
+    int acc = 0;
+    for (int w = 0; w < kSortThreads / kWave; w++) {
+      int c = wsum[w];
+      wsum[w] = acc;
+      acc += c;
+    }
+  }
+  __syncthreads();
+  int start = wsum[t / kWave] + x - run;
+#pragma unroll
+  for (int q = 0; q < kPer; q++) {
+    hoff[t * kPer + q] = start;
+    start += hcnt[t * kPer + q];
+  }
+  __syncthreads();
+  if (longest > kGroupMaxRun) {
+    if (t == 0) *too_many = 1;
+    return;
+  }
+  // each entry into its run at the run's cursor (hoff moves to the run's end)
+  int32_t at[kSortPerThread];
+#pragma unroll
+  for (int r = 0; r < kSortPerThread; r++) {
+    at[r] = -1;
+    if (slot[r] >= 0) {
+      at[r] = atomicAdd(&hoff[slot[r]], 1);
+      tmp_key[at[r]] = key[r];
+    }
+  }
+  __threadfence_block();
+  __syncthreads();
+  // rank inside the run: its keys below this one (keys are distinct)
+#pragma unroll
+  for (int r = 0; r < kSortPerThread; r++) {
+    if (slot[r] < 0) continue;
+    int32_t len = hcnt[slot[r]];
+    int32_t lo = hoff[slot[r]] - len;
+    int32_t rank = 0;
+    for (int32_t j = lo; j < lo + len; j++) rank += tmp_key[j] < key[r];
+    kout[lo + rank] = key[r];
+    vout[lo + rank] = dst[r];
+  }
+}
+
+// The grouped multi entries (skey/sdst, *count of them): the first entry of each source's run
+// places the run (one thread per run, a block-wide reservation per pass) and zeroes the
+// source's count. Nothing when *skip is set (the entries were not sorted).
+__global__ void k_place_multi(int64_t* aoff, int32_t* alen, int32_t* acap, int32_t* acol, int32_t* aown,
+                              const uint64_t* skey, const int32_t* sdst, const int64_t* count, int64_t cap, int ks,
+                              int32_t* ncnt, int64_t* top, int64_t pool_cap, MoveList mv, int64_t* overflow,
+                              const int64_t* skip) {
+  if (skip && *skip) return;
+  int64_t cnt = min(*count, cap);
+  // the trip count is uniform across the block, so the block-wide reservations see every thread
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < cnt; base += (int64_t)gridDim.x * blockDim.x) {
+    int64_t i = base + threadIdx.x;
+    bool head = i < cnt && (i == 0 || (skey[i] >> ks) != (skey[i - 1] >> ks));
+    int32_t u = -1, len = 0;
+    int64_t k = 0, need = 0;
+    if (head) {
+      u = (int32_t)(skey[i] >> ks);
+      int64_t e = i + 1;
+      while (e < cnt && (int32_t)(skey[e] >> ks) == u) e++;
+      k = e - i;
+      len = alen[u];
+      if (len + k > acap[u]) need = grown_capacity(len + k);
+    }
+    int64_t pos = block_reserve(need, top);
+    if (head) {
+      ncnt[u] = 0;
+      place_group(aoff, alen, acap, acol, aown, &sdst[i], k, u, len, need, pos, pool_cap, i, mv, overflow);
+    }
+  }
+}
+
+// Wave per long row that moves: its entries, then the new ones, into the reserved span.
 __global__ void k_move_rows(int64_t* aoff, int32_t* alen, int32_t* acap, int32_t* acol, int32_t* aown,
-                            const uint64_t* nkey, const int32_t* ndst, int64_t cnt, int ks, const int32_t* heads,
-                            const int32_t* moves, const int64_t* nmoves, const int64_t* move_pos) {
-  int64_t nm = *nmoves;
+                            const int32_t* ndst, const int32_t* sdst, MoveList mv) {
+  int64_t nm = *mv.count;
   for (int64_t i = global_wave(); i < nm; i += total_waves()) {
-    int32_t gi = moves[i];
-    int64_t s = heads ? heads[gi] : gi;
-    int64_t e = heads ? group_end(nkey, cnt, ks, s) : s + 1;
-    int32_t u = (int32_t)(nkey[s] >> ks);
-    int64_t len = alen[u], k = e - s, old = aoff[u], pos = move_pos[gi];
+    int32_t u = mv.u[i];
+    int64_t src = mv.src[i], k = mv.k[i], pos = mv.pos[i];
+    const int32_t* d = src < 0 ? ndst + (-1 - src) : sdst + src;
+    int64_t len = alen[u], old = aoff[u];
     for (int64_t j = lane_id(); j < len; j += kWave) {
       acol[pos + j] = acol[old + j];
       aown[pos + j] = u;
     }
     for (int64_t j = lane_id(); j < k; j += kWave) {
-      acol[pos + len + j] = ndst[s + j];
+      acol[pos + len + j] = d[j];
       aown[pos + len + j] = u;
     }
     if (lane_id() == 0) {
@@ -375,6 +507,19 @@ __global__ void k_move_rows(int64_t* aoff, int32_t* alen, int32_t* acap, int32_t
       alen[u] = (int32_t)(len + k);
     }
   }
+}
+
+// Mailbox slots of an apply: [0] new entries, [1] holes created, [3] rows listed for
+// k_move_rows, [4] pool top, [5] reservations past the pool; the repeated-pair count k_apply
+// checks, the multi-entry count and the flag of a multi list k_group_small leaves to the host.
+constexpr int kRepeatSlot = 48, kMultiSlot = 51, kSortSlot = 52;
+
+// First kernel of an apply: zeroes the counts and flags above and sets the pool top.
+__global__ void k_apply_begin(int64_t* box, int64_t pool_top) {
+  int t = (int)threadIdx.x;
+  if (t < 6) box[t] = t == 4 ? pool_top : 0;
+  if (t == 6) box[kMultiSlot] = 0;
+  if (t == 7) box[kSortSlot] = 0;
 }
 
 // --- inserted rows: full re-pack (holes dropped, fresh slack) --------------------------
@@ -552,115 +697,116 @@ int bits_for(int64_t x) {
   return b;
 }
 
-// Mailbox slots of the repeated-pair count that k_apply checks on the device, and of the
-// single and multi entry counts of k_src_split.
-constexpr int kRepeatSlot = 48, kSingleSlot = 50, kMultiSlot = 51;
 
-// Places the batch's new inserted-row entries as split by k_src_split: ns whose source has no
-// other (nkey2/ndst2, counted in box[kSingleSlot]) and nm whose source has several (mkey/mdst).
-int insert_new(Ctx& c, int64_t cnt_new, int ks, int64_t ns, int64_t nm) {
-  DevGraph& g = c.g;
-  if (nm > 0) {
-    // Stable by (src, line): the new entries of a row keep the batch's line order. Keys are
-    // src << ks | line offset, so only their low ks + bits(n - 1) bits are sorted. The sorted
-    // entries go to nkey/ndst, whose unsplit contents are no longer needed.
-    SCOUT_CHECK(sort_pairs_u64(c, g.mkey.p, g.nkey.p, g.mdst.p, g.ndst.p, nm,
-                               ks + bits_for(std::max<int64_t>(g.n - 1, 0))));
-  }
-  // Worst case: every touched row moves to twice its new size, and the rows hold at most the
-  // pool's used part: 2 * (A + cnt_new) + 4 per group <= 2 * A + 6 * cnt_new.
-  int64_t worst = 2 * g.A + 6 * cnt_new + 1024;
-  if (g.A + worst > pool_cap(g)) {
-    if (g.A > 2 * g.live + (1 << 20)) SCOUT_CHECK(repack_inserted(c, 2 * g.live + 6 * cnt_new));
-    worst = 2 * g.A + 6 * cnt_new + 1024;
-    if (g.A + worst > pool_cap(g)) SCOUT_CHECK(repack_inserted(c, 2 * (g.A + worst)));
-  }
-  SCOUT_CHECK(g.heads.reserve(cnt_new));
-  SCOUT_CHECK(g.moves.reserve(cnt_new));
-  SCOUT_CHECK(g.move_pos.reserve(cnt_new));
-  // box: [2] groups, [3] moved rows, [4] pool top, [5] overflow
-  int64_t init[4] = {0, 0, g.A, 0};
-  SCOUT_TRY(hipMemcpyAsync(c.d_box + 2, init, sizeof(init), hipMemcpyHostToDevice, c.s));
-  // the single entries: each one a group of its own (no head list)
-  if (ns > 0) {
-    k_append_groups<<<grid_for(ns), kBlock, 0, c.s>>>(g.aoff.p, g.alen.p, g.acap.p, g.acol.p, g.aown.p, g.nkey2.p,
-                                                      g.ndst2.p, ns, ks, nullptr, c.d_box + kSingleSlot, c.d_box + 4,
-                                                      pool_cap(g), g.moves.p, c.d_box + 3, g.move_pos.p, c.d_box + 5);
-    k_move_rows<<<kListGrid, kBlock, 0, c.s>>>(g.aoff.p, g.alen.p, g.acap.p, g.acol.p, g.aown.p, g.nkey2.p,
-                                               g.ndst2.p, ns, ks, nullptr, g.moves.p, c.d_box + 3, g.move_pos.p);
-  }
-  // the sorted multi entries: one group per source (their sources differ from the singles')
-  if (nm > 0) {
-    k_zero_slot<<<1, 1, 0, c.s>>>(c.d_box + 3);
-    k_group_heads<<<grid_for(nm), kBlock, 0, c.s>>>(g.nkey.p, nm, ks, g.heads.p, c.d_box + 2, g.heads.cap);
-    k_append_groups<<<grid_for(nm), kBlock, 0, c.s>>>(g.aoff.p, g.alen.p, g.acap.p, g.acol.p, g.aown.p, g.nkey.p,
-                                                      g.ndst.p, nm, ks, g.heads.p, c.d_box + 2, c.d_box + 4,
-                                                      pool_cap(g), g.moves.p, c.d_box + 3, g.move_pos.p, c.d_box + 5);
-    k_move_rows<<<kListGrid, kBlock, 0, c.s>>>(g.aoff.p, g.alen.p, g.acap.p, g.acol.p, g.aown.p, g.nkey.p,
-                                               g.ndst.p, nm, ks, g.heads.p, g.moves.p, c.d_box + 3, g.move_pos.p);
-  }
-  // the counts of k_src_claim go back to zero for the next batch
-  if (ns > 0) k_src_release<<<grid_for(ns), kBlock, 0, c.s>>>(g.nkey2.p, ns, ks, g.ncnt.p);
-  if (nm > 0) k_src_release<<<grid_for(nm), kBlock, 0, c.s>>>(g.nkey.p, nm, ks, g.ncnt.p);
-  SCOUT_TRY(hipMemcpyAsync(c.h_box + 2, c.d_box + 2, 4 * sizeof(int64_t), hipMemcpyDeviceToHost, c.s));
-  SCOUT_TRY(hipStreamSynchronize(c.s));
-  c.readbacks++;
-  if (c.h_box[5] != 0) {
-    set_error("apply: inserted-row pool overflow");
-    return SCOUT_ERR_STATE;
-  }
-  g.A = c.h_box[4];
-  g.live += cnt_new;
-  return SCOUT_OK;
+// Blocks of k_place_multi (grid-stride: the multi entries are few, their count is on the device).
+constexpr unsigned kPlaceGrid = 256;
+
+// Placement counters reset for a second pass over the multi entries.
+__global__ void k_place_reset(int64_t* box, int64_t pool_top) {
+  box[3] = 0;
+  box[4] = pool_top;
+  box[5] = 0;
 }
 
-// Applies lines [lo, hi). With `skip` (a device count of repeated pairs in the range, read back
-// with the new-entry count), nothing is applied when the count is not zero, and *skipped says so.
+MoveList move_list(Ctx& c) {
+  DevGraph& g = c.g;
+  return MoveList{g.moves.p, g.move_pos.p, g.move_src.p, g.move_k.p, c.d_box + 3};
+}
+
+// Applies lines [lo, hi): k_apply, then the new inserted-row entries placed as described above
+// the placement kernels, with one read-back at the end. With `skip` (a device count of repeated
+// pairs in the range, read back with the rest), nothing is applied when the count is not zero,
+// and *skipped says so.
 int apply_range(Ctx& c, Channel& ch, int64_t lo, int64_t hi, int conditional, const int64_t* skip = nullptr,
                 bool* skipped = nullptr) {
   DevGraph& g = c.g;
-  int64_t cap = 4 * (hi - lo) + 16;
+  int64_t m = hi - lo;
+  int64_t cap = 4 * m + 16;
   SCOUT_CHECK(g.nkey.reserve(cap));
   SCOUT_CHECK(g.ndst.reserve(cap));
-  SCOUT_TRY(hipMemsetAsync(c.d_box, 0, 4 * sizeof(int64_t), c.s));
-  SCOUT_CHECK(prof_begin(c, PROF_APPLY));
-  const int64_t* pos = (ch.pos_gen == g.base_gen && ch.pos_lo <= lo && hi <= ch.pos_hi) ? ch.d_pos.p : nullptr;
-  int ks = bits_for(std::max<int64_t>(hi - lo - 1, 0));  // key bits of a line offset
-  if (hi > lo)
-    k_apply<<<grid_for(hi - lo, kBlock * kApplyLines), kBlock, 0, c.s>>>(g.view(), g.sorted, ch.d_uv.p, ch.d_ins.p,
-                                                                         pos, lo, hi, conditional, ch.d_done.p,
-                                                                         g.nkey.p, g.ndst.p, g.nkey.cap, ks, c.d_box,
-                                                                         skip);
-  // algorithmic bytes per line: the pair and its label and done flag (10 B), both rows'
-  // base and inserted offsets (32 B) and the entries flipped or appended (8 B)
-  SCOUT_CHECK(prof_end(c, PROF_APPLY, 50 * (hi - lo)));
+  SCOUT_CHECK(g.nkey2.reserve(cap));
+  SCOUT_CHECK(g.ndst2.reserve(cap));
+  SCOUT_CHECK(g.mkey.reserve(cap));
+  SCOUT_CHECK(g.mdst.reserve(cap));
+  SCOUT_CHECK(g.moves.reserve(cap));
+  SCOUT_CHECK(g.move_pos.reserve(cap));
+  SCOUT_CHECK(g.move_src.reserve(cap));
+  SCOUT_CHECK(g.move_k.reserve(cap));
   if (!g.ncnt.p || g.ncnt.cap < (int64_t)g.n + 1) {
     g.ncnt.release();
     SCOUT_CHECK(g.ncnt.reserve((int64_t)g.n + 1));
     SCOUT_TRY(hipMemsetAsync(g.ncnt.p, 0, (size_t)g.ncnt.cap * sizeof(int32_t), c.s));
   }
-  SCOUT_CHECK(g.nkey2.reserve(g.nkey.cap));
-  SCOUT_CHECK(g.ndst2.reserve(g.nkey.cap));
-  SCOUT_CHECK(g.mkey.reserve(g.nkey.cap));
-  SCOUT_CHECK(g.mdst.reserve(g.nkey.cap));
-  static_assert(kMultiSlot == kSingleSlot + 1, "split counts in adjacent slots");
-  k_zero_slots<<<1, 64, 0, c.s>>>(c.d_box + kSingleSlot, 2);
-  k_src_claim<<<kListGrid, kBlock, 0, c.s>>>(g.nkey.p, c.d_box, g.nkey.cap, ks, g.ncnt.p);
-  k_src_split<<<kListGrid, kBlock, 0, c.s>>>(g.nkey.p, g.ndst.p, c.d_box, g.nkey.cap, ks, g.ncnt.p, g.nkey2.p,
-                                              g.ndst2.p, c.d_box + kSingleSlot, g.mkey.p, g.mdst.p,
-                                              c.d_box + kMultiSlot);
-  SCOUT_CHECK(box_read(c, kMultiSlot + 1));
+  // Room for the worst case before anything is placed: at most two new entries per line, and
+  // every touched row moving to twice its new size (rows hold at most the pool's used part):
+  // 2 * (A + new) + 4 per group <= 2 * A + 6 * new.
+  int64_t bound = 2 * m;
+  int64_t worst = 2 * g.A + 6 * bound + 1024;
+  if (g.A + worst > pool_cap(g)) {
+    if (g.A > 2 * g.live + (1 << 20)) SCOUT_CHECK(repack_inserted(c, 2 * g.live + 6 * bound));
+    worst = 2 * g.A + 6 * bound + 1024;
+    if (g.A + worst > pool_cap(g)) SCOUT_CHECK(repack_inserted(c, 2 * (g.A + worst)));
+  }
+  k_apply_begin<<<1, 64, 0, c.s>>>(c.d_box, g.A);
+  SCOUT_CHECK(prof_begin(c, PROF_APPLY));
+  const int64_t* pos = (ch.pos_gen == g.base_gen && ch.pos_lo <= lo && hi <= ch.pos_hi) ? ch.d_pos.p : nullptr;
+  int ks = bits_for(std::max<int64_t>(m - 1, 0));  // key bits of a line offset
+  if (m > 0)
+    k_apply<<<grid_for(m, kBlock * kApplyLines), kBlock, 0, c.s>>>(g.view(), g.sorted, ch.d_uv.p, ch.d_ins.p, pos, lo,
+                                                                   hi, conditional, ch.d_done.p, g.nkey.p, g.ndst.p,
+                                                                   g.nkey.cap, ks, c.d_box, skip, g.ncnt.p);
+  // algorithmic bytes per line: the pair and its label and done flag (10 B), both rows'
+  // base and inserted offsets (32 B) and the entries flipped or appended (8 B)
+  SCOUT_CHECK(prof_end(c, PROF_APPLY, 50 * m));
+  MoveList mv = move_list(c);
+  int64_t pcap = pool_cap(g);
+  if (m > 0) {
+    k_place_single<<<grid_for(bound), kBlock, 0, c.s>>>(g.aoff.p, g.alen.p, g.acap.p, g.acol.p, g.aown.p, g.nkey.p,
+                                                       g.ndst.p, c.d_box, cap, ks, g.ncnt.p, g.mkey.p, g.mdst.p,
+                                                       c.d_box + kMultiSlot, c.d_box + 4, pcap, mv, c.d_box + 5);
+    k_group_small<<<1, kSortThreads, 0, c.s>>>(g.mkey.p, g.mdst.p, c.d_box + kMultiSlot, cap, ks, g.nkey.p,
+                                               g.nkey2.p, g.ndst2.p, c.d_box + kSortSlot);
+    k_place_multi<<<kPlaceGrid, kBlock, 0, c.s>>>(g.aoff.p, g.alen.p, g.acap.p, g.acol.p, g.aown.p, g.nkey2.p,
+                                                  g.ndst2.p, c.d_box + kMultiSlot, cap, ks, g.ncnt.p, c.d_box + 4, pcap,
+                                                  mv, c.d_box + 5, c.d_box + kSortSlot);
+    k_move_rows<<<kListGrid, kBlock, 0, c.s>>>(g.aoff.p, g.alen.p, g.acap.p, g.acol.p, g.aown.p, g.ndst.p,
+                                               g.ndst2.p, mv);
+  }
+  SCOUT_CHECK(box_read(c, kSortSlot + 1));
   if (skip && c.h_box[skip - c.d_box] != 0) {
     *skipped = true;
     return SCOUT_OK;
   }
   int64_t cnt_new = c.h_box[0];
-  g.holes += c.h_box[1];
-  if (cnt_new > g.nkey.cap) {
+  if (cnt_new > cap) {
     set_error("apply: inserted-entry staging overflow");
     return SCOUT_ERR_STATE;
   }
-  if (cnt_new > 0) SCOUT_CHECK(insert_new(c, cnt_new, ks, c.h_box[kSingleSlot], c.h_box[kMultiSlot]));
+  if (c.h_box[5] != 0) {
+    set_error("apply: inserted-row pool overflow");
+    return SCOUT_ERR_STATE;
+  }
+  g.holes += c.h_box[1];
+  g.A = c.h_box[4];
+  if (c.h_box[kSortSlot] != 0) {
+    // more multi entries than one block sorts: the library sorts them, then they are placed
+    int64_t nm = c.h_box[kMultiSlot];
+    SCOUT_CHECK(sort_pairs_u64(c, g.mkey.p, g.nkey2.p, g.mdst.p, g.ndst2.p, nm,
+                               ks + bits_for(std::max<int64_t>(g.n - 1, 0))));
+    k_place_reset<<<1, 1, 0, c.s>>>(c.d_box, g.A);
+    k_place_multi<<<kPlaceGrid, kBlock, 0, c.s>>>(g.aoff.p, g.alen.p, g.acap.p, g.acol.p, g.aown.p, g.nkey2.p,
+                                                  g.ndst2.p, c.d_box + kMultiSlot, cap, ks, g.ncnt.p, c.d_box + 4, pcap,
+                                                  mv, c.d_box + 5, nullptr);
+    k_move_rows<<<kListGrid, kBlock, 0, c.s>>>(g.aoff.p, g.alen.p, g.acap.p, g.acol.p, g.aown.p, g.ndst.p,
+                                               g.ndst2.p, mv);
+    SCOUT_CHECK(box_read(c, 6));
+    if (c.h_box[5] != 0) {
+      set_error("apply: inserted-row pool overflow");
+      return SCOUT_ERR_STATE;
+    }
+    g.A = c.h_box[4];
+  }
+  g.live += cnt_new;
   if (g.holes > (g.live >> 1) + (1 << 20)) SCOUT_CHECK(repack_inserted(c, 2 * g.live + (1 << 20)));
   return SCOUT_OK;
 }
@@ -701,9 +847,10 @@ int graph_free(Ctx& c) {
   g.acol2.release();
   g.aown.release();
   g.aown2.release();
-  g.heads.release();
   g.moves.release();
+  g.move_k.release();
   g.move_pos.release();
+  g.move_src.release();
   g.nkey.release();
   g.nkey2.release();
   g.ndst.release();

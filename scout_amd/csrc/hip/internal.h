@@ -134,9 +134,9 @@ struct DevGraph {
   DevBuf<int64_t> aoff, aoff2;
   DevBuf<int32_t> alen, acap, acap2;
   DevBuf<int32_t> acol, acol2, aown, aown2;
-  // groups of this batch's new entries (one per source vertex) and moved rows
-  DevBuf<int32_t> heads, moves;
-  DevBuf<int64_t> move_pos;
+  // rows of this batch's new entries that move to fresh pool space (row, span, entries)
+  DevBuf<int32_t> moves, move_k;
+  DevBuf<int64_t> move_pos, move_src;
   // new inserted-row entries of the current batch: key = src << 32 | line
   DevBuf<uint64_t> nkey, nkey2, mkey;
   DevBuf<int32_t> ndst, ndst2, mdst;
@@ -227,10 +227,11 @@ struct Ctx {
   int32_t* cc_level = nullptr;   // BFS levels from vertex 0 kept for CC when BFS is not maintained
   int32_t* cc_parent = nullptr;  // parents for cc_level
   int64_t apply_seq = 0;         // batches applied to the graph
-  int64_t bfs_bound = 0;
-  double fetch_ms = 0;           // copy-stream time of streamed base-entry chunks (storage-resident mode)
-  int64_t fetch_bytes = 0;         // scout_bfs_bound: invalidated vertices per deletion candidate past which a
+  int64_t bfs_bound = 0;          // scout_bfs_bound: invalidated vertices per deletion candidate past which a
                                  // batch's BFS repair is finished by a full recompute (0 = never)
+  bool bfs_async = true;         // scout_bfs_mode: asynchronous (true) or level-synchronous BFS rounds
+  double fetch_ms = 0;           // copy-stream time of streamed base-entry chunks (storage-resident mode)
+  int64_t fetch_bytes = 0;       // bytes of those chunks
   int64_t level_seq = -1;        // apply_seq at which `level` was last made exact
   int64_t cc_level_seq = -1;     // same for cc_level
   int64_t cc_components = -1;    // component count after the last CC repair
@@ -264,6 +265,7 @@ struct Ctx {
   float last_repair_ms = 0.f;
   int64_t readbacks = 0;         // host read-backs of device counters so far
   double pred_over = 0;          // running mean of how far the predicted last level overshoots
+  double pred_bias = 0;          // running mean of (rounds run - predicted depth), asynchronous BFS
   int64_t pred_cut = 0;          // split head: expected vertices the batch cuts off vertex 0's component
   int64_t pred_cut_seq = -1;     // the batch (apply_seq) pred_cut was predicted for
   int hint_inval = 0, hint_relax = 0;  // scout_bfs_schedule: first group sizes of the next repair
@@ -275,10 +277,11 @@ struct Ctx {
 constexpr int kBoxSlots = 64;
 // Mailbox slots 44 and 45: each channel's repeated-pair count from its last full classify.
 constexpr int kChannelRepeatSlot = 44;
-// Long-row code paths counted on the device (scout_path_counts): chunk tasks of split rows
-// per kernel family, and invalid rows whose children did not fit the LDS list.
+// Code paths counted on the device (scout_path_counts): chunk tasks of split rows per kernel
+// family, invalid rows whose children did not fit the LDS list, and vertices the asynchronous
+// BFS invalidation made wait a round.
 enum PathId { PATH_INVAL_TASKS = 0, PATH_INVAL_PUSH_TASKS, PATH_INVAL_KIDS_OVERFLOW, PATH_CC_TASKS,
-              PATH_RELAX_TASKS, PATH_TENTATIVE_TASKS, PATH_CC_WALK_ROWS, kPathCount };
+              PATH_RELAX_TASKS, PATH_TENTATIVE_TASKS, PATH_CC_WALK_ROWS, PATH_INVAL_DEFERRED, kPathCount };
 
 // prims.hip (library scans/sorts)
 int scan_exclusive_i64(Ctx& c, const int64_t* in, int64_t* out, int64_t count);

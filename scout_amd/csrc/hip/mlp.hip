@@ -338,24 +338,26 @@ __global__ void k_pred_init(int64_t* box) {
 }
 
 // Aggregates of the predictions: box[0] = sum of predicted cascade sizes (rounded), box[1] =
-// the largest predicted last level L[c] + depth - 1, box[3] = the expected number of vertices
+// the largest predicted last level L[c] + depth - 1, box[2] = minus the largest predicted depth
+// (levels from a candidate to the deepest vertex it invalidates), box[3] = the expected number of vertices
 // cut off vertex 0's component in sixteenths: per candidate P(c ends unreached) times its
 // predicted cascade (c and the vertices it invalidates).
 __global__ void k_mlp_aggregate(const float* out, const int32_t* lvl0, const int64_t* ncand, int64_t sample,
                                 int64_t* box) {
   int64_t ns = *ncand < sample ? *ncand : sample;
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  int64_t size = 0, last = -1, cut16 = 0;
+  int64_t size = 0, last = -1, cut16 = 0, neg_depth = INT64_MAX;
   if (i < ns) {
     float s = expm1f(fmaxf(out[i * kMlpOut + 1], 0.f));
     float d = fmaxf(out[i * kMlpOut + 2], 0.f);
     float p = 1.f / (1.f + expf(-fminf(fmaxf(out[i * kMlpOut + 0], -30.f), 30.f)));
     size = (int64_t)fminf(s + 0.5f, 1e15f);
     last = lvl0[i] + (int64_t)ceilf(d) - 1;
+    neg_depth = -(int64_t)ceilf(d);
     cut16 = (int64_t)fminf(16.f * p * (1.f + s) + 0.5f, 1e15f);
   }
   block_add(size, &box[0]);
-  block_min_max(INT64_MAX, last, &box[2], &box[1]);
+  block_min_max(neg_depth, last, &box[2], &box[1]);
   block_add(cut16, &box[3]);
 }
 

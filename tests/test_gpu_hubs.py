@@ -165,7 +165,7 @@ def test_hub_rows_bfs_cc_exact_every_batch(tmp_path):
     assert np.array_equal(eng.download(sa.STATE_LABEL), og.cc())
     paths = eng.path_counts()
     say(f"[hubs] path counts {paths}")
-    for name in sa.PATH_NAMES:
+    for name in sa.LONG_ROW_PATHS:
         assert paths[name] > 0, f"the {name} path never ran"
     assert cc_only.path_counts()["cc_tasks"] > 0 and cc_only.path_counts()["cc_walk_rows"] == 0
     eng.close()

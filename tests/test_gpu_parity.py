@@ -104,6 +104,98 @@ def test_bfs_cc_exact_every_batch(data, name, mode):
     eng.close()
 
 
+@pytest.mark.parametrize("name", ["sample", "er", "rmat", "deg", "heavy"])
+def test_bfs_level_sync_exact_every_batch(data, name):
+    """The level-synchronous BFS round order (one level per round) gives the oracle's levels
+    and labels after every batch too; the tests above run the default asynchronous order."""
+    egr, bdir, nb = data[name]
+    say(f"[bfs level-sync] {name}")
+    eng, g = engine_for(egr)
+    eng.bfs_mode(sa.BFS_LEVEL_SYNC)
+    og = OracleGraph(egr)
+    eng.upload(sa.STATE_LEVEL, og.bfs(0))
+    eng.upload(sa.STATE_LABEL, og.cc())
+    for k, uv in batches(bdir, nb, g.n):
+        eng.stage(uv)
+        ins = eng.classify(sa.CLASSIFY_CURRENT)
+        og.apply(uv, og.classify(uv))
+        eng.apply(False)
+        eng.repair_bfs()
+        eng.repair_cc()
+        lv, want_lv = eng.download(sa.STATE_LEVEL), og.bfs(0)
+        assert np.array_equal(lv, want_lv), f"{name} batch {k}: {np.count_nonzero(lv != want_lv)} levels differ"
+        assert np.array_equal(eng.download(sa.STATE_LABEL), og.cc()), f"{name} batch {k}: labels differ"
+        assert ins is not None
+    eng.close()
+
+
+def test_inserted_rows_keep_line_order_past_the_block_sort():
+    """New inserted-row entries are placed in line order per row. A batch of 30,000 insertions
+    over 1,500 vertices gives almost every source several new entries (more than one block
+    groups, so the library sort runs) and rows of more than 64 entries move to fresh spans; a
+    mixed batch then appends to, tombstones and forgets those entries (grouped by one block);
+    a last batch gives one source 400 new entries (a run longer than the block ranks: the
+    library sort again). The logical rows equal the oracle's after each batch."""
+    rng = np.random.default_rng(21)
+    n = 1500
+    pairs = rng.integers(0, n, size=(3000, 2))
+    pairs = pairs[pairs[:, 0] != pairs[:, 1]]
+    key = np.unique(np.minimum(pairs[:, 0], pairs[:, 1]) * n + np.maximum(pairs[:, 0], pairs[:, 1]))
+    u, v = key // n, key % n
+    both = np.concatenate([np.stack([u, v], 1), np.stack([v, u], 1)])
+    both = both[np.lexsort((both[:, 1], both[:, 0]))]
+    off = np.zeros(n + 1, dtype=np.int64)
+    np.add.at(off, both[:, 0] + 1, 1)
+    off = np.cumsum(off)
+    col = both[:, 1].astype(np.int32)
+    og = OracleGraph(n=n, off=off, col=col)
+    eng = sa.Engine(0)
+    eng.load_graph(sa.Graph(n, off, col))
+    present = set(key.tolist())
+
+    def fresh(count):
+        out, seen = [], set()
+        while len(out) < count:
+            a, b = (int(x) for x in rng.integers(0, n, 2))
+            k = min(a, b) * n + max(a, b)
+            if a == b or k in seen or k in present:
+                continue
+            seen.add(k)
+            out.append((a, b))
+        return np.array(out, dtype=np.int32)
+
+    first = fresh(30000)
+    for k in (1, 2, 3):
+        if k == 1:
+            uv = first
+        elif k == 2:
+            # new pairs (not present after batch 1) and deletions of batch 1's: all distinct;
+            # ~8,000 new entries over 1,500 sources, grouped by one block
+            second = np.concatenate([fresh(4000), first[rng.choice(len(first), 3000, replace=False)]])
+            uv = np.ascontiguousarray(second[rng.permutation(len(second))])
+        else:
+            # one source gains 400 new entries (a run longer than one block ranks), among others
+            hub = 7
+            new_nb = [w for w in range(n) if w != hub and min(hub, w) * n + max(hub, w) not in present][:400]
+            others = fresh(600)
+            others = others[(others[:, 0] != hub) & (others[:, 1] != hub)]
+            uv = np.ascontiguousarray(np.concatenate([np.array([[hub, w] for w in new_nb], dtype=np.int32),
+                                                      others])[rng.permutation(len(new_nb) + len(others))])
+        eng.stage(uv)
+        ins = eng.classify(sa.CLASSIFY_CURRENT)
+        ins_o = og.classify(uv)
+        assert np.array_equal(ins, ins_o), f"classification differs in batch {k}"
+        eng.apply(False)
+        og.apply(uv, ins_o)
+        want_off, want_col = og.logical_csr()
+        dg = eng.download_graph()
+        assert np.array_equal(dg.off, want_off), f"batch {k}: row lengths differ"
+        assert np.array_equal(dg.col, want_col), f"batch {k}: row entries or their order differ"
+        for a, b in uv.tolist():
+            present ^= {min(a, b) * n + max(a, b)}
+    eng.close()
+
+
 def test_segments_match_sequential_batches(data):
     """Three batches staged back to back and applied as segments (one serving step) give the
     labels, the logical graph and the BFS/CC solutions of applying them one by one, also when
