@@ -342,7 +342,9 @@ int scout_prof_read(scout_ctx* ctx, int id, double* total_ms, int64_t* launches,
 enum {
   SCOUT_PATH_INVAL_TASKS = 0,        /* BFS invalidation: chunk tasks searched for support */
   SCOUT_PATH_INVAL_PUSH_TASKS = 1,   /* BFS invalidation: chunk tasks of unsupported long rows */
-  SCOUT_PATH_INVAL_KIDS_OVERFLOW = 2,/* BFS invalidation: invalid rows whose children were re-read */
+  SCOUT_PATH_INVAL_KIDS_OVERFLOW = 2,/* BFS invalidation: invalid rows whose children were found by a
+                                        pass over the row (too many for the LDS list, or a level-1
+                                        row, which skips the support search) */
   SCOUT_PATH_CC_TASKS = 3,           /* CC union-find: chunk tasks of long rows */
   SCOUT_PATH_RELAX_TASKS = 4,        /* BFS relax: chunk tasks of long rows */
   SCOUT_PATH_TENTATIVE_TASKS = 5,    /* BFS tentative levels: chunk tasks of long rows */

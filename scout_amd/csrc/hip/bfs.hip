@@ -27,9 +27,10 @@
 // between consecutive levels that is not a parent edge changes nothing.
 //
 // Rows are walked by length: up to kShort entries by one thread (the entries and their
-// levels are gathered first, so the loads of one row are in flight together), up to kTask
-// (relax, tentative levels) or kChunk (invalidation, which stops at the first support) by one
-// wave, longer rows as kTask-entry tasks spread over many waves.
+// levels are gathered first, so the loads of one row are in flight together), up to kSmall by
+// a group of kGroup lanes (invalidation), up to kTask (relax, tentative levels) or kChunk
+// (invalidation, which stops at the first support) by one wave, longer rows as kTask-entry
+// tasks spread over many waves.
 #include <algorithm>
 
 #include "common.cuh"
@@ -39,6 +40,12 @@ namespace scout {
 namespace {
 
 constexpr int kShort = 16;
+// Rows of kShort < length <= kSmall entries are walked by kGroup lanes of a wave each (four
+// rows per wave at once, kU entries per lane in flight: the whole row in one step), so a wave
+// does not take such rows one after another.
+constexpr int kSmall = 64;
+constexpr int kGroup = 16;
+static_assert(kGroup * 4 == kSmall && kWave % kGroup == 0, "a group's four loads per lane cover a small row");
 // Entries per chunk task of a split row, and the longest row the relax and tentative passes
 // give one wave: a wave walks at most kTask entries (kTask / (kU * kWave) dependent steps),
 // so one long row does not set a round's duration.
@@ -101,10 +108,11 @@ struct RoundBook {
   int64_t* sum;
   int64_t* nonempty;
   int64_t* max_tasks;
-  int64_t* other_tasks;  // the other set: chunk tasks, then up to two more counters (or null)
+  int64_t* other_tasks;  // the other set: chunk tasks, then up to three more counters (or null)
   int64_t* other_a;
   int64_t* other_b;
   int side;
+  int64_t* other_c = nullptr;
 };
 
 __device__ __forceinline__ void round_book(const RoundBook& b) {
@@ -116,6 +124,7 @@ __device__ __forceinline__ void round_book(const RoundBook& b) {
   *b.other_tasks = 0;
   if (b.other_a) *b.other_a = 0;
   if (b.other_b) *b.other_b = 0;
+  if (b.other_c) *b.other_c = 0;
   *b.fcur = 0;
 }
 
@@ -133,6 +142,29 @@ __device__ __forceinline__ void load_steps(const GView& g, const Row& r, int64_t
     int64_t k = base + (int64_t)j * kWave + lane_id();
     xs[j] = k < hi ? row_at(g, r, k) : -1;
   }
+}
+
+// Group lanes: kGroup consecutive lanes walk one row of at most kSmall entries.
+__device__ __forceinline__ int group_lane() { return lane_id() & (kGroup - 1); }
+
+// Loads entries group_lane() + j * kGroup (j < kU) of row r, -1 at or past len (<= kSmall).
+__device__ __forceinline__ void group_load(const GView& g, const Row& r, int64_t len, bool active, int32_t (&xs)[kU]) {
+#pragma unroll
+  for (int j = 0; j < kU; j++) {
+    int64_t k = group_lane() + (int64_t)j * kGroup;
+    xs[j] = active && k < len ? row_at(g, r, k) : -1;
+  }
+}
+
+// Minimum and OR over the kGroup lanes of a group (the xor offsets stay inside the group).
+__device__ __forceinline__ int32_t group_min(int32_t v) {
+  for (int off = kGroup / 2; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off));
+  return v;
+}
+__device__ __forceinline__ bool group_any(bool b) {
+  int v = b ? 1 : 0;
+  for (int off = kGroup / 2; off > 0; off >>= 1) v |= __shfl_xor(v, off);
+  return v != 0;
 }
 
 // Cascade attribution while the learned scheduler logs training data: every invalid vertex
@@ -523,7 +555,8 @@ __global__ void k_mark_pending(const int32_t* cand, const int64_t* ncand, int32_
 // (cur) and the batch's deletion candidates of this level (cand[0..ncand), or cand[cb[0]..cb[1])
 // when the candidates are grouped by level; filtered by level and stamp; ncand = 0 past the
 // deepest candidate; *listed += those taken). Thread per candidate; rows longer than kShort go
-// to the wave list (<= kChunk) or the long list, whose chunk tasks are pushed here.
+// to the group list (<= kSmall), the wave list (<= kChunk) or the long list, whose chunk tasks
+// are pushed here.
 // Asynchronous (A): the round's list (cur) holds pending vertices of any level, each decided
 // at its own level l = kPend - level: a neighbour at l - 1 that is not pending supports it; if
 // none does but one is pending, it waits for the next round (queued again, still pending);
@@ -536,10 +569,10 @@ template <bool A>
 __global__ void k_invalidate_short(GView g, int32_t* level, int32_t* parent, const int32_t* cur,
                                    const int64_t* cur_count, const int32_t* cand, int64_t ncand, const int64_t* cb,
                                    int64_t* listed, int32_t lvl, int32_t* inval, int64_t* inval_count, int32_t* next,
-                                   int64_t* next_count, int64_t cap, int32_t* stamp, int32_t epoch, int32_t* medium,
-                                   int64_t* medium_count, int32_t* longs, int32_t* long_lvl, int64_t* long_count,
-                                   int64_t* tasks, int64_t* task_count, int64_t task_cap, int64_t* scanned,
-                                   int64_t* paths, Attrib at) {
+                                   int64_t* next_count, int64_t cap, int32_t* stamp, int32_t epoch, int32_t* small,
+                                   int64_t* small_count, int32_t* medium, int64_t* medium_count, int32_t* longs,
+                                   int32_t* long_lvl, int64_t* long_count, int64_t* tasks, int64_t* task_count,
+                                   int64_t task_cap, int64_t* scanned, int64_t* paths, Attrib at) {
   int64_t nc = *cur_count;
   if (A) ncand = 0;
   if (!A && cb && ncand > 0) {
@@ -599,7 +632,8 @@ __global__ void k_invalidate_short(GView g, int32_t* level, int32_t* parent, con
         if (A) level[c] = lm;
       }
     }
-    bool is_medium = active && !is_short && len <= kChunk;
+    bool is_small = active && !is_short && len <= kSmall;
+    bool is_medium = active && len > kSmall && len <= kChunk;
     bool is_long = active && len > kChunk;
     // children of an invalid candidate: neighbours one level down (levels read above; no
     // vertex of that level changes in this kernel, except to pending) whose parent it is
@@ -620,12 +654,13 @@ __global__ void k_invalidate_short(GView g, int32_t* level, int32_t* parent, con
     // one reservation for the block's invalid, medium and long rows, the long rows' chunk
     // tasks (index in longs << 32 | chunk) and the children (and a deferred vertex); a vertex
     // is listed once per round, so the lists (n + 1 slots) never fill
-    int32_t need[5] = {invalid, is_medium, is_long, is_long ? (int32_t)((len + kTask - 1) / kTask) : 0,
-                       nkids + (int32_t)defer};
-    int64_t pos[5];
-    block_reserve_n(need, pos, inval_count, medium_count, long_count, task_count, next_count);
+    int32_t need[6] = {invalid, is_medium, is_long, is_long ? (int32_t)((len + kTask - 1) / kTask) : 0,
+                       nkids + (int32_t)defer, is_small};
+    int64_t pos[6];
+    block_reserve_n(need, pos, inval_count, medium_count, long_count, task_count, next_count, small_count);
     if (invalid && pos[0] < cap) inval[pos[0]] = c;
     if (is_medium && pos[1] < cap) medium[pos[1]] = c;
+    if (is_small && pos[5] < cap) small[pos[5]] = c;
     if (is_long && pos[2] < cap) {
       longs[pos[2]] = c;
       if (A) long_lvl[pos[2]] = lm;
@@ -649,14 +684,20 @@ __global__ void k_invalidate_short(GView g, int32_t* level, int32_t* parent, con
 // Asynchronous (A): levels are the candidates' own (as in k_invalidate_short); a medium row
 // with no support but a pending neighbour one level up waits for the next round, and a chunk
 // that sees such a neighbour raises supported[c] to supepoch - 1 (atomicMax: support wins).
+// Rows of at most kSmall entries (the group list) are walked first, kGroup lanes per row: the
+// row, its levels and (for an invalid row) its children's parents are read in one step each.
+// A candidate at level 1 can only be supported by the source src: it is supported exactly when
+// still adjacent to it (one pair lookup instead of a search of its whole row, usually a hub's
+// that lost its edge to the source); an unsupported one then reads its row once, for children.
 // own[0] += entries and parents read, own[1] += rows and tasks.
 template <bool A>
-__global__ void k_invalidate(GView g, int32_t* level, int32_t* parent, const int32_t* cur, const int64_t* cur_count,
+__global__ void k_invalidate(GView g, int32_t* level, int32_t* parent, const int32_t* small,
+                             const int64_t* small_count, const int32_t* cur, const int64_t* cur_count,
                              const int32_t* longs, const int32_t* long_lvl, const int64_t* tasks,
                              const int64_t* task_count, int64_t task_cap, int32_t* supported, int32_t supepoch,
                              int32_t lvl, int32_t* inval, int64_t* inval_count, int64_t inval_cap, int32_t* next,
                              int64_t* next_count, int64_t cap, int32_t* stamp, int32_t epoch, int64_t* scanned,
-                             int64_t* own, int64_t* paths, Attrib ab) {
+                             int64_t* own, int64_t* paths, bool sorted, int32_t src, Attrib ab) {
   // neighbours one level down met while searching for support: the only possible children
   constexpr int kKids = 1024;
   constexpr int kInvSeg = 64;
@@ -669,6 +710,80 @@ __global__ void k_invalidate(GView g, int32_t* level, int32_t* parent, const int
   int64_t nc = *cur_count;
   int64_t nt = min(*task_count, task_cap);
   int64_t work = 0, reads = 0, rows = 0, ntask = 0, overflow = 0, deferred = 0;
+  // per-lane counts of the group walks (a group's leader counts its row)
+  int64_t gwork = 0, greads = 0, grows = 0, gdeferred = 0;
+  constexpr int kPerWave = kWave / kGroup;
+  int64_t nsm = *small_count;
+  for (int64_t base = global_wave() * kPerWave; base < nsm; base += total_waves() * kPerWave) {
+    int64_t idx = base + lane_id() / kGroup;
+    bool act = idx < nsm;
+    bool lead = act && group_lane() == 0;
+    int32_t c = act ? small[idx] : -1;
+    Row r{0, 0, 0, 0};
+    int64_t len = 0;
+    int32_t lm = lvl;
+    if (act) {
+      r = row_of(g, c);
+      len = row_len(r);
+      if constexpr (A) lm = kPend - level[c];
+    }
+    int32_t xs[kU];
+    group_load(g, r, len, act, xs);
+    int32_t lx[kU];
+#pragma unroll
+    for (int j = 0; j < kU; j++) lx[j] = xs[j] >= 0 ? level[xs[j]] : INT32_MIN;
+    // the first support in row order (entry group_lane() + j * kGroup), then the group's minimum
+    int32_t mine = INT32_MAX;
+    bool pend = false;
+#pragma unroll
+    for (int j = 0; j < kU; j++) {
+      if (mine == INT32_MAX && xs[j] >= 0 && lx[j] == lm - 1) mine = group_lane() + j * kGroup;
+      if (A) pend |= xs[j] >= 0 && lx[j] == pend_of(lm - 1);
+    }
+    int32_t at = group_min(mine);
+    bool any_pend = A && group_any(pend);
+    bool supported = act && at != INT32_MAX;
+    bool defer = act && !supported && any_pend;
+    bool invalid = act && !supported && !defer;
+    if (supported) {
+#pragma unroll
+      for (int j = 0; j < kU; j++) {
+        if (group_lane() + j * kGroup == at) {
+          parent[c] = xs[j];
+          if (A) level[c] = lm;
+        }
+      }
+    }
+    if (invalid && lead) {
+      level[c] = kInvalid;
+      attrib_invalid(ab, c, lm);
+    }
+    stage_push(sinv, invalid && lead, c, inval, inval_count, inval_cap);
+    stage_push(st, defer && lead, c, next, next_count, cap);
+    // children of an invalid row: its neighbours one level down whose parent it is
+    int32_t pw[kU];
+#pragma unroll
+    for (int j = 0; j < kU; j++) pw[j] = (invalid && xs[j] >= 0 && lx[j] == lm + 1) ? parent[xs[j]] : -1;
+#pragma unroll
+    for (int j = 0; j < kU; j++) {
+      // only an invalid row's lanes take children (an idle lane's c and pw[j] are both -1)
+      bool take;
+      if constexpr (A)
+        take = invalid && pw[j] == c && atomicCAS(&level[xs[j]], lm + 1, pend_of(lm + 1)) == lm + 1;
+      else
+        take = invalid && pw[j] == c && atomicExch(&stamp[xs[j]], epoch) != epoch;
+      if (take) attrib_child(ab, c, xs[j]);
+      greads += invalid && xs[j] >= 0 && lx[j] == lm + 1;
+      stage_push(st, take, xs[j], next, next_count, cap);
+    }
+    if (lead) {
+      int64_t nread = supported ? at + 1 : len;
+      gwork += nread;
+      greads += nread;
+      grows++;
+      gdeferred += defer;
+    }
+  }
   for (int64_t i = global_wave(); i < nc + nt; i += total_waves()) {
     if (i >= nc) {
       int64_t task = tasks[i - nc];
@@ -678,6 +793,20 @@ __global__ void k_invalidate(GView g, int32_t* level, int32_t* parent, const int
       if (supported[c] == supepoch) continue;
       ntask++;
       Row r = row_of(g, c);
+      if (lm == 1) {
+        // the source alone can support it: chunk 0 asks, the other chunks have nothing to do
+        rows++;
+        if ((task & 0xffffffffLL) == 0 && wave_adjacent(g, sorted, c, src) && lane_id() == 0) {
+          parent[c] = src;
+          if constexpr (A) {
+            level[c] = lm;
+            atomicMax(&supported[c], supepoch);
+          } else {
+            supported[c] = supepoch;
+          }
+        }
+        continue;
+      }
       int64_t lo, hi;
       task_range(r, task, lo, hi);
       int64_t at = -1;
@@ -720,9 +849,23 @@ __global__ void k_invalidate(GView g, int32_t* level, int32_t* parent, const int
     int64_t len = row_len(r);
     int64_t at = -1, nk = 0;
     bool pending = false;
+    bool source_only = lm == 1;
+    if (source_only) {
+      // the source alone can support it (no search; children by one pass over the row below)
+      rows++;
+      if (wave_adjacent(g, sorted, c, src)) {
+        if (lane_id() == 0) {
+          parent[c] = src;
+          if (A) level[c] = lm;
+        }
+        continue;
+      }
+      nk = kKids + 1;
+    }
     // one row step first (support is usually among the first entries of a row, whose hubs
     // come first), then kU steps in flight per iteration; `at` is uniform across the wave
-    for (int64_t base = 0, step = kWave; base < len && at < 0; base += step, step = (int64_t)kU * kWave) {
+    for (int64_t base = 0, step = kWave; !source_only && base < len && at < 0;
+         base += step, step = (int64_t)kU * kWave) {
       int32_t xs[kU];
       load_steps(g, r, base, min(len, base + step), xs);
       int32_t lx[kU];
@@ -744,7 +887,7 @@ __global__ void k_invalidate(GView g, int32_t* level, int32_t* parent, const int
         nk += __popcll(km);
       }
     }
-    rows++;
+    if (!source_only) rows++;
     if (at >= 0) {
       work += at + 1;
       reads += at + 1;
@@ -794,7 +937,8 @@ __global__ void k_invalidate(GView g, int32_t* level, int32_t* parent, const int
         for (int j = 0; j < kU; j++) stage_push(st, take[j], ws[j], next, next_count, cap);
       }
     } else {
-      reads += 2 * len;
+      // the row again (the support search read it too, except for a level-1 candidate)
+      reads += source_only ? len : 2 * len;
       overflow++;
       for (int64_t base = 0; base < len; base += (int64_t)kU * kWave) {
         int32_t ws[kU];
@@ -821,11 +965,13 @@ __global__ void k_invalidate(GView g, int32_t* level, int32_t* parent, const int
   count_path(ntask, &paths[PATH_INVAL_TASKS]);
   count_path(overflow, &paths[PATH_INVAL_KIDS_OVERFLOW]);
   count_path(deferred, &paths[PATH_INVAL_DEFERRED]);
-  // work, reads and rows are uniform across the wave: counted once per wave
+  if (A) block_add(gdeferred, &paths[PATH_INVAL_DEFERRED]);
+  // work, reads and rows of the wave walks are uniform across the wave: counted once per wave;
+  // the group walks' by their leaders
   bool l0 = lane_id() == 0;
-  block_add(l0 ? work : 0, scanned);
-  block_add(l0 ? reads : 0, &own[0]);
-  block_add(l0 ? rows : 0, &own[1]);
+  block_add((l0 ? work : 0) + gwork, scanned);
+  block_add((l0 ? reads : 0) + greads, &own[0]);
+  block_add((l0 ? rows : 0) + grows, &own[1]);
 }
 
 // Wave per chunk of a long candidate that no chunk supported: it is invalid (recorded by its
@@ -1240,6 +1386,7 @@ int ensure_bfs(Ctx& c, int64_t extra) {
   SCOUT_CHECK(c.le.reserve(n + 1));
   SCOUT_CHECK(c.lf.reserve(n + 1));
   SCOUT_CHECK(c.lg.reserve(n + 2 * extra + 1));
+  SCOUT_CHECK(c.lh.reserve(n + 1));
   SCOUT_CHECK(c.wa.reserve(n + 2 * extra + 1));
   SCOUT_CHECK(c.wc.reserve(n + 2 * extra + 1));
   SCOUT_CHECK(c.lvlb.reserve(2 * kMaxBuckets + 2));
@@ -1285,6 +1432,8 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, int32_t* P, int strategy, scou
   int64_t cap = (int64_t)n + 1;
   int64_t seed_cap = std::min(c.wa.cap, c.lg.cap);  // seeds and their parents
   int64_t* box = c.d_box;
+  // the maintained levels grow from the BFS source; CC's own levels from vertex 0
+  int32_t src = L == c.level ? c.source : 0;
   // box: [0] cand, [1] min, [2] max, [3] deletions, [4] seeds, [5] seed min, [6] seed max, [7] insertions
   k_init_box<<<1, 64, 0, c.s>>>(box);
   // the learned scheduler logs from, and predicts for, the maintained levels only
@@ -1305,7 +1454,7 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, int32_t* P, int strategy, scou
   int64_t ncand = c.h_box[0];
   int32_t cmin = (int32_t)c.h_box[1], cmax = (int32_t)c.h_box[2];
   st->deletions += c.h_box[3];
-  int64_t nx = 0, scanned = 0, rounds = 0, st_inval_rounds = 0;
+  int64_t nx = 0, scanned = 0, rounds = 0;
   int64_t nsample = std::min<int64_t>(ncand, sample);
   int first_group = kFirstGroup;
   int64_t predicted_last = -1, predicted_depth = -1;
@@ -1318,7 +1467,7 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, int32_t* P, int strategy, scou
     if (predicted > kRecomputeFraction * (double)n) {
       // cascades over a large part of the graph: one full BFS costs less than local repair
       st->fallback = 1;
-      return bfs_full_on(c, L, P, c.source, &st->rounds, &st->edges_scanned);
+      return bfs_full_on(c, L, P, src, &st->rounds, &st->edges_scanned);
     }
     // rounds cmin .. max(predicted last level, deepest candidate): one read-back when the
     // prediction holds; an empty extra round costs as much as a read-back, so no slack. The
@@ -1371,8 +1520,9 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, int32_t* P, int strategy, scou
     int64_t* medium_of[2] = {box + 12, box + 28};
     int64_t* longs_of[2] = {box + 13, box + 29};
     int64_t* tasks_of[2] = {box + 14, box + 30};
+    int64_t* small_of[2] = {box + 31, box + 32};  // rows walked by lane groups
     // [15] entries read by k_inval_push, [23] entries of short rows read by k_invalidate_short
-    k_zero2<<<1, 64, 0, c.s>>>(box + 8, 16, box + 28, 3);
+    k_zero2<<<1, 64, 0, c.s>>>(box + 8, 16, box + 28, 5);
     if (async)
       k_mark_pending<<<grid_for(ncand, kBlock * kLinesPerThread), kBlock, 0, c.s>>>(c.la.p, box, L, cur, box + 8,
                                                                                     cap);
@@ -1390,6 +1540,7 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, int32_t* P, int strategy, scou
         int64_t nlevel = !async && lvl <= cmax ? ncand : 0;
         RoundBook book{bcur,           box + 18,          box + 20,        box + 19,
                        tasks_of[p ^ 1], medium_of[p ^ 1], longs_of[p ^ 1], nlevel > 0};
+        book.other_c = small_of[p ^ 1];
         const int64_t* cb = (cbuckets && nlevel > 0) ? cbuckets + (lvl - cmin) : nullptr;
         // two fresh consecutive epochs: supepoch marks support, supepoch - 1 a pending neighbour
         int32_t supepoch = next_epoch(c);
@@ -1398,12 +1549,13 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, int32_t* P, int strategy, scou
           if (supepoch == 1) supepoch = next_epoch(c);  // the counter started over
           SCOUT_CHECK(prof_launch(c, PROF_K_INVALIDATE_SHORT, k_invalidate_short<true>, dim3(kListGrid),
                                   dim3(kBlock), c.g.view(), L, P, cur, bcur, cand, nlevel, cb, box + 18, lvl, c.ld.p,
-                                  binval, nxt, bnext, cap, c.stamp, sepoch, c.le.p, medium_of[p], c.lf.p, c.lg.p,
-                                  longs_of[p], c.wb.p, tasks_of[p], c.wb.cap, box + 23, c.d_paths, ab));
+                                  binval, nxt, bnext, cap, c.stamp, sepoch, c.lh.p, small_of[p], c.le.p, medium_of[p],
+                                  c.lf.p, c.lg.p, longs_of[p], c.wb.p, tasks_of[p], c.wb.cap, box + 23, c.d_paths,
+                                  ab));
           SCOUT_CHECK(prof_launch(c, PROF_BFS_INVALIDATE_KERNEL, k_invalidate<true>, dim3(kListGrid), dim3(kBlock),
-                                  c.g.view(), L, P, c.le.p, medium_of[p], c.lf.p, c.lg.p, c.wb.p, tasks_of[p],
+                                  c.g.view(), L, P, c.lh.p, small_of[p], c.le.p, medium_of[p], c.lf.p, c.lg.p, c.wb.p, tasks_of[p],
                                   c.wb.cap, c.aux, supepoch, lvl, c.ld.p, binval, cap, nxt, bnext, cap, c.stamp,
-                                  sepoch, bscan, box + 21, c.d_paths, ab));
+                                  sepoch, bscan, box + 21, c.d_paths, c.g.sorted, src, ab));
           SCOUT_CHECK(prof_launch(c, PROF_K_INVAL_PUSH, k_inval_push<true>, dim3(kListGrid), dim3(kBlock),
                                   c.g.view(), L, P, lvl, c.lf.p, c.lg.p, c.wb.p, tasks_of[p], c.wb.cap, c.aux,
                                   supepoch, c.ld.p, binval, cap, nxt, bnext, cap, c.stamp, sepoch, box + 15,
@@ -1411,12 +1563,13 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, int32_t* P, int strategy, scou
         } else {
           SCOUT_CHECK(prof_launch(c, PROF_K_INVALIDATE_SHORT, k_invalidate_short<false>, dim3(kListGrid),
                                   dim3(kBlock), c.g.view(), L, P, cur, bcur, cand, nlevel, cb, box + 18, lvl, c.ld.p,
-                                  binval, nxt, bnext, cap, c.stamp, sepoch, c.le.p, medium_of[p], c.lf.p, c.lg.p,
-                                  longs_of[p], c.wb.p, tasks_of[p], c.wb.cap, box + 23, c.d_paths, ab));
+                                  binval, nxt, bnext, cap, c.stamp, sepoch, c.lh.p, small_of[p], c.le.p, medium_of[p],
+                                  c.lf.p, c.lg.p, longs_of[p], c.wb.p, tasks_of[p], c.wb.cap, box + 23, c.d_paths,
+                                  ab));
           SCOUT_CHECK(prof_launch(c, PROF_BFS_INVALIDATE_KERNEL, k_invalidate<false>, dim3(kListGrid),
-                                  dim3(kBlock), c.g.view(), L, P, c.le.p, medium_of[p], c.lf.p, c.lg.p, c.wb.p,
+                                  dim3(kBlock), c.g.view(), L, P, c.lh.p, small_of[p], c.le.p, medium_of[p], c.lf.p, c.lg.p, c.wb.p,
                                   tasks_of[p], c.wb.cap, c.aux, supepoch, lvl, c.ld.p, binval, cap, nxt, bnext, cap,
-                                  c.stamp, sepoch, bscan, box + 21, c.d_paths, ab));
+                                  c.stamp, sepoch, bscan, box + 21, c.d_paths, c.g.sorted, src, ab));
           SCOUT_CHECK(prof_launch(c, PROF_K_INVAL_PUSH, k_inval_push<false>, dim3(kListGrid), dim3(kBlock),
                                   c.g.view(), L, P, lvl, c.lf.p, c.lg.p, c.wb.p, tasks_of[p], c.wb.cap, c.aux,
                                   supepoch, c.ld.p, binval, cap, nxt, bnext, cap, c.stamp, sepoch, box + 15,
@@ -1442,7 +1595,6 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, int32_t* P, int strategy, scou
     }
     rounds += c.h_box[20];
     st->inval_rounds += c.h_box[20];
-    st_inval_rounds = c.h_box[20];
     if (predicted_depth >= 0) {
       c.pred_bias = 0.75 * c.pred_bias + 0.25 * (double)(c.h_box[20] - predicted_depth);
     } else if (predicted_last >= 0) {
@@ -1470,7 +1622,7 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, int32_t* P, int strategy, scou
       st->touched += nx;
       st->rounds += rounds;
       st->edges_scanned += scanned;
-      SCOUT_CHECK(bfs_full_on(c, L, P, c.source, &st->rounds, &st->edges_scanned));
+      SCOUT_CHECK(bfs_full_on(c, L, P, src, &st->rounds, &st->edges_scanned));
       if (logging) SCOUT_CHECK(mlp_end_batch(c, c.la.p, nsample, L));
       return SCOUT_OK;
     }
@@ -1513,15 +1665,19 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, int32_t* P, int strategy, scou
     if (learned && predicted_last >= 0)
       relax_group = (int)std::min<int64_t>(
           std::max<int64_t>(std::max<int64_t>(predicted_last, smax) - smin + 1, kFirstGroup), kMaxGroup);
-    // asynchronous: the lowered regions are the invalidated cascades and the insertions' far
-    // sides, as deep as the invalidation rounds that ran
+    // asynchronous: the relax re-attaches the invalidated cascades and runs out from the
+    // insertions; its rounds follow the same predicted depth, corrected by the running mean of
+    // their difference on earlier batches (rounded up, as for the invalidation)
     if (learned && c.bfs_async && predicted_depth >= 0)
-      relax_group = (int)std::min<int64_t>(std::max<int64_t>(st_inval_rounds + 1, kFirstGroup), kMaxGroup);
+      relax_group = (int)std::min<int64_t>(
+          std::max<int64_t>(predicted_depth + (int64_t)std::ceil(c.relax_bias), kFirstGroup), kMaxGroup);
     if (L == c.level && hint_relax > 0) relax_group = hint_relax;
     int64_t r0 = rounds;
     SCOUT_CHECK(relax_levels(c, L, P, smin, smax, c.wa.p, nseeds, epoch, &rounds, &scanned, relax_group,
                              c.bfs_async));
     st->relax_rounds += rounds - r0;
+    if (learned && c.bfs_async && predicted_depth >= 0)
+      c.relax_bias = 0.75 * c.relax_bias + 0.25 * (double)(rounds - r0 - predicted_depth);
   }
   st->rounds += rounds;
   st->edges_scanned += scanned;

@@ -520,4 +520,44 @@ __device__ __forceinline__ bool adjacent_now(const PairState& s) {
   return s.base_live;
 }
 
+// adjacent_now(pair_state(g, sorted, u, v)) computed by a whole wave, for long rows: lane 0
+// searches a sorted base row (through the samples), an unsorted one and the inserted row are
+// compared 64 entries at a time. Uniform across the wave.
+__device__ __forceinline__ bool wave_adjacent(const GView& g, bool sorted, int32_t u, int32_t v) {
+  int64_t b = g.boff[u], e = g.boff[u + 1];
+  unsigned long long live = 0, tomb = 0;
+  if (sorted) {
+    int flags = 0;
+    if (lane_id() == 0) {
+      int64_t lo, hi;
+      base_range_sampled(g, true, u, v, lo, hi);
+      for (int64_t k = lo; k < hi; k++) {
+        int32_t x = g.bcol[k];
+        flags |= (x == v ? 1 : 0) | (x == ~v ? 2 : 0);
+      }
+    }
+    flags = __shfl(flags, 0);
+    live = flags & 1;
+    tomb = flags & 2;
+  } else {
+    for (int64_t base = b; base < e; base += kWave) {
+      int64_t k = base + lane_id();
+      int32_t x = k < e ? g.bcol[k] : HOLE;
+      live |= __ballot(x == v);
+      tomb |= __ballot(x == ~v);
+    }
+  }
+  unsigned long long alive = 0, atomb = 0;
+  int64_t ab = g.aoff[u], ae = ab + g.alen[u];
+  for (int64_t base = ab; base < ae; base += kWave) {
+    int64_t k = base + lane_id();
+    int32_t x = k < ae ? g.acol[k] : HOLE;
+    alive |= __ballot(x == v);
+    atomb |= __ballot(x == ~v);
+  }
+  if (alive) return true;
+  if (tomb || atomb) return false;
+  return live != 0;
+}
+
 }  // namespace scout

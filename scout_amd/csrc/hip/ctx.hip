@@ -297,7 +297,7 @@ int scout_ctx_destroy(scout_ctx* x) {
     for (hipEvent_t ev : {c.ch[k].staged, c.ch[k].consumed, c.ch[k].h2d0, c.ch[k].h2d1})
       if (ev) (void)hipEventDestroy(ev);
   }
-  DevBuf<int32_t>* ls[] = {&c.la, &c.lb, &c.lc, &c.ld, &c.le, &c.lf, &c.lg};
+  DevBuf<int32_t>* ls[] = {&c.la, &c.lb, &c.lc, &c.ld, &c.le, &c.lf, &c.lg, &c.lh};
   for (auto* l : ls) l->release();
   c.wa.release();
   c.wb.release();

@@ -243,7 +243,7 @@ struct Ctx {
   // cumulative counts of the long-row code paths (kPathCount entries after the mailbox)
   int64_t* d_paths = nullptr;
   // generic work lists
-  DevBuf<int32_t> la, lb, lc, ld, le, lf, lg;
+  DevBuf<int32_t> la, lb, lc, ld, le, lf, lg, lh;
   DevBuf<int64_t> wa, wb, wc;
   DevBuf<int64_t> lvlb;        // BFS: per-level bucket offsets and cursors of candidates / seeds
   // rows read by repairs (SCOUT_DUMP_TOUCHED): one bit per vertex while tracking is on
@@ -265,7 +265,8 @@ struct Ctx {
   float last_repair_ms = 0.f;
   int64_t readbacks = 0;         // host read-backs of device counters so far
   double pred_over = 0;          // running mean of how far the predicted last level overshoots
-  double pred_bias = 0;          // running mean of (rounds run - predicted depth), asynchronous BFS
+  double pred_bias = 0;          // running mean of (invalidation rounds - predicted depth), asynchronous BFS
+  double relax_bias = 0;         // running mean of (relax rounds - predicted depth), asynchronous BFS
   int64_t pred_cut = 0;          // split head: expected vertices the batch cuts off vertex 0's component
   int64_t pred_cut_seq = -1;     // the batch (apply_seq) pred_cut was predicted for
   int hint_inval = 0, hint_relax = 0;  // scout_bfs_schedule: first group sizes of the next repair
