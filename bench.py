@@ -98,11 +98,11 @@ def cpu_baseline(eng, n, batches_host, seconds):
 
 # Kernels of each profiled family (the launches scout_prof_* brackets on the compute stream).
 FAMILY_KERNELS = {
-    "bfs_invalidate": ["k_invalidate_short", "k_invalidate", "k_inval_push"],
+    "bfs_invalidate": ["k_mark_pending", "k_invalidate_short", "k_invalidate", "k_inval_push"],
     "bfs_relax": ["k_relax_short", "k_relax_rows"],
     "cc_expand": ["k_cc_zero", "k_cc_seed", "k_cc_first_ranges", "k_cc_explore", "k_zero_u32", "k_cc_mark",
                   "k_cc_collect", "k_cc_union_list", "k_cc_unreached_hook", "k_cc_hook_tasks", "k_cc_final_list"],
-    "apply": ["k_apply", "k_group_heads", "k_append_groups", "k_move_rows"],
+    "apply": ["k_apply_begin", "k_apply", "k_place_single", "k_group_small", "k_place_multi", "k_move_rows"],
     "classify": ["k_dup_count", "k_classify"],
 }
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_rmat26.json")
@@ -116,8 +116,11 @@ ROOFLINE_KERNELS = {"k_invalidate": "bfs_invalidate_kernel", "k_apply": "apply",
 
 
 def _kernel_short_name(name):
+    """Kernel name without namespace, arguments, return type or template arguments
+    ("void scout::(anonymous namespace)::k_invalidate<true>(...)" -> "k_invalidate")."""
     import re
-    return re.sub(r"\(.*", "", name.replace("scout::(anonymous namespace)::", ""))
+    name = re.sub(r"\(.*", "", name.replace("scout::(anonymous namespace)::", ""))
+    return re.sub(r"<.*", "", name.replace("void ", "")).strip()
 
 
 def summarize_pmc(pmc_dir, last_batches=0):
