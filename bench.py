@@ -541,7 +541,7 @@ def main():
                     help="plain batches whose repairs log the learned scheduler's training data (0: plain only)")
     ap.add_argument("--epochs", type=int, default=4)
     ap.add_argument("--repeats", type=int, default=3, help="alternating plain / learned replays (medians reported)")
-    ap.add_argument("--bfs-mode", choices=["async", "level"], default="async",
+    ap.add_argument("--bfs-mode", choices=["async", "level", "inval"], default="async",
                     help="BFS round order: async (every level's candidates and seeds in the first rounds) or "
                          "level (one level per round); the levels are the same")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -593,7 +593,7 @@ def main():
     ntrain = args.train_batches
     total = ntrain + args.warmup + args.steps
     eng = sa.Engine(0)
-    eng.bfs_mode(sa.BFS_ASYNC if args.bfs_mode == "async" else sa.BFS_LEVEL_SYNC)
+    eng.bfs_mode({"async": sa.BFS_ASYNC, "level": sa.BFS_LEVEL_SYNC, "inval": sa.BFS_ASYNC_INVALIDATION}[args.bfs_mode])
     t0 = time.perf_counter()
     eng.gen_rmat(args.scale, 16, 0.57, 0.19, 0.19, args.seed)
     info = eng.graph_info()

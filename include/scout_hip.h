@@ -173,7 +173,7 @@ int scout_bfs_bound(scout_ctx* ctx, int64_t bound);
  * rounds follow the longest cascade instead of the span of levels the batch touches.
  * SCOUT_BFS_LEVEL_SYNC runs one level per round, in increasing level order. No counterpart in the
  * reference, which repairs one edit at a time (src/bfs/scout_bfs.cu:436-515). */
-enum { SCOUT_BFS_ASYNC = 0, SCOUT_BFS_LEVEL_SYNC = 1 };
+enum { SCOUT_BFS_ASYNC = 0, SCOUT_BFS_LEVEL_SYNC = 1, SCOUT_BFS_ASYNC_INVALIDATION = 2 };
 int scout_bfs_mode(scout_ctx* ctx, int mode);
 /* Round groups of the next BFS repair only: its first invalidation group queues inval_rounds
  * levels and its first relax group relax_rounds levels before the first read-back of each

@@ -25,7 +25,7 @@ CLASSIFY_CURRENT, CLASSIFY_BASE = 0, 1
 STRATEGY_LOCAL, STRATEGY_RECOMPUTE, STRATEGY_LEARNED = 0, 1, 2
 MIS_RULES, MIS_COMPLETE = 0, 1
 CC_AUTO, CC_WALK, CC_SCAN = 0, 1, 2
-BFS_ASYNC, BFS_LEVEL_SYNC = 0, 1
+BFS_ASYNC, BFS_LEVEL_SYNC, BFS_ASYNC_INVALIDATION = 0, 1, 2
 GRAPH_INFO_COUNT = 8  # entries scout_graph_info writes
 
 

@@ -21,7 +21,7 @@
 // (req_<r>.bin, int32 pairs), every step's request range (steps.txt) and the committed levels
 // and labels after every step (lv_<k>.bin, lab_<k>.bin), for a check against the oracle.
 //
-//   scout_serve [--scale 25] [--rate 2e6] [--min 1000] [--max 1000000] [--requests 200]
+//   scout_serve [--scale 25] [--rate 2e6] [--min 1000] [--max 1000000] [--requests 200] [--bfs-mode async|level]
 //               [--delete 0.6] [--pinned-mb 64] [--hbm-mb 512] [--seed 1] [--gpu 0] [--dump DIR]
 //
 // Prints one JSON line. The reference has no serving loop: this is new (the serving
@@ -96,7 +96,7 @@ int main(int argc, char** argv) {
   double rate = 2e6, min_size = 1000, max_size = 1e6, del = 0.6;
   int64_t pinned_mb = 64, hbm_mb = 512;
   uint64_t seed = 1;
-  std::string dump;
+  std::string dump, bfs_mode = "async";
   for (int i = 1; i + 1 < argc; i += 2) {
     std::string k = argv[i];
     const char* v = argv[i + 1];
@@ -111,9 +111,11 @@ int main(int argc, char** argv) {
     else if (k == "--seed") seed = strtoull(v, nullptr, 10);
     else if (k == "--gpu") gpu = atoi(v);
     else if (k == "--dump") dump = v;
+    else if (k == "--bfs-mode") bfs_mode = v;
     else {
       fprintf(stderr, "usage: scout_serve [--scale S] [--rate EDGES_PER_S] [--min L] [--max L] [--requests N] "
-                      "[--delete F] [--pinned-mb MB] [--hbm-mb MB] [--seed S] [--gpu G] [--dump DIR]\n");
+                      "[--delete F] [--pinned-mb MB] [--hbm-mb MB] [--seed S] [--gpu G] [--dump DIR] "
+                      "[--bfs-mode async|level|inval]\n");
       return 1;
     }
   }
@@ -129,6 +131,9 @@ int main(int argc, char** argv) {
   scout_budget budget{0, pinned_bytes / 2};
   scout_ctx* ctx = nullptr;
   CHECK(scout_ctx_create(gpu, &budget, &ctx));
+  CHECK(scout_bfs_mode(ctx, bfs_mode == "level"   ? SCOUT_BFS_LEVEL_SYNC
+                            : bfs_mode == "inval" ? SCOUT_BFS_ASYNC_INVALIDATION
+                                                  : SCOUT_BFS_ASYNC));
   CHECK(scout_gen_rmat(ctx, scale, 16, 0.57, 0.19, 0.19, seed));
   int64_t info[SCOUT_GRAPH_INFO_COUNT];
   CHECK(scout_graph_info(ctx, info));

@@ -1668,15 +1668,15 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, int32_t* P, int strategy, scou
     // asynchronous: the relax re-attaches the invalidated cascades and runs out from the
     // insertions; its rounds follow the same predicted depth, corrected by the running mean of
     // their difference on earlier batches (rounded up, as for the invalidation)
-    if (learned && c.bfs_async && predicted_depth >= 0)
+    if (learned && c.relax_async && predicted_depth >= 0)
       relax_group = (int)std::min<int64_t>(
           std::max<int64_t>(predicted_depth + (int64_t)std::ceil(c.relax_bias), kFirstGroup), kMaxGroup);
     if (L == c.level && hint_relax > 0) relax_group = hint_relax;
     int64_t r0 = rounds;
     SCOUT_CHECK(relax_levels(c, L, P, smin, smax, c.wa.p, nseeds, epoch, &rounds, &scanned, relax_group,
-                             c.bfs_async));
+                             c.relax_async));
     st->relax_rounds += rounds - r0;
-    if (learned && c.bfs_async && predicted_depth >= 0)
+    if (learned && c.relax_async && predicted_depth >= 0)
       c.relax_bias = 0.75 * c.relax_bias + 0.25 * (double)(rounds - r0 - predicted_depth);
   }
   st->rounds += rounds;
@@ -1743,11 +1743,12 @@ int scout_bfs_bound(scout_ctx* x, int64_t bound) {
 }
 
 int scout_bfs_mode(scout_ctx* x, int mode) {
-  if (!x || (mode != SCOUT_BFS_ASYNC && mode != SCOUT_BFS_LEVEL_SYNC)) {
-    set_error("bfs_mode: mode must be SCOUT_BFS_ASYNC or SCOUT_BFS_LEVEL_SYNC");
+  if (!x || (mode != SCOUT_BFS_ASYNC && mode != SCOUT_BFS_LEVEL_SYNC && mode != SCOUT_BFS_ASYNC_INVALIDATION)) {
+    set_error("bfs_mode: mode must be SCOUT_BFS_ASYNC, SCOUT_BFS_LEVEL_SYNC or SCOUT_BFS_ASYNC_INVALIDATION");
     return SCOUT_ERR_ARG;
   }
-  x->c.bfs_async = mode == SCOUT_BFS_ASYNC;
+  x->c.bfs_async = mode != SCOUT_BFS_LEVEL_SYNC;
+  x->c.relax_async = mode == SCOUT_BFS_ASYNC;
   return SCOUT_OK;
 }
 

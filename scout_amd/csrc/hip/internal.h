@@ -229,7 +229,8 @@ struct Ctx {
   int64_t apply_seq = 0;         // batches applied to the graph
   int64_t bfs_bound = 0;          // scout_bfs_bound: invalidated vertices per deletion candidate past which a
                                  // batch's BFS repair is finished by a full recompute (0 = never)
-  bool bfs_async = true;         // scout_bfs_mode: asynchronous (true) or level-synchronous BFS rounds
+  bool bfs_async = true;         // scout_bfs_mode: asynchronous invalidation rounds (else level-synchronous)
+  bool relax_async = true;       // scout_bfs_mode: asynchronous relax rounds (else level-synchronous)
   double fetch_ms = 0;           // copy-stream time of streamed base-entry chunks (storage-resident mode)
   int64_t fetch_bytes = 0;       // bytes of those chunks
   int64_t level_seq = -1;        // apply_seq at which `level` was last made exact
