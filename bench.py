@@ -13,6 +13,11 @@ the repair itself (the batches depend on the levels it produces); the state is t
 and the W + K batches replayed: W untimed, K timed. Every committed state is exact: the final
 levels and labels are compared with full recomputes on the device.
 
+After the headline measurement, the other configurations of the benchmark plan run as child
+processes (colouring + MIS at R-MAT scale 24, 64 Barabasi-Albert instances, the serving loop at
+R-MAT scale 25 offered 2e6 and 1e9 edges/s), and their results go under "other_configs"
+(--no-side skips them).
+
 Prints one JSON line (see the keys below). --gpus N > 1 exits: this build runs on one GPU.
 """
 import argparse
@@ -186,6 +191,49 @@ def summarize_pmc(pmc_dir, last_batches=0):
     out["per_batch"] = {f: {"fetch_bytes": round(v["fetch_bytes"] / batches), "write_bytes": round(v["write_bytes"] / batches),
                             "traffic_bytes": round((v["fetch_bytes"] + v["write_bytes"]) / batches)}
                         for f, v in fam.items()}
+    return out
+
+
+# The other configurations of the benchmark plan, each measured by its own command in a child
+# process after the headline line's measurement: (command, time limit in s, keys kept from its
+# JSON line). Child stderr (progress) passes through; their JSON line is taken from stdout.
+SIDE_CONFIGS = {
+    "coloring_mis_rmat24": (["bench.py", "--workload", "color-mis", "--steps", "10", "--warmup", "2", "--no-cpu"], 240,
+                            ["value", "unit", "p50_ms", "p99_ms", "verified_exact", "config"]),
+    "multi_instance_ba64": (["scout_amd/bin/scout_multi"], 240,
+                            ["updates_per_s", "alone_updates_per_s", "speedup_vs_alone", "step_ms_p50",
+                             "same_as_alone", "proper_coloring", "independent_mis", "color_config", "mis_config",
+                             "workload"]),
+    "serving_rmat25_2e6": (["scout_amd/bin/scout_serve"], 120,
+                           ["updates_per_s", "offered_per_s", "p50_ms", "p99_ms", "max_ms", "steps", "pcie_h2d_GBps",
+                            "same_as_alone", "verified_exact", "workload"]),
+    "serving_rmat25_saturated": (["scout_amd/bin/scout_serve", "--rate", "1e9", "--requests", "300"], 120,
+                                 ["updates_per_s", "offered_per_s", "p50_ms", "p99_ms", "steps", "max_requests_per_step",
+                                  "steps_staged_while_device_busy", "same_as_alone", "verified_exact"]),
+}
+
+
+def side_configs():
+    """Runs SIDE_CONFIGS one after another; a failure or time-out is recorded, not raised."""
+    out = {}
+    for name, (cmd, limit, keys) in SIDE_CONFIGS.items():
+        argv = ([sys.executable, "-u", os.path.join(ROOT, cmd[0])] if cmd[0].endswith(".py")
+                else [os.path.join(ROOT, cmd[0])]) + cmd[1:]
+        log(f"side configuration {name}: {' '.join(cmd)}")
+        t0 = time.perf_counter()
+        try:
+            r = subprocess.run(argv, cwd=ROOT, stdout=subprocess.PIPE, text=True, timeout=limit)
+            lines = [l for l in r.stdout.splitlines() if l.startswith("{")]
+            if r.returncode != 0 or not lines:
+                out[name] = {"error": f"exit status {r.returncode}", "command": " ".join(cmd)}
+            else:
+                d = json.loads(lines[-1])
+                out[name] = {k: d.get(k) for k in keys}
+                out[name]["command"] = " ".join(cmd)
+        except subprocess.TimeoutExpired:
+            out[name] = {"error": f"time limit {limit} s", "command": " ".join(cmd)}
+        out[name]["seconds"] = round(time.perf_counter() - t0, 1)
+        log(f"side configuration {name}: {out[name]}")
     return out
 
 
@@ -547,6 +595,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-verify", action="store_true")
+    ap.add_argument("--no-side", action="store_true",
+                    help="skip the other configurations (colouring + MIS, 64 instances, serving) measured after the "
+                         "headline")
     ap.add_argument("--no-graph", action="store_true",
                     help="color-mis: launch the repair rounds kernel by kernel instead of replaying captured graphs")
     ap.add_argument("--summarize-pmc", metavar="DIR",
@@ -813,6 +864,10 @@ def main():
         "work_per_batch": {k: round(v / steps, 1) for k, v in head["work"].items()},
         "cpu_baseline": cpu,
     }
+    if not args.no_side:
+        # the headline's context holds ~12 GB of HBM: freed before the child processes start
+        eng.close()
+        out["other_configs"] = side_configs()
     print(json.dumps(out), flush=True)
 
 
