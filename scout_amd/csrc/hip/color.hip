@@ -317,7 +317,7 @@ void launch_group(Ctx& c, ColorState& s, const RoundBufs& b) {
 std::vector<const void*> graph_key(Ctx& c, ColorState& s, const RoundBufs& b) {
   GView g = c.g.view();
   return {b.la, b.lb, b.prop, (const void*)(intptr_t)b.cap, c.color, c.col_marker, c.d_box, s.wave_list.p,
-          s.block_list.p, g.boff, g.bcol, g.aoff, g.alen, g.acol, g.bsamp, g.touched, (const void*)(intptr_t)g.n,
+          s.block_list.p, g.boff, g.bcol, g.ahead, g.acol, g.bsamp, g.touched, (const void*)(intptr_t)g.n,
           (const void*)(intptr_t)s.thread_max, (const void*)(intptr_t)s.block_min, (const void*)(intptr_t)s.grid,
           (const void*)(intptr_t)s.group_rounds};
 }

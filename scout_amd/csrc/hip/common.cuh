@@ -27,8 +27,8 @@ __device__ __forceinline__ Row row_of(const GView& g, int32_t v) {
     uint32_t bit = 1u << (v & 31);
     if (!(g.touched[v >> 5] & bit)) atomicOr(&g.touched[v >> 5], bit);
   }
-  int64_t a0 = g.aoff[v];
-  return Row{g.boff[v], g.boff[v + 1], a0, a0 + g.alen[v]};
+  AHead h = g.ahead[v];
+  return Row{g.boff[v], g.boff[v + 1], h.off, h.off + h.len};
 }
 __device__ __forceinline__ int64_t row_len(const Row& r) { return (r.b1 - r.b0) + (r.a1 - r.a0); }
 // k-th physical entry of the row (base segment first, then the inserted segment). Negative
@@ -482,7 +482,8 @@ __device__ __forceinline__ PairState pair_state_in(const GView& g, int32_t u, in
     else if (x == ~v) s.base_tomb = true;
   }
   s.base_any = s.base_live || s.base_tomb;
-  int64_t ab = g.aoff[u], ae = ab + g.alen[u];
+  AHead hu = g.ahead[u];
+  int64_t ab = hu.off, ae = ab + hu.len;
   for (int64_t k = ab; k < ae; k++) {
     int32_t x = g.acol[k];
     if (x == v) s.added_live = true;
@@ -503,7 +504,8 @@ __device__ __forceinline__ PairState pair_state(const GView& g, bool sorted, int
     else if (x == ~v) s.base_tomb = true;
   }
   s.base_any = s.base_live || s.base_tomb;
-  int64_t ab = g.aoff[u], ae = ab + g.alen[u];
+  AHead hu = g.ahead[u];
+  int64_t ab = hu.off, ae = ab + hu.len;
   for (int64_t k = ab; k < ae; k++) {
     int32_t x = g.acol[k];
     if (x == v) s.added_live = true;
@@ -548,7 +550,8 @@ __device__ __forceinline__ bool wave_adjacent(const GView& g, bool sorted, int32
     }
   }
   unsigned long long alive = 0, atomb = 0;
-  int64_t ab = g.aoff[u], ae = ab + g.alen[u];
+  AHead hu = g.ahead[u];
+  int64_t ab = hu.off, ae = ab + hu.len;
   for (int64_t base = ab; base < ae; base += kWave) {
     int64_t k = base + lane_id();
     int32_t x = k < ae ? g.acol[k] : HOLE;

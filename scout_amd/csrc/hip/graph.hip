@@ -145,7 +145,8 @@ __device__ __forceinline__ void flip_base(const GView& g, const PairState& s, in
 // First inserted-row entry of u equal to x set to HOLE. True if one was found.
 __device__ __forceinline__ bool hole_first(const GView& g, int32_t u, int32_t x) {
   int32_t* col = const_cast<int32_t*>(g.acol);
-  for (int64_t k = g.aoff[u], e = k + g.alen[u]; k < e; k++) {
+  AHead h = g.ahead[u];
+  for (int64_t k = h.off, e = k + h.len; k < e; k++) {
     if (col[k] == x) {
       col[k] = HOLE;
       return true;
@@ -266,34 +267,31 @@ struct MoveList {
   int64_t* count;
 };
 
-// Places k new entries of row u (dst[0..k)) after its len entries, or moves the row to the
-// span of `need` slots reserved at pos (need = 0: the span has room).
-__device__ __forceinline__ void place_group(int64_t* aoff, int32_t* alen, int32_t* acap, int32_t* acol, int32_t* aown,
-                                            const int32_t* dst, int64_t k, int32_t u, int32_t len, int64_t need,
-                                            int64_t pos, int64_t pool_cap, int64_t src, const MoveList& mv,
-                                            int64_t* overflow) {
+// Places k new entries of row u (dst[0..k)) after the h.len entries of its header h, or moves
+// the row to the span of `need` slots reserved at pos (need = 0: the span has room).
+__device__ __forceinline__ void place_group(AHead* ahead, int32_t* acol, int32_t* aown, const int32_t* dst, int64_t k,
+                                            int32_t u, AHead h, int64_t need, int64_t pos, int64_t pool_cap,
+                                            int64_t src, const MoveList& mv, int64_t* overflow) {
+  int32_t len = h.len;
   if (need == 0) {
-    int64_t at = aoff[u] + len;
+    int64_t at = h.off + len;
     for (int64_t j = 0; j < k; j++) {
       acol[at + j] = dst[j];
       aown[at + j] = u;
     }
-    alen[u] = (int32_t)(len + k);
+    ahead[u].len = (int32_t)(len + k);
   } else if (pos + need > pool_cap) {
     atomicAdd((unsigned long long*)overflow, 1ULL);
   } else if (len <= kMoveThread) {
-    int64_t old = aoff[u];
     for (int64_t j = 0; j < len; j++) {
-      acol[pos + j] = acol[old + j];
+      acol[pos + j] = acol[h.off + j];
       aown[pos + j] = u;
     }
     for (int64_t j = 0; j < k; j++) {
       acol[pos + len + j] = dst[j];
       aown[pos + len + j] = u;
     }
-    aoff[u] = pos;
-    acap[u] = (int32_t)need;
-    alen[u] = (int32_t)(len + k);
+    ahead[u] = AHead{pos, (int32_t)(len + k), (int32_t)need};
   } else {
     int64_t slot = (int64_t)atomicAdd((unsigned long long*)mv.count, 1ULL);
     mv.u[slot] = u;
@@ -308,7 +306,7 @@ __device__ __forceinline__ void place_group(int64_t* aoff, int32_t* alen, int32_
 // list (mkey/mdst, *nmulti). The span of a row that must move is reserved on *top with one
 // atomic per block; the pool has room for the worst case (checked on the host before the
 // batch), and *overflow counts reservations past it all the same.
-__global__ void k_place_single(int64_t* aoff, int32_t* alen, int32_t* acap, int32_t* acol, int32_t* aown,
+__global__ void k_place_single(AHead* ahead, int32_t* acol, int32_t* aown,
                                const uint64_t* nkey, const int32_t* ndst, const int64_t* count, int64_t cap, int ks,
                                int32_t* ncnt, uint64_t* mkey, int32_t* mdst, int64_t* nmulti, int64_t* top,
                                int64_t pool_cap, MoveList mv, int64_t* overflow) {
@@ -321,16 +319,16 @@ __global__ void k_place_single(int64_t* aoff, int32_t* alen, int32_t* acap, int3
   bool single = have && ncnt[u] == 1;
   int64_t pm = block_append(have && !single, key, mkey, nmulti, cap);
   if (pm >= 0) mdst[pm] = dst;
-  int32_t len = 0;
+  AHead h{0, 0, 0};
   int64_t need = 0;
   if (single) {
-    len = alen[u];
-    if (len + 1 > acap[u]) need = grown_capacity(len + 1);
+    h = ahead[u];
+    if (h.len + 1 > h.cap) need = grown_capacity(h.len + 1);
   }
   int64_t pos = block_reserve(need, top);
   if (!single) return;
   ncnt[u] = 0;
-  place_group(aoff, alen, acap, acol, aown, &ndst[i], 1, u, len, need, pos, pool_cap, -1 - i, mv, overflow);
+  place_group(ahead, acol, aown, &ndst[i], 1, u, h, need, pos, pool_cap, -1 - i, mv, overflow);
 }
 
 // Multi entries one block groups (at most kSmallSort / 2 sources, each with two or more
@@ -456,7 +454,7 @@ __global__ __launch_bounds__(kSortThreads) void k_group_small(const uint64_t* ki
 // The grouped multi entries (skey/sdst, *count of them): the first entry of each source's run
 // places the run (one thread per run, a block-wide reservation per pass) and zeroes the
 // source's count. Nothing when *skip is set (the entries were not sorted).
-__global__ void k_place_multi(int64_t* aoff, int32_t* alen, int32_t* acap, int32_t* acol, int32_t* aown,
+__global__ void k_place_multi(AHead* ahead, int32_t* acol, int32_t* aown,
                               const uint64_t* skey, const int32_t* sdst, const int64_t* count, int64_t cap, int ks,
                               int32_t* ncnt, int64_t* top, int64_t pool_cap, MoveList mv, int64_t* overflow,
                               const int64_t* skip) {
@@ -466,33 +464,35 @@ __global__ void k_place_multi(int64_t* aoff, int32_t* alen, int32_t* acap, int32
   for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < cnt; base += (int64_t)gridDim.x * blockDim.x) {
     int64_t i = base + threadIdx.x;
     bool head = i < cnt && (i == 0 || (skey[i] >> ks) != (skey[i - 1] >> ks));
-    int32_t u = -1, len = 0;
+    int32_t u = -1;
+    AHead h{0, 0, 0};
     int64_t k = 0, need = 0;
     if (head) {
       u = (int32_t)(skey[i] >> ks);
       int64_t e = i + 1;
       while (e < cnt && (int32_t)(skey[e] >> ks) == u) e++;
       k = e - i;
-      len = alen[u];
-      if (len + k > acap[u]) need = grown_capacity(len + k);
+      h = ahead[u];
+      if (h.len + k > h.cap) need = grown_capacity(h.len + k);
     }
     int64_t pos = block_reserve(need, top);
     if (head) {
       ncnt[u] = 0;
-      place_group(aoff, alen, acap, acol, aown, &sdst[i], k, u, len, need, pos, pool_cap, i, mv, overflow);
+      place_group(ahead, acol, aown, &sdst[i], k, u, h, need, pos, pool_cap, i, mv, overflow);
     }
   }
 }
 
 // Wave per long row that moves: its entries, then the new ones, into the reserved span.
-__global__ void k_move_rows(int64_t* aoff, int32_t* alen, int32_t* acap, int32_t* acol, int32_t* aown,
-                            const int32_t* ndst, const int32_t* sdst, MoveList mv) {
+__global__ void k_move_rows(AHead* ahead, int32_t* acol, int32_t* aown, const int32_t* ndst, const int32_t* sdst,
+                            MoveList mv) {
   int64_t nm = *mv.count;
   for (int64_t i = global_wave(); i < nm; i += total_waves()) {
     int32_t u = mv.u[i];
     int64_t src = mv.src[i], k = mv.k[i], pos = mv.pos[i];
     const int32_t* d = src < 0 ? ndst + (-1 - src) : sdst + src;
-    int64_t len = alen[u], old = aoff[u];
+    AHead h = ahead[u];
+    int64_t len = h.len, old = h.off;
     for (int64_t j = lane_id(); j < len; j += kWave) {
       acol[pos + j] = acol[old + j];
       aown[pos + j] = u;
@@ -501,11 +501,7 @@ __global__ void k_move_rows(int64_t* aoff, int32_t* alen, int32_t* acap, int32_t
       acol[pos + len + j] = d[j];
       aown[pos + len + j] = u;
     }
-    if (lane_id() == 0) {
-      aoff[u] = pos;
-      acap[u] = grown_capacity(len + k);
-      alen[u] = (int32_t)(len + k);
-    }
+    if (lane_id() == 0) ahead[u] = AHead{pos, (int32_t)(len + k), grown_capacity(len + k)};
   }
 }
 
@@ -530,7 +526,8 @@ __global__ void k_repack_count(GView g, int32_t* cnt, int32_t* cap2, int64_t* li
   for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base <= g.n; base += (int64_t)gridDim.x * blockDim.x) {
     int64_t v = base + threadIdx.x;
     if (v < g.n) {
-      int64_t a0 = g.aoff[v], a1 = a0 + g.alen[v];
+      AHead h = g.ahead[v];
+      int64_t a0 = h.off, a1 = a0 + h.len;
       int32_t k = 0;
       for (int64_t j = a0; j < a1; j++) k += g.acol[j] != HOLE;
       cnt[v] = k;
@@ -545,7 +542,8 @@ __global__ void k_repack_count(GView g, int32_t* cnt, int32_t* cap2, int64_t* li
 
 __global__ void k_repack_copy(GView g, const int64_t* aoff2, int32_t* acol2, int32_t* aown2) {
   for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < g.n; v += (int64_t)gridDim.x * blockDim.x) {
-    int64_t a0 = g.aoff[v], a1 = a0 + g.alen[v], o = aoff2[v];
+    AHead h = g.ahead[v];
+    int64_t a0 = h.off, a1 = a0 + h.len, o = aoff2[v];
     for (int64_t j = a0; j < a1; j++) {
       int32_t x = g.acol[j];
       if (x != HOLE) {
@@ -555,6 +553,12 @@ __global__ void k_repack_copy(GView g, const int64_t* aoff2, int32_t* acol2, int
       }
     }
   }
+}
+
+// The re-packed headers: start, live count and capacity of every row (and the n-th).
+__global__ void k_repack_heads(AHead* ahead2, const int64_t* off2, const int32_t* cnt, const int32_t* cap2, int64_t n) {
+  int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v <= n) ahead2[v] = AHead{off2[v], v < n ? cnt[v] : 0, cap2[v]};
 }
 
 // Live entries per vertex of the logical graph: thread per vertex for rows up to 64 physical
@@ -660,24 +664,24 @@ int upload_chunked(Ctx& c, void* dst, const void* src, size_t bytes) {
 int repack_inserted(Ctx& c, int64_t headroom) {
   DevGraph& g = c.g;
   int32_t n = g.n;
-  SCOUT_CHECK(g.acap2.reserve((int64_t)n + 1));
-  SCOUT_CHECK(g.aoff2.reserve((int64_t)n + 1));
+  SCOUT_CHECK(g.rcap.reserve((int64_t)n + 1));
+  SCOUT_CHECK(g.roff.reserve((int64_t)n + 1));
+  SCOUT_CHECK(g.ahead2.reserve((int64_t)n + 1));
   SCOUT_TRY(hipMemsetAsync(c.d_box + 32, 0, sizeof(int64_t), c.s));
-  // counts land in alen (read by the copy through the old aoff... so stage them in moves)
+  // live counts staged in moves, new capacities in rcap, new starts (their scan) in roff
   SCOUT_CHECK(g.moves.reserve((int64_t)n + 1));
-  k_repack_count<<<kListGrid, kBlock, 0, c.s>>>(g.view(), g.moves.p, g.acap2.p, c.d_box + 32);
-  SCOUT_CHECK(scan_exclusive_i32_to_i64(c, g.acap2.p, g.aoff2.p, (int64_t)n + 1));
+  k_repack_count<<<kListGrid, kBlock, 0, c.s>>>(g.view(), g.moves.p, g.rcap.p, c.d_box + 32);
+  SCOUT_CHECK(scan_exclusive_i32_to_i64(c, g.rcap.p, g.roff.p, (int64_t)n + 1));
   int64_t total = 0;
-  SCOUT_TRY(hipMemcpyAsync(&total, g.aoff2.p + n, sizeof(int64_t), hipMemcpyDeviceToHost, c.s));
+  SCOUT_TRY(hipMemcpyAsync(&total, g.roff.p + n, sizeof(int64_t), hipMemcpyDeviceToHost, c.s));
   SCOUT_TRY(hipMemcpyAsync(c.h_box + 32, c.d_box + 32, sizeof(int64_t), hipMemcpyDeviceToHost, c.s));
   SCOUT_TRY(hipStreamSynchronize(c.s));
   c.readbacks++;
   SCOUT_CHECK(g.acol2.reserve(total + headroom + 1024));
   SCOUT_CHECK(g.aown2.reserve(total + headroom + 1024));
-  k_repack_copy<<<kListGrid, kBlock, 0, c.s>>>(g.view(), g.aoff2.p, g.acol2.p, g.aown2.p);
-  SCOUT_TRY(hipMemcpyAsync(g.alen.p, g.moves.p, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToDevice, c.s));
-  std::swap(g.aoff, g.aoff2);
-  std::swap(g.acap, g.acap2);
+  k_repack_copy<<<kListGrid, kBlock, 0, c.s>>>(g.view(), g.roff.p, g.acol2.p, g.aown2.p);
+  k_repack_heads<<<grid_for((int64_t)n + 1), kBlock, 0, c.s>>>(g.ahead2.p, g.roff.p, g.moves.p, g.rcap.p, n);
+  std::swap(g.ahead, g.ahead2);
   std::swap(g.acol, g.acol2);
   std::swap(g.aown, g.aown2);
   g.A = total;
@@ -761,15 +765,15 @@ int apply_range(Ctx& c, Channel& ch, int64_t lo, int64_t hi, int conditional, co
   MoveList mv = move_list(c);
   int64_t pcap = pool_cap(g);
   if (m > 0) {
-    k_place_single<<<grid_for(bound), kBlock, 0, c.s>>>(g.aoff.p, g.alen.p, g.acap.p, g.acol.p, g.aown.p, g.nkey.p,
+    k_place_single<<<grid_for(bound), kBlock, 0, c.s>>>(g.ahead.p, g.acol.p, g.aown.p, g.nkey.p,
                                                        g.ndst.p, c.d_box, cap, ks, g.ncnt.p, g.mkey.p, g.mdst.p,
                                                        c.d_box + kMultiSlot, c.d_box + 4, pcap, mv, c.d_box + 5);
     k_group_small<<<1, kSortThreads, 0, c.s>>>(g.mkey.p, g.mdst.p, c.d_box + kMultiSlot, cap, ks, g.nkey.p,
                                                g.nkey2.p, g.ndst2.p, c.d_box + kSortSlot);
-    k_place_multi<<<kPlaceGrid, kBlock, 0, c.s>>>(g.aoff.p, g.alen.p, g.acap.p, g.acol.p, g.aown.p, g.nkey2.p,
+    k_place_multi<<<kPlaceGrid, kBlock, 0, c.s>>>(g.ahead.p, g.acol.p, g.aown.p, g.nkey2.p,
                                                   g.ndst2.p, c.d_box + kMultiSlot, cap, ks, g.ncnt.p, c.d_box + 4, pcap,
                                                   mv, c.d_box + 5, c.d_box + kSortSlot);
-    k_move_rows<<<kListGrid, kBlock, 0, c.s>>>(g.aoff.p, g.alen.p, g.acap.p, g.acol.p, g.aown.p, g.ndst.p,
+    k_move_rows<<<kListGrid, kBlock, 0, c.s>>>(g.ahead.p, g.acol.p, g.aown.p, g.ndst.p,
                                                g.ndst2.p, mv);
   }
   SCOUT_CHECK(box_read(c, kSortSlot + 1));
@@ -794,10 +798,10 @@ int apply_range(Ctx& c, Channel& ch, int64_t lo, int64_t hi, int conditional, co
     SCOUT_CHECK(sort_pairs_u64(c, g.mkey.p, g.nkey2.p, g.mdst.p, g.ndst2.p, nm,
                                ks + bits_for(std::max<int64_t>(g.n - 1, 0))));
     k_place_reset<<<1, 1, 0, c.s>>>(c.d_box, g.A);
-    k_place_multi<<<kPlaceGrid, kBlock, 0, c.s>>>(g.aoff.p, g.alen.p, g.acap.p, g.acol.p, g.aown.p, g.nkey2.p,
+    k_place_multi<<<kPlaceGrid, kBlock, 0, c.s>>>(g.ahead.p, g.acol.p, g.aown.p, g.nkey2.p,
                                                   g.ndst2.p, c.d_box + kMultiSlot, cap, ks, g.ncnt.p, c.d_box + 4, pcap,
                                                   mv, c.d_box + 5, nullptr);
-    k_move_rows<<<kListGrid, kBlock, 0, c.s>>>(g.aoff.p, g.alen.p, g.acap.p, g.acol.p, g.aown.p, g.ndst.p,
+    k_move_rows<<<kListGrid, kBlock, 0, c.s>>>(g.ahead.p, g.acol.p, g.aown.p, g.ndst.p,
                                                g.ndst2.p, mv);
     SCOUT_CHECK(box_read(c, 6));
     if (c.h_box[5] != 0) {
@@ -838,11 +842,10 @@ int graph_free(Ctx& c) {
   g.host_bcol = false;
   g.hoff.clear();
   g.hoff.shrink_to_fit();
-  g.aoff.release();
-  g.aoff2.release();
-  g.alen.release();
-  g.acap.release();
-  g.acap2.release();
+  g.ahead.release();
+  g.ahead2.release();
+  g.roff.release();
+  g.rcap.release();
   g.acol.release();
   g.acol2.release();
   g.aown.release();
@@ -869,12 +872,8 @@ int graph_free(Ctx& c) {
 int reset_inserted(Ctx& c) {
   DevGraph& g = c.g;
   size_t n1 = (size_t)g.n + 1;
-  SCOUT_CHECK(g.aoff.reserve((int64_t)n1));
-  SCOUT_CHECK(g.alen.reserve((int64_t)n1));
-  SCOUT_CHECK(g.acap.reserve((int64_t)n1));
-  SCOUT_TRY(hipMemsetAsync(g.aoff.p, 0, n1 * sizeof(int64_t), c.s));
-  SCOUT_TRY(hipMemsetAsync(g.alen.p, 0, n1 * sizeof(int32_t), c.s));
-  SCOUT_TRY(hipMemsetAsync(g.acap.p, 0, n1 * sizeof(int32_t), c.s));
+  SCOUT_CHECK(g.ahead.reserve((int64_t)n1));
+  SCOUT_TRY(hipMemsetAsync(g.ahead.p, 0, n1 * sizeof(AHead), c.s));
   SCOUT_CHECK(g.acol.reserve(1 << 16));
   SCOUT_CHECK(g.aown.reserve(g.acol.cap));
   g.A = g.live = g.holes = 0;

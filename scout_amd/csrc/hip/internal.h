@@ -3,10 +3,11 @@
 // HBM layout of the dynamic graph (one copy per context):
 //   boff[n+1] int64, bcol[E] int32   base CSR; a deleted base entry keeps its slot and is
 //                                    stored as ~v (negative), so rows never move
-//   aoff[n] int64, alen/acap[n] int32, acol/aown[pool] int32
+//   ahead[n] {off int64, len int32, cap int32}, acol/aown[pool] int32
 //                                    inserted edges of v in insertion order at
-//                                    acol[aoff[v] .. aoff[v] + alen[v]), inside a reserved
-//                                    span of acap[v] slots of a bump-allocated pool; aown
+//                                    acol[off .. off + len) of ahead[v], inside a reserved
+//                                    span of cap slots of a bump-allocated pool (the header is
+//                                    one 16-byte load); aown
 //                                    names the vertex a pool slot was written for. An entry is
 //                                    v (live), ~v (a tombstone on a pair the base row does not
 //                                    hold) or HOLE (a forgotten insertion). A row that
@@ -99,12 +100,19 @@ struct PinBuf {
   }
 };
 
+// Header of a vertex's inserted row: its span [off, off + cap) of the pool, of which the first
+// len slots are in use. One aligned 16-byte record, so a row's header is one load.
+struct alignas(16) AHead {
+  int64_t off;
+  int32_t len;
+  int32_t cap;
+};
+
 // Read-only view of the logical graph for kernels.
 struct GView {
   const int64_t* boff;
   const int32_t* bcol;
-  const int64_t* aoff;
-  const int32_t* alen;
+  const AHead* ahead;
   const int32_t* acol;
   int32_t n;
   const int32_t* bsamp;  // decoded bcol[kSampleStride * k], or nullptr (rows searched directly)
@@ -120,7 +128,7 @@ struct DevGraph {
   int32_t n = 0;
   int64_t E = 0;        // base entries
   int64_t A = 0;        // pool slots handed out (rows, their slack and moved-away spans)
-  int64_t live = 0;     // entries inside rows (sum of alen), holes included
+  int64_t live = 0;     // entries inside rows (sum of the headers' len), holes included
   int64_t holes = 0;    // HOLE entries inside rows
   bool sorted = false;  // every base row ascending (by decoded id)
   int64_t base_gen = 0; // bumped whenever the base arrays are replaced or restored
@@ -131,8 +139,9 @@ struct DevGraph {
   bool host_bcol = false;
   int32_t* hbcol = nullptr;
   std::vector<int64_t> hoff;
-  DevBuf<int64_t> aoff, aoff2;
-  DevBuf<int32_t> alen, acap, acap2;
+  DevBuf<AHead> ahead, ahead2;
+  DevBuf<int64_t> roff;   // re-pack scratch: new row starts
+  DevBuf<int32_t> rcap;   // re-pack scratch: new row capacities
   DevBuf<int32_t> acol, acol2, aown, aown2;
   // rows of this batch's new entries that move to fresh pool space (row, span, entries)
   DevBuf<int32_t> moves, move_k;
@@ -149,7 +158,7 @@ struct DevGraph {
   // set only while a repair runs with row tracking on (scout_touched_enable)
   uint32_t* touched = nullptr;
   GView view() const {
-    return GView{boff, bcol, aoff.p, alen.p, acol.p, n, (sorted && bsamp_gen == base_gen) ? bsamp.p : nullptr,
+    return GView{boff, bcol, ahead.p, acol.p, n, (sorted && bsamp_gen == base_gen) ? bsamp.p : nullptr,
                  touched};
   }
 };
@@ -288,6 +297,8 @@ enum PathId { PATH_INVAL_TASKS = 0, PATH_INVAL_PUSH_TASKS, PATH_INVAL_KIDS_OVERF
 // prims.hip (library scans/sorts)
 int scan_exclusive_i64(Ctx& c, const int64_t* in, int64_t* out, int64_t count);
 int scan_exclusive_i32_to_i64(Ctx& c, const int32_t* in, int64_t* out, int64_t count);
+// out[k] = the sum of the inserted-row lengths before row k (exclusive scan of ahead[].len).
+int scan_row_lens(Ctx& c, const AHead* in, int64_t* out, int64_t count);
 int sort_pairs_u64(Ctx& c, const uint64_t* kin, uint64_t* kout, const int32_t* vin, int32_t* vout, int64_t count,
                    int end_bit);
 int sort_keys_u64(Ctx& c, const uint64_t* kin, uint64_t* kout, int64_t count, int begin_bit, int end_bit);

@@ -91,8 +91,8 @@ __device__ __forceinline__ bool member(const MisView& s, int32_t x, bool* confli
 // Live entries equal to y in row x (base entries not marked deleted, inserted entries); the
 // shorter of the two rows is searched (the adjacency is symmetric).
 __device__ __forceinline__ int32_t live_count(const MisView& s, int32_t x, int32_t y) {
-  int64_t lx = (s.g.boff[x + 1] - s.g.boff[x]) + s.g.alen[x];
-  int64_t ly = (s.g.boff[y + 1] - s.g.boff[y]) + s.g.alen[y];
+  int64_t lx = (s.g.boff[x + 1] - s.g.boff[x]) + s.g.ahead[x].len;
+  int64_t ly = (s.g.boff[y + 1] - s.g.boff[y]) + s.g.ahead[y].len;
   if (ly < lx) {
     int32_t tmp = x;
     x = y;
@@ -102,7 +102,8 @@ __device__ __forceinline__ int32_t live_count(const MisView& s, int32_t x, int32
   base_range(s.g, s.sorted, x, y, lo, hi);
   int32_t c = 0;
   for (int64_t k = lo; k < hi; k++) c += s.g.bcol[k] == y;
-  for (int64_t k = s.g.aoff[x], e = k + s.g.alen[x]; k < e; k++) c += s.g.acol[k] == y;
+  AHead hx = s.g.ahead[x];
+  for (int64_t k = hx.off, e = k + hx.len; k < e; k++) c += s.g.acol[k] == y;
   return c;
 }
 
@@ -1121,7 +1122,7 @@ int mm_graph(Ctx& c, MisState& ms, SegView sv) {
                                   (const void*)(intptr_t)c.g.sorted, ms.h2_line.p, ms.h2_seg.p, ms.h2_flag.p,
                                   ms.h2_best.p, ms.h2_off.p, ms.h2_task.p, ms.h2_y.p, ms.h2_rsb.p, ms.h2_keyA.p,
                                   (const void*)(intptr_t)ms.h2_rcap,
-                                  (const void*)(intptr_t)ms.h2_tcap, g.boff, g.bcol, g.aoff, g.alen, g.acol, g.bsamp, g.touched,
+                                  (const void*)(intptr_t)ms.h2_tcap, g.boff, g.bcol, g.ahead, g.acol, g.bsamp, g.touched,
                                   (const void*)(intptr_t)g.n,
                                   (const void*)(intptr_t)sv.stride, (const void*)(intptr_t)ms.grid,
                                   (const void*)(intptr_t)ms.group_rounds};

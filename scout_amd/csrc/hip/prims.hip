@@ -33,6 +33,20 @@ int scan_exclusive_i32_to_i64(Ctx& c, const int32_t* in, int64_t* out, int64_t c
   });
 }
 
+namespace {
+struct RowLen {
+  __host__ __device__ int64_t operator()(const AHead& h) const { return h.len; }
+};
+}  // namespace
+
+int scan_row_lens(Ctx& c, const AHead* in, int64_t* out, int64_t count) {
+  if (count <= 0) return SCOUT_OK;
+  hipcub::TransformInputIterator<int64_t, RowLen, const AHead*> it(in, RowLen());
+  return with_temp(c, [&](void* t, size_t& b) {
+    return hipcub::DeviceScan::ExclusiveSum(t, b, it, out, count, c.s);
+  });
+}
+
 int sort_pairs_u64(Ctx& c, const uint64_t* kin, uint64_t* kout, const int32_t* vin, int32_t* vout, int64_t count,
                    int end_bit) {
   if (count <= 0) return SCOUT_OK;

@@ -100,8 +100,8 @@ __global__ void k_gen_emit(const uint64_t* cand, const int32_t* acc, const int64
 // The inserted entries in logical order (rows by id, entries in row order), independent of
 // where rows sit in the pool: rowpre[u] = inserted-row slots of the rows before u
 // (rowpre[n] = all of them), so entry j lies in the last row u with rowpre[u] <= j.
-__global__ void k_rowpre_total(const int32_t* alen, int32_t n, int64_t* rowpre) {
-  rowpre[n] = n > 0 ? rowpre[n - 1] + alen[n - 1] : 0;
+__global__ void k_rowpre_total(const AHead* ahead, int32_t n, int64_t* rowpre) {
+  rowpre[n] = n > 0 ? rowpre[n - 1] + ahead[n - 1].len : 0;
 }
 
 // Inserted entry j of the logical order: its row and value; false if not live.
@@ -114,7 +114,7 @@ __device__ __forceinline__ bool pool_entry(const GView& g, const int64_t* rowpre
     else hi = mid - 1;
   }
   *owner = (uint32_t)lo;
-  *w = g.acol[g.aoff[lo] + (j - rowpre[lo])];
+  *w = g.acol[g.ahead[lo].off + (j - rowpre[lo])];
   return *w >= 0;
 }
 
@@ -327,8 +327,8 @@ int gen_batch(Ctx& c, Channel& ch, int64_t size, double delete_frac, double tree
   int64_t V = 0;
   SCOUT_CHECK(s.rowpre.reserve((int64_t)c.g.n + 1));
   if (c.g.A > 0) {
-    SCOUT_CHECK(scan_exclusive_i32_to_i64(c, c.g.alen.p, s.rowpre.p, c.g.n));
-    k_rowpre_total<<<1, 1, 0, c.s>>>(c.g.alen.p, c.g.n, s.rowpre.p);
+    SCOUT_CHECK(scan_row_lens(c, c.g.ahead.p, s.rowpre.p, c.g.n));
+    k_rowpre_total<<<1, 1, 0, c.s>>>(c.g.ahead.p, c.g.n, s.rowpre.p);
     SCOUT_TRY(hipMemcpyAsync(c.h_box, s.rowpre.p + c.g.n, sizeof(int64_t), hipMemcpyDeviceToHost, c.s));
     SCOUT_TRY(hipStreamSynchronize(c.s));
     c.readbacks++;
@@ -499,8 +499,9 @@ void stream_release(Ctx& c) {
 
 // Device copies of the graph and every allocated solution, for replaying a stream.
 struct Snapshot {
-  DevBuf<int64_t> boff, aoff;
-  DevBuf<int32_t> bcol, acol, aown, alen, acap;
+  DevBuf<int64_t> boff;
+  DevBuf<AHead> ahead;
+  DevBuf<int32_t> bcol, acol, aown;
   int64_t live = 0;
   DevBuf<int32_t> level, parent, label, color;
   DevBuf<uint8_t> mis;
@@ -518,16 +519,12 @@ int snapshot_save(Ctx& c) {
   Snapshot& s = *c.snap;
   int64_t n = c.g.n;
   SCOUT_CHECK(s.boff.reserve(n + 1));
-  SCOUT_CHECK(s.aoff.reserve(n + 1));
+  SCOUT_CHECK(s.ahead.reserve(n + 1));
   SCOUT_CHECK(s.bcol.reserve(c.g.E + 1));
   SCOUT_CHECK(s.acol.reserve(c.g.A + 1));
   SCOUT_CHECK(s.aown.reserve(c.g.A + 1));
-  SCOUT_CHECK(s.alen.reserve(n + 1));
-  SCOUT_CHECK(s.acap.reserve(n + 1));
   SCOUT_TRY(hipMemcpyAsync(s.boff.p, c.g.boff, (size_t)(n + 1) * 8, hipMemcpyDeviceToDevice, c.s));
-  SCOUT_TRY(hipMemcpyAsync(s.aoff.p, c.g.aoff.p, (size_t)(n + 1) * 8, hipMemcpyDeviceToDevice, c.s));
-  SCOUT_TRY(hipMemcpyAsync(s.alen.p, c.g.alen.p, (size_t)(n + 1) * 4, hipMemcpyDeviceToDevice, c.s));
-  SCOUT_TRY(hipMemcpyAsync(s.acap.p, c.g.acap.p, (size_t)(n + 1) * 4, hipMemcpyDeviceToDevice, c.s));
+  SCOUT_TRY(hipMemcpyAsync(s.ahead.p, c.g.ahead.p, (size_t)(n + 1) * sizeof(AHead), hipMemcpyDeviceToDevice, c.s));
   SCOUT_TRY(hipMemcpyAsync(s.bcol.p, c.g.bcol, (size_t)c.g.E * 4, hipMemcpyDeviceToDevice, c.s));
   if (c.g.A > 0) {
     SCOUT_TRY(hipMemcpyAsync(s.acol.p, c.g.acol.p, (size_t)c.g.A * 4, hipMemcpyDeviceToDevice, c.s));
@@ -576,10 +573,9 @@ int snapshot_restore(Ctx& c) {
     SCOUT_TRY(hipMalloc((void**)&c.g.bcol, (size_t)std::max<int64_t>(s.E, 1) * 4));
   }
   SCOUT_TRY(hipMemcpyAsync(c.g.boff, s.boff.p, (size_t)(n + 1) * 8, hipMemcpyDeviceToDevice, c.s));
-  SCOUT_TRY(hipMemcpyAsync(c.g.aoff.p, s.aoff.p, (size_t)(n + 1) * 8, hipMemcpyDeviceToDevice, c.s));
+  SCOUT_TRY(hipMemcpyAsync(c.g.ahead.p, s.ahead.p, (size_t)(n + 1) * sizeof(AHead), hipMemcpyDeviceToDevice, c.s));
   SCOUT_TRY(hipMemcpyAsync(c.g.bcol, s.bcol.p, (size_t)s.E * 4, hipMemcpyDeviceToDevice, c.s));
-  SCOUT_TRY(hipMemcpyAsync(c.g.alen.p, s.alen.p, (size_t)(n + 1) * 4, hipMemcpyDeviceToDevice, c.s));
-  SCOUT_TRY(hipMemcpyAsync(c.g.acap.p, s.acap.p, (size_t)(n + 1) * 4, hipMemcpyDeviceToDevice, c.s));
+
   SCOUT_CHECK(c.g.acol.reserve(s.A + 1));
   SCOUT_CHECK(c.g.aown.reserve(c.g.acol.cap));
   if (s.A > 0) {
@@ -618,12 +614,10 @@ void snapshot_release(Ctx& c) {
   if (!c.snap) return;
   Snapshot& s = *c.snap;
   s.boff.release();
-  s.aoff.release();
+  s.ahead.release();
   s.bcol.release();
   s.acol.release();
   s.aown.release();
-  s.alen.release();
-  s.acap.release();
   s.level.release();
   s.parent.release();
   s.label.release();
