@@ -337,7 +337,8 @@ void bucket_by_level(hipStream_t s, K src, int64_t count, int32_t lmin, int nb, 
 // here; rows up to kTask go to the wave list, longer rows to chunk tasks.
 // *seeded += the seeds taken.
 // Asynchronous (A): every frontier vertex relaxes from its current level, whatever the round;
-// the first round takes every seed (seeds[0..ns), once each under seed_epoch). A vertex may be
+// the first round takes every seed (seeds[0..ns), or [0..*ns_dev) when the count is on the
+// device, at most seed_cap; once each under seed_epoch). A vertex may be
 // lowered in several rounds, and each lowering queues it for the next round (epoch is that
 // round's). A frontier vertex also takes a neighbour one level up as its parent: concurrent
 // lowerings of one vertex in one round may leave the parent of a level that lost the race,
@@ -347,8 +348,10 @@ __global__ void k_relax_short(GView g, int32_t* level, int32_t* parent, const in
                               const int64_t* seeds, int64_t ns, const int64_t* sb, int64_t* seeded, int32_t lvl,
                               int32_t* next, int64_t* next_count, int64_t cap, int32_t* stamp, int32_t epoch,
                               int32_t seed_epoch, int32_t* medium, int64_t* medium_count, int64_t* tasks,
-                              int64_t* task_count, int64_t task_cap, int64_t* scanned) {
+                              int64_t* task_count, int64_t task_cap, int64_t* scanned, const int64_t* ns_dev,
+                              int64_t seed_cap) {
   int64_t nf = *count;
+  if (ns_dev) ns = min(*ns_dev, seed_cap);
   if (!A && sb && ns > 0) {
     seeds += sb[0];
     ns = sb[1] - sb[0];
@@ -1227,13 +1230,15 @@ __global__ void k_insert_seeds(GView g, bool sorted, bool distinct, const int32_
 
 // A seed that still holds the level it was lowered to takes the vertex that lowered it as
 // its parent (several seeds of one vertex at one level are all valid parents).
-__global__ void k_seed_parents(const int64_t* seeds, const int32_t* seed_par, int64_t ns, const int32_t* level,
-                               int32_t* parent) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= ns) return;
-  int64_t s = seeds[i];
-  int32_t lv = (int32_t)(s >> 32), v = (int32_t)(s & 0xffffffff);
-  if (level[v] == lv) parent[v] = seed_par[i];
+// ns_dev (optional): the count on the device, at most cap (grid-stride over it).
+__global__ void k_seed_parents(const int64_t* seeds, const int32_t* seed_par, int64_t ns, const int64_t* ns_dev,
+                               int64_t cap, const int32_t* level, int32_t* parent) {
+  if (ns_dev) ns = min(*ns_dev, cap);
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < ns; i += (int64_t)gridDim.x * blockDim.x) {
+    int64_t s = seeds[i];
+    int32_t lv = (int32_t)(s >> 32), v = (int32_t)(s & 0xffffffff);
+    if (level[v] == lv) parent[v] = seed_par[i];
+  }
 }
 
 // ---------------------------------------------------------------------------------------
@@ -1294,7 +1299,7 @@ __global__ void k_find_parents(GView g, const int32_t* level, int32_t* parent, i
 // Adds the non-empty rounds to *rounds.
 int relax_levels(Ctx& c, int32_t* L, int32_t* P, int32_t lvl0, int32_t max_seed, const int64_t* seeds, int64_t nseeds,
                  int32_t epoch, int64_t* rounds, int64_t* scanned_total, int first_group = kFirstGroup,
-                 bool async = false) {
+                 bool async = false, const int64_t* nseeds_dev = nullptr, int64_t seed_cap = 0) {
   int32_t n = c.g.n;
   int64_t cap = (int64_t)n + 1;
   int64_t task_cap = c.wb.cap;
@@ -1331,20 +1336,24 @@ int relax_levels(Ctx& c, int32_t* L, int32_t* P, int32_t lvl0, int32_t max_seed,
       // level-synchronous: the seeds of this level are taken by the round's first kernel;
       // asynchronous: all of them by the first round, each round queueing under its own epoch
       int64_t ns = async ? (done == 0 ? nseeds : 0) : (nseeds > 0 && lvl <= max_seed) ? nseeds : 0;
+      const int64_t* ns_dev = async && done == 0 ? nseeds_dev : nullptr;
       const int64_t* sb = (!async && sbuckets && ns > 0) ? sbuckets + (lvl - lvl0) : nullptr;
       int32_t ep = async ? next_epoch(c) : epoch;
-      RoundBook book{bcur, box + 15, box + 17, box + 16, tasks_of[p ^ 1], medium_of[p ^ 1], nullptr, ns > 0};
+      RoundBook book{bcur, box + 15, box + 17, box + 16, tasks_of[p ^ 1], medium_of[p ^ 1], nullptr,
+                     ns > 0 || ns_dev != nullptr};
       if (async) {
         SCOUT_CHECK(prof_launch(c, PROF_K_RELAX_SHORT, k_relax_short<true>, dim3(kListGrid), dim3(kBlock),
                                 c.g.view(), L, P, cur, bcur, seeds, ns, sb, box + 15, lvl, nxt, bnext, cap, c.stamp,
-                                ep, epoch, c.le.p, medium_of[p], c.wb.p, tasks_of[p], task_cap, box + 13));
+                                ep, epoch, c.le.p, medium_of[p], c.wb.p, tasks_of[p], task_cap, box + 13, ns_dev,
+                                seed_cap));
         SCOUT_CHECK(prof_launch(c, PROF_K_RELAX_ROWS, k_relax_rows<true>, dim3(kListGrid), dim3(kBlock), c.g.view(),
                                 L, P, c.le.p, medium_of[p], c.wb.p, tasks_of[p], task_cap, lvl, nxt, bnext, cap,
                                 c.stamp, ep, box + 12, c.d_paths, book));
       } else {
         SCOUT_CHECK(prof_launch(c, PROF_K_RELAX_SHORT, k_relax_short<false>, dim3(kListGrid), dim3(kBlock),
                                 c.g.view(), L, P, cur, bcur, seeds, ns, sb, box + 15, lvl, nxt, bnext, cap, c.stamp,
-                                ep, epoch, c.le.p, medium_of[p], c.wb.p, tasks_of[p], task_cap, box + 13));
+                                ep, epoch, c.le.p, medium_of[p], c.wb.p, tasks_of[p], task_cap, box + 13, nullptr,
+                                0));
         SCOUT_CHECK(prof_launch(c, PROF_K_RELAX_ROWS, k_relax_rows<false>, dim3(kListGrid), dim3(kBlock),
                                 c.g.view(), L, P, c.le.p, medium_of[p], c.wb.p, tasks_of[p], task_cap, lvl, nxt,
                                 bnext, cap, c.stamp, ep, box + 12, c.d_paths, book));
@@ -1354,7 +1363,9 @@ int relax_levels(Ctx& c, int32_t* L, int32_t* P, int32_t lvl0, int32_t max_seed,
       p ^= 1;
     }
     SCOUT_CHECK(prof_end(c, PROF_BFS_RELAX, 0));
-    SCOUT_TRY(hipMemcpyAsync(c.h_box + 8, box + 8, 12 * sizeof(int64_t), hipMemcpyDeviceToHost, c.s));
+    // [8..19] this relax's counters; [4..7] and [36..38] come back too (the seed and
+    // tentative-level counters of a local repair, which read them after the relax)
+    SCOUT_TRY(hipMemcpyAsync(c.h_box + 4, box + 4, 35 * sizeof(int64_t), hipMemcpyDeviceToHost, c.s));
     SCOUT_TRY(hipStreamSynchronize(c.s));
     c.readbacks++;
     if (std::max({c.h_box[10], c.h_box[16], c.h_box[18]}) > task_cap) {
@@ -1627,13 +1638,13 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, int32_t* P, int strategy, scou
       return SCOUT_OK;
     }
     if (nx > 0) {
-      // [11] scanned, [12] medium rows, [13] chunk tasks
-      k_zero<<<1, 64, 0, c.s>>>(box + 11, 3);
+      // [36] scanned, [37] medium rows, [38] chunk tasks (slots the relax leaves alone)
+      k_zero<<<1, 64, 0, c.s>>>(box + 36, 3);
       uint64_t* tent = reinterpret_cast<uint64_t*>(c.wc.p);
-      k_tentative_short<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, c.ld.p, nx, tent, c.lf.p, box + 12, c.lf.cap,
-                                                        c.wb.p, box + 13, c.wb.cap, box + 11);
-      k_tentative_rows<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, c.ld.p, c.lf.p, box + 12, c.wb.p, box + 13,
-                                                       c.wb.cap, tent, box + 11, c.d_paths);
+      k_tentative_short<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, c.ld.p, nx, tent, c.lf.p, box + 37, c.lf.cap,
+                                                        c.wb.p, box + 38, c.wb.cap, box + 36);
+      k_tentative_rows<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, c.ld.p, c.lf.p, box + 37, c.wb.p, box + 38,
+                                                       c.wb.cap, tent, box + 36, c.d_paths);
       k_apply_tentative<<<grid_for(nx, kBlock * kLinesPerThread), kBlock, 0, c.s>>>(c.ld.p, tent, nx, L, P, c.wa.p,
                                                                                     c.lg.p, seed_cap, box);
     }
@@ -1641,23 +1652,46 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, int32_t* P, int strategy, scou
   if (m > 0)
     k_insert_seeds<<<grid_for(m, kBlock * kLinesPerThread), kBlock, 0, c.s>>>(
         c.g.view(), c.g.sorted, ch.pairs_distinct, ch.d_uv.p, ch.d_ins.p, m, L, c.wa.p, c.lg.p, seed_cap, box);
-  SCOUT_CHECK(box_read(c, 14));
-  if (nx > 0) scanned += c.h_box[11];
-  if (nx > 0 && c.h_box[13] > c.wb.cap) {
+  int64_t nseeds = 0;
+  if (c.relax_async) {
+    // No read-back here: the seeds' count stays on the device for the relax's first round, and
+    // the seed and tentative-level counters come back with the relax's read-back.
+    k_seed_parents<<<kListGrid, kBlock, 0, c.s>>>(c.wa.p, c.lg.p, 0, box + 4, seed_cap, L, P);
+    int32_t epoch = next_epoch(c);
+    k_zero<<<1, 64, 0, c.s>>>(box + 8, 1);
+    // the relax re-attaches the invalidated cascades and runs out from the insertions; under
+    // the learned strategy its rounds follow the predicted depth, corrected by the running mean
+    // of their difference on earlier batches (rounded up, as for the invalidation)
+    int relax_group = kFirstGroup;
+    if (learned && predicted_depth >= 0)
+      relax_group = (int)std::min<int64_t>(
+          std::max<int64_t>(predicted_depth + (int64_t)std::ceil(c.relax_bias), kFirstGroup), kMaxGroup);
+    if (L == c.level && hint_relax > 0) relax_group = hint_relax;
+    int64_t r0 = rounds;
+    SCOUT_CHECK(relax_levels(c, L, P, 0, 0, c.wa.p, 0, epoch, &rounds, &scanned, relax_group, true, box + 4,
+                             seed_cap));
+    st->relax_rounds += rounds - r0;
+    if (learned && predicted_depth >= 0)
+      c.relax_bias = 0.75 * c.relax_bias + 0.25 * (double)(rounds - r0 - predicted_depth);
+    nseeds = c.h_box[4];
+  }
+  if (!c.relax_async) SCOUT_CHECK(box_read(c, 39));
+  if (nx > 0) scanned += c.h_box[36];
+  if (nx > 0 && c.h_box[38] > c.wb.cap) {
     set_error("bfs tentative levels: chunk task list overflow");
     return SCOUT_ERR_STATE;
   }
   st->insertions += c.h_box[7];
-  int64_t nseeds = c.h_box[4];
+  nseeds = c.h_box[4];
   if (nseeds > seed_cap) {
     set_error("bfs: seed list overflow");
     return SCOUT_ERR_STATE;
   }
   st->seeds += nseeds;
   st->touched += nx;
-  if (nseeds > 0) {
+  if (!c.relax_async && nseeds > 0) {
     int32_t smin = (int32_t)c.h_box[5], smax = (int32_t)c.h_box[6];
-    k_seed_parents<<<grid_for(nseeds), kBlock, 0, c.s>>>(c.wa.p, c.lg.p, nseeds, L, P);
+    k_seed_parents<<<grid_for(nseeds), kBlock, 0, c.s>>>(c.wa.p, c.lg.p, nseeds, nullptr, seed_cap, L, P);
     int32_t epoch = next_epoch(c);
     k_zero<<<1, 64, 0, c.s>>>(box + 8, 1);
     // the learned scheduler's predicted last level bounds the relax as well
@@ -1665,19 +1699,10 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, int32_t* P, int strategy, scou
     if (learned && predicted_last >= 0)
       relax_group = (int)std::min<int64_t>(
           std::max<int64_t>(std::max<int64_t>(predicted_last, smax) - smin + 1, kFirstGroup), kMaxGroup);
-    // asynchronous: the relax re-attaches the invalidated cascades and runs out from the
-    // insertions; its rounds follow the same predicted depth, corrected by the running mean of
-    // their difference on earlier batches (rounded up, as for the invalidation)
-    if (learned && c.relax_async && predicted_depth >= 0)
-      relax_group = (int)std::min<int64_t>(
-          std::max<int64_t>(predicted_depth + (int64_t)std::ceil(c.relax_bias), kFirstGroup), kMaxGroup);
     if (L == c.level && hint_relax > 0) relax_group = hint_relax;
     int64_t r0 = rounds;
-    SCOUT_CHECK(relax_levels(c, L, P, smin, smax, c.wa.p, nseeds, epoch, &rounds, &scanned, relax_group,
-                             c.relax_async));
+    SCOUT_CHECK(relax_levels(c, L, P, smin, smax, c.wa.p, nseeds, epoch, &rounds, &scanned, relax_group, false));
     st->relax_rounds += rounds - r0;
-    if (learned && c.relax_async && predicted_depth >= 0)
-      c.relax_bias = 0.75 * c.relax_bias + 0.25 * (double)(rounds - r0 - predicted_depth);
   }
   st->rounds += rounds;
   st->edges_scanned += scanned;

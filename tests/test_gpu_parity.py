@@ -104,14 +104,16 @@ def test_bfs_cc_exact_every_batch(data, name, mode):
     eng.close()
 
 
+@pytest.mark.parametrize("bfs_mode", [sa.BFS_LEVEL_SYNC, sa.BFS_ASYNC_INVALIDATION])
 @pytest.mark.parametrize("name", ["sample", "er", "rmat", "deg", "heavy"])
-def test_bfs_level_sync_exact_every_batch(data, name):
-    """The level-synchronous BFS round order (one level per round) gives the oracle's levels
-    and labels after every batch too; the tests above run the default asynchronous order."""
+def test_bfs_level_sync_exact_every_batch(data, name, bfs_mode):
+    """The level-synchronous BFS round order (one level per round), and the asynchronous
+    invalidation with a level-synchronous relax, give the oracle's levels and labels after every
+    batch too; the tests above run the default asynchronous order."""
     egr, bdir, nb = data[name]
-    say(f"[bfs level-sync] {name}")
+    say(f"[bfs mode {bfs_mode}] {name}")
     eng, g = engine_for(egr)
-    eng.bfs_mode(sa.BFS_LEVEL_SYNC)
+    eng.bfs_mode(bfs_mode)
     og = OracleGraph(egr)
     eng.upload(sa.STATE_LEVEL, og.bfs(0))
     eng.upload(sa.STATE_LABEL, og.cc())
