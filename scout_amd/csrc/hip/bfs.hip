@@ -334,7 +334,8 @@ void bucket_by_level(hipStream_t s, K src, int64_t count, int32_t lmin, int nb, 
 // Thread per frontier vertex still at level lvl, and per seed (level << 32 | vertex) of this
 // level not queued before: seeds[0..ns) filtered by level, or, when the seeds are grouped by
 // level, seeds[sb[0]..sb[1]) (ns = 0 in rounds past the deepest seed). Short rows are relaxed
-// here; rows up to kTask go to the wave list, longer rows to chunk tasks.
+// here; rows up to kSmall go to the group list, up to kTask to the wave list, longer rows to
+// chunk tasks.
 // *seeded += the seeds taken.
 // Asynchronous (A): every frontier vertex relaxes from its current level, whatever the round;
 // the first round takes every seed (seeds[0..ns), or [0..*ns_dev) when the count is on the
@@ -347,7 +348,8 @@ template <bool A>
 __global__ void k_relax_short(GView g, int32_t* level, int32_t* parent, const int32_t* front, const int64_t* count,
                               const int64_t* seeds, int64_t ns, const int64_t* sb, int64_t* seeded, int32_t lvl,
                               int32_t* next, int64_t* next_count, int64_t cap, int32_t* stamp, int32_t epoch,
-                              int32_t seed_epoch, int32_t* medium, int64_t* medium_count, int64_t* tasks,
+                              int32_t seed_epoch, int32_t* small, int64_t* small_count, int32_t* medium,
+                              int64_t* medium_count, int64_t* tasks,
                               int64_t* task_count, int64_t task_cap, int64_t* scanned, const int64_t* ns_dev,
                               int64_t seed_cap) {
   int64_t nf = *count;
@@ -389,7 +391,8 @@ __global__ void k_relax_short(GView g, int32_t* level, int32_t* parent, const in
       len = row_len(r);
     }
     bool is_short = active && len <= kShort;
-    bool is_medium = active && !is_short && len <= kTask;
+    bool is_small = active && !is_short && len <= kSmall;
+    bool is_medium = active && len > kSmall && len <= kTask;
     bool is_long = active && len > kTask;
     int32_t xs[kShort];
     gather_short(g, r, len, is_short, xs);
@@ -411,11 +414,12 @@ __global__ void k_relax_short(GView g, int32_t* level, int32_t* parent, const in
       take[k] = xs[k] >= 0 && (lv[k] == -1 || lv[k] > lf + 1) && relax_to(level, parent, stamp, epoch, f, xs[k], lf);
       nnext += take[k];
     }
-    // one reservation for the block's medium rows, chunk tasks and lowered neighbours
-    int32_t need[3] = {is_medium, is_long ? (int32_t)((len + kTask - 1) / kTask) : 0, nnext};
-    int64_t pos[3];
-    block_reserve_n(need, pos, medium_count, task_count, next_count);
+    // one reservation for the block's group and wave rows, chunk tasks and lowered neighbours
+    int32_t need[4] = {is_medium, is_long ? (int32_t)((len + kTask - 1) / kTask) : 0, nnext, is_small};
+    int64_t pos[4];
+    block_reserve_n(need, pos, medium_count, task_count, next_count, small_count);
     if (is_medium && pos[0] < cap) medium[pos[0]] = f;
+    if (is_small && pos[3] < cap) small[pos[3]] = f;
     for (int64_t k = 0; k < need[1] && pos[1] + k < task_cap; k++) tasks[pos[1] + k] = ((int64_t)f << 32) | k;
     write_taken(take, xs, next, pos[2], cap);
   }
@@ -423,19 +427,60 @@ __global__ void k_relax_short(GView g, int32_t* level, int32_t* parent, const in
   block_add(taken, seeded);
 }
 
-// Wave per medium frontier vertex (listed by k_relax_short) or per chunk task of a long one.
+// Group of kGroup lanes per small frontier vertex (four per wave), then wave per medium one
+// (both listed by k_relax_short) or per chunk task of a long one.
 // Asynchronous (A): relaxes from the vertex's current level and sets its parent to a
 // neighbour one level up, as k_relax_short does.
 template <bool A>
-__global__ void k_relax_rows(GView g, int32_t* level, int32_t* parent, const int32_t* medium,
-                             const int64_t* medium_count, const int64_t* tasks, const int64_t* task_count,
-                             int64_t task_cap, int32_t lvl, int32_t* next, int64_t* next_count, int64_t cap,
-                             int32_t* stamp, int32_t epoch, int64_t* scanned, int64_t* paths, RoundBook book) {
+__global__ void k_relax_rows(GView g, int32_t* level, int32_t* parent, const int32_t* small,
+                             const int64_t* small_count, const int32_t* medium, const int64_t* medium_count,
+                             const int64_t* tasks, const int64_t* task_count, int64_t task_cap, int32_t lvl,
+                             int32_t* next, int64_t* next_count, int64_t cap, int32_t* stamp, int32_t epoch,
+                             int64_t* scanned, int64_t* paths, RoundBook book) {
   __shared__ int32_t seg_all[kWavesPerBlock * kNextSeg];
   WaveStage<kNextSeg> st{seg_all + (threadIdx.x / kWave) * kNextSeg, 0};
   int64_t nm = *medium_count;
   int64_t nt = min(*task_count, task_cap);
-  int64_t work = 0, ntask = 0;
+  int64_t work = 0, ntask = 0, gwork = 0;
+  constexpr int kPerWave = kWave / kGroup;
+  int64_t nsm = *small_count;
+  for (int64_t base = global_wave() * kPerWave; base < nsm; base += total_waves() * kPerWave) {
+    int64_t idx = base + lane_id() / kGroup;
+    bool act = idx < nsm;
+    int32_t f = act ? small[idx] : -1;
+    Row r{0, 0, 0, 0};
+    int64_t len = 0;
+    int32_t lf = lvl;
+    if (act) {
+      r = row_of(g, f);
+      len = row_len(r);
+      if constexpr (A) lf = level[f];
+    }
+    act = act && lf >= 0;
+    int32_t ws[kU];
+    group_load(g, r, len, act, ws);
+    int32_t lw[kU];
+#pragma unroll
+    for (int j = 0; j < kU; j++) lw[j] = ws[j] >= 0 ? level[ws[j]] : 0;
+    if constexpr (A) {
+      // a neighbour one level up as the parent: the first in row order over the group
+      int32_t mine = INT32_MAX;
+#pragma unroll
+      for (int j = 0; j < kU; j++)
+        if (mine == INT32_MAX && ws[j] >= 0 && lw[j] == lf - 1) mine = group_lane() + j * kGroup;
+      int32_t at = group_min(mine);
+#pragma unroll
+      for (int j = 0; j < kU; j++)
+        if (act && lf > 0 && group_lane() + j * kGroup == at) parent[f] = ws[j];
+    }
+#pragma unroll
+    for (int j = 0; j < kU; j++) {
+      bool take = act && ws[j] >= 0 && (lw[j] == -1 || lw[j] > lf + 1) && relax_to(level, parent, stamp, epoch, f,
+                                                                                     ws[j], lf);
+      stage_push(st, take, ws[j], next, next_count, cap);
+    }
+    if (act && group_lane() == 0) gwork += len;
+  }
   for (int64_t t = global_wave(); t < nm + nt; t += total_waves()) {
     int32_t f;
     int64_t lo, hi;
@@ -482,7 +527,7 @@ __global__ void k_relax_rows(GView g, int32_t* level, int32_t* parent, const int
   }
   stage_finish(st, next, next_count, cap);
   count_path(ntask, &paths[PATH_RELAX_TASKS]);
-  block_add(lane_id() == 0 ? work : 0, scanned);
+  block_add((lane_id() == 0 ? work : 0) + gwork, scanned);
   round_book(book);
 }
 
@@ -1292,6 +1337,28 @@ __global__ void k_find_parents(GView g, const int32_t* level, int32_t* parent, i
   }
 }
 
+// Launch shape of the asynchronous rounds. A phase's first rounds carry nearly all of its work
+// (every candidate or seed and its first cascade step) and take the full list grid; the rounds
+// after them hold a few hundred vertices or fewer, where dispatching kListGrid mostly idle
+// blocks costs more than the work: they take one block per CU (R-MAT scale 26, 1M-line
+// batches: 5.32e8 -> 5.44e8 updates/s; 128 and 512 blocks measured no better). A read-back
+// that finds a frontier larger than one pass of the small grid restores the full grid for the
+// next group's first rounds. The level-synchronous order, whose later levels can be the widest,
+// always takes the full grid.
+constexpr int kFullRounds = 2;
+constexpr unsigned kTailGrid = 256;
+constexpr int64_t kTailFront = (int64_t)kTailGrid * kBlock;
+
+// Grid of the next round; counts down the full-grid rounds left.
+unsigned round_grid(bool async, int& full_rounds) {
+  if (!async) return kListGrid;
+  if (full_rounds > 0) {
+    full_rounds--;
+    return kListGrid;
+  }
+  return kTailGrid;
+}
+
 // Relax with the current list already in `cur` (count in box slot 8) and optional seeds.
 // Level-synchronous (async = false): starts at level `lvl0` and runs until no frontier and no
 // seed above the current level remains. Asynchronous: the first round takes every seed and
@@ -1307,13 +1374,14 @@ int relax_levels(Ctx& c, int32_t* L, int32_t* P, int32_t lvl0, int32_t max_seed,
   int32_t* nxt = c.lc.p;
   int64_t* box = c.d_box;
   // box slots: [8]/[9] frontier counts (current and next alternate), [10]/[11] and [18]/[19]
-  // chunk tasks and medium rows (the counter sets of alternate rounds), [12] scanned entries,
-  // [13] entries of short rows, [15] frontier vertices over all rounds, [16] largest task
-  // count, [17] non-empty rounds
+  // chunk tasks and medium rows, [20]/[21] small rows (the counter sets of alternate rounds),
+  // [12] scanned entries, [13] entries of short rows, [15] frontier vertices over all rounds,
+  // [16] largest task count, [17] non-empty rounds
   int64_t* cnt[2] = {box + 8, box + 9};
   int64_t* tasks_of[2] = {box + 10, box + 18};
   int64_t* medium_of[2] = {box + 11, box + 19};
-  k_zero2<<<1, 64, 0, c.s>>>(box + 9, 5, box + 15, 5);
+  int64_t* small_of[2] = {box + 20, box + 21};  // rows walked by lane groups
+  k_zero2<<<1, 64, 0, c.s>>>(box + 9, 5, box + 15, 7);
   // seeds grouped by level once (each round then reads only its own), when the levels they
   // span fit the bucket counters and the list fits the scratch
   const int64_t* sbuckets = nullptr;
@@ -1328,6 +1396,7 @@ int relax_levels(Ctx& c, int32_t* L, int32_t* P, int32_t lvl0, int32_t max_seed,
   int p = 0;
   int group = first_group;
   int64_t done = 0;  // rounds queued so far
+  int full_rounds = kFullRounds;
   while (true) {
     SCOUT_CHECK(prof_begin(c, PROF_BFS_RELAX));
     for (int k = 0; k < group; k++, done++) {
@@ -1339,23 +1408,26 @@ int relax_levels(Ctx& c, int32_t* L, int32_t* P, int32_t lvl0, int32_t max_seed,
       const int64_t* ns_dev = async && done == 0 ? nseeds_dev : nullptr;
       const int64_t* sb = (!async && sbuckets && ns > 0) ? sbuckets + (lvl - lvl0) : nullptr;
       int32_t ep = async ? next_epoch(c) : epoch;
-      RoundBook book{bcur, box + 15, box + 17, box + 16, tasks_of[p ^ 1], medium_of[p ^ 1], nullptr,
+      unsigned rg = round_grid(async, full_rounds);
+      RoundBook book{bcur, box + 15, box + 17, box + 16, tasks_of[p ^ 1], medium_of[p ^ 1], small_of[p ^ 1],
                      ns > 0 || ns_dev != nullptr};
       if (async) {
-        SCOUT_CHECK(prof_launch(c, PROF_K_RELAX_SHORT, k_relax_short<true>, dim3(kListGrid), dim3(kBlock),
+        SCOUT_CHECK(prof_launch(c, PROF_K_RELAX_SHORT, k_relax_short<true>, dim3(rg), dim3(kBlock),
                                 c.g.view(), L, P, cur, bcur, seeds, ns, sb, box + 15, lvl, nxt, bnext, cap, c.stamp,
-                                ep, epoch, c.le.p, medium_of[p], c.wb.p, tasks_of[p], task_cap, box + 13, ns_dev,
+                                ep, epoch, c.lh.p, small_of[p], c.le.p, medium_of[p], c.wb.p, tasks_of[p], task_cap,
+                                box + 13, ns_dev,
                                 seed_cap));
-        SCOUT_CHECK(prof_launch(c, PROF_K_RELAX_ROWS, k_relax_rows<true>, dim3(kListGrid), dim3(kBlock), c.g.view(),
-                                L, P, c.le.p, medium_of[p], c.wb.p, tasks_of[p], task_cap, lvl, nxt, bnext, cap,
+        SCOUT_CHECK(prof_launch(c, PROF_K_RELAX_ROWS, k_relax_rows<true>, dim3(rg), dim3(kBlock), c.g.view(),
+                                L, P, c.lh.p, small_of[p], c.le.p, medium_of[p], c.wb.p, tasks_of[p], task_cap, lvl, nxt, bnext, cap,
                                 c.stamp, ep, box + 12, c.d_paths, book));
       } else {
-        SCOUT_CHECK(prof_launch(c, PROF_K_RELAX_SHORT, k_relax_short<false>, dim3(kListGrid), dim3(kBlock),
+        SCOUT_CHECK(prof_launch(c, PROF_K_RELAX_SHORT, k_relax_short<false>, dim3(rg), dim3(kBlock),
                                 c.g.view(), L, P, cur, bcur, seeds, ns, sb, box + 15, lvl, nxt, bnext, cap, c.stamp,
-                                ep, epoch, c.le.p, medium_of[p], c.wb.p, tasks_of[p], task_cap, box + 13, nullptr,
+                                ep, epoch, c.lh.p, small_of[p], c.le.p, medium_of[p], c.wb.p, tasks_of[p], task_cap,
+                                box + 13, nullptr,
                                 0));
-        SCOUT_CHECK(prof_launch(c, PROF_K_RELAX_ROWS, k_relax_rows<false>, dim3(kListGrid), dim3(kBlock),
-                                c.g.view(), L, P, c.le.p, medium_of[p], c.wb.p, tasks_of[p], task_cap, lvl, nxt,
+        SCOUT_CHECK(prof_launch(c, PROF_K_RELAX_ROWS, k_relax_rows<false>, dim3(rg), dim3(kBlock),
+                                c.g.view(), L, P, c.lh.p, small_of[p], c.le.p, medium_of[p], c.wb.p, tasks_of[p], task_cap, lvl, nxt,
                                 bnext, cap, c.stamp, ep, box + 12, c.d_paths, book));
       }
       lvl++;
@@ -1373,6 +1445,7 @@ int relax_levels(Ctx& c, int32_t* L, int32_t* P, int32_t lvl0, int32_t max_seed,
       return SCOUT_ERR_STATE;
     }
     if (c.h_box[8 + p] == 0 && (async || nseeds == 0 || lvl > max_seed)) break;
+    full_rounds = c.h_box[8 + p] > kTailFront ? kFullRounds : 0;
     // after a predicted first group the repair is near its end: start again from a short group
     group = group == first_group && first_group != kFirstGroup ? kFirstGroup : std::min(2 * group, kMaxGroup);
     first_group = kFirstGroup;
@@ -1540,9 +1613,11 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, int32_t* P, int strategy, scou
     int p = 0;
     int group = first_group;
     int32_t lvl = cmin;
+    int full_rounds = kFullRounds;
     while (true) {
       SCOUT_CHECK(prof_begin(c, PROF_BFS_INVALIDATE));
       for (int k = 0; k < group; k++) {
+        unsigned ig = round_grid(async, full_rounds);
         int64_t* bcur = cnt[p];
         int64_t* bnext = cnt[p ^ 1];
         // candidates of this level are taken by the round's first kernel, which also lists the
@@ -1558,30 +1633,30 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, int32_t* P, int strategy, scou
         if (async) {
           supepoch = next_epoch(c);
           if (supepoch == 1) supepoch = next_epoch(c);  // the counter started over
-          SCOUT_CHECK(prof_launch(c, PROF_K_INVALIDATE_SHORT, k_invalidate_short<true>, dim3(kListGrid),
+          SCOUT_CHECK(prof_launch(c, PROF_K_INVALIDATE_SHORT, k_invalidate_short<true>, dim3(ig),
                                   dim3(kBlock), c.g.view(), L, P, cur, bcur, cand, nlevel, cb, box + 18, lvl, c.ld.p,
                                   binval, nxt, bnext, cap, c.stamp, sepoch, c.lh.p, small_of[p], c.le.p, medium_of[p],
                                   c.lf.p, c.lg.p, longs_of[p], c.wb.p, tasks_of[p], c.wb.cap, box + 23, c.d_paths,
                                   ab));
-          SCOUT_CHECK(prof_launch(c, PROF_BFS_INVALIDATE_KERNEL, k_invalidate<true>, dim3(kListGrid), dim3(kBlock),
+          SCOUT_CHECK(prof_launch(c, PROF_BFS_INVALIDATE_KERNEL, k_invalidate<true>, dim3(ig), dim3(kBlock),
                                   c.g.view(), L, P, c.lh.p, small_of[p], c.le.p, medium_of[p], c.lf.p, c.lg.p, c.wb.p, tasks_of[p],
                                   c.wb.cap, c.aux, supepoch, lvl, c.ld.p, binval, cap, nxt, bnext, cap, c.stamp,
                                   sepoch, bscan, box + 21, c.d_paths, c.g.sorted, src, ab));
-          SCOUT_CHECK(prof_launch(c, PROF_K_INVAL_PUSH, k_inval_push<true>, dim3(kListGrid), dim3(kBlock),
+          SCOUT_CHECK(prof_launch(c, PROF_K_INVAL_PUSH, k_inval_push<true>, dim3(ig), dim3(kBlock),
                                   c.g.view(), L, P, lvl, c.lf.p, c.lg.p, c.wb.p, tasks_of[p], c.wb.cap, c.aux,
                                   supepoch, c.ld.p, binval, cap, nxt, bnext, cap, c.stamp, sepoch, box + 15,
                                   c.d_paths, ab, book));
         } else {
-          SCOUT_CHECK(prof_launch(c, PROF_K_INVALIDATE_SHORT, k_invalidate_short<false>, dim3(kListGrid),
+          SCOUT_CHECK(prof_launch(c, PROF_K_INVALIDATE_SHORT, k_invalidate_short<false>, dim3(ig),
                                   dim3(kBlock), c.g.view(), L, P, cur, bcur, cand, nlevel, cb, box + 18, lvl, c.ld.p,
                                   binval, nxt, bnext, cap, c.stamp, sepoch, c.lh.p, small_of[p], c.le.p, medium_of[p],
                                   c.lf.p, c.lg.p, longs_of[p], c.wb.p, tasks_of[p], c.wb.cap, box + 23, c.d_paths,
                                   ab));
-          SCOUT_CHECK(prof_launch(c, PROF_BFS_INVALIDATE_KERNEL, k_invalidate<false>, dim3(kListGrid),
+          SCOUT_CHECK(prof_launch(c, PROF_BFS_INVALIDATE_KERNEL, k_invalidate<false>, dim3(ig),
                                   dim3(kBlock), c.g.view(), L, P, c.lh.p, small_of[p], c.le.p, medium_of[p], c.lf.p, c.lg.p, c.wb.p,
                                   tasks_of[p], c.wb.cap, c.aux, supepoch, lvl, c.ld.p, binval, cap, nxt, bnext, cap,
                                   c.stamp, sepoch, bscan, box + 21, c.d_paths, c.g.sorted, src, ab));
-          SCOUT_CHECK(prof_launch(c, PROF_K_INVAL_PUSH, k_inval_push<false>, dim3(kListGrid), dim3(kBlock),
+          SCOUT_CHECK(prof_launch(c, PROF_K_INVAL_PUSH, k_inval_push<false>, dim3(ig), dim3(kBlock),
                                   c.g.view(), L, P, lvl, c.lf.p, c.lg.p, c.wb.p, tasks_of[p], c.wb.cap, c.aux,
                                   supepoch, c.ld.p, binval, cap, nxt, bnext, cap, c.stamp, sepoch, box + 15,
                                   c.d_paths, ab, book));
@@ -1600,6 +1675,7 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, int32_t* P, int strategy, scou
         return SCOUT_ERR_STATE;
       }
       if (c.h_box[8 + p] == 0 && (async || lvl > cmax)) break;
+      full_rounds = c.h_box[8 + p] > kTailFront ? kFullRounds : 0;
       // after a predicted first group the repair is near its end: start again from a short group
       group = group == first_group && first_group != kFirstGroup ? kFirstGroup : std::min(2 * group, kMaxGroup);
       first_group = kFirstGroup;
