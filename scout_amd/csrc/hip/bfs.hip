@@ -1534,9 +1534,107 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, int32_t* P, int strategy, scou
   int64_t sample = logging ? kMlpSample : kMlpInfer;
   if (logging || learned) SCOUT_CHECK(mlp_begin_batch(c, c.la.p, box, L, P, sample));
   if (learned) SCOUT_CHECK(mlp_predict_async(c, box, sample, box + 24));
-  SCOUT_CHECK(box_read(c, learned ? 28 : 8));
-  int64_t ncand = c.h_box[0];
-  int32_t cmin = (int32_t)c.h_box[1], cmax = (int32_t)c.h_box[2];
+  // Invalidation, queued in groups of rounds. Counts: [8]/[9] the round's list / the next
+  // (alternating), [10] invalid vertices, [11] scanned entries, [12..14] / [28..30] medium
+  // rows, long rows and chunk tasks (alternating), [31]/[32] rows walked by lane groups, [18]
+  // candidates over all rounds, [19] largest task count, [20] non-empty rounds, [15] entries
+  // read by k_inval_push, [23] entries of short rows read by k_invalidate_short.
+  // Level-synchronous: one level per round. Asynchronous (c.bfs_async): the first round holds
+  // every candidate, marked pending, and the rounds run until one queues nothing.
+  bool async = c.bfs_async;
+  int32_t sepoch = 0;
+  int32_t* cur = c.lb.p;
+  int32_t* nxt = c.lc.p;
+  const int32_t* cand = c.la.p;
+  const int64_t* cbuckets = nullptr;  // candidates grouped by level (level-synchronous)
+  int64_t* cnt[2] = {box + 8, box + 9};
+  int64_t* binval = box + 10;
+  int64_t* bscan = box + 11;
+  int64_t* medium_of[2] = {box + 12, box + 28};
+  int64_t* longs_of[2] = {box + 13, box + 29};
+  int64_t* tasks_of[2] = {box + 14, box + 30};
+  int64_t* small_of[2] = {box + 31, box + 32};
+  int p = 0;
+  int32_t lvl = 1;
+  int full_rounds = kFullRounds;
+  int64_t ncand = 0;
+  int32_t cmin = 1, cmax = 0;
+  // zeroed counters; in the asynchronous order every candidate marked pending into the first
+  // round's list (the count comes from the device; there are at most m candidates)
+  auto inval_begin = [&]() {
+    sepoch = next_epoch(c);
+    k_zero2<<<1, 64, 0, c.s>>>(box + 8, 16, box + 28, 5);
+    if (async && m > 0)
+      k_mark_pending<<<grid_for(m, kBlock * kLinesPerThread), kBlock, 0, c.s>>>(c.la.p, box, L, cur, box + 8, cap);
+  };
+  // queues one round
+  auto inval_round = [&]() -> int {
+    unsigned ig = round_grid(async, full_rounds);
+    int64_t* bcur = cnt[p];
+    int64_t* bnext = cnt[p ^ 1];
+    // candidates of this level are taken by the round's first kernel, which also lists the
+    // long rows and their chunk tasks; k_invalidate searches the medium rows and the
+    // tasks; k_inval_push settles the long rows no task supported
+    int64_t nlevel = !async && lvl <= cmax ? ncand : 0;
+    RoundBook book{bcur,           box + 18,          box + 20,        box + 19,
+                   tasks_of[p ^ 1], medium_of[p ^ 1], longs_of[p ^ 1], nlevel > 0};
+    book.other_c = small_of[p ^ 1];
+    const int64_t* cb = (cbuckets && nlevel > 0) ? cbuckets + (lvl - cmin) : nullptr;
+    // two fresh consecutive epochs: supepoch marks support, supepoch - 1 a pending neighbour
+    int32_t supepoch = next_epoch(c);
+    if (async) {
+      supepoch = next_epoch(c);
+      if (supepoch == 1) supepoch = next_epoch(c);  // the counter started over
+      SCOUT_CHECK(prof_launch(c, PROF_K_INVALIDATE_SHORT, k_invalidate_short<true>, dim3(ig),
+                              dim3(kBlock), c.g.view(), L, P, cur, bcur, cand, nlevel, cb, box + 18, lvl, c.ld.p,
+                              binval, nxt, bnext, cap, c.stamp, sepoch, c.lh.p, small_of[p], c.le.p, medium_of[p],
+                              c.lf.p, c.lg.p, longs_of[p], c.wb.p, tasks_of[p], c.wb.cap, box + 23, c.d_paths,
+                              ab));
+      SCOUT_CHECK(prof_launch(c, PROF_BFS_INVALIDATE_KERNEL, k_invalidate<true>, dim3(ig), dim3(kBlock),
+                              c.g.view(), L, P, c.lh.p, small_of[p], c.le.p, medium_of[p], c.lf.p, c.lg.p, c.wb.p, tasks_of[p],
+                              c.wb.cap, c.aux, supepoch, lvl, c.ld.p, binval, cap, nxt, bnext, cap, c.stamp,
+                              sepoch, bscan, box + 21, c.d_paths, c.g.sorted, src, ab));
+      SCOUT_CHECK(prof_launch(c, PROF_K_INVAL_PUSH, k_inval_push<true>, dim3(ig), dim3(kBlock),
+                              c.g.view(), L, P, lvl, c.lf.p, c.lg.p, c.wb.p, tasks_of[p], c.wb.cap, c.aux,
+                              supepoch, c.ld.p, binval, cap, nxt, bnext, cap, c.stamp, sepoch, box + 15,
+                              c.d_paths, ab, book));
+    } else {
+      SCOUT_CHECK(prof_launch(c, PROF_K_INVALIDATE_SHORT, k_invalidate_short<false>, dim3(ig),
+                              dim3(kBlock), c.g.view(), L, P, cur, bcur, cand, nlevel, cb, box + 18, lvl, c.ld.p,
+                              binval, nxt, bnext, cap, c.stamp, sepoch, c.lh.p, small_of[p], c.le.p, medium_of[p],
+                              c.lf.p, c.lg.p, longs_of[p], c.wb.p, tasks_of[p], c.wb.cap, box + 23, c.d_paths,
+                              ab));
+      SCOUT_CHECK(prof_launch(c, PROF_BFS_INVALIDATE_KERNEL, k_invalidate<false>, dim3(ig),
+                              dim3(kBlock), c.g.view(), L, P, c.lh.p, small_of[p], c.le.p, medium_of[p], c.lf.p, c.lg.p, c.wb.p,
+                              tasks_of[p], c.wb.cap, c.aux, supepoch, lvl, c.ld.p, binval, cap, nxt, bnext, cap,
+                              c.stamp, sepoch, bscan, box + 21, c.d_paths, c.g.sorted, src, ab));
+      SCOUT_CHECK(prof_launch(c, PROF_K_INVAL_PUSH, k_inval_push<false>, dim3(ig), dim3(kBlock),
+                              c.g.view(), L, P, lvl, c.lf.p, c.lg.p, c.wb.p, tasks_of[p], c.wb.cap, c.aux,
+                              supepoch, c.ld.p, binval, cap, nxt, bnext, cap, c.stamp, sepoch, box + 15,
+                              c.d_paths, ab, book));
+    }
+    lvl++;
+    std::swap(cur, nxt);
+    p ^= 1;
+    return SCOUT_OK;
+  };
+  // The asynchronous order takes nothing from the host in its first rounds: they are queued
+  // behind the read-back of the candidate counts and predictions, and run while the host
+  // reads those.
+  bool early = async && m > 0;
+  int queued = 0;  // rounds of the first group already queued
+  if (early) {
+    SCOUT_CHECK(box_read_async(c, learned ? 28 : 8));
+    SCOUT_CHECK(prof_begin(c, PROF_BFS_INVALIDATE));
+    inval_begin();
+    for (; queued < kFirstGroup; queued++) SCOUT_CHECK(inval_round());
+    SCOUT_CHECK(box_wait(c));
+  } else {
+    SCOUT_CHECK(box_read(c, learned ? 28 : 8));
+  }
+  ncand = c.h_box[0];
+  cmin = (int32_t)c.h_box[1];
+  cmax = (int32_t)c.h_box[2];
   st->deletions += c.h_box[3];
   int64_t nx = 0, scanned = 0, rounds = 0;
   int64_t nsample = std::min<int64_t>(ncand, sample);
@@ -1575,96 +1673,25 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, int32_t* P, int strategy, scou
   int hint_inval = c.hint_inval, hint_relax = c.hint_relax;
   c.hint_inval = c.hint_relax = 0;
   if (L == c.level && hint_inval > 0) first_group = hint_inval;
+  if (ncand == 0 && queued > 0) SCOUT_CHECK(prof_end(c, PROF_BFS_INVALIDATE, 0));
   if (ncand > 0) {
-    // Invalidation, level by level, queued in groups of rounds. Counts: [8]/[9] candidates at
-    // this level / the next (alternating), [10] invalid vertices, [11] scanned entries,
-    // [12..14] / [28..30] medium rows, long rows and chunk tasks (alternating), [18]
-    // candidates over all rounds, [19] largest task count, [20] non-empty rounds.
-    // Asynchronous (c.bfs_async): the first round holds every candidate, marked pending, and
-    // the rounds run until one queues nothing.
-    bool async = c.bfs_async;
-    int32_t sepoch = next_epoch(c);
-    int32_t* cur = c.lb.p;
-    int32_t* nxt = c.lc.p;
-    // candidates grouped by level once (each round then reads only its own level's)
-    const int32_t* cand = c.la.p;
-    const int64_t* cbuckets = nullptr;
-    int64_t cspan = (int64_t)cmax - cmin + 1;
-    if (!async && cspan >= 1 && cspan <= kMaxBuckets) {
-      bucket_by_level(c.s, VertexKey{c.la.p, L}, ncand, cmin, (int)cspan, c.lvlb.p, c.lvlb.p + kMaxBuckets + 1,
-                      c.lg.p);
-      cand = c.lg.p;
-      cbuckets = c.lvlb.p;
-    }
-    int64_t* cnt[2] = {box + 8, box + 9};
-    int64_t* binval = box + 10;
-    int64_t* bscan = box + 11;
-    // the counter sets of alternate rounds: medium rows, long rows and chunk tasks at
-    // [12..14] and [28..30]
-    int64_t* medium_of[2] = {box + 12, box + 28};
-    int64_t* longs_of[2] = {box + 13, box + 29};
-    int64_t* tasks_of[2] = {box + 14, box + 30};
-    int64_t* small_of[2] = {box + 31, box + 32};  // rows walked by lane groups
-    // [15] entries read by k_inval_push, [23] entries of short rows read by k_invalidate_short
-    k_zero2<<<1, 64, 0, c.s>>>(box + 8, 16, box + 28, 5);
-    if (async)
-      k_mark_pending<<<grid_for(ncand, kBlock * kLinesPerThread), kBlock, 0, c.s>>>(c.la.p, box, L, cur, box + 8,
-                                                                                    cap);
-    int p = 0;
-    int group = first_group;
-    int32_t lvl = cmin;
-    int full_rounds = kFullRounds;
-    while (true) {
-      SCOUT_CHECK(prof_begin(c, PROF_BFS_INVALIDATE));
-      for (int k = 0; k < group; k++) {
-        unsigned ig = round_grid(async, full_rounds);
-        int64_t* bcur = cnt[p];
-        int64_t* bnext = cnt[p ^ 1];
-        // candidates of this level are taken by the round's first kernel, which also lists the
-        // long rows and their chunk tasks; k_invalidate searches the medium rows and the
-        // tasks; k_inval_push settles the long rows no task supported
-        int64_t nlevel = !async && lvl <= cmax ? ncand : 0;
-        RoundBook book{bcur,           box + 18,          box + 20,        box + 19,
-                       tasks_of[p ^ 1], medium_of[p ^ 1], longs_of[p ^ 1], nlevel > 0};
-        book.other_c = small_of[p ^ 1];
-        const int64_t* cb = (cbuckets && nlevel > 0) ? cbuckets + (lvl - cmin) : nullptr;
-        // two fresh consecutive epochs: supepoch marks support, supepoch - 1 a pending neighbour
-        int32_t supepoch = next_epoch(c);
-        if (async) {
-          supepoch = next_epoch(c);
-          if (supepoch == 1) supepoch = next_epoch(c);  // the counter started over
-          SCOUT_CHECK(prof_launch(c, PROF_K_INVALIDATE_SHORT, k_invalidate_short<true>, dim3(ig),
-                                  dim3(kBlock), c.g.view(), L, P, cur, bcur, cand, nlevel, cb, box + 18, lvl, c.ld.p,
-                                  binval, nxt, bnext, cap, c.stamp, sepoch, c.lh.p, small_of[p], c.le.p, medium_of[p],
-                                  c.lf.p, c.lg.p, longs_of[p], c.wb.p, tasks_of[p], c.wb.cap, box + 23, c.d_paths,
-                                  ab));
-          SCOUT_CHECK(prof_launch(c, PROF_BFS_INVALIDATE_KERNEL, k_invalidate<true>, dim3(ig), dim3(kBlock),
-                                  c.g.view(), L, P, c.lh.p, small_of[p], c.le.p, medium_of[p], c.lf.p, c.lg.p, c.wb.p, tasks_of[p],
-                                  c.wb.cap, c.aux, supepoch, lvl, c.ld.p, binval, cap, nxt, bnext, cap, c.stamp,
-                                  sepoch, bscan, box + 21, c.d_paths, c.g.sorted, src, ab));
-          SCOUT_CHECK(prof_launch(c, PROF_K_INVAL_PUSH, k_inval_push<true>, dim3(ig), dim3(kBlock),
-                                  c.g.view(), L, P, lvl, c.lf.p, c.lg.p, c.wb.p, tasks_of[p], c.wb.cap, c.aux,
-                                  supepoch, c.ld.p, binval, cap, nxt, bnext, cap, c.stamp, sepoch, box + 15,
-                                  c.d_paths, ab, book));
-        } else {
-          SCOUT_CHECK(prof_launch(c, PROF_K_INVALIDATE_SHORT, k_invalidate_short<false>, dim3(ig),
-                                  dim3(kBlock), c.g.view(), L, P, cur, bcur, cand, nlevel, cb, box + 18, lvl, c.ld.p,
-                                  binval, nxt, bnext, cap, c.stamp, sepoch, c.lh.p, small_of[p], c.le.p, medium_of[p],
-                                  c.lf.p, c.lg.p, longs_of[p], c.wb.p, tasks_of[p], c.wb.cap, box + 23, c.d_paths,
-                                  ab));
-          SCOUT_CHECK(prof_launch(c, PROF_BFS_INVALIDATE_KERNEL, k_invalidate<false>, dim3(ig),
-                                  dim3(kBlock), c.g.view(), L, P, c.lh.p, small_of[p], c.le.p, medium_of[p], c.lf.p, c.lg.p, c.wb.p,
-                                  tasks_of[p], c.wb.cap, c.aux, supepoch, lvl, c.ld.p, binval, cap, nxt, bnext, cap,
-                                  c.stamp, sepoch, bscan, box + 21, c.d_paths, c.g.sorted, src, ab));
-          SCOUT_CHECK(prof_launch(c, PROF_K_INVAL_PUSH, k_inval_push<false>, dim3(ig), dim3(kBlock),
-                                  c.g.view(), L, P, lvl, c.lf.p, c.lg.p, c.wb.p, tasks_of[p], c.wb.cap, c.aux,
-                                  supepoch, c.ld.p, binval, cap, nxt, bnext, cap, c.stamp, sepoch, box + 15,
-                                  c.d_paths, ab, book));
-        }
-        lvl++;
-        std::swap(cur, nxt);
-        p ^= 1;
+    if (!early) {
+      lvl = cmin;
+      int64_t cspan = (int64_t)cmax - cmin + 1;
+      if (!async && cspan >= 1 && cspan <= kMaxBuckets) {
+        // each round then reads only its own level's candidates
+        bucket_by_level(c.s, VertexKey{c.la.p, L}, ncand, cmin, (int)cspan, c.lvlb.p, c.lvlb.p + kMaxBuckets + 1,
+                        c.lg.p);
+        cand = c.lg.p;
+        cbuckets = c.lvlb.p;
       }
+      inval_begin();
+    }
+    int group = first_group;
+    while (true) {
+      if (queued == 0) SCOUT_CHECK(prof_begin(c, PROF_BFS_INVALIDATE));
+      for (; queued < group; queued++) SCOUT_CHECK(inval_round());
+      queued = 0;
       SCOUT_CHECK(prof_end(c, PROF_BFS_INVALIDATE, 0));
       // [8..30]: the prediction slots [24..27] come back unchanged
       SCOUT_TRY(hipMemcpyAsync(c.h_box + 8, box + 8, 23 * sizeof(int64_t), hipMemcpyDeviceToHost, c.s));

@@ -23,6 +23,19 @@ int box_read(Ctx& c, int count) {
   return SCOUT_OK;
 }
 
+int box_read_async(Ctx& c, int count) {
+  if (!c.box_ev) SCOUT_TRY(hipEventCreateWithFlags(&c.box_ev, hipEventDisableTiming));
+  SCOUT_TRY(hipMemcpyAsync(c.h_box, c.d_box, (size_t)count * sizeof(int64_t), hipMemcpyDeviceToHost, c.s));
+  SCOUT_TRY(hipEventRecord(c.box_ev, c.s));
+  return SCOUT_OK;
+}
+
+int box_wait(Ctx& c) {
+  SCOUT_TRY(hipEventSynchronize(c.box_ev));
+  c.readbacks++;
+  return SCOUT_OK;
+}
+
 int next_epoch(Ctx& c) {
   if (c.epoch >= 0x7ffffff0) {
     // every array compared against epochs starts over (epochs are > 0)
@@ -305,6 +318,7 @@ int scout_ctx_destroy(scout_ctx* x) {
   c.lvlb.release();
   c.cc_bits.release();
   c.tmp.release();
+  if (c.box_ev) (void)hipEventDestroy(c.box_ev);
   if (c.h_box) (void)hipHostFree(c.h_box);
   if (c.d_box) (void)hipFree(c.d_box);
   if (c.s) (void)hipStreamDestroy(c.s);
@@ -556,7 +570,7 @@ int scout_classify(scout_ctx* x, int channel, int rule, uint8_t* ins) {
     set_error("classify: unknown rule");
     return SCOUT_ERR_ARG;
   }
-  SCOUT_TRY(hipStreamWaitEvent(c.s, ch->staged, 0));
+  SCOUT_CHECK(channel_wait(c.s, *ch));
   SCOUT_CHECK(graph_classify(c, *ch, rule));
   SCOUT_CHECK(channel_used(c.s, *ch));
   if (ins && ch->m > 0) {
@@ -571,7 +585,7 @@ int scout_apply(scout_ctx* x, int channel, int conditional) {
   Ctx& c = x->c;
   Channel* ch = channel_of(c, channel);
   if (!ch) return SCOUT_ERR_ARG;
-  SCOUT_TRY(hipStreamWaitEvent(c.s, ch->staged, 0));
+  SCOUT_CHECK(channel_wait(c.s, *ch));
   SCOUT_CHECK(graph_apply(c, *ch, conditional));
   return channel_used(c.s, *ch);
 }
@@ -595,7 +609,7 @@ int scout_apply_segments(scout_ctx* x, int channel, int32_t nseg, const int64_t*
       set_error("apply_segments: segment offsets must not decrease");
       return SCOUT_ERR_ARG;
     }
-  SCOUT_TRY(hipStreamWaitEvent(c.s, ch->staged, 0));
+  SCOUT_CHECK(channel_wait(c.s, *ch));
   c.apply_seq++;  // one commit: the repairs that follow see one batch of all segments
   // When no pair occurs twice in the whole range, no segment touches a pair of an earlier one:
   // each line's pair stands in the graph as it stood before the segments, so one

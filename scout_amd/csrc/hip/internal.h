@@ -181,6 +181,8 @@ struct Channel {
   int64_t rep_hi = -1;
   int64_t m = 0;
   hipEvent_t staged = nullptr;
+  // the last staging ran on the compute stream, which orders the work after it by itself
+  bool staged_on_compute = false;
   // recorded on the compute stream after the last work that reads this channel's lines: the
   // next upload into the channel waits for it (write after read)
   hipEvent_t consumed = nullptr;
@@ -195,8 +197,15 @@ inline int channel_used(hipStream_t s, Channel& ch) {
   return hipEventRecord(ch.consumed, s) == hipSuccess ? SCOUT_OK : SCOUT_ERR_HIP;
 }
 
+// Orders stream s (the compute stream) after the channel's last staging.
+inline int channel_wait(hipStream_t s, Channel& ch) {
+  if (ch.staged_on_compute) return SCOUT_OK;
+  return hipStreamWaitEvent(s, ch.staged, 0) == hipSuccess ? SCOUT_OK : SCOUT_ERR_HIP;
+}
+
 // Sets the staged line count; base ranges found by an earlier classify no longer apply.
-inline void channel_staged(Channel& ch, int64_t m) {
+inline void channel_staged(Channel& ch, int64_t m, bool on_compute = false) {
+  ch.staged_on_compute = on_compute;
   ch.m = m;
   ch.pos_lo = ch.pos_hi = 0;
   ch.pairs_distinct = false;
@@ -249,6 +258,7 @@ struct Ctx {
   int64_t mis_version = 0;       // bumped when `mis` is overwritten outside the MIS repair
   // small mailbox read back by the host after a round (kBoxSlots entries)
   int64_t* h_box = nullptr;    // pinned
+  hipEvent_t box_ev = nullptr;  // marks the last box_read_async copy
   int64_t* d_box = nullptr;
   // cumulative counts of the long-row code paths (kPathCount entries after the mailbox)
   int64_t* d_paths = nullptr;
@@ -396,6 +406,10 @@ enum ProfId { PROF_BFS_RELAX = 0, PROF_BFS_INVALIDATE, PROF_CC_EXPAND, PROF_APPL
 
 // helpers in ctx.hip
 int box_read(Ctx& c, int count);  // copies d_box[0..count) into h_box, synchronises c.s
+// box_read in two halves: the copy is queued behind the work queued so far, and box_wait waits
+// for that copy alone, so work queued after it runs on while the host reads the values
+int box_read_async(Ctx& c, int count);
+int box_wait(Ctx& c);
 int next_epoch(Ctx& c);
 int ensure_state(Ctx& c, int kind);
 // n-sized int32 scratch filled with `fill` on first allocation

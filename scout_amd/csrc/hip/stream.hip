@@ -319,7 +319,7 @@ int gen_batch(Ctx& c, Channel& ch, int64_t size, double delete_frac, double tree
   SCOUT_CHECK(ch.d_uv.reserve(2 * size + 2));
   SCOUT_CHECK(ch.d_ins.reserve(size + 1));
   SCOUT_CHECK(ch.d_done.reserve(size + 1));
-  SCOUT_TRY(hipStreamWaitEvent(c.s, ch.staged, 0));
+  SCOUT_CHECK(channel_wait(c.s, ch));
   uint64_t keys[3] = {stream_key(seed, RNG_STREAM_TREE), stream_key(seed, RNG_STREAM_DEL), stream_key(seed, RNG_STREAM_INS)};
   int64_t want[3] = {ntree, nuni, nins};
   int64_t produced = 0;
@@ -398,7 +398,7 @@ int gen_batch(Ctx& c, Channel& ch, int64_t size, double delete_frac, double tree
   }
   s.batches++;
   SCOUT_TRY(hipEventRecord(ch.staged, c.s));
-  channel_staged(ch, produced);
+  channel_staged(ch, produced, true);
   if (out_m) *out_m = produced;
   return SCOUT_OK;
 }
@@ -416,7 +416,7 @@ int batch_store(Ctx& c, Channel& ch, int slot) {
     s.slot_m.resize((size_t)slot + 1, 0);
   }
   SCOUT_CHECK(s.slots[(size_t)slot].reserve(2 * ch.m + 2));
-  SCOUT_TRY(hipStreamWaitEvent(c.s, ch.staged, 0));
+  SCOUT_CHECK(channel_wait(c.s, ch));
   if (ch.m > 0)
     SCOUT_TRY(hipMemcpyAsync(s.slots[(size_t)slot].p, ch.d_uv.p, (size_t)(2 * ch.m) * 4, hipMemcpyDeviceToDevice, c.s));
   s.slot_m[(size_t)slot] = ch.m;
@@ -431,18 +431,17 @@ int batch_load(Ctx& c, Channel& ch, int slot) {
     return SCOUT_ERR_ARG;
   }
   int64_t m = c.stream->slot_m[(size_t)slot];
-  SCOUT_TRY(hipEventSynchronize(ch.staged));
+  SCOUT_CHECK(channel_wait(c.s, ch));  // an upload into the channel may still be in flight
   SCOUT_CHECK(ch.d_uv.reserve(2 * m + 2));
   SCOUT_CHECK(ch.d_ins.reserve(m + 1));
   SCOUT_CHECK(ch.d_done.reserve(m + 1));
-  // the compute stream may still read this channel's previous batch
-  SCOUT_TRY(hipEventRecord(ch.staged, c.s));
-  SCOUT_TRY(hipStreamWaitEvent(c.cs, ch.staged, 0));
+  // a device-to-device copy on the compute stream: ordered after every earlier reader of the
+  // channel's lines and before the work that follows, with no cross-stream wait
   if (m > 0)
     SCOUT_TRY(hipMemcpyAsync(ch.d_uv.p, c.stream->slots[(size_t)slot].p, (size_t)(2 * m) * 4, hipMemcpyDeviceToDevice,
-                             c.cs));
-  SCOUT_TRY(hipEventRecord(ch.staged, c.cs));
-  channel_staged(ch, m);
+                             c.s));
+  SCOUT_TRY(hipEventRecord(ch.staged, c.s));
+  channel_staged(ch, m, true);
   return SCOUT_OK;
 }
 
@@ -666,7 +665,7 @@ int scout_batch_download(scout_ctx* x, int channel, int32_t* uv, int64_t cap_lin
   Channel* ch = channel_of(c, channel);
   if (!ch) return SCOUT_ERR_ARG;
   int64_t m = std::min(ch->m, cap_lines);
-  SCOUT_TRY(hipStreamWaitEvent(c.s, ch->staged, 0));
+  SCOUT_CHECK(channel_wait(c.s, *ch));
   if (m > 0) SCOUT_TRY(hipMemcpyAsync(uv, ch->d_uv.p, (size_t)(2 * m) * 4, hipMemcpyDeviceToHost, c.s));
   SCOUT_TRY(hipStreamSynchronize(c.s));
   return SCOUT_OK;
