@@ -351,7 +351,8 @@ __global__ void k_relax_short(GView g, int32_t* level, int32_t* parent, const in
                               int32_t seed_epoch, int32_t* small, int64_t* small_count, int32_t* medium,
                               int64_t* medium_count, int64_t* tasks,
                               int64_t* task_count, int64_t task_cap, int64_t* scanned, const int64_t* ns_dev,
-                              int64_t seed_cap) {
+                              int64_t seed_cap, const int64_t* gate) {
+  if (gate && *gate != 0) return;
   int64_t nf = *count;
   if (ns_dev) ns = min(*ns_dev, seed_cap);
   if (!A && sb && ns > 0) {
@@ -436,7 +437,8 @@ __global__ void k_relax_rows(GView g, int32_t* level, int32_t* parent, const int
                              const int64_t* small_count, const int32_t* medium, const int64_t* medium_count,
                              const int64_t* tasks, const int64_t* task_count, int64_t task_cap, int32_t lvl,
                              int32_t* next, int64_t* next_count, int64_t cap, int32_t* stamp, int32_t epoch,
-                             int64_t* scanned, int64_t* paths, RoundBook book) {
+                             int64_t* scanned, int64_t* paths, RoundBook book, const int64_t* gate) {
+  if (gate && *gate != 0) return;
   __shared__ int32_t seg_all[kWavesPerBlock * kNextSeg];
   WaveStage<kNextSeg> st{seg_all + (threadIdx.x / kWave) * kNextSeg, 0};
   int64_t nm = *medium_count;
@@ -1098,9 +1100,12 @@ __global__ void k_inval_push(GView g, int32_t* level, const int32_t* parent, int
 // Smallest (level << 32 | neighbour) over the reached neighbours of each invalidated vertex
 // (UINT64_MAX if none). Thread per vertex for rows of at most kShort entries; longer rows go
 // to the wave list (<= kTask) or to chunk tasks.
-__global__ void k_tentative_short(GView g, const int32_t* level, const int32_t* xs, int64_t nx, uint64_t* tent,
-                                  int32_t* medium, int64_t* medium_count, int64_t medium_cap, int64_t* tasks,
-                                  int64_t* task_count, int64_t task_cap, int64_t* scanned) {
+__global__ void k_tentative_short(GView g, const int32_t* level, const int32_t* xs, const int64_t* nx_dev,
+                                  uint64_t* tent, int32_t* medium, int64_t* medium_count, int64_t medium_cap,
+                                  int64_t* tasks, int64_t* task_count, int64_t task_cap, int64_t* scanned,
+                                  const int64_t* gate) {
+  if (gate && *gate != 0) return;
+  int64_t nx = *nx_dev;
   int64_t work = 0;
   for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < nx; base += (int64_t)gridDim.x * blockDim.x) {
     int64_t i = base + threadIdx.x;
@@ -1146,7 +1151,9 @@ __device__ __forceinline__ uint64_t wave_min_u64(uint64_t v) {
 // Wave per medium row (index list) or per chunk of a long row; atomicMin into tent.
 __global__ void k_tentative_rows(GView g, const int32_t* level, const int32_t* xs, const int32_t* medium,
                                  const int64_t* medium_count, const int64_t* tasks, const int64_t* task_count,
-                                 int64_t task_cap, uint64_t* tent, int64_t* scanned, int64_t* paths) {
+                                 int64_t task_cap, uint64_t* tent, int64_t* scanned, int64_t* paths,
+                                 const int64_t* gate) {
+  if (gate && *gate != 0) return;
   int64_t nm = *medium_count;
   int64_t nt = min(*task_count, task_cap);
   int64_t work = 0, ntask = 0;
@@ -1184,38 +1191,44 @@ __global__ void k_tentative_rows(GView g, const int32_t* level, const int32_t* x
   block_add(lane_id() == 0 ? work : 0, scanned);
 }
 
-__global__ void k_apply_tentative(const int32_t* xs, const uint64_t* tent, int64_t nx, int32_t* level,
-                                  int32_t* parent, int64_t* seeds, int32_t* seed_par, int64_t cap, int64_t* box) {
+// Grid-stride over the nx_dev invalidated vertices, kLinesPerThread per thread and pass (the
+// trip count is uniform across the block, so the block-wide appends see every thread).
+__global__ void k_apply_tentative(const int32_t* xs, const uint64_t* tent, const int64_t* nx_dev, int32_t* level,
+                                  int32_t* parent, int64_t* seeds, int32_t* seed_par, int64_t cap, int64_t* box,
+                                  const int64_t* gate) {
+  if (gate && *gate != 0) return;
   constexpr int K = kLinesPerThread;
-  int64_t first = (int64_t)blockIdx.x * blockDim.x * K + threadIdx.x;
-  bool take[K];
-  int64_t packed[K];
-  int32_t par[K];
+  int64_t nx = *nx_dev;
   int64_t lmin = INT64_MAX, lmax = -1;
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x * K; base < nx; base += (int64_t)gridDim.x * blockDim.x * K) {
+    bool take[K];
+    int64_t packed[K];
+    int32_t par[K];
 #pragma unroll
-  for (int q = 0; q < K; q++) {
-    int64_t i = first + (int64_t)q * blockDim.x;
-    int32_t t = -1, x = 0, p = -1;
-    if (i < nx) {
-      uint64_t best = tent[i];
-      if (best != UINT64_MAX) {
-        t = (int32_t)(best >> 32) + 1;
-        p = (int32_t)(best & 0xffffffffu);
+    for (int q = 0; q < K; q++) {
+      int64_t i = base + threadIdx.x + (int64_t)q * blockDim.x;
+      int32_t t = -1, x = 0, p = -1;
+      if (i < nx) {
+        uint64_t best = tent[i];
+        if (best != UINT64_MAX) {
+          t = (int32_t)(best >> 32) + 1;
+          p = (int32_t)(best & 0xffffffffu);
+        }
+        x = xs[i];
+        level[x] = t;
+        parent[x] = p;
       }
-      x = xs[i];
-      level[x] = t;
-      parent[x] = p;
+      take[q] = t >= 0;
+      packed[q] = ((int64_t)t << 32) | (int64_t)(uint32_t)x;
+      par[q] = p;
+      if (t >= 0) lmin = min(lmin, (int64_t)t), lmax = max(lmax, (int64_t)t);
     }
-    take[q] = t >= 0;
-    packed[q] = ((int64_t)t << 32) | (int64_t)(uint32_t)x;
-    par[q] = p;
-    if (t >= 0) lmin = min(lmin, (int64_t)t), lmax = max(lmax, (int64_t)t);
-  }
-  int64_t at[K];
-  block_append_k(take, packed, seeds, &box[4], cap, at);
+    int64_t at[K];
+    block_append_k(take, packed, seeds, &box[4], cap, at);
 #pragma unroll
-  for (int q = 0; q < K; q++)
-    if (at[q] >= 0) seed_par[at[q]] = par[q];
+    for (int q = 0; q < K; q++)
+      if (at[q] >= 0) seed_par[at[q]] = par[q];
+  }
   block_min_max(lmin, lmax, &box[5], &box[6]);
 }
 
@@ -1224,7 +1237,9 @@ __global__ void k_apply_tentative(const int32_t* xs, const uint64_t* tent, int64
 // by a conditional apply because it already was), so the row search is needed only when a
 // pair repeats.
 __global__ void k_insert_seeds(GView g, bool sorted, bool distinct, const int32_t* uv, const uint8_t* ins, int64_t m,
-                               int32_t* level, int64_t* seeds, int32_t* seed_par, int64_t cap, int64_t* box) {
+                               int32_t* level, int64_t* seeds, int32_t* seed_par, int64_t cap, int64_t* box,
+                               const int64_t* gate) {
+  if (gate && *gate != 0) return;
   // kLinesPerThread lines per thread (each pass coalesced over the block): one append of the
   // seeds and one reduction of their levels per block. Seeds are packed level << 32 | vertex,
   // with the vertex that lowered them in seed_par; box[4] = seeds, box[5] / box[6] = their
@@ -1277,7 +1292,8 @@ __global__ void k_insert_seeds(GView g, bool sorted, bool distinct, const int32_
 // its parent (several seeds of one vertex at one level are all valid parents).
 // ns_dev (optional): the count on the device, at most cap (grid-stride over it).
 __global__ void k_seed_parents(const int64_t* seeds, const int32_t* seed_par, int64_t ns, const int64_t* ns_dev,
-                               int64_t cap, const int32_t* level, int32_t* parent) {
+                               int64_t cap, const int32_t* level, int32_t* parent, const int64_t* gate) {
+  if (gate && *gate != 0) return;
   if (ns_dev) ns = min(*ns_dev, cap);
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < ns; i += (int64_t)gridDim.x * blockDim.x) {
     int64_t s = seeds[i];
@@ -1293,6 +1309,23 @@ __global__ void k_seed_parents(const int64_t* seeds, const int32_t* seed_par, in
 // Zeroes `count` consecutive counters (round bookkeeping without a copy or fill kernel).
 __global__ void k_zero(int64_t* p, int count) {
   if ((int)threadIdx.x < count) p[threadIdx.x] = 0;
+}
+
+// Box slots of the work queued ahead of the read-back that ends an invalidation group.
+constexpr int kGateSlot = 39;  // the gate: zero once the invalidation has ended
+constexpr int kNxSlot = 33;    // the invalidated vertices' count, kept for the tentative levels
+
+// Opens (gate = 0) or closes that work: the gate takes the next round's list count. When the
+// invalidation has ended, the invalidated count moves to kNxSlot and the tentative-level and
+// relax counters ([8..21], [36..38]) are zeroed.
+__global__ void k_spec_gate(const int64_t* front, int64_t* box) {
+  if (threadIdx.x != 0) return;
+  int64_t f = *front;
+  box[kGateSlot] = f;
+  if (f != 0) return;
+  box[kNxSlot] = box[10];
+  for (int k = 8; k <= 21; k++) box[k] = 0;
+  for (int k = 36; k <= 38; k++) box[k] = 0;
 }
 
 // Zeroes two runs of counters.
@@ -1346,6 +1379,8 @@ __global__ void k_find_parents(GView g, const int32_t* level, int32_t* parent, i
 // next group's first rounds. The level-synchronous order, whose later levels can be the widest,
 // always takes the full grid.
 constexpr int kFullRounds = 2;
+// Grid of k_apply_tentative, whose count is on the device: one pass over 512k vertices.
+constexpr unsigned kTentGrid = 512;
 constexpr unsigned kTailGrid = 256;
 constexpr int64_t kTailFront = (int64_t)kTailGrid * kBlock;
 
@@ -1359,46 +1394,45 @@ unsigned round_grid(bool async, int& full_rounds) {
   return kTailGrid;
 }
 
-// Relax with the current list already in `cur` (count in box slot 8) and optional seeds.
-// Level-synchronous (async = false): starts at level `lvl0` and runs until no frontier and no
-// seed above the current level remains. Asynchronous: the first round takes every seed and
-// later rounds their lowered neighbours, whatever their levels, until a round lowers nothing.
-// Adds the non-empty rounds to *rounds.
-int relax_levels(Ctx& c, int32_t* L, int32_t* P, int32_t lvl0, int32_t max_seed, const int64_t* seeds, int64_t nseeds,
-                 int32_t epoch, int64_t* rounds, int64_t* scanned_total, int first_group = kFirstGroup,
-                 bool async = false, const int64_t* nseeds_dev = nullptr, int64_t seed_cap = 0) {
-  int32_t n = c.g.n;
-  int64_t cap = (int64_t)n + 1;
-  int64_t task_cap = c.wb.cap;
-  int32_t* cur = c.lb.p;
-  int32_t* nxt = c.lc.p;
-  int64_t* box = c.d_box;
-  // box slots: [8]/[9] frontier counts (current and next alternate), [10]/[11] and [18]/[19]
-  // chunk tasks and medium rows, [20]/[21] small rows (the counter sets of alternate rounds),
-  // [12] scanned entries, [13] entries of short rows, [15] frontier vertices over all rounds,
-  // [16] largest task count, [17] non-empty rounds
-  int64_t* cnt[2] = {box + 8, box + 9};
-  int64_t* tasks_of[2] = {box + 10, box + 18};
-  int64_t* medium_of[2] = {box + 11, box + 19};
-  int64_t* small_of[2] = {box + 20, box + 21};  // rows walked by lane groups
-  k_zero2<<<1, 64, 0, c.s>>>(box + 9, 5, box + 15, 7);
-  // seeds grouped by level once (each round then reads only its own), when the levels they
-  // span fit the bucket counters and the list fits the scratch
+// One relax: rounds queued in groups, each group followed by one read-back of the counters.
+// Box slots: [8]/[9] frontier counts (current and next alternate), [10]/[11] and [18]/[19]
+// chunk tasks and medium rows, [20]/[21] small rows (the counter sets of alternate rounds),
+// [12] scanned entries, [13] entries of short rows, [15] frontier vertices over all rounds,
+// [16] largest task count, [17] non-empty rounds.
+// start() zeroes the counters (unless the caller has) and queues the first group; finish()
+// reads back and queues further groups until a round lowers nothing. Under a gate, start()
+// runs ahead of the read-back that decides whether the relax may start: every kernel it queues
+// returns at once while *gate is non-zero.
+struct Relax {
+  Ctx& c;
+  int32_t* L;
+  int32_t* P;
+  int32_t lvl0, max_seed;
+  const int64_t* seeds;
+  int64_t nseeds;
+  int32_t epoch;
+  bool async;
+  const int64_t* nseeds_dev;
+  int64_t seed_cap;
+  // progress
+  int32_t* cur = nullptr;
+  int32_t* nxt = nullptr;
   const int64_t* sbuckets = nullptr;
-  int64_t span = (int64_t)max_seed - lvl0 + 1;
-  if (!async && nseeds > 0 && span >= 1 && span <= kMaxBuckets && c.wc.cap >= nseeds) {
-    bucket_by_level(c.s, SeedKey{seeds}, nseeds, lvl0, (int)span, c.lvlb.p, c.lvlb.p + kMaxBuckets + 1,
-                    reinterpret_cast<int64_t*>(c.wc.p));
-    seeds = reinterpret_cast<const int64_t*>(c.wc.p);
-    sbuckets = c.lvlb.p;
-  }
-  int32_t lvl = lvl0;
+  int32_t lvl = 0;
   int p = 0;
-  int group = first_group;
+  int group = kFirstGroup, first_group = kFirstGroup;
   int64_t done = 0;  // rounds queued so far
   int full_rounds = kFullRounds;
-  while (true) {
-    SCOUT_CHECK(prof_begin(c, PROF_BFS_RELAX));
+
+  int queue_group(const int64_t* gate) {
+    int32_t n = c.g.n;
+    int64_t cap = (int64_t)n + 1;
+    int64_t task_cap = c.wb.cap;
+    int64_t* box = c.d_box;
+    int64_t* cnt[2] = {box + 8, box + 9};
+    int64_t* tasks_of[2] = {box + 10, box + 18};
+    int64_t* medium_of[2] = {box + 11, box + 19};
+    int64_t* small_of[2] = {box + 20, box + 21};
     for (int k = 0; k < group; k++, done++) {
       int64_t* bcur = cnt[p];
       int64_t* bnext = cnt[p ^ 1];
@@ -1412,53 +1446,97 @@ int relax_levels(Ctx& c, int32_t* L, int32_t* P, int32_t lvl0, int32_t max_seed,
       RoundBook book{bcur, box + 15, box + 17, box + 16, tasks_of[p ^ 1], medium_of[p ^ 1], small_of[p ^ 1],
                      ns > 0 || ns_dev != nullptr};
       if (async) {
-        SCOUT_CHECK(prof_launch(c, PROF_K_RELAX_SHORT, k_relax_short<true>, dim3(rg), dim3(kBlock),
-                                c.g.view(), L, P, cur, bcur, seeds, ns, sb, box + 15, lvl, nxt, bnext, cap, c.stamp,
-                                ep, epoch, c.lh.p, small_of[p], c.le.p, medium_of[p], c.wb.p, tasks_of[p], task_cap,
-                                box + 13, ns_dev,
-                                seed_cap));
-        SCOUT_CHECK(prof_launch(c, PROF_K_RELAX_ROWS, k_relax_rows<true>, dim3(rg), dim3(kBlock), c.g.view(),
-                                L, P, c.lh.p, small_of[p], c.le.p, medium_of[p], c.wb.p, tasks_of[p], task_cap, lvl, nxt, bnext, cap,
-                                c.stamp, ep, box + 12, c.d_paths, book));
+        SCOUT_CHECK(prof_launch(c, PROF_K_RELAX_SHORT, k_relax_short<true>, dim3(rg), dim3(kBlock), c.g.view(), L,
+                                P, cur, bcur, seeds, ns, sb, box + 15, lvl, nxt, bnext, cap, c.stamp, ep, epoch,
+                                c.lh.p, small_of[p], c.le.p, medium_of[p], c.wb.p, tasks_of[p], task_cap, box + 13,
+                                ns_dev, seed_cap, gate));
+        SCOUT_CHECK(prof_launch(c, PROF_K_RELAX_ROWS, k_relax_rows<true>, dim3(rg), dim3(kBlock), c.g.view(), L, P,
+                                c.lh.p, small_of[p], c.le.p, medium_of[p], c.wb.p, tasks_of[p], task_cap, lvl, nxt,
+                                bnext, cap, c.stamp, ep, box + 12, c.d_paths, book, gate));
       } else {
-        SCOUT_CHECK(prof_launch(c, PROF_K_RELAX_SHORT, k_relax_short<false>, dim3(rg), dim3(kBlock),
-                                c.g.view(), L, P, cur, bcur, seeds, ns, sb, box + 15, lvl, nxt, bnext, cap, c.stamp,
-                                ep, epoch, c.lh.p, small_of[p], c.le.p, medium_of[p], c.wb.p, tasks_of[p], task_cap,
-                                box + 13, nullptr,
-                                0));
-        SCOUT_CHECK(prof_launch(c, PROF_K_RELAX_ROWS, k_relax_rows<false>, dim3(rg), dim3(kBlock),
-                                c.g.view(), L, P, c.lh.p, small_of[p], c.le.p, medium_of[p], c.wb.p, tasks_of[p], task_cap, lvl, nxt,
-                                bnext, cap, c.stamp, ep, box + 12, c.d_paths, book));
+        SCOUT_CHECK(prof_launch(c, PROF_K_RELAX_SHORT, k_relax_short<false>, dim3(rg), dim3(kBlock), c.g.view(), L,
+                                P, cur, bcur, seeds, ns, sb, box + 15, lvl, nxt, bnext, cap, c.stamp, ep, epoch,
+                                c.lh.p, small_of[p], c.le.p, medium_of[p], c.wb.p, tasks_of[p], task_cap, box + 13,
+                                nullptr, 0, gate));
+        SCOUT_CHECK(prof_launch(c, PROF_K_RELAX_ROWS, k_relax_rows<false>, dim3(rg), dim3(kBlock), c.g.view(), L,
+                                P, c.lh.p, small_of[p], c.le.p, medium_of[p], c.wb.p, tasks_of[p], task_cap, lvl,
+                                nxt, bnext, cap, c.stamp, ep, box + 12, c.d_paths, book, gate));
       }
       lvl++;
       std::swap(cur, nxt);
       p ^= 1;
     }
-    SCOUT_CHECK(prof_end(c, PROF_BFS_RELAX, 0));
-    // [8..19] this relax's counters; [4..7] and [36..38] come back too (the seed and
-    // tentative-level counters of a local repair, which read them after the relax)
-    SCOUT_TRY(hipMemcpyAsync(c.h_box + 4, box + 4, 35 * sizeof(int64_t), hipMemcpyDeviceToHost, c.s));
-    SCOUT_TRY(hipStreamSynchronize(c.s));
-    c.readbacks++;
-    if (std::max({c.h_box[10], c.h_box[16], c.h_box[18]}) > task_cap) {
-      set_error("bfs relax: chunk task list overflow");
-      return SCOUT_ERR_STATE;
-    }
-    if (c.h_box[8 + p] == 0 && (async || nseeds == 0 || lvl > max_seed)) break;
-    full_rounds = c.h_box[8 + p] > kTailFront ? kFullRounds : 0;
-    // after a predicted first group the repair is near its end: start again from a short group
-    group = group == first_group && first_group != kFirstGroup ? kFirstGroup : std::min(2 * group, kMaxGroup);
-    first_group = kFirstGroup;
+    return SCOUT_OK;
   }
-  // algorithmic bytes: per frontier vertex its id, level, the four row offsets and its queue
-  // entry (40 B); per scanned entry the neighbour id and its level (8 B). k_relax_short reads
-  // every frontier vertex and the short rows ([13]), k_relax_rows the longer rows ([12]).
-  prof_bytes(c, PROF_BFS_RELAX, 40 * c.h_box[15] + 8 * (c.h_box[12] + c.h_box[13]));
-  prof_bytes(c, PROF_K_RELAX_SHORT, 40 * c.h_box[15] + 8 * c.h_box[13]);
-  prof_bytes(c, PROF_K_RELAX_ROWS, 8 * c.h_box[12]);
-  *rounds += c.h_box[17];
-  *scanned_total += c.h_box[12] + c.h_box[13];
-  return SCOUT_OK;
+
+  int start(int first, const int64_t* gate, bool zeroed) {
+    int64_t* box = c.d_box;
+    cur = c.lb.p;
+    nxt = c.lc.p;
+    lvl = lvl0;
+    p = 0;
+    done = 0;
+    full_rounds = kFullRounds;
+    group = first_group = first;
+    if (!zeroed) k_zero2<<<1, 64, 0, c.s>>>(box + 9, 5, box + 15, 7);
+    // seeds grouped by level once (each round then reads only its own), when the levels they
+    // span fit the bucket counters and the list fits the scratch
+    int64_t span = (int64_t)max_seed - lvl0 + 1;
+    if (!async && nseeds > 0 && span >= 1 && span <= kMaxBuckets && c.wc.cap >= nseeds) {
+      bucket_by_level(c.s, SeedKey{seeds}, nseeds, lvl0, (int)span, c.lvlb.p, c.lvlb.p + kMaxBuckets + 1,
+                      reinterpret_cast<int64_t*>(c.wc.p));
+      seeds = reinterpret_cast<const int64_t*>(c.wc.p);
+      sbuckets = c.lvlb.p;
+    }
+    SCOUT_CHECK(prof_begin(c, PROF_BFS_RELAX));
+    return queue_group(gate);
+  }
+
+  // Adds the non-empty rounds to *rounds.
+  int finish(int64_t* rounds, int64_t* scanned_total) {
+    int64_t* box = c.d_box;
+    while (true) {
+      SCOUT_CHECK(prof_end(c, PROF_BFS_RELAX, 0));
+      // [8..21] this relax's counters; [4..7] and [36..38] come back too (the seed and
+      // tentative-level counters of a local repair, which read them after the relax)
+      SCOUT_TRY(hipMemcpyAsync(c.h_box + 4, box + 4, 35 * sizeof(int64_t), hipMemcpyDeviceToHost, c.s));
+      SCOUT_TRY(hipStreamSynchronize(c.s));
+      c.readbacks++;
+      if (std::max({c.h_box[10], c.h_box[16], c.h_box[18]}) > c.wb.cap) {
+        set_error("bfs relax: chunk task list overflow");
+        return SCOUT_ERR_STATE;
+      }
+      if (c.h_box[8 + p] == 0 && (async || nseeds == 0 || lvl > max_seed)) break;
+      full_rounds = c.h_box[8 + p] > kTailFront ? kFullRounds : 0;
+      // after a predicted first group the repair is near its end: start again from a short group
+      group = group == first_group && first_group != kFirstGroup ? kFirstGroup : std::min(2 * group, kMaxGroup);
+      first_group = kFirstGroup;
+      SCOUT_CHECK(prof_begin(c, PROF_BFS_RELAX));
+      SCOUT_CHECK(queue_group(nullptr));
+    }
+    // algorithmic bytes: per frontier vertex its id, level, the four row offsets and its queue
+    // entry (40 B); per scanned entry the neighbour id and its level (8 B). k_relax_short reads
+    // every frontier vertex and the short rows ([13]), k_relax_rows the longer rows ([12]).
+    prof_bytes(c, PROF_BFS_RELAX, 40 * c.h_box[15] + 8 * (c.h_box[12] + c.h_box[13]));
+    prof_bytes(c, PROF_K_RELAX_SHORT, 40 * c.h_box[15] + 8 * c.h_box[13]);
+    prof_bytes(c, PROF_K_RELAX_ROWS, 8 * c.h_box[12]);
+    *rounds += c.h_box[17];
+    *scanned_total += c.h_box[12] + c.h_box[13];
+    return SCOUT_OK;
+  }
+};
+
+// Relax with the current list already in `cur` (count in box slot 8) and optional seeds.
+// Level-synchronous (async = false): starts at level `lvl0` and runs until no frontier and no
+// seed above the current level remains. Asynchronous: the first round takes every seed and
+// later rounds their lowered neighbours, whatever their levels, until a round lowers nothing.
+// Adds the non-empty rounds to *rounds.
+int relax_levels(Ctx& c, int32_t* L, int32_t* P, int32_t lvl0, int32_t max_seed, const int64_t* seeds, int64_t nseeds,
+                 int32_t epoch, int64_t* rounds, int64_t* scanned_total, int first_group = kFirstGroup,
+                 bool async = false, const int64_t* nseeds_dev = nullptr, int64_t seed_cap = 0) {
+  Relax r{c, L, P, lvl0, max_seed, seeds, nseeds, epoch, async, nseeds_dev, seed_cap};
+  SCOUT_CHECK(r.start(first_group, nullptr, false));
+  return r.finish(rounds, scanned_total);
 }
 
 int ensure_bfs(Ctx& c, int64_t extra) {
@@ -1624,7 +1702,7 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, int32_t* P, int strategy, scou
   bool early = async && m > 0;
   int queued = 0;  // rounds of the first group already queued
   if (early) {
-    SCOUT_CHECK(box_read_async(c, learned ? 28 : 8));
+    SCOUT_CHECK(box_read_async(c, 0, learned ? 28 : 8));
     SCOUT_CHECK(prof_begin(c, PROF_BFS_INVALIDATE));
     inval_begin();
     for (; queued < kFirstGroup; queued++) SCOUT_CHECK(inval_round());
@@ -1673,6 +1751,49 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, int32_t* P, int strategy, scou
   int hint_inval = c.hint_inval, hint_relax = c.hint_relax;
   c.hint_inval = c.hint_relax = 0;
   if (L == c.level && hint_inval > 0) first_group = hint_inval;
+  // The asynchronous relax re-attaches the invalidated cascades and runs out from the
+  // insertions; under the learned strategy its rounds follow the predicted depth, corrected by
+  // the running mean of their difference on earlier batches (rounded up, as for the
+  // invalidation).
+  int relax_group = kFirstGroup;
+  if (learned && predicted_depth >= 0)
+    relax_group = (int)std::min<int64_t>(
+        std::max<int64_t>(predicted_depth + (int64_t)std::ceil(c.relax_bias), kFirstGroup), kMaxGroup);
+  if (L == c.level && hint_relax > 0) relax_group = hint_relax;
+  Relax relax{c, L, P, 0, 0, c.wa.p, 0, 0, true, box + 4, seed_cap};
+  bool relax_queued = false;
+  // Queues what follows the invalidation in the asynchronous relax: the tentative levels of the
+  // invalidated vertices (their count from the device), the insertion seeds and the relax's
+  // first group. No read-back: the seeds' count stays on the device for the relax's first
+  // round, and the seed and tentative-level counters come back with the relax's read-back.
+  // gate: queued ahead of the read-back that ends an invalidation group (k_spec_gate has
+  // zeroed the counters and kept the invalidated count).
+  auto queue_relax = [&](const int64_t* gate, bool tentative) -> int {
+    if (tentative) {
+      // [36] scanned, [37] medium rows, [38] chunk tasks (slots the relax leaves alone)
+      if (!gate) k_zero<<<1, 64, 0, c.s>>>(box + 36, 3);
+      const int64_t* nx_dev = gate ? box + kNxSlot : box + 10;
+      uint64_t* tent = reinterpret_cast<uint64_t*>(c.wc.p);
+      k_tentative_short<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, c.ld.p, nx_dev, tent, c.lf.p, box + 37,
+                                                        c.lf.cap, c.wb.p, box + 38, c.wb.cap, box + 36, gate);
+      k_tentative_rows<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, c.ld.p, c.lf.p, box + 37, c.wb.p, box + 38,
+                                                       c.wb.cap, tent, box + 36, c.d_paths, gate);
+      k_apply_tentative<<<kTentGrid, kBlock, 0, c.s>>>(c.ld.p, tent, nx_dev, L, P, c.wa.p, c.lg.p, seed_cap, box,
+                                                        gate);
+    }
+    if (m > 0)
+      k_insert_seeds<<<grid_for(m, kBlock * kLinesPerThread), kBlock, 0, c.s>>>(
+          c.g.view(), c.g.sorted, ch.pairs_distinct, ch.d_uv.p, ch.d_ins.p, m, L, c.wa.p, c.lg.p, seed_cap, box, gate);
+    k_seed_parents<<<kListGrid, kBlock, 0, c.s>>>(c.wa.p, c.lg.p, 0, box + 4, seed_cap, L, P, gate);
+    relax.epoch = next_epoch(c);
+    if (!gate) k_zero<<<1, 64, 0, c.s>>>(box + 8, 1);
+    relax_queued = true;
+    return relax.start(relax_group, gate, gate != nullptr);
+  };
+  // A first invalidation group sized by a prediction (or a schedule handed in) is expected to
+  // end the invalidation: the relax is queued behind its read-back under the gate, so the
+  // device does not wait for the host between the two phases.
+  bool speculate = async && c.relax_async && ((learned && predicted_depth >= 0) || (L == c.level && hint_inval > 0));
   if (ncand == 0 && queued > 0) SCOUT_CHECK(prof_end(c, PROF_BFS_INVALIDATE, 0));
   if (ncand > 0) {
     if (!early) {
@@ -1694,14 +1815,24 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, int32_t* P, int strategy, scou
       queued = 0;
       SCOUT_CHECK(prof_end(c, PROF_BFS_INVALIDATE, 0));
       // [8..30]: the prediction slots [24..27] come back unchanged
-      SCOUT_TRY(hipMemcpyAsync(c.h_box + 8, box + 8, 23 * sizeof(int64_t), hipMemcpyDeviceToHost, c.s));
-      SCOUT_TRY(hipStreamSynchronize(c.s));
-      c.readbacks++;
+      if (speculate) {
+        SCOUT_CHECK(box_read_async(c, 8, 23));
+        k_spec_gate<<<1, 64, 0, c.s>>>(box + 8 + p, box);
+        SCOUT_CHECK(queue_relax(box + kGateSlot, true));
+        SCOUT_CHECK(box_wait(c));
+      } else {
+        SCOUT_TRY(hipMemcpyAsync(c.h_box + 8, box + 8, 23 * sizeof(int64_t), hipMemcpyDeviceToHost, c.s));
+        SCOUT_TRY(hipStreamSynchronize(c.s));
+        c.readbacks++;
+      }
       if (std::max({c.h_box[14], c.h_box[19], c.h_box[30]}) > c.wb.cap) {
         set_error("bfs invalidation: chunk task list overflow");
         return SCOUT_ERR_STATE;
       }
       if (c.h_box[8 + p] == 0 && (async || lvl > cmax)) break;
+      // the gate stayed shut: nothing queued behind the read-back ran
+      speculate = false;
+      relax_queued = false;
       full_rounds = c.h_box[8 + p] > kTailFront ? kFullRounds : 0;
       // after a predicted first group the repair is near its end: start again from a short group
       group = group == first_group && first_group != kFirstGroup ? kFirstGroup : std::min(2 * group, kMaxGroup);
@@ -1740,39 +1871,27 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, int32_t* P, int strategy, scou
       if (logging) SCOUT_CHECK(mlp_end_batch(c, c.la.p, nsample, L));
       return SCOUT_OK;
     }
-    if (nx > 0) {
+    if (nx > 0 && !c.relax_async) {
       // [36] scanned, [37] medium rows, [38] chunk tasks (slots the relax leaves alone)
       k_zero<<<1, 64, 0, c.s>>>(box + 36, 3);
       uint64_t* tent = reinterpret_cast<uint64_t*>(c.wc.p);
-      k_tentative_short<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, c.ld.p, nx, tent, c.lf.p, box + 37, c.lf.cap,
-                                                        c.wb.p, box + 38, c.wb.cap, box + 36);
+      k_tentative_short<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, c.ld.p, box + 10, tent, c.lf.p, box + 37,
+                                                        c.lf.cap, c.wb.p, box + 38, c.wb.cap, box + 36, nullptr);
       k_tentative_rows<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), L, c.ld.p, c.lf.p, box + 37, c.wb.p, box + 38,
-                                                       c.wb.cap, tent, box + 36, c.d_paths);
-      k_apply_tentative<<<grid_for(nx, kBlock * kLinesPerThread), kBlock, 0, c.s>>>(c.ld.p, tent, nx, L, P, c.wa.p,
-                                                                                    c.lg.p, seed_cap, box);
+                                                       c.wb.cap, tent, box + 36, c.d_paths, nullptr);
+      k_apply_tentative<<<kTentGrid, kBlock, 0, c.s>>>(c.ld.p, tent, box + 10, L, P, c.wa.p, c.lg.p, seed_cap, box,
+                                                        nullptr);
     }
   }
-  if (m > 0)
+  if (m > 0 && !c.relax_async)
     k_insert_seeds<<<grid_for(m, kBlock * kLinesPerThread), kBlock, 0, c.s>>>(
-        c.g.view(), c.g.sorted, ch.pairs_distinct, ch.d_uv.p, ch.d_ins.p, m, L, c.wa.p, c.lg.p, seed_cap, box);
+        c.g.view(), c.g.sorted, ch.pairs_distinct, ch.d_uv.p, ch.d_ins.p, m, L, c.wa.p, c.lg.p, seed_cap, box,
+        nullptr);
   int64_t nseeds = 0;
   if (c.relax_async) {
-    // No read-back here: the seeds' count stays on the device for the relax's first round, and
-    // the seed and tentative-level counters come back with the relax's read-back.
-    k_seed_parents<<<kListGrid, kBlock, 0, c.s>>>(c.wa.p, c.lg.p, 0, box + 4, seed_cap, L, P);
-    int32_t epoch = next_epoch(c);
-    k_zero<<<1, 64, 0, c.s>>>(box + 8, 1);
-    // the relax re-attaches the invalidated cascades and runs out from the insertions; under
-    // the learned strategy its rounds follow the predicted depth, corrected by the running mean
-    // of their difference on earlier batches (rounded up, as for the invalidation)
-    int relax_group = kFirstGroup;
-    if (learned && predicted_depth >= 0)
-      relax_group = (int)std::min<int64_t>(
-          std::max<int64_t>(predicted_depth + (int64_t)std::ceil(c.relax_bias), kFirstGroup), kMaxGroup);
-    if (L == c.level && hint_relax > 0) relax_group = hint_relax;
+    if (!relax_queued) SCOUT_CHECK(queue_relax(nullptr, nx > 0));
     int64_t r0 = rounds;
-    SCOUT_CHECK(relax_levels(c, L, P, 0, 0, c.wa.p, 0, epoch, &rounds, &scanned, relax_group, true, box + 4,
-                             seed_cap));
+    SCOUT_CHECK(relax.finish(&rounds, &scanned));
     st->relax_rounds += rounds - r0;
     if (learned && predicted_depth >= 0)
       c.relax_bias = 0.75 * c.relax_bias + 0.25 * (double)(rounds - r0 - predicted_depth);
@@ -1794,7 +1913,7 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, int32_t* P, int strategy, scou
   st->touched += nx;
   if (!c.relax_async && nseeds > 0) {
     int32_t smin = (int32_t)c.h_box[5], smax = (int32_t)c.h_box[6];
-    k_seed_parents<<<grid_for(nseeds), kBlock, 0, c.s>>>(c.wa.p, c.lg.p, nseeds, nullptr, seed_cap, L, P);
+    k_seed_parents<<<grid_for(nseeds), kBlock, 0, c.s>>>(c.wa.p, c.lg.p, nseeds, nullptr, seed_cap, L, P, nullptr);
     int32_t epoch = next_epoch(c);
     k_zero<<<1, 64, 0, c.s>>>(box + 8, 1);
     // the learned scheduler's predicted last level bounds the relax as well

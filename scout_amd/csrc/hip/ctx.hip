@@ -23,9 +23,10 @@ int box_read(Ctx& c, int count) {
   return SCOUT_OK;
 }
 
-int box_read_async(Ctx& c, int count) {
+int box_read_async(Ctx& c, int first, int count) {
   if (!c.box_ev) SCOUT_TRY(hipEventCreateWithFlags(&c.box_ev, hipEventDisableTiming));
-  SCOUT_TRY(hipMemcpyAsync(c.h_box, c.d_box, (size_t)count * sizeof(int64_t), hipMemcpyDeviceToHost, c.s));
+  SCOUT_TRY(hipMemcpyAsync(c.h_box + first, c.d_box + first, (size_t)count * sizeof(int64_t), hipMemcpyDeviceToHost,
+                           c.s));
   SCOUT_TRY(hipEventRecord(c.box_ev, c.s));
   return SCOUT_OK;
 }

@@ -406,9 +406,10 @@ enum ProfId { PROF_BFS_RELAX = 0, PROF_BFS_INVALIDATE, PROF_CC_EXPAND, PROF_APPL
 
 // helpers in ctx.hip
 int box_read(Ctx& c, int count);  // copies d_box[0..count) into h_box, synchronises c.s
-// box_read in two halves: the copy is queued behind the work queued so far, and box_wait waits
-// for that copy alone, so work queued after it runs on while the host reads the values
-int box_read_async(Ctx& c, int count);
+// box_read in two halves: the copy of d_box[first..first+count) is queued behind the work
+// queued so far, and box_wait waits for that copy alone, so work queued after it runs on while
+// the host reads the values
+int box_read_async(Ctx& c, int first, int count);
 int box_wait(Ctx& c);
 int next_epoch(Ctx& c);
 int ensure_state(Ctx& c, int kind);

@@ -60,7 +60,7 @@ def gadget_graph(short, medium, leaves=20):
     return n, off, col, np.array(dels, dtype=np.int32)
 
 
-def run_stream(mode, n, off, col, batches):
+def run_stream(mode, n, off, col, batches, schedule=None):
     og = OracleGraph(n=n, off=off, col=col)
     eng = sa.Engine(0)
     eng.load_graph(sa.Graph(n, off, col))
@@ -74,6 +74,8 @@ def run_stream(mode, n, off, col, batches):
         assert np.array_equal(ins, ins_o), f"classification differs in batch {k}"
         eng.apply(False)
         og.apply(uv, ins_o)
+        if schedule:
+            eng.bfs_schedule(*schedule)
         st = eng.repair_bfs()
         want, got = og.bfs(0), eng.download(sa.STATE_LEVEL)
         assert np.array_equal(got, want), f"mode {mode} batch {k}: {np.count_nonzero(got != want)} levels differ"
@@ -119,3 +121,28 @@ def test_bfs_modes_exact_on_deep_cascades(mode):
                shortcuts[:, ::-1].copy()]
     rounds, _ = run_stream(mode, n, off, col, batches)
     say(f"[bfs modes] path mode {mode}: rounds {rounds}")
+
+
+@pytest.mark.parametrize("schedule", [(1, 1), (40, 40)])
+def test_relax_queued_behind_the_invalidation_read_back_exact(schedule):
+    """A first invalidation group sized in advance (here handed in with bfs_schedule; under the
+    learned strategy by the prediction) is expected to end the invalidation, so the asynchronous
+    repair queues the tentative levels, the seeds and the relax's first group behind that
+    group's read-back, each kernel shut by a device gate unless the invalidation has ended. The
+    asynchronous order queues two rounds whatever the schedule, so under both schedules the
+    gadget graph's two-round cascades open the gate, and the 753-round path keeps it shut (the
+    repair goes on with more invalidation rounds and queues the relax again). Levels equal the
+    oracle's after every batch either way."""
+    n, off, col, dels = gadget_graph(short=300, medium=150)
+    a_w = dels[0::2]
+    batches = [dels, dels[:, ::-1].copy(), a_w]
+    rounds, _ = run_stream(sa.BFS_ASYNC, n, off, col, batches, schedule=schedule)
+    say(f"[bfs modes] gadgets, schedule {schedule}: rounds {rounds}")
+    L = 3000
+    path = [(i, i + 1) for i in range(L)]
+    det = list(range(L + 1, L + 6))
+    extra = [(0, det[0])] + [(det[i], det[i + 1]) for i in range(4)] + [(det[-1], 1500)]
+    n, off, col = csr_from_pairs(L + 6, path + extra)
+    batches = [np.array([[0, 1]], dtype=np.int32), np.array([[0, 2000], [5, 2900]], dtype=np.int32)]
+    rounds, _ = run_stream(sa.BFS_ASYNC, n, off, col, batches, schedule=schedule)
+    say(f"[bfs modes] path, schedule {schedule}: rounds {rounds}")
