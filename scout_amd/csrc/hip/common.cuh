@@ -28,7 +28,7 @@ __device__ __forceinline__ Row row_of(const GView& g, int32_t v) {
     if (!(g.touched[v >> 5] & bit)) atomicOr(&g.touched[v >> 5], bit);
   }
   AHead h = g.ahead[v];
-  return Row{g.boff[v], g.boff[v + 1], h.off, h.off + h.len};
+  return Row{h.b0, h.b1, h.off, h.off + h.len};
 }
 __device__ __forceinline__ int64_t row_len(const Row& r) { return (r.b1 - r.b0) + (r.a1 - r.a0); }
 // k-th physical entry of the row (base segment first, then the inserted segment). Negative
@@ -408,11 +408,10 @@ __device__ __forceinline__ int64_t wave_find_first(const GView& g, const Row& r,
   return -1;
 }
 
-// Range [lo, hi) of base row u whose entries decode to v (sorted rows: binary search).
-// For unsorted rows returns the whole row and callers compare every entry.
-__device__ __forceinline__ void base_range(const GView& g, bool sorted, int32_t u, int32_t v, int64_t& lo,
-                                           int64_t& hi) {
-  int64_t b = g.boff[u], e = g.boff[u + 1];
+// Range [lo, hi) of the base row [b, e) whose entries decode to v (sorted rows: binary
+// search). For unsorted rows returns the whole row and callers compare every entry.
+__device__ __forceinline__ void base_range_in(const GView& g, bool sorted, int64_t b, int64_t e, int32_t v,
+                                              int64_t& lo, int64_t& hi) {
   if (!sorted) {
     lo = b;
     hi = e;
@@ -430,14 +429,20 @@ __device__ __forceinline__ void base_range(const GView& g, bool sorted, int32_t 
   hi = k;
 }
 
-// base_range through the entry samples when the row is long: the search first narrows to one
-// sample stride, so a hub row costs a few cached sample reads and one stride of the row. (Kept
-// apart from base_range, whose smaller code suits the kernels that inline it many times.)
-__device__ __forceinline__ void base_range_sampled(const GView& g, bool sorted, int32_t u, int32_t v, int64_t& lo,
-                                                   int64_t& hi) {
-  int64_t b = g.boff[u], e = g.boff[u + 1];
+// base_range_in over base row u.
+__device__ __forceinline__ void base_range(const GView& g, bool sorted, int32_t u, int32_t v, int64_t& lo,
+                                           int64_t& hi) {
+  base_range_in(g, sorted, g.boff[u], g.boff[u + 1], v, lo, hi);
+}
+
+// base_range_in through the entry samples when the row is long: the search first narrows to
+// one sample stride, so a hub row costs a few cached sample reads and one stride of the row.
+// (Kept apart from base_range_in, whose smaller code suits the kernels that inline it many
+// times.)
+__device__ __forceinline__ void base_range_sampled_in(const GView& g, bool sorted, int64_t b, int64_t e, int32_t v,
+                                                      int64_t& lo, int64_t& hi) {
   if (!sorted || !g.bsamp || e - b <= 4 * kSampleStride) {
-    base_range(g, sorted, u, v, lo, hi);
+    base_range_in(g, sorted, b, e, v, lo, hi);
     return;
   }
   // first sample position p = kSampleStride * k in [b, e) whose entry is >= v: the lower
@@ -462,6 +467,12 @@ __device__ __forceinline__ void base_range_sampled(const GView& g, bool sorted, 
   hi = k;
 }
 
+// base_range_sampled_in over base row u.
+__device__ __forceinline__ void base_range_sampled(const GView& g, bool sorted, int32_t u, int32_t v, int64_t& lo,
+                                                   int64_t& hi) {
+  base_range_sampled_in(g, sorted, g.boff[u], g.boff[u + 1], v, lo, hi);
+}
+
 struct PairState {
   bool base_live;     // base row u holds a live v
   bool base_tomb;     // base row u holds ~v
@@ -471,8 +482,9 @@ struct PairState {
   int64_t lo, hi;     // base entries of row u searched for v (base_range)
 };
 
-// pair_state over a base range [lo, hi) of row u already located.
-__device__ __forceinline__ PairState pair_state_in(const GView& g, int32_t u, int32_t v, int64_t lo, int64_t hi) {
+// pair_state over the base range [lo, hi) of row u already located; hu is u's header.
+__device__ __forceinline__ PairState pair_state_range(const GView& g, const AHead& hu, int32_t v, int64_t lo,
+                                                      int64_t hi) {
   PairState s{false, false, false, false, false, 0, 0};
   s.lo = lo;
   s.hi = hi;
@@ -482,7 +494,6 @@ __device__ __forceinline__ PairState pair_state_in(const GView& g, int32_t u, in
     else if (x == ~v) s.base_tomb = true;
   }
   s.base_any = s.base_live || s.base_tomb;
-  AHead hu = g.ahead[u];
   int64_t ab = hu.off, ae = ab + hu.len;
   for (int64_t k = ab; k < ae; k++) {
     int32_t x = g.acol[k];
@@ -492,26 +503,37 @@ __device__ __forceinline__ PairState pair_state_in(const GView& g, int32_t u, in
   return s;
 }
 
-__device__ __forceinline__ PairState pair_state(const GView& g, bool sorted, int32_t u, int32_t v) {
-  PairState s{false, false, false, false, false, 0, 0};
+// pair_state over a base range [lo, hi) of row u already located.
+__device__ __forceinline__ PairState pair_state_in(const GView& g, int32_t u, int32_t v, int64_t lo, int64_t hi) {
+  return pair_state_range(g, g.ahead[u], v, lo, hi);
+}
+
+// pair_state with u's header hu already loaded.
+__device__ __forceinline__ PairState pair_state_h(const GView& g, bool sorted, int32_t u, const AHead& hu,
+                                                  int32_t v) {
   int64_t lo, hi;
-  base_range_sampled(g, sorted, u, v, lo, hi);
-  s.lo = lo;
-  s.hi = hi;
-  for (int64_t k = lo; k < hi; k++) {
-    int32_t x = g.bcol[k];
-    if (x == v) s.base_live = true;
-    else if (x == ~v) s.base_tomb = true;
-  }
-  s.base_any = s.base_live || s.base_tomb;
+  base_range_sampled_in(g, sorted, hu.b0, hu.b1, v, lo, hi);
+  return pair_state_range(g, hu, v, lo, hi);
+}
+
+__device__ __forceinline__ PairState pair_state(const GView& g, bool sorted, int32_t u, int32_t v) {
+  return pair_state_h(g, sorted, u, g.ahead[u], v);
+}
+
+// The mirror offset of row u's one base entry for v (su.lo, when su found exactly one): the
+// offset of v's entry for u in base row v, or -1.
+__device__ __forceinline__ int32_t mirror_offset(const GView& g, const PairState& su) {
+  return g.twin && su.hi - su.lo == 1 ? g.twin[su.lo] : -1;
+}
+
+// pair_state(g, sorted, u, v) when row u's one base entry for v is expected at offset t of the
+// base row (a mirror offset): that entry is read instead of searching the row (the search runs
+// when t is -1 or the entry holds another neighbour).
+__device__ __forceinline__ PairState pair_state_twin(const GView& g, bool sorted, int32_t u, int32_t v, int32_t t) {
   AHead hu = g.ahead[u];
-  int64_t ab = hu.off, ae = ab + hu.len;
-  for (int64_t k = ab; k < ae; k++) {
-    int32_t x = g.acol[k];
-    if (x == v) s.added_live = true;
-    else if (x == ~v) s.added_tomb = true;
-  }
-  return s;
+  if (t >= 0 && hu.b0 + t < hu.b1 && decode(g.bcol[hu.b0 + t]) == v)
+    return pair_state_range(g, hu, v, hu.b0 + t, hu.b0 + t + 1);
+  return pair_state_h(g, sorted, u, hu, v);
 }
 
 // HybridGraph::isAdjacent (core/FlashGraph.h:356-366): inserted copy -> true; tombstone ->
@@ -526,13 +548,14 @@ __device__ __forceinline__ bool adjacent_now(const PairState& s) {
 // searches a sorted base row (through the samples), an unsorted one and the inserted row are
 // compared 64 entries at a time. Uniform across the wave.
 __device__ __forceinline__ bool wave_adjacent(const GView& g, bool sorted, int32_t u, int32_t v) {
-  int64_t b = g.boff[u], e = g.boff[u + 1];
+  AHead hu = g.ahead[u];
+  int64_t b = hu.b0, e = hu.b1;
   unsigned long long live = 0, tomb = 0;
   if (sorted) {
     int flags = 0;
     if (lane_id() == 0) {
       int64_t lo, hi;
-      base_range_sampled(g, true, u, v, lo, hi);
+      base_range_sampled_in(g, true, b, e, v, lo, hi);
       for (int64_t k = lo; k < hi; k++) {
         int32_t x = g.bcol[k];
         flags |= (x == v ? 1 : 0) | (x == ~v ? 2 : 0);
@@ -550,7 +573,6 @@ __device__ __forceinline__ bool wave_adjacent(const GView& g, bool sorted, int32
     }
   }
   unsigned long long alive = 0, atomb = 0;
-  AHead hu = g.ahead[u];
   int64_t ab = hu.off, ae = ab + hu.len;
   for (int64_t base = ab; base < ae; base += kWave) {
     int64_t k = base + lane_id();

@@ -277,6 +277,7 @@ int gen_rmat(Ctx& c, int scale, int ef, double a, double b, double cc, uint64_t 
   (void)hipFree(d_bin_bucket);
   g.E = E;
   g.base_gen++;
+  g.shape_gen++;
   SCOUT_CHECK(reset_inserted(c));
   return check_rows(c);
 }

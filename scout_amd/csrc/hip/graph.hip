@@ -116,17 +116,62 @@ __global__ void k_base_samples(const int32_t* bcol, int64_t E, int32_t* samp) {
   if (k * kSampleStride < E) samp[k] = decode(bcol[k * kSampleStride]);
 }
 
+// The row of every base entry, into out (a wave per row).
+__global__ void k_twin_rows(const int64_t* boff, int32_t n, int32_t* out) {
+  for (int64_t u = global_wave(); u < n; u += total_waves()) {
+    int64_t b = boff[u], e = boff[u + 1];
+    for (int64_t k = b + lane_id(); k < e; k += kWave) out[k] = (int32_t)u;
+  }
+}
+
+// twin[k] (holding the row u of entry k on entry) becomes the offset in row v = entry k's
+// neighbour of its one entry for u, or -1; *missing counts the -1s.
+__global__ void k_twin_search(GView g, int64_t E, int32_t* twin, int64_t* missing) {
+  int64_t miss = 0;
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < E; base += (int64_t)gridDim.x * blockDim.x) {
+    int64_t k = base + threadIdx.x;
+    if (k < E) {
+      int32_t u = twin[k];
+      int32_t v = decode(g.bcol[k]);
+      int64_t lo, hi;
+      base_range_sampled(g, true, v, u, lo, hi);
+      twin[k] = hi - lo == 1 ? (int32_t)(lo - g.boff[v]) : -1;
+      miss += hi - lo != 1;
+    }
+  }
+  block_add(miss, missing);
+}
+
 // pos (optional): the base range of row u found for each line, reused by k_apply.
 // hkey (optional): the repeated-pair count of k_dup_count is taken here too, into *repeated.
+// by_short_row: every base entry has exactly one mirror entry (g.twin complete), so the pair
+// stands in row u as it stands in row v; the shorter base row of the two is searched, and row
+// u's entry is found from the mirror offset.
 __global__ void k_classify(GView g, bool sorted, const int32_t* uv, int64_t m, int rule, uint8_t* ins,
-                           int64_t* pos, uint64_t* hkey, uint64_t mask, int64_t* repeated) {
+                           int64_t* pos, uint64_t* hkey, uint64_t mask, int64_t* repeated, bool by_short_row) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < m) {
     int32_t u = uv[2 * i], v = uv[2 * i + 1];
-    PairState s = pair_state(g, sorted, u, v);
+    PairState s;
+    int64_t lo, hi;
+    AHead hu = g.ahead[u];
+    AHead hv;
+    if (by_short_row) hv = g.ahead[v];
+    if (by_short_row && hv.b1 - hv.b0 < hu.b1 - hu.b0) {
+      s = pair_state_h(g, sorted, v, hv, u);
+      lo = hi = hu.b0;
+      if (s.hi - s.lo == 1) {
+        lo = hu.b0 + g.twin[s.lo];
+        hi = lo + 1;
+      }
+    } else {
+      s = pair_state_h(g, sorted, u, hu, v);
+      lo = s.lo;
+      hi = s.hi;
+    }
     if (pos) {
-      pos[2 * i] = s.lo;
-      pos[2 * i + 1] = s.hi;
+      pos[2 * i] = lo;
+      pos[2 * i + 1] = hi;
     }
     bool adj = rule == SCOUT_CLASSIFY_BASE ? s.base_any : adjacent_now(s);
     ins[i] = adj ? 0 : 1;
@@ -198,7 +243,7 @@ __global__ void k_apply(GView g, bool sorted, const int32_t* uv, const uint8_t* 
         if (su.base_tomb || su.added_tomb) {
           if (su.base_tomb) flip_base(g, su, v, false);
           if (su.added_tomb) holes += hole_first(g, u, ~v);
-          PairState sv = pair_state(g, sorted, v, u);
+          PairState sv = pair_state_twin(g, sorted, v, u, mirror_offset(g, su));
           if (sv.base_tomb) flip_base(g, sv, u, false);
           if (sv.added_tomb) holes += hole_first(g, v, ~u);
         } else {
@@ -213,7 +258,7 @@ __global__ void k_apply(GView g, bool sorted, const int32_t* uv, const uint8_t* 
         } else {
           if (su.base_live) flip_base(g, su, v, true);
           else if (!su.base_any && !su.added_tomb) e1 = NewEntry{true, u, ~v};
-          PairState sv = pair_state(g, sorted, v, u);
+          PairState sv = pair_state_twin(g, sorted, v, u, mirror_offset(g, su));
           if (sv.base_live) flip_base(g, sv, u, true);
           else if (!sv.base_any && !sv.added_tomb) e2 = NewEntry{true, v, ~u};
         }
@@ -291,7 +336,7 @@ __device__ __forceinline__ void place_group(AHead* ahead, int32_t* acol, int32_t
       acol[pos + len + j] = dst[j];
       aown[pos + len + j] = u;
     }
-    ahead[u] = AHead{pos, (int32_t)(len + k), (int32_t)need};
+    set_inserted(ahead[u], pos, (int32_t)(len + k), (int32_t)need);
   } else {
     int64_t slot = (int64_t)atomicAdd((unsigned long long*)mv.count, 1ULL);
     mv.u[slot] = u;
@@ -501,7 +546,7 @@ __global__ void k_move_rows(AHead* ahead, int32_t* acol, int32_t* aown, const in
       acol[pos + len + j] = d[j];
       aown[pos + len + j] = u;
     }
-    if (lane_id() == 0) ahead[u] = AHead{pos, (int32_t)(len + k), grown_capacity(len + k)};
+    if (lane_id() == 0) set_inserted(ahead[u], pos, (int32_t)(len + k), grown_capacity(len + k));
   }
 }
 
@@ -555,10 +600,32 @@ __global__ void k_repack_copy(GView g, const int64_t* aoff2, int32_t* acol2, int
   }
 }
 
-// The re-packed headers: start, live count and capacity of every row (and the n-th).
-__global__ void k_repack_heads(AHead* ahead2, const int64_t* off2, const int32_t* cnt, const int32_t* cap2, int64_t n) {
+// The re-packed headers: start, live count and capacity of every row (and the n-th), with the
+// base rows of boff.
+__global__ void k_repack_heads(AHead* ahead2, const int64_t* off2, const int32_t* cnt, const int32_t* cap2,
+                               const int64_t* boff, int64_t n) {
   int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (v <= n) ahead2[v] = AHead{off2[v], v < n ? cnt[v] : 0, cap2[v]};
+  if (v > n) return;
+  AHead h;
+  h.off = off2[v];
+  h.len = v < n ? cnt[v] : 0;
+  h.cap = cap2[v];
+  h.b0 = boff[v];
+  h.b1 = v < n ? boff[v + 1] : boff[v];
+  ahead2[v] = h;
+}
+
+// Empty inserted rows with the base rows of boff (and the n-th header).
+__global__ void k_reset_heads(AHead* ahead, const int64_t* boff, int64_t n) {
+  int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v > n) return;
+  AHead h;
+  h.off = 0;
+  h.len = 0;
+  h.cap = 0;
+  h.b0 = boff[v];
+  h.b1 = v < n ? boff[v + 1] : boff[v];
+  ahead[v] = h;
 }
 
 // Live entries per vertex of the logical graph: thread per vertex for rows up to 64 physical
@@ -680,7 +747,8 @@ int repack_inserted(Ctx& c, int64_t headroom) {
   SCOUT_CHECK(g.acol2.reserve(total + headroom + 1024));
   SCOUT_CHECK(g.aown2.reserve(total + headroom + 1024));
   k_repack_copy<<<kListGrid, kBlock, 0, c.s>>>(g.view(), g.roff.p, g.acol2.p, g.aown2.p);
-  k_repack_heads<<<grid_for((int64_t)n + 1), kBlock, 0, c.s>>>(g.ahead2.p, g.roff.p, g.moves.p, g.rcap.p, n);
+  k_repack_heads<<<grid_for((int64_t)n + 1), kBlock, 0, c.s>>>(g.ahead2.p, g.roff.p, g.moves.p, g.rcap.p, g.boff,
+                                                                n);
   std::swap(g.ahead, g.ahead2);
   std::swap(g.acol, g.acol2);
   std::swap(g.aown, g.aown2);
@@ -863,9 +931,11 @@ int graph_free(Ctx& c) {
   g.mkey.release();
   g.mdst.release();
   g.bsamp.release();
+  g.twin.release();
   g.n = 0;
   g.E = g.A = g.live = g.holes = 0;
   g.base_gen++;
+  g.shape_gen++;
   return SCOUT_OK;
 }
 
@@ -873,7 +943,7 @@ int reset_inserted(Ctx& c) {
   DevGraph& g = c.g;
   size_t n1 = (size_t)g.n + 1;
   SCOUT_CHECK(g.ahead.reserve((int64_t)n1));
-  SCOUT_TRY(hipMemsetAsync(g.ahead.p, 0, n1 * sizeof(AHead), c.s));
+  k_reset_heads<<<grid_for((int64_t)n1), kBlock, 0, c.s>>>(g.ahead.p, g.boff, g.n);
   SCOUT_CHECK(g.acol.reserve(1 << 16));
   SCOUT_CHECK(g.aown.reserve(g.acol.cap));
   g.A = g.live = g.holes = 0;
@@ -911,6 +981,7 @@ int graph_upload_egr(Ctx& c, int32_t n, int64_t E, const void* nindex, int wide,
   g.n = n;
   g.E = E;
   g.base_gen++;
+  g.shape_gen++;
   SCOUT_TRY(hipMalloc((void**)&g.boff, ((size_t)n + 1) * sizeof(int64_t)));
   if (host_resident) {
     // the base entries stay in pinned host memory, filled by OpenMP threads from the (mmap'd)
@@ -976,12 +1047,41 @@ int graph_samples(Ctx& c) {
   return SCOUT_OK;
 }
 
+// Mirror offsets of the base entries (GView::twin): for the entry of row u holding v, the
+// offset in row v of its entry for u (-1 if row v holds u other than exactly once). k_apply
+// removes or restores both directions of a base pair; with the offset the second direction is
+// one read instead of a search of row v. Built once per base layout (tombstone flips keep it
+// valid) when its 4 bytes per entry take at most half the free device memory and no HBM budget
+// is set (a budget bounds the graph and the solution state, which the index would crowd).
+constexpr int kTwinSlot = 34;  // mailbox slot of the build's count of entries without a mirror
+
+int graph_twins(Ctx& c) {
+  DevGraph& g = c.g;
+  if (!g.sorted || g.host_bcol || g.E == 0 || c.budget.hbm_bytes > 0 || g.twin_gen == g.shape_gen) return SCOUT_OK;
+  if (g.twin.cap < g.E) {
+    size_t free_b = 0, total_b = 0;
+    SCOUT_TRY(hipMemGetInfo(&free_b, &total_b));
+    if (4.0 * (double)g.E > 0.5 * (double)free_b) return SCOUT_OK;
+    g.twin.release();
+    SCOUT_CHECK(g.twin.reserve(g.E));
+  }
+  k_twin_rows<<<kListGrid, kBlock, 0, c.s>>>(g.boff, g.n, g.twin.p);
+  SCOUT_TRY(hipMemsetAsync(c.d_box + kTwinSlot, 0, sizeof(int64_t), c.s));
+  k_twin_search<<<4 * kListGrid, kBlock, 0, c.s>>>(g.view(), g.E, g.twin.p, c.d_box + kTwinSlot);
+  SCOUT_TRY(hipMemcpyAsync(c.h_box + kTwinSlot, c.d_box + kTwinSlot, sizeof(int64_t), hipMemcpyDeviceToHost, c.s));
+  SCOUT_TRY(hipStreamSynchronize(c.s));
+  g.twin_gen = g.shape_gen;
+  g.twin_symmetric = c.h_box[kTwinSlot] == 0;
+  return SCOUT_OK;
+}
+
 int graph_classify_lines(Ctx& c, Channel& ch, int64_t lo, int64_t hi, int rule) {
   if (!c.g.boff) {
     set_error("classify: no graph loaded");
     return SCOUT_ERR_STATE;
   }
   SCOUT_CHECK(graph_samples(c));
+  SCOUT_CHECK(graph_twins(c));
   int64_t m = hi - lo;
   SCOUT_CHECK(ch.d_pos.reserve(2 * ch.m));
   if (ch.pos_gen != c.g.base_gen || ch.pos_hi != lo) ch.pos_lo = lo;  // else extend the valid range
@@ -1001,7 +1101,8 @@ int graph_classify_lines(Ctx& c, Channel& ch, int64_t lo, int64_t hi, int rule) 
   SCOUT_CHECK(prof_begin(c, PROF_CLASSIFY));
   if (m > 0)
     k_classify<<<grid_for(m), kBlock, 0, c.s>>>(c.g.view(), c.g.sorted, ch.d_uv.p + 2 * lo, m, rule, ch.d_ins.p + lo,
-                                                ch.d_pos.p + 2 * lo, hkey, size - 1, repeated);
+                                                ch.d_pos.p + 2 * lo, hkey, size - 1, repeated,
+                                                c.g.view().twin != nullptr && c.g.twin_symmetric);
   // per line: the pair (8 B), row u's base and inserted offsets (16 B), the located entry
   // (4 B), the label written (1 B) and, when counting repeats, the pair's set slot (8 B)
   SCOUT_CHECK(prof_end(c, PROF_CLASSIFY, (count_repeats ? 37 : 29) * m));
@@ -1143,6 +1244,7 @@ int graph_compact(Ctx& c) {
   col2.p = nullptr;
   g.E = total;
   g.base_gen++;
+  g.shape_gen++;
   SCOUT_CHECK(reset_inserted(c));
   return check_rows(c);
 }

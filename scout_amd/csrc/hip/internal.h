@@ -100,13 +100,23 @@ struct PinBuf {
   }
 };
 
-// Header of a vertex's inserted row: its span [off, off + cap) of the pool, of which the first
-// len slots are in use. One aligned 16-byte record, so a row's header is one load.
-struct alignas(16) AHead {
+// Header of a vertex's rows: its inserted row's span [off, off + cap) of the pool, of which the
+// first len slots are in use, and its base row [b0, b1) of the base entries (a copy of the
+// base offsets). One aligned 32-byte record, so a row lookup is one load of one cache line.
+struct alignas(32) AHead {
   int64_t off;
   int32_t len;
   int32_t cap;
+  int64_t b0;
+  int64_t b1;
 };
+
+// Sets the inserted row of a header, keeping its base row.
+__host__ __device__ inline void set_inserted(AHead& h, int64_t off, int32_t len, int32_t cap) {
+  h.off = off;
+  h.len = len;
+  h.cap = cap;
+}
 
 // Read-only view of the logical graph for kernels.
 struct GView {
@@ -117,6 +127,9 @@ struct GView {
   int32_t n;
   const int32_t* bsamp;  // decoded bcol[kSampleStride * k], or nullptr (rows searched directly)
   uint32_t* touched;     // bitmap of vertices whose rows a repair read, or nullptr (not tracked)
+  // per base entry (row u, neighbour v): the offset in row v of its one entry for u, or -1;
+  // nullptr when not built (rows are searched instead)
+  const int32_t* twin;
 };
 
 // Every kSampleStride-th base entry is sampled (decoded): a search in a long sorted row first
@@ -132,6 +145,9 @@ struct DevGraph {
   int64_t holes = 0;    // HOLE entries inside rows
   bool sorted = false;  // every base row ascending (by decoded id)
   int64_t base_gen = 0; // bumped whenever the base arrays are replaced or restored
+  // bumped whenever the base rows' layout changes; a restore takes the snapshot's (its
+  // tombstones differ, its layout is the snapshot's)
+  int64_t shape_gen = 0;
   int64_t* boff = nullptr;
   int32_t* bcol = nullptr;      // device address of the base entries (HBM, or mapped host memory)
   // storage-resident mode: the base entries stay in pinned host memory that kernels read over
@@ -155,11 +171,15 @@ struct DevGraph {
   // base entry samples and the base generation they were taken from
   DevBuf<int32_t> bsamp;
   int64_t bsamp_gen = -1;
+  // mirror offsets of the base entries (GView::twin) and the layout they were built for
+  DevBuf<int32_t> twin;
+  int64_t twin_gen = -1;
+  bool twin_symmetric = false;  // every base entry has exactly one mirror entry
   // set only while a repair runs with row tracking on (scout_touched_enable)
   uint32_t* touched = nullptr;
   GView view() const {
     return GView{boff, bcol, ahead.p, acol.p, n, (sorted && bsamp_gen == base_gen) ? bsamp.p : nullptr,
-                 touched};
+                 touched, (sorted && twin_gen == shape_gen) ? twin.p : nullptr};
   }
 };
 

@@ -91,18 +91,17 @@ __device__ __forceinline__ bool member(const MisView& s, int32_t x, bool* confli
 // Live entries equal to y in row x (base entries not marked deleted, inserted entries); the
 // shorter of the two rows is searched (the adjacency is symmetric).
 __device__ __forceinline__ int32_t live_count(const MisView& s, int32_t x, int32_t y) {
-  int64_t lx = (s.g.boff[x + 1] - s.g.boff[x]) + s.g.ahead[x].len;
-  int64_t ly = (s.g.boff[y + 1] - s.g.boff[y]) + s.g.ahead[y].len;
-  if (ly < lx) {
+  AHead hx = s.g.ahead[x], hy = s.g.ahead[y];
+  if ((hy.b1 - hy.b0) + hy.len < (hx.b1 - hx.b0) + hx.len) {
     int32_t tmp = x;
     x = y;
     y = tmp;
+    hx = hy;
   }
   int64_t lo, hi;
-  base_range(s.g, s.sorted, x, y, lo, hi);
+  base_range_in(s.g, s.sorted, hx.b0, hx.b1, y, lo, hi);
   int32_t c = 0;
   for (int64_t k = lo; k < hi; k++) c += s.g.bcol[k] == y;
-  AHead hx = s.g.ahead[x];
   for (int64_t k = hx.off, e = k + hx.len; k < e; k++) c += s.g.acol[k] == y;
   return c;
 }

@@ -505,6 +505,7 @@ struct Snapshot {
   DevBuf<int32_t> level, parent, label, color;
   DevBuf<uint8_t> mis;
   int64_t E = 0, A = 0, holes = 0;
+  int64_t shape_gen = 0;
   bool sorted = false;
   bool has_level = false, has_label = false, has_color = false, has_mis = false;
 };
@@ -530,6 +531,7 @@ int snapshot_save(Ctx& c) {
     SCOUT_TRY(hipMemcpyAsync(s.aown.p, c.g.aown.p, (size_t)c.g.A * 4, hipMemcpyDeviceToDevice, c.s));
   }
   s.E = c.g.E;
+  s.shape_gen = c.g.shape_gen;
   s.A = c.g.A;
   s.live = c.g.live;
   s.holes = c.g.holes;
@@ -583,6 +585,7 @@ int snapshot_restore(Ctx& c) {
   }
   c.g.E = s.E;
   c.g.base_gen++;
+  c.g.shape_gen = s.shape_gen;
   c.g.A = s.A;
   c.g.live = s.live;
   c.g.holes = s.holes;
