@@ -694,10 +694,7 @@ def main():
         eng.restore()
         for b in range(ntrain + args.warmup):
             eng.batch_load(b)
-            eng.classify(download=False)
-            eng.apply()
-            eng.repair_bfs(sa.STRATEGY_LOCAL if b < ntrain else strategy)
-            eng.repair_cc(sa.STRATEGY_LOCAL if b < ntrain else strategy)
+            eng.commit(sa.STRATEGY_LOCAL if b < ntrain else strategy)
         eng.sync()
         if profile:
             eng.prof_enable(True)
@@ -706,19 +703,23 @@ def main():
         work = {"bfs_invalidated": 0, "bfs_seeds": 0, "bfs_rounds": 0, "bfs_entries_scanned": 0,
                 "cc_entries_scanned": 0, "readbacks": 0, "fallbacks": 0, "insertions": 0, "deletions": 0}
         rounds = []
+        stats = []
         eng.sync()
         t_start = time.perf_counter()
         for i, b in enumerate(range(ntrain + args.warmup, total)):
             ts = time.perf_counter()
             eng.batch_load(b)
-            eng.classify(download=False)
-            eng.apply()
             if schedule:
                 eng.bfs_schedule(*schedule[i])
-            sb = eng.repair_bfs(strategy)
-            sc = eng.repair_cc(strategy)
+            # classify, apply and both repairs in one call (scout_commit), the same work as the
+            # four calls one by one
+            stats.append(eng.commit(strategy))
             eng.sync()
             lat.append((time.perf_counter() - ts) * 1e3)
+        eng.sync()
+        elapsed = time.perf_counter() - t_start
+        # the harness's bookkeeping stays out of the timed region
+        for sb, sc in stats:
             lines += sb.lines
             work["bfs_invalidated"] += sb.touched
             work["bfs_seeds"] += sb.seeds
@@ -730,8 +731,6 @@ def main():
             work["insertions"] += sb.insertions
             work["deletions"] += sb.deletions
             rounds.append((max(int(sb.inval_rounds), 1), max(int(sb.relax_rounds), 1)))
-        eng.sync()
-        elapsed = time.perf_counter() - t_start
         prof = None
         if profile:
             prof = {name: eng.prof_read(i) for i, name in enumerate(sa.PROF_NAMES)}

@@ -72,7 +72,7 @@ typedef struct {
   int64_t conflicts;    /* coloring: insertions with equal colours; mis: edits that wrote */
   int64_t sub_batches;  /* mis: sub-batches */
   int64_t edges_scanned;/* adjacency entries read by the repair kernels */
-  double repair_ms;     /* device time of the repair, from HIP events on the compute stream */
+  double repair_ms;     /* time of the repair call on the host clock, until its device work is done */
   double apply_ms;      /* device time of classification + graph update */
   int64_t readbacks;    /* host read-backs of device counters (each waits for the stream) */
   int64_t inval_rounds; /* bfs: invalidation levels with work */
@@ -161,6 +161,14 @@ int scout_apply(scout_ctx* ctx, int channel, int conditional);
 /* BFS levels: replaces processBatch + gpu_relax_round + gpu_deletion_round
  * (src/bfs/scout_bfs.cu:177-272,325-515). */
 int scout_repair_bfs(scout_ctx* ctx, int channel, int strategy, scout_stats* st);
+/* One batch committed in one call: scout_classify (SCOUT_CLASSIFY_CURRENT, labels kept on the
+ * device), scout_apply (unconditional), scout_repair_bfs and scout_repair_cc on the channel,
+ * in that order, with the same results and statistics. Needs levels and labels. The
+ * reference's BFS and CC binaries each run processBatch on their own solution
+ * (src/bfs/scout_bfs.cu:177-272, src/cc/scout_cc.cu:300-441); this context keeps both
+ * solutions, and one call lets the host go from each phase's last read-back straight to the
+ * next phase's launches. */
+int scout_commit(scout_ctx* ctx, int channel, int strategy, scout_stats* bfs_st, scout_stats* cc_st);
 /* The cascade bound b of the reference (SCOUT_BFS_BOUND, src/bfs/scout_bfs.cu:475-488): a
  * batch whose invalidated vertices exceed b per deleted parent edge is finished by a full BFS
  * (st->fallback = 1) instead of the local tentative levels and relax. The answer is exact either

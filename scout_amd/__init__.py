@@ -69,6 +69,7 @@ _SIGNATURES = {
     "scout_apply": [_P, ctypes.c_int, ctypes.c_int],
     "scout_repair_bfs": [_P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(Stats)],
     "scout_repair_cc": [_P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(Stats)],
+    "scout_commit": [_P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(Stats), ctypes.POINTER(Stats)],
     "scout_repair_color": [_P, ctypes.c_int, ctypes.POINTER(Stats)],
     "scout_repair_mis": [_P, ctypes.c_int, _I64, ctypes.POINTER(Stats)],
     "scout_repair_mis_multi": [_P, ctypes.c_int, _I32, _P, _I64, ctypes.POINTER(Stats)],
@@ -301,6 +302,13 @@ class Engine:
         st = Stats()
         self._check(self.lib.scout_repair_cc(self.h, channel, strategy, ctypes.byref(st)), "repair_cc")
         return st
+
+    def commit(self, strategy: int = STRATEGY_LOCAL, channel: int = 0):
+        """classify (labels kept on the device), apply, repair_bfs and repair_cc of the staged
+        lines in one call; returns the BFS and the CC stats."""
+        sb, sc = Stats(), Stats()
+        self._check(self.lib.scout_commit(self.h, channel, strategy, ctypes.byref(sb), ctypes.byref(sc)), "commit")
+        return sb, sc
 
     def repair_color(self, channel: int = 0) -> Stats:
         st = Stats()

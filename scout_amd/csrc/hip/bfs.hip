@@ -1946,6 +1946,30 @@ int bfs_local(Ctx& c, Channel& ch, int strategy, scout_stats* st) {
   return SCOUT_OK;
 }
 
+int repair_bfs(Ctx& c, Channel& ch, int strategy, scout_stats* st) {
+  if (!c.g.boff || !c.level) {
+    set_error("repair_bfs: graph and levels must be loaded first");
+    return SCOUT_ERR_STATE;
+  }
+  TouchScope touch(c);
+  Span span(c);
+  SCOUT_CHECK(span.begin(c.s));
+  st->lines += ch.m;
+  int64_t rb0 = c.readbacks;
+  // a local repair needs levels that were exact for the graph before this batch
+  bool stale = c.level_seq != c.apply_seq - 1;
+  if (strategy == SCOUT_STRATEGY_RECOMPUTE || stale) {
+    st->fallback = 1;
+    SCOUT_CHECK(bfs_full(c, c.source, &st->rounds, &st->edges_scanned));
+  } else {
+    SCOUT_CHECK(bfs_local(c, ch, strategy, st));
+  }
+  SCOUT_CHECK(span.end(c.s));
+  st->repair_ms += span.ms();
+  st->readbacks += c.readbacks - rb0;
+  return SCOUT_OK;
+}
+
 }  // namespace scout
 
 using namespace scout;
@@ -2005,28 +2029,7 @@ int scout_repair_bfs(scout_ctx* x, int channel, int strategy, scout_stats* st) {
   SCOUT_TRY(hipSetDevice(c.dev));
   Channel* ch = channel_of(c, channel);
   if (!ch) return SCOUT_ERR_ARG;
-  if (!c.g.boff || !c.level) {
-    set_error("repair_bfs: graph and levels must be loaded first");
-    return SCOUT_ERR_STATE;
-  }
-  TouchScope touch(c);
-  Span span;
-  SCOUT_CHECK(span.begin(c.s));
-  st->lines += ch->m;
-  int64_t rb0 = c.readbacks;
-  int rc;
-  // a local repair needs levels that were exact for the graph before this batch
-  bool stale = c.level_seq != c.apply_seq - 1;
-  if (strategy == SCOUT_STRATEGY_RECOMPUTE || stale) {
-    st->fallback = 1;
-    rc = bfs_full(c, c.source, &st->rounds, &st->edges_scanned);
-  } else {
-    rc = bfs_local(c, *ch, strategy, st);
-  }
-  SCOUT_CHECK(rc);
-  SCOUT_CHECK(span.end(c.s));
-  st->repair_ms += span.ms();
-  st->readbacks += c.readbacks - rb0;
+  SCOUT_CHECK(repair_bfs(c, *ch, strategy, st));
   return channel_used(c.s, *ch);
 }
 

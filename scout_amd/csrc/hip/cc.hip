@@ -368,9 +368,11 @@ __global__ void k_cc_mark(const int32_t* uv, int64_t m, const int32_t* L, const 
   if (!(coarse[cl >> 5] & cbit)) atomicOr(&coarse[cl >> 5], cbit);
 }
 
-__global__ void k_zero_u32(uint32_t* p, int64_t count) {
+// Zeroes count words, and the first nbox counters of box (one launch for both).
+__global__ void k_zero_u32(uint32_t* p, int64_t count, int64_t* box, int nbox) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < count) p[i] = 0;
+  if (i < nbox) box[i] = 0;
 }
 
 // Grid-stride pass over all vertices: newly reached vertices take label 0; unreached vertices
@@ -695,9 +697,8 @@ int cc_scan(Ctx& c, Channel& ch, const int32_t* L, scout_stats* st) {
   int64_t* box = c.d_box;
   // box: [0] listed vertices, [1] chunk tasks, [2] scanned entries, [3] roots among the listed,
   // [4] roots of untouched components, [5] long rows
-  k_cc_zero<<<1, 64, 0, c.s>>>(box, 6);
   SCOUT_CHECK(prof_begin(c, PROF_CC_EXPAND));
-  k_zero_u32<<<grid_for(words + cwords), kBlock, 0, c.s>>>(bits, words + cwords);
+  k_zero_u32<<<grid_for(std::max<int64_t>(words + cwords, 6)), kBlock, 0, c.s>>>(bits, words + cwords, box, 6);
   if (ch.m > 0) k_cc_mark<<<grid_for(2 * ch.m), kBlock, 0, c.s>>>(ch.d_uv.p, ch.m, L, c.label, bits, coarse);
   SCOUT_CHECK(prof_launch(c, PROF_K_CC_COLLECT, k_cc_collect, dim3(kListGrid), dim3(kBlock), L, c.label, bits, coarse,
                           n, par, c.la.p, box, (int64_t)n + 1, box + 4));
@@ -766,6 +767,30 @@ int cc_local(Ctx& c, Channel& ch, int strategy, scout_stats* st) {
   return walk ? cc_walk(c, ch, L, st) : cc_scan(c, ch, L, st);
 }
 
+int repair_cc(Ctx& c, Channel& ch, int strategy, scout_stats* st) {
+  if (!c.g.boff || !c.label) {
+    set_error("repair_cc: graph and labels must be loaded first");
+    return SCOUT_ERR_STATE;
+  }
+  TouchScope touch(c);
+  Span span(c);
+  SCOUT_CHECK(span.begin(c.s));
+  st->lines += ch.m;
+  int64_t rb0 = c.readbacks;
+  // relabelling only the touched components needs labels exact for the graph before this batch
+  bool stale = c.label_seq != c.apply_seq - 1;
+  if (strategy == SCOUT_STRATEGY_RECOMPUTE || stale) {
+    st->fallback = 1;
+    SCOUT_CHECK(cc_full(c, &st->edges_scanned));
+  } else {
+    SCOUT_CHECK(cc_local(c, ch, strategy, st));
+  }
+  SCOUT_CHECK(span.end(c.s));
+  st->repair_ms += span.ms();
+  st->readbacks += c.readbacks - rb0;
+  return SCOUT_OK;
+}
+
 void cc_release(Ctx& c) { (void)c; }
 
 }  // namespace scout
@@ -804,26 +829,7 @@ int scout_repair_cc(scout_ctx* x, int channel, int strategy, scout_stats* st) {
   SCOUT_TRY(hipSetDevice(c.dev));
   Channel* ch = channel_of(c, channel);
   if (!ch) return SCOUT_ERR_ARG;
-  if (!c.g.boff || !c.label) {
-    set_error("repair_cc: graph and labels must be loaded first");
-    return SCOUT_ERR_STATE;
-  }
-  TouchScope touch(c);
-  Span span;
-  SCOUT_CHECK(span.begin(c.s));
-  st->lines += ch->m;
-  int64_t rb0 = c.readbacks;
-  // relabelling only the touched components needs labels exact for the graph before this batch
-  bool stale = c.label_seq != c.apply_seq - 1;
-  if (strategy == SCOUT_STRATEGY_RECOMPUTE || stale) {
-    st->fallback = 1;
-    SCOUT_CHECK(cc_full(c, &st->edges_scanned));
-  } else {
-    SCOUT_CHECK(cc_local(c, *ch, strategy, st));
-  }
-  SCOUT_CHECK(span.end(c.s));
-  st->repair_ms += span.ms();
-  st->readbacks += c.readbacks - rb0;
+  SCOUT_CHECK(repair_cc(c, *ch, strategy, st));
   return channel_used(c.s, *ch);
 }
 

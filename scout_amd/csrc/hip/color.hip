@@ -424,7 +424,7 @@ extern "C" int scout_repair_color(scout_ctx* x, int channel, scout_stats* st) {
     return SCOUT_ERR_STATE;
   }
   TouchScope touch(c);
-  Span span;
+  Span span(c);
   SCOUT_CHECK(span.begin(c.s));
   st->lines += ch->m;
   int64_t rb0 = c.readbacks;
@@ -493,7 +493,7 @@ extern "C" int scout_color_autotune(scout_ctx* x, int channel, double* best_ms) 
         for (int rep = 0; rep < 2; rep++) {
           SCOUT_TRY(hipMemcpyAsync(c.color, saved.p, (size_t)n * 4, hipMemcpyDeviceToDevice, c.s));
           scout_stats st{};
-          Span span;
+          Span span(c);
           SCOUT_CHECK(span.begin(c.s));
           SCOUT_CHECK(color_repair(c, *ch, &st));
           SCOUT_CHECK(span.end(c.s));

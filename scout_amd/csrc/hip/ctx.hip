@@ -130,25 +130,16 @@ int ensure_scratch(Ctx& c, int32_t** p, int32_t fill) {
 int ensure_list(DevBuf<int32_t>& l, int64_t count) { return l.reserve(count); }
 
 int Span::begin(hipStream_t s) {
-  if (!a) {
-    SCOUT_TRY(hipEventCreate(&a));
-    SCOUT_TRY(hipEventCreate(&b));
-  }
-  SCOUT_TRY(hipEventRecord(a, s));
+  (void)s;
+  t0 = std::chrono::steady_clock::now();
   return SCOUT_OK;
 }
 int Span::end(hipStream_t s) {
-  SCOUT_TRY(hipEventRecord(b, s));
+  SCOUT_TRY(hipStreamSynchronize(s));
   return SCOUT_OK;
 }
 double Span::ms() {
-  float t = 0.f;
-  if (hipEventSynchronize(b) != hipSuccess || hipEventElapsedTime(&t, a, b) != hipSuccess) return 0.0;
-  return (double)t;
-}
-Span::~Span() {
-  if (a) (void)hipEventDestroy(a);
-  if (b) (void)hipEventDestroy(b);
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
 struct ProfState {
@@ -588,6 +579,23 @@ int scout_apply(scout_ctx* x, int channel, int conditional) {
   if (!ch) return SCOUT_ERR_ARG;
   SCOUT_CHECK(channel_wait(c.s, *ch));
   SCOUT_CHECK(graph_apply(c, *ch, conditional));
+  return channel_used(c.s, *ch);
+}
+
+int scout_commit(scout_ctx* x, int channel, int strategy, scout_stats* bfs_st, scout_stats* cc_st) {
+  SCOUT_CHECK(check_graph(x));
+  Ctx& c = x->c;
+  Channel* ch = channel_of(c, channel);
+  if (!ch || !bfs_st || !cc_st) return SCOUT_ERR_ARG;
+  if (!c.level || !c.label) {
+    set_error("commit: levels and labels must be loaded first");
+    return SCOUT_ERR_STATE;
+  }
+  SCOUT_CHECK(channel_wait(c.s, *ch));
+  SCOUT_CHECK(graph_classify(c, *ch, SCOUT_CLASSIFY_CURRENT));
+  SCOUT_CHECK(graph_apply(c, *ch, 0));
+  SCOUT_CHECK(repair_bfs(c, *ch, strategy, bfs_st));
+  SCOUT_CHECK(repair_cc(c, *ch, strategy, cc_st));
   return channel_used(c.s, *ch);
 }
 

@@ -883,13 +883,22 @@ int apply_range(Ctx& c, Channel& ch, int64_t lo, int64_t hi, int conditional, co
   return SCOUT_OK;
 }
 
-// Clears the pair set of k_dup_count and k_classify for m lines: a power of two of at least
-// 4m slots, so it is at most a quarter full. *size = its slot count.
-int pair_set_clear(Ctx& c, int64_t m, uint64_t* size) {
-  uint64_t s = 1;
+// Empties size slots of the pair set (16 bytes per store) and zeroes its repeat counter.
+__global__ void k_pair_set_clear(uint64_t* hkey, uint64_t size, int64_t* count) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) *count = 0;
+  ulonglong2* h2 = reinterpret_cast<ulonglong2*>(hkey);
+  for (uint64_t k = i; 2 * k < size; k += (uint64_t)gridDim.x * blockDim.x) h2[k] = make_ulonglong2(kEmpty, kEmpty);
+}
+
+// Clears the pair set of k_dup_count and k_classify for m lines (a power of two of at least
+// 4m slots, so it is at most a quarter full) and zeroes *count, in one launch. *size = its
+// slot count.
+int pair_set_clear(Ctx& c, int64_t m, uint64_t* size, int64_t* count) {
+  uint64_t s = 2;
   while (s < (uint64_t)(4 * m)) s <<= 1;
   SCOUT_CHECK(c.g.hkey.reserve((int64_t)s));
-  SCOUT_TRY(hipMemsetAsync(c.g.hkey.p, 0xff, s * sizeof(uint64_t), c.s));
+  k_pair_set_clear<<<kListGrid, kBlock, 0, c.s>>>(c.g.hkey.p, s, count);
   *size = s;
   return SCOUT_OK;
 }
@@ -1094,9 +1103,8 @@ int graph_classify_lines(Ctx& c, Channel& ch, int64_t lo, int64_t hi, int rule) 
   uint64_t size = 0;
   int64_t* repeated = c.d_box + kChannelRepeatSlot + (&ch - c.ch);
   if (count_repeats) {
-    SCOUT_CHECK(pair_set_clear(c, m, &size));
+    SCOUT_CHECK(pair_set_clear(c, m, &size, repeated));
     hkey = c.g.hkey.p;
-    SCOUT_TRY(hipMemsetAsync(repeated, 0, sizeof(int64_t), c.s));
   }
   SCOUT_CHECK(prof_begin(c, PROF_CLASSIFY));
   if (m > 0)
@@ -1118,8 +1126,7 @@ int queue_repeat_count(Ctx& c, Channel& ch, int64_t lo, int64_t hi, int64_t* out
   DevGraph& g = c.g;
   int64_t m = hi - lo;
   uint64_t size = 0;
-  SCOUT_CHECK(pair_set_clear(c, m, &size));
-  SCOUT_TRY(hipMemsetAsync(out, 0, sizeof(int64_t), c.s));
+  SCOUT_CHECK(pair_set_clear(c, m, &size, out));
   if (m > 0) k_dup_count<<<grid_for(m), kBlock, 0, c.s>>>(ch.d_uv.p + 2 * lo, m, g.hkey.p, size - 1, out);
   return SCOUT_OK;
 }

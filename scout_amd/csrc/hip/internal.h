@@ -17,6 +17,7 @@
 // inserted row -- the order of core/FlashGraph.h:246-265.
 #pragma once
 
+#include <chrono>
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
@@ -359,6 +360,10 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, int32_t* P, int strategy, scou
 // array of the current graph.
 int compute_parents(Ctx& c, const int32_t* L, int32_t* P);
 int bfs_local(Ctx& c, Channel& ch, int strategy, scout_stats* st);
+// scout_repair_bfs / scout_repair_cc on one channel: the local repair, or a full recompute when
+// asked for or when the state is not exact for the graph before this batch; stats and timing
+int repair_bfs(Ctx& c, Channel& ch, int strategy, scout_stats* st);
+int repair_cc(Ctx& c, Channel& ch, int strategy, scout_stats* st);
 int cc_full(Ctx& c, int64_t* scanned);
 // Upper bound of the chunk tasks of a full CC recompute of the current graph.
 int64_t cc_full_task_bound(const Ctx& c);
@@ -447,12 +452,17 @@ struct TouchScope {
 };
 
 // Device time between two events recorded around a repair.
+// Time of a repair call on the host clock, from its first launch until the compute stream has
+// finished its work (ms() synchronises the stream, which returns at once when the call's last
+// read-back has already done so). The reference reports its stage times from host timers
+// around each batch (src/bfs/scout_bfs.cu:578-589).
 struct Span {
-  hipEvent_t a = nullptr, b = nullptr;
+  Ctx& c;
+  std::chrono::steady_clock::time_point t0;
+  explicit Span(Ctx& ctx) : c(ctx) {}
   int begin(hipStream_t s);
   int end(hipStream_t s);
   double ms();
-  ~Span();
 };
 
 }  // namespace scout

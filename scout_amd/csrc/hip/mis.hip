@@ -1513,7 +1513,7 @@ int mis_entry(scout_ctx* x, int channel, int32_t ninst, const int64_t* line_off,
       return SCOUT_ERR_ARG;
     }
   TouchScope touch(c);
-  Span span;
+  Span span(c);
   SCOUT_CHECK(span.begin(c.s));
   st->lines += ch->m;
   int64_t rb0 = c.readbacks;
