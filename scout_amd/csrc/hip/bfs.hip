@@ -1328,6 +1328,13 @@ __global__ void k_spec_gate(const int64_t* front, int64_t* box) {
   for (int k = 36; k <= 38; k++) box[k] = 0;
 }
 
+// The gate of the work queued behind a relax read-back (the CC scan under scout_commit):
+// the next round's frontier count, zero once the relax has ended.
+constexpr int kRelaxGateSlot = 35;
+__global__ void k_copy_gate(const int64_t* front, int64_t* gate) {
+  if (threadIdx.x == 0) *gate = *front;
+}
+
 // Zeroes two runs of counters.
 __global__ void k_zero2(int64_t* a, int na, int64_t* b, int nb) {
   int t = (int)threadIdx.x;
@@ -1423,6 +1430,8 @@ struct Relax {
   int group = kFirstGroup, first_group = kFirstGroup;
   int64_t done = 0;  // rounds queued so far
   int full_rounds = kFullRounds;
+  // work to queue behind each read-back, shut until the relax has ended (Ctx::after_relax)
+  const std::function<int(const int64_t*, bool*)>* ahead = nullptr;
 
   int queue_group(const int64_t* gate) {
     int32_t n = c.g.n;
@@ -1499,14 +1508,25 @@ struct Relax {
       SCOUT_CHECK(prof_end(c, PROF_BFS_RELAX, 0));
       // [8..21] this relax's counters; [4..7] and [36..38] come back too (the seed and
       // tentative-level counters of a local repair, which read them after the relax)
-      SCOUT_TRY(hipMemcpyAsync(c.h_box + 4, box + 4, 35 * sizeof(int64_t), hipMemcpyDeviceToHost, c.s));
-      SCOUT_TRY(hipStreamSynchronize(c.s));
-      c.readbacks++;
+      bool queued = false;
+      if (ahead && *ahead) {
+        SCOUT_CHECK(box_read_async(c, 4, 35));
+        k_copy_gate<<<1, 64, 0, c.s>>>(box + 8 + p, box + kRelaxGateSlot);
+        SCOUT_CHECK((*ahead)(box + kRelaxGateSlot, &queued));
+        SCOUT_CHECK(box_wait(c));
+      } else {
+        SCOUT_TRY(hipMemcpyAsync(c.h_box + 4, box + 4, 35 * sizeof(int64_t), hipMemcpyDeviceToHost, c.s));
+        SCOUT_TRY(hipStreamSynchronize(c.s));
+        c.readbacks++;
+      }
       if (std::max({c.h_box[10], c.h_box[16], c.h_box[18]}) > c.wb.cap) {
         set_error("bfs relax: chunk task list overflow");
         return SCOUT_ERR_STATE;
       }
-      if (c.h_box[8 + p] == 0 && (async || nseeds == 0 || lvl > max_seed)) break;
+      if (c.h_box[8 + p] == 0 && (async || nseeds == 0 || lvl > max_seed)) {
+        if (queued) c.cc_ahead_seq = c.apply_seq;  // the gate opened: what was queued ran
+        break;
+      }
       full_rounds = c.h_box[8 + p] > kTailFront ? kFullRounds : 0;
       // after a predicted first group the repair is near its end: start again from a short group
       group = group == first_group && first_group != kFirstGroup ? kFirstGroup : std::min(2 * group, kMaxGroup);
@@ -1761,6 +1781,9 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, int32_t* P, int strategy, scou
         std::max<int64_t>(predicted_depth + (int64_t)std::ceil(c.relax_bias), kFirstGroup), kMaxGroup);
   if (L == c.level && hint_relax > 0) relax_group = hint_relax;
   Relax relax{c, L, P, 0, 0, c.wa.p, 0, 0, true, box + 4, seed_cap};
+  // work that follows the relax (scout_commit's CC scan) may run behind its read-backs, on the
+  // maintained levels and when no training data is logged (the log reads the candidate list)
+  if (L == c.level && !logging && c.after_relax) relax.ahead = &c.after_relax;
   bool relax_queued = false;
   // Queues what follows the invalidation in the asynchronous relax: the tentative levels of the
   // invalidated vertices (their count from the device), the insertion seeds and the relax's

@@ -89,7 +89,8 @@ __global__ void k_cc_hook(GView g, int32_t* parent, int64_t* tasks, int64_t* tas
 }
 
 __global__ void k_cc_hook_tasks(GView g, int32_t* parent, const int64_t* tasks, const int64_t* task_count,
-                                int64_t task_cap, int64_t* scanned, int64_t* paths) {
+                                int64_t task_cap, int64_t* scanned, int64_t* paths, const int64_t* gate) {
+  if (gate && *gate != 0) return;
   int64_t nt = min(*task_count, task_cap);
   int64_t work = 0, ntask = 0;
   for (int64_t i = global_wave(); i < nt; i += total_waves()) {
@@ -149,7 +150,8 @@ __global__ void k_cc_flatten(int32_t* parent, int32_t n) {
 // Wave per listed long row v: union(v, w) for every neighbour w > v; rows longer than
 // kChunk become chunk tasks.
 __global__ void k_cc_unreached_hook(GView g, const int32_t* list, const int64_t* count, int32_t* par, int64_t* tasks,
-                                    int64_t* task_count, int64_t task_cap, int64_t* scanned) {
+                                    int64_t* task_count, int64_t task_cap, int64_t* scanned, const int64_t* gate) {
+  if (gate && *gate != 0) return;
   int64_t nl = *count;
   int64_t work = 0;
   for (int64_t i = global_wave(); i < nl; i += total_waves()) {
@@ -354,7 +356,8 @@ __global__ void k_cc_count_roots(const int32_t* label, int32_t n, int64_t* roots
 constexpr int kCoarseShift = 4;
 
 __global__ void k_cc_mark(const int32_t* uv, int64_t m, const int32_t* L, const int32_t* label, uint32_t* bits,
-                          uint32_t* coarse) {
+                          uint32_t* coarse, const int64_t* gate) {
+  if (gate && *gate != 0) return;
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= 2 * m) return;
   int32_t x = uv[i];
@@ -369,7 +372,8 @@ __global__ void k_cc_mark(const int32_t* uv, int64_t m, const int32_t* L, const 
 }
 
 // Zeroes count words, and the first nbox counters of box (one launch for both).
-__global__ void k_zero_u32(uint32_t* p, int64_t count, int64_t* box, int nbox) {
+__global__ void k_zero_u32(uint32_t* p, int64_t count, int64_t* box, int nbox, const int64_t* gate) {
+  if (gate && *gate != 0) return;
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < count) p[i] = 0;
   if (i < nbox) box[i] = 0;
@@ -380,7 +384,9 @@ __global__ void k_zero_u32(uint32_t* p, int64_t count, int64_t* box, int nbox) {
 // listed for relabelling with par[v] = v. kept counts the roots of untouched components.
 // Four consecutive vertices per thread (one 16-byte load of levels and of labels).
 __global__ void k_cc_collect(const int32_t* L, int32_t* label, const uint32_t* bits, const uint32_t* coarse, int32_t n,
-                             int32_t* par, int32_t* list, int64_t* count, int64_t cap, int64_t* kept) {
+                             int32_t* par, int32_t* list, int64_t* count, int64_t cap, int64_t* kept,
+                             const int64_t* gate) {
+  if (gate && *gate != 0) return;
   int64_t roots = 0;
   // listed vertices are sparse and spread over the whole range: staged per wave in LDS
   constexpr int kSeg = 512;
@@ -454,7 +460,8 @@ __global__ void k_cc_collect(const int32_t* L, int32_t* label, const uint32_t* b
 // Thread per listed vertex: union with neighbours w > v for rows up to 32 entries; longer
 // rows are listed for k_cc_unreached_hook.
 __global__ void k_cc_union_list(GView g, const int32_t* list, const int64_t* count, int32_t* par, int32_t* longs,
-                                int64_t* nlong, int64_t cap, int64_t* scanned) {
+                                int64_t* nlong, int64_t cap, int64_t* scanned, const int64_t* gate) {
+  if (gate && *gate != 0) return;
   int64_t nl = *count;
   int64_t work = 0;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nl; i += (int64_t)gridDim.x * blockDim.x) {
@@ -477,7 +484,8 @@ __global__ void k_cc_union_list(GView g, const int32_t* list, const int64_t* cou
 
 // Thread per listed vertex: its label is its set's root (the component minimum).
 __global__ void k_cc_final_list(const int32_t* list, const int64_t* count, int32_t* par, int32_t* label,
-                                int64_t* roots) {
+                                int64_t* roots, const int64_t* gate) {
+  if (gate && *gate != 0) return;
   int64_t nl = *count;
   int64_t cnt = 0;
   for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < nl; base += (int64_t)gridDim.x * blockDim.x) {
@@ -502,7 +510,7 @@ int cc_full_into(Ctx& c, int32_t* label, int64_t* tasks, int64_t task_cap, int64
   k_cc_zero_counters<<<1, 64, 0, c.s>>>(counters, 2);
   k_cc_hook<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), label, tasks, counters, task_cap, counters + 1);
   k_cc_hook_tasks<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), label, tasks, counters, task_cap, counters + 1,
-                                                  c.d_paths);
+                                                  c.d_paths, nullptr);
   k_cc_flatten<<<grid_for(n), kBlock, 0, c.s>>>(label, n);
   SCOUT_TRY(hipGetLastError());
   return SCOUT_OK;
@@ -561,7 +569,8 @@ int cc_full_streamed(Ctx& c, int64_t* scanned) {
     gv.bsamp = nullptr;
     k_cc_zero_counters<<<1, 64, 0, c.s>>>(c.d_box, 1);
     k_cc_hook_range<<<kListGrid, kBlock, 0, c.s>>>(gv, v0, v1, c.label, tasks.p, c.d_box, task_cap, c.d_box + 1);
-    k_cc_hook_tasks<<<kListGrid, kBlock, 0, c.s>>>(gv, c.label, tasks.p, c.d_box, task_cap, c.d_box + 1, c.d_paths);
+    k_cc_hook_tasks<<<kListGrid, kBlock, 0, c.s>>>(gv, c.label, tasks.p, c.d_box, task_cap, c.d_box + 1, c.d_paths,
+                                                    nullptr);
     SCOUT_TRY(hipEventRecord(ev[4 * k + 2], c.s));
   }
   k_cc_flatten<<<grid_for(n), kBlock, 0, c.s>>>(c.label, n);
@@ -644,11 +653,13 @@ int cc_walk(Ctx& c, Channel& ch, const int32_t* L, scout_stats* st) {
   }
   c.hint_cc = (int)rounds;
   // union-find over the unreached components' rows, then their roots as labels
-  k_cc_union_list<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), c.la.p, box, par, c.lb.p, box + 5, cap, box + 2);
+  k_cc_union_list<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), c.la.p, box, par, c.lb.p, box + 5, cap, box + 2,
+                                                  nullptr);
   k_cc_unreached_hook<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), c.lb.p, box + 5, par, c.wb.p, box + 1, c.wb.cap,
-                                                      box + 2);
-  k_cc_hook_tasks<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), par, c.wb.p, box + 1, c.wb.cap, box + 2, c.d_paths);
-  k_cc_final_list<<<kListGrid, kBlock, 0, c.s>>>(c.la.p, box, par, c.label, box + 3);
+                                                      box + 2, nullptr);
+  k_cc_hook_tasks<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), par, c.wb.p, box + 1, c.wb.cap, box + 2, c.d_paths,
+                                                  nullptr);
+  k_cc_final_list<<<kListGrid, kBlock, 0, c.s>>>(c.la.p, box, par, c.label, box + 3, nullptr);
   SCOUT_CHECK(prof_end(c, PROF_CC_EXPAND, 0));
   SCOUT_CHECK(box_read(c, 12));
   if (c.h_box[1] > c.wb.cap) {
@@ -683,7 +694,9 @@ int cc_walk(Ctx& c, Channel& ch, const int32_t* L, scout_stats* st) {
 
 // Scan mode: one pass over every vertex lists the affected ones (cheaper than the walk's
 // dependent rounds when the batch touches a large share of the graph).
-int cc_scan(Ctx& c, Channel& ch, const int32_t* L, scout_stats* st) {
+// Queues the scan listing and the union-find (every kernel shut while *gate is non-zero, when
+// a gate is given: the scan queued ahead of the BFS relax's last read-back).
+int cc_scan_queue(Ctx& c, Channel& ch, const int32_t* L, const int64_t* gate) {
   int32_t n = c.g.n;
   SCOUT_CHECK(ensure_scratch(c, &c.cc_owner, -1));
   SCOUT_CHECK(c.la.reserve((int64_t)n + 1));
@@ -696,19 +709,27 @@ int cc_scan(Ctx& c, Channel& ch, const int32_t* L, scout_stats* st) {
   int32_t* par = c.cc_owner;
   int64_t* box = c.d_box;
   // box: [0] listed vertices, [1] chunk tasks, [2] scanned entries, [3] roots among the listed,
-  // [4] roots of untouched components, [5] long rows
+  // [4] roots of untouched components, [5] long rows (zeroed with the bitmaps)
   SCOUT_CHECK(prof_begin(c, PROF_CC_EXPAND));
-  k_zero_u32<<<grid_for(std::max<int64_t>(words + cwords, 6)), kBlock, 0, c.s>>>(bits, words + cwords, box, 6);
-  if (ch.m > 0) k_cc_mark<<<grid_for(2 * ch.m), kBlock, 0, c.s>>>(ch.d_uv.p, ch.m, L, c.label, bits, coarse);
+  k_zero_u32<<<grid_for(std::max<int64_t>(words + cwords, 6)), kBlock, 0, c.s>>>(bits, words + cwords, box, 6, gate);
+  if (ch.m > 0)
+    k_cc_mark<<<grid_for(2 * ch.m), kBlock, 0, c.s>>>(ch.d_uv.p, ch.m, L, c.label, bits, coarse, gate);
   SCOUT_CHECK(prof_launch(c, PROF_K_CC_COLLECT, k_cc_collect, dim3(kListGrid), dim3(kBlock), L, c.label, bits, coarse,
-                          n, par, c.la.p, box, (int64_t)n + 1, box + 4));
+                          n, par, c.la.p, box, (int64_t)n + 1, box + 4, gate));
   k_cc_union_list<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), c.la.p, box, par, c.lb.p, box + 5, (int64_t)n + 1,
-                                                  box + 2);
+                                                  box + 2, gate);
   k_cc_unreached_hook<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), c.lb.p, box + 5, par, c.wb.p, box + 1, c.wb.cap,
-                                                      box + 2);
-  k_cc_hook_tasks<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), par, c.wb.p, box + 1, c.wb.cap, box + 2, c.d_paths);
-  k_cc_final_list<<<kListGrid, kBlock, 0, c.s>>>(c.la.p, box, par, c.label, box + 3);
+                                                      box + 2, gate);
+  k_cc_hook_tasks<<<kListGrid, kBlock, 0, c.s>>>(c.g.view(), par, c.wb.p, box + 1, c.wb.cap, box + 2, c.d_paths,
+                                                  gate);
+  k_cc_final_list<<<kListGrid, kBlock, 0, c.s>>>(c.la.p, box, par, c.label, box + 3, gate);
   SCOUT_CHECK(prof_end(c, PROF_CC_EXPAND, 0));
+  return SCOUT_OK;
+}
+
+// Reads back the queued scan's counters: overflow check, component count and statistics.
+int cc_scan_finish(Ctx& c, Channel& ch, scout_stats* st) {
+  int32_t n = c.g.n;
   SCOUT_CHECK(box_read(c, 6));
   if (c.h_box[1] > c.wb.cap) {
     set_error("cc: chunk task overflow");
@@ -730,11 +751,38 @@ int cc_scan(Ctx& c, Channel& ch, const int32_t* L, scout_stats* st) {
   return SCOUT_OK;
 }
 
+int cc_scan(Ctx& c, Channel& ch, const int32_t* L, scout_stats* st) {
+  SCOUT_CHECK(cc_scan_queue(c, ch, L, nullptr));
+  return cc_scan_finish(c, ch, st);
+}
+
+// The listing a local repair of this batch takes: the walk lists the endpoints' components;
+// under the learned strategy the split head adds the vertices it expects the batch to cut off
+// vertex 0's component (the walk must list them all, the scan's cost does not depend on them).
+bool cc_walks(const Ctx& c, const Channel& ch, int strategy, bool levels_maintained) {
+  int64_t est = 2 * ch.m;
+  if (strategy == SCOUT_STRATEGY_LEARNED && c.pred_cut_seq == c.apply_seq && levels_maintained) est += c.pred_cut;
+  return c.cc_mode == SCOUT_CC_WALK || (c.cc_mode == SCOUT_CC_AUTO && est * kWalkRatio <= (int64_t)c.g.n);
+}
+
+// True when repair_cc of this batch will scan with the maintained BFS levels from vertex 0
+// (the local repair on exact labels): the scan may then be queued behind the BFS relax.
+bool cc_scan_ahead_ok(const Ctx& c, const Channel& ch, int strategy) {
+  return c.label && c.level && c.source == 0 && c.g.n > 0 && strategy != SCOUT_STRATEGY_RECOMPUTE &&
+         c.label_seq == c.apply_seq - 1 && !cc_walks(c, ch, strategy, true);
+}
+
 // Exact labels after the batch: the BFS levels from vertex 0 decide vertex 0's component;
 // the rest is a union-find over the unreached subgraph.
 int cc_local(Ctx& c, Channel& ch, int strategy, scout_stats* st) {
   int32_t n = c.g.n;
   if (n == 0) return SCOUT_OK;
+  if (c.cc_ahead_seq == c.apply_seq) {
+    // the scan ran behind the BFS relax of this batch (scout_commit)
+    c.cc_ahead_seq = -1;
+    st->cc_mode = SCOUT_CC_SCAN;
+    return cc_scan_finish(c, ch, st);
+  }
   const int32_t* L = nullptr;
   if (c.level && c.source == 0 && c.level_seq == c.apply_seq) {
     L = c.level;  // this batch's BFS repair from vertex 0 already ran on this context
@@ -757,12 +805,7 @@ int cc_local(Ctx& c, Channel& ch, int strategy, scout_stats* st) {
     st->fallback += bst.fallback;
     L = c.cc_level;
   }
-  // the walk lists the endpoints' components; under the learned strategy the split head adds
-  // the vertices it expects the batch to cut off vertex 0's component (the walk must list them
-  // all, the scan's cost does not depend on them)
-  int64_t est = 2 * ch.m;
-  if (strategy == SCOUT_STRATEGY_LEARNED && c.pred_cut_seq == c.apply_seq && L == c.level) est += c.pred_cut;
-  bool walk = c.cc_mode == SCOUT_CC_WALK || (c.cc_mode == SCOUT_CC_AUTO && est * kWalkRatio <= (int64_t)n);
+  bool walk = cc_walks(c, ch, strategy, L == c.level);
   st->cc_mode = walk ? SCOUT_CC_WALK : SCOUT_CC_SCAN;
   return walk ? cc_walk(c, ch, L, st) : cc_scan(c, ch, L, st);
 }

@@ -496,6 +496,7 @@ extern "C" int scout_color_autotune(scout_ctx* x, int channel, double* best_ms) 
           Span span(c);
           SCOUT_CHECK(span.begin(c.s));
           SCOUT_CHECK(color_repair(c, *ch, &st));
+          SCOUT_TRY(hipStreamSynchronize(c.s));
           SCOUT_CHECK(span.end(c.s));
           double ms = span.ms();
           if (rep == 1 && ms < best) {

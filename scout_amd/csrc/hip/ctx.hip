@@ -135,7 +135,7 @@ int Span::begin(hipStream_t s) {
   return SCOUT_OK;
 }
 int Span::end(hipStream_t s) {
-  SCOUT_TRY(hipStreamSynchronize(s));
+  (void)s;
   return SCOUT_OK;
 }
 double Span::ms() {
@@ -594,7 +594,17 @@ int scout_commit(scout_ctx* x, int channel, int strategy, scout_stats* bfs_st, s
   SCOUT_CHECK(channel_wait(c.s, *ch));
   SCOUT_CHECK(graph_classify(c, *ch, SCOUT_CLASSIFY_CURRENT));
   SCOUT_CHECK(graph_apply(c, *ch, 0));
-  SCOUT_CHECK(repair_bfs(c, *ch, strategy, bfs_st));
+  // The CC scan reads only the final levels: it is queued behind each read-back of the BFS
+  // relax, shut until the relax has ended, so the device goes from one repair to the next
+  // without waiting for the host.
+  Channel* chp = ch;
+  c.after_relax = [&c, chp, strategy](const int64_t* gate, bool* queued) -> int {
+    *queued = cc_scan_ahead_ok(c, *chp, strategy);
+    return *queued ? cc_scan_queue(c, *chp, c.level, gate) : SCOUT_OK;
+  };
+  int rc = repair_bfs(c, *ch, strategy, bfs_st);
+  c.after_relax = nullptr;
+  SCOUT_CHECK(rc);
   SCOUT_CHECK(repair_cc(c, *ch, strategy, cc_st));
   return channel_used(c.s, *ch);
 }

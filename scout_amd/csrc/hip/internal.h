@@ -18,6 +18,7 @@
 #pragma once
 
 #include <chrono>
+#include <functional>
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
@@ -280,6 +281,11 @@ struct Ctx {
   // small mailbox read back by the host after a round (kBoxSlots entries)
   int64_t* h_box = nullptr;    // pinned
   hipEvent_t box_ev = nullptr;  // marks the last box_read_async copy
+  // Set by scout_commit while its BFS repair runs: queues work that follows the BFS relax behind
+  // each relax read-back, shut by the gate (zero once the relax has ended); *queued tells
+  // whether it queued anything. cc_ahead_seq: the batch whose CC scan ran that way.
+  std::function<int(const int64_t* gate, bool* queued)> after_relax;
+  int64_t cc_ahead_seq = -1;
   int64_t* d_box = nullptr;
   // cumulative counts of the long-row code paths (kPathCount entries after the mailbox)
   int64_t* d_paths = nullptr;
@@ -374,6 +380,10 @@ int cc_full_into(Ctx& c, int32_t* label, int64_t* tasks, int64_t task_cap, int64
 // (once per device; called by scout_ctx_create with `dev` current).
 int noalloc_init(int dev);
 int cc_local(Ctx& c, Channel& ch, int strategy, scout_stats* st);
+// The CC scan's kernels, shut while *gate is non-zero (gate may be null), and whether
+// repair_cc of this batch will take that scan on the maintained levels from vertex 0.
+int cc_scan_queue(Ctx& c, Channel& ch, const int32_t* L, const int64_t* gate);
+bool cc_scan_ahead_ok(const Ctx& c, const Channel& ch, int strategy);
 void cc_release(Ctx& c);
 void mis_release(Ctx& c);
 void stream_release(Ctx& c);
@@ -452,10 +462,11 @@ struct TouchScope {
 };
 
 // Device time between two events recorded around a repair.
-// Time of a repair call on the host clock, from its first launch until the compute stream has
-// finished its work (ms() synchronises the stream, which returns at once when the call's last
-// read-back has already done so). The reference reports its stage times from host timers
-// around each batch (src/bfs/scout_bfs.cu:578-589).
+// Time of a repair call on the host clock, from its first launch to its end. Every repair ends
+// with a read-back that waits for its device work (work queued behind that read-back for the
+// next repair, scout_commit's CC scan, is not waited for); a caller timing other work
+// synchronises before end(). The reference reports its stage times from host timers around
+// each batch (src/bfs/scout_bfs.cu:578-589).
 struct Span {
   Ctx& c;
   std::chrono::steady_clock::time_point t0;
