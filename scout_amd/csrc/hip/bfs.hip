@@ -582,10 +582,13 @@ __global__ void k_del_candidates(const int32_t* uv, const uint8_t* ins, int64_t 
 // first round's list, each marked pending (stored as pend_of(level)); a vertex listed twice
 // is kept once. Runs after every reader of the plain candidate levels (the candidate pass and
 // the scheduler's features).
-__global__ void k_mark_pending(const int32_t* cand, const int64_t* ncand, int32_t* level, int32_t* out,
-                               int64_t* out_count, int64_t cap) {
+// n_host: the candidate count when the host has it (-1: read *ncand, queued before the host has
+// read it back).
+__global__ void k_mark_pending(const int32_t* cand, const int64_t* ncand, int64_t n_host, int32_t* level,
+                               int32_t* out, int64_t* out_count, int64_t cap, const int64_t* gate) {
+  if (gate && *gate != 0) return;
   constexpr int K = kLinesPerThread;
-  int64_t n = *ncand;
+  int64_t n = n_host >= 0 ? n_host : *ncand;
   int64_t first = (int64_t)blockIdx.x * blockDim.x * K + threadIdx.x;
   bool take[K];
   int32_t cs[K];
@@ -622,7 +625,9 @@ __global__ void k_invalidate_short(GView g, int32_t* level, int32_t* parent, con
                                    int64_t* next_count, int64_t cap, int32_t* stamp, int32_t epoch, int32_t* small,
                                    int64_t* small_count, int32_t* medium, int64_t* medium_count, int32_t* longs,
                                    int32_t* long_lvl, int64_t* long_count, int64_t* tasks, int64_t* task_count,
-                                   int64_t task_cap, int64_t* scanned, int64_t* paths, Attrib at) {
+                                   int64_t task_cap, int64_t* scanned, int64_t* paths, Attrib at,
+                                   const int64_t* gate) {
+  if (gate && *gate != 0) return;
   int64_t nc = *cur_count;
   if (A) ncand = 0;
   if (!A && cb && ncand > 0) {
@@ -747,7 +752,9 @@ __global__ void k_invalidate(GView g, int32_t* level, int32_t* parent, const int
                              const int64_t* task_count, int64_t task_cap, int32_t* supported, int32_t supepoch,
                              int32_t lvl, int32_t* inval, int64_t* inval_count, int64_t inval_cap, int32_t* next,
                              int64_t* next_count, int64_t cap, int32_t* stamp, int32_t epoch, int64_t* scanned,
-                             int64_t* own, int64_t* paths, bool sorted, int32_t src, Attrib ab) {
+                             int64_t* own, int64_t* paths, bool sorted, int32_t src, Attrib ab,
+                             const int64_t* gate) {
+  if (gate && *gate != 0) return;
   // neighbours one level down met while searching for support: the only possible children
   constexpr int kKids = 1024;
   constexpr int kInvSeg = 64;
@@ -1035,7 +1042,8 @@ __global__ void k_inval_push(GView g, int32_t* level, const int32_t* parent, int
                              int64_t task_cap, const int32_t* supported, int32_t supepoch, int32_t* inval,
                              int64_t* inval_count, int64_t inval_cap, int32_t* next, int64_t* next_count, int64_t cap,
                              int32_t* stamp, int32_t epoch, int64_t* scanned, int64_t* paths, Attrib ab,
-                             RoundBook book) {
+                             RoundBook book, const int64_t* gate) {
+  if (gate && *gate != 0) return;
   constexpr int kInvSeg = 64;
   __shared__ int32_t seg_all[kWavesPerBlock * kNextSeg];
   __shared__ int32_t inv_all[kWavesPerBlock * kInvSeg];
@@ -1336,7 +1344,8 @@ __global__ void k_copy_gate(const int64_t* front, int64_t* gate) {
 }
 
 // Zeroes two runs of counters.
-__global__ void k_zero2(int64_t* a, int na, int64_t* b, int nb) {
+__global__ void k_zero2(int64_t* a, int na, int64_t* b, int nb, const int64_t* gate = nullptr) {
+  if (gate && *gate != 0) return;
   int t = (int)threadIdx.x;
   if (t < na) a[t] = 0;
   if (t < nb) b[t] = 0;
@@ -1659,11 +1668,16 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, int32_t* P, int strategy, scou
   int32_t cmin = 1, cmax = 0;
   // zeroed counters; in the asynchronous order every candidate marked pending into the first
   // round's list (the count comes from the device; there are at most m candidates)
+  // rgate: the gate of rounds queued before a deferred graph update is settled; ncand_host: the
+  // candidate count once read back (the device slot may then hold a host-side apply's counter)
+  const int64_t* rgate = nullptr;
+  int64_t ncand_host = -1;
   auto inval_begin = [&]() {
     sepoch = next_epoch(c);
-    k_zero2<<<1, 64, 0, c.s>>>(box + 8, 16, box + 28, 5);
+    k_zero2<<<1, 64, 0, c.s>>>(box + 8, 16, box + 28, 5, rgate);
     if (async && m > 0)
-      k_mark_pending<<<grid_for(m, kBlock * kLinesPerThread), kBlock, 0, c.s>>>(c.la.p, box, L, cur, box + 8, cap);
+      k_mark_pending<<<grid_for(m, kBlock * kLinesPerThread), kBlock, 0, c.s>>>(c.la.p, box, ncand_host, L, cur,
+                                                                                box + 8, cap, rgate);
   };
   // queues one round
   auto inval_round = [&]() -> int {
@@ -1687,29 +1701,29 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, int32_t* P, int strategy, scou
                               dim3(kBlock), c.g.view(), L, P, cur, bcur, cand, nlevel, cb, box + 18, lvl, c.ld.p,
                               binval, nxt, bnext, cap, c.stamp, sepoch, c.lh.p, small_of[p], c.le.p, medium_of[p],
                               c.lf.p, c.lg.p, longs_of[p], c.wb.p, tasks_of[p], c.wb.cap, box + 23, c.d_paths,
-                              ab));
+                              ab, rgate));
       SCOUT_CHECK(prof_launch(c, PROF_BFS_INVALIDATE_KERNEL, k_invalidate<true>, dim3(ig), dim3(kBlock),
                               c.g.view(), L, P, c.lh.p, small_of[p], c.le.p, medium_of[p], c.lf.p, c.lg.p, c.wb.p, tasks_of[p],
                               c.wb.cap, c.aux, supepoch, lvl, c.ld.p, binval, cap, nxt, bnext, cap, c.stamp,
-                              sepoch, bscan, box + 21, c.d_paths, c.g.sorted, src, ab));
+                              sepoch, bscan, box + 21, c.d_paths, c.g.sorted, src, ab, rgate));
       SCOUT_CHECK(prof_launch(c, PROF_K_INVAL_PUSH, k_inval_push<true>, dim3(ig), dim3(kBlock),
                               c.g.view(), L, P, lvl, c.lf.p, c.lg.p, c.wb.p, tasks_of[p], c.wb.cap, c.aux,
                               supepoch, c.ld.p, binval, cap, nxt, bnext, cap, c.stamp, sepoch, box + 15,
-                              c.d_paths, ab, book));
+                              c.d_paths, ab, book, rgate));
     } else {
       SCOUT_CHECK(prof_launch(c, PROF_K_INVALIDATE_SHORT, k_invalidate_short<false>, dim3(ig),
                               dim3(kBlock), c.g.view(), L, P, cur, bcur, cand, nlevel, cb, box + 18, lvl, c.ld.p,
                               binval, nxt, bnext, cap, c.stamp, sepoch, c.lh.p, small_of[p], c.le.p, medium_of[p],
                               c.lf.p, c.lg.p, longs_of[p], c.wb.p, tasks_of[p], c.wb.cap, box + 23, c.d_paths,
-                              ab));
+                              ab, rgate));
       SCOUT_CHECK(prof_launch(c, PROF_BFS_INVALIDATE_KERNEL, k_invalidate<false>, dim3(ig),
                               dim3(kBlock), c.g.view(), L, P, c.lh.p, small_of[p], c.le.p, medium_of[p], c.lf.p, c.lg.p, c.wb.p,
                               tasks_of[p], c.wb.cap, c.aux, supepoch, lvl, c.ld.p, binval, cap, nxt, bnext, cap,
-                              c.stamp, sepoch, bscan, box + 21, c.d_paths, c.g.sorted, src, ab));
+                              c.stamp, sepoch, bscan, box + 21, c.d_paths, c.g.sorted, src, ab, rgate));
       SCOUT_CHECK(prof_launch(c, PROF_K_INVAL_PUSH, k_inval_push<false>, dim3(ig), dim3(kBlock),
                               c.g.view(), L, P, lvl, c.lf.p, c.lg.p, c.wb.p, tasks_of[p], c.wb.cap, c.aux,
                               supepoch, c.ld.p, binval, cap, nxt, bnext, cap, c.stamp, sepoch, box + 15,
-                              c.d_paths, ab, book));
+                              c.d_paths, ab, book, rgate));
     }
     lvl++;
     std::swap(cur, nxt);
@@ -1719,30 +1733,45 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, int32_t* P, int strategy, scou
   // The asynchronous order takes nothing from the host in its first rounds: they are queued
   // behind the read-back of the candidate counts and predictions, and run while the host
   // reads those.
+  // A graph update still to settle (scout_commit) gates the early rounds, which read rows; it
+  // settles after the counts below are taken, and if it was not complete on the device the
+  // early rounds did not run and are queued again.
   bool early = async && m > 0;
   int queued = 0;  // rounds of the first group already queued
   if (early) {
     SCOUT_CHECK(box_read_async(c, 0, learned ? 28 : 8));
     SCOUT_CHECK(prof_begin(c, PROF_BFS_INVALIDATE));
+    if (c.apply_pend.on) rgate = box + kApplyGateSlot;
     inval_begin();
     for (; queued < kFirstGroup; queued++) SCOUT_CHECK(inval_round());
+    rgate = nullptr;
     SCOUT_CHECK(box_wait(c));
   } else {
     SCOUT_CHECK(box_read(c, learned ? 28 : 8));
   }
   ncand = c.h_box[0];
+  ncand_host = ncand;
   cmin = (int32_t)c.h_box[1];
   cmax = (int32_t)c.h_box[2];
   st->deletions += c.h_box[3];
+  int64_t pred_box[4] = {c.h_box[24], c.h_box[25], c.h_box[26], c.h_box[27]};
+  if (c.apply_pend.on) {
+    bool complete = true;
+    SCOUT_CHECK(apply_settle(c, &complete));
+    if (!complete && early) {
+      early = false;
+      queued = 0;
+    }
+  }
   int64_t nx = 0, scanned = 0, rounds = 0;
   int64_t nsample = std::min<int64_t>(ncand, sample);
   int first_group = kFirstGroup;
   int64_t predicted_last = -1, predicted_depth = -1;
   if (learned && ncand > 0) {
     // predictions size and order the work; the plain search below decides every level
-    double predicted = (double)c.h_box[24] * (double)ncand / (double)nsample;
+    double predicted = (double)pred_box[0] * (double)ncand / (double)nsample;
     // the split head's expected cut-off, kept for this batch's CC repair
-    c.pred_cut = (int64_t)((double)c.h_box[27] / 16.0 * (double)ncand / (double)nsample);
+    c.pred_cut = (int64_t)((double)pred_box[3] / 16.0 * (double)ncand / (double)nsample);
     c.pred_cut_seq = c.apply_seq;
     if (predicted > kRecomputeFraction * (double)n) {
       // cascades over a large part of the graph: one full BFS costs less than local repair
@@ -1759,11 +1788,11 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, int32_t* P, int strategy, scou
       // sampled candidates; the deepest cascade of the whole batch is rarely among them, so
       // the prediction is corrected by the running mean of its error on earlier batches
       // (rounded up: an empty extra round costs less than a second read-back).
-      predicted_depth = std::max<int64_t>(-c.h_box[26], 1) + 1;
+      predicted_depth = std::max<int64_t>(-pred_box[2], 1) + 1;
       int64_t want = predicted_depth + (int64_t)std::ceil(c.pred_bias);
       first_group = (int)std::min<int64_t>(std::max<int64_t>(want, kFirstGroup), kMaxGroup);
     } else {
-      predicted_last = std::max<int64_t>(c.h_box[25] - (int64_t)std::floor(c.pred_over), cmax);
+      predicted_last = std::max<int64_t>(pred_box[1] - (int64_t)std::floor(c.pred_over), cmax);
       first_group = (int)std::min<int64_t>(std::max<int64_t>(predicted_last - cmin + 1, kFirstGroup), kMaxGroup);
     }
   }
@@ -1867,7 +1896,7 @@ int bfs_local_on(Ctx& c, Channel& ch, int32_t* L, int32_t* P, int strategy, scou
       c.pred_bias = 0.75 * c.pred_bias + 0.25 * (double)(c.h_box[20] - predicted_depth);
     } else if (predicted_last >= 0) {
       int64_t last = cmin + c.h_box[20] - 1;  // deepest invalidation level
-      double over = std::max<double>(0.0, (double)(c.h_box[25] - last));
+      double over = std::max<double>(0.0, (double)(pred_box[1] - last));
       c.pred_over = 0.75 * c.pred_over + 0.25 * over;
     }
     // per candidate its id, level, row offsets and queue entry (40 B); per scanned entry the
@@ -1982,6 +2011,8 @@ int repair_bfs(Ctx& c, Channel& ch, int strategy, scout_stats* st) {
   // a local repair needs levels that were exact for the graph before this batch
   bool stale = c.level_seq != c.apply_seq - 1;
   if (strategy == SCOUT_STRATEGY_RECOMPUTE || stale) {
+    bool complete;
+    SCOUT_CHECK(apply_settle(c, &complete));  // a full BFS reads every row
     st->fallback = 1;
     SCOUT_CHECK(bfs_full(c, c.source, &st->rounds, &st->edges_scanned));
   } else {

@@ -259,6 +259,7 @@ int scout_ctx_create(int gpu_id, const scout_budget* budget, scout_ctx** out) {
       hipStreamCreateWithFlags(&c.cs, hipStreamNonBlocking) != hipSuccess ||
       hipHostMalloc((void**)&c.h_box, (kBoxSlots + kPathCount) * sizeof(int64_t), hipHostMallocDefault) !=
           hipSuccess ||
+      hipHostMalloc((void**)&c.h_apply, kBoxSlots * sizeof(int64_t), hipHostMallocDefault) != hipSuccess ||
       hipMalloc((void**)&c.d_box, (kBoxSlots + kPathCount) * sizeof(int64_t)) != hipSuccess ||
       hipMemset(c.d_box, 0, (kBoxSlots + kPathCount) * sizeof(int64_t)) != hipSuccess) {
     set_error("context set-up failed");
@@ -311,6 +312,8 @@ int scout_ctx_destroy(scout_ctx* x) {
   c.cc_bits.release();
   c.tmp.release();
   if (c.box_ev) (void)hipEventDestroy(c.box_ev);
+  if (c.apply_ev) (void)hipEventDestroy(c.apply_ev);
+  if (c.h_apply) (void)hipHostFree(c.h_apply);
   if (c.h_box) (void)hipHostFree(c.h_box);
   if (c.d_box) (void)hipFree(c.d_box);
   if (c.s) (void)hipStreamDestroy(c.s);
@@ -593,7 +596,8 @@ int scout_commit(scout_ctx* x, int channel, int strategy, scout_stats* bfs_st, s
   }
   SCOUT_CHECK(channel_wait(c.s, *ch));
   SCOUT_CHECK(graph_classify(c, *ch, SCOUT_CLASSIFY_CURRENT));
-  SCOUT_CHECK(graph_apply(c, *ch, 0));
+  // the update's counters are read with the BFS repair's first read-back, which settles it
+  SCOUT_CHECK(graph_apply_deferred(c, *ch));
   // The CC scan reads only the final levels: it is queued behind each read-back of the BFS
   // relax, shut until the relax has ended, so the device goes from one repair to the next
   // without waiting for the host.
@@ -604,7 +608,10 @@ int scout_commit(scout_ctx* x, int channel, int strategy, scout_stats* bfs_st, s
   };
   int rc = repair_bfs(c, *ch, strategy, bfs_st);
   c.after_relax = nullptr;
+  bool complete;
+  int rs = apply_settle(c, &complete);  // settled by the repair on every path; kept for errors
   SCOUT_CHECK(rc);
+  SCOUT_CHECK(rs);
   SCOUT_CHECK(repair_cc(c, *ch, strategy, cc_st));
   return channel_used(c.s, *ch);
 }

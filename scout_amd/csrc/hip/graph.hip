@@ -555,6 +555,17 @@ __global__ void k_move_rows(AHead* ahead, int32_t* acol, int32_t* aown, const in
 // checks, the multi-entry count and the flag of a multi list k_group_small leaves to the host.
 constexpr int kRepeatSlot = 48, kMultiSlot = 51, kSortSlot = 52;
 
+// Zeroes mailbox slots [4..7] (seed counters) after a host-side apply finish.
+__global__ void k_apply_rezero(int64_t* box) {
+  if (threadIdx.x >= 4 && threadIdx.x < 8) box[threadIdx.x] = 0;
+}
+
+// Non-zero when the apply is not complete on the device: the range repeats a pair (*skip) or
+// holds more multi entries than one block groups (box[kSortSlot]).
+__global__ void k_apply_gate(const int64_t* box, const int64_t* skip, int64_t* gate) {
+  if (threadIdx.x == 0) *gate = (skip ? *skip : 0) | box[kSortSlot];
+}
+
 // First kernel of an apply: zeroes the counts and flags above and sets the pool top.
 __global__ void k_apply_begin(int64_t* box, int64_t pool_top) {
   int t = (int)threadIdx.x;
@@ -789,8 +800,13 @@ MoveList move_list(Ctx& c) {
 // the placement kernels, with one read-back at the end. With `skip` (a device count of repeated
 // pairs in the range, read back with the rest), nothing is applied when the count is not zero,
 // and *skipped says so.
+int apply_counts(Ctx& c, Channel& ch, int64_t lo, int64_t hi, int conditional, const int64_t* h, int skip_slot,
+                 bool* skipped);
+
+// defer: the counters are not read back here; they go to c.h_apply with the work queued next,
+// and apply_settle finishes the apply (scout_commit's graph update).
 int apply_range(Ctx& c, Channel& ch, int64_t lo, int64_t hi, int conditional, const int64_t* skip = nullptr,
-                bool* skipped = nullptr) {
+                bool* skipped = nullptr, bool defer = false) {
   DevGraph& g = c.g;
   int64_t m = hi - lo;
   int64_t cap = 4 * m + 16;
@@ -844,25 +860,52 @@ int apply_range(Ctx& c, Channel& ch, int64_t lo, int64_t hi, int conditional, co
     k_move_rows<<<kListGrid, kBlock, 0, c.s>>>(g.ahead.p, g.acol.p, g.aown.p, g.ndst.p,
                                                g.ndst2.p, mv);
   }
+  if (defer) {
+    // the counters travel to c.h_apply behind the placement; the gate tells the kernels queued
+    // before the host has read them whether the apply is complete on the device
+    SCOUT_TRY(hipMemcpyAsync(c.h_apply, c.d_box, (size_t)(kSortSlot + 1) * sizeof(int64_t), hipMemcpyDeviceToHost,
+                             c.s));
+    if (!c.apply_ev) SCOUT_TRY(hipEventCreateWithFlags(&c.apply_ev, hipEventDisableTiming));
+    SCOUT_TRY(hipEventRecord(c.apply_ev, c.s));
+    k_apply_gate<<<1, 64, 0, c.s>>>(c.d_box, skip, c.d_box + kApplyGateSlot);
+    c.apply_pend = ApplyPending{true, &ch, lo, hi, conditional, skip ? (int)(skip - c.d_box) : -1};
+    return SCOUT_OK;
+  }
   SCOUT_CHECK(box_read(c, kSortSlot + 1));
-  if (skip && c.h_box[skip - c.d_box] != 0) {
+  return apply_counts(c, ch, lo, hi, conditional, c.h_box, skip ? (int)(skip - c.d_box) : -1, skipped);
+}
+
+// The host side of apply_range once its counters h (mailbox slots 0..kSortSlot) are on the
+// host: the checks, the bookkeeping and, for more multi entries than one block groups, the
+// library sort and placement.
+int apply_counts(Ctx& c, Channel& ch, int64_t lo, int64_t hi, int conditional, const int64_t* h, int skip_slot,
+                 bool* skipped) {
+  (void)ch;
+  (void)conditional;
+  DevGraph& g = c.g;
+  int64_t m = hi - lo;
+  int64_t cap = 4 * m + 16;
+  int ks = bits_for(std::max<int64_t>(m - 1, 0));
+  MoveList mv = move_list(c);
+  int64_t pcap = pool_cap(g);
+  if (skip_slot >= 0 && h[skip_slot] != 0) {
     *skipped = true;
     return SCOUT_OK;
   }
-  int64_t cnt_new = c.h_box[0];
+  int64_t cnt_new = h[0];
   if (cnt_new > cap) {
     set_error("apply: inserted-entry staging overflow");
     return SCOUT_ERR_STATE;
   }
-  if (c.h_box[5] != 0) {
+  if (h[5] != 0) {
     set_error("apply: inserted-row pool overflow");
     return SCOUT_ERR_STATE;
   }
-  g.holes += c.h_box[1];
-  g.A = c.h_box[4];
-  if (c.h_box[kSortSlot] != 0) {
+  g.holes += h[1];
+  g.A = h[4];
+  if (h[kSortSlot] != 0) {
     // more multi entries than one block sorts: the library sorts them, then they are placed
-    int64_t nm = c.h_box[kMultiSlot];
+    int64_t nm = h[kMultiSlot];
     SCOUT_CHECK(sort_pairs_u64(c, g.mkey.p, g.nkey2.p, g.mdst.p, g.ndst2.p, nm,
                                ks + bits_for(std::max<int64_t>(g.n - 1, 0))));
     k_place_reset<<<1, 1, 0, c.s>>>(c.d_box, g.A);
@@ -1159,6 +1202,12 @@ int graph_apply_lines(Ctx& c, Channel& ch, int64_t lo, int64_t hi, int condition
   SCOUT_CHECK(apply_range(c, ch, lo, hi, conditional, c.d_box + slot, &skipped));
   ch.pairs_distinct = !skipped && lo == 0 && hi == ch.m;
   if (!skipped) return SCOUT_OK;
+  return apply_runs(c, ch, lo, hi, conditional);
+}
+
+// Lines [lo, hi) that repeat a pair, applied as runs of distinct pairs in line order.
+int apply_runs(Ctx& c, Channel& ch, int64_t lo, int64_t hi, int conditional) {
+  int64_t m = hi - lo;
   // Repeated pairs: cut the range into runs of distinct pairs, applied in order.
   std::vector<int32_t> uv((size_t)(2 * m));
   SCOUT_TRY(hipMemcpy(uv.data(), ch.d_uv.p + 2 * lo, (size_t)(2 * m) * sizeof(int32_t), hipMemcpyDeviceToHost));
@@ -1180,6 +1229,38 @@ int graph_apply_lines(Ctx& c, Channel& ch, int64_t lo, int64_t hi, int condition
 int graph_apply(Ctx& c, Channel& ch, int conditional) {
   c.apply_seq++;
   return graph_apply_lines(c, ch, 0, ch.m, conditional);
+}
+
+int graph_apply_deferred(Ctx& c, Channel& ch) {
+  c.apply_seq++;
+  if (ch.m <= 0) return SCOUT_OK;
+  int slot = kRepeatSlot;
+  if (ch.rep_hi == ch.m) {
+    slot = kChannelRepeatSlot + (int)(&ch - c.ch);
+  } else {
+    SCOUT_CHECK(queue_repeat_count(c, ch, 0, ch.m, c.d_box + kRepeatSlot));
+  }
+  ch.pairs_distinct = false;  // known once settled
+  return apply_range(c, ch, 0, ch.m, 0, c.d_box + slot, nullptr, true);
+}
+
+int apply_settle(Ctx& c, bool* complete) {
+  *complete = true;
+  if (!c.apply_pend.on) return SCOUT_OK;
+  ApplyPending a = c.apply_pend;
+  c.apply_pend.on = false;
+  Channel& ch = *a.ch;
+  SCOUT_TRY(hipEventSynchronize(c.apply_ev));
+  bool skipped = false;
+  int64_t sorted_fallback = c.h_apply[kSortSlot];
+  SCOUT_CHECK(apply_counts(c, ch, a.lo, a.hi, a.conditional, c.h_apply, a.skip_slot, &skipped));
+  if (skipped) SCOUT_CHECK(apply_runs(c, ch, a.lo, a.hi, a.conditional));
+  ch.pairs_distinct = !skipped && a.lo == 0 && a.hi == ch.m;
+  *complete = !skipped && sorted_fallback == 0;
+  // the host-side finish reused mailbox slots [4..7], which the work queued since expects
+  // zeroed (the repair's seed counters)
+  if (!*complete) k_apply_rezero<<<1, 64, 0, c.s>>>(c.d_box);
+  return SCOUT_OK;
 }
 
 int graph_logical_degrees(Ctx& c, int64_t* d_deg) {

@@ -185,6 +185,17 @@ struct DevGraph {
   }
 };
 
+struct Channel;
+// A graph update queued without its read-back (graph_apply_deferred).
+struct ApplyPending {
+  bool on = false;
+  Channel* ch = nullptr;
+  int64_t lo = 0, hi = 0;
+  int conditional = 0;
+  int skip_slot = -1;  // mailbox slot of the repeated-pair count, or -1
+};
+constexpr int kApplyGateSlot = 53;
+
 // Per-batch edit staging: a channel holds one uploaded batch.
 struct Channel {
   PinBuf<int32_t> h_uv;
@@ -281,6 +292,11 @@ struct Ctx {
   // small mailbox read back by the host after a round (kBoxSlots entries)
   int64_t* h_box = nullptr;    // pinned
   hipEvent_t box_ev = nullptr;  // marks the last box_read_async copy
+  // a graph update whose counters have not been read yet (graph_apply_deferred); h_apply
+  // (pinned) receives them, apply_ev marks the copy
+  ApplyPending apply_pend;
+  int64_t* h_apply = nullptr;
+  hipEvent_t apply_ev = nullptr;
   // Set by scout_commit while its BFS repair runs: queues work that follows the BFS relax behind
   // each relax read-back, shut by the gate (zero once the relax has ended); *queued tells
   // whether it queued anything. cc_ahead_seq: the batch whose CC scan ran that way.
@@ -348,6 +364,15 @@ int graph_upload_egr(Ctx& c, int32_t n, int64_t E, const void* nindex, int wide,
                      bool host_resident = false);
 int graph_classify(Ctx& c, Channel& ch, int rule);
 int graph_apply(Ctx& c, Channel& ch, int conditional);
+// graph_apply (unconditional) without its read-back: the device work is queued, the counters
+// follow to the host, and mailbox slot kApplyGateSlot becomes non-zero when the host must
+// finish the apply (a repeated pair or more multi entries than one block groups). Kernels
+// queued before apply_settle that read rows take that slot as their gate. apply_settle finishes
+// the host side; *complete is false when the gated kernels did not run.
+int graph_apply_deferred(Ctx& c, Channel& ch);
+// Lines [lo, hi) that repeat a pair, applied as runs of distinct pairs in line order.
+int apply_runs(Ctx& c, Channel& ch, int64_t lo, int64_t hi, int conditional);
+int apply_settle(Ctx& c, bool* complete);
 int graph_classify_lines(Ctx& c, Channel& ch, int64_t lo, int64_t hi, int rule);
 int graph_apply_lines(Ctx& c, Channel& ch, int64_t lo, int64_t hi, int conditional);
 int graph_repeated_pairs(Ctx& c, Channel& ch, int64_t lo, int64_t hi, int64_t* repeated);

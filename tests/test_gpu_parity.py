@@ -526,6 +526,66 @@ def test_repeated_pairs_in_one_batch(data):
         assert all(np.array_equal(a, b) for a, b in zip(rows_o, rows_g)), "rows differ"
 
 
+def test_commit_settles_repeated_pairs_and_the_library_sort(data):
+    """scout_commit queues the BFS repair's first kernels before reading the graph update's
+    counters back; rounds that read rows are shut by a device gate while the update is not
+    complete. Batches that leave it incomplete: repeated pairs (the device applies nothing, the
+    host applies runs of distinct pairs) and more multi entries than one block groups (the
+    library sort places them), one batch with a 400-entry run. Levels, labels and the logical
+    rows equal the oracle's after every batch, and the stream ends with batches that settle on
+    the device."""
+    egr, _, _ = data["er"]
+    say("[commit] incomplete updates")
+    eng, g = engine_for(egr)
+    og = OracleGraph(egr)
+    eng.upload(sa.STATE_LEVEL, og.bfs(0))
+    eng.upload(sa.STATE_LABEL, og.cc())
+    rng = np.random.default_rng(9)
+    n = g.n
+    edges = np.stack([np.repeat(np.arange(n), np.diff(g.off)), g.col], 1)
+
+    def absent(count, avoid=()):
+        out, seen = [], set()
+        while len(out) < count:
+            a, b = (int(x) for x in rng.integers(0, n, 2))
+            k = (min(a, b), max(a, b))
+            if a == b or k in seen or og.is_adjacent(a, b) or a in avoid or b in avoid:
+                continue
+            seen.add(k)
+            out.append((a, b))
+        return np.array(out, dtype=np.int32)
+
+    def present(count):
+        return edges[rng.integers(0, len(edges), count)].astype(np.int32)
+
+    hub = 11
+    for k in range(1, 7):
+        if k == 1:  # repeated pairs
+            pick, rand = present(300), absent(300)
+            uv = np.concatenate([pick, rand, pick[:100, ::-1], rand[:100]])
+        elif k == 2:  # ~15,000 multi entries (most sources gain several): the library sort
+            uv = absent(15000)
+        elif k == 3:  # one source gains 400 entries among others
+            nb = [w for w in range(n) if w != hub and not og.is_adjacent(hub, w)][:400]
+            uv = np.concatenate([np.array([[hub, w] for w in nb], dtype=np.int32), absent(500, {hub})])
+            uv = uv[rng.permutation(len(uv))]
+        else:  # settled on the device
+            uv = np.concatenate([present(400), absent(400)])
+        uv = np.ascontiguousarray(uv[uv[:, 0] != uv[:, 1]])
+        eng.stage(uv)
+        eng.commit(sa.STRATEGY_LOCAL)
+        og.apply(uv, og.classify(uv))
+        assert np.array_equal(eng.download(sa.STATE_LEVEL), og.bfs(0)), f"batch {k}: levels differ"
+        assert np.array_equal(eng.download(sa.STATE_LABEL), og.cc()), f"batch {k}: labels differ"
+        # row by row (repeated pairs can leave the delta in a state whose compaction the
+        # reference refuses, so the oracle's logical CSR is not taken)
+        dg = eng.download_graph()
+        rows_g = [dg.col[dg.off[v]:dg.off[v + 1]] for v in range(n)]
+        assert all(np.array_equal(og.row(v), rows_g[v]) for v in range(n)), f"batch {k}: rows differ"
+        edges = np.stack([np.repeat(np.arange(n), np.diff(dg.off)), dg.col], 1)
+    eng.close()
+
+
 def test_device_generator_matches_host(tmp_path):
     say("[gen] rmat scale 12")
     p = tmp_path / "h.egr"
