@@ -28,7 +28,7 @@ __device__ __forceinline__ Row row_of(const GView& g, int32_t v) {
     if (!(g.touched[v >> 5] & bit)) atomicOr(&g.touched[v >> 5], bit);
   }
   AHead h = g.ahead[v];
-  return Row{h.b0, h.b1, h.off, h.off + h.len};
+  return Row{h.b0, h.b1(), h.off, h.off + h.len};
 }
 __device__ __forceinline__ int64_t row_len(const Row& r) { return (r.b1 - r.b0) + (r.a1 - r.a0); }
 // k-th physical entry of the row (base segment first, then the inserted segment). Negative
@@ -512,7 +512,7 @@ __device__ __forceinline__ PairState pair_state_in(const GView& g, int32_t u, in
 __device__ __forceinline__ PairState pair_state_h(const GView& g, bool sorted, int32_t u, const AHead& hu,
                                                   int32_t v) {
   int64_t lo, hi;
-  base_range_sampled_in(g, sorted, hu.b0, hu.b1, v, lo, hi);
+  base_range_sampled_in(g, sorted, hu.b0, hu.b1(), v, lo, hi);
   return pair_state_range(g, hu, v, lo, hi);
 }
 
@@ -531,7 +531,7 @@ __device__ __forceinline__ int32_t mirror_offset(const GView& g, const PairState
 // when t is -1 or the entry holds another neighbour).
 __device__ __forceinline__ PairState pair_state_twin(const GView& g, bool sorted, int32_t u, int32_t v, int32_t t) {
   AHead hu = g.ahead[u];
-  if (t >= 0 && hu.b0 + t < hu.b1 && decode(g.bcol[hu.b0 + t]) == v)
+  if (t >= 0 && t < hu.blen && decode(g.bcol[hu.b0 + t]) == v)
     return pair_state_range(g, hu, v, hu.b0 + t, hu.b0 + t + 1);
   return pair_state_h(g, sorted, u, hu, v);
 }
@@ -549,7 +549,7 @@ __device__ __forceinline__ bool adjacent_now(const PairState& s) {
 // compared 64 entries at a time. Uniform across the wave.
 __device__ __forceinline__ bool wave_adjacent(const GView& g, bool sorted, int32_t u, int32_t v) {
   AHead hu = g.ahead[u];
-  int64_t b = hu.b0, e = hu.b1;
+  int64_t b = hu.b0, e = hu.b1();
   unsigned long long live = 0, tomb = 0;
   if (sorted) {
     int flags = 0;

@@ -157,7 +157,7 @@ __global__ void k_classify(GView g, bool sorted, const int32_t* uv, int64_t m, i
     AHead hu = g.ahead[u];
     AHead hv;
     if (by_short_row) hv = g.ahead[v];
-    if (by_short_row && hv.b1 - hv.b0 < hu.b1 - hu.b0) {
+    if (by_short_row && hv.blen < hu.blen) {
       s = pair_state_h(g, sorted, v, hv, u);
       lo = hi = hu.b0;
       if (s.hi - s.lo == 1) {
@@ -314,7 +314,8 @@ struct MoveList {
 
 // Places k new entries of row u (dst[0..k)) after the h.len entries of its header h, or moves
 // the row to the span of `need` slots reserved at pos (need = 0: the span has room).
-__device__ __forceinline__ void place_group(AHead* ahead, int32_t* acol, int32_t* aown, const int32_t* dst, int64_t k,
+__device__ __forceinline__ void place_group(AHead* ahead, int32_t* acap, int32_t* acol, int32_t* aown,
+                                            const int32_t* dst, int64_t k,
                                             int32_t u, AHead h, int64_t need, int64_t pos, int64_t pool_cap,
                                             int64_t src, const MoveList& mv, int64_t* overflow) {
   int32_t len = h.len;
@@ -336,7 +337,7 @@ __device__ __forceinline__ void place_group(AHead* ahead, int32_t* acol, int32_t
       acol[pos + len + j] = dst[j];
       aown[pos + len + j] = u;
     }
-    set_inserted(ahead[u], pos, (int32_t)(len + k), (int32_t)need);
+    set_inserted(ahead, acap, u, pos, (int32_t)(len + k), (int32_t)need);
   } else {
     int64_t slot = (int64_t)atomicAdd((unsigned long long*)mv.count, 1ULL);
     mv.u[slot] = u;
@@ -351,7 +352,7 @@ __device__ __forceinline__ void place_group(AHead* ahead, int32_t* acol, int32_t
 // list (mkey/mdst, *nmulti). The span of a row that must move is reserved on *top with one
 // atomic per block; the pool has room for the worst case (checked on the host before the
 // batch), and *overflow counts reservations past it all the same.
-__global__ void k_place_single(AHead* ahead, int32_t* acol, int32_t* aown,
+__global__ void k_place_single(AHead* ahead, int32_t* acap, int32_t* acol, int32_t* aown,
                                const uint64_t* nkey, const int32_t* ndst, const int64_t* count, int64_t cap, int ks,
                                int32_t* ncnt, uint64_t* mkey, int32_t* mdst, int64_t* nmulti, int64_t* top,
                                int64_t pool_cap, MoveList mv, int64_t* overflow) {
@@ -368,12 +369,12 @@ __global__ void k_place_single(AHead* ahead, int32_t* acol, int32_t* aown,
   int64_t need = 0;
   if (single) {
     h = ahead[u];
-    if (h.len + 1 > h.cap) need = grown_capacity(h.len + 1);
+    if (h.len + 1 > acap[u]) need = grown_capacity(h.len + 1);
   }
   int64_t pos = block_reserve(need, top);
   if (!single) return;
   ncnt[u] = 0;
-  place_group(ahead, acol, aown, &ndst[i], 1, u, h, need, pos, pool_cap, -1 - i, mv, overflow);
+  place_group(ahead, acap, acol, aown, &ndst[i], 1, u, h, need, pos, pool_cap, -1 - i, mv, overflow);
 }
 
 // Multi entries one block groups (at most kSmallSort / 2 sources, each with two or more
@@ -499,7 +500,7 @@ __global__ __launch_bounds__(kSortThreads) void k_group_small(const uint64_t* ki
 // The grouped multi entries (skey/sdst, *count of them): the first entry of each source's run
 // places the run (one thread per run, a block-wide reservation per pass) and zeroes the
 // source's count. Nothing when *skip is set (the entries were not sorted).
-__global__ void k_place_multi(AHead* ahead, int32_t* acol, int32_t* aown,
+__global__ void k_place_multi(AHead* ahead, int32_t* acap, int32_t* acol, int32_t* aown,
                               const uint64_t* skey, const int32_t* sdst, const int64_t* count, int64_t cap, int ks,
                               int32_t* ncnt, int64_t* top, int64_t pool_cap, MoveList mv, int64_t* overflow,
                               const int64_t* skip) {
@@ -518,18 +519,19 @@ __global__ void k_place_multi(AHead* ahead, int32_t* acol, int32_t* aown,
       while (e < cnt && (int32_t)(skey[e] >> ks) == u) e++;
       k = e - i;
       h = ahead[u];
-      if (h.len + k > h.cap) need = grown_capacity(h.len + k);
+      if (h.len + k > acap[u]) need = grown_capacity(h.len + k);
     }
     int64_t pos = block_reserve(need, top);
     if (head) {
       ncnt[u] = 0;
-      place_group(ahead, acol, aown, &sdst[i], k, u, h, need, pos, pool_cap, i, mv, overflow);
+      place_group(ahead, acap, acol, aown, &sdst[i], k, u, h, need, pos, pool_cap, i, mv, overflow);
     }
   }
 }
 
 // Wave per long row that moves: its entries, then the new ones, into the reserved span.
-__global__ void k_move_rows(AHead* ahead, int32_t* acol, int32_t* aown, const int32_t* ndst, const int32_t* sdst,
+__global__ void k_move_rows(AHead* ahead, int32_t* acap, int32_t* acol, int32_t* aown, const int32_t* ndst,
+                            const int32_t* sdst,
                             MoveList mv) {
   int64_t nm = *mv.count;
   for (int64_t i = global_wave(); i < nm; i += total_waves()) {
@@ -546,7 +548,7 @@ __global__ void k_move_rows(AHead* ahead, int32_t* acol, int32_t* aown, const in
       acol[pos + len + j] = d[j];
       aown[pos + len + j] = u;
     }
-    if (lane_id() == 0) set_inserted(ahead[u], pos, (int32_t)(len + k), grown_capacity(len + k));
+    if (lane_id() == 0) set_inserted(ahead, acap, u, pos, (int32_t)(len + k), grown_capacity(len + k));
   }
 }
 
@@ -613,30 +615,30 @@ __global__ void k_repack_copy(GView g, const int64_t* aoff2, int32_t* acol2, int
 
 // The re-packed headers: start, live count and capacity of every row (and the n-th), with the
 // base rows of boff.
-__global__ void k_repack_heads(AHead* ahead2, const int64_t* off2, const int32_t* cnt, const int32_t* cap2,
-                               const int64_t* boff, int64_t n) {
+__global__ void k_repack_heads(AHead* ahead2, int32_t* acap, const int64_t* off2, const int32_t* cnt,
+                               const int32_t* cap2, const int64_t* boff, int64_t n) {
   int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (v > n) return;
   AHead h;
   h.off = off2[v];
   h.len = v < n ? cnt[v] : 0;
-  h.cap = cap2[v];
   h.b0 = boff[v];
-  h.b1 = v < n ? boff[v + 1] : boff[v];
+  h.blen = v < n ? (int32_t)(boff[v + 1] - boff[v]) : 0;
   ahead2[v] = h;
+  acap[v] = cap2[v];
 }
 
 // Empty inserted rows with the base rows of boff (and the n-th header).
-__global__ void k_reset_heads(AHead* ahead, const int64_t* boff, int64_t n) {
+__global__ void k_reset_heads(AHead* ahead, int32_t* acap, const int64_t* boff, int64_t n) {
   int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (v > n) return;
   AHead h;
   h.off = 0;
   h.len = 0;
-  h.cap = 0;
   h.b0 = boff[v];
-  h.b1 = v < n ? boff[v + 1] : boff[v];
+  h.blen = v < n ? (int32_t)(boff[v + 1] - boff[v]) : 0;
   ahead[v] = h;
+  acap[v] = 0;
 }
 
 // Live entries per vertex of the logical graph: thread per vertex for rows up to 64 physical
@@ -758,8 +760,8 @@ int repack_inserted(Ctx& c, int64_t headroom) {
   SCOUT_CHECK(g.acol2.reserve(total + headroom + 1024));
   SCOUT_CHECK(g.aown2.reserve(total + headroom + 1024));
   k_repack_copy<<<kListGrid, kBlock, 0, c.s>>>(g.view(), g.roff.p, g.acol2.p, g.aown2.p);
-  k_repack_heads<<<grid_for((int64_t)n + 1), kBlock, 0, c.s>>>(g.ahead2.p, g.roff.p, g.moves.p, g.rcap.p, g.boff,
-                                                                n);
+  k_repack_heads<<<grid_for((int64_t)n + 1), kBlock, 0, c.s>>>(g.ahead2.p, g.acap.p, g.roff.p, g.moves.p, g.rcap.p,
+                                                                g.boff, n);
   std::swap(g.ahead, g.ahead2);
   std::swap(g.acol, g.acol2);
   std::swap(g.aown, g.aown2);
@@ -849,15 +851,15 @@ int apply_range(Ctx& c, Channel& ch, int64_t lo, int64_t hi, int conditional, co
   MoveList mv = move_list(c);
   int64_t pcap = pool_cap(g);
   if (m > 0) {
-    k_place_single<<<grid_for(bound), kBlock, 0, c.s>>>(g.ahead.p, g.acol.p, g.aown.p, g.nkey.p,
+    k_place_single<<<grid_for(bound), kBlock, 0, c.s>>>(g.ahead.p, g.acap.p, g.acol.p, g.aown.p, g.nkey.p,
                                                        g.ndst.p, c.d_box, cap, ks, g.ncnt.p, g.mkey.p, g.mdst.p,
                                                        c.d_box + kMultiSlot, c.d_box + 4, pcap, mv, c.d_box + 5);
     k_group_small<<<1, kSortThreads, 0, c.s>>>(g.mkey.p, g.mdst.p, c.d_box + kMultiSlot, cap, ks, g.nkey.p,
                                                g.nkey2.p, g.ndst2.p, c.d_box + kSortSlot);
-    k_place_multi<<<kPlaceGrid, kBlock, 0, c.s>>>(g.ahead.p, g.acol.p, g.aown.p, g.nkey2.p,
+    k_place_multi<<<kPlaceGrid, kBlock, 0, c.s>>>(g.ahead.p, g.acap.p, g.acol.p, g.aown.p, g.nkey2.p,
                                                   g.ndst2.p, c.d_box + kMultiSlot, cap, ks, g.ncnt.p, c.d_box + 4, pcap,
                                                   mv, c.d_box + 5, c.d_box + kSortSlot);
-    k_move_rows<<<kListGrid, kBlock, 0, c.s>>>(g.ahead.p, g.acol.p, g.aown.p, g.ndst.p,
+    k_move_rows<<<kListGrid, kBlock, 0, c.s>>>(g.ahead.p, g.acap.p, g.acol.p, g.aown.p, g.ndst.p,
                                                g.ndst2.p, mv);
   }
   if (defer) {
@@ -909,10 +911,10 @@ int apply_counts(Ctx& c, Channel& ch, int64_t lo, int64_t hi, int conditional, c
     SCOUT_CHECK(sort_pairs_u64(c, g.mkey.p, g.nkey2.p, g.mdst.p, g.ndst2.p, nm,
                                ks + bits_for(std::max<int64_t>(g.n - 1, 0))));
     k_place_reset<<<1, 1, 0, c.s>>>(c.d_box, g.A);
-    k_place_multi<<<kPlaceGrid, kBlock, 0, c.s>>>(g.ahead.p, g.acol.p, g.aown.p, g.nkey2.p,
+    k_place_multi<<<kPlaceGrid, kBlock, 0, c.s>>>(g.ahead.p, g.acap.p, g.acol.p, g.aown.p, g.nkey2.p,
                                                   g.ndst2.p, c.d_box + kMultiSlot, cap, ks, g.ncnt.p, c.d_box + 4, pcap,
                                                   mv, c.d_box + 5, nullptr);
-    k_move_rows<<<kListGrid, kBlock, 0, c.s>>>(g.ahead.p, g.acol.p, g.aown.p, g.ndst.p,
+    k_move_rows<<<kListGrid, kBlock, 0, c.s>>>(g.ahead.p, g.acap.p, g.acol.p, g.aown.p, g.ndst.p,
                                                g.ndst2.p, mv);
     SCOUT_CHECK(box_read(c, 6));
     if (c.h_box[5] != 0) {
@@ -964,6 +966,7 @@ int graph_free(Ctx& c) {
   g.hoff.shrink_to_fit();
   g.ahead.release();
   g.ahead2.release();
+  g.acap.release();
   g.roff.release();
   g.rcap.release();
   g.acol.release();
@@ -995,7 +998,8 @@ int reset_inserted(Ctx& c) {
   DevGraph& g = c.g;
   size_t n1 = (size_t)g.n + 1;
   SCOUT_CHECK(g.ahead.reserve((int64_t)n1));
-  k_reset_heads<<<grid_for((int64_t)n1), kBlock, 0, c.s>>>(g.ahead.p, g.boff, g.n);
+  SCOUT_CHECK(g.acap.reserve((int64_t)n1));
+  k_reset_heads<<<grid_for((int64_t)n1), kBlock, 0, c.s>>>(g.ahead.p, g.acap.p, g.boff, g.n);
   SCOUT_CHECK(g.acol.reserve(1 << 16));
   SCOUT_CHECK(g.aown.reserve(g.acol.cap));
   g.A = g.live = g.holes = 0;

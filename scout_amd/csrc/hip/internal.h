@@ -102,22 +102,25 @@ struct PinBuf {
   }
 };
 
-// Header of a vertex's rows: its inserted row's span [off, off + cap) of the pool, of which the
-// first len slots are in use, and its base row [b0, b1) of the base entries (a copy of the
-// base offsets). One aligned 32-byte record, so a row lookup is one load of one cache line.
-struct alignas(32) AHead {
+// Header of a vertex's rows: its inserted row acol[off, off + len) and its base row
+// bcol[b0, b0 + blen) (a copy of the base offsets). One 24-byte record, so a row lookup reads
+// one cache line (two for the one record in eight that straddles a line boundary). The
+// inserted row's reserved span capacity, read only when entries are placed, is kept apart
+// (DevGraph::acap), so the headers take what the offsets and a 16-byte header took together.
+struct alignas(8) AHead {
   int64_t off;
   int32_t len;
-  int32_t cap;
+  int32_t blen;
   int64_t b0;
-  int64_t b1;
+  __host__ __device__ int64_t b1() const { return b0 + blen; }
 };
 
-// Sets the inserted row of a header, keeping its base row.
-__host__ __device__ inline void set_inserted(AHead& h, int64_t off, int32_t len, int32_t cap) {
-  h.off = off;
-  h.len = len;
-  h.cap = cap;
+// Sets the inserted row of vertex u (start, length and span capacity), keeping its base row.
+__host__ __device__ inline void set_inserted(AHead* ahead, int32_t* acap, int64_t u, int64_t off, int32_t len,
+                                             int32_t cap) {
+  ahead[u].off = off;
+  ahead[u].len = len;
+  acap[u] = cap;
 }
 
 // Read-only view of the logical graph for kernels.
@@ -158,6 +161,7 @@ struct DevGraph {
   int32_t* hbcol = nullptr;
   std::vector<int64_t> hoff;
   DevBuf<AHead> ahead, ahead2;
+  DevBuf<int32_t> acap;  // per vertex: slots reserved for its inserted row
   DevBuf<int64_t> roff;   // re-pack scratch: new row starts
   DevBuf<int32_t> rcap;   // re-pack scratch: new row capacities
   DevBuf<int32_t> acol, acol2, aown, aown2;

@@ -91,17 +91,18 @@ __device__ __forceinline__ bool member(const MisView& s, int32_t x, bool* confli
 // Live entries equal to y in row x (base entries not marked deleted, inserted entries); the
 // shorter of the two rows is searched (the adjacency is symmetric).
 __device__ __forceinline__ int32_t live_count(const MisView& s, int32_t x, int32_t y) {
-  AHead hx = s.g.ahead[x], hy = s.g.ahead[y];
-  if ((hy.b1 - hy.b0) + hy.len < (hx.b1 - hx.b0) + hx.len) {
+  int64_t lx = (s.g.boff[x + 1] - s.g.boff[x]) + s.g.ahead[x].len;
+  int64_t ly = (s.g.boff[y + 1] - s.g.boff[y]) + s.g.ahead[y].len;
+  if (ly < lx) {
     int32_t tmp = x;
     x = y;
     y = tmp;
-    hx = hy;
   }
   int64_t lo, hi;
-  base_range_in(s.g, s.sorted, hx.b0, hx.b1, y, lo, hi);
+  base_range(s.g, s.sorted, x, y, lo, hi);
   int32_t c = 0;
   for (int64_t k = lo; k < hi; k++) c += s.g.bcol[k] == y;
+  AHead hx = s.g.ahead[x];
   for (int64_t k = hx.off, e = k + hx.len; k < e; k++) c += s.g.acol[k] == y;
   return c;
 }
@@ -550,7 +551,9 @@ __device__ __forceinline__ bool hop2_request(const MisDev& d, int64_t* count, in
 
 // Pass A: wave per window line. Writes go to w0/w1 and are marked (minw[w] = smallest writer
 // of w, and the neighbour marks of w's row). A deferred hop-2 search leaves w1 pending.
-__global__ void k_mm_eval(MisDev d, SegView sv) {
+// Launched with kBlock threads: the bound lets the compiler give it more than the 128 VGPRs a
+// 1024-thread block would allow (at 128 it spilled to scratch, 3x slower).
+__global__ void __launch_bounds__(kBlock) k_mm_eval(MisDev d, SegView sv) {
   int64_t slots = sv.t[kHSlots];
   int64_t scanned = 0;
   for (int64_t t = global_wave(); t < slots; t += total_waves()) {

@@ -500,6 +500,7 @@ void stream_release(Ctx& c) {
 struct Snapshot {
   DevBuf<int64_t> boff;
   DevBuf<AHead> ahead;
+  DevBuf<int32_t> acap;
   DevBuf<int32_t> bcol, acol, aown;
   int64_t live = 0;
   DevBuf<int32_t> level, parent, label, color;
@@ -520,11 +521,13 @@ int snapshot_save(Ctx& c) {
   int64_t n = c.g.n;
   SCOUT_CHECK(s.boff.reserve(n + 1));
   SCOUT_CHECK(s.ahead.reserve(n + 1));
+  SCOUT_CHECK(s.acap.reserve(n + 1));
   SCOUT_CHECK(s.bcol.reserve(c.g.E + 1));
   SCOUT_CHECK(s.acol.reserve(c.g.A + 1));
   SCOUT_CHECK(s.aown.reserve(c.g.A + 1));
   SCOUT_TRY(hipMemcpyAsync(s.boff.p, c.g.boff, (size_t)(n + 1) * 8, hipMemcpyDeviceToDevice, c.s));
   SCOUT_TRY(hipMemcpyAsync(s.ahead.p, c.g.ahead.p, (size_t)(n + 1) * sizeof(AHead), hipMemcpyDeviceToDevice, c.s));
+  SCOUT_TRY(hipMemcpyAsync(s.acap.p, c.g.acap.p, (size_t)(n + 1) * sizeof(int32_t), hipMemcpyDeviceToDevice, c.s));
   SCOUT_TRY(hipMemcpyAsync(s.bcol.p, c.g.bcol, (size_t)c.g.E * 4, hipMemcpyDeviceToDevice, c.s));
   if (c.g.A > 0) {
     SCOUT_TRY(hipMemcpyAsync(s.acol.p, c.g.acol.p, (size_t)c.g.A * 4, hipMemcpyDeviceToDevice, c.s));
@@ -575,6 +578,7 @@ int snapshot_restore(Ctx& c) {
   }
   SCOUT_TRY(hipMemcpyAsync(c.g.boff, s.boff.p, (size_t)(n + 1) * 8, hipMemcpyDeviceToDevice, c.s));
   SCOUT_TRY(hipMemcpyAsync(c.g.ahead.p, s.ahead.p, (size_t)(n + 1) * sizeof(AHead), hipMemcpyDeviceToDevice, c.s));
+  SCOUT_TRY(hipMemcpyAsync(c.g.acap.p, s.acap.p, (size_t)(n + 1) * sizeof(int32_t), hipMemcpyDeviceToDevice, c.s));
   SCOUT_TRY(hipMemcpyAsync(c.g.bcol, s.bcol.p, (size_t)s.E * 4, hipMemcpyDeviceToDevice, c.s));
 
   SCOUT_CHECK(c.g.acol.reserve(s.A + 1));
@@ -617,6 +621,7 @@ void snapshot_release(Ctx& c) {
   Snapshot& s = *c.snap;
   s.boff.release();
   s.ahead.release();
+  s.acap.release();
   s.bcol.release();
   s.acol.release();
   s.aown.release();
